@@ -1,0 +1,157 @@
+#!/usr/bin/env python3
+"""Throughput of the FP64 IMEX step on one MI355X.
+
+Workload: the lid-driven cavity with a deformable disk at N = 1024 (ACDI,
+seed 0: radius ~ U[0.15, 0.25], centre ~ U[0.4, 0.6]^2, dt = 1e-3*128/N),
+solved at the reference's tolerances (velocity 1e-8, projection 1e-10).  A
+"step" is one integrate_step! of the whole grid with the state resident in
+HBM.  W warm-up steps develop the flow from rest; K steps are timed.
+
+The JSON line carries:
+  roofline     -- the dominant kernel's algorithmic bytes / its mean launch
+                  time, measured with HIP events on the library's own stream;
+  cpu_baseline -- the CPU oracle (a C restatement of the reference step, one
+                  thread) timed on a bounded sample: S steps from the GPU's
+                  state after the timed region, same tolerances.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "IMEX steps/s at N=1024 (FP64 vs mixed @1e-8), HBM roofline %, adjoint grads/s"
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+
+# Algorithmic HBM passes per step (fields of N^2 doubles): tendencies 3 x 20,
+# stage combinations 20 + 28 + 32, velocity CG 20 / iteration + 6 per solve,
+# projection PCG (V-cycle + vector kernels) per iteration + 18 per projection.
+def step_bytes(N, vel_iters, proj_iters, nlev_passes):
+    cell = 8.0 * N * N
+    passes = 60 + 80 + sum(6 + 20 * k for k in vel_iters) + sum(18 + nlev_passes * k for k in proj_iters)
+    return passes * cell
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--N", type=int, default=1024)
+    ap.add_argument("--method", default="acdi")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--only-kernel", default=None, help="time one kernel standalone and exit (profiling runs)")
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build's step runs on a single MI355X (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    import eulerianfsi_amd as fsi
+    from eulerianfsi_amd import _abi, setups
+
+    case = setups.seeded_cavity(args.N, args.method, args.seed, velocity_rtol=1e-8, projection_rtol=1e-10)
+    ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
+    work = fsi.allocate_imex_workspace(ctx)
+    state = case.state
+    dt = case.dt
+    t = 0.0
+    fsi.make_velocity_compatible(state, ctx, work)
+    for _ in range(args.warmup):
+        fsi.integrate_step(state, t, dt, ctx, work)
+        t += dt
+        print(f"warmup step t={t:.5f} iters={work.last_stats}", file=sys.stderr, flush=True)
+
+    def bench_kernel(name, reps):
+        import ctypes
+        ms, nbytes = ctypes.c_double(), ctypes.c_double()
+        st = ctx.lib.fsi_benchmark_kernel(ctx.handle, state._device(ctx), work.handle, name.encode(), reps,
+                                          ctypes.byref(ms), ctypes.byref(nbytes))
+        _abi.check(ctx.lib, ctx.handle, st, f"benchmark {name}")
+        return ms.value, nbytes.value
+
+    if args.only_kernel:
+        ms, nb = bench_kernel(args.only_kernel, args.kernel_reps)
+        print(json.dumps({"kernel": args.only_kernel, "avg_ms": ms, "bytes": nb, "GBps": nb / ms / 1e6}))
+        return 0
+
+    vel_tot, proj_tot = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        rep = fsi.integrate_step(state, t, dt, ctx, work)
+        t += dt
+        vel_tot.append(work.last_stats.velocity_iterations)
+        proj_tot.append(work.last_stats.projection_iterations)
+    elapsed = time.perf_counter() - t0
+    steps_per_s = args.steps / elapsed
+    ms_per_step = 1e3 * elapsed / args.steps
+    vel_mean = np.mean(np.array(vel_tot), axis=0)
+    proj_mean = np.mean(np.array(proj_tot), axis=0)
+    print(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, vel iters {vel_mean}, proj iters {proj_mean}, "
+          f"report {rep}", file=sys.stderr, flush=True)
+
+    # per-kernel measurement: the candidates for the dominant kernel
+    kernels = {}
+    for name in ("vel_dir", "vel_upd", "pcg_dir", "pcg_upd", "mg_jacobi", "mg_resid", "momentum", "cells", "copy"):
+        ms, nb = bench_kernel(name, args.kernel_reps)
+        kernels[name] = {"avg_ms": ms, "bytes": nb, "GBps": nb / ms / 1e6}
+        print(f"kernel {name}: {ms * 1e3:.1f} us, {nb / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
+    # launches per step of each kernel
+    nv = float(np.sum(vel_mean))
+    npj = float(np.sum(proj_mean))
+    launches = {"vel_dir": nv, "vel_upd": nv, "pcg_dir": npj, "pcg_upd": npj, "mg_jacobi": 4 * npj,
+                "mg_resid": npj, "momentum": 3, "cells": 3}
+    share = {k: launches[k] * kernels[k]["avg_ms"] for k in launches}
+    dominant = max(share, key=share.get)
+    dk = kernels[dominant]
+    roofline = {"kernel": dominant, "bound": "hbm", "achieved": dk["GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": dk["GBps"] * 1e9 / HBM_PEAK, "traffic": None,
+                "bytes_per_launch": dk["bytes"], "avg_launch_us": dk["avg_ms"] * 1e3}
+
+    nlev_passes = 20.0
+    b_step = step_bytes(args.N, vel_mean, proj_mean, nlev_passes)
+    whole_step = {"algorithmic_GB": b_step / 1e9, "achieved_GBps": b_step / (ms_per_step * 1e-3) / 1e9,
+                  "frac_of_peak": b_step / (ms_per_step * 1e-3) / HBM_PEAK}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle.oracle import Oracle, state_dict
+        o = Oracle(case.metrics, case.bcs, case.models, case.numerics)
+        s = state_dict(state)
+        o.correction[:] = work.correction
+        c0 = time.perf_counter()
+        for k in range(args.cpu_steps):
+            o.integrate_step(s, t + k * dt, dt)
+            print(f"cpu oracle step {k} done", file=sys.stderr, flush=True)
+        cpu_s = (time.perf_counter() - c0) / args.cpu_steps
+        cpu = {"value": 1.0 / cpu_s, "unit": "IMEX steps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} step(s) of the same N={args.N} {args.method} cavity from the GPU state "
+                         f"after the timed region, same tolerances, 1 thread"}
+
+    out = {
+        "metric": METRIC, "value": steps_per_s, "unit": "IMEX steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded cavity initial condition)",
+        "config": {"workload": f"lid-driven cavity with deformable disk, N={args.N}x{args.N}, {args.method.upper()}, "
+                               f"FP64, dt={dt:.4g}, seed {args.seed}, velocity rtol 1e-8, projection rtol 1e-10",
+                   "N": args.N, "method": args.method, "cell_updates_per_s": steps_per_s * args.N ** 2,
+                   "velocity_iterations": vel_mean.tolist(), "projection_iterations": proj_mean.tolist(),
+                   "mixed_precision": None, "adjoint_grads_per_s": None},
+        "roofline": roofline, "whole_step": whole_step, "kernels": kernels, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

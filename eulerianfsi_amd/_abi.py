@@ -1,0 +1,271 @@
+"""ctypes mirror of include/fsi_api.h and the loader of libfsi_hip.so.
+
+The structures here are the C-ABI configuration blocks; `abi_config` turns
+the Python closure objects (the mirror of the reference's FSIModels /
+FSIBoundaryConditions / IMEXNumerics) into them.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+from pathlib import Path
+
+from . import boundary as _bc
+from . import closures as _cl
+from .errors import raise_for_status
+
+# ---- enums (fsi_api.h) ------------------------------------------------------
+FSI_OK = 0
+BC_PERIODIC, BC_DIRICHLET, BC_NEUMANN = 0, 1, 2
+PHASE_UPWIND, PHASE_CDI, PHASE_ACDI, PHASE_UWCDI = 0, 1, 2, 3
+VISC_NONE, VISC_CONSTANT = 0, 1
+ELASTIC_NONE, ELASTIC_LINEAR = 0, 1
+RELAX_NONE, RELAX_MAXWELL = 0, 1
+FORCE_NONE, FORCE_CONSTANT, FORCE_KOLMOGOROV = 0, 1, 2
+EXT_NONE, EXT_IDENTITY, EXT_GEOMETRIC = 0, 1, 2
+PRECISION_FP64, PRECISION_MIXED = 0, 1
+
+
+class BcSide(ctypes.Structure):
+    _fields_ = [("kind", c_int32), ("value", c_double)]
+
+
+class BcSetup(ctypes.Structure):
+    _fields_ = [("left", BcSide), ("right", BcSide), ("bottom", BcSide), ("top", BcSide)]
+
+
+class Bcs(ctypes.Structure):
+    _fields_ = [(n, BcSetup) for n in ("U", "V", "correction", "phi", "resource", "sigma_xx", "sigma_yy", "sigma_xy")]
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("lx", c_double), ("ly", c_double),
+                ("periodic_x", c_int32), ("periodic_y", c_int32)]
+
+
+class Models(ctypes.Structure):
+    _fields_ = [
+        ("phase_method", c_int32),
+        ("cdi_inverse_peclet", c_double), ("cdi_epsilon", c_double), ("cdi_delta_normal", c_double),
+        ("acdi_gamma", c_double), ("acdi_epsilon", c_double), ("acdi_psi_floor", c_double),
+        ("acdi_delta_normal", c_double), ("acdi_dynamic_gamma", c_int32),
+        ("uwcdi_alpha", c_double), ("uwcdi_delta_normal", c_double),
+        ("viscosity_kind", c_int32), ("viscosity", c_double),
+        ("elastic_kind", c_int32), ("shear_modulus", c_double),
+        ("relaxation_kind", c_int32), ("relaxation_rate", c_double),
+        ("body_force_kind", c_int32),
+        ("force_fx", c_double), ("force_fy", c_double),
+        ("force_amplitude", c_double), ("force_wavenumber", c_double), ("force_phase", c_double),
+    ]
+
+
+class Numerics(ctypes.Structure):
+    _fields_ = [
+        ("extension_kind", c_int32),
+        ("ext_max_rate", c_double), ("ext_exponent", c_double), ("ext_cutoff", c_double),
+        ("velocity_rtol", c_double), ("velocity_maxiter", c_int32),
+        ("projection_rtol", c_double), ("projection_maxiter", c_int32),
+        ("stage1_atol", c_double), ("stage1_rtol", c_double), ("check_stage1", c_int32),
+        ("precision", c_int32),
+    ]
+
+
+class StepReportC(ctypes.Structure):
+    _fields_ = [
+        ("maximum_projection_constraint_linf", c_double),
+        ("maximum_growth_trace_error", c_double),
+        ("minimum_phi", c_double),
+        ("maximum_phi", c_double),
+        ("minimum_resource", c_double),
+        ("minimum_sigma_eigenvalue", c_double),
+        ("maximum_abs_sigma_determinant_error", c_double),
+        ("non_spd_cells", c_int64),
+    ]
+
+
+class ProjectionResultC(ctypes.Structure):
+    _fields_ = [
+        ("linear_relative_residual", c_double),
+        ("constraint_linf", c_double),
+        ("constraint_l2", c_double),
+        ("compatibility_defect", c_double),
+        ("iterations", c_int32),
+    ]
+
+
+class StepStatsC(ctypes.Structure):
+    _fields_ = [("velocity_iterations", c_int32 * 3), ("projection_iterations", c_int32 * 3)]
+
+
+# ---- configuration conversion ------------------------------------------------
+def _side(s) -> BcSide:
+    if isinstance(s, _bc.Periodic):
+        return BcSide(BC_PERIODIC, 0.0)
+    if isinstance(s, _bc.Dirichlet):
+        return BcSide(BC_DIRICHLET, _bc.scalar_value(s))
+    if isinstance(s, _bc.Neumann):
+        return BcSide(BC_NEUMANN, _bc.scalar_value(s))
+    raise _bc.UnsupportedBoundary(f"{type(s).__name__} boundaries are outside the device path")
+
+
+def _setup(b) -> BcSetup:
+    return BcSetup(_side(b.left), _side(b.right), _side(b.bottom), _side(b.top))
+
+
+def abi_config(metrics, bcs, models, numerics):
+    """(Grid, Bcs, Models, Numerics) for a MeshMetrics / FSIBoundaryConditions /
+    FSIModels / IMEXNumerics quadruple."""
+    grid = Grid(metrics.Nx, metrics.Ny, metrics.Lx, metrics.Ly, int(metrics.periodic_x), int(metrics.periodic_y))
+    cb = Bcs(
+        _setup(bcs.U), _setup(bcs.V), _setup(bcs.correction), _setup(bcs.phi), _setup(bcs.resource),
+        _setup(bcs.sigma["xx"]), _setup(bcs.sigma["yy"]), _setup(bcs.sigma["xy"]),
+    )
+    m = Models()
+    pf = models.phase_field_method
+    if isinstance(pf, _cl.LegacySplitPhaseFieldMethod):
+        flux = pf.interface_flux
+        if isinstance(flux, _cl.NoInterfaceFlux):
+            m.phase_method = PHASE_UPWIND
+        else:
+            m.phase_method = PHASE_CDI
+            m.cdi_inverse_peclet, m.cdi_epsilon, m.cdi_delta_normal = flux.inverse_peclet, flux.epsilon, flux.delta_normal
+    elif isinstance(pf, _cl.NoPhaseFieldRegularization):
+        m.phase_method = PHASE_UPWIND
+    elif isinstance(pf, _cl.CurrentCDIPhaseField):
+        f = pf.flux
+        m.phase_method = PHASE_CDI
+        m.cdi_inverse_peclet, m.cdi_epsilon, m.cdi_delta_normal = f.inverse_peclet, f.epsilon, f.delta_normal
+    elif isinstance(pf, _cl.JainACDIPhaseField):
+        m.phase_method = PHASE_ACDI
+        m.acdi_gamma, m.acdi_epsilon, m.acdi_psi_floor = pf.gamma, pf.epsilon, pf.psi_floor
+        m.acdi_delta_normal, m.acdi_dynamic_gamma = pf.delta_normal, int(pf.dynamic_gamma)
+    elif isinstance(pf, _cl.PirozzoliUWCDIPhaseField):
+        if pf.normal_reconstruction != "standard":
+            raise _cl.UnsupportedModel("UWCDI normal reconstruction other than :standard is outside the device path")
+        m.phase_method = PHASE_UWCDI
+        m.uwcdi_alpha, m.uwcdi_delta_normal = pf.alpha, pf.delta_normal
+    else:
+        raise _cl.UnsupportedModel(f"{type(pf).__name__} is outside the device path")
+
+    v = models.viscosity
+    if isinstance(v, _cl.ConstantViscosity):
+        m.viscosity_kind, m.viscosity = VISC_CONSTANT, v.value
+    elif isinstance(v, _cl.NoViscosity):
+        m.viscosity_kind = VISC_NONE
+    else:
+        raise _cl.UnsupportedModel(f"{type(v).__name__} is outside the device path")
+    e = models.elastic_stress
+    if isinstance(e, _cl.LinearInternalTensorStress):
+        m.elastic_kind, m.shear_modulus = ELASTIC_LINEAR, e.coefficient
+    elif isinstance(e, _cl.NoElasticStress):
+        m.elastic_kind = ELASTIC_NONE
+    else:
+        raise _cl.UnsupportedModel(f"{type(e).__name__} is outside the device path")
+    r = models.tensor_relaxation
+    if isinstance(r, _cl.MaxwellRelaxation):
+        m.relaxation_kind, m.relaxation_rate = RELAX_MAXWELL, r.rate
+    elif isinstance(r, _cl.NoRelaxation):
+        m.relaxation_kind = RELAX_NONE
+    else:
+        raise _cl.UnsupportedModel(f"{type(r).__name__} is outside the device path")
+    f = models.body_force
+    if isinstance(f, _cl.KolmogorovBodyForce):
+        m.body_force_kind = FORCE_KOLMOGOROV
+        m.force_amplitude, m.force_wavenumber, m.force_phase = f.amplitude, f.wavenumber, f.phase
+    elif isinstance(f, _cl.ConstantBodyForce):
+        m.body_force_kind, m.force_fx, m.force_fy = FORCE_CONSTANT, f.fx, f.fy
+    elif isinstance(f, _cl.NoBodyForce):
+        m.body_force_kind = FORCE_NONE
+    else:
+        raise _cl.UnsupportedModel(f"{type(f).__name__} is outside the device path")
+    for name in ("growth_source", "growth_tensor", "diffusivity", "resource_consumption", "capillarity"):
+        obj = getattr(models, name)
+        if not isinstance(obj, (_cl.NoGrowth, _cl.IsotropicGrowthTensor, _cl.NoDiffusion,
+                                _cl.NoResourceConsumption, _cl.NoCapillarity)):
+            raise _cl.UnsupportedModel(f"{name}={type(obj).__name__} is outside the device path")
+
+    n = Numerics()
+    ext = numerics.structure_extension
+    if isinstance(ext, _cl.GeometricStructureExtension):
+        n.extension_kind, n.ext_max_rate, n.ext_exponent, n.ext_cutoff = (
+            EXT_GEOMETRIC, ext.max_rate, ext.exponent, ext.safety_cutoff)
+    elif isinstance(ext, _cl.IdentityOutsideStructure):
+        n.extension_kind, n.ext_cutoff = EXT_IDENTITY, ext.cutoff
+    else:
+        n.extension_kind = EXT_NONE
+    ls = numerics.linear_solvers
+    n.velocity_rtol, n.velocity_maxiter = ls.reltol, ls.maxiter
+    n.projection_rtol, n.projection_maxiter = numerics.projection_reltol, numerics.projection_maxiter
+    n.stage1_atol, n.stage1_rtol = numerics.stage1_constraint_atol, numerics.stage1_constraint_rtol
+    n.check_stage1 = int(numerics.check_stage1_constraint)
+    n.precision = PRECISION_MIXED if numerics.precision == "mixed" else PRECISION_FP64
+    return grid, cb, m, n
+
+
+# ---- library -----------------------------------------------------------------
+_LIB = None
+LIB_PATH = Path(__file__).resolve().parent / "libfsi_hip.so"
+
+_dp = POINTER(c_double)
+
+
+def _declare(lib):
+    P = c_void_p
+    sig = {
+        "fsi_context_create": (c_int, [POINTER(Grid), POINTER(Bcs), POINTER(Models), POINTER(Numerics), POINTER(P)]),
+        "fsi_context_destroy": (None, [P]),
+        "fsi_last_error": (c_char_p, [P]),
+        "fsi_state_create": (c_int, [P, POINTER(P)]),
+        "fsi_state_destroy": (None, [P]),
+        "fsi_state_upload": (c_int, [P, P] + [_dp] * 7),
+        "fsi_state_download": (c_int, [P, P] + [_dp] * 7),
+        "fsi_workspace_create": (c_int, [P, POINTER(P)]),
+        "fsi_workspace_destroy": (None, [P]),
+        "fsi_workspace_download_correction": (c_int, [P, P, _dp]),
+        "fsi_workspace_upload_correction": (c_int, [P, P, _dp]),
+        "fsi_make_velocity_compatible": (c_int, [P, P, P, POINTER(ProjectionResultC)]),
+        "fsi_integrate_step": (c_int, [P, P, P, c_double, c_double, POINTER(StepReportC), POINTER(StepStatsC)]),
+        "fsi_project": (c_int, [P, P, P, POINTER(ProjectionResultC)]),
+        "fsi_eval_tendencies": (c_int, [P, P, P, c_double] + [_dp] * 8),
+        "fsi_benchmark_kernel": (c_int, [P, P, P, c_char_p, c_int32, _dp, _dp]),
+        "fsi_version": (c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+EXPORTED = (
+    "fsi_context_create", "fsi_context_destroy", "fsi_last_error", "fsi_state_create", "fsi_state_destroy",
+    "fsi_state_upload", "fsi_state_download", "fsi_workspace_create", "fsi_workspace_destroy",
+    "fsi_workspace_download_correction", "fsi_workspace_upload_correction", "fsi_make_velocity_compatible",
+    "fsi_integrate_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel", "fsi_version",
+)
+
+
+def load_library(path: str | os.PathLike | None = None):
+    """Load libfsi_hip.so.  Raises ImportError when the library has not been
+    built: there is no CPU fallback behind this API."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise ImportError(f"{p} is missing: build it with `python -m eulerianfsi_amd.build`")
+    lib = _declare(ctypes.CDLL(str(p)))
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def check(lib, ctx, status: int, what: str) -> None:
+    if status != FSI_OK:
+        msg = lib.fsi_last_error(ctx)
+        raise_for_status(status, f"{what}: {msg.decode() if msg else ''}")
+
+
+def as_ptr(arr):
+    return arr.ctypes.data_as(_dp) if arr is not None else None

@@ -1,0 +1,925 @@
+// fsi_api.hip -- C ABI of libfsi_hip.so: context, device-resident state and
+// workspace, and the host side of the four-stage IMEX step
+// (src/TimeIntegrationIMEX.jl:1152-1256).
+//
+// One HIP stream per context.  A step enqueues every kernel of the four
+// stages and reads back one small Status block at the end (the StepReport,
+// the first error raised by any kernel, iteration counts).  The Krylov solves
+// run with their convergence flags in device memory: the host enqueues
+// iterations in chunks and polls the flag between chunks, sized from the
+// previous step's iteration counts.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fsi_kernels.hpp"
+
+using namespace fsi;
+
+namespace {
+
+const double AE[4][4] = {{0, 0, 0, 0},
+                         {8.0 / 15.0, 0, 0, 0},
+                         {1.0 / 4.0, 5.0 / 12.0, 0, 0},
+                         {1.0 / 4.0, 0, 3.0 / 4.0, 0}};
+const double AI[4][4] = {{0, 0, 0, 0},
+                         {4.0 / 15.0, 4.0 / 15.0, 0, 0},
+                         {4.0 / 15.0, 1.0 / 3.0, 1.0 / 15.0, 0},
+                         {4.0 / 15.0, 1.0 / 3.0, 7.0 / 30.0, 1.0 / 6.0}};
+
+constexpr int kMaxLevels = 24;
+
+}  // namespace
+
+struct fsi_context {
+    fsi_grid grid{};
+    fsi_bcs bcs{};
+    fsi_models models{};
+    fsi_numerics num{};
+    Params prm{};
+    BcSet bpsi{};
+    int P = 0;
+    size_t felems = 0;
+    int has_null = 0;
+    int has_reset = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    double* force_row = nullptr;
+    double* lap_bc = nullptr;
+    double* bvisc_u = nullptr;
+    double* bvisc_v = nullptr;
+    double* partials = nullptr;
+    Status* st = nullptr;       // device
+    Status* st_host = nullptr;  // pinned mirror
+    std::vector<MgGeom> geo;
+    int last_vel_iters[3] = {8, 8, 8};
+    int last_proj_iters[3] = {6, 6, 6};
+};
+
+struct fsi_state {
+    fsi_context* ctx = nullptr;
+    double *U = nullptr, *V = nullptr, *phi = nullptr, *res = nullptr;
+    double *sxx = nullptr, *syy = nullptr, *sxy = nullptr;
+    double res_min = INFINITY;
+    long long res_nonfinite = 0;
+};
+
+struct fsi_workspace {
+    fsi_context* ctx = nullptr;
+    double *Us = nullptr, *Vs = nullptr, *phis = nullptr, *sxxs = nullptr, *syys = nullptr, *sxys = nullptr;
+    double *Eu[3] = {}, *Ev[3] = {}, *Iu[3] = {}, *Iv[3] = {};
+    double *Ep[3] = {}, *Exx[3] = {}, *Eyy[3] = {}, *Exy[3] = {};
+    double *ru = nullptr, *rv = nullptr, *r_u = nullptr, *r_v = nullptr;
+    double *p_u[2] = {}, *p_v[2] = {}, *q_u = nullptr, *q_v = nullptr;
+    double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
+    int corr_cur = 0;
+    std::vector<MgLevel> lv;
+    std::vector<double*> owned;
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static fsi_status fail(fsi_context* c, fsi_status code, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    return code;
+}
+
+#define HIPCHECK(c, expr)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((c), FSI_E_HIP, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, \
+                        __LINE__);                                                               \
+    } while (0)
+
+static hipError_t dalloc(fsi_context* c, double** p, size_t n) {
+    hipError_t e = hipMalloc((void**)p, n * sizeof(double));
+    if (e == hipSuccess) e = hipMemsetAsync(*p, 0, n * sizeof(double), c->stream);
+    return e;
+}
+
+static BcSet to_bcset(const fsi_bc_setup& s) {
+    BcSet b;
+    b.l = {s.left.kind, s.left.value};
+    b.r = {s.right.kind, s.right.value};
+    b.b = {s.bottom.kind, s.bottom.value};
+    b.t = {s.top.kind, s.top.value};
+    return b;
+}
+
+static fsi_status upload2d(fsi_context* c, double* dst, const double* src, int width, int height) {
+    HIPCHECK(c, hipMemcpy2DAsync(dst, (size_t)c->P * sizeof(double), src, (size_t)width * sizeof(double),
+                                 (size_t)width * sizeof(double), height, hipMemcpyHostToDevice, c->stream));
+    return FSI_OK;
+}
+static fsi_status download2d(fsi_context* c, double* dst, const double* src, int width, int height) {
+    HIPCHECK(c, hipMemcpy2DAsync(dst, (size_t)width * sizeof(double), src, (size_t)c->P * sizeof(double),
+                                 (size_t)width * sizeof(double), height, hipMemcpyDeviceToHost, c->stream));
+    return FSI_OK;
+}
+
+static const char* side_name(int k) {
+    static const char* n[4] = {"left", "right", "bottom", "top"};
+    return n[k];
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static fsi_status validate(fsi_context* c) {
+    const fsi_grid& g = c->grid;
+    if (g.nx < 2 || g.ny < 2) return fail(c, FSI_E_ARGUMENT, "Nx and Ny must be at least 2.");
+    if (!(g.lx > 0.0) || !(g.ly > 0.0)) return fail(c, FSI_E_ARGUMENT, "Lx and Ly must be positive.");
+    const fsi_bc_setup* all[8] = {&c->bcs.U, &c->bcs.V, &c->bcs.correction, &c->bcs.phi,
+                                  &c->bcs.resource, &c->bcs.sigma_xx, &c->bcs.sigma_yy, &c->bcs.sigma_xy};
+    const char* names[8] = {"U", "V", "correction", "phi", "resource", "sigma_xx", "sigma_yy", "sigma_xy"};
+    for (int k = 0; k < 8; ++k) {
+        const fsi_bc_side* sd[4] = {&all[k]->left, &all[k]->right, &all[k]->bottom, &all[k]->top};
+        for (int q = 0; q < 4; ++q) {
+            if (sd[q]->kind < 0 || sd[q]->kind > 2)
+                return fail(c, FSI_E_UNSUPPORTED, "%s %s boundary kind %d is not supported on the device.", names[k],
+                            side_name(q), sd[q]->kind);
+            if (!std::isfinite(sd[q]->value))
+                return fail(c, FSI_E_ARGUMENT, "%s %s boundary value must be finite.", names[k], side_name(q));
+        }
+        int lp = all[k]->left.kind == FSI_BC_PERIODIC, rp = all[k]->right.kind == FSI_BC_PERIODIC;
+        int bp = all[k]->bottom.kind == FSI_BC_PERIODIC, tp = all[k]->top.kind == FSI_BC_PERIODIC;
+        if (lp != rp || bp != tp)  // BoundaryConditions.jl:74-84
+            return fail(c, FSI_E_ARGUMENT, "Periodic boundaries must be specified as an opposing pair (%s).",
+                        names[k]);
+        if (lp != (g.periodic_x != 0))  // TimeIntegrationIMEX.jl:447-466
+            return fail(c, FSI_E_ARGUMENT, "%s boundary conditions do not match MeshMetrics periodicity in x.",
+                        names[k]);
+        if (bp != (g.periodic_y != 0))
+            return fail(c, FSI_E_ARGUMENT, "%s boundary conditions do not match MeshMetrics periodicity in y.",
+                        names[k]);
+    }
+    const fsi_models& m = c->models;
+    switch (m.phase_method) {
+    case FSI_PHASE_UPWIND: break;
+    case FSI_PHASE_CDI:  // ConservativeDiffuseInterfaceFlux, InterfaceRegularization.jl:30-63
+        if (!(m.cdi_inverse_peclet >= 0.0) || !std::isfinite(m.cdi_inverse_peclet))
+            return fail(c, FSI_E_ARGUMENT, "inverse_peclet must be finite and non-negative.");
+        if (!(m.cdi_epsilon > 0.0)) return fail(c, FSI_E_ARGUMENT, "epsilon must be strictly positive.");
+        if (!(m.cdi_delta_normal > 0.0)) return fail(c, FSI_E_ARGUMENT, "delta_normal must be strictly positive.");
+        break;
+    case FSI_PHASE_ACDI:  // PhaseFieldMethods.jl:58-91
+        if (!(m.acdi_gamma >= 0.0) || !std::isfinite(m.acdi_gamma))
+            return fail(c, FSI_E_ARGUMENT, "gamma must be finite and non-negative.");
+        if (!(m.acdi_epsilon > 0.0) || !std::isfinite(m.acdi_epsilon))
+            return fail(c, FSI_E_ARGUMENT, "epsilon must be finite and strictly positive.");
+        if (!(m.acdi_psi_floor > 0.0 && m.acdi_psi_floor < 0.5))
+            return fail(c, FSI_E_ARGUMENT, "psi_floor must lie strictly between 0 and 0.5.");
+        if (!(m.acdi_delta_normal > 0.0)) return fail(c, FSI_E_ARGUMENT, "delta_normal must be strictly positive.");
+        break;
+    case FSI_PHASE_UWCDI:  // PhaseFieldMethods.jl:100-119
+        if (!(m.uwcdi_alpha >= 1.0) || !std::isfinite(m.uwcdi_alpha))
+            return fail(c, FSI_E_ARGUMENT, "alpha must be finite and >= 1 for the published UWCDI stability range.");
+        if (!(m.uwcdi_delta_normal > 0.0)) return fail(c, FSI_E_ARGUMENT, "delta_normal must be strictly positive.");
+        break;
+    default: return fail(c, FSI_E_UNSUPPORTED, "phase method %d is not supported on the device.", m.phase_method);
+    }
+    if (m.viscosity_kind == FSI_VISC_CONSTANT) {
+        if (!(m.viscosity > 0.0) || !std::isfinite(m.viscosity))
+            return fail(c, FSI_E_ARGUMENT, "Viscosity must be strictly positive.");
+        // _require_supported_viscous_boundaries, Operators.jl:91-107
+        const fsi_bc_side* vs[8] = {&c->bcs.U.left, &c->bcs.U.right, &c->bcs.U.bottom, &c->bcs.U.top,
+                                    &c->bcs.V.left, &c->bcs.V.right, &c->bcs.V.bottom, &c->bcs.V.top};
+        for (int q = 0; q < 8; ++q)
+            if (vs[q]->kind == FSI_BC_NEUMANN)
+                return fail(c, FSI_E_ARGUMENT,
+                            "%s-%s uses Neumann. The current compact viscous operator supports Dirichlet or "
+                            "Periodic velocity boundaries.", q < 4 ? "U" : "V", side_name(q & 3));
+    } else if (m.viscosity_kind != FSI_VISC_NONE)
+        return fail(c, FSI_E_UNSUPPORTED, "viscosity model %d is not supported on the device.", m.viscosity_kind);
+    if (m.elastic_kind == FSI_ELASTIC_LINEAR) {
+        if (!(m.shear_modulus >= 0.0) || !std::isfinite(m.shear_modulus))
+            return fail(c, FSI_E_ARGUMENT, "Elastic-stress coefficient must be finite and non-negative.");
+    } else if (m.elastic_kind != FSI_ELASTIC_NONE)
+        return fail(c, FSI_E_UNSUPPORTED, "elastic model %d is not supported on the device.", m.elastic_kind);
+    if (m.relaxation_kind == FSI_RELAX_MAXWELL) {
+        if (!(m.relaxation_rate >= 0.0) || !std::isfinite(m.relaxation_rate))
+            return fail(c, FSI_E_ARGUMENT, "Maxwell relaxation rate must be finite and non-negative.");
+    } else if (m.relaxation_kind != FSI_RELAX_NONE)
+        return fail(c, FSI_E_UNSUPPORTED, "relaxation model %d is not supported on the device.", m.relaxation_kind);
+    if (m.body_force_kind == FSI_FORCE_KOLMOGOROV) {
+        if (!(m.force_wavenumber > 0.0)) return fail(c, FSI_E_ARGUMENT, "Kolmogorov-force wavenumber must be positive.");
+    } else if (m.body_force_kind != FSI_FORCE_NONE && m.body_force_kind != FSI_FORCE_CONSTANT)
+        return fail(c, FSI_E_UNSUPPORTED, "body force %d is not supported on the device.", m.body_force_kind);
+    const fsi_numerics& n = c->num;
+    if (n.extension_kind == FSI_EXT_GEOMETRIC) {  // StructureTensor.jl:106-131
+        if (!(n.ext_max_rate >= 0.0) || !std::isfinite(n.ext_max_rate))
+            return fail(c, FSI_E_ARGUMENT, "Geometric extension rate must be non-negative.");
+        if (!(n.ext_exponent > 0.0) || !std::isfinite(n.ext_exponent))
+            return fail(c, FSI_E_ARGUMENT, "Geometric extension exponent must be positive.");
+        if (!(n.ext_cutoff >= 0.0 && n.ext_cutoff < 1.0))
+            return fail(c, FSI_E_ARGUMENT, "Geometric extension safety cutoff must lie in [0, 1).");
+    } else if (n.extension_kind == FSI_EXT_IDENTITY) {
+        if (!(n.ext_cutoff >= 0.0 && n.ext_cutoff <= 1.0))
+            return fail(c, FSI_E_ARGUMENT, "Structure-extension cutoff must lie in [0, 1].");
+    } else if (n.extension_kind != FSI_EXT_NONE)
+        return fail(c, FSI_E_UNSUPPORTED, "extension model %d is not supported on the device.", n.extension_kind);
+    if (!(n.projection_rtol > 0.0)) return fail(c, FSI_E_ARGUMENT, "projection_reltol must be positive.");
+    if (n.projection_maxiter <= 0) return fail(c, FSI_E_ARGUMENT, "projection_maxiter must be positive.");
+    if (!(n.velocity_rtol > 0.0)) return fail(c, FSI_E_ARGUMENT, "Linear-solver reltol must be positive.");
+    if (n.velocity_maxiter <= 0) return fail(c, FSI_E_ARGUMENT, "Linear-solver maxiter must be positive.");
+    if (!(n.stage1_atol >= 0.0) || !(n.stage1_rtol >= 0.0))
+        return fail(c, FSI_E_ARGUMENT, "stage-1 constraint tolerances must be non-negative.");
+    if (n.precision != FSI_PRECISION_FP64)
+        return fail(c, FSI_E_UNSUPPORTED, "only the FP64 precision mode is available on the device.");
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bcs, const fsi_models* models,
+                                         const fsi_numerics* numerics, fsi_context** out) {
+    if (!out) return FSI_E_ARGUMENT;
+    *out = nullptr;
+    fsi_context* c = new fsi_context();
+    *out = c;
+    if (!grid || !bcs || !models || !numerics) return fail(c, FSI_E_ARGUMENT, "null configuration pointer");
+    c->grid = *grid;
+    c->bcs = *bcs;
+    c->models = *models;
+    c->num = *numerics;
+    fsi_status rc = validate(c);
+    if (rc != FSI_OK) return rc;
+
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(c, FSI_E_HIP, "no HIP device is available");
+    HIPCHECK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    const int nx = grid->nx, ny = grid->ny;
+    c->P = ((nx + 1 + 31) / 32) * 32;
+    c->felems = (size_t)(ny + 1) * c->P;
+    Params& p = c->prm;
+    p.nx = nx;
+    p.ny = ny;
+    p.P = c->P;
+    p.px = grid->periodic_x != 0;
+    p.py = grid->periodic_y != 0;
+    // generate_grid(:uniform): h = L/N on both the face and the centre grids
+    p.hx = grid->lx / nx;
+    p.hy = grid->ly / ny;
+    p.ihx = 1.0 / p.hx;
+    p.ihy = 1.0 / p.hy;
+    p.bU = to_bcset(bcs->U);
+    p.bV = to_bcset(bcs->V);
+    p.bQ = to_bcset(bcs->correction);
+    p.bPhi = to_bcset(bcs->phi);
+    p.bSxx = to_bcset(bcs->sigma_xx);
+    p.bSyy = to_bcset(bcs->sigma_yy);
+    p.bSxy = to_bcset(bcs->sigma_xy);
+    const fsi_models& m = *models;
+    p.phase = m.phase_method;
+    p.cdi_G = m.cdi_inverse_peclet;
+    p.cdi_eps = m.cdi_epsilon;
+    p.cdi_d2 = m.cdi_delta_normal * m.cdi_delta_normal;
+    p.acdi_gamma = m.acdi_gamma;
+    p.acdi_eps = m.acdi_epsilon;
+    p.acdi_floor = m.acdi_psi_floor;
+    p.acdi_d2 = m.acdi_delta_normal * m.acdi_delta_normal;
+    p.acdi_dyn = m.acdi_dynamic_gamma;
+    p.uw_alpha = m.uwcdi_alpha;
+    p.uw_d2 = m.uwcdi_delta_normal * m.uwcdi_delta_normal;
+    p.visc = m.viscosity_kind == FSI_VISC_CONSTANT;
+    p.mu = m.viscosity;
+    p.elastic = m.elastic_kind == FSI_ELASTIC_LINEAR;
+    p.Gmod = m.shear_modulus;
+    p.relax = m.relaxation_kind == FSI_RELAX_MAXWELL;
+    p.relax_rate = m.relaxation_rate;
+    p.ext = numerics->extension_kind;
+    p.ext_rate = numerics->ext_max_rate;
+    p.ext_exp = numerics->ext_exponent;
+    p.ext_cutoff = numerics->ext_cutoff;
+    c->has_reset = (p.ext == FSI_EXT_IDENTITY) || (p.ext == FSI_EXT_GEOMETRIC && p.ext_cutoff > 0.0);
+    p.force = m.body_force_kind;
+    p.ffx = m.force_fx;
+    p.ffy = m.force_fy;
+    // ACDI psi boundary setup (_psi_boundary_setup, PhaseFieldMethods.jl:493-529)
+    c->bpsi = p.bPhi;
+    BcSide* ps[4] = {&c->bpsi.l, &c->bpsi.r, &c->bpsi.b, &c->bpsi.t};
+    for (int q = 0; q < 4; ++q) {
+        if (ps[q]->kind == FSI_BC_NEUMANN) {
+            if (p.phase == FSI_PHASE_ACDI && std::fabs(ps[q]->value) > 100 * 2.220446049250313e-16)
+                return fail(c, FSI_E_ARGUMENT, "ACDI currently supports only homogeneous Neumann phase gradients.");
+            ps[q]->value = 0.0;
+        } else if (ps[q]->kind == FSI_BC_DIRICHLET) {
+            double q0 = std::min(std::max(ps[q]->value, 0.0), 1.0), e = m.acdi_psi_floor;
+            ps[q]->value = m.acdi_epsilon * std::log((q0 + e) / (1.0 - q0 + e));
+        }
+    }
+    const fsi_bc_setup& bq = bcs->correction;
+    c->has_null = !(bq.left.kind == FSI_BC_DIRICHLET || bq.right.kind == FSI_BC_DIRICHLET ||
+                    bq.bottom.kind == FSI_BC_DIRICHLET || bq.top.kind == FSI_BC_DIRICHLET);
+
+    // constant device data
+    std::vector<double> host((size_t)ny, 0.0);
+    HIPCHECK(c, dalloc(c, &c->force_row, (size_t)ny));
+    if (m.body_force_kind == FSI_FORCE_KOLMOGOROV) {  // TransportConstitutive.jl:385-397
+        double y = 0.0;
+        for (int j = 0; j < ny; ++j) {
+            y += 0.5 * p.hy;
+            host[j] = m.force_amplitude * std::sin(m.force_wavenumber * y + m.force_phase);
+            y += 0.5 * p.hy;
+        }
+        HIPCHECK(c, hipMemcpyAsync(c->force_row, host.data(), ny * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
+    p.force_row = c->force_row;
+    // b_bc of the projection Laplacian: D * gradient_bc (Operators.jl:361-384)
+    {
+        std::vector<double> bc((size_t)nx * ny, 0.0);
+        auto gx = [&](int i, int j) -> double {  // boundary part of the x-face gradient
+            (void)j;
+            if (p.px) return 0.0;
+            if (i == 0) return bq.left.kind == FSI_BC_DIRICHLET ? -bq.left.value / (p.hx / 2.0) : -bq.left.value;
+            if (i == nx) return bq.right.kind == FSI_BC_DIRICHLET ? bq.right.value / (p.hx / 2.0) : bq.right.value;
+            return 0.0;
+        };
+        auto gy = [&](int i, int j) -> double {
+            (void)i;
+            if (p.py) return 0.0;
+            if (j == 0) return bq.bottom.kind == FSI_BC_DIRICHLET ? -bq.bottom.value / (p.hy / 2.0) : -bq.bottom.value;
+            if (j == ny) return bq.top.kind == FSI_BC_DIRICHLET ? bq.top.value / (p.hy / 2.0) : bq.top.value;
+            return 0.0;
+        };
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i)
+                bc[(size_t)j * nx + i] = (gx(i + 1, j) - gx(i, j)) / p.hx + (gy(i, j + 1) - gy(i, j)) / p.hy;
+        HIPCHECK(c, dalloc(c, &c->lap_bc, c->felems));
+        rc = upload2d(c, c->lap_bc, bc.data(), nx, ny);
+        if (rc != FSI_OK) return rc;
+    }
+    HIPCHECK(c, dalloc(c, &c->bvisc_u, c->felems));
+    HIPCHECK(c, dalloc(c, &c->bvisc_v, c->felems));
+    if (p.visc) launch_visc_boundary(c->stream, p, nullptr, c->bvisc_u, c->bvisc_v);
+    // multigrid geometry: halve while both sizes are even, >= 4, and the level
+    // is larger than the single-workgroup coarse solve
+    {
+        MgGeom g{};
+        g.nx = nx;
+        g.ny = ny;
+        g.P = c->P;
+        g.px = p.px;
+        g.py = p.py;
+        g.dl = bq.left.kind == FSI_BC_DIRICHLET;
+        g.dr = bq.right.kind == FSI_BC_DIRICHLET;
+        g.db = bq.bottom.kind == FSI_BC_DIRICHLET;
+        g.dt = bq.top.kind == FSI_BC_DIRICHLET;
+        g.ihx2 = 1.0 / (p.hx * p.hx);
+        g.ihy2 = 1.0 / (p.hy * p.hy);
+        c->geo.push_back(g);
+        while ((int)c->geo.size() < kMaxLevels && (long)g.nx * g.ny > 1024 && !(g.nx & 1) && !(g.ny & 1) &&
+               g.nx >= 8 && g.ny >= 8) {
+            g.nx /= 2;
+            g.ny /= 2;
+            g.P = ((g.nx + 31) / 32) * 32;
+            g.ihx2 *= 0.25;
+            g.ihy2 *= 0.25;
+            c->geo.push_back(g);
+        }
+    }
+    const int nb = grid_blocks(p);
+    HIPCHECK(c, dalloc(c, &c->partials, (size_t)8 * (nb + 64)));
+    HIPCHECK(c, hipMalloc((void**)&c->st, sizeof(Status)));
+    HIPCHECK(c, hipMemsetAsync(c->st, 0, sizeof(Status), c->stream));
+    HIPCHECK(c, hipHostMalloc((void**)&c->st_host, sizeof(Status), hipHostMallocDefault));
+    std::memset(c->st_host, 0, sizeof(Status));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipGetLastError());
+    return FSI_OK;
+}
+
+extern "C" void fsi_context_destroy(fsi_context* c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials};
+    for (double* b : bufs)
+        if (b) (void)hipFree(b);
+    if (c->st) (void)hipFree(c->st);
+    if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char* fsi_last_error(const fsi_context* c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" const char* fsi_version(void) {
+    return "libfsi_hip 0.1 (gfx950, FP64 IMEX step: CG velocity predictor, MG-PCG projection)";
+}
+
+// ---------------------------------------------------------------------------
+// state and workspace
+// ---------------------------------------------------------------------------
+extern "C" fsi_status fsi_state_create(fsi_context* c, fsi_state** out) {
+    if (!c || !out) return FSI_E_ARGUMENT;
+    fsi_state* s = new fsi_state();
+    s->ctx = c;
+    *out = s;
+    double** f[7] = {&s->U, &s->V, &s->phi, &s->res, &s->sxx, &s->syy, &s->sxy};
+    for (auto ptr : f) HIPCHECK(c, dalloc(c, ptr, c->felems));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+
+extern "C" void fsi_state_destroy(fsi_state* s) {
+    if (!s) return;
+    double* f[7] = {s->U, s->V, s->phi, s->res, s->sxx, s->syy, s->sxy};
+    for (double* p : f)
+        if (p) (void)hipFree(p);
+    delete s;
+}
+
+extern "C" fsi_status fsi_state_upload(fsi_context* c, fsi_state* s, const double* U, const double* V,
+                                       const double* phi, const double* resource, const double* sxx,
+                                       const double* syy, const double* sxy) {
+    if (!c || !s) return FSI_E_ARGUMENT;
+    const int nx = c->grid.nx, ny = c->grid.ny;
+    fsi_status rc;
+    if (U && (rc = upload2d(c, s->U, U, nx + 1, ny)) != FSI_OK) return rc;
+    if (V && (rc = upload2d(c, s->V, V, nx, ny + 1)) != FSI_OK) return rc;
+    if (phi && (rc = upload2d(c, s->phi, phi, nx, ny)) != FSI_OK) return rc;
+    if (resource) {
+        if ((rc = upload2d(c, s->res, resource, nx, ny)) != FSI_OK) return rc;
+        // the resource is not transported under NoGrowth: its minimum and
+        // finiteness are fixed at upload (StepReport.minimum_resource)
+        double mn = INFINITY;
+        long long bad = 0;
+        for (size_t k = 0; k < (size_t)nx * ny; ++k) {
+            if (!std::isfinite(resource[k])) ++bad;
+            mn = std::fmin(mn, resource[k]);
+        }
+        s->res_min = mn;
+        s->res_nonfinite = bad;
+    }
+    if (sxx && (rc = upload2d(c, s->sxx, sxx, nx, ny)) != FSI_OK) return rc;
+    if (syy && (rc = upload2d(c, s->syy, syy, nx, ny)) != FSI_OK) return rc;
+    if (sxy && (rc = upload2d(c, s->sxy, sxy, nx, ny)) != FSI_OK) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_state_download(fsi_context* c, const fsi_state* s, double* U, double* V, double* phi,
+                                         double* resource, double* sxx, double* syy, double* sxy) {
+    if (!c || !s) return FSI_E_ARGUMENT;
+    const int nx = c->grid.nx, ny = c->grid.ny;
+    fsi_status rc;
+    if (U && (rc = download2d(c, U, s->U, nx + 1, ny)) != FSI_OK) return rc;
+    if (V && (rc = download2d(c, V, s->V, nx, ny + 1)) != FSI_OK) return rc;
+    if (phi && (rc = download2d(c, phi, s->phi, nx, ny)) != FSI_OK) return rc;
+    if (resource && (rc = download2d(c, resource, s->res, nx, ny)) != FSI_OK) return rc;
+    if (sxx && (rc = download2d(c, sxx, s->sxx, nx, ny)) != FSI_OK) return rc;
+    if (syy && (rc = download2d(c, syy, s->syy, nx, ny)) != FSI_OK) return rc;
+    if (sxy && (rc = download2d(c, sxy, s->sxy, nx, ny)) != FSI_OK) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) {
+    if (!c || !out) return FSI_E_ARGUMENT;
+    fsi_workspace* w = new fsi_workspace();
+    w->ctx = c;
+    *out = w;
+    auto A = [&](double** p, size_t n) -> hipError_t {
+        hipError_t e = dalloc(c, p, n);
+        if (e == hipSuccess) w->owned.push_back(*p);
+        return e;
+    };
+    const size_t F = c->felems;
+    double** singles[] = {&w->Us, &w->Vs, &w->phis, &w->sxxs, &w->syys, &w->sxys, &w->ru,  &w->rv,
+                          &w->r_u, &w->r_v, &w->p_u[0], &w->p_u[1], &w->p_v[0], &w->p_v[1], &w->q_u,
+                          &w->q_v, &w->rhs, &w->corr[0], &w->corr[1], &w->r, &w->z, &w->pp[0], &w->pp[1], &w->w};
+    for (auto p : singles) HIPCHECK(c, A(p, F));
+    for (int s = 0; s < 3; ++s) {
+        double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
+        for (auto p : h) HIPCHECK(c, A(p, F));
+    }
+    // multigrid levels: level 0 solves into z with b = r; x and r are scratch
+    w->lv.resize(c->geo.size());
+    for (size_t l = 0; l < c->geo.size(); ++l) {
+        const MgGeom& g = c->geo[l];
+        MgLevel& L = w->lv[l];
+        L.nx = g.nx;
+        L.ny = g.ny;
+        L.P = g.P;
+        const size_t n = (size_t)g.P * (g.ny + 1);
+        HIPCHECK(c, A(&L.x, n));
+        HIPCHECK(c, A(&L.r, n));
+        if (l == 0) {
+            L.t = w->z;
+            L.b = w->r;
+        } else {
+            HIPCHECK(c, A(&L.t, n));
+            HIPCHECK(c, A(&L.b, n));
+        }
+    }
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+
+extern "C" void fsi_workspace_destroy(fsi_workspace* w) {
+    if (!w) return;
+    for (double* p : w->owned) (void)hipFree(p);
+    delete w;
+}
+
+extern "C" fsi_status fsi_workspace_download_correction(fsi_context* c, const fsi_workspace* w, double* corr) {
+    if (!c || !w || !corr) return FSI_E_ARGUMENT;
+    fsi_status rc = download2d(c, corr, w->corr[w->corr_cur], c->grid.nx, c->grid.ny);
+    if (rc != FSI_OK) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_workspace_upload_correction(fsi_context* c, fsi_workspace* w, const double* corr) {
+    if (!c || !w || !corr) return FSI_E_ARGUMENT;
+    for (size_t k = 0; k < (size_t)c->grid.nx * c->grid.ny; ++k)
+        if (!std::isfinite(corr[k])) return fail(c, FSI_E_ARGUMENT, "correction_guess contains non-finite values.");
+    fsi_status rc = upload2d(c, w->corr[w->corr_cur], corr, c->grid.nx, c->grid.ny);
+    if (rc != FSI_OK) return rc;
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// step pieces
+// ---------------------------------------------------------------------------
+static void status_reset(fsi_context* c) {
+    Status& s = *c->st_host;
+    std::memset(&s, 0, sizeof(Status));
+    s.phi_min = INFINITY;
+    s.phi_max = -INFINITY;
+    s.min_eig = INFINITY;
+}
+
+static fsi_status status_push(fsi_context* c) {
+    HIPCHECK(c, hipMemcpyAsync(c->st, c->st_host, sizeof(Status), hipMemcpyHostToDevice, c->stream));
+    // the pinned buffer is rewritten by the next status_pull; wait for the copy
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+static fsi_status status_pull(fsi_context* c) {
+    HIPCHECK(c, hipMemcpyAsync(c->st_host, c->st, sizeof(Status), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    HIPCHECK(c, hipGetLastError());
+    return FSI_OK;
+}
+
+// Poll the `done` flag of one Krylov solve (a small device->host copy).
+static fsi_status poll_done(fsi_context* c, const KrylovScalars* dev, int* done, int* iter) {
+    int buf[2];
+    HIPCHECK(c, hipMemcpyAsync(buf, &dev->done, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    *done = buf[0];
+    *iter = buf[1];
+    return FSI_OK;
+}
+
+static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot) {
+    const Params& p = c->prm;
+    VelBufs b{};
+    b.Un = s->U;
+    b.Vn = s->V;
+    b.ru = w->ru;
+    b.rv = w->rv;
+    b.xu = w->Us;
+    b.xv = w->Vs;
+    b.r_u = w->r_u;
+    b.r_v = w->r_v;
+    b.p_u[0] = w->p_u[0];
+    b.p_u[1] = w->p_u[1];
+    b.p_v[0] = w->p_v[0];
+    b.p_v[1] = w->p_v[1];
+    b.q_u = w->q_u;
+    b.q_v = w->q_v;
+    b.partials = c->partials;
+    if (!p.visc) {
+        launch_vel_copy(c->stream, p, b);
+        return FSI_OK;
+    }
+    launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter);
+    int total = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
+    while (total < c->num.velocity_maxiter) {
+        int n = std::min(chunk, c->num.velocity_maxiter - total);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, n, slot);
+        total += n;
+        fsi_status rc = poll_done(c, &c->st->vel, &done, &iter);
+        if (rc != FSI_OK) return rc;
+        if (done) break;
+        chunk = 4;
+    }
+    c->last_vel_iters[slot] = std::max(1, iter);
+    return FSI_OK;
+}
+
+static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double* V, int inject, int slot) {
+    const Params& p = c->prm;
+    ProjBufs b{};
+    b.U = U;
+    b.V = V;
+    b.inject = inject;
+    b.lap_bc = c->lap_bc;
+    b.rhs = w->rhs;
+    b.x = w->corr[w->corr_cur];
+    b.q_out = w->corr[1 - w->corr_cur];
+    b.r = w->r;
+    b.z = w->z;
+    b.p[0] = w->pp[0];
+    b.p[1] = w->pp[1];
+    b.w = w->w;
+    b.partials = c->partials;
+    launch_proj_rhs(c->stream, p, b, c->st, c->has_null, c->num.projection_rtol, c->num.projection_maxiter);
+    launch_pcg_init(c->stream, p, c->geo[0], b, c->st);
+    const int ps = slot < 0 ? 0 : slot;
+    int total = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
+    while (total < c->num.projection_maxiter) {
+        int n = std::min(chunk, c->num.projection_maxiter - total);
+        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->st, n, slot);
+        total += n;
+        fsi_status rc = poll_done(c, &c->st->pcg, &done, &iter);
+        if (rc != FSI_OK) return rc;
+        if (done) break;
+        chunk = 4;
+    }
+    c->last_proj_iters[ps] = std::max(1, iter);
+    launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
+    w->corr_cur = 1 - w->corr_cur;
+    return FSI_OK;
+}
+
+static fsi_status device_error(fsi_context* c) {
+    const Status& s = *c->st_host;
+    switch (s.code) {
+    case DEV_OK: return FSI_OK;
+    case DEV_NEUMANN_INFLOW:
+        return fail(c, FSI_E_ARGUMENT,
+                    "A Neumann condition does not define the incoming advected state (face velocity %.6e). "
+                    "Prescribe Dirichlet data for inflow.", s.v0);
+    case DEV_NONSPD:
+        return fail(c, FSI_E_DOMAIN,
+                    "GeometricStructureExtension requires an SPD tensor; encountered a non-positive or "
+                    "non-finite eigenvalue (sigma_xx = %.6e, sigma_xy = %.6e).", s.v0, s.v1);
+    case DEV_INCOMPATIBLE:
+        return fail(c, FSI_E_ARGUMENT,
+                    "The prescribed divergence constraint is incompatible with the Neumann/periodic correction "
+                    "boundaries. The volume integral of the Poisson right-hand side is %.6e, while the "
+                    "compatibility tolerance is %.6e.", s.v0, s.v1);
+    case DEV_PROJ_RESIDUAL:
+        return fail(c, FSI_E_SOLVER,
+                    "Projection Poisson solve did not reach the required true-residual accuracy. Relative "
+                    "residual = %.6e, tolerance = %.1e.", s.v0, s.v1);
+    case DEV_PROJ_CONSTRAINT:
+        return fail(c, FSI_E_SOLVER,
+                    "Projection did not satisfy the prescribed divergence constraint. Weighted L2 residual = "
+                    "%.6e, tolerance = %.6e.", s.v0, s.v1);
+    case DEV_NONHOMOG_NEUMANN:
+        return fail(c, FSI_E_ARGUMENT,
+                    "Only homogeneous Neumann phase boundaries are supported by the ACDI/UWCDI operators.");
+    case DEV_VEL_NOCONV:
+        return fail(c, FSI_E_SOLVER, "velocity predictor CG residual is too large: %.6e after %d iterations",
+                    s.v0, (int)s.v1);
+    default: return fail(c, FSI_E_HIP, "device reported error code %d", s.code);
+    }
+}
+
+static void fill_projection_result(const fsi_context* c, fsi_projection_result* r) {
+    if (!r) return;
+    const Status& s = *c->st_host;
+    r->linear_relative_residual = s.proj.lin_rel;
+    r->constraint_linf = s.proj.linf;
+    r->constraint_l2 = s.proj.l2;
+    r->compatibility_defect = s.proj.defect;
+    r->iterations = s.pcg.iter;
+}
+
+static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, fsi_projection_result* res) {
+    if (!c || !s || !w) return FSI_E_ARGUMENT;
+    status_reset(c);
+    fsi_status rc = status_push(c);
+    if (rc != FSI_OK) return rc;
+    const int saved = w->corr_cur;
+    rc = projection(c, w, s->U, s->V, 0, -1);
+    if (rc != FSI_OK) return rc;
+    if ((rc = status_pull(c)) != FSI_OK) return rc;
+    fill_projection_result(c, res);
+    rc = device_error(c);
+    if (rc != FSI_OK) w->corr_cur = saved;
+    return rc;
+}
+
+extern "C" fsi_status fsi_make_velocity_compatible(fsi_context* c, fsi_state* s, fsi_workspace* w,
+                                                   fsi_projection_result* res) {
+    // growth_source!(NoGrowth) gives G = 0 (GrowthModels.jl:214-223)
+    return project_state(c, s, w, res);
+}
+
+extern "C" fsi_status fsi_project(fsi_context* c, fsi_state* s, fsi_workspace* w, fsi_projection_result* res) {
+    return project_state(c, s, w, res);
+}
+
+static void tendencies(fsi_context* c, fsi_workspace* w, int slot, const double* U, const double* V,
+                       const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
+                       int stage1) {
+    const Params& p = c->prm;
+    launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, w->Eu[slot], w->Ev[slot], w->Iu[slot],
+                    w->Iv[slot]);
+    launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], w->Exx[slot], w->Eyy[slot],
+                 w->Exy[slot], c->st, stage1);
+}
+
+extern "C" fsi_status fsi_eval_tendencies(fsi_context* c, const fsi_state* s, fsi_workspace* w, double t,
+                                          double* E_u, double* E_v, double* I_u, double* I_v, double* E_phi,
+                                          double* E_sxx, double* E_syy, double* E_sxy) {
+    if (!c || !s || !w) return FSI_E_ARGUMENT;
+    (void)t;  // the supported body forces are steady
+    status_reset(c);
+    fsi_status rc = status_push(c);
+    if (rc != FSI_OK) return rc;
+    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, c->has_reset, 0);
+    const int nx = c->grid.nx, ny = c->grid.ny;
+    if (E_u && (rc = download2d(c, E_u, w->Eu[0], nx, ny)) != FSI_OK) return rc;
+    if (E_v && (rc = download2d(c, E_v, w->Ev[0], nx, ny)) != FSI_OK) return rc;
+    if (I_u && (rc = download2d(c, I_u, w->Iu[0], nx, ny)) != FSI_OK) return rc;
+    if (I_v && (rc = download2d(c, I_v, w->Iv[0], nx, ny)) != FSI_OK) return rc;
+    if (E_phi && (rc = download2d(c, E_phi, w->Ep[0], nx, ny)) != FSI_OK) return rc;
+    if (E_sxx && (rc = download2d(c, E_sxx, w->Exx[0], nx, ny)) != FSI_OK) return rc;
+    if (E_syy && (rc = download2d(c, E_syy, w->Eyy[0], nx, ny)) != FSI_OK) return rc;
+    if (E_sxy && (rc = download2d(c, E_sxy, w->Exy[0], nx, ny)) != FSI_OK) return rc;
+    if ((rc = status_pull(c)) != FSI_OK) return rc;
+    return device_error(c);
+}
+
+extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double t, double dt,
+                                         fsi_step_report* rep, fsi_step_stats* stats) {
+    if (!c || !s || !w) return FSI_E_ARGUMENT;
+    if (s->res_nonfinite) return fail(c, FSI_E_ARGUMENT, "state.resource contains non-finite values.");
+    if (!std::isfinite(t)) return fail(c, FSI_E_ARGUMENT, "t must be finite.");
+    if (!std::isfinite(dt)) return fail(c, FSI_E_ARGUMENT, "dt must be finite.");
+    if (!(dt > 0.0)) return fail(c, FSI_E_ARGUMENT, "dt must be strictly positive.");
+    const Params& p = c->prm;
+    // _inject_compact_predictor! requirements (:556-578)
+    if (!p.px && c->bcs.U.right.kind != FSI_BC_DIRICHLET)
+        return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic U-right boundary.");
+    if (!p.py && c->bcs.V.top.kind != FSI_BC_DIRICHLET)
+        return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic V-top boundary.");
+
+    status_reset(c);
+    fsi_status rc = status_push(c);
+    if (rc != FSI_OK) return rc;
+    const int saved_corr = w->corr_cur;
+    const int reset = c->has_reset;
+    // stage 1: the state itself (with the safety reset on sigma_n), tendencies
+    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1);
+    for (int si = 1; si < 4; ++si) {
+        // phi_s, sigma_s (_build_explicit_stage!)
+        double coef[3] = {0, 0, 0};
+        for (int q = 0; q < si; ++q) coef[q] = dt * AE[si][q];
+        launch_combine_cells(c->stream, p, s->phi, s->sxx, s->syy, s->sxy, reset, w->Ep, w->Exx, w->Eyy, w->Exy, si,
+                             coef, w->phis, w->sxxs, w->syys, w->sxys, c->st, si == 3);
+        // velocity right-hand side and predictor (_build_velocity_stage!)
+        double ce[3] = {0, 0, 0}, ci[3] = {0, 0, 0};
+        for (int q = 0; q < si; ++q) {
+            ce[q] = AE[si][q];
+            ci[q] = AI[si][q];
+        }
+        const double alpha = dt * AI[si][si];
+        launch_combine_faces(c->stream, p, s->U, s->V, w->Eu, w->Ev, w->Iu, w->Iv, si, dt, ce, ci, alpha,
+                             p.visc ? c->bvisc_u : nullptr, p.visc ? c->bvisc_v : nullptr, w->ru, w->rv);
+        if ((rc = velocity_solve(c, s, w, alpha, si - 1)) != FSI_OK) return rc;
+        if ((rc = projection(c, w, w->Us, w->Vs, 1, si - 1)) != FSI_OK) return rc;
+        if (si <= 2)
+            tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0);
+    }
+    if ((rc = status_pull(c)) != FSI_OK) return rc;
+    const Status& st = *c->st_host;
+    // error priority follows the reference's order of checks
+    if (st.nonfinite_in) {
+        w->corr_cur = saved_corr;
+        return fail(c, FSI_E_ARGUMENT, "the state contains non-finite values.");
+    }
+    if (c->num.check_stage1) {  // _check_stage_one_constraint! :1116-1127
+        double tol = c->num.stage1_atol + c->num.stage1_rtol * 1.0;
+        if (!(st.stage1_linf <= tol)) {
+            w->corr_cur = saved_corr;
+            return fail(c, FSI_E_ARGUMENT,
+                        "The velocity entering stage 1 does not satisfy the prescribed divergence constraint. "
+                        "Maximum |div(v)-G| = %.6e, tolerance = %.6e. Initialize the state with "
+                        "make_velocity_compatible! or provide a compatible velocity.", st.stage1_linf, tol);
+        }
+    }
+    if ((rc = device_error(c)) != FSI_OK) {
+        w->corr_cur = saved_corr;
+        return rc;
+    }
+    // the fourth stage is the new state (:1232-1238): swap buffers
+    std::swap(s->U, w->Us);
+    std::swap(s->V, w->Vs);
+    std::swap(s->phi, w->phis);
+    std::swap(s->sxx, w->sxxs);
+    std::swap(s->syy, w->syys);
+    std::swap(s->sxy, w->sxys);
+    if (rep) {
+        rep->maximum_projection_constraint_linf = std::fmax(st.stage1_linf, st.max_constraint_linf);
+        rep->maximum_growth_trace_error = 0.0;
+        rep->minimum_phi = st.phi_min;
+        rep->maximum_phi = st.phi_max;
+        rep->minimum_resource = s->res_min;
+        rep->minimum_sigma_eigenvalue = st.min_eig;
+        rep->maximum_abs_sigma_determinant_error = st.max_det_err;
+        rep->non_spd_cells = st.nonspd;
+    }
+    if (stats)
+        for (int k = 0; k < 3; ++k) {
+            stats->velocity_iterations[k] = st.vel_iters[k];
+            stats->projection_iterations[k] = st.proj_iters[k];
+        }
+    return FSI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// measurement hook
+// ---------------------------------------------------------------------------
+extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_workspace* w, const char* name,
+                                           int32_t reps, double* avg_ms, double* bytes) {
+    if (!c || !s || !w || !name || reps <= 0) return FSI_E_ARGUMENT;
+    const Params& p = c->prm;
+    VelBufs vb{};
+    vb.Un = s->U;
+    vb.Vn = s->V;
+    vb.ru = w->ru;
+    vb.rv = w->rv;
+    vb.xu = w->Us;
+    vb.xv = w->Vs;
+    vb.r_u = w->r_u;
+    vb.r_v = w->r_v;
+    vb.p_u[0] = w->p_u[0];
+    vb.p_u[1] = w->p_u[1];
+    vb.p_v[0] = w->p_v[0];
+    vb.p_v[1] = w->p_v[1];
+    vb.q_u = w->q_u;
+    vb.q_v = w->q_v;
+    vb.partials = c->partials;
+    ProjBufs pb{};
+    pb.U = w->Us;
+    pb.V = w->Vs;
+    pb.lap_bc = c->lap_bc;
+    pb.rhs = w->rhs;
+    pb.x = w->corr[0];
+    pb.q_out = w->corr[1];
+    pb.r = w->r;
+    pb.z = w->z;
+    pb.p[0] = w->pp[0];
+    pb.p[1] = w->pp[1];
+    pb.w = w->w;
+    pb.partials = c->partials;
+    // keep every kernel running (done = 0) on an iteration > 0 state
+    status_reset(c);
+    c->st_host->vel.iter = 1;
+    c->st_host->pcg.iter = 1;
+    c->st_host->vel.rr = c->st_host->pcg.rz = 1.0;
+    fsi_status rc = status_push(c);
+    if (rc != FSI_OK) return rc;
+    const std::string n(name);
+    auto launch = [&]() -> double {
+        if (n == "momentum") {
+            launch_momentum(c->stream, p, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, c->has_reset, w->Eu[0],
+                            w->Ev[0], w->Iu[0], w->Iv[0]);
+            return 10.0 * 8.0 * p.nx * (double)p.ny;
+        }
+        if (n == "cells") {
+            launch_cells(c->stream, p, c->bpsi, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, c->has_reset,
+                         w->Ep[0], w->Exx[0], w->Eyy[0], w->Exy[0], c->st, 0);
+            return 10.0 * 8.0 * p.nx * (double)p.ny;
+        }
+        return bench_launch(c->stream, name, p, vb, pb, w->lv.data(), c->geo.data(), c->st);
+    };
+    double b = launch();
+    if (b < 0) return fail(c, FSI_E_ARGUMENT, "unknown kernel name %s", name);
+    if (c->geo.size() < 2 && (n == "mg_prolong" || n == "mg_restrict"))
+        return fail(c, FSI_E_ARGUMENT, "the grid has a single multigrid level");
+    hipEvent_t e0, e1;
+    HIPCHECK(c, hipEventCreate(&e0));
+    HIPCHECK(c, hipEventCreate(&e1));
+    HIPCHECK(c, hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) launch();
+    HIPCHECK(c, hipEventRecord(e1, c->stream));
+    HIPCHECK(c, hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHECK(c, hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    HIPCHECK(c, hipGetLastError());
+    if (avg_ms) *avg_ms = (double)ms / reps;
+    if (bytes) *bytes = b;
+    return FSI_OK;
+}

@@ -1,0 +1,174 @@
+// fsi_device.hpp -- device-side data layout and shared helpers of libfsi_hip.
+//
+// HBM layout: every field is stored row-major in y with x contiguous (the
+// reference's column-major (i,j) order), padded to a pitch P that is a
+// multiple of 32 doubles (256 B), so that each grid row starts on a 256-byte
+// boundary and a 64-lane wavefront reads 512 contiguous bytes.  All fields of
+// a grid share one allocation size (ny+1)*P so x-faces (nx+1 columns), y-faces
+// (ny+1 rows) and cells use the same indexing at(i,j) = j*P + i.
+//
+// The compact velocity vector of the implicit solve, uv = [U(1:nx,1:ny);
+// V(1:nx,1:ny)] (src/TimeIntegrationIMEX.jl:529-541), is stored in place in
+// the U / V arrays: the compact u view ignores column nx, the compact v view
+// ignores row ny.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/fsi_api.h"
+
+namespace fsi {
+
+constexpr int TILE_X = 64;         // threads along x (one wavefront row)
+constexpr int TILE_Y = 4;          // thread rows per block
+constexpr int ROWS = 4;            // grid rows processed per thread
+constexpr int BLOCK = TILE_X * TILE_Y;
+constexpr int TILE_ROWS = TILE_Y * ROWS;  // 16 grid rows per block
+
+struct BcSide {
+    int kind;      // fsi_bc_kind
+    double value;
+};
+struct BcSet {
+    BcSide l, r, b, t;  // left, right, bottom, top
+};
+
+// Parameters passed by value to every physics kernel.
+struct Params {
+    int nx, ny, P;
+    int px, py;
+    double hx, hy, ihx, ihy;
+    BcSet bU, bV, bQ, bPhi, bSxx, bSyy, bSxy;
+    // phase field
+    int phase;
+    double cdi_G, cdi_eps, cdi_d2;
+    double acdi_gamma, acdi_eps, acdi_floor, acdi_d2;
+    int acdi_dyn;
+    double uw_alpha, uw_d2;
+    // closures
+    int visc;
+    double mu;
+    int elastic;
+    double Gmod;
+    int relax;
+    double relax_rate;
+    int ext;
+    double ext_rate, ext_exp, ext_cutoff;
+    int force;
+    double ffx, ffy;
+    const double* force_row;  // KolmogorovBodyForce per-row value (ny), or null
+};
+
+// Error codes raised from kernels into Status::code (first error wins).
+enum DevError : int {
+    DEV_OK = 0,
+    DEV_NEUMANN_INFLOW = 1,      // ArgumentError: Neumann BC on an inflow face
+    DEV_NONSPD = 2,              // DomainError: geometric extension of non-SPD sigma
+    DEV_NONFINITE_STATE = 3,     // ArgumentError: non-finite state entry
+    DEV_STAGE1 = 4,              // ArgumentError: stage-1 divergence check
+    DEV_INCOMPATIBLE = 5,        // ArgumentError: projection compatibility
+    DEV_PROJ_RESIDUAL = 6,       // ErrorException: projection true residual
+    DEV_PROJ_CONSTRAINT = 7,     // ErrorException: projection constraint
+    DEV_NONHOMOG_NEUMANN = 8,    // ArgumentError: non-homogeneous Neumann phase BC
+    DEV_VEL_NOCONV = 9           // ErrorException: velocity predictor solve not converged
+};
+
+// Scalars of one Krylov solve, resident in device memory.
+struct KrylovScalars {
+    double rr, rz, rz_old, pq, alpha, beta, bnorm2, tol2;
+    int done, iter, maxiter, pad;
+};
+
+// Projection scalars (src/Projection.jl:170-307).
+struct ProjScalars {
+    double defect, scale, mean_rhs, rhs2, pdiv_l2sq, mean_q;
+    double resid2, linf, l2sq;
+    double lin_rel, l2;
+};
+
+// Everything the host reads back once per step.
+struct Status {
+    int code;        // DevError of the first failure
+    int pad0;
+    double v0, v1;   // values describing the first failure
+    // StepReport accumulators (src/TimeIntegrationIMEX.jl:318-327)
+    double max_constraint_linf;
+    double phi_min, phi_max;
+    double min_eig, max_det_err;
+    long long nonspd, nonfinite;
+    long long nonfinite_in;  // non-finite entries of the input state (stage 1)
+    double stage1_linf;      // max |div(v) - G| of the input velocity (stage 1)
+    double gamma_umax;       // dynamic ACDI gamma: max |u_c|
+    int vel_iters[3], proj_iters[3];
+    KrylovScalars vel;
+    KrylovScalars pcg;
+    ProjScalars proj;
+    unsigned int counters[16];  // last-block arrival counters
+};
+
+__device__ __forceinline__ size_t at(const Params& p, int i, int j) { return (size_t)j * p.P + i; }
+
+__device__ __forceinline__ void raise_error(Status* st, int code, double v0, double v1) {
+    if (atomicCAS(&st->code, 0, code) == 0) {
+        st->v0 = v0;
+        st->v1 = v1;
+    }
+}
+
+// ---- deterministic block reduction (wave64 shuffles, then LDS in wave order)
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double* lds /* >= NV*BLOCK/64 */) {
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nthreads = blockDim.x * blockDim.y;
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+    const int wave = lane >> 6, nwaves = (nthreads + 63) >> 6;
+    if ((lane & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) lds[k * 16 + wave] = v[k];
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NV; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nwaves; ++w) s += lds[k * 16 + w];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Block-wide max / min (order independent, hence deterministic).
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+// Publish a block partial and elect the last-arriving block.  Returns true in
+// every thread of the last block, after which all partials are visible.
+__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int nblocks, int* lds_flag) {
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    __threadfence();
+    __syncthreads();
+    if (lane == 0) {
+        unsigned int prev = atomicAdd(counter, 1u);
+        *lds_flag = (prev == nblocks - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    bool last = *lds_flag != 0;
+    if (last) __threadfence();
+    return last;
+}
+
+// Loads that must observe other blocks' writes from the same kernel.
+__device__ __forceinline__ double load_coherent(const double* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+}  // namespace fsi

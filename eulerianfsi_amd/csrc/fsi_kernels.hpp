@@ -1,0 +1,115 @@
+// fsi_kernels.hpp -- kernel launchers shared by the physics, solver and API
+// translation units of libfsi_hip.
+#pragma once
+
+#include "fsi_device.hpp"
+
+namespace fsi {
+
+__device__ __forceinline__ void atomic_min_double(double* addr, double v) {
+    unsigned long long* a = (unsigned long long*)addr;
+    unsigned long long old = *a;
+    while (v < __longlong_as_double((long long)old)) {
+        unsigned long long assumed = old;
+        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
+        if (old == assumed) break;
+    }
+}
+__device__ __forceinline__ void atomic_max_double(double* addr, double v) {
+    unsigned long long* a = (unsigned long long*)addr;
+    unsigned long long old = *a;
+    while (v > __longlong_as_double((long long)old)) {
+        unsigned long long assumed = old;
+        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
+        if (old == assumed) break;
+    }
+}
+
+__global__ void k_zero_scalar(double* x);
+
+// One level of the projection multigrid hierarchy (cell-centred, uniform).
+struct MgLevel {
+    int nx, ny, P;
+    double hx, hy;
+    double* x;  // smoothed iterate / correction (ping)
+    double* t;  // pong
+    double* b;  // right-hand side
+    double* r;  // residual
+};
+
+// Level geometry passed to the multigrid kernels.
+struct MgGeom {
+    int nx, ny, P;
+    int px, py;
+    int dl, dr, db, dt;  // 1 where the side is Dirichlet (else Neumann/periodic)
+    double ihx2, ihy2;
+};
+
+// ---- physics (fsi_physics.hip) ----
+void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
+                     const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
+                     double* Iu, double* Iv);
+void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
+                  const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
+                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1);
+void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, const double* sxx_n,
+                          const double* syy_n, const double* sxy_n, int reset, const double* const* E,
+                          const double* const* Sxx, const double* const* Syy, const double* const* Sxy, int ns,
+                          const double* coef, double* phi_s, double* sxx_s, double* syy_s, double* sxy_s,
+                          Status* st, int report);
+void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, const double* Vn,
+                          const double* const* Eu, const double* const* Ev, const double* const* Iu,
+                          const double* const* Iv, int ns, double dt, const double* ce, const double* ci,
+                          double alpha, const double* bu, const double* bv, double* ru, double* rv);
+void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv);
+
+// ---- velocity predictor CG (fsi_solvers.hip) ----
+struct VelBufs {
+    const double* Un;  // warm start uv_n (state velocity)
+    const double* Vn;
+    const double* ru;  // right-hand side (stage sum + alpha*b)
+    const double* rv;
+    double* xu;  // solution, written into the stage velocity arrays
+    double* xv;
+    double* r_u;
+    double* r_v;
+    double* p_u[2];
+    double* p_v[2];
+    double* q_u;
+    double* q_v;
+    double* partials;
+};
+void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
+                     int maxiter);
+void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
+                      int slot);
+void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
+
+// ---- projection (fsi_solvers.hip) ----
+struct ProjBufs {
+    double* U;  // velocity being projected (state or stage arrays)
+    double* V;
+    int inject;            // write the injected boundary faces first (_inject_compact_predictor!)
+    const double* lap_bc;  // b_bc of the Laplacian
+    double* rhs;
+    double* x;      // PCG iterate (warm start in, solution out)
+    double* q_out;  // mean-free correction written here
+    double* r;      // PCG residual = right-hand side of the V-cycle
+    double* z;      // preconditioned residual = V-cycle result
+    double* p[2];
+    double* w;
+    double* partials;
+};
+void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int has_null, double rtol,
+                     int maxiter);
+void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st);
+void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
+                      int nlev, Status* st, int count, int slot);
+void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
+                        int has_null, double lxly);
+
+int grid_blocks(const Params& p);
+double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
+                    const MgLevel* lv, const MgGeom* geo, Status* st);
+
+}  // namespace fsi

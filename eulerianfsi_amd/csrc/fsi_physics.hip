@@ -1,0 +1,892 @@
+// fsi_physics.hip -- explicit stage tendencies and stage combinations of the
+// four-stage IMEX step on gfx950.
+//
+// Every kernel is an HBM-bound stencil: one thread per output point, 64
+// threads along x (one wavefront reads one 512-byte row segment), 4 rows per
+// thread.  Neighbour values come from the same or adjacent cache lines and
+// are served by L1/L2.  No MFMA: nothing here is a dense contraction.
+#include "fsi_kernels.hpp"
+#include "fsi_stencils.hpp"
+
+namespace fsi {
+
+// ---------------------------------------------------------------------------
+// Small device helpers shared by the stencils.
+// ---------------------------------------------------------------------------
+struct Stage {
+    const double* U;
+    const double* V;
+    const double* phi;
+    const double* sxx;
+    const double* syy;
+    const double* sxy;
+    int reset;  // sigma reads apply the safety reset (phi < cutoff -> identity)
+};
+
+__device__ __forceinline__ double clamp01(double x) { return x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x); }
+
+__device__ __forceinline__ void load_sigma(const Params& p, const Stage& s, int i, int j,
+                                           double& a, double& d, double& b) {
+    size_t k = at(p, i, j);
+    a = s.sxx[k];
+    d = s.syy[k];
+    b = s.sxy[k];
+    if (s.reset && s.phi[k] < p.ext_cutoff) {  // apply_structure_extension! StructureTensor.jl:590-613
+        a = 1.0;
+        d = 1.0;
+        b = 0.0;
+    }
+}
+
+// phi * elastic_stress(sigma)  (StructuralConstitutive.jl:100-113, TimeIntegrationIMEX.jl:911-913)
+__device__ __forceinline__ double Txx_c(const Params& p, const Stage& s, int i, int j) {
+    double a, d, b;
+    load_sigma(p, s, i, j, a, d, b);
+    return s.phi[at(p, i, j)] * (p.Gmod * (a - 1.0));
+}
+__device__ __forceinline__ double Tyy_c(const Params& p, const Stage& s, int i, int j) {
+    double a, d, b;
+    load_sigma(p, s, i, j, a, d, b);
+    return s.phi[at(p, i, j)] * (p.Gmod * (d - 1.0));
+}
+__device__ __forceinline__ double Txy_c(const Params& p, const Stage& s, int i, int j) {
+    double a, d, b;
+    load_sigma(p, s, i, j, a, d, b);
+    return s.phi[at(p, i, j)] * (p.Gmod * b);
+}
+
+// ---- material convection pieces (MomentumTransport.jl:141-219) -----------
+__device__ __forceinline__ double u2c(const Params& p, const double* U, int k, int j) {
+    double ur = (p.px && k == p.nx - 1) ? U[at(p, 0, j)] : U[at(p, k + 1, j)];
+    double uc = 0.5 * U[at(p, k, j)] + 0.5 * ur;
+    return uc * uc;
+}
+__device__ __forceinline__ double v2c(const Params& p, const double* V, int i, int k) {
+    double vt = (p.py && k == p.ny - 1) ? V[at(p, i, 0)] : V[at(p, i, k + 1)];
+    double vc = 0.5 * V[at(p, i, k)] + 0.5 * vt;
+    return vc * vc;
+}
+__device__ __forceinline__ double u_corner(const Params& p, const double* U, int i, int jc) {
+    int ii = (p.px && i == p.nx) ? 0 : i;
+    if (p.py) {
+        if (jc == 0 || jc == p.ny) return 0.5 * U[at(p, ii, p.ny - 1)] + 0.5 * U[at(p, ii, 0)];
+        return 0.5 * U[at(p, ii, jc - 1)] + 0.5 * U[at(p, ii, jc)];
+    }
+    if (jc == 0) {
+        if (p.bU.b.kind == FSI_BC_DIRICHLET) return p.bU.b.value;
+        return U[at(p, ii, 0)] + p.bU.b.value * (p.hy / 2.0);
+    }
+    if (jc == p.ny) {
+        if (p.bU.t.kind == FSI_BC_DIRICHLET) return p.bU.t.value;
+        return U[at(p, ii, p.ny - 1)] + p.bU.t.value * (p.hy / 2.0);
+    }
+    return 0.5 * U[at(p, ii, jc - 1)] + 0.5 * U[at(p, ii, jc)];
+}
+__device__ __forceinline__ double v_corner(const Params& p, const double* V, int i, int jc) {
+    int jj = (p.py && jc == p.ny) ? 0 : jc;
+    if (p.px) {
+        if (i == 0 || i == p.nx) return 0.5 * V[at(p, p.nx - 1, jj)] + 0.5 * V[at(p, 0, jj)];
+        return 0.5 * V[at(p, i - 1, jj)] + 0.5 * V[at(p, i, jj)];
+    }
+    if (i == 0) {
+        if (p.bV.l.kind == FSI_BC_DIRICHLET) return p.bV.l.value;
+        return V[at(p, 0, jj)] + p.bV.l.value * (p.hx / 2.0);
+    }
+    if (i == p.nx) {
+        if (p.bV.r.kind == FSI_BC_DIRICHLET) return p.bV.r.value;
+        return V[at(p, p.nx - 1, jj)] + p.bV.r.value * (p.hx / 2.0);
+    }
+    return 0.5 * V[at(p, i - 1, jj)] + 0.5 * V[at(p, i, jj)];
+}
+__device__ __forceinline__ double uv_corner(const Params& p, const double* U, const double* V, int i, int jc) {
+    return u_corner(p, U, i, jc) * v_corner(p, V, i, jc);
+}
+
+// ---- linear wall extrapolation (Operators.jl:738-842) ---------------------
+template <class F>
+__device__ __forceinline__ double lex_lo(F f, double h) {  // value at the low wall from f(0), f(1)
+    double d = 0.5 * h;
+    return f(0) - (f(1) - f(0)) * d / h;
+}
+template <class F>
+__device__ __forceinline__ double lex_hi(F f, int n, double h) {  // value at the high wall
+    double d = 0.5 * h;
+    return f(n - 1) + (f(n - 1) - f(n - 2)) * d / h;
+}
+
+// ---------------------------------------------------------------------------
+// k_momentum: E_u, E_v (explicit momentum, TimeIntegrationIMEX.jl:962-988) and
+// I_u, I_v (viscous operator, :1057-1061) on the compact faces.
+// ---------------------------------------------------------------------------
+__device__ double elastic_fx(const Params& p, const Stage& s, int i, int j) {
+    // symmetric_stress_divergence x-part at x-face i (Operators.jl:642-683)
+    const int nx = p.nx, ny = p.ny;
+    double fx;
+    if (p.px && i == 0) fx = (Txx_c(p, s, 0, j) - Txx_c(p, s, nx - 1, j)) / p.hx;
+    else if (i == 0) {
+        auto f = [&](int k) { return Txx_c(p, s, k, j); };
+        double lw = lex_lo(f, p.hx);
+        fx = (Txx_c(p, s, 0, j) - lw) / (0.5 * p.hx);
+    } else fx = (Txx_c(p, s, i, j) - Txx_c(p, s, i - 1, j)) / p.hx;
+    // Txy on this x-face column, any row jj
+    auto txy_x = [&](int jj) -> double {
+        if (p.px && i == 0) return 0.5 * Txy_c(p, s, nx - 1, jj) + 0.5 * Txy_c(p, s, 0, jj);
+        if (i == 0) {
+            auto f = [&](int k) { return Txy_c(p, s, k, jj); };
+            return lex_lo(f, p.hx);
+        }
+        return 0.5 * Txy_c(p, s, i - 1, jj) + 0.5 * Txy_c(p, s, i, jj);
+    };
+    if (p.py) {
+        int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
+        fx += (txy_x(jp) - txy_x(jm)) / (p.hy + p.hy);
+    } else if (j == 0) {
+        double t0 = txy_x(0), t1 = txy_x(1);
+        double bw = t0 - (t1 - t0) * (0.5 * p.hy) / p.hy;
+        fx += (t0 - bw) / (0.5 * p.hy);
+    } else if (j == ny - 1) {
+        double t0 = txy_x(ny - 1), t1 = txy_x(ny - 2);
+        double tw = t0 + (t0 - t1) * (0.5 * p.hy) / p.hy;
+        fx += (tw - t0) / (0.5 * p.hy);
+    } else {
+        fx += (txy_x(j + 1) - txy_x(j - 1)) / (p.hy + p.hy);
+    }
+    return fx;
+}
+
+__device__ double elastic_fy(const Params& p, const Stage& s, int i, int j) {
+    const int nx = p.nx, ny = p.ny;
+    double fy;
+    if (p.py && j == 0) fy = (Tyy_c(p, s, i, 0) - Tyy_c(p, s, i, ny - 1)) / p.hy;
+    else if (j == 0) {
+        auto f = [&](int k) { return Tyy_c(p, s, i, k); };
+        double bw = lex_lo(f, p.hy);
+        fy = (Tyy_c(p, s, i, 0) - bw) / (0.5 * p.hy);
+    } else fy = (Tyy_c(p, s, i, j) - Tyy_c(p, s, i, j - 1)) / p.hy;
+    auto txy_y = [&](int ii) -> double {
+        if (p.py && j == 0) return 0.5 * Txy_c(p, s, ii, ny - 1) + 0.5 * Txy_c(p, s, ii, 0);
+        if (j == 0) {
+            auto f = [&](int k) { return Txy_c(p, s, ii, k); };
+            return lex_lo(f, p.hy);
+        }
+        return 0.5 * Txy_c(p, s, ii, j - 1) + 0.5 * Txy_c(p, s, ii, j);
+    };
+    if (p.px) {
+        int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
+        fy += (txy_y(ip) - txy_y(im)) / (p.hx + p.hx);
+    } else if (i == 0) {
+        double t0 = txy_y(0), t1 = txy_y(1);
+        double lw = t0 - (t1 - t0) * (0.5 * p.hx) / p.hx;
+        fy += (t0 - lw) / (0.5 * p.hx);
+    } else if (i == nx - 1) {
+        double t0 = txy_y(nx - 1), t1 = txy_y(nx - 2);
+        double rw = t0 + (t0 - t1) * (0.5 * p.hx) / p.hx;
+        fy += (rw - t0) / (0.5 * p.hx);
+    } else {
+        fy += (txy_y(i + 1) - txy_y(i - 1)) / (p.hx + p.hx);
+    }
+    return fy;
+}
+
+__global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* __restrict__ Eu,
+                                                    double* __restrict__ Ev, double* __restrict__ Iu,
+                                                    double* __restrict__ Iv) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    if (i >= p.nx) return;
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (j >= p.ny) return;
+        // ---- x-momentum at x-face (i, j)
+        double cu;
+        if (!p.px && i == 0) cu = 0.0;  // wall-normal row (MomentumTransport.jl:322-327)
+        else {
+            int im = (i == 0) ? p.nx - 1 : i - 1;
+            cu = (u2c(p, s.U, i, j) - u2c(p, s.U, im, j)) / p.hx +
+                 (uv_corner(p, s.U, s.V, i, j + 1) - uv_corner(p, s.U, s.V, i, j)) / p.hy;
+        }
+        double eu = -cu;
+        if (p.force == FSI_FORCE_CONSTANT) eu += p.ffx;
+        else if (p.force == FSI_FORCE_KOLMOGOROV) eu += p.force_row[j];
+        if (p.elastic) eu += elastic_fx(p, s, i, j);
+        // ---- y-momentum at y-face (i, j)
+        double cv;
+        if (!p.py && j == 0) cv = 0.0;
+        else {
+            int jm = (j == 0) ? p.ny - 1 : j - 1;
+            cv = (uv_corner(p, s.U, s.V, i + 1, j) - uv_corner(p, s.U, s.V, i, j)) / p.hx +
+                 (v2c(p, s.V, i, j) - v2c(p, s.V, i, jm)) / p.hy;
+        }
+        double ev = -cv;
+        if (p.force == FSI_FORCE_CONSTANT) ev += p.ffy;
+        if (p.elastic) ev += elastic_fy(p, s, i, j);
+        const size_t k = at(p, i, j);
+        Eu[k] = eu;
+        Ev[k] = ev;
+        if (Iu) {
+            Iu[k] = p.visc ? visc_u(p, PtrAcc{s.U, p.P}, PtrAcc{s.V, p.P}, i, j, 1) : 0.0;
+            Iv[k] = p.visc ? visc_v(p, PtrAcc{s.U, p.P}, PtrAcc{s.V, p.P}, i, j, 1) : 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Phase-field and structure-tensor right-hand sides (cell centred).
+// ---------------------------------------------------------------------------
+
+// scalar_gradient on x-face k (Operators.jl:189-356) of the cell function f.
+template <class F>
+__device__ __forceinline__ double grad_xface(const Params& p, const BcSet& bc, F f, int k, int j) {
+    if (p.px) {
+        if (k == 0 || k == p.nx) return p.ihx * f(0, j) - p.ihx * f(p.nx - 1, j);
+    } else if (k == 0) {
+        if (bc.l.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hx / 2.0;
+            return (1.0 / dist) * f(0, j) + (-bc.l.value / dist);
+        }
+        return -bc.l.value;
+    } else if (k == p.nx) {
+        if (bc.r.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hx / 2.0;
+            return (-1.0 / dist) * f(p.nx - 1, j) + bc.r.value / dist;
+        }
+        return bc.r.value;
+    }
+    return p.ihx * f(k, j) - p.ihx * f(k - 1, j);
+}
+template <class F>
+__device__ __forceinline__ double grad_yface(const Params& p, const BcSet& bc, F f, int i, int k) {
+    if (p.py) {
+        if (k == 0 || k == p.ny) return p.ihy * f(i, 0) - p.ihy * f(i, p.ny - 1);
+    } else if (k == 0) {
+        if (bc.b.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hy / 2.0;
+            return (1.0 / dist) * f(i, 0) + (-bc.b.value / dist);
+        }
+        return -bc.b.value;
+    } else if (k == p.ny) {
+        if (bc.t.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hy / 2.0;
+            return (-1.0 / dist) * f(i, p.ny - 1) + bc.t.value / dist;
+        }
+        return bc.t.value;
+    }
+    return p.ihy * f(i, k) - p.ihy * f(i, k - 1);
+}
+// centre gradient = face gradients averaged (PhaseFieldMethods.jl:219-231)
+template <class F>
+__device__ __forceinline__ void grad_center(const Params& p, const BcSet& bc, F f, int i, int j,
+                                            double& gx, double& gy) {
+    gx = 0.5 * grad_xface(p, bc, f, i, j) + 0.5 * grad_xface(p, bc, f, i + 1, j);
+    gy = 0.5 * grad_yface(p, bc, f, i, j) + 0.5 * grad_yface(p, bc, f, i, j + 1);
+}
+// interpolate_center_to_x_faces (Operators.jl:110-134) of f at face k
+template <class F>
+__device__ __forceinline__ double c2fx(const Params& p, F f, int k, int j) {
+    if (p.px && (k == 0 || k == p.nx)) return 0.5 * f(p.nx - 1, j) + 0.5 * f(0, j);
+    if (k == 0) return f(0, j);
+    if (k == p.nx) return f(p.nx - 1, j);
+    return 0.5 * f(k - 1, j) + 0.5 * f(k, j);
+}
+template <class F>
+__device__ __forceinline__ double c2fy(const Params& p, F f, int i, int k) {
+    if (p.py && (k == 0 || k == p.ny)) return 0.5 * f(i, p.ny - 1) + 0.5 * f(i, 0);
+    if (k == 0) return f(i, 0);
+    if (k == p.ny) return f(i, p.ny - 1);
+    return 0.5 * f(i, k - 1) + 0.5 * f(i, k);
+}
+
+// Incoming state at a wall face for first-order upwinding
+// (InterfaceTransport.jl:112-159, StructureTensor.jl:195-265).
+__device__ __forceinline__ double upwind_state(const BcSide& b, double vel, bool low_side, double interior,
+                                               Status* st) {
+    bool inflow = low_side ? vel > 0.0 : vel < 0.0;
+    if (vel == 0.0 || !inflow) return interior;
+    if (b.kind == FSI_BC_DIRICHLET) return b.value;
+    raise_error(st, DEV_NEUMANN_INFLOW, vel, 0.0);
+    return interior;
+}
+
+// -div(upwind v f) at cell (i,j) (InterfaceTransport.jl:161-258).
+template <class F>
+__device__ double upwind_rhs(const Params& p, const double* U, const double* V, const BcSet& bc, F f,
+                             int i, int j, Status* st) {
+    auto ax = [&](int k) -> double {
+        if (p.px) {
+            int kk = (k == p.nx) ? 0 : k;
+            double u = U[at(p, kk, j)];
+            double L = f(kk == 0 ? p.nx - 1 : kk - 1, j), R = f(kk, j);
+            return u * (u >= 0.0 ? L : R);
+        }
+        if (k == 0) {
+            double u = U[at(p, 0, j)];
+            double bs = upwind_state(bc.l, u, true, f(0, j), st);
+            return u * (u >= 0.0 ? bs : f(0, j));
+        }
+        if (k == p.nx) {
+            double u = U[at(p, p.nx, j)];
+            double bs = upwind_state(bc.r, u, false, f(p.nx - 1, j), st);
+            return u * (u >= 0.0 ? f(p.nx - 1, j) : bs);
+        }
+        double u = U[at(p, k, j)];
+        return u * (u >= 0.0 ? f(k - 1, j) : f(k, j));
+    };
+    auto ay = [&](int k) -> double {
+        if (p.py) {
+            int kk = (k == p.ny) ? 0 : k;
+            double v = V[at(p, i, kk)];
+            double B = f(i, kk == 0 ? p.ny - 1 : kk - 1), T = f(i, kk);
+            return v * (v >= 0.0 ? B : T);
+        }
+        if (k == 0) {
+            double v = V[at(p, i, 0)];
+            double bs = upwind_state(bc.b, v, true, f(i, 0), st);
+            return v * (v >= 0.0 ? bs : f(i, 0));
+        }
+        if (k == p.ny) {
+            double v = V[at(p, i, p.ny)];
+            double bs = upwind_state(bc.t, v, false, f(i, p.ny - 1), st);
+            return v * (v >= 0.0 ? f(i, p.ny - 1) : bs);
+        }
+        double v = V[at(p, i, k)];
+        return v * (v >= 0.0 ? f(i, k - 1) : f(i, k));
+    };
+    return -((ax(i + 1) - ax(i)) / p.hx + (ay(j + 1) - ay(j)) / p.hy);
+}
+
+// _phase_boundary_value (PhaseFieldMethods.jl:174-196)
+__device__ __forceinline__ double phase_bvalue(const BcSide& b, double interior, Status* st) {
+    if (b.kind == FSI_BC_DIRICHLET) return b.value;
+    if (fabs(b.value) > 100 * 2.220446049250313e-16) raise_error(st, DEV_NONHOMOG_NEUMANN, b.value, 0.0);
+    return interior;
+}
+
+// ---- CDI (InterfaceRegularization.jl:187-228 + InterfaceTransport.jl:260-276)
+__device__ double phase_cdi(const Params& p, const Stage& s, int i, int j, Status* st, int with_flux) {
+    const double* phi = s.phi;
+    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
+    double adv = upwind_rhs(p, s.U, s.V, p.bPhi, f, i, j, st);
+    if (!with_flux) return adv - 0.0;
+    auto sharp = [&](int a, int b, int comp) -> double {
+        double gx, gy;
+        grad_center(p, p.bPhi, f, a, b, gx, gy);
+        double mag = sqrt(gx * gx + gy * gy + p.cdi_d2);
+        double ph = f(a, b);
+        double amp = ph * (1.0 - ph);
+        return comp == 0 ? amp * gx / mag : amp * gy / mag;
+    };
+    const double ell = 0.5 * p.cdi_eps;
+    auto jx = [&](int k) -> double {
+        if (!p.px && (k == 0 || k == p.nx)) return 0.0;
+        auto shx = [&](int a, int b) { return sharp(a, b, 0); };
+        return p.cdi_G * (c2fx(p, shx, k, j) - ell * grad_xface(p, p.bPhi, f, k, j));
+    };
+    auto jy = [&](int k) -> double {
+        if (!p.py && (k == 0 || k == p.ny)) return 0.0;
+        auto shy = [&](int a, int b) { return sharp(a, b, 1); };
+        return p.cdi_G * (c2fy(p, shy, i, k) - ell * grad_yface(p, p.bPhi, f, i, k));
+    };
+    double divj = (jx(i + 1) - jx(i)) / p.hx + (jy(j + 1) - jy(j)) / p.hy;
+    return adv - divj;
+}
+
+// ---- ACDI (PhaseFieldMethods.jl:479-671)
+__device__ __forceinline__ double psi_value(const Params& p, double ph) {
+    double q = clamp01(ph), e = p.acdi_floor;
+    return p.acdi_eps * log((q + e) / (1.0 - q + e));
+}
+
+__device__ double phase_acdi(const Params& p, const Stage& s, const BcSet& bpsi, int i, int j,
+                             double gamma, Status* st) {
+    const double* phi = s.phi;
+    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
+    auto psi = [&](int a, int b) { return psi_value(p, phi[at(p, a, b)]); };
+    // convective flux U * phi_face (:531-574)
+    auto fx = [&](int k) -> double {
+        if (p.px) {
+            int kk = (k == p.nx) ? 0 : k;
+            double pf = kk == 0 ? 0.5 * f(p.nx - 1, j) + 0.5 * f(0, j) : 0.5 * f(kk - 1, j) + 0.5 * f(kk, j);
+            return s.U[at(p, kk, j)] * pf;
+        }
+        if (k == 0) return s.U[at(p, 0, j)] * phase_bvalue(p.bPhi.l, f(0, j), st);
+        if (k == p.nx) return s.U[at(p, p.nx, j)] * phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
+        return s.U[at(p, k, j)] * (0.5 * f(k - 1, j) + 0.5 * f(k, j));
+    };
+    auto fy = [&](int k) -> double {
+        if (p.py) {
+            int kk = (k == p.ny) ? 0 : k;
+            return s.V[at(p, i, kk)] * (kk == 0 ? 0.5 * f(i, p.ny - 1) + 0.5 * f(i, 0)
+                                                : 0.5 * f(i, kk - 1) + 0.5 * f(i, kk));
+        }
+        if (k == 0) return s.V[at(p, i, 0)] * phase_bvalue(p.bPhi.b, f(i, 0), st);
+        if (k == p.ny) return s.V[at(p, i, p.ny)] * phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
+        return s.V[at(p, i, k)] * (0.5 * f(i, k - 1) + 0.5 * f(i, k));
+    };
+    // centre normals of psi (:233-244)
+    auto normal = [&](int a, int b, int comp) -> double {
+        double gx, gy;
+        grad_center(p, bpsi, psi, a, b, gx, gy);
+        double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
+        return comp == 0 ? gx / mag : gy / mag;
+    };
+    const double eps = p.acdi_eps;
+    auto Ax = [&](int k) -> double {
+        if (!p.px && (k == 0 || k == p.nx)) return 0.0;
+        auto nxc = [&](int a, int b) { return normal(a, b, 0); };
+        double t = tanh(c2fx(p, psi, k, j) / (2.0 * eps));
+        double amp = 0.25 * (1.0 - t * t);
+        return gamma * (eps * grad_xface(p, p.bPhi, f, k, j) - amp * c2fx(p, nxc, k, j));
+    };
+    auto Ay = [&](int k) -> double {
+        if (!p.py && (k == 0 || k == p.ny)) return 0.0;
+        auto nyc = [&](int a, int b) { return normal(a, b, 1); };
+        double t = tanh(c2fy(p, psi, i, k) / (2.0 * eps));
+        double amp = 0.25 * (1.0 - t * t);
+        return gamma * (eps * grad_yface(p, p.bPhi, f, i, k) - amp * c2fy(p, nyc, i, k));
+    };
+    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
+    double dA = (Ax(i + 1) - Ax(i)) / p.hx + (Ay(j + 1) - Ay(j)) / p.hy;
+    return -dF + dA;
+}
+
+// ---- UWCDI (PhaseFieldMethods.jl:675-867, :standard normals :246-278)
+__device__ double phase_uwcdi(const Params& p, const Stage& s, int i, int j, Status* st) {
+    const double* phi = s.phi;
+    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
+    auto ncomp = [&](int a, int b, int comp) -> double {
+        double gx, gy;
+        grad_center(p, p.bPhi, f, a, b, gx, gy);
+        double mag = sqrt(gx * gx + gy * gy + p.uw_d2);
+        return comp == 0 ? gx / mag : gy / mag;
+    };
+    auto nxc = [&](int a, int b) { return ncomp(a, b, 0); };
+    auto nyc = [&](int a, int b) { return ncomp(a, b, 1); };
+    auto fx = [&](int k) -> double {
+        int kk = (p.px && k == p.nx) ? 0 : k;  // periodic duplicate face
+        double u = s.U[at(p, kk, j)], pu;
+        if (!p.px && kk == 0) {
+            double bnd = phase_bvalue(p.bPhi.l, f(0, j), st);
+            pu = u >= 0.0 ? bnd : f(0, j);
+        } else if (!p.px && kk == p.nx) {
+            double bnd = phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
+            pu = u >= 0.0 ? f(p.nx - 1, j) : bnd;
+        } else {
+            int iL = kk == 0 ? p.nx - 1 : kk - 1, iR = kk;
+            pu = u >= 0.0 ? f(iL, j) : f(iR, j);
+        }
+        double nf = c2fx(p, nxc, kk, j);
+        double ny_on = c2fx(p, nyc, kk, j);
+        nf /= sqrt(nf * nf + ny_on * ny_on + p.uw_d2);
+        double ax = p.uw_alpha * fmax(1.0, p.hy / p.hx);
+        double beta = fabs(u) / ax;
+        return pu * (u + beta * nf * (1.0 - pu));
+    };
+    auto fy = [&](int k) -> double {
+        int kk = (p.py && k == p.ny) ? 0 : k;
+        double v = s.V[at(p, i, kk)], pu;
+        if (!p.py && kk == 0) {
+            double bnd = phase_bvalue(p.bPhi.b, f(i, 0), st);
+            pu = v >= 0.0 ? bnd : f(i, 0);
+        } else if (!p.py && kk == p.ny) {
+            double bnd = phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
+            pu = v >= 0.0 ? f(i, p.ny - 1) : bnd;
+        } else {
+            int jB = kk == 0 ? p.ny - 1 : kk - 1, jT = kk;
+            pu = v >= 0.0 ? f(i, jB) : f(i, jT);
+        }
+        double nf = c2fy(p, nyc, i, kk);
+        double nx_on = c2fy(p, nxc, i, kk);
+        nf /= sqrt(nx_on * nx_on + nf * nf + p.uw_d2);
+        double ay = p.uw_alpha * fmax(1.0, p.hx / p.hy);
+        double beta = fabs(v) / ay;
+        return pu * (v + beta * nf * (1.0 - pu));
+    };
+    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
+    return -dF;
+}
+
+// ---- structure tensor (StructureTensor.jl:377-551, Operators.jl:431-586)
+__device__ __forceinline__ bool sigma_log_product(double a, double d, double b, double& ox, double& oy,
+                                                  double& oxy) {
+    double m = 0.5 * (a + d), delta = 0.5 * (a - d);
+    double r = hypot(delta, b);
+    double lhi = m + r, llo = m - r;
+    if (!(isfinite(llo) && isfinite(lhi) && llo > 0.0)) return false;
+    double ghi = lhi * log(lhi), glo = llo * log(llo);
+    double scale = fmax(1.0, fabs(m)), alpha, beta;
+    if (r <= 1.0e-12 * scale) {
+        alpha = log(m) + 1.0;
+        beta = -m;
+    } else {
+        double den = lhi - llo;
+        alpha = (ghi - glo) / den;
+        beta = (lhi * glo - llo * ghi) / den;
+    }
+    ox = alpha * a + beta;
+    oy = alpha * d + beta;
+    oxy = alpha * b;
+    return true;
+}
+
+__device__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status* st, double& oxx,
+                          double& oyy, double& oxy) {
+    const double* U = s.U;
+    const double* V = s.V;
+    auto fxx = [&](int a, int b) { double x, y, z; load_sigma(p, s, a, b, x, y, z); return x; };
+    auto fyy = [&](int a, int b) { double x, y, z; load_sigma(p, s, a, b, x, y, z); return y; };
+    auto fxy = [&](int a, int b) { double x, y, z; load_sigma(p, s, a, b, x, y, z); return z; };
+    double axx = upwind_rhs(p, U, V, p.bSxx, fxx, i, j, st);
+    double ayy = upwind_rhs(p, U, V, p.bSyy, fyy, i, j, st);
+    double axy = upwind_rhs(p, U, V, p.bSxy, fxy, i, j, st);
+    double uL = U[at(p, i, j)], uR = U[at(p, i + 1, j)], vB = V[at(p, i, j)], vT = V[at(p, i, j + 1)];
+    double dv = (uR - uL) / p.hx + (vT - vB) / p.hy;
+    double ux = (uR - uL) / p.hx, vy = (vT - vB) / p.hy;
+    auto Uc = [&](int jj) { return 0.5 * U[at(p, i, jj)] + 0.5 * U[at(p, i + 1, jj)]; };
+    auto Vc = [&](int ii) { return 0.5 * V[at(p, ii, j)] + 0.5 * V[at(p, ii, j + 1)]; };
+    double uy, vx;
+    if (p.py) {
+        int jm = j == 0 ? p.ny - 1 : j - 1, jp = j == p.ny - 1 ? 0 : j + 1;
+        uy = (Uc(jp) - Uc(jm)) / (p.hy + p.hy);
+    } else if (j == 0) {
+        uy = p.bU.b.kind == FSI_BC_DIRICHLET ? (Uc(0) - p.bU.b.value) / (0.5 * p.hy) : -p.bU.b.value;
+    } else if (j == p.ny - 1) {
+        uy = p.bU.t.kind == FSI_BC_DIRICHLET ? (p.bU.t.value - Uc(p.ny - 1)) / (0.5 * p.hy) : p.bU.t.value;
+    } else {
+        uy = (Uc(j + 1) - Uc(j - 1)) / (p.hy + p.hy);
+    }
+    if (p.px) {
+        int im = i == 0 ? p.nx - 1 : i - 1, ip = i == p.nx - 1 ? 0 : i + 1;
+        vx = (Vc(ip) - Vc(im)) / (p.hx + p.hx);
+    } else if (i == 0) {
+        vx = p.bV.l.kind == FSI_BC_DIRICHLET ? (Vc(0) - p.bV.l.value) / (0.5 * p.hx) : -p.bV.l.value;
+    } else if (i == p.nx - 1) {
+        vx = p.bV.r.kind == FSI_BC_DIRICHLET ? (p.bV.r.value - Vc(p.nx - 1)) / (0.5 * p.hx) : p.bV.r.value;
+    } else {
+        vx = (Vc(i + 1) - Vc(i - 1)) / (p.hx + p.hx);
+    }
+    double sa, sd, sb;
+    load_sigma(p, s, i, j, sa, sd, sb);
+    double A11 = ux - 0.0, A12 = uy - 0.0, A21 = vx - 0.0, A22 = vy - 0.0;
+    double dxx = 2.0 * (A11 * sa + A12 * sb);
+    double dyy = 2.0 * (A21 * sb + A22 * sd);
+    double dxy = A21 * sa + A12 * sd + (A11 + A22) * sb;
+    double rxx = 0.0, ryy = 0.0, rxy = 0.0;
+    if (p.relax) {
+        rxx = -p.relax_rate * (sa - 1.0);
+        ryy = -p.relax_rate * (sd - 1.0);
+        rxy = -p.relax_rate * sb;
+    }
+    double exx = 0.0, eyy = 0.0, exy = 0.0;
+    if (p.ext == FSI_EXT_GEOMETRIC) {
+        double kappa = p.ext_rate * pow(1.0 - clamp01(s.phi[at(p, i, j)]), p.ext_exp);
+        if (kappa != 0.0) {
+            double gx, gy, gxy;
+            if (sigma_log_product(sa, sd, sb, gx, gy, gxy)) {
+                exx = -kappa * gx;
+                eyy = -kappa * gy;
+                exy = -kappa * gxy;
+            } else {
+                raise_error(st, DEV_NONSPD, sa, sb);
+            }
+        }
+    }
+    oxx = (axx + sa * dv) + dxx + rxx + exx;
+    oyy = (ayy + sd * dv) + dyy + ryy + eyy;
+    oxy = (axy + sb * dv) + dxy + rxy + exy;
+}
+
+// Max |u_c| for dynamic ACDI gamma (PhaseFieldMethods.jl:576-596).
+__global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, const double* V, Status* st) {
+    __shared__ double lds[16];
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    double m = 0.0;
+    if (i < p.nx)
+        for (int r = 0; r < ROWS; ++r) {
+            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            if (j >= p.ny) break;
+            double uc = 0.5 * U[at(p, i, j)] + 0.5 * U[at(p, i + 1, j)];
+            double vc = 0.5 * V[at(p, i, j)] + 0.5 * V[at(p, i, j + 1)];
+            m = fmax(m, hypot(uc, vc));
+        }
+    m = wave_max(m);
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) lds[lane >> 6] = m;
+    __syncthreads();
+    if (lane == 0) {
+        double mm = lds[0];
+        for (int w = 1; w < BLOCK / 64; ++w) mm = fmax(mm, lds[w]);
+        atomic_max_double(&st->gamma_umax, mm);
+    }
+}
+
+// k_cells: E_phi (phase_field_rhs) and E_sigma (structure_tensor_rhs) of a
+// stage; on stage 1 also the finiteness validation of the state
+// (TimeIntegrationIMEX.jl:495-523) and the stage-1 divergence L-infinity
+// (:1111-1114).
+__global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
+                                                 double* __restrict__ Exx, double* __restrict__ Eyy,
+                                                 double* __restrict__ Exy, Status* st, int stage1) {
+    __shared__ double lds[16];
+    __shared__ long long lds_n[16];
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    double dmax = 0.0;
+    long long nonfinite = 0;
+    double gamma = p.acdi_gamma;
+    if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn) gamma = p.acdi_gamma * st->gamma_umax;
+    if (i < p.nx)
+        for (int r = 0; r < ROWS; ++r) {
+            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            if (j >= p.ny) break;
+            double e;
+            switch (p.phase) {
+            case FSI_PHASE_UPWIND: e = phase_cdi(p, s, i, j, st, 0); break;
+            case FSI_PHASE_CDI: e = phase_cdi(p, s, i, j, st, 1); break;
+            case FSI_PHASE_ACDI: e = phase_acdi(p, s, bpsi, i, j, gamma, st); break;
+            default: e = phase_uwcdi(p, s, i, j, st); break;
+            }
+            double oxx, oyy, oxy;
+            sigma_rhs(p, s, i, j, st, oxx, oyy, oxy);
+            const size_t k = at(p, i, j);
+            Ephi[k] = e;
+            Exx[k] = oxx;
+            Eyy[k] = oyy;
+            Exy[k] = oxy;
+            if (stage1) {
+                double uL = s.U[k], uR = s.U[at(p, i + 1, j)], vB = s.V[k], vT = s.V[at(p, i, j + 1)];
+                double dv = (uR - uL) / p.hx + (vT - vB) / p.hy;
+                dmax = fmax(dmax, fabs(dv - 0.0));
+                bool ok = isfinite(uL) && isfinite(vB) && isfinite(s.phi[k]) && isfinite(s.sxx[k]) &&
+                          isfinite(s.syy[k]) && isfinite(s.sxy[k]);
+                if (i == p.nx - 1) ok = ok && isfinite(uR);
+                if (j == p.ny - 1) ok = ok && isfinite(vT);
+                nonfinite += ok ? 0 : 1;
+                if (!isfinite(dv)) dmax = INFINITY;
+            }
+        }
+    if (stage1) {
+        dmax = wave_max(dmax);
+        for (int off = 32; off > 0; off >>= 1) nonfinite += __shfl_down(nonfinite, off, 64);
+        const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+        if ((lane & 63) == 0) {
+            lds[lane >> 6] = dmax;
+            lds_n[lane >> 6] = nonfinite;
+        }
+        __syncthreads();
+        if (lane == 0) {
+            double mm = lds[0];
+            long long nn = lds_n[0];
+            for (int w = 1; w < BLOCK / 64; ++w) {
+                mm = fmax(mm, lds[w]);
+                nn += lds_n[w];
+            }
+            atomic_max_double(&st->stage1_linf, mm);
+            if (nn) atomicAdd((unsigned long long*)&st->nonfinite_in, (unsigned long long)nn);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage combinations (_build_explicit_stage! :716-740, _build_velocity_stage!
+// :777-793) for s = 2..4.  coef[j] = dt * A_EXPLICIT[s][j] for the cells,
+// ce[j] = A_EXPLICIT[s][j], ci[j] = A_IMPLICIT[s][j] for the velocity.
+// ---------------------------------------------------------------------------
+struct Hist {
+    const double* E[3];
+    const double* Sxx[3];
+    const double* Syy[3];
+    const double* Sxy[3];
+};
+
+__global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, Hist h, int ns, double c0,
+                                                         double c1, double c2, double* __restrict__ phi_s,
+                                                         double* __restrict__ sxx_s, double* __restrict__ syy_s,
+                                                         double* __restrict__ sxy_s, Status* st, int report) {
+    __shared__ double lds[4][16];
+    __shared__ long long lds_n[2][16];
+    const double cf[3] = {c0, c1, c2};
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    double pmin = INFINITY, pmax = -INFINITY, emin = INFINITY, dmax = 0.0;
+    long long nonspd = 0, nonfinite = 0;
+    if (i < p.nx)
+        for (int r = 0; r < ROWS; ++r) {
+            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            if (j >= p.ny) break;
+            const size_t k = at(p, i, j);
+            double ph = base.phi[k], a, d, b;
+            load_sigma(p, base, i, j, a, d, b);
+            for (int q = 0; q < ns; ++q) {
+                if (cf[q] == 0.0) continue;
+                ph += cf[q] * h.E[q][k];
+                a += cf[q] * h.Sxx[q][k];
+                d += cf[q] * h.Syy[q][k];
+                b += cf[q] * h.Sxy[q][k];
+            }
+            if (base.reset && ph < p.ext_cutoff) {
+                a = 1.0;
+                d = 1.0;
+                b = 0.0;
+            }
+            phi_s[k] = ph;
+            sxx_s[k] = a;
+            syy_s[k] = d;
+            sxy_s[k] = b;
+            if (report) {  // structure_tensor_diagnostics StructureTensor.jl:627-662
+                double det = a * d - b * b;
+                double lmin = 0.5 * (a + d - hypot(a - d, 2.0 * b));
+                dmax = fmax(dmax, fabs(det - 1.0));
+                emin = fmin(emin, lmin);
+                nonspd += (isfinite(lmin) && lmin > 0.0) ? 0 : 1;
+                pmin = fmin(pmin, ph);
+                pmax = fmax(pmax, ph);
+                nonfinite += (isfinite(ph) && isfinite(a) && isfinite(d) && isfinite(b)) ? 0 : 1;
+            }
+        }
+    if (!report) return;
+    pmin = wave_min(pmin);
+    pmax = wave_max(pmax);
+    emin = wave_min(emin);
+    dmax = wave_max(dmax);
+    for (int off = 32; off > 0; off >>= 1) {
+        nonspd += __shfl_down(nonspd, off, 64);
+        nonfinite += __shfl_down(nonfinite, off, 64);
+    }
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) {
+        int w = lane >> 6;
+        lds[0][w] = pmin;
+        lds[1][w] = pmax;
+        lds[2][w] = emin;
+        lds[3][w] = dmax;
+        lds_n[0][w] = nonspd;
+        lds_n[1][w] = nonfinite;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double v0 = lds[0][0], v1 = lds[1][0], v2 = lds[2][0], v3 = lds[3][0];
+        long long n0 = lds_n[0][0], n1 = lds_n[1][0];
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            v0 = fmin(v0, lds[0][w]);
+            v1 = fmax(v1, lds[1][w]);
+            v2 = fmin(v2, lds[2][w]);
+            v3 = fmax(v3, lds[3][w]);
+            n0 += lds_n[0][w];
+            n1 += lds_n[1][w];
+        }
+        atomic_min_double(&st->phi_min, v0);
+        atomic_max_double(&st->phi_max, v1);
+        atomic_min_double(&st->min_eig, v2);
+        atomic_max_double(&st->max_det_err, v3);
+        if (n0) atomicAdd((unsigned long long*)&st->nonspd, (unsigned long long)n0);
+        if (n1) atomicAdd((unsigned long long*)&st->nonfinite, (unsigned long long)n1);
+    }
+}
+
+struct VHist {
+    const double* Eu[3];
+    const double* Ev[3];
+    const double* Iu[3];
+    const double* Iv[3];
+};
+
+// rhs_uv = uv_n + dt * sum(aE E_uv + aI I_uv) + alpha * b_visc
+__global__ void __launch_bounds__(BLOCK) k_combine_faces(Params p, const double* __restrict__ Un,
+                                                         const double* __restrict__ Vn, VHist h, int ns, double dt,
+                                                         double e0, double e1, double e2, double i0, double i1,
+                                                         double i2, double alpha, const double* __restrict__ bu,
+                                                         const double* __restrict__ bv, double* __restrict__ ru,
+                                                         double* __restrict__ rv) {
+    const double ce[3] = {e0, e1, e2}, ci[3] = {i0, i1, i2};
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    if (i >= p.nx) return;
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (j >= p.ny) return;
+        const size_t k = at(p, i, j);
+        double u = Un[k], v = Vn[k];
+        for (int q = 0; q < ns; ++q) {
+            if (ce[q] == 0.0 && ci[q] == 0.0) continue;
+            u += dt * (ce[q] * h.Eu[q][k] + ci[q] * h.Iu[q][k]);
+            v += dt * (ce[q] * h.Ev[q][k] + ci[q] * h.Iv[q][k]);
+        }
+        if (bu) {
+            u += alpha * bu[k];
+            v += alpha * bv[k];
+        }
+        ru[k] = u;
+        rv[k] = v;
+    }
+}
+
+// b_visc = M*0 + b (the affine part of the viscous operator), computed once.
+__global__ void __launch_bounds__(BLOCK) k_visc_boundary(Params p, const double* zero, double* bu, double* bv) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    if (i >= p.nx) return;
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (j >= p.ny) return;
+        bu[at(p, i, j)] = visc_u(p, ZeroAcc{}, ZeroAcc{}, i, j, 1);
+        bv[at(p, i, j)] = visc_v(p, ZeroAcc{}, ZeroAcc{}, i, j, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static dim3 grid2d(const Params& p) {
+    return dim3((p.nx + TILE_X - 1) / TILE_X, (p.ny + TILE_ROWS - 1) / TILE_ROWS);
+}
+
+void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
+                     const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
+                     double* Iu, double* Iv) {
+    Stage st{U, V, phi, sxx, syy, sxy, reset};
+    k_momentum<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, st, Eu, Ev, Iu, Iv);
+}
+
+void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
+                  const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
+                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1) {
+    Stage sg{U, V, phi, sxx, syy, sxy, reset};
+    if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn) {
+        k_zero_scalar<<<1, 1, 0, s>>>(&st->gamma_umax);
+        k_max_speed<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, st);
+    }
+    k_cells<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, sg, bpsi, Ephi, Exx, Eyy, Exy, st, stage1);
+}
+
+void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, const double* sxx_n,
+                          const double* syy_n, const double* sxy_n, int reset, const double* const* E,
+                          const double* const* Sxx, const double* const* Syy, const double* const* Sxy, int ns,
+                          const double* coef, double* phi_s, double* sxx_s, double* syy_s, double* sxy_s,
+                          Status* st, int report) {
+    Stage base{nullptr, nullptr, phi_n, sxx_n, syy_n, sxy_n, reset};
+    Hist h{};
+    for (int q = 0; q < ns; ++q) {
+        h.E[q] = E[q];
+        h.Sxx[q] = Sxx[q];
+        h.Syy[q] = Syy[q];
+        h.Sxy[q] = Sxy[q];
+    }
+    k_combine_cells<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, base, h, ns, coef[0], coef[1], coef[2], phi_s,
+                                                               sxx_s, syy_s, sxy_s, st, report);
+}
+
+void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, const double* Vn,
+                          const double* const* Eu, const double* const* Ev, const double* const* Iu,
+                          const double* const* Iv, int ns, double dt, const double* ce, const double* ci,
+                          double alpha, const double* bu, const double* bv, double* ru, double* rv) {
+    VHist h{};
+    for (int q = 0; q < ns; ++q) {
+        h.Eu[q] = Eu[q];
+        h.Ev[q] = Ev[q];
+        h.Iu[q] = Iu[q];
+        h.Iv[q] = Iv[q];
+    }
+    k_combine_faces<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, Un, Vn, h, ns, dt, ce[0], ce[1], ce[2], ci[0],
+                                                               ci[1], ci[2], alpha, bu, bv, ru, rv);
+}
+
+void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv) {
+    k_visc_boundary<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, zero, bu, bv);
+}
+
+}  // namespace fsi

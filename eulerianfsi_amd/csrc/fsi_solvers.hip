@@ -1,0 +1,866 @@
+// fsi_solvers.hip -- the two linear solves of the IMEX step on gfx950.
+//
+// (1) Velocity predictor: (I - alpha M) uv = rhs with Dirichlet rows
+//     (src/TimeIntegrationIMEX.jl:794-812).  M is symmetric negative
+//     semidefinite on uniform grids, so after eliminating the Dirichlet rows
+//     the system is SPD and is solved by matrix-free CG.  The reference uses
+//     UMFPACK or GMRES(100); both return the same unique solution.
+// (2) Projection: A q = div(v*) - G - b_bc with the unit-coefficient
+//     Laplacian (src/Projection.jl:170-307), solved by CG preconditioned with
+//     one geometric-multigrid V-cycle (cell-centred, damped-Jacobi smoothing,
+//     bilinear prolongation P, restriction R = P^T/4, rediscretised coarse
+//     operators, a single-workgroup LDS solve for the coarsest levels).  The
+//     reference uses GMRES(50) + Ruge-Stueben AMG; A q = b has a unique
+//     zero-mean solution, so the projected velocity is the same.
+//
+// All dot products are deterministic: wave64 shuffles, LDS in wave order, a
+// per-block partial, and the last-arriving block summing the partials in a
+// fixed order and updating the Krylov scalars in device memory.  Every kernel
+// of an iteration first checks the solver's `done` flag, so the host can
+// enqueue iterations ahead without reading the residual back.
+#include "fsi_kernels.hpp"
+#include "fsi_stencils.hpp"
+
+#include <string>
+
+namespace fsi {
+
+__global__ void k_zero_scalar(double* x) { *x = 0.0; }
+
+static __device__ __forceinline__ int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
+static __device__ __forceinline__ int lane_id() { return threadIdx.x + threadIdx.y * blockDim.x; }
+
+// Grid-wide deterministic sum of NV values; returns true in the last block,
+// whose lane 0 then holds the totals in v.
+template <int NV>
+__device__ bool grid_sum(double (&v)[NV], double* partials, int nblocks, unsigned int* counter) {
+    __shared__ double lds[NV * 16];
+    __shared__ int flag;
+    block_sum<NV>(v, lds);
+    const int lane = lane_id(), bid = block_id();
+    if (lane == 0)
+        for (int k = 0; k < NV; ++k) partials[(size_t)k * nblocks + bid] = v[k];
+    if (!arrive_last(counter, (unsigned)nblocks, &flag)) return false;
+    double acc[NV];
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    const int nthreads = blockDim.x * blockDim.y;
+    for (int b = lane; b < nblocks; b += nthreads)
+        for (int k = 0; k < NV; ++k) acc[k] += load_coherent(&partials[(size_t)k * nblocks + b]);
+    block_sum<NV>(acc, lds);
+    if (lane == 0) {
+        for (int k = 0; k < NV; ++k) v[k] = acc[k];
+        *counter = 0u;
+    }
+    return true;
+}
+
+__device__ __forceinline__ void block_max_to(double m, double* dst) {
+    __shared__ double lds[16];
+    m = wave_max(m);
+    const int lane = lane_id();
+    if ((lane & 63) == 0) lds[lane >> 6] = m;
+    __syncthreads();
+    if (lane == 0) {
+        double mm = lds[0];
+        for (int w = 1; w < (int)(blockDim.x * blockDim.y + 63) / 64; ++w) mm = fmax(mm, lds[w]);
+        atomic_max_double(dst, mm);
+    }
+}
+
+#define FOR_TILE(p)                                                              \
+    const int i = blockIdx.x * TILE_X + threadIdx.x;                             \
+    for (int rr_ = 0; rr_ < ROWS; ++rr_)                                         \
+        if (const int j = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;  \
+            i < (p).nx && j < (p).ny)
+
+// ===========================================================================
+// Velocity predictor CG
+// ===========================================================================
+struct VelScal {
+    double alpha;  // dt * a_ss
+};
+
+// x = warm start uv_n with Dirichlet values, r = rhs - A x (masked), rr, |b_F|^2
+__global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double alpha, Status* st, int nblocks,
+                                                    double rtol, int maxiter) {
+    const double gul = p.bU.l.value, gvb = p.bV.b.value;
+    auto xu = [&](int a, int c) { return dir_u(p, a) ? gul : b.Un[at(p, a, c)]; };
+    auto xv = [&](int a, int c) { return dir_v(p, c) ? gvb : b.Vn[at(p, a, c)]; };
+    double acc[2] = {0.0, 0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double x_u = xu(i, j), x_v = xv(i, j);
+        double ru = 0.0, rv = 0.0;
+        if (!dir_u(p, i)) {
+            ru = b.ru[k] - (x_u - alpha * visc_u(p, xu, xv, i, j, 0));
+            acc[1] += b.ru[k] * b.ru[k];
+        }
+        if (!dir_v(p, j)) {
+            rv = b.rv[k] - (x_v - alpha * visc_v(p, xu, xv, i, j, 0));
+            acc[1] += b.rv[k] * b.rv[k];
+        }
+        b.xu[k] = x_u;
+        b.xv[k] = x_v;
+        b.r_u[k] = ru;
+        b.r_v[k] = rv;
+        acc[0] += ru * ru + rv * rv;
+    }
+    if (grid_sum<2>(acc, b.partials, nblocks, &st->counters[0]) && lane_id() == 0) {
+        KrylovScalars& s = st->vel;
+        s.rr = acc[0];
+        s.bnorm2 = acc[1];
+        s.tol2 = rtol * rtol * acc[1];
+        s.iter = 0;
+        s.maxiter = maxiter;
+        s.beta = 0.0;
+        s.done = (acc[0] <= s.tol2) ? 1 : 0;
+    }
+}
+
+// p_new = r + beta p_old (p_new = r on the first iteration); q = A p_new; alpha = rr / (p.q)
+__global__ void __launch_bounds__(BLOCK) k_vel_dir(Params p, VelBufs b, double alpha, Status* st, int nblocks) {
+    if (st->vel.done) return;
+    const int it = st->vel.iter;
+    const double beta = st->vel.beta;
+    const double* pou = b.p_u[it & 1];
+    const double* pov = b.p_v[it & 1];
+    double* pnu = b.p_u[(it + 1) & 1];
+    double* pnv = b.p_v[(it + 1) & 1];
+    auto pu = [&](int a, int c) {
+        size_t k = at(p, a, c);
+        return it == 0 ? b.r_u[k] : b.r_u[k] + beta * pou[k];
+    };
+    auto pv = [&](int a, int c) {
+        size_t k = at(p, a, c);
+        return it == 0 ? b.r_v[k] : b.r_v[k] + beta * pov[k];
+    };
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double vu = pu(i, j), vv = pv(i, j);
+        double qu = dir_u(p, i) ? 0.0 : vu - alpha * visc_u(p, pu, pv, i, j, 0);
+        double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, pu, pv, i, j, 0);
+        pnu[k] = vu;
+        pnv[k] = vv;
+        b.q_u[k] = qu;
+        b.q_v[k] = qv;
+        acc[0] += vu * qu + vv * qv;
+    }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[1]) && lane_id() == 0) {
+        st->vel.pq = acc[0];
+        st->vel.alpha = st->vel.rr / acc[0];
+    }
+}
+
+// x += a p; r -= a q; convergence test on |r|
+__global__ void __launch_bounds__(BLOCK) k_vel_upd(Params p, VelBufs b, Status* st, int nblocks, int slot) {
+    if (st->vel.done) return;
+    const int it = st->vel.iter;
+    const double a = st->vel.alpha;
+    const double* pu = b.p_u[(it + 1) & 1];
+    const double* pv = b.p_v[(it + 1) & 1];
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        b.xu[k] += a * pu[k];
+        b.xv[k] += a * pv[k];
+        double ru = b.r_u[k] - a * b.q_u[k];
+        double rv = b.r_v[k] - a * b.q_v[k];
+        b.r_u[k] = ru;
+        b.r_v[k] = rv;
+        acc[0] += ru * ru + rv * rv;
+    }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[2]) && lane_id() == 0) {
+        KrylovScalars& s = st->vel;
+        s.beta = acc[0] / s.rr;
+        s.rr = acc[0];
+        s.iter = it + 1;
+        st->vel_iters[slot] = s.iter;
+        if (s.rr <= s.tol2) s.done = 1;
+        else if (s.iter >= s.maxiter) {
+            s.done = 1;
+            // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
+            if (s.rr > 1e-12 * s.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(s.rr / s.bnorm2), (double)s.iter);
+        }
+    }
+}
+
+// NoViscosity: the predictor is the stage sum with Dirichlet values (:794-796).
+__global__ void __launch_bounds__(BLOCK) k_vel_copy(Params p, VelBufs b) {
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        b.xu[k] = dir_u(p, i) ? p.bU.l.value : b.ru[k];
+        b.xv[k] = dir_v(p, j) ? p.bV.b.value : b.rv[k];
+    }
+}
+
+static dim3 grid2(const Params& p) { return dim3((p.nx + TILE_X - 1) / TILE_X, (p.ny + TILE_ROWS - 1) / TILE_ROWS); }
+int grid_blocks(const Params& p) {
+    dim3 g = grid2(p);
+    return (int)(g.x * g.y);
+}
+
+
+// ===========================================================================
+// Projection: rhs, PCG, multigrid V-cycle, finish
+// ===========================================================================
+
+// -A x at a cell of a uniform level (A = unit Laplacian with the correction
+// BCs, homogeneous data).  Dirichlet sides add 2/h^2 to the diagonal
+// (gradient (q1 - g)/(h/2)), Neumann sides add nothing, periodic sides wrap.
+template <class FX>
+__device__ __forceinline__ double neg_lap(const MgGeom& g, const FX& x, int i, int j, double* diag) {
+    double c = x(i, j), sx = 0.0, sy = 0.0, cx = 0.0, cy = 0.0;
+    if (i > 0) { sx += x(i - 1, j); cx += 1.0; }
+    else if (g.px) { sx += x(g.nx - 1, j); cx += 1.0; }
+    else if (g.dl) cx += 2.0;
+    if (i < g.nx - 1) { sx += x(i + 1, j); cx += 1.0; }
+    else if (g.px) { sx += x(0, j); cx += 1.0; }
+    else if (g.dr) cx += 2.0;
+    if (j > 0) { sy += x(i, j - 1); cy += 1.0; }
+    else if (g.py) { sy += x(i, g.ny - 1); cy += 1.0; }
+    else if (g.db) cy += 2.0;
+    if (j < g.ny - 1) { sy += x(i, j + 1); cy += 1.0; }
+    else if (g.py) { sy += x(i, 0); cy += 1.0; }
+    else if (g.dt) cy += 2.0;
+    if (diag) *diag = g.ihx2 * cx + g.ihy2 * cy;
+    return g.ihx2 * (cx * c - sx) + g.ihy2 * (cy * c - sy);
+}
+
+struct GAcc {
+    const double* a;
+    int P;
+    __device__ __forceinline__ double operator()(int i, int j) const { return a[(size_t)j * P + i]; }
+};
+
+// 1-D bilinear prolongation of fine index f from the coarse grid: parent
+// weight 3/4, nearer neighbour 1/4; outside a non-periodic wall the ghost is
+// the mirrored parent (+, Neumann) or its negative (-, Dirichlet).
+__device__ __forceinline__ void prol_1d(int f, int nf, int per, int dlo, int dhi, int& I0, double& w0, int& I1,
+                                        double& w1) {
+    const int nc = nf >> 1, parent = f >> 1;
+    int other = (f & 1) ? parent + 1 : parent - 1;
+    I0 = parent;
+    w0 = 0.75;
+    I1 = other;
+    w1 = 0.25;
+    if (other < 0 || other >= nc) {
+        if (per) I1 = (other + nc) % nc;
+        else {
+            w0 += ((other < 0 ? dlo : dhi) ? -0.25 : 0.25);
+            I1 = parent;
+            w1 = 0.0;
+        }
+    }
+}
+__device__ __forceinline__ double pweight(int f, int I, int nf, int per, int dlo, int dhi) {
+    int I0, I1;
+    double w0, w1;
+    prol_1d(f, nf, per, dlo, dhi, I0, w0, I1, w1);
+    double w = 0.0;
+    if (I0 == I) w += w0;
+    if (I1 == I) w += w1;
+    return w;
+}
+
+// rhs = (div(v*) - G) - b_bc, with the injected boundary faces of the
+// predictor (_inject_compact_predictor! :543-579) written first.
+__global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, Status* st, int nblocks, int has_null,
+                                                    double rtol, int maxiter) {
+    const double area = p.hx * p.hy;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    FOR_TILE(p) {
+        double uR, vT;
+        if (b.inject && i == p.nx - 1) {
+            uR = p.px ? b.U[at(p, 0, j)] : p.bU.r.value;
+            b.U[at(p, p.nx, j)] = uR;
+        } else uR = b.U[at(p, i + 1, j)];
+        if (b.inject && j == p.ny - 1) {
+            vT = p.py ? b.V[at(p, i, 0)] : p.bV.t.value;
+            b.V[at(p, i, p.ny)] = vT;
+        } else vT = b.V[at(p, i, j + 1)];
+        const size_t k = at(p, i, j);
+        double div = (uR - b.U[k]) / p.hx + (vT - b.V[k]) / p.hy;
+        double rhs = (div - 0.0) - b.lap_bc[k];
+        b.rhs[k] = rhs;
+        acc[0] += rhs * area;
+        acc[1] += fabs(rhs) * area;
+        acc[2] += rhs * rhs;
+        acc[3] += div * div * area;
+    }
+    if (grid_sum<4>(acc, b.partials, nblocks, &st->counters[3]) && lane_id() == 0) {
+        ProjScalars& q = st->proj;
+        q.defect = has_null ? acc[0] : 0.0;
+        q.scale = acc[1];
+        q.rhs2 = acc[2];
+        q.pdiv_l2sq = acc[3];
+        q.linf = 0.0;
+        if (has_null) {  // compatibility check, src/Projection.jl:206-221
+            double tol = 1e-12 + 1e-10 * fmax(acc[1], 2.220446049250313e-16);
+            if (!(fabs(acc[0]) <= tol)) raise_error(st, DEV_INCOMPATIBLE, acc[0], tol);
+        }
+        const double n = (double)p.nx * (double)p.ny;
+        q.mean_rhs = has_null ? (acc[0] / area) / n : 0.0;
+        KrylovScalars& s = st->pcg;
+        s.bnorm2 = fmax(acc[2] - n * q.mean_rhs * q.mean_rhs, 0.0);
+        s.tol2 = rtol * rtol * s.bnorm2;
+        s.iter = 0;
+        s.maxiter = maxiter;
+        s.done = 0;
+        s.beta = 0.0;
+        s.rz = 0.0;
+    }
+}
+
+// r = b - (-A) x with b = -(rhs - mean)
+__global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nblocks) {
+    const double mean = st->proj.mean_rhs;
+    GAcc X{b.x, p.P};
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double r = -(b.rhs[k] - mean) - neg_lap(g, X, i, j, nullptr);
+        b.r[k] = r;
+        acc[0] += r * r;
+    }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[4]) && lane_id() == 0) {
+        st->pcg.rr = acc[0];
+        st->pcg.done = (acc[0] <= st->pcg.tol2) ? 1 : 0;
+    }
+}
+
+// ---- multigrid level kernels (every one returns at once once PCG is done)
+__global__ void __launch_bounds__(BLOCK) k_mg_jacobi0(MgGeom g, const double* b, double* x, double omega,
+                                                      const int* done) {
+    if (*done) return;
+    FOR_TILE(g) {
+        double d;
+        neg_lap(g, GAcc{b, g.P}, i, j, &d);
+        x[(size_t)j * g.P + i] = omega * b[(size_t)j * g.P + i] / d;
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_jacobi(MgGeom g, const double* xin, const double* b, double* xout,
+                                                     double omega, const int* done) {
+    if (*done) return;
+    FOR_TILE(g) {
+        double d;
+        double ax = neg_lap(g, GAcc{xin, g.P}, i, j, &d);
+        const size_t k = (size_t)j * g.P + i;
+        xout[k] = xin[k] + omega * (b[k] - ax) / d;
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_mg_resid(MgGeom g, const double* x, const double* b, double* r,
+                                                    const int* done) {
+    if (*done) return;
+    FOR_TILE(g) {
+        const size_t k = (size_t)j * g.P + i;
+        r[k] = b[k] - neg_lap(g, GAcc{x, g.P}, i, j, nullptr);
+    }
+}
+// b_c(I,J) = sum_f 0.25 * P(f,I) * r_f  (R = P^T / 4)
+__global__ void __launch_bounds__(BLOCK) k_mg_restrict(MgGeom gf, MgGeom gc, const double* rf, double* bc,
+                                                       const int* done) {
+    if (*done) return;
+    const int I = blockIdx.x * TILE_X + threadIdx.x;
+    for (int rr_ = 0; rr_ < ROWS; ++rr_) {
+        const int J = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;
+        if (I >= gc.nx || J >= gc.ny) continue;
+        double s = 0.0;
+        for (int dy = -1; dy <= 2; ++dy) {
+            int fy = 2 * J + dy;
+            if (fy < 0 || fy >= gf.ny) {
+                if (!gf.py) continue;
+                fy = (fy + gf.ny) % gf.ny;
+            }
+            double wy = pweight(fy, J, gf.ny, gf.py, gf.db, gf.dt);
+            if (wy == 0.0) continue;
+            for (int dx = -1; dx <= 2; ++dx) {
+                int fx = 2 * I + dx;
+                if (fx < 0 || fx >= gf.nx) {
+                    if (!gf.px) continue;
+                    fx = (fx + gf.nx) % gf.nx;
+                }
+                double wx = pweight(fx, I, gf.nx, gf.px, gf.dl, gf.dr);
+                if (wx == 0.0) continue;
+                s += 0.25 * wx * wy * rf[(size_t)fy * gf.P + fx];
+            }
+        }
+        bc[(size_t)J * gc.P + I] = s;
+    }
+}
+// x_f += P x_c
+__global__ void __launch_bounds__(BLOCK) k_mg_prolong(MgGeom gf, MgGeom gc, const double* xc, double* xf,
+                                                      const int* done) {
+    if (*done) return;
+    FOR_TILE(gf) {
+        int X0, X1, Y0, Y1;
+        double wx0, wx1, wy0, wy1;
+        prol_1d(i, gf.nx, gf.px, gf.dl, gf.dr, X0, wx0, X1, wx1);
+        prol_1d(j, gf.ny, gf.py, gf.db, gf.dt, Y0, wy0, Y1, wy1);
+        double e = wy0 * (wx0 * xc[(size_t)Y0 * gc.P + X0] + wx1 * xc[(size_t)Y0 * gc.P + X1]) +
+                   wy1 * (wx0 * xc[(size_t)Y1 * gc.P + X0] + wx1 * xc[(size_t)Y1 * gc.P + X1]);
+        xf[(size_t)j * gf.P + i] += e;
+    }
+}
+
+// ---- coarsest levels: one workgroup, the whole remaining V-cycle in LDS
+constexpr int CO_THREADS = 1024;
+constexpr int CO_MAX_CELLS = 1024;
+constexpr int CO_MAX_LEVELS = 12;
+constexpr int CO_BOTTOM_SWEEPS = 64;
+
+struct CoLevel {
+    MgGeom g;
+    int off;  // offset of this level's arrays (x, t, b, r consecutive, each n doubles)
+};
+
+__device__ void co_jacobi0(const CoLevel& L, double* s) {
+    const int n = L.g.nx * L.g.ny;
+    double* x = s + L.off;
+    const double* b = s + L.off + 2 * n;
+    for (int k = threadIdx.x; k < n; k += CO_THREADS) {
+        int i = k % L.g.nx, j = k / L.g.nx;
+        double d;
+        neg_lap(L.g, GAcc{b, L.g.P}, i, j, &d);
+        x[k] = 0.8 * b[k] / d;
+    }
+    __syncthreads();
+}
+__device__ void co_jacobi(const CoLevel& L, double* s, int from_x) {
+    const int n = L.g.nx * L.g.ny;
+    double* xin = s + L.off + (from_x ? 0 : n);
+    double* xout = s + L.off + (from_x ? n : 0);
+    const double* b = s + L.off + 2 * n;
+    for (int k = threadIdx.x; k < n; k += CO_THREADS) {
+        int i = k % L.g.nx, j = k / L.g.nx;
+        double d;
+        double ax = neg_lap(L.g, GAcc{xin, L.g.P}, i, j, &d);
+        xout[k] = xin[k] + 0.8 * (b[k] - ax) / d;
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, const double* bglob, double* tglob,
+                                                          const int* done) {
+    extern __shared__ double s[];
+    if (*done) return;
+    CoLevel L[CO_MAX_LEVELS];
+    int nl = 0, off = 0;
+    MgGeom g = g0;
+    g.P = g0.nx;
+    for (;;) {
+        L[nl].g = g;
+        L[nl].off = off;
+        off += 4 * g.nx * g.ny;
+        ++nl;
+        if (nl == CO_MAX_LEVELS || (g.nx & 1) || (g.ny & 1) || g.nx < 4 || g.ny < 4) break;
+        g.nx >>= 1;
+        g.ny >>= 1;
+        g.P = g.nx;
+        g.ihx2 *= 0.25;
+        g.ihy2 *= 0.25;
+    }
+    // load b of the top coarse level
+    {
+        const int n = g0.nx * g0.ny;
+        for (int k = threadIdx.x; k < n; k += CO_THREADS) {
+            int i = k % g0.nx, j = k / g0.nx;
+            s[L[0].off + 2 * n + k] = bglob[(size_t)j * g0.P + i];
+        }
+        __syncthreads();
+    }
+    // down
+    for (int l = 0; l < nl - 1; ++l) {
+        const MgGeom& gf = L[l].g;
+        const MgGeom& gc = L[l + 1].g;
+        const int nf = gf.nx * gf.ny, nc = gc.nx * gc.ny;
+        co_jacobi0(L[l], s);
+        co_jacobi(L[l], s, 1);  // result in t
+        double* t = s + L[l].off + nf;
+        double* bf = s + L[l].off + 2 * nf;
+        double* r = s + L[l].off + 3 * nf;
+        for (int k = threadIdx.x; k < nf; k += CO_THREADS) {
+            int i = k % gf.nx, j = k / gf.nx;
+            r[k] = bf[k] - neg_lap(gf, GAcc{t, gf.P}, i, j, nullptr);
+        }
+        __syncthreads();
+        double* bc = s + L[l + 1].off + 2 * nc;
+        for (int k = threadIdx.x; k < nc; k += CO_THREADS) {
+            int I = k % gc.nx, J = k / gc.nx;
+            double acc = 0.0;
+            for (int dy = -1; dy <= 2; ++dy) {
+                int fy = 2 * J + dy;
+                if (fy < 0 || fy >= gf.ny) {
+                    if (!gf.py) continue;
+                    fy = (fy + gf.ny) % gf.ny;
+                }
+                double wy = pweight(fy, J, gf.ny, gf.py, gf.db, gf.dt);
+                for (int dx = -1; dx <= 2; ++dx) {
+                    int fx = 2 * I + dx;
+                    if (fx < 0 || fx >= gf.nx) {
+                        if (!gf.px) continue;
+                        fx = (fx + gf.nx) % gf.nx;
+                    }
+                    double wx = pweight(fx, I, gf.nx, gf.px, gf.dl, gf.dr);
+                    acc += 0.25 * wx * wy * r[(size_t)fy * gf.P + fx];
+                }
+            }
+            bc[k] = acc;
+        }
+        __syncthreads();
+    }
+    // bottom: fixed number of damped-Jacobi sweeps from zero (result in t)
+    {
+        const CoLevel& B = L[nl - 1];
+        co_jacobi0(B, s);
+        for (int it = 0; it < CO_BOTTOM_SWEEPS; ++it) {
+            co_jacobi(B, s, 1);
+            co_jacobi(B, s, 0);
+        }
+        co_jacobi(B, s, 1);
+    }
+    // up: t_l += P t_{l+1}; two sweeps t -> x -> t
+    for (int l = nl - 2; l >= 0; --l) {
+        const MgGeom& gf = L[l].g;
+        const MgGeom& gc = L[l + 1].g;
+        const int nf = gf.nx * gf.ny, nc = gc.nx * gc.ny;
+        double* t = s + L[l].off + nf;
+        const double* tc = s + L[l + 1].off + nc;
+        for (int k = threadIdx.x; k < nf; k += CO_THREADS) {
+            int i = k % gf.nx, j = k / gf.nx;
+            int X0, X1, Y0, Y1;
+            double wx0, wx1, wy0, wy1;
+            prol_1d(i, gf.nx, gf.px, gf.dl, gf.dr, X0, wx0, X1, wx1);
+            prol_1d(j, gf.ny, gf.py, gf.db, gf.dt, Y0, wy0, Y1, wy1);
+            t[k] += wy0 * (wx0 * tc[Y0 * gc.nx + X0] + wx1 * tc[Y0 * gc.nx + X1]) +
+                    wy1 * (wx0 * tc[Y1 * gc.nx + X0] + wx1 * tc[Y1 * gc.nx + X1]);
+        }
+        __syncthreads();
+        co_jacobi(L[l], s, 0);  // t -> x
+        co_jacobi(L[l], s, 1);  // x -> t
+    }
+    {
+        const int n = g0.nx * g0.ny;
+        const double* t = s + L[0].off + n;
+        for (int k = threadIdx.x; k < n; k += CO_THREADS) {
+            int i = k % g0.nx, j = k / g0.nx;
+            tglob[(size_t)j * g0.P + i] = t[k];
+        }
+    }
+}
+
+static size_t coarse_lds_bytes(MgGeom g) {
+    size_t total = 0;
+    for (int l = 0; l < CO_MAX_LEVELS; ++l) {
+        total += 4 * (size_t)g.nx * g.ny;
+        if (l + 1 == CO_MAX_LEVELS || (g.nx & 1) || (g.ny & 1) || g.nx < 4 || g.ny < 4) break;
+        g.nx >>= 1;
+        g.ny >>= 1;
+    }
+    return total * sizeof(double);
+}
+
+// rz = r.z; beta = rz / rz_old
+__global__ void __launch_bounds__(BLOCK) k_pcg_rz(Params p, ProjBufs b, Status* st, int nblocks) {
+    if (st->pcg.done) return;
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        acc[0] += b.r[k] * b.z[k];
+    }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[5]) && lane_id() == 0) {
+        KrylovScalars& s = st->pcg;
+        s.rz_old = s.rz;
+        s.rz = acc[0];
+        s.beta = s.iter == 0 ? 0.0 : acc[0] / s.rz_old;
+    }
+}
+// p_new = z + beta p_old; w = (-A) p_new; alpha = rz / (p.w)
+__global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs b, Status* st, int nblocks) {
+    if (st->pcg.done) return;
+    const int it = st->pcg.iter;
+    const double beta = st->pcg.beta;
+    const double* po = b.p[it & 1];
+    double* pn = b.p[(it + 1) & 1];
+    auto P = [&](int a, int c) {
+        size_t k = at(p, a, c);
+        return it == 0 ? b.z[k] : b.z[k] + beta * po[k];
+    };
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double pv = P(i, j);
+        double w = neg_lap(g, P, i, j, nullptr);
+        pn[k] = pv;
+        b.w[k] = w;
+        acc[0] += pv * w;
+    }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[6]) && lane_id() == 0) {
+        st->pcg.pq = acc[0];
+        st->pcg.alpha = st->pcg.rz / acc[0];
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_pcg_upd(Params p, ProjBufs b, Status* st, int nblocks, int slot) {
+    if (st->pcg.done) return;
+    const int it = st->pcg.iter;
+    const double a = st->pcg.alpha;
+    const double* pn = b.p[(it + 1) & 1];
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        b.x[k] += a * pn[k];
+        double r = b.r[k] - a * b.w[k];
+        b.r[k] = r;
+        acc[0] += r * r;
+    }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[7]) && lane_id() == 0) {
+        KrylovScalars& s = st->pcg;
+        s.rr = acc[0];
+        s.iter = it + 1;
+        if (slot >= 0) st->proj_iters[slot] = s.iter;
+        if (s.rr <= s.tol2 || s.iter >= s.maxiter) s.done = 1;
+    }
+}
+
+// mean of q (area weighted, src/Projection.jl:255-259)
+__global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, Status* st, int nblocks, int has_null,
+                                                     double lxly) {
+    double acc[1] = {0.0};
+    const double area = p.hx * p.hy;
+    FOR_TILE(p) { acc[0] += b.x[at(p, i, j)] * area; }
+    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[8]) && lane_id() == 0)
+        st->proj.mean_q = has_null ? acc[0] / lxly : 0.0;
+}
+
+// v -= grad q on every face (src/Projection.jl:266-273), q with its BC data.
+__global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    GAcc Q{b.x, p.P};
+    for (int rr_ = 0; rr_ < ROWS; ++rr_) {
+        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;
+        if (i > p.nx || j > p.ny) continue;
+        if (j < p.ny) {  // x-face (i, j)
+            double gx;
+            if (p.px) {
+                int k = (i == p.nx) ? 0 : i;
+                int km = k == 0 ? p.nx - 1 : k - 1;
+                gx = p.ihx * Q(k, j) - p.ihx * Q(km, j);
+            } else if (i == 0) {
+                if (p.bQ.l.kind == FSI_BC_DIRICHLET) {
+                    double dist = p.hx / 2.0;
+                    gx = (1.0 / dist) * Q(0, j) + (-p.bQ.l.value / dist);
+                } else gx = -p.bQ.l.value;
+            } else if (i == p.nx) {
+                if (p.bQ.r.kind == FSI_BC_DIRICHLET) {
+                    double dist = p.hx / 2.0;
+                    gx = (-1.0 / dist) * Q(p.nx - 1, j) + p.bQ.r.value / dist;
+                } else gx = p.bQ.r.value;
+            } else gx = p.ihx * Q(i, j) - p.ihx * Q(i - 1, j);
+            b.U[at(p, i, j)] -= gx;
+        }
+        if (i < p.nx) {  // y-face (i, j)
+            double gy;
+            if (p.py) {
+                int k = (j == p.ny) ? 0 : j;
+                int km = k == 0 ? p.ny - 1 : k - 1;
+                gy = p.ihy * Q(i, k) - p.ihy * Q(i, km);
+            } else if (j == 0) {
+                if (p.bQ.b.kind == FSI_BC_DIRICHLET) {
+                    double dist = p.hy / 2.0;
+                    gy = (1.0 / dist) * Q(i, 0) + (-p.bQ.b.value / dist);
+                } else gy = -p.bQ.b.value;
+            } else if (j == p.ny) {
+                if (p.bQ.t.kind == FSI_BC_DIRICHLET) {
+                    double dist = p.hy / 2.0;
+                    gy = (-1.0 / dist) * Q(i, p.ny - 1) + p.bQ.t.value / dist;
+                } else gy = p.bQ.t.value;
+            } else gy = p.ihy * Q(i, j) - p.ihy * Q(i, j - 1);
+            b.V[at(p, i, j)] -= gy;
+        }
+    }
+}
+
+// q_out = q - mean; true residual |A q - rhs| (:261-264); constraint div(v) - G
+// in L-inf and area-weighted L2 (:275-297), with the checks.
+__global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nblocks,
+                                                      double lxly) {
+    const double mean = st->proj.mean_q;
+    const double area = p.hx * p.hy;
+    auto Q = [&](int a, int c) { return b.x[at(p, a, c)] - mean; };
+    double acc[2] = {0.0, 0.0};
+    double linf = 0.0;
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double q = Q(i, j);
+        b.q_out[k] = q;
+        double aq = -neg_lap(g, Q, i, j, nullptr);
+        double res = aq - b.rhs[k];
+        acc[0] += res * res;
+        double div = (b.U[at(p, i + 1, j)] - b.U[k]) / p.hx + (b.V[at(p, i, j + 1)] - b.V[k]) / p.hy;
+        double c = div - 0.0;
+        linf = fmax(linf, fabs(c));
+        if (!isfinite(c)) linf = INFINITY;
+        acc[1] += c * c * area;
+    }
+    block_max_to(linf, &st->proj.linf);
+    if (grid_sum<2>(acc, b.partials, nblocks, &st->counters[9]) && lane_id() == 0) {
+        ProjScalars& q = st->proj;
+        q.resid2 = acc[0];
+        q.l2sq = acc[1];
+        double rn = sqrt(q.rhs2);
+        q.lin_rel = sqrt(acc[0]) / fmax(rn, 2.220446049250313e-16);
+        q.l2 = sqrt(acc[1] / lxly);
+        double lin = load_coherent(&q.linf);
+        atomic_max_double(&st->max_constraint_linf, lin);
+        if (!(q.lin_rel <= 1e-8)) raise_error(st, DEV_PROJ_RESIDUAL, q.lin_rel, 1e-8);
+        double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), 0.0), 1.0);
+        double ctol = 1e-12 + 1e-8 * cs;
+        if (!(q.l2 <= ctol)) raise_error(st, DEV_PROJ_CONSTRAINT, q.l2, ctol);
+    }
+}
+
+// Plain copy, the calibration kernel of the traffic measurement.
+__global__ void __launch_bounds__(BLOCK) k_copy(Params p, const double* __restrict__ a, double* __restrict__ b) {
+    FOR_TILE(p) { b[at(p, i, j)] = a[at(p, i, j)]; }
+}
+
+// ===========================================================================
+// launchers
+// ===========================================================================
+static dim3 grid_of(int nx, int ny) { return dim3((nx + TILE_X - 1) / TILE_X, (ny + TILE_ROWS - 1) / TILE_ROWS); }
+static const dim3 kBlock(TILE_X, TILE_Y);
+
+void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
+                     int maxiter) {
+    k_vel_init<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p), rtol, maxiter);
+}
+void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
+                      int slot) {
+    const int nb = grid_blocks(p);
+    for (int c = 0; c < count; ++c) {
+        k_vel_dir<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb);
+        k_vel_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, slot);
+    }
+}
+void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
+    k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
+}
+
+void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int has_null, double rtol,
+                     int maxiter) {
+    k_proj_rhs<<<grid2(p), kBlock, 0, s>>>(p, b, st, grid_blocks(p), has_null, rtol, maxiter);
+}
+void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st) {
+    k_pcg_init<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, grid_blocks(p));
+}
+
+static void vcycle(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done) {
+    const double omega = 0.8;
+    for (int l = 0; l < nlev - 1; ++l) {
+        const MgGeom& g = geo[l];
+        dim3 gr = grid_of(g.nx, g.ny);
+        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+        k_mg_resid<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].r, done);
+        const MgGeom& gc = geo[l + 1];
+        k_mg_restrict<<<grid_of(gc.nx, gc.ny), kBlock, 0, s>>>(g, gc, lv[l].r, lv[l + 1].b, done);
+    }
+    {
+        const int l = nlev - 1;
+        const MgGeom& g = geo[l];
+        if ((long)g.nx * g.ny <= CO_MAX_CELLS) {
+            k_mg_coarse<<<1, CO_THREADS, coarse_lds_bytes(g), s>>>(g, lv[l].b, lv[l].t, done);
+        } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
+            dim3 gr = grid_of(g.nx, g.ny);
+            k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
+            const int sweeps = 2 * (g.nx > g.ny ? g.nx : g.ny);
+            for (int it = 0; it < sweeps; ++it) {
+                k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+                k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
+            }
+            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+        }
+    }
+    for (int l = nlev - 2; l >= 0; --l) {
+        const MgGeom& g = geo[l];
+        dim3 gr = grid_of(g.nx, g.ny);
+        k_mg_prolong<<<gr, kBlock, 0, s>>>(g, geo[l + 1], lv[l + 1].t, lv[l].t, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+    }
+}
+
+void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
+                      int nlev, Status* st, int count, int slot) {
+    const int nb = grid_blocks(p);
+    for (int c = 0; c < count; ++c) {
+        vcycle(s, lv, geo, nlev, &st->pcg.done);
+        k_pcg_rz<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb);
+        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], b, st, nb);
+        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, slot);
+    }
+}
+
+void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
+                        int has_null, double lxly) {
+    const int nb = grid_blocks(p);
+    k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, has_null, lxly);
+    k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
+    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, lxly);
+}
+
+// One launch of a named solver kernel with the buffers of a live solve, for
+// timing.  Returns the algorithmic HBM bytes of one launch (fields read +
+// written, 8 bytes per cell), or a negative value for an unknown name.
+double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
+                    const MgLevel* lv, const MgGeom* geo, Status* st) {
+    const int nb = grid_blocks(p);
+    const double cell = 8.0 * (double)p.nx * (double)p.ny;
+    const std::string n(name);
+    if (n == "copy") {
+        k_copy<<<grid2(p), kBlock, 0, s>>>(p, pb.rhs, pb.w);
+        return 2 * cell;
+    }
+    if (n == "vel_dir") {
+        k_vel_dir<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb);
+        return 8 * cell;  // r, p_old (u, v) read; p_new, q (u, v) written
+    }
+    if (n == "vel_upd") {
+        k_vel_upd<<<grid2(p), kBlock, 0, s>>>(p, vb, st, nb, 0);
+        return 12 * cell;  // x, p, r, q (u, v) read; x, r (u, v) written
+    }
+    if (n == "pcg_dir") {
+        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], pb, st, nb);
+        return 4 * cell;  // z, p_old read; p_new, w written
+    }
+    if (n == "pcg_upd") {
+        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb, 0);
+        return 6 * cell;  // x, p, r, w read; x, r written
+    }
+    if (n == "pcg_rz") {
+        k_pcg_rz<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb);
+        return 2 * cell;
+    }
+    if (n == "mg_jacobi") {
+        k_mg_jacobi<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], lv[0].x, lv[0].b, lv[0].t, 0.8,
+                                                                    &st->pcg.done);
+        return 3 * cell;  // x, b read; x' written
+    }
+    if (n == "mg_resid") {
+        k_mg_resid<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], lv[0].t, lv[0].b, lv[0].r, &st->pcg.done);
+        return 3 * cell;
+    }
+    if (n == "mg_prolong") {
+        k_mg_prolong<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], geo[1], lv[1].t, lv[0].t,
+                                                                      &st->pcg.done);
+        return 2.25 * cell;
+    }
+    if (n == "mg_restrict") {
+        k_mg_restrict<<<grid_of(geo[1].nx, geo[1].ny), kBlock, 0, s>>>(geo[0], geo[1], lv[0].r, lv[1].b,
+                                                                       &st->pcg.done);
+        return 1.25 * cell;
+    }
+    return -1.0;
+}
+
+}  // namespace fsi

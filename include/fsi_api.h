@@ -1,0 +1,249 @@
+/*
+ * fsi_api.h -- C ABI of libfsi_hip.so, the MI355X (gfx950) backend of the
+ * EulerianFSI.jl four-stage IMEX fractional step.
+ *
+ * The entry points mirror the reference's Julia step API one to one:
+ *
+ *   build_imex_context(metrics, bcs, models; numerics)  src/TimeIntegrationIMEX.jl:468-493
+ *   allocate_imex_workspace(metrics)                     src/TimeIntegrationIMEX.jl:389-441
+ *   make_velocity_compatible!(state, ctx, work)          src/TimeIntegrationIMEX.jl:1088-1109
+ *   integrate_step!(state, t, dt, ctx, work)            src/TimeIntegrationIMEX.jl:1152-1256
+ *   project!(U, V, target, system, metrics; ...)        src/Projection.jl:170-307
+ *
+ * Array convention (identical to the reference's Julia matrices, so a Julia
+ * caller passes pointer(A) with no transposition):
+ *   column-major, x index fastest, 0-based here;
+ *   U   : (nx+1) x ny   x-face velocity,  element (i,j) at U[i + j*(nx+1)]
+ *   V   : nx x (ny+1)   y-face velocity,  element (i,j) at V[i + j*nx]
+ *   cell: nx x ny       phi, resource, sigma_xx, sigma_yy, sigma_xy
+ * All values are IEEE double.  The state is resident in device memory
+ * between calls; host copies happen only in fsi_state_upload/_download.
+ *
+ * Error model: every call returns an fsi_status.  The codes map to the
+ * exceptions the reference throws (ArgumentError, DimensionMismatch,
+ * ErrorException, DomainError); fsi_last_error() returns the message of the
+ * last failing call on a context.  A context, its states and workspaces use
+ * one HIP stream and are not safe for concurrent use.
+ */
+#ifndef FSI_API_H
+#define FSI_API_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+typedef enum {
+    FSI_OK = 0,
+    FSI_E_ARGUMENT = 1,    /* ArgumentError    (invalid value, failed stage-1/compatibility check) */
+    FSI_E_DIMENSION = 2,   /* DimensionMismatch                                           */
+    FSI_E_SOLVER = 3,      /* ErrorException   (projection residual / constraint check)   */
+    FSI_E_DOMAIN = 4,      /* DomainError      (non-SPD sigma under geometric extension)  */
+    FSI_E_UNSUPPORTED = 5, /* configuration outside the device path; caller uses the CPU path */
+    FSI_E_HIP = 6          /* device runtime error                                        */
+} fsi_status;
+
+/* ---- boundary conditions (src/BoundaryConditions.jl:26-72) -------------- */
+typedef enum {
+    FSI_BC_PERIODIC = 0,
+    FSI_BC_DIRICHLET = 1,
+    FSI_BC_NEUMANN = 2
+} fsi_bc_kind;
+
+typedef struct {
+    int32_t kind;   /* fsi_bc_kind                                   */
+    double value;   /* scalar datum (Dirichlet value / outward normal derivative) */
+} fsi_bc_side;
+
+typedef struct {
+    fsi_bc_side left, right, bottom, top;
+} fsi_bc_setup;
+
+/* FSIBoundaryConditions (src/TimeIntegrationIMEX.jl:127-147) */
+typedef struct {
+    fsi_bc_setup U, V, correction, phi, resource, sigma_xx, sigma_yy, sigma_xy;
+} fsi_bcs;
+
+/* ---- uniform MAC grid (src/MeshGeneration.jl:179-233, :uniform stretching) */
+typedef struct {
+    int32_t nx, ny;
+    double lx, ly;
+    int32_t periodic_x, periodic_y;
+} fsi_grid;
+
+/* ---- closures (FSIModels, src/TimeIntegrationIMEX.jl:156-208) ----------- */
+typedef enum {
+    FSI_PHASE_UPWIND = 0, /* LegacySplitPhaseFieldMethod(NoInterfaceFlux())  PhaseFieldMethods.jl:408-435 */
+    FSI_PHASE_CDI = 1,    /* CurrentCDIPhaseField                            PhaseFieldMethods.jl:457-475 */
+    FSI_PHASE_ACDI = 2,   /* JainACDIPhaseField                              PhaseFieldMethods.jl:652-671 */
+    FSI_PHASE_UWCDI = 3   /* PirozzoliUWCDIPhaseField(:standard)             PhaseFieldMethods.jl:852-867 */
+} fsi_phase_method;
+
+typedef enum { FSI_VISC_NONE = 0, FSI_VISC_CONSTANT = 1 } fsi_viscosity_kind;
+typedef enum { FSI_ELASTIC_NONE = 0, FSI_ELASTIC_LINEAR = 1 } fsi_elastic_kind;
+typedef enum { FSI_RELAX_NONE = 0, FSI_RELAX_MAXWELL = 1 } fsi_relaxation_kind;
+typedef enum {
+    FSI_FORCE_NONE = 0,
+    FSI_FORCE_CONSTANT = 1,  /* ConstantBodyForce(fx, fy)                        */
+    FSI_FORCE_KOLMOGOROV = 2 /* KolmogorovBodyForce(amplitude, wavenumber; phase) */
+} fsi_body_force_kind;
+
+typedef struct {
+    int32_t phase_method;      /* fsi_phase_method */
+    /* CDI: ConservativeDiffuseInterfaceFlux(inverse_peclet, epsilon; delta_normal) */
+    double cdi_inverse_peclet, cdi_epsilon, cdi_delta_normal;
+    /* ACDI: JainACDIPhaseField(gamma, epsilon; psi_floor, delta_normal, dynamic_gamma) */
+    double acdi_gamma, acdi_epsilon, acdi_psi_floor, acdi_delta_normal;
+    int32_t acdi_dynamic_gamma;
+    /* UWCDI: PirozzoliUWCDIPhaseField(alpha; delta_normal) */
+    double uwcdi_alpha, uwcdi_delta_normal;
+
+    int32_t viscosity_kind;    double viscosity;       /* ConstantViscosity(mu)          */
+    int32_t elastic_kind;      double shear_modulus;   /* LinearInternalTensorStress(G)  */
+    int32_t relaxation_kind;   double relaxation_rate; /* MaxwellRelaxation(rate)        */
+    int32_t body_force_kind;
+    double force_fx, force_fy;                         /* ConstantBodyForce              */
+    double force_amplitude, force_wavenumber, force_phase; /* KolmogorovBodyForce        */
+} fsi_models;
+
+/* ---- numerics (IMEXNumerics, src/TimeIntegrationIMEX.jl:262-301) -------- */
+typedef enum {
+    FSI_EXT_NONE = 0,      /* NoStructureExtension                        */
+    FSI_EXT_IDENTITY = 1,  /* IdentityOutsideStructure(cutoff)            */
+    FSI_EXT_GEOMETRIC = 2  /* GeometricStructureExtension(rate, exponent; safety_cutoff) */
+} fsi_extension_kind;
+
+typedef enum { FSI_PRECISION_FP64 = 0, FSI_PRECISION_MIXED = 1 } fsi_precision;
+
+typedef struct {
+    int32_t extension_kind;
+    double ext_max_rate, ext_exponent, ext_cutoff;
+    /* velocity predictor solve (LinearSolverOptions.reltol / maxiter) */
+    double velocity_rtol;
+    int32_t velocity_maxiter;
+    /* projection (projection_reltol / projection_maxiter) */
+    double projection_rtol;
+    int32_t projection_maxiter;
+    /* stage-1 divergence check (stage1_constraint_atol/rtol, check_stage1_constraint) */
+    double stage1_atol, stage1_rtol;
+    int32_t check_stage1;
+    int32_t precision;         /* fsi_precision */
+} fsi_numerics;
+
+/* ---- results ------------------------------------------------------------ */
+/* StepReport, field order of src/TimeIntegrationIMEX.jl:318-327 */
+typedef struct {
+    double maximum_projection_constraint_linf;
+    double maximum_growth_trace_error;
+    double minimum_phi;
+    double maximum_phi;
+    double minimum_resource;
+    double minimum_sigma_eigenvalue;
+    double maximum_abs_sigma_determinant_error;
+    int64_t non_spd_cells;
+} fsi_step_report;
+
+/* ProjectionResult scalars, src/Projection.jl:31-37 (the correction field
+ * itself stays in the workspace; fsi_workspace_download_correction reads it) */
+typedef struct {
+    double linear_relative_residual;
+    double constraint_linf;
+    double constraint_l2;
+    double compatibility_defect;
+    int32_t iterations;
+} fsi_projection_result;
+
+/* Per-step solver counters, for measurement (no reference counterpart). */
+typedef struct {
+    int32_t velocity_iterations[3];   /* stages 2..4 */
+    int32_t projection_iterations[3]; /* stages 2..4 */
+} fsi_step_stats;
+
+typedef struct fsi_context fsi_context;
+typedef struct fsi_state fsi_state;
+typedef struct fsi_workspace fsi_workspace;
+
+/* ---- entry points --------------------------------------------------------- */
+
+/* build_imex_context, src/TimeIntegrationIMEX.jl:468-493 (periodicity
+ * validation :447-466, projection system + multigrid hierarchy
+ * src/Projection.jl:149-166).  Returns FSI_E_UNSUPPORTED for closures the
+ * device path does not cover. */
+fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bcs,
+                              const fsi_models* models, const fsi_numerics* numerics,
+                              fsi_context** out);
+void fsi_context_destroy(fsi_context* ctx);
+const char* fsi_last_error(const fsi_context* ctx);
+
+/* FSIState, src/TimeIntegrationIMEX.jl:103-118: device-resident 7 fields. */
+fsi_status fsi_state_create(fsi_context* ctx, fsi_state** out);
+void fsi_state_destroy(fsi_state* st);
+/* Copy host arrays (column-major, shapes above) to/from the device.  Any
+ * pointer may be NULL on download to skip that field. */
+fsi_status fsi_state_upload(fsi_context* ctx, fsi_state* st,
+                            const double* U, const double* V, const double* phi,
+                            const double* resource, const double* sigma_xx,
+                            const double* sigma_yy, const double* sigma_xy);
+fsi_status fsi_state_download(fsi_context* ctx, const fsi_state* st,
+                              double* U, double* V, double* phi, double* resource,
+                              double* sigma_xx, double* sigma_yy, double* sigma_xy);
+
+/* IMEXWorkspace, src/TimeIntegrationIMEX.jl:333-441: stage buffers,
+ * histories, solver scratch and the projection warm start `correction`. */
+fsi_status fsi_workspace_create(fsi_context* ctx, fsi_workspace** out);
+void fsi_workspace_destroy(fsi_workspace* ws);
+fsi_status fsi_workspace_download_correction(fsi_context* ctx, const fsi_workspace* ws,
+                                             double* correction);
+fsi_status fsi_workspace_upload_correction(fsi_context* ctx, fsi_workspace* ws,
+                                           const double* correction);
+
+/* make_velocity_compatible!, src/TimeIntegrationIMEX.jl:1088-1109 */
+fsi_status fsi_make_velocity_compatible(fsi_context* ctx, fsi_state* st, fsi_workspace* ws,
+                                        fsi_projection_result* result);
+
+/* integrate_step!, src/TimeIntegrationIMEX.jl:1152-1256.  On any error the
+ * state is left unchanged. `stats` may be NULL. */
+fsi_status fsi_integrate_step(fsi_context* ctx, fsi_state* st, fsi_workspace* ws,
+                              double t, double dt, fsi_step_report* report,
+                              fsi_step_stats* stats);
+
+/* project! on the state's velocity with target divergence G = 0 (NoGrowth),
+ * src/Projection.jl:170-307, warm-started from the workspace correction. */
+fsi_status fsi_project(fsi_context* ctx, fsi_state* st, fsi_workspace* ws,
+                       fsi_projection_result* result);
+
+/* Stage tendencies of the state, as _evaluate_stage_tendencies! computes
+ * them for one stage (src/TimeIntegrationIMEX.jl:944-1071), for kernel
+ * parity: E_u (nx+1)x ny and E_v nx x (ny+1) are the explicit momentum
+ * terms (-material convection + body force + elastic force,
+ * :962-988), I_u / I_v the viscous operator applied to the state velocity
+ * (:1057-1061) on the compact faces (other faces are written as 0),
+ * E_phi the phase_field_rhs (:991-999), E_sxx/E_syy/E_sxy the
+ * structure_tensor_rhs (:1032-1052).  Output pointers may be NULL. */
+fsi_status fsi_eval_tendencies(fsi_context* ctx, const fsi_state* st, fsi_workspace* ws,
+                               double t,
+                               double* E_u, double* E_v, double* I_u, double* I_v,
+                               double* E_phi, double* E_sxx, double* E_syy, double* E_sxy);
+
+/* Measurement hook; no counterpart in the reference (whose code is not
+ * instrumented).  Launches the named kernel `reps` times on the context's
+ * stream with the buffers of the workspace (after at least one step), timed
+ * with HIP events on that stream; returns the mean duration of one launch in
+ * milliseconds and the kernel's algorithmic HBM bytes per launch.  Names:
+ * "copy" (calibration), "vel_dir", "vel_upd", "pcg_dir", "pcg_upd",
+ * "pcg_rz", "mg_jacobi", "mg_resid", "mg_restrict", "mg_prolong",
+ * "momentum", "cells". */
+fsi_status fsi_benchmark_kernel(fsi_context* ctx, fsi_state* st, fsi_workspace* ws, const char* name,
+                                int32_t reps, double* avg_ms, double* bytes_per_launch);
+
+/* Library identification: device name, architecture and build. */
+const char* fsi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FSI_API_H */

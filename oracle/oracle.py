@@ -1,0 +1,233 @@
+"""ctypes binding of the CPU oracle (liboracle_fsi.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product (eulerianfsi_amd)
+never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, c_char_p, c_double, c_int, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+from eulerianfsi_amd import _abi
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle_fsi.so"
+_dp = POINTER(c_double)
+_LIB = None
+
+
+def build_oracle(verbose=False) -> Path:
+    subprocess.run(["make", "-C", str(HERE), "-s"], check=True, capture_output=not verbose)
+    return LIB_PATH
+
+
+def load():
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not LIB_PATH.exists():
+        build_oracle()
+    lib = ctypes.CDLL(str(LIB_PATH))
+    P = c_void_p
+    sig = {
+        "fsio_create": (c_int, [POINTER(_abi.Grid), POINTER(_abi.Bcs), POINTER(_abi.Models), POINTER(_abi.Numerics),
+                                POINTER(P)]),
+        "fsio_destroy": (None, [P]),
+        "fsio_last_error": (c_char_p, [P]),
+        "fsio_integrate_step": (c_int, [P] + [_dp] * 8 + [c_double, c_double, POINTER(_abi.StepReportC),
+                                                          POINTER(_abi.StepStatsC)]),
+        "fsio_make_velocity_compatible": (c_int, [P] + [_dp] * 5 + [POINTER(_abi.ProjectionResultC)]),
+        "fsio_project": (c_int, [P, _dp, _dp, _dp, POINTER(_abi.ProjectionResultC)]),
+        "fsio_eval_tendencies": (c_int, [P] + [_dp] * 6 + [c_double] + [_dp] * 8),
+        "fsio_divergence": (None, [P, _dp, _dp, _dp]),
+        "fsio_scalar_gradient": (None, [P, c_int, _dp, _dp, _dp]),
+        "fsio_velocity_gradient": (c_int, [P] + [_dp] * 6),
+        "fsio_stress_divergence": (None, [P] + [_dp] * 5),
+        "fsio_material_convection": (None, [P] + [_dp] * 5),
+        "fsio_phase_rhs": (c_int, [P] + [_dp] * 5),
+        "fsio_structure_rhs": (c_int, [P] + [_dp] * 9),
+        "fsio_structure_extension_rhs": (c_int, [P, _dp, _dp, _dp, _dp, c_int, _dp, _dp, _dp]),
+        "fsio_viscous_apply": (None, [P, _dp, _dp, c_int]),
+        "fsio_laplacian_apply": (None, [P, _dp, _dp, c_int]),
+        "fsio_velocity_solve": (c_int, [P, c_double, _dp, _dp, _dp, POINTER(c_int)]),
+        "fsio_metric": (_dp, [P, c_int, POINTER(c_int)]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _F(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"[{status}] {msg}")
+        self.status = status
+
+
+class Oracle:
+    """One oracle context for a (metrics, bcs, models, numerics) configuration."""
+
+    def __init__(self, metrics, bcs, models, numerics):
+        self.lib = load()
+        self.metrics = metrics
+        self.cfg = _abi.abi_config(metrics, bcs, models, numerics)
+        g, b, m, n = self.cfg
+        h = c_void_p()
+        st = self.lib.fsio_create(ctypes.byref(g), ctypes.byref(b), ctypes.byref(m), ctypes.byref(n), ctypes.byref(h))
+        self.h = h
+        if st:
+            raise OracleError(st, self.lib.fsio_last_error(h).decode())
+        nx, ny = metrics.Nx, metrics.Ny
+        self.nx, self.ny = nx, ny
+        self.correction = np.zeros((nx, ny), order="F")
+
+    def close(self):
+        if self.h:
+            self.lib.fsio_destroy(self.h)
+            self.h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, st):
+        if st:
+            raise OracleError(st, self.lib.fsio_last_error(self.h).decode())
+
+    # -- step API ---------------------------------------------------------------
+    def integrate_step(self, s: dict, t, dt):
+        """s: dict of Fortran arrays U, V, phi, resource, sigma_xx, sigma_yy, sigma_xy (updated in place)."""
+        rep, stats = _abi.StepReportC(), _abi.StepStatsC()
+        st = self.lib.fsio_integrate_step(self.h, _p(s["U"]), _p(s["V"]), _p(s["phi"]), _p(s["resource"]),
+                                          _p(s["sigma_xx"]), _p(s["sigma_yy"]), _p(s["sigma_xy"]),
+                                          _p(self.correction), float(t), float(dt), ctypes.byref(rep),
+                                          ctypes.byref(stats))
+        self._check(st)
+        return rep, stats
+
+    def make_velocity_compatible(self, s: dict):
+        res = _abi.ProjectionResultC()
+        st = self.lib.fsio_make_velocity_compatible(self.h, _p(s["U"]), _p(s["V"]), _p(s["phi"]), _p(s["resource"]),
+                                                    _p(self.correction), ctypes.byref(res))
+        self._check(st)
+        return res
+
+    def project(self, U, V):
+        res = _abi.ProjectionResultC()
+        st = self.lib.fsio_project(self.h, _p(U), _p(V), _p(self.correction), ctypes.byref(res))
+        self._check(st)
+        return res
+
+    def tendencies(self, s: dict, t=0.0):
+        nx, ny = self.nx, self.ny
+        out = dict(E_u=np.zeros((nx + 1, ny), order="F"), E_v=np.zeros((nx, ny + 1), order="F"),
+                   I_u=np.zeros((nx + 1, ny), order="F"), I_v=np.zeros((nx, ny + 1), order="F"),
+                   E_phi=np.zeros((nx, ny), order="F"), E_sigma_xx=np.zeros((nx, ny), order="F"),
+                   E_sigma_yy=np.zeros((nx, ny), order="F"), E_sigma_xy=np.zeros((nx, ny), order="F"))
+        st = self.lib.fsio_eval_tendencies(self.h, _p(s["U"]), _p(s["V"]), _p(s["phi"]), _p(s["sigma_xx"]),
+                                           _p(s["sigma_yy"]), _p(s["sigma_xy"]), float(t),
+                                           *(_p(out[k]) for k in out))
+        self._check(st)
+        return out
+
+    # -- operators ----------------------------------------------------------------
+    def divergence(self, U, V):
+        out = np.zeros((self.nx, self.ny), order="F")
+        self.lib.fsio_divergence(self.h, _p(_F(U)), _p(_F(V)), _p(out))
+        return out
+
+    def scalar_gradient(self, q, which=0):
+        gx = np.zeros((self.nx + 1, self.ny), order="F")
+        gy = np.zeros((self.nx, self.ny + 1), order="F")
+        self.lib.fsio_scalar_gradient(self.h, which, _p(_F(q)), _p(gx), _p(gy))
+        return gx, gy
+
+    def velocity_gradient(self, U, V):
+        outs = [np.zeros((self.nx, self.ny), order="F") for _ in range(4)]
+        self._check(self.lib.fsio_velocity_gradient(self.h, _p(_F(U)), _p(_F(V)), *(_p(o) for o in outs)))
+        return outs
+
+    def stress_divergence(self, Txx, Tyy, Txy):
+        Fx = np.zeros((self.nx + 1, self.ny), order="F")
+        Fy = np.zeros((self.nx, self.ny + 1), order="F")
+        self.lib.fsio_stress_divergence(self.h, _p(_F(Txx)), _p(_F(Tyy)), _p(_F(Txy)), _p(Fx), _p(Fy))
+        return Fx, Fy
+
+    def material_convection(self, U, V, S=None):
+        S = np.zeros((self.nx, self.ny), order="F") if S is None else _F(S)
+        Cu = np.zeros((self.nx + 1, self.ny), order="F")
+        Cv = np.zeros((self.nx, self.ny + 1), order="F")
+        self.lib.fsio_material_convection(self.h, _p(_F(U)), _p(_F(V)), _p(S), _p(Cu), _p(Cv))
+        return Cu, Cv
+
+    def phase_rhs(self, U, V, phi, S=None):
+        S = np.zeros((self.nx, self.ny), order="F") if S is None else _F(S)
+        out = np.zeros((self.nx, self.ny), order="F")
+        self._check(self.lib.fsio_phase_rhs(self.h, _p(_F(U)), _p(_F(V)), _p(_F(phi)), _p(S), _p(out)))
+        return out
+
+    def structure_rhs(self, sxx, syy, sxy, U, V, phi):
+        outs = [np.zeros((self.nx, self.ny), order="F") for _ in range(3)]
+        self._check(self.lib.fsio_structure_rhs(self.h, _p(_F(sxx)), _p(_F(syy)), _p(_F(sxy)), _p(_F(U)), _p(_F(V)),
+                                                _p(_F(phi)), *(_p(o) for o in outs)))
+        return outs
+
+    def extension_rhs(self, sxx, syy, sxy, phi):
+        sxx, syy, sxy, phi = (np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+                              for a in (sxx, syy, sxy, phi))
+        n = sxx.size
+        outs = [np.zeros(n) for _ in range(3)]
+        self._check(self.lib.fsio_structure_extension_rhs(self.h, _p(sxx), _p(syy), _p(sxy), _p(phi), n,
+                                                          *(_p(o) for o in outs)))
+        return outs
+
+    def viscous_apply(self, uv, with_boundary=True):
+        uv = np.ascontiguousarray(uv, dtype=np.float64)
+        out = np.zeros_like(uv)
+        self.lib.fsio_viscous_apply(self.h, _p(uv), _p(out), int(with_boundary))
+        return out
+
+    def laplacian_apply(self, q, with_boundary=False):
+        out = np.zeros((self.nx, self.ny), order="F")
+        self.lib.fsio_laplacian_apply(self.h, _p(_F(q)), _p(out), int(with_boundary))
+        return out
+
+    METRICS = ("dx_f", "dx_c", "dy_f", "dy_c", "wL_c2fx", "wR_c2fx", "wB_c2fy", "wT_c2fy", "wL_f2cx", "wR_f2cx",
+               "wB_f2cy", "wT_f2cy", "dx_west_corner", "dx_east_corner", "dy_south_corner", "dy_north_corner",
+               "x_p", "y_p", "x_u", "y_v")
+
+    def metric(self, name):
+        n = c_int(0)
+        ptr = self.lib.fsio_metric(self.h, self.METRICS.index(name), ctypes.byref(n))
+        return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+
+    def velocity_solve(self, alpha, rhs, x0=None):
+        rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+        x0 = np.zeros_like(rhs) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
+        x = np.zeros_like(rhs)
+        it = c_int(0)
+        self._check(self.lib.fsio_velocity_solve(self.h, float(alpha), _p(rhs), _p(x0), _p(x), ctypes.byref(it)))
+        return x, it.value
+
+
+def state_dict(state) -> dict:
+    """Fortran-ordered copies of an FSIState's host arrays."""
+    return {k: np.array(getattr(state, k), dtype=np.float64, order="F", copy=True)
+            for k in ("U", "V", "phi", "resource", "sigma_xx", "sigma_yy", "sigma_xy")}

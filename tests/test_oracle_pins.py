@@ -1,0 +1,451 @@
+"""Pins of the CPU oracle (oracle/fsi_oracle.c), on the CPU.
+
+The reference ships no golden vectors for this path, so the oracle is pinned
+by (1) the reference's own known-answer tests re-expressed
+(test/runtests.jl, test/growth_tests.jl), (2) closed forms, (3) invariants,
+and (4) a brute-force sparse assembly of the reference's operators that
+follows src/Operators.jl index by index (tests/reference_assembly.py).
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import eulerianfsi_amd as fsi
+from eulerianfsi_amd import closures as C
+from tests import cases
+from tests import reference_assembly as RA
+
+
+def geo_numerics(**kw):
+    return fsi.IMEXNumerics(structure_extension=C.GeometricStructureExtension(20.0, 4.0, safety_cutoff=1e-6), **kw)
+
+
+# ---------------------------------------------------------------------------
+# (1) reference known-answer tests
+# ---------------------------------------------------------------------------
+class TestGeometricExtension:  # test/runtests.jl:7-76
+    def oracle(self):
+        return cases.oracle_for(cases.periodic_box(2, numerics=geo_numerics()))
+
+    def test_identity_is_fixed_point(self):
+        o = self.oracle()
+        rx, ry, rxy = o.extension_rhs(np.ones(4), np.ones(4), np.zeros(4), [1.0, 0.2, 0.5, 0.0])
+        assert np.all(rx == 0.0) and np.all(ry == 0.0) and np.all(rxy == 0.0)
+
+    def test_unimodular_trace_vanishes(self):
+        o = self.oracle()
+        rx, ry, rxy = o.extension_rhs([2.0], [0.5], [0.0], [0.2])
+        S = np.array([[2.0, 0.0], [0.0, 0.5]])
+        R = np.array([[rx[0], rxy[0]], [rxy[0], ry[0]]])
+        assert abs(np.trace(np.linalg.inv(S) @ R)) <= 1e-12
+        # dissipative for the elastic free energy
+        G = 0.5 * (np.eye(2) - np.linalg.inv(S))
+        assert np.trace(G.T @ R) <= 1e-12
+
+    def test_rotated_spd_closed_form(self):
+        o = self.oracle()
+        th = 0.37
+        Q = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+        lam = np.array([3.0, 1.0 / 3.0])
+        S = Q @ np.diag(lam) @ Q.T
+        exp = Q @ np.diag(lam * np.log(lam)) @ Q.T
+        rx, ry, rxy = o.extension_rhs([S[0, 0]], [S[1, 1]], [S[0, 1]], [0.3])
+        kappa = 20.0 * (1.0 - 0.3) ** 4.0
+        assert rx[0] == pytest.approx(-kappa * exp[0, 0], abs=1e-12)
+        assert ry[0] == pytest.approx(-kappa * exp[1, 1], abs=1e-12)
+        assert rxy[0] == pytest.approx(-kappa * exp[0, 1], abs=1e-12)
+
+    def test_repeated_eigenvalue_hermite_limit(self):
+        o = self.oracle()
+        rx, ry, rxy = o.extension_rhs([2.0], [2.0], [0.0], [0.0])
+        # S log S for S = 2 I is 2 log 2 I; kappa(phi=0) = 20
+        assert rx[0] == pytest.approx(-20.0 * 2.0 * math.log(2.0), rel=1e-14)
+        assert ry[0] == pytest.approx(-20.0 * 2.0 * math.log(2.0), rel=1e-14)
+        assert rxy[0] == 0.0
+
+    def test_non_spd_is_a_domain_error(self):
+        from oracle.oracle import OracleError
+        o = self.oracle()
+        with pytest.raises(OracleError) as e:
+            o.extension_rhs([1.0], [1.0], [2.0], [0.5])
+        assert e.value.status == 4
+
+
+def test_maxwell_relaxation_values():  # test/runtests.jl:78-87
+    cfg = cases.periodic_box(2, models=fsi.FSIModels(tensor_relaxation=C.MaxwellRelaxation(0.5)))
+    o = cases.oracle_for(cfg)
+    z = np.zeros((3, 2)), np.zeros((2, 3))
+    rx, ry, rxy = o.structure_rhs(np.full((2, 2), 2.0), np.full((2, 2), 0.5), np.full((2, 2), 0.1),
+                                  z[0], z[1], np.ones((2, 2)))
+    assert np.all(rx == -0.5) and np.all(ry == 0.25) and np.all(rxy == -0.05)
+
+
+def test_kolmogorov_body_force():  # test/runtests.jl:90-119
+    grid = fsi.generate_grid(6.0, 2.0, 24, 8)
+    metrics = fsi.build_mesh_metrics(grid, periodic_x=True)
+    p, w, n0 = fsi.Periodic(), fsi.Dirichlet(0.0), fsi.Neumann(0.0)
+    bu = fsi.BoundaryConditionSetup(p, p, w, w)
+    sc = fsi.BoundaryConditionSetup(p, p, n0, n0)
+    ky = 2 * math.pi * 4 / 2.0
+    amp = ky ** 2 / 100.0
+    models = fsi.FSIModels(body_force=C.KolmogorovBodyForce(amp, ky))
+    o = cases.oracle_for((grid, metrics, fsi.FSIBoundaryConditions(bu, bu, sc, sc, {"xx": sc, "yy": sc, "xy": sc}),
+                          models, fsi.IMEXNumerics()))
+    s = dict(U=np.zeros((25, 8), order="F"), V=np.zeros((24, 9), order="F"), phi=np.zeros((24, 8), order="F"),
+             sigma_xx=np.ones((24, 8), order="F"), sigma_yy=np.ones((24, 8), order="F"),
+             sigma_xy=np.zeros((24, 8), order="F"))
+    out = o.tendencies(s)
+    assert np.all(out["E_v"] == 0.0)
+    y = 0.0
+    for j in range(8):
+        y += 0.5 * metrics.dy_c[j]
+        assert np.max(np.abs(out["E_u"][:, j] - amp * math.sin(ky * y))) <= 1e-12
+        y += 0.5 * metrics.dy_c[j]
+
+
+PHASE_MODELS = {
+    "cdi": C.CurrentCDIPhaseField(4.0, 2 / 16, delta_normal=1e-6),
+    "acdi": C.JainACDIPhaseField(0.2, 0.51 / 16),
+    "uwcdi": C.PirozzoliUWCDIPhaseField(1.0),
+}
+
+
+def _uniform_flow(N=16):
+    U = np.full((N + 1, N), 0.2, order="F")
+    V = np.full((N, N + 1), -0.1, order="F")
+    return U, V
+
+
+@pytest.mark.parametrize("name", sorted(PHASE_MODELS))
+def test_phase_constant_field_is_stationary(name):  # test/runtests.jl:181-187
+    o = cases.oracle_for(cases.periodic_box(16, models=fsi.FSIModels(phase_field_method=PHASE_MODELS[name])))
+    U, V = _uniform_flow()
+    rhs = o.phase_rhs(U, V, np.full((16, 16), 0.37))
+    assert np.max(np.abs(rhs)) <= 1e-11
+
+
+@pytest.mark.parametrize("name", sorted(PHASE_MODELS))
+def test_phase_periodic_integral_vanishes(name):  # test/runtests.jl:189-202
+    cfg = cases.periodic_box(16, models=fsi.FSIModels(phase_field_method=PHASE_MODELS[name]))
+    o = cases.oracle_for(cfg)
+    g, m = cfg[0], cfg[1]
+    X, Y = np.meshgrid(g.x_p, g.y_p, indexing="ij")
+    phi = 0.5 + 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    U, V = _uniform_flow()
+    rhs = o.phase_rhs(U, V, phi)
+    assert abs(np.sum(rhs * np.outer(m.dx_c, m.dy_c))) <= 1e-11
+    assert np.max(np.abs(rhs)) > 1e-3  # the field is genuinely transported
+
+
+def test_dynamic_acdi_vanishes_at_rest():  # test/runtests.jl:204-209
+    model = C.JainACDIPhaseField(1.0, 0.51 / 16, dynamic_gamma=True)
+    cfg = cases.periodic_box(16, models=fsi.FSIModels(phase_field_method=model))
+    o = cases.oracle_for(cfg)
+    X, Y = np.meshgrid(cfg[0].x_p, cfg[0].y_p, indexing="ij")
+    phi = 0.5 + 0.2 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y)
+    rhs = o.phase_rhs(np.zeros((17, 16)), np.zeros((16, 17)), phi)
+    assert np.max(np.abs(rhs)) <= 1e-12
+
+
+def test_ftac_rest_state_is_preserved():  # test/growth_tests.jl:144-162
+    from oracle.oracle import state_dict
+    cfg = cases.periodic_box(8)
+    o = cases.oracle_for(cfg)
+    st = fsi.FSIState(np.zeros((9, 8)), np.zeros((8, 9)), np.full((8, 8), 0.5), np.ones((8, 8)), np.ones((8, 8)),
+                      np.zeros((8, 8)))
+    s = state_dict(st)
+    assert np.all(s["resource"] == 0.0)
+    o.make_velocity_compatible(s)
+    rep, _ = o.integrate_step(s, 0.0, 1e-4)
+    assert rep.maximum_growth_trace_error == 0.0
+    assert np.max(np.abs(s["U"])) < 1e-9 and np.max(np.abs(s["V"])) < 1e-9
+    assert np.max(np.abs(s["phi"] - 0.5)) < 1e-9
+
+
+def test_projection_open_box_meets_constraint():  # test/growth_tests.jl:73-94 (open-box part)
+    d = fsi.Dirichlet(0.0)
+    cfg = cases.closed_box(8, correction=fsi.BoundaryConditionSetup(d, d, d, d),
+                           numerics=fsi.IMEXNumerics(projection_reltol=1e-12))
+    o = cases.oracle_for(cfg)
+    rng = np.random.default_rng(0)
+    U = np.asfortranarray(rng.standard_normal((9, 8)))
+    V = np.asfortranarray(rng.standard_normal((8, 9)))
+    res = o.project(U, V)
+    assert res.constraint_linf < 1e-7
+    assert np.max(np.abs(o.divergence(U, V))) < 1e-7
+
+
+def test_projection_incompatible_flux_is_an_argument_error():  # test/growth_tests.jl:76-83
+    from oracle.oracle import OracleError
+    n0 = fsi.Neumann(0.0)
+    cfg = cases.closed_box(8, correction=fsi.BoundaryConditionSetup(n0, fsi.Neumann(1.0), n0, n0))
+    o = cases.oracle_for(cfg)
+    with pytest.raises(OracleError) as e:
+        o.project(np.zeros((9, 8), order="F"), np.zeros((8, 9), order="F"))
+    assert e.value.status == 1
+
+
+# ---------------------------------------------------------------------------
+# (2) closed forms
+# ---------------------------------------------------------------------------
+def test_metrics_match_uniform_closed_form():
+    o = cases.oracle_for(cases.closed_box(8))
+    h = 1.0 / 8
+    assert np.all(o.metric("dx_c") == h) and np.all(o.metric("dx_f") == h)
+    assert np.all(o.metric("wL_f2cx") == 0.5) and np.all(o.metric("wL_c2fx")[1:-1] == 0.5)
+    assert o.metric("wL_c2fx")[0] == 0.0 and o.metric("wR_c2fx")[0] == 1.0
+    assert np.all(o.metric("dx_east_corner") == h / 2) and np.all(o.metric("dy_south_corner") == h / 2)
+
+
+def test_explicit_tableau_is_third_order():
+    A = np.array(fsi.A_EXPLICIT)
+    c = np.array(fsi.STAGE_ABSCISSA[:3])
+    b = A[3, :3]
+    assert np.allclose(A[:, :].sum(axis=1), fsi.STAGE_ABSCISSA)
+    assert b.sum() == pytest.approx(1.0) and b @ c == pytest.approx(0.5)
+    assert b @ c ** 2 == pytest.approx(1 / 3) and b @ (A[:3, :3] @ c) == pytest.approx(1 / 6)
+
+
+def test_step_reproduces_rk3_stability_polynomial():
+    """Zero velocity and Maxwell relaxation make the sigma update the linear
+    ODE y' = -k y; one step must multiply y by R(z) = 1 + z + z^2/2 + z^3/6."""
+    from oracle.oracle import state_dict
+    k, dt = 3.0, 0.05
+    cfg = cases.periodic_box(4, models=fsi.FSIModels(tensor_relaxation=C.MaxwellRelaxation(k)))
+    o = cases.oracle_for(cfg)
+    st = fsi.FSIState(np.zeros((5, 4)), np.zeros((4, 5)), np.full((4, 4), 0.3), np.full((4, 4), 1.4),
+                      np.full((4, 4), 0.9), np.full((4, 4), 0.25))
+    s = state_dict(st)
+    o.integrate_step(s, 0.0, dt)
+    z = -k * dt
+    R = 1 + z + z * z / 2 + z ** 3 / 6
+    assert np.allclose(s["sigma_xy"], 0.25 * R, rtol=0, atol=1e-15)
+    assert np.allclose(s["sigma_xx"] - 1.0, 0.4 * R, rtol=0, atol=1e-15)
+    assert np.allclose(s["sigma_yy"] - 1.0, -0.1 * R, rtol=0, atol=1e-15)
+    assert np.all(s["phi"] == 0.3)
+
+
+def test_simple_shear_structure_rhs():
+    gamma = 0.7
+    cfg = cases.shear_channel(8, gamma)
+    o = cases.oracle_for(cfg)
+    g = cfg[0]
+    U = np.asfortranarray(np.tile(gamma * g.y_p, (9, 1)))
+    V = np.zeros((8, 9), order="F")
+    a, d, b = 1.3, 0.8, 0.2
+    rx, ry, rxy = o.structure_rhs(np.full((8, 8), a), np.full((8, 8), d), np.full((8, 8), b), U, V, np.ones((8, 8)))
+    assert np.allclose(rx, 2 * gamma * b, rtol=1e-13, atol=1e-13)
+    assert np.allclose(ry, 0.0, atol=1e-13)
+    assert np.allclose(rxy, gamma * d, rtol=1e-13, atol=1e-13)
+    du_dx, du_dy, dv_dx, dv_dy = o.velocity_gradient(U, V)
+    assert np.allclose(du_dy, gamma, rtol=1e-13) and np.all(du_dx == 0) and np.all(dv_dx == 0)
+
+
+def test_stress_divergence_exact_on_linear_fields():
+    cfg = cases.closed_box(8)
+    o = cases.oracle_for(cfg)
+    g = cfg[0]
+    X, Y = np.meshgrid(g.x_p, g.y_p, indexing="ij")
+    a, b1, b2, c = 0.7, -0.4, 1.3, 2.1
+    Fx, Fy = o.stress_divergence(a * X, c * Y, b1 * X + b2 * Y)
+    assert np.allclose(Fx, a + b2, rtol=1e-12, atol=1e-12)
+    assert np.allclose(Fy, c + b1, rtol=1e-12, atol=1e-12)
+
+
+def test_gradient_and_divergence_exact_on_linear_fields():
+    cfg = cases.closed_box(8)
+    o = cases.oracle_for(cfg)
+    g = cfg[0]
+    X, Y = np.meshgrid(g.x_p, g.y_p, indexing="ij")
+    gx, gy = o.scalar_gradient(0.3 * X - 1.7 * Y)
+    assert np.allclose(gx[1:-1, :], 0.3, atol=1e-12) and np.allclose(gy[:, 1:-1], -1.7, atol=1e-12)
+    assert np.all(gx[0, :] == 0.0) and np.all(gy[:, -1] == 0.0)  # homogeneous Neumann faces
+    XU, YU = np.meshgrid(g.x_u, g.y_p, indexing="ij")
+    XV, YV = np.meshgrid(g.x_p, g.y_v, indexing="ij")
+    div = o.divergence(2.0 * XU + YU, -0.5 * YV + XV)
+    assert np.allclose(div, 1.5, atol=1e-12)
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_neumann_laplacian_eigenpairs(k):
+    N = 16
+    cfg = cases.closed_box(N)
+    o = cases.oracle_for(cfg)
+    g = cfg[0]
+    q = np.asfortranarray(np.tile(np.cos(k * np.pi * g.x_p)[:, None], (1, N)))
+    lam = -(2.0 - 2.0 * math.cos(k * math.pi / N)) * N * N
+    assert np.allclose(o.laplacian_apply(q), lam * q, atol=1e-10 * abs(lam))
+
+
+# ---------------------------------------------------------------------------
+# (3) invariants
+# ---------------------------------------------------------------------------
+def _streamfunction_velocity(g, amp=0.3, periodic=True):
+    X, Y = np.meshgrid(g.x_u, g.y_v, indexing="ij")
+    if periodic:
+        psi = amp * (np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y) + 0.5 * np.cos(2 * np.pi * (X + Y)))
+    else:
+        psi = amp * np.sin(np.pi * X) ** 2 * np.sin(np.pi * Y) ** 2
+    U = (psi[:, 1:] - psi[:, :-1]) * g.Ny
+    V = -(psi[1:, :] - psi[:-1, :]) * g.Nx
+    return np.asfortranarray(U), np.asfortranarray(V)
+
+
+def test_mac_convection_conserves_energy_periodic():
+    cfg = cases.periodic_box(16)
+    o = cases.oracle_for(cfg)
+    U, V = _streamfunction_velocity(cfg[0])
+    assert np.max(np.abs(o.divergence(U, V))) < 1e-12
+    Cu, Cv = o.material_convection(U, V)
+    e = np.sum(U[:-1, :] * Cu[:-1, :]) + np.sum(V[:, :-1] * Cv[:, :-1])
+    assert abs(e) <= 1e-12 * (np.sum(np.abs(U[:-1] * Cu[:-1])) + np.sum(np.abs(V[:, :-1] * Cv[:, :-1])))
+    assert np.max(np.abs(Cu)) > 1e-2
+
+
+def test_projection_is_idempotent_and_keeps_solenoidal_fields():
+    cfg = cases.closed_box(16, numerics=fsi.IMEXNumerics(projection_reltol=1e-13))
+    o = cases.oracle_for(cfg)
+    U0, V0 = _streamfunction_velocity(cfg[0], periodic=False)
+    U, V = U0.copy(order="F"), V0.copy(order="F")
+    o.project(U, V)
+    assert np.max(np.abs(U - U0)) < 1e-12 and np.max(np.abs(V - V0)) < 1e-12
+    rng = np.random.default_rng(5)
+    U = np.asfortranarray(rng.standard_normal((17, 16)))
+    V = np.asfortranarray(rng.standard_normal((16, 17)))
+    U[0, :] = U[-1, :] = 0.0
+    V[:, 0] = V[:, -1] = 0.0
+    o.correction[:] = 0.0
+    o.project(U, V)
+    U1, V1 = U.copy(order="F"), V.copy(order="F")
+    # P(P v + grad f) = P v: the projection removes any discrete gradient.
+    # (Re-projecting P v itself would leave a round-off right-hand side that
+    # the reference's own true-residual check, Projection.jl:279-285, rejects.)
+    gx, gy = o.scalar_gradient(rng.standard_normal((16, 16)))
+    U, V = np.asfortranarray(U1 + gx), np.asfortranarray(V1 + gy)
+    o.correction[:] = 0.0
+    o.project(U, V)
+    assert np.max(np.abs(U - U1)) < 1e-11 and np.max(np.abs(V - V1)) < 1e-11
+
+
+@pytest.mark.parametrize("method", ["acdi", "cdi", "uwcdi"])
+def test_cavity_step_conserves_phase_volume(method):
+    from eulerianfsi_amd import setups
+    from oracle.oracle import Oracle, state_dict
+    case = setups.seeded_cavity(32, method, seed=3, velocity_rtol=1e-13, projection_rtol=1e-13)
+    o = Oracle(case.metrics, case.bcs, case.models, case.numerics)
+    s = state_dict(case.state)
+    v0 = s["phi"].sum()
+    for k in range(4):
+        rep, _ = o.integrate_step(s, k * case.dt, case.dt)
+    assert abs(s["phi"].sum() - v0) <= 1e-13 * v0
+    assert rep.maximum_projection_constraint_linf < 1e-10
+    assert np.max(np.abs(o.divergence(s["U"], s["V"]))) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# (4) brute force: the reference's sparse assembly vs the matrix-free oracle
+# ---------------------------------------------------------------------------
+VISC_CFGS = {
+    "cavity": lambda: cases.closed_box(7),
+    "channel": lambda: cases.shear_channel(6, 0.4, models=fsi.FSIModels(viscosity=C.ConstantViscosity(0.03))),
+    "periodic": lambda: cases.periodic_box(6, models=fsi.FSIModels(viscosity=C.ConstantViscosity(0.02))),
+}
+
+
+@pytest.mark.parametrize("name", sorted(VISC_CFGS))
+def test_viscous_operator_matches_sparse_assembly(name):
+    cfg = VISC_CFGS[name]()
+    _, metrics, bcs, models, _ = cfg
+    o = cases.oracle_for(cfg)
+    m = RA.Met(o)
+    M, b = RA.viscous_operator(m, models.viscosity.value, bcs.U, bcs.V, metrics.periodic_x, metrics.periodic_y)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(2 * o.nx * o.ny)
+    assert np.allclose(o.viscous_apply(x, True), M @ x + b, rtol=1e-13, atol=1e-10)
+    assert np.allclose(o.viscous_apply(x, False), M @ x, rtol=1e-13, atol=1e-10)
+    # The assembled M couples wall-normal Dirichlet unknowns (U-left, V-bottom)
+    # non-symmetrically; those rows become identity rows and their columns move
+    # to the right-hand side, and the remaining block is symmetric negative
+    # definite on uniform grids, which is what makes the predictor solve CG-able.
+    idx, _ = RA.dirichlet_rows(o.nx, o.ny, bcs.U, bcs.V)
+    free = np.setdiff1d(np.arange(2 * o.nx * o.ny), idx)
+    Mf = M.toarray()[np.ix_(free, free)]
+    assert np.allclose(Mf, Mf.T, rtol=0, atol=1e-12 * np.max(np.abs(Mf)))
+    assert np.max(np.linalg.eigvalsh(0.5 * (Mf + Mf.T))) <= 1e-9
+
+
+@pytest.mark.parametrize("name", ["cavity", "channel"])
+def test_velocity_solve_matches_direct_solve(name):
+    cfg = VISC_CFGS[name]()
+    _, metrics, bcs, models, _ = cfg
+    cfg = (cfg[0], cfg[1], cfg[2], cfg[3], fsi.IMEXNumerics(
+        linear_solvers=fsi.LinearSolverOptions(reltol=1e-15, maxiter=2000)))
+    o = cases.oracle_for(cfg)
+    m = RA.Met(o)
+    M, b = RA.viscous_operator(m, models.viscosity.value, bcs.U, bcs.V, metrics.periodic_x, metrics.periodic_y)
+    n = 2 * o.nx * o.ny
+    alpha = 0.7
+    rng = np.random.default_rng(2)
+    rhs = rng.standard_normal(n)
+    # _shifted_identity + _impose_row_dirichlet, TimeIntegrationIMEX.jl:643-672
+    A = np.eye(n) - alpha * M.toarray()
+    r = rhs + alpha * b
+    idx, val = RA.dirichlet_rows(o.nx, o.ny, bcs.U, bcs.V)
+    for k, v in zip(idx, val):
+        A[k, :] = 0.0
+        A[k, k] = 1.0
+        r[k] = v
+    x_ref = np.linalg.solve(A, r)
+    x, it = o.velocity_solve(alpha, rhs)
+    assert np.allclose(x, x_ref, rtol=0, atol=1e-12 * np.max(np.abs(x_ref)))
+
+
+@pytest.mark.parametrize("corr", ["neumann", "dirichlet", "periodic"])
+def test_laplacian_and_pinned_projection_match_assembly(corr):
+    if corr == "periodic":
+        cfg = cases.periodic_box(8, numerics=fsi.IMEXNumerics(projection_reltol=1e-14))
+    else:
+        side = fsi.Neumann(0.0) if corr == "neumann" else fsi.Dirichlet(0.0)
+        cfg = cases.closed_box(8, correction=fsi.BoundaryConditionSetup(side, side, side, side),
+                               numerics=fsi.IMEXNumerics(projection_reltol=1e-14))
+    _, metrics, bcs, _, _ = cfg
+    o = cases.oracle_for(cfg)
+    m = RA.Met(o)
+    px, py = metrics.periodic_x, metrics.periodic_y
+    A, bb = RA.pressure_laplacian(m, bcs.correction, px, py)
+    rng = np.random.default_rng(4)
+    q = rng.standard_normal((8, 8))
+    assert np.allclose(o.laplacian_apply(q).ravel(order="F"), A @ q.ravel(order="F"), rtol=1e-13, atol=1e-9)
+    # project! with the gauge-fixed matrix (Projection.jl:106-113, 223-259), dense
+    U = np.asfortranarray(rng.standard_normal((9, 8)))
+    V = np.asfortranarray(rng.standard_normal((8, 9)))
+    if not px:
+        U[0, :] = U[-1, :] = 0.0
+    else:
+        U[-1, :] = U[0, :]
+    if not py:
+        V[:, 0] = V[:, -1] = 0.0
+    else:
+        V[:, -1] = V[:, 0]
+    Gx, Gy, Dx, Dy, gbx, gby = RA.scalar_operators(m, bcs.correction, px, py)
+    div = Dx @ U.ravel(order="F") + Dy @ V.ravel(order="F")
+    rhs = div - bb
+    Ad = A.toarray()
+    nullspace = corr != "dirichlet"
+    if nullspace:
+        Ad[0, :] = 0.0
+        Ad[:, 0] = 0.0
+        Ad[0, 0] = 1.0
+        rhs = rhs.copy()
+        rhs[0] = 0.0
+    x = np.linalg.solve(Ad, rhs)
+    if nullspace:
+        x -= x.mean()
+    U_ref = U.ravel(order="F") - (Gx @ x + gbx)
+    V_ref = V.ravel(order="F") - (Gy @ x + gby)
+    o.project(U, V)
+    assert np.allclose(U.ravel(order="F"), U_ref, atol=1e-11)
+    assert np.allclose(V.ravel(order="F"), V_ref, atol=1e-11)
+    assert np.allclose(o.correction.ravel(order="F"), x, atol=1e-11)
