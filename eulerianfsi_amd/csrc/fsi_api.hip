@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -58,6 +59,8 @@ struct fsi_context {
     std::vector<MgGeom> geo;
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
+    int mg_fused = 1;   // fused LDS-tiled V-cycle level kernels (FSI_MG_SIMPLE=1: one kernel per operation)
+    int vel_fused = 1;  // single-reduction predictor CG (FSI_CG_SIMPLE=1: textbook two-kernel CG)
 };
 
 struct fsi_state {
@@ -73,8 +76,8 @@ struct fsi_workspace {
     double *Us = nullptr, *Vs = nullptr, *phis = nullptr, *sxxs = nullptr, *syys = nullptr, *sxys = nullptr;
     double *Eu[3] = {}, *Ev[3] = {}, *Iu[3] = {}, *Iv[3] = {};
     double *Ep[3] = {}, *Exx[3] = {}, *Eyy[3] = {}, *Exy[3] = {};
-    double *ru = nullptr, *rv = nullptr, *r_u = nullptr, *r_v = nullptr;
-    double *p_u[2] = {}, *p_v[2] = {}, *q_u = nullptr, *q_v = nullptr;
+    double *ru = nullptr, *rv = nullptr, *r_u[2] = {}, *r_v[2] = {};
+    double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
     double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
     int corr_cur = 0;
     std::vector<MgLevel> lv;
@@ -248,6 +251,12 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     *out = c;
     if (!grid || !bcs || !models || !numerics) return fail(c, FSI_E_ARGUMENT, "null configuration pointer");
     c->grid = *grid;
+    {
+        const char* e = std::getenv("FSI_MG_SIMPLE");
+        c->mg_fused = (e && e[0] == '1') ? 0 : 1;
+        const char* f = std::getenv("FSI_CG_SIMPLE");
+        c->vel_fused = (f && f[0] == '1') ? 0 : 1;
+    }
     c->bcs = *bcs;
     c->models = *models;
     c->num = *numerics;
@@ -496,9 +505,10 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
         return e;
     };
     const size_t F = c->felems;
-    double** singles[] = {&w->Us, &w->Vs, &w->phis, &w->sxxs, &w->syys, &w->sxys, &w->ru,  &w->rv,
-                          &w->r_u, &w->r_v, &w->p_u[0], &w->p_u[1], &w->p_v[0], &w->p_v[1], &w->q_u,
-                          &w->q_v, &w->rhs, &w->corr[0], &w->corr[1], &w->r, &w->z, &w->pp[0], &w->pp[1], &w->w};
+    double** singles[] = {&w->Us,     &w->Vs,     &w->phis,   &w->sxxs,   &w->syys,    &w->sxys,   &w->ru,
+                          &w->rv,     &w->r_u[0], &w->r_u[1], &w->r_v[0], &w->r_v[1],  &w->p_u[0], &w->p_u[1],
+                          &w->p_v[0], &w->p_v[1], &w->q_u[0], &w->q_u[1], &w->q_v[0],  &w->q_v[1], &w->rhs,
+                          &w->corr[0], &w->corr[1], &w->r, &w->z, &w->pp[0], &w->pp[1], &w->w};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -594,24 +604,26 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     b.rv = w->rv;
     b.xu = w->Us;
     b.xv = w->Vs;
-    b.r_u = w->r_u;
-    b.r_v = w->r_v;
-    b.p_u[0] = w->p_u[0];
-    b.p_u[1] = w->p_u[1];
-    b.p_v[0] = w->p_v[0];
-    b.p_v[1] = w->p_v[1];
-    b.q_u = w->q_u;
-    b.q_v = w->q_v;
+    for (int k = 0; k < 2; ++k) {
+        b.r_u[k] = w->r_u[k];
+        b.r_v[k] = w->r_v[k];
+        b.p_u[k] = w->p_u[k];
+        b.p_v[k] = w->p_v[k];
+        b.q_u[k] = w->q_u[k];
+        b.q_v[k] = w->q_v[k];
+    }
     b.partials = c->partials;
     if (!p.visc) {
         launch_vel_copy(c->stream, p, b);
         return FSI_OK;
     }
     launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter);
+    if (c->vel_fused) launch_vel_first(c->stream, p, b, alpha, c->st);
     int total = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
     while (total < c->num.velocity_maxiter) {
         int n = std::min(chunk, c->num.velocity_maxiter - total);
-        launch_vel_iters(c->stream, p, b, alpha, c->st, n, slot);
+        if (c->vel_fused) launch_vel_fused(c->stream, p, b, alpha, c->st, n, slot);
+        else launch_vel_iters(c->stream, p, b, alpha, c->st, n, slot);
         total += n;
         fsi_status rc = poll_done(c, &c->st->vel, &done, &iter);
         if (rc != FSI_OK) return rc;
@@ -644,7 +656,8 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     int total = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
     while (total < c->num.projection_maxiter) {
         int n = std::min(chunk, c->num.projection_maxiter - total);
-        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->st, n, slot);
+        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->st, n, slot,
+                         c->mg_fused);
         total += n;
         fsi_status rc = poll_done(c, &c->st->pcg, &done, &iter);
         if (rc != FSI_OK) return rc;
@@ -860,14 +873,14 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
     vb.rv = w->rv;
     vb.xu = w->Us;
     vb.xv = w->Vs;
-    vb.r_u = w->r_u;
-    vb.r_v = w->r_v;
-    vb.p_u[0] = w->p_u[0];
-    vb.p_u[1] = w->p_u[1];
-    vb.p_v[0] = w->p_v[0];
-    vb.p_v[1] = w->p_v[1];
-    vb.q_u = w->q_u;
-    vb.q_v = w->q_v;
+    for (int k = 0; k < 2; ++k) {
+        vb.r_u[k] = w->r_u[k];
+        vb.r_v[k] = w->r_v[k];
+        vb.p_u[k] = w->p_u[k];
+        vb.p_v[k] = w->p_v[k];
+        vb.q_u[k] = w->q_u[k];
+        vb.q_v[k] = w->q_v[k];
+    }
     vb.partials = c->partials;
     ProjBufs pb{};
     pb.U = w->Us;

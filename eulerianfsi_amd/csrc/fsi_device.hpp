@@ -78,6 +78,7 @@ enum DevError : int {
 struct KrylovScalars {
     double rr, rz, rz_old, pq, alpha, beta, bnorm2, tol2;
     int done, iter, maxiter, pad;
+    double rq, qq;  // r.q and q.q of the single-reduction CG recurrence
 };
 
 // Projection scalars (src/Projection.jl:170-307).

@@ -71,17 +71,20 @@ struct VelBufs {
     const double* rv;
     double* xu;  // solution, written into the stage velocity arrays
     double* xv;
-    double* r_u;
-    double* r_v;
+    double* r_u[2];  // residual (ping-pong for the fused iteration)
+    double* r_v[2];
     double* p_u[2];
     double* p_v[2];
-    double* q_u;
-    double* q_v;
+    double* q_u[2];  // q = A p
+    double* q_v[2];
     double* partials;
 };
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
                      int maxiter);
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
+                      int slot);
+void launch_vel_first(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st);
+void launch_vel_fused(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
                       int slot);
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
 
@@ -104,11 +107,18 @@ void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b, Status* 
                      int maxiter);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
-                      int nlev, Status* st, int count, int slot);
+                      int nlev, Status* st, int count, int slot, int fused);
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly);
 
 int grid_blocks(const Params& p);
+
+// ---- fused multigrid levels (fsi_mg.hip) ----
+void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
+                    double omega, const int* done);
+void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
+                  const double* ec, double* xout, double omega, const int* done, Status* st, double* partials);
+int mg_up_blocks(const MgGeom& gf);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
                     const MgLevel* lv, const MgGeom* geo, Status* st);
 

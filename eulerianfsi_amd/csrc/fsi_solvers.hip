@@ -101,8 +101,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double 
         }
         b.xu[k] = x_u;
         b.xv[k] = x_v;
-        b.r_u[k] = ru;
-        b.r_v[k] = rv;
+        b.r_u[0][k] = ru;
+        b.r_v[0][k] = rv;
         acc[0] += ru * ru + rv * rv;
     }
     if (grid_sum<2>(acc, b.partials, nblocks, &st->counters[0]) && lane_id() == 0) {
@@ -128,11 +128,11 @@ __global__ void __launch_bounds__(BLOCK) k_vel_dir(Params p, VelBufs b, double a
     double* pnv = b.p_v[(it + 1) & 1];
     auto pu = [&](int a, int c) {
         size_t k = at(p, a, c);
-        return it == 0 ? b.r_u[k] : b.r_u[k] + beta * pou[k];
+        return it == 0 ? b.r_u[0][k] : b.r_u[0][k] + beta * pou[k];
     };
     auto pv = [&](int a, int c) {
         size_t k = at(p, a, c);
-        return it == 0 ? b.r_v[k] : b.r_v[k] + beta * pov[k];
+        return it == 0 ? b.r_v[0][k] : b.r_v[0][k] + beta * pov[k];
     };
     double acc[1] = {0.0};
     FOR_TILE(p) {
@@ -142,8 +142,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_dir(Params p, VelBufs b, double a
         double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, pu, pv, i, j, 0);
         pnu[k] = vu;
         pnv[k] = vv;
-        b.q_u[k] = qu;
-        b.q_v[k] = qv;
+        b.q_u[0][k] = qu;
+        b.q_v[0][k] = qv;
         acc[0] += vu * qu + vv * qv;
     }
     if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[1]) && lane_id() == 0) {
@@ -164,10 +164,10 @@ __global__ void __launch_bounds__(BLOCK) k_vel_upd(Params p, VelBufs b, Status* 
         const size_t k = at(p, i, j);
         b.xu[k] += a * pu[k];
         b.xv[k] += a * pv[k];
-        double ru = b.r_u[k] - a * b.q_u[k];
-        double rv = b.r_v[k] - a * b.q_v[k];
-        b.r_u[k] = ru;
-        b.r_v[k] = rv;
+        double ru = b.r_u[0][k] - a * b.q_u[0][k];
+        double rv = b.r_v[0][k] - a * b.q_v[0][k];
+        b.r_u[0][k] = ru;
+        b.r_v[0][k] = rv;
         acc[0] += ru * ru + rv * rv;
     }
     if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[2]) && lane_id() == 0) {
@@ -181,6 +181,98 @@ __global__ void __launch_bounds__(BLOCK) k_vel_upd(Params p, VelBufs b, Status* 
             s.done = 1;
             // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
             if (s.rr > 1e-12 * s.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(s.rr / s.bnorm2), (double)s.iter);
+        }
+    }
+}
+
+// ---- single-reduction CG: one kernel per iteration ----------------------
+// Iteration k+1 forms r_{k+1} = r_k - a_k q_k and p_{k+1} = r_{k+1} + b_{k+1} p_k
+// on the fly (own point and stencil neighbours), q_{k+1} = A p_{k+1},
+// x_{k+1} = x_k + a_k p_k, and the dot products p.q, r.q, q.q, r.r.  The
+// next beta comes from rr_{k+2} = rr_{k+1} - 2 a r.q + a^2 q.q; convergence is
+// tested on the true r.r.  r, p, q are ping-ponged; x is updated in place.
+__device__ __forceinline__ void vel_finish_scalars(KrylovScalars& s, double pq, double rq, double qq, double rr) {
+    s.pq = pq;
+    s.rr = rr;
+    s.alpha = rr / pq;
+    double next = rr - 2.0 * s.alpha * rq + s.alpha * s.alpha * qq;
+    s.beta = fmax(next, 0.0) / rr;
+    s.rq = rq;
+    s.qq = qq;
+}
+
+// p_0 = r_0, q_0 = A p_0 and the first scalars
+__global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double alpha, Status* st, int nblocks) {
+    if (st->vel.done) return;
+    auto pu = [&](int a, int c) { return b.r_u[0][at(p, a, c)]; };
+    auto pv = [&](int a, int c) { return b.r_v[0][at(p, a, c)]; };
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double vu = pu(i, j), vv = pv(i, j);
+        double qu = dir_u(p, i) ? 0.0 : vu - alpha * visc_u(p, pu, pv, i, j, 0);
+        double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, pu, pv, i, j, 0);
+        b.p_u[0][k] = vu;
+        b.p_v[0][k] = vv;
+        b.q_u[0][k] = qu;
+        b.q_v[0][k] = qv;
+        acc[0] += vu * qu + vv * qv;
+        acc[1] += vu * qu + vv * qv;  // r = p on the first iteration
+        acc[2] += qu * qu + qv * qv;
+        acc[3] += vu * vu + vv * vv;
+    }
+    if (grid_sum<4>(acc, b.partials, nblocks, &st->counters[11]) && lane_id() == 0)
+        vel_finish_scalars(st->vel, acc[0], acc[1], acc[2], acc[3]);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_vel_fused(Params p, VelBufs b, double alpha, Status* st, int nblocks,
+                                                     int slot) {
+    if (st->vel.done) return;
+    const int it = st->vel.iter;
+    const double a = st->vel.alpha, beta = st->vel.beta;
+    const int o = it & 1, n = (it + 1) & 1;
+    const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
+    auto pnu = [&](int c0, int c1) {
+        size_t k = at(p, c0, c1);
+        return (ru[k] - a * qu0[k]) + beta * pu0[k];
+    };
+    auto pnv = [&](int c0, int c1) {
+        size_t k = at(p, c0, c1);
+        return (rv[k] - a * qv0[k]) + beta * pv0[k];
+    };
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double r1u = ru[k] - a * qu0[k], r1v = rv[k] - a * qv0[k];
+        double p1u = r1u + beta * pu0[k], p1v = r1v + beta * pv0[k];
+        double q1u = dir_u(p, i) ? 0.0 : p1u - alpha * visc_u(p, pnu, pnv, i, j, 0);
+        double q1v = dir_v(p, j) ? 0.0 : p1v - alpha * visc_v(p, pnu, pnv, i, j, 0);
+        b.xu[k] += a * pu0[k];
+        b.xv[k] += a * pv0[k];
+        b.r_u[n][k] = r1u;
+        b.r_v[n][k] = r1v;
+        b.p_u[n][k] = p1u;
+        b.p_v[n][k] = p1v;
+        b.q_u[n][k] = q1u;
+        b.q_v[n][k] = q1v;
+        acc[0] += p1u * q1u + p1v * q1v;
+        acc[1] += r1u * q1u + r1v * q1v;
+        acc[2] += q1u * q1u + q1v * q1v;
+        acc[3] += r1u * r1u + r1v * r1v;
+    }
+    if (grid_sum<4>(acc, b.partials, nblocks, &st->counters[12]) && lane_id() == 0) {
+        KrylovScalars& s = st->vel;
+        s.iter = it + 1;
+        st->vel_iters[slot] = s.iter;
+        if (acc[3] <= s.tol2) {
+            s.rr = acc[3];
+            s.done = 1;
+        } else if (s.iter >= s.maxiter) {
+            s.rr = acc[3];
+            s.done = 1;
+            if (s.rr > 1e-12 * s.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(s.rr / s.bnorm2), (double)s.iter);
+        } else {
+            vel_finish_scalars(s, acc[0], acc[1], acc[2], acc[3]);
         }
     }
 }
@@ -742,6 +834,14 @@ void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double a
         k_vel_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, slot);
     }
 }
+void launch_vel_first(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st) {
+    k_vel_first<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
+}
+void launch_vel_fused(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
+                      int slot) {
+    const int nb = grid_blocks(p);
+    for (int c = 0; c < count; ++c) k_vel_fused<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, slot);
+}
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
     k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
 }
@@ -752,6 +852,21 @@ void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b, Status* 
 }
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st) {
     k_pcg_init<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, grid_blocks(p));
+}
+
+static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done);
+
+// Fused V-cycle: one LDS-tiled kernel down and one up per level; the finest
+// up-kernel also forms r.z and beta (returns true: no separate dot needed).
+static void vcycle_fused(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, Status* st,
+                         double* partials) {
+    const double omega = 0.8;
+    const int* done = &st->pcg.done;
+    for (int l = 0; l < nlev - 1; ++l) launch_mg_down(s, geo[l], geo[l + 1], lv[l].b, lv[l].x, lv[l + 1].b, omega, done);
+    coarse_solve(s, lv, geo, nlev, done);
+    for (int l = nlev - 2; l >= 0; --l)
+        launch_mg_up(s, geo[l], geo[l + 1], lv[l].x, lv[l].b, lv[l + 1].t, lv[l].t, omega, done,
+                     l == 0 ? st : nullptr, partials);
 }
 
 static void vcycle(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done) {
@@ -765,6 +880,18 @@ static void vcycle(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev
         const MgGeom& gc = geo[l + 1];
         k_mg_restrict<<<grid_of(gc.nx, gc.ny), kBlock, 0, s>>>(g, gc, lv[l].r, lv[l + 1].b, done);
     }
+    coarse_solve(s, lv, geo, nlev, done);
+    for (int l = nlev - 2; l >= 0; --l) {
+        const MgGeom& g = geo[l];
+        dim3 gr = grid_of(g.nx, g.ny);
+        k_mg_prolong<<<gr, kBlock, 0, s>>>(g, geo[l + 1], lv[l + 1].t, lv[l].t, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+    }
+}
+
+static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done) {
+    const double omega = 0.8;
     {
         const int l = nlev - 1;
         const MgGeom& g = geo[l];
@@ -781,21 +908,18 @@ static void vcycle(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev
             k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
         }
     }
-    for (int l = nlev - 2; l >= 0; --l) {
-        const MgGeom& g = geo[l];
-        dim3 gr = grid_of(g.nx, g.ny);
-        k_mg_prolong<<<gr, kBlock, 0, s>>>(g, geo[l + 1], lv[l + 1].t, lv[l].t, done);
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
-    }
 }
 
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
-                      int nlev, Status* st, int count, int slot) {
+                      int nlev, Status* st, int count, int slot, int fused) {
     const int nb = grid_blocks(p);
     for (int c = 0; c < count; ++c) {
-        vcycle(s, lv, geo, nlev, &st->pcg.done);
-        k_pcg_rz<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb);
+        if (fused && nlev > 1) {
+            vcycle_fused(s, lv, geo, nlev, st, b.partials);
+        } else {
+            vcycle(s, lv, geo, nlev, &st->pcg.done);
+            k_pcg_rz<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb);
+        }
         k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], b, st, nb);
         k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, slot);
     }
@@ -824,6 +948,10 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
     if (n == "vel_dir") {
         k_vel_dir<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb);
         return 8 * cell;  // r, p_old (u, v) read; p_new, q (u, v) written
+    }
+    if (n == "vel_fused") {
+        k_vel_fused<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 0);
+        return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
     }
     if (n == "vel_upd") {
         k_vel_upd<<<grid2(p), kBlock, 0, s>>>(p, vb, st, nb, 0);
