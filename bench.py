@@ -31,11 +31,13 @@ METRIC = "IMEX steps/s at N=1024 (FP64 vs mixed @1e-8), HBM roofline %, adjoint 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 # Algorithmic HBM passes per step (fields of N^2 doubles): tendencies 3 x 20,
-# stage combinations 20 + 28 + 32, velocity CG 20 / iteration + 6 per solve,
-# projection PCG (V-cycle + vector kernels) per iteration + 18 per projection.
-def step_bytes(N, vel_iters, proj_iters, nlev_passes):
+# stage combinations 22 + 30 + 38, velocity CG 16 / iteration + 16 per solve
+# (init + first), projection PCG per iteration: V-cycle 5.5 at the finest
+# level x 4/3 for the coarser ones, + 10 for the Krylov vector kernels; + 16
+# per projection (rhs, init, mean, update, check).
+def step_bytes(N, vel_iters, proj_iters, pcg_passes=17.3):
     cell = 8.0 * N * N
-    passes = 60 + 80 + sum(6 + 20 * k for k in vel_iters) + sum(18 + nlev_passes * k for k in proj_iters)
+    passes = 60 + 90 + sum(16 + 16 * k for k in vel_iters) + sum(16 + pcg_passes * k for k in proj_iters)
     return passes * cell
 
 
@@ -102,24 +104,25 @@ def main(argv=None):
 
     # per-kernel measurement: the candidates for the dominant kernel
     kernels = {}
-    for name in ("vel_dir", "vel_upd", "pcg_dir", "pcg_upd", "mg_jacobi", "mg_resid", "momentum", "cells", "copy"):
+    for name in ("vel_iter", "pcg_dir", "pcg_upd", "mg_down0", "mg_up0", "momentum", "cells", "copy"):
         ms, nb = bench_kernel(name, args.kernel_reps)
         kernels[name] = {"avg_ms": ms, "bytes": nb, "GBps": nb / ms / 1e6}
         print(f"kernel {name}: {ms * 1e3:.1f} us, {nb / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
-    # launches per step of each kernel
+    # launches per step of each kernel (one velocity-CG kernel per iteration,
+    # one finest down/up pair and one dir/upd pair per PCG iteration)
     nv = float(np.sum(vel_mean))
     npj = float(np.sum(proj_mean))
-    launches = {"vel_dir": nv, "vel_upd": nv, "pcg_dir": npj, "pcg_upd": npj, "mg_jacobi": 4 * npj,
-                "mg_resid": npj, "momentum": 3, "cells": 3}
+    launches = {"vel_iter": nv + 3, "pcg_dir": npj, "pcg_upd": npj, "mg_down0": npj + 3, "mg_up0": npj,
+                "momentum": 3, "cells": 3}
     share = {k: launches[k] * kernels[k]["avg_ms"] for k in launches}
     dominant = max(share, key=share.get)
     dk = kernels[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": dk["GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": dk["GBps"] * 1e9 / HBM_PEAK, "traffic": None,
-                "bytes_per_launch": dk["bytes"], "avg_launch_us": dk["avg_ms"] * 1e3}
+                "bytes_per_launch": dk["bytes"], "avg_launch_us": dk["avg_ms"] * 1e3,
+                "share_of_step": share[dominant] / ms_per_step}
 
-    nlev_passes = 20.0
-    b_step = step_bytes(args.N, vel_mean, proj_mean, nlev_passes)
+    b_step = step_bytes(args.N, vel_mean, proj_mean)
     whole_step = {"algorithmic_GB": b_step / 1e9, "achieved_GBps": b_step / (ms_per_step * 1e-3) / 1e9,
                   "frac_of_peak": b_step / (ms_per_step * 1e-3) / HBM_PEAK}
 

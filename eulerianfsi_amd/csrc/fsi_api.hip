@@ -59,8 +59,6 @@ struct fsi_context {
     std::vector<MgGeom> geo;
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
-    int mg_fused = 1;   // fused LDS-tiled V-cycle level kernels (FSI_MG_SIMPLE=1: one kernel per operation)
-    int vel_fused = 1;  // single-reduction predictor CG (FSI_CG_SIMPLE=1: textbook two-kernel CG)
 };
 
 struct fsi_state {
@@ -251,12 +249,6 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     *out = c;
     if (!grid || !bcs || !models || !numerics) return fail(c, FSI_E_ARGUMENT, "null configuration pointer");
     c->grid = *grid;
-    {
-        const char* e = std::getenv("FSI_MG_SIMPLE");
-        c->mg_fused = (e && e[0] == '1') ? 0 : 1;
-        const char* f = std::getenv("FSI_CG_SIMPLE");
-        c->vel_fused = (f && f[0] == '1') ? 0 : 1;
-    }
     c->bcs = *bcs;
     c->models = *models;
     c->num = *numerics;
@@ -398,8 +390,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             c->geo.push_back(g);
         }
     }
-    const int nb = grid_blocks(p);
-    HIPCHECK(c, dalloc(c, &c->partials, (size_t)8 * (nb + 64)));
+    HIPCHECK(c, dalloc(c, &c->partials, partials_size(p)));
     HIPCHECK(c, hipMalloc((void**)&c->st, sizeof(Status)));
     HIPCHECK(c, hipMemsetAsync(c->st, 0, sizeof(Status), c->stream));
     HIPCHECK(c, hipHostMalloc((void**)&c->st_host, sizeof(Status), hipHostMallocDefault));
@@ -505,10 +496,13 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
         return e;
     };
     const size_t F = c->felems;
-    double** singles[] = {&w->Us,     &w->Vs,     &w->phis,   &w->sxxs,   &w->syys,    &w->sxys,   &w->ru,
-                          &w->rv,     &w->r_u[0], &w->r_u[1], &w->r_v[0], &w->r_v[1],  &w->p_u[0], &w->p_u[1],
-                          &w->p_v[0], &w->p_v[1], &w->q_u[0], &w->q_u[1], &w->q_v[0],  &w->q_v[1], &w->rhs,
-                          &w->corr[0], &w->corr[1], &w->r, &w->z, &w->pp[0], &w->pp[1], &w->w};
+    // the six stage fields trade places with the state's fields at the end of
+    // every step, so they are freed through their current owner (not `owned`)
+    double** stage[] = {&w->Us, &w->Vs, &w->phis, &w->sxxs, &w->syys, &w->sxys};
+    for (auto p : stage) HIPCHECK(c, dalloc(c, p, F));
+    double** singles[] = {&w->ru,     &w->rv,     &w->r_u[0], &w->r_u[1], &w->r_v[0], &w->r_v[1], &w->p_u[0],
+                          &w->p_u[1], &w->p_v[0], &w->p_v[1], &w->q_u[0], &w->q_u[1], &w->q_v[0], &w->q_v[1],
+                          &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -540,6 +534,9 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
 extern "C" void fsi_workspace_destroy(fsi_workspace* w) {
     if (!w) return;
     for (double* p : w->owned) (void)hipFree(p);
+    double* stage[] = {w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys};
+    for (double* p : stage)
+        if (p) (void)hipFree(p);
     delete w;
 }
 
@@ -617,14 +614,14 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         launch_vel_copy(c->stream, p, b);
         return FSI_OK;
     }
-    launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter);
-    if (c->vel_fused) launch_vel_first(c->stream, p, b, alpha, c->st);
-    int total = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
-    while (total < c->num.velocity_maxiter) {
-        int n = std::min(chunk, c->num.velocity_maxiter - total);
-        if (c->vel_fused) launch_vel_fused(c->stream, p, b, alpha, c->st, n, slot);
-        else launch_vel_iters(c->stream, p, b, alpha, c->st, n, slot);
-        total += n;
+    launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter, slot);
+    // kernel m tests the residual of iterate m and performs update m+1
+    const int last = c->num.velocity_maxiter;
+    int m = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
+    while (m <= last) {
+        int n = std::min(chunk, last + 1 - m);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot);
+        m += n;
         fsi_status rc = poll_done(c, &c->st->vel, &done, &iter);
         if (rc != FSI_OK) return rc;
         if (done) break;
@@ -650,15 +647,16 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     b.p[1] = w->pp[1];
     b.w = w->w;
     b.partials = c->partials;
-    launch_proj_rhs(c->stream, p, b, c->st, c->has_null, c->num.projection_rtol, c->num.projection_maxiter);
-    launch_pcg_init(c->stream, p, c->geo[0], b, c->st);
+    launch_proj_rhs(c->stream, p, b);
+    launch_pcg_init(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol,
+                    c->num.projection_maxiter);
     const int ps = slot < 0 ? 0 : slot;
-    int total = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
-    while (total < c->num.projection_maxiter) {
-        int n = std::min(chunk, c->num.projection_maxiter - total);
-        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->st, n, slot,
-                         c->mg_fused);
-        total += n;
+    const int last = c->num.projection_maxiter;
+    int m = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
+    while (m <= last) {
+        int n = std::min(chunk, last + 1 - m);
+        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->st, m, n, slot);
+        m += n;
         fsi_status rc = poll_done(c, &c->st->pcg, &done, &iter);
         if (rc != FSI_OK) return rc;
         if (done) break;
@@ -895,13 +893,14 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
     pb.p[1] = w->pp[1];
     pb.w = w->w;
     pb.partials = c->partials;
-    // keep every kernel running (done = 0) on an iteration > 0 state
+    // keep every kernel running: done = 0, no tolerance reached, no maxiter
     status_reset(c);
-    c->st_host->vel.iter = 1;
-    c->st_host->pcg.iter = 1;
-    c->st_host->vel.rr = c->st_host->pcg.rz = 1.0;
+    c->st_host->vel.tol2 = c->st_host->pcg.tol2 = -1.0;
+    c->st_host->vel.maxiter = c->st_host->pcg.maxiter = 1 << 30;
+    c->st_host->pcg.rzs[0] = c->st_host->pcg.rzs[1] = 1.0;
     fsi_status rc = status_push(c);
     if (rc != FSI_OK) return rc;
+    pb.x = w->lv[0].r;  // scratch: the correction warm start is left untouched
     const std::string n(name);
     auto launch = [&]() -> double {
         if (n == "momentum") {

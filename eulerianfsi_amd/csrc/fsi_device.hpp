@@ -76,9 +76,9 @@ enum DevError : int {
 
 // Scalars of one Krylov solve, resident in device memory.
 struct KrylovScalars {
-    double rr, rz, rz_old, pq, alpha, beta, bnorm2, tol2;
+    double rr, bnorm2, tol2;
+    double rzs[2];  // r.z of the last two PCG iterations (by parity)
     int done, iter, maxiter, pad;
-    double rq, qq;  // r.q and q.q of the single-reduction CG recurrence
 };
 
 // Projection scalars (src/Projection.jl:170-307).
@@ -105,7 +105,6 @@ struct Status {
     KrylovScalars vel;
     KrylovScalars pcg;
     ProjScalars proj;
-    unsigned int counters[16];  // last-block arrival counters
 };
 
 __device__ __forceinline__ size_t at(const Params& p, int i, int j) { return (size_t)j * p.P + i; }
@@ -151,25 +150,41 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
-// Publish a block partial and elect the last-arriving block.  Returns true in
-// every thread of the last block, after which all partials are visible.
-__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int nblocks, int* lds_flag) {
+// Cross-block reductions never synchronise blocks inside a kernel: a kernel
+// writes one partial per block (put_partials), and every block of the next
+// kernel sums all partials in the same fixed order (get_totals), so every
+// block derives bitwise-identical totals and the result is deterministic.
+// The kernel boundary orders the writes before the reads.
+template <int NV>
+__device__ __forceinline__ void put_partials(double (&v)[NV], double* dst, int nb) {
+    __shared__ double lds[NV * 16];
+    block_sum<NV>(v, lds);
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    __threadfence();
-    __syncthreads();
-    if (lane == 0) {
-        unsigned int prev = atomicAdd(counter, 1u);
-        *lds_flag = (prev == nblocks - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    bool last = *lds_flag != 0;
-    if (last) __threadfence();
-    return last;
+    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) dst[(size_t)k * nb + bid] = v[k];
 }
 
-// Loads that must observe other blocks' writes from the same kernel.
-__device__ __forceinline__ double load_coherent(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+template <int NV>
+__device__ __forceinline__ void get_totals(const double* src, int nb, double (&out)[NV]) {
+    __shared__ double lds[NV * 16];
+    __shared__ double res[NV];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    const int nthreads = blockDim.x * blockDim.y;
+    double acc[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+    for (int b = lane; b < nb; b += nthreads)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) acc[k] += src[(size_t)k * nb + b];
+    block_sum<NV>(acc, lds);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k) res[k] = acc[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; ++k) out[k] = res[k];
 }
 
 }  // namespace fsi

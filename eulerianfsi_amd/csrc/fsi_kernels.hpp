@@ -27,6 +27,20 @@ __device__ __forceinline__ void atomic_max_double(double* addr, double v) {
 
 __global__ void k_zero_scalar(double* x);
 
+// block maximum of a non-negative value, merged into *dst with an atomic
+__device__ __forceinline__ void block_max_to(double m, double* dst) {
+    __shared__ double lds[16];
+    m = wave_max(m);
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) lds[lane >> 6] = m;
+    __syncthreads();
+    if (lane == 0) {
+        double mm = lds[0];
+        for (int w = 1; w < (int)(blockDim.x * blockDim.y + 63) / 64; ++w) mm = fmax(mm, lds[w]);
+        atomic_max_double(dst, mm);
+    }
+}
+
 // One level of the projection multigrid hierarchy (cell-centred, uniform).
 struct MgLevel {
     int nx, ny, P;
@@ -80,11 +94,8 @@ struct VelBufs {
     double* partials;
 };
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
-                     int maxiter);
-void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
-                      int slot);
-void launch_vel_first(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st);
-void launch_vel_fused(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
+                     int maxiter, int slot);
+void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot);
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
 
@@ -103,22 +114,27 @@ struct ProjBufs {
     double* w;
     double* partials;
 };
-void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int has_null, double rtol,
-                     int maxiter);
-void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st);
+void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
+void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
+                     double rtol, int maxiter);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
-                      int nlev, Status* st, int count, int slot, int fused);
+                      int nlev, Status* st, int m0, int count, int slot);
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly);
+void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot);
+void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, int nb, const int* done);
 
 int grid_blocks(const Params& p);
 
 // ---- fused multigrid levels (fsi_mg.hip) ----
+// The finest down-kernel (st != nullptr) first sums the r.r partials of the
+// previous PCG kernel and ends the solve when converged; the finest up-kernel
+// (rz != nullptr) writes the r.z block partials.
 void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
-                    double omega, const int* done);
+                    double omega, const int* done, Status* st, const double* rr, int nb, int m, int slot);
 void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
-                  const double* ec, double* xout, double omega, const int* done, Status* st, double* partials);
-int mg_up_blocks(const MgGeom& gf);
+                  const double* ec, double* xout, double omega, const int* done, double* rz, int nb);
+size_t partials_size(const Params& p);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
                     const MgLevel* lv, const MgGeom* geo, Status* st);
 

@@ -90,8 +90,22 @@ constexpr int DHh = FT_Y + 2 * DH;    // 22
 
 __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, const double* __restrict__ b,
                                                         double* __restrict__ x2out, double* __restrict__ bc,
-                                                        double omega, const int* done) {
+                                                        double omega, const int* done, Status* st,
+                                                        const double* __restrict__ rr, int nb, int m, int slot) {
     if (*done) return;
+    if (st) {  // finest level: convergence test on |r|^2 of the previous PCG kernel
+        double tot[1];
+        get_totals<1>(rr, nb, tot);
+        if (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter) {
+            if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
+                st->pcg.done = 1;
+                st->pcg.iter = m;
+                st->pcg.rr = tot[0];
+                if (slot >= 0) st->proj_iters[slot] = m;
+            }
+            return;
+        }
+    }
     __shared__ double sb[DHh * DW];                       // b on halo 3
     __shared__ double sx2[(DHh - 2) * (DW - 2)];          // x2 on halo 2 (68 x 20)
     __shared__ double sr[(DHh - 4) * (DW - 4)];           // r on halo 1 (66 x 18)
@@ -195,14 +209,12 @@ constexpr int CH = FT_Y / 2 + 4;    // 12
 __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const double* __restrict__ x2,
                                                       const double* __restrict__ b, const double* __restrict__ ec,
                                                       double* __restrict__ xout, double omega, const int* done,
-                                                      Status* st, double* partials, int nblocks) {
+                                                      double* rz, int nb) {
     if (*done) return;
     __shared__ double sx3[UHh * UW];              // x3 on halo 2
     __shared__ double sx4[(UHh - 2) * (UW - 2)];  // x4 on halo 1
     __shared__ double sbb[(UHh - 2) * (UW - 2)];  // b on halo 1
     __shared__ double sec[CH * CW];               // e_c around the tile
-    __shared__ double lds_red[16];
-    __shared__ int flag;
     const int tid = threadIdx.x;
     const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
@@ -274,38 +286,21 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
         xout[(size_t)gj * gf.P + gi] = v;
         acc += bv * v;
     }
-    if (!st) return;
-    // deterministic grid reduction of r.z (same scheme as the Krylov kernels)
+    if (!rz) return;
     double v[1] = {acc};
-    block_sum<1>(v, lds_red);
-    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
-    if (tid == 0) partials[bid] = v[0];
-    if (!arrive_last(&st->counters[10], (unsigned)nblocks, &flag)) return;
-    double a = 0.0;
-    for (int q = tid; q < nblocks; q += MG_THREADS) a += load_coherent(&partials[q]);
-    double w[1] = {a};
-    block_sum<1>(w, lds_red);
-    if (tid == 0) {
-        KrylovScalars& s = st->pcg;
-        s.rz_old = s.rz;
-        s.rz = w[0];
-        s.beta = s.iter == 0 ? 0.0 : w[0] / s.rz_old;
-        st->counters[10] = 0u;
-    }
+    put_partials<1>(v, rz, nb);
 }
 
 void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
-                    double omega, const int* done) {
+                    double omega, const int* done, Status* st, const double* rr, int nb, int m, int slot) {
     dim3 g((gf.nx + FT_X - 1) / FT_X, (gf.ny + FT_Y - 1) / FT_Y);
-    k_mg_down<<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, omega, done);
+    k_mg_down<<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, omega, done, st, rr, nb, m, slot);
 }
 
-int mg_up_blocks(const MgGeom& gf) { return ((gf.nx + FT_X - 1) / FT_X) * ((gf.ny + FT_Y - 1) / FT_Y); }
-
 void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
-                  const double* ec, double* xout, double omega, const int* done, Status* st, double* partials) {
+                  const double* ec, double* xout, double omega, const int* done, double* rz, int nb) {
     dim3 g((gf.nx + FT_X - 1) / FT_X, (gf.ny + FT_Y - 1) / FT_Y);
-    k_mg_up<<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, omega, done, st, partials, mg_up_blocks(gf));
+    k_mg_up<<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, omega, done, rz, nb);
 }
 
 }  // namespace fsi

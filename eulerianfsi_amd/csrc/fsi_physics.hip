@@ -360,148 +360,10 @@ __device__ __forceinline__ double phase_bvalue(const BcSide& b, double interior,
     return interior;
 }
 
-// ---- CDI (InterfaceRegularization.jl:187-228 + InterfaceTransport.jl:260-276)
-__device__ double phase_cdi(const Params& p, const Stage& s, int i, int j, Status* st, int with_flux) {
-    const double* phi = s.phi;
-    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
-    double adv = upwind_rhs(p, s.U, s.V, p.bPhi, f, i, j, st);
-    if (!with_flux) return adv - 0.0;
-    auto sharp = [&](int a, int b, int comp) -> double {
-        double gx, gy;
-        grad_center(p, p.bPhi, f, a, b, gx, gy);
-        double mag = sqrt(gx * gx + gy * gy + p.cdi_d2);
-        double ph = f(a, b);
-        double amp = ph * (1.0 - ph);
-        return comp == 0 ? amp * gx / mag : amp * gy / mag;
-    };
-    const double ell = 0.5 * p.cdi_eps;
-    auto jx = [&](int k) -> double {
-        if (!p.px && (k == 0 || k == p.nx)) return 0.0;
-        auto shx = [&](int a, int b) { return sharp(a, b, 0); };
-        return p.cdi_G * (c2fx(p, shx, k, j) - ell * grad_xface(p, p.bPhi, f, k, j));
-    };
-    auto jy = [&](int k) -> double {
-        if (!p.py && (k == 0 || k == p.ny)) return 0.0;
-        auto shy = [&](int a, int b) { return sharp(a, b, 1); };
-        return p.cdi_G * (c2fy(p, shy, i, k) - ell * grad_yface(p, p.bPhi, f, i, k));
-    };
-    double divj = (jx(i + 1) - jx(i)) / p.hx + (jy(j + 1) - jy(j)) / p.hy;
-    return adv - divj;
-}
-
-// ---- ACDI (PhaseFieldMethods.jl:479-671)
+// ---- ACDI psi (PhaseFieldMethods.jl:479-483)
 __device__ __forceinline__ double psi_value(const Params& p, double ph) {
     double q = clamp01(ph), e = p.acdi_floor;
     return p.acdi_eps * log((q + e) / (1.0 - q + e));
-}
-
-__device__ double phase_acdi(const Params& p, const Stage& s, const BcSet& bpsi, int i, int j,
-                             double gamma, Status* st) {
-    const double* phi = s.phi;
-    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
-    auto psi = [&](int a, int b) { return psi_value(p, phi[at(p, a, b)]); };
-    // convective flux U * phi_face (:531-574)
-    auto fx = [&](int k) -> double {
-        if (p.px) {
-            int kk = (k == p.nx) ? 0 : k;
-            double pf = kk == 0 ? 0.5 * f(p.nx - 1, j) + 0.5 * f(0, j) : 0.5 * f(kk - 1, j) + 0.5 * f(kk, j);
-            return s.U[at(p, kk, j)] * pf;
-        }
-        if (k == 0) return s.U[at(p, 0, j)] * phase_bvalue(p.bPhi.l, f(0, j), st);
-        if (k == p.nx) return s.U[at(p, p.nx, j)] * phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
-        return s.U[at(p, k, j)] * (0.5 * f(k - 1, j) + 0.5 * f(k, j));
-    };
-    auto fy = [&](int k) -> double {
-        if (p.py) {
-            int kk = (k == p.ny) ? 0 : k;
-            return s.V[at(p, i, kk)] * (kk == 0 ? 0.5 * f(i, p.ny - 1) + 0.5 * f(i, 0)
-                                                : 0.5 * f(i, kk - 1) + 0.5 * f(i, kk));
-        }
-        if (k == 0) return s.V[at(p, i, 0)] * phase_bvalue(p.bPhi.b, f(i, 0), st);
-        if (k == p.ny) return s.V[at(p, i, p.ny)] * phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
-        return s.V[at(p, i, k)] * (0.5 * f(i, k - 1) + 0.5 * f(i, k));
-    };
-    // centre normals of psi (:233-244)
-    auto normal = [&](int a, int b, int comp) -> double {
-        double gx, gy;
-        grad_center(p, bpsi, psi, a, b, gx, gy);
-        double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
-        return comp == 0 ? gx / mag : gy / mag;
-    };
-    const double eps = p.acdi_eps;
-    auto Ax = [&](int k) -> double {
-        if (!p.px && (k == 0 || k == p.nx)) return 0.0;
-        auto nxc = [&](int a, int b) { return normal(a, b, 0); };
-        double t = tanh(c2fx(p, psi, k, j) / (2.0 * eps));
-        double amp = 0.25 * (1.0 - t * t);
-        return gamma * (eps * grad_xface(p, p.bPhi, f, k, j) - amp * c2fx(p, nxc, k, j));
-    };
-    auto Ay = [&](int k) -> double {
-        if (!p.py && (k == 0 || k == p.ny)) return 0.0;
-        auto nyc = [&](int a, int b) { return normal(a, b, 1); };
-        double t = tanh(c2fy(p, psi, i, k) / (2.0 * eps));
-        double amp = 0.25 * (1.0 - t * t);
-        return gamma * (eps * grad_yface(p, p.bPhi, f, i, k) - amp * c2fy(p, nyc, i, k));
-    };
-    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
-    double dA = (Ax(i + 1) - Ax(i)) / p.hx + (Ay(j + 1) - Ay(j)) / p.hy;
-    return -dF + dA;
-}
-
-// ---- UWCDI (PhaseFieldMethods.jl:675-867, :standard normals :246-278)
-__device__ double phase_uwcdi(const Params& p, const Stage& s, int i, int j, Status* st) {
-    const double* phi = s.phi;
-    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
-    auto ncomp = [&](int a, int b, int comp) -> double {
-        double gx, gy;
-        grad_center(p, p.bPhi, f, a, b, gx, gy);
-        double mag = sqrt(gx * gx + gy * gy + p.uw_d2);
-        return comp == 0 ? gx / mag : gy / mag;
-    };
-    auto nxc = [&](int a, int b) { return ncomp(a, b, 0); };
-    auto nyc = [&](int a, int b) { return ncomp(a, b, 1); };
-    auto fx = [&](int k) -> double {
-        int kk = (p.px && k == p.nx) ? 0 : k;  // periodic duplicate face
-        double u = s.U[at(p, kk, j)], pu;
-        if (!p.px && kk == 0) {
-            double bnd = phase_bvalue(p.bPhi.l, f(0, j), st);
-            pu = u >= 0.0 ? bnd : f(0, j);
-        } else if (!p.px && kk == p.nx) {
-            double bnd = phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
-            pu = u >= 0.0 ? f(p.nx - 1, j) : bnd;
-        } else {
-            int iL = kk == 0 ? p.nx - 1 : kk - 1, iR = kk;
-            pu = u >= 0.0 ? f(iL, j) : f(iR, j);
-        }
-        double nf = c2fx(p, nxc, kk, j);
-        double ny_on = c2fx(p, nyc, kk, j);
-        nf /= sqrt(nf * nf + ny_on * ny_on + p.uw_d2);
-        double ax = p.uw_alpha * fmax(1.0, p.hy / p.hx);
-        double beta = fabs(u) / ax;
-        return pu * (u + beta * nf * (1.0 - pu));
-    };
-    auto fy = [&](int k) -> double {
-        int kk = (p.py && k == p.ny) ? 0 : k;
-        double v = s.V[at(p, i, kk)], pu;
-        if (!p.py && kk == 0) {
-            double bnd = phase_bvalue(p.bPhi.b, f(i, 0), st);
-            pu = v >= 0.0 ? bnd : f(i, 0);
-        } else if (!p.py && kk == p.ny) {
-            double bnd = phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
-            pu = v >= 0.0 ? f(i, p.ny - 1) : bnd;
-        } else {
-            int jB = kk == 0 ? p.ny - 1 : kk - 1, jT = kk;
-            pu = v >= 0.0 ? f(i, jB) : f(i, jT);
-        }
-        double nf = c2fy(p, nyc, i, kk);
-        double nx_on = c2fy(p, nxc, i, kk);
-        nf /= sqrt(nx_on * nx_on + nf * nf + p.uw_d2);
-        double ay = p.uw_alpha * fmax(1.0, p.hx / p.hy);
-        double beta = fabs(v) / ay;
-        return pu * (v + beta * nf * (1.0 - pu));
-    };
-    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
-    return -dF;
 }
 
 // ---- structure tensor (StructureTensor.jl:377-551, Operators.jl:431-586)
@@ -594,6 +456,115 @@ __device__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status*
     oxy = (axy + sb * dv) + dxy + rxy + exy;
 }
 
+// ---- phase right-hand sides on tile accessors ------------------------------
+// f: phi, fpsi: psi, N0/N1: cell normals (CDI: sharpening vector); all take
+// global (wrapped) indices.
+template <class F, class NX, class NY>
+__device__ double phase_cdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i, int j,
+                              Status* st, int with_flux) {
+    double adv = upwind_rhs(p, s.U, s.V, p.bPhi, f, i, j, st);
+    if (!with_flux) return adv - 0.0;
+    const double ell = 0.5 * p.cdi_eps;
+    auto jx = [&](int k) -> double {
+        if (!p.px && (k == 0 || k == p.nx)) return 0.0;
+        return p.cdi_G * (c2fx(p, N0, k, j) - ell * grad_xface(p, p.bPhi, f, k, j));
+    };
+    auto jy = [&](int k) -> double {
+        if (!p.py && (k == 0 || k == p.ny)) return 0.0;
+        return p.cdi_G * (c2fy(p, N1, i, k) - ell * grad_yface(p, p.bPhi, f, i, k));
+    };
+    double divj = (jx(i + 1) - jx(i)) / p.hx + (jy(j + 1) - jy(j)) / p.hy;
+    return adv - divj;
+}
+
+template <class F, class FP, class NX, class NY>
+__device__ double phase_acdi_t(const Params& p, const Stage& s, const F& f, const FP& psi, const NX& N0,
+                               const NY& N1, int i, int j, double gamma, Status* st) {
+    auto fx = [&](int k) -> double {
+        if (p.px) {
+            int kk = (k == p.nx) ? 0 : k;
+            double pf = kk == 0 ? 0.5 * f(p.nx - 1, j) + 0.5 * f(0, j) : 0.5 * f(kk - 1, j) + 0.5 * f(kk, j);
+            return s.U[at(p, kk, j)] * pf;
+        }
+        if (k == 0) return s.U[at(p, 0, j)] * phase_bvalue(p.bPhi.l, f(0, j), st);
+        if (k == p.nx) return s.U[at(p, p.nx, j)] * phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
+        return s.U[at(p, k, j)] * (0.5 * f(k - 1, j) + 0.5 * f(k, j));
+    };
+    auto fy = [&](int k) -> double {
+        if (p.py) {
+            int kk = (k == p.ny) ? 0 : k;
+            double pf = kk == 0 ? 0.5 * f(i, p.ny - 1) + 0.5 * f(i, 0) : 0.5 * f(i, kk - 1) + 0.5 * f(i, kk);
+            return s.V[at(p, i, kk)] * pf;
+        }
+        if (k == 0) return s.V[at(p, i, 0)] * phase_bvalue(p.bPhi.b, f(i, 0), st);
+        if (k == p.ny) return s.V[at(p, i, p.ny)] * phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
+        return s.V[at(p, i, k)] * (0.5 * f(i, k - 1) + 0.5 * f(i, k));
+    };
+    const double eps = p.acdi_eps;
+    auto Ax = [&](int k) -> double {
+        if (!p.px && (k == 0 || k == p.nx)) return 0.0;
+        double t = tanh(c2fx(p, psi, k, j) / (2.0 * eps));
+        double amp = 0.25 * (1.0 - t * t);
+        return gamma * (eps * grad_xface(p, p.bPhi, f, k, j) - amp * c2fx(p, N0, k, j));
+    };
+    auto Ay = [&](int k) -> double {
+        if (!p.py && (k == 0 || k == p.ny)) return 0.0;
+        double t = tanh(c2fy(p, psi, i, k) / (2.0 * eps));
+        double amp = 0.25 * (1.0 - t * t);
+        return gamma * (eps * grad_yface(p, p.bPhi, f, i, k) - amp * c2fy(p, N1, i, k));
+    };
+    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
+    double dA = (Ax(i + 1) - Ax(i)) / p.hx + (Ay(j + 1) - Ay(j)) / p.hy;
+    return -dF + dA;
+}
+
+template <class F, class NX, class NY>
+__device__ double phase_uwcdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i,
+                                int j, Status* st) {
+    auto fx = [&](int k) -> double {
+        int kk = (p.px && k == p.nx) ? 0 : k;
+        double u = s.U[at(p, kk, j)], pu;
+        if (!p.px && kk == 0) {
+            double bnd = phase_bvalue(p.bPhi.l, f(0, j), st);
+            pu = u >= 0.0 ? bnd : f(0, j);
+        } else if (!p.px && kk == p.nx) {
+            double bnd = phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
+            pu = u >= 0.0 ? f(p.nx - 1, j) : bnd;
+        } else {
+            int iL = kk == 0 ? p.nx - 1 : kk - 1, iR = kk;
+            pu = u >= 0.0 ? f(iL, j) : f(iR, j);
+        }
+        double nf = c2fx(p, N0, kk, j);
+        double ny_on = c2fx(p, N1, kk, j);
+        nf /= sqrt(nf * nf + ny_on * ny_on + p.uw_d2);
+        double ax = p.uw_alpha * fmax(1.0, p.hy / p.hx);
+        double beta = fabs(u) / ax;
+        return pu * (u + beta * nf * (1.0 - pu));
+    };
+    auto fy = [&](int k) -> double {
+        int kk = (p.py && k == p.ny) ? 0 : k;
+        double v = s.V[at(p, i, kk)], pu;
+        if (!p.py && kk == 0) {
+            double bnd = phase_bvalue(p.bPhi.b, f(i, 0), st);
+            pu = v >= 0.0 ? bnd : f(i, 0);
+        } else if (!p.py && kk == p.ny) {
+            double bnd = phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
+            pu = v >= 0.0 ? f(i, p.ny - 1) : bnd;
+        } else {
+            int jB = kk == 0 ? p.ny - 1 : kk - 1, jT = kk;
+            pu = v >= 0.0 ? f(i, jB) : f(i, jT);
+        }
+        double nf = c2fy(p, N1, i, kk);
+        double nx_on = c2fy(p, N0, i, kk);
+        nf /= sqrt(nx_on * nx_on + nf * nf + p.uw_d2);
+        double ay = p.uw_alpha * fmax(1.0, p.hx / p.hy);
+        double beta = fabs(v) / ay;
+        return pu * (v + beta * nf * (1.0 - pu));
+    };
+    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
+    return -dF;
+}
+
 // Max |u_c| for dynamic ACDI gamma (PhaseFieldMethods.jl:576-596).
 __global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, const double* V, Status* st) {
     __shared__ double lds[16];
@@ -618,31 +589,124 @@ __global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, 
     }
 }
 
+// ---------------------------------------------------------------------------
 // k_cells: E_phi (phase_field_rhs) and E_sigma (structure_tensor_rhs) of a
 // stage; on stage 1 also the finiteness validation of the state
 // (TimeIntegrationIMEX.jl:495-523) and the stage-1 divergence L-infinity
 // (:1111-1114).
+//
+// LDS tile of the 64 x 16 output cells: phi (and, for ACDI, psi) on a halo of
+// 2, then the cell-centred normals (CDI: the sharpening vector) on a halo of
+// 1, each computed once per cell; the face fluxes are formed from the tile.
+// Across a periodic side the halo holds the wrapped values; outside a
+// non-periodic wall the halo is never read (walls enter through the BC
+// branches of the stencils, exactly as in the reference).
+// ---------------------------------------------------------------------------
+constexpr int CH2 = 2, CW2 = TILE_X + 2 * CH2, CR2 = TILE_ROWS + 2 * CH2;  // 68 x 20
+constexpr int CW1 = TILE_X + 2, CR1 = TILE_ROWS + 2;                      // 66 x 18
+
+struct TileAcc {  // accessor of an LDS tile by global (wrapped) index
+    const double* a;
+    int i0, j0, H, W, R, nx, ny;
+    __device__ __forceinline__ double operator()(int gi, int gj) const {
+        int li = gi - i0 + H, lj = gj - j0 + H;
+        if (li < 0) li += nx;
+        else if (li >= W) li -= nx;
+        if (lj < 0) lj += ny;
+        else if (lj >= R) lj -= ny;
+        return a[lj * W + li];
+    }
+};
+
 __global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
                                                  double* __restrict__ Exx, double* __restrict__ Eyy,
                                                  double* __restrict__ Exy, Status* st, int stage1) {
+    __shared__ double s_phi[CR2 * CW2];
+    __shared__ double s_psi[CR2 * CW2];
+    __shared__ double s_n0[CR1 * CW1];
+    __shared__ double s_n1[CR1 * CW1];
     __shared__ double lds[16];
     __shared__ long long lds_n[16];
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
+    const int phase = p.phase;
+    // 1. phi (and psi) on the halo-2 tile
+    for (int k = tid; k < CR2 * CW2; k += BLOCK) {
+        int li = k % CW2, lj = k / CW2;
+        int gi = i0 + li - CH2, gj = j0 + lj - CH2;
+        bool in = true;
+        if (gi < 0 || gi >= p.nx) {
+            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+            else in = false;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+            else in = false;
+        }
+        double v = in ? s.phi[at(p, gi, gj)] : 0.0;
+        s_phi[k] = v;
+        if (phase == FSI_PHASE_ACDI) s_psi[k] = in ? psi_value(p, v) : 0.0;
+    }
+    __syncthreads();
+    const TileAcc f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TileAcc fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    // 2. cell-centred normals / sharpening vector on the halo-1 tile
+    if (phase != FSI_PHASE_UPWIND) {
+        for (int k = tid; k < CR1 * CW1; k += BLOCK) {
+            int li = k % CW1, lj = k / CW1;
+            int gi = i0 + li - 1, gj = j0 + lj - 1;
+            bool in = true;
+            if (gi < 0 || gi >= p.nx) {
+                if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+                else in = false;
+            }
+            if (gj < 0 || gj >= p.ny) {
+                if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+                else in = false;
+            }
+            double n0 = 0.0, n1 = 0.0;
+            if (in) {
+                double gx, gy;
+                if (phase == FSI_PHASE_ACDI) {  // _center_normals of psi (:233-244)
+                    grad_center(p, bpsi, fpsi, gi, gj, gx, gy);
+                    double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
+                    n0 = gx / mag;
+                    n1 = gy / mag;
+                } else if (phase == FSI_PHASE_UWCDI) {  // _center_normals of phi
+                    grad_center(p, p.bPhi, f, gi, gj, gx, gy);
+                    double mag = sqrt(gx * gx + gy * gy + p.uw_d2);
+                    n0 = gx / mag;
+                    n1 = gy / mag;
+                } else {  // CDI _sharpening_vector (InterfaceRegularization.jl:123-145)
+                    grad_center(p, p.bPhi, f, gi, gj, gx, gy);
+                    double mag = sqrt(gx * gx + gy * gy + p.cdi_d2);
+                    double ph = f(gi, gj);
+                    double amp = ph * (1.0 - ph);
+                    n0 = amp * gx / mag;
+                    n1 = amp * gy / mag;
+                }
+            }
+            s_n0[k] = n0;
+            s_n1[k] = n1;
+        }
+        __syncthreads();
+    }
+    const TileAcc N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    const TileAcc N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+
     double dmax = 0.0;
     long long nonfinite = 0;
     double gamma = p.acdi_gamma;
-    if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn) gamma = p.acdi_gamma * st->gamma_umax;
+    if (phase == FSI_PHASE_ACDI && p.acdi_dyn) gamma = p.acdi_gamma * st->gamma_umax;
+    const int i = i0 + threadIdx.x;
     if (i < p.nx)
         for (int r = 0; r < ROWS; ++r) {
-            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            const int j = j0 + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
             double e;
-            switch (p.phase) {
-            case FSI_PHASE_UPWIND: e = phase_cdi(p, s, i, j, st, 0); break;
-            case FSI_PHASE_CDI: e = phase_cdi(p, s, i, j, st, 1); break;
-            case FSI_PHASE_ACDI: e = phase_acdi(p, s, bpsi, i, j, gamma, st); break;
-            default: e = phase_uwcdi(p, s, i, j, st); break;
-            }
+            if (phase == FSI_PHASE_ACDI) e = phase_acdi_t(p, s, f, fpsi, N0, N1, i, j, gamma, st);
+            else if (phase == FSI_PHASE_UWCDI) e = phase_uwcdi_t(p, s, f, N0, N1, i, j, st);
+            else e = phase_cdi_t(p, s, f, N0, N1, i, j, st, phase == FSI_PHASE_CDI);
             double oxx, oyy, oxy;
             sigma_rhs(p, s, i, j, st, oxx, oyy, oxy);
             const size_t k = at(p, i, j);
@@ -665,13 +729,12 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, 
     if (stage1) {
         dmax = wave_max(dmax);
         for (int off = 32; off > 0; off >>= 1) nonfinite += __shfl_down(nonfinite, off, 64);
-        const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-        if ((lane & 63) == 0) {
-            lds[lane >> 6] = dmax;
-            lds_n[lane >> 6] = nonfinite;
+        if ((tid & 63) == 0) {
+            lds[tid >> 6] = dmax;
+            lds_n[tid >> 6] = nonfinite;
         }
         __syncthreads();
-        if (lane == 0) {
+        if (tid == 0) {
             double mm = lds[0];
             long long nn = lds_n[0];
             for (int w = 1; w < BLOCK / 64; ++w) {

@@ -27,45 +27,9 @@ namespace fsi {
 
 __global__ void k_zero_scalar(double* x) { *x = 0.0; }
 
+
 static __device__ __forceinline__ int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
 static __device__ __forceinline__ int lane_id() { return threadIdx.x + threadIdx.y * blockDim.x; }
-
-// Grid-wide deterministic sum of NV values; returns true in the last block,
-// whose lane 0 then holds the totals in v.
-template <int NV>
-__device__ bool grid_sum(double (&v)[NV], double* partials, int nblocks, unsigned int* counter) {
-    __shared__ double lds[NV * 16];
-    __shared__ int flag;
-    block_sum<NV>(v, lds);
-    const int lane = lane_id(), bid = block_id();
-    if (lane == 0)
-        for (int k = 0; k < NV; ++k) partials[(size_t)k * nblocks + bid] = v[k];
-    if (!arrive_last(counter, (unsigned)nblocks, &flag)) return false;
-    double acc[NV];
-    for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    const int nthreads = blockDim.x * blockDim.y;
-    for (int b = lane; b < nblocks; b += nthreads)
-        for (int k = 0; k < NV; ++k) acc[k] += load_coherent(&partials[(size_t)k * nblocks + b]);
-    block_sum<NV>(acc, lds);
-    if (lane == 0) {
-        for (int k = 0; k < NV; ++k) v[k] = acc[k];
-        *counter = 0u;
-    }
-    return true;
-}
-
-__device__ __forceinline__ void block_max_to(double m, double* dst) {
-    __shared__ double lds[16];
-    m = wave_max(m);
-    const int lane = lane_id();
-    if ((lane & 63) == 0) lds[lane >> 6] = m;
-    __syncthreads();
-    if (lane == 0) {
-        double mm = lds[0];
-        for (int w = 1; w < (int)(blockDim.x * blockDim.y + 63) / 64; ++w) mm = fmax(mm, lds[w]);
-        atomic_max_double(dst, mm);
-    }
-}
 
 #define FOR_TILE(p)                                                              \
     const int i = blockIdx.x * TILE_X + threadIdx.x;                             \
@@ -73,16 +37,25 @@ __device__ __forceinline__ void block_max_to(double m, double* dst) {
         if (const int j = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;  \
             i < (p).nx && j < (p).ny)
 
+// Partial-sum sets (see put_partials / get_totals in fsi_device.hpp): each
+// set holds up to 4 quantities x nb blocks.
+enum PartialSet : int { PS_VEL0 = 0, PS_VEL1 = 1, PS_VELI = 2, PS_RHS = 3, PS_RR = 4, PS_RZ = 5, PS_PW = 6,
+                        PS_MEAN = 7, PS_CHECK = 8, PS_COUNT = 9 };
+__device__ __forceinline__ double* pset(double* base, int nb, int set) { return base + (size_t)set * 4 * nb; }
+
 // ===========================================================================
-// Velocity predictor CG
+// Velocity predictor: single-reduction CG, one kernel per iteration
 // ===========================================================================
-struct VelScal {
-    double alpha;  // dt * a_ss
-};
+// Kernel m forms r_{m+1} = r_m - a_m q_m and p_{m+1} = r_{m+1} + b_{m+1} p_m on
+// the fly (own point and stencil neighbours), q_{m+1} = (I - alpha M) p_{m+1},
+// x_{m+1} = x_m + a_m p_m, and the block partials of p.q, r.q, q.q, r.r.  a_m
+// and b_{m+1} come from the partials of kernel m-1 (every block sums them in
+// the same order): a_m = rr_m / pq_m, b_{m+1} = rr_{m+1} / rr_m with
+// rr_{m+1} = rr_m - 2 a_m r.q + a_m^2 q.q.  Convergence is tested on the true
+// rr_m.  r, p, q are ping-ponged by m; x is updated in place.
 
 // x = warm start uv_n with Dirichlet values, r = rhs - A x (masked), rr, |b_F|^2
-__global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double alpha, Status* st, int nblocks,
-                                                    double rtol, int maxiter) {
+__global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double alpha, int nb) {
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
     auto xu = [&](int a, int c) { return dir_u(p, a) ? gul : b.Un[at(p, a, c)]; };
     auto xv = [&](int a, int c) { return dir_v(p, c) ? gvb : b.Vn[at(p, a, c)]; };
@@ -105,105 +78,26 @@ __global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double 
         b.r_v[0][k] = rv;
         acc[0] += ru * ru + rv * rv;
     }
-    if (grid_sum<2>(acc, b.partials, nblocks, &st->counters[0]) && lane_id() == 0) {
+    put_partials<2>(acc, pset(b.partials, nb, PS_VELI), nb);
+}
+
+// p_0 = r_0, q_0 = A p_0
+__global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double alpha, Status* st, int nb,
+                                                     double rtol, int maxiter, int slot) {
+    double tot[2];
+    get_totals<2>(pset(b.partials, nb, PS_VELI), nb, tot);
+    const double tol2 = rtol * rtol * tot[1];
+    if (block_id() == 0 && lane_id() == 0) {
         KrylovScalars& s = st->vel;
-        s.rr = acc[0];
-        s.bnorm2 = acc[1];
-        s.tol2 = rtol * rtol * acc[1];
-        s.iter = 0;
+        s.rr = tot[0];
+        s.bnorm2 = tot[1];
+        s.tol2 = tol2;
         s.maxiter = maxiter;
-        s.beta = 0.0;
-        s.done = (acc[0] <= s.tol2) ? 1 : 0;
+        s.iter = 0;
+        s.done = (tot[0] <= tol2) ? 1 : 0;
+        st->vel_iters[slot] = 0;
     }
-}
-
-// p_new = r + beta p_old (p_new = r on the first iteration); q = A p_new; alpha = rr / (p.q)
-__global__ void __launch_bounds__(BLOCK) k_vel_dir(Params p, VelBufs b, double alpha, Status* st, int nblocks) {
-    if (st->vel.done) return;
-    const int it = st->vel.iter;
-    const double beta = st->vel.beta;
-    const double* pou = b.p_u[it & 1];
-    const double* pov = b.p_v[it & 1];
-    double* pnu = b.p_u[(it + 1) & 1];
-    double* pnv = b.p_v[(it + 1) & 1];
-    auto pu = [&](int a, int c) {
-        size_t k = at(p, a, c);
-        return it == 0 ? b.r_u[0][k] : b.r_u[0][k] + beta * pou[k];
-    };
-    auto pv = [&](int a, int c) {
-        size_t k = at(p, a, c);
-        return it == 0 ? b.r_v[0][k] : b.r_v[0][k] + beta * pov[k];
-    };
-    double acc[1] = {0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        double vu = pu(i, j), vv = pv(i, j);
-        double qu = dir_u(p, i) ? 0.0 : vu - alpha * visc_u(p, pu, pv, i, j, 0);
-        double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, pu, pv, i, j, 0);
-        pnu[k] = vu;
-        pnv[k] = vv;
-        b.q_u[0][k] = qu;
-        b.q_v[0][k] = qv;
-        acc[0] += vu * qu + vv * qv;
-    }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[1]) && lane_id() == 0) {
-        st->vel.pq = acc[0];
-        st->vel.alpha = st->vel.rr / acc[0];
-    }
-}
-
-// x += a p; r -= a q; convergence test on |r|
-__global__ void __launch_bounds__(BLOCK) k_vel_upd(Params p, VelBufs b, Status* st, int nblocks, int slot) {
-    if (st->vel.done) return;
-    const int it = st->vel.iter;
-    const double a = st->vel.alpha;
-    const double* pu = b.p_u[(it + 1) & 1];
-    const double* pv = b.p_v[(it + 1) & 1];
-    double acc[1] = {0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        b.xu[k] += a * pu[k];
-        b.xv[k] += a * pv[k];
-        double ru = b.r_u[0][k] - a * b.q_u[0][k];
-        double rv = b.r_v[0][k] - a * b.q_v[0][k];
-        b.r_u[0][k] = ru;
-        b.r_v[0][k] = rv;
-        acc[0] += ru * ru + rv * rv;
-    }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[2]) && lane_id() == 0) {
-        KrylovScalars& s = st->vel;
-        s.beta = acc[0] / s.rr;
-        s.rr = acc[0];
-        s.iter = it + 1;
-        st->vel_iters[slot] = s.iter;
-        if (s.rr <= s.tol2) s.done = 1;
-        else if (s.iter >= s.maxiter) {
-            s.done = 1;
-            // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
-            if (s.rr > 1e-12 * s.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(s.rr / s.bnorm2), (double)s.iter);
-        }
-    }
-}
-
-// ---- single-reduction CG: one kernel per iteration ----------------------
-// Iteration k+1 forms r_{k+1} = r_k - a_k q_k and p_{k+1} = r_{k+1} + b_{k+1} p_k
-// on the fly (own point and stencil neighbours), q_{k+1} = A p_{k+1},
-// x_{k+1} = x_k + a_k p_k, and the dot products p.q, r.q, q.q, r.r.  The
-// next beta comes from rr_{k+2} = rr_{k+1} - 2 a r.q + a^2 q.q; convergence is
-// tested on the true r.r.  r, p, q are ping-ponged; x is updated in place.
-__device__ __forceinline__ void vel_finish_scalars(KrylovScalars& s, double pq, double rq, double qq, double rr) {
-    s.pq = pq;
-    s.rr = rr;
-    s.alpha = rr / pq;
-    double next = rr - 2.0 * s.alpha * rq + s.alpha * s.alpha * qq;
-    s.beta = fmax(next, 0.0) / rr;
-    s.rq = rq;
-    s.qq = qq;
-}
-
-// p_0 = r_0, q_0 = A p_0 and the first scalars
-__global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double alpha, Status* st, int nblocks) {
-    if (st->vel.done) return;
+    if (tot[0] <= tol2) return;
     auto pu = [&](int a, int c) { return b.r_u[0][at(p, a, c)]; };
     auto pv = [&](int a, int c) { return b.r_v[0][at(p, a, c)]; };
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -221,16 +115,31 @@ __global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double
         acc[2] += qu * qu + qv * qv;
         acc[3] += vu * vu + vv * vv;
     }
-    if (grid_sum<4>(acc, b.partials, nblocks, &st->counters[11]) && lane_id() == 0)
-        vel_finish_scalars(st->vel, acc[0], acc[1], acc[2], acc[3]);
+    put_partials<4>(acc, pset(b.partials, nb, PS_VEL0), nb);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_vel_fused(Params p, VelBufs b, double alpha, Status* st, int nblocks,
-                                                     int slot) {
+__global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
+                                                    int slot) {
     if (st->vel.done) return;
-    const int it = st->vel.iter;
-    const double a = st->vel.alpha, beta = st->vel.beta;
-    const int o = it & 1, n = (it + 1) & 1;
+    double tot[4];
+    get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
+    const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
+    const KrylovScalars& sc = st->vel;
+    const bool conv = rr <= sc.tol2, stop = m >= sc.maxiter;
+    if (conv || stop) {
+        if (block_id() == 0 && lane_id() == 0) {
+            st->vel.done = 1;
+            st->vel.rr = rr;
+            st->vel.iter = m;
+            st->vel_iters[slot] = m;
+            // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
+            if (!conv && rr > 1e-12 * sc.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / sc.bnorm2), (double)m);
+        }
+        return;
+    }
+    const double a = rr / pq;
+    const double beta = fmax(rr - 2.0 * a * rq + a * a * qq, 0.0) / rr;
+    const int o = m & 1, n = (m + 1) & 1;
     const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
     auto pnu = [&](int c0, int c1) {
         size_t k = at(p, c0, c1);
@@ -260,21 +169,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_fused(Params p, VelBufs b, double
         acc[2] += q1u * q1u + q1v * q1v;
         acc[3] += r1u * r1u + r1v * r1v;
     }
-    if (grid_sum<4>(acc, b.partials, nblocks, &st->counters[12]) && lane_id() == 0) {
-        KrylovScalars& s = st->vel;
-        s.iter = it + 1;
-        st->vel_iters[slot] = s.iter;
-        if (acc[3] <= s.tol2) {
-            s.rr = acc[3];
-            s.done = 1;
-        } else if (s.iter >= s.maxiter) {
-            s.rr = acc[3];
-            s.done = 1;
-            if (s.rr > 1e-12 * s.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(s.rr / s.bnorm2), (double)s.iter);
-        } else {
-            vel_finish_scalars(s, acc[0], acc[1], acc[2], acc[3]);
-        }
-    }
+    put_partials<4>(acc, pset(b.partials, nb, n ? PS_VEL1 : PS_VEL0), nb);
+    if (block_id() == 0 && lane_id() == 0) st->vel_iters[slot] = m + 1;
 }
 
 // NoViscosity: the predictor is the stage sum with Dirichlet values (:794-796).
@@ -291,7 +187,7 @@ int grid_blocks(const Params& p) {
     dim3 g = grid2(p);
     return (int)(g.x * g.y);
 }
-
+size_t partials_size(const Params& p) { return (size_t)PS_COUNT * 4 * (size_t)grid_blocks(p); }
 
 // ===========================================================================
 // Projection: rhs, PCG, multigrid V-cycle, finish
@@ -357,8 +253,7 @@ __device__ __forceinline__ double pweight(int f, int I, int nf, int per, int dlo
 
 // rhs = (div(v*) - G) - b_bc, with the injected boundary faces of the
 // predictor (_inject_compact_predictor! :543-579) written first.
-__global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, Status* st, int nblocks, int has_null,
-                                                    double rtol, int maxiter) {
+__global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb) {
     const double area = p.hx * p.hy;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     FOR_TILE(p) {
@@ -380,33 +275,37 @@ __global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, Status
         acc[2] += rhs * rhs;
         acc[3] += div * div * area;
     }
-    if (grid_sum<4>(acc, b.partials, nblocks, &st->counters[3]) && lane_id() == 0) {
+    put_partials<4>(acc, pset(b.partials, nb, PS_RHS), nb);
+}
+
+// compatibility check (src/Projection.jl:206-221), solver tolerance, and
+// r = b - (-A) x with b = -(rhs - mean)
+__global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int has_null,
+                                                    double rtol, int maxiter) {
+    double tot[4];
+    get_totals<4>(pset(b.partials, nb, PS_RHS), nb, tot);
+    const double area = p.hx * p.hy;
+    const double n = (double)p.nx * (double)p.ny;
+    const double mean = has_null ? (tot[0] / area) / n : 0.0;
+    if (block_id() == 0 && lane_id() == 0) {
         ProjScalars& q = st->proj;
-        q.defect = has_null ? acc[0] : 0.0;
-        q.scale = acc[1];
-        q.rhs2 = acc[2];
-        q.pdiv_l2sq = acc[3];
+        q.defect = has_null ? tot[0] : 0.0;
+        q.scale = tot[1];
+        q.rhs2 = tot[2];
+        q.pdiv_l2sq = tot[3];
+        q.mean_rhs = mean;
         q.linf = 0.0;
-        if (has_null) {  // compatibility check, src/Projection.jl:206-221
-            double tol = 1e-12 + 1e-10 * fmax(acc[1], 2.220446049250313e-16);
-            if (!(fabs(acc[0]) <= tol)) raise_error(st, DEV_INCOMPATIBLE, acc[0], tol);
+        if (has_null) {
+            double tol = 1e-12 + 1e-10 * fmax(tot[1], 2.220446049250313e-16);
+            if (!(fabs(tot[0]) <= tol)) raise_error(st, DEV_INCOMPATIBLE, tot[0], tol);
         }
-        const double n = (double)p.nx * (double)p.ny;
-        q.mean_rhs = has_null ? (acc[0] / area) / n : 0.0;
         KrylovScalars& s = st->pcg;
-        s.bnorm2 = fmax(acc[2] - n * q.mean_rhs * q.mean_rhs, 0.0);
+        s.bnorm2 = fmax(tot[2] - n * mean * mean, 0.0);
         s.tol2 = rtol * rtol * s.bnorm2;
         s.iter = 0;
         s.maxiter = maxiter;
         s.done = 0;
-        s.beta = 0.0;
-        s.rz = 0.0;
     }
-}
-
-// r = b - (-A) x with b = -(rhs - mean)
-__global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nblocks) {
-    const double mean = st->proj.mean_rhs;
     GAcc X{b.x, p.P};
     double acc[1] = {0.0};
     FOR_TILE(p) {
@@ -415,10 +314,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs
         b.r[k] = r;
         acc[0] += r * r;
     }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[4]) && lane_id() == 0) {
-        st->pcg.rr = acc[0];
-        st->pcg.done = (acc[0] <= st->pcg.tol2) ? 1 : 0;
-    }
+    put_partials<1>(acc, pset(b.partials, nb, PS_RR), nb);
 }
 
 // ---- multigrid level kernels (every one returns at once once PCG is done)
@@ -441,58 +337,26 @@ __global__ void __launch_bounds__(BLOCK) k_mg_jacobi(MgGeom g, const double* xin
         xout[k] = xin[k] + omega * (b[k] - ax) / d;
     }
 }
-__global__ void __launch_bounds__(BLOCK) k_mg_resid(MgGeom g, const double* x, const double* b, double* r,
-                                                    const int* done) {
-    if (*done) return;
-    FOR_TILE(g) {
-        const size_t k = (size_t)j * g.P + i;
-        r[k] = b[k] - neg_lap(g, GAcc{x, g.P}, i, j, nullptr);
+// single-level hierarchy: the convergence test the finest down-kernel
+// would do, and the r.z partials the finest up-kernel would write
+__global__ void k_pcg_check(Status* st, const double* rr, int nb, int m, int slot) {
+    if (st->pcg.done) return;
+    double tot[1];
+    get_totals<1>(rr, nb, tot);
+    if (threadIdx.x == 0 && (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter)) {
+        st->pcg.done = 1;
+        st->pcg.iter = m;
+        if (slot >= 0) st->proj_iters[slot] = m;
     }
 }
-// b_c(I,J) = sum_f 0.25 * P(f,I) * r_f  (R = P^T / 4)
-__global__ void __launch_bounds__(BLOCK) k_mg_restrict(MgGeom gf, MgGeom gc, const double* rf, double* bc,
-                                                       const int* done) {
+__global__ void __launch_bounds__(BLOCK) k_rz(Params p, ProjBufs b, double* rz, int nb, const int* done) {
     if (*done) return;
-    const int I = blockIdx.x * TILE_X + threadIdx.x;
-    for (int rr_ = 0; rr_ < ROWS; ++rr_) {
-        const int J = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;
-        if (I >= gc.nx || J >= gc.ny) continue;
-        double s = 0.0;
-        for (int dy = -1; dy <= 2; ++dy) {
-            int fy = 2 * J + dy;
-            if (fy < 0 || fy >= gf.ny) {
-                if (!gf.py) continue;
-                fy = (fy + gf.ny) % gf.ny;
-            }
-            double wy = pweight(fy, J, gf.ny, gf.py, gf.db, gf.dt);
-            if (wy == 0.0) continue;
-            for (int dx = -1; dx <= 2; ++dx) {
-                int fx = 2 * I + dx;
-                if (fx < 0 || fx >= gf.nx) {
-                    if (!gf.px) continue;
-                    fx = (fx + gf.nx) % gf.nx;
-                }
-                double wx = pweight(fx, I, gf.nx, gf.px, gf.dl, gf.dr);
-                if (wx == 0.0) continue;
-                s += 0.25 * wx * wy * rf[(size_t)fy * gf.P + fx];
-            }
-        }
-        bc[(size_t)J * gc.P + I] = s;
+    double acc[1] = {0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        acc[0] += b.r[k] * b.z[k];
     }
-}
-// x_f += P x_c
-__global__ void __launch_bounds__(BLOCK) k_mg_prolong(MgGeom gf, MgGeom gc, const double* xc, double* xf,
-                                                      const int* done) {
-    if (*done) return;
-    FOR_TILE(gf) {
-        int X0, X1, Y0, Y1;
-        double wx0, wx1, wy0, wy1;
-        prol_1d(i, gf.nx, gf.px, gf.dl, gf.dr, X0, wx0, X1, wx1);
-        prol_1d(j, gf.ny, gf.py, gf.db, gf.dt, Y0, wy0, Y1, wy1);
-        double e = wy0 * (wx0 * xc[(size_t)Y0 * gc.P + X0] + wx1 * xc[(size_t)Y0 * gc.P + X1]) +
-                   wy1 * (wx0 * xc[(size_t)Y1 * gc.P + X0] + wx1 * xc[(size_t)Y1 * gc.P + X1]);
-        xf[(size_t)j * gf.P + i] += e;
-    }
+    put_partials<1>(acc, rz, nb);
 }
 
 // ---- coarsest levels: one workgroup, the whole remaining V-cycle in LDS
@@ -652,31 +516,19 @@ static size_t coarse_lds_bytes(MgGeom g) {
     return total * sizeof(double);
 }
 
-// rz = r.z; beta = rz / rz_old
-__global__ void __launch_bounds__(BLOCK) k_pcg_rz(Params p, ProjBufs b, Status* st, int nblocks) {
+// p_m = z + beta p_{m-1} (beta = rz_m / rz_{m-1}), w = (-A) p_m, partial p.w
+__global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int m) {
     if (st->pcg.done) return;
-    double acc[1] = {0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        acc[0] += b.r[k] * b.z[k];
-    }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[5]) && lane_id() == 0) {
-        KrylovScalars& s = st->pcg;
-        s.rz_old = s.rz;
-        s.rz = acc[0];
-        s.beta = s.iter == 0 ? 0.0 : acc[0] / s.rz_old;
-    }
-}
-// p_new = z + beta p_old; w = (-A) p_new; alpha = rz / (p.w)
-__global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs b, Status* st, int nblocks) {
-    if (st->pcg.done) return;
-    const int it = st->pcg.iter;
-    const double beta = st->pcg.beta;
-    const double* po = b.p[it & 1];
-    double* pn = b.p[(it + 1) & 1];
+    double tot[1];
+    get_totals<1>(pset(b.partials, nb, PS_RZ), nb, tot);
+    const double rz = tot[0];
+    const double beta = m == 0 ? 0.0 : rz / st->pcg.rzs[(m - 1) & 1];
+    if (block_id() == 0 && lane_id() == 0) st->pcg.rzs[m & 1] = rz;
+    const double* po = b.p[(m + 1) & 1];
+    double* pn = b.p[m & 1];
     auto P = [&](int a, int c) {
         size_t k = at(p, a, c);
-        return it == 0 ? b.z[k] : b.z[k] + beta * po[k];
+        return m == 0 ? b.z[k] : b.z[k] + beta * po[k];
     };
     double acc[1] = {0.0};
     FOR_TILE(p) {
@@ -687,16 +539,15 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs 
         b.w[k] = w;
         acc[0] += pv * w;
     }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[6]) && lane_id() == 0) {
-        st->pcg.pq = acc[0];
-        st->pcg.alpha = st->pcg.rz / acc[0];
-    }
+    put_partials<1>(acc, pset(b.partials, nb, PS_PW), nb);
 }
-__global__ void __launch_bounds__(BLOCK) k_pcg_upd(Params p, ProjBufs b, Status* st, int nblocks, int slot) {
+// x += a p_m; r -= a w (a = rz_m / p.w); partial r.r
+__global__ void __launch_bounds__(BLOCK) k_pcg_upd(Params p, ProjBufs b, Status* st, int nb, int m, int slot) {
     if (st->pcg.done) return;
-    const int it = st->pcg.iter;
-    const double a = st->pcg.alpha;
-    const double* pn = b.p[(it + 1) & 1];
+    double tot[1];
+    get_totals<1>(pset(b.partials, nb, PS_PW), nb, tot);
+    const double a = st->pcg.rzs[m & 1] / tot[0];
+    const double* pn = b.p[m & 1];
     double acc[1] = {0.0};
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
@@ -705,23 +556,16 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_upd(Params p, ProjBufs b, Status*
         b.r[k] = r;
         acc[0] += r * r;
     }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[7]) && lane_id() == 0) {
-        KrylovScalars& s = st->pcg;
-        s.rr = acc[0];
-        s.iter = it + 1;
-        if (slot >= 0) st->proj_iters[slot] = s.iter;
-        if (s.rr <= s.tol2 || s.iter >= s.maxiter) s.done = 1;
-    }
+    put_partials<1>(acc, pset(b.partials, nb, PS_RR), nb);
+    if (block_id() == 0 && lane_id() == 0 && slot >= 0) st->proj_iters[slot] = m + 1;
 }
 
 // mean of q (area weighted, src/Projection.jl:255-259)
-__global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, Status* st, int nblocks, int has_null,
-                                                     double lxly) {
+__global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int nb) {
     double acc[1] = {0.0};
     const double area = p.hx * p.hy;
     FOR_TILE(p) { acc[0] += b.x[at(p, i, j)] * area; }
-    if (grid_sum<1>(acc, b.partials, nblocks, &st->counters[8]) && lane_id() == 0)
-        st->proj.mean_q = has_null ? acc[0] / lxly : 0.0;
+    put_partials<1>(acc, pset(b.partials, nb, PS_MEAN), nb);
 }
 
 // v -= grad q on every face (src/Projection.jl:266-273), q with its BC data.
@@ -773,10 +617,12 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
 }
 
 // q_out = q - mean; true residual |A q - rhs| (:261-264); constraint div(v) - G
-// in L-inf and area-weighted L2 (:275-297), with the checks.
-__global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nblocks,
-                                                      double lxly) {
-    const double mean = st->proj.mean_q;
+// in L-inf and area-weighted L2 (:275-297)
+__global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
+                                                      int has_null, double lxly) {
+    double tm[1];
+    get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
+    const double mean = has_null ? tm[0] / lxly : 0.0;
     const double area = p.hx * p.hy;
     auto Q = [&](int a, int c) { return b.x[at(p, a, c)] - mean; };
     double acc[2] = {0.0, 0.0};
@@ -795,20 +641,24 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
         acc[1] += c * c * area;
     }
     block_max_to(linf, &st->proj.linf);
-    if (grid_sum<2>(acc, b.partials, nblocks, &st->counters[9]) && lane_id() == 0) {
-        ProjScalars& q = st->proj;
-        q.resid2 = acc[0];
-        q.l2sq = acc[1];
-        double rn = sqrt(q.rhs2);
-        q.lin_rel = sqrt(acc[0]) / fmax(rn, 2.220446049250313e-16);
-        q.l2 = sqrt(acc[1] / lxly);
-        double lin = load_coherent(&q.linf);
-        atomic_max_double(&st->max_constraint_linf, lin);
-        if (!(q.lin_rel <= 1e-8)) raise_error(st, DEV_PROJ_RESIDUAL, q.lin_rel, 1e-8);
-        double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), 0.0), 1.0);
-        double ctol = 1e-12 + 1e-8 * cs;
-        if (!(q.l2 <= ctol)) raise_error(st, DEV_PROJ_CONSTRAINT, q.l2, ctol);
-    }
+    put_partials<2>(acc, pset(b.partials, nb, PS_CHECK), nb);
+}
+
+// the checks of project! (:279-298), one block
+__global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, int nb, double lxly) {
+    double tot[2];
+    get_totals<2>(pset(b.partials, nb, PS_CHECK), nb, tot);
+    if (lane_id() != 0) return;
+    ProjScalars& q = st->proj;
+    q.resid2 = tot[0];
+    q.l2sq = tot[1];
+    q.lin_rel = sqrt(tot[0]) / fmax(sqrt(q.rhs2), 2.220446049250313e-16);
+    q.l2 = sqrt(tot[1] / lxly);
+    st->max_constraint_linf = fmax(st->max_constraint_linf, q.linf);
+    if (!(q.lin_rel <= 1e-8)) raise_error(st, DEV_PROJ_RESIDUAL, q.lin_rel, 1e-8);
+    double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), 0.0), 1.0);
+    double ctol = 1e-12 + 1e-8 * cs;
+    if (!(q.l2 <= ctol)) raise_error(st, DEV_PROJ_CONSTRAINT, q.l2, ctol);
 }
 
 // Plain copy, the calibration kernel of the traffic measurement.
@@ -823,114 +673,86 @@ static dim3 grid_of(int nx, int ny) { return dim3((nx + TILE_X - 1) / TILE_X, (n
 static const dim3 kBlock(TILE_X, TILE_Y);
 
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
-                     int maxiter) {
-    k_vel_init<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p), rtol, maxiter);
+                     int maxiter, int slot) {
+    const int nb = grid_blocks(p);
+    k_vel_init<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, nb);
+    k_vel_first<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, rtol, maxiter, slot);
 }
-void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
+void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot) {
     const int nb = grid_blocks(p);
-    for (int c = 0; c < count; ++c) {
-        k_vel_dir<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb);
-        k_vel_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, slot);
-    }
-}
-void launch_vel_first(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st) {
-    k_vel_first<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
-}
-void launch_vel_fused(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int count,
-                      int slot) {
-    const int nb = grid_blocks(p);
-    for (int c = 0; c < count; ++c) k_vel_fused<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, slot);
+    for (int c = 0; c < count; ++c) k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m0 + c, slot);
 }
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
     k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
 }
-
-void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int has_null, double rtol,
-                     int maxiter) {
-    k_proj_rhs<<<grid2(p), kBlock, 0, s>>>(p, b, st, grid_blocks(p), has_null, rtol, maxiter);
+void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot) {
+    k_pcg_check<<<1, BLOCK, 0, s>>>(st, rr, nb, m, slot);
 }
-void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st) {
-    k_pcg_init<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, grid_blocks(p));
+void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, int nb, const int* done) {
+    k_rz<<<grid2(p), kBlock, 0, s>>>(p, b, rz, nb, done);
 }
 
-static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done);
-
-// Fused V-cycle: one LDS-tiled kernel down and one up per level; the finest
-// up-kernel also forms r.z and beta (returns true: no separate dot needed).
-static void vcycle_fused(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, Status* st,
-                         double* partials) {
-    const double omega = 0.8;
-    const int* done = &st->pcg.done;
-    for (int l = 0; l < nlev - 1; ++l) launch_mg_down(s, geo[l], geo[l + 1], lv[l].b, lv[l].x, lv[l + 1].b, omega, done);
-    coarse_solve(s, lv, geo, nlev, done);
-    for (int l = nlev - 2; l >= 0; --l)
-        launch_mg_up(s, geo[l], geo[l + 1], lv[l].x, lv[l].b, lv[l + 1].t, lv[l].t, omega, done,
-                     l == 0 ? st : nullptr, partials);
+void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b) {
+    k_proj_rhs<<<grid2(p), kBlock, 0, s>>>(p, b, grid_blocks(p));
 }
-
-static void vcycle(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done) {
-    const double omega = 0.8;
-    for (int l = 0; l < nlev - 1; ++l) {
-        const MgGeom& g = geo[l];
-        dim3 gr = grid_of(g.nx, g.ny);
-        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
-        k_mg_resid<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].r, done);
-        const MgGeom& gc = geo[l + 1];
-        k_mg_restrict<<<grid_of(gc.nx, gc.ny), kBlock, 0, s>>>(g, gc, lv[l].r, lv[l + 1].b, done);
-    }
-    coarse_solve(s, lv, geo, nlev, done);
-    for (int l = nlev - 2; l >= 0; --l) {
-        const MgGeom& g = geo[l];
-        dim3 gr = grid_of(g.nx, g.ny);
-        k_mg_prolong<<<gr, kBlock, 0, s>>>(g, geo[l + 1], lv[l + 1].t, lv[l].t, done);
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
-    }
+void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
+                     double rtol, int maxiter) {
+    k_pcg_init<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, grid_blocks(p), has_null, rtol, maxiter);
 }
 
 static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done) {
     const double omega = 0.8;
-    {
-        const int l = nlev - 1;
-        const MgGeom& g = geo[l];
-        if ((long)g.nx * g.ny <= CO_MAX_CELLS) {
-            k_mg_coarse<<<1, CO_THREADS, coarse_lds_bytes(g), s>>>(g, lv[l].b, lv[l].t, done);
-        } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
-            dim3 gr = grid_of(g.nx, g.ny);
-            k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
-            const int sweeps = 2 * (g.nx > g.ny ? g.nx : g.ny);
-            for (int it = 0; it < sweeps; ++it) {
-                k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
-                k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
-            }
+    const int l = nlev - 1;
+    const MgGeom& g = geo[l];
+    if ((long)g.nx * g.ny <= CO_MAX_CELLS) {
+        k_mg_coarse<<<1, CO_THREADS, coarse_lds_bytes(g), s>>>(g, lv[l].b, lv[l].t, done);
+    } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
+        dim3 gr = grid_of(g.nx, g.ny);
+        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
+        const int sweeps = 2 * (g.nx > g.ny ? g.nx : g.ny);
+        for (int it = 0; it < sweeps; ++it) {
             k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
         }
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
     }
 }
 
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
-                      int nlev, Status* st, int count, int slot, int fused) {
+                      int nlev, Status* st, int m0, int count, int slot) {
     const int nb = grid_blocks(p);
+    const double omega = 0.8;
+    const int* done = &st->pcg.done;
+    double* rr = b.partials + (size_t)PS_RR * 4 * nb;
+    double* rz = b.partials + (size_t)PS_RZ * 4 * nb;
     for (int c = 0; c < count; ++c) {
-        if (fused && nlev > 1) {
-            vcycle_fused(s, lv, geo, nlev, st, b.partials);
+        const int m = m0 + c;
+        if (nlev > 1) {
+            for (int l = 0; l < nlev - 1; ++l)
+                launch_mg_down(s, geo[l], geo[l + 1], lv[l].b, lv[l].x, lv[l + 1].b, omega, done,
+                               l == 0 ? st : nullptr, rr, nb, m, slot);
+            coarse_solve(s, lv, geo, nlev, done);
+            for (int l = nlev - 2; l >= 0; --l)
+                launch_mg_up(s, geo[l], geo[l + 1], lv[l].x, lv[l].b, lv[l + 1].t, lv[l].t, omega, done,
+                             l == 0 ? rz : nullptr, nb);
         } else {
-            vcycle(s, lv, geo, nlev, &st->pcg.done);
-            k_pcg_rz<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb);
+            launch_pcg_check(s, st, rr, nb, m, slot);
+            coarse_solve(s, lv, geo, nlev, done);
+            launch_rz(s, p, b, rz, nb, done);
         }
-        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], b, st, nb);
-        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, slot);
+        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], b, st, nb, m);
+        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, m, slot);
     }
 }
 
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly) {
     const int nb = grid_blocks(p);
-    k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, has_null, lxly);
+    k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, nb);
     k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
-    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, lxly);
+    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
+    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly);
 }
 
 // One launch of a named solver kernel with the buffers of a live solve, for
@@ -945,48 +767,30 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         k_copy<<<grid2(p), kBlock, 0, s>>>(p, pb.rhs, pb.w);
         return 2 * cell;
     }
-    if (n == "vel_dir") {
-        k_vel_dir<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb);
-        return 8 * cell;  // r, p_old (u, v) read; p_new, q (u, v) written
-    }
-    if (n == "vel_fused") {
-        k_vel_fused<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 0);
+    if (n == "vel_iter") {
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0);
         return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
     }
-    if (n == "vel_upd") {
-        k_vel_upd<<<grid2(p), kBlock, 0, s>>>(p, vb, st, nb, 0);
-        return 12 * cell;  // x, p, r, q (u, v) read; x, r (u, v) written
-    }
     if (n == "pcg_dir") {
-        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], pb, st, nb);
+        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], pb, st, nb, 2);
         return 4 * cell;  // z, p_old read; p_new, w written
     }
     if (n == "pcg_upd") {
-        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb, 0);
+        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb, 2, 0);
         return 6 * cell;  // x, p, r, w read; x, r written
     }
-    if (n == "pcg_rz") {
-        k_pcg_rz<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb);
-        return 2 * cell;
+    if (n == "mg_down0" && geo[0].nx != geo[1].nx) {
+        launch_mg_down(s, geo[0], geo[1], lv[0].b, lv[0].x, lv[1].b, 0.8, &st->pcg.done, nullptr, nullptr, nb, 2, 0);
+        return 2.25 * cell;  // b read; x2 and b_c written
+    }
+    if (n == "mg_up0" && geo[0].nx != geo[1].nx) {
+        launch_mg_up(s, geo[0], geo[1], lv[0].x, lv[0].b, lv[1].t, lv[0].t, 0.8, &st->pcg.done, nullptr, nb);
+        return 3.25 * cell;  // x2, b, e_c read; x5 written
     }
     if (n == "mg_jacobi") {
         k_mg_jacobi<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], lv[0].x, lv[0].b, lv[0].t, 0.8,
                                                                     &st->pcg.done);
         return 3 * cell;  // x, b read; x' written
-    }
-    if (n == "mg_resid") {
-        k_mg_resid<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], lv[0].t, lv[0].b, lv[0].r, &st->pcg.done);
-        return 3 * cell;
-    }
-    if (n == "mg_prolong") {
-        k_mg_prolong<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], geo[1], lv[1].t, lv[0].t,
-                                                                      &st->pcg.done);
-        return 2.25 * cell;
-    }
-    if (n == "mg_restrict") {
-        k_mg_restrict<<<grid_of(geo[1].nx, geo[1].ny), kBlock, 0, s>>>(geo[0], geo[1], lv[0].r, lv[1].b,
-                                                                       &st->pcg.done);
-        return 1.25 * cell;
     }
     return -1.0;
 }

@@ -233,9 +233,8 @@ fsi_status fsi_eval_tendencies(fsi_context* ctx, const fsi_state* st, fsi_worksp
  * stream with the buffers of the workspace (after at least one step), timed
  * with HIP events on that stream; returns the mean duration of one launch in
  * milliseconds and the kernel's algorithmic HBM bytes per launch.  Names:
- * "copy" (calibration), "vel_dir", "vel_upd", "pcg_dir", "pcg_upd",
- * "pcg_rz", "mg_jacobi", "mg_resid", "mg_restrict", "mg_prolong",
- * "momentum", "cells". */
+ * "copy" (calibration), "vel_iter", "pcg_dir", "pcg_upd", "mg_down0",
+ * "mg_up0", "mg_jacobi", "momentum", "cells". */
 fsi_status fsi_benchmark_kernel(fsi_context* ctx, fsi_state* st, fsi_workspace* ws, const char* name,
                                 int32_t reps, double* avg_ms, double* bytes_per_launch);
 

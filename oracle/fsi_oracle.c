@@ -67,6 +67,8 @@ struct fsio_ctx {
 };
 
 /* ------------------------------------------------------------------------- */
+static void apply_extension_reset(fsio_ctx* c, double* sxx, double* syy, double* sxy, const double* phi);
+
 static int set_err(fsio_ctx* c, int code, const char* fmt, ...) {
     va_list ap;
     va_start(ap, fmt);
@@ -1671,14 +1673,23 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
     return rc;
 }
 
+/* Tendencies of the stage-1 state: sigma is taken after the safety reset of
+ * _copy_time_n_state! (:686-702), as the first stage of integrate_step! sees it. */
 int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const double* phi,
                          const double* sxx, const double* syy, const double* sxy, double t,
                          double* E_u, double* E_v, double* I_u, double* I_v, double* E_phi,
                          double* E_sxx, double* E_syy, double* E_sxy) {
     double* G = dalloc(c->N);
-    int rc = tendencies(c, U, V, phi, sxx, syy, sxy, G, t, E_u, E_v, I_u, I_v, E_phi, E_sxx,
-                        E_syy, E_sxy);
+    double *a = dalloc(c->N), *d = dalloc(c->N), *b = dalloc(c->N);
+    memcpy(a, sxx, sizeof(double) * c->N);
+    memcpy(d, syy, sizeof(double) * c->N);
+    memcpy(b, sxy, sizeof(double) * c->N);
+    apply_extension_reset(c, a, d, b, phi);
+    int rc = tendencies(c, U, V, phi, a, d, b, G, t, E_u, E_v, I_u, I_v, E_phi, E_sxx, E_syy, E_sxy);
     free(G);
+    free(a);
+    free(d);
+    free(b);
     return rc;
 }
 

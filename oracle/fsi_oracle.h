@@ -37,7 +37,7 @@ int fsio_make_velocity_compatible(fsio_ctx* c, double* U, double* V, const doubl
 /* project! with target divergence G = 0. */
 int fsio_project(fsio_ctx* c, double* U, double* V, double* correction,
                  fsi_projection_result* res);
-int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const double* phi,
+int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const double* phi, /* sigma after the safety reset */
                          const double* sxx, const double* syy, const double* sxy, double t,
                          double* E_u, double* E_v, double* I_u, double* I_v, double* E_phi,
                          double* E_sxx, double* E_syy, double* E_sxy);
