@@ -49,7 +49,7 @@ def main(argv=None):
     ap.add_argument("--N", type=int, default=1024)
     ap.add_argument("--method", default="acdi")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=1, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--only-kernel", default=None, help="time one kernel standalone and exit (profiling runs)")
     args = ap.parse_args(argv)
@@ -104,7 +104,8 @@ def main(argv=None):
 
     # per-kernel measurement: the candidates for the dominant kernel
     kernels = {}
-    for name in ("vel_iter", "pcg_dir", "pcg_upd", "mg_down0", "mg_up0", "momentum", "cells", "copy"):
+    for name in ("vel_iter", "pcg_dir", "pcg_upd", "mg_down0", "mg_up0", "mg_coarse", "pcg_iter", "momentum",
+                 "cells", "copy"):
         ms, nb = bench_kernel(name, args.kernel_reps)
         kernels[name] = {"avg_ms": ms, "bytes": nb, "GBps": nb / ms / 1e6}
         print(f"kernel {name}: {ms * 1e3:.1f} us, {nb / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
@@ -113,7 +114,7 @@ def main(argv=None):
     nv = float(np.sum(vel_mean))
     npj = float(np.sum(proj_mean))
     launches = {"vel_iter": nv + 3, "pcg_dir": npj, "pcg_upd": npj, "mg_down0": npj + 3, "mg_up0": npj,
-                "momentum": 3, "cells": 3}
+                "mg_coarse": npj, "momentum": 3, "cells": 3}
     share = {k: launches[k] * kernels[k]["avg_ms"] for k in launches}
     dominant = max(share, key=share.get)
     dk = kernels[dominant]

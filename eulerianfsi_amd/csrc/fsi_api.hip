@@ -57,6 +57,8 @@ struct fsi_context {
     Status* st = nullptr;       // device
     Status* st_host = nullptr;  // pinned mirror
     std::vector<MgGeom> geo;
+    CoarsePlan cplan{};         // single-workgroup coarse solve of geo.back()
+    double* co_dense = nullptr;  // its dense bottom inverse
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
 };
@@ -379,6 +381,9 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         g.dt = bq.top.kind == FSI_BC_DIRICHLET;
         g.ihx2 = 1.0 / (p.hx * p.hx);
         g.ihy2 = 1.0 / (p.hy * p.hy);
+        g.omega = 0.8;
+        g.d0 = g.ihx2 * 2.0 + g.ihy2 * 2.0;
+        g.wd0 = g.omega / g.d0;
         c->geo.push_back(g);
         while ((int)c->geo.size() < kMaxLevels && (long)g.nx * g.ny > 1024 && !(g.nx & 1) && !(g.ny & 1) &&
                g.nx >= 8 && g.ny >= 8) {
@@ -387,7 +392,17 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             g.P = ((g.nx + 31) / 32) * 32;
             g.ihx2 *= 0.25;
             g.ihy2 *= 0.25;
+            g.d0 = g.ihx2 * 2.0 + g.ihy2 * 2.0;
+            g.wd0 = g.omega / g.d0;
             c->geo.push_back(g);
+        }
+        std::vector<double> dense;
+        if (coarse_plan(c->geo.back(), c->cplan, dense) && c->cplan.dense_n > 0) {
+            HIPCHECK(c, dalloc(c, &c->co_dense, dense.size()));
+            HIPCHECK(c, hipMemcpyAsync(c->co_dense, dense.data(), dense.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, c->stream));
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+            c->cplan.dense = c->co_dense;
         }
     }
     HIPCHECK(c, dalloc(c, &c->partials, partials_size(p)));
@@ -403,7 +418,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
 extern "C" void fsi_context_destroy(fsi_context* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials};
+    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (c->st) (void)hipFree(c->st);
@@ -655,7 +670,8 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     int m = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
     while (m <= last) {
         int n = std::min(chunk, last + 1 - m);
-        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->st, m, n, slot);
+        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan, c->st, m, n,
+                         slot);
         m += n;
         fsi_status rc = poll_done(c, &c->st->pcg, &done, &iter);
         if (rc != FSI_OK) return rc;
@@ -913,7 +929,8 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
                          w->Ep[0], w->Exx[0], w->Eyy[0], w->Exy[0], c->st, 0);
             return 10.0 * 8.0 * p.nx * (double)p.ny;
         }
-        return bench_launch(c->stream, name, p, vb, pb, w->lv.data(), c->geo.data(), c->st);
+        return bench_launch(c->stream, name, p, vb, pb, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan,
+                            c->st);
     };
     double b = launch();
     if (b < 0) return fail(c, FSI_E_ARGUMENT, "unknown kernel name %s", name);

@@ -2,6 +2,8 @@
 // translation units of libfsi_hip.
 #pragma once
 
+#include <vector>
+
 #include "fsi_device.hpp"
 
 namespace fsi {
@@ -57,6 +59,20 @@ struct MgGeom {
     int px, py;
     int dl, dr, db, dt;  // 1 where the side is Dirichlet (else Neumann/periodic)
     double ihx2, ihy2;
+    double omega;        // damped-Jacobi weight
+    double d0, wd0;      // interior diagonal of -A (2/hx^2 + 2/hy^2) and omega / d0
+};
+
+// Coarse levels solved in one workgroup (fsi_mg.hip): a top level of at most
+// CO_MAX_CELLS cells and its halvings, with a bottom of at most CO_DENSE_MAX
+// cells solved by a dense (pseudo-)inverse.
+constexpr int CO_MAX_CELLS = 1024;
+constexpr int CO_DENSE_MAX = 64;
+struct CoarsePlan {
+    int levels;            // levels in the workgroup, top included
+    int dense_n;           // cells of the bottom when solved densely, else 0
+    size_t lds_bytes;      // dynamic LDS of the launch
+    const double* dense;   // device copy of the dense inverse (dense_n^2)
 };
 
 // ---- physics (fsi_physics.hip) ----
@@ -118,7 +134,7 @@ void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
-                      int nlev, Status* st, int m0, int count, int slot);
+                      int nlev, const CoarsePlan& cp, Status* st, int m0, int count, int slot);
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly);
 void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot);
@@ -131,11 +147,16 @@ int grid_blocks(const Params& p);
 // previous PCG kernel and ends the solve when converged; the finest up-kernel
 // (rz != nullptr) writes the r.z block partials.
 void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
-                    double omega, const int* done, Status* st, const double* rr, int nb, int m, int slot);
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot);
 void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
-                  const double* ec, double* xout, double omega, const int* done, double* rz, int nb);
+                  const double* ec, double* xout, const int* done, double* rz, int nb);
+// Plan of the single-workgroup coarse solve for a top level (false when the
+// level is too large); `dense` receives the bottom's inverse, row-major.
+bool coarse_plan(const MgGeom& top, CoarsePlan& cp, std::vector<double>& dense);
+void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
+                      const int* done);
 size_t partials_size(const Params& p);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
-                    const MgLevel* lv, const MgGeom* geo, Status* st);
+                    const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st);
 
 }  // namespace fsi

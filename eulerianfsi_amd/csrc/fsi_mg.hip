@@ -1,15 +1,23 @@
-// fsi_mg.hip -- fused multigrid level kernels of the projection V-cycle.
+// fsi_mg.hip -- multigrid V-cycle of the projection preconditioner.
 //
 // Per fine level the V-cycle needs, from a zero initial guess,
 //   down:  x1 = w D^-1 b;  x2 = x1 + w D^-1 (b - A x1);  b_c = R (b - A x2)
 //   up:    x3 = x2 + P e_c;  x4 = x3 + w D^-1 (b - A x3);  x5 = x4 + w D^-1 (b - A x4)
-// (A = minus the unit Laplacian, R = P^T/4).  Each is one kernel on an LDS
-// tile with the halo the chained stencils need (3 cells down, 2 up), so a
-// level costs 2 launches and reads b once, writes x2 once, and writes x5 once.
+// (A = minus the unit Laplacian, R = P^T/4, P bilinear).  Each is one kernel
+// on an LDS tile with the halo the chained stencils need (3 cells down, 2
+// up), so a level costs 2 launches, reads b once and writes x2, b_c and x5
+// once.  The levels of at most CO_MAX_CELLS cells run in one workgroup
+// (k_mg_coarse), whose bottom level is solved with a precomputed dense
+// (pseudo-)inverse.
 //
-// Tiles are 64 x 16 fine cells for 256 threads; halo cells outside a
-// non-periodic wall are never read (the wall enters the stencil through its
-// coefficient), halo cells across a periodic side hold the wrapped values.
+// Tiles hold 0 at points outside a non-periodic wall, so -A x at any cell is
+// d x - (x_W + x_E)/hx^2 - (x_S + x_N)/hy^2 with the wall's contribution in
+// the diagonal d alone; across a periodic side the tile holds the wrapped
+// values.  w/d is the precomputed interior constant except on wall rows and
+// columns.
+#include <cmath>
+#include <vector>
+
 #include "fsi_kernels.hpp"
 
 namespace fsi {
@@ -18,80 +26,69 @@ constexpr int FT_X = 64;  // fine tile
 constexpr int FT_Y = 16;
 constexpr int MG_THREADS = 256;
 
-// stencil of -A at a cell from its four neighbours; `has` flags which
-// neighbours exist (interior or periodic), walls add their coefficient.
-struct Nb {
-    int wx_lo, wx_hi, wy_lo, wy_hi;  // 0: neighbour exists, 1: Neumann wall, 2: Dirichlet wall
+// diagonal of -A at a cell and w / diagonal
+struct Diag {
+    double d, wd;
 };
-
-__device__ __forceinline__ Nb neighbours(const MgGeom& g, int gi, int gj) {
-    Nb n;
-    n.wx_lo = (gi > 0 || g.px) ? 0 : (g.dl ? 2 : 1);
-    n.wx_hi = (gi < g.nx - 1 || g.px) ? 0 : (g.dr ? 2 : 1);
-    n.wy_lo = (gj > 0 || g.py) ? 0 : (g.db ? 2 : 1);
-    n.wy_hi = (gj < g.ny - 1 || g.py) ? 0 : (g.dt ? 2 : 1);
-    return n;
+__device__ __forceinline__ Diag mg_diag(const MgGeom& g, int gi, int gj) {
+    const bool bx = !g.px && (gi == 0 || gi == g.nx - 1);
+    const bool by = !g.py && (gj == 0 || gj == g.ny - 1);
+    if (!(bx || by)) return {g.d0, g.wd0};
+    double cx = 2.0, cy = 2.0;
+    if (!g.px) {
+        if (gi == 0) cx += g.dl ? 1.0 : -1.0;
+        if (gi == g.nx - 1) cx += g.dr ? 1.0 : -1.0;
+    }
+    if (!g.py) {
+        if (gj == 0) cy += g.db ? 1.0 : -1.0;
+        if (gj == g.ny - 1) cy += g.dt ? 1.0 : -1.0;
+    }
+    const double d = g.ihx2 * cx + g.ihy2 * cy;
+    return {d, g.omega / d};
 }
 
-__device__ __forceinline__ double nb_diag(const MgGeom& g, const Nb& n) {
-    double cx = (n.wx_lo == 0 ? 1.0 : (n.wx_lo == 2 ? 2.0 : 0.0)) + (n.wx_hi == 0 ? 1.0 : (n.wx_hi == 2 ? 2.0 : 0.0));
-    double cy = (n.wy_lo == 0 ? 1.0 : (n.wy_lo == 2 ? 2.0 : 0.0)) + (n.wy_hi == 0 ? 1.0 : (n.wy_hi == 2 ? 2.0 : 0.0));
-    return g.ihx2 * cx + g.ihy2 * cy;
-}
-
-// -A x at the LDS point (li, lj) of an array with row length W.
-__device__ __forceinline__ double nb_apply(const MgGeom& g, const Nb& n, const double* a, int W, int li, int lj,
-                                          double d) {
-    const double c = a[lj * W + li];
-    double sx = 0.0, sy = 0.0;
-    if (n.wx_lo == 0) sx += a[lj * W + li - 1];
-    if (n.wx_hi == 0) sx += a[lj * W + li + 1];
-    if (n.wy_lo == 0) sy += a[(lj - 1) * W + li];
-    if (n.wy_hi == 0) sy += a[(lj + 1) * W + li];
-    return d * c - g.ihx2 * sx - g.ihy2 * sy;
-}
-
-// global index of tile-local coordinate, wrapped across periodic sides;
-// returns false when the point lies outside a non-periodic domain.
-__device__ __forceinline__ bool gcoord(int n, int per, int k, int& out) {
+// tile coordinate -> global index, wrapped across a periodic side; false
+// outside a non-periodic domain
+__device__ __forceinline__ bool wrap_idx(int n, int per, int k, int& out) {
     if (k >= 0 && k < n) {
         out = k;
         return true;
     }
     if (!per) return false;
-    out = ((k % n) + n) % n;
+    out = k < 0 ? k + n : k - n;
+    if (out < 0 || out >= n) out = ((k % n) + n) % n;
     return true;
 }
 
-__device__ __forceinline__ void prol_1d_mg(int f, int nf, int per, int dlo, int dhi, int& I0, double& w0, int& I1,
-                                           double& w1) {
-    const int nc = nf >> 1, parent = f >> 1;
-    int other = (f & 1) ? parent + 1 : parent - 1;
-    I0 = parent;
+// -A x at tile position c of an array with row length W (zero outside walls)
+__device__ __forceinline__ double tile_negA(const MgGeom& g, const double* a, int c, int W, double d) {
+    return d * a[c] - g.ihx2 * (a[c - 1] + a[c + 1]) - g.ihy2 * (a[c - W] + a[c + W]);
+}
+
+// 1-D restriction weights of fine 2I and 2I+1 onto coarse I (R = P^T/4 up to
+// the factor 1/4): 3/4 inside; at a non-periodic wall the fine cell's ghost
+// is the mirrored parent (Neumann, weight 1) or its negative (Dirichlet, 1/2).
+__device__ __forceinline__ void rweights(int I, int nf, int per, int dlo, int dhi, double& w0, double& w1) {
     w0 = 0.75;
-    I1 = other;
-    w1 = 0.25;
-    if (other < 0 || other >= nc) {
-        if (per) I1 = (other + nc) % nc;
-        else {
-            w0 += ((other < 0 ? dlo : dhi) ? -0.25 : 0.25);
-            I1 = parent;
-            w1 = 0.0;
-        }
+    w1 = 0.75;
+    if (!per) {
+        if (2 * I == 0) w0 = dlo ? 0.5 : 1.0;
+        if (2 * I + 1 == nf - 1) w1 = dhi ? 0.5 : 1.0;
     }
 }
 
 // ---------------------------------------------------------------------------
 // down: b (fine) -> x2 (fine, stored) and b_c (coarse)
 // ---------------------------------------------------------------------------
-constexpr int DH = 3;                 // halo of b
-constexpr int DW = FT_X + 2 * DH;     // 70
-constexpr int DHh = FT_Y + 2 * DH;    // 22
+constexpr int D3W = FT_X + 6, D3H = FT_Y + 6;  // halo 3: 70 x 22
+constexpr int D2W = FT_X + 4, D2H = FT_Y + 4;  // halo 2: 68 x 20
+constexpr int D1W = FT_X + 2, D1H = FT_Y + 2;  // halo 1: 66 x 18
+constexpr int CTX = FT_X / 2, CTY = FT_Y / 2;  // coarse tile 32 x 8
 
 __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, const double* __restrict__ b,
                                                         double* __restrict__ x2out, double* __restrict__ bc,
-                                                        double omega, const int* done, Status* st,
-                                                        const double* __restrict__ rr, int nb, int m, int slot) {
+                                                        const int* done, Status* st, const double* __restrict__ rr,
+                                                        int nb, int m, int slot) {
     if (*done) return;
     if (st) {  // finest level: convergence test on |r|^2 of the previous PCG kernel
         double tot[1];
@@ -106,183 +103,168 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
             return;
         }
     }
-    __shared__ double sb[DHh * DW];                       // b on halo 3
-    __shared__ double sx2[(DHh - 2) * (DW - 2)];          // x2 on halo 2 (68 x 20)
-    __shared__ double sr[(DHh - 4) * (DW - 4)];           // r on halo 1 (66 x 18)
+    __shared__ double sb[D3H * D3W];  // b on halo 3
+    __shared__ double sx[D3H * D3W];  // x1 on halo 3, then r on halo 1 (row length D1W)
+    __shared__ double s2[D2H * D2W];  // x2 on halo 2, then row sums of the restriction (D1H x CTX)
     const int tid = threadIdx.x;
     const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
-    // load b with halo 3 (points outside a wall stay unused; store 0)
-    for (int k = tid; k < DHh * DW; k += MG_THREADS) {
-        int li = k % DW, lj = k / DW;
+    // b and x1 = w D^-1 b on halo 3
+    for (int k = tid; k < D3H * D3W; k += MG_THREADS) {
+        const int li = k % D3W, lj = k / D3W;
         int gi, gj;
-        double v = 0.0;
-        if (gcoord(gf.nx, gf.px, fi0 + li - DH, gi) && gcoord(gf.ny, gf.py, fj0 + lj - DH, gj))
-            v = b[(size_t)gj * gf.P + gi];
-        sb[k] = v;
-    }
-    __syncthreads();
-    // x2 on halo 2: x2 = x1 + w (b - A x1)/d with x1 = w b / d (pointwise)
-    constexpr int W2 = DW - 2, H2 = DHh - 2;
-    for (int k = tid; k < W2 * H2; k += MG_THREADS) {
-        int li = k % W2, lj = k / W2;  // sb coordinates li+1, lj+1
-        int gi, gj;
-        double v = 0.0;
-        if (gcoord(gf.nx, gf.px, fi0 + li - 2, gi) && gcoord(gf.ny, gf.py, fj0 + lj - 2, gj)) {
-            Nb n = neighbours(gf, gi, gj);
-            const int si = li + 1, sj = lj + 1;
-            double d = nb_diag(gf, n);
-            // x1 at the centre and the existing neighbours
-            auto x1 = [&](int ii, int jj) -> double {
-                int ggi, ggj;
-                gcoord(gf.nx, gf.px, fi0 + ii - DH, ggi);
-                gcoord(gf.ny, gf.py, fj0 + jj - DH, ggj);
-                return omega * sb[jj * DW + ii] / nb_diag(gf, neighbours(gf, ggi, ggj));
-            };
-            double c = x1(si, sj), sx = 0.0, sy = 0.0;
-            if (n.wx_lo == 0) sx += x1(si - 1, sj);
-            if (n.wx_hi == 0) sx += x1(si + 1, sj);
-            if (n.wy_lo == 0) sy += x1(si, sj - 1);
-            if (n.wy_hi == 0) sy += x1(si, sj + 1);
-            double ax = d * c - gf.ihx2 * sx - gf.ihy2 * sy;
-            v = c + omega * (sb[sj * DW + si] - ax) / d;
+        double bv = 0.0, xv = 0.0;
+        if (wrap_idx(gf.nx, gf.px, fi0 + li - 3, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 3, gj)) {
+            bv = b[(size_t)gj * gf.P + gi];
+            xv = mg_diag(gf, gi, gj).wd * bv;
         }
-        sx2[k] = v;
+        sb[k] = bv;
+        sx[k] = xv;
     }
     __syncthreads();
-    // r on halo 1, and store x2 on the interior
-    constexpr int W1 = DW - 4, H1 = DHh - 4;
-    for (int k = tid; k < W1 * H1; k += MG_THREADS) {
-        int li = k % W1, lj = k / W1;  // sx2 coordinates li+1, lj+1
+    // x2 = x1 + w D^-1 (b - A x1) on halo 2
+    for (int k = tid; k < D2H * D2W; k += MG_THREADS) {
+        const int li = k % D2W, lj = k / D2W;
         int gi, gj;
         double v = 0.0;
-        if (gcoord(gf.nx, gf.px, fi0 + li - 1, gi) && gcoord(gf.ny, gf.py, fj0 + lj - 1, gj)) {
-            Nb n = neighbours(gf, gi, gj);
-            double d = nb_diag(gf, n);
-            double ax = nb_apply(gf, n, sx2, W2, li + 1, lj + 1, d);
-            v = sb[(lj + 2) * DW + li + 2] - ax;
-            const bool interior = li >= 1 && li <= FT_X && lj >= 1 && lj <= FT_Y;
-            if (interior && fi0 + li - 1 < gf.nx && fj0 + lj - 1 < gf.ny)
-                x2out[(size_t)(fj0 + lj - 1) * gf.P + fi0 + li - 1] = sx2[(lj + 1) * W2 + li + 1];
+        if (wrap_idx(gf.nx, gf.px, fi0 + li - 2, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 2, gj)) {
+            const Diag D = mg_diag(gf, gi, gj);
+            const int c = (lj + 1) * D3W + li + 1;
+            v = sx[c] + D.wd * (sb[c] - tile_negA(gf, sx, c, D3W, D.d));
         }
-        sr[k] = v;
+        s2[k] = v;
     }
     __syncthreads();
-    // restriction to the coarse tile (FT_X/2 x FT_Y/2), b_c = sum 0.25 P(f,I) r_f
-    const int ci0 = fi0 / 2, cj0 = fj0 / 2;
-    for (int k = tid; k < (FT_X / 2) * (FT_Y / 2); k += MG_THREADS) {
-        int I = ci0 + k % (FT_X / 2), J = cj0 + k / (FT_X / 2);
+    // r = b - A x2 on halo 1 (into sx), x2 stored on the tile
+    for (int k = tid; k < D1H * D1W; k += MG_THREADS) {
+        const int li = k % D1W, lj = k / D1W;
+        const int ui = fi0 + li - 1, uj = fj0 + lj - 1;
+        int gi, gj;
+        double v = 0.0;
+        if (wrap_idx(gf.nx, gf.px, ui, gi) && wrap_idx(gf.ny, gf.py, uj, gj)) {
+            const Diag D = mg_diag(gf, gi, gj);
+            const int c = (lj + 1) * D2W + li + 1;
+            v = sb[(lj + 2) * D3W + li + 2] - tile_negA(gf, s2, c, D2W, D.d);
+            if (li >= 1 && li <= FT_X && lj >= 1 && lj <= FT_Y && ui < gf.nx && uj < gf.ny)
+                x2out[(size_t)uj * gf.P + ui] = s2[c];
+        }
+        sx[k] = v;
+    }
+    __syncthreads();
+    // restriction, x then y: row sums of r over fine 2I-1 .. 2I+2
+    double* sh = s2;
+    for (int k = tid; k < D1H * CTX; k += MG_THREADS) {
+        const int ci = k % CTX, lj = k / CTX;
+        double w0, w1;
+        rweights(fi0 / 2 + ci, gf.nx, gf.px, gf.dl, gf.dr, w0, w1);
+        const double* row = sx + lj * D1W + 2 * ci;  // fine 2I-1 at halo-1 index 2 ci
+        sh[lj * CTX + ci] = 0.25 * row[0] + w0 * row[1] + w1 * row[2] + 0.25 * row[3];
+    }
+    __syncthreads();
+    for (int k = tid; k < CTY * CTX; k += MG_THREADS) {
+        const int ci = k % CTX, cj = k / CTX;
+        const int I = fi0 / 2 + ci, J = fj0 / 2 + cj;
         if (I >= gc.nx || J >= gc.ny) continue;
-        double s = 0.0;
-        for (int dy = -1; dy <= 2; ++dy) {
-            int fy = 2 * J + dy, gfy;
-            if (!gcoord(gf.ny, gf.py, fy, gfy)) continue;
-            int I0, I1;
-            double w0, w1;
-            prol_1d_mg(gfy, gf.ny, gf.py, gf.db, gf.dt, I0, w0, I1, w1);
-            double wy = (I0 == J ? w0 : 0.0) + (I1 == J ? w1 : 0.0);
-            if (wy == 0.0) continue;
-            for (int dx = -1; dx <= 2; ++dx) {
-                int fx = 2 * I + dx, gfx;
-                if (!gcoord(gf.nx, gf.px, fx, gfx)) continue;
-                prol_1d_mg(gfx, gf.nx, gf.px, gf.dl, gf.dr, I0, w0, I1, w1);
-                double wx = (I0 == I ? w0 : 0.0) + (I1 == I ? w1 : 0.0);
-                if (wx == 0.0) continue;
-                // local position of fine (fx, fy) in sr (halo 1)
-                int li = fx - fi0 + 1, lj = fy - fj0 + 1;
-                s += 0.25 * wx * wy * sr[lj * W1 + li];
-            }
-        }
-        bc[(size_t)J * gc.P + I] = s;
+        double w0, w1;
+        rweights(J, gf.ny, gf.py, gf.db, gf.dt, w0, w1);
+        const double* col = sh + 2 * cj * CTX + ci;
+        bc[(size_t)J * gc.P + I] =
+            0.25 * (0.25 * col[0] + w0 * col[CTX] + w1 * col[2 * CTX] + 0.25 * col[3 * CTX]);
     }
 }
 
 // ---------------------------------------------------------------------------
 // up: x2 (fine), e_c (coarse) -> x5 (fine); on the finest level also the
-// partial of r.z (b = r there) with the last block forming rz and beta.
+// block partials of r.z (b = r there).
 // ---------------------------------------------------------------------------
-constexpr int UH = 2;
-constexpr int UW = FT_X + 2 * UH;   // 68
-constexpr int UHh = FT_Y + 2 * UH;  // 20
-constexpr int CW = FT_X / 2 + 4;    // coarse tile with halo 2: 36
-constexpr int CH = FT_Y / 2 + 4;    // 12
+constexpr int CW = FT_X / 2 + 4;  // coarse values around the tile: 36 x 12
+constexpr int CH = FT_Y / 2 + 4;
+
+// bilinear weights of the parent (w0) and the nearer coarse neighbour (1/4)
+// of fine f; the tile stores 0 beyond a non-periodic wall, where the ghost
+// is the mirrored parent (Neumann, w0 = 1) or its negative (Dirichlet, 1/2).
+__device__ __forceinline__ void pweights(int f, int nf, int per, int dlo, int dhi, int& parent, int& other,
+                                         double& w0) {
+    parent = f >> 1;  // floor for negative tile coordinates as well
+    other = (f & 1) ? parent + 1 : parent - 1;
+    w0 = 0.75;
+    if (!per) {
+        if (f == 0) w0 = dlo ? 0.5 : 1.0;
+        if (f == nf - 1) w0 = dhi ? 0.5 : 1.0;
+    }
+}
 
 __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const double* __restrict__ x2,
                                                       const double* __restrict__ b, const double* __restrict__ ec,
-                                                      double* __restrict__ xout, double omega, const int* done,
-                                                      double* rz, int nb) {
+                                                      double* __restrict__ xout, const int* done, double* rz,
+                                                      int nb) {
     if (*done) return;
-    __shared__ double sx3[UHh * UW];              // x3 on halo 2
-    __shared__ double sx4[(UHh - 2) * (UW - 2)];  // x4 on halo 1
-    __shared__ double sbb[(UHh - 2) * (UW - 2)];  // b on halo 1
-    __shared__ double sec[CH * CW];               // e_c around the tile
+    __shared__ double sx3[D2H * D2W];  // x3 on halo 2
+    __shared__ double sx4[D1H * D1W];  // x4 on halo 1
+    __shared__ double sbb[D1H * D1W];  // b on halo 1
+    __shared__ double sec[CH * CW];    // e_c around the tile
     const int tid = threadIdx.x;
     const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
     for (int k = tid; k < CH * CW; k += MG_THREADS) {
-        int li = k % CW, lj = k / CW, gi, gj;
+        const int li = k % CW, lj = k / CW;
+        int gi, gj;
         double v = 0.0;
-        if (gcoord(gc.nx, gc.px, ci0 + li, gi) && gcoord(gc.ny, gc.py, cj0 + lj, gj)) v = ec[(size_t)gj * gc.P + gi];
+        if (wrap_idx(gc.nx, gc.px, ci0 + li, gi) && wrap_idx(gc.ny, gc.py, cj0 + lj, gj))
+            v = ec[(size_t)gj * gc.P + gi];
         sec[k] = v;
     }
-    constexpr int W1 = UW - 2, H1 = UHh - 2;
-    for (int k = tid; k < W1 * H1; k += MG_THREADS) {
-        int li = k % W1, lj = k / W1, gi, gj;
+    for (int k = tid; k < D1H * D1W; k += MG_THREADS) {
+        const int li = k % D1W, lj = k / D1W;
+        int gi, gj;
         double v = 0.0;
-        if (gcoord(gf.nx, gf.px, fi0 + li - 1, gi) && gcoord(gf.ny, gf.py, fj0 + lj - 1, gj))
+        if (wrap_idx(gf.nx, gf.px, fi0 + li - 1, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 1, gj))
             v = b[(size_t)gj * gf.P + gi];
         sbb[k] = v;
     }
     __syncthreads();
     // x3 = x2 + P e_c on halo 2
-    for (int k = tid; k < UHh * UW; k += MG_THREADS) {
-        int li = k % UW, lj = k / UW, gi, gj;
+    for (int k = tid; k < D2H * D2W; k += MG_THREADS) {
+        const int li = k % D2W, lj = k / D2W;
+        const int fx = fi0 + li - 2, fy = fj0 + lj - 2;
+        int gi, gj;
         double v = 0.0;
-        if (gcoord(gf.nx, gf.px, fi0 + li - UH, gi) && gcoord(gf.ny, gf.py, fj0 + lj - UH, gj)) {
-            int X0, X1, Y0, Y1;
-            double wx0, wx1, wy0, wy1;
-            prol_1d_mg(gi, gf.nx, gf.px, gf.dl, gf.dr, X0, wx0, X1, wx1);
-            prol_1d_mg(gj, gf.ny, gf.py, gf.db, gf.dt, Y0, wy0, Y1, wy1);
-            // parent / neighbour of the unwrapped fine coordinate, relative
-            // to the coarse tile loaded at unwrapped coordinates ci0, cj0
-            const int fx = fi0 + li - UH, fy = fj0 + lj - UH;
-            const int pxc = (fx >= 0) ? fx / 2 : -((-fx + 1) / 2);
-            const int pyc = (fy >= 0) ? fy / 2 : -((-fy + 1) / 2);
-            const int ox = (fx & 1) ? pxc + 1 : pxc - 1;
-            const int oy = (fy & 1) ? pyc + 1 : pyc - 1;
-            const int a0 = pxc - ci0, a1 = (wx1 != 0.0) ? ox - ci0 : a0;
-            const int b0 = pyc - cj0, b1 = (wy1 != 0.0) ? oy - cj0 : b0;
-            (void)X0; (void)X1; (void)Y0; (void)Y1;
-            double e = wy0 * (wx0 * sec[b0 * CW + a0] + wx1 * sec[b0 * CW + a1]) +
-                       wy1 * (wx0 * sec[b1 * CW + a0] + wx1 * sec[b1 * CW + a1]);
+        if (wrap_idx(gf.nx, gf.px, fx, gi) && wrap_idx(gf.ny, gf.py, fy, gj)) {
+            int px0, px1, py0, py1;
+            double wx0, wy0;
+            pweights(fx, gf.nx, gf.px, gf.dl, gf.dr, px0, px1, wx0);
+            pweights(fy, gf.ny, gf.py, gf.db, gf.dt, py0, py1, wy0);
+            const double* r0 = sec + (py0 - cj0) * CW;
+            const double* r1 = sec + (py1 - cj0) * CW;
+            const int a0 = px0 - ci0, a1 = px1 - ci0;
+            const double e = wy0 * (wx0 * r0[a0] + 0.25 * r0[a1]) + 0.25 * (wx0 * r1[a0] + 0.25 * r1[a1]);
             v = x2[(size_t)gj * gf.P + gi] + e;
         }
         sx3[k] = v;
     }
     __syncthreads();
     // x4 on halo 1
-    for (int k = tid; k < W1 * H1; k += MG_THREADS) {
-        int li = k % W1, lj = k / W1, gi, gj;
+    for (int k = tid; k < D1H * D1W; k += MG_THREADS) {
+        const int li = k % D1W, lj = k / D1W;
+        int gi, gj;
         double v = 0.0;
-        if (gcoord(gf.nx, gf.px, fi0 + li - 1, gi) && gcoord(gf.ny, gf.py, fj0 + lj - 1, gj)) {
-            Nb n = neighbours(gf, gi, gj);
-            double d = nb_diag(gf, n);
-            double ax = nb_apply(gf, n, sx3, UW, li + 1, lj + 1, d);
-            v = sx3[(lj + 1) * UW + li + 1] + omega * (sbb[k] - ax) / d;
+        if (wrap_idx(gf.nx, gf.px, fi0 + li - 1, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 1, gj)) {
+            const Diag D = mg_diag(gf, gi, gj);
+            const int c = (lj + 1) * D2W + li + 1;
+            v = sx3[c] + D.wd * (sbb[k] - tile_negA(gf, sx3, c, D2W, D.d));
         }
         sx4[k] = v;
     }
     __syncthreads();
-    // x5 on the interior, plus r.z on the finest level
+    // x5 on the tile, plus r.z on the finest level
     double acc = 0.0;
     for (int k = tid; k < FT_X * FT_Y; k += MG_THREADS) {
-        int li = k % FT_X, lj = k / FT_X;
-        int gi = fi0 + li, gj = fj0 + lj;
+        const int li = k % FT_X, lj = k / FT_X;
+        const int gi = fi0 + li, gj = fj0 + lj;
         if (gi >= gf.nx || gj >= gf.ny) continue;
-        Nb n = neighbours(gf, gi, gj);
-        double d = nb_diag(gf, n);
-        double ax = nb_apply(gf, n, sx4, W1, li + 1, lj + 1, d);
-        double bv = sbb[(lj + 1) * W1 + li + 1];
-        double v = sx4[(lj + 1) * W1 + li + 1] + omega * (bv - ax) / d;
+        const Diag D = mg_diag(gf, gi, gj);
+        const int c = (lj + 1) * D1W + li + 1;
+        const double bv = sbb[c];
+        const double v = sx4[c] + D.wd * (bv - tile_negA(gf, sx4, c, D1W, D.d));
         xout[(size_t)gj * gf.P + gi] = v;
         acc += bv * v;
     }
@@ -291,16 +273,338 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
     put_partials<1>(v, rz, nb);
 }
 
+// ---------------------------------------------------------------------------
+// Coarse levels in one workgroup: a top level of at most CO_MAX_CELLS cells,
+// halving while both sizes stay even, down to a bottom of at most
+// CO_DENSE_MAX cells solved with its dense (pseudo-)inverse (otherwise:
+// CO_SWEEPS pairs of damped-Jacobi sweeps).  Level l keeps x, t and b (n_l
+// doubles each) consecutively in LDS.
+// ---------------------------------------------------------------------------
+constexpr int CO_THREADS = 1024;
+constexpr int CO_SWEEPS = 64;
+
+__device__ __forceinline__ MgGeom co_geom(const MgGeom& g0, int l) {
+    MgGeom g = g0;
+    g.nx = g0.nx >> l;
+    g.ny = g0.ny >> l;
+    g.P = g.nx;
+    const double s = 1.0 / (double)(1 << (2 * l));
+    g.ihx2 *= s;
+    g.ihy2 *= s;
+    g.d0 *= s;
+    g.wd0 /= s;
+    return g;
+}
+
+// -A a at (i, j) of a compact level array, with its diagonal
+__device__ __forceinline__ double co_negA(const MgGeom& g, const double* a, int i, int j, double& d) {
+    const int k = j * g.nx + i;
+    double sx = 0.0, sy = 0.0, cx = 0.0, cy = 0.0;
+    if (i > 0) { sx += a[k - 1]; cx += 1.0; }
+    else if (g.px) { sx += a[k + g.nx - 1]; cx += 1.0; }
+    else if (g.dl) cx += 2.0;
+    if (i < g.nx - 1) { sx += a[k + 1]; cx += 1.0; }
+    else if (g.px) { sx += a[k - g.nx + 1]; cx += 1.0; }
+    else if (g.dr) cx += 2.0;
+    if (j > 0) { sy += a[k - g.nx]; cy += 1.0; }
+    else if (g.py) { sy += a[k + (g.ny - 1) * g.nx]; cy += 1.0; }
+    else if (g.db) cy += 2.0;
+    if (j < g.ny - 1) { sy += a[k + g.nx]; cy += 1.0; }
+    else if (g.py) { sy += a[k - (g.ny - 1) * g.nx]; cy += 1.0; }
+    else if (g.dt) cy += 2.0;
+    d = g.ihx2 * cx + g.ihy2 * cy;
+    return d * a[k] - g.ihx2 * sx - g.ihy2 * sy;
+}
+__device__ __forceinline__ double co_wd(const MgGeom& g, double d) { return d == g.d0 ? g.wd0 : g.omega / d; }
+
+// restriction weight of fine f = 2I + dx onto coarse I, with f wrapped
+__device__ __forceinline__ double co_rw(int I, int dx, int nf, int per, int dlo, int dhi, int& f) {
+    f = 2 * I + dx;
+    if (f < 0 || f >= nf) {
+        if (!per) return 0.0;
+        f = f < 0 ? f + nf : f - nf;
+        return 0.25;
+    }
+    if (dx == -1 || dx == 2) return 0.25;
+    if (!per) {
+        if (f == 0) return dlo ? 0.5 : 1.0;
+        if (f == nf - 1) return dhi ? 0.5 : 1.0;
+    }
+    return 0.75;
+}
+
+// prolongation of fine f: parent I0 (w0), nearer neighbour I1 (w1), wrapped
+__device__ __forceinline__ void co_pw(int f, int nf, int per, int dlo, int dhi, int& I0, double& w0, int& I1,
+                                      double& w1) {
+    const int nc = nf >> 1;
+    I0 = f >> 1;
+    I1 = (f & 1) ? I0 + 1 : I0 - 1;
+    w0 = 0.75;
+    w1 = 0.25;
+    if (I1 < 0 || I1 >= nc) {
+        if (per) I1 = I1 < 0 ? I1 + nc : I1 - nc;
+        else {
+            w0 = ((I1 < 0 ? dlo : dhi) ? 0.5 : 1.0);
+            I1 = I0;
+            w1 = 0.0;
+        }
+    }
+}
+
+#define CO_FOR(g) \
+    for (int j = ty; j < (g).ny; j += 32) \
+        for (int i = tx; i < (g).nx; i += 32)
+
+__global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels, const double* __restrict__ dense,
+                                                          int dense_n, const double* __restrict__ bglob,
+                                                          double* __restrict__ tglob, const int* done) {
+    extern __shared__ double s[];
+    if (*done) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    {  // b of the top level and x = w D^-1 b
+        const MgGeom g = co_geom(g0, 0);
+        const int n = g.nx * g.ny;
+        double* x = s;
+        double* bb = s + 2 * n;
+        CO_FOR(g) {
+            const double bv = bglob[(size_t)j * g0.P + i];
+            const Diag D = mg_diag(g, i, j);
+            bb[j * g.nx + i] = bv;
+            x[j * g.nx + i] = D.wd * bv;
+        }
+    }
+    __syncthreads();
+    int off = 0;
+    for (int l = 0; l < levels - 1; ++l) {
+        const MgGeom g = co_geom(g0, l), gc = co_geom(g0, l + 1);
+        const int n = g.nx * g.ny, nc = gc.nx * gc.ny;
+        double* x = s + off;
+        double* t = x + n;
+        double* bb = x + 2 * n;
+        CO_FOR(g) {  // t = x + w D^-1 (b - A x)
+            double d;
+            const double ax = co_negA(g, x, i, j, d);
+            const int k = j * g.nx + i;
+            t[k] = x[k] + co_wd(g, d) * (bb[k] - ax);
+        }
+        __syncthreads();
+        CO_FOR(g) {  // r = b - A t, into x
+            double d;
+            const double at = co_negA(g, t, i, j, d);
+            x[j * g.nx + i] = bb[j * g.nx + i] - at;
+        }
+        __syncthreads();
+        double* xc = x + 3 * n;
+        double* bc = xc + 2 * nc;
+        CO_FOR(gc) {  // b_c = R r, x_c = w D^-1 b_c
+            double acc = 0.0;
+            for (int dy = -1; dy <= 2; ++dy) {
+                int fy;
+                const double wy = co_rw(j, dy, g.ny, g.py, g.db, g.dt, fy);
+                if (wy == 0.0) continue;
+                double row = 0.0;
+                for (int dx = -1; dx <= 2; ++dx) {
+                    int fx;
+                    const double wx = co_rw(i, dx, g.nx, g.px, g.dl, g.dr, fx);
+                    if (wx != 0.0) row += wx * x[fy * g.nx + fx];
+                }
+                acc += wy * row;
+            }
+            const double bv = 0.25 * acc;
+            const Diag D = mg_diag(gc, i, j);
+            bc[j * gc.nx + i] = bv;
+            xc[j * gc.nx + i] = D.wd * bv;
+        }
+        __syncthreads();
+        off += 3 * n;
+    }
+    {  // bottom
+        const MgGeom g = co_geom(g0, levels - 1);
+        const int n = g.nx * g.ny;
+        double* x = s + off;
+        double* t = x + n;
+        double* bb = x + 2 * n;
+        if (dense_n) {  // t = M b, 16 lanes per row
+            const int row = threadIdx.x >> 4, lane = threadIdx.x & 15;
+            double acc = 0.0;
+            if (row < n)
+                for (int c = lane; c < n; c += 16) acc += dense[row * n + c] * bb[c];
+            for (int msk = 8; msk >= 1; msk >>= 1) acc += __shfl_xor(acc, msk, 16);
+            if (row < n && lane == 0) t[row] = acc;
+            __syncthreads();
+        } else {
+            for (int it = 0; it <= 2 * CO_SWEEPS; ++it) {
+                double* in = (it & 1) ? t : x;
+                double* out = (it & 1) ? x : t;
+                CO_FOR(g) {
+                    double d;
+                    const double ax = co_negA(g, in, i, j, d);
+                    const int k = j * g.nx + i;
+                    out[k] = in[k] + co_wd(g, d) * (bb[k] - ax);
+                }
+                __syncthreads();
+            }
+        }
+    }
+    // up: t_l += P t_{l+1}; x = t + w D^-1 (b - A t); t = x + w D^-1 (b - A x)
+    for (int l = levels - 2; l >= 0; --l) {
+        const MgGeom g = co_geom(g0, l), gc = co_geom(g0, l + 1);
+        const int n = g.nx * g.ny;
+        off -= 3 * n;
+        double* x = s + off;
+        double* t = x + n;
+        double* bb = x + 2 * n;
+        const double* tc = x + 3 * n + gc.nx * gc.ny;
+        CO_FOR(g) {
+            int X0, X1, Y0, Y1;
+            double wx0, wx1, wy0, wy1;
+            co_pw(i, g.nx, g.px, g.dl, g.dr, X0, wx0, X1, wx1);
+            co_pw(j, g.ny, g.py, g.db, g.dt, Y0, wy0, Y1, wy1);
+            t[j * g.nx + i] += wy0 * (wx0 * tc[Y0 * gc.nx + X0] + wx1 * tc[Y0 * gc.nx + X1]) +
+                               wy1 * (wx0 * tc[Y1 * gc.nx + X0] + wx1 * tc[Y1 * gc.nx + X1]);
+        }
+        __syncthreads();
+        CO_FOR(g) {
+            double d;
+            const double at = co_negA(g, t, i, j, d);
+            const int k = j * g.nx + i;
+            x[k] = t[k] + co_wd(g, d) * (bb[k] - at);
+        }
+        __syncthreads();
+        CO_FOR(g) {
+            double d;
+            const double ax = co_negA(g, x, i, j, d);
+            const int k = j * g.nx + i;
+            const double v = x[k] + co_wd(g, d) * (bb[k] - ax);
+            if (l == 0) tglob[(size_t)j * g0.P + i] = v;
+            else t[k] = v;
+        }
+        if (l > 0) __syncthreads();
+    }
+    if (levels == 1) {  // the top level is the bottom
+        const int n = g0.nx * g0.ny;
+        const double* t = s + n;
+        CO_FOR(g0) tglob[(size_t)j * g0.P + i] = t[j * g0.nx + i];
+    }
+}
+
+// ---- host side -------------------------------------------------------------
+
+// -A of a level as a dense row-major matrix (the operator of co_negA)
+static std::vector<double> dense_operator(const MgGeom& g) {
+    const int nx = g.nx, ny = g.ny, n = nx * ny;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            const int k = j * nx + i;
+            double* row = &A[(size_t)k * n];
+            double cx = 0.0, cy = 0.0;
+            auto nb = [&](int ii, int jj, double h) { row[jj * nx + ii] -= h; };
+            if (i > 0) { nb(i - 1, j, g.ihx2); cx += 1.0; }
+            else if (g.px) { nb(nx - 1, j, g.ihx2); cx += 1.0; }
+            else if (g.dl) cx += 2.0;
+            if (i < nx - 1) { nb(i + 1, j, g.ihx2); cx += 1.0; }
+            else if (g.px) { nb(0, j, g.ihx2); cx += 1.0; }
+            else if (g.dr) cx += 2.0;
+            if (j > 0) { nb(i, j - 1, g.ihy2); cy += 1.0; }
+            else if (g.py) { nb(i, ny - 1, g.ihy2); cy += 1.0; }
+            else if (g.db) cy += 2.0;
+            if (j < ny - 1) { nb(i, j + 1, g.ihy2); cy += 1.0; }
+            else if (g.py) { nb(i, 0, g.ihy2); cy += 1.0; }
+            else if (g.dt) cy += 2.0;
+            row[k] += g.ihx2 * cx + g.ihy2 * cy;
+        }
+    return A;
+}
+
+// Inverse of -A at the bottom level.  With only Neumann/periodic sides -A is
+// singular with the constants as null space e, and its pseudo-inverse is
+// (A + a e e^T)^-1 - e e^T / (a n^2).  Gauss-Jordan with partial pivoting.
+static bool dense_inverse(const MgGeom& g, std::vector<double>& M) {
+    const int n = g.nx * g.ny;
+    std::vector<double> A = dense_operator(g);
+    const bool singular = (g.px || (!g.dl && !g.dr)) && (g.py || (!g.db && !g.dt));
+    const double a = g.d0 / n;
+    if (singular)
+        for (double& v : A) v += a;
+    M.assign((size_t)n * n, 0.0);
+    for (int k = 0; k < n; ++k) M[(size_t)k * n + k] = 1.0;
+    for (int c = 0; c < n; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(A[(size_t)r * n + c]) > std::fabs(A[(size_t)piv * n + c])) piv = r;
+        if (A[(size_t)piv * n + c] == 0.0) return false;
+        if (piv != c)
+            for (int k = 0; k < n; ++k) {
+                std::swap(A[(size_t)piv * n + k], A[(size_t)c * n + k]);
+                std::swap(M[(size_t)piv * n + k], M[(size_t)c * n + k]);
+            }
+        const double inv = 1.0 / A[(size_t)c * n + c];
+        for (int k = 0; k < n; ++k) {
+            A[(size_t)c * n + k] *= inv;
+            M[(size_t)c * n + k] *= inv;
+        }
+        for (int r = 0; r < n; ++r) {
+            if (r == c) continue;
+            const double f = A[(size_t)r * n + c];
+            if (f == 0.0) continue;
+            for (int k = 0; k < n; ++k) {
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k];
+                M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
+            }
+        }
+    }
+    if (singular) {
+        const double e = 1.0 / (a * (double)n * (double)n);
+        for (double& v : M) v -= e;
+    }
+    return true;
+}
+
+bool coarse_plan(const MgGeom& top, CoarsePlan& cp, std::vector<double>& dense) {
+    cp = CoarsePlan{};
+    dense.clear();
+    if ((long)top.nx * top.ny > CO_MAX_CELLS) return false;
+    int levels = 1, nx = top.nx, ny = top.ny;
+    size_t cells = (size_t)nx * ny;
+    while ((long)nx * ny > CO_DENSE_MAX && !(nx & 1) && !(ny & 1) && nx >= 4 && ny >= 4) {
+        nx >>= 1;
+        ny >>= 1;
+        cells += (size_t)nx * ny;
+        ++levels;
+    }
+    cp.levels = levels;
+    cp.lds_bytes = 3 * cells * sizeof(double);
+    if ((long)nx * ny <= CO_DENSE_MAX) {
+        MgGeom gb = top;
+        gb.nx = nx;
+        gb.ny = ny;
+        const double sc = 1.0 / (double)(1 << (2 * (levels - 1)));
+        gb.ihx2 *= sc;
+        gb.ihy2 *= sc;
+        gb.d0 *= sc;
+        gb.wd0 /= sc;
+        if (dense_inverse(gb, dense)) cp.dense_n = nx * ny;
+        else dense.clear();
+    }
+    return true;
+}
+
+void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
+                      const int* done) {
+    k_mg_coarse<<<1, CO_THREADS, cp.lds_bytes, s>>>(top, cp.levels, cp.dense, cp.dense_n, b, t, done);
+}
+
 void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
-                    double omega, const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot) {
     dim3 g((gf.nx + FT_X - 1) / FT_X, (gf.ny + FT_Y - 1) / FT_Y);
-    k_mg_down<<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, omega, done, st, rr, nb, m, slot);
+    k_mg_down<<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
 }
 
 void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
-                  const double* ec, double* xout, double omega, const int* done, double* rz, int nb) {
+                  const double* ec, double* xout, const int* done, double* rz, int nb) {
     dim3 g((gf.nx + FT_X - 1) / FT_X, (gf.ny + FT_Y - 1) / FT_Y);
-    k_mg_up<<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, omega, done, rz, nb);
+    k_mg_up<<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, done, rz, nb);
 }
 
 }  // namespace fsi

@@ -193,6 +193,7 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* _
                                                     double* __restrict__ Iv) {
     const int i = blockIdx.x * TILE_X + threadIdx.x;
     if (i >= p.nx) return;
+#pragma unroll 1
     for (int r = 0; r < ROWS; ++r) {
         const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
         if (j >= p.ny) return;
@@ -308,7 +309,7 @@ __device__ __forceinline__ double upwind_state(const BcSide& b, double vel, bool
 
 // -div(upwind v f) at cell (i,j) (InterfaceTransport.jl:161-258).
 template <class F>
-__device__ double upwind_rhs(const Params& p, const double* U, const double* V, const BcSet& bc, F f,
+__device__ __forceinline__ double upwind_rhs(const Params& p, const double* U, const double* V, const BcSet& bc, F f,
                              int i, int j, Status* st) {
     auto ax = [&](int k) -> double {
         if (p.px) {
@@ -389,16 +390,68 @@ __device__ __forceinline__ bool sigma_log_product(double a, double d, double b, 
     return true;
 }
 
-__device__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status* st, double& oxx,
-                          double& oyy, double& oxy) {
+struct Sym3 {
+    double xx, yy, xy;
+};
+__device__ __forceinline__ Sym3 load_sym3(const Params& p, const Stage& s, int i, int j) {
+    Sym3 v;
+    load_sigma(p, s, i, j, v.xx, v.yy, v.xy);
+    return v;
+}
+// upwind flux u * state at a face for the three components
+__device__ __forceinline__ Sym3 upwind_flux3(double u, const Sym3& lo, const Sym3& hi) {
+    const Sym3& q = u >= 0.0 ? lo : hi;
+    return {u * q.xx, u * q.yy, u * q.xy};
+}
+__device__ __forceinline__ const BcSide& side_of(const BcSet& b, int side) {
+    return side == 0 ? b.l : side == 1 ? b.r : side == 2 ? b.b : b.t;
+}
+// incoming state at a wall face, per component (upwind_state)
+__device__ __forceinline__ Sym3 wall_state3(const Params& p, int side, double u, bool low_side, const Sym3& c,
+                                            Status* st) {
+    return {upwind_state(side_of(p.bSxx, side), u, low_side, c.xx, st),
+            upwind_state(side_of(p.bSyy, side), u, low_side, c.yy, st),
+            upwind_state(side_of(p.bSxy, side), u, low_side, c.xy, st)};
+}
+
+// Structure-tensor right-hand side at cell (i, j): upwind transport of the
+// three components (InterfaceTransport.jl:161-258 per component, the 5-point
+// stencil loaded once), stretching, relaxation and geometric extension.
+__device__ __forceinline__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status* st, double& oxx,
+                                          double& oyy, double& oxy) {
     const double* U = s.U;
     const double* V = s.V;
-    auto fxx = [&](int a, int b) { double x, y, z; load_sigma(p, s, a, b, x, y, z); return x; };
-    auto fyy = [&](int a, int b) { double x, y, z; load_sigma(p, s, a, b, x, y, z); return y; };
-    auto fxy = [&](int a, int b) { double x, y, z; load_sigma(p, s, a, b, x, y, z); return z; };
-    double axx = upwind_rhs(p, U, V, p.bSxx, fxx, i, j, st);
-    double ayy = upwind_rhs(p, U, V, p.bSyy, fyy, i, j, st);
-    double axy = upwind_rhs(p, U, V, p.bSxy, fxy, i, j, st);
+    const Sym3 C = load_sym3(p, s, i, j);
+    Sym3 fl, fr, fb, ft;
+    {  // x faces i (left) and i+1 (right); periodic face nx is face 0
+        const double ul = U[at(p, i, j)];
+        if (p.px || i > 0) fl = upwind_flux3(ul, load_sym3(p, s, i > 0 ? i - 1 : p.nx - 1, j), C);
+        else fl = upwind_flux3(ul, wall_state3(p, 0, ul, true, C, st), C);
+        if (i < p.nx - 1) {
+            fr = upwind_flux3(U[at(p, i + 1, j)], C, load_sym3(p, s, i + 1, j));
+        } else if (p.px) {
+            fr = upwind_flux3(U[at(p, 0, j)], C, load_sym3(p, s, 0, j));
+        } else {
+            const double ur = U[at(p, p.nx, j)];
+            fr = upwind_flux3(ur, C, wall_state3(p, 1, ur, false, C, st));
+        }
+    }
+    {  // y faces j (bottom) and j+1 (top)
+        const double vb = V[at(p, i, j)];
+        if (p.py || j > 0) fb = upwind_flux3(vb, load_sym3(p, s, i, j > 0 ? j - 1 : p.ny - 1), C);
+        else fb = upwind_flux3(vb, wall_state3(p, 2, vb, true, C, st), C);
+        if (j < p.ny - 1) {
+            ft = upwind_flux3(V[at(p, i, j + 1)], C, load_sym3(p, s, i, j + 1));
+        } else if (p.py) {
+            ft = upwind_flux3(V[at(p, i, 0)], C, load_sym3(p, s, i, 0));
+        } else {
+            const double vt = V[at(p, i, p.ny)];
+            ft = upwind_flux3(vt, C, wall_state3(p, 3, vt, false, C, st));
+        }
+    }
+    const double axx = -((fr.xx - fl.xx) / p.hx + (ft.xx - fb.xx) / p.hy);
+    const double ayy = -((fr.yy - fl.yy) / p.hx + (ft.yy - fb.yy) / p.hy);
+    const double axy = -((fr.xy - fl.xy) / p.hx + (ft.xy - fb.xy) / p.hy);
     double uL = U[at(p, i, j)], uR = U[at(p, i + 1, j)], vB = V[at(p, i, j)], vT = V[at(p, i, j + 1)];
     double dv = (uR - uL) / p.hx + (vT - vB) / p.hy;
     double ux = (uR - uL) / p.hx, vy = (vT - vB) / p.hy;
@@ -425,8 +478,7 @@ __device__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status*
     } else {
         vx = (Vc(i + 1) - Vc(i - 1)) / (p.hx + p.hx);
     }
-    double sa, sd, sb;
-    load_sigma(p, s, i, j, sa, sd, sb);
+    const double sa = C.xx, sd = C.yy, sb = C.xy;
     double A11 = ux - 0.0, A12 = uy - 0.0, A21 = vx - 0.0, A22 = vy - 0.0;
     double dxx = 2.0 * (A11 * sa + A12 * sb);
     double dyy = 2.0 * (A21 * sb + A22 * sd);
@@ -460,7 +512,7 @@ __device__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status*
 // f: phi, fpsi: psi, N0/N1: cell normals (CDI: sharpening vector); all take
 // global (wrapped) indices.
 template <class F, class NX, class NY>
-__device__ double phase_cdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i, int j,
+__device__ __forceinline__ double phase_cdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i, int j,
                               Status* st, int with_flux) {
     double adv = upwind_rhs(p, s.U, s.V, p.bPhi, f, i, j, st);
     if (!with_flux) return adv - 0.0;
@@ -478,7 +530,7 @@ __device__ double phase_cdi_t(const Params& p, const Stage& s, const F& f, const
 }
 
 template <class F, class FP, class NX, class NY>
-__device__ double phase_acdi_t(const Params& p, const Stage& s, const F& f, const FP& psi, const NX& N0,
+__device__ __forceinline__ double phase_acdi_t(const Params& p, const Stage& s, const F& f, const FP& psi, const NX& N0,
                                const NY& N1, int i, int j, double gamma, Status* st) {
     auto fx = [&](int k) -> double {
         if (p.px) {
@@ -519,7 +571,7 @@ __device__ double phase_acdi_t(const Params& p, const Stage& s, const F& f, cons
 }
 
 template <class F, class NX, class NY>
-__device__ double phase_uwcdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i,
+__device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i,
                                 int j, Status* st) {
     auto fx = [&](int k) -> double {
         int kk = (p.px && k == p.nx) ? 0 : k;
@@ -618,18 +670,18 @@ struct TileAcc {  // accessor of an LDS tile by global (wrapped) index
     }
 };
 
-__global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
-                                                 double* __restrict__ Exx, double* __restrict__ Eyy,
-                                                 double* __restrict__ Exy, Status* st, int stage1) {
+// Phase-field right-hand side, one instantiation per phase method.
+template <int PH>
+__global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
+                                                 Status* st) {
+    constexpr bool kPsi = PH == FSI_PHASE_ACDI;
+    constexpr bool kNormals = PH != FSI_PHASE_UPWIND;
     __shared__ double s_phi[CR2 * CW2];
-    __shared__ double s_psi[CR2 * CW2];
-    __shared__ double s_n0[CR1 * CW1];
-    __shared__ double s_n1[CR1 * CW1];
-    __shared__ double lds[16];
-    __shared__ long long lds_n[16];
+    __shared__ double s_psi[kPsi ? CR2 * CW2 : 1];
+    __shared__ double s_n0[kNormals ? CR1 * CW1 : 1];
+    __shared__ double s_n1[kNormals ? CR1 * CW1 : 1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
-    const int phase = p.phase;
     // 1. phi (and psi) on the halo-2 tile
     for (int k = tid; k < CR2 * CW2; k += BLOCK) {
         int li = k % CW2, lj = k / CW2;
@@ -645,13 +697,13 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, 
         }
         double v = in ? s.phi[at(p, gi, gj)] : 0.0;
         s_phi[k] = v;
-        if (phase == FSI_PHASE_ACDI) s_psi[k] = in ? psi_value(p, v) : 0.0;
+        if (kPsi) s_psi[k] = in ? psi_value(p, v) : 0.0;
     }
     __syncthreads();
     const TileAcc f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
     const TileAcc fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
     // 2. cell-centred normals / sharpening vector on the halo-1 tile
-    if (phase != FSI_PHASE_UPWIND) {
+    if (kNormals) {
         for (int k = tid; k < CR1 * CW1; k += BLOCK) {
             int li = k % CW1, lj = k / CW1;
             int gi = i0 + li - 1, gj = j0 + lj - 1;
@@ -667,12 +719,12 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, 
             double n0 = 0.0, n1 = 0.0;
             if (in) {
                 double gx, gy;
-                if (phase == FSI_PHASE_ACDI) {  // _center_normals of psi (:233-244)
+                if (PH == FSI_PHASE_ACDI) {  // _center_normals of psi (:233-244)
                     grad_center(p, bpsi, fpsi, gi, gj, gx, gy);
                     double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
                     n0 = gx / mag;
                     n1 = gy / mag;
-                } else if (phase == FSI_PHASE_UWCDI) {  // _center_normals of phi
+                } else if (PH == FSI_PHASE_UWCDI) {  // _center_normals of phi
                     grad_center(p, p.bPhi, f, gi, gj, gx, gy);
                     double mag = sqrt(gx * gx + gy * gy + p.uw_d2);
                     n0 = gx / mag;
@@ -693,24 +745,40 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, 
     }
     const TileAcc N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
     const TileAcc N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    double gamma = p.acdi_gamma;
+    if (PH == FSI_PHASE_ACDI && p.acdi_dyn) gamma = p.acdi_gamma * st->gamma_umax;
+    const int i = i0 + threadIdx.x;
+    if (i >= p.nx) return;
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = j0 + threadIdx.y + r * TILE_Y;
+        if (j >= p.ny) break;
+        double e;
+        if (PH == FSI_PHASE_ACDI) e = phase_acdi_t(p, s, f, fpsi, N0, N1, i, j, gamma, st);
+        else if (PH == FSI_PHASE_UWCDI) e = phase_uwcdi_t(p, s, f, N0, N1, i, j, st);
+        else e = phase_cdi_t(p, s, f, N0, N1, i, j, st, PH == FSI_PHASE_CDI);
+        Ephi[at(p, i, j)] = e;
+    }
+}
 
+// Structure-tensor right-hand side, plus on stage 1 the divergence
+// L-infinity and the finiteness check of the stage-1 state.
+__global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
+                                                 double* __restrict__ Eyy, double* __restrict__ Exy, Status* st,
+                                                 int stage1) {
+    __shared__ double lds[16];
+    __shared__ long long lds_n[16];
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
     double dmax = 0.0;
     long long nonfinite = 0;
-    double gamma = p.acdi_gamma;
-    if (phase == FSI_PHASE_ACDI && p.acdi_dyn) gamma = p.acdi_gamma * st->gamma_umax;
-    const int i = i0 + threadIdx.x;
+    const int i = blockIdx.x * TILE_X + threadIdx.x;
     if (i < p.nx)
+#pragma unroll 1
         for (int r = 0; r < ROWS; ++r) {
-            const int j = j0 + threadIdx.y + r * TILE_Y;
+            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
-            double e;
-            if (phase == FSI_PHASE_ACDI) e = phase_acdi_t(p, s, f, fpsi, N0, N1, i, j, gamma, st);
-            else if (phase == FSI_PHASE_UWCDI) e = phase_uwcdi_t(p, s, f, N0, N1, i, j, st);
-            else e = phase_cdi_t(p, s, f, N0, N1, i, j, st, phase == FSI_PHASE_CDI);
             double oxx, oyy, oxy;
             sigma_rhs(p, s, i, j, st, oxx, oyy, oxy);
             const size_t k = at(p, i, j);
-            Ephi[k] = e;
             Exx[k] = oxx;
             Eyy[k] = oyy;
             Exy[k] = oxy;
@@ -726,24 +794,23 @@ __global__ void __launch_bounds__(BLOCK) k_cells(Params p, Stage s, BcSet bpsi, 
                 if (!isfinite(dv)) dmax = INFINITY;
             }
         }
-    if (stage1) {
-        dmax = wave_max(dmax);
-        for (int off = 32; off > 0; off >>= 1) nonfinite += __shfl_down(nonfinite, off, 64);
-        if ((tid & 63) == 0) {
-            lds[tid >> 6] = dmax;
-            lds_n[tid >> 6] = nonfinite;
+    if (!stage1) return;
+    dmax = wave_max(dmax);
+    for (int off = 32; off > 0; off >>= 1) nonfinite += __shfl_down(nonfinite, off, 64);
+    if ((tid & 63) == 0) {
+        lds[tid >> 6] = dmax;
+        lds_n[tid >> 6] = nonfinite;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double mm = lds[0];
+        long long nn = lds_n[0];
+        for (int w = 1; w < BLOCK / 64; ++w) {
+            mm = fmax(mm, lds[w]);
+            nn += lds_n[w];
         }
-        __syncthreads();
-        if (tid == 0) {
-            double mm = lds[0];
-            long long nn = lds_n[0];
-            for (int w = 1; w < BLOCK / 64; ++w) {
-                mm = fmax(mm, lds[w]);
-                nn += lds_n[w];
-            }
-            atomic_max_double(&st->stage1_linf, mm);
-            if (nn) atomicAdd((unsigned long long*)&st->nonfinite_in, (unsigned long long)nn);
-        }
+        atomic_max_double(&st->stage1_linf, mm);
+        if (nn) atomicAdd((unsigned long long*)&st->nonfinite_in, (unsigned long long)nn);
     }
 }
 
@@ -913,7 +980,14 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
         k_zero_scalar<<<1, 1, 0, s>>>(&st->gamma_umax);
         k_max_speed<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, st);
     }
-    k_cells<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, sg, bpsi, Ephi, Exx, Eyy, Exy, st, stage1);
+    const dim3 g = grid2d(p), blk(TILE_X, TILE_Y);
+    switch (p.phase) {
+        case FSI_PHASE_ACDI: k_phase<FSI_PHASE_ACDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+        case FSI_PHASE_UWCDI: k_phase<FSI_PHASE_UWCDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+        case FSI_PHASE_CDI: k_phase<FSI_PHASE_CDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+        default: k_phase<FSI_PHASE_UPWIND><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+    }
+    k_sigma<<<g, blk, 0, s>>>(p, sg, Exx, Eyy, Exy, st, stage1);
 }
 
 void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, const double* sxx_n,

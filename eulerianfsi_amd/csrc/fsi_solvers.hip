@@ -359,163 +359,6 @@ __global__ void __launch_bounds__(BLOCK) k_rz(Params p, ProjBufs b, double* rz, 
     put_partials<1>(acc, rz, nb);
 }
 
-// ---- coarsest levels: one workgroup, the whole remaining V-cycle in LDS
-constexpr int CO_THREADS = 1024;
-constexpr int CO_MAX_CELLS = 1024;
-constexpr int CO_MAX_LEVELS = 12;
-constexpr int CO_BOTTOM_SWEEPS = 64;
-
-struct CoLevel {
-    MgGeom g;
-    int off;  // offset of this level's arrays (x, t, b, r consecutive, each n doubles)
-};
-
-__device__ void co_jacobi0(const CoLevel& L, double* s) {
-    const int n = L.g.nx * L.g.ny;
-    double* x = s + L.off;
-    const double* b = s + L.off + 2 * n;
-    for (int k = threadIdx.x; k < n; k += CO_THREADS) {
-        int i = k % L.g.nx, j = k / L.g.nx;
-        double d;
-        neg_lap(L.g, GAcc{b, L.g.P}, i, j, &d);
-        x[k] = 0.8 * b[k] / d;
-    }
-    __syncthreads();
-}
-__device__ void co_jacobi(const CoLevel& L, double* s, int from_x) {
-    const int n = L.g.nx * L.g.ny;
-    double* xin = s + L.off + (from_x ? 0 : n);
-    double* xout = s + L.off + (from_x ? n : 0);
-    const double* b = s + L.off + 2 * n;
-    for (int k = threadIdx.x; k < n; k += CO_THREADS) {
-        int i = k % L.g.nx, j = k / L.g.nx;
-        double d;
-        double ax = neg_lap(L.g, GAcc{xin, L.g.P}, i, j, &d);
-        xout[k] = xin[k] + 0.8 * (b[k] - ax) / d;
-    }
-    __syncthreads();
-}
-
-__global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, const double* bglob, double* tglob,
-                                                          const int* done) {
-    extern __shared__ double s[];
-    if (*done) return;
-    CoLevel L[CO_MAX_LEVELS];
-    int nl = 0, off = 0;
-    MgGeom g = g0;
-    g.P = g0.nx;
-    for (;;) {
-        L[nl].g = g;
-        L[nl].off = off;
-        off += 4 * g.nx * g.ny;
-        ++nl;
-        if (nl == CO_MAX_LEVELS || (g.nx & 1) || (g.ny & 1) || g.nx < 4 || g.ny < 4) break;
-        g.nx >>= 1;
-        g.ny >>= 1;
-        g.P = g.nx;
-        g.ihx2 *= 0.25;
-        g.ihy2 *= 0.25;
-    }
-    // load b of the top coarse level
-    {
-        const int n = g0.nx * g0.ny;
-        for (int k = threadIdx.x; k < n; k += CO_THREADS) {
-            int i = k % g0.nx, j = k / g0.nx;
-            s[L[0].off + 2 * n + k] = bglob[(size_t)j * g0.P + i];
-        }
-        __syncthreads();
-    }
-    // down
-    for (int l = 0; l < nl - 1; ++l) {
-        const MgGeom& gf = L[l].g;
-        const MgGeom& gc = L[l + 1].g;
-        const int nf = gf.nx * gf.ny, nc = gc.nx * gc.ny;
-        co_jacobi0(L[l], s);
-        co_jacobi(L[l], s, 1);  // result in t
-        double* t = s + L[l].off + nf;
-        double* bf = s + L[l].off + 2 * nf;
-        double* r = s + L[l].off + 3 * nf;
-        for (int k = threadIdx.x; k < nf; k += CO_THREADS) {
-            int i = k % gf.nx, j = k / gf.nx;
-            r[k] = bf[k] - neg_lap(gf, GAcc{t, gf.P}, i, j, nullptr);
-        }
-        __syncthreads();
-        double* bc = s + L[l + 1].off + 2 * nc;
-        for (int k = threadIdx.x; k < nc; k += CO_THREADS) {
-            int I = k % gc.nx, J = k / gc.nx;
-            double acc = 0.0;
-            for (int dy = -1; dy <= 2; ++dy) {
-                int fy = 2 * J + dy;
-                if (fy < 0 || fy >= gf.ny) {
-                    if (!gf.py) continue;
-                    fy = (fy + gf.ny) % gf.ny;
-                }
-                double wy = pweight(fy, J, gf.ny, gf.py, gf.db, gf.dt);
-                for (int dx = -1; dx <= 2; ++dx) {
-                    int fx = 2 * I + dx;
-                    if (fx < 0 || fx >= gf.nx) {
-                        if (!gf.px) continue;
-                        fx = (fx + gf.nx) % gf.nx;
-                    }
-                    double wx = pweight(fx, I, gf.nx, gf.px, gf.dl, gf.dr);
-                    acc += 0.25 * wx * wy * r[(size_t)fy * gf.P + fx];
-                }
-            }
-            bc[k] = acc;
-        }
-        __syncthreads();
-    }
-    // bottom: fixed number of damped-Jacobi sweeps from zero (result in t)
-    {
-        const CoLevel& B = L[nl - 1];
-        co_jacobi0(B, s);
-        for (int it = 0; it < CO_BOTTOM_SWEEPS; ++it) {
-            co_jacobi(B, s, 1);
-            co_jacobi(B, s, 0);
-        }
-        co_jacobi(B, s, 1);
-    }
-    // up: t_l += P t_{l+1}; two sweeps t -> x -> t
-    for (int l = nl - 2; l >= 0; --l) {
-        const MgGeom& gf = L[l].g;
-        const MgGeom& gc = L[l + 1].g;
-        const int nf = gf.nx * gf.ny, nc = gc.nx * gc.ny;
-        double* t = s + L[l].off + nf;
-        const double* tc = s + L[l + 1].off + nc;
-        for (int k = threadIdx.x; k < nf; k += CO_THREADS) {
-            int i = k % gf.nx, j = k / gf.nx;
-            int X0, X1, Y0, Y1;
-            double wx0, wx1, wy0, wy1;
-            prol_1d(i, gf.nx, gf.px, gf.dl, gf.dr, X0, wx0, X1, wx1);
-            prol_1d(j, gf.ny, gf.py, gf.db, gf.dt, Y0, wy0, Y1, wy1);
-            t[k] += wy0 * (wx0 * tc[Y0 * gc.nx + X0] + wx1 * tc[Y0 * gc.nx + X1]) +
-                    wy1 * (wx0 * tc[Y1 * gc.nx + X0] + wx1 * tc[Y1 * gc.nx + X1]);
-        }
-        __syncthreads();
-        co_jacobi(L[l], s, 0);  // t -> x
-        co_jacobi(L[l], s, 1);  // x -> t
-    }
-    {
-        const int n = g0.nx * g0.ny;
-        const double* t = s + L[0].off + n;
-        for (int k = threadIdx.x; k < n; k += CO_THREADS) {
-            int i = k % g0.nx, j = k / g0.nx;
-            tglob[(size_t)j * g0.P + i] = t[k];
-        }
-    }
-}
-
-static size_t coarse_lds_bytes(MgGeom g) {
-    size_t total = 0;
-    for (int l = 0; l < CO_MAX_LEVELS; ++l) {
-        total += 4 * (size_t)g.nx * g.ny;
-        if (l + 1 == CO_MAX_LEVELS || (g.nx & 1) || (g.ny & 1) || g.nx < 4 || g.ny < 4) break;
-        g.nx >>= 1;
-        g.ny >>= 1;
-    }
-    return total * sizeof(double);
-}
-
 // p_m = z + beta p_{m-1} (beta = rz_m / rz_{m-1}), w = (-A) p_m, partial p.w
 __global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int m) {
     if (st->pcg.done) return;
@@ -701,28 +544,27 @@ void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const Pro
     k_pcg_init<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, grid_blocks(p), has_null, rtol, maxiter);
 }
 
-static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const int* done) {
-    const double omega = 0.8;
+static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp,
+                         const int* done) {
     const int l = nlev - 1;
     const MgGeom& g = geo[l];
-    if ((long)g.nx * g.ny <= CO_MAX_CELLS) {
-        k_mg_coarse<<<1, CO_THREADS, coarse_lds_bytes(g), s>>>(g, lv[l].b, lv[l].t, done);
+    if (cp.levels > 0) {
+        launch_mg_coarse(s, g, cp, lv[l].b, lv[l].t, done);
     } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
         dim3 gr = grid_of(g.nx, g.ny);
-        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
+        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, g.omega, done);
         const int sweeps = 2 * (g.nx > g.ny ? g.nx : g.ny);
         for (int it = 0; it < sweeps; ++it) {
-            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
-            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
+            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, g.omega, done);
+            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, g.omega, done);
         }
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, g.omega, done);
     }
 }
 
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
-                      int nlev, Status* st, int m0, int count, int slot) {
+                      int nlev, const CoarsePlan& cp, Status* st, int m0, int count, int slot) {
     const int nb = grid_blocks(p);
-    const double omega = 0.8;
     const int* done = &st->pcg.done;
     double* rr = b.partials + (size_t)PS_RR * 4 * nb;
     double* rz = b.partials + (size_t)PS_RZ * 4 * nb;
@@ -730,15 +572,15 @@ void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const M
         const int m = m0 + c;
         if (nlev > 1) {
             for (int l = 0; l < nlev - 1; ++l)
-                launch_mg_down(s, geo[l], geo[l + 1], lv[l].b, lv[l].x, lv[l + 1].b, omega, done,
-                               l == 0 ? st : nullptr, rr, nb, m, slot);
-            coarse_solve(s, lv, geo, nlev, done);
+                launch_mg_down(s, geo[l], geo[l + 1], lv[l].b, lv[l].x, lv[l + 1].b, done, l == 0 ? st : nullptr,
+                               rr, nb, m, slot);
+            coarse_solve(s, lv, geo, nlev, cp, done);
             for (int l = nlev - 2; l >= 0; --l)
-                launch_mg_up(s, geo[l], geo[l + 1], lv[l].x, lv[l].b, lv[l + 1].t, lv[l].t, omega, done,
+                launch_mg_up(s, geo[l], geo[l + 1], lv[l].x, lv[l].b, lv[l + 1].t, lv[l].t, done,
                              l == 0 ? rz : nullptr, nb);
         } else {
             launch_pcg_check(s, st, rr, nb, m, slot);
-            coarse_solve(s, lv, geo, nlev, done);
+            coarse_solve(s, lv, geo, nlev, cp, done);
             launch_rz(s, p, b, rz, nb, done);
         }
         k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], b, st, nb, m);
@@ -759,7 +601,7 @@ void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const 
 // timing.  Returns the algorithmic HBM bytes of one launch (fields read +
 // written, 8 bytes per cell), or a negative value for an unknown name.
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
-                    const MgLevel* lv, const MgGeom* geo, Status* st) {
+                    const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st) {
     const int nb = grid_blocks(p);
     const double cell = 8.0 * (double)p.nx * (double)p.ny;
     const std::string n(name);
@@ -779,13 +621,22 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb, 2, 0);
         return 6 * cell;  // x, p, r, w read; x, r written
     }
-    if (n == "mg_down0" && geo[0].nx != geo[1].nx) {
-        launch_mg_down(s, geo[0], geo[1], lv[0].b, lv[0].x, lv[1].b, 0.8, &st->pcg.done, nullptr, nullptr, nb, 2, 0);
+    if (n == "mg_down0" && nlev > 1) {
+        launch_mg_down(s, geo[0], geo[1], lv[0].b, lv[0].x, lv[1].b, &st->pcg.done, nullptr, nullptr, nb, 2, 0);
         return 2.25 * cell;  // b read; x2 and b_c written
     }
-    if (n == "mg_up0" && geo[0].nx != geo[1].nx) {
-        launch_mg_up(s, geo[0], geo[1], lv[0].x, lv[0].b, lv[1].t, lv[0].t, 0.8, &st->pcg.done, nullptr, nb);
+    if (n == "mg_up0" && nlev > 1) {
+        launch_mg_up(s, geo[0], geo[1], lv[0].x, lv[0].b, lv[1].t, lv[0].t, &st->pcg.done, nullptr, nb);
         return 3.25 * cell;  // x2, b, e_c read; x5 written
+    }
+    if (n == "mg_coarse" && cp.levels > 0) {
+        const MgGeom& g = geo[nlev - 1];
+        launch_mg_coarse(s, g, cp, lv[nlev - 1].b, lv[nlev - 1].t, &st->pcg.done);
+        return 16.0 * g.nx * g.ny;  // b read, t written
+    }
+    if (n == "pcg_iter") {  // one whole PCG iteration: V-cycle, direction, update
+        launch_pcg_iters(s, p, pb, lv, geo, nlev, cp, st, 2, 1, -1);
+        return 0.0;
     }
     if (n == "mg_jacobi") {
         k_mg_jacobi<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], lv[0].x, lv[0].b, lv[0].t, 0.8,
