@@ -26,6 +26,12 @@ constexpr int FT_X = 64;  // fine tile
 constexpr int FT_Y = 16;
 constexpr int MG_THREADS = 256;
 
+// Loop over the N points of a tile phase with a compile-time trip count, so
+// the unrolled iterations issue their global loads back to back.
+#define TILE_LOOP(N)                                                               \
+    _Pragma("unroll") for (int it_ = 0; it_ < ((N) + MG_THREADS - 1) / MG_THREADS; \
+                           ++it_) if (const int k = tid + it_ * MG_THREADS; k < (N))
+
 // diagonal of -A at a cell and w / diagonal
 struct Diag {
     double d, wd;
@@ -59,6 +65,37 @@ __device__ __forceinline__ bool wrap_idx(int n, int per, int k, int& out) {
     if (out < 0 || out >= n) out = ((k % n) + n) % n;
     return true;
 }
+
+// Global loads of the N points of a tile (row length W, origin (x0, y0) in
+// unwrapped coordinates) into registers: indices first, then every load back
+// to back, so a thread has all of its loads in flight at once.  Points
+// outside a non-periodic wall read as 0; val[it] belongs to tile point
+// tid + it * MG_THREADS (clamped to the last point).
+template <int N, int W>
+struct TileGather {
+    static constexpr int NIT = (N + MG_THREADS - 1) / MG_THREADS;
+    double val[NIT];
+    int gi[NIT], gj[NIT];
+    bool in[NIT];
+    __device__ __forceinline__ void load(const double* __restrict__ a, int P, int nx, int ny, int px, int py,
+                                         int x0, int y0, int tid) {
+        int idx[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int k = min(tid + it * MG_THREADS, N - 1);
+            int i = 0, j = 0;
+            const bool ok = wrap_idx(nx, px, x0 + k % W, i) & wrap_idx(ny, py, y0 + k / W, j);
+            in[it] = ok;
+            gi[it] = i;
+            gj[it] = j;
+            idx[it] = ok ? j * P + i : 0;
+        }
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) val[it] = a[idx[it]];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) val[it] = in[it] ? val[it] : 0.0;
+    }
+};
 
 // -A x at tile position c of an array with row length W (zero outside walls)
 __device__ __forceinline__ double tile_negA(const MgGeom& g, const double* a, int c, int W, double d) {
@@ -109,20 +146,20 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
     const int tid = threadIdx.x;
     const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
     // b and x1 = w D^-1 b on halo 3
-    for (int k = tid; k < D3H * D3W; k += MG_THREADS) {
-        const int li = k % D3W, lj = k / D3W;
-        int gi, gj;
-        double bv = 0.0, xv = 0.0;
-        if (wrap_idx(gf.nx, gf.px, fi0 + li - 3, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 3, gj)) {
-            bv = b[(size_t)gj * gf.P + gi];
-            xv = mg_diag(gf, gi, gj).wd * bv;
+    {
+        TileGather<D3H * D3W, D3W> tb;
+        tb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 3, fj0 - 3, tid);
+#pragma unroll
+        for (int it = 0; it < tb.NIT; ++it) {
+            const int k = tid + it * MG_THREADS;
+            if (k >= D3H * D3W) break;
+            sb[k] = tb.val[it];
+            sx[k] = tb.in[it] ? mg_diag(gf, tb.gi[it], tb.gj[it]).wd * tb.val[it] : 0.0;
         }
-        sb[k] = bv;
-        sx[k] = xv;
     }
     __syncthreads();
     // x2 = x1 + w D^-1 (b - A x1) on halo 2
-    for (int k = tid; k < D2H * D2W; k += MG_THREADS) {
+    TILE_LOOP(D2H * D2W) {
         const int li = k % D2W, lj = k / D2W;
         int gi, gj;
         double v = 0.0;
@@ -135,7 +172,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
     }
     __syncthreads();
     // r = b - A x2 on halo 1 (into sx), x2 stored on the tile
-    for (int k = tid; k < D1H * D1W; k += MG_THREADS) {
+    TILE_LOOP(D1H * D1W) {
         const int li = k % D1W, lj = k / D1W;
         const int ui = fi0 + li - 1, uj = fj0 + lj - 1;
         int gi, gj;
@@ -152,7 +189,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
     __syncthreads();
     // restriction, x then y: row sums of r over fine 2I-1 .. 2I+2
     double* sh = s2;
-    for (int k = tid; k < D1H * CTX; k += MG_THREADS) {
+    TILE_LOOP(D1H * CTX) {
         const int ci = k % CTX, lj = k / CTX;
         double w0, w1;
         rweights(fi0 / 2 + ci, gf.nx, gf.px, gf.dl, gf.dr, w0, w1);
@@ -160,7 +197,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
         sh[lj * CTX + ci] = 0.25 * row[0] + w0 * row[1] + w1 * row[2] + 0.25 * row[3];
     }
     __syncthreads();
-    for (int k = tid; k < CTY * CTX; k += MG_THREADS) {
+    TILE_LOOP(CTY * CTX) {
         const int ci = k % CTX, cj = k / CTX;
         const int I = fi0 / 2 + ci, J = fj0 / 2 + cj;
         if (I >= gc.nx || J >= gc.ny) continue;
@@ -205,30 +242,32 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
     const int tid = threadIdx.x;
     const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
-    for (int k = tid; k < CH * CW; k += MG_THREADS) {
-        const int li = k % CW, lj = k / CW;
-        int gi, gj;
-        double v = 0.0;
-        if (wrap_idx(gc.nx, gc.px, ci0 + li, gi) && wrap_idx(gc.ny, gc.py, cj0 + lj, gj))
-            v = ec[(size_t)gj * gc.P + gi];
-        sec[k] = v;
+    TileGather<CH * CW, CW> tc;
+    TileGather<D1H * D1W, D1W> tbb;
+    TileGather<D2H * D2W, D2W> tx2;
+    tc.load(ec, gc.P, gc.nx, gc.ny, gc.px, gc.py, ci0, cj0, tid);
+    tbb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 1, fj0 - 1, tid);
+    tx2.load(x2, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 2, fj0 - 2, tid);
+#pragma unroll
+    for (int it = 0; it < tc.NIT; ++it) {
+        const int k = tid + it * MG_THREADS;
+        if (k < CH * CW) sec[k] = tc.val[it];
     }
-    for (int k = tid; k < D1H * D1W; k += MG_THREADS) {
-        const int li = k % D1W, lj = k / D1W;
-        int gi, gj;
-        double v = 0.0;
-        if (wrap_idx(gf.nx, gf.px, fi0 + li - 1, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 1, gj))
-            v = b[(size_t)gj * gf.P + gi];
-        sbb[k] = v;
+#pragma unroll
+    for (int it = 0; it < tbb.NIT; ++it) {
+        const int k = tid + it * MG_THREADS;
+        if (k < D1H * D1W) sbb[k] = tbb.val[it];
     }
     __syncthreads();
     // x3 = x2 + P e_c on halo 2
-    for (int k = tid; k < D2H * D2W; k += MG_THREADS) {
+#pragma unroll
+    for (int it = 0; it < tx2.NIT; ++it) {
+        const int k = tid + it * MG_THREADS;
+        if (k >= D2H * D2W) break;
         const int li = k % D2W, lj = k / D2W;
         const int fx = fi0 + li - 2, fy = fj0 + lj - 2;
-        int gi, gj;
         double v = 0.0;
-        if (wrap_idx(gf.nx, gf.px, fx, gi) && wrap_idx(gf.ny, gf.py, fy, gj)) {
+        if (tx2.in[it]) {
             int px0, px1, py0, py1;
             double wx0, wy0;
             pweights(fx, gf.nx, gf.px, gf.dl, gf.dr, px0, px1, wx0);
@@ -237,13 +276,13 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
             const double* r1 = sec + (py1 - cj0) * CW;
             const int a0 = px0 - ci0, a1 = px1 - ci0;
             const double e = wy0 * (wx0 * r0[a0] + 0.25 * r0[a1]) + 0.25 * (wx0 * r1[a0] + 0.25 * r1[a1]);
-            v = x2[(size_t)gj * gf.P + gi] + e;
+            v = tx2.val[it] + e;
         }
         sx3[k] = v;
     }
     __syncthreads();
     // x4 on halo 1
-    for (int k = tid; k < D1H * D1W; k += MG_THREADS) {
+    TILE_LOOP(D1H * D1W) {
         const int li = k % D1W, lj = k / D1W;
         int gi, gj;
         double v = 0.0;
@@ -257,7 +296,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
     __syncthreads();
     // x5 on the tile, plus r.z on the finest level
     double acc = 0.0;
-    for (int k = tid; k < FT_X * FT_Y; k += MG_THREADS) {
+    TILE_LOOP(FT_X * FT_Y) {
         const int li = k % FT_X, lj = k / FT_X;
         const int gi = fi0 + li, gj = fj0 + lj;
         if (gi >= gf.nx || gj >= gf.ny) continue;

@@ -13,11 +13,11 @@
 //     reference uses GMRES(50) + Ruge-Stueben AMG; A q = b has a unique
 //     zero-mean solution, so the projected velocity is the same.
 //
-// All dot products are deterministic: wave64 shuffles, LDS in wave order, a
-// per-block partial, and the last-arriving block summing the partials in a
-// fixed order and updating the Krylov scalars in device memory.  Every kernel
-// of an iteration first checks the solver's `done` flag, so the host can
-// enqueue iterations ahead without reading the residual back.
+// All dot products are deterministic: wave64 shuffles, LDS in wave order and
+// one partial per block; every block of the consuming kernel sums all the
+// partials in the same fixed order, so no kernel synchronises across blocks.
+// Every kernel of an iteration first checks the solver's `done` flag, so the
+// host can enqueue iterations ahead without reading the residual back.
 #include "fsi_kernels.hpp"
 #include "fsi_stencils.hpp"
 
@@ -46,13 +46,13 @@ __device__ __forceinline__ double* pset(double* base, int nb, int set) { return 
 // ===========================================================================
 // Velocity predictor: single-reduction CG, one kernel per iteration
 // ===========================================================================
-// Kernel m forms r_{m+1} = r_m - a_m q_m and p_{m+1} = r_{m+1} + b_{m+1} p_m on
-// the fly (own point and stencil neighbours), q_{m+1} = (I - alpha M) p_{m+1},
+// Kernel m forms r_{m+1} = r_m - a_m q_m and p_{m+1} = r_{m+1} + b_{m+1} p_m
+// (once per point of a halo-1 LDS tile), q_{m+1} = (I - alpha M) p_{m+1},
 // x_{m+1} = x_m + a_m p_m, and the block partials of p.q, r.q, q.q, r.r.  a_m
-// and b_{m+1} come from the partials of kernel m-1 (every block sums them in
-// the same order): a_m = rr_m / pq_m, b_{m+1} = rr_{m+1} / rr_m with
-// rr_{m+1} = rr_m - 2 a_m r.q + a_m^2 q.q.  Convergence is tested on the true
-// rr_m.  r, p, q are ping-ponged by m; x is updated in place.
+// and b_{m+1} come from the partials of kernel m-1: a_m = rr_m / pq_m,
+// b_{m+1} = rr_{m+1} / rr_m with rr_{m+1} = rr_m - 2 a_m r.q + a_m^2 q.q.
+// Convergence is tested on rr_m.  r, p, q are ping-ponged by m; x is updated
+// in place.
 
 // x = warm start uv_n with Dirichlet values, r = rhs - A x (masked), rr, |b_F|^2
 __global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double alpha, int nb) {
@@ -118,6 +118,27 @@ __global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double
     put_partials<4>(acc, pset(b.partials, nb, PS_VEL0), nb);
 }
 
+// LDS tile of a velocity component on halo 1 around the 64 x 16 cells of a
+// block, read by global (wrapped) index.
+constexpr int VT_W = TILE_X + 2, VT_H = TILE_ROWS + 2;  // 66 x 18
+struct VelTile {
+    const double* a;
+    int i0, j0, nx, ny;
+    __device__ __forceinline__ double operator()(int gi, int gj) const {
+        int li = gi - i0 + 1, lj = gj - j0 + 1;
+        if (li < 0) li += nx;
+        else if (li >= VT_W) li -= nx;
+        if (lj < 0) lj += ny;
+        else if (lj >= VT_H) lj -= ny;
+        return a[lj * VT_W + li];
+    }
+};
+
+// One CG iteration m >= 1 with a single reduction phase: the totals of
+// iteration m-1 give a = rr/pq and, from rr' = rr - 2a rq + a^2 qq,
+// beta = rr'/rr; then r' = r - a q, p' = r' + beta p, x += a p, q' = A p'
+// with the partials of p'.q', r'.q', q'.q', r'.r' for iteration m+1.  p' is
+// formed once per point on a halo-1 LDS tile and A is applied from there.
 __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
                                                     int slot) {
     if (st->vel.done) return;
@@ -141,33 +162,98 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     const double beta = fmax(rr - 2.0 * a * rq + a * a * qq, 0.0) / rr;
     const int o = m & 1, n = (m + 1) & 1;
     const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
-    auto pnu = [&](int c0, int c1) {
-        size_t k = at(p, c0, c1);
-        return (ru[k] - a * qu0[k]) + beta * pu0[k];
-    };
-    auto pnv = [&](int c0, int c1) {
-        size_t k = at(p, c0, c1);
-        return (rv[k] - a * qv0[k]) + beta * pv0[k];
-    };
+    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];             // p' on halo 1
+    __shared__ double sru[TILE_ROWS * TILE_X], srv[TILE_ROWS * TILE_X];  // r' on the cells
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
+    constexpr int NPT = VT_H * VT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
+    // indices first, then all loads back to back (points outside a
+    // non-periodic wall read index 0 and are discarded)
+    size_t idx[NIT];
+    int flag[NIT];  // 1: inside the domain, 2: a cell of this block
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = min(tid + it * BLOCK, NPT - 1);
+        const int li = k % VT_W, lj = k / VT_W;
+        const int ui = i0 + li - 1, uj = j0 + lj - 1;
+        int gi = ui, gj = uj;
+        bool in = true;
+        if (gi < 0 || gi >= p.nx) {
+            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+            else in = false;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+            else in = false;
+        }
+        const bool centre = in && li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
+        idx[it] = in ? at(p, gi, gj) : 0;
+        flag[it] = (in ? 1 : 0) | (centre ? 2 : 0);
+    }
+    double vru[NIT], vrv[NIT], vqu[NIT], vqv[NIT], vpu[NIT], vpv[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        vru[it] = ru[idx[it]];
+        vrv[it] = rv[idx[it]];
+        vqu[it] = qu0[idx[it]];
+        vqv[it] = qv0[idx[it]];
+        vpu[it] = pu0[idx[it]];
+        vpv[it] = pv0[idx[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k >= NPT) break;
+        const size_t kk = idx[it];
+        const double r1u = vru[it] - a * vqu[it], r1v = vrv[it] - a * vqv[it];
+        const double p1u = r1u + beta * vpu[it], p1v = r1v + beta * vpv[it];
+        if (flag[it] & 2) {
+            const int li = k % VT_W, lj = k / VT_W;
+            b.r_u[n][kk] = r1u;
+            b.r_v[n][kk] = r1v;
+            b.p_u[n][kk] = p1u;
+            b.p_v[n][kk] = p1v;
+            sru[(lj - 1) * TILE_X + li - 1] = r1u;
+            srv[(lj - 1) * TILE_X + li - 1] = r1v;
+        }
+        su[k] = (flag[it] & 1) ? p1u : 0.0;
+        sv[k] = (flag[it] & 1) ? p1v : 0.0;
+    }
+    __syncthreads();
+    const VelTile tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        double r1u = ru[k] - a * qu0[k], r1v = rv[k] - a * qv0[k];
-        double p1u = r1u + beta * pu0[k], p1v = r1v + beta * pv0[k];
-        double q1u = dir_u(p, i) ? 0.0 : p1u - alpha * visc_u(p, pnu, pnv, i, j, 0);
-        double q1v = dir_v(p, j) ? 0.0 : p1v - alpha * visc_v(p, pnu, pnv, i, j, 0);
-        b.xu[k] += a * pu0[k];
-        b.xv[k] += a * pv0[k];
-        b.r_u[n][k] = r1u;
-        b.r_v[n][k] = r1v;
-        b.p_u[n][k] = p1u;
-        b.p_v[n][k] = p1v;
-        b.q_u[n][k] = q1u;
-        b.q_v[n][k] = q1v;
-        acc[0] += p1u * q1u + p1v * q1v;
-        acc[1] += r1u * q1u + r1v * q1v;
-        acc[2] += q1u * q1u + q1v * q1v;
-        acc[3] += r1u * r1u + r1v * r1v;
+    const int i = i0 + threadIdx.x;
+    if (i < p.nx) {
+        // x += a p on the cells (x and p of this block's rows, loaded together)
+        double vx[2 * ROWS], vp[2 * ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int j = min(j0 + threadIdx.y + r * TILE_Y, p.ny - 1);
+            const size_t k = at(p, i, j);
+            vx[2 * r] = b.xu[k];
+            vx[2 * r + 1] = b.xv[k];
+            vp[2 * r] = pu0[k];
+            vp[2 * r + 1] = pv0[k];
+        }
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            const int lj = threadIdx.y + r * TILE_Y;
+            const int j = j0 + lj;
+            if (j >= p.ny) break;
+            const size_t k = at(p, i, j);
+            b.xu[k] = vx[2 * r] + a * vp[2 * r];
+            b.xv[k] = vx[2 * r + 1] + a * vp[2 * r + 1];
+            const double p1u = su[(lj + 1) * VT_W + threadIdx.x + 1], p1v = sv[(lj + 1) * VT_W + threadIdx.x + 1];
+            const double r1u = sru[lj * TILE_X + threadIdx.x], r1v = srv[lj * TILE_X + threadIdx.x];
+            const double q1u = dir_u(p, i) ? 0.0 : p1u - alpha * visc_u(p, tu, tv, i, j, 0);
+            const double q1v = dir_v(p, j) ? 0.0 : p1v - alpha * visc_v(p, tu, tv, i, j, 0);
+            b.q_u[n][k] = q1u;
+            b.q_v[n][k] = q1v;
+            acc[0] += p1u * q1u + p1v * q1v;
+            acc[1] += r1u * q1u + r1v * q1v;
+            acc[2] += q1u * q1u + q1v * q1v;
+            acc[3] += r1u * r1u + r1v * r1v;
+        }
     }
     put_partials<4>(acc, pset(b.partials, nb, n ? PS_VEL1 : PS_VEL0), nb);
     if (block_id() == 0 && lane_id() == 0) st->vel_iters[slot] = m + 1;
@@ -369,15 +455,47 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs 
     if (block_id() == 0 && lane_id() == 0) st->pcg.rzs[m & 1] = rz;
     const double* po = b.p[(m + 1) & 1];
     double* pn = b.p[m & 1];
-    auto P = [&](int a, int c) {
-        size_t k = at(p, a, c);
-        return m == 0 ? b.z[k] : b.z[k] + beta * po[k];
-    };
+    // p_m on a halo-1 LDS tile, each point formed once (loads batched)
+    __shared__ double sp[VT_H * VT_W];
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
+    constexpr int NPT = VT_H * VT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
+    size_t idx[NIT];
+    bool in[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = min(tid + it * BLOCK, NPT - 1);
+        int gi = i0 + k % VT_W - 1, gj = j0 + k / VT_W - 1;
+        bool ok = true;
+        if (gi < 0 || gi >= p.nx) {
+            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+            else ok = false;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+            else ok = false;
+        }
+        in[it] = ok;
+        idx[it] = ok ? at(p, gi, gj) : 0;
+    }
+    double vz[NIT], vo[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        vz[it] = b.z[idx[it]];
+        vo[it] = m == 0 ? 0.0 : po[idx[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k < NPT) sp[k] = in[it] ? (m == 0 ? vz[it] : vz[it] + beta * vo[it]) : 0.0;
+    }
+    __syncthreads();
+    const VelTile P{sp, i0, j0, p.nx, p.ny};
     double acc[1] = {0.0};
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
-        double pv = P(i, j);
-        double w = neg_lap(g, P, i, j, nullptr);
+        const double pv = sp[(j - j0 + 1) * VT_W + (i - i0 + 1)];
+        const double w = neg_lap(g, P, i, j, nullptr);
         pn[k] = pv;
         b.w[k] = w;
         acc[0] += pv * w;
