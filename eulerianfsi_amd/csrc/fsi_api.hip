@@ -61,6 +61,13 @@ struct fsi_context {
     double* co_dense = nullptr;  // its dense bottom inverse
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
+    // Speculative steps: each solve enqueued with a planned iteration count,
+    // the whole step replayed as a captured HIP graph (see fsi_integrate_step).
+    int use_graph = 1;
+    int have_history = 0;  // iteration counts of a completed step are known
+    int plan_vel[3] = {0, 0, 0};
+    int plan_proj[3] = {0, 0, 0};
+    long long spec_steps = 0, spec_misses = 0;
 };
 
 struct fsi_state {
@@ -82,6 +89,15 @@ struct fsi_workspace {
     int corr_cur = 0;
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
+    // captured steps, keyed by everything their kernel arguments depend on
+    struct StepGraph {
+        int vel[3], proj[3];
+        double dt;
+        const double* sU;  // state/stage buffers alternate between steps
+        int corr_in, corr_out;
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> graphs;
 };
 
 // ---------------------------------------------------------------------------
@@ -254,6 +270,10 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     c->bcs = *bcs;
     c->models = *models;
     c->num = *numerics;
+    {  // FSI_STEP_GRAPH=0 keeps every step on the polling path
+        const char* e = std::getenv("FSI_STEP_GRAPH");
+        if (e && e[0] == '0') c->use_graph = 0;
+    }
     fsi_status rc = validate(c);
     if (rc != FSI_OK) return rc;
 
@@ -381,9 +401,19 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         g.dt = bq.top.kind == FSI_BC_DIRICHLET;
         g.ihx2 = 1.0 / (p.hx * p.hx);
         g.ihy2 = 1.0 / (p.hy * p.hy);
-        g.omega = 0.8;
+        // degree-2 Chebyshev smoothing on [1/3, 2] of the spectrum of D^-1 A
+        // (eigenvalues in [0, 2]): weights 1/r, r = 7/6 -+ (5/6) cos(pi/4)
+        g.om1 = 1.0 / (7.0 / 6.0 + 5.0 / 6.0 * std::sqrt(0.5));
+        g.om2 = 1.0 / (7.0 / 6.0 - 5.0 / 6.0 * std::sqrt(0.5));
+        if (const char* e = std::getenv("FSI_MG_SMOOTH")) {  // "w1,w2" overrides (tuning)
+            double a = 0.0, b = 0.0;
+            if (std::sscanf(e, "%lf,%lf", &a, &b) == 2 && a > 0.0 && b > 0.0) {
+                g.om1 = a;
+                g.om2 = b;
+            }
+        }
         g.d0 = g.ihx2 * 2.0 + g.ihy2 * 2.0;
-        g.wd0 = g.omega / g.d0;
+        g.id0 = 1.0 / g.d0;
         c->geo.push_back(g);
         while ((int)c->geo.size() < kMaxLevels && (long)g.nx * g.ny > 1024 && !(g.nx & 1) && !(g.ny & 1) &&
                g.nx >= 8 && g.ny >= 8) {
@@ -393,7 +423,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             g.ihx2 *= 0.25;
             g.ihy2 *= 0.25;
             g.d0 = g.ihx2 * 2.0 + g.ihy2 * 2.0;
-            g.wd0 = g.omega / g.d0;
+            g.id0 = 1.0 / g.d0;
             c->geo.push_back(g);
         }
         std::vector<double> dense;
@@ -548,6 +578,8 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
 
 extern "C" void fsi_workspace_destroy(fsi_workspace* w) {
     if (!w) return;
+    if (w->ctx && w->ctx->stream) (void)hipStreamSynchronize(w->ctx->stream);
+    for (auto& g : w->graphs) (void)hipGraphExecDestroy(g.exec);
     for (double* p : w->owned) (void)hipFree(p);
     double* stage[] = {w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys};
     for (double* p : stage)
@@ -607,7 +639,10 @@ static fsi_status poll_done(fsi_context* c, const KrylovScalars* dev, int* done,
     return FSI_OK;
 }
 
-static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot) {
+// fixed > 0: enqueue exactly `fixed` iteration kernels without reading the
+// residual back (speculative step); otherwise poll until converged.
+static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot,
+                                 int fixed) {
     const Params& p = c->prm;
     VelBufs b{};
     b.Un = s->U;
@@ -630,6 +665,10 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         return FSI_OK;
     }
     launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter, slot);
+    if (fixed > 0) {
+        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, c->num.velocity_maxiter + 1), slot);
+        return FSI_OK;
+    }
     // kernel m tests the residual of iterate m and performs update m+1
     const int last = c->num.velocity_maxiter;
     int m = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
@@ -646,7 +685,8 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     return FSI_OK;
 }
 
-static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double* V, int inject, int slot) {
+static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double* V, int inject, int slot,
+                             int fixed) {
     const Params& p = c->prm;
     ProjBufs b{};
     b.U = U;
@@ -667,6 +707,13 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
                     c->num.projection_maxiter);
     const int ps = slot < 0 ? 0 : slot;
     const int last = c->num.projection_maxiter;
+    if (fixed > 0) {
+        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan, c->st, 0,
+                         std::min(fixed, last + 1), slot);
+        launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
+        w->corr_cur = 1 - w->corr_cur;
+        return FSI_OK;
+    }
     int m = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
     while (m <= last) {
         int n = std::min(chunk, last + 1 - m);
@@ -735,7 +782,7 @@ static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, 
     fsi_status rc = status_push(c);
     if (rc != FSI_OK) return rc;
     const int saved = w->corr_cur;
-    rc = projection(c, w, s->U, s->V, 0, -1);
+    rc = projection(c, w, s->U, s->V, 0, -1, 0);
     if (rc != FSI_OK) return rc;
     if ((rc = status_pull(c)) != FSI_OK) return rc;
     fill_projection_result(c, res);
@@ -786,25 +833,13 @@ extern "C" fsi_status fsi_eval_tendencies(fsi_context* c, const fsi_state* s, fs
     return device_error(c);
 }
 
-extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double t, double dt,
-                                         fsi_step_report* rep, fsi_step_stats* stats) {
-    if (!c || !s || !w) return FSI_E_ARGUMENT;
-    if (s->res_nonfinite) return fail(c, FSI_E_ARGUMENT, "state.resource contains non-finite values.");
-    if (!std::isfinite(t)) return fail(c, FSI_E_ARGUMENT, "t must be finite.");
-    if (!std::isfinite(dt)) return fail(c, FSI_E_ARGUMENT, "dt must be finite.");
-    if (!(dt > 0.0)) return fail(c, FSI_E_ARGUMENT, "dt must be strictly positive.");
+// The stages of one step (TimeIntegrationIMEX.jl:1152-1256) on the stream;
+// with `planned` each solve runs the planned number of iterations and the
+// launch sequence is fixed (it can be captured into a graph).
+static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, bool planned) {
     const Params& p = c->prm;
-    // _inject_compact_predictor! requirements (:556-578)
-    if (!p.px && c->bcs.U.right.kind != FSI_BC_DIRICHLET)
-        return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic U-right boundary.");
-    if (!p.py && c->bcs.V.top.kind != FSI_BC_DIRICHLET)
-        return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic V-top boundary.");
-
-    status_reset(c);
-    fsi_status rc = status_push(c);
-    if (rc != FSI_OK) return rc;
-    const int saved_corr = w->corr_cur;
     const int reset = c->has_reset;
+    fsi_status rc = FSI_OK;
     // stage 1: the state itself (with the safety reset on sigma_n), tendencies
     tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1);
     for (int si = 1; si < 4; ++si) {
@@ -822,12 +857,122 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         const double alpha = dt * AI[si][si];
         launch_combine_faces(c->stream, p, s->U, s->V, w->Eu, w->Ev, w->Iu, w->Iv, si, dt, ce, ci, alpha,
                              p.visc ? c->bvisc_u : nullptr, p.visc ? c->bvisc_v : nullptr, w->ru, w->rv);
-        if ((rc = velocity_solve(c, s, w, alpha, si - 1)) != FSI_OK) return rc;
-        if ((rc = projection(c, w, w->Us, w->Vs, 1, si - 1)) != FSI_OK) return rc;
-        if (si <= 2)
-            tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0);
+        const int k = si - 1;
+        if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0)) != FSI_OK) return rc;
+        if ((rc = projection(c, w, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
+        if (si <= 2) tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0);
     }
-    if ((rc = status_pull(c)) != FSI_OK) return rc;
+    return FSI_OK;
+}
+
+// Launch the planned step as a graph, capturing it on first use.  A graph
+// is valid for the buffers it was captured with: the state/stage pair and
+// the correction warm start alternate between steps, so two graphs serve a
+// run at a fixed dt and plan.
+static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt) {
+    fsi_workspace::StepGraph* hit = nullptr;
+    for (auto& g : w->graphs) {
+        bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur;
+        for (int k = 0; k < 3; ++k) same = same && g.vel[k] == c->plan_vel[k] && g.proj[k] == c->plan_proj[k];
+        if (same) {
+            hit = &g;
+            break;
+        }
+    }
+    if (!hit) {
+        fsi_workspace::StepGraph g{};
+        for (int k = 0; k < 3; ++k) {
+            g.vel[k] = c->plan_vel[k];
+            g.proj[k] = c->plan_proj[k];
+        }
+        g.dt = dt;
+        g.sU = s->U;
+        g.corr_in = w->corr_cur;
+        hipGraph_t graph = nullptr;
+        HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        fsi_status rc = enqueue_step(c, s, w, dt, true);
+        hipError_t e = hipStreamEndCapture(c->stream, &graph);
+        if (rc != FSI_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+        }
+        HIPCHECK(c, e);
+        g.corr_out = w->corr_cur;
+        w->corr_cur = g.corr_in;  // nothing ran yet
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIPCHECK(c, e);
+        if (w->graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(w->graphs.front().exec);
+            w->graphs.erase(w->graphs.begin());
+        }
+        w->graphs.push_back(g);
+        hit = &w->graphs.back();
+    }
+    HIPCHECK(c, hipGraphLaunch(hit->exec, c->stream));
+    w->corr_cur = hit->corr_out;
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double t, double dt,
+                                         fsi_step_report* rep, fsi_step_stats* stats) {
+    if (!c || !s || !w) return FSI_E_ARGUMENT;
+    if (s->res_nonfinite) return fail(c, FSI_E_ARGUMENT, "state.resource contains non-finite values.");
+    if (!std::isfinite(t)) return fail(c, FSI_E_ARGUMENT, "t must be finite.");
+    if (!std::isfinite(dt)) return fail(c, FSI_E_ARGUMENT, "dt must be finite.");
+    if (!(dt > 0.0)) return fail(c, FSI_E_ARGUMENT, "dt must be strictly positive.");
+    const Params& p = c->prm;
+    // _inject_compact_predictor! requirements (:556-578)
+    if (!p.px && c->bcs.U.right.kind != FSI_BC_DIRICHLET)
+        return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic U-right boundary.");
+    if (!p.py && c->bcs.V.top.kind != FSI_BC_DIRICHLET)
+        return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic V-top boundary.");
+
+    const int saved_corr = w->corr_cur;
+    fsi_status rc = FSI_OK;
+    bool done = false, replayed = false;
+    if (c->use_graph && c->have_history) {
+        // speculative step: every solve gets its planned iteration count
+        // (the last step's count plus a margin; kernels past convergence
+        // return at once), replayed as one captured graph, one read-back
+        static const int kVelMargin = 2, kProjMargin = 1;
+        for (int k = 0; k < 3; ++k) {
+            const int need_v = c->last_vel_iters[k] + kVelMargin, need_p = c->last_proj_iters[k] + kProjMargin;
+            // grow at once; shrink only when the plan is well above the need
+            if (c->plan_vel[k] < need_v || c->plan_vel[k] > need_v + 3) c->plan_vel[k] = need_v;
+            if (c->plan_proj[k] < need_p || c->plan_proj[k] > need_p + 2) c->plan_proj[k] = need_p;
+        }
+        status_reset(c);
+        if ((rc = status_push(c)) != FSI_OK) return rc;
+        if ((rc = run_step_graph(c, s, w, dt)) != FSI_OK) return rc;
+        if ((rc = status_pull(c)) != FSI_OK) return rc;
+        const Status& st = *c->st_host;
+        bool ok = true;
+        for (int k = 0; k < 3; ++k) ok = ok && (!p.visc || st.vel_ok[k]) && st.proj_ok[k];
+        ++c->spec_steps;
+        if (ok) {
+            done = replayed = true;
+        } else {  // a solve needed more iterations than planned: redo with polling
+            ++c->spec_misses;
+            w->corr_cur = saved_corr;
+        }
+    }
+    if (!done) {
+        status_reset(c);
+        if ((rc = status_push(c)) != FSI_OK) return rc;
+        if ((rc = enqueue_step(c, s, w, dt, false)) != FSI_OK) return rc;
+        if ((rc = status_pull(c)) != FSI_OK) return rc;
+    }
+    {
+        const Status& st = *c->st_host;
+        for (int k = 0; k < 3; ++k) {
+            if (!p.visc || st.vel_ok[k]) c->last_vel_iters[k] = std::max(1, st.vel_iters[k]);
+            if (st.proj_ok[k]) c->last_proj_iters[k] = std::max(1, st.proj_iters[k]);
+        }
+        bool all = true;
+        for (int k = 0; k < 3; ++k) all = all && (!p.visc || st.vel_ok[k]) && st.proj_ok[k];
+        c->have_history = all ? 1 : 0;
+    }
     const Status& st = *c->st_host;
     // error priority follows the reference's order of checks
     if (st.nonfinite_in) {
@@ -865,11 +1010,13 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         rep->maximum_abs_sigma_determinant_error = st.max_det_err;
         rep->non_spd_cells = st.nonspd;
     }
-    if (stats)
+    if (stats) {
         for (int k = 0; k < 3; ++k) {
             stats->velocity_iterations[k] = st.vel_iters[k];
             stats->projection_iterations[k] = st.proj_iters[k];
         }
+        stats->graph_replayed = replayed ? 1 : 0;
+    }
     return FSI_OK;
 }
 

@@ -102,6 +102,7 @@ struct Status {
     double stage1_linf;      // max |div(v) - G| of the input velocity (stage 1)
     double gamma_umax;       // dynamic ACDI gamma: max |u_c|
     int vel_iters[3], proj_iters[3];
+    int vel_ok[3], proj_ok[3];  // 1 once the solve of that stage has converged
     KrylovScalars vel;
     KrylovScalars pcg;
     ProjScalars proj;

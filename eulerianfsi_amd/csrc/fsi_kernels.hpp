@@ -59,8 +59,8 @@ struct MgGeom {
     int px, py;
     int dl, dr, db, dt;  // 1 where the side is Dirichlet (else Neumann/periodic)
     double ihx2, ihy2;
-    double omega;        // damped-Jacobi weight
-    double d0, wd0;      // interior diagonal of -A (2/hx^2 + 2/hy^2) and omega / d0
+    double om1, om2;     // weights of the two Jacobi-preconditioned smoothing sweeps
+    double d0, id0;      // interior diagonal of -A (2/hx^2 + 2/hy^2) and 1 / d0
 };
 
 // Coarse levels solved in one workgroup (fsi_mg.hip): a top level of at most

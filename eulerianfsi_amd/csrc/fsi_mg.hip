@@ -1,9 +1,12 @@
 // fsi_mg.hip -- multigrid V-cycle of the projection preconditioner.
 //
 // Per fine level the V-cycle needs, from a zero initial guess,
-//   down:  x1 = w D^-1 b;  x2 = x1 + w D^-1 (b - A x1);  b_c = R (b - A x2)
-//   up:    x3 = x2 + P e_c;  x4 = x3 + w D^-1 (b - A x3);  x5 = x4 + w D^-1 (b - A x4)
-// (A = minus the unit Laplacian, R = P^T/4, P bilinear).  Each is one kernel
+//   down:  x1 = w1 D^-1 b;  x2 = x1 + w2 D^-1 (b - A x1);  b_c = R (b - A x2)
+//   up:    x3 = x2 + P e_c;  x4 = x3 + w2 D^-1 (b - A x3);  x5 = x4 + w1 D^-1 (b - A x4)
+// (A = minus the unit Laplacian, R = P^T/4, P bilinear).  The two smoothing
+// weights are the reciprocal roots of the degree-2 Chebyshev polynomial on
+// [1/3, 2] of the spectrum of D^-1 A (which lies in [0, 2]), so the smoother
+// is a fixed polynomial in D^-1 A and the V-cycle stays symmetric.  Each is one kernel
 // on an LDS tile with the halo the chained stencils need (3 cells down, 2
 // up), so a level costs 2 launches, reads b once and writes x2, b_c and x5
 // once.  The levels of at most CO_MAX_CELLS cells run in one workgroup
@@ -13,7 +16,7 @@
 // Tiles hold 0 at points outside a non-periodic wall, so -A x at any cell is
 // d x - (x_W + x_E)/hx^2 - (x_S + x_N)/hy^2 with the wall's contribution in
 // the diagonal d alone; across a periodic side the tile holds the wrapped
-// values.  w/d is the precomputed interior constant except on wall rows and
+// values.  1/d is the precomputed interior constant except on wall rows and
 // columns.
 #include <cmath>
 #include <vector>
@@ -32,14 +35,14 @@ constexpr int MG_THREADS = 256;
     _Pragma("unroll") for (int it_ = 0; it_ < ((N) + MG_THREADS - 1) / MG_THREADS; \
                            ++it_) if (const int k = tid + it_ * MG_THREADS; k < (N))
 
-// diagonal of -A at a cell and w / diagonal
+// diagonal of -A at a cell and its inverse
 struct Diag {
-    double d, wd;
+    double d, id;
 };
 __device__ __forceinline__ Diag mg_diag(const MgGeom& g, int gi, int gj) {
     const bool bx = !g.px && (gi == 0 || gi == g.nx - 1);
     const bool by = !g.py && (gj == 0 || gj == g.ny - 1);
-    if (!(bx || by)) return {g.d0, g.wd0};
+    if (!(bx || by)) return {g.d0, g.id0};
     double cx = 2.0, cy = 2.0;
     if (!g.px) {
         if (gi == 0) cx += g.dl ? 1.0 : -1.0;
@@ -50,7 +53,7 @@ __device__ __forceinline__ Diag mg_diag(const MgGeom& g, int gi, int gj) {
         if (gj == g.ny - 1) cy += g.dt ? 1.0 : -1.0;
     }
     const double d = g.ihx2 * cx + g.ihy2 * cy;
-    return {d, g.omega / d};
+    return {d, 1.0 / d};
 }
 
 // tile coordinate -> global index, wrapped across a periodic side; false
@@ -84,7 +87,9 @@ struct TileGather {
         for (int it = 0; it < NIT; ++it) {
             const int k = min(tid + it * MG_THREADS, N - 1);
             int i = 0, j = 0;
-            const bool ok = wrap_idx(nx, px, x0 + k % W, i) & wrap_idx(ny, py, y0 + k / W, j);
+            const bool okx = wrap_idx(nx, px, x0 + k % W, i);
+            const bool oky = wrap_idx(ny, py, y0 + k / W, j);
+            const bool ok = okx && oky;
             in[it] = ok;
             gi[it] = i;
             gj[it] = j;
@@ -126,7 +131,13 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
                                                         double* __restrict__ x2out, double* __restrict__ bc,
                                                         const int* done, Status* st, const double* __restrict__ rr,
                                                         int nb, int m, int slot) {
-    if (*done) return;
+    const int tid = threadIdx.x;
+    const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
+    // the tile loads go out with the done-flag read (harmless if done)
+    const int finished = *done;
+    TileGather<D3H * D3W, D3W> tb;
+    tb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 3, fj0 - 3, tid);
+    if (finished) return;
     if (st) {  // finest level: convergence test on |r|^2 of the previous PCG kernel
         double tot[1];
         get_totals<1>(rr, nb, tot);
@@ -136,6 +147,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
                 st->pcg.iter = m;
                 st->pcg.rr = tot[0];
                 if (slot >= 0) st->proj_iters[slot] = m;
+                if (slot >= 0 && tot[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
             }
             return;
         }
@@ -143,18 +155,14 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
     __shared__ double sb[D3H * D3W];  // b on halo 3
     __shared__ double sx[D3H * D3W];  // x1 on halo 3, then r on halo 1 (row length D1W)
     __shared__ double s2[D2H * D2W];  // x2 on halo 2, then row sums of the restriction (D1H x CTX)
-    const int tid = threadIdx.x;
-    const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
-    // b and x1 = w D^-1 b on halo 3
+    // b and x1 = w1 D^-1 b on halo 3
     {
-        TileGather<D3H * D3W, D3W> tb;
-        tb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 3, fj0 - 3, tid);
 #pragma unroll
         for (int it = 0; it < tb.NIT; ++it) {
             const int k = tid + it * MG_THREADS;
             if (k >= D3H * D3W) break;
             sb[k] = tb.val[it];
-            sx[k] = tb.in[it] ? mg_diag(gf, tb.gi[it], tb.gj[it]).wd * tb.val[it] : 0.0;
+            sx[k] = tb.in[it] ? gf.om1 * mg_diag(gf, tb.gi[it], tb.gj[it]).id * tb.val[it] : 0.0;
         }
     }
     __syncthreads();
@@ -166,7 +174,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
         if (wrap_idx(gf.nx, gf.px, fi0 + li - 2, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 2, gj)) {
             const Diag D = mg_diag(gf, gi, gj);
             const int c = (lj + 1) * D3W + li + 1;
-            v = sx[c] + D.wd * (sb[c] - tile_negA(gf, sx, c, D3W, D.d));
+            v = sx[c] + gf.om2 * D.id * (sb[c] - tile_negA(gf, sx, c, D3W, D.d));
         }
         s2[k] = v;
     }
@@ -234,7 +242,6 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
                                                       const double* __restrict__ b, const double* __restrict__ ec,
                                                       double* __restrict__ xout, const int* done, double* rz,
                                                       int nb) {
-    if (*done) return;
     __shared__ double sx3[D2H * D2W];  // x3 on halo 2
     __shared__ double sx4[D1H * D1W];  // x4 on halo 1
     __shared__ double sbb[D1H * D1W];  // b on halo 1
@@ -242,12 +249,15 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
     const int tid = threadIdx.x;
     const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
+    // the tile loads go out with the done-flag read (harmless if done)
+    const int finished = *done;
     TileGather<CH * CW, CW> tc;
     TileGather<D1H * D1W, D1W> tbb;
     TileGather<D2H * D2W, D2W> tx2;
     tc.load(ec, gc.P, gc.nx, gc.ny, gc.px, gc.py, ci0, cj0, tid);
     tbb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 1, fj0 - 1, tid);
     tx2.load(x2, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 2, fj0 - 2, tid);
+    if (finished) return;
 #pragma unroll
     for (int it = 0; it < tc.NIT; ++it) {
         const int k = tid + it * MG_THREADS;
@@ -289,7 +299,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
         if (wrap_idx(gf.nx, gf.px, fi0 + li - 1, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 1, gj)) {
             const Diag D = mg_diag(gf, gi, gj);
             const int c = (lj + 1) * D2W + li + 1;
-            v = sx3[c] + D.wd * (sbb[k] - tile_negA(gf, sx3, c, D2W, D.d));
+            v = sx3[c] + gf.om2 * D.id * (sbb[k] - tile_negA(gf, sx3, c, D2W, D.d));
         }
         sx4[k] = v;
     }
@@ -303,7 +313,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
         const Diag D = mg_diag(gf, gi, gj);
         const int c = (lj + 1) * D1W + li + 1;
         const double bv = sbb[c];
-        const double v = sx4[c] + D.wd * (bv - tile_negA(gf, sx4, c, D1W, D.d));
+        const double v = sx4[c] + gf.om1 * D.id * (bv - tile_negA(gf, sx4, c, D1W, D.d));
         xout[(size_t)gj * gf.P + gi] = v;
         acc += bv * v;
     }
@@ -331,7 +341,7 @@ __device__ __forceinline__ MgGeom co_geom(const MgGeom& g0, int l) {
     g.ihx2 *= s;
     g.ihy2 *= s;
     g.d0 *= s;
-    g.wd0 /= s;
+    g.id0 /= s;
     return g;
 }
 
@@ -354,7 +364,8 @@ __device__ __forceinline__ double co_negA(const MgGeom& g, const double* a, int 
     d = g.ihx2 * cx + g.ihy2 * cy;
     return d * a[k] - g.ihx2 * sx - g.ihy2 * sy;
 }
-__device__ __forceinline__ double co_wd(const MgGeom& g, double d) { return d == g.d0 ? g.wd0 : g.omega / d; }
+__device__ __forceinline__ double co_id(const MgGeom& g, double d) { return d == g.d0 ? g.id0 : 1.0 / d; }
+constexpr double CO_OMEGA = 0.8;  // damped-Jacobi weight of the fallback bottom sweeps
 
 // restriction weight of fine f = 2I + dx onto coarse I, with f wrapped
 __device__ __forceinline__ double co_rw(int I, int dx, int nf, int per, int dlo, int dhi, int& f) {
@@ -398,18 +409,21 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
                                                           int dense_n, const double* __restrict__ bglob,
                                                           double* __restrict__ tglob, const int* done) {
     extern __shared__ double s[];
-    if (*done) return;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    {  // b of the top level and x = w D^-1 b
+    {  // b of the top level (loads issued with the done-flag read) and x = w1 D^-1 b
+        const int finished = *done;
         const MgGeom g = co_geom(g0, 0);
         const int n = g.nx * g.ny;
         double* x = s;
         double* bb = s + 2 * n;
+        double bv0 = 0.0;
+        if (ty < g.ny && tx < g.nx) bv0 = bglob[(size_t)ty * g0.P + tx];
+        if (finished) return;
         CO_FOR(g) {
-            const double bv = bglob[(size_t)j * g0.P + i];
+            const double bv = (i == tx && j == ty) ? bv0 : bglob[(size_t)j * g0.P + i];
             const Diag D = mg_diag(g, i, j);
             bb[j * g.nx + i] = bv;
-            x[j * g.nx + i] = D.wd * bv;
+            x[j * g.nx + i] = g.om1 * D.id * bv;
         }
     }
     __syncthreads();
@@ -420,11 +434,11 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
         double* x = s + off;
         double* t = x + n;
         double* bb = x + 2 * n;
-        CO_FOR(g) {  // t = x + w D^-1 (b - A x)
+        CO_FOR(g) {  // t = x + w2 D^-1 (b - A x)
             double d;
             const double ax = co_negA(g, x, i, j, d);
             const int k = j * g.nx + i;
-            t[k] = x[k] + co_wd(g, d) * (bb[k] - ax);
+            t[k] = x[k] + g.om2 * co_id(g, d) * (bb[k] - ax);
         }
         __syncthreads();
         CO_FOR(g) {  // r = b - A t, into x
@@ -435,7 +449,7 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
         __syncthreads();
         double* xc = x + 3 * n;
         double* bc = xc + 2 * nc;
-        CO_FOR(gc) {  // b_c = R r, x_c = w D^-1 b_c
+        CO_FOR(gc) {  // b_c = R r, x_c = w1 D^-1 b_c
             double acc = 0.0;
             for (int dy = -1; dy <= 2; ++dy) {
                 int fy;
@@ -452,7 +466,7 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
             const double bv = 0.25 * acc;
             const Diag D = mg_diag(gc, i, j);
             bc[j * gc.nx + i] = bv;
-            xc[j * gc.nx + i] = D.wd * bv;
+            xc[j * gc.nx + i] = gc.om1 * D.id * bv;
         }
         __syncthreads();
         off += 3 * n;
@@ -479,13 +493,13 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
                     double d;
                     const double ax = co_negA(g, in, i, j, d);
                     const int k = j * g.nx + i;
-                    out[k] = in[k] + co_wd(g, d) * (bb[k] - ax);
+                    out[k] = in[k] + CO_OMEGA * co_id(g, d) * (bb[k] - ax);
                 }
                 __syncthreads();
             }
         }
     }
-    // up: t_l += P t_{l+1}; x = t + w D^-1 (b - A t); t = x + w D^-1 (b - A x)
+    // up: t_l += P t_{l+1}; x = t + w2 D^-1 (b - A t); t = x + w1 D^-1 (b - A x)
     for (int l = levels - 2; l >= 0; --l) {
         const MgGeom g = co_geom(g0, l), gc = co_geom(g0, l + 1);
         const int n = g.nx * g.ny;
@@ -507,14 +521,14 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
             double d;
             const double at = co_negA(g, t, i, j, d);
             const int k = j * g.nx + i;
-            x[k] = t[k] + co_wd(g, d) * (bb[k] - at);
+            x[k] = t[k] + g.om2 * co_id(g, d) * (bb[k] - at);
         }
         __syncthreads();
         CO_FOR(g) {
             double d;
             const double ax = co_negA(g, x, i, j, d);
             const int k = j * g.nx + i;
-            const double v = x[k] + co_wd(g, d) * (bb[k] - ax);
+            const double v = x[k] + g.om1 * co_id(g, d) * (bb[k] - ax);
             if (l == 0) tglob[(size_t)j * g0.P + i] = v;
             else t[k] = v;
         }
@@ -622,7 +636,7 @@ bool coarse_plan(const MgGeom& top, CoarsePlan& cp, std::vector<double>& dense) 
         gb.ihx2 *= sc;
         gb.ihy2 *= sc;
         gb.d0 *= sc;
-        gb.wd0 /= sc;
+        gb.id0 /= sc;
         if (dense_inverse(gb, dense)) cp.dense_n = nx * ny;
         else dense.clear();
     }

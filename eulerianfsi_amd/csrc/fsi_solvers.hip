@@ -96,6 +96,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double
         s.iter = 0;
         s.done = (tot[0] <= tol2) ? 1 : 0;
         st->vel_iters[slot] = 0;
+        if (tot[0] <= tol2) st->vel_ok[slot] = 1;
     }
     if (tot[0] <= tol2) return;
     auto pu = [&](int a, int c) { return b.r_u[0][at(p, a, c)]; };
@@ -153,6 +154,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
             st->vel.rr = rr;
             st->vel.iter = m;
             st->vel_iters[slot] = m;
+            if (conv) st->vel_ok[slot] = 1;
             // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
             if (!conv && rr > 1e-12 * sc.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / sc.bnorm2), (double)m);
         }
@@ -433,6 +435,7 @@ __global__ void k_pcg_check(Status* st, const double* rr, int nb, int m, int slo
         st->pcg.done = 1;
         st->pcg.iter = m;
         if (slot >= 0) st->proj_iters[slot] = m;
+        if (slot >= 0 && tot[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
     }
 }
 __global__ void __launch_bounds__(BLOCK) k_rz(Params p, ProjBufs b, double* rz, int nb, const int* done) {
@@ -670,13 +673,14 @@ static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, in
         launch_mg_coarse(s, g, cp, lv[l].b, lv[l].t, done);
     } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
         dim3 gr = grid_of(g.nx, g.ny);
-        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, g.omega, done);
+        const double omega = 0.8;  // damped-Jacobi sweeps of the fallback
+        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
         const int sweeps = 2 * (g.nx > g.ny ? g.nx : g.ny);
         for (int it = 0; it < sweeps; ++it) {
-            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, g.omega, done);
-            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, g.omega, done);
+            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
         }
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, g.omega, done);
+        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
     }
 }
 

@@ -160,6 +160,7 @@ typedef struct {
 typedef struct {
     int32_t velocity_iterations[3];   /* stages 2..4 */
     int32_t projection_iterations[3]; /* stages 2..4 */
+    int32_t graph_replayed;           /* 1: the step ran as a replayed speculative graph */
 } fsi_step_stats;
 
 typedef struct fsi_context fsi_context;

@@ -161,3 +161,25 @@ def test_periodic_filament_steps_match_oracle(method):
         print(method, k, {f: f"{v:.2e}" for f, v in e.items()}, flush=True)
         for f, v in e.items():
             assert v <= 1e-10, (k, f, v)
+
+
+def test_graph_replayed_steps_match_polling_steps(monkeypatch):
+    """Speculative steps (planned iteration counts, one captured graph per
+    buffer parity, replayed) give the same bits as the polling path."""
+    def run(graph):
+        monkeypatch.setenv("FSI_STEP_GRAPH", "1" if graph else "0")
+        case = setups.seeded_cavity(64, "acdi", seed=11, **TIGHT)
+        ctx, work = _device(case)
+        fsi.make_velocity_compatible(case.state, ctx, work)
+        replayed = []
+        for k in range(6):
+            fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+            replayed.append(work.last_stats.graph_replayed)
+        return {f: getattr(case.state, f).copy() for f in FIELDS}, replayed
+
+    a, ra = run(True)
+    b, rb = run(False)
+    print("replayed", ra, rb, flush=True)
+    assert not any(rb) and sum(ra) >= 4
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
