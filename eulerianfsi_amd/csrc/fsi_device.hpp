@@ -110,6 +110,22 @@ struct Status {
 
 __device__ __forceinline__ size_t at(const Params& p, int i, int j) { return (size_t)j * p.P + i; }
 
+// XCD-aware tile order.  The hardware deals consecutive workgroups round-robin
+// to the 8 XCDs, each with its own L2; renumbering the tiles so that every XCD
+// works a contiguous run of them (row-major) keeps the halo rows its tiles
+// share in that XCD's L2.  A bijection whenever the block count is a
+// multiple of 8, otherwise the identity.
+constexpr int kXcds = 8;
+struct TileId {
+    int x, y;
+};
+__device__ __forceinline__ TileId tile_id() {
+    const int nbx = gridDim.x, nb = gridDim.x * gridDim.y;
+    const int b = blockIdx.x + blockIdx.y * nbx;
+    const int t = (nb % kXcds == 0) ? (b % kXcds) * (nb / kXcds) + b / kXcds : b;
+    return {t % nbx, t / nbx};
+}
+
 __device__ __forceinline__ void raise_error(Status* st, int code, double v0, double v1) {
     if (atomicCAS(&st->code, 0, code) == 0) {
         st->v0 = v0;

@@ -132,7 +132,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
                                                         const int* done, Status* st, const double* __restrict__ rr,
                                                         int nb, int m, int slot) {
     const int tid = threadIdx.x;
-    const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
+    const int fi0 = tile_id().x * FT_X, fj0 = tile_id().y * FT_Y;
     // the tile loads go out with the done-flag read (harmless if done)
     const int finished = *done;
     TileGather<D3H * D3W, D3W> tb;
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
     __shared__ double sbb[D1H * D1W];  // b on halo 1
     __shared__ double sec[CH * CW];    // e_c around the tile
     const int tid = threadIdx.x;
-    const int fi0 = blockIdx.x * FT_X, fj0 = blockIdx.y * FT_Y;
+    const int fi0 = tile_id().x * FT_X, fj0 = tile_id().y * FT_Y;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
     // the tile loads go out with the done-flag read (harmless if done)
     const int finished = *done;

@@ -191,11 +191,11 @@ __device__ double elastic_fy(const Params& p, const Stage& s, int i, int j) {
 __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* __restrict__ Eu,
                                                     double* __restrict__ Ev, double* __restrict__ Iu,
                                                     double* __restrict__ Iv) {
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     if (i >= p.nx) return;
 #pragma unroll 1
     for (int r = 0; r < ROWS; ++r) {
-        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
         if (j >= p.ny) return;
         // ---- x-momentum at x-face (i, j)
         double cu;
@@ -620,11 +620,11 @@ __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s,
 // Max |u_c| for dynamic ACDI gamma (PhaseFieldMethods.jl:576-596).
 __global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, const double* V, Status* st) {
     __shared__ double lds[16];
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     double m = 0.0;
     if (i < p.nx)
         for (int r = 0; r < ROWS; ++r) {
-            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
             double uc = 0.5 * U[at(p, i, j)] + 0.5 * U[at(p, i + 1, j)];
             double vc = 0.5 * V[at(p, i, j)] + 0.5 * V[at(p, i, j + 1)];
@@ -681,7 +681,7 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
     __shared__ double s_n0[kNormals ? CR1 * CW1 : 1];
     __shared__ double s_n1[kNormals ? CR1 * CW1 : 1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
+    const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
     // 1. phi (and psi) on the halo-2 tile
     for (int k = tid; k < CR2 * CW2; k += BLOCK) {
         int li = k % CW2, lj = k / CW2;
@@ -770,11 +770,11 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     double dmax = 0.0;
     long long nonfinite = 0;
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     if (i < p.nx)
 #pragma unroll 1
         for (int r = 0; r < ROWS; ++r) {
-            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
             double oxx, oyy, oxy;
             sigma_rhs(p, s, i, j, st, oxx, oyy, oxy);
@@ -833,12 +833,12 @@ __global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, H
     __shared__ double lds[4][16];
     __shared__ long long lds_n[2][16];
     const double cf[3] = {c0, c1, c2};
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     double pmin = INFINITY, pmax = -INFINITY, emin = INFINITY, dmax = 0.0;
     long long nonspd = 0, nonfinite = 0;
     if (i < p.nx)
         for (int r = 0; r < ROWS; ++r) {
-            const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
             const size_t k = at(p, i, j);
             double ph = base.phi[k], a, d, b;
@@ -925,10 +925,10 @@ __global__ void __launch_bounds__(BLOCK) k_combine_faces(Params p, const double*
                                                          const double* __restrict__ bv, double* __restrict__ ru,
                                                          double* __restrict__ rv) {
     const double ce[3] = {e0, e1, e2}, ci[3] = {i0, i1, i2};
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     if (i >= p.nx) return;
     for (int r = 0; r < ROWS; ++r) {
-        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
         if (j >= p.ny) return;
         const size_t k = at(p, i, j);
         double u = Un[k], v = Vn[k];
@@ -948,10 +948,10 @@ __global__ void __launch_bounds__(BLOCK) k_combine_faces(Params p, const double*
 
 // b_visc = M*0 + b (the affine part of the viscous operator), computed once.
 __global__ void __launch_bounds__(BLOCK) k_visc_boundary(Params p, const double* zero, double* bu, double* bv) {
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     if (i >= p.nx) return;
     for (int r = 0; r < ROWS; ++r) {
-        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
         if (j >= p.ny) return;
         bu[at(p, i, j)] = visc_u(p, ZeroAcc{}, ZeroAcc{}, i, j, 1);
         bv[at(p, i, j)] = visc_v(p, ZeroAcc{}, ZeroAcc{}, i, j, 1);

@@ -32,9 +32,9 @@ static __device__ __forceinline__ int block_id() { return blockIdx.x + blockIdx.
 static __device__ __forceinline__ int lane_id() { return threadIdx.x + threadIdx.y * blockDim.x; }
 
 #define FOR_TILE(p)                                                              \
-    const int i = blockIdx.x * TILE_X + threadIdx.x;                             \
+    const int i = tile_id().x * TILE_X + threadIdx.x;                             \
     for (int rr_ = 0; rr_ < ROWS; ++rr_)                                         \
-        if (const int j = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;  \
+        if (const int j = tile_id().y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;  \
             i < (p).nx && j < (p).ny)
 
 // Partial-sum sets (see put_partials / get_totals in fsi_device.hpp): each
@@ -164,98 +164,112 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     const double beta = fmax(rr - 2.0 * a * rq + a * a * qq, 0.0) / rr;
     const int o = m & 1, n = (m + 1) & 1;
     const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
-    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];             // p' on halo 1
-    __shared__ double sru[TILE_ROWS * TILE_X], srv[TILE_ROWS * TILE_X];  // r' on the cells
+    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
-    constexpr int NPT = VT_H * VT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
-    // indices first, then all loads back to back (points outside a
-    // non-periodic wall read index 0 and are discarded)
-    size_t idx[NIT];
-    int flag[NIT];  // 1: inside the domain, 2: a cell of this block
-#pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int k = min(tid + it * BLOCK, NPT - 1);
-        const int li = k % VT_W, lj = k / VT_W;
-        const int ui = i0 + li - 1, uj = j0 + lj - 1;
-        int gi = ui, gj = uj;
-        bool in = true;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    // wrapped global index of tile point (li, lj) of the halo-1 tile, or -1
+    // outside a non-periodic wall
+    auto wrapped = [&](int li, int lj) -> long {
+        int gi = i0 + li - 1, gj = j0 + lj - 1;
         if (gi < 0 || gi >= p.nx) {
-            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
-            else in = false;
+            if (!p.px) return -1;
+            gi = ((gi % p.nx) + p.nx) % p.nx;
         }
         if (gj < 0 || gj >= p.ny) {
-            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
-            else in = false;
+            if (!p.py) return -1;
+            gj = ((gj % p.ny) + p.ny) % p.ny;
         }
-        const bool centre = in && li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
-        idx[it] = in ? at(p, gi, gj) : 0;
-        flag[it] = (in ? 1 : 0) | (centre ? 2 : 0);
-    }
-    double vru[NIT], vrv[NIT], vqu[NIT], vqv[NIT], vpu[NIT], vpv[NIT];
+        return (long)at(p, gi, gj);
+    };
+    // (1) the block's own cells (column tx, rows ty + 4r) in registers: all
+    // loads issued together; tile slots of cells past the domain edge get
+    // the wrapped values (periodic) or 0
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx;
+    size_t kc[ROWS];
+    bool own[ROWS], fill[ROWS];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        vru[it] = ru[idx[it]];
-        vrv[it] = rv[idx[it]];
-        vqu[it] = qu0[idx[it]];
-        vqv[it] = qv0[idx[it]];
-        vpu[it] = pu0[idx[it]];
-        vpv[it] = pv0[idx[it]];
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = j0 + ty + r * TILE_Y;
+        own[r] = i < p.nx && j < p.ny;
+        const long w = wrapped(tx + 1, ty + r * TILE_Y + 1);
+        fill[r] = w >= 0;
+        kc[r] = fill[r] ? (size_t)w : 0;
     }
+    double cru[ROWS], crv[ROWS], cqu[ROWS], cqv[ROWS], cpu[ROWS], cpv[ROWS], cxu[ROWS], cxv[ROWS];
 #pragma unroll
-    for (int it = 0; it < NIT; ++it) {
-        const int k = tid + it * BLOCK;
-        if (k >= NPT) break;
-        const size_t kk = idx[it];
-        const double r1u = vru[it] - a * vqu[it], r1v = vrv[it] - a * vqv[it];
-        const double p1u = r1u + beta * vpu[it], p1v = r1v + beta * vpv[it];
-        if (flag[it] & 2) {
-            const int li = k % VT_W, lj = k / VT_W;
-            b.r_u[n][kk] = r1u;
-            b.r_v[n][kk] = r1v;
-            b.p_u[n][kk] = p1u;
-            b.p_v[n][kk] = p1v;
-            sru[(lj - 1) * TILE_X + li - 1] = r1u;
-            srv[(lj - 1) * TILE_X + li - 1] = r1v;
+    for (int r = 0; r < ROWS; ++r) {
+        cru[r] = ru[kc[r]];
+        crv[r] = rv[kc[r]];
+        cqu[r] = qu0[kc[r]];
+        cqv[r] = qv0[kc[r]];
+        cpu[r] = pu0[kc[r]];
+        cpv[r] = pv0[kc[r]];
+        const size_t kx = own[r] ? kc[r] : 0;
+        cxu[r] = b.xu[kx];
+        cxv[r] = b.xv[kx];
+    }
+    // (2) the halo ring of the tile: 2 rows of VT_W and 2 columns of TILE_ROWS
+    constexpr int NRING = 2 * VT_W + 2 * TILE_ROWS;
+    double hp_u = 0.0, hp_v = 0.0;
+    int hslot = -1;
+    if (tid < NRING) {
+        int li, lj;
+        if (tid < VT_W) { li = tid; lj = 0; }
+        else if (tid < 2 * VT_W) { li = tid - VT_W; lj = VT_H - 1; }
+        else if (tid < 2 * VT_W + TILE_ROWS) { li = 0; lj = tid - 2 * VT_W + 1; }
+        else { li = VT_W - 1; lj = tid - 2 * VT_W - TILE_ROWS + 1; }
+        hslot = lj * VT_W + li;
+        const long w = wrapped(li, lj);
+        const size_t kk = w >= 0 ? (size_t)w : 0;
+        const double hru = ru[kk], hrv = rv[kk], hqu = qu0[kk], hqv = qv0[kk], hpu = pu0[kk], hpv = pv0[kk];
+        if (w >= 0) {
+            hp_u = (hru - a * hqu) + beta * hpu;
+            hp_v = (hrv - a * hqv) + beta * hpv;
         }
-        su[k] = (flag[it] & 1) ? p1u : 0.0;
-        sv[k] = (flag[it] & 1) ? p1v : 0.0;
+    }
+    double r1u[ROWS], r1v[ROWS], p1u[ROWS], p1v[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        r1u[r] = cru[r] - a * cqu[r];
+        r1v[r] = crv[r] - a * cqv[r];
+        p1u[r] = r1u[r] + beta * cpu[r];
+        p1v[r] = r1v[r] + beta * cpv[r];
+        const int slot_c = (ty + r * TILE_Y + 1) * VT_W + tx + 1;
+        su[slot_c] = fill[r] ? p1u[r] : 0.0;
+        sv[slot_c] = fill[r] ? p1v[r] : 0.0;
+        if (own[r]) {
+            const size_t k = kc[r];
+            b.xu[k] = cxu[r] + a * cpu[r];
+            b.xv[k] = cxv[r] + a * cpv[r];
+            b.r_u[n][k] = r1u[r];
+            b.r_v[n][k] = r1v[r];
+            b.p_u[n][k] = p1u[r];
+            b.p_v[n][k] = p1v[r];
+        }
+    }
+    if (hslot >= 0) {
+        su[hslot] = hp_u;
+        sv[hslot] = hp_v;
     }
     __syncthreads();
+    // (3) q' = (I - alpha M) p' from the tile, and the four partials
     const VelTile tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    const int i = i0 + threadIdx.x;
-    if (i < p.nx) {
-        // x += a p on the cells (x and p of this block's rows, loaded together)
-        double vx[2 * ROWS], vp[2 * ROWS];
 #pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            const int j = min(j0 + threadIdx.y + r * TILE_Y, p.ny - 1);
-            const size_t k = at(p, i, j);
-            vx[2 * r] = b.xu[k];
-            vx[2 * r + 1] = b.xv[k];
-            vp[2 * r] = pu0[k];
-            vp[2 * r + 1] = pv0[k];
-        }
-#pragma unroll
-        for (int r = 0; r < ROWS; ++r) {
-            const int lj = threadIdx.y + r * TILE_Y;
-            const int j = j0 + lj;
-            if (j >= p.ny) break;
-            const size_t k = at(p, i, j);
-            b.xu[k] = vx[2 * r] + a * vp[2 * r];
-            b.xv[k] = vx[2 * r + 1] + a * vp[2 * r + 1];
-            const double p1u = su[(lj + 1) * VT_W + threadIdx.x + 1], p1v = sv[(lj + 1) * VT_W + threadIdx.x + 1];
-            const double r1u = sru[lj * TILE_X + threadIdx.x], r1v = srv[lj * TILE_X + threadIdx.x];
-            const double q1u = dir_u(p, i) ? 0.0 : p1u - alpha * visc_u(p, tu, tv, i, j, 0);
-            const double q1v = dir_v(p, j) ? 0.0 : p1v - alpha * visc_v(p, tu, tv, i, j, 0);
-            b.q_u[n][k] = q1u;
-            b.q_v[n][k] = q1v;
-            acc[0] += p1u * q1u + p1v * q1v;
-            acc[1] += r1u * q1u + r1v * q1v;
-            acc[2] += q1u * q1u + q1v * q1v;
-            acc[3] += r1u * r1u + r1v * r1v;
-        }
+    for (int r = 0; r < ROWS; ++r) {
+        if (!own[r]) continue;
+        const int j = j0 + ty + r * TILE_Y;
+        const size_t k = kc[r];
+        const double q1u = dir_u(p, i) ? 0.0 : p1u[r] - alpha * visc_u(p, tu, tv, i, j, 0);
+        const double q1v = dir_v(p, j) ? 0.0 : p1v[r] - alpha * visc_v(p, tu, tv, i, j, 0);
+        b.q_u[n][k] = q1u;
+        b.q_v[n][k] = q1v;
+        acc[0] += p1u[r] * q1u + p1v[r] * q1v;
+        acc[1] += r1u[r] * q1u + r1v[r] * q1v;
+        acc[2] += q1u * q1u + q1v * q1v;
+        acc[3] += r1u[r] * r1u[r] + r1v[r] * r1v[r];
     }
     put_partials<4>(acc, pset(b.partials, nb, n ? PS_VEL1 : PS_VEL0), nb);
     if (block_id() == 0 && lane_id() == 0) st->vel_iters[slot] = m + 1;
@@ -461,7 +475,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs 
     // p_m on a halo-1 LDS tile, each point formed once (loads batched)
     __shared__ double sp[VT_H * VT_W];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const int i0 = blockIdx.x * TILE_X, j0 = blockIdx.y * TILE_ROWS;
+    const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
     constexpr int NPT = VT_H * VT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
     size_t idx[NIT];
     bool in[NIT];
@@ -534,10 +548,10 @@ __global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int n
 
 // v -= grad q on every face (src/Projection.jl:266-273), q with its BC data.
 __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
-    const int i = blockIdx.x * TILE_X + threadIdx.x;
+    const int i = tile_id().x * TILE_X + threadIdx.x;
     GAcc Q{b.x, p.P};
     for (int rr_ = 0; rr_ < ROWS; ++rr_) {
-        const int j = blockIdx.y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;
+        const int j = tile_id().y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;
         if (i > p.nx || j > p.ny) continue;
         if (j < p.ny) {  // x-face (i, j)
             double gx;
