@@ -415,7 +415,8 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         g.d0 = g.ihx2 * 2.0 + g.ihy2 * 2.0;
         g.id0 = 1.0 / g.d0;
         c->geo.push_back(g);
-        while ((int)c->geo.size() < kMaxLevels && (long)g.nx * g.ny > 1024 && !(g.nx & 1) && !(g.ny & 1) &&
+        const int co_cells = coarse_max_cells();
+        while ((int)c->geo.size() < kMaxLevels && (long)g.nx * g.ny > co_cells && !(g.nx & 1) && !(g.ny & 1) &&
                g.nx >= 8 && g.ny >= 8) {
             g.nx /= 2;
             g.ny /= 2;
@@ -427,7 +428,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             c->geo.push_back(g);
         }
         std::vector<double> dense;
-        if (coarse_plan(c->geo.back(), c->cplan, dense) && c->cplan.dense_n > 0) {
+        if (coarse_plan(c->geo.back(), co_cells, c->cplan, dense) && c->cplan.dense_n > 0) {
             HIPCHECK(c, dalloc(c, &c->co_dense, dense.size()));
             HIPCHECK(c, hipMemcpyAsync(c->co_dense, dense.data(), dense.size() * sizeof(double),
                                        hipMemcpyHostToDevice, c->stream));

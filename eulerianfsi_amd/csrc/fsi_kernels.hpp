@@ -64,9 +64,8 @@ struct MgGeom {
 };
 
 // Coarse levels solved in one workgroup (fsi_mg.hip): a top level of at most
-// CO_MAX_CELLS cells and its halvings, with a bottom of at most CO_DENSE_MAX
-// cells solved by a dense (pseudo-)inverse.
-constexpr int CO_MAX_CELLS = 1024;
+// coarse_max_cells() cells (1024 by default) and its halvings, with a bottom
+// of at most CO_DENSE_MAX cells solved by a dense (pseudo-)inverse.
 constexpr int CO_DENSE_MAX = 64;
 struct CoarsePlan {
     int levels;            // levels in the workgroup, top included
@@ -152,7 +151,8 @@ void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const doubl
                   const double* ec, double* xout, const int* done, double* rz, int nb);
 // Plan of the single-workgroup coarse solve for a top level (false when the
 // level is too large); `dense` receives the bottom's inverse, row-major.
-bool coarse_plan(const MgGeom& top, CoarsePlan& cp, std::vector<double>& dense);
+bool coarse_plan(const MgGeom& top, int max_cells, CoarsePlan& cp, std::vector<double>& dense);
+int coarse_max_cells();
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
                       const int* done);
 size_t partials_size(const Params& p);

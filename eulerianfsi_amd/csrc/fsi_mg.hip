@@ -9,7 +9,8 @@
 // is a fixed polynomial in D^-1 A and the V-cycle stays symmetric.  Each is one kernel
 // on an LDS tile with the halo the chained stencils need (3 cells down, 2
 // up), so a level costs 2 launches, reads b once and writes x2, b_c and x5
-// once.  The levels of at most CO_MAX_CELLS cells run in one workgroup
+// once; 256 threads work a tile of 64 x 16 cells on the finest level and of
+// 16 x 16 or 16 x 8 on coarse ones.  The levels of at most coarse_max_cells() cells run in one workgroup
 // (k_mg_coarse), whose bottom level is solved with a precomputed dense
 // (pseudo-)inverse.
 //
@@ -19,14 +20,13 @@
 // values.  1/d is the precomputed interior constant except on wall rows and
 // columns.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "fsi_kernels.hpp"
 
 namespace fsi {
 
-constexpr int FT_X = 64;  // fine tile
-constexpr int FT_Y = 16;
 constexpr int MG_THREADS = 256;
 
 // Loop over the N points of a tile phase with a compile-time trip count, so
@@ -69,6 +69,35 @@ __device__ __forceinline__ bool wrap_idx(int n, int per, int k, int& out) {
     return true;
 }
 
+// Where a tile sits.  IN = true: every point of the tile and its halo is a
+// cell strictly inside the domain (no wall row or column, no periodic wrap),
+// so indices are direct, the diagonal is the interior constant and the
+// transfer weights are the interior ones; the kernels pick this variant per
+// block (about 85 % of the finest-level tiles at N = 1024).
+template <bool IN>
+struct Where {
+    __device__ static __forceinline__ bool locate(const MgGeom& g, int ux, int uy, int& gi, int& gj) {
+        if (IN) {
+            gi = ux;
+            gj = uy;
+            return true;
+        }
+        const bool okx = wrap_idx(g.nx, g.px, ux, gi);
+        const bool oky = wrap_idx(g.ny, g.py, uy, gj);
+        return okx && oky;
+    }
+    __device__ static __forceinline__ Diag diag(const MgGeom& g, int gi, int gj) {
+        if (IN) return {g.d0, g.id0};
+        return mg_diag(g, gi, gj);
+    }
+};
+// is the fine region [x0, x1] x [y0, y1] strictly inside (see Where<true>)?
+__device__ __forceinline__ bool inside(const MgGeom& g, int x0, int x1, int y0, int y1) {
+    const int lo_x = g.px ? 0 : 1, hi_x = g.px ? g.nx - 1 : g.nx - 2;
+    const int lo_y = g.py ? 0 : 1, hi_y = g.py ? g.ny - 1 : g.ny - 2;
+    return x0 >= lo_x && x1 <= hi_x && y0 >= lo_y && y1 <= hi_y;
+}
+
 // Global loads of the N points of a tile (row length W, origin (x0, y0) in
 // unwrapped coordinates) into registers: indices first, then every load back
 // to back, so a thread has all of its loads in flight at once.  Points
@@ -80,25 +109,25 @@ struct TileGather {
     double val[NIT];
     int gi[NIT], gj[NIT];
     bool in[NIT];
-    __device__ __forceinline__ void load(const double* __restrict__ a, int P, int nx, int ny, int px, int py,
-                                         int x0, int y0, int tid) {
+    template <bool IN>
+    __device__ __forceinline__ void load(const double* __restrict__ a, const MgGeom& g, int x0, int y0, int tid) {
         int idx[NIT];
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int k = min(tid + it * MG_THREADS, N - 1);
             int i = 0, j = 0;
-            const bool okx = wrap_idx(nx, px, x0 + k % W, i);
-            const bool oky = wrap_idx(ny, py, y0 + k / W, j);
-            const bool ok = okx && oky;
+            const bool ok = Where<IN>::locate(g, x0 + k % W, y0 + k / W, i, j);
             in[it] = ok;
             gi[it] = i;
             gj[it] = j;
-            idx[it] = ok ? j * P + i : 0;
+            idx[it] = ok ? j * g.P + i : 0;
         }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) val[it] = a[idx[it]];
+        if (!IN) {
 #pragma unroll
-        for (int it = 0; it < NIT; ++it) val[it] = in[it] ? val[it] : 0.0;
+            for (int it = 0; it < NIT; ++it) val[it] = in[it] ? val[it] : 0.0;
+        }
     }
 };
 
@@ -110,10 +139,11 @@ __device__ __forceinline__ double tile_negA(const MgGeom& g, const double* a, in
 // 1-D restriction weights of fine 2I and 2I+1 onto coarse I (R = P^T/4 up to
 // the factor 1/4): 3/4 inside; at a non-periodic wall the fine cell's ghost
 // is the mirrored parent (Neumann, weight 1) or its negative (Dirichlet, 1/2).
+template <bool IN>
 __device__ __forceinline__ void rweights(int I, int nf, int per, int dlo, int dhi, double& w0, double& w1) {
     w0 = 0.75;
     w1 = 0.75;
-    if (!per) {
+    if (!IN && !per) {
         if (2 * I == 0) w0 = dlo ? 0.5 : 1.0;
         if (2 * I + 1 == nf - 1) w1 = dhi ? 0.5 : 1.0;
     }
@@ -122,22 +152,95 @@ __device__ __forceinline__ void rweights(int I, int nf, int per, int dlo, int dh
 // ---------------------------------------------------------------------------
 // down: b (fine) -> x2 (fine, stored) and b_c (coarse)
 // ---------------------------------------------------------------------------
-constexpr int D3W = FT_X + 6, D3H = FT_Y + 6;  // halo 3: 70 x 22
-constexpr int D2W = FT_X + 4, D2H = FT_Y + 4;  // halo 2: 68 x 20
-constexpr int D1W = FT_X + 2, D1H = FT_Y + 2;  // halo 1: 66 x 18
-constexpr int CTX = FT_X / 2, CTY = FT_Y / 2;  // coarse tile 32 x 8
+template <int FX, int FY>
+struct DownTile {
+    static constexpr int D3W = FX + 6, D3H = FY + 6;  // halo 3
+    static constexpr int D2W = FX + 4, D2H = FY + 4;  // halo 2
+    static constexpr int D1W = FX + 2, D1H = FY + 2;  // halo 1
+    static constexpr int CTX = FX / 2, CTY = FY / 2;  // coarse tile
+    static_assert(D1H * CTX <= D2H * D2W, "row sums fit the x2 buffer");
+};
 
+template <int FX, int FY, bool IN>
+__device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, const double* __restrict__ b,
+                                          double* __restrict__ x2out, double* __restrict__ bc, int fi0, int fj0,
+                                          int tid, double* sb, double* sx, double* s2) {
+    using T = DownTile<FX, FY>;
+    constexpr int D3W = T::D3W, D3H = T::D3H, D2W = T::D2W, D2H = T::D2H, D1W = T::D1W, D1H = T::D1H;
+    constexpr int CTX = T::CTX, CTY = T::CTY;
+    {  // b and x1 = w1 D^-1 b on halo 3
+        TileGather<D3H * D3W, D3W> tb;
+        tb.template load<IN>(b, gf, fi0 - 3, fj0 - 3, tid);
+#pragma unroll
+        for (int it = 0; it < tb.NIT; ++it) {
+            const int k = tid + it * MG_THREADS;
+            if (k >= D3H * D3W) break;
+            sb[k] = tb.val[it];
+            sx[k] = tb.in[it] ? gf.om1 * Where<IN>::diag(gf, tb.gi[it], tb.gj[it]).id * tb.val[it] : 0.0;
+        }
+    }
+    __syncthreads();
+    // x2 = x1 + w2 D^-1 (b - A x1) on halo 2
+    TILE_LOOP(D2H * D2W) {
+        const int li = k % D2W, lj = k / D2W;
+        int gi, gj;
+        double v = 0.0;
+        if (Where<IN>::locate(gf, fi0 + li - 2, fj0 + lj - 2, gi, gj)) {
+            const Diag D = Where<IN>::diag(gf, gi, gj);
+            const int c = (lj + 1) * D3W + li + 1;
+            v = sx[c] + gf.om2 * D.id * (sb[c] - tile_negA(gf, sx, c, D3W, D.d));
+        }
+        s2[k] = v;
+    }
+    __syncthreads();
+    // r = b - A x2 on halo 1 (into sx), x2 stored on the tile
+    TILE_LOOP(D1H * D1W) {
+        const int li = k % D1W, lj = k / D1W;
+        const int ui = fi0 + li - 1, uj = fj0 + lj - 1;
+        int gi, gj;
+        double v = 0.0;
+        if (Where<IN>::locate(gf, ui, uj, gi, gj)) {
+            const Diag D = Where<IN>::diag(gf, gi, gj);
+            const int c = (lj + 1) * D2W + li + 1;
+            v = sb[(lj + 2) * D3W + li + 2] - tile_negA(gf, s2, c, D2W, D.d);
+            if (li >= 1 && li <= FX && lj >= 1 && lj <= FY && (IN || (ui < gf.nx && uj < gf.ny)))
+                x2out[(size_t)uj * gf.P + ui] = s2[c];
+        }
+        sx[k] = v;
+    }
+    __syncthreads();
+    // restriction, x then y: row sums of r over fine 2I-1 .. 2I+2
+    double* sh = s2;
+    TILE_LOOP(D1H * CTX) {
+        const int ci = k % CTX, lj = k / CTX;
+        double w0, w1;
+        rweights<IN>(fi0 / 2 + ci, gf.nx, gf.px, gf.dl, gf.dr, w0, w1);
+        const double* row = sx + lj * D1W + 2 * ci;  // fine 2I-1 at halo-1 index 2 ci
+        sh[lj * CTX + ci] = 0.25 * row[0] + w0 * row[1] + w1 * row[2] + 0.25 * row[3];
+    }
+    __syncthreads();
+    TILE_LOOP(CTY * CTX) {
+        const int ci = k % CTX, cj = k / CTX;
+        const int I = fi0 / 2 + ci, J = fj0 / 2 + cj;
+        if (!IN && (I >= gc.nx || J >= gc.ny)) continue;
+        double w0, w1;
+        rweights<IN>(J, gf.ny, gf.py, gf.db, gf.dt, w0, w1);
+        const double* col = sh + 2 * cj * CTX + ci;
+        bc[(size_t)J * gc.P + I] =
+            0.25 * (0.25 * col[0] + w0 * col[CTX] + w1 * col[2 * CTX] + 0.25 * col[3 * CTX]);
+    }
+}
+
+// Tiles are FX x FY fine cells (64 x 16 on the finest level, whose partials
+// must line up with the PCG kernels' grid; smaller on coarser levels so that
+// they still spread over many CUs).
+template <int FX, int FY>
 __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, const double* __restrict__ b,
                                                         double* __restrict__ x2out, double* __restrict__ bc,
                                                         const int* done, Status* st, const double* __restrict__ rr,
                                                         int nb, int m, int slot) {
-    const int tid = threadIdx.x;
-    const int fi0 = tile_id().x * FT_X, fj0 = tile_id().y * FT_Y;
-    // the tile loads go out with the done-flag read (harmless if done)
-    const int finished = *done;
-    TileGather<D3H * D3W, D3W> tb;
-    tb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 3, fj0 - 3, tid);
-    if (finished) return;
+    using T = DownTile<FX, FY>;
+    if (*done) return;
     if (st) {  // finest level: convergence test on |r|^2 of the previous PCG kernel
         double tot[1];
         get_totals<1>(rr, nb, tot);
@@ -152,143 +255,91 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
             return;
         }
     }
-    __shared__ double sb[D3H * D3W];  // b on halo 3
-    __shared__ double sx[D3H * D3W];  // x1 on halo 3, then r on halo 1 (row length D1W)
-    __shared__ double s2[D2H * D2W];  // x2 on halo 2, then row sums of the restriction (D1H x CTX)
-    // b and x1 = w1 D^-1 b on halo 3
-    {
-#pragma unroll
-        for (int it = 0; it < tb.NIT; ++it) {
-            const int k = tid + it * MG_THREADS;
-            if (k >= D3H * D3W) break;
-            sb[k] = tb.val[it];
-            sx[k] = tb.in[it] ? gf.om1 * mg_diag(gf, tb.gi[it], tb.gj[it]).id * tb.val[it] : 0.0;
-        }
-    }
-    __syncthreads();
-    // x2 = x1 + w D^-1 (b - A x1) on halo 2
-    TILE_LOOP(D2H * D2W) {
-        const int li = k % D2W, lj = k / D2W;
-        int gi, gj;
-        double v = 0.0;
-        if (wrap_idx(gf.nx, gf.px, fi0 + li - 2, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 2, gj)) {
-            const Diag D = mg_diag(gf, gi, gj);
-            const int c = (lj + 1) * D3W + li + 1;
-            v = sx[c] + gf.om2 * D.id * (sb[c] - tile_negA(gf, sx, c, D3W, D.d));
-        }
-        s2[k] = v;
-    }
-    __syncthreads();
-    // r = b - A x2 on halo 1 (into sx), x2 stored on the tile
-    TILE_LOOP(D1H * D1W) {
-        const int li = k % D1W, lj = k / D1W;
-        const int ui = fi0 + li - 1, uj = fj0 + lj - 1;
-        int gi, gj;
-        double v = 0.0;
-        if (wrap_idx(gf.nx, gf.px, ui, gi) && wrap_idx(gf.ny, gf.py, uj, gj)) {
-            const Diag D = mg_diag(gf, gi, gj);
-            const int c = (lj + 1) * D2W + li + 1;
-            v = sb[(lj + 2) * D3W + li + 2] - tile_negA(gf, s2, c, D2W, D.d);
-            if (li >= 1 && li <= FT_X && lj >= 1 && lj <= FT_Y && ui < gf.nx && uj < gf.ny)
-                x2out[(size_t)uj * gf.P + ui] = s2[c];
-        }
-        sx[k] = v;
-    }
-    __syncthreads();
-    // restriction, x then y: row sums of r over fine 2I-1 .. 2I+2
-    double* sh = s2;
-    TILE_LOOP(D1H * CTX) {
-        const int ci = k % CTX, lj = k / CTX;
-        double w0, w1;
-        rweights(fi0 / 2 + ci, gf.nx, gf.px, gf.dl, gf.dr, w0, w1);
-        const double* row = sx + lj * D1W + 2 * ci;  // fine 2I-1 at halo-1 index 2 ci
-        sh[lj * CTX + ci] = 0.25 * row[0] + w0 * row[1] + w1 * row[2] + 0.25 * row[3];
-    }
-    __syncthreads();
-    TILE_LOOP(CTY * CTX) {
-        const int ci = k % CTX, cj = k / CTX;
-        const int I = fi0 / 2 + ci, J = fj0 / 2 + cj;
-        if (I >= gc.nx || J >= gc.ny) continue;
-        double w0, w1;
-        rweights(J, gf.ny, gf.py, gf.db, gf.dt, w0, w1);
-        const double* col = sh + 2 * cj * CTX + ci;
-        bc[(size_t)J * gc.P + I] =
-            0.25 * (0.25 * col[0] + w0 * col[CTX] + w1 * col[2 * CTX] + 0.25 * col[3 * CTX]);
-    }
+    __shared__ double sb[T::D3H * T::D3W];  // b on halo 3
+    __shared__ double sx[T::D3H * T::D3W];  // x1 on halo 3, then r on halo 1
+    __shared__ double s2[T::D2H * T::D2W];  // x2 on halo 2, then row sums of the restriction
+    const int fi0 = tile_id().x * FX, fj0 = tile_id().y * FY;
+    if (inside(gf, fi0 - 3, fi0 + FX + 2, fj0 - 3, fj0 + FY + 2))
+        down_tile<FX, FY, true>(gf, gc, b, x2out, bc, fi0, fj0, threadIdx.x, sb, sx, s2);
+    else
+        down_tile<FX, FY, false>(gf, gc, b, x2out, bc, fi0, fj0, threadIdx.x, sb, sx, s2);
 }
 
 // ---------------------------------------------------------------------------
 // up: x2 (fine), e_c (coarse) -> x5 (fine); on the finest level also the
 // block partials of r.z (b = r there).
 // ---------------------------------------------------------------------------
-constexpr int CW = FT_X / 2 + 4;  // coarse values around the tile: 36 x 12
-constexpr int CH = FT_Y / 2 + 4;
 
 // bilinear weights of the parent (w0) and the nearer coarse neighbour (1/4)
 // of fine f; the tile stores 0 beyond a non-periodic wall, where the ghost
 // is the mirrored parent (Neumann, w0 = 1) or its negative (Dirichlet, 1/2).
+template <bool IN>
 __device__ __forceinline__ void pweights(int f, int nf, int per, int dlo, int dhi, int& parent, int& other,
                                          double& w0) {
     parent = f >> 1;  // floor for negative tile coordinates as well
     other = (f & 1) ? parent + 1 : parent - 1;
     w0 = 0.75;
-    if (!per) {
+    if (!IN && !per) {
         if (f == 0) w0 = dlo ? 0.5 : 1.0;
         if (f == nf - 1) w0 = dhi ? 0.5 : 1.0;
     }
 }
 
-__global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const double* __restrict__ x2,
-                                                      const double* __restrict__ b, const double* __restrict__ ec,
-                                                      double* __restrict__ xout, const int* done, double* rz,
-                                                      int nb) {
-    __shared__ double sx3[D2H * D2W];  // x3 on halo 2
-    __shared__ double sx4[D1H * D1W];  // x4 on halo 1
-    __shared__ double sbb[D1H * D1W];  // b on halo 1
-    __shared__ double sec[CH * CW];    // e_c around the tile
-    const int tid = threadIdx.x;
-    const int fi0 = tile_id().x * FT_X, fj0 = tile_id().y * FT_Y;
+template <int FX, int FY>
+struct UpTile {
+    static constexpr int D2W = FX + 4, D2H = FY + 4;         // halo 2
+    static constexpr int D1W = FX + 2, D1H = FY + 2;         // halo 1
+    static constexpr int CW = FX / 2 + 4, CH = FY / 2 + 4;   // coarse values around the tile
+};
+
+template <int FX, int FY, bool IN>
+__device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, const double* __restrict__ x2,
+                                          const double* __restrict__ b, const double* __restrict__ ec,
+                                          double* __restrict__ xout, int fi0, int fj0, int tid, double* sx3,
+                                          double* sx4, double* sbb, double* sec) {
+    using T = UpTile<FX, FY>;
+    constexpr int D2W = T::D2W, D2H = T::D2H, D1W = T::D1W, D1H = T::D1H, CW = T::CW, CH = T::CH;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
-    // the tile loads go out with the done-flag read (harmless if done)
-    const int finished = *done;
-    TileGather<CH * CW, CW> tc;
-    TileGather<D1H * D1W, D1W> tbb;
-    TileGather<D2H * D2W, D2W> tx2;
-    tc.load(ec, gc.P, gc.nx, gc.ny, gc.px, gc.py, ci0, cj0, tid);
-    tbb.load(b, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 1, fj0 - 1, tid);
-    tx2.load(x2, gf.P, gf.nx, gf.ny, gf.px, gf.py, fi0 - 2, fj0 - 2, tid);
-    if (finished) return;
+    {
+        TileGather<CH * CW, CW> tc;
+        TileGather<D1H * D1W, D1W> tbb;
+        TileGather<D2H * D2W, D2W> tx2;
+        tc.template load<IN>(ec, gc, ci0, cj0, tid);
+        tbb.template load<IN>(b, gf, fi0 - 1, fj0 - 1, tid);
+        tx2.template load<IN>(x2, gf, fi0 - 2, fj0 - 2, tid);
 #pragma unroll
-    for (int it = 0; it < tc.NIT; ++it) {
-        const int k = tid + it * MG_THREADS;
-        if (k < CH * CW) sec[k] = tc.val[it];
-    }
-#pragma unroll
-    for (int it = 0; it < tbb.NIT; ++it) {
-        const int k = tid + it * MG_THREADS;
-        if (k < D1H * D1W) sbb[k] = tbb.val[it];
-    }
-    __syncthreads();
-    // x3 = x2 + P e_c on halo 2
-#pragma unroll
-    for (int it = 0; it < tx2.NIT; ++it) {
-        const int k = tid + it * MG_THREADS;
-        if (k >= D2H * D2W) break;
-        const int li = k % D2W, lj = k / D2W;
-        const int fx = fi0 + li - 2, fy = fj0 + lj - 2;
-        double v = 0.0;
-        if (tx2.in[it]) {
-            int px0, px1, py0, py1;
-            double wx0, wy0;
-            pweights(fx, gf.nx, gf.px, gf.dl, gf.dr, px0, px1, wx0);
-            pweights(fy, gf.ny, gf.py, gf.db, gf.dt, py0, py1, wy0);
-            const double* r0 = sec + (py0 - cj0) * CW;
-            const double* r1 = sec + (py1 - cj0) * CW;
-            const int a0 = px0 - ci0, a1 = px1 - ci0;
-            const double e = wy0 * (wx0 * r0[a0] + 0.25 * r0[a1]) + 0.25 * (wx0 * r1[a0] + 0.25 * r1[a1]);
-            v = tx2.val[it] + e;
+        for (int it = 0; it < tc.NIT; ++it) {
+            const int k = tid + it * MG_THREADS;
+            if (k < CH * CW) sec[k] = tc.val[it];
         }
-        sx3[k] = v;
+#pragma unroll
+        for (int it = 0; it < tbb.NIT; ++it) {
+            const int k = tid + it * MG_THREADS;
+            if (k < D1H * D1W) sbb[k] = tbb.val[it];
+        }
+        __syncthreads();
+        // x3 = x2 + P e_c on halo 2
+#pragma unroll
+        for (int it = 0; it < tx2.NIT; ++it) {
+            const int k = tid + it * MG_THREADS;
+            if (k >= D2H * D2W) break;
+            const int li = k % D2W, lj = k / D2W;
+            const int fx = fi0 + li - 2, fy = fj0 + lj - 2;
+            double v = 0.0;
+            if (tx2.in[it]) {
+                int px0, px1, py0, py1;
+                double wx0, wy0;
+                pweights<IN>(fx, gf.nx, gf.px, gf.dl, gf.dr, px0, px1, wx0);
+                pweights<IN>(fy, gf.ny, gf.py, gf.db, gf.dt, py0, py1, wy0);
+                const double* r0 = sec + (py0 - cj0) * CW;
+                const double* r1 = sec + (py1 - cj0) * CW;
+                const int a0 = px0 - ci0, a1 = px1 - ci0;
+                const double e =
+                    wy0 * (wx0 * r0[a0] + 0.25 * r0[a1]) + 0.25 * (wx0 * r1[a0] + 0.25 * r1[a1]);
+                v = tx2.val[it] + e;
+            }
+            sx3[k] = v;
+        }
     }
     __syncthreads();
     // x4 on halo 1
@@ -296,34 +347,54 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, cons
         const int li = k % D1W, lj = k / D1W;
         int gi, gj;
         double v = 0.0;
-        if (wrap_idx(gf.nx, gf.px, fi0 + li - 1, gi) && wrap_idx(gf.ny, gf.py, fj0 + lj - 1, gj)) {
-            const Diag D = mg_diag(gf, gi, gj);
+        if (Where<IN>::locate(gf, fi0 + li - 1, fj0 + lj - 1, gi, gj)) {
+            const Diag D = Where<IN>::diag(gf, gi, gj);
             const int c = (lj + 1) * D2W + li + 1;
             v = sx3[c] + gf.om2 * D.id * (sbb[k] - tile_negA(gf, sx3, c, D2W, D.d));
         }
         sx4[k] = v;
     }
     __syncthreads();
-    // x5 on the tile, plus r.z on the finest level
+    // x5 on the tile, and this thread's part of r.z
     double acc = 0.0;
-    TILE_LOOP(FT_X * FT_Y) {
-        const int li = k % FT_X, lj = k / FT_X;
+    TILE_LOOP(FX * FY) {
+        const int li = k % FX, lj = k / FX;
         const int gi = fi0 + li, gj = fj0 + lj;
-        if (gi >= gf.nx || gj >= gf.ny) continue;
-        const Diag D = mg_diag(gf, gi, gj);
+        if (!IN && (gi >= gf.nx || gj >= gf.ny)) continue;
+        const Diag D = Where<IN>::diag(gf, gi, gj);
         const int c = (lj + 1) * D1W + li + 1;
         const double bv = sbb[c];
         const double v = sx4[c] + gf.om1 * D.id * (bv - tile_negA(gf, sx4, c, D1W, D.d));
         xout[(size_t)gj * gf.P + gi] = v;
         acc += bv * v;
     }
+    return acc;
+}
+
+template <int FX, int FY>
+__global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const double* __restrict__ x2,
+                                                      const double* __restrict__ b, const double* __restrict__ ec,
+                                                      double* __restrict__ xout, const int* done, double* rz,
+                                                      int nb) {
+    using T = UpTile<FX, FY>;
+    if (*done) return;
+    __shared__ double sx3[T::D2H * T::D2W];  // x3 on halo 2
+    __shared__ double sx4[T::D1H * T::D1W];  // x4 on halo 1
+    __shared__ double sbb[T::D1H * T::D1W];  // b on halo 1
+    __shared__ double sec[T::CH * T::CW];    // e_c around the tile
+    const int fi0 = tile_id().x * FX, fj0 = tile_id().y * FY;
+    const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
+    const bool in = inside(gf, fi0 - 2, fi0 + FX + 1, fj0 - 2, fj0 + FY + 1) && ci0 >= 0 && cj0 >= 0 &&
+                    ci0 + T::CW <= gc.nx && cj0 + T::CH <= gc.ny;
+    const double acc = in ? up_tile<FX, FY, true>(gf, gc, x2, b, ec, xout, fi0, fj0, threadIdx.x, sx3, sx4, sbb, sec)
+                          : up_tile<FX, FY, false>(gf, gc, x2, b, ec, xout, fi0, fj0, threadIdx.x, sx3, sx4, sbb, sec);
     if (!rz) return;
     double v[1] = {acc};
     put_partials<1>(v, rz, nb);
 }
 
 // ---------------------------------------------------------------------------
-// Coarse levels in one workgroup: a top level of at most CO_MAX_CELLS cells,
+// Coarse levels in one workgroup: a top level of at most coarse_max_cells() cells,
 // halving while both sizes stay even, down to a bottom of at most
 // CO_DENSE_MAX cells solved with its dense (pseudo-)inverse (otherwise:
 // CO_SWEEPS pairs of damped-Jacobi sweeps).  Level l keeps x, t and b (n_l
@@ -410,6 +481,15 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
                                                           double* __restrict__ tglob, const int* done) {
     extern __shared__ double s[];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    // this thread's entries of the dense bottom inverse (row tid/16, columns
+    // tid%16 + 16q), loaded now so they are in registers at the bottom
+    const int drow = threadIdx.x >> 4, dlane = threadIdx.x & 15;
+    double dm[CO_DENSE_MAX / 16];
+#pragma unroll
+    for (int q = 0; q < CO_DENSE_MAX / 16; ++q) {
+        const int c = dlane + 16 * q;
+        dm[q] = (drow < dense_n && c < dense_n) ? dense[drow * dense_n + c] : 0.0;
+    }
     {  // b of the top level (loads issued with the done-flag read) and x = w1 D^-1 b
         const int finished = *done;
         const MgGeom g = co_geom(g0, 0);
@@ -478,12 +558,14 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
         double* t = x + n;
         double* bb = x + 2 * n;
         if (dense_n) {  // t = M b, 16 lanes per row
-            const int row = threadIdx.x >> 4, lane = threadIdx.x & 15;
             double acc = 0.0;
-            if (row < n)
-                for (int c = lane; c < n; c += 16) acc += dense[row * n + c] * bb[c];
+#pragma unroll
+            for (int q = 0; q < CO_DENSE_MAX / 16; ++q) {
+                const int c = dlane + 16 * q;
+                if (c < n) acc += dm[q] * bb[c];
+            }
             for (int msk = 8; msk >= 1; msk >>= 1) acc += __shfl_xor(acc, msk, 16);
-            if (row < n && lane == 0) t[row] = acc;
+            if (drow < n && dlane == 0) t[drow] = acc;
             __syncthreads();
         } else {
             for (int it = 0; it <= 2 * CO_SWEEPS; ++it) {
@@ -614,10 +696,10 @@ static bool dense_inverse(const MgGeom& g, std::vector<double>& M) {
     return true;
 }
 
-bool coarse_plan(const MgGeom& top, CoarsePlan& cp, std::vector<double>& dense) {
+bool coarse_plan(const MgGeom& top, int max_cells, CoarsePlan& cp, std::vector<double>& dense) {
     cp = CoarsePlan{};
     dense.clear();
-    if ((long)top.nx * top.ny > CO_MAX_CELLS) return false;
+    if ((long)top.nx * top.ny > max_cells) return false;
     int levels = 1, nx = top.nx, ny = top.ny;
     size_t cells = (size_t)nx * ny;
     while ((long)nx * ny > CO_DENSE_MAX && !(nx & 1) && !(ny & 1) && nx >= 4 && ny >= 4) {
@@ -643,21 +725,72 @@ bool coarse_plan(const MgGeom& top, CoarsePlan& cp, std::vector<double>& dense) 
     return true;
 }
 
+int coarse_max_cells() {
+    // 1024 cells: measured faster on the 1024^2 cavity than a 4096-cell top
+    // level (one CU then works 4 cells per thread in every phase, slower
+    // than two extra level launches).  FSI_MG_COARSE_CELLS=4096 selects the
+    // larger top level, which needs 3 x 8 x (4096 + 1024 + 256 + 64) bytes of
+    // LDS in one workgroup.
+    const char* e = std::getenv("FSI_MG_COARSE_CELLS");
+    if (!e || std::atoi(e) != 4096) return 1024;
+    int dev = 0, lds = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 1024;
+    const size_t need = 3 * sizeof(double) * (4096 + 1024 + 256 + 64);
+    if ((size_t)lds < need) return 1024;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)need) != hipSuccess) {
+        (void)hipGetLastError();
+        return 1024;
+    }
+    return 4096;
+}
+
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
                       const int* done) {
     k_mg_coarse<<<1, CO_THREADS, cp.lds_bytes, s>>>(top, cp.levels, cp.dense, cp.dense_n, b, t, done);
 }
 
+// tile of a level: 64 x 16 on the finest (rz/rr partials follow the PCG
+// grid), else the largest of 64 x 16, 16 x 16, 16 x 8 that still gives 1024
+// blocks (four per CU), or 16 x 8
+static int level_tile(const MgGeom& g, bool finest) {
+    if (finest) return 0;
+    auto blocks = [&](int fx, int fy) { return (long)((g.nx + fx - 1) / fx) * ((g.ny + fy - 1) / fy); };
+    if (blocks(64, 16) >= 1024) return 0;
+    if (blocks(16, 16) >= 1024) return 1;
+    return 2;
+}
+
+template <int FX, int FY>
+static void down_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
+                   const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+    dim3 g((gf.nx + FX - 1) / FX, (gf.ny + FY - 1) / FY);
+    k_mg_down<FX, FY><<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+}
+template <int FX, int FY>
+static void up_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
+                 const double* ec, double* xout, const int* done, double* rz, int nb) {
+    dim3 g((gf.nx + FX - 1) / FX, (gf.ny + FY - 1) / FY);
+    k_mg_up<FX, FY><<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, done, rz, nb);
+}
+
 void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
                     const int* done, Status* st, const double* rr, int nb, int m, int slot) {
-    dim3 g((gf.nx + FT_X - 1) / FT_X, (gf.ny + FT_Y - 1) / FT_Y);
-    k_mg_down<<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+    switch (level_tile(gf, st != nullptr)) {
+        case 0: down_t<64, 16>(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot); break;
+        case 1: down_t<16, 16>(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot); break;
+        default: down_t<16, 8>(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot); break;
+    }
 }
 
 void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
                   const double* ec, double* xout, const int* done, double* rz, int nb) {
-    dim3 g((gf.nx + FT_X - 1) / FT_X, (gf.ny + FT_Y - 1) / FT_Y);
-    k_mg_up<<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, done, rz, nb);
+    switch (level_tile(gf, rz != nullptr)) {
+        case 0: up_t<64, 16>(s, gf, gc, x2, b, ec, xout, done, rz, nb); break;
+        case 1: up_t<16, 16>(s, gf, gc, x2, b, ec, xout, done, rz, nb); break;
+        default: up_t<16, 8>(s, gf, gc, x2, b, ec, xout, done, rz, nb); break;
+    }
 }
 
 }  // namespace fsi
