@@ -41,6 +41,30 @@ def step_bytes(N, vel_iters, proj_iters, pcg_passes=17.3):
     return passes * cell
 
 
+# Compulsory passes of a textbook implementation of the same step (the
+# accounting the throughput target is quoted in): tendencies 3 x 14, stage
+# combinations 80, textbook CG 28 passes per iteration + 6 per solve, textbook
+# PCG with a V-cycle 30 per iteration + 18 per projection, report 6.
+def textbook_step_bytes(N, vel_iters, proj_iters):
+    cell = 8.0 * N * N
+    passes = 42 + 80 + sum(6 + 28 * k for k in vel_iters) + sum(18 + 30 * k for k in proj_iters) + 6
+    return passes * cell
+
+
+def profiled_traffic(kernel, N, method):
+    """HBM bytes per launch of `kernel` from its committed rocprof summary
+    (FETCH_SIZE + WRITE_SIZE, calibrated by a copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_cavity{N}_{method}.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,13 +143,15 @@ def main(argv=None):
     dominant = max(share, key=share.get)
     dk = kernels[dominant]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": dk["GBps"], "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": dk["GBps"] * 1e9 / HBM_PEAK, "traffic": None,
+                "frac": dk["GBps"] * 1e9 / HBM_PEAK, "traffic": profiled_traffic(dominant, args.N, args.method),
                 "bytes_per_launch": dk["bytes"], "avg_launch_us": dk["avg_ms"] * 1e3,
                 "share_of_step": share[dominant] / ms_per_step}
 
     b_step = step_bytes(args.N, vel_mean, proj_mean)
+    b_text = textbook_step_bytes(args.N, vel_mean, proj_mean)
     whole_step = {"algorithmic_GB": b_step / 1e9, "achieved_GBps": b_step / (ms_per_step * 1e-3) / 1e9,
-                  "frac_of_peak": b_step / (ms_per_step * 1e-3) / HBM_PEAK}
+                  "frac_of_peak": b_step / (ms_per_step * 1e-3) / HBM_PEAK,
+                  "textbook_GB": b_text / 1e9, "textbook_frac_of_peak": b_text / (ms_per_step * 1e-3) / HBM_PEAK}
 
     cpu = None
     if args.cpu_steps > 0:
