@@ -96,7 +96,7 @@ class ProjectionResultC(ctypes.Structure):
 
 class StepStatsC(ctypes.Structure):
     _fields_ = [("velocity_iterations", c_int32 * 3), ("projection_iterations", c_int32 * 3),
-                ("graph_replayed", c_int32)]
+                ("planned", c_int32)]
 
 
 # ---- configuration conversion ------------------------------------------------

@@ -61,12 +61,22 @@ struct fsi_context {
     double* co_dense = nullptr;  // its dense bottom inverse
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
-    // Speculative steps: each solve enqueued with a planned iteration count,
-    // the whole step replayed as a captured HIP graph (see fsi_integrate_step).
-    int use_graph = 1;
+    // Planned steps: each solve enqueued with a planned iteration count and no
+    // read-back until the end of the step (see fsi_integrate_step).  Mode 2
+    // launches the kernels directly, mode 1 replays the step as a captured
+    // HIP graph (measured slower on ROCm 7 for the ~460-launch step), mode 0
+    // polls each solve.
+    int use_graph = 2;
+    int stage_warm = 1;    // per-stage projection warm starts (FSI_STAGE_WARM=0: shared correction)
     int have_history = 0;  // iteration counts of a completed step are known
     int plan_vel[3] = {0, 0, 0};
     int plan_proj[3] = {0, 0, 0};
+    // iteration counts of the last kHist completed steps per stage (the plan
+    // is their maximum plus a margin, so alternating counts keep one plan)
+    static constexpr int kHist = 4;
+    int hist_vel[3][kHist] = {};
+    int hist_proj[3][kHist] = {};
+    int hist_n = 0;
     long long spec_steps = 0, spec_misses = 0;
 };
 
@@ -87,6 +97,13 @@ struct fsi_workspace {
     double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
     double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
     int corr_cur = 0;
+    // Warm starts of the three stage projections of a step: stage s starts
+    // from stage s of the previous step (closer than the previous stage's
+    // correction, which the shared `corr` holds).  Only iteration counts
+    // depend on it; `corr` keeps the reference's meaning for callers.
+    double* corr_stage[3] = {};
+    double* corr_prev[3] = {};  // the step before, for the linear extrapolation 2 q^{n-1} - q^{n-2}
+    int stage_warm = 0;         // completed steps whose stage solutions corr_stage / corr_prev hold (0, 1, 2)
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
     // captured steps, keyed by everything their kernel arguments depend on
@@ -95,6 +112,7 @@ struct fsi_workspace {
         double dt;
         const double* sU;  // state/stage buffers alternate between steps
         int corr_in, corr_out;
+        int warm;          // stage warm-start history (0, 1, 2): changes the launch sequence
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> graphs;
@@ -271,8 +289,10 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     c->models = *models;
     c->num = *numerics;
     {  // FSI_STEP_GRAPH=0 keeps every step on the polling path
-        const char* e = std::getenv("FSI_STEP_GRAPH");
-        if (e && e[0] == '0') c->use_graph = 0;
+        const char* e = std::getenv("FSI_STEP_GRAPH");  // 0 polling, 1 planned graph, 2 planned direct
+        if (e && (e[0] == '0' || e[0] == '1' || e[0] == '2')) c->use_graph = e[0] - '0';
+        e = std::getenv("FSI_STAGE_WARM");
+        if (e && e[0] == '0') c->stage_warm = 0;
     }
     fsi_status rc = validate(c);
     if (rc != FSI_OK) return rc;
@@ -548,7 +568,9 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
     for (auto p : stage) HIPCHECK(c, dalloc(c, p, F));
     double** singles[] = {&w->ru,     &w->rv,     &w->r_u[0], &w->r_u[1], &w->r_v[0], &w->r_v[1], &w->p_u[0],
                           &w->p_u[1], &w->p_v[0], &w->p_v[1], &w->q_u[0], &w->q_u[1], &w->q_v[0], &w->q_v[1],
-                          &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w};
+                          &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w,
+                          &w->corr_stage[0], &w->corr_stage[1], &w->corr_stage[2],
+                          &w->corr_prev[0],  &w->corr_prev[1],  &w->corr_prev[2]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -695,7 +717,9 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     b.inject = inject;
     b.lap_bc = c->lap_bc;
     b.rhs = w->rhs;
-    b.x = w->corr[w->corr_cur];
+    b.x = (slot >= 0 && c->stage_warm) ? w->corr_stage[slot] : w->corr[w->corr_cur];
+    if (slot >= 0 && c->stage_warm && w->stage_warm >= 1)
+        launch_extrapolate(c->stream, p, b.x, w->corr_prev[slot], w->stage_warm >= 2);
     b.q_out = w->corr[1 - w->corr_cur];
     b.r = w->r;
     b.z = w->z;
@@ -873,7 +897,7 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
 static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt) {
     fsi_workspace::StepGraph* hit = nullptr;
     for (auto& g : w->graphs) {
-        bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur;
+        bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur && g.warm == w->stage_warm;
         for (int k = 0; k < 3; ++k) same = same && g.vel[k] == c->plan_vel[k] && g.proj[k] == c->plan_proj[k];
         if (same) {
             hit = &g;
@@ -889,6 +913,7 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
         g.dt = dt;
         g.sU = s->U;
         g.corr_in = w->corr_cur;
+        g.warm = w->stage_warm;
         hipGraph_t graph = nullptr;
         HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         fsi_status rc = enqueue_step(c, s, w, dt, true);
@@ -903,7 +928,7 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
         e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIPCHECK(c, e);
-        if (w->graphs.size() >= 8) {
+        if (w->graphs.size() >= 16) {
             (void)hipGraphExecDestroy(w->graphs.front().exec);
             w->graphs.erase(w->graphs.begin());
         }
@@ -932,20 +957,35 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
     const int saved_corr = w->corr_cur;
     fsi_status rc = FSI_OK;
     bool done = false, replayed = false;
+    if (c->stage_warm && !w->stage_warm) {  // first step: every stage starts from the shared correction
+        for (int k = 0; k < 3; ++k)
+            HIPCHECK(c, hipMemcpyAsync(w->corr_stage[k], w->corr[w->corr_cur], c->felems * sizeof(double),
+                                       hipMemcpyDeviceToDevice, c->stream));
+    }
     if (c->use_graph && c->have_history) {
-        // speculative step: every solve gets its planned iteration count
-        // (the last step's count plus a margin; kernels past convergence
-        // return at once), replayed as one captured graph, one read-back
-        static const int kVelMargin = 2, kProjMargin = 1;
+        // planned step: every solve gets its planned iteration count (kernels
+        // past convergence return at once), one read-back at the end
+        // a solve of k iterations is seen converged by launch k + 1 (kernel m
+        // tests the residual of iterate m), so the plan is the maximum count
+        // of the last kHist steps plus one; a larger count redoes the step
+        static const int kVelMargin = 1, kProjMargin = 1;
+        const int nh = std::min(c->hist_n, fsi_context::kHist);
         for (int k = 0; k < 3; ++k) {
-            const int need_v = c->last_vel_iters[k] + kVelMargin, need_p = c->last_proj_iters[k] + kProjMargin;
-            // grow at once; shrink only when the plan is well above the need
-            if (c->plan_vel[k] < need_v || c->plan_vel[k] > need_v + 3) c->plan_vel[k] = need_v;
-            if (c->plan_proj[k] < need_p || c->plan_proj[k] > need_p + 2) c->plan_proj[k] = need_p;
+            int mv = c->last_vel_iters[k], mp = c->last_proj_iters[k];
+            for (int h = 0; h < nh; ++h) {
+                mv = std::max(mv, c->hist_vel[k][h]);
+                mp = std::max(mp, c->hist_proj[k][h]);
+            }
+            c->plan_vel[k] = mv + kVelMargin;
+            c->plan_proj[k] = mp + kProjMargin;
         }
         status_reset(c);
         if ((rc = status_push(c)) != FSI_OK) return rc;
-        if ((rc = run_step_graph(c, s, w, dt)) != FSI_OK) return rc;
+        if (c->use_graph == 2) {  // planned counts, launched directly
+            if ((rc = enqueue_step(c, s, w, dt, true)) != FSI_OK) return rc;
+        } else if ((rc = run_step_graph(c, s, w, dt)) != FSI_OK) {
+            return rc;
+        }
         if ((rc = status_pull(c)) != FSI_OK) return rc;
         const Status& st = *c->st_host;
         bool ok = true;
@@ -973,6 +1013,14 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         bool all = true;
         for (int k = 0; k < 3; ++k) all = all && (!p.visc || st.vel_ok[k]) && st.proj_ok[k];
         c->have_history = all ? 1 : 0;
+        if (all) {
+            const int h = c->hist_n % fsi_context::kHist;
+            for (int k = 0; k < 3; ++k) {
+                c->hist_vel[k][h] = c->last_vel_iters[k];
+                c->hist_proj[k][h] = c->last_proj_iters[k];
+            }
+            ++c->hist_n;
+        }
     }
     const Status& st = *c->st_host;
     // error priority follows the reference's order of checks
@@ -1001,6 +1049,7 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
     std::swap(s->sxx, w->sxxs);
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
+    if (c->stage_warm && w->stage_warm < 2) ++w->stage_warm;
     if (rep) {
         rep->maximum_projection_constraint_linf = std::fmax(st.stage1_linf, st.max_constraint_linf);
         rep->maximum_growth_trace_error = 0.0;
@@ -1016,7 +1065,7 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
             stats->velocity_iterations[k] = st.vel_iters[k];
             stats->projection_iterations[k] = st.proj_iters[k];
         }
-        stats->graph_replayed = replayed ? 1 : 0;
+        stats->planned = replayed ? 1 : 0;
     }
     return FSI_OK;
 }

@@ -130,6 +130,7 @@ struct ProjBufs {
     double* partials;
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
+void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,

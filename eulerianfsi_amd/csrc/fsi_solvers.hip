@@ -671,6 +671,21 @@ void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, in
     k_rz<<<grid2(p), kBlock, 0, s>>>(p, b, rz, nb, done);
 }
 
+// Warm start extrapolated in time: cur = 2 cur - prev (extrap) or unchanged,
+// and prev = the old cur.
+__global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restrict__ cur, double* __restrict__ prev,
+                                                       int extrap) {
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        const double c = cur[k];
+        if (extrap) cur[k] = 2.0 * c - prev[k];
+        prev[k] = c;
+    }
+}
+void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap) {
+    k_extrapolate<<<grid2(p), kBlock, 0, s>>>(p, cur, prev, extrap);
+}
+
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b) {
     k_proj_rhs<<<grid2(p), kBlock, 0, s>>>(p, b, grid_blocks(p));
 }

@@ -258,7 +258,7 @@ class ProjectionResult:
 class StepStats:
     velocity_iterations: tuple
     projection_iterations: tuple
-    graph_replayed: bool = False
+    planned: bool = False
 
 
 # ---------------------------------------------------------------------------
@@ -393,7 +393,7 @@ def integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work:
     _abi.check(ctx.lib, ctx.handle, st, "integrate_step!")
     state._device_modified()
     work.last_stats = StepStats(tuple(stats.velocity_iterations), tuple(stats.projection_iterations),
-                                bool(stats.graph_replayed))
+                                bool(stats.planned))
     return StepReport(rep.maximum_projection_constraint_linf, rep.maximum_growth_trace_error, rep.minimum_phi,
                       rep.maximum_phi, rep.minimum_resource, rep.minimum_sigma_eigenvalue,
                       rep.maximum_abs_sigma_determinant_error, int(rep.non_spd_cells))

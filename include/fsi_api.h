@@ -160,7 +160,7 @@ typedef struct {
 typedef struct {
     int32_t velocity_iterations[3];   /* stages 2..4 */
     int32_t projection_iterations[3]; /* stages 2..4 */
-    int32_t graph_replayed;           /* 1: the step ran as a replayed speculative graph */
+    int32_t planned;                  /* 1: the step ran with planned iteration counts and one read-back */
 } fsi_step_stats;
 
 typedef struct fsi_context fsi_context;

@@ -163,23 +163,25 @@ def test_periodic_filament_steps_match_oracle(method):
             assert v <= 1e-10, (k, f, v)
 
 
-def test_graph_replayed_steps_match_polling_steps(monkeypatch):
-    """Speculative steps (planned iteration counts, one captured graph per
-    buffer parity, replayed) give the same bits as the polling path."""
-    def run(graph):
-        monkeypatch.setenv("FSI_STEP_GRAPH", "1" if graph else "0")
+@pytest.mark.parametrize("mode", ["1", "2"])
+def test_planned_steps_match_polling_steps(monkeypatch, mode):
+    """Planned steps (each solve enqueued with its planned iteration count,
+    one read-back per step; mode 1 replays a captured graph, mode 2 launches
+    directly) give the same bits as the polling path."""
+    def run(m):
+        monkeypatch.setenv("FSI_STEP_GRAPH", m)
         case = setups.seeded_cavity(64, "acdi", seed=11, **TIGHT)
         ctx, work = _device(case)
         fsi.make_velocity_compatible(case.state, ctx, work)
-        replayed = []
+        planned = []
         for k in range(6):
             fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
-            replayed.append(work.last_stats.graph_replayed)
-        return {f: getattr(case.state, f).copy() for f in FIELDS}, replayed
+            planned.append(work.last_stats.planned)
+        return {f: getattr(case.state, f).copy() for f in FIELDS}, planned
 
-    a, ra = run(True)
-    b, rb = run(False)
-    print("replayed", ra, rb, flush=True)
+    a, ra = run(mode)
+    b, rb = run("0")
+    print("planned", ra, rb, flush=True)
     assert not any(rb) and sum(ra) >= 4
     for f in FIELDS:
         assert np.array_equal(a[f], b[f]), f
