@@ -103,6 +103,8 @@ struct fsi_workspace {
     // depend on it; `corr` keeps the reference's meaning for callers.
     double* corr_stage[3] = {};
     double* corr_prev[3] = {};  // the step before, for the linear extrapolation 2 q^{n-1} - q^{n-2}
+    // the same for the predictor solves: stage s of the previous two steps
+    double *vel_cur_u[3] = {}, *vel_cur_v[3] = {}, *vel_prev_u[3] = {}, *vel_prev_v[3] = {};
     int stage_warm = 0;         // completed steps whose stage solutions corr_stage / corr_prev hold (0, 1, 2)
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
@@ -570,7 +572,10 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->p_u[1], &w->p_v[0], &w->p_v[1], &w->q_u[0], &w->q_u[1], &w->q_v[0], &w->q_v[1],
                           &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w,
                           &w->corr_stage[0], &w->corr_stage[1], &w->corr_stage[2],
-                          &w->corr_prev[0],  &w->corr_prev[1],  &w->corr_prev[2]};
+                          &w->corr_prev[0],  &w->corr_prev[1],  &w->corr_prev[2],
+                          &w->vel_cur_u[0], &w->vel_cur_u[1], &w->vel_cur_u[2], &w->vel_cur_v[0], &w->vel_cur_v[1],
+                          &w->vel_cur_v[2], &w->vel_prev_u[0], &w->vel_prev_u[1], &w->vel_prev_u[2],
+                          &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -687,9 +692,19 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         launch_vel_copy(c->stream, p, b);
         return FSI_OK;
     }
+    // warm start: stage `slot` of the previous step (extrapolated in time
+    // once two steps are known) instead of u^n
+    const bool warm = slot >= 0 && c->stage_warm && w->stage_warm >= 1;
+    if (warm) {
+        launch_extrapolate2(c->stream, p, w->vel_cur_u[slot], w->vel_prev_u[slot], w->vel_cur_v[slot],
+                            w->vel_prev_v[slot], w->stage_warm >= 2);
+        b.Un = w->vel_cur_u[slot];
+        b.Vn = w->vel_cur_v[slot];
+    }
     launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter, slot);
     if (fixed > 0) {
         launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, c->num.velocity_maxiter + 1), slot);
+        if (slot >= 0 && c->stage_warm) launch_copy2(c->stream, p, w->Us, w->vel_cur_u[slot], w->Vs, w->vel_cur_v[slot]);
         return FSI_OK;
     }
     // kernel m tests the residual of iterate m and performs update m+1
@@ -705,6 +720,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         chunk = 4;
     }
     c->last_vel_iters[slot] = std::max(1, iter);
+    if (slot >= 0 && c->stage_warm) launch_copy2(c->stream, p, w->Us, w->vel_cur_u[slot], w->Vs, w->vel_cur_v[slot]);
     return FSI_OK;
 }
 

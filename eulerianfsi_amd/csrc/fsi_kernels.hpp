@@ -131,6 +131,8 @@ struct ProjBufs {
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
 void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap);
+void launch_extrapolate2(hipStream_t s, const Params& p, double* cu, double* pu, double* cv, double* pv, int extrap);
+void launch_copy2(hipStream_t s, const Params& p, const double* a1, double* b1, const double* a2, double* b2);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,

@@ -682,6 +682,36 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restr
         prev[k] = c;
     }
 }
+// the same for the two velocity components, and a two-field copy
+__global__ void __launch_bounds__(BLOCK) k_extrapolate2(Params p, double* __restrict__ cu, double* __restrict__ pu,
+                                                        double* __restrict__ cv, double* __restrict__ pv,
+                                                        int extrap) {
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        const double a = cu[k], b = cv[k];
+        if (extrap) {
+            cu[k] = 2.0 * a - pu[k];
+            cv[k] = 2.0 * b - pv[k];
+        }
+        pu[k] = a;
+        pv[k] = b;
+    }
+}
+__global__ void __launch_bounds__(BLOCK) k_copy2(Params p, const double* __restrict__ a1, double* __restrict__ b1,
+                                                 const double* __restrict__ a2, double* __restrict__ b2) {
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        b1[k] = a1[k];
+        b2[k] = a2[k];
+    }
+}
+void launch_extrapolate2(hipStream_t s, const Params& p, double* cu, double* pu, double* cv, double* pv,
+                         int extrap) {
+    k_extrapolate2<<<grid2(p), kBlock, 0, s>>>(p, cu, pu, cv, pv, extrap);
+}
+void launch_copy2(hipStream_t s, const Params& p, const double* a1, double* b1, const double* a2, double* b2) {
+    k_copy2<<<grid2(p), kBlock, 0, s>>>(p, a1, b1, a2, b2);
+}
 void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap) {
     k_extrapolate<<<grid2(p), kBlock, 0, s>>>(p, cur, prev, extrap);
 }
