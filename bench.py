@@ -76,6 +76,8 @@ def main(argv=None):
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--only-kernel", default=None, help="time one kernel standalone and exit (profiling runs)")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="precision of the main run")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -86,7 +88,8 @@ def main(argv=None):
     import eulerianfsi_amd as fsi
     from eulerianfsi_amd import _abi, setups
 
-    case = setups.seeded_cavity(args.N, args.method, args.seed, velocity_rtol=1e-8, projection_rtol=1e-10)
+    case = setups.seeded_cavity(args.N, args.method, args.seed, velocity_rtol=1e-8, projection_rtol=1e-10,
+                                precision=args.precision)
     ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
     work = fsi.allocate_imex_workspace(ctx)
     state = case.state
@@ -123,6 +126,7 @@ def main(argv=None):
     ms_per_step = 1e3 * elapsed / args.steps
     vel_mean = np.mean(np.array(vel_tot), axis=0)
     proj_mean = np.mean(np.array(proj_tot), axis=0)
+    final = {f: getattr(state, f).copy() for f in ("U", "V", "phi", "sigma_xx", "sigma_yy", "sigma_xy")}
     print(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step, vel iters {vel_mean}, proj iters {proj_mean}, "
           f"report {rep}", file=sys.stderr, flush=True)
 
@@ -153,6 +157,35 @@ def main(argv=None):
                   "frac_of_peak": b_step / (ms_per_step * 1e-3) / HBM_PEAK,
                   "textbook_GB": b_text / 1e9, "textbook_frac_of_peak": b_text / (ms_per_step * 1e-3) / HBM_PEAK}
 
+    # mixed precision: the same run (same seed, W + K steps) with the FP32
+    # V-cycle preconditioner; error of every field against this FP64 run
+    mixed = None
+    if not args.no_mixed and args.precision == "fp64":
+        mcase = setups.seeded_cavity(args.N, args.method, args.seed, velocity_rtol=1e-8, projection_rtol=1e-10,
+                                     precision="mixed")
+        mctx = fsi.build_imex_context(mcase.metrics, mcase.bcs, mcase.models, mcase.numerics)
+        mwork = fsi.allocate_imex_workspace(mctx)
+        mstate, mt = mcase.state, 0.0
+        fsi.make_velocity_compatible(mstate, mctx, mwork)
+        for _ in range(args.warmup):
+            fsi.integrate_step(mstate, mt, dt, mctx, mwork)
+            mt += dt
+        m0 = time.perf_counter()
+        miters = []
+        for _ in range(args.steps):
+            fsi.integrate_step(mstate, mt, dt, mctx, mwork)
+            mt += dt
+            miters.append(mwork.last_stats.projection_iterations)
+        m_el = (time.perf_counter() - m0) / args.steps
+        errs = {f: float(np.linalg.norm(getattr(mstate, f) - final[f]) / np.linalg.norm(final[f])) for f in final}
+        mixed = {"value": 1.0 / m_el, "ms_per_step": 1e3 * m_el, "speedup_vs_fp64": (1.0 / m_el) / steps_per_s,
+                 "rel_l2_vs_fp64": errs, "projection_iterations": np.mean(np.array(miters), axis=0).tolist(),
+                 "scheme": "FP32 V-cycle preconditioner (levels, LDS tiles, arithmetic) inside the FP64 PCG; "
+                           "FP64 residuals, dot products, stencils and state"}
+        print(f"mixed: {1e3 * m_el:.3f} ms/step, speedup {mixed['speedup_vs_fp64']:.3f}, errors {errs}",
+              file=sys.stderr, flush=True)
+        del mwork, mctx
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle.oracle import Oracle, state_dict
@@ -176,7 +209,7 @@ def main(argv=None):
                                f"FP64, dt={dt:.4g}, seed {args.seed}, velocity rtol 1e-8, projection rtol 1e-10",
                    "N": args.N, "method": args.method, "cell_updates_per_s": steps_per_s * args.N ** 2,
                    "velocity_iterations": vel_mean.tolist(), "projection_iterations": proj_mean.tolist(),
-                   "mixed_precision": None, "adjoint_grads_per_s": None},
+                   "precision": args.precision, "mixed_precision": mixed, "adjoint_grads_per_s": None},
         "roofline": roofline, "whole_step": whole_step, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -59,6 +59,7 @@ struct fsi_context {
     std::vector<MgGeom> geo;
     CoarsePlan cplan{};         // single-workgroup coarse solve of geo.back()
     double* co_dense = nullptr;  // its dense bottom inverse
+    float* co_dense_f = nullptr; // the same in FP32 (mixed precision)
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
     // Planned steps: each solve enqueued with a planned iteration count and no
@@ -274,8 +275,8 @@ static fsi_status validate(fsi_context* c) {
     if (n.velocity_maxiter <= 0) return fail(c, FSI_E_ARGUMENT, "Linear-solver maxiter must be positive.");
     if (!(n.stage1_atol >= 0.0) || !(n.stage1_rtol >= 0.0))
         return fail(c, FSI_E_ARGUMENT, "stage-1 constraint tolerances must be non-negative.");
-    if (n.precision != FSI_PRECISION_FP64)
-        return fail(c, FSI_E_UNSUPPORTED, "only the FP64 precision mode is available on the device.");
+    if (n.precision != FSI_PRECISION_FP64 && n.precision != FSI_PRECISION_MIXED)
+        return fail(c, FSI_E_ARGUMENT, "precision must be FSI_PRECISION_FP64 or FSI_PRECISION_MIXED.");
     return FSI_OK;
 }
 
@@ -456,6 +457,12 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
                                        hipMemcpyHostToDevice, c->stream));
             HIPCHECK(c, hipStreamSynchronize(c->stream));
             c->cplan.dense = c->co_dense;
+            std::vector<float> df(dense.begin(), dense.end());
+            HIPCHECK(c, hipMalloc((void**)&c->co_dense_f, df.size() * sizeof(float)));
+            HIPCHECK(c, hipMemcpyAsync(c->co_dense_f, df.data(), df.size() * sizeof(float), hipMemcpyHostToDevice,
+                                       c->stream));
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+            c->cplan.dense_f = c->co_dense_f;
         }
     }
     HIPCHECK(c, dalloc(c, &c->partials, partials_size(p)));
@@ -474,6 +481,7 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
+    if (c->co_dense_f) (void)hipFree(c->co_dense_f);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -598,6 +606,16 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
         } else {
             HIPCHECK(c, A(&L.t, n));
             HIPCHECK(c, A(&L.b, n));
+        }
+        L.xf = L.tf = L.bf = nullptr;
+        if (c->num.precision == FSI_PRECISION_MIXED && c->cplan.levels > 0 && c->geo.size() > 1) {
+            // FP32 levels of the mixed-precision V-cycle (the finest keeps r and z in FP64)
+            float** fl[3] = {&L.xf, &L.tf, &L.bf};
+            for (int q = 0; q < (l == 0 ? 1 : 3); ++q) {
+                HIPCHECK(c, hipMalloc((void**)fl[q], n * sizeof(float)));
+                HIPCHECK(c, hipMemsetAsync(*fl[q], 0, n * sizeof(float), c->stream));
+                w->owned.push_back(reinterpret_cast<double*>(*fl[q]));
+            }
         }
     }
     HIPCHECK(c, hipStreamSynchronize(c->stream));

@@ -51,6 +51,7 @@ struct MgLevel {
     double* t;  // pong
     double* b;  // right-hand side
     double* r;  // residual
+    float *xf, *tf, *bf;  // FP32 x, t, b of the mixed-precision V-cycle (null in FP64 mode)
 };
 
 // Level geometry passed to the multigrid kernels.
@@ -68,10 +69,11 @@ struct MgGeom {
 // of at most CO_DENSE_MAX cells solved by a dense (pseudo-)inverse.
 constexpr int CO_DENSE_MAX = 64;
 struct CoarsePlan {
-    int levels;            // levels in the workgroup, top included
-    int dense_n;           // cells of the bottom when solved densely, else 0
-    size_t lds_bytes;      // dynamic LDS of the launch
-    const double* dense;   // device copy of the dense inverse (dense_n^2)
+    int levels;             // levels in the workgroup, top included
+    int dense_n;            // cells of the bottom when solved densely, else 0
+    size_t lds_cells;       // dynamic LDS of the launch, in values
+    const double* dense;    // device copy of the dense inverse (dense_n^2)
+    const float* dense_f;   // the same in FP32 (mixed-precision V-cycle)
 };
 
 // ---- physics (fsi_physics.hip) ----
@@ -148,15 +150,27 @@ int grid_blocks(const Params& p);
 // The finest down-kernel (st != nullptr) first sums the r.r partials of the
 // previous PCG kernel and ends the solve when converged; the finest up-kernel
 // (rz != nullptr) writes the r.z block partials.
+// FP64 levels, and the mixed-precision variants (FP32 levels; the finest
+// level reads the FP64 residual and writes the FP64 z).
 void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot);
+void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, float* x2, float* bc,
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot);
+void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const float* b, float* x2, float* bc,
                     const int* done, Status* st, const double* rr, int nb, int m, int slot);
 void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
                   const double* ec, double* xout, const int* done, double* rz, int nb);
+void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const float* x2, const double* b,
+                  const float* ec, double* xout, const int* done, double* rz, int nb);
+void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const float* x2, const float* b,
+                  const float* ec, float* xout, const int* done, double* rz, int nb);
 // Plan of the single-workgroup coarse solve for a top level (false when the
 // level is too large); `dense` receives the bottom's inverse, row-major.
 bool coarse_plan(const MgGeom& top, int max_cells, CoarsePlan& cp, std::vector<double>& dense);
 int coarse_max_cells();
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
+                      const int* done);
+void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const float* b, float* t,
                       const int* done);
 size_t partials_size(const Params& p);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,

@@ -14,6 +14,10 @@
 // (k_mg_coarse), whose bottom level is solved with a precomputed dense
 // (pseudo-)inverse.
 //
+// The kernels are templated on the stored and computed floating type: FP64
+// throughout, or (mixed precision) FP32 levels, tiles and arithmetic with the
+// finest level reading the FP64 PCG residual and writing the FP64 z.
+//
 // Tiles hold 0 at points outside a non-periodic wall, so -A x at any cell is
 // d x - (x_W + x_E)/hx^2 - (x_S + x_N)/hy^2 with the wall's contribution in
 // the diagonal d alone; across a periodic side the tile holds the wrapped
@@ -21,6 +25,7 @@
 // columns.
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "fsi_kernels.hpp"
@@ -36,13 +41,15 @@ constexpr int MG_THREADS = 256;
                            ++it_) if (const int k = tid + it_ * MG_THREADS; k < (N))
 
 // diagonal of -A at a cell and its inverse
-struct Diag {
-    double d, id;
+template <typename T>
+struct DiagT {
+    T d, id;
 };
-__device__ __forceinline__ Diag mg_diag(const MgGeom& g, int gi, int gj) {
+template <typename T>
+__device__ __forceinline__ DiagT<T> mg_diag(const MgGeom& g, int gi, int gj) {
     const bool bx = !g.px && (gi == 0 || gi == g.nx - 1);
     const bool by = !g.py && (gj == 0 || gj == g.ny - 1);
-    if (!(bx || by)) return {g.d0, g.id0};
+    if (!(bx || by)) return {(T)g.d0, (T)g.id0};
     double cx = 2.0, cy = 2.0;
     if (!g.px) {
         if (gi == 0) cx += g.dl ? 1.0 : -1.0;
@@ -53,7 +60,7 @@ __device__ __forceinline__ Diag mg_diag(const MgGeom& g, int gi, int gj) {
         if (gj == g.ny - 1) cy += g.dt ? 1.0 : -1.0;
     }
     const double d = g.ihx2 * cx + g.ihy2 * cy;
-    return {d, 1.0 / d};
+    return {(T)d, (T)(1.0 / d)};
 }
 
 // tile coordinate -> global index, wrapped across a periodic side; false
@@ -86,9 +93,10 @@ struct Where {
         const bool oky = wrap_idx(g.ny, g.py, uy, gj);
         return okx && oky;
     }
-    __device__ static __forceinline__ Diag diag(const MgGeom& g, int gi, int gj) {
-        if (IN) return {g.d0, g.id0};
-        return mg_diag(g, gi, gj);
+    template <typename T>
+    __device__ static __forceinline__ DiagT<T> diag(const MgGeom& g, int gi, int gj) {
+        if (IN) return {(T)g.d0, (T)g.id0};
+        return mg_diag<T>(g, gi, gj);
     }
 };
 // is the fine region [x0, x1] x [y0, y1] strictly inside (see Where<true>)?
@@ -103,14 +111,14 @@ __device__ __forceinline__ bool inside(const MgGeom& g, int x0, int x1, int y0, 
 // to back, so a thread has all of its loads in flight at once.  Points
 // outside a non-periodic wall read as 0; val[it] belongs to tile point
 // tid + it * MG_THREADS (clamped to the last point).
-template <int N, int W>
+template <int N, int W, typename TS>
 struct TileGather {
     static constexpr int NIT = (N + MG_THREADS - 1) / MG_THREADS;
-    double val[NIT];
+    TS val[NIT];
     int gi[NIT], gj[NIT];
     bool in[NIT];
     template <bool IN>
-    __device__ __forceinline__ void load(const double* __restrict__ a, const MgGeom& g, int x0, int y0, int tid) {
+    __device__ __forceinline__ void load(const TS* __restrict__ a, const MgGeom& g, int x0, int y0, int tid) {
         int idx[NIT];
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
@@ -126,26 +134,27 @@ struct TileGather {
         for (int it = 0; it < NIT; ++it) val[it] = a[idx[it]];
         if (!IN) {
 #pragma unroll
-            for (int it = 0; it < NIT; ++it) val[it] = in[it] ? val[it] : 0.0;
+            for (int it = 0; it < NIT; ++it) val[it] = in[it] ? val[it] : TS(0);
         }
     }
 };
 
 // -A x at tile position c of an array with row length W (zero outside walls)
-__device__ __forceinline__ double tile_negA(const MgGeom& g, const double* a, int c, int W, double d) {
-    return d * a[c] - g.ihx2 * (a[c - 1] + a[c + 1]) - g.ihy2 * (a[c - W] + a[c + W]);
+template <typename T>
+__device__ __forceinline__ T tile_negA(const MgGeom& g, const T* a, int c, int W, T d) {
+    return d * a[c] - (T)g.ihx2 * (a[c - 1] + a[c + 1]) - (T)g.ihy2 * (a[c - W] + a[c + W]);
 }
 
 // 1-D restriction weights of fine 2I and 2I+1 onto coarse I (R = P^T/4 up to
 // the factor 1/4): 3/4 inside; at a non-periodic wall the fine cell's ghost
 // is the mirrored parent (Neumann, weight 1) or its negative (Dirichlet, 1/2).
-template <bool IN>
-__device__ __forceinline__ void rweights(int I, int nf, int per, int dlo, int dhi, double& w0, double& w1) {
-    w0 = 0.75;
-    w1 = 0.75;
+template <bool IN, typename T>
+__device__ __forceinline__ void rweights(int I, int nf, int per, int dlo, int dhi, T& w0, T& w1) {
+    w0 = T(0.75);
+    w1 = T(0.75);
     if (!IN && !per) {
-        if (2 * I == 0) w0 = dlo ? 0.5 : 1.0;
-        if (2 * I + 1 == nf - 1) w1 = dhi ? 0.5 : 1.0;
+        if (2 * I == 0) w0 = dlo ? T(0.5) : T(1);
+        if (2 * I + 1 == nf - 1) w1 = dhi ? T(0.5) : T(1);
     }
 }
 
@@ -161,22 +170,24 @@ struct DownTile {
     static_assert(D1H * CTX <= D2H * D2W, "row sums fit the x2 buffer");
 };
 
-template <int FX, int FY, bool IN>
-__device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, const double* __restrict__ b,
-                                          double* __restrict__ x2out, double* __restrict__ bc, int fi0, int fj0,
-                                          int tid, double* sb, double* sx, double* s2) {
-    using T = DownTile<FX, FY>;
-    constexpr int D3W = T::D3W, D3H = T::D3H, D2W = T::D2W, D2H = T::D2H, D1W = T::D1W, D1H = T::D1H;
-    constexpr int CTX = T::CTX, CTY = T::CTY;
+template <int FX, int FY, bool IN, typename TB, typename T>
+__device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, const TB* __restrict__ b,
+                                          T* __restrict__ x2out, T* __restrict__ bc, int fi0, int fj0, int tid, T* sb,
+                                          T* sx, T* s2) {
+    using DT = DownTile<FX, FY>;
+    constexpr int D3W = DT::D3W, D3H = DT::D3H, D2W = DT::D2W, D2H = DT::D2H, D1W = DT::D1W, D1H = DT::D1H;
+    constexpr int CTX = DT::CTX, CTY = DT::CTY;
+    const T om1 = (T)gf.om1, om2 = (T)gf.om2;
     {  // b and x1 = w1 D^-1 b on halo 3
-        TileGather<D3H * D3W, D3W> tb;
+        TileGather<D3H * D3W, D3W, TB> tb;
         tb.template load<IN>(b, gf, fi0 - 3, fj0 - 3, tid);
 #pragma unroll
         for (int it = 0; it < tb.NIT; ++it) {
             const int k = tid + it * MG_THREADS;
             if (k >= D3H * D3W) break;
-            sb[k] = tb.val[it];
-            sx[k] = tb.in[it] ? gf.om1 * Where<IN>::diag(gf, tb.gi[it], tb.gj[it]).id * tb.val[it] : 0.0;
+            const T bv = (T)tb.val[it];
+            sb[k] = bv;
+            sx[k] = tb.in[it] ? om1 * Where<IN>::template diag<T>(gf, tb.gi[it], tb.gj[it]).id * bv : T(0);
         }
     }
     __syncthreads();
@@ -184,11 +195,11 @@ __device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, co
     TILE_LOOP(D2H * D2W) {
         const int li = k % D2W, lj = k / D2W;
         int gi, gj;
-        double v = 0.0;
+        T v = T(0);
         if (Where<IN>::locate(gf, fi0 + li - 2, fj0 + lj - 2, gi, gj)) {
-            const Diag D = Where<IN>::diag(gf, gi, gj);
+            const DiagT<T> D = Where<IN>::template diag<T>(gf, gi, gj);
             const int c = (lj + 1) * D3W + li + 1;
-            v = sx[c] + gf.om2 * D.id * (sb[c] - tile_negA(gf, sx, c, D3W, D.d));
+            v = sx[c] + om2 * D.id * (sb[c] - tile_negA<T>(gf, sx, c, D3W, D.d));
         }
         s2[k] = v;
     }
@@ -198,11 +209,11 @@ __device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, co
         const int li = k % D1W, lj = k / D1W;
         const int ui = fi0 + li - 1, uj = fj0 + lj - 1;
         int gi, gj;
-        double v = 0.0;
+        T v = T(0);
         if (Where<IN>::locate(gf, ui, uj, gi, gj)) {
-            const Diag D = Where<IN>::diag(gf, gi, gj);
+            const DiagT<T> D = Where<IN>::template diag<T>(gf, gi, gj);
             const int c = (lj + 1) * D2W + li + 1;
-            v = sb[(lj + 2) * D3W + li + 2] - tile_negA(gf, s2, c, D2W, D.d);
+            v = sb[(lj + 2) * D3W + li + 2] - tile_negA<T>(gf, s2, c, D2W, D.d);
             if (li >= 1 && li <= FX && lj >= 1 && lj <= FY && (IN || (ui < gf.nx && uj < gf.ny)))
                 x2out[(size_t)uj * gf.P + ui] = s2[c];
         }
@@ -210,36 +221,36 @@ __device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, co
     }
     __syncthreads();
     // restriction, x then y: row sums of r over fine 2I-1 .. 2I+2
-    double* sh = s2;
+    T* sh = s2;
+    const T q = T(0.25);
     TILE_LOOP(D1H * CTX) {
         const int ci = k % CTX, lj = k / CTX;
-        double w0, w1;
-        rweights<IN>(fi0 / 2 + ci, gf.nx, gf.px, gf.dl, gf.dr, w0, w1);
-        const double* row = sx + lj * D1W + 2 * ci;  // fine 2I-1 at halo-1 index 2 ci
-        sh[lj * CTX + ci] = 0.25 * row[0] + w0 * row[1] + w1 * row[2] + 0.25 * row[3];
+        T w0, w1;
+        rweights<IN, T>(fi0 / 2 + ci, gf.nx, gf.px, gf.dl, gf.dr, w0, w1);
+        const T* row = sx + lj * D1W + 2 * ci;  // fine 2I-1 at halo-1 index 2 ci
+        sh[lj * CTX + ci] = q * row[0] + w0 * row[1] + w1 * row[2] + q * row[3];
     }
     __syncthreads();
     TILE_LOOP(CTY * CTX) {
         const int ci = k % CTX, cj = k / CTX;
         const int I = fi0 / 2 + ci, J = fj0 / 2 + cj;
         if (!IN && (I >= gc.nx || J >= gc.ny)) continue;
-        double w0, w1;
-        rweights<IN>(J, gf.ny, gf.py, gf.db, gf.dt, w0, w1);
-        const double* col = sh + 2 * cj * CTX + ci;
-        bc[(size_t)J * gc.P + I] =
-            0.25 * (0.25 * col[0] + w0 * col[CTX] + w1 * col[2 * CTX] + 0.25 * col[3 * CTX]);
+        T w0, w1;
+        rweights<IN, T>(J, gf.ny, gf.py, gf.db, gf.dt, w0, w1);
+        const T* col = sh + 2 * cj * CTX + ci;
+        bc[(size_t)J * gc.P + I] = q * (q * col[0] + w0 * col[CTX] + w1 * col[2 * CTX] + q * col[3 * CTX]);
     }
 }
 
 // Tiles are FX x FY fine cells (64 x 16 on the finest level, whose partials
 // must line up with the PCG kernels' grid; smaller on coarser levels so that
 // they still spread over many CUs).
-template <int FX, int FY>
-__global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, const double* __restrict__ b,
-                                                        double* __restrict__ x2out, double* __restrict__ bc,
-                                                        const int* done, Status* st, const double* __restrict__ rr,
-                                                        int nb, int m, int slot) {
-    using T = DownTile<FX, FY>;
+template <int FX, int FY, typename TB, typename T>
+__global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, const TB* __restrict__ b,
+                                                        T* __restrict__ x2out, T* __restrict__ bc, const int* done,
+                                                        Status* st, const double* __restrict__ rr, int nb, int m,
+                                                        int slot) {
+    using DT = DownTile<FX, FY>;
     if (*done) return;
     if (st) {  // finest level: convergence test on |r|^2 of the previous PCG kernel
         double tot[1];
@@ -255,14 +266,14 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
             return;
         }
     }
-    __shared__ double sb[T::D3H * T::D3W];  // b on halo 3
-    __shared__ double sx[T::D3H * T::D3W];  // x1 on halo 3, then r on halo 1
-    __shared__ double s2[T::D2H * T::D2W];  // x2 on halo 2, then row sums of the restriction
+    __shared__ T sb[DT::D3H * DT::D3W];  // b on halo 3
+    __shared__ T sx[DT::D3H * DT::D3W];  // x1 on halo 3, then r on halo 1
+    __shared__ T s2[DT::D2H * DT::D2W];  // x2 on halo 2, then row sums of the restriction
     const int fi0 = tile_id().x * FX, fj0 = tile_id().y * FY;
     if (inside(gf, fi0 - 3, fi0 + FX + 2, fj0 - 3, fj0 + FY + 2))
-        down_tile<FX, FY, true>(gf, gc, b, x2out, bc, fi0, fj0, threadIdx.x, sb, sx, s2);
+        down_tile<FX, FY, true, TB, T>(gf, gc, b, x2out, bc, fi0, fj0, threadIdx.x, sb, sx, s2);
     else
-        down_tile<FX, FY, false>(gf, gc, b, x2out, bc, fi0, fj0, threadIdx.x, sb, sx, s2);
+        down_tile<FX, FY, false, TB, T>(gf, gc, b, x2out, bc, fi0, fj0, threadIdx.x, sb, sx, s2);
 }
 
 // ---------------------------------------------------------------------------
@@ -273,15 +284,14 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
 // bilinear weights of the parent (w0) and the nearer coarse neighbour (1/4)
 // of fine f; the tile stores 0 beyond a non-periodic wall, where the ghost
 // is the mirrored parent (Neumann, w0 = 1) or its negative (Dirichlet, 1/2).
-template <bool IN>
-__device__ __forceinline__ void pweights(int f, int nf, int per, int dlo, int dhi, int& parent, int& other,
-                                         double& w0) {
+template <bool IN, typename T>
+__device__ __forceinline__ void pweights(int f, int nf, int per, int dlo, int dhi, int& parent, int& other, T& w0) {
     parent = f >> 1;  // floor for negative tile coordinates as well
     other = (f & 1) ? parent + 1 : parent - 1;
-    w0 = 0.75;
+    w0 = T(0.75);
     if (!IN && !per) {
-        if (f == 0) w0 = dlo ? 0.5 : 1.0;
-        if (f == nf - 1) w0 = dhi ? 0.5 : 1.0;
+        if (f == 0) w0 = dlo ? T(0.5) : T(1);
+        if (f == nf - 1) w0 = dhi ? T(0.5) : T(1);
     }
 }
 
@@ -292,18 +302,21 @@ struct UpTile {
     static constexpr int CW = FX / 2 + 4, CH = FY / 2 + 4;   // coarse values around the tile
 };
 
-template <int FX, int FY, bool IN>
-__device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, const double* __restrict__ x2,
-                                          const double* __restrict__ b, const double* __restrict__ ec,
-                                          double* __restrict__ xout, int fi0, int fj0, int tid, double* sx3,
-                                          double* sx4, double* sbb, double* sec) {
-    using T = UpTile<FX, FY>;
-    constexpr int D2W = T::D2W, D2H = T::D2H, D1W = T::D1W, D1H = T::D1H, CW = T::CW, CH = T::CH;
+// sbc (TB, the tile's own cells) keeps b at its stored precision for r.z
+// when the tile works in a lower one.
+template <int FX, int FY, bool IN, typename TB, typename T, typename TO>
+__device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, const T* __restrict__ x2,
+                                          const TB* __restrict__ b, const T* __restrict__ ec, TO* __restrict__ xout,
+                                          int fi0, int fj0, int tid, T* sx3, T* sx4, T* sbb, T* sec, TB* sbc) {
+    using UT = UpTile<FX, FY>;
+    constexpr int D2W = UT::D2W, D2H = UT::D2H, D1W = UT::D1W, D1H = UT::D1H, CW = UT::CW, CH = UT::CH;
+    constexpr bool kSplit = !std::is_same<TB, T>::value;
+    const T om1 = (T)gf.om1, om2 = (T)gf.om2, q = T(0.25);
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
     {
-        TileGather<CH * CW, CW> tc;
-        TileGather<D1H * D1W, D1W> tbb;
-        TileGather<D2H * D2W, D2W> tx2;
+        TileGather<CH * CW, CW, T> tc;
+        TileGather<D1H * D1W, D1W, TB> tbb;
+        TileGather<D2H * D2W, D2W, T> tx2;
         tc.template load<IN>(ec, gc, ci0, cj0, tid);
         tbb.template load<IN>(b, gf, fi0 - 1, fj0 - 1, tid);
         tx2.template load<IN>(x2, gf, fi0 - 2, fj0 - 2, tid);
@@ -315,7 +328,13 @@ __device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, co
 #pragma unroll
         for (int it = 0; it < tbb.NIT; ++it) {
             const int k = tid + it * MG_THREADS;
-            if (k < D1H * D1W) sbb[k] = tbb.val[it];
+            if (k < D1H * D1W) {
+                sbb[k] = (T)tbb.val[it];
+                if (kSplit) {
+                    const int li = k % D1W, lj = k / D1W;
+                    if (li >= 1 && li <= FX && lj >= 1 && lj <= FY) sbc[(lj - 1) * FX + li - 1] = tbb.val[it];
+                }
+            }
         }
         __syncthreads();
         // x3 = x2 + P e_c on halo 2
@@ -325,17 +344,16 @@ __device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, co
             if (k >= D2H * D2W) break;
             const int li = k % D2W, lj = k / D2W;
             const int fx = fi0 + li - 2, fy = fj0 + lj - 2;
-            double v = 0.0;
+            T v = T(0);
             if (tx2.in[it]) {
                 int px0, px1, py0, py1;
-                double wx0, wy0;
-                pweights<IN>(fx, gf.nx, gf.px, gf.dl, gf.dr, px0, px1, wx0);
-                pweights<IN>(fy, gf.ny, gf.py, gf.db, gf.dt, py0, py1, wy0);
-                const double* r0 = sec + (py0 - cj0) * CW;
-                const double* r1 = sec + (py1 - cj0) * CW;
+                T wx0, wy0;
+                pweights<IN, T>(fx, gf.nx, gf.px, gf.dl, gf.dr, px0, px1, wx0);
+                pweights<IN, T>(fy, gf.ny, gf.py, gf.db, gf.dt, py0, py1, wy0);
+                const T* r0 = sec + (py0 - cj0) * CW;
+                const T* r1 = sec + (py1 - cj0) * CW;
                 const int a0 = px0 - ci0, a1 = px1 - ci0;
-                const double e =
-                    wy0 * (wx0 * r0[a0] + 0.25 * r0[a1]) + 0.25 * (wx0 * r1[a0] + 0.25 * r1[a1]);
+                const T e = wy0 * (wx0 * r0[a0] + q * r0[a1]) + q * (wx0 * r1[a0] + q * r1[a1]);
                 v = tx2.val[it] + e;
             }
             sx3[k] = v;
@@ -346,11 +364,11 @@ __device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, co
     TILE_LOOP(D1H * D1W) {
         const int li = k % D1W, lj = k / D1W;
         int gi, gj;
-        double v = 0.0;
+        T v = T(0);
         if (Where<IN>::locate(gf, fi0 + li - 1, fj0 + lj - 1, gi, gj)) {
-            const Diag D = Where<IN>::diag(gf, gi, gj);
+            const DiagT<T> D = Where<IN>::template diag<T>(gf, gi, gj);
             const int c = (lj + 1) * D2W + li + 1;
-            v = sx3[c] + gf.om2 * D.id * (sbb[k] - tile_negA(gf, sx3, c, D2W, D.d));
+            v = sx3[c] + om2 * D.id * (sbb[k] - tile_negA<T>(gf, sx3, c, D2W, D.d));
         }
         sx4[k] = v;
     }
@@ -361,33 +379,36 @@ __device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, co
         const int li = k % FX, lj = k / FX;
         const int gi = fi0 + li, gj = fj0 + lj;
         if (!IN && (gi >= gf.nx || gj >= gf.ny)) continue;
-        const Diag D = Where<IN>::diag(gf, gi, gj);
+        const DiagT<T> D = Where<IN>::template diag<T>(gf, gi, gj);
         const int c = (lj + 1) * D1W + li + 1;
-        const double bv = sbb[c];
-        const double v = sx4[c] + gf.om1 * D.id * (bv - tile_negA(gf, sx4, c, D1W, D.d));
-        xout[(size_t)gj * gf.P + gi] = v;
-        acc += bv * v;
+        const T bv = sbb[c];
+        const T v = sx4[c] + om1 * D.id * (bv - tile_negA<T>(gf, sx4, c, D1W, D.d));
+        xout[(size_t)gj * gf.P + gi] = (TO)v;
+        const double bd = kSplit ? (double)sbc[lj * FX + li] : (double)bv;
+        acc += bd * (double)(TO)v;
     }
     return acc;
 }
 
-template <int FX, int FY>
-__global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const double* __restrict__ x2,
-                                                      const double* __restrict__ b, const double* __restrict__ ec,
-                                                      double* __restrict__ xout, const int* done, double* rz,
-                                                      int nb) {
-    using T = UpTile<FX, FY>;
+template <int FX, int FY, typename TB, typename T, typename TO>
+__global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const T* __restrict__ x2,
+                                                      const TB* __restrict__ b, const T* __restrict__ ec,
+                                                      TO* __restrict__ xout, const int* done, double* rz, int nb) {
+    using UT = UpTile<FX, FY>;
+    constexpr bool kSplit = !std::is_same<TB, T>::value;
     if (*done) return;
-    __shared__ double sx3[T::D2H * T::D2W];  // x3 on halo 2
-    __shared__ double sx4[T::D1H * T::D1W];  // x4 on halo 1
-    __shared__ double sbb[T::D1H * T::D1W];  // b on halo 1
-    __shared__ double sec[T::CH * T::CW];    // e_c around the tile
+    __shared__ T sx3[UT::D2H * UT::D2W];         // x3 on halo 2
+    __shared__ T sx4[UT::D1H * UT::D1W];         // x4 on halo 1
+    __shared__ T sbb[UT::D1H * UT::D1W];         // b on halo 1
+    __shared__ T sec[UT::CH * UT::CW];           // e_c around the tile
+    __shared__ TB sbc[kSplit ? FX * FY : 1];     // b of the tile's cells at its stored precision
     const int fi0 = tile_id().x * FX, fj0 = tile_id().y * FY;
     const int ci0 = fi0 / 2 - 2, cj0 = fj0 / 2 - 2;
     const bool in = inside(gf, fi0 - 2, fi0 + FX + 1, fj0 - 2, fj0 + FY + 1) && ci0 >= 0 && cj0 >= 0 &&
-                    ci0 + T::CW <= gc.nx && cj0 + T::CH <= gc.ny;
-    const double acc = in ? up_tile<FX, FY, true>(gf, gc, x2, b, ec, xout, fi0, fj0, threadIdx.x, sx3, sx4, sbb, sec)
-                          : up_tile<FX, FY, false>(gf, gc, x2, b, ec, xout, fi0, fj0, threadIdx.x, sx3, sx4, sbb, sec);
+                    ci0 + UT::CW <= gc.nx && cj0 + UT::CH <= gc.ny;
+    const double acc =
+        in ? up_tile<FX, FY, true, TB, T, TO>(gf, gc, x2, b, ec, xout, fi0, fj0, threadIdx.x, sx3, sx4, sbb, sec, sbc)
+           : up_tile<FX, FY, false, TB, T, TO>(gf, gc, x2, b, ec, xout, fi0, fj0, threadIdx.x, sx3, sx4, sbb, sec, sbc);
     if (!rz) return;
     double v[1] = {acc};
     put_partials<1>(v, rz, nb);
@@ -417,25 +438,30 @@ __device__ __forceinline__ MgGeom co_geom(const MgGeom& g0, int l) {
 }
 
 // -A a at (i, j) of a compact level array, with its diagonal
-__device__ __forceinline__ double co_negA(const MgGeom& g, const double* a, int i, int j, double& d) {
+template <typename T>
+__device__ __forceinline__ T co_negA(const MgGeom& g, const T* a, int i, int j, T& d) {
     const int k = j * g.nx + i;
-    double sx = 0.0, sy = 0.0, cx = 0.0, cy = 0.0;
-    if (i > 0) { sx += a[k - 1]; cx += 1.0; }
-    else if (g.px) { sx += a[k + g.nx - 1]; cx += 1.0; }
-    else if (g.dl) cx += 2.0;
-    if (i < g.nx - 1) { sx += a[k + 1]; cx += 1.0; }
-    else if (g.px) { sx += a[k - g.nx + 1]; cx += 1.0; }
-    else if (g.dr) cx += 2.0;
-    if (j > 0) { sy += a[k - g.nx]; cy += 1.0; }
-    else if (g.py) { sy += a[k + (g.ny - 1) * g.nx]; cy += 1.0; }
-    else if (g.db) cy += 2.0;
-    if (j < g.ny - 1) { sy += a[k + g.nx]; cy += 1.0; }
-    else if (g.py) { sy += a[k - (g.ny - 1) * g.nx]; cy += 1.0; }
-    else if (g.dt) cy += 2.0;
-    d = g.ihx2 * cx + g.ihy2 * cy;
-    return d * a[k] - g.ihx2 * sx - g.ihy2 * sy;
+    T sx = T(0), sy = T(0), cx = T(0), cy = T(0);
+    if (i > 0) { sx += a[k - 1]; cx += T(1); }
+    else if (g.px) { sx += a[k + g.nx - 1]; cx += T(1); }
+    else if (g.dl) cx += T(2);
+    if (i < g.nx - 1) { sx += a[k + 1]; cx += T(1); }
+    else if (g.px) { sx += a[k - g.nx + 1]; cx += T(1); }
+    else if (g.dr) cx += T(2);
+    if (j > 0) { sy += a[k - g.nx]; cy += T(1); }
+    else if (g.py) { sy += a[k + (g.ny - 1) * g.nx]; cy += T(1); }
+    else if (g.db) cy += T(2);
+    if (j < g.ny - 1) { sy += a[k + g.nx]; cy += T(1); }
+    else if (g.py) { sy += a[k - (g.ny - 1) * g.nx]; cy += T(1); }
+    else if (g.dt) cy += T(2);
+    const T ihx2 = (T)g.ihx2, ihy2 = (T)g.ihy2;
+    d = ihx2 * cx + ihy2 * cy;
+    return d * a[k] - ihx2 * sx - ihy2 * sy;
 }
-__device__ __forceinline__ double co_id(const MgGeom& g, double d) { return d == g.d0 ? g.id0 : 1.0 / d; }
+template <typename T>
+__device__ __forceinline__ T co_id(const MgGeom& g, T d) {
+    return d == (T)g.d0 ? (T)g.id0 : T(1) / d;
+}
 constexpr double CO_OMEGA = 0.8;  // damped-Jacobi weight of the fallback bottom sweeps
 
 // restriction weight of fine f = 2I + dx onto coarse I, with f wrapped
@@ -455,19 +481,19 @@ __device__ __forceinline__ double co_rw(int I, int dx, int nf, int per, int dlo,
 }
 
 // prolongation of fine f: parent I0 (w0), nearer neighbour I1 (w1), wrapped
-__device__ __forceinline__ void co_pw(int f, int nf, int per, int dlo, int dhi, int& I0, double& w0, int& I1,
-                                      double& w1) {
+template <typename T>
+__device__ __forceinline__ void co_pw(int f, int nf, int per, int dlo, int dhi, int& I0, T& w0, int& I1, T& w1) {
     const int nc = nf >> 1;
     I0 = f >> 1;
     I1 = (f & 1) ? I0 + 1 : I0 - 1;
-    w0 = 0.75;
-    w1 = 0.25;
+    w0 = T(0.75);
+    w1 = T(0.25);
     if (I1 < 0 || I1 >= nc) {
         if (per) I1 = I1 < 0 ? I1 + nc : I1 - nc;
         else {
-            w0 = ((I1 < 0 ? dlo : dhi) ? 0.5 : 1.0);
+            w0 = ((I1 < 0 ? dlo : dhi) ? T(0.5) : T(1));
             I1 = I0;
-            w1 = 0.0;
+            w1 = T(0);
         }
     }
 }
@@ -476,34 +502,37 @@ __device__ __forceinline__ void co_pw(int f, int nf, int per, int dlo, int dhi, 
     for (int j = ty; j < (g).ny; j += 32) \
         for (int i = tx; i < (g).nx; i += 32)
 
-__global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels, const double* __restrict__ dense,
-                                                          int dense_n, const double* __restrict__ bglob,
-                                                          double* __restrict__ tglob, const int* done) {
-    extern __shared__ double s[];
+template <typename T>
+__global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels, const T* __restrict__ dense,
+                                                          int dense_n, const T* __restrict__ bglob,
+                                                          T* __restrict__ tglob, const int* done) {
+    extern __shared__ __align__(16) unsigned char co_smem[];
+    T* s = reinterpret_cast<T*>(co_smem);
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     // this thread's entries of the dense bottom inverse (row tid/16, columns
     // tid%16 + 16q), loaded now so they are in registers at the bottom
     const int drow = threadIdx.x >> 4, dlane = threadIdx.x & 15;
-    double dm[CO_DENSE_MAX / 16];
+    T dm[CO_DENSE_MAX / 16];
 #pragma unroll
     for (int q = 0; q < CO_DENSE_MAX / 16; ++q) {
         const int c = dlane + 16 * q;
-        dm[q] = (drow < dense_n && c < dense_n) ? dense[drow * dense_n + c] : 0.0;
+        dm[q] = (drow < dense_n && c < dense_n) ? dense[drow * dense_n + c] : T(0);
     }
     {  // b of the top level (loads issued with the done-flag read) and x = w1 D^-1 b
         const int finished = *done;
         const MgGeom g = co_geom(g0, 0);
         const int n = g.nx * g.ny;
-        double* x = s;
-        double* bb = s + 2 * n;
-        double bv0 = 0.0;
+        T* x = s;
+        T* bb = s + 2 * n;
+        T bv0 = T(0);
         if (ty < g.ny && tx < g.nx) bv0 = bglob[(size_t)ty * g0.P + tx];
         if (finished) return;
+        const T om1 = (T)g.om1;
         CO_FOR(g) {
-            const double bv = (i == tx && j == ty) ? bv0 : bglob[(size_t)j * g0.P + i];
-            const Diag D = mg_diag(g, i, j);
+            const T bv = (i == tx && j == ty) ? bv0 : bglob[(size_t)j * g0.P + i];
+            const DiagT<T> D = mg_diag<T>(g, i, j);
             bb[j * g.nx + i] = bv;
-            x[j * g.nx + i] = g.om1 * D.id * bv;
+            x[j * g.nx + i] = om1 * D.id * bv;
         }
     }
     __syncthreads();
@@ -511,42 +540,43 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
     for (int l = 0; l < levels - 1; ++l) {
         const MgGeom g = co_geom(g0, l), gc = co_geom(g0, l + 1);
         const int n = g.nx * g.ny, nc = gc.nx * gc.ny;
-        double* x = s + off;
-        double* t = x + n;
-        double* bb = x + 2 * n;
+        const T om1 = (T)g.om1, om2 = (T)g.om2;
+        T* x = s + off;
+        T* t = x + n;
+        T* bb = x + 2 * n;
         CO_FOR(g) {  // t = x + w2 D^-1 (b - A x)
-            double d;
-            const double ax = co_negA(g, x, i, j, d);
+            T d;
+            const T ax = co_negA<T>(g, x, i, j, d);
             const int k = j * g.nx + i;
-            t[k] = x[k] + g.om2 * co_id(g, d) * (bb[k] - ax);
+            t[k] = x[k] + om2 * co_id<T>(g, d) * (bb[k] - ax);
         }
         __syncthreads();
         CO_FOR(g) {  // r = b - A t, into x
-            double d;
-            const double at = co_negA(g, t, i, j, d);
+            T d;
+            const T at = co_negA<T>(g, t, i, j, d);
             x[j * g.nx + i] = bb[j * g.nx + i] - at;
         }
         __syncthreads();
-        double* xc = x + 3 * n;
-        double* bc = xc + 2 * nc;
+        T* xc = x + 3 * n;
+        T* bc = xc + 2 * nc;
         CO_FOR(gc) {  // b_c = R r, x_c = w1 D^-1 b_c
-            double acc = 0.0;
+            T acc = T(0);
             for (int dy = -1; dy <= 2; ++dy) {
                 int fy;
-                const double wy = co_rw(j, dy, g.ny, g.py, g.db, g.dt, fy);
-                if (wy == 0.0) continue;
-                double row = 0.0;
+                const T wy = (T)co_rw(j, dy, g.ny, g.py, g.db, g.dt, fy);
+                if (wy == T(0)) continue;
+                T row = T(0);
                 for (int dx = -1; dx <= 2; ++dx) {
                     int fx;
-                    const double wx = co_rw(i, dx, g.nx, g.px, g.dl, g.dr, fx);
-                    if (wx != 0.0) row += wx * x[fy * g.nx + fx];
+                    const T wx = (T)co_rw(i, dx, g.nx, g.px, g.dl, g.dr, fx);
+                    if (wx != T(0)) row += wx * x[fy * g.nx + fx];
                 }
                 acc += wy * row;
             }
-            const double bv = 0.25 * acc;
-            const Diag D = mg_diag(gc, i, j);
+            const T bv = T(0.25) * acc;
+            const DiagT<T> D = mg_diag<T>(gc, i, j);
             bc[j * gc.nx + i] = bv;
-            xc[j * gc.nx + i] = gc.om1 * D.id * bv;
+            xc[j * gc.nx + i] = (T)gc.om1 * D.id * bv;
         }
         __syncthreads();
         off += 3 * n;
@@ -554,11 +584,11 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
     {  // bottom
         const MgGeom g = co_geom(g0, levels - 1);
         const int n = g.nx * g.ny;
-        double* x = s + off;
-        double* t = x + n;
-        double* bb = x + 2 * n;
+        T* x = s + off;
+        T* t = x + n;
+        T* bb = x + 2 * n;
         if (dense_n) {  // t = M b, 16 lanes per row
-            double acc = 0.0;
+            T acc = T(0);
 #pragma unroll
             for (int q = 0; q < CO_DENSE_MAX / 16; ++q) {
                 const int c = dlane + 16 * q;
@@ -569,13 +599,13 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
             __syncthreads();
         } else {
             for (int it = 0; it <= 2 * CO_SWEEPS; ++it) {
-                double* in = (it & 1) ? t : x;
-                double* out = (it & 1) ? x : t;
+                T* in = (it & 1) ? t : x;
+                T* out = (it & 1) ? x : t;
                 CO_FOR(g) {
-                    double d;
-                    const double ax = co_negA(g, in, i, j, d);
+                    T d;
+                    const T ax = co_negA<T>(g, in, i, j, d);
                     const int k = j * g.nx + i;
-                    out[k] = in[k] + CO_OMEGA * co_id(g, d) * (bb[k] - ax);
+                    out[k] = in[k] + (T)CO_OMEGA * co_id<T>(g, d) * (bb[k] - ax);
                 }
                 __syncthreads();
             }
@@ -585,32 +615,33 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
     for (int l = levels - 2; l >= 0; --l) {
         const MgGeom g = co_geom(g0, l), gc = co_geom(g0, l + 1);
         const int n = g.nx * g.ny;
+        const T om1 = (T)g.om1, om2 = (T)g.om2;
         off -= 3 * n;
-        double* x = s + off;
-        double* t = x + n;
-        double* bb = x + 2 * n;
-        const double* tc = x + 3 * n + gc.nx * gc.ny;
+        T* x = s + off;
+        T* t = x + n;
+        T* bb = x + 2 * n;
+        const T* tc = x + 3 * n + gc.nx * gc.ny;
         CO_FOR(g) {
             int X0, X1, Y0, Y1;
-            double wx0, wx1, wy0, wy1;
-            co_pw(i, g.nx, g.px, g.dl, g.dr, X0, wx0, X1, wx1);
-            co_pw(j, g.ny, g.py, g.db, g.dt, Y0, wy0, Y1, wy1);
+            T wx0, wx1, wy0, wy1;
+            co_pw<T>(i, g.nx, g.px, g.dl, g.dr, X0, wx0, X1, wx1);
+            co_pw<T>(j, g.ny, g.py, g.db, g.dt, Y0, wy0, Y1, wy1);
             t[j * g.nx + i] += wy0 * (wx0 * tc[Y0 * gc.nx + X0] + wx1 * tc[Y0 * gc.nx + X1]) +
                                wy1 * (wx0 * tc[Y1 * gc.nx + X0] + wx1 * tc[Y1 * gc.nx + X1]);
         }
         __syncthreads();
         CO_FOR(g) {
-            double d;
-            const double at = co_negA(g, t, i, j, d);
+            T d;
+            const T at = co_negA<T>(g, t, i, j, d);
             const int k = j * g.nx + i;
-            x[k] = t[k] + g.om2 * co_id(g, d) * (bb[k] - at);
+            x[k] = t[k] + om2 * co_id<T>(g, d) * (bb[k] - at);
         }
         __syncthreads();
         CO_FOR(g) {
-            double d;
-            const double ax = co_negA(g, x, i, j, d);
+            T d;
+            const T ax = co_negA<T>(g, x, i, j, d);
             const int k = j * g.nx + i;
-            const double v = x[k] + g.om1 * co_id(g, d) * (bb[k] - ax);
+            const T v = x[k] + om1 * co_id<T>(g, d) * (bb[k] - ax);
             if (l == 0) tglob[(size_t)j * g0.P + i] = v;
             else t[k] = v;
         }
@@ -618,7 +649,7 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
     }
     if (levels == 1) {  // the top level is the bottom
         const int n = g0.nx * g0.ny;
-        const double* t = s + n;
+        const T* t = s + n;
         CO_FOR(g0) tglob[(size_t)j * g0.P + i] = t[j * g0.nx + i];
     }
 }
@@ -709,7 +740,7 @@ bool coarse_plan(const MgGeom& top, int max_cells, CoarsePlan& cp, std::vector<d
         ++levels;
     }
     cp.levels = levels;
-    cp.lds_bytes = 3 * cells * sizeof(double);
+    cp.lds_cells = 3 * cells;
     if ((long)nx * ny <= CO_DENSE_MAX) {
         MgGeom gb = top;
         gb.nx = nx;
@@ -738,8 +769,10 @@ int coarse_max_cells() {
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 1024;
     const size_t need = 3 * sizeof(double) * (4096 + 1024 + 256 + 64);
     if ((size_t)lds < need) return 1024;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)need) != hipSuccess) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse<double>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse<float>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) != hipSuccess) {
         (void)hipGetLastError();
         return 1024;
     }
@@ -748,7 +781,13 @@ int coarse_max_cells() {
 
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
                       const int* done) {
-    k_mg_coarse<<<1, CO_THREADS, cp.lds_bytes, s>>>(top, cp.levels, cp.dense, cp.dense_n, b, t, done);
+    k_mg_coarse<double><<<1, CO_THREADS, cp.lds_cells * sizeof(double), s>>>(top, cp.levels, cp.dense, cp.dense_n,
+                                                                             b, t, done);
+}
+void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const float* b, float* t,
+                      const int* done) {
+    k_mg_coarse<float><<<1, CO_THREADS, cp.lds_cells * sizeof(float), s>>>(top, cp.levels, cp.dense_f, cp.dense_n,
+                                                                           b, t, done);
 }
 
 // tile of a level: 64 x 16 on the finest (rz/rr partials follow the PCG
@@ -762,35 +801,61 @@ static int level_tile(const MgGeom& g, bool finest) {
     return 2;
 }
 
-template <int FX, int FY>
-static void down_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
-                   const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+template <int FX, int FY, typename TB, typename T>
+static void down_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const TB* b, T* x2, T* bc, const int* done,
+                   Status* st, const double* rr, int nb, int m, int slot) {
     dim3 g((gf.nx + FX - 1) / FX, (gf.ny + FY - 1) / FY);
-    k_mg_down<FX, FY><<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+    k_mg_down<FX, FY, TB, T><<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
 }
-template <int FX, int FY>
-static void up_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
-                 const double* ec, double* xout, const int* done, double* rz, int nb) {
+template <int FX, int FY, typename TB, typename T, typename TO>
+static void up_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const T* x2, const TB* b, const T* ec, TO* xout,
+                 const int* done, double* rz, int nb) {
     dim3 g((gf.nx + FX - 1) / FX, (gf.ny + FY - 1) / FY);
-    k_mg_up<FX, FY><<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, done, rz, nb);
+    k_mg_up<FX, FY, TB, T, TO><<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, done, rz, nb);
 }
 
-void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
-                    const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+template <typename TB, typename T>
+static void down_any(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const TB* b, T* x2, T* bc, const int* done,
+                     Status* st, const double* rr, int nb, int m, int slot) {
     switch (level_tile(gf, st != nullptr)) {
         case 0: down_t<64, 16>(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot); break;
         case 1: down_t<16, 16>(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot); break;
         default: down_t<16, 8>(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot); break;
     }
 }
-
-void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
-                  const double* ec, double* xout, const int* done, double* rz, int nb) {
+template <typename TB, typename T, typename TO>
+static void up_any(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const T* x2, const TB* b, const T* ec,
+                   TO* xout, const int* done, double* rz, int nb) {
     switch (level_tile(gf, rz != nullptr)) {
         case 0: up_t<64, 16>(s, gf, gc, x2, b, ec, xout, done, rz, nb); break;
         case 1: up_t<16, 16>(s, gf, gc, x2, b, ec, xout, done, rz, nb); break;
         default: up_t<16, 8>(s, gf, gc, x2, b, ec, xout, done, rz, nb); break;
     }
+}
+
+void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, double* x2, double* bc,
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+    down_any(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+}
+void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* b, float* x2, float* bc,
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+    down_any(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+}
+void launch_mg_down(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const float* b, float* x2, float* bc,
+                    const int* done, Status* st, const double* rr, int nb, int m, int slot) {
+    down_any(s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+}
+void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const double* x2, const double* b,
+                  const double* ec, double* xout, const int* done, double* rz, int nb) {
+    up_any(s, gf, gc, x2, b, ec, xout, done, rz, nb);
+}
+void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const float* x2, const double* b,
+                  const float* ec, double* xout, const int* done, double* rz, int nb) {
+    up_any(s, gf, gc, x2, b, ec, xout, done, rz, nb);
+}
+void launch_mg_up(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const float* x2, const float* b,
+                  const float* ec, float* xout, const int* done, double* rz, int nb) {
+    up_any(s, gf, gc, x2, b, ec, xout, done, rz, nb);
 }
 
 }  // namespace fsi

@@ -728,7 +728,9 @@ static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, in
                          const int* done) {
     const int l = nlev - 1;
     const MgGeom& g = geo[l];
-    if (cp.levels > 0) {
+    if (cp.levels > 0 && lv[0].xf) {  // mixed precision: FP32 coarse levels
+        launch_mg_coarse(s, g, cp, lv[l].bf, lv[l].tf, done);
+    } else if (cp.levels > 0) {
         launch_mg_coarse(s, g, cp, lv[l].b, lv[l].t, done);
     } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
         dim3 gr = grid_of(g.nx, g.ny);
@@ -751,7 +753,16 @@ void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const M
     double* rz = b.partials + (size_t)PS_RZ * 4 * nb;
     for (int c = 0; c < count; ++c) {
         const int m = m0 + c;
-        if (nlev > 1) {
+        if (nlev > 1 && lv[0].xf) {  // mixed precision: FP32 V-cycle, FP64 r in and z out
+            launch_mg_down(s, geo[0], geo[1], lv[0].b, lv[0].xf, lv[1].bf, done, st, rr, nb, m, slot);
+            for (int l = 1; l < nlev - 1; ++l)
+                launch_mg_down(s, geo[l], geo[l + 1], lv[l].bf, lv[l].xf, lv[l + 1].bf, done, nullptr, rr, nb, m,
+                               slot);
+            coarse_solve(s, lv, geo, nlev, cp, done);
+            for (int l = nlev - 2; l >= 1; --l)
+                launch_mg_up(s, geo[l], geo[l + 1], lv[l].xf, lv[l].bf, lv[l + 1].tf, lv[l].tf, done, nullptr, nb);
+            launch_mg_up(s, geo[0], geo[1], lv[0].xf, lv[0].b, lv[1].tf, lv[0].t, done, rz, nb);
+        } else if (nlev > 1) {
             for (int l = 0; l < nlev - 1; ++l)
                 launch_mg_down(s, geo[l], geo[l + 1], lv[l].b, lv[l].x, lv[l + 1].b, done, l == 0 ? st : nullptr,
                                rr, nb, m, slot);
@@ -802,6 +813,14 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb, 2, 0);
         return 6 * cell;  // x, p, r, w read; x, r written
     }
+    if (n == "mg_down0" && nlev > 1 && lv[0].xf) {
+        launch_mg_down(s, geo[0], geo[1], lv[0].b, lv[0].xf, lv[1].bf, &st->pcg.done, nullptr, nullptr, nb, 2, 0);
+        return 1.625 * cell;  // b read (FP64); x2 and b_c written (FP32)
+    }
+    if (n == "mg_up0" && nlev > 1 && lv[0].xf) {
+        launch_mg_up(s, geo[0], geo[1], lv[0].xf, lv[0].b, lv[1].tf, lv[0].t, &st->pcg.done, nullptr, nb);
+        return 2.625 * cell;  // x2, e_c (FP32) and b (FP64) read; x5 written (FP64)
+    }
     if (n == "mg_down0" && nlev > 1) {
         launch_mg_down(s, geo[0], geo[1], lv[0].b, lv[0].x, lv[1].b, &st->pcg.done, nullptr, nullptr, nb, 2, 0);
         return 2.25 * cell;  // b read; x2 and b_c written
@@ -812,6 +831,10 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
     }
     if (n == "mg_coarse" && cp.levels > 0) {
         const MgGeom& g = geo[nlev - 1];
+        if (lv[0].xf) {
+            launch_mg_coarse(s, g, cp, lv[nlev - 1].bf, lv[nlev - 1].tf, &st->pcg.done);
+            return 8.0 * g.nx * g.ny;
+        }
         launch_mg_coarse(s, g, cp, lv[nlev - 1].b, lv[nlev - 1].t, &st->pcg.done);
         return 16.0 * g.nx * g.ny;  // b read, t written
     }
