@@ -719,9 +719,11 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         b.Un = w->vel_cur_u[slot];
         b.Vn = w->vel_cur_v[slot];
     }
-    launch_vel_init(c->stream, p, b, alpha, c->st, c->num.velocity_rtol, c->num.velocity_maxiter, slot);
+    launch_vel_init(c->stream, p, b, alpha, c->st);
+    const double vrtol = c->num.velocity_rtol;
+    const int vmax = c->num.velocity_maxiter;
     if (fixed > 0) {
-        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, c->num.velocity_maxiter + 1), slot);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax);
         if (slot >= 0 && c->stage_warm) launch_copy2(c->stream, p, w->Us, w->vel_cur_u[slot], w->Vs, w->vel_cur_v[slot]);
         return FSI_OK;
     }
@@ -730,7 +732,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     int m = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
     while (m <= last) {
         int n = std::min(chunk, last + 1 - m);
-        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot, vrtol, vmax);
         m += n;
         fsi_status rc = poll_done(c, &c->st->vel, &done, &iter);
         if (rc != FSI_OK) return rc;

@@ -110,10 +110,9 @@ struct VelBufs {
     double* q_v[2];
     double* partials;
 };
-void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
-                     int maxiter, int slot);
+void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st);
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
-                      int slot);
+                      int slot, double rtol, int maxiter);
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
 
 // ---- projection (fsi_solvers.hip) ----

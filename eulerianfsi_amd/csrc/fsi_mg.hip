@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels,
     for (int l = 0; l < levels - 1; ++l) {
         const MgGeom g = co_geom(g0, l), gc = co_geom(g0, l + 1);
         const int n = g.nx * g.ny, nc = gc.nx * gc.ny;
-        const T om1 = (T)g.om1, om2 = (T)g.om2;
+        const T om2 = (T)g.om2;
         T* x = s + off;
         T* t = x + n;
         T* bb = x + 2 * n;

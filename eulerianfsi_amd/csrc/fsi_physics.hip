@@ -38,67 +38,73 @@ __device__ __forceinline__ void load_sigma(const Params& p, const Stage& s, int 
     }
 }
 
-// phi * elastic_stress(sigma)  (StructuralConstitutive.jl:100-113, TimeIntegrationIMEX.jl:911-913)
-__device__ __forceinline__ double Txx_c(const Params& p, const Stage& s, int i, int j) {
-    double a, d, b;
-    load_sigma(p, s, i, j, a, d, b);
-    return s.phi[at(p, i, j)] * (p.Gmod * (a - 1.0));
-}
-__device__ __forceinline__ double Tyy_c(const Params& p, const Stage& s, int i, int j) {
-    double a, d, b;
-    load_sigma(p, s, i, j, a, d, b);
-    return s.phi[at(p, i, j)] * (p.Gmod * (d - 1.0));
-}
-__device__ __forceinline__ double Txy_c(const Params& p, const Stage& s, int i, int j) {
-    double a, d, b;
-    load_sigma(p, s, i, j, a, d, b);
-    return s.phi[at(p, i, j)] * (p.Gmod * b);
-}
+// ---- LDS tiles of the momentum kernel ------------------------------------
+// A halo-1 tile around the 64 x 16 faces/cells of a block, read by global
+// (wrapped) index: tile position (li, lj) holds global (i0 + li - 1,
+// j0 + lj - 1), wrapped across a periodic side.
+constexpr int MT_W = TILE_X + 2, MT_H = TILE_ROWS + 2;  // 66 x 18
+struct HaloTile {
+    const double* a;
+    int i0, j0, nx, ny;
+    __device__ __forceinline__ double operator()(int gi, int gj) const {
+        int li = gi - i0 + 1, lj = gj - j0 + 1;
+        if (li < 0) li += nx;
+        else if (li >= MT_W) li -= nx;
+        if (lj < 0) lj += ny;
+        else if (lj >= MT_H) lj -= ny;
+        return a[lj * MT_W + li];
+    }
+};
 
 // ---- material convection pieces (MomentumTransport.jl:141-219) -----------
-__device__ __forceinline__ double u2c(const Params& p, const double* U, int k, int j) {
-    double ur = (p.px && k == p.nx - 1) ? U[at(p, 0, j)] : U[at(p, k + 1, j)];
-    double uc = 0.5 * U[at(p, k, j)] + 0.5 * ur;
+template <class FU>
+__device__ __forceinline__ double u2c(const Params& p, const FU& U, int k, int j) {
+    double ur = (p.px && k == p.nx - 1) ? U(0, j) : U(k + 1, j);
+    double uc = 0.5 * U(k, j) + 0.5 * ur;
     return uc * uc;
 }
-__device__ __forceinline__ double v2c(const Params& p, const double* V, int i, int k) {
-    double vt = (p.py && k == p.ny - 1) ? V[at(p, i, 0)] : V[at(p, i, k + 1)];
-    double vc = 0.5 * V[at(p, i, k)] + 0.5 * vt;
+template <class FV>
+__device__ __forceinline__ double v2c(const Params& p, const FV& V, int i, int k) {
+    double vt = (p.py && k == p.ny - 1) ? V(i, 0) : V(i, k + 1);
+    double vc = 0.5 * V(i, k) + 0.5 * vt;
     return vc * vc;
 }
-__device__ __forceinline__ double u_corner(const Params& p, const double* U, int i, int jc) {
+template <class FU>
+__device__ __forceinline__ double u_corner(const Params& p, const FU& U, int i, int jc) {
     int ii = (p.px && i == p.nx) ? 0 : i;
     if (p.py) {
-        if (jc == 0 || jc == p.ny) return 0.5 * U[at(p, ii, p.ny - 1)] + 0.5 * U[at(p, ii, 0)];
-        return 0.5 * U[at(p, ii, jc - 1)] + 0.5 * U[at(p, ii, jc)];
+        if (jc == 0 || jc == p.ny) return 0.5 * U(ii, p.ny - 1) + 0.5 * U(ii, 0);
+        return 0.5 * U(ii, jc - 1) + 0.5 * U(ii, jc);
     }
     if (jc == 0) {
         if (p.bU.b.kind == FSI_BC_DIRICHLET) return p.bU.b.value;
-        return U[at(p, ii, 0)] + p.bU.b.value * (p.hy / 2.0);
+        return U(ii, 0) + p.bU.b.value * (p.hy / 2.0);
     }
     if (jc == p.ny) {
         if (p.bU.t.kind == FSI_BC_DIRICHLET) return p.bU.t.value;
-        return U[at(p, ii, p.ny - 1)] + p.bU.t.value * (p.hy / 2.0);
+        return U(ii, p.ny - 1) + p.bU.t.value * (p.hy / 2.0);
     }
-    return 0.5 * U[at(p, ii, jc - 1)] + 0.5 * U[at(p, ii, jc)];
+    return 0.5 * U(ii, jc - 1) + 0.5 * U(ii, jc);
 }
-__device__ __forceinline__ double v_corner(const Params& p, const double* V, int i, int jc) {
+template <class FV>
+__device__ __forceinline__ double v_corner(const Params& p, const FV& V, int i, int jc) {
     int jj = (p.py && jc == p.ny) ? 0 : jc;
     if (p.px) {
-        if (i == 0 || i == p.nx) return 0.5 * V[at(p, p.nx - 1, jj)] + 0.5 * V[at(p, 0, jj)];
-        return 0.5 * V[at(p, i - 1, jj)] + 0.5 * V[at(p, i, jj)];
+        if (i == 0 || i == p.nx) return 0.5 * V(p.nx - 1, jj) + 0.5 * V(0, jj);
+        return 0.5 * V(i - 1, jj) + 0.5 * V(i, jj);
     }
     if (i == 0) {
         if (p.bV.l.kind == FSI_BC_DIRICHLET) return p.bV.l.value;
-        return V[at(p, 0, jj)] + p.bV.l.value * (p.hx / 2.0);
+        return V(0, jj) + p.bV.l.value * (p.hx / 2.0);
     }
     if (i == p.nx) {
         if (p.bV.r.kind == FSI_BC_DIRICHLET) return p.bV.r.value;
-        return V[at(p, p.nx - 1, jj)] + p.bV.r.value * (p.hx / 2.0);
+        return V(p.nx - 1, jj) + p.bV.r.value * (p.hx / 2.0);
     }
-    return 0.5 * V[at(p, i - 1, jj)] + 0.5 * V[at(p, i, jj)];
+    return 0.5 * V(i - 1, jj) + 0.5 * V(i, jj);
 }
-__device__ __forceinline__ double uv_corner(const Params& p, const double* U, const double* V, int i, int jc) {
+template <class FU, class FV>
+__device__ __forceinline__ double uv_corner(const Params& p, const FU& U, const FV& V, int i, int jc) {
     return u_corner(p, U, i, jc) * v_corner(p, V, i, jc);
 }
 
@@ -108,34 +114,31 @@ __device__ __forceinline__ double lex_lo(F f, double h) {  // value at the low w
     double d = 0.5 * h;
     return f(0) - (f(1) - f(0)) * d / h;
 }
-template <class F>
-__device__ __forceinline__ double lex_hi(F f, int n, double h) {  // value at the high wall
-    double d = 0.5 * h;
-    return f(n - 1) + (f(n - 1) - f(n - 2)) * d / h;
-}
 
 // ---------------------------------------------------------------------------
-// k_momentum: E_u, E_v (explicit momentum, TimeIntegrationIMEX.jl:962-988) and
-// I_u, I_v (viscous operator, :1057-1061) on the compact faces.
+// Elastic force: symmetric_stress_divergence (Operators.jl:620-735) of
+// T = phi G (sigma - I) (StructuralConstitutive.jl:100-113,
+// TimeIntegrationIMEX.jl:911-913), from cell-centred accessors Txx, Tyy, Txy.
 // ---------------------------------------------------------------------------
-__device__ double elastic_fx(const Params& p, const Stage& s, int i, int j) {
-    // symmetric_stress_divergence x-part at x-face i (Operators.jl:642-683)
+template <class FT>
+__device__ __forceinline__ double elastic_fx(const Params& p, const FT& Txx, const FT& Txy, int i, int j) {
+    // x-part at x-face i (Operators.jl:642-683)
     const int nx = p.nx, ny = p.ny;
     double fx;
-    if (p.px && i == 0) fx = (Txx_c(p, s, 0, j) - Txx_c(p, s, nx - 1, j)) / p.hx;
+    if (p.px && i == 0) fx = (Txx(0, j) - Txx(nx - 1, j)) / p.hx;
     else if (i == 0) {
-        auto f = [&](int k) { return Txx_c(p, s, k, j); };
+        auto f = [&](int k) { return Txx(k, j); };
         double lw = lex_lo(f, p.hx);
-        fx = (Txx_c(p, s, 0, j) - lw) / (0.5 * p.hx);
-    } else fx = (Txx_c(p, s, i, j) - Txx_c(p, s, i - 1, j)) / p.hx;
+        fx = (Txx(0, j) - lw) / (0.5 * p.hx);
+    } else fx = (Txx(i, j) - Txx(i - 1, j)) / p.hx;
     // Txy on this x-face column, any row jj
     auto txy_x = [&](int jj) -> double {
-        if (p.px && i == 0) return 0.5 * Txy_c(p, s, nx - 1, jj) + 0.5 * Txy_c(p, s, 0, jj);
+        if (p.px && i == 0) return 0.5 * Txy(nx - 1, jj) + 0.5 * Txy(0, jj);
         if (i == 0) {
-            auto f = [&](int k) { return Txy_c(p, s, k, jj); };
+            auto f = [&](int k) { return Txy(k, jj); };
             return lex_lo(f, p.hx);
         }
-        return 0.5 * Txy_c(p, s, i - 1, jj) + 0.5 * Txy_c(p, s, i, jj);
+        return 0.5 * Txy(i - 1, jj) + 0.5 * Txy(i, jj);
     };
     if (p.py) {
         int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
@@ -154,22 +157,23 @@ __device__ double elastic_fx(const Params& p, const Stage& s, int i, int j) {
     return fx;
 }
 
-__device__ double elastic_fy(const Params& p, const Stage& s, int i, int j) {
+template <class FT>
+__device__ __forceinline__ double elastic_fy(const Params& p, const FT& Tyy, const FT& Txy, int i, int j) {
     const int nx = p.nx, ny = p.ny;
     double fy;
-    if (p.py && j == 0) fy = (Tyy_c(p, s, i, 0) - Tyy_c(p, s, i, ny - 1)) / p.hy;
+    if (p.py && j == 0) fy = (Tyy(i, 0) - Tyy(i, ny - 1)) / p.hy;
     else if (j == 0) {
-        auto f = [&](int k) { return Tyy_c(p, s, i, k); };
+        auto f = [&](int k) { return Tyy(i, k); };
         double bw = lex_lo(f, p.hy);
-        fy = (Tyy_c(p, s, i, 0) - bw) / (0.5 * p.hy);
-    } else fy = (Tyy_c(p, s, i, j) - Tyy_c(p, s, i, j - 1)) / p.hy;
+        fy = (Tyy(i, 0) - bw) / (0.5 * p.hy);
+    } else fy = (Tyy(i, j) - Tyy(i, j - 1)) / p.hy;
     auto txy_y = [&](int ii) -> double {
-        if (p.py && j == 0) return 0.5 * Txy_c(p, s, ii, ny - 1) + 0.5 * Txy_c(p, s, ii, 0);
+        if (p.py && j == 0) return 0.5 * Txy(ii, ny - 1) + 0.5 * Txy(ii, 0);
         if (j == 0) {
-            auto f = [&](int k) { return Txy_c(p, s, ii, k); };
+            auto f = [&](int k) { return Txy(ii, k); };
             return lex_lo(f, p.hy);
         }
-        return 0.5 * Txy_c(p, s, ii, j - 1) + 0.5 * Txy_c(p, s, ii, j);
+        return 0.5 * Txy(ii, j - 1) + 0.5 * Txy(ii, j);
     };
     if (p.px) {
         int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
@@ -188,44 +192,112 @@ __device__ double elastic_fy(const Params& p, const Stage& s, int i, int j) {
     return fy;
 }
 
+// ---------------------------------------------------------------------------
+// k_momentum: E_u, E_v (explicit momentum, TimeIntegrationIMEX.jl:962-988) and
+// I_u, I_v (viscous operator, :1057-1061) on the compact faces.  U, V and
+// the three components of phi G (sigma - I) (safety reset applied) sit on
+// halo-1 LDS tiles, each value formed once; the face formulas read them
+// from there.
+// ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* __restrict__ Eu,
                                                     double* __restrict__ Ev, double* __restrict__ Iu,
                                                     double* __restrict__ Iv) {
-    const int i = tile_id().x * TILE_X + threadIdx.x;
+    __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
+    __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    const bool elastic = p.elastic != 0;
+    constexpr int NPT = MT_H * MT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
+    // indices of every tile point for U (faces 0..nx), V (rows 0..ny) and
+    // the cells; then all loads back to back
+    size_t ku[NIT], kv[NIT], kc[NIT];
+    bool okc[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = min(tid + it * BLOCK, NPT - 1);
+        const int ui = i0 + k % MT_W - 1, uj = j0 + k / MT_W - 1;
+        auto wrap = [](int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); };
+        // x index for faces (U) and cells; y index for rows (V) and cells
+        const int xf = p.px ? wrap(ui, p.nx) : ui, xc = p.px ? wrap(ui, p.nx) : ui;
+        const int yr = p.py ? wrap(uj, p.ny) : uj, yc = p.py ? wrap(uj, p.ny) : uj;
+        const bool xu_ok = xf >= 0 && xf <= (p.px ? p.nx - 1 : p.nx);
+        const bool yv_ok = yr >= 0 && yr <= (p.py ? p.ny - 1 : p.ny);
+        const bool xc_ok = xc >= 0 && xc < p.nx, yc_ok = yc >= 0 && yc < p.ny;
+        ku[it] = (xu_ok && yc_ok) ? at(p, xf, yc) : 0;
+        kv[it] = (xc_ok && yv_ok) ? at(p, xc, yr) : 0;
+        okc[it] = xc_ok && yc_ok;
+        kc[it] = okc[it] ? at(p, xc, yc) : 0;
+    }
+    double vu[NIT], vvv[NIT], vph[NIT], va[NIT], vd[NIT], vb[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        vu[it] = s.U[ku[it]];
+        vvv[it] = s.V[kv[it]];
+        if (elastic) {
+            vph[it] = s.phi[kc[it]];
+            va[it] = s.sxx[kc[it]];
+            vd[it] = s.syy[kc[it]];
+            vb[it] = s.sxy[kc[it]];
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k >= NPT) break;
+        su[k] = vu[it];
+        sv[k] = vvv[it];
+        if (elastic) {
+            double a = va[it], d = vd[it], b = vb[it];
+            if (s.reset && vph[it] < p.ext_cutoff) {  // apply_structure_extension! StructureTensor.jl:590-613
+                a = 1.0;
+                d = 1.0;
+                b = 0.0;
+            }
+            const double ph = okc[it] ? vph[it] : 0.0;
+            sxx[k] = ph * (p.Gmod * (a - 1.0));
+            syy[k] = ph * (p.Gmod * (d - 1.0));
+            sxy[k] = ph * (p.Gmod * b);
+        }
+    }
+    __syncthreads();
+    const HaloTile U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
+    const HaloTile Txx{sxx, i0, j0, p.nx, p.ny}, Tyy{syy, i0, j0, p.nx, p.ny}, Txy{sxy, i0, j0, p.nx, p.ny};
+    const int i = i0 + threadIdx.x;
     if (i >= p.nx) return;
 #pragma unroll 1
     for (int r = 0; r < ROWS; ++r) {
-        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        const int j = j0 + threadIdx.y + r * TILE_Y;
         if (j >= p.ny) return;
         // ---- x-momentum at x-face (i, j)
         double cu;
         if (!p.px && i == 0) cu = 0.0;  // wall-normal row (MomentumTransport.jl:322-327)
         else {
             int im = (i == 0) ? p.nx - 1 : i - 1;
-            cu = (u2c(p, s.U, i, j) - u2c(p, s.U, im, j)) / p.hx +
-                 (uv_corner(p, s.U, s.V, i, j + 1) - uv_corner(p, s.U, s.V, i, j)) / p.hy;
+            cu = (u2c(p, U, i, j) - u2c(p, U, im, j)) / p.hx +
+                 (uv_corner(p, U, V, i, j + 1) - uv_corner(p, U, V, i, j)) / p.hy;
         }
         double eu = -cu;
         if (p.force == FSI_FORCE_CONSTANT) eu += p.ffx;
         else if (p.force == FSI_FORCE_KOLMOGOROV) eu += p.force_row[j];
-        if (p.elastic) eu += elastic_fx(p, s, i, j);
+        if (elastic) eu += elastic_fx(p, Txx, Txy, i, j);
         // ---- y-momentum at y-face (i, j)
         double cv;
         if (!p.py && j == 0) cv = 0.0;
         else {
             int jm = (j == 0) ? p.ny - 1 : j - 1;
-            cv = (uv_corner(p, s.U, s.V, i + 1, j) - uv_corner(p, s.U, s.V, i, j)) / p.hx +
-                 (v2c(p, s.V, i, j) - v2c(p, s.V, i, jm)) / p.hy;
+            cv = (uv_corner(p, U, V, i + 1, j) - uv_corner(p, U, V, i, j)) / p.hx +
+                 (v2c(p, V, i, j) - v2c(p, V, i, jm)) / p.hy;
         }
         double ev = -cv;
         if (p.force == FSI_FORCE_CONSTANT) ev += p.ffy;
-        if (p.elastic) ev += elastic_fy(p, s, i, j);
+        if (elastic) ev += elastic_fy(p, Tyy, Txy, i, j);
         const size_t k = at(p, i, j);
         Eu[k] = eu;
         Ev[k] = ev;
         if (Iu) {
-            Iu[k] = p.visc ? visc_u(p, PtrAcc{s.U, p.P}, PtrAcc{s.V, p.P}, i, j, 1) : 0.0;
-            Iv[k] = p.visc ? visc_v(p, PtrAcc{s.U, p.P}, PtrAcc{s.V, p.P}, i, j, 1) : 0.0;
+            Iu[k] = p.visc ? visc_u(p, U, V, i, j, 1) : 0.0;
+            Iv[k] = p.visc ? visc_v(p, U, V, i, j, 1) : 0.0;
         }
     }
 }

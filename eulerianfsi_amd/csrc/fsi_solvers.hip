@@ -54,71 +54,6 @@ __device__ __forceinline__ double* pset(double* base, int nb, int set) { return 
 // Convergence is tested on rr_m.  r, p, q are ping-ponged by m; x is updated
 // in place.
 
-// x = warm start uv_n with Dirichlet values, r = rhs - A x (masked), rr, |b_F|^2
-__global__ void __launch_bounds__(BLOCK) k_vel_init(Params p, VelBufs b, double alpha, int nb) {
-    const double gul = p.bU.l.value, gvb = p.bV.b.value;
-    auto xu = [&](int a, int c) { return dir_u(p, a) ? gul : b.Un[at(p, a, c)]; };
-    auto xv = [&](int a, int c) { return dir_v(p, c) ? gvb : b.Vn[at(p, a, c)]; };
-    double acc[2] = {0.0, 0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        double x_u = xu(i, j), x_v = xv(i, j);
-        double ru = 0.0, rv = 0.0;
-        if (!dir_u(p, i)) {
-            ru = b.ru[k] - (x_u - alpha * visc_u(p, xu, xv, i, j, 0));
-            acc[1] += b.ru[k] * b.ru[k];
-        }
-        if (!dir_v(p, j)) {
-            rv = b.rv[k] - (x_v - alpha * visc_v(p, xu, xv, i, j, 0));
-            acc[1] += b.rv[k] * b.rv[k];
-        }
-        b.xu[k] = x_u;
-        b.xv[k] = x_v;
-        b.r_u[0][k] = ru;
-        b.r_v[0][k] = rv;
-        acc[0] += ru * ru + rv * rv;
-    }
-    put_partials<2>(acc, pset(b.partials, nb, PS_VELI), nb);
-}
-
-// p_0 = r_0, q_0 = A p_0
-__global__ void __launch_bounds__(BLOCK) k_vel_first(Params p, VelBufs b, double alpha, Status* st, int nb,
-                                                     double rtol, int maxiter, int slot) {
-    double tot[2];
-    get_totals<2>(pset(b.partials, nb, PS_VELI), nb, tot);
-    const double tol2 = rtol * rtol * tot[1];
-    if (block_id() == 0 && lane_id() == 0) {
-        KrylovScalars& s = st->vel;
-        s.rr = tot[0];
-        s.bnorm2 = tot[1];
-        s.tol2 = tol2;
-        s.maxiter = maxiter;
-        s.iter = 0;
-        s.done = (tot[0] <= tol2) ? 1 : 0;
-        st->vel_iters[slot] = 0;
-        if (tot[0] <= tol2) st->vel_ok[slot] = 1;
-    }
-    if (tot[0] <= tol2) return;
-    auto pu = [&](int a, int c) { return b.r_u[0][at(p, a, c)]; };
-    auto pv = [&](int a, int c) { return b.r_v[0][at(p, a, c)]; };
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        double vu = pu(i, j), vv = pv(i, j);
-        double qu = dir_u(p, i) ? 0.0 : vu - alpha * visc_u(p, pu, pv, i, j, 0);
-        double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, pu, pv, i, j, 0);
-        b.p_u[0][k] = vu;
-        b.p_v[0][k] = vv;
-        b.q_u[0][k] = qu;
-        b.q_v[0][k] = qv;
-        acc[0] += vu * qu + vv * qv;
-        acc[1] += vu * qu + vv * qv;  // r = p on the first iteration
-        acc[2] += qu * qu + qv * qv;
-        acc[3] += vu * vu + vv * vv;
-    }
-    put_partials<4>(acc, pset(b.partials, nb, PS_VEL0), nb);
-}
-
 // LDS tile of a velocity component on halo 1 around the 64 x 16 cells of a
 // block, read by global (wrapped) index.
 constexpr int VT_W = TILE_X + 2, VT_H = TILE_ROWS + 2;  // 66 x 18
@@ -135,19 +70,154 @@ struct VelTile {
     }
 };
 
+// LDS tile of halo H around the 64 x 16 cells of a block, by global index.
+template <int H>
+struct HTile {
+    static constexpr int W = TILE_X + 2 * H, R = TILE_ROWS + 2 * H;
+    const double* a;
+    int i0, j0, nx, ny;
+    __device__ __forceinline__ double operator()(int gi, int gj) const {
+        int li = gi - i0 + H, lj = gj - j0 + H;
+        if (li < 0) li += nx;
+        else if (li >= W) li -= nx;
+        if (lj < 0) lj += ny;
+        else if (lj >= R) lj -= ny;
+        return a[lj * W + li];
+    }
+};
+
+// Start of a predictor solve: x0 = the warm start with the Dirichlet
+// values, r0 = rhs - (I - alpha M) x0 (zero on Dirichlet rows), p0 = r0,
+// q0 = (I - alpha M) p0, and the partials of |r0|^2, |b_F|^2 (free rows) and
+// of the first iteration (p.q, r.q, q.q, r.r).  x0 sits on a halo-2 LDS
+// tile and r0 on a halo-1 tile, so both stencils read LDS.
+__global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
+    using T2 = HTile<2>;
+    using T1 = HTile<1>;
+    __shared__ double sxu[T2::R * T2::W], sxv[T2::R * T2::W];
+    __shared__ double sru[T1::R * T1::W], srv[T1::R * T1::W];
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    if (T.x == 0 && T.y == 0 && tid == 0) st->vel.done = 0;  // the iteration kernels follow this one
+    const double gul = p.bU.l.value, gvb = p.bV.b.value;
+    auto wrapped = [&](int ui, int uj, int& gi, int& gj) -> bool {
+        gi = ui;
+        gj = uj;
+        if (gi < 0 || gi >= p.nx) {
+            if (!p.px) return false;
+            gi = ((gi % p.nx) + p.nx) % p.nx;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (!p.py) return false;
+            gj = ((gj % p.ny) + p.ny) % p.ny;
+        }
+        return true;
+    };
+    // x0 on halo 2
+    for (int k = tid; k < T2::R * T2::W; k += BLOCK) {
+        int gi, gj;
+        double xu = 0.0, xv = 0.0;
+        if (wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj)) {
+            const size_t kk = at(p, gi, gj);
+            xu = dir_u(p, gi) ? gul : b.Un[kk];
+            xv = dir_v(p, gj) ? gvb : b.Vn[kk];
+        }
+        sxu[k] = xu;
+        sxv[k] = xv;
+    }
+    __syncthreads();
+    const T2 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
+    // r0 on halo 1; |r0|^2 and |b_F|^2 over the block's own cells
+    double acc2[2] = {0.0, 0.0};
+    for (int k = tid; k < T1::R * T1::W; k += BLOCK) {
+        const int li = k % T1::W, lj = k / T1::W;
+        const int ui = i0 + li - 1, uj = j0 + lj - 1;
+        int gi, gj;
+        double ru = 0.0, rv = 0.0;
+        if (wrapped(ui, uj, gi, gj)) {
+            const size_t kk = at(p, gi, gj);
+            const bool own = li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
+            if (!dir_u(p, gi)) {
+                const double bu = b.ru[kk];
+                ru = bu - (Xu(gi, gj) - alpha * visc_u(p, Xu, Xv, gi, gj, 0));
+                if (own) acc2[1] += bu * bu;
+            }
+            if (!dir_v(p, gj)) {
+                const double bv = b.rv[kk];
+                rv = bv - (Xv(gi, gj) - alpha * visc_v(p, Xu, Xv, gi, gj, 0));
+                if (own) acc2[1] += bv * bv;
+            }
+            if (own) {
+                b.xu[kk] = Xu(gi, gj);
+                b.xv[kk] = Xv(gi, gj);
+                b.r_u[0][kk] = ru;
+                b.r_v[0][kk] = rv;
+                b.p_u[0][kk] = ru;
+                b.p_v[0][kk] = rv;
+                acc2[0] += ru * ru + rv * rv;
+            }
+        }
+        sru[k] = ru;
+        srv[k] = rv;
+    }
+    __syncthreads();
+    const T1 Ru{sru, i0, j0, p.nx, p.ny}, Rv{srv, i0, j0, p.nx, p.ny};
+    double acc4[4] = {0.0, 0.0, 0.0, 0.0};
+    const int i = i0 + threadIdx.x;
+    if (i < p.nx)
+        for (int r = 0; r < ROWS; ++r) {
+            const int j = j0 + threadIdx.y + r * TILE_Y;
+            if (j >= p.ny) break;
+            const size_t k = at(p, i, j);
+            const double vu = Ru(i, j), vv = Rv(i, j);
+            const double qu = dir_u(p, i) ? 0.0 : vu - alpha * visc_u(p, Ru, Rv, i, j, 0);
+            const double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, Ru, Rv, i, j, 0);
+            b.q_u[0][k] = qu;
+            b.q_v[0][k] = qv;
+            acc4[0] += vu * qu + vv * qv;
+            acc4[1] += vu * qu + vv * qv;  // r = p on the first iteration
+            acc4[2] += qu * qu + qv * qv;
+            acc4[3] += vu * vu + vv * vv;
+        }
+    put_partials<2>(acc2, pset(b.partials, nb, PS_VELI), nb);
+    put_partials<4>(acc4, pset(b.partials, nb, PS_VEL0), nb);
+}
+
 // One CG iteration m >= 1 with a single reduction phase: the totals of
 // iteration m-1 give a = rr/pq and, from rr' = rr - 2a rq + a^2 qq,
 // beta = rr'/rr; then r' = r - a q, p' = r' + beta p, x += a p, q' = A p'
 // with the partials of p'.q', r'.q', q'.q', r'.r' for iteration m+1.  p' is
 // formed once per point on a halo-1 LDS tile and A is applied from there.
 __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
-                                                    int slot) {
+                                                    int slot, double rtol, int maxiter_arg) {
     if (st->vel.done) return;
     double tot[4];
     get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
-    const KrylovScalars& sc = st->vel;
-    const bool conv = rr <= sc.tol2, stop = m >= sc.maxiter;
+    // the first iteration sets the solve's scalars from the partials of k_vel_start
+    double tol2, bnorm2;
+    int maxiter;
+    if (m == 0) {
+        double ti[2];
+        get_totals<2>(pset(b.partials, nb, PS_VELI), nb, ti);
+        bnorm2 = ti[1];
+        tol2 = rtol * rtol * bnorm2;
+        maxiter = maxiter_arg;
+        if (block_id() == 0 && lane_id() == 0) {
+            KrylovScalars& sk = st->vel;
+            sk.bnorm2 = bnorm2;
+            sk.tol2 = tol2;
+            sk.maxiter = maxiter;
+            sk.iter = 0;
+            st->vel_iters[slot] = 0;
+        }
+    } else {
+        tol2 = st->vel.tol2;
+        bnorm2 = st->vel.bnorm2;
+        maxiter = st->vel.maxiter;
+    }
+    const bool conv = rr <= tol2, stop = m >= maxiter;
     if (conv || stop) {
         if (block_id() == 0 && lane_id() == 0) {
             st->vel.done = 1;
@@ -156,7 +226,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
             st->vel_iters[slot] = m;
             if (conv) st->vel_ok[slot] = 1;
             // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
-            if (!conv && rr > 1e-12 * sc.bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / sc.bnorm2), (double)m);
+            if (!conv && rr > 1e-12 * bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / bnorm2), (double)m);
         }
         return;
     }
@@ -650,16 +720,14 @@ __global__ void __launch_bounds__(BLOCK) k_copy(Params p, const double* __restri
 static dim3 grid_of(int nx, int ny) { return dim3((nx + TILE_X - 1) / TILE_X, (ny + TILE_ROWS - 1) / TILE_ROWS); }
 static const dim3 kBlock(TILE_X, TILE_Y);
 
-void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, double rtol,
-                     int maxiter, int slot) {
-    const int nb = grid_blocks(p);
-    k_vel_init<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, nb);
-    k_vel_first<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, rtol, maxiter, slot);
+void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st) {
+    k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
-                      int slot) {
+                      int slot, double rtol, int maxiter) {
     const int nb = grid_blocks(p);
-    for (int c = 0; c < count; ++c) k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m0 + c, slot);
+    for (int c = 0; c < count; ++c)
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m0 + c, slot, rtol, maxiter);
 }
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
     k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
@@ -802,7 +870,7 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         return 2 * cell;
     }
     if (n == "vel_iter") {
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0);
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0, 0.0, 1 << 30);
         return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
     }
     if (n == "pcg_dir") {
