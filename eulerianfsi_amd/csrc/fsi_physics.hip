@@ -440,18 +440,30 @@ __device__ __forceinline__ double psi_value(const Params& p, double ph) {
 }
 
 // ---- structure tensor (StructureTensor.jl:377-551, Operators.jl:431-586)
+// x^e; small non-negative integer exponents (the geometric extension's
+// default is 4) by repeated squaring, others by pow
+__device__ __forceinline__ double pow_fast(double x, double e) {
+    if (e == 4.0) {
+        const double x2 = x * x;
+        return x2 * x2;
+    }
+    if (e == 2.0) return x * x;
+    if (e == 1.0) return x;
+    return pow(x, e);
+}
+
 __device__ __forceinline__ bool sigma_log_product(double a, double d, double b, double& ox, double& oy,
                                                   double& oxy) {
     double m = 0.5 * (a + d), delta = 0.5 * (a - d);
     double r = hypot(delta, b);
     double lhi = m + r, llo = m - r;
     if (!(isfinite(llo) && isfinite(lhi) && llo > 0.0)) return false;
-    double ghi = lhi * log(lhi), glo = llo * log(llo);
     double scale = fmax(1.0, fabs(m)), alpha, beta;
-    if (r <= 1.0e-12 * scale) {
+    if (r <= 1.0e-12 * scale) {  // Hermite limit: lhi log lhi, llo log llo are not needed
         alpha = log(m) + 1.0;
         beta = -m;
     } else {
+        double ghi = lhi * log(lhi), glo = llo * log(llo);
         double den = lhi - llo;
         alpha = (ghi - glo) / den;
         beta = (lhi * glo - llo * ghi) / den;
@@ -489,46 +501,47 @@ __device__ __forceinline__ Sym3 wall_state3(const Params& p, int side, double u,
 // Structure-tensor right-hand side at cell (i, j): upwind transport of the
 // three components (InterfaceTransport.jl:161-258 per component, the 5-point
 // stencil loaded once), stretching, relaxation and geometric extension.
-__device__ __forceinline__ void sigma_rhs(const Params& p, const Stage& s, int i, int j, Status* st, double& oxx,
-                                          double& oyy, double& oxy) {
-    const double* U = s.U;
-    const double* V = s.V;
-    const Sym3 C = load_sym3(p, s, i, j);
+// S(i, j): sigma after the safety reset; U, V: face velocities; phic: phi
+// at the cell.
+template <class FS, class FU, class FV>
+__device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU& U, const FV& V, double phic, int i,
+                                          int j, Status* st, double& oxx, double& oyy, double& oxy) {
+    const Sym3 C = S(i, j);
     Sym3 fl, fr, fb, ft;
     {  // x faces i (left) and i+1 (right); periodic face nx is face 0
-        const double ul = U[at(p, i, j)];
-        if (p.px || i > 0) fl = upwind_flux3(ul, load_sym3(p, s, i > 0 ? i - 1 : p.nx - 1, j), C);
+        const double ul = U(i, j);
+        if (p.px || i > 0) fl = upwind_flux3(ul, S(i > 0 ? i - 1 : p.nx - 1, j), C);
         else fl = upwind_flux3(ul, wall_state3(p, 0, ul, true, C, st), C);
         if (i < p.nx - 1) {
-            fr = upwind_flux3(U[at(p, i + 1, j)], C, load_sym3(p, s, i + 1, j));
+            fr = upwind_flux3(U(i + 1, j), C, S(i + 1, j));
         } else if (p.px) {
-            fr = upwind_flux3(U[at(p, 0, j)], C, load_sym3(p, s, 0, j));
+            fr = upwind_flux3(U(0, j), C, S(0, j));
         } else {
-            const double ur = U[at(p, p.nx, j)];
+            const double ur = U(p.nx, j);
             fr = upwind_flux3(ur, C, wall_state3(p, 1, ur, false, C, st));
         }
     }
     {  // y faces j (bottom) and j+1 (top)
-        const double vb = V[at(p, i, j)];
-        if (p.py || j > 0) fb = upwind_flux3(vb, load_sym3(p, s, i, j > 0 ? j - 1 : p.ny - 1), C);
+        const double vb = V(i, j);
+        if (p.py || j > 0) fb = upwind_flux3(vb, S(i, j > 0 ? j - 1 : p.ny - 1), C);
         else fb = upwind_flux3(vb, wall_state3(p, 2, vb, true, C, st), C);
         if (j < p.ny - 1) {
-            ft = upwind_flux3(V[at(p, i, j + 1)], C, load_sym3(p, s, i, j + 1));
+            ft = upwind_flux3(V(i, j + 1), C, S(i, j + 1));
         } else if (p.py) {
-            ft = upwind_flux3(V[at(p, i, 0)], C, load_sym3(p, s, i, 0));
+            ft = upwind_flux3(V(i, 0), C, S(i, 0));
         } else {
-            const double vt = V[at(p, i, p.ny)];
+            const double vt = V(i, p.ny);
             ft = upwind_flux3(vt, C, wall_state3(p, 3, vt, false, C, st));
         }
     }
     const double axx = -((fr.xx - fl.xx) / p.hx + (ft.xx - fb.xx) / p.hy);
     const double ayy = -((fr.yy - fl.yy) / p.hx + (ft.yy - fb.yy) / p.hy);
     const double axy = -((fr.xy - fl.xy) / p.hx + (ft.xy - fb.xy) / p.hy);
-    double uL = U[at(p, i, j)], uR = U[at(p, i + 1, j)], vB = V[at(p, i, j)], vT = V[at(p, i, j + 1)];
+    double uL = U(i, j), uR = U(i + 1, j), vB = V(i, j), vT = V(i, j + 1);
     double dv = (uR - uL) / p.hx + (vT - vB) / p.hy;
     double ux = (uR - uL) / p.hx, vy = (vT - vB) / p.hy;
-    auto Uc = [&](int jj) { return 0.5 * U[at(p, i, jj)] + 0.5 * U[at(p, i + 1, jj)]; };
-    auto Vc = [&](int ii) { return 0.5 * V[at(p, ii, j)] + 0.5 * V[at(p, ii, j + 1)]; };
+    auto Uc = [&](int jj) { return 0.5 * U(i, jj) + 0.5 * U(i + 1, jj); };
+    auto Vc = [&](int ii) { return 0.5 * V(ii, j) + 0.5 * V(ii, j + 1); };
     double uy, vx;
     if (p.py) {
         int jm = j == 0 ? p.ny - 1 : j - 1, jp = j == p.ny - 1 ? 0 : j + 1;
@@ -563,7 +576,7 @@ __device__ __forceinline__ void sigma_rhs(const Params& p, const Stage& s, int i
     }
     double exx = 0.0, eyy = 0.0, exy = 0.0;
     if (p.ext == FSI_EXT_GEOMETRIC) {
-        double kappa = p.ext_rate * pow(1.0 - clamp01(s.phi[at(p, i, j)]), p.ext_exp);
+        double kappa = p.ext_rate * pow_fast(1.0 - clamp01(phic), p.ext_exp);
         if (kappa != 0.0) {
             double gx, gy, gxy;
             if (sigma_log_product(sa, sd, sb, gx, gy, gxy)) {
@@ -833,24 +846,87 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
 }
 
 // Structure-tensor right-hand side, plus on stage 1 the divergence
-// L-infinity and the finiteness check of the stage-1 state.
+// L-infinity and the finiteness check of the stage-1 state.  sigma (safety
+// reset applied), U and V sit on halo-1 LDS tiles, each value loaded once.
+struct SymTile {
+    const double *xx, *yy, *xy;
+    int i0, j0, nx, ny;
+    __device__ __forceinline__ Sym3 operator()(int gi, int gj) const {
+        int li = gi - i0 + 1, lj = gj - j0 + 1;
+        if (li < 0) li += nx;
+        else if (li >= MT_W) li -= nx;
+        if (lj < 0) lj += ny;
+        else if (lj >= MT_H) lj -= ny;
+        const int k = lj * MT_W + li;
+        return {xx[k], yy[k], xy[k]};
+    }
+};
+
 __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
                                                  double* __restrict__ Eyy, double* __restrict__ Exy, Status* st,
                                                  int stage1) {
     __shared__ double lds[16];
     __shared__ long long lds_n[16];
+    __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
+    __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    constexpr int NPT = MT_H * MT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
+    size_t ku[NIT], kv[NIT], kc[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = min(tid + it * BLOCK, NPT - 1);
+        const int ui = i0 + k % MT_W - 1, uj = j0 + k / MT_W - 1;
+        auto wrap = [](int u, int n) { return u < 0 ? u + n : (u >= n ? u - n : u); };
+        const int x = p.px ? wrap(ui, p.nx) : ui, y = p.py ? wrap(uj, p.ny) : uj;
+        const bool xu_ok = x >= 0 && x <= (p.px ? p.nx - 1 : p.nx);  // faces 0..nx
+        const bool yv_ok = y >= 0 && y <= (p.py ? p.ny - 1 : p.ny);  // rows 0..ny
+        const bool xc_ok = x >= 0 && x < p.nx, yc_ok = y >= 0 && y < p.ny;
+        ku[it] = (xu_ok && yc_ok) ? at(p, x, y) : 0;
+        kv[it] = (xc_ok && yv_ok) ? at(p, x, y) : 0;
+        kc[it] = (xc_ok && yc_ok) ? at(p, x, y) : 0;
+    }
+    double vu[NIT], vvv[NIT], vph[NIT], va[NIT], vd[NIT], vb[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        vu[it] = s.U[ku[it]];
+        vvv[it] = s.V[kv[it]];
+        vph[it] = s.phi[kc[it]];
+        va[it] = s.sxx[kc[it]];
+        vd[it] = s.syy[kc[it]];
+        vb[it] = s.sxy[kc[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k >= NPT) break;
+        su[k] = vu[it];
+        sv[k] = vvv[it];
+        double a = va[it], d = vd[it], b = vb[it];
+        if (s.reset && vph[it] < p.ext_cutoff) {  // apply_structure_extension! StructureTensor.jl:590-613
+            a = 1.0;
+            d = 1.0;
+            b = 0.0;
+        }
+        sxx[k] = a;
+        syy[k] = d;
+        sxy[k] = b;
+    }
+    __syncthreads();
+    const HaloTile U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
+    const SymTile S{sxx, syy, sxy, i0, j0, p.nx, p.ny};
     double dmax = 0.0;
     long long nonfinite = 0;
-    const int i = tile_id().x * TILE_X + threadIdx.x;
+    const int i = i0 + threadIdx.x;
     if (i < p.nx)
 #pragma unroll 1
         for (int r = 0; r < ROWS; ++r) {
-            const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+            const int j = j0 + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
-            double oxx, oyy, oxy;
-            sigma_rhs(p, s, i, j, st, oxx, oyy, oxy);
             const size_t k = at(p, i, j);
+            double oxx, oyy, oxy;
+            sigma_rhs(p, S, U, V, s.phi[k], i, j, st, oxx, oyy, oxy);
             Exx[k] = oxx;
             Eyy[k] = oyy;
             Exy[k] = oxy;
