@@ -767,22 +767,38 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
     __shared__ double s_n1[kNormals ? CR1 * CW1 : 1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
-    // 1. phi (and psi) on the halo-2 tile
-    for (int k = tid; k < CR2 * CW2; k += BLOCK) {
-        int li = k % CW2, lj = k / CW2;
-        int gi = i0 + li - CH2, gj = j0 + lj - CH2;
-        bool in = true;
-        if (gi < 0 || gi >= p.nx) {
-            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
-            else in = false;
+    // 1. phi (and psi) on the halo-2 tile: indices, then all loads back to back
+    {
+        constexpr int NPT = CR2 * CW2, NIT = (NPT + BLOCK - 1) / BLOCK;
+        size_t idx[NIT];
+        bool inb[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int k = min(tid + it * BLOCK, NPT - 1);
+            int gi = i0 + k % CW2 - CH2, gj = j0 + k / CW2 - CH2;
+            bool in = true;
+            if (gi < 0 || gi >= p.nx) {
+                if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+                else in = false;
+            }
+            if (gj < 0 || gj >= p.ny) {
+                if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+                else in = false;
+            }
+            inb[it] = in;
+            idx[it] = in ? at(p, gi, gj) : 0;
         }
-        if (gj < 0 || gj >= p.ny) {
-            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
-            else in = false;
+        double vals[NIT];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) vals[it] = s.phi[idx[it]];
+#pragma unroll
+        for (int it = 0; it < NIT; ++it) {
+            const int k = tid + it * BLOCK;
+            if (k >= NPT) break;
+            const double v = inb[it] ? vals[it] : 0.0;
+            s_phi[k] = v;
+            if (kPsi) s_psi[k] = inb[it] ? psi_value(p, v) : 0.0;
         }
-        double v = in ? s.phi[at(p, gi, gj)] : 0.0;
-        s_phi[k] = v;
-        if (kPsi) s_psi[k] = in ? psi_value(p, v) : 0.0;
     }
     __syncthreads();
     const TileAcc f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};

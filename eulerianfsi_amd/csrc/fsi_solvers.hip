@@ -114,52 +114,87 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
         }
         return true;
     };
-    // x0 on halo 2
-    for (int k = tid; k < T2::R * T2::W; k += BLOCK) {
-        int gi, gj;
-        double xu = 0.0, xv = 0.0;
-        if (wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj)) {
-            const size_t kk = at(p, gi, gj);
-            xu = dir_u(p, gi) ? gul : b.Un[kk];
-            xv = dir_v(p, gj) ? gvb : b.Vn[kk];
+    // x0 on halo 2 (indices, then all loads back to back)
+    {
+        constexpr int N2 = T2::R * T2::W, NI2 = (N2 + BLOCK - 1) / BLOCK;
+        size_t idx[NI2];
+        int fl[NI2];  // 1: inside, 2: U-left Dirichlet column, 4: V-bottom Dirichlet row
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            const int k = min(tid + it * BLOCK, N2 - 1);
+            int gi, gj;
+            const bool in = wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj);
+            idx[it] = in ? at(p, gi, gj) : 0;
+            fl[it] = in ? (1 | (dir_u(p, gi) ? 2 : 0) | (dir_v(p, gj) ? 4 : 0)) : 0;
         }
-        sxu[k] = xu;
-        sxv[k] = xv;
+        double xu[NI2], xv[NI2];
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            xu[it] = b.Un[idx[it]];
+            xv[it] = b.Vn[idx[it]];
+        }
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            const int k = tid + it * BLOCK;
+            if (k >= N2) break;
+            sxu[k] = (fl[it] & 1) ? ((fl[it] & 2) ? gul : xu[it]) : 0.0;
+            sxv[k] = (fl[it] & 1) ? ((fl[it] & 4) ? gvb : xv[it]) : 0.0;
+        }
     }
     __syncthreads();
     const T2 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
     // r0 on halo 1; |r0|^2 and |b_F|^2 over the block's own cells
     double acc2[2] = {0.0, 0.0};
-    for (int k = tid; k < T1::R * T1::W; k += BLOCK) {
-        const int li = k % T1::W, lj = k / T1::W;
-        const int ui = i0 + li - 1, uj = j0 + lj - 1;
-        int gi, gj;
-        double ru = 0.0, rv = 0.0;
-        if (wrapped(ui, uj, gi, gj)) {
-            const size_t kk = at(p, gi, gj);
-            const bool own = li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
-            if (!dir_u(p, gi)) {
-                const double bu = b.ru[kk];
-                ru = bu - (Xu(gi, gj) - alpha * visc_u(p, Xu, Xv, gi, gj, 0));
-                if (own) acc2[1] += bu * bu;
-            }
-            if (!dir_v(p, gj)) {
-                const double bv = b.rv[kk];
-                rv = bv - (Xv(gi, gj) - alpha * visc_v(p, Xu, Xv, gi, gj, 0));
-                if (own) acc2[1] += bv * bv;
-            }
-            if (own) {
-                b.xu[kk] = Xu(gi, gj);
-                b.xv[kk] = Xv(gi, gj);
-                b.r_u[0][kk] = ru;
-                b.r_v[0][kk] = rv;
-                b.p_u[0][kk] = ru;
-                b.p_v[0][kk] = rv;
-                acc2[0] += ru * ru + rv * rv;
-            }
+    {
+        constexpr int N1 = T1::R * T1::W, NI1 = (N1 + BLOCK - 1) / BLOCK;
+        size_t idx[NI1];
+        bool inb[NI1];
+#pragma unroll
+        for (int it = 0; it < NI1; ++it) {
+            const int k = min(tid + it * BLOCK, N1 - 1);
+            int gi, gj;
+            inb[it] = wrapped(i0 + k % T1::W - 1, j0 + k / T1::W - 1, gi, gj);
+            idx[it] = inb[it] ? at(p, gi, gj) : 0;
         }
-        sru[k] = ru;
-        srv[k] = rv;
+        double bu[NI1], bv[NI1];
+#pragma unroll
+        for (int it = 0; it < NI1; ++it) {
+            bu[it] = b.ru[idx[it]];
+            bv[it] = b.rv[idx[it]];
+        }
+#pragma unroll
+        for (int it = 0; it < NI1; ++it) {
+            const int k = tid + it * BLOCK;
+            if (k >= N1) break;
+            const int li = k % T1::W, lj = k / T1::W;
+            const int ui = i0 + li - 1, uj = j0 + lj - 1;
+            double ru = 0.0, rv = 0.0;
+            if (inb[it]) {
+                int gi, gj;
+                wrapped(ui, uj, gi, gj);
+                const size_t kk = idx[it];
+                const bool own = li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
+                if (!dir_u(p, gi)) {
+                    ru = bu[it] - (Xu(gi, gj) - alpha * visc_u(p, Xu, Xv, gi, gj, 0));
+                    if (own) acc2[1] += bu[it] * bu[it];
+                }
+                if (!dir_v(p, gj)) {
+                    rv = bv[it] - (Xv(gi, gj) - alpha * visc_v(p, Xu, Xv, gi, gj, 0));
+                    if (own) acc2[1] += bv[it] * bv[it];
+                }
+                if (own) {
+                    b.xu[kk] = Xu(gi, gj);
+                    b.xv[kk] = Xv(gi, gj);
+                    b.r_u[0][kk] = ru;
+                    b.r_v[0][kk] = rv;
+                    b.p_u[0][kk] = ru;
+                    b.p_v[0][kk] = rv;
+                    acc2[0] += ru * ru + rv * rv;
+                }
+            }
+            sru[k] = ru;
+            srv[k] = rv;
+        }
     }
     __syncthreads();
     const T1 Ru{sru, i0, j0, p.nx, p.ny}, Rv{srv, i0, j0, p.nx, p.ny};
