@@ -100,6 +100,39 @@ def test_tendencies_match_oracle(method):
         assert err <= 1e-12, (name, err)
 
 
+@pytest.mark.parametrize("mname", ["upwind", "cdi", "acdi_fixed", "acdi_dynamic", "uwcdi", "uwcdi_alpha"])
+@pytest.mark.parametrize("cname", ["channel", "dirichlet", "xperiodic"])
+def test_phase_boundary_branches_match_oracle(cname, mname):
+    """The device's phase kernels on the boundary branches the cavity never
+    reaches: Dirichlet phi with inflow through a wall (channel: left; all four
+    walls in `dirichlet`, where the right wall is an inflow), homogeneous
+    Neumann walls with outflow (channel right, xperiodic bottom and top),
+    anisotropic cells and phi values at and outside [0, 1]."""
+    from tests.golden import make_phase_golden as G
+    model = G.models()[mname]
+    cfg = G.configuration(cname, model, n=48, walls=True)
+    U, V, phi, _ = G.inputs(cname, 5, n=48, walls=True)
+    one, zero = np.ones_like(phi), np.zeros_like(phi)
+    ref = _oracle_cfg(cfg).tendencies(dict(U=np.asfortranarray(U), V=np.asfortranarray(V),
+                                           phi=np.asfortranarray(phi), sigma_xx=one, sigma_yy=one,
+                                           sigma_xy=zero))["E_phi"]
+    if os.environ.get("FSI_TEST_DRYRUN") == "1":
+        pytest.skip("dry run: oracle half only")
+    ctx = fsi.build_imex_context(*cfg)
+    work = fsi.allocate_imex_workspace(ctx)
+    state = fsi.FSIState(U, V, phi, one, one, zero)
+    dev = fsi.evaluate_stage_tendencies(state, ctx, work)["E_phi"]
+    err = rel_l2(dev, ref)
+    print(cname, mname, f"{err:.2e}", flush=True)
+    assert np.max(np.abs(ref)) > 1.0
+    assert err <= 1e-12, err
+
+
+def _oracle_cfg(cfg):
+    from oracle.oracle import Oracle
+    return Oracle(*cfg)
+
+
 @pytest.mark.parametrize("method", ["acdi", "cdi", "uwcdi"])
 def test_cavity_steps_match_oracle(method):
     case = setups.seeded_cavity(64, method, seed=7, **TIGHT)
