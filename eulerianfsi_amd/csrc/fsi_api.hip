@@ -60,6 +60,13 @@ struct fsi_context {
     CoarsePlan cplan{};         // single-workgroup coarse solve of geo.back()
     double* co_dense = nullptr;  // its dense bottom inverse
     float* co_dense_f = nullptr; // the same in FP32 (mixed precision)
+    // Direct solve of the projection system by fast transforms (power-of-two
+    // axes with Neumann/Neumann, Dirichlet/Dirichlet or periodic sides): it
+    // is the PCG's initial guess, which then usually meets the tolerance at
+    // iteration 0.  FSI_SPECTRAL=0 keeps the multigrid-PCG solve alone.
+    int use_spectral = 1;
+    SpecPlan spec{};
+    double* spec_tab = nullptr;  // device tables of both axes
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
     // Planned steps: each solve enqueued with a planned iteration count and no
@@ -296,6 +303,8 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         if (e && (e[0] == '0' || e[0] == '1' || e[0] == '2')) c->use_graph = e[0] - '0';
         e = std::getenv("FSI_STAGE_WARM");
         if (e && e[0] == '0') c->stage_warm = 0;
+        e = std::getenv("FSI_SPECTRAL");
+        if (e && e[0] == '0') c->use_spectral = 0;
     }
     fsi_status rc = validate(c);
     if (rc != FSI_OK) return rc;
@@ -464,6 +473,31 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             HIPCHECK(c, hipStreamSynchronize(c->stream));
             c->cplan.dense_f = c->co_dense_f;
         }
+        const MgGeom& g0 = c->geo[0];
+        const int kx = spec_kind(g0.nx, g0.px, g0.dl, g0.dr), ky = spec_kind(g0.ny, g0.py, g0.db, g0.dt);
+        if (c->use_spectral && kx != SPEC_NONE && ky != SPEC_NONE) {
+            const std::vector<double> tx = spec_tables(g0.nx, kx, g0.ihx2), ty = spec_tables(g0.ny, ky, g0.ihy2);
+            HIPCHECK(c, dalloc(c, &c->spec_tab, tx.size() + ty.size()));
+            HIPCHECK(c, hipMemcpyAsync(c->spec_tab, tx.data(), tx.size() * sizeof(double), hipMemcpyHostToDevice,
+                                       c->stream));
+            HIPCHECK(c, hipMemcpyAsync(c->spec_tab + tx.size(), ty.data(), ty.size() * sizeof(double),
+                                       hipMemcpyHostToDevice, c->stream));
+            HIPCHECK(c, hipStreamSynchronize(c->stream));
+            auto axis = [](int n, int kind, double* t) {
+                SpecAxis a{};
+                a.n = n;
+                a.m = 0;
+                while ((1 << a.m) < n) ++a.m;
+                a.kind = kind;
+                a.wfft = reinterpret_cast<const double2*>(t);
+                a.wpost = reinterpret_cast<const double2*>(t + n);
+                a.mu = t + 3 * n;
+                return a;
+            };
+            c->spec.x = axis(g0.nx, kx, c->spec_tab);
+            c->spec.y = axis(g0.ny, ky, c->spec_tab + tx.size());
+            c->spec.enabled = 1;
+        }
     }
     HIPCHECK(c, dalloc(c, &c->partials, partials_size(p)));
     HIPCHECK(c, hipMalloc((void**)&c->st, sizeof(Status)));
@@ -478,7 +512,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
 extern "C" void fsi_context_destroy(fsi_context* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense};
+    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense, c->spec_tab};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
@@ -754,7 +788,7 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     b.lap_bc = c->lap_bc;
     b.rhs = w->rhs;
     b.x = (slot >= 0 && c->stage_warm) ? w->corr_stage[slot] : w->corr[w->corr_cur];
-    if (slot >= 0 && c->stage_warm && w->stage_warm >= 1)
+    if (!c->spec.enabled && slot >= 0 && c->stage_warm && w->stage_warm >= 1)
         launch_extrapolate(c->stream, p, b.x, w->corr_prev[slot], w->stage_warm >= 2);
     b.q_out = w->corr[1 - w->corr_cur];
     b.r = w->r;
@@ -764,6 +798,7 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     b.w = w->w;
     b.partials = c->partials;
     launch_proj_rhs(c->stream, p, b);
+    if (c->spec.enabled) launch_spectral_solve(c->stream, c->spec, p.P, b.rhs, b.x);
     launch_pcg_init(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol,
                     c->num.projection_maxiter);
     const int ps = slot < 0 ? 0 : slot;
@@ -786,7 +821,7 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
         if (done) break;
         chunk = 4;
     }
-    c->last_proj_iters[ps] = std::max(1, iter);
+    c->last_proj_iters[ps] = std::max(0, iter);
     launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
     w->corr_cur = 1 - w->corr_cur;
     return FSI_OK;
@@ -1044,7 +1079,7 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         const Status& st = *c->st_host;
         for (int k = 0; k < 3; ++k) {
             if (!p.visc || st.vel_ok[k]) c->last_vel_iters[k] = std::max(1, st.vel_iters[k]);
-            if (st.proj_ok[k]) c->last_proj_iters[k] = std::max(1, st.proj_iters[k]);
+            if (st.proj_ok[k]) c->last_proj_iters[k] = std::max(0, st.proj_iters[k]);
         }
         bool all = true;
         for (int k = 0; k < 3; ++k) all = all && (!p.visc || st.vel_ok[k]) && st.proj_ok[k];
@@ -1163,7 +1198,7 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
             return 10.0 * 8.0 * p.nx * (double)p.ny;
         }
         return bench_launch(c->stream, name, p, vb, pb, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan,
-                            c->st);
+                            c->st, &c->spec);
     };
     double b = launch();
     if (b < 0) return fail(c, FSI_E_ARGUMENT, "unknown kernel name %s", name);

@@ -171,8 +171,37 @@ void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, co
                       const int* done);
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const float* b, float* t,
                       const int* done);
+// ---- direct Poisson solve by fast real transforms (fsi_spectral.hip) ----
+// Per axis the 5-point operator -A is diagonalised by
+//   SPEC_COS  (Neumann on both sides)   the DCT-II basis cos(pi k (i + 1/2) / n)
+//   SPEC_ALT  (Dirichlet on both sides) the same basis applied to (-1)^i q
+//   SPEC_RDFT (periodic)                the real DFT in half-complex order
+enum SpecKind { SPEC_NONE = -1, SPEC_COS = 0, SPEC_ALT = 1, SPEC_RDFT = 2 };
+constexpr int SPEC_MIN_N = 8, SPEC_MAX_N = 2048;
+struct SpecAxis {
+    int n, m, kind;        // length (power of two), log2 n, SpecKind
+    const double2* wfft;   // e^{-2 pi i t / n}, t < n/2
+    const double2* wpost;  // e^{-i pi k / (2n)}, k < n (cosine kinds)
+    const double* mu;      // eigenvalue of the 1-D operator per coefficient index
+};
+struct SpecPlan {
+    int enabled;
+    SpecAxis x, y;
+};
+// The kind of one axis from its geometry (SPEC_NONE if not diagonalisable
+// here: mixed Neumann/Dirichlet sides, or a length that is not a power of two
+// in [SPEC_MIN_N, SPEC_MAX_N]).
+int spec_kind(int n, int periodic, int dlo, int dhi);
+// Host tables of one axis, packed as wfft (n/2 complex), wpost (n complex),
+// mu (n); h2inv = 1 / h^2.
+std::vector<double> spec_tables(int n, int kind, double h2inv);
+// x = solution of (-A) x = -rhs with the (0, 0) mode set to zero when A is
+// singular (the zero-mean gauge); rhs and x are cell fields of pitch P.
+void launch_spectral_solve(hipStream_t s, const SpecPlan& sp, int P, const double* rhs, double* x);
+
 size_t partials_size(const Params& p);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
-                    const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st);
+                    const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st,
+                    const SpecPlan* sp = nullptr);
 
 }  // namespace fsi

@@ -896,7 +896,8 @@ void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const 
 // timing.  Returns the algorithmic HBM bytes of one launch (fields read +
 // written, 8 bytes per cell), or a negative value for an unknown name.
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
-                    const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st) {
+                    const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st,
+                    const SpecPlan* sp) {
     const int nb = grid_blocks(p);
     const double cell = 8.0 * (double)p.nx * (double)p.ny;
     const std::string n(name);
@@ -940,6 +941,10 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         }
         launch_mg_coarse(s, g, cp, lv[nlev - 1].b, lv[nlev - 1].t, &st->pcg.done);
         return 16.0 * g.nx * g.ny;  // b read, t written
+    }
+    if (n == "spec_solve" && sp && sp->enabled) {  // direct solve: three passes, each reading and writing once
+        launch_spectral_solve(s, *sp, p.P, pb.rhs, pb.w);
+        return 6.0 * cell;
     }
     if (n == "pcg_iter") {  // one whole PCG iteration: V-cycle, direction, update
         launch_pcg_iters(s, p, pb, lv, geo, nlev, cp, st, 2, 1, -1);

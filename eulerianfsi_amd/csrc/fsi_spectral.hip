@@ -1,0 +1,405 @@
+// fsi_spectral.hip -- direct solve of the projection Poisson system
+// (build_pressure_laplacian, src/Operators.jl:361-390, solved in project!,
+// src/Projection.jl:170-307) on uniform grids by fast real transforms.
+//
+// The cell-centred 5-point operator -A (wall ghosts folded into the
+// diagonal: 1/h^2 at a Neumann wall, 3/h^2 at a Dirichlet wall) is the
+// Kronecker sum of two 1-D operators.  Per axis, with n cells of width h:
+//   SPEC_COS  Neumann/Neumann:      basis cos(pi k (i+1/2)/n),
+//             eigenvalue 4 sin^2(pi k / 2n) / h^2
+//   SPEC_ALT  Dirichlet/Dirichlet:  (-1)^i times the same basis (the sign
+//             flip maps the Dirichlet operator onto 4/h^2 - Neumann),
+//             eigenvalue 4 cos^2(pi k / 2n) / h^2
+//   SPEC_RDFT periodic:             real DFT in half-complex order (c <= n/2:
+//             Re X_c, c > n/2: Im X_{n-c}), eigenvalue 4 sin^2(pi f / n) / h^2
+//             with f = min(c, n - c)
+// The solve is three passes over the field, each reading and writing it once:
+//   rows:    forward transform along x                    (k_spec_rows_fwd)
+//   columns: forward along y, divide by the eigenvalue,
+//            inverse along y                              (k_spec_cols)
+//   rows:    inverse along x                              (k_spec_rows_inv)
+// The (0, 0) mode is zeroed where both axes have the constant null vector,
+// which is the zero-mean gauge of the singular Neumann/periodic system.
+//
+// Every transform of two real lines is one complex FFT of length n = 2^m in
+// LDS: the lines are packed as z = x1 + i x2, loaded in bit-reversed order
+// and transformed in place by decimation-in-time radix-4 stages (one radix-2
+// stage first when m is odd), then separated with X1 = (Z_k + conj Z_{n-k})/2,
+// X2 = (Z_k - conj Z_{n-k})/(2i).  The cosine transform of a real line uses
+// Makhoul's reordering v = (x0, x2, x4, ..., x5, x3, x1): C_k = Re(e^{-i pi
+// k/2n} V_k); its inverse rebuilds V_k = e^{i pi k/2n} (C_k - i C_{n-k}) and
+// applies the inverse FFT (the forward one on conjugated data).
+#include <cmath>
+#include <vector>
+
+#include "fsi_kernels.hpp"
+
+namespace fsi {
+
+constexpr int SP_THREADS = 256;
+constexpr int SP_ITEMS = 8;  // values per thread of a 2048-point line pair
+
+__device__ __forceinline__ int brev(int j, int m) { return (int)(__brev((unsigned)j) >> (32 - m)); }
+
+// position of x_i in Makhoul's reordered sequence v
+__device__ __forceinline__ int mk_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
+
+// F complex FFTs of length n = 2^m in place on re/im (F * n each, input in
+// bit-reversed order, output in natural order); twiddles wr + i wi =
+// e^{-2 pi i t/n}, t < n/2.  Ends with a barrier.
+__device__ void lds_fft(double* re, double* im, const double* wr, const double* wi, int m, int F, int tid) {
+    const int n = 1 << m;
+    int s = 0;
+    if (m & 1) {  // radix 2, span 1: butterflies (2q, 2q+1), twiddle 1
+        for (int q = tid; q < F * (n >> 1); q += SP_THREADS) {
+            const int j = 2 * q;
+            const double ar = re[j], ai = im[j], br = re[j + 1], bi = im[j + 1];
+            re[j] = ar + br;
+            im[j] = ai + bi;
+            re[j + 1] = ar - br;
+            im[j + 1] = ai - bi;
+        }
+        s = 1;
+        __syncthreads();
+    }
+    // radix 4 = the radix-2 stages of spans h and 2h on j, j+h, j+2h, j+3h:
+    // span h pairs (j, j+h), (j+2h, j+3h) with w1 = e^{-2 pi i t/2h}; span 2h
+    // pairs (j, j+2h) with w2 = e^{-2 pi i t/4h} and (j+h, j+3h) with -i w2
+    for (; s < m; s += 2) {
+        const int h = 1 << s;
+        for (int q = tid; q < F * (n >> 2); q += SP_THREADS) {
+            const int f = q >> (m - 2), r = q & ((n >> 2) - 1);
+            const int t = r & (h - 1);
+            const int j = f * n + ((r >> s) << (s + 2)) + t;
+            const int i1 = t << (m - s - 1), i2 = t << (m - s - 2);
+            const double w1r = wr[i1], w1i = wi[i1], w2r = wr[i2], w2i = wi[i2];
+            const double Ar = re[j], Ai = im[j], Br = re[j + h], Bi = im[j + h];
+            const double Cr = re[j + 2 * h], Ci = im[j + 2 * h], Dr = re[j + 3 * h], Di = im[j + 3 * h];
+            const double bR = w1r * Br - w1i * Bi, bI = w1r * Bi + w1i * Br;
+            const double dR = w1r * Dr - w1i * Di, dI = w1r * Di + w1i * Dr;
+            const double A1r = Ar + bR, A1i = Ai + bI, B1r = Ar - bR, B1i = Ai - bI;
+            const double C1r = Cr + dR, C1i = Ci + dI, D1r = Cr - dR, D1i = Ci - dI;
+            const double cR = w2r * C1r - w2i * C1i, cI = w2r * C1i + w2i * C1r;
+            const double gR = w2r * D1r - w2i * D1i, gI = w2r * D1i + w2i * D1r;  // w2 D1
+            const double eR = gI, eI = -gR;                                         // -i w2 D1
+            re[j] = A1r + cR;
+            im[j] = A1i + cI;
+            re[j + 2 * h] = A1r - cR;
+            im[j + 2 * h] = A1i - cI;
+            re[j + h] = B1r + eR;
+            im[j + h] = B1i + eI;
+            re[j + 3 * h] = B1r - eR;
+            im[j + 3 * h] = B1i - eI;
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void load_twiddles(const SpecAxis& a, double* wr, double* wi, int tid) {
+    for (int t = tid; t < (a.n >> 1); t += SP_THREADS) {
+        const double2 w = a.wfft[t];
+        wr[t] = w.x;
+        wi[t] = w.y;
+    }
+}
+
+// the two real coefficient lines of output index k from the packed spectrum
+template <int KIND>
+__device__ __forceinline__ void split_coeffs(const SpecAxis& a, const double* re, const double* im, int k,
+                                             double& o1, double& o2) {
+    const int n = a.n, kr = (n - k) & (n - 1);
+    const double zr = re[k], zi = im[k], cr = re[kr], ci = im[kr];
+    const double x1r = 0.5 * (zr + cr), x1i = 0.5 * (zi - ci);
+    const double x2r = 0.5 * (zi + ci), x2i = -0.5 * (zr - cr);
+    if (KIND == SPEC_RDFT) {
+        const bool lo = k <= (n >> 1);
+        o1 = lo ? x1r : -x1i;
+        o2 = lo ? x2r : -x2i;
+    } else {
+        const double2 t = a.wpost[k];
+        o1 = t.x * x1r - t.y * x1i;
+        o2 = t.x * x2r - t.y * x2i;
+    }
+}
+
+// the spectrum V_k (or X_k) of one real line from its coefficient line c
+template <int KIND>
+__device__ __forceinline__ void line_spectrum(const SpecAxis& a, const double* c, int k, double& vr, double& vi) {
+    const int n = a.n, h = n >> 1;
+    if (KIND == SPEC_RDFT) {
+        vr = k <= h ? c[k] : c[n - k];
+        vi = (k == 0 || k == h) ? 0.0 : (k < h ? c[n - k] : -c[k]);
+    } else {
+        const double ck = c[k], cr = k ? c[n - k] : 0.0;
+        const double2 t = a.wpost[k];  // V = conj(t) (ck - i cr)
+        vr = t.x * ck - t.y * cr;
+        vi = -t.x * cr - t.y * ck;
+    }
+}
+
+// ---- rows: forward transform along x of two rows per workgroup ----------
+template <int KIND>
+__global__ void __launch_bounds__(SP_THREADS) k_spec_rows_fwd(SpecAxis ax, int nrows, int P,
+                                                              const double* __restrict__ in,
+                                                              double* __restrict__ out, double scale) {
+    extern __shared__ double sp_smem[];
+    const int n = ax.n, m = ax.m, tid = threadIdx.x;
+    double* re = sp_smem;
+    double* im = re + n;
+    double* wr = im + n;
+    double* wi = wr + (n >> 1);
+    const int j0 = 2 * blockIdx.x;
+    const bool two = j0 + 1 < nrows;
+    const double* r0 = in + (size_t)j0 * P;
+    double a[SP_ITEMS], b[SP_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {  // every load in flight before the first use
+        const int i = tid + it * SP_THREADS;
+        a[it] = i < n ? r0[i] : 0.0;
+        b[it] = (i < n && two) ? r0[i + P] : 0.0;
+    }
+    load_twiddles(ax, wr, wi, tid);
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int i = tid + it * SP_THREADS;
+        if (i >= n) break;
+        const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
+        const double s = (KIND == SPEC_ALT && (i & 1)) ? -scale : scale;
+        const int p = brev(jv, m);
+        re[p] = s * a[it];
+        im[p] = s * b[it];
+    }
+    __syncthreads();
+    lds_fft(re, im, wr, wi, m, 1, tid);
+    double* o0 = out + (size_t)j0 * P;
+    for (int k = tid; k < n; k += SP_THREADS) {
+        double o1, o2;
+        split_coeffs<KIND>(ax, re, im, k, o1, o2);
+        o0[k] = o1;
+        if (two) o0[k + P] = o2;
+    }
+}
+
+// ---- rows: inverse transform along x, in place ---------------------------
+template <int KIND>
+__global__ void __launch_bounds__(SP_THREADS) k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x) {
+    extern __shared__ double sp_smem[];
+    const int n = ax.n, m = ax.m, tid = threadIdx.x;
+    double* re = sp_smem;  // the two coefficient lines, then the packed spectrum
+    double* im = re + n;
+    double* wr = im + n;
+    double* wi = wr + (n >> 1);
+    const int j0 = 2 * blockIdx.x;
+    const bool two = j0 + 1 < nrows;
+    double* r0 = x + (size_t)j0 * P;
+    double a[SP_ITEMS], b[SP_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int i = tid + it * SP_THREADS;
+        a[it] = i < n ? r0[i] : 0.0;
+        b[it] = (i < n && two) ? r0[i + P] : 0.0;
+    }
+    load_twiddles(ax, wr, wi, tid);
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int i = tid + it * SP_THREADS;
+        if (i >= n) break;
+        re[i] = a[it];
+        im[i] = b[it];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {  // conj(V1 + i V2), V from the coefficient lines
+        const int k = tid + it * SP_THREADS;
+        if (k >= n) break;
+        double v1r, v1i, v2r, v2i;
+        line_spectrum<KIND>(ax, re, k, v1r, v1i);
+        line_spectrum<KIND>(ax, im, k, v2r, v2i);
+        a[it] = v1r - v2i;
+        b[it] = -(v1i + v2r);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int k = tid + it * SP_THREADS;
+        if (k >= n) break;
+        const int p = brev(k, m);
+        re[p] = a[it];
+        im[p] = b[it];
+    }
+    __syncthreads();
+    lds_fft(re, im, wr, wi, m, 1, tid);
+    const double inv_n = 1.0 / (double)n;
+    for (int i = tid; i < n; i += SP_THREADS) {  // z = conj(FFT(conj Z)) / n
+        const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
+        const double s = (KIND == SPEC_ALT && (i & 1)) ? -inv_n : inv_n;
+        r0[i] = s * re[jv];
+        if (two) r0[i + P] = -s * im[jv];
+    }
+}
+
+// ---- columns: forward along y, divide, inverse along y; 2C columns -------
+template <int KIND, int C>
+__global__ void __launch_bounds__(SP_THREADS) k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P,
+                                                          double* x) {
+    extern __shared__ double sp_smem[];
+    const int n = ay.n, m = ay.m, tid = threadIdx.x;
+    double* re = sp_smem;  // C transforms of n
+    double* im = re + C * n;
+    double* wr = im + C * n;
+    double* wi = wr + (n >> 1);
+    const int x0 = blockIdx.x * 2 * C;
+    constexpr int W = 2 * C;  // columns per row of the block
+    const int total = n * W;
+    double v[SP_ITEMS * 2];
+#pragma unroll
+    for (int it = 0; it < 2 * SP_ITEMS; ++it) {  // element e: row e / W, column e % W
+        const int e = tid + it * SP_THREADS;
+        v[it] = e < total ? x[(size_t)(e / W) * P + x0 + (e % W)] : 0.0;
+    }
+    load_twiddles(ay, wr, wi, tid);
+#pragma unroll
+    for (int it = 0; it < 2 * SP_ITEMS; ++it) {
+        const int e = tid + it * SP_THREADS;
+        if (e >= total) break;
+        const int y = e / W, c = e % W;
+        const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
+        const double s = (KIND == SPEC_ALT && (y & 1)) ? -1.0 : 1.0;
+        const int p = (c >> 1) * n + brev(jv, m);
+        if (c & 1) im[p] = s * v[it];
+        else re[p] = s * v[it];
+    }
+    __syncthreads();
+    lds_fft(re, im, wr, wi, m, C, tid);
+    // coefficients of the column pair f at row mode l, divided by the eigenvalue
+    double o1[SP_ITEMS], o2[SP_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int q = tid + it * SP_THREADS;
+        if (q >= C * n) break;
+        const int f = q >> m, l = q & (n - 1);
+        split_coeffs<KIND>(ay, re + f * n, im + f * n, l, o1[it], o2[it]);
+        const double my = ay.mu[l];
+        const double l1 = mux[x0 + 2 * f] + my, l2 = mux[x0 + 2 * f + 1] + my;
+        o1[it] = l1 != 0.0 ? o1[it] / l1 : 0.0;
+        o2[it] = l2 != 0.0 ? o2[it] / l2 : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int q = tid + it * SP_THREADS;
+        if (q >= C * n) break;
+        re[q] = o1[it];  // coefficient line of column 2f in re[f n ...], 2f+1 in im[f n ...]
+        im[q] = o2[it];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int q = tid + it * SP_THREADS;
+        if (q >= C * n) break;
+        const int f = q >> m, l = q & (n - 1);
+        double v1r, v1i, v2r, v2i;
+        line_spectrum<KIND>(ay, re + f * n, l, v1r, v1i);
+        line_spectrum<KIND>(ay, im + f * n, l, v2r, v2i);
+        o1[it] = v1r - v2i;
+        o2[it] = -(v1i + v2r);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SP_ITEMS; ++it) {
+        const int q = tid + it * SP_THREADS;
+        if (q >= C * n) break;
+        const int p = (q >> m) * n + brev(q & (n - 1), m);
+        re[p] = o1[it];
+        im[p] = o2[it];
+    }
+    __syncthreads();
+    lds_fft(re, im, wr, wi, m, C, tid);
+    const double inv_n = 1.0 / (double)n;
+#pragma unroll
+    for (int it = 0; it < 2 * SP_ITEMS; ++it) {
+        const int e = tid + it * SP_THREADS;
+        if (e >= total) break;
+        const int y = e / W, c = e % W;
+        const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
+        const double s = (KIND == SPEC_ALT && (y & 1)) ? -inv_n : inv_n;
+        const int p = (c >> 1) * n + jv;
+        x[(size_t)y * P + x0 + c] = (c & 1) ? -s * im[p] : s * re[p];
+    }
+}
+
+// ---- host side -------------------------------------------------------------
+
+int spec_kind(int n, int periodic, int dlo, int dhi) {
+    if (n < SPEC_MIN_N || n > SPEC_MAX_N || (n & (n - 1))) return SPEC_NONE;
+    if (periodic) return SPEC_RDFT;
+    if (!dlo && !dhi) return SPEC_COS;
+    if (dlo && dhi) return SPEC_ALT;
+    return SPEC_NONE;
+}
+
+std::vector<double> spec_tables(int n, int kind, double h2inv) {
+    std::vector<double> t((size_t)(n + 2 * n + n), 0.0);  // wfft (n/2 complex), wpost (n complex), mu (n)
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < n / 2; ++k) {
+        const long double a = -2.0L * pi * k / n;
+        t[2 * k] = (double)std::cos(a);
+        t[2 * k + 1] = (double)std::sin(a);
+    }
+    double* wpost = t.data() + n;
+    for (int k = 0; k < n; ++k) {
+        const long double a = -pi * k / (2.0L * n);
+        wpost[2 * k] = (double)std::cos(a);
+        wpost[2 * k + 1] = (double)std::sin(a);
+    }
+    double* mu = t.data() + 3 * n;
+    for (int k = 0; k < n; ++k) {
+        long double s;
+        if (kind == SPEC_COS) s = std::sin(pi * k / (2.0L * n));
+        else if (kind == SPEC_ALT) s = std::cos(pi * k / (2.0L * n));
+        else s = std::sin(pi * (k <= n / 2 ? k : n - k) / n);
+        mu[k] = (double)(4.0L * s * s * h2inv);
+    }
+    return t;
+}
+
+template <int KX>
+static void rows_fwd(hipStream_t s, const SpecPlan& sp, int P, const double* in, double* out) {
+    const int n = sp.x.n, rows = sp.y.n;
+    k_spec_rows_fwd<KX><<<(rows + 1) / 2, SP_THREADS, (size_t)3 * n * sizeof(double), s>>>(sp.x, rows, P, in, out,
+                                                                                            -1.0);
+}
+template <int KX>
+static void rows_inv(hipStream_t s, const SpecPlan& sp, int P, double* x) {
+    const int n = sp.x.n, rows = sp.y.n;
+    k_spec_rows_inv<KX><<<(rows + 1) / 2, SP_THREADS, (size_t)3 * n * sizeof(double), s>>>(sp.x, rows, P, x);
+}
+template <int KY>
+static void cols(hipStream_t s, const SpecPlan& sp, int P, double* x) {
+    const int n = sp.y.n, nx = sp.x.n;
+    const size_t lds2 = ((size_t)4 * n + n) * sizeof(double), lds1 = ((size_t)2 * n + n) * sizeof(double);
+    if (n <= SPEC_MAX_N / 2 && nx % 4 == 0)
+        k_spec_cols<KY, 2><<<nx / 4, SP_THREADS, lds2, s>>>(sp.y, sp.x.mu, P, x);
+    else
+        k_spec_cols<KY, 1><<<nx / 2, SP_THREADS, lds1, s>>>(sp.y, sp.x.mu, P, x);
+}
+
+void launch_spectral_solve(hipStream_t s, const SpecPlan& sp, int P, const double* rhs, double* x) {
+    switch (sp.x.kind) {
+    case SPEC_COS: rows_fwd<SPEC_COS>(s, sp, P, rhs, x); break;
+    case SPEC_ALT: rows_fwd<SPEC_ALT>(s, sp, P, rhs, x); break;
+    default: rows_fwd<SPEC_RDFT>(s, sp, P, rhs, x); break;
+    }
+    switch (sp.y.kind) {
+    case SPEC_COS: cols<SPEC_COS>(s, sp, P, x); break;
+    case SPEC_ALT: cols<SPEC_ALT>(s, sp, P, x); break;
+    default: cols<SPEC_RDFT>(s, sp, P, x); break;
+    }
+    switch (sp.x.kind) {
+    case SPEC_COS: rows_inv<SPEC_COS>(s, sp, P, x); break;
+    case SPEC_ALT: rows_inv<SPEC_ALT>(s, sp, P, x); break;
+    default: rows_inv<SPEC_RDFT>(s, sp, P, x); break;
+    }
+}
+
+}  // namespace fsi

@@ -1,0 +1,127 @@
+"""The projection's direct solve by fast transforms (cosine / sign-flipped
+cosine / periodic real DFT per axis) on the GPU: projections agree with the
+oracle's tight projection for every axis pair, the PCG that follows it ends
+at iteration 0 at the default tolerance, and IMEX steps agree with steps
+whose projection is the multigrid-PCG alone."""
+import numpy as np
+import pytest
+
+import eulerianfsi_amd as fsi
+from eulerianfsi_amd import closures as C
+from eulerianfsi_amd import setups
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U", "V", "phi", "sigma_xx", "sigma_yy", "sigma_xy")
+
+# correction sides (left, right, bottom, top); P periodic, N Neumann(0), D Dirichlet(0)
+CONFIGS = {
+    "neumann_64x32": (64, 32, "NNNN"),
+    "dirichlet_32x64": (32, 64, "DDDD"),
+    "dirichlet_x_neumann_y": (64, 64, "DDNN"),
+    "periodic_x_neumann_y": (64, 32, "PPNN"),
+    "periodic": (32, 32, "PPPP"),
+    "neumann_128x256": (128, 256, "NNNN"),
+    "fallback_48x40": (48, 40, "NNNN"),  # not a power of two: multigrid-PCG alone
+}
+
+
+def _config(nx, ny, sides, rtol):
+    px, py = sides[0] == "P", sides[2] == "P"
+    grid = fsi.generate_grid(1.0, 0.8, nx, ny)
+    metrics = fsi.build_mesh_metrics(grid, periodic_x=px, periodic_y=py)
+    side = {"P": fsi.Periodic(), "N": fsi.Neumann(0.0), "D": fsi.Dirichlet(0.0)}
+    corr = fsi.BoundaryConditionSetup(*(side[s] for s in sides))
+    xs = fsi.Periodic() if px else fsi.Dirichlet(0.0)
+    ys = fsi.Periodic() if py else fsi.Dirichlet(0.0)
+    vel = fsi.BoundaryConditionSetup(xs, xs, ys, ys)
+    xn = fsi.Periodic() if px else fsi.Neumann(0.0)
+    yn = fsi.Periodic() if py else fsi.Neumann(0.0)
+    sc = fsi.BoundaryConditionSetup(xn, xn, yn, yn)
+    bcs = fsi.FSIBoundaryConditions(vel, vel, corr, sc, {"xx": sc, "yy": sc, "xy": sc})
+    models = fsi.FSIModels(viscosity=C.ConstantViscosity(0.01))
+    return metrics, bcs, models, fsi.IMEXNumerics(projection_reltol=rtol)
+
+
+def _velocity(nx, ny, sides, seed):
+    rng = np.random.default_rng(seed)
+    U = rng.standard_normal((nx + 1, ny))
+    V = rng.standard_normal((nx, ny + 1))
+    if sides[0] == "P":
+        U[-1] = U[0]
+    else:
+        U[0] = U[-1] = 0.0
+    if sides[2] == "P":
+        V[:, -1] = V[:, 0]
+    else:
+        V[:, 0] = V[:, -1] = 0.0
+    return np.asfortranarray(U), np.asfortranarray(V)
+
+
+def _project_on_device(cfg, U, V):
+    """Projection of copies of U, V (an FSIState writes results back into the
+    arrays it was given, as the reference's state does)."""
+    U, V = U.copy(order="F"), V.copy(order="F")
+    ctx = fsi.build_imex_context(*cfg)
+    work = fsi.allocate_imex_workspace(ctx)
+    n = U.shape[1]
+    one, zero = np.ones((U.shape[0] - 1, n)), np.zeros((U.shape[0] - 1, n))
+    state = fsi.FSIState(U, V, 0.5 * one, one, one, zero)
+    res = fsi.project(state, ctx, work)
+    return state.U.copy(), state.V.copy(), res
+
+
+@pytest.mark.parametrize("name", sorted(CONFIGS))
+def test_projection_matches_oracle(name):
+    from oracle.oracle import Oracle
+    nx, ny, sides = CONFIGS[name]
+    U, V = _velocity(nx, ny, sides, seed=sum(map(ord, name)))
+    # 1e-13: the oracle's PCG cannot attain 1e-14 on a white-noise right-hand
+    # side at 128 x 256 and would run to its iteration limit
+    cfg = _config(nx, ny, sides, 1e-13)
+    Uo, Vo = U.copy(order="F"), V.copy(order="F")
+    Oracle(*cfg).project(Uo, Vo)
+    Ud, Vd, res = _project_on_device(cfg, U, V)
+    eu, ev = rel_l2(Ud, Uo), rel_l2(Vd, Vo)
+    print(name, "iterations", res.iterations, f"U {eu:.2e} V {ev:.2e}", flush=True)
+    assert eu <= 1e-11 and ev <= 1e-11
+    # at the default tolerance the direct solve alone meets it
+    _, _, res = _project_on_device(_config(nx, ny, sides, 1e-10), U, V)
+    print(name, "iterations at rtol 1e-10:", res.iterations, "residual", res.linear_relative_residual, flush=True)
+    if not name.startswith("fallback"):
+        assert res.iterations == 0
+    assert res.linear_relative_residual <= 1e-10
+
+
+@pytest.mark.parametrize("method", ["acdi", "uwcdi"])
+def test_steps_match_multigrid_projection(monkeypatch, method):
+    """Five tight steps of the N = 128 cavity (ACDI) or the x-periodic
+    filament (UWCDI): the direct-solve projection and the multigrid-PCG
+    projection give the same fields to solver accuracy."""
+    def run(flag):
+        monkeypatch.setenv("FSI_SPECTRAL", flag)
+        if method == "acdi":
+            case = setups.seeded_cavity(128, method, seed=3, velocity_rtol=1e-14, projection_rtol=1e-14)
+        else:
+            case = setups.kolmogorov_filament(128, method, velocity_rtol=1e-14, projection_rtol=1e-14)
+        ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
+        work = fsi.allocate_imex_workspace(ctx)
+        fsi.make_velocity_compatible(case.state, ctx, work)
+        its = []
+        for k in range(5):
+            fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+            its.append(work.last_stats.projection_iterations)
+        return {f: getattr(case.state, f).copy() for f in FIELDS}, its
+
+    a, ia = run("1")
+    b, ib = run("0")
+    print(method, "projection iterations direct", ia, "multigrid", ib, flush=True)
+    vel = np.sqrt(np.sum(b["U"] ** 2) + np.sum(b["V"] ** 2))
+    duv = np.sqrt(np.sum((a["U"] - b["U"]) ** 2) + np.sum((a["V"] - b["V"]) ** 2)) / vel
+    print("UV", f"{duv:.2e}", flush=True)
+    assert duv <= 1e-10
+    for f in ("phi", "sigma_xx", "sigma_yy", "sigma_xy"):
+        err = rel_l2(a[f], b[f])
+        print(f, f"{err:.2e}", flush=True)
+        assert err <= 1e-10, (f, err)
