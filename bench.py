@@ -132,17 +132,20 @@ def main(argv=None):
 
     # per-kernel measurement: the candidates for the dominant kernel
     kernels = {}
-    for name in ("vel_iter", "pcg_dir", "pcg_upd", "mg_down0", "mg_up0", "mg_coarse", "pcg_iter", "momentum",
-                 "cells", "copy"):
+    for name in ("vel_iter", "spec_solve", "pcg_dir", "pcg_upd", "mg_down0", "mg_up0", "mg_coarse", "pcg_iter",
+                 "momentum", "cells", "copy"):
         ms, nb = bench_kernel(name, args.kernel_reps)
         kernels[name] = {"avg_ms": ms, "bytes": nb, "GBps": nb / ms / 1e6}
         print(f"kernel {name}: {ms * 1e3:.1f} us, {nb / ms / 1e6:.0f} GB/s", file=sys.stderr, flush=True)
     # launches per step of each kernel (one velocity-CG kernel per iteration,
-    # one finest down/up pair and one dir/upd pair per PCG iteration)
+    # one finest down/up pair and one dir/upd pair per PCG iteration, one
+    # direct solve per projection)
     nv = float(np.sum(vel_mean))
     npj = float(np.sum(proj_mean))
-    launches = {"vel_iter": nv + 3, "pcg_dir": npj, "pcg_upd": npj, "mg_down0": npj + 3, "mg_up0": npj,
+    launches = {"vel_iter": nv + 3, "pcg_dir": npj, "pcg_upd": npj, "mg_down0": npj, "mg_up0": npj,
                 "mg_coarse": npj, "momentum": 3, "cells": 3}
+    if kernels["spec_solve"]["avg_ms"] > 0:
+        launches["spec_solve"] = 3
     share = {k: launches[k] * kernels[k]["avg_ms"] for k in launches}
     dominant = max(share, key=share.get)
     dk = kernels[dominant]

@@ -50,6 +50,7 @@ struct fsi_context {
     hipStream_t stream = nullptr;
     std::string err;
     double* force_row = nullptr;
+    double* ext = nullptr;  // per-block extrema of the report and checks (EX_* slots)
     double* lap_bc = nullptr;
     double* bvisc_u = nullptr;
     double* bvisc_v = nullptr;
@@ -391,6 +392,9 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         HIPCHECK(c, hipMemcpyAsync(c->force_row, host.data(), ny * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
     p.force_row = c->force_row;
+    p.exnb = grid_blocks(p);
+    HIPCHECK(c, dalloc(c, &c->ext, (size_t)EX_COUNT * p.exnb));
+    p.exbuf = c->ext;
     // b_bc of the projection Laplacian: D * gradient_bc (Operators.jl:361-384)
     {
         std::vector<double> bc((size_t)nx * ny, 0.0);
@@ -496,7 +500,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             };
             c->spec.x = axis(g0.nx, kx, c->spec_tab);
             c->spec.y = axis(g0.ny, ky, c->spec_tab + tx.size());
-            c->spec.enabled = 1;
+            c->spec.enabled = spec_prepare(c->spec) ? 1 : 0;
         }
     }
     HIPCHECK(c, dalloc(c, &c->partials, partials_size(p)));
@@ -512,7 +516,8 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
 extern "C" void fsi_context_destroy(fsi_context* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense, c->spec_tab};
+    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense, c->spec_tab,
+                      c->ext};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
@@ -804,8 +809,14 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     const int ps = slot < 0 ? 0 : slot;
     const int last = c->num.projection_maxiter;
     if (fixed > 0) {
-        launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan, c->st, 0,
-                         std::min(fixed, last + 1), slot);
+        // Of the planned iterations only the convergence test of the last one
+        // matters (its update is never tested: converged earlier, it is
+        // skipped; not converged, the step is redone), so it is launched alone.
+        const int n = std::min(fixed, last + 1);
+        if (n > 1)
+            launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan, c->st, 0,
+                             n - 1, slot);
+        launch_pcg_test(c->stream, p, b, c->st, n - 1, slot);
         launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
         w->corr_cur = 1 - w->corr_cur;
         return FSI_OK;
@@ -958,6 +969,7 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         if ((rc = projection(c, w, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
         if (si <= 2) tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0);
     }
+    launch_ext_final(c->stream, p, c->st);  // the report of the last stage and the stage-1 checks
     return FSI_OK;
 }
 

@@ -58,6 +58,23 @@ struct Params {
     int force;
     double ffx, ffy;
     const double* force_row;  // KolmogorovBodyForce per-row value (ny), or null
+    // Per-block extrema and counts of the report and checks (EX_* slots of
+    // exnb values each, one per block of the cell grid); reduced in a fixed
+    // order by their consumer, so no global atomics are needed.
+    double* exbuf;
+    int exnb;
+};
+enum ExtSlot : int {
+    EX_PHI_NEGMIN = 0,  // -min phi (max-reduced)
+    EX_PHI_MAX = 1,
+    EX_EIG_NEGMIN = 2,  // -min eigenvalue of sigma
+    EX_DET_ERR = 3,     // max |det sigma - 1|
+    EX_NONSPD = 4,      // counts (sum-reduced)
+    EX_NONFINITE = 5,
+    EX_STAGE1 = 6,      // max |div v - G| of the input velocity
+    EX_NONFINITE_IN = 7,
+    EX_SPEED = 8,       // max |u_c| (dynamic ACDI gamma)
+    EX_COUNT = 9
 };
 
 // Error codes raised from kernels into Status::code (first error wins).

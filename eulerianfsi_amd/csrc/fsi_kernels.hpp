@@ -8,41 +8,6 @@
 
 namespace fsi {
 
-__device__ __forceinline__ void atomic_min_double(double* addr, double v) {
-    unsigned long long* a = (unsigned long long*)addr;
-    unsigned long long old = *a;
-    while (v < __longlong_as_double((long long)old)) {
-        unsigned long long assumed = old;
-        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
-        if (old == assumed) break;
-    }
-}
-__device__ __forceinline__ void atomic_max_double(double* addr, double v) {
-    unsigned long long* a = (unsigned long long*)addr;
-    unsigned long long old = *a;
-    while (v > __longlong_as_double((long long)old)) {
-        unsigned long long assumed = old;
-        old = atomicCAS(a, assumed, (unsigned long long)__double_as_longlong(v));
-        if (old == assumed) break;
-    }
-}
-
-__global__ void k_zero_scalar(double* x);
-
-// block maximum of a non-negative value, merged into *dst with an atomic
-__device__ __forceinline__ void block_max_to(double m, double* dst) {
-    __shared__ double lds[16];
-    m = wave_max(m);
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((lane & 63) == 0) lds[lane >> 6] = m;
-    __syncthreads();
-    if (lane == 0) {
-        double mm = lds[0];
-        for (int w = 1; w < (int)(blockDim.x * blockDim.y + 63) / 64; ++w) mm = fmax(mm, lds[w]);
-        atomic_max_double(dst, mm);
-    }
-}
-
 // One level of the projection multigrid hierarchy (cell-centred, uniform).
 struct MgLevel {
     int nx, ny, P;
@@ -93,6 +58,9 @@ void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, cons
                           const double* const* Iv, int ns, double dt, const double* ce, const double* ci,
                           double alpha, const double* bu, const double* bv, double* ru, double* rv);
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv);
+// StepReport extrema and counts, and the stage-1 checks, from the per-block
+// partials of the step's last combination and of its stage-1 tendencies
+void launch_ext_final(hipStream_t s, const Params& p, Status* st);
 
 // ---- velocity predictor CG (fsi_solvers.hip) ----
 struct VelBufs {
@@ -141,6 +109,9 @@ void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const M
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly);
 void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot);
+// the convergence test of PCG iteration m alone (the first step of that
+// iteration), on the r.r partials of the previous one
+void launch_pcg_test(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int m, int slot);
 void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, int nb, const int* done);
 
 int grid_blocks(const Params& p);
@@ -198,6 +169,9 @@ std::vector<double> spec_tables(int n, int kind, double h2inv);
 // x = solution of (-A) x = -rhs with the (0, 0) mode set to zero when A is
 // singular (the zero-mean gauge); rhs and x are cell fields of pitch P.
 void launch_spectral_solve(hipStream_t s, const SpecPlan& sp, int P, const double* rhs, double* x);
+// kernel attributes the plan's launches need (LDS above 64 KiB); false if the
+// device cannot run them
+bool spec_prepare(const SpecPlan& sp);
 
 size_t partials_size(const Params& p);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,

@@ -722,7 +722,7 @@ __global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, 
     if (lane == 0) {
         double mm = lds[0];
         for (int w = 1; w < BLOCK / 64; ++w) mm = fmax(mm, lds[w]);
-        atomic_max_double(&st->gamma_umax, mm);
+        p.exbuf[(size_t)EX_SPEED * p.exnb + blockIdx.x + blockIdx.y * gridDim.x] = mm;
     }
 }
 
@@ -847,7 +847,17 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
     const TileAcc N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
     const TileAcc N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
     double gamma = p.acdi_gamma;
-    if (PH == FSI_PHASE_ACDI && p.acdi_dyn) gamma = p.acdi_gamma * st->gamma_umax;
+    if (PH == FSI_PHASE_ACDI && p.acdi_dyn) {  // max |u_c| over the block maxima of k_max_speed
+        __shared__ double smax[BLOCK / 64];
+        double m = 0.0;
+        for (int b = tid; b < p.exnb; b += BLOCK) m = fmax(m, p.exbuf[(size_t)EX_SPEED * p.exnb + b]);
+        m = wave_max(m);
+        if ((tid & 63) == 0) smax[tid >> 6] = m;
+        __syncthreads();
+        m = smax[0];
+        for (int w = 1; w < BLOCK / 64; ++w) m = fmax(m, smax[w]);
+        gamma = p.acdi_gamma * m;
+    }
     const int i = i0 + threadIdx.x;
     if (i >= p.nx) return;
     for (int r = 0; r < ROWS; ++r) {
@@ -973,8 +983,9 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
             mm = fmax(mm, lds[w]);
             nn += lds_n[w];
         }
-        atomic_max_double(&st->stage1_linf, mm);
-        if (nn) atomicAdd((unsigned long long*)&st->nonfinite_in, (unsigned long long)nn);
+        const size_t bid = blockIdx.x + blockIdx.y * gridDim.x;
+        p.exbuf[(size_t)EX_STAGE1 * p.exnb + bid] = mm;
+        p.exbuf[(size_t)EX_NONFINITE_IN * p.exnb + bid] = (double)nn;
     }
 }
 
@@ -1065,12 +1076,13 @@ __global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, H
             n0 += lds_n[0][w];
             n1 += lds_n[1][w];
         }
-        atomic_min_double(&st->phi_min, v0);
-        atomic_max_double(&st->phi_max, v1);
-        atomic_min_double(&st->min_eig, v2);
-        atomic_max_double(&st->max_det_err, v3);
-        if (n0) atomicAdd((unsigned long long*)&st->nonspd, (unsigned long long)n0);
-        if (n1) atomicAdd((unsigned long long*)&st->nonfinite, (unsigned long long)n1);
+        const size_t bid = blockIdx.x + blockIdx.y * gridDim.x, nb = p.exnb;
+        p.exbuf[EX_PHI_NEGMIN * nb + bid] = -v0;
+        p.exbuf[EX_PHI_MAX * nb + bid] = v1;
+        p.exbuf[EX_EIG_NEGMIN * nb + bid] = -v2;
+        p.exbuf[EX_DET_ERR * nb + bid] = v3;
+        p.exbuf[EX_NONSPD * nb + bid] = (double)n0;
+        p.exbuf[EX_NONFINITE * nb + bid] = (double)n1;
     }
 }
 
@@ -1129,6 +1141,56 @@ static dim3 grid2d(const Params& p) {
     return dim3((p.nx + TILE_X - 1) / TILE_X, (p.ny + TILE_ROWS - 1) / TILE_ROWS);
 }
 
+// One block: maxima and sums of the per-block partials in a fixed order.
+__global__ void __launch_bounds__(BLOCK) k_ext_final(Params p, Status* st) {
+    constexpr int NM = 5, NS = 3;  // maxima: -phi_min, phi_max, -eig_min, det_err, stage1; sums: counts
+    const int mslot[NM] = {EX_PHI_NEGMIN, EX_PHI_MAX, EX_EIG_NEGMIN, EX_DET_ERR, EX_STAGE1};
+    const int sslot[NS] = {EX_NONSPD, EX_NONFINITE, EX_NONFINITE_IN};
+    __shared__ double lm[NM][BLOCK / 64], ls[NS][BLOCK / 64];
+    const int tid = threadIdx.x;
+    double m[NM], sm[NS];
+#pragma unroll
+    for (int q = 0; q < NM; ++q) m[q] = -INFINITY;
+#pragma unroll
+    for (int q = 0; q < NS; ++q) sm[q] = 0.0;
+    const size_t nb = p.exnb;
+    for (size_t b = tid; b < nb; b += BLOCK) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q) m[q] = fmax(m[q], p.exbuf[mslot[q] * nb + b]);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) sm[q] += p.exbuf[sslot[q] * nb + b];
+    }
+#pragma unroll
+    for (int q = 0; q < NM; ++q) m[q] = wave_max(m[q]);
+#pragma unroll
+    for (int q = 0; q < NS; ++q)
+        for (int off = 32; off > 0; off >>= 1) sm[q] += __shfl_down(sm[q], off, 64);
+    if ((tid & 63) == 0) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q) lm[q][tid >> 6] = m[q];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) ls[q][tid >> 6] = sm[q];
+    }
+    __syncthreads();
+    if (tid != 0) return;
+    for (int w = 1; w < BLOCK / 64; ++w) {
+#pragma unroll
+        for (int q = 0; q < NM; ++q) lm[q][0] = fmax(lm[q][0], lm[q][w]);
+#pragma unroll
+        for (int q = 0; q < NS; ++q) ls[q][0] += ls[q][w];
+    }
+    st->phi_min = -lm[0][0];
+    st->phi_max = lm[1][0];
+    st->min_eig = -lm[2][0];
+    st->max_det_err = fmax(lm[3][0], 0.0);
+    st->stage1_linf = fmax(lm[4][0], 0.0);
+    st->nonspd = (long long)ls[0][0];
+    st->nonfinite = (long long)ls[1][0];
+    st->nonfinite_in = (long long)ls[2][0];
+}
+
+void launch_ext_final(hipStream_t s, const Params& p, Status* st) { k_ext_final<<<1, BLOCK, 0, s>>>(p, st); }
+
 void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
                      const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
                      double* Iu, double* Iv) {
@@ -1140,10 +1202,8 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
                   const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
                   double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1) {
     Stage sg{U, V, phi, sxx, syy, sxy, reset};
-    if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn) {
-        k_zero_scalar<<<1, 1, 0, s>>>(&st->gamma_umax);
+    if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn)
         k_max_speed<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, st);
-    }
     const dim3 g = grid2d(p), blk(TILE_X, TILE_Y);
     switch (p.phase) {
         case FSI_PHASE_ACDI: k_phase<FSI_PHASE_ACDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;

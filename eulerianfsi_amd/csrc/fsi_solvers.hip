@@ -25,7 +25,6 @@
 
 namespace fsi {
 
-__global__ void k_zero_scalar(double* x) { *x = 0.0; }
 
 
 static __device__ __forceinline__ int block_id() { return blockIdx.x + blockIdx.y * gridDim.x; }
@@ -227,6 +226,77 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
 __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
                                                     int slot, double rtol, int maxiter_arg) {
     if (st->vel.done) return;
+    const int o = m & 1, n = (m + 1) & 1;
+    const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
+    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    // wrapped global index of tile point (li, lj) of the halo-1 tile, or -1
+    // outside a non-periodic wall
+    auto wrapped = [&](int li, int lj) -> long {
+        int gi = i0 + li - 1, gj = j0 + lj - 1;
+        if (gi < 0 || gi >= p.nx) {
+            if (!p.px) return -1;
+            gi = ((gi % p.nx) + p.nx) % p.nx;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (!p.py) return -1;
+            gj = ((gj % p.ny) + p.ny) % p.ny;
+        }
+        return (long)at(p, gi, gj);
+    };
+    // (1) the block's own cells (column tx, rows ty + 4r) and the halo ring
+    // (2 rows of VT_W, 2 columns of TILE_ROWS) in registers: every load is
+    // issued before the partials of the previous iteration are summed, so the
+    // two latencies overlap.  Tile slots of cells past the domain edge get the
+    // wrapped values (periodic) or 0.
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx;
+    size_t kc[ROWS];
+    bool own[ROWS], fill[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = j0 + ty + r * TILE_Y;
+        own[r] = i < p.nx && j < p.ny;
+        const long w = wrapped(tx + 1, ty + r * TILE_Y + 1);
+        fill[r] = w >= 0;
+        kc[r] = fill[r] ? (size_t)w : 0;
+    }
+    double cru[ROWS], crv[ROWS], cqu[ROWS], cqv[ROWS], cpu[ROWS], cpv[ROWS], cxu[ROWS], cxv[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        cru[r] = ru[kc[r]];
+        crv[r] = rv[kc[r]];
+        cqu[r] = qu0[kc[r]];
+        cqv[r] = qv0[kc[r]];
+        cpu[r] = pu0[kc[r]];
+        cpv[r] = pv0[kc[r]];
+        const size_t kx = own[r] ? kc[r] : 0;
+        cxu[r] = b.xu[kx];
+        cxv[r] = b.xv[kx];
+    }
+    constexpr int NRING = 2 * VT_W + 2 * TILE_ROWS;
+    int hslot = -1;
+    bool hin = false;
+    double hru = 0.0, hrv = 0.0, hqu = 0.0, hqv = 0.0, hpu = 0.0, hpv = 0.0;
+    if (tid < NRING) {
+        int li, lj;
+        if (tid < VT_W) { li = tid; lj = 0; }
+        else if (tid < 2 * VT_W) { li = tid - VT_W; lj = VT_H - 1; }
+        else if (tid < 2 * VT_W + TILE_ROWS) { li = 0; lj = tid - 2 * VT_W + 1; }
+        else { li = VT_W - 1; lj = tid - 2 * VT_W - TILE_ROWS + 1; }
+        hslot = lj * VT_W + li;
+        const long w = wrapped(li, lj);
+        hin = w >= 0;
+        const size_t kk = hin ? (size_t)w : 0;
+        hru = ru[kk];
+        hrv = rv[kk];
+        hqu = qu0[kk];
+        hqv = qv0[kk];
+        hpu = pu0[kk];
+        hpv = pv0[kk];
+    }
     double tot[4];
     get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
@@ -267,72 +337,10 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     }
     const double a = rr / pq;
     const double beta = fmax(rr - 2.0 * a * rq + a * a * qq, 0.0) / rr;
-    const int o = m & 1, n = (m + 1) & 1;
-    const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
-    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
-    const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const TileId T = tile_id();
-    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
-    // wrapped global index of tile point (li, lj) of the halo-1 tile, or -1
-    // outside a non-periodic wall
-    auto wrapped = [&](int li, int lj) -> long {
-        int gi = i0 + li - 1, gj = j0 + lj - 1;
-        if (gi < 0 || gi >= p.nx) {
-            if (!p.px) return -1;
-            gi = ((gi % p.nx) + p.nx) % p.nx;
-        }
-        if (gj < 0 || gj >= p.ny) {
-            if (!p.py) return -1;
-            gj = ((gj % p.ny) + p.ny) % p.ny;
-        }
-        return (long)at(p, gi, gj);
-    };
-    // (1) the block's own cells (column tx, rows ty + 4r) in registers: all
-    // loads issued together; tile slots of cells past the domain edge get
-    // the wrapped values (periodic) or 0
-    const int tx = threadIdx.x, ty = threadIdx.y;
-    const int i = i0 + tx;
-    size_t kc[ROWS];
-    bool own[ROWS], fill[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        const int j = j0 + ty + r * TILE_Y;
-        own[r] = i < p.nx && j < p.ny;
-        const long w = wrapped(tx + 1, ty + r * TILE_Y + 1);
-        fill[r] = w >= 0;
-        kc[r] = fill[r] ? (size_t)w : 0;
-    }
-    double cru[ROWS], crv[ROWS], cqu[ROWS], cqv[ROWS], cpu[ROWS], cpv[ROWS], cxu[ROWS], cxv[ROWS];
-#pragma unroll
-    for (int r = 0; r < ROWS; ++r) {
-        cru[r] = ru[kc[r]];
-        crv[r] = rv[kc[r]];
-        cqu[r] = qu0[kc[r]];
-        cqv[r] = qv0[kc[r]];
-        cpu[r] = pu0[kc[r]];
-        cpv[r] = pv0[kc[r]];
-        const size_t kx = own[r] ? kc[r] : 0;
-        cxu[r] = b.xu[kx];
-        cxv[r] = b.xv[kx];
-    }
-    // (2) the halo ring of the tile: 2 rows of VT_W and 2 columns of TILE_ROWS
-    constexpr int NRING = 2 * VT_W + 2 * TILE_ROWS;
     double hp_u = 0.0, hp_v = 0.0;
-    int hslot = -1;
-    if (tid < NRING) {
-        int li, lj;
-        if (tid < VT_W) { li = tid; lj = 0; }
-        else if (tid < 2 * VT_W) { li = tid - VT_W; lj = VT_H - 1; }
-        else if (tid < 2 * VT_W + TILE_ROWS) { li = 0; lj = tid - 2 * VT_W + 1; }
-        else { li = VT_W - 1; lj = tid - 2 * VT_W - TILE_ROWS + 1; }
-        hslot = lj * VT_W + li;
-        const long w = wrapped(li, lj);
-        const size_t kk = w >= 0 ? (size_t)w : 0;
-        const double hru = ru[kk], hrv = rv[kk], hqu = qu0[kk], hqv = qv0[kk], hpu = pu0[kk], hpv = pv0[kk];
-        if (w >= 0) {
-            hp_u = (hru - a * hqu) + beta * hpu;
-            hp_v = (hrv - a * hqv) + beta * hpv;
-        }
+    if (hin) {
+        hp_u = (hru - a * hqu) + beta * hpu;
+        hp_v = (hrv - a * hqv) + beta * hpv;
     }
     double r1u[ROWS], r1v[ROWS], p1u[ROWS], p1v[ROWS];
 #pragma unroll
@@ -553,6 +561,7 @@ __global__ void k_pcg_check(Status* st, const double* rr, int nb, int m, int slo
     if (threadIdx.x == 0 && (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter)) {
         st->pcg.done = 1;
         st->pcg.iter = m;
+        st->pcg.rr = tot[0];
         if (slot >= 0) st->proj_iters[slot] = m;
         if (slot >= 0 && tot[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
     }
@@ -723,16 +732,36 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
         if (!isfinite(c)) linf = INFINITY;
         acc[1] += c * c * area;
     }
-    block_max_to(linf, &st->proj.linf);
     put_partials<2>(acc, pset(b.partials, nb, PS_CHECK), nb);
+    // the block maximum as a third partial of the set (maximum taken by k_proj_final)
+    __shared__ double lmax[BLOCK / 64];
+    linf = wave_max(linf);
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) lmax[lane >> 6] = linf;
+    __syncthreads();
+    if (lane == 0) {
+        double mm = lmax[0];
+        for (int w = 1; w < BLOCK / 64; ++w) mm = fmax(mm, lmax[w]);
+        pset(b.partials, nb, PS_CHECK)[2 * (size_t)nb + blockIdx.x + blockIdx.y * gridDim.x] = mm;
+    }
 }
 
 // the checks of project! (:279-298), one block
 __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, int nb, double lxly) {
     double tot[2];
     get_totals<2>(pset(b.partials, nb, PS_CHECK), nb, tot);
+    __shared__ double lmax[BLOCK / 64];
+    double linf = 0.0;
+    const double* bmax = pset(b.partials, nb, PS_CHECK) + 2 * (size_t)nb;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) linf = fmax(linf, bmax[k]);
+    linf = wave_max(linf);
+    if ((threadIdx.x & 63) == 0) lmax[threadIdx.x >> 6] = linf;
+    __syncthreads();
     if (lane_id() != 0) return;
+    for (int w = 1; w < (int)blockDim.x / 64; ++w) linf = fmax(linf, lmax[w]);
+    linf = fmax(linf, lmax[0]);
     ProjScalars& q = st->proj;
+    q.linf = linf;
     q.resid2 = tot[0];
     q.l2sq = tot[1];
     q.lin_rel = sqrt(tot[0]) / fmax(sqrt(q.rhs2), 2.220446049250313e-16);
@@ -769,6 +798,10 @@ void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
 }
 void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot) {
     k_pcg_check<<<1, BLOCK, 0, s>>>(st, rr, nb, m, slot);
+}
+void launch_pcg_test(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int m, int slot) {
+    const int nb = grid_blocks(p);
+    k_pcg_check<<<1, BLOCK, 0, s>>>(st, b.partials + (size_t)PS_RR * 4 * nb, nb, m, slot);
 }
 void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, int nb, const int* done) {
     k_rz<<<grid2(p), kBlock, 0, s>>>(p, b, rz, nb, done);

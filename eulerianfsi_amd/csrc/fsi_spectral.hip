@@ -36,22 +36,56 @@
 
 namespace fsi {
 
-constexpr int SP_THREADS = 256;
-constexpr int SP_ITEMS = 8;  // values per thread of a 2048-point line pair
 
 __device__ __forceinline__ int brev(int j, int m) { return (int)(__brev((unsigned)j) >> (32 - m)); }
 
 // position of x_i in Makhoul's reordered sequence v
 __device__ __forceinline__ int mk_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
 
+// Per-axis tables staged in LDS at kernel start: FFT twiddles (n/2 complex),
+// the cosine kinds' post-twiddles (n complex) and, for the column pass, the
+// eigenvalues of the axis (n).
+struct SpecLds {
+    double *wr, *wi, *pr, *pi, *mu;
+};
+template <int NT, int KIND>
+__device__ __forceinline__ SpecLds load_tables(const SpecAxis& a, double* base, bool with_mu, int tid) {
+    const int n = a.n;
+    SpecLds t;
+    t.wr = base;
+    t.wi = t.wr + (n >> 1);
+    t.pr = t.wi + (n >> 1);
+    t.pi = t.pr + (KIND == SPEC_RDFT ? 0 : n);
+    t.mu = t.pi + (KIND == SPEC_RDFT ? 0 : n);
+    for (int k = tid; k < (n >> 1); k += NT) {
+        const double2 w = a.wfft[k];
+        t.wr[k] = w.x;
+        t.wi[k] = w.y;
+    }
+    if (KIND != SPEC_RDFT)
+        for (int k = tid; k < n; k += NT) {
+            const double2 w = a.wpost[k];
+            t.pr[k] = w.x;
+            t.pi[k] = w.y;
+        }
+    if (with_mu)
+        for (int k = tid; k < n; k += NT) t.mu[k] = a.mu[k];
+    return t;
+}
+// LDS values of those tables
+__host__ __device__ constexpr int spec_table_values(int n, int kind, bool with_mu) {
+    return n + (kind == SPEC_RDFT ? 0 : 2 * n) + (with_mu ? n : 0);
+}
+
 // F complex FFTs of length n = 2^m in place on re/im (F * n each, input in
 // bit-reversed order, output in natural order); twiddles wr + i wi =
 // e^{-2 pi i t/n}, t < n/2.  Ends with a barrier.
+template <int NT>
 __device__ void lds_fft(double* re, double* im, const double* wr, const double* wi, int m, int F, int tid) {
     const int n = 1 << m;
     int s = 0;
     if (m & 1) {  // radix 2, span 1: butterflies (2q, 2q+1), twiddle 1
-        for (int q = tid; q < F * (n >> 1); q += SP_THREADS) {
+        for (int q = tid; q < F * (n >> 1); q += NT) {
             const int j = 2 * q;
             const double ar = re[j], ai = im[j], br = re[j + 1], bi = im[j + 1];
             re[j] = ar + br;
@@ -67,7 +101,7 @@ __device__ void lds_fft(double* re, double* im, const double* wr, const double* 
     // pairs (j, j+2h) with w2 = e^{-2 pi i t/4h} and (j+h, j+3h) with -i w2
     for (; s < m; s += 2) {
         const int h = 1 << s;
-        for (int q = tid; q < F * (n >> 2); q += SP_THREADS) {
+        for (int q = tid; q < F * (n >> 2); q += NT) {
             const int f = q >> (m - 2), r = q & ((n >> 2) - 1);
             const int t = r & (h - 1);
             const int j = f * n + ((r >> s) << (s + 2)) + t;
@@ -95,19 +129,11 @@ __device__ void lds_fft(double* re, double* im, const double* wr, const double* 
     }
 }
 
-__device__ __forceinline__ void load_twiddles(const SpecAxis& a, double* wr, double* wi, int tid) {
-    for (int t = tid; t < (a.n >> 1); t += SP_THREADS) {
-        const double2 w = a.wfft[t];
-        wr[t] = w.x;
-        wi[t] = w.y;
-    }
-}
-
 // the two real coefficient lines of output index k from the packed spectrum
 template <int KIND>
-__device__ __forceinline__ void split_coeffs(const SpecAxis& a, const double* re, const double* im, int k,
+__device__ __forceinline__ void split_coeffs(int n, const SpecLds& t, const double* re, const double* im, int k,
                                              double& o1, double& o2) {
-    const int n = a.n, kr = (n - k) & (n - 1);
+    const int kr = (n - k) & (n - 1);
     const double zr = re[k], zi = im[k], cr = re[kr], ci = im[kr];
     const double x1r = 0.5 * (zr + cr), x1i = 0.5 * (zi - ci);
     const double x2r = 0.5 * (zi + ci), x2i = -0.5 * (zr - cr);
@@ -116,52 +142,51 @@ __device__ __forceinline__ void split_coeffs(const SpecAxis& a, const double* re
         o1 = lo ? x1r : -x1i;
         o2 = lo ? x2r : -x2i;
     } else {
-        const double2 t = a.wpost[k];
-        o1 = t.x * x1r - t.y * x1i;
-        o2 = t.x * x2r - t.y * x2i;
+        const double tr = t.pr[k], ti = t.pi[k];
+        o1 = tr * x1r - ti * x1i;
+        o2 = tr * x2r - ti * x2i;
     }
 }
 
 // the spectrum V_k (or X_k) of one real line from its coefficient line c
 template <int KIND>
-__device__ __forceinline__ void line_spectrum(const SpecAxis& a, const double* c, int k, double& vr, double& vi) {
-    const int n = a.n, h = n >> 1;
+__device__ __forceinline__ void line_spectrum(int n, const SpecLds& t, const double* c, int k, double& vr,
+                                              double& vi) {
+    const int h = n >> 1;
     if (KIND == SPEC_RDFT) {
         vr = k <= h ? c[k] : c[n - k];
         vi = (k == 0 || k == h) ? 0.0 : (k < h ? c[n - k] : -c[k]);
     } else {
         const double ck = c[k], cr = k ? c[n - k] : 0.0;
-        const double2 t = a.wpost[k];  // V = conj(t) (ck - i cr)
-        vr = t.x * ck - t.y * cr;
-        vi = -t.x * cr - t.y * ck;
+        const double tr = t.pr[k], ti = t.pi[k];  // V = conj(t) (ck - i cr)
+        vr = tr * ck - ti * cr;
+        vi = -tr * cr - ti * ck;
     }
 }
 
 // ---- rows: forward transform along x of two rows per workgroup ----------
-template <int KIND>
-__global__ void __launch_bounds__(SP_THREADS) k_spec_rows_fwd(SpecAxis ax, int nrows, int P,
-                                                              const double* __restrict__ in,
-                                                              double* __restrict__ out, double scale) {
+template <int KIND, int NT>
+__global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, int P, const double* __restrict__ in,
+                                                      double* __restrict__ out, double scale) {
+    constexpr int IT = 2048 / NT;
     extern __shared__ double sp_smem[];
     const int n = ax.n, m = ax.m, tid = threadIdx.x;
     double* re = sp_smem;
     double* im = re + n;
-    double* wr = im + n;
-    double* wi = wr + (n >> 1);
     const int j0 = 2 * blockIdx.x;
     const bool two = j0 + 1 < nrows;
     const double* r0 = in + (size_t)j0 * P;
-    double a[SP_ITEMS], b[SP_ITEMS];
+    double a[IT], b[IT];
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {  // every load in flight before the first use
-        const int i = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {  // every load in flight before the first use
+        const int i = tid + it * NT;
         a[it] = i < n ? r0[i] : 0.0;
         b[it] = (i < n && two) ? r0[i + P] : 0.0;
     }
-    load_twiddles(ax, wr, wi, tid);
+    const SpecLds t = load_tables<NT, KIND>(ax, im + n, false, tid);
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int i = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int i = tid + it * NT;
         if (i >= n) break;
         const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
         const double s = (KIND == SPEC_ALT && (i & 1)) ? -scale : scale;
@@ -170,67 +195,66 @@ __global__ void __launch_bounds__(SP_THREADS) k_spec_rows_fwd(SpecAxis ax, int n
         im[p] = s * b[it];
     }
     __syncthreads();
-    lds_fft(re, im, wr, wi, m, 1, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, tid);
     double* o0 = out + (size_t)j0 * P;
-    for (int k = tid; k < n; k += SP_THREADS) {
+    for (int k = tid; k < n; k += NT) {
         double o1, o2;
-        split_coeffs<KIND>(ax, re, im, k, o1, o2);
+        split_coeffs<KIND>(n, t, re, im, k, o1, o2);
         o0[k] = o1;
         if (two) o0[k + P] = o2;
     }
 }
 
 // ---- rows: inverse transform along x, in place ---------------------------
-template <int KIND>
-__global__ void __launch_bounds__(SP_THREADS) k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x) {
+template <int KIND, int NT>
+__global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x) {
+    constexpr int IT = 2048 / NT;
     extern __shared__ double sp_smem[];
     const int n = ax.n, m = ax.m, tid = threadIdx.x;
     double* re = sp_smem;  // the two coefficient lines, then the packed spectrum
     double* im = re + n;
-    double* wr = im + n;
-    double* wi = wr + (n >> 1);
     const int j0 = 2 * blockIdx.x;
     const bool two = j0 + 1 < nrows;
     double* r0 = x + (size_t)j0 * P;
-    double a[SP_ITEMS], b[SP_ITEMS];
+    double a[IT], b[IT];
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int i = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int i = tid + it * NT;
         a[it] = i < n ? r0[i] : 0.0;
         b[it] = (i < n && two) ? r0[i + P] : 0.0;
     }
-    load_twiddles(ax, wr, wi, tid);
+    const SpecLds t = load_tables<NT, KIND>(ax, im + n, false, tid);
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int i = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int i = tid + it * NT;
         if (i >= n) break;
         re[i] = a[it];
         im[i] = b[it];
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {  // conj(V1 + i V2), V from the coefficient lines
-        const int k = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {  // conj(V1 + i V2), V from the coefficient lines
+        const int k = tid + it * NT;
         if (k >= n) break;
         double v1r, v1i, v2r, v2i;
-        line_spectrum<KIND>(ax, re, k, v1r, v1i);
-        line_spectrum<KIND>(ax, im, k, v2r, v2i);
+        line_spectrum<KIND>(n, t, re, k, v1r, v1i);
+        line_spectrum<KIND>(n, t, im, k, v2r, v2i);
         a[it] = v1r - v2i;
         b[it] = -(v1i + v2r);
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int k = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int k = tid + it * NT;
         if (k >= n) break;
         const int p = brev(k, m);
         re[p] = a[it];
         im[p] = b[it];
     }
     __syncthreads();
-    lds_fft(re, im, wr, wi, m, 1, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, tid);
     const double inv_n = 1.0 / (double)n;
-    for (int i = tid; i < n; i += SP_THREADS) {  // z = conj(FFT(conj Z)) / n
+    for (int i = tid; i < n; i += NT) {  // z = conj(FFT(conj Z)) / n
         const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
         const double s = (KIND == SPEC_ALT && (i & 1)) ? -inv_n : inv_n;
         r0[i] = s * re[jv];
@@ -239,28 +263,29 @@ __global__ void __launch_bounds__(SP_THREADS) k_spec_rows_inv(SpecAxis ax, int n
 }
 
 // ---- columns: forward along y, divide, inverse along y; 2C columns -------
-template <int KIND, int C>
-__global__ void __launch_bounds__(SP_THREADS) k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P,
-                                                          double* x) {
+template <int KIND, int C, int NT>
+__global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, double* x) {
+    constexpr int IT = 2048 / NT;  // C n <= 2048 line-pair values per block
     extern __shared__ double sp_smem[];
     const int n = ay.n, m = ay.m, tid = threadIdx.x;
     double* re = sp_smem;  // C transforms of n
     double* im = re + C * n;
-    double* wr = im + C * n;
-    double* wi = wr + (n >> 1);
     const int x0 = blockIdx.x * 2 * C;
     constexpr int W = 2 * C;  // columns per row of the block
     const int total = n * W;
-    double v[SP_ITEMS * 2];
+    double v[2 * IT];
 #pragma unroll
-    for (int it = 0; it < 2 * SP_ITEMS; ++it) {  // element e: row e / W, column e % W
-        const int e = tid + it * SP_THREADS;
+    for (int it = 0; it < 2 * IT; ++it) {  // element e: row e / W, column e % W
+        const int e = tid + it * NT;
         v[it] = e < total ? x[(size_t)(e / W) * P + x0 + (e % W)] : 0.0;
     }
-    load_twiddles(ay, wr, wi, tid);
+    double lx[W];  // eigenvalues of the block's columns
 #pragma unroll
-    for (int it = 0; it < 2 * SP_ITEMS; ++it) {
-        const int e = tid + it * SP_THREADS;
+    for (int c = 0; c < W; ++c) lx[c] = mux[x0 + c];
+    const SpecLds t = load_tables<NT, KIND>(ay, im + C * n, true, tid);
+#pragma unroll
+    for (int it = 0; it < 2 * IT; ++it) {
+        const int e = tid + it * NT;
         if (e >= total) break;
         const int y = e / W, c = e % W;
         const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
@@ -270,55 +295,55 @@ __global__ void __launch_bounds__(SP_THREADS) k_spec_cols(SpecAxis ay, const dou
         else re[p] = s * v[it];
     }
     __syncthreads();
-    lds_fft(re, im, wr, wi, m, C, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, C, tid);
     // coefficients of the column pair f at row mode l, divided by the eigenvalue
-    double o1[SP_ITEMS], o2[SP_ITEMS];
+    double o1[IT], o2[IT];
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int q = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int q = tid + it * NT;
         if (q >= C * n) break;
         const int f = q >> m, l = q & (n - 1);
-        split_coeffs<KIND>(ay, re + f * n, im + f * n, l, o1[it], o2[it]);
-        const double my = ay.mu[l];
-        const double l1 = mux[x0 + 2 * f] + my, l2 = mux[x0 + 2 * f + 1] + my;
+        split_coeffs<KIND>(n, t, re + f * n, im + f * n, l, o1[it], o2[it]);
+        const double my = t.mu[l];
+        const double l1 = lx[2 * f] + my, l2 = lx[2 * f + 1] + my;
         o1[it] = l1 != 0.0 ? o1[it] / l1 : 0.0;
         o2[it] = l2 != 0.0 ? o2[it] / l2 : 0.0;
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int q = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int q = tid + it * NT;
         if (q >= C * n) break;
         re[q] = o1[it];  // coefficient line of column 2f in re[f n ...], 2f+1 in im[f n ...]
         im[q] = o2[it];
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int q = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int q = tid + it * NT;
         if (q >= C * n) break;
         const int f = q >> m, l = q & (n - 1);
         double v1r, v1i, v2r, v2i;
-        line_spectrum<KIND>(ay, re + f * n, l, v1r, v1i);
-        line_spectrum<KIND>(ay, im + f * n, l, v2r, v2i);
+        line_spectrum<KIND>(n, t, re + f * n, l, v1r, v1i);
+        line_spectrum<KIND>(n, t, im + f * n, l, v2r, v2i);
         o1[it] = v1r - v2i;
         o2[it] = -(v1i + v2r);
     }
     __syncthreads();
 #pragma unroll
-    for (int it = 0; it < SP_ITEMS; ++it) {
-        const int q = tid + it * SP_THREADS;
+    for (int it = 0; it < IT; ++it) {
+        const int q = tid + it * NT;
         if (q >= C * n) break;
         const int p = (q >> m) * n + brev(q & (n - 1), m);
         re[p] = o1[it];
         im[p] = o2[it];
     }
     __syncthreads();
-    lds_fft(re, im, wr, wi, m, C, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, C, tid);
     const double inv_n = 1.0 / (double)n;
 #pragma unroll
-    for (int it = 0; it < 2 * SP_ITEMS; ++it) {
-        const int e = tid + it * SP_THREADS;
+    for (int it = 0; it < 2 * IT; ++it) {
+        const int e = tid + it * NT;
         if (e >= total) break;
         const int y = e / W, c = e % W;
         const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
@@ -363,25 +388,70 @@ std::vector<double> spec_tables(int n, int kind, double h2inv) {
     return t;
 }
 
+constexpr int SP_ROW_THREADS = 256, SP_COL_THREADS = 512;
+
+static size_t rows_lds(int n, int kind) { return (2 * (size_t)n + spec_table_values(n, kind, false)) * sizeof(double); }
+// two column pairs per block where n allows (rows of 32 bytes), else one
+static int cols_pairs(int n, int nx) { return (n <= SPEC_MAX_N / 2 && nx % 4 == 0) ? 2 : 1; }
+static size_t cols_lds(int n, int kind, int C) {
+    return (2 * (size_t)C * n + spec_table_values(n, kind, true)) * sizeof(double);
+}
+
 template <int KX>
 static void rows_fwd(hipStream_t s, const SpecPlan& sp, int P, const double* in, double* out) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_fwd<KX><<<(rows + 1) / 2, SP_THREADS, (size_t)3 * n * sizeof(double), s>>>(sp.x, rows, P, in, out,
-                                                                                            -1.0);
+    k_spec_rows_fwd<KX, SP_ROW_THREADS><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(sp.x, rows, P, in,
+                                                                                              out, -1.0);
 }
 template <int KX>
 static void rows_inv(hipStream_t s, const SpecPlan& sp, int P, double* x) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_inv<KX><<<(rows + 1) / 2, SP_THREADS, (size_t)3 * n * sizeof(double), s>>>(sp.x, rows, P, x);
+    k_spec_rows_inv<KX, SP_ROW_THREADS><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(sp.x, rows, P, x);
 }
 template <int KY>
 static void cols(hipStream_t s, const SpecPlan& sp, int P, double* x) {
     const int n = sp.y.n, nx = sp.x.n;
-    const size_t lds2 = ((size_t)4 * n + n) * sizeof(double), lds1 = ((size_t)2 * n + n) * sizeof(double);
-    if (n <= SPEC_MAX_N / 2 && nx % 4 == 0)
-        k_spec_cols<KY, 2><<<nx / 4, SP_THREADS, lds2, s>>>(sp.y, sp.x.mu, P, x);
+    if (cols_pairs(n, nx) == 2)
+        k_spec_cols<KY, 2, SP_COL_THREADS><<<nx / 4, SP_COL_THREADS, cols_lds(n, KY, 2), s>>>(sp.y, sp.x.mu, P, x);
     else
-        k_spec_cols<KY, 1><<<nx / 2, SP_THREADS, lds1, s>>>(sp.y, sp.x.mu, P, x);
+        k_spec_cols<KY, 1, SP_COL_THREADS><<<nx / 2, SP_COL_THREADS, cols_lds(n, KY, 1), s>>>(sp.y, sp.x.mu, P, x);
+}
+
+// dynamic LDS above the 64 KiB default needs the kernel attribute (up to the
+// device's per-block maximum)
+static bool allow_lds(const void* kernel, size_t bytes) {
+    if (bytes <= 65536) return true;
+    int dev = 0, mx = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        bytes > (size_t)mx ||
+        hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return true;
+}
+template <int KX>
+static bool prepare_rows(int n) {
+    return allow_lds(reinterpret_cast<const void*>(&k_spec_rows_fwd<KX, SP_ROW_THREADS>), rows_lds(n, KX)) &&
+           allow_lds(reinterpret_cast<const void*>(&k_spec_rows_inv<KX, SP_ROW_THREADS>), rows_lds(n, KX));
+}
+template <int KY>
+static bool prepare_cols(int n, int nx) {
+    const int C = cols_pairs(n, nx);
+    const void* k = C == 2 ? reinterpret_cast<const void*>(&k_spec_cols<KY, 2, SP_COL_THREADS>)
+                           : reinterpret_cast<const void*>(&k_spec_cols<KY, 1, SP_COL_THREADS>);
+    return allow_lds(k, cols_lds(n, KY, C));
+}
+
+bool spec_prepare(const SpecPlan& sp) {
+    bool ok = sp.x.kind == SPEC_COS ? prepare_rows<SPEC_COS>(sp.x.n)
+              : sp.x.kind == SPEC_ALT ? prepare_rows<SPEC_ALT>(sp.x.n)
+                                      : prepare_rows<SPEC_RDFT>(sp.x.n);
+    ok = ok && (sp.y.kind == SPEC_COS ? prepare_cols<SPEC_COS>(sp.y.n, sp.x.n)
+                : sp.y.kind == SPEC_ALT ? prepare_cols<SPEC_ALT>(sp.y.n, sp.x.n)
+                                        : prepare_cols<SPEC_RDFT>(sp.y.n, sp.x.n));
+    return ok;
 }
 
 void launch_spectral_solve(hipStream_t s, const SpecPlan& sp, int P, const double* rhs, double* x) {
