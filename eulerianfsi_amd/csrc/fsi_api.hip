@@ -749,21 +749,24 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         launch_vel_copy(c->stream, p, b);
         return FSI_OK;
     }
-    // warm start: stage `slot` of the previous step (extrapolated in time
-    // once two steps are known) instead of u^n
+    // warm start: stage `slot` of the previous step, extrapolated in time
+    // (2 u^{n-1} - u^{n-2}, formed by the start kernel) once two steps are
+    // known, instead of u^n
     const bool warm = slot >= 0 && c->stage_warm && w->stage_warm >= 1;
     if (warm) {
-        launch_extrapolate2(c->stream, p, w->vel_cur_u[slot], w->vel_prev_u[slot], w->vel_cur_v[slot],
-                            w->vel_prev_v[slot], w->stage_warm >= 2);
         b.Un = w->vel_cur_u[slot];
         b.Vn = w->vel_cur_v[slot];
+        b.Pu = w->vel_prev_u[slot];
+        b.Pv = w->vel_prev_v[slot];
+        b.extrap = w->stage_warm >= 2;
     }
     launch_vel_init(c->stream, p, b, alpha, c->st);
     const double vrtol = c->num.velocity_rtol;
     const int vmax = c->num.velocity_maxiter;
     if (fixed > 0) {
         launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax);
-        if (slot >= 0 && c->stage_warm) launch_copy2(c->stream, p, w->Us, w->vel_cur_u[slot], w->Vs, w->vel_cur_v[slot]);
+        if (slot >= 0 && c->stage_warm)
+            launch_copy2(c->stream, p, w->Us, w->vel_prev_u[slot], w->Vs, w->vel_prev_v[slot]);
         return FSI_OK;
     }
     // kernel m tests the residual of iterate m and performs update m+1
@@ -779,7 +782,11 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         chunk = 4;
     }
     c->last_vel_iters[slot] = std::max(1, iter);
-    if (slot >= 0 && c->stage_warm) launch_copy2(c->stream, p, w->Us, w->vel_cur_u[slot], w->Vs, w->vel_cur_v[slot]);
+    // the solution goes to the older history buffer; the two trade roles once
+    // the step has succeeded (fsi_integrate_step), so a redone step still
+    // starts from the same history
+    if (slot >= 0 && c->stage_warm)
+        launch_copy2(c->stream, p, w->Us, w->vel_prev_u[slot], w->Vs, w->vel_prev_v[slot]);
     return FSI_OK;
 }
 
@@ -1132,7 +1139,13 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
     std::swap(s->sxx, w->sxxs);
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
-    if (c->stage_warm && w->stage_warm < 2) ++w->stage_warm;
+    if (c->stage_warm) {
+        for (int k = 0; k < 3; ++k) {
+            std::swap(w->vel_cur_u[k], w->vel_prev_u[k]);
+            std::swap(w->vel_cur_v[k], w->vel_prev_v[k]);
+        }
+        if (w->stage_warm < 2) ++w->stage_warm;
+    }
     if (rep) {
         rep->maximum_projection_constraint_linf = std::fmax(st.stage1_linf, st.max_constraint_linf);
         rep->maximum_growth_trace_error = 0.0;

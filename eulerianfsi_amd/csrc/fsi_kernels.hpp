@@ -64,8 +64,11 @@ void launch_ext_final(hipStream_t s, const Params& p, Status* st);
 
 // ---- velocity predictor CG (fsi_solvers.hip) ----
 struct VelBufs {
-    const double* Un;  // warm start uv_n (state velocity)
+    const double* Un;  // warm start: uv_n (state velocity) or the previous step's stage solution
     const double* Vn;
+    const double* Pu;  // the stage solution of the step before (extrap: x0 = 2 Un - Pu)
+    const double* Pv;
+    int extrap;
     const double* ru;  // right-hand side (stage sum + alpha*b)
     const double* rv;
     double* xu;  // solution, written into the stage velocity arrays
@@ -100,7 +103,6 @@ struct ProjBufs {
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
 void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap);
-void launch_extrapolate2(hipStream_t s, const Params& p, double* cu, double* pu, double* cv, double* pv, int extrap);
 void launch_copy2(hipStream_t s, const Params& p, const double* a1, double* b1, const double* a2, double* b2);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter);

@@ -113,54 +113,65 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
         }
         return true;
     };
-    // x0 on halo 2 (indices, then all loads back to back)
-    {
-        constexpr int N2 = T2::R * T2::W, NI2 = (N2 + BLOCK - 1) / BLOCK;
-        size_t idx[NI2];
-        int fl[NI2];  // 1: inside, 2: U-left Dirichlet column, 4: V-bottom Dirichlet row
+    // x0 on halo 2 and the right-hand side on halo 1: indices, then every
+    // load back to back (one latency for both tiles).  x0 = 2 cur - prev
+    // (extrapolated warm start) or cur.
+    constexpr int N2 = T2::R * T2::W, NI2 = (N2 + BLOCK - 1) / BLOCK;
+    constexpr int N1 = T1::R * T1::W, NI1 = (N1 + BLOCK - 1) / BLOCK;
+    size_t idx2[NI2], idx[NI1];
+    int fl[NI2];  // 1: inside, 2: U-left Dirichlet column, 4: V-bottom Dirichlet row
+    bool inb[NI1];
+#pragma unroll
+    for (int it = 0; it < NI2; ++it) {
+        const int k = min(tid + it * BLOCK, N2 - 1);
+        int gi, gj;
+        const bool in = wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj);
+        idx2[it] = in ? at(p, gi, gj) : 0;
+        fl[it] = in ? (1 | (dir_u(p, gi) ? 2 : 0) | (dir_v(p, gj) ? 4 : 0)) : 0;
+    }
+#pragma unroll
+    for (int it = 0; it < NI1; ++it) {
+        const int k = min(tid + it * BLOCK, N1 - 1);
+        int gi, gj;
+        inb[it] = wrapped(i0 + k % T1::W - 1, j0 + k / T1::W - 1, gi, gj);
+        idx[it] = inb[it] ? at(p, gi, gj) : 0;
+    }
+    double xu[NI2], xv[NI2], bu[NI1], bv[NI1];
+#pragma unroll
+    for (int it = 0; it < NI2; ++it) {
+        xu[it] = b.Un[idx2[it]];
+        xv[it] = b.Vn[idx2[it]];
+    }
+#pragma unroll
+    for (int it = 0; it < NI1; ++it) {
+        bu[it] = b.ru[idx[it]];
+        bv[it] = b.rv[idx[it]];
+    }
+    if (b.extrap) {
+        double yu[NI2], yv[NI2];
 #pragma unroll
         for (int it = 0; it < NI2; ++it) {
-            const int k = min(tid + it * BLOCK, N2 - 1);
-            int gi, gj;
-            const bool in = wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj);
-            idx[it] = in ? at(p, gi, gj) : 0;
-            fl[it] = in ? (1 | (dir_u(p, gi) ? 2 : 0) | (dir_v(p, gj) ? 4 : 0)) : 0;
-        }
-        double xu[NI2], xv[NI2];
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            xu[it] = b.Un[idx[it]];
-            xv[it] = b.Vn[idx[it]];
+            yu[it] = b.Pu[idx2[it]];
+            yv[it] = b.Pv[idx2[it]];
         }
 #pragma unroll
         for (int it = 0; it < NI2; ++it) {
-            const int k = tid + it * BLOCK;
-            if (k >= N2) break;
-            sxu[k] = (fl[it] & 1) ? ((fl[it] & 2) ? gul : xu[it]) : 0.0;
-            sxv[k] = (fl[it] & 1) ? ((fl[it] & 4) ? gvb : xv[it]) : 0.0;
+            xu[it] = 2.0 * xu[it] - yu[it];
+            xv[it] = 2.0 * xv[it] - yv[it];
         }
+    }
+#pragma unroll
+    for (int it = 0; it < NI2; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k >= N2) break;
+        sxu[k] = (fl[it] & 1) ? ((fl[it] & 2) ? gul : xu[it]) : 0.0;
+        sxv[k] = (fl[it] & 1) ? ((fl[it] & 4) ? gvb : xv[it]) : 0.0;
     }
     __syncthreads();
     const T2 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
     // r0 on halo 1; |r0|^2 and |b_F|^2 over the block's own cells
     double acc2[2] = {0.0, 0.0};
     {
-        constexpr int N1 = T1::R * T1::W, NI1 = (N1 + BLOCK - 1) / BLOCK;
-        size_t idx[NI1];
-        bool inb[NI1];
-#pragma unroll
-        for (int it = 0; it < NI1; ++it) {
-            const int k = min(tid + it * BLOCK, N1 - 1);
-            int gi, gj;
-            inb[it] = wrapped(i0 + k % T1::W - 1, j0 + k / T1::W - 1, gi, gj);
-            idx[it] = inb[it] ? at(p, gi, gj) : 0;
-        }
-        double bu[NI1], bv[NI1];
-#pragma unroll
-        for (int it = 0; it < NI1; ++it) {
-            bu[it] = b.ru[idx[it]];
-            bv[it] = b.rv[idx[it]];
-        }
 #pragma unroll
         for (int it = 0; it < NI1; ++it) {
             const int k = tid + it * BLOCK;
@@ -819,20 +830,6 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restr
     }
 }
 // the same for the two velocity components, and a two-field copy
-__global__ void __launch_bounds__(BLOCK) k_extrapolate2(Params p, double* __restrict__ cu, double* __restrict__ pu,
-                                                        double* __restrict__ cv, double* __restrict__ pv,
-                                                        int extrap) {
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        const double a = cu[k], b = cv[k];
-        if (extrap) {
-            cu[k] = 2.0 * a - pu[k];
-            cv[k] = 2.0 * b - pv[k];
-        }
-        pu[k] = a;
-        pv[k] = b;
-    }
-}
 __global__ void __launch_bounds__(BLOCK) k_copy2(Params p, const double* __restrict__ a1, double* __restrict__ b1,
                                                  const double* __restrict__ a2, double* __restrict__ b2) {
     FOR_TILE(p) {
@@ -840,10 +837,6 @@ __global__ void __launch_bounds__(BLOCK) k_copy2(Params p, const double* __restr
         b1[k] = a1[k];
         b2[k] = a2[k];
     }
-}
-void launch_extrapolate2(hipStream_t s, const Params& p, double* cu, double* pu, double* cv, double* pv,
-                         int extrap) {
-    k_extrapolate2<<<grid2(p), kBlock, 0, s>>>(p, cu, pu, cv, pv, extrap);
 }
 void launch_copy2(hipStream_t s, const Params& p, const double* a1, double* b1, const double* a2, double* b2) {
     k_copy2<<<grid2(p), kBlock, 0, s>>>(p, a1, b1, a2, b2);
