@@ -755,6 +755,208 @@ struct TileAcc {  // accessor of an LDS tile by global (wrapped) index
     }
 };
 
+// ACDI right-hand side (phase_field_rhs, PhaseFieldMethods.jl:652-671), formed
+// face by face: every x- and y-face flux -f + A of the tile (convective flux
+// f, regularisation flux A) is evaluated once into LDS and each cell takes
+// the divergence.  With q = clamp(phi), r = (q + e)/(1 - q + e), psi = eps log r
+// and the interior face value psi_f = (psi_a + psi_b)/2 of interpolate_center_to_*,
+// the amplitude 1/4 (1 - tanh^2(psi_f / 2 eps)) (:603-614) equals s/(1 + s)^2
+// with s = sqrt(r_a) sqrt(r_b), so no tanh is evaluated.  The face
+// velocities are loaded with the phi tile.
+// s/(1 + s)^2 = 1/4 (1 - tanh^2(y)) with s = e^{2y}; zero where tanh(y) rounds
+// to +-1 in double (|y| > 55 ln2 / 2, i.e. s outside [2^-55, 2^55]), as the
+// tanh form is, so regions of phi = 0 or 1 receive no flux.
+__device__ __forceinline__ double acdi_amp(double sv) {
+    if (!(sv >= 0x1p-55 && sv <= 0x1p55)) return 0.0;
+    return sv / ((1.0 + sv) * (1.0 + sv));
+}
+constexpr int AF_XW = TILE_X + 1, AF_YH = TILE_ROWS + 1;  // 65 x-faces per row, 17 rows of y-faces
+constexpr int AF_NX = TILE_ROWS * AF_XW, AF_NY = AF_YH * TILE_X;
+__global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
+                                                      Status* st) {
+    __shared__ double s_phi[CR2 * CW2];
+    __shared__ double s_psi[CR2 * CW2];  // psi; after the normals, the x-face fluxes
+    __shared__ double s_sr[CR2 * CW2];   // sqrt(r)
+    __shared__ double s_n0[CR1 * CW1];
+    __shared__ double s_n1[CR1 * CW1];
+    __shared__ double s_fy[AF_NY];
+    static_assert(AF_NX <= CR2 * CW2, "x-face fluxes fit the psi tile");
+    double* s_fx = s_psi;
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
+    // 1. phi on the halo-2 tile and the tile's face velocities: indices, then
+    // every load back to back
+    constexpr int NPT = CR2 * CW2, NIT = (NPT + BLOCK - 1) / BLOCK;
+    constexpr int NUX = (AF_NX + BLOCK - 1) / BLOCK, NVY = (AF_NY + BLOCK - 1) / BLOCK;
+    size_t idx[NIT], iu[NUX], iv[NVY];
+    bool inb[NIT];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = min(tid + it * BLOCK, NPT - 1);
+        int gi = i0 + k % CW2 - CH2, gj = j0 + k / CW2 - CH2;
+        bool in = true;
+        if (gi < 0 || gi >= p.nx) {
+            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+            else in = false;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+            else in = false;
+        }
+        inb[it] = in;
+        idx[it] = in ? at(p, gi, gj) : 0;
+    }
+#pragma unroll
+    for (int it = 0; it < NUX; ++it) {  // x-face (i0 + fi, j0 + fj); face nx of a periodic x is face 0
+        const int q = min(tid + it * BLOCK, AF_NX - 1);
+        const int k = i0 + q % AF_XW, j = j0 + q / AF_XW;
+        const bool ok = k <= p.nx && j < p.ny;
+        iu[it] = ok ? at(p, (p.px && k == p.nx) ? 0 : k, j) : 0;
+    }
+#pragma unroll
+    for (int it = 0; it < NVY; ++it) {  // y-face (i0 + fi, j0 + fj)
+        const int q = min(tid + it * BLOCK, AF_NY - 1);
+        const int i = i0 + q % TILE_X, k = j0 + q / TILE_X;
+        const bool ok = i < p.nx && k <= p.ny;
+        iv[it] = ok ? at(p, i, (p.py && k == p.ny) ? 0 : k) : 0;
+    }
+    double vals[NIT], uf[NUX], vf[NVY];
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) vals[it] = s.phi[idx[it]];
+#pragma unroll
+    for (int it = 0; it < NUX; ++it) uf[it] = s.U[iu[it]];
+#pragma unroll
+    for (int it = 0; it < NVY; ++it) vf[it] = s.V[iv[it]];
+    const double eps = p.acdi_eps, floor_ = p.acdi_floor;
+#pragma unroll
+    for (int it = 0; it < NIT; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k >= NPT) break;
+        double v = 0.0, ps = 0.0, sr = 0.0;
+        if (inb[it]) {
+            v = vals[it];
+            const double q = clamp01(v);
+            const double r = (q + floor_) / (1.0 - q + floor_);
+            ps = eps * log(r);
+            sr = sqrt(r);
+        }
+        s_phi[k] = v;
+        s_psi[k] = ps;
+        s_sr[k] = sr;
+    }
+    double gamma = p.acdi_gamma;
+    if (p.acdi_dyn) {  // max |u_c| over the block maxima of k_max_speed
+        __shared__ double smax[BLOCK / 64];
+        double m = 0.0;
+        for (int b = tid; b < p.exnb; b += BLOCK) m = fmax(m, p.exbuf[(size_t)EX_SPEED * p.exnb + b]);
+        m = wave_max(m);
+        if ((tid & 63) == 0) smax[tid >> 6] = m;
+        __syncthreads();
+        m = smax[0];
+        for (int w = 1; w < BLOCK / 64; ++w) m = fmax(m, smax[w]);
+        gamma = p.acdi_gamma * m;
+    }
+    __syncthreads();
+    const TileAcc f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TileAcc fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TileAcc fsr{s_sr, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    // 2. normals of psi (_center_normals, :233-244) on the halo-1 tile
+    for (int k = tid; k < CR1 * CW1; k += BLOCK) {
+        int gi = i0 + k % CW1 - 1, gj = j0 + k / CW1 - 1;
+        bool in = true;
+        if (gi < 0 || gi >= p.nx) {
+            if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
+            else in = false;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
+            else in = false;
+        }
+        double n0 = 0.0, n1 = 0.0;
+        if (in) {
+            double gx, gy;
+            grad_center(p, bpsi, fpsi, gi, gj, gx, gy);
+            const double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
+            n0 = gx / mag;
+            n1 = gy / mag;
+        }
+        s_n0[k] = n0;
+        s_n1[k] = n1;
+    }
+    __syncthreads();
+    const TileAcc N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    const TileAcc N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    // 3. face fluxes -f + A, once per face (psi is no longer needed: its tile
+    // takes the x-face fluxes)
+    double fxv[NUX], fyv[NVY];
+#pragma unroll
+    for (int it = 0; it < NUX; ++it) {
+        const int q = tid + it * BLOCK;
+        const int k = i0 + q % AF_XW, j = j0 + q / AF_XW;
+        double F = 0.0;
+        if (q < AF_NX && k <= p.nx && j < p.ny) {
+            const double u = uf[it];
+            if (!p.px && k == 0) {  // walls: boundary data in f, no regularisation flux (:637-648)
+                F = -u * phase_bvalue(p.bPhi.l, f(0, j), st);
+            } else if (!p.px && k == p.nx) {
+                F = -u * phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
+            } else {
+                const int ka = k == 0 ? p.nx - 1 : k - 1, kb = k == p.nx ? 0 : k;
+                const double pa = f(ka, j), pb = f(kb, j);
+                const double amp = acdi_amp(fsr(ka, j) * fsr(kb, j));
+                const double nf = 0.5 * N0(ka, j) + 0.5 * N0(kb, j);
+                const double A = gamma * (eps * (p.ihx * pb - p.ihx * pa) - amp * nf);
+                F = A - u * (0.5 * pa + 0.5 * pb);
+            }
+        }
+        fxv[it] = F;
+    }
+#pragma unroll
+    for (int it = 0; it < NVY; ++it) {
+        const int q = tid + it * BLOCK;
+        const int i = i0 + q % TILE_X, k = j0 + q / TILE_X;
+        double F = 0.0;
+        if (q < AF_NY && i < p.nx && k <= p.ny) {
+            const double v = vf[it];
+            if (!p.py && k == 0) {
+                F = -v * phase_bvalue(p.bPhi.b, f(i, 0), st);
+            } else if (!p.py && k == p.ny) {
+                F = -v * phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
+            } else {
+                const int ka = k == 0 ? p.ny - 1 : k - 1, kb = k == p.ny ? 0 : k;
+                const double pa = f(i, ka), pb = f(i, kb);
+                const double amp = acdi_amp(fsr(i, ka) * fsr(i, kb));
+                const double nf = 0.5 * N1(i, ka) + 0.5 * N1(i, kb);
+                const double A = gamma * (eps * (p.ihy * pb - p.ihy * pa) - amp * nf);
+                F = A - v * (0.5 * pa + 0.5 * pb);
+            }
+        }
+        fyv[it] = F;
+    }
+    __syncthreads();  // the psi tile is free now
+#pragma unroll
+    for (int it = 0; it < NUX; ++it) {
+        const int q = tid + it * BLOCK;
+        if (q < AF_NX) s_fx[q] = fxv[it];
+    }
+#pragma unroll
+    for (int it = 0; it < NVY; ++it) {
+        const int q = tid + it * BLOCK;
+        if (q < AF_NY) s_fy[q] = fyv[it];
+    }
+    __syncthreads();
+    // 4. E_phi = div of the face fluxes
+    const int i = i0 + threadIdx.x;
+    if (i >= p.nx) return;
+    for (int r = 0; r < ROWS; ++r) {
+        const int lj = threadIdx.y + r * TILE_Y, j = j0 + lj;
+        if (j >= p.ny) break;
+        const double* fx = s_fx + lj * AF_XW + threadIdx.x;
+        const double* fy = s_fy + lj * TILE_X + threadIdx.x;
+        Ephi[at(p, i, j)] = (fx[1] - fx[0]) / p.hx + (fy[TILE_X] - fy[0]) / p.hy;
+    }
+}
+
 // Phase-field right-hand side, one instantiation per phase method.
 template <int PH>
 __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
@@ -1206,7 +1408,7 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
         k_max_speed<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, st);
     const dim3 g = grid2d(p), blk(TILE_X, TILE_Y);
     switch (p.phase) {
-        case FSI_PHASE_ACDI: k_phase<FSI_PHASE_ACDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+        case FSI_PHASE_ACDI: k_phase_acdi<<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
         case FSI_PHASE_UWCDI: k_phase<FSI_PHASE_UWCDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
         case FSI_PHASE_CDI: k_phase<FSI_PHASE_CDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
         default: k_phase<FSI_PHASE_UPWIND><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
