@@ -82,6 +82,9 @@ def main(argv=None):
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--only-kernel", default=None, help="time one kernel standalone and exit (profiling runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--ensemble-steps", type=int, default=10,
+                    help="timed steps of the 64-cavity ensemble, stepped one by one and batched (0: skip)")
+    ap.add_argument("--ensemble-warmup", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="precision of the main run")
     args = ap.parse_args(argv)
 
@@ -198,6 +201,52 @@ def main(argv=None):
               file=sys.stderr, flush=True)
         del mwork, mctx
 
+    # batched ensemble: 64 cavities (N in {128, 192, 256, 384}, G, disk drawn
+    # from a seed), the same steps once one instance after another
+    # (integrate_step) and once batched (ensemble_step, every instance's step
+    # in flight together); aggregate cell updates per second
+    ensemble = None
+    if args.ensemble_steps > 0:
+        ecases = setups.ensemble_cases(velocity_rtol=1e-8, projection_rtol=1e-10)
+        ectx, ework, ets = [], [], []
+        for c in ecases:
+            cx = fsi.build_imex_context(c.metrics, c.bcs, c.models, c.numerics)
+            wk = fsi.allocate_imex_workspace(cx)
+            fsi.make_velocity_compatible(c.state, cx, wk)
+            ectx.append(cx)
+            ework.append(wk)
+            ets.append(0.0)
+        states, dts = [c.state for c in ecases], [c.dt for c in ecases]
+        cells = sum(c.params["N"] ** 2 for c in ecases)
+
+        def batched_step():
+            fsi.ensemble_step(states, ets, dts, ectx, ework)
+            for k in range(len(ets)):
+                ets[k] += dts[k]
+
+        for _ in range(args.ensemble_warmup):  # the first steps poll and record iteration counts
+            batched_step()
+        print(f"ensemble: {len(ecases)} instances, {cells / 1e6:.2f} M cells, warm", file=sys.stderr, flush=True)
+        s0 = time.perf_counter()
+        for _ in range(args.ensemble_steps):
+            for k in range(len(ecases)):
+                fsi.integrate_step(states[k], ets[k], dts[k], ectx[k], ework[k])
+                ets[k] += dts[k]
+        t_seq = time.perf_counter() - s0
+        s0 = time.perf_counter()
+        for _ in range(args.ensemble_steps):
+            batched_step()
+        t_ens = time.perf_counter() - s0
+        ensemble = {"workload": "64 lid-driven cavities, N ~ U{128, 192, 256, 384}, G log-uniform [0.05, 0.5], "
+                                "disk r ~ U[0.15, 0.25], centre ~ U[0.4, 0.6]^2, ACDI, FP64, dt = 1e-3*128/N, seed 4",
+                    "instances": len(ecases), "cells": cells, "steps": args.ensemble_steps,
+                    "cell_updates_per_s_batched": cells * args.ensemble_steps / t_ens,
+                    "cell_updates_per_s_one_by_one": cells * args.ensemble_steps / t_seq,
+                    "speedup": t_seq / t_ens}
+        print(f"ensemble: one by one {t_seq:.3f} s, batched {t_ens:.3f} s, speedup {t_seq / t_ens:.2f}",
+              file=sys.stderr, flush=True)
+        del ework, ectx, states
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle.oracle import Oracle, state_dict
@@ -221,7 +270,8 @@ def main(argv=None):
                                f"FP64, dt={dt:.4g}, seed {args.seed}, velocity rtol 1e-8, projection rtol 1e-10",
                    "N": args.N, "method": args.method, "cell_updates_per_s": steps_per_s * args.N ** 2,
                    "velocity_iterations": vel_mean.tolist(), "projection_iterations": proj_mean.tolist(),
-                   "precision": args.precision, "mixed_precision": mixed, "adjoint_grads_per_s": None},
+                   "precision": args.precision, "mixed_precision": mixed, "ensemble": ensemble,
+                   "adjoint_grads_per_s": None},
         "roofline": roofline, "whole_step": whole_step, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -20,7 +20,7 @@ from .errors import ArgumentError, DeviceError, DimensionMismatch, DomainError, 
 from .imex import (A_EXPLICIT, A_IMPLICIT, STAGE_ABSCISSA, FourStageIMEXFractionalStep, FSIBoundaryConditions,
                    FSIModels, FSIState, IMEXContext, IMEXNumerics, IMEXWorkspace, LinearSolverOptions,
                    ProjectionResult, StepReport, allocate_imex_workspace, build_imex_context,
-                   evaluate_stage_tendencies, integrate_step, make_velocity_compatible, project)
+                   ensemble_step, evaluate_stage_tendencies, integrate_step, make_velocity_compatible, project)
 from .mesh import MACGrid, MeshMetrics, build_mesh_metrics, generate_grid
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -227,6 +227,8 @@ def _declare(lib):
         "fsi_workspace_upload_correction": (c_int, [P, P, _dp]),
         "fsi_make_velocity_compatible": (c_int, [P, P, P, POINTER(ProjectionResultC)]),
         "fsi_integrate_step": (c_int, [P, P, P, c_double, c_double, POINTER(StepReportC), POINTER(StepStatsC)]),
+        "fsi_ensemble_step": (c_int, [c_int32, POINTER(P), POINTER(P), POINTER(P), _dp, _dp, POINTER(StepReportC),
+                                      POINTER(StepStatsC), POINTER(c_int32)]),
         "fsi_project": (c_int, [P, P, P, POINTER(ProjectionResultC)]),
         "fsi_eval_tendencies": (c_int, [P, P, P, c_double] + [_dp] * 8),
         "fsi_benchmark_kernel": (c_int, [P, P, P, c_char_p, c_int32, _dp, _dp]),
@@ -243,7 +245,8 @@ EXPORTED = (
     "fsi_context_create", "fsi_context_destroy", "fsi_last_error", "fsi_state_create", "fsi_state_destroy",
     "fsi_state_upload", "fsi_state_download", "fsi_workspace_create", "fsi_workspace_destroy",
     "fsi_workspace_download_correction", "fsi_workspace_upload_correction", "fsi_make_velocity_compatible",
-    "fsi_integrate_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel", "fsi_version",
+    "fsi_integrate_step", "fsi_ensemble_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel",
+    "fsi_version",
 )
 
 

@@ -57,6 +57,7 @@ struct fsi_context {
     double* partials = nullptr;
     Status* st = nullptr;       // device
     Status* st_host = nullptr;  // pinned mirror
+    Status* st_reset = nullptr; // pinned, the Status every step starts from (never written after set-up)
     std::vector<MgGeom> geo;
     CoarsePlan cplan{};         // single-workgroup coarse solve of geo.back()
     double* co_dense = nullptr;  // its dense bottom inverse
@@ -508,6 +509,11 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     HIPCHECK(c, hipMemsetAsync(c->st, 0, sizeof(Status), c->stream));
     HIPCHECK(c, hipHostMalloc((void**)&c->st_host, sizeof(Status), hipHostMallocDefault));
     std::memset(c->st_host, 0, sizeof(Status));
+    HIPCHECK(c, hipHostMalloc((void**)&c->st_reset, sizeof(Status), hipHostMallocDefault));
+    std::memset(c->st_reset, 0, sizeof(Status));
+    c->st_reset->phi_min = INFINITY;
+    c->st_reset->phi_max = -INFINITY;
+    c->st_reset->min_eig = INFINITY;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipGetLastError());
     return FSI_OK;
@@ -523,6 +529,7 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->st_reset) (void)hipHostFree(c->st_reset);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -705,6 +712,11 @@ static fsi_status status_push(fsi_context* c) {
     HIPCHECK(c, hipMemcpyAsync(c->st, c->st_host, sizeof(Status), hipMemcpyHostToDevice, c->stream));
     // the pinned buffer is rewritten by the next status_pull; wait for the copy
     HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+// The step's starting Status from the constant pinned copy: nothing to wait for.
+static fsi_status status_push_reset(fsi_context* c) {
+    HIPCHECK(c, hipMemcpyAsync(c->st, c->st_reset, sizeof(Status), hipMemcpyHostToDevice, c->stream));
     return FSI_OK;
 }
 static fsi_status status_pull(fsi_context* c) {
@@ -1030,8 +1042,7 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
     return FSI_OK;
 }
 
-extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double t, double dt,
-                                         fsi_step_report* rep, fsi_step_stats* stats) {
+static fsi_status step_check(fsi_context* c, fsi_state* s, fsi_workspace* w, double t, double dt) {
     if (!c || !s || !w) return FSI_E_ARGUMENT;
     if (s->res_nonfinite) return fail(c, FSI_E_ARGUMENT, "state.resource contains non-finite values.");
     if (!std::isfinite(t)) return fail(c, FSI_E_ARGUMENT, "t must be finite.");
@@ -1043,40 +1054,66 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic U-right boundary.");
     if (!p.py && c->bcs.V.top.kind != FSI_BC_DIRICHLET)
         return fail(c, FSI_E_ARGUMENT, "The compact predictor currently requires a Dirichlet or Periodic V-top boundary.");
+    return FSI_OK;
+}
 
-    const int saved_corr = w->corr_cur;
+struct StepRun {
+    int saved_corr = 0;
+    bool planned = false;  // the planned step and its status read-back are enqueued
+};
+
+// Enqueues the planned step (once the iteration history allows one) and the
+// read-back of its Status; nothing is awaited, so the steps of several
+// contexts can be in flight together.
+static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, StepRun& run) {
+    run.saved_corr = w->corr_cur;
+    run.planned = false;
     fsi_status rc = FSI_OK;
-    bool done = false, replayed = false;
     if (c->stage_warm && !w->stage_warm) {  // first step: every stage starts from the shared correction
         for (int k = 0; k < 3; ++k)
             HIPCHECK(c, hipMemcpyAsync(w->corr_stage[k], w->corr[w->corr_cur], c->felems * sizeof(double),
                                        hipMemcpyDeviceToDevice, c->stream));
     }
-    if (c->use_graph && c->have_history) {
-        // planned step: every solve gets its planned iteration count (kernels
-        // past convergence return at once), one read-back at the end
-        // a solve of k iterations is seen converged by launch k + 1 (kernel m
-        // tests the residual of iterate m), so the plan is the maximum count
-        // of the last kHist steps plus one; a larger count redoes the step
-        static const int kVelMargin = 1, kProjMargin = 1;
-        const int nh = std::min(c->hist_n, fsi_context::kHist);
-        for (int k = 0; k < 3; ++k) {
-            int mv = c->last_vel_iters[k], mp = c->last_proj_iters[k];
-            for (int h = 0; h < nh; ++h) {
-                mv = std::max(mv, c->hist_vel[k][h]);
-                mp = std::max(mp, c->hist_proj[k][h]);
-            }
-            c->plan_vel[k] = mv + kVelMargin;
-            c->plan_proj[k] = mp + kProjMargin;
+    if (!(c->use_graph && c->have_history)) return FSI_OK;
+    // planned step: every solve gets its planned iteration count (kernels
+    // past convergence return at once), one read-back at the end
+    // a solve of k iterations is seen converged by launch k + 1 (kernel m
+    // tests the residual of iterate m), so the plan is the maximum count
+    // of the last kHist steps plus one; a larger count redoes the step
+    static const int kVelMargin = 1, kProjMargin = 1;
+    const int nh = std::min(c->hist_n, fsi_context::kHist);
+    for (int k = 0; k < 3; ++k) {
+        int mv = c->last_vel_iters[k], mp = c->last_proj_iters[k];
+        for (int h = 0; h < nh; ++h) {
+            mv = std::max(mv, c->hist_vel[k][h]);
+            mp = std::max(mp, c->hist_proj[k][h]);
         }
-        status_reset(c);
-        if ((rc = status_push(c)) != FSI_OK) return rc;
-        if (c->use_graph == 2) {  // planned counts, launched directly
-            if ((rc = enqueue_step(c, s, w, dt, true)) != FSI_OK) return rc;
-        } else if ((rc = run_step_graph(c, s, w, dt)) != FSI_OK) {
-            return rc;
-        }
-        if ((rc = status_pull(c)) != FSI_OK) return rc;
+        c->plan_vel[k] = mv + kVelMargin;
+        c->plan_proj[k] = mp + kProjMargin;
+    }
+    if ((rc = status_push_reset(c)) != FSI_OK) return rc;
+    if (c->use_graph == 2) {  // planned counts, launched directly
+        if ((rc = enqueue_step(c, s, w, dt, true)) != FSI_OK) return rc;
+    } else if ((rc = run_step_graph(c, s, w, dt)) != FSI_OK) {
+        return rc;
+    }
+    HIPCHECK(c, hipMemcpyAsync(c->st_host, c->st, sizeof(Status), hipMemcpyDeviceToHost, c->stream));
+    run.planned = true;
+    return FSI_OK;
+}
+
+// Waits for a planned step (redoing it on the polling path if a solve needed
+// more iterations than planned) or runs the polling path, then applies the
+// reference's checks and swaps the new state in.
+static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, const StepRun& run,
+                              fsi_step_report* rep, fsi_step_stats* stats) {
+    const Params& p = c->prm;
+    const int saved_corr = run.saved_corr;
+    fsi_status rc = FSI_OK;
+    bool done = false, replayed = false;
+    if (run.planned) {
+        HIPCHECK(c, hipStreamSynchronize(c->stream));
+        HIPCHECK(c, hipGetLastError());
         const Status& st = *c->st_host;
         bool ok = true;
         for (int k = 0; k < 3; ++k) ok = ok && (!p.visc || st.vel_ok[k]) && st.proj_ok[k];
@@ -1089,8 +1126,7 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         }
     }
     if (!done) {
-        status_reset(c);
-        if ((rc = status_push(c)) != FSI_OK) return rc;
+        if ((rc = status_push_reset(c)) != FSI_OK) return rc;
         if ((rc = enqueue_step(c, s, w, dt, false)) != FSI_OK) return rc;
         if ((rc = status_pull(c)) != FSI_OK) return rc;
     }
@@ -1164,6 +1200,37 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
         stats->planned = replayed ? 1 : 0;
     }
     return FSI_OK;
+}
+
+extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double t, double dt,
+                                         fsi_step_report* rep, fsi_step_stats* stats) {
+    fsi_status rc = step_check(c, s, w, t, dt);
+    if (rc != FSI_OK) return rc;
+    StepRun run;
+    if ((rc = step_begin(c, s, w, dt, run)) != FSI_OK) return rc;
+    return step_finish(c, s, w, dt, run, rep, stats);
+}
+
+extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_state* const* st,
+                                        fsi_workspace* const* w, const double* t, const double* dt,
+                                        fsi_step_report* reps, fsi_step_stats* stats, int32_t* status) {
+    if (n < 0) return FSI_E_ARGUMENT;
+    if (n == 0) return FSI_OK;
+    if (!ctx || !st || !w || !t || !dt || !status) return FSI_E_ARGUMENT;
+    std::vector<StepRun> runs((size_t)n);
+    for (int k = 0; k < n; ++k) {  // every instance's planned step in flight before any is awaited
+        fsi_status rc = step_check(ctx[k], st[k], w[k], t[k], dt[k]);
+        if (rc == FSI_OK) rc = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
+        status[k] = rc;
+    }
+    fsi_status first = FSI_OK;
+    for (int k = 0; k < n; ++k) {
+        if (status[k] == FSI_OK)
+            status[k] = step_finish(ctx[k], st[k], w[k], dt[k], runs[k], reps ? &reps[k] : nullptr,
+                                    stats ? &stats[k] : nullptr);
+        if (status[k] != FSI_OK && first == FSI_OK) first = (fsi_status)status[k];
+    }
+    return first;
 }
 
 // ---------------------------------------------------------------------------

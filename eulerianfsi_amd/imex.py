@@ -399,6 +399,54 @@ def integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work:
                       rep.maximum_abs_sigma_determinant_error, int(rep.non_spd_cells))
 
 
+def ensemble_step(states, ts, dts, ctxs, works) -> list:
+    """integrate_step! on n independent instances (states[k], ts[k], dts[k],
+    ctxs[k], works[k]; pairwise distinct contexts) with their device work in
+    flight together (fsi_ensemble_step).  Every instance advances exactly as
+    integrate_step would advance it; the first failing instance's error is
+    raised after all instances have been processed (the others have
+    advanced)."""
+    n = len(states)
+    if not (len(ts) == len(dts) == len(ctxs) == len(works) == n):
+        raise DimensionMismatch("ensemble_step needs one t, dt, context and workspace per state.")
+    if len({id(c) for c in ctxs}) != n:
+        raise ArgumentError("ensemble_step needs pairwise distinct contexts.")
+    if n == 0:
+        return []
+    lib = ctxs[0].lib
+    for c in ctxs:
+        if not isinstance(c.numerics.scheme, FourStageIMEXFractionalStep):
+            raise ArgumentError(f"Unsupported IMEX scheme {type(c.numerics.scheme).__name__}.")
+    P = ctypes.c_void_p
+    hc = (P * n)(*(c.handle for c in ctxs))
+    hs = (P * n)(*(s._device(c) for s, c in zip(states, ctxs)))
+    hw = (P * n)(*(w.handle for w in works))
+    t_arr = (ctypes.c_double * n)(*map(float, ts))
+    dt_arr = (ctypes.c_double * n)(*map(float, dts))
+    reps = (_abi.StepReportC * n)()
+    stats = (_abi.StepStatsC * n)()
+    status = (ctypes.c_int32 * n)()
+    lib.fsi_ensemble_step(n, hc, hs, hw, t_arr, dt_arr, reps, stats, status)
+    out, first = [], None
+    for k in range(n):
+        if status[k] == 0:
+            states[k]._device_modified()
+            st = stats[k]
+            works[k].last_stats = StepStats(tuple(st.velocity_iterations), tuple(st.projection_iterations),
+                                            bool(st.planned))
+            r = reps[k]
+            out.append(StepReport(r.maximum_projection_constraint_linf, r.maximum_growth_trace_error,
+                                  r.minimum_phi, r.maximum_phi, r.minimum_resource, r.minimum_sigma_eigenvalue,
+                                  r.maximum_abs_sigma_determinant_error, int(r.non_spd_cells)))
+        else:
+            out.append(None)
+            if first is None:
+                first = k
+    if first is not None:
+        _abi.check(ctxs[first].lib, ctxs[first].handle, status[first], f"ensemble_step (instance {first})")
+    return out
+
+
 def evaluate_stage_tendencies(state: FSIState, ctx: IMEXContext, work: IMEXWorkspace, t: float = 0.0) -> dict:
     """Stage tendencies of the state as _evaluate_stage_tendencies! forms
     them (:944-1071): compact E_u, E_v, I_u, I_v (Nx, Ny) and E_phi,

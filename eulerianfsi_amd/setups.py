@@ -199,3 +199,20 @@ def kolmogorov_filament(N=1024, method="uwcdi", shear_modulus=0.5, viscosity=0.0
         projection_reltol=projection_rtol, precision=precision)
     return Case(f"filament_N{N}_{method}", grid, metrics, bcs, models, num, state, dt,
                 dict(N=N, method=method, shear_modulus=shear_modulus, viscosity=viscosity))
+
+
+def ensemble_cases(n=64, seed=4, sizes=(128, 192, 256, 384), method="acdi", **kw) -> list:
+    """Batched-ensemble workload: n cavities with N drawn uniformly from
+    `sizes`, shear modulus log-uniform in [0.05, 0.5], disk radius ~ U[0.15,
+    0.25], centre ~ U[0.4, 0.6]^2, and dt = 1e-3 * 128 / N each."""
+    rng = SplitMix64(seed)
+    out = []
+    for k in range(n):
+        N = sizes[min(int(rng.uniform() * len(sizes)), len(sizes) - 1)]
+        G = math.exp(rng.uniform(math.log(0.05), math.log(0.5)))
+        r = rng.uniform(0.15, 0.25)
+        cx, cy = rng.uniform(0.4, 0.6), rng.uniform(0.4, 0.6)
+        case = lid_driven_cavity(N, method, radius=r, center=(cx, cy), shear_modulus=G, **kw)
+        case.params["instance"] = k
+        out.append(case)
+    return out

@@ -211,6 +211,20 @@ fsi_status fsi_integrate_step(fsi_context* ctx, fsi_state* st, fsi_workspace* ws
                               double t, double dt, fsi_step_report* report,
                               fsi_step_stats* stats);
 
+/* Batched ensemble: integrate_step! (src/TimeIntegrationIMEX.jl:1152-1256)
+ * applied to n independent instances, instance k being (ctx[k], st[k],
+ * ws[k], t[k], dt[k]) with pairwise distinct contexts.  The reference steps
+ * instances one after another from its drivers; this entry point has no
+ * single counterpart there: it enqueues every instance's planned step on its
+ * context's stream before waiting for any, so small instances run
+ * concurrently on the device.  Each instance's result equals that of
+ * fsi_integrate_step on it; status[k] receives its status (the message via
+ * fsi_last_error(ctx[k])), and the first non-OK status is returned.
+ * `reports` and `stats` (n entries each) may be NULL. */
+fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_state* const* st,
+                             fsi_workspace* const* ws, const double* t, const double* dt,
+                             fsi_step_report* reports, fsi_step_stats* stats, int32_t* status);
+
 /* project! on the state's velocity with target divergence G = 0 (NoGrowth),
  * src/Projection.jl:170-307, warm-started from the workspace correction. */
 fsi_status fsi_project(fsi_context* ctx, fsi_state* st, fsi_workspace* ws,
@@ -234,8 +248,9 @@ fsi_status fsi_eval_tendencies(fsi_context* ctx, const fsi_state* st, fsi_worksp
  * stream with the buffers of the workspace (after at least one step), timed
  * with HIP events on that stream; returns the mean duration of one launch in
  * milliseconds and the kernel's algorithmic HBM bytes per launch.  Names:
- * "copy" (calibration), "vel_iter", "pcg_dir", "pcg_upd", "mg_down0",
- * "mg_up0", "mg_jacobi", "momentum", "cells". */
+ * "copy" (calibration), "vel_iter", "spec_solve" (the projection's direct
+ * solve, three kernels), "pcg_dir", "pcg_upd", "mg_down0", "mg_up0",
+ * "mg_coarse", "pcg_iter", "mg_jacobi", "momentum", "cells". */
 fsi_status fsi_benchmark_kernel(fsi_context* ctx, fsi_state* st, fsi_workspace* ws, const char* name,
                                 int32_t reps, double* avg_ms, double* bytes_per_launch);
 
