@@ -70,17 +70,20 @@ struct VelTile {
 };
 
 // LDS tile of halo H around the 64 x 16 cells of a block, by global index.
-template <int H>
+// IN: the tile and its halo lie inside the domain, so no index wraps.
+template <int H, bool IN = false>
 struct HTile {
     static constexpr int W = TILE_X + 2 * H, R = TILE_ROWS + 2 * H;
     const double* a;
     int i0, j0, nx, ny;
     __device__ __forceinline__ double operator()(int gi, int gj) const {
         int li = gi - i0 + H, lj = gj - j0 + H;
-        if (li < 0) li += nx;
-        else if (li >= W) li -= nx;
-        if (lj < 0) lj += ny;
-        else if (lj >= R) lj -= ny;
+        if (!IN) {
+            if (li < 0) li += nx;
+            else if (li >= W) li -= nx;
+            if (lj < 0) lj += ny;
+            else if (lj >= R) lj -= ny;
+        }
         return a[lj * W + li];
     }
 };
@@ -90,19 +93,16 @@ struct HTile {
 // q0 = (I - alpha M) p0, and the partials of |r0|^2, |b_F|^2 (free rows) and
 // of the first iteration (p.q, r.q, q.q, r.r).  x0 sits on a halo-2 LDS
 // tile and r0 on a halo-1 tile, so both stencils read LDS.
-__global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
-    using T2 = HTile<2>;
-    using T1 = HTile<1>;
-    __shared__ double sxu[T2::R * T2::W], sxv[T2::R * T2::W];
-    __shared__ double sru[T1::R * T1::W], srv[T1::R * T1::W];
-    const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const TileId T = tile_id();
-    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
-    if (T.x == 0 && T.y == 0 && tid == 0) st->vel.done = 0;  // the iteration kernels follow this one
+template <bool IN>
+__device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b, double alpha, int nb, int i0, int j0,
+                                               int tid, double* sxu, double* sxv, double* sru, double* srv) {
+    using T2 = HTile<2, IN>;
+    using T1 = HTile<1, IN>;
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
     auto wrapped = [&](int ui, int uj, int& gi, int& gj) -> bool {
         gi = ui;
         gj = uj;
+        if (IN) return true;
         if (gi < 0 || gi >= p.nx) {
             if (!p.px) return false;
             gi = ((gi % p.nx) + p.nx) % p.nx;
@@ -127,7 +127,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
         int gi, gj;
         const bool in = wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj);
         idx2[it] = in ? at(p, gi, gj) : 0;
-        fl[it] = in ? (1 | (dir_u(p, gi) ? 2 : 0) | (dir_v(p, gj) ? 4 : 0)) : 0;
+        fl[it] = in ? (1 | (dir_u<IN>(p, gi) ? 2 : 0) | (dir_v<IN>(p, gj) ? 4 : 0)) : 0;
     }
 #pragma unroll
     for (int it = 0; it < NI1; ++it) {
@@ -184,12 +184,12 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
                 wrapped(ui, uj, gi, gj);
                 const size_t kk = idx[it];
                 const bool own = li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
-                if (!dir_u(p, gi)) {
-                    ru = bu[it] - (Xu(gi, gj) - alpha * visc_u(p, Xu, Xv, gi, gj, 0));
+                if (!dir_u<IN>(p, gi)) {
+                    ru = bu[it] - (Xu(gi, gj) - alpha * visc_u<IN>(p, Xu, Xv, gi, gj, 0));
                     if (own) acc2[1] += bu[it] * bu[it];
                 }
-                if (!dir_v(p, gj)) {
-                    rv = bv[it] - (Xv(gi, gj) - alpha * visc_v(p, Xu, Xv, gi, gj, 0));
+                if (!dir_v<IN>(p, gj)) {
+                    rv = bv[it] - (Xv(gi, gj) - alpha * visc_v<IN>(p, Xu, Xv, gi, gj, 0));
                     if (own) acc2[1] += bv[it] * bv[it];
                 }
                 if (own) {
@@ -216,8 +216,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
             if (j >= p.ny) break;
             const size_t k = at(p, i, j);
             const double vu = Ru(i, j), vv = Rv(i, j);
-            const double qu = dir_u(p, i) ? 0.0 : vu - alpha * visc_u(p, Ru, Rv, i, j, 0);
-            const double qv = dir_v(p, j) ? 0.0 : vv - alpha * visc_v(p, Ru, Rv, i, j, 0);
+            const double qu = dir_u<IN>(p, i) ? 0.0 : vu - alpha * visc_u<IN>(p, Ru, Rv, i, j, 0);
+            const double qv = dir_v<IN>(p, j) ? 0.0 : vv - alpha * visc_v<IN>(p, Ru, Rv, i, j, 0);
             b.q_u[0][k] = qu;
             b.q_v[0][k] = qv;
             acc4[0] += vu * qu + vv * qv;
@@ -227,6 +227,21 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
         }
     put_partials<2>(acc2, pset(b.partials, nb, PS_VELI), nb);
     put_partials<4>(acc4, pset(b.partials, nb, PS_VEL0), nb);
+}
+
+__global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
+    constexpr int N2 = HTile<2>::R * HTile<2>::W, N1 = HTile<1>::R * HTile<1>::W;
+    __shared__ double sxu[N2], sxv[N2];
+    __shared__ double sru[N1], srv[N1];
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    if (T.x == 0 && T.y == 0 && tid == 0) st->vel.done = 0;  // the iteration kernels follow this one
+    // every point whose operator row the block evaluates (halo 1 included)
+    // has its stencil inside the domain: no boundary rows, no wraps
+    const bool inner = i0 - 1 >= 1 && i0 + TILE_X <= p.nx - 2 && j0 - 1 >= 1 && j0 + TILE_ROWS <= p.ny - 2;
+    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, sru, srv);
+    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, sru, srv);
 }
 
 // One CG iteration m >= 1 with a single reduction phase: the totals of
