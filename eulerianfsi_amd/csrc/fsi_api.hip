@@ -114,8 +114,12 @@ struct fsi_workspace {
     double* corr_stage[3] = {};
     double* corr_prev[3] = {};  // the step before, for the linear extrapolation 2 q^{n-1} - q^{n-2}
     // the same for the predictor solves: stage s of the previous two steps
+    // velocity stage solutions of the last three steps (newest first); the
+    // solve writes into vel_old, and the three trade roles once a step has
+    // succeeded
     double *vel_cur_u[3] = {}, *vel_cur_v[3] = {}, *vel_prev_u[3] = {}, *vel_prev_v[3] = {};
-    int stage_warm = 0;         // completed steps whose stage solutions corr_stage / corr_prev hold (0, 1, 2)
+    double *vel_old_u[3] = {}, *vel_old_v[3] = {};
+    int stage_warm = 0;         // completed steps whose stage solutions the history buffers hold (0 .. 3)
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
     // captured steps, keyed by everything their kernel arguments depend on
@@ -124,7 +128,8 @@ struct fsi_workspace {
         double dt;
         const double* sU;  // state/stage buffers alternate between steps
         int corr_in, corr_out;
-        int warm;          // stage warm-start history (0, 1, 2): changes the launch sequence
+        int warm;          // stage warm-start history (0 .. 3): changes the launch sequence
+        const double* vcur;  // newest velocity history buffer (the three rotate between steps)
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> graphs;
@@ -629,7 +634,9 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->corr_prev[0],  &w->corr_prev[1],  &w->corr_prev[2],
                           &w->vel_cur_u[0], &w->vel_cur_u[1], &w->vel_cur_u[2], &w->vel_cur_v[0], &w->vel_cur_v[1],
                           &w->vel_cur_v[2], &w->vel_prev_u[0], &w->vel_prev_u[1], &w->vel_prev_u[2],
-                          &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2]};
+                          &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2], &w->vel_old_u[0],
+                          &w->vel_old_u[1],  &w->vel_old_u[2],  &w->vel_old_v[0],  &w->vel_old_v[1],
+                          &w->vel_old_v[2]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -761,16 +768,18 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         launch_vel_copy(c->stream, p, b);
         return FSI_OK;
     }
-    // warm start: stage `slot` of the previous step, extrapolated in time
-    // (2 u^{n-1} - u^{n-2}, formed by the start kernel) once two steps are
-    // known, instead of u^n
+    // warm start: stage `slot` of the previous step, extrapolated in time by
+    // the start kernel once two (2 u^{n-1} - u^{n-2}) or three
+    // (3 (u^{n-1} - u^{n-2}) + u^{n-3}) steps are known, instead of u^n
     const bool warm = slot >= 0 && c->stage_warm && w->stage_warm >= 1;
     if (warm) {
         b.Un = w->vel_cur_u[slot];
         b.Vn = w->vel_cur_v[slot];
         b.Pu = w->vel_prev_u[slot];
         b.Pv = w->vel_prev_v[slot];
-        b.extrap = w->stage_warm >= 2;
+        b.Ou = w->vel_old_u[slot];
+        b.Ov = w->vel_old_v[slot];
+        b.extrap = std::min(w->stage_warm, 3) - 1;
     }
     launch_vel_init(c->stream, p, b, alpha, c->st);
     const double vrtol = c->num.velocity_rtol;
@@ -778,7 +787,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     if (fixed > 0) {
         launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax);
         if (slot >= 0 && c->stage_warm)
-            launch_copy2(c->stream, p, w->Us, w->vel_prev_u[slot], w->Vs, w->vel_prev_v[slot]);
+            launch_copy2(c->stream, p, w->Us, w->vel_old_u[slot], w->Vs, w->vel_old_v[slot]);
         return FSI_OK;
     }
     // kernel m tests the residual of iterate m and performs update m+1
@@ -794,11 +803,10 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         chunk = 4;
     }
     c->last_vel_iters[slot] = std::max(1, iter);
-    // the solution goes to the older history buffer; the two trade roles once
-    // the step has succeeded (fsi_integrate_step), so a redone step still
-    // starts from the same history
+    // the solution goes to the oldest history buffer, which the start kernel
+    // has read; the three trade roles once the step has succeeded
     if (slot >= 0 && c->stage_warm)
-        launch_copy2(c->stream, p, w->Us, w->vel_prev_u[slot], w->Vs, w->vel_prev_v[slot]);
+        launch_copy2(c->stream, p, w->Us, w->vel_old_u[slot], w->Vs, w->vel_old_v[slot]);
     return FSI_OK;
 }
 
@@ -999,7 +1007,8 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
 static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt) {
     fsi_workspace::StepGraph* hit = nullptr;
     for (auto& g : w->graphs) {
-        bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur && g.warm == w->stage_warm;
+        bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur && g.warm == w->stage_warm &&
+                    g.vcur == w->vel_cur_u[0];
         for (int k = 0; k < 3; ++k) same = same && g.vel[k] == c->plan_vel[k] && g.proj[k] == c->plan_proj[k];
         if (same) {
             hit = &g;
@@ -1016,6 +1025,7 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
         g.sU = s->U;
         g.corr_in = w->corr_cur;
         g.warm = w->stage_warm;
+        g.vcur = w->vel_cur_u[0];
         hipGraph_t graph = nullptr;
         HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         fsi_status rc = enqueue_step(c, s, w, dt, true);
@@ -1176,11 +1186,17 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
     if (c->stage_warm) {
-        for (int k = 0; k < 3; ++k) {
-            std::swap(w->vel_cur_u[k], w->vel_prev_u[k]);
-            std::swap(w->vel_cur_v[k], w->vel_prev_v[k]);
+        for (int k = 0; k < 3; ++k) {  // (cur, prev, old) <- (old, cur, prev)
+            double* nu = w->vel_old_u[k];
+            double* nv = w->vel_old_v[k];
+            w->vel_old_u[k] = w->vel_prev_u[k];
+            w->vel_old_v[k] = w->vel_prev_v[k];
+            w->vel_prev_u[k] = w->vel_cur_u[k];
+            w->vel_prev_v[k] = w->vel_cur_v[k];
+            w->vel_cur_u[k] = nu;
+            w->vel_cur_v[k] = nv;
         }
-        if (w->stage_warm < 2) ++w->stage_warm;
+        if (w->stage_warm < 3) ++w->stage_warm;
     }
     if (rep) {
         rep->maximum_projection_constraint_linf = std::fmax(st.stage1_linf, st.max_constraint_linf);

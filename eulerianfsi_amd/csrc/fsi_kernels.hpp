@@ -66,9 +66,11 @@ void launch_ext_final(hipStream_t s, const Params& p, Status* st);
 struct VelBufs {
     const double* Un;  // warm start: uv_n (state velocity) or the previous step's stage solution
     const double* Vn;
-    const double* Pu;  // the stage solution of the step before (extrap: x0 = 2 Un - Pu)
+    const double* Pu;  // the stage solution of the step before
     const double* Pv;
-    int extrap;
+    const double* Ou;  // and of the step before that
+    const double* Ov;
+    int extrap;        // 0: x0 = Un; 1: x0 = 2 Un - Pu; 2: x0 = 3 (Un - Pu) + Ou
     const double* ru;  // right-hand side (stage sum + alpha*b)
     const double* rv;
     double* xu;  // solution, written into the stage velocity arrays

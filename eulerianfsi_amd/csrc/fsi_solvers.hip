@@ -114,8 +114,9 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         return true;
     };
     // x0 on halo 2 and the right-hand side on halo 1: indices, then every
-    // load back to back (one latency for both tiles).  x0 = 2 cur - prev
-    // (extrapolated warm start) or cur.
+    // load back to back (one latency for both tiles).  x0 = cur, or the
+    // warm start extrapolated in time from the last two (2 cur - prev) or
+    // three (3 (cur - prev) + old) stage solutions.
     constexpr int N2 = T2::R * T2::W, NI2 = (N2 + BLOCK - 1) / BLOCK;
     constexpr int N1 = T1::R * T1::W, NI1 = (N1 + BLOCK - 1) / BLOCK;
     size_t idx2[NI2], idx[NI1];
@@ -147,7 +148,7 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         bu[it] = b.ru[idx[it]];
         bv[it] = b.rv[idx[it]];
     }
-    if (b.extrap) {
+    if (b.extrap == 1) {
         double yu[NI2], yv[NI2];
 #pragma unroll
         for (int it = 0; it < NI2; ++it) {
@@ -158,6 +159,20 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         for (int it = 0; it < NI2; ++it) {
             xu[it] = 2.0 * xu[it] - yu[it];
             xv[it] = 2.0 * xv[it] - yv[it];
+        }
+    } else if (b.extrap == 2) {
+        double yu[NI2], yv[NI2], zu[NI2], zv[NI2];
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            yu[it] = b.Pu[idx2[it]];
+            yv[it] = b.Pv[idx2[it]];
+            zu[it] = b.Ou[idx2[it]];
+            zv[it] = b.Ov[idx2[it]];
+        }
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            xu[it] = 3.0 * (xu[it] - yu[it]) + zu[it];
+            xv[it] = 3.0 * (xv[it] - yv[it]) + zv[it];
         }
     }
 #pragma unroll
