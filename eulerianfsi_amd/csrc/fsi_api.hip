@@ -118,7 +118,7 @@ struct fsi_workspace {
     // solve writes into vel_old, and the three trade roles once a step has
     // succeeded
     double *vel_cur_u[3] = {}, *vel_cur_v[3] = {}, *vel_prev_u[3] = {}, *vel_prev_v[3] = {};
-    double *vel_old_u[3] = {}, *vel_old_v[3] = {};
+    double *vel_old_u[3] = {}, *vel_old_v[3] = {}, *vel_old2_u[3] = {}, *vel_old2_v[3] = {};
     int stage_warm = 0;         // completed steps whose stage solutions the history buffers hold (0 .. 3)
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
@@ -636,7 +636,8 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->vel_cur_v[2], &w->vel_prev_u[0], &w->vel_prev_u[1], &w->vel_prev_u[2],
                           &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2], &w->vel_old_u[0],
                           &w->vel_old_u[1],  &w->vel_old_u[2],  &w->vel_old_v[0],  &w->vel_old_v[1],
-                          &w->vel_old_v[2]};
+                          &w->vel_old_v[2],  &w->vel_old2_u[0], &w->vel_old2_u[1], &w->vel_old2_u[2],
+                          &w->vel_old2_v[0], &w->vel_old2_v[1], &w->vel_old2_v[2]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -779,7 +780,9 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         b.Pv = w->vel_prev_v[slot];
         b.Ou = w->vel_old_u[slot];
         b.Ov = w->vel_old_v[slot];
-        b.extrap = std::min(w->stage_warm, 3) - 1;
+        b.Wu = w->vel_old2_u[slot];
+        b.Wv = w->vel_old2_v[slot];
+        b.extrap = std::min(w->stage_warm, 4) - 1;
     }
     launch_vel_init(c->stream, p, b, alpha, c->st);
     const double vrtol = c->num.velocity_rtol;
@@ -787,7 +790,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     if (fixed > 0) {
         launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax);
         if (slot >= 0 && c->stage_warm)
-            launch_copy2(c->stream, p, w->Us, w->vel_old_u[slot], w->Vs, w->vel_old_v[slot]);
+            launch_copy2(c->stream, p, w->Us, w->vel_old2_u[slot], w->Vs, w->vel_old2_v[slot]);
         return FSI_OK;
     }
     // kernel m tests the residual of iterate m and performs update m+1
@@ -806,7 +809,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     // the solution goes to the oldest history buffer, which the start kernel
     // has read; the three trade roles once the step has succeeded
     if (slot >= 0 && c->stage_warm)
-        launch_copy2(c->stream, p, w->Us, w->vel_old_u[slot], w->Vs, w->vel_old_v[slot]);
+        launch_copy2(c->stream, p, w->Us, w->vel_old2_u[slot], w->Vs, w->vel_old2_v[slot]);
     return FSI_OK;
 }
 
@@ -1186,9 +1189,11 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
     if (c->stage_warm) {
-        for (int k = 0; k < 3; ++k) {  // (cur, prev, old) <- (old, cur, prev)
-            double* nu = w->vel_old_u[k];
-            double* nv = w->vel_old_v[k];
+        for (int k = 0; k < 3; ++k) {  // (cur, prev, old, old2) <- (old2, cur, prev, old)
+            double* nu = w->vel_old2_u[k];
+            double* nv = w->vel_old2_v[k];
+            w->vel_old2_u[k] = w->vel_old_u[k];
+            w->vel_old2_v[k] = w->vel_old_v[k];
             w->vel_old_u[k] = w->vel_prev_u[k];
             w->vel_old_v[k] = w->vel_prev_v[k];
             w->vel_prev_u[k] = w->vel_cur_u[k];
@@ -1196,7 +1201,7 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
             w->vel_cur_u[k] = nu;
             w->vel_cur_v[k] = nv;
         }
-        if (w->stage_warm < 3) ++w->stage_warm;
+        if (w->stage_warm < 4) ++w->stage_warm;
     }
     if (rep) {
         rep->maximum_projection_constraint_linf = std::fmax(st.stage1_linf, st.max_constraint_linf);

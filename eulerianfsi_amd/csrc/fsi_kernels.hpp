@@ -70,7 +70,10 @@ struct VelBufs {
     const double* Pv;
     const double* Ou;  // and of the step before that
     const double* Ov;
-    int extrap;        // 0: x0 = Un; 1: x0 = 2 Un - Pu; 2: x0 = 3 (Un - Pu) + Ou
+    const double* Wu;  // and of the one before
+    const double* Wv;
+    int extrap;        // 0: x0 = Un; 1: x0 = 2 Un - Pu; 2: x0 = 3 (Un - Pu) + Ou;
+                       // 3: x0 = 4 (Un + Ou) - 6 Pu - Wu
     const double* ru;  // right-hand side (stage sum + alpha*b)
     const double* rv;
     double* xu;  // solution, written into the stage velocity arrays

@@ -110,9 +110,8 @@ __device__ __forceinline__ double uv_corner(const Params& p, const FU& U, const 
 
 // ---- linear wall extrapolation (Operators.jl:738-842) ---------------------
 template <class F>
-__device__ __forceinline__ double lex_lo(F f, double h) {  // value at the low wall from f(0), f(1)
-    double d = 0.5 * h;
-    return f(0) - (f(1) - f(0)) * d / h;
+__device__ __forceinline__ double lex_lo(F f) {  // value at the low wall (half a cell out) from f(0), f(1)
+    return f(0) - (f(1) - f(0)) * 0.5;
 }
 
 // ---------------------------------------------------------------------------
@@ -125,34 +124,34 @@ __device__ __forceinline__ double elastic_fx(const Params& p, const FT& Txx, con
     // x-part at x-face i (Operators.jl:642-683)
     const int nx = p.nx, ny = p.ny;
     double fx;
-    if (p.px && i == 0) fx = (Txx(0, j) - Txx(nx - 1, j)) / p.hx;
+    if (p.px && i == 0) fx = (Txx(0, j) - Txx(nx - 1, j)) * p.ihx;
     else if (i == 0) {
         auto f = [&](int k) { return Txx(k, j); };
-        double lw = lex_lo(f, p.hx);
-        fx = (Txx(0, j) - lw) / (0.5 * p.hx);
-    } else fx = (Txx(i, j) - Txx(i - 1, j)) / p.hx;
+        double lw = lex_lo(f);
+        fx = (Txx(0, j) - lw) * (2.0 * p.ihx);
+    } else fx = (Txx(i, j) - Txx(i - 1, j)) * p.ihx;
     // Txy on this x-face column, any row jj
     auto txy_x = [&](int jj) -> double {
         if (p.px && i == 0) return 0.5 * Txy(nx - 1, jj) + 0.5 * Txy(0, jj);
         if (i == 0) {
             auto f = [&](int k) { return Txy(k, jj); };
-            return lex_lo(f, p.hx);
+            return lex_lo(f);
         }
         return 0.5 * Txy(i - 1, jj) + 0.5 * Txy(i, jj);
     };
     if (p.py) {
         int jm = j == 0 ? ny - 1 : j - 1, jp = j == ny - 1 ? 0 : j + 1;
-        fx += (txy_x(jp) - txy_x(jm)) / (p.hy + p.hy);
+        fx += (txy_x(jp) - txy_x(jm)) * (0.5 * p.ihy);
     } else if (j == 0) {
         double t0 = txy_x(0), t1 = txy_x(1);
-        double bw = t0 - (t1 - t0) * (0.5 * p.hy) / p.hy;
-        fx += (t0 - bw) / (0.5 * p.hy);
+        double bw = t0 - (t1 - t0) * 0.5;
+        fx += (t0 - bw) * (2.0 * p.ihy);
     } else if (j == ny - 1) {
         double t0 = txy_x(ny - 1), t1 = txy_x(ny - 2);
-        double tw = t0 + (t0 - t1) * (0.5 * p.hy) / p.hy;
-        fx += (tw - t0) / (0.5 * p.hy);
+        double tw = t0 + (t0 - t1) * 0.5;
+        fx += (tw - t0) * (2.0 * p.ihy);
     } else {
-        fx += (txy_x(j + 1) - txy_x(j - 1)) / (p.hy + p.hy);
+        fx += (txy_x(j + 1) - txy_x(j - 1)) * (0.5 * p.ihy);
     }
     return fx;
 }
@@ -161,33 +160,33 @@ template <class FT>
 __device__ __forceinline__ double elastic_fy(const Params& p, const FT& Tyy, const FT& Txy, int i, int j) {
     const int nx = p.nx, ny = p.ny;
     double fy;
-    if (p.py && j == 0) fy = (Tyy(i, 0) - Tyy(i, ny - 1)) / p.hy;
+    if (p.py && j == 0) fy = (Tyy(i, 0) - Tyy(i, ny - 1)) * p.ihy;
     else if (j == 0) {
         auto f = [&](int k) { return Tyy(i, k); };
-        double bw = lex_lo(f, p.hy);
-        fy = (Tyy(i, 0) - bw) / (0.5 * p.hy);
-    } else fy = (Tyy(i, j) - Tyy(i, j - 1)) / p.hy;
+        double bw = lex_lo(f);
+        fy = (Tyy(i, 0) - bw) * (2.0 * p.ihy);
+    } else fy = (Tyy(i, j) - Tyy(i, j - 1)) * p.ihy;
     auto txy_y = [&](int ii) -> double {
         if (p.py && j == 0) return 0.5 * Txy(ii, ny - 1) + 0.5 * Txy(ii, 0);
         if (j == 0) {
             auto f = [&](int k) { return Txy(ii, k); };
-            return lex_lo(f, p.hy);
+            return lex_lo(f);
         }
         return 0.5 * Txy(ii, j - 1) + 0.5 * Txy(ii, j);
     };
     if (p.px) {
         int im = i == 0 ? nx - 1 : i - 1, ip = i == nx - 1 ? 0 : i + 1;
-        fy += (txy_y(ip) - txy_y(im)) / (p.hx + p.hx);
+        fy += (txy_y(ip) - txy_y(im)) * (0.5 * p.ihx);
     } else if (i == 0) {
         double t0 = txy_y(0), t1 = txy_y(1);
-        double lw = t0 - (t1 - t0) * (0.5 * p.hx) / p.hx;
-        fy += (t0 - lw) / (0.5 * p.hx);
+        double lw = t0 - (t1 - t0) * 0.5;
+        fy += (t0 - lw) * (2.0 * p.ihx);
     } else if (i == nx - 1) {
         double t0 = txy_y(nx - 1), t1 = txy_y(nx - 2);
-        double rw = t0 + (t0 - t1) * (0.5 * p.hx) / p.hx;
-        fy += (rw - t0) / (0.5 * p.hx);
+        double rw = t0 + (t0 - t1) * 0.5;
+        fy += (rw - t0) * (2.0 * p.ihx);
     } else {
-        fy += (txy_y(i + 1) - txy_y(i - 1)) / (p.hx + p.hx);
+        fy += (txy_y(i + 1) - txy_y(i - 1)) * (0.5 * p.ihx);
     }
     return fy;
 }
@@ -274,8 +273,8 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* _
         if (!p.px && i == 0) cu = 0.0;  // wall-normal row (MomentumTransport.jl:322-327)
         else {
             int im = (i == 0) ? p.nx - 1 : i - 1;
-            cu = (u2c(p, U, i, j) - u2c(p, U, im, j)) / p.hx +
-                 (uv_corner(p, U, V, i, j + 1) - uv_corner(p, U, V, i, j)) / p.hy;
+            cu = (u2c(p, U, i, j) - u2c(p, U, im, j)) * p.ihx +
+                 (uv_corner(p, U, V, i, j + 1) - uv_corner(p, U, V, i, j)) * p.ihy;
         }
         double eu = -cu;
         if (p.force == FSI_FORCE_CONSTANT) eu += p.ffx;
@@ -286,8 +285,8 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* _
         if (!p.py && j == 0) cv = 0.0;
         else {
             int jm = (j == 0) ? p.ny - 1 : j - 1;
-            cv = (uv_corner(p, U, V, i + 1, j) - uv_corner(p, U, V, i, j)) / p.hx +
-                 (v2c(p, V, i, j) - v2c(p, V, i, jm)) / p.hy;
+            cv = (uv_corner(p, U, V, i + 1, j) - uv_corner(p, U, V, i, j)) * p.ihx +
+                 (v2c(p, V, i, j) - v2c(p, V, i, jm)) * p.ihy;
         }
         double ev = -cv;
         if (p.force == FSI_FORCE_CONSTANT) ev += p.ffy;
@@ -423,7 +422,7 @@ __device__ __forceinline__ double upwind_rhs(const Params& p, const double* U, c
         double v = V[at(p, i, k)];
         return v * (v >= 0.0 ? f(i, k - 1) : f(i, k));
     };
-    return -((ax(i + 1) - ax(i)) / p.hx + (ay(j + 1) - ay(j)) / p.hy);
+    return -((ax(i + 1) - ax(i)) * p.ihx + (ay(j + 1) - ay(j)) * p.ihy);
 }
 
 // _phase_boundary_value (PhaseFieldMethods.jl:174-196)
@@ -534,34 +533,34 @@ __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU
             ft = upwind_flux3(vt, C, wall_state3(p, 3, vt, false, C, st));
         }
     }
-    const double axx = -((fr.xx - fl.xx) / p.hx + (ft.xx - fb.xx) / p.hy);
-    const double ayy = -((fr.yy - fl.yy) / p.hx + (ft.yy - fb.yy) / p.hy);
-    const double axy = -((fr.xy - fl.xy) / p.hx + (ft.xy - fb.xy) / p.hy);
+    const double axx = -((fr.xx - fl.xx) * p.ihx + (ft.xx - fb.xx) * p.ihy);
+    const double ayy = -((fr.yy - fl.yy) * p.ihx + (ft.yy - fb.yy) * p.ihy);
+    const double axy = -((fr.xy - fl.xy) * p.ihx + (ft.xy - fb.xy) * p.ihy);
     double uL = U(i, j), uR = U(i + 1, j), vB = V(i, j), vT = V(i, j + 1);
-    double dv = (uR - uL) / p.hx + (vT - vB) / p.hy;
-    double ux = (uR - uL) / p.hx, vy = (vT - vB) / p.hy;
+    double dv = (uR - uL) * p.ihx + (vT - vB) * p.ihy;
+    double ux = (uR - uL) * p.ihx, vy = (vT - vB) * p.ihy;
     auto Uc = [&](int jj) { return 0.5 * U(i, jj) + 0.5 * U(i + 1, jj); };
     auto Vc = [&](int ii) { return 0.5 * V(ii, j) + 0.5 * V(ii, j + 1); };
     double uy, vx;
     if (p.py) {
         int jm = j == 0 ? p.ny - 1 : j - 1, jp = j == p.ny - 1 ? 0 : j + 1;
-        uy = (Uc(jp) - Uc(jm)) / (p.hy + p.hy);
+        uy = (Uc(jp) - Uc(jm)) * (0.5 * p.ihy);
     } else if (j == 0) {
-        uy = p.bU.b.kind == FSI_BC_DIRICHLET ? (Uc(0) - p.bU.b.value) / (0.5 * p.hy) : -p.bU.b.value;
+        uy = p.bU.b.kind == FSI_BC_DIRICHLET ? (Uc(0) - p.bU.b.value) * (2.0 * p.ihy) : -p.bU.b.value;
     } else if (j == p.ny - 1) {
-        uy = p.bU.t.kind == FSI_BC_DIRICHLET ? (p.bU.t.value - Uc(p.ny - 1)) / (0.5 * p.hy) : p.bU.t.value;
+        uy = p.bU.t.kind == FSI_BC_DIRICHLET ? (p.bU.t.value - Uc(p.ny - 1)) * (2.0 * p.ihy) : p.bU.t.value;
     } else {
-        uy = (Uc(j + 1) - Uc(j - 1)) / (p.hy + p.hy);
+        uy = (Uc(j + 1) - Uc(j - 1)) * (0.5 * p.ihy);
     }
     if (p.px) {
         int im = i == 0 ? p.nx - 1 : i - 1, ip = i == p.nx - 1 ? 0 : i + 1;
-        vx = (Vc(ip) - Vc(im)) / (p.hx + p.hx);
+        vx = (Vc(ip) - Vc(im)) * (0.5 * p.ihx);
     } else if (i == 0) {
-        vx = p.bV.l.kind == FSI_BC_DIRICHLET ? (Vc(0) - p.bV.l.value) / (0.5 * p.hx) : -p.bV.l.value;
+        vx = p.bV.l.kind == FSI_BC_DIRICHLET ? (Vc(0) - p.bV.l.value) * (2.0 * p.ihx) : -p.bV.l.value;
     } else if (i == p.nx - 1) {
-        vx = p.bV.r.kind == FSI_BC_DIRICHLET ? (p.bV.r.value - Vc(p.nx - 1)) / (0.5 * p.hx) : p.bV.r.value;
+        vx = p.bV.r.kind == FSI_BC_DIRICHLET ? (p.bV.r.value - Vc(p.nx - 1)) * (2.0 * p.ihx) : p.bV.r.value;
     } else {
-        vx = (Vc(i + 1) - Vc(i - 1)) / (p.hx + p.hx);
+        vx = (Vc(i + 1) - Vc(i - 1)) * (0.5 * p.ihx);
     }
     const double sa = C.xx, sd = C.yy, sb = C.xy;
     double A11 = ux - 0.0, A12 = uy - 0.0, A21 = vx - 0.0, A22 = vy - 0.0;
@@ -610,7 +609,7 @@ __device__ __forceinline__ double phase_cdi_t(const Params& p, const Stage& s, c
         if (!p.py && (k == 0 || k == p.ny)) return 0.0;
         return p.cdi_G * (c2fy(p, N1, i, k) - ell * grad_yface(p, p.bPhi, f, i, k));
     };
-    double divj = (jx(i + 1) - jx(i)) / p.hx + (jy(j + 1) - jy(j)) / p.hy;
+    double divj = (jx(i + 1) - jx(i)) * p.ihx + (jy(j + 1) - jy(j)) * p.ihy;
     return adv - divj;
 }
 
@@ -650,8 +649,8 @@ __device__ __forceinline__ double phase_acdi_t(const Params& p, const Stage& s, 
         double amp = 0.25 * (1.0 - t * t);
         return gamma * (eps * grad_yface(p, p.bPhi, f, i, k) - amp * c2fy(p, N1, i, k));
     };
-    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
-    double dA = (Ax(i + 1) - Ax(i)) / p.hx + (Ay(j + 1) - Ay(j)) / p.hy;
+    double dF = (fx(i + 1) - fx(i)) * p.ihx + (fy(j + 1) - fy(j)) * p.ihy;
+    double dA = (Ax(i + 1) - Ax(i)) * p.ihx + (Ay(j + 1) - Ay(j)) * p.ihy;
     return -dF + dA;
 }
 
@@ -698,7 +697,7 @@ __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s,
         double beta = fabs(v) / ay;
         return pu * (v + beta * nf * (1.0 - pu));
     };
-    double dF = (fx(i + 1) - fx(i)) / p.hx + (fy(j + 1) - fy(j)) / p.hy;
+    double dF = (fx(i + 1) - fx(i)) * p.ihx + (fy(j + 1) - fy(j)) * p.ihy;
     return -dF;
 }
 
@@ -876,9 +875,9 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         if (in) {
             double gx, gy;
             grad_center(p, bpsi, fpsi, gi, gj, gx, gy);
-            const double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
-            n0 = gx / mag;
-            n1 = gy / mag;
+            const double imag = 1.0 / sqrt(gx * gx + gy * gy + p.acdi_d2);
+            n0 = gx * imag;
+            n1 = gy * imag;
         }
         s_n0[k] = n0;
         s_n1[k] = n1;
@@ -953,7 +952,7 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         if (j >= p.ny) break;
         const double* fx = s_fx + lj * AF_XW + threadIdx.x;
         const double* fy = s_fy + lj * TILE_X + threadIdx.x;
-        Ephi[at(p, i, j)] = (fx[1] - fx[0]) / p.hx + (fy[TILE_X] - fy[0]) / p.hy;
+        Ephi[at(p, i, j)] = (fx[1] - fx[0]) * p.ihx + (fy[TILE_X] - fy[0]) * p.ihy;
     }
 }
 
@@ -1160,7 +1159,7 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
             Exy[k] = oxy;
             if (stage1) {
                 double uL = s.U[k], uR = s.U[at(p, i + 1, j)], vB = s.V[k], vT = s.V[at(p, i, j + 1)];
-                double dv = (uR - uL) / p.hx + (vT - vB) / p.hy;
+                double dv = (uR - uL) * p.ihx + (vT - vB) * p.ihy;
                 dmax = fmax(dmax, fabs(dv - 0.0));
                 bool ok = isfinite(uL) && isfinite(vB) && isfinite(s.phi[k]) && isfinite(s.sxx[k]) &&
                           isfinite(s.syy[k]) && isfinite(s.sxy[k]);

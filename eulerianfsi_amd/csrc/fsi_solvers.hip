@@ -174,6 +174,22 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
             xu[it] = 3.0 * (xu[it] - yu[it]) + zu[it];
             xv[it] = 3.0 * (xv[it] - yv[it]) + zv[it];
         }
+    } else if (b.extrap == 3) {
+        double yu[NI2], yv[NI2], zu[NI2], zv[NI2], wu[NI2], wv[NI2];
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            yu[it] = b.Pu[idx2[it]];
+            yv[it] = b.Pv[idx2[it]];
+            zu[it] = b.Ou[idx2[it]];
+            zv[it] = b.Ov[idx2[it]];
+            wu[it] = b.Wu[idx2[it]];
+            wv[it] = b.Wv[idx2[it]];
+        }
+#pragma unroll
+        for (int it = 0; it < NI2; ++it) {
+            xu[it] = (4.0 * (xu[it] + zu[it]) - 6.0 * yu[it]) - wu[it];
+            xv[it] = (4.0 * (xv[it] + zv[it]) - 6.0 * yv[it]) - wv[it];
+        }
     }
 #pragma unroll
     for (int it = 0; it < NI2; ++it) {
@@ -523,7 +539,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb
             b.V[at(p, i, p.ny)] = vT;
         } else vT = b.V[at(p, i, j + 1)];
         const size_t k = at(p, i, j);
-        double div = (uR - b.U[k]) / p.hx + (vT - b.V[k]) / p.hy;
+        double div = (uR - b.U[k]) * p.ihx + (vT - b.V[k]) * p.ihy;
         double rhs = (div - 0.0) - b.lap_bc[k];
         b.rhs[k] = rhs;
         acc[0] += rhs * area;
@@ -767,7 +783,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
         double aq = -neg_lap(g, Q, i, j, nullptr);
         double res = aq - b.rhs[k];
         acc[0] += res * res;
-        double div = (b.U[at(p, i + 1, j)] - b.U[k]) / p.hx + (b.V[at(p, i, j + 1)] - b.V[k]) / p.hy;
+        double div = (b.U[at(p, i + 1, j)] - b.U[k]) * p.ihx + (b.V[at(p, i, j + 1)] - b.V[k]) * p.ihy;
         double c = div - 0.0;
         linf = fmax(linf, fabs(c));
         if (!isfinite(c)) linf = INFINITY;
