@@ -834,10 +834,18 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double
     b.partials = c->partials;
     launch_proj_rhs(c->stream, p, b);
     if (c->spec.enabled) launch_spectral_solve(c->stream, c->spec, p.P, b.rhs, b.x);
-    launch_pcg_init(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol,
-                    c->num.projection_maxiter);
     const int ps = slot < 0 ? 0 : slot;
     const int last = c->num.projection_maxiter;
+    if (fixed > 0 && std::min(fixed, last + 1) == 1) {
+        // planned with no PCG iteration (the direct solve, or an exact warm
+        // start): the initial guess is only tested
+        launch_proj_tested(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol, last,
+                           c->grid.lx * c->grid.ly, slot);
+        w->corr_cur = 1 - w->corr_cur;
+        return FSI_OK;
+    }
+    launch_pcg_init(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol,
+                    c->num.projection_maxiter);
     if (fixed > 0) {
         // Of the planned iterations only the convergence test of the last one
         // matters (its update is never tested: converged earlier, it is

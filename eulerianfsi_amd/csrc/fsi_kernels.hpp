@@ -115,6 +115,13 @@ void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const M
                       int nlev, const CoarsePlan& cp, Status* st, int m0, int count, int slot);
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly);
+// The whole projection after the right-hand side when the warm start x is
+// only tested (PCG iteration 0; a miss is left to the caller): one pass over
+// x for the convergence test, the true residual and the mean, then the
+// velocity update, its constraint check and the final reduction (which makes
+// the convergence decision).
+void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
+                        int has_null, double rtol, int maxiter, double lxly, int slot);
 void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot);
 // the convergence test of PCG iteration m alone (the first step of that
 // iteration), on the r.r partials of the previous one

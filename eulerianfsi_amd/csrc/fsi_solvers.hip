@@ -550,10 +550,11 @@ __global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb
     put_partials<4>(acc, pset(b.partials, nb, PS_RHS), nb);
 }
 
-// compatibility check (src/Projection.jl:206-221), solver tolerance, and
-// r = b - (-A) x with b = -(rhs - mean)
-__global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int has_null,
-                                                    double rtol, int maxiter) {
+// The projection scalars from the right-hand side totals: compatibility
+// check (src/Projection.jl:206-221) and solver tolerance.  Returns the mean
+// of the right-hand side (removed from b when A is singular).
+__device__ __forceinline__ double proj_scalars(const Params& p, const ProjBufs& b, Status* st, int nb, int has_null,
+                                               double rtol, int maxiter) {
     double tot[4];
     get_totals<4>(pset(b.partials, nb, PS_RHS), nb, tot);
     const double area = p.hx * p.hy;
@@ -578,6 +579,13 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs
         s.maxiter = maxiter;
         s.done = 0;
     }
+    return mean;
+}
+
+// the projection scalars and r = b - (-A) x with b = -(rhs - mean)
+__global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int has_null,
+                                                    double rtol, int maxiter) {
+    const double mean = proj_scalars(p, b, st, nb, has_null, rtol, maxiter);
     GAcc X{b.x, p.P};
     double acc[1] = {0.0};
     FOR_TILE(p) {
@@ -587,6 +595,39 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs
         acc[0] += r * r;
     }
     put_partials<1>(acc, pset(b.partials, nb, PS_RR), nb);
+}
+
+// A solution that is only tested (no PCG iteration follows it: a planned
+// projection whose initial guess is expected to meet the tolerance, such as
+// the direct solve), in one pass over x: the projection scalars, and per
+// block the partials of r.r (r = b - (-A) x, the convergence test of
+// iteration 0), of the true residual |A x - rhs|^2 (:261-264; x and
+// x - mean(x) have the same image under A) and of the area-weighted sum of x
+// (its mean, :255-259).  r is not stored.
+__global__ void __launch_bounds__(BLOCK) k_proj_direct(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
+                                                       int has_null, double rtol, int maxiter) {
+    const double mean = proj_scalars(p, b, st, nb, has_null, rtol, maxiter);
+    const double area = p.hx * p.hy;
+    GAcc X{b.x, p.P};
+    double acc[3] = {0.0, 0.0, 0.0};
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        const double nl = neg_lap(g, X, i, j, nullptr);
+        const double rhs = b.rhs[k];
+        const double r = -(rhs - mean) - nl;
+        const double res = -nl - rhs;
+        acc[0] += r * r;
+        acc[1] += res * res;
+        acc[2] += X(i, j) * area;
+    }
+    __shared__ double lds[3 * 16];
+    block_sum<3>(acc, lds);
+    if (lane_id() == 0) {
+        const int bid = block_id();
+        pset(b.partials, nb, PS_RR)[bid] = acc[0];
+        pset(b.partials, nb, PS_CHECK)[bid] = acc[1];
+        pset(b.partials, nb, PS_MEAN)[bid] = acc[2];
+    }
 }
 
 // ---- multigrid level kernels (every one returns at once once PCG is done)
@@ -768,7 +809,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
 // q_out = q - mean; true residual |A q - rhs| (:261-264); constraint div(v) - G
 // in L-inf and area-weighted L2 (:275-297)
 __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
-                                                      int has_null, double lxly) {
+                                                      int has_null, double lxly, int resid) {
     double tm[1];
     get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
     const double mean = has_null ? tm[0] / lxly : 0.0;
@@ -780,16 +821,26 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
         const size_t k = at(p, i, j);
         double q = Q(i, j);
         b.q_out[k] = q;
-        double aq = -neg_lap(g, Q, i, j, nullptr);
-        double res = aq - b.rhs[k];
-        acc[0] += res * res;
+        if (resid) {
+            double aq = -neg_lap(g, Q, i, j, nullptr);
+            double res = aq - b.rhs[k];
+            acc[0] += res * res;
+        }
         double div = (b.U[at(p, i + 1, j)] - b.U[k]) * p.ihx + (b.V[at(p, i, j + 1)] - b.V[k]) * p.ihy;
         double c = div - 0.0;
         linf = fmax(linf, fabs(c));
         if (!isfinite(c)) linf = INFINITY;
         acc[1] += c * c * area;
     }
-    put_partials<2>(acc, pset(b.partials, nb, PS_CHECK), nb);
+    {  // slot 0 (true residual) comes from k_proj_direct when resid == 0
+        __shared__ double lds[2 * 16];
+        block_sum<2>(acc, lds);
+        if (lane_id() == 0) {
+            double* dst = pset(b.partials, nb, PS_CHECK);
+            if (resid) dst[block_id()] = acc[0];
+            dst[(size_t)nb + block_id()] = acc[1];
+        }
+    }
     // the block maximum as a third partial of the set (maximum taken by k_proj_final)
     __shared__ double lmax[BLOCK / 64];
     linf = wave_max(linf);
@@ -804,7 +855,19 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
 }
 
 // the checks of project! (:279-298), one block
-__global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, int nb, double lxly) {
+__global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, int nb, double lxly, int pcg_m,
+                                                      int slot) {
+    if (pcg_m >= 0) {  // the convergence test of PCG iteration pcg_m (k_pcg_check)
+        double rr[1];
+        get_totals<1>(pset(b.partials, nb, PS_RR), nb, rr);
+        if (lane_id() == 0 && !st->pcg.done && (rr[0] <= st->pcg.tol2 || pcg_m >= st->pcg.maxiter)) {
+            st->pcg.done = 1;
+            st->pcg.iter = pcg_m;
+            st->pcg.rr = rr[0];
+            if (slot >= 0) st->proj_iters[slot] = pcg_m;
+            if (slot >= 0 && rr[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
+        }
+    }
     double tot[2];
     get_totals<2>(pset(b.partials, nb, PS_CHECK), nb, tot);
     __shared__ double lmax[BLOCK / 64];
@@ -960,8 +1023,16 @@ void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const 
     const int nb = grid_blocks(p);
     k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, nb);
     k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
-    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
-    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly);
+    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly, 1);
+    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, -1, -1);
+}
+void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
+                        int has_null, double rtol, int maxiter, double lxly, int slot) {
+    const int nb = grid_blocks(p);
+    k_proj_direct<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter);
+    k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
+    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly, 0);
+    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
 }
 
 // One launch of a named solver kernel with the buffers of a live solve, for
