@@ -30,19 +30,18 @@ sys.path.insert(0, ROOT)
 METRIC = "IMEX steps/s at N=1024 (FP64 vs mixed @1e-8), HBM roofline %, adjoint grads/s"
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
-# Algorithmic HBM passes per step (fields of N^2 doubles): tendencies 3 x 20,
-# stage combinations 22 + 30 + 38, velocity CG 16 / iteration + 16 per solve
-# (init + first), projection PCG per iteration: V-cycle 5.5 at the finest
-# level x 4/3 for the coarser ones, + 10 for the Krylov vector kernels; + 16
-# per projection (rhs, init, mean, update, check).
+# Algorithmic HBM passes per step, in fields of N^2 doubles.
 def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
-    """Passes this implementation makes per step: tendencies 60, stage
-    combinations 90, per velocity solve 16 + 16 per CG iteration, per
-    projection 16, plus 6 for the direct transform solve (three kernels each
-    reading and writing the field once) and 17.3 per PCG iteration."""
+    """Passes this implementation makes per step: tendencies 69 (3 x (10 + 4
+    + 9)), stage combinations 90; per velocity solve 22 (start: right-hand
+    side and four history fields read, x r p q written; the solution copied
+    into the history) + 16 per CG iteration; per projection 14 (rhs 3, the
+    tested solution 2, update 5, check 4) + 6 for the direct transform solve
+    (three kernels each reading and writing the field once), or 17 + 17.3 per
+    PCG iteration when the multigrid-PCG solves it."""
     cell = 8.0 * N * N
-    per_proj = 16 + (6 if direct else 0)
-    passes = 60 + 90 + sum(16 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
+    per_proj = 14 + 6 if direct else 17
+    passes = 69 + 90 + sum(22 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
     return passes * cell
 
 
