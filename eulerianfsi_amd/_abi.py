@@ -204,6 +204,38 @@ def abi_config(metrics, bcs, models, numerics):
     return grid, cb, m, n
 
 
+# ---- driver diagnostics (fsi_api.h) -------------------------------------------
+DIAG_MAX_MODES = 32
+
+
+class DiagParamsC(ctypes.Structure):
+    _fields_ = [("density", c_double), ("elastic_modulus", c_double), ("sigma_core_cutoff", c_double),
+                ("sigma_interface_cutoff", c_double), ("sigma_safety_cutoff", c_double),
+                ("h_interface", c_double), ("deformation_mode_max", c_int32)]
+
+
+class TensorRegionC(ctypes.Structure):
+    _fields_ = [("cells", c_int64), ("area", c_double), ("determinant_min", c_double),
+                ("determinant_max", c_double), ("max_abs_determinant_error", c_double),
+                ("mean_abs_determinant_error", c_double), ("lambda_min", c_double), ("lambda_max", c_double),
+                ("non_spd_cells", c_int64), ("max_effective_stress", c_double),
+                ("mean_effective_stress", c_double)]
+
+
+class DiagnosticsC(ctypes.Structure):
+    _fields_ = ([(n, c_double) for n in (
+        "x_centroid", "y_centroid", "phase_integral", "perimeter_proxy", "mixed_layer_measure",
+        "effective_interface_width", "effective_interface_cells", "covariance_major", "covariance_minor",
+        "aspect_ratio", "orientation_radians", "R0")]
+        + [("nmax", c_int32), ("Rc", c_double * DIAG_MAX_MODES), ("Rs", c_double * DIAG_MAX_MODES),
+           ("amplitudes", c_double * DIAG_MAX_MODES)]
+        + [(n, c_double) for n in ("phi_min", "phi_max", "lower_violation", "upper_violation")]
+        + [("out_of_bounds_cells", c_int64)]
+        + [(n, c_double) for n in ("maximum_speed", "kinetic_energy", "divergence_linf", "divergence_l2",
+                                   "max_abs_u", "max_abs_v")]
+        + [(n, TensorRegionC) for n in ("global_region", "core", "interface_region", "tail")])
+
+
 # ---- library -----------------------------------------------------------------
 _LIB = None
 LIB_PATH = Path(__file__).resolve().parent / "libfsi_hip.so"
@@ -232,6 +264,9 @@ def _declare(lib):
         "fsi_project": (c_int, [P, P, P, POINTER(ProjectionResultC)]),
         "fsi_eval_tendencies": (c_int, [P, P, P, c_double] + [_dp] * 8),
         "fsi_benchmark_kernel": (c_int, [P, P, P, c_char_p, c_int32, _dp, _dp]),
+        "fsi_velocity_extrema": (c_int, [P, P, _dp, _dp]),
+        "fsi_phase_centroid": (c_int, [P, P, _dp, _dp, _dp]),
+        "fsi_diagnostics_compute": (c_int, [P, P, POINTER(DiagParamsC), POINTER(DiagnosticsC)]),
         "fsi_version": (c_char_p, []),
     }
     for name, (res, args) in sig.items():
@@ -246,7 +281,7 @@ EXPORTED = (
     "fsi_state_upload", "fsi_state_download", "fsi_workspace_create", "fsi_workspace_destroy",
     "fsi_workspace_download_correction", "fsi_workspace_upload_correction", "fsi_make_velocity_compatible",
     "fsi_integrate_step", "fsi_ensemble_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel",
-    "fsi_version",
+    "fsi_velocity_extrema", "fsi_phase_centroid", "fsi_diagnostics_compute", "fsi_version",
 )
 
 

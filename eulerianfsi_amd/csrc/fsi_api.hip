@@ -51,6 +51,8 @@ struct fsi_context {
     std::string err;
     double* force_row = nullptr;
     double* ext = nullptr;  // per-block extrema of the report and checks (EX_* slots)
+    double* diag = nullptr;      // per-block partials of the driver diagnostics
+    double* diag_out = nullptr;  // their reduced values
     double* lap_bc = nullptr;
     double* bvisc_u = nullptr;
     double* bvisc_v = nullptr;
@@ -400,6 +402,11 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     p.force_row = c->force_row;
     p.exnb = grid_blocks(p);
     HIPCHECK(c, dalloc(c, &c->ext, (size_t)EX_COUNT * p.exnb));
+    {
+        const int nv = std::max(DIAG_STATE_VALUES, std::max(DIAG_TENSOR_VALUES, DIAG_GEOMETRY_VALUES));
+        HIPCHECK(c, dalloc(c, &c->diag, (size_t)nv * p.exnb));
+        HIPCHECK(c, dalloc(c, &c->diag_out, (size_t)nv));
+    }
     p.exbuf = c->ext;
     // b_bc of the projection Laplacian: D * gradient_bc (Operators.jl:361-384)
     {
@@ -528,7 +535,7 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense, c->spec_tab,
-                      c->ext};
+                      c->ext,       c->diag,   c->diag_out};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
@@ -540,6 +547,160 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
 }
 
 extern "C" const char* fsi_last_error(const fsi_context* c) { return c ? c->err.c_str() : "null context"; }
+
+// ---- driver diagnostics -----------------------------------------------------
+static fsi_status diag_fetch(fsi_context* c, double* v, int n) {
+    HIPCHECK(c, hipMemcpyAsync(v, c->diag_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+// pass 1 of the diagnostics: sums 0..10 then maxima 11..16 (k_diag_state)
+static fsi_status diag_state(fsi_context* c, const fsi_state* s, double density, double* v) {
+    launch_diag_state(c->stream, c->prm, s->U, s->V, s->phi, c->grid.lx, c->grid.ly, density, c->diag,
+                      c->diag_out);
+    return diag_fetch(c, v, DIAG_STATE_VALUES);
+}
+static double nan_if(double count, double x) { return count > 0.0 ? NAN : x; }
+
+extern "C" fsi_status fsi_velocity_extrema(fsi_context* c, const fsi_state* s, double* max_abs_u, double* max_abs_v) {
+    if (!c || !s || s->ctx != c || !max_abs_u || !max_abs_v) return FSI_E_ARGUMENT;
+    double v[DIAG_STATE_VALUES];
+    fsi_status rc = diag_state(c, s, 1.0, v);
+    if (rc != FSI_OK) return rc;
+    *max_abs_u = nan_if(v[10], v[11 + 4]);
+    *max_abs_v = nan_if(v[10], v[11 + 5]);
+    return FSI_OK;
+}
+
+// phase_centroid: mass > eps(Float64) || error(...) (:497-498)
+static fsi_status centroid_of(fsi_context* c, const double* v, double& xc, double& yc, double& mass) {
+    mass = v[0];
+    if (!(mass > 2.220446049250313e-16)) return fail(c, FSI_E_SOLVER, "Phase integral is too small to define a centroid.");
+    xc = v[1] / mass;
+    yc = v[2] / mass;
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_phase_centroid(fsi_context* c, const fsi_state* s, double* x_centroid, double* y_centroid,
+                                         double* phase_integral) {
+    if (!c || !s || s->ctx != c || !x_centroid || !y_centroid || !phase_integral) return FSI_E_ARGUMENT;
+    double v[DIAG_STATE_VALUES];
+    fsi_status rc = diag_state(c, s, 1.0, v);
+    if (rc != FSI_OK) return rc;
+    return centroid_of(c, v, *x_centroid, *y_centroid, *phase_integral);
+}
+
+// tensor_region_diagnostics (:753-822) from the sums 0..5 and maxima 6..11
+// of k_diag_tensor
+static void fill_region(const double* v, fsi_tensor_region& r) {
+    const double cells = v[0], area = v[1];
+    if (cells == 0.0 || area == 0.0) {
+        r = fsi_tensor_region{0, 0.0, NAN, NAN, NAN, NAN, NAN, NAN, 0, NAN, NAN};
+        return;
+    }
+    const double nan = v[5];
+    r.cells = (int64_t)cells;
+    r.area = area;
+    r.determinant_min = nan_if(nan, -v[6 + 1]);
+    r.determinant_max = nan_if(nan, v[6 + 0]);
+    r.max_abs_determinant_error = nan_if(nan, v[6 + 2]);
+    r.mean_abs_determinant_error = v[2] / area;
+    r.lambda_min = nan_if(nan, -v[6 + 4]);
+    r.lambda_max = nan_if(nan, v[6 + 3]);
+    r.non_spd_cells = (int64_t)v[4];
+    r.max_effective_stress = nan_if(nan, v[6 + 5]);
+    r.mean_effective_stress = v[3] / area;
+}
+
+extern "C" fsi_status fsi_diagnostics_compute(fsi_context* c, const fsi_state* s, const fsi_diag_params* prm,
+                                              fsi_diagnostics* out) {
+    if (!c || !s || s->ctx != c || !prm || !out) return FSI_E_ARGUMENT;
+    const fsi_diag_params& q = *prm;
+    if (q.deformation_mode_max < 0 || q.deformation_mode_max > FSI_DIAG_MAX_MODES)
+        return fail(c, FSI_E_ARGUMENT, "deformation_mode_max must lie in 0..%d.", FSI_DIAG_MAX_MODES);
+    if (!(0.0 <= q.sigma_safety_cutoff && q.sigma_safety_cutoff < 1.0))
+        return fail(c, FSI_E_ARGUMENT, "sigma_safety_cutoff must lie in [0,1).");
+    if (!(0.0 < q.sigma_interface_cutoff && q.sigma_interface_cutoff < q.sigma_core_cutoff &&
+          q.sigma_core_cutoff <= 1.0))
+        return fail(c, FSI_E_ARGUMENT, "Require 0 < sigma_interface_cutoff < sigma_core_cutoff <= 1.");
+    if (!(q.sigma_safety_cutoff < q.sigma_interface_cutoff))
+        return fail(c, FSI_E_ARGUMENT, "sigma_safety_cutoff must be smaller than sigma_interface_cutoff.");
+    *out = fsi_diagnostics{};
+    double v1[DIAG_STATE_VALUES];
+    fsi_status rc = diag_state(c, s, q.density, v1);
+    if (rc != FSI_OK) return rc;
+    double xc, yc, mass;
+    if ((rc = centroid_of(c, v1, xc, yc, mass)) != FSI_OK) return rc;
+    const double* m1 = v1 + 11;
+    // phase_bound_diagnostics
+    out->phi_min = nan_if(v1[8], -m1[1]);
+    out->phi_max = nan_if(v1[8], m1[0]);
+    out->lower_violation = std::isnan(out->phi_min) ? NAN : std::fmax(0.0, -out->phi_min);
+    out->upper_violation = std::isnan(out->phi_max) ? NAN : std::fmax(0.0, out->phi_max - 1.0);
+    out->out_of_bounds_cells = (int64_t)v1[7];
+    // flow_diagnostics (max_speed starts from 0.0)
+    out->maximum_speed = nan_if(v1[9], std::fmax(0.0, m1[2]));
+    out->kinetic_energy = v1[4];
+    out->divergence_linf = nan_if(v1[9], m1[3]);
+    out->divergence_l2 = std::sqrt(v1[5] / v1[6]);
+    out->max_abs_u = nan_if(v1[10], m1[4]);
+    out->max_abs_v = nan_if(v1[10], m1[5]);
+    // phase_geometry_diagnostics: second pass about the centroid, the modes
+    // in launches of diag_modes_per_launch()
+    const int nmax = q.deformation_mode_max, per = diag_modes_per_launch();
+    double perimeter = 0.0, cxx = 0.0, cyy = 0.0, cxy = 0.0;
+    for (int n0 = 0; n0 == 0 || n0 < nmax; n0 += per) {
+        double v2[DIAG_GEOMETRY_VALUES];
+        launch_diag_geometry(c->stream, c->prm, s->phi, c->grid.lx, c->grid.ly, xc, yc, n0, nmax, c->diag,
+                             c->diag_out);
+        if ((rc = diag_fetch(c, v2, DIAG_GEOMETRY_VALUES)) != FSI_OK) return rc;
+        if (n0 == 0) {
+            perimeter = v2[0];
+            cxx = v2[1];
+            cyy = v2[2];
+            cxy = v2[3];
+        }
+        for (int k = 0; k < per && n0 + k < nmax; ++k) {
+            out->Rc[n0 + k] = v2[4 + 2 * k];
+            out->Rs[n0 + k] = v2[5 + 2 * k];
+        }
+    }
+    cxx /= mass;
+    cyy /= mass;
+    cxy /= mass;
+    const double gap = std::hypot(cxx - cyy, 2.0 * cxy);
+    out->x_centroid = xc;
+    out->y_centroid = yc;
+    out->phase_integral = mass;
+    out->perimeter_proxy = perimeter;
+    out->mixed_layer_measure = v1[3];
+    out->covariance_major = 0.5 * (cxx + cyy + gap);
+    out->covariance_minor = 0.5 * (cxx + cyy - gap);
+    out->aspect_ratio = out->covariance_minor > 0.0 ? std::sqrt(out->covariance_major / out->covariance_minor)
+                                                    : INFINITY;
+    out->orientation_radians = gap > 100.0 * 2.220446049250313e-16 ? 0.5 * std::atan2(2.0 * cxy, cxx - cyy) : 0.0;
+    out->R0 = perimeter / (2.0 * M_PI);
+    out->nmax = nmax;
+    for (int n = 0; n < nmax; ++n) out->amplitudes[n] = std::hypot(out->Rc[n], out->Rs[n]);
+    out->effective_interface_width = perimeter > 2.220446049250313e-16 ? v1[3] / perimeter : NAN;
+    out->effective_interface_cells = out->effective_interface_width / q.h_interface;
+    // tensor_diagnostics_detailed: global, core, interface, tail
+    const DiagRegion regions[4] = {
+        {1, 0.0, 0.0, 0, 0},
+        {0, q.sigma_core_cutoff, INFINITY, 1, 1},
+        {0, q.sigma_interface_cutoff, q.sigma_core_cutoff, 0, 0},
+        {0, q.sigma_safety_cutoff, q.sigma_interface_cutoff, 0, 1},
+    };
+    fsi_tensor_region* dst[4] = {&out->global_region, &out->core, &out->interface_region, &out->tail};
+    for (int r = 0; r < 4; ++r) {
+        double vt[DIAG_TENSOR_VALUES];
+        launch_diag_tensor(c->stream, c->prm, s->phi, s->sxx, s->syy, s->sxy, q.elastic_modulus, regions[r],
+                           c->diag, c->diag_out);
+        if ((rc = diag_fetch(c, vt, DIAG_TENSOR_VALUES)) != FSI_OK) return rc;
+        fill_region(vt, *dst[r]);
+    }
+    return FSI_OK;
+}
 
 extern "C" const char* fsi_version(void) {
     return "libfsi_hip 0.1 (gfx950, FP64 IMEX step: CG velocity predictor, MG-PCG projection)";

@@ -62,6 +62,24 @@ void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, do
 // partials of the step's last combination and of its stage-1 tendencies
 void launch_ext_final(hipStream_t s, const Params& p, Status* st);
 
+// ---- driver diagnostics (fsi_physics.hip) ----
+// Each launcher runs a per-block reduction into buf (rows of p.exnb partials)
+// and a one-block final reduction into out: the sums, then the maxima (a
+// minimum negated), in the order documented at each kernel.
+struct DiagRegion {
+    int all;           // every cell
+    double lo, hi;     // else lo < phi < hi, each bound inclusive when *_in
+    int lo_in, hi_in;
+};
+constexpr int DIAG_STATE_VALUES = 11 + 6, DIAG_TENSOR_VALUES = 6 + 6, DIAG_GEOMETRY_VALUES = 4 + 2 * 8 + 1;
+void launch_diag_state(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
+                       double lx, double ly, double density, double* buf, double* out);
+void launch_diag_tensor(hipStream_t s, const Params& p, const double* phi, const double* sxx, const double* syy,
+                        const double* sxy, double G, const DiagRegion& rg, double* buf, double* out);
+void launch_diag_geometry(hipStream_t s, const Params& p, const double* phi, double lx, double ly, double xc,
+                          double yc, int n0, int nmax, double* buf, double* out);
+int diag_modes_per_launch();
+
 // ---- velocity predictor CG (fsi_solvers.hip) ----
 struct VelBufs {
     const double* Un;  // warm start: uv_n (state velocity) or the previous step's stage solution

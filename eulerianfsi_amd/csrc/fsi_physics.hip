@@ -1451,4 +1451,243 @@ void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, do
     k_visc_boundary<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, zero, bu, bv);
 }
 
+// ===========================================================================
+// Driver diagnostics (scripts/benchmarks/run_sugiyama_lid_driven_cavity.jl
+// :483-934, BenchmarkUtilities.jl:62-88).  Each kernel writes per block NS
+// sums and NM maxima (a minimum is stored negated) as rows of nb values;
+// k_diag_final reduces them in a fixed order, so the results are
+// deterministic.  Sums that count cells are exact in double up to 2^53.
+// Julia's minimum/maximum propagate NaN while fmax skips it, so every kernel
+// also counts the NaN values its extrema see; the host reports NaN for those
+// extrema when the count is not zero.
+// ===========================================================================
+template <int NS, int NM>
+__device__ __forceinline__ void diag_put(double (&sm)[NS], double (&mx)[NM], double* buf, int nb) {
+    __shared__ double ls[NS * 16];
+    __shared__ double lm[NM][BLOCK / 64];
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+    block_sum<NS>(sm, ls);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) mx[q] = wave_max(mx[q]);
+    if ((lane & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NM; ++q) lm[q][lane >> 6] = mx[q];
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) buf[(size_t)q * nb + bid] = sm[q];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            double v = lm[q][0];
+            for (int w = 1; w < BLOCK / 64; ++w) v = fmax(v, lm[q][w]);
+            buf[(size_t)(NS + q) * nb + bid] = v;
+        }
+    }
+}
+
+// x_p, y_p of a uniform grid (MeshGeneration.jl:215: L * ((i - 0.5) / N), 1-based)
+__device__ __forceinline__ double centre_x(double lx, int nx, int i) { return lx * (((double)i + 0.5) / (double)nx); }
+
+// pass 1: phase integral and first moments (phase_centroid :496-509), mixed
+// layer (:656), bounds (phase_bound_diagnostics :709-722), flow
+// (flow_diagnostics :724-751, with the MAC divergence), velocity extrema
+// (comparison_timestep_bounds :862-866).
+// sums: 0 sum phi a, 1 sum x phi a, 2 sum y phi a, 3 sum a 4 phi~ (1 - phi~),
+//       4 kinetic energy, 5 sum a div^2, 6 sum a, 7 cells outside [0, 1],
+//       NaN counts: 8 phi, 9 speed or divergence, 10 U or V
+// maxima: 0 phi max, 1 -phi min, 2 speed, 3 |div|, 4 |U|, 5 |V|
+constexpr int DG1_S = 11, DG1_M = 6;
+__global__ void __launch_bounds__(BLOCK) k_diag_state(Params p, const double* U, const double* V, const double* phi,
+                                                      double lx, double ly, double density, double* buf) {
+    double sm[DG1_S], mx[DG1_M];
+#pragma unroll
+    for (int q = 0; q < DG1_S; ++q) sm[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < DG1_M; ++q) mx[q] = -INFINITY;
+    const double area = p.hx * p.hy;
+    const TileId T = tile_id();
+    const int i = T.x * TILE_X + threadIdx.x;
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = T.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (i >= p.nx || j >= p.ny) break;
+        const double ph = phi[at(p, i, j)];
+        const double pb = clamp01(ph);
+        sm[0] += ph * area;
+        const double w = ph * area;
+        sm[1] += centre_x(lx, p.nx, i) * w;
+        sm[2] += centre_x(ly, p.ny, j) * w;
+        sm[3] += area * 4.0 * pb * (1.0 - pb);
+        const double uL = U[at(p, i, j)], uR = U[at(p, i + 1, j)], vB = V[at(p, i, j)], vT = V[at(p, i, j + 1)];
+        const double uc = 0.5 * (uL + uR), vc = 0.5 * (vB + vT);
+        const double s2 = uc * uc + vc * vc;
+        sm[4] += 0.5 * density * s2 * area;
+        const double div = (uR - uL) / p.hx + (vT - vB) / p.hy;
+        sm[5] += area * div * div;
+        sm[6] += area;
+        if (ph < 0.0 || ph > 1.0) sm[7] += 1.0;
+        mx[0] = fmax(mx[0], ph);
+        mx[1] = fmax(mx[1], -ph);
+        mx[2] = fmax(mx[2], sqrt(s2));
+        mx[3] = fmax(mx[3], fabs(div));
+        mx[4] = fmax(mx[4], fabs(uL));
+        mx[5] = fmax(mx[5], fabs(vB));
+        double nan_uv = (isnan(uL) ? 1.0 : 0.0) + (isnan(vB) ? 1.0 : 0.0);
+        if (i == p.nx - 1) {  // the last x-face column
+            mx[4] = fmax(mx[4], fabs(uR));
+            nan_uv += isnan(uR) ? 1.0 : 0.0;
+        }
+        if (j == p.ny - 1) {  // the last y-face row
+            mx[5] = fmax(mx[5], fabs(vT));
+            nan_uv += isnan(vT) ? 1.0 : 0.0;
+        }
+        sm[8] += isnan(ph) ? 1.0 : 0.0;
+        sm[9] += (isnan(s2) || isnan(div)) ? 1.0 : 0.0;
+        sm[10] += nan_uv;
+    }
+    diag_put<DG1_S, DG1_M>(sm, mx, buf, p.exnb);
+}
+
+// one region of tensor_region_diagnostics (:753-822) with
+// tensor_cell_diagnostics_2d (BenchmarkUtilities.jl:62-88): the cells with
+// lo <(=) phi <(=) hi, or every cell when all != 0.
+// sums: 0 cells, 1 area, 2 sum a |det - 1|, 3 sum a stress, 4 non-SPD cells, 5 NaN count
+// maxima: 0 det, 1 -det, 2 |det - 1|, 3 lambda_max, 4 -lambda_min, 5 stress
+constexpr int DGT_S = 6, DGT_M = 6;
+__global__ void __launch_bounds__(BLOCK) k_diag_tensor(Params p, const double* phi, const double* sxx,
+                                                       const double* syy, const double* sxy, double G, int all,
+                                                       double lo, int lo_in, double hi, int hi_in, double* buf) {
+    double sm[DGT_S], mx[DGT_M];
+#pragma unroll
+    for (int q = 0; q < DGT_S; ++q) sm[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < DGT_M; ++q) mx[q] = -INFINITY;
+    const double area = p.hx * p.hy;
+    const TileId T = tile_id();
+    const int i = T.x * TILE_X + threadIdx.x;
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = T.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (i >= p.nx || j >= p.ny) break;
+        const size_t k = at(p, i, j);
+        const double ph = phi[k];
+        const bool sel = all || ((lo_in ? ph >= lo : ph > lo) && (hi_in ? ph <= hi : ph < hi));
+        if (!sel) continue;
+        const double xx = sxx[k], yy = syy[k], xy = sxy[k];
+        const double det = xx * yy - xy * xy;
+        const double tr = xx + yy;
+        const double gap = hypot(xx - yy, 2.0 * xy);
+        const double lmin = 0.5 * (tr - gap), lmax = 0.5 * (tr + gap);
+        const double stress = fabs(ph * G) * sqrt((xx - 1.0) * (xx - 1.0) + (yy - 1.0) * (yy - 1.0) + 2.0 * (xy * xy));
+        const double derr = fabs(det - 1.0);
+        sm[0] += 1.0;
+        sm[1] += area;
+        sm[2] += area * derr;
+        sm[3] += area * stress;
+        if (!(isfinite(lmin) && lmin > 0.0)) sm[4] += 1.0;
+        sm[5] += (isnan(det) || isnan(lmin) || isnan(lmax) || isnan(stress)) ? 1.0 : 0.0;
+        mx[0] = fmax(mx[0], det);
+        mx[1] = fmax(mx[1], -det);
+        mx[2] = fmax(mx[2], derr);
+        mx[3] = fmax(mx[3], lmax);
+        mx[4] = fmax(mx[4], -lmin);
+        mx[5] = fmax(mx[5], stress);
+    }
+    diag_put<DGT_S, DGT_M>(sm, mx, buf, p.exnb);
+}
+
+// pass 2 of phase_geometry_diagnostics (:632-707), about the centroid
+// (xc, yc): modes n0 + 1 .. n0 + DG2_MODES of the interface-weighted Fourier
+// series, and with n0 == 0 the perimeter proxy and the second moments.
+// sums: 0 perimeter, 1 cxx, 2 cyy, 3 cxy, 4 + 2q Rc[n0 + q + 1], 5 + 2q Rs[...]
+constexpr int DG2_MODES = 8, DG2_S = 4 + 2 * DG2_MODES;
+__global__ void __launch_bounds__(BLOCK) k_diag_geometry(Params p, const double* phi, double lx, double ly, double xc,
+                                                         double yc, int n0, int nmax, double* buf) {
+    double sm[DG2_S], mx[1] = {-INFINITY};
+#pragma unroll
+    for (int q = 0; q < DG2_S; ++q) sm[q] = 0.0;
+    const double area = p.hx * p.hy;
+    const TileId T = tile_id();
+    const int i = T.x * TILE_X + threadIdx.x;
+    auto f = [&](int a, int b) { return phi[at(p, a, b)]; };
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = T.y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (i >= p.nx || j >= p.ny) break;
+        double gx, gy;
+        grad_center(p, p.bPhi, f, i, j, gx, gy);
+        const double iw = area * hypot(gx, gy);
+        const double dx = centre_x(lx, p.nx, i) - xc, dy = centre_x(ly, p.ny, j) - yc;
+        if (n0 == 0) {
+            const double mw = area * phi[at(p, i, j)];
+            sm[0] += iw;
+            sm[1] += mw * dx * dx;
+            sm[2] += mw * dy * dy;
+            sm[3] += mw * dx * dy;
+        }
+        const double theta = atan2(dy, dx);
+#pragma unroll
+        for (int q = 0; q < DG2_MODES; ++q) {
+            const int n = n0 + q + 1;
+            if (n > nmax) break;
+            double sn, cn;
+            sincos((double)n * theta, &sn, &cn);
+            sm[4 + 2 * q] += iw * cn / M_PI;
+            sm[5 + 2 * q] += iw * sn / M_PI;
+        }
+    }
+    diag_put<DG2_S, 1>(sm, mx, buf, p.exnb);
+}
+
+// one block: out[q] = total of row q (sums in a fixed order, then maxima)
+template <int NS, int NM>
+__global__ void __launch_bounds__(BLOCK) k_diag_final(const double* buf, int nb, double* out) {
+    double sm[NS], mx[NM];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) sm[q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < NM; ++q) mx[q] = -INFINITY;
+    for (int b = threadIdx.x; b < nb; b += BLOCK) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) sm[q] += buf[(size_t)q * nb + b];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) mx[q] = fmax(mx[q], buf[(size_t)(NS + q) * nb + b]);
+    }
+    __shared__ double ls[NS * 16];
+    __shared__ double lm[NM][BLOCK / 64];
+    block_sum<NS>(sm, ls);
+#pragma unroll
+    for (int q = 0; q < NM; ++q) mx[q] = wave_max(mx[q]);
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int q = 0; q < NM; ++q) lm[q][threadIdx.x >> 6] = mx[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) out[q] = sm[q];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            double v = lm[q][0];
+            for (int w = 1; w < BLOCK / 64; ++w) v = fmax(v, lm[q][w]);
+            out[NS + q] = v;
+        }
+    }
+}
+
+void launch_diag_state(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
+                       double lx, double ly, double density, double* buf, double* out) {
+    k_diag_state<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, phi, lx, ly, density, buf);
+    k_diag_final<DG1_S, DG1_M><<<1, BLOCK, 0, s>>>(buf, p.exnb, out);
+}
+void launch_diag_tensor(hipStream_t s, const Params& p, const double* phi, const double* sxx, const double* syy,
+                        const double* sxy, double G, const DiagRegion& rg, double* buf, double* out) {
+    k_diag_tensor<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, phi, sxx, syy, sxy, G, rg.all, rg.lo, rg.lo_in,
+                                                             rg.hi, rg.hi_in, buf);
+    k_diag_final<DGT_S, DGT_M><<<1, BLOCK, 0, s>>>(buf, p.exnb, out);
+}
+void launch_diag_geometry(hipStream_t s, const Params& p, const double* phi, double lx, double ly, double xc,
+                          double yc, int n0, int nmax, double* buf, double* out) {
+    k_diag_geometry<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, phi, lx, ly, xc, yc, n0, nmax, buf);
+    k_diag_final<DG2_S, 1><<<1, BLOCK, 0, s>>>(buf, p.exnb, out);
+}
+int diag_modes_per_launch() { return DG2_MODES; }
+
 }  // namespace fsi

@@ -243,6 +243,101 @@ fsi_status fsi_eval_tendencies(fsi_context* ctx, const fsi_state* st, fsi_worksp
                                double* E_u, double* E_v, double* I_u, double* I_v,
                                double* E_phi, double* E_sxx, double* E_syy, double* E_sxy);
 
+/* ---- driver diagnostics ---------------------------------------------------
+ * The per-step and per-interval reductions of the reference's cavity driver
+ * (scripts/benchmarks/run_sugiyama_lid_driven_cavity.jl, "DIAGNOSTICS" and
+ * "TIMESTEP SELECTION", :483-934), computed on the device-resident state so
+ * that a driver loop never copies fields to the host.  Uniform grids with the
+ * origin at (0, 0): x_p[i] = lx (i + 1/2) / nx (MeshGeneration.jl:215).
+ * Sums are reduced in a fixed order (deterministic); a NaN seen by an
+ * extremum makes that extremum NaN, as Julia's minimum/maximum do. */
+
+/* max |U| over every x-face and max |V| over every y-face of the state: the
+ * data comparison_timestep_bounds reads before every step (:862-866). */
+fsi_status fsi_velocity_extrema(fsi_context* ctx, const fsi_state* st, double* max_abs_u,
+                                double* max_abs_v);
+
+/* phase_centroid (:496-509).  FSI_E_SOLVER (the reference's error()) when
+ * the phase integral is not above eps(Float64). */
+fsi_status fsi_phase_centroid(fsi_context* ctx, const fsi_state* st, double* x_centroid,
+                              double* y_centroid, double* phase_integral);
+
+#define FSI_DIAG_MAX_MODES 32
+
+/* BenchmarkConfig fields the interval diagnostics read (:40-128). */
+typedef struct {
+    double density;                 /* flow_diagnostics kinetic energy */
+    double elastic_modulus;         /* tensor_cell_diagnostics_2d stress */
+    double sigma_core_cutoff;       /* tensor_diagnostics_detailed regions :824-849 */
+    double sigma_interface_cutoff;
+    double sigma_safety_cutoff;
+    double h_interface;             /* effective_interface_cells = width / h_interface */
+    int32_t deformation_mode_max;   /* nmax of the Fourier modes, 0..FSI_DIAG_MAX_MODES */
+} fsi_diag_params;
+
+/* tensor_region_diagnostics (:753-822), fields in TENSOR_REGION_FIELDS
+ * order (:1069-1081); an empty region has NaN extrema and means. */
+typedef struct {
+    int64_t cells;
+    double area;
+    double determinant_min;
+    double determinant_max;
+    double max_abs_determinant_error;
+    double mean_abs_determinant_error;
+    double lambda_min;
+    double lambda_max;
+    int64_t non_spd_cells;
+    double max_effective_stress;
+    double mean_effective_stress;
+} fsi_tensor_region;
+
+typedef struct {
+    /* phase_geometry_diagnostics (:632-707) */
+    double x_centroid;
+    double y_centroid;
+    double phase_integral;
+    double perimeter_proxy;
+    double mixed_layer_measure;
+    double effective_interface_width;
+    double effective_interface_cells;
+    double covariance_major;
+    double covariance_minor;
+    double aspect_ratio;
+    double orientation_radians;
+    double R0;
+    int32_t nmax;
+    double Rc[FSI_DIAG_MAX_MODES];
+    double Rs[FSI_DIAG_MAX_MODES];
+    double amplitudes[FSI_DIAG_MAX_MODES];
+    /* phase_bound_diagnostics (:709-722) */
+    double phi_min;
+    double phi_max;
+    double lower_violation;
+    double upper_violation;
+    int64_t out_of_bounds_cells;
+    /* flow_diagnostics (:724-751) */
+    double maximum_speed;
+    double kinetic_energy;
+    double divergence_linf;
+    double divergence_l2;
+    /* the velocity extrema of fsi_velocity_extrema */
+    double max_abs_u;
+    double max_abs_v;
+    /* tensor_diagnostics_detailed (:824-849): every cell; phi >= core;
+     * interface < phi < core; safety < phi <= interface */
+    fsi_tensor_region global_region;
+    fsi_tensor_region core;
+    fsi_tensor_region interface_region;
+    fsi_tensor_region tail;
+} fsi_diagnostics;
+
+/* Every interval diagnostic of the driver in one call (two passes over the
+ * state, the second about the centroid).  FSI_E_ARGUMENT for parameters the
+ * driver's validation rejects (:205-236) or nmax outside 0..32; FSI_E_SOLVER
+ * when the phase integral cannot define a centroid. */
+fsi_status fsi_diagnostics_compute(fsi_context* ctx, const fsi_state* st, const fsi_diag_params* params,
+                                   fsi_diagnostics* out);
+
 /* Measurement hook; no counterpart in the reference (whose code is not
  * instrumented).  Launches the named kernel `reps` times on the context's
  * stream with the buffers of the workspace (after at least one step), timed

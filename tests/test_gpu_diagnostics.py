@@ -1,0 +1,136 @@
+"""The driver diagnostics on the device (fsi_velocity_extrema,
+fsi_phase_centroid, fsi_diagnostics_compute) against the NumPy restatement of
+the reference driver (oracle/diagnostics.py) on stepped states and on a
+synthetic state with out-of-range phase, non-SPD tensors and a NaN.
+Tolerance: 1e-12 relative for sums (summation order differs), 1e-14 for
+extrema of derived quantities; field extrema must be exact."""
+import math
+
+import numpy as np
+import pytest
+
+import eulerianfsi_amd as fsi
+from eulerianfsi_amd import diagnostics as D
+from eulerianfsi_amd import setups
+from oracle import diagnostics as O
+
+pytestmark = pytest.mark.gpu
+
+NMAX = 12
+
+
+def _close(a, b, rtol, what):
+    if isinstance(b, float) and math.isnan(b):
+        assert math.isnan(a), what
+        return
+    scale = max(abs(b), 1e-300)
+    assert abs(a - b) <= rtol * scale + 1e-300, (what, a, b)
+
+
+def _stepped(name):
+    if name == "cavity":
+        case = setups.seeded_cavity(96, "acdi", seed=5)
+        bc = (("N", 0.0),) * 4
+    else:
+        case = setups.kolmogorov_filament(64, "uwcdi")
+        bc = (("P", 0.0), ("P", 0.0), ("N", 0.0), ("N", 0.0))
+    ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
+    work = fsi.allocate_imex_workspace(ctx)
+    fsi.make_velocity_compatible(case.state, ctx, work)
+    for k in range(3):
+        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+    return case, ctx, bc
+
+
+def _check(case, ctx, bc, cfg):
+    n = case.grid.Nx
+    g = O.Grid(case.grid.Lx, case.grid.Ly, n, case.grid.Ny, bc)
+    s = case.state
+    U, V, phi, sxx, syy, sxy = s.U, s.V, s.phi, s.sigma_xx, s.sigma_yy, s.sigma_xy
+    h = g.hx
+    dev = D.compute_diagnostics(s, ctx, cfg, h)
+    print("centroid", dev.geometry.x_centroid, dev.geometry.y_centroid, "R0", dev.geometry.R0, flush=True)
+    # velocity extrema: exact
+    mu, mv = D.velocity_extrema(s, ctx)
+    assert (mu, mv) == (float(np.max(np.abs(U))), float(np.max(np.abs(V))))
+    assert (dev.max_abs_u, dev.max_abs_v) == (mu, mv)
+    geo = O.phase_geometry_diagnostics(phi, g, h, NMAX)
+    xc, yc, mass = D.phase_centroid(s, ctx)
+    _close(xc, geo["x_centroid"], 1e-12, "xc")
+    _close(yc, geo["y_centroid"], 1e-12, "yc")
+    _close(mass, geo["phase_integral"], 1e-12, "mass")
+    for f in ("x_centroid", "y_centroid", "phase_integral", "perimeter_proxy", "mixed_layer_measure",
+              "effective_interface_width", "effective_interface_cells", "covariance_major", "covariance_minor",
+              "aspect_ratio", "R0"):
+        _close(getattr(dev.geometry, f), geo[f], 1e-11, f)
+    assert abs(dev.geometry.orientation_radians - geo["orientation_radians"]) < 1e-9
+    amp_scale = max(geo["R0"], 1e-300)
+    for k in range(NMAX):  # modes relative to R0 (small modes carry cancellation)
+        assert abs(dev.geometry.Rc[k] - geo["Rc"][k]) <= 1e-11 * amp_scale, ("Rc", k)
+        assert abs(dev.geometry.Rs[k] - geo["Rs"][k]) <= 1e-11 * amp_scale, ("Rs", k)
+    b = O.phase_bound_diagnostics(phi)
+    assert (dev.bounds.minimum, dev.bounds.maximum, dev.bounds.out_of_bounds_cells) == (
+        b["minimum"], b["maximum"], b["out_of_bounds_cells"])
+    fl = O.flow_diagnostics(U, V, g, cfg.density)
+    _close(dev.flow.maximum_speed, fl["maximum_speed"], 1e-14, "max speed")
+    _close(dev.flow.kinetic_energy, fl["kinetic_energy"], 1e-12, "kinetic energy")
+    _close(dev.flow.divergence_linf, fl["divergence_linf"], 1e-12, "div linf")
+    _close(dev.flow.divergence_l2, fl["divergence_l2"], 1e-10, "div l2")
+    t = O.tensor_diagnostics_detailed(phi, sxx, syy, sxy, g, cfg.elastic_modulus, cfg.sigma_core_cutoff,
+                                      cfg.sigma_interface_cutoff, cfg.sigma_safety_cutoff)
+    for region, ref in zip(dev.tensor, (t["global_region"], t["core"], t["interface"], t["tail"])):
+        assert region.cells == ref["cells"] and region.non_spd_cells == ref["non_spd_cells"]
+        for f in ("area", "mean_effective_stress"):
+            _close(getattr(region, f), ref[f], 1e-12, f)
+        for f in ("determinant_min", "determinant_max", "lambda_min", "lambda_max", "max_effective_stress"):
+            _close(getattr(region, f), ref[f], 1e-14, f)
+        # |det - 1| cancels: compared on the scale of det (the device may fuse
+        # xx*yy - xy*xy into one rounding)
+        for f in ("max_abs_determinant_error", "mean_abs_determinant_error"):
+            a, b = getattr(region, f), ref[f]
+            assert (math.isnan(a) and math.isnan(b)) or abs(a - b) <= 1e-14 * (1.0 + abs(region.determinant_max)), f
+    return dev
+
+
+@pytest.mark.parametrize("name", ["cavity", "filament"])
+def test_diagnostics_match_oracle(name):
+    case, ctx, bc = _stepped(name)
+    cfg = D.BenchmarkConfig(elastic_modulus=case.params["shear_modulus"], deformation_mode_max=NMAX)
+    dev = _check(case, ctx, bc, cfg)
+    assert dev.tensor.global_region.cells == case.grid.Nx * case.grid.Ny
+
+
+def test_out_of_range_state_and_nan():
+    """phi outside [0, 1], tensors with negative eigenvalues, then a NaN in
+    sigma_xy of one core cell: the core's extrema become NaN, as Julia's
+    min/max make them, and the cell counts as non-SPD."""
+    case, ctx, bc = _stepped("cavity")
+    n = case.grid.Nx
+    rng = np.random.default_rng(11)
+    s = case.state
+    s.phi = rng.uniform(-0.2, 1.2, (n, n))
+    s.sigma_xx = rng.uniform(-0.5, 2.0, (n, n))
+    s.sigma_yy = rng.uniform(0.2, 2.0, (n, n))
+    s.sigma_xy = rng.uniform(-0.5, 0.5, (n, n))
+    cfg = D.BenchmarkConfig(elastic_modulus=0.1, deformation_mode_max=NMAX)
+    dev = _check(case, ctx, bc, cfg)
+    assert dev.bounds.out_of_bounds_cells > 0 and dev.tensor.global_region.non_spd_cells > 0
+    sxy = s.sigma_xy.copy()
+    i, j = np.argwhere(s.phi >= 0.9)[0]
+    sxy[i, j] = math.nan
+    s.sigma_xy = sxy
+    dev = _check(case, ctx, bc, cfg)
+    assert math.isnan(dev.tensor.core.lambda_min) and math.isnan(dev.tensor.global_region.determinant_max)
+    assert not math.isnan(dev.tensor.tail.lambda_min)
+
+
+def test_argument_and_centroid_errors():
+    case, ctx, _ = _stepped("cavity")
+    h = 1.0 / case.grid.Nx
+    with pytest.raises(fsi.ArgumentError):
+        D.compute_diagnostics(case.state, ctx, D.BenchmarkConfig(deformation_mode_max=33), h)
+    with pytest.raises(fsi.ArgumentError):
+        D.compute_diagnostics(case.state, ctx, D.BenchmarkConfig(sigma_interface_cutoff=0.95), h)
+    case.state.phi = np.zeros_like(case.state.phi)
+    with pytest.raises(fsi.SolverError, match="too small to define a centroid"):
+        D.phase_centroid(case.state, ctx)
