@@ -43,34 +43,41 @@ __device__ __forceinline__ void load_sigma(const Params& p, const Stage& s, int 
 // (wrapped) index: tile position (li, lj) holds global (i0 + li - 1,
 // j0 + lj - 1), wrapped across a periodic side.
 constexpr int MT_W = TILE_X + 2, MT_H = TILE_ROWS + 2;  // 66 x 18
+// IN: the tile and its halo lie inside the domain (no wrap).
+template <bool IN = false>
 struct HaloTile {
     const double* a;
     int i0, j0, nx, ny;
     __device__ __forceinline__ double operator()(int gi, int gj) const {
         int li = gi - i0 + 1, lj = gj - j0 + 1;
-        if (li < 0) li += nx;
-        else if (li >= MT_W) li -= nx;
-        if (lj < 0) lj += ny;
-        else if (lj >= MT_H) lj -= ny;
+        if (!IN) {
+            if (li < 0) li += nx;
+            else if (li >= MT_W) li -= nx;
+            if (lj < 0) lj += ny;
+            else if (lj >= MT_H) lj -= ny;
+        }
         return a[lj * MT_W + li];
     }
 };
 
 // ---- material convection pieces (MomentumTransport.jl:141-219) -----------
-template <class FU>
+// IN = true: every index lies at least one cell inside the walls, so the wall
+// and wrap branches compile out (same arithmetic on interior points).
+template <bool IN = false, class FU>
 __device__ __forceinline__ double u2c(const Params& p, const FU& U, int k, int j) {
-    double ur = (p.px && k == p.nx - 1) ? U(0, j) : U(k + 1, j);
+    double ur = (!IN && p.px && k == p.nx - 1) ? U(0, j) : U(k + 1, j);
     double uc = 0.5 * U(k, j) + 0.5 * ur;
     return uc * uc;
 }
-template <class FV>
+template <bool IN = false, class FV>
 __device__ __forceinline__ double v2c(const Params& p, const FV& V, int i, int k) {
-    double vt = (p.py && k == p.ny - 1) ? V(i, 0) : V(i, k + 1);
+    double vt = (!IN && p.py && k == p.ny - 1) ? V(i, 0) : V(i, k + 1);
     double vc = 0.5 * V(i, k) + 0.5 * vt;
     return vc * vc;
 }
-template <class FU>
+template <bool IN = false, class FU>
 __device__ __forceinline__ double u_corner(const Params& p, const FU& U, int i, int jc) {
+    if (IN) return 0.5 * U(i, jc - 1) + 0.5 * U(i, jc);
     int ii = (p.px && i == p.nx) ? 0 : i;
     if (p.py) {
         if (jc == 0 || jc == p.ny) return 0.5 * U(ii, p.ny - 1) + 0.5 * U(ii, 0);
@@ -86,8 +93,9 @@ __device__ __forceinline__ double u_corner(const Params& p, const FU& U, int i, 
     }
     return 0.5 * U(ii, jc - 1) + 0.5 * U(ii, jc);
 }
-template <class FV>
+template <bool IN = false, class FV>
 __device__ __forceinline__ double v_corner(const Params& p, const FV& V, int i, int jc) {
+    if (IN) return 0.5 * V(i - 1, jc) + 0.5 * V(i, jc);
     int jj = (p.py && jc == p.ny) ? 0 : jc;
     if (p.px) {
         if (i == 0 || i == p.nx) return 0.5 * V(p.nx - 1, jj) + 0.5 * V(0, jj);
@@ -103,9 +111,9 @@ __device__ __forceinline__ double v_corner(const Params& p, const FV& V, int i, 
     }
     return 0.5 * V(i - 1, jj) + 0.5 * V(i, jj);
 }
-template <class FU, class FV>
+template <bool IN = false, class FU, class FV>
 __device__ __forceinline__ double uv_corner(const Params& p, const FU& U, const FV& V, int i, int jc) {
-    return u_corner(p, U, i, jc) * v_corner(p, V, i, jc);
+    return u_corner<IN>(p, U, i, jc) * v_corner<IN>(p, V, i, jc);
 }
 
 // ---- linear wall extrapolation (Operators.jl:738-842) ---------------------
@@ -119,9 +127,15 @@ __device__ __forceinline__ double lex_lo(F f) {  // value at the low wall (half 
 // T = phi G (sigma - I) (StructuralConstitutive.jl:100-113,
 // TimeIntegrationIMEX.jl:911-913), from cell-centred accessors Txx, Tyy, Txy.
 // ---------------------------------------------------------------------------
-template <class FT>
+template <bool IN = false, class FT>
 __device__ __forceinline__ double elastic_fx(const Params& p, const FT& Txx, const FT& Txy, int i, int j) {
     // x-part at x-face i (Operators.jl:642-683)
+    if (IN) {
+        double fx = (Txx(i, j) - Txx(i - 1, j)) * p.ihx;
+        const double tp = 0.5 * Txy(i - 1, j + 1) + 0.5 * Txy(i, j + 1);
+        const double tm = 0.5 * Txy(i - 1, j - 1) + 0.5 * Txy(i, j - 1);
+        return fx + (tp - tm) * (0.5 * p.ihy);
+    }
     const int nx = p.nx, ny = p.ny;
     double fx;
     if (p.px && i == 0) fx = (Txx(0, j) - Txx(nx - 1, j)) * p.ihx;
@@ -156,8 +170,14 @@ __device__ __forceinline__ double elastic_fx(const Params& p, const FT& Txx, con
     return fx;
 }
 
-template <class FT>
+template <bool IN = false, class FT>
 __device__ __forceinline__ double elastic_fy(const Params& p, const FT& Tyy, const FT& Txy, int i, int j) {
+    if (IN) {
+        double fy = (Tyy(i, j) - Tyy(i, j - 1)) * p.ihy;
+        const double tp = 0.5 * Txy(i + 1, j - 1) + 0.5 * Txy(i + 1, j);
+        const double tm = 0.5 * Txy(i - 1, j - 1) + 0.5 * Txy(i - 1, j);
+        return fy + (tp - tm) * (0.5 * p.ihx);
+    }
     const int nx = p.nx, ny = p.ny;
     double fy;
     if (p.py && j == 0) fy = (Tyy(i, 0) - Tyy(i, ny - 1)) * p.ihy;
@@ -198,14 +218,12 @@ __device__ __forceinline__ double elastic_fy(const Params& p, const FT& Tyy, con
 // halo-1 LDS tiles, each value formed once; the face formulas read them
 // from there.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* __restrict__ Eu,
-                                                    double* __restrict__ Ev, double* __restrict__ Iu,
-                                                    double* __restrict__ Iv) {
-    __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
-    __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
+template <bool IN>
+__device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, double* __restrict__ Eu,
+                                              double* __restrict__ Ev, double* __restrict__ Iu,
+                                              double* __restrict__ Iv, int i0, int j0, double* su, double* sv,
+                                              double* sxx, double* syy, double* sxy) {
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const TileId T = tile_id();
-    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
     const bool elastic = p.elastic != 0;
     constexpr int NPT = MT_H * MT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
     // indices of every tile point for U (faces 0..nx), V (rows 0..ny) and
@@ -260,8 +278,9 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* _
         }
     }
     __syncthreads();
-    const HaloTile U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
-    const HaloTile Txx{sxx, i0, j0, p.nx, p.ny}, Tyy{syy, i0, j0, p.nx, p.ny}, Txy{sxy, i0, j0, p.nx, p.ny};
+    using HT = HaloTile<IN>;
+    const HT U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
+    const HT Txx{sxx, i0, j0, p.nx, p.ny}, Tyy{syy, i0, j0, p.nx, p.ny}, Txy{sxy, i0, j0, p.nx, p.ny};
     const int i = i0 + threadIdx.x;
     if (i >= p.nx) return;
 #pragma unroll 1
@@ -270,35 +289,50 @@ __global__ void __launch_bounds__(BLOCK) k_momentum(Params p, Stage s, double* _
         if (j >= p.ny) return;
         // ---- x-momentum at x-face (i, j)
         double cu;
-        if (!p.px && i == 0) cu = 0.0;  // wall-normal row (MomentumTransport.jl:322-327)
+        if (!IN && !p.px && i == 0) cu = 0.0;  // wall-normal row (MomentumTransport.jl:322-327)
         else {
-            int im = (i == 0) ? p.nx - 1 : i - 1;
-            cu = (u2c(p, U, i, j) - u2c(p, U, im, j)) * p.ihx +
-                 (uv_corner(p, U, V, i, j + 1) - uv_corner(p, U, V, i, j)) * p.ihy;
+            int im = (!IN && i == 0) ? p.nx - 1 : i - 1;
+            cu = (u2c<IN>(p, U, i, j) - u2c<IN>(p, U, im, j)) * p.ihx +
+                 (uv_corner<IN>(p, U, V, i, j + 1) - uv_corner<IN>(p, U, V, i, j)) * p.ihy;
         }
         double eu = -cu;
         if (p.force == FSI_FORCE_CONSTANT) eu += p.ffx;
         else if (p.force == FSI_FORCE_KOLMOGOROV) eu += p.force_row[j];
-        if (elastic) eu += elastic_fx(p, Txx, Txy, i, j);
+        if (elastic) eu += elastic_fx<IN>(p, Txx, Txy, i, j);
         // ---- y-momentum at y-face (i, j)
         double cv;
-        if (!p.py && j == 0) cv = 0.0;
+        if (!IN && !p.py && j == 0) cv = 0.0;
         else {
-            int jm = (j == 0) ? p.ny - 1 : j - 1;
-            cv = (uv_corner(p, U, V, i + 1, j) - uv_corner(p, U, V, i, j)) * p.ihx +
-                 (v2c(p, V, i, j) - v2c(p, V, i, jm)) * p.ihy;
+            int jm = (!IN && j == 0) ? p.ny - 1 : j - 1;
+            cv = (uv_corner<IN>(p, U, V, i + 1, j) - uv_corner<IN>(p, U, V, i, j)) * p.ihx +
+                 (v2c<IN>(p, V, i, j) - v2c<IN>(p, V, i, jm)) * p.ihy;
         }
         double ev = -cv;
         if (p.force == FSI_FORCE_CONSTANT) ev += p.ffy;
-        if (elastic) ev += elastic_fy(p, Tyy, Txy, i, j);
+        if (elastic) ev += elastic_fy<IN>(p, Tyy, Txy, i, j);
         const size_t k = at(p, i, j);
         Eu[k] = eu;
         Ev[k] = ev;
         if (Iu) {
-            Iu[k] = p.visc ? visc_u(p, U, V, i, j, 1) : 0.0;
-            Iv[k] = p.visc ? visc_v(p, U, V, i, j, 1) : 0.0;
+            Iu[k] = p.visc ? visc_u<IN>(p, U, V, i, j, 1) : 0.0;
+            Iv[k] = p.visc ? visc_v<IN>(p, U, V, i, j, 1) : 0.0;
         }
     }
+}
+
+// (47.5 KB of LDS: at most 3 waves per SIMD, so registers up to that occupancy)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 3)))
+k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, double* __restrict__ Iu,
+           double* __restrict__ Iv) {
+    __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
+    __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    // every face of the tile at least one cell inside the walls: no wall
+    // row, no wrap, no boundary lifting
+    const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
+    if (inner) momentum_tile<true>(p, s, Eu, Ev, Iu, Iv, i0, j0, su, sv, sxx, syy, sxy);
+    else momentum_tile<false>(p, s, Eu, Ev, Iu, Iv, i0, j0, su, sv, sxx, syy, sxy);
 }
 
 // ---------------------------------------------------------------------------
@@ -1141,7 +1175,7 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
         sxy[k] = b;
     }
     __syncthreads();
-    const HaloTile U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
+    const HaloTile<> U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
     const SymTile S{sxx, syy, sxy, i0, j0, p.nx, p.ny};
     double dmax = 0.0;
     long long nonfinite = 0;
