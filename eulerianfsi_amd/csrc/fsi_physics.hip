@@ -536,16 +536,18 @@ __device__ __forceinline__ Sym3 wall_state3(const Params& p, int side, double u,
 // stencil loaded once), stretching, relaxation and geometric extension.
 // S(i, j): sigma after the safety reset; U, V: face velocities; phic: phi
 // at the cell.
-template <class FS, class FU, class FV>
+// IN = true: 1 <= i <= nx-2 and 1 <= j <= ny-2, so the wall states, wraps
+// and one-sided wall gradients compile out (same arithmetic).
+template <bool IN = false, class FS, class FU, class FV>
 __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU& U, const FV& V, double phic, int i,
                                           int j, Status* st, double& oxx, double& oyy, double& oxy) {
     const Sym3 C = S(i, j);
     Sym3 fl, fr, fb, ft;
     {  // x faces i (left) and i+1 (right); periodic face nx is face 0
         const double ul = U(i, j);
-        if (p.px || i > 0) fl = upwind_flux3(ul, S(i > 0 ? i - 1 : p.nx - 1, j), C);
+        if (IN || p.px || i > 0) fl = upwind_flux3(ul, S((IN || i > 0) ? i - 1 : p.nx - 1, j), C);
         else fl = upwind_flux3(ul, wall_state3(p, 0, ul, true, C, st), C);
-        if (i < p.nx - 1) {
+        if (IN || i < p.nx - 1) {
             fr = upwind_flux3(U(i + 1, j), C, S(i + 1, j));
         } else if (p.px) {
             fr = upwind_flux3(U(0, j), C, S(0, j));
@@ -556,9 +558,9 @@ __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU
     }
     {  // y faces j (bottom) and j+1 (top)
         const double vb = V(i, j);
-        if (p.py || j > 0) fb = upwind_flux3(vb, S(i, j > 0 ? j - 1 : p.ny - 1), C);
+        if (IN || p.py || j > 0) fb = upwind_flux3(vb, S(i, (IN || j > 0) ? j - 1 : p.ny - 1), C);
         else fb = upwind_flux3(vb, wall_state3(p, 2, vb, true, C, st), C);
-        if (j < p.ny - 1) {
+        if (IN || j < p.ny - 1) {
             ft = upwind_flux3(V(i, j + 1), C, S(i, j + 1));
         } else if (p.py) {
             ft = upwind_flux3(V(i, 0), C, S(i, 0));
@@ -576,7 +578,9 @@ __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU
     auto Uc = [&](int jj) { return 0.5 * U(i, jj) + 0.5 * U(i + 1, jj); };
     auto Vc = [&](int ii) { return 0.5 * V(ii, j) + 0.5 * V(ii, j + 1); };
     double uy, vx;
-    if (p.py) {
+    if (IN) {
+        uy = (Uc(j + 1) - Uc(j - 1)) * (0.5 * p.ihy);
+    } else if (p.py) {
         int jm = j == 0 ? p.ny - 1 : j - 1, jp = j == p.ny - 1 ? 0 : j + 1;
         uy = (Uc(jp) - Uc(jm)) * (0.5 * p.ihy);
     } else if (j == 0) {
@@ -586,7 +590,9 @@ __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU
     } else {
         uy = (Uc(j + 1) - Uc(j - 1)) * (0.5 * p.ihy);
     }
-    if (p.px) {
+    if (IN) {
+        vx = (Vc(i + 1) - Vc(i - 1)) * (0.5 * p.ihx);
+    } else if (p.px) {
         int im = i == 0 ? p.nx - 1 : i - 1, ip = i == p.nx - 1 ? 0 : i + 1;
         vx = (Vc(ip) - Vc(im)) * (0.5 * p.ihx);
     } else if (i == 0) {
@@ -1109,30 +1115,31 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
 // Structure-tensor right-hand side, plus on stage 1 the divergence
 // L-infinity and the finiteness check of the stage-1 state.  sigma (safety
 // reset applied), U and V sit on halo-1 LDS tiles, each value loaded once.
+template <bool IN = false>
 struct SymTile {
     const double *xx, *yy, *xy;
     int i0, j0, nx, ny;
     __device__ __forceinline__ Sym3 operator()(int gi, int gj) const {
         int li = gi - i0 + 1, lj = gj - j0 + 1;
-        if (li < 0) li += nx;
-        else if (li >= MT_W) li -= nx;
-        if (lj < 0) lj += ny;
-        else if (lj >= MT_H) lj -= ny;
+        if (!IN) {
+            if (li < 0) li += nx;
+            else if (li >= MT_W) li -= nx;
+            if (lj < 0) lj += ny;
+            else if (lj >= MT_H) lj -= ny;
+        }
         const int k = lj * MT_W + li;
         return {xx[k], yy[k], xy[k]};
     }
 };
 
-__global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
-                                                 double* __restrict__ Eyy, double* __restrict__ Exy, Status* st,
-                                                 int stage1) {
+template <bool IN>
+__device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, double* __restrict__ Exx,
+                                           double* __restrict__ Eyy, double* __restrict__ Exy, Status* st, int stage1,
+                                           int i0, int j0, double* su, double* sv, double* sxx, double* syy,
+                                           double* sxy) {
     __shared__ double lds[16];
     __shared__ long long lds_n[16];
-    __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
-    __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const TileId T = tile_id();
-    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
     constexpr int NPT = MT_H * MT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
     size_t ku[NIT], kv[NIT], kc[NIT];
 #pragma unroll
@@ -1175,8 +1182,8 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
         sxy[k] = b;
     }
     __syncthreads();
-    const HaloTile<> U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
-    const SymTile S{sxx, syy, sxy, i0, j0, p.nx, p.ny};
+    const HaloTile<IN> U{su, i0, j0, p.nx, p.ny}, V{sv, i0, j0, p.nx, p.ny};
+    const SymTile<IN> S{sxx, syy, sxy, i0, j0, p.nx, p.ny};
     double dmax = 0.0;
     long long nonfinite = 0;
     const int i = i0 + threadIdx.x;
@@ -1187,7 +1194,7 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
             if (j >= p.ny) break;
             const size_t k = at(p, i, j);
             double oxx, oyy, oxy;
-            sigma_rhs(p, S, U, V, s.phi[k], i, j, st, oxx, oyy, oxy);
+            sigma_rhs<IN>(p, S, U, V, s.phi[k], i, j, st, oxx, oyy, oxy);
             Exx[k] = oxx;
             Eyy[k] = oyy;
             Exy[k] = oxy;
@@ -1222,6 +1229,19 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
         p.exbuf[(size_t)EX_STAGE1 * p.exnb + bid] = mm;
         p.exbuf[(size_t)EX_NONFINITE_IN * p.exnb + bid] = (double)nn;
     }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
+                                                 double* __restrict__ Eyy, double* __restrict__ Exy, Status* st,
+                                                 int stage1) {
+    __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
+    __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    // every cell of the tile at least one cell inside the walls
+    const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
+    if (inner) sigma_tile<true>(p, s, Exx, Eyy, Exy, st, stage1, i0, j0, su, sv, sxx, syy, sxy);
+    else sigma_tile<false>(p, s, Exx, Eyy, Exy, st, stage1, i0, j0, su, sv, sxx, syy, sxy);
 }
 
 // ---------------------------------------------------------------------------
