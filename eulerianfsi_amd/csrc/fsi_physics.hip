@@ -340,8 +340,10 @@ k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, 
 // ---------------------------------------------------------------------------
 
 // scalar_gradient on x-face k (Operators.jl:189-356) of the cell function f.
-template <class F>
+// IN = true: 1 <= k <= nx-1, an interior face (no wall row, no wrap).
+template <bool IN = false, class F>
 __device__ __forceinline__ double grad_xface(const Params& p, const BcSet& bc, F f, int k, int j) {
+    if (IN) return p.ihx * f(k, j) - p.ihx * f(k - 1, j);
     if (p.px) {
         if (k == 0 || k == p.nx) return p.ihx * f(0, j) - p.ihx * f(p.nx - 1, j);
     } else if (k == 0) {
@@ -359,8 +361,9 @@ __device__ __forceinline__ double grad_xface(const Params& p, const BcSet& bc, F
     }
     return p.ihx * f(k, j) - p.ihx * f(k - 1, j);
 }
-template <class F>
+template <bool IN = false, class F>
 __device__ __forceinline__ double grad_yface(const Params& p, const BcSet& bc, F f, int i, int k) {
+    if (IN) return p.ihy * f(i, k) - p.ihy * f(i, k - 1);
     if (p.py) {
         if (k == 0 || k == p.ny) return p.ihy * f(i, 0) - p.ihy * f(i, p.ny - 1);
     } else if (k == 0) {
@@ -379,11 +382,11 @@ __device__ __forceinline__ double grad_yface(const Params& p, const BcSet& bc, F
     return p.ihy * f(i, k) - p.ihy * f(i, k - 1);
 }
 // centre gradient = face gradients averaged (PhaseFieldMethods.jl:219-231)
-template <class F>
+template <bool IN = false, class F>
 __device__ __forceinline__ void grad_center(const Params& p, const BcSet& bc, F f, int i, int j,
                                             double& gx, double& gy) {
-    gx = 0.5 * grad_xface(p, bc, f, i, j) + 0.5 * grad_xface(p, bc, f, i + 1, j);
-    gy = 0.5 * grad_yface(p, bc, f, i, j) + 0.5 * grad_yface(p, bc, f, i, j + 1);
+    gx = 0.5 * grad_xface<IN>(p, bc, f, i, j) + 0.5 * grad_xface<IN>(p, bc, f, i + 1, j);
+    gy = 0.5 * grad_yface<IN>(p, bc, f, i, j) + 0.5 * grad_yface<IN>(p, bc, f, i, j + 1);
 }
 // interpolate_center_to_x_faces (Operators.jl:110-134) of f at face k
 template <class F>
@@ -781,15 +784,18 @@ __global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, 
 constexpr int CH2 = 2, CW2 = TILE_X + 2 * CH2, CR2 = TILE_ROWS + 2 * CH2;  // 68 x 20
 constexpr int CW1 = TILE_X + 2, CR1 = TILE_ROWS + 2;                      // 66 x 18
 
-struct TileAcc {  // accessor of an LDS tile by global (wrapped) index
+template <bool IN = false>
+struct TileAcc {  // accessor of an LDS tile by global (wrapped) index; IN: no wrap
     const double* a;
     int i0, j0, H, W, R, nx, ny;
     __device__ __forceinline__ double operator()(int gi, int gj) const {
         int li = gi - i0 + H, lj = gj - j0 + H;
-        if (li < 0) li += nx;
-        else if (li >= W) li -= nx;
-        if (lj < 0) lj += ny;
-        else if (lj >= R) lj -= ny;
+        if (!IN) {
+            if (li < 0) li += nx;
+            else if (li >= W) li -= nx;
+            if (lj < 0) lj += ny;
+            else if (lj >= R) lj -= ny;
+        }
         return a[lj * W + li];
     }
 };
@@ -811,18 +817,13 @@ __device__ __forceinline__ double acdi_amp(double sv) {
 }
 constexpr int AF_XW = TILE_X + 1, AF_YH = TILE_ROWS + 1;  // 65 x-faces per row, 17 rows of y-faces
 constexpr int AF_NX = TILE_ROWS * AF_XW, AF_NY = AF_YH * TILE_X;
-__global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
-                                                      Status* st) {
-    __shared__ double s_phi[CR2 * CW2];
-    __shared__ double s_psi[CR2 * CW2];  // psi; after the normals, the x-face fluxes
-    __shared__ double s_sr[CR2 * CW2];   // sqrt(r)
-    __shared__ double s_n0[CR1 * CW1];
-    __shared__ double s_n1[CR1 * CW1];
-    __shared__ double s_fy[AF_NY];
+template <bool IN>
+__device__ __forceinline__ void acdi_tile(const Params& p, const Stage& s, const BcSet& bpsi,
+                                          double* __restrict__ Ephi, Status* st, int i0, int j0, double* s_phi,
+                                          double* s_psi, double* s_sr, double* s_n0, double* s_n1, double* s_fy) {
     static_assert(AF_NX <= CR2 * CW2, "x-face fluxes fit the psi tile");
-    double* s_fx = s_psi;
+    double* s_fx = s_psi;  // psi; after the normals, the x-face fluxes
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
     // 1. phi on the halo-2 tile and the tile's face velocities: indices, then
     // every load back to back
     constexpr int NPT = CR2 * CW2, NIT = (NPT + BLOCK - 1) / BLOCK;
@@ -834,11 +835,11 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         const int k = min(tid + it * BLOCK, NPT - 1);
         int gi = i0 + k % CW2 - CH2, gj = j0 + k / CW2 - CH2;
         bool in = true;
-        if (gi < 0 || gi >= p.nx) {
+        if (!IN && (gi < 0 || gi >= p.nx)) {
             if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
             else in = false;
         }
-        if (gj < 0 || gj >= p.ny) {
+        if (!IN && (gj < 0 || gj >= p.ny)) {
             if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
             else in = false;
         }
@@ -850,14 +851,14 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         const int q = min(tid + it * BLOCK, AF_NX - 1);
         const int k = i0 + q % AF_XW, j = j0 + q / AF_XW;
         const bool ok = k <= p.nx && j < p.ny;
-        iu[it] = ok ? at(p, (p.px && k == p.nx) ? 0 : k, j) : 0;
+        iu[it] = ok ? at(p, (!IN && p.px && k == p.nx) ? 0 : k, j) : 0;
     }
 #pragma unroll
     for (int it = 0; it < NVY; ++it) {  // y-face (i0 + fi, j0 + fj)
         const int q = min(tid + it * BLOCK, AF_NY - 1);
         const int i = i0 + q % TILE_X, k = j0 + q / TILE_X;
         const bool ok = i < p.nx && k <= p.ny;
-        iv[it] = ok ? at(p, i, (p.py && k == p.ny) ? 0 : k) : 0;
+        iv[it] = ok ? at(p, i, (!IN && p.py && k == p.ny) ? 0 : k) : 0;
     }
     double vals[NIT], uf[NUX], vf[NVY];
 #pragma unroll
@@ -896,25 +897,26 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         gamma = p.acdi_gamma * m;
     }
     __syncthreads();
-    const TileAcc f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
-    const TileAcc fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
-    const TileAcc fsr{s_sr, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    using TA = TileAcc<IN>;
+    const TA f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TA fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TA fsr{s_sr, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
     // 2. normals of psi (_center_normals, :233-244) on the halo-1 tile
     for (int k = tid; k < CR1 * CW1; k += BLOCK) {
         int gi = i0 + k % CW1 - 1, gj = j0 + k / CW1 - 1;
         bool in = true;
-        if (gi < 0 || gi >= p.nx) {
+        if (!IN && (gi < 0 || gi >= p.nx)) {
             if (p.px) gi = ((gi % p.nx) + p.nx) % p.nx;
             else in = false;
         }
-        if (gj < 0 || gj >= p.ny) {
+        if (!IN && (gj < 0 || gj >= p.ny)) {
             if (p.py) gj = ((gj % p.ny) + p.ny) % p.ny;
             else in = false;
         }
         double n0 = 0.0, n1 = 0.0;
         if (in) {
             double gx, gy;
-            grad_center(p, bpsi, fpsi, gi, gj, gx, gy);
+            grad_center<IN>(p, bpsi, fpsi, gi, gj, gx, gy);
             const double imag = 1.0 / sqrt(gx * gx + gy * gy + p.acdi_d2);
             n0 = gx * imag;
             n1 = gy * imag;
@@ -923,8 +925,8 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         s_n1[k] = n1;
     }
     __syncthreads();
-    const TileAcc N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
-    const TileAcc N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    const TA N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    const TA N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
     // 3. face fluxes -f + A, once per face (psi is no longer needed: its tile
     // takes the x-face fluxes)
     double fxv[NUX], fyv[NVY];
@@ -935,12 +937,12 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         double F = 0.0;
         if (q < AF_NX && k <= p.nx && j < p.ny) {
             const double u = uf[it];
-            if (!p.px && k == 0) {  // walls: boundary data in f, no regularisation flux (:637-648)
+            if (!IN && !p.px && k == 0) {  // walls: boundary data in f, no regularisation flux (:637-648)
                 F = -u * phase_bvalue(p.bPhi.l, f(0, j), st);
-            } else if (!p.px && k == p.nx) {
+            } else if (!IN && !p.px && k == p.nx) {
                 F = -u * phase_bvalue(p.bPhi.r, f(p.nx - 1, j), st);
             } else {
-                const int ka = k == 0 ? p.nx - 1 : k - 1, kb = k == p.nx ? 0 : k;
+                const int ka = (!IN && k == 0) ? p.nx - 1 : k - 1, kb = (!IN && k == p.nx) ? 0 : k;
                 const double pa = f(ka, j), pb = f(kb, j);
                 const double amp = acdi_amp(fsr(ka, j) * fsr(kb, j));
                 const double nf = 0.5 * N0(ka, j) + 0.5 * N0(kb, j);
@@ -957,12 +959,12 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         double F = 0.0;
         if (q < AF_NY && i < p.nx && k <= p.ny) {
             const double v = vf[it];
-            if (!p.py && k == 0) {
+            if (!IN && !p.py && k == 0) {
                 F = -v * phase_bvalue(p.bPhi.b, f(i, 0), st);
-            } else if (!p.py && k == p.ny) {
+            } else if (!IN && !p.py && k == p.ny) {
                 F = -v * phase_bvalue(p.bPhi.t, f(i, p.ny - 1), st);
             } else {
-                const int ka = k == 0 ? p.ny - 1 : k - 1, kb = k == p.ny ? 0 : k;
+                const int ka = (!IN && k == 0) ? p.ny - 1 : k - 1, kb = (!IN && k == p.ny) ? 0 : k;
                 const double pa = f(i, ka), pb = f(i, kb);
                 const double amp = acdi_amp(fsr(i, ka) * fsr(i, kb));
                 const double nf = 0.5 * N1(i, ka) + 0.5 * N1(i, kb);
@@ -994,6 +996,22 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
         const double* fy = s_fy + lj * TILE_X + threadIdx.x;
         Ephi[at(p, i, j)] = (fx[1] - fx[0]) * p.ihx + (fy[TILE_X] - fy[0]) * p.ihy;
     }
+}
+
+__global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
+                                                      Status* st) {
+    __shared__ double s_phi[CR2 * CW2];
+    __shared__ double s_psi[CR2 * CW2];
+    __shared__ double s_sr[CR2 * CW2];  // sqrt(r)
+    __shared__ double s_n0[CR1 * CW1];
+    __shared__ double s_n1[CR1 * CW1];
+    __shared__ double s_fy[AF_NY];
+    const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
+    // the halo-2 tile inside the domain: no wrap, every face and normal interior
+    const bool inner = i0 >= CH2 && i0 + TILE_X + CH2 - 1 <= p.nx - 1 && j0 >= CH2 &&
+                       j0 + TILE_ROWS + CH2 - 1 <= p.ny - 1;
+    if (inner) acdi_tile<true>(p, s, bpsi, Ephi, st, i0, j0, s_phi, s_psi, s_sr, s_n0, s_n1, s_fy);
+    else acdi_tile<false>(p, s, bpsi, Ephi, st, i0, j0, s_phi, s_psi, s_sr, s_n0, s_n1, s_fy);
 }
 
 // Phase-field right-hand side, one instantiation per phase method.
@@ -1042,8 +1060,8 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
         }
     }
     __syncthreads();
-    const TileAcc f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
-    const TileAcc fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TileAcc<> f{s_phi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
+    const TileAcc<> fpsi{s_psi, i0, j0, CH2, CW2, CR2, p.nx, p.ny};
     // 2. cell-centred normals / sharpening vector on the halo-1 tile
     if (kNormals) {
         for (int k = tid; k < CR1 * CW1; k += BLOCK) {
@@ -1085,8 +1103,8 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
         }
         __syncthreads();
     }
-    const TileAcc N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
-    const TileAcc N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    const TileAcc<> N0{s_n0, i0, j0, 1, CW1, CR1, p.nx, p.ny};
+    const TileAcc<> N1{s_n1, i0, j0, 1, CW1, CR1, p.nx, p.ny};
     double gamma = p.acdi_gamma;
     if (PH == FSI_PHASE_ACDI && p.acdi_dyn) {  // max |u_c| over the block maxima of k_max_speed
         __shared__ double smax[BLOCK / 64];
