@@ -606,19 +606,31 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs
 // (its mean, :255-259).  r is not stored.
 __global__ void __launch_bounds__(BLOCK) k_proj_direct(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
                                                        int has_null, double rtol, int maxiter) {
+    // the block's own cells loaded before the right-hand side totals are
+    // summed, so the two latencies overlap
+    const int i = tile_id().x * TILE_X + threadIdx.x;
+    double xr[ROWS], rh[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        const size_t k = (i < p.nx && j < p.ny) ? at(p, i, j) : 0;
+        xr[r] = b.x[k];
+        rh[r] = b.rhs[k];
+    }
     const double mean = proj_scalars(p, b, st, nb, has_null, rtol, maxiter);
     const double area = p.hx * p.hy;
     GAcc X{b.x, p.P};
     double acc[3] = {0.0, 0.0, 0.0};
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
+        if (i >= p.nx || j >= p.ny) continue;
         const double nl = neg_lap(g, X, i, j, nullptr);
-        const double rhs = b.rhs[k];
-        const double r = -(rhs - mean) - nl;
-        const double res = -nl - rhs;
-        acc[0] += r * r;
+        const double rr = -(rh[r] - mean) - nl;
+        const double res = -nl - rh[r];
+        acc[0] += rr * rr;
         acc[1] += res * res;
-        acc[2] += X(i, j) * area;
+        acc[2] += xr[r] * area;
     }
     __shared__ double lds[3 * 16];
     block_sum<3>(acc, lds);
@@ -857,35 +869,43 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
 // the checks of project! (:279-298), one block
 __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, int nb, double lxly, int pcg_m,
                                                       int slot) {
-    if (pcg_m >= 0) {  // the convergence test of PCG iteration pcg_m (k_pcg_check)
-        double rr[1];
-        get_totals<1>(pset(b.partials, nb, PS_RR), nb, rr);
-        if (lane_id() == 0 && !st->pcg.done && (rr[0] <= st->pcg.tol2 || pcg_m >= st->pcg.maxiter)) {
-            st->pcg.done = 1;
-            st->pcg.iter = pcg_m;
-            st->pcg.rr = rr[0];
-            if (slot >= 0) st->proj_iters[slot] = pcg_m;
-            if (slot >= 0 && rr[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
-        }
-    }
-    double tot[2];
-    get_totals<2>(pset(b.partials, nb, PS_CHECK), nb, tot);
-    __shared__ double lmax[BLOCK / 64];
+    // one pass over the partials: the r.r of PCG iteration pcg_m (its
+    // convergence test, as k_pcg_check makes it), the true-residual and
+    // constraint sums (each in the order of get_totals) and the constraint
+    // maximum
+    const double* rrp = pset(b.partials, nb, PS_RR);
+    const double* chk = pset(b.partials, nb, PS_CHECK);
+    const int lane = lane_id();
+    double acc[3] = {0.0, 0.0, 0.0};
     double linf = 0.0;
-    const double* bmax = pset(b.partials, nb, PS_CHECK) + 2 * (size_t)nb;
-    for (int k = threadIdx.x; k < nb; k += blockDim.x) linf = fmax(linf, bmax[k]);
+    for (int k = lane; k < nb; k += BLOCK) {
+        if (pcg_m >= 0) acc[0] += rrp[k];
+        acc[1] += chk[k];
+        acc[2] += chk[(size_t)nb + k];
+        linf = fmax(linf, chk[2 * (size_t)nb + k]);
+    }
+    __shared__ double lds[3 * 16];
+    __shared__ double lmax[BLOCK / 64];
+    block_sum<3>(acc, lds);
     linf = wave_max(linf);
-    if ((threadIdx.x & 63) == 0) lmax[threadIdx.x >> 6] = linf;
+    if ((lane & 63) == 0) lmax[lane >> 6] = linf;
     __syncthreads();
-    if (lane_id() != 0) return;
-    for (int w = 1; w < (int)blockDim.x / 64; ++w) linf = fmax(linf, lmax[w]);
+    if (lane != 0) return;
+    if (pcg_m >= 0 && !st->pcg.done && (acc[0] <= st->pcg.tol2 || pcg_m >= st->pcg.maxiter)) {
+        st->pcg.done = 1;
+        st->pcg.iter = pcg_m;
+        st->pcg.rr = acc[0];
+        if (slot >= 0) st->proj_iters[slot] = pcg_m;
+        if (slot >= 0 && acc[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
+    }
+    for (int w = 1; w < BLOCK / 64; ++w) linf = fmax(linf, lmax[w]);
     linf = fmax(linf, lmax[0]);
     ProjScalars& q = st->proj;
     q.linf = linf;
-    q.resid2 = tot[0];
-    q.l2sq = tot[1];
-    q.lin_rel = sqrt(tot[0]) / fmax(sqrt(q.rhs2), 2.220446049250313e-16);
-    q.l2 = sqrt(tot[1] / lxly);
+    q.resid2 = acc[1];
+    q.l2sq = acc[2];
+    q.lin_rel = sqrt(acc[1]) / fmax(sqrt(q.rhs2), 2.220446049250313e-16);
+    q.l2 = sqrt(acc[2] / lxly);
     st->max_constraint_linf = fmax(st->max_constraint_linf, q.linf);
     if (!(q.lin_rel <= 1e-8)) raise_error(st, DEV_PROJ_RESIDUAL, q.lin_rel, 1e-8);
     double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), 0.0), 1.0);
