@@ -96,7 +96,7 @@ class ProjectionResultC(ctypes.Structure):
 
 class StepStatsC(ctypes.Structure):
     _fields_ = [("velocity_iterations", c_int32 * 3), ("projection_iterations", c_int32 * 3),
-                ("planned", c_int32)]
+                ("planned", c_int32), ("max_abs_u", c_double), ("max_abs_v", c_double)]
 
 
 # ---- configuration conversion ------------------------------------------------

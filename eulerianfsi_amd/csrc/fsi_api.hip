@@ -1388,6 +1388,8 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
             stats->projection_iterations[k] = st.proj_iters[k];
         }
         stats->planned = replayed ? 1 : 0;
+        stats->max_abs_u = st.max_abs_u;
+        stats->max_abs_v = st.max_abs_v;
     }
     return FSI_OK;
 }

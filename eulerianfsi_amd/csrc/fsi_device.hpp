@@ -118,6 +118,7 @@ struct Status {
     long long nonfinite_in;  // non-finite entries of the input state (stage 1)
     double stage1_linf;      // max |div(v) - G| of the input velocity (stage 1)
     double gamma_umax;       // dynamic ACDI gamma: max |u_c|
+    double max_abs_u, max_abs_v;  // face-velocity extrema after the last projection
     int vel_iters[3], proj_iters[3];
     int vel_ok[3], proj_ok[3];  // 1 once the solve of that stage has converged
     KrylovScalars vel;

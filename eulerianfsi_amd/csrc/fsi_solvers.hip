@@ -39,7 +39,7 @@ static __device__ __forceinline__ int lane_id() { return threadIdx.x + threadIdx
 // Partial-sum sets (see put_partials / get_totals in fsi_device.hpp): each
 // set holds up to 4 quantities x nb blocks.
 enum PartialSet : int { PS_VEL0 = 0, PS_VEL1 = 1, PS_VELI = 2, PS_RHS = 3, PS_RR = 4, PS_RZ = 5, PS_PW = 6,
-                        PS_MEAN = 7, PS_CHECK = 8, PS_COUNT = 9 };
+                        PS_MEAN = 7, PS_CHECK = 8, PS_VMAX = 9, PS_COUNT = 10 };
 __device__ __forceinline__ double* pset(double* base, int nb, int set) { return base + (size_t)set * 4 * nb; }
 
 // ===========================================================================
@@ -829,6 +829,9 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
     auto Q = [&](int a, int c) { return b.x[at(p, a, c)] - mean; };
     double acc[2] = {0.0, 0.0};
     double linf = 0.0;
+    // max |U| over every x-face, max |V| over every y-face and their NaN count
+    // (the velocity extrema of the driver's timestep bound, for free here)
+    double umax = 0.0, vmax = 0.0, nan_uv = 0.0;
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
         double q = Q(i, j);
@@ -838,11 +841,23 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
             double res = aq - b.rhs[k];
             acc[0] += res * res;
         }
-        double div = (b.U[at(p, i + 1, j)] - b.U[k]) * p.ihx + (b.V[at(p, i, j + 1)] - b.V[k]) * p.ihy;
+        const double uL = b.U[k], uR = b.U[at(p, i + 1, j)], vB = b.V[k], vT = b.V[at(p, i, j + 1)];
+        double div = (uR - uL) * p.ihx + (vT - vB) * p.ihy;
         double c = div - 0.0;
         linf = fmax(linf, fabs(c));
         if (!isfinite(c)) linf = INFINITY;
         acc[1] += c * c * area;
+        umax = fmax(umax, fabs(uL));
+        vmax = fmax(vmax, fabs(vB));
+        nan_uv += (isnan(uL) ? 1.0 : 0.0) + (isnan(vB) ? 1.0 : 0.0);
+        if (i == p.nx - 1) {
+            umax = fmax(umax, fabs(uR));
+            nan_uv += isnan(uR) ? 1.0 : 0.0;
+        }
+        if (j == p.ny - 1) {
+            vmax = fmax(vmax, fabs(vT));
+            nan_uv += isnan(vT) ? 1.0 : 0.0;
+        }
     }
     {  // slot 0 (true residual) comes from k_proj_direct when resid == 0
         __shared__ double lds[2 * 16];
@@ -853,16 +868,34 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
             dst[(size_t)nb + block_id()] = acc[1];
         }
     }
-    // the block maximum as a third partial of the set (maximum taken by k_proj_final)
-    __shared__ double lmax[BLOCK / 64];
+    // block maxima: the constraint as a third partial of PS_CHECK, the face
+    // velocities as PS_VMAX (maxima taken by k_proj_final); the NaN count sums
+    __shared__ double lmax[3][BLOCK / 64];
+    __shared__ double lnan[16];
+    double nn[1] = {nan_uv};
+    block_sum<1>(nn, lnan);
     linf = wave_max(linf);
+    umax = wave_max(umax);
+    vmax = wave_max(vmax);
     const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((lane & 63) == 0) lmax[lane >> 6] = linf;
+    if ((lane & 63) == 0) {
+        lmax[0][lane >> 6] = linf;
+        lmax[1][lane >> 6] = umax;
+        lmax[2][lane >> 6] = vmax;
+    }
     __syncthreads();
     if (lane == 0) {
-        double mm = lmax[0];
-        for (int w = 1; w < BLOCK / 64; ++w) mm = fmax(mm, lmax[w]);
-        pset(b.partials, nb, PS_CHECK)[2 * (size_t)nb + blockIdx.x + blockIdx.y * gridDim.x] = mm;
+        double m[3];
+        for (int q = 0; q < 3; ++q) {
+            m[q] = lmax[q][0];
+            for (int w = 1; w < BLOCK / 64; ++w) m[q] = fmax(m[q], lmax[q][w]);
+        }
+        const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+        pset(b.partials, nb, PS_CHECK)[2 * (size_t)nb + bid] = m[0];
+        double* vm = pset(b.partials, nb, PS_VMAX);
+        vm[bid] = m[1];
+        vm[(size_t)nb + bid] = m[2];
+        vm[2 * (size_t)nb + bid] = nn[0];
     }
 }
 
@@ -875,22 +908,38 @@ __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, in
     // maximum
     const double* rrp = pset(b.partials, nb, PS_RR);
     const double* chk = pset(b.partials, nb, PS_CHECK);
+    const double* vm = pset(b.partials, nb, PS_VMAX);
     const int lane = lane_id();
-    double acc[3] = {0.0, 0.0, 0.0};
-    double linf = 0.0;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    double linf = 0.0, umax = 0.0, vmax = 0.0;
     for (int k = lane; k < nb; k += BLOCK) {
         if (pcg_m >= 0) acc[0] += rrp[k];
         acc[1] += chk[k];
         acc[2] += chk[(size_t)nb + k];
+        acc[3] += vm[2 * (size_t)nb + k];
         linf = fmax(linf, chk[2 * (size_t)nb + k]);
+        umax = fmax(umax, vm[k]);
+        vmax = fmax(vmax, vm[(size_t)nb + k]);
     }
-    __shared__ double lds[3 * 16];
-    __shared__ double lmax[BLOCK / 64];
-    block_sum<3>(acc, lds);
+    __shared__ double lds[4 * 16];
+    __shared__ double lmax[3][BLOCK / 64];
+    block_sum<4>(acc, lds);
     linf = wave_max(linf);
-    if ((lane & 63) == 0) lmax[lane >> 6] = linf;
+    umax = wave_max(umax);
+    vmax = wave_max(vmax);
+    if ((lane & 63) == 0) {
+        lmax[0][lane >> 6] = linf;
+        lmax[1][lane >> 6] = umax;
+        lmax[2][lane >> 6] = vmax;
+    }
     __syncthreads();
     if (lane != 0) return;
+    for (int w = 0; w < BLOCK / 64; ++w) {
+        umax = fmax(umax, lmax[1][w]);
+        vmax = fmax(vmax, lmax[2][w]);
+    }
+    st->max_abs_u = acc[3] > 0.0 ? NAN : umax;  // Julia's maximum propagates NaN
+    st->max_abs_v = acc[3] > 0.0 ? NAN : vmax;
     if (pcg_m >= 0 && !st->pcg.done && (acc[0] <= st->pcg.tol2 || pcg_m >= st->pcg.maxiter)) {
         st->pcg.done = 1;
         st->pcg.iter = pcg_m;
@@ -898,8 +947,8 @@ __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, in
         if (slot >= 0) st->proj_iters[slot] = pcg_m;
         if (slot >= 0 && acc[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
     }
-    for (int w = 1; w < BLOCK / 64; ++w) linf = fmax(linf, lmax[w]);
-    linf = fmax(linf, lmax[0]);
+    for (int w = 1; w < BLOCK / 64; ++w) linf = fmax(linf, lmax[0][w]);
+    linf = fmax(linf, lmax[0][0]);
     ProjScalars& q = st->proj;
     q.linf = linf;
     q.resid2 = acc[1];

@@ -211,10 +211,14 @@ def timestep_bounds_from_speed(cfg: BenchmarkConfig, hx: float, hy: float, max_a
                           elastic_wave_speed, gamma_acdi)
 
 
-def comparison_timestep_bounds(cfg: BenchmarkConfig, state: FSIState, ctx: IMEXContext, setup) -> TimestepBounds:
+def comparison_timestep_bounds(cfg: BenchmarkConfig, state: FSIState, ctx: IMEXContext, setup,
+                               extrema: tuple[float, float] | None = None) -> TimestepBounds:
     """comparison_timestep_bounds(cfg, state, metrics, setup) (:852-934);
-    `setup` carries cdi_epsilon and acdi_epsilon as the driver's setup does."""
-    mu, mv = velocity_extrema(state, ctx)
+    `setup` carries cdi_epsilon and acdi_epsilon as the driver's setup does.
+    `extrema` = (max |U|, max |V|) of the state when the caller has them (the
+    step statistics of the step that produced the state carry them); else
+    they are reduced on the device."""
+    mu, mv = velocity_extrema(state, ctx) if extrema is None else extrema
     m = ctx.metrics
     return timestep_bounds_from_speed(cfg, float(min(m.dx_c)), float(min(m.dy_c)), mu, mv, setup.cdi_epsilon,
                                       setup.acdi_epsilon)

@@ -2,10 +2,11 @@
 
 Follows scripts/benchmarks/run_sugiyama_lid_driven_cavity.jl:1540-1935: the
 timestep of every step is the smallest of the comparison bounds
-(comparison_timestep_bounds, from the device velocity extrema) and of the
-time to the next output event; trajectory rows are written every
-trajectory_interval and the full diagnostics every diagnostics_interval, to
-the seven CSV files of the reference with its formats.  Snapshot figures
+(comparison_timestep_bounds, from the velocity extrema that each step's
+statistics carry) and of the time to the next output event; trajectory rows
+are written every trajectory_interval and the full diagnostics every
+diagnostics_interval, to the seven CSV files of the reference with its
+formats.  Snapshot figures
 (the plotting part of the driver) are not produced.  Nothing but a few
 scalars per step crosses to the host.
 """
@@ -104,8 +105,9 @@ def run_cavity(case, cfg: D.BenchmarkConfig, setup: PhaseFieldSetup, output_dir:
         last_report = None
         dt = 0.0
         timestep = initial_bounds
+        extrema = None  # velocity extrema of the current state, from the step that made it
         while t < cfg.T - EVENT_TOL:  # :1723-1830
-            timestep = D.comparison_timestep_bounds(cfg, state, ctx, setup)
+            timestep = D.comparison_timestep_bounds(cfg, state, ctx, setup, extrema)
             dt_physical = timestep.selected
             if not dt_physical >= cfg.dt_min_physical:
                 raise RuntimeError(f"Physical timestep bound {dt_physical:.6e} is below dt_min_physical "
@@ -116,6 +118,7 @@ def run_cavity(case, cfg: D.BenchmarkConfig, setup: PhaseFieldSetup, output_dir:
                 t = next_event
             else:
                 last_report = integrate_step(state, t, dt, ctx, work)
+                extrema = (work.last_stats.max_abs_u, work.last_stats.max_abs_v)
                 t += dt
                 step += 1
                 if abs(t - next_event) <= EVENT_TOL:
@@ -142,7 +145,8 @@ def run_cavity(case, cfg: D.BenchmarkConfig, setup: PhaseFieldSetup, output_dir:
         if abs(last_diag - t) > EVENT_TOL:
             constraint = (init_projection.constraint_linf if last_report is None
                           else last_report.maximum_projection_constraint_linf)
-            diagnostics_event(step, t, 0.0, D.comparison_timestep_bounds(cfg, state, ctx, setup), constraint, mass0)
+            diagnostics_event(step, t, 0.0, D.comparison_timestep_bounds(cfg, state, ctx, setup, extrema), constraint,
+                              mass0)
     finally:
         for f in ios.values():
             f.close()

@@ -259,6 +259,9 @@ class StepStats:
     velocity_iterations: tuple
     projection_iterations: tuple
     planned: bool = False
+    # max |U|, max |V| over the faces of the new state (NaN if a face is NaN)
+    max_abs_u: float = float("nan")
+    max_abs_v: float = float("nan")
 
 
 # ---------------------------------------------------------------------------
@@ -380,6 +383,11 @@ def _project(state, ctx, work, fn, what):
                             res.constraint_l2, res.compatibility_defect, res.iterations)
 
 
+def _step_stats(s) -> StepStats:
+    return StepStats(tuple(s.velocity_iterations), tuple(s.projection_iterations), bool(s.planned),
+                     float(s.max_abs_u), float(s.max_abs_v))
+
+
 def integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work: IMEXWorkspace) -> StepReport:
     """integrate_step! (:1152-1256): advance `state` from t to t+dt in place.
     On error the state is unchanged and the mapped exception is raised."""
@@ -392,8 +400,7 @@ def integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work:
                                     ctypes.byref(stats))
     _abi.check(ctx.lib, ctx.handle, st, "integrate_step!")
     state._device_modified()
-    work.last_stats = StepStats(tuple(stats.velocity_iterations), tuple(stats.projection_iterations),
-                                bool(stats.planned))
+    work.last_stats = _step_stats(stats)
     return StepReport(rep.maximum_projection_constraint_linf, rep.maximum_growth_trace_error, rep.minimum_phi,
                       rep.maximum_phi, rep.minimum_resource, rep.minimum_sigma_eigenvalue,
                       rep.maximum_abs_sigma_determinant_error, int(rep.non_spd_cells))
@@ -432,8 +439,7 @@ def ensemble_step(states, ts, dts, ctxs, works) -> list:
         if status[k] == 0:
             states[k]._device_modified()
             st = stats[k]
-            works[k].last_stats = StepStats(tuple(st.velocity_iterations), tuple(st.projection_iterations),
-                                            bool(st.planned))
+            works[k].last_stats = _step_stats(st)
             r = reps[k]
             out.append(StepReport(r.maximum_projection_constraint_linf, r.maximum_growth_trace_error,
                                   r.minimum_phi, r.maximum_phi, r.minimum_resource, r.minimum_sigma_eigenvalue,

@@ -161,6 +161,12 @@ typedef struct {
     int32_t velocity_iterations[3];   /* stages 2..4 */
     int32_t projection_iterations[3]; /* stages 2..4 */
     int32_t planned;                  /* 1: the step ran with planned iteration counts and one read-back */
+    /* max |U| over the x-faces and max |V| over the y-faces of the new
+     * state (NaN if any face is NaN), as the driver's timestep bound reads
+     * them (scripts/benchmarks/run_sugiyama_lid_driven_cavity.jl:862-866);
+     * formed by the step's last projection check, so they cost no launch */
+    double max_abs_u;
+    double max_abs_v;
 } fsi_step_stats;
 
 typedef struct fsi_context fsi_context;

@@ -1,5 +1,6 @@
 """The driver diagnostics on the device (fsi_velocity_extrema,
-fsi_phase_centroid, fsi_diagnostics_compute) against the NumPy restatement of
+fsi_phase_centroid, fsi_diagnostics_compute, the velocity extrema of the step
+statistics) against the NumPy restatement of
 the reference driver (oracle/diagnostics.py) on stepped states and on a
 synthetic state with out-of-range phase, non-SPD tensors and a NaN.
 Tolerance: 1e-12 relative for sums (summation order differs), 1e-14 for
@@ -39,6 +40,11 @@ def _stepped(name):
     fsi.make_velocity_compatible(case.state, ctx, work)
     for k in range(3):
         fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+        # the step statistics carry the new state's velocity extrema, exactly
+        # (the first step polls its solves, the later ones run planned)
+        U, V = case.state.U, case.state.V
+        st = work.last_stats
+        assert (st.max_abs_u, st.max_abs_v) == (float(np.max(np.abs(U))), float(np.max(np.abs(V)))), k
     return case, ctx, bc
 
 
