@@ -270,7 +270,9 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
     const int n = ay.n, m = ay.m, tid = threadIdx.x;
     double* re = sp_smem;  // C transforms of n
     double* im = re + C * n;
-    const int x0 = blockIdx.x * 2 * C;
+    // XCD-aware: the 2C-column strips of one XCD are contiguous, so the
+    // 128-byte lines that neighbouring strips share are fetched by one L2
+    const int x0 = tile_id().x * 2 * C;
     constexpr int W = 2 * C;  // columns per row of the block
     const int total = n * W;
     double v[2 * IT];
