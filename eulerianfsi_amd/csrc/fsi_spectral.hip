@@ -39,6 +39,14 @@ namespace fsi {
 
 __device__ __forceinline__ int brev(int j, int m) { return (int)(__brev((unsigned)j) >> (32 - m)); }
 
+// LDS slot of index j with one pad double per 32: the power-of-two strides
+// of the bit-reversed scatter, of the first butterfly spans and of the
+// twiddle reads then fall on distinct banks
+__host__ __device__ constexpr int pd(int j) { return j + (j >> 5); }
+// LDS doubles that indices [0, v) occupy (odd for v >= 32, which also
+// offsets the banks of an array placed right after)
+__host__ __device__ constexpr int pdn(int v) { return v + (v >> 5) + 1; }
+
 // position of x_i in Makhoul's reordered sequence v
 __device__ __forceinline__ int mk_pos(int i, int n) { return (i & 1) ? n - 1 - (i >> 1) : (i >> 1); }
 
@@ -53,14 +61,14 @@ __device__ __forceinline__ SpecLds load_tables(const SpecAxis& a, double* base, 
     const int n = a.n;
     SpecLds t;
     t.wr = base;
-    t.wi = t.wr + (n >> 1);
-    t.pr = t.wi + (n >> 1);
+    t.wi = t.wr + pdn(n >> 1);
+    t.pr = t.wi + pdn(n >> 1);
     t.pi = t.pr + (KIND == SPEC_RDFT ? 0 : n);
     t.mu = t.pi + (KIND == SPEC_RDFT ? 0 : n);
     for (int k = tid; k < (n >> 1); k += NT) {
         const double2 w = a.wfft[k];
-        t.wr[k] = w.x;
-        t.wi[k] = w.y;
+        t.wr[pd(k)] = w.x;
+        t.wi[pd(k)] = w.y;
     }
     if (KIND != SPEC_RDFT)
         for (int k = tid; k < n; k += NT) {
@@ -74,24 +82,26 @@ __device__ __forceinline__ SpecLds load_tables(const SpecAxis& a, double* base, 
 }
 // LDS values of those tables
 __host__ __device__ constexpr int spec_table_values(int n, int kind, bool with_mu) {
-    return n + (kind == SPEC_RDFT ? 0 : 2 * n) + (with_mu ? n : 0);
+    return 2 * pdn(n >> 1) + (kind == SPEC_RDFT ? 0 : 2 * n) + (with_mu ? n : 0);
 }
 
-// F complex FFTs of length n = 2^m in place on re/im (F * n each, input in
-// bit-reversed order, output in natural order); twiddles wr + i wi =
-// e^{-2 pi i t/n}, t < n/2.  Ends with a barrier.
+// F complex FFTs of length n = 2^m in place on re/im (transform f at
+// f * S, S >= pdn(n), input in bit-reversed order, output in natural order);
+// twiddles wr + i wi = e^{-2 pi i t/n}, t < n/2.  Every line is addressed
+// through pd().  Ends with a barrier.
 template <int NT>
-__device__ void lds_fft(double* re, double* im, const double* wr, const double* wi, int m, int F, int tid) {
+__device__ void lds_fft(double* re, double* im, const double* wr, const double* wi, int m, int F, int S, int tid) {
     const int n = 1 << m;
     int s = 0;
     if (m & 1) {  // radix 2, span 1: butterflies (2q, 2q+1), twiddle 1
         for (int q = tid; q < F * (n >> 1); q += NT) {
-            const int j = 2 * q;
-            const double ar = re[j], ai = im[j], br = re[j + 1], bi = im[j + 1];
-            re[j] = ar + br;
-            im[j] = ai + bi;
-            re[j + 1] = ar - br;
-            im[j + 1] = ai - bi;
+            const int l = 2 * (q & ((n >> 1) - 1)), b = (q >> (m - 1)) * S;
+            const int j0 = b + pd(l), j1 = b + pd(l + 1);
+            const double ar = re[j0], ai = im[j0], br = re[j1], bi = im[j1];
+            re[j0] = ar + br;
+            im[j0] = ai + bi;
+            re[j1] = ar - br;
+            im[j1] = ai - bi;
         }
         s = 1;
         __syncthreads();
@@ -104,11 +114,12 @@ __device__ void lds_fft(double* re, double* im, const double* wr, const double* 
         for (int q = tid; q < F * (n >> 2); q += NT) {
             const int f = q >> (m - 2), r = q & ((n >> 2) - 1);
             const int t = r & (h - 1);
-            const int j = f * n + ((r >> s) << (s + 2)) + t;
-            const int i1 = t << (m - s - 1), i2 = t << (m - s - 2);
+            const int l = ((r >> s) << (s + 2)) + t, b = f * S;
+            const int i1 = pd(t << (m - s - 1)), i2 = pd(t << (m - s - 2));
+            const int ja = b + pd(l), jb = b + pd(l + h), jc = b + pd(l + 2 * h), jd = b + pd(l + 3 * h);
             const double w1r = wr[i1], w1i = wi[i1], w2r = wr[i2], w2i = wi[i2];
-            const double Ar = re[j], Ai = im[j], Br = re[j + h], Bi = im[j + h];
-            const double Cr = re[j + 2 * h], Ci = im[j + 2 * h], Dr = re[j + 3 * h], Di = im[j + 3 * h];
+            const double Ar = re[ja], Ai = im[ja], Br = re[jb], Bi = im[jb];
+            const double Cr = re[jc], Ci = im[jc], Dr = re[jd], Di = im[jd];
             const double bR = w1r * Br - w1i * Bi, bI = w1r * Bi + w1i * Br;
             const double dR = w1r * Dr - w1i * Di, dI = w1r * Di + w1i * Dr;
             const double A1r = Ar + bR, A1i = Ai + bI, B1r = Ar - bR, B1i = Ai - bI;
@@ -116,25 +127,26 @@ __device__ void lds_fft(double* re, double* im, const double* wr, const double* 
             const double cR = w2r * C1r - w2i * C1i, cI = w2r * C1i + w2i * C1r;
             const double gR = w2r * D1r - w2i * D1i, gI = w2r * D1i + w2i * D1r;  // w2 D1
             const double eR = gI, eI = -gR;                                         // -i w2 D1
-            re[j] = A1r + cR;
-            im[j] = A1i + cI;
-            re[j + 2 * h] = A1r - cR;
-            im[j + 2 * h] = A1i - cI;
-            re[j + h] = B1r + eR;
-            im[j + h] = B1i + eI;
-            re[j + 3 * h] = B1r - eR;
-            im[j + 3 * h] = B1i - eI;
+            re[ja] = A1r + cR;
+            im[ja] = A1i + cI;
+            re[jc] = A1r - cR;
+            im[jc] = A1i - cI;
+            re[jb] = B1r + eR;
+            im[jb] = B1i + eI;
+            re[jd] = B1r - eR;
+            im[jd] = B1i - eI;
         }
         __syncthreads();
     }
 }
 
 // the two real coefficient lines of output index k from the packed spectrum
+// (re, im: one transform, addressed through pd())
 template <int KIND>
 __device__ __forceinline__ void split_coeffs(int n, const SpecLds& t, const double* re, const double* im, int k,
                                              double& o1, double& o2) {
     const int kr = (n - k) & (n - 1);
-    const double zr = re[k], zi = im[k], cr = re[kr], ci = im[kr];
+    const double zr = re[pd(k)], zi = im[pd(k)], cr = re[pd(kr)], ci = im[pd(kr)];
     const double x1r = 0.5 * (zr + cr), x1i = 0.5 * (zi - ci);
     const double x2r = 0.5 * (zi + ci), x2i = -0.5 * (zr - cr);
     if (KIND == SPEC_RDFT) {
@@ -149,15 +161,16 @@ __device__ __forceinline__ void split_coeffs(int n, const SpecLds& t, const doub
 }
 
 // the spectrum V_k (or X_k) of one real line from its coefficient line c
+// (addressed through pd())
 template <int KIND>
 __device__ __forceinline__ void line_spectrum(int n, const SpecLds& t, const double* c, int k, double& vr,
                                               double& vi) {
     const int h = n >> 1;
     if (KIND == SPEC_RDFT) {
-        vr = k <= h ? c[k] : c[n - k];
-        vi = (k == 0 || k == h) ? 0.0 : (k < h ? c[n - k] : -c[k]);
+        vr = k <= h ? c[pd(k)] : c[pd(n - k)];
+        vi = (k == 0 || k == h) ? 0.0 : (k < h ? c[pd(n - k)] : -c[pd(k)]);
     } else {
-        const double ck = c[k], cr = k ? c[n - k] : 0.0;
+        const double ck = c[pd(k)], cr = k ? c[pd(n - k)] : 0.0;
         const double tr = t.pr[k], ti = t.pi[k];  // V = conj(t) (ck - i cr)
         vr = tr * ck - ti * cr;
         vi = -tr * cr - ti * ck;
@@ -172,7 +185,7 @@ __global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, in
     extern __shared__ double sp_smem[];
     const int n = ax.n, m = ax.m, tid = threadIdx.x;
     double* re = sp_smem;
-    double* im = re + n;
+    double* im = re + pdn(n);
     const int j0 = 2 * blockIdx.x;
     const bool two = j0 + 1 < nrows;
     const double* r0 = in + (size_t)j0 * P;
@@ -183,19 +196,19 @@ __global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, in
         a[it] = i < n ? r0[i] : 0.0;
         b[it] = (i < n && two) ? r0[i + P] : 0.0;
     }
-    const SpecLds t = load_tables<NT, KIND>(ax, im + n, false, tid);
+    const SpecLds t = load_tables<NT, KIND>(ax, im + pdn(n), false, tid);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int i = tid + it * NT;
         if (i >= n) break;
         const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
         const double s = (KIND == SPEC_ALT && (i & 1)) ? -scale : scale;
-        const int p = brev(jv, m);
+        const int p = pd(brev(jv, m));
         re[p] = s * a[it];
         im[p] = s * b[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, pdn(n), tid);
     double* o0 = out + (size_t)j0 * P;
     for (int k = tid; k < n; k += NT) {
         double o1, o2;
@@ -212,7 +225,7 @@ __global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, in
     extern __shared__ double sp_smem[];
     const int n = ax.n, m = ax.m, tid = threadIdx.x;
     double* re = sp_smem;  // the two coefficient lines, then the packed spectrum
-    double* im = re + n;
+    double* im = re + pdn(n);
     const int j0 = 2 * blockIdx.x;
     const bool two = j0 + 1 < nrows;
     double* r0 = x + (size_t)j0 * P;
@@ -223,13 +236,13 @@ __global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, in
         a[it] = i < n ? r0[i] : 0.0;
         b[it] = (i < n && two) ? r0[i + P] : 0.0;
     }
-    const SpecLds t = load_tables<NT, KIND>(ax, im + n, false, tid);
+    const SpecLds t = load_tables<NT, KIND>(ax, im + pdn(n), false, tid);
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int i = tid + it * NT;
         if (i >= n) break;
-        re[i] = a[it];
-        im[i] = b[it];
+        re[pd(i)] = a[it];
+        im[pd(i)] = b[it];
     }
     __syncthreads();
 #pragma unroll
@@ -247,15 +260,15 @@ __global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, in
     for (int it = 0; it < IT; ++it) {
         const int k = tid + it * NT;
         if (k >= n) break;
-        const int p = brev(k, m);
+        const int p = pd(brev(k, m));
         re[p] = a[it];
         im[p] = b[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, pdn(n), tid);
     const double inv_n = 1.0 / (double)n;
     for (int i = tid; i < n; i += NT) {  // z = conj(FFT(conj Z)) / n
-        const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
+        const int jv = pd(KIND == SPEC_RDFT ? i : mk_pos(i, n));
         const double s = (KIND == SPEC_ALT && (i & 1)) ? -inv_n : inv_n;
         r0[i] = s * re[jv];
         if (two) r0[i + P] = -s * im[jv];
@@ -268,8 +281,11 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
     constexpr int IT = 2048 / NT;  // C n <= 2048 line-pair values per block
     extern __shared__ double sp_smem[];
     const int n = ay.n, m = ay.m, tid = threadIdx.x;
-    double* re = sp_smem;  // C transforms of n
-    double* im = re + C * n;
+    // C transforms of n, transform f at f S: the odd stride puts the lines of
+    // one row (read and written together) on different banks
+    const int S = pdn(n);
+    double* re = sp_smem;
+    double* im = re + C * S;
     // XCD-aware: the 2C-column strips of one XCD are contiguous, so the
     // 128-byte lines that neighbouring strips share are fetched by one L2
     const int x0 = tile_id().x * 2 * C;
@@ -284,7 +300,7 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
     double lx[W];  // eigenvalues of the block's columns
 #pragma unroll
     for (int c = 0; c < W; ++c) lx[c] = mux[x0 + c];
-    const SpecLds t = load_tables<NT, KIND>(ay, im + C * n, true, tid);
+    const SpecLds t = load_tables<NT, KIND>(ay, im + C * S, true, tid);
 #pragma unroll
     for (int it = 0; it < 2 * IT; ++it) {
         const int e = tid + it * NT;
@@ -292,12 +308,12 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
         const int y = e / W, c = e % W;
         const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
         const double s = (KIND == SPEC_ALT && (y & 1)) ? -1.0 : 1.0;
-        const int p = (c >> 1) * n + brev(jv, m);
+        const int p = (c >> 1) * S + pd(brev(jv, m));
         if (c & 1) im[p] = s * v[it];
         else re[p] = s * v[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, C, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, C, S, tid);
     // coefficients of the column pair f at row mode l, divided by the eigenvalue
     double o1[IT], o2[IT];
 #pragma unroll
@@ -305,7 +321,7 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
         const int q = tid + it * NT;
         if (q >= C * n) break;
         const int f = q >> m, l = q & (n - 1);
-        split_coeffs<KIND>(n, t, re + f * n, im + f * n, l, o1[it], o2[it]);
+        split_coeffs<KIND>(n, t, re + f * S, im + f * S, l, o1[it], o2[it]);
         const double my = t.mu[l];
         const double l1 = lx[2 * f] + my, l2 = lx[2 * f + 1] + my;
         o1[it] = l1 != 0.0 ? o1[it] / l1 : 0.0;
@@ -316,8 +332,9 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
     for (int it = 0; it < IT; ++it) {
         const int q = tid + it * NT;
         if (q >= C * n) break;
-        re[q] = o1[it];  // coefficient line of column 2f in re[f n ...], 2f+1 in im[f n ...]
-        im[q] = o2[it];
+        const int p = (q >> m) * S + pd(q & (n - 1));
+        re[p] = o1[it];  // coefficient line of column 2f in re[f S ...], 2f+1 in im[f S ...]
+        im[p] = o2[it];
     }
     __syncthreads();
 #pragma unroll
@@ -326,8 +343,8 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
         if (q >= C * n) break;
         const int f = q >> m, l = q & (n - 1);
         double v1r, v1i, v2r, v2i;
-        line_spectrum<KIND>(n, t, re + f * n, l, v1r, v1i);
-        line_spectrum<KIND>(n, t, im + f * n, l, v2r, v2i);
+        line_spectrum<KIND>(n, t, re + f * S, l, v1r, v1i);
+        line_spectrum<KIND>(n, t, im + f * S, l, v2r, v2i);
         o1[it] = v1r - v2i;
         o2[it] = -(v1i + v2r);
     }
@@ -336,12 +353,12 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
     for (int it = 0; it < IT; ++it) {
         const int q = tid + it * NT;
         if (q >= C * n) break;
-        const int p = (q >> m) * n + brev(q & (n - 1), m);
+        const int p = (q >> m) * S + pd(brev(q & (n - 1), m));
         re[p] = o1[it];
         im[p] = o2[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, C, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, C, S, tid);
     const double inv_n = 1.0 / (double)n;
 #pragma unroll
     for (int it = 0; it < 2 * IT; ++it) {
@@ -350,7 +367,7 @@ __global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __r
         const int y = e / W, c = e % W;
         const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
         const double s = (KIND == SPEC_ALT && (y & 1)) ? -inv_n : inv_n;
-        const int p = (c >> 1) * n + jv;
+        const int p = (c >> 1) * S + pd(jv);
         x[(size_t)y * P + x0 + c] = (c & 1) ? -s * im[p] : s * re[p];
     }
 }
@@ -392,11 +409,13 @@ std::vector<double> spec_tables(int n, int kind, double h2inv) {
 
 constexpr int SP_ROW_THREADS = 256, SP_COL_THREADS = 512;
 
-static size_t rows_lds(int n, int kind) { return (2 * (size_t)n + spec_table_values(n, kind, false)) * sizeof(double); }
+static size_t rows_lds(int n, int kind) {
+    return (2 * (size_t)pdn(n) + spec_table_values(n, kind, false)) * sizeof(double);
+}
 // two column pairs per block where n allows (rows of 32 bytes), else one
 static int cols_pairs(int n, int nx) { return (n <= SPEC_MAX_N / 2 && nx % 4 == 0) ? 2 : 1; }
 static size_t cols_lds(int n, int kind, int C) {
-    return (2 * (size_t)C * n + spec_table_values(n, kind, true)) * sizeof(double);
+    return (2 * (size_t)C * pdn(n) + spec_table_values(n, kind, true)) * sizeof(double);
 }
 
 template <int KX>
