@@ -82,6 +82,7 @@ struct fsi_context {
     int stage_warm = 1;    // per-stage projection warm starts (FSI_STAGE_WARM=0: shared correction)
     int have_history = 0;  // iteration counts of a completed step are known
     int plan_vel[3] = {0, 0, 0};
+    int vel_check_from[3] = {0, 0, 0};  // the fewest iterations of the same history
     int plan_proj[3] = {0, 0, 0};
     // iteration counts of the last kHist completed steps per stage (the plan
     // is their maximum plus a margin, so alternating counts keep one plan)
@@ -126,7 +127,7 @@ struct fsi_workspace {
     std::vector<double*> owned;
     // captured steps, keyed by everything their kernel arguments depend on
     struct StepGraph {
-        int vel[3], proj[3];
+        int vel[3], proj[3], vchk[3];
         double dt;
         const double* sU;  // state/stage buffers alternate between steps
         int corr_in, corr_out;
@@ -949,7 +950,8 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     const double vrtol = c->num.velocity_rtol;
     const int vmax = c->num.velocity_maxiter;
     if (fixed > 0) {
-        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax,
+                         c->vel_check_from[slot]);
         if (slot >= 0 && c->stage_warm)
             launch_copy2(c->stream, p, w->Us, w->vel_old2_u[slot], w->Vs, w->vel_old2_v[slot]);
         return FSI_OK;
@@ -959,7 +961,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     int m = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
     while (m <= last) {
         int n = std::min(chunk, last + 1 - m);
-        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot, vrtol, vmax);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot, vrtol, vmax, 1 << 30);
         m += n;
         fsi_status rc = poll_done(c, &c->st->vel, &done, &iter);
         if (rc != FSI_OK) return rc;
@@ -1181,7 +1183,9 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
     for (auto& g : w->graphs) {
         bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur && g.warm == w->stage_warm &&
                     g.vcur == w->vel_cur_u[0];
-        for (int k = 0; k < 3; ++k) same = same && g.vel[k] == c->plan_vel[k] && g.proj[k] == c->plan_proj[k];
+        for (int k = 0; k < 3; ++k)
+            same = same && g.vel[k] == c->plan_vel[k] && g.proj[k] == c->plan_proj[k] &&
+                   g.vchk[k] == c->vel_check_from[k];
         if (same) {
             hit = &g;
             break;
@@ -1191,6 +1195,7 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
         fsi_workspace::StepGraph g{};
         for (int k = 0; k < 3; ++k) {
             g.vel[k] = c->plan_vel[k];
+            g.vchk[k] = c->vel_check_from[k];
             g.proj[k] = c->plan_proj[k];
         }
         g.dt = dt;
@@ -1264,13 +1269,18 @@ static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, dou
     // of the last kHist steps plus one; a larger count redoes the step
     static const int kVelMargin = 1, kProjMargin = 1;
     const int nh = std::min(c->hist_n, fsi_context::kHist);
+    // Kernels from the fewest iterations of that history on test the
+    // previous iterate before loading the fields, so the launch that sees
+    // convergence is short.
     for (int k = 0; k < 3; ++k) {
-        int mv = c->last_vel_iters[k], mp = c->last_proj_iters[k];
+        int mv = c->last_vel_iters[k], mp = c->last_proj_iters[k], lo = mv;
         for (int h = 0; h < nh; ++h) {
             mv = std::max(mv, c->hist_vel[k][h]);
             mp = std::max(mp, c->hist_proj[k][h]);
+            lo = std::min(lo, c->hist_vel[k][h]);
         }
         c->plan_vel[k] = mv + kVelMargin;
+        c->vel_check_from[k] = lo;
         c->plan_proj[k] = mp + kProjMargin;
     }
     if ((rc = status_push_reset(c)) != FSI_OK) return rc;

@@ -105,8 +105,10 @@ struct VelBufs {
     double* partials;
 };
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st);
+// kernels m0 .. m0+count-1; those with m >= max(check_from, 1) test the
+// previous iterate before loading any field
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
-                      int slot, double rtol, int maxiter);
+                      int slot, double rtol, int maxiter, int check_from);
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
 
 // ---- projection (fsi_solvers.hip) ----

@@ -280,9 +280,35 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
 // beta = rr'/rr; then r' = r - a q, p' = r' + beta p, x += a p, q' = A p'
 // with the partials of p'.q', r'.q', q'.q', r'.r' for iteration m+1.  p' is
 // formed once per point on a halo-1 LDS tile and A is applied from there.
+// The field loads are normally issued before the totals are summed, so the
+// two latencies overlap; with check_first (m >= 1, an iteration at which
+// convergence is expected) the totals come first, so the launch that sees
+// convergence ends without touching the fields.
 __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
-                                                    int slot, double rtol, int maxiter_arg) {
+                                                    int slot, double rtol, int maxiter_arg, int check_first) {
     if (st->vel.done) return;
+    // iterate m passes the test or the iteration limit: the solve ends here
+    auto finish = [&](double rr, bool conv, double bnorm2) {
+        if (block_id() == 0 && lane_id() == 0) {
+            st->vel.done = 1;
+            st->vel.rr = rr;
+            st->vel.iter = m;
+            st->vel_iters[slot] = m;
+            if (conv) st->vel_ok[slot] = 1;
+            // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
+            if (!conv && rr > 1e-12 * bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / bnorm2), (double)m);
+        }
+    };
+    double tot[4];
+    if (check_first) {
+        get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
+        const double rr = tot[3], tol2 = st->vel.tol2;
+        const bool conv = rr <= tol2;
+        if (conv || m >= st->vel.maxiter) {
+            finish(rr, conv, st->vel.bnorm2);
+            return;
+        }
+    }
     const int o = m & 1, n = (m + 1) & 1;
     const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
     __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
@@ -354,8 +380,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
         hpu = pu0[kk];
         hpv = pv0[kk];
     }
-    double tot[4];
-    get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
+    if (!check_first) get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
     // the first iteration sets the solve's scalars from the partials of k_vel_start
     double tol2, bnorm2;
@@ -381,15 +406,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     }
     const bool conv = rr <= tol2, stop = m >= maxiter;
     if (conv || stop) {
-        if (block_id() == 0 && lane_id() == 0) {
-            st->vel.done = 1;
-            st->vel.rr = rr;
-            st->vel.iter = m;
-            st->vel_iters[slot] = m;
-            if (conv) st->vel_ok[slot] = 1;
-            // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
-            if (!conv && rr > 1e-12 * bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / bnorm2), (double)m);
-        }
+        finish(rr, conv, bnorm2);
         return;
     }
     const double a = rr / pq;
@@ -977,10 +994,13 @@ void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double al
     k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
-                      int slot, double rtol, int maxiter) {
+                      int slot, double rtol, int maxiter, int check_from) {
     const int nb = grid_blocks(p);
-    for (int c = 0; c < count; ++c)
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m0 + c, slot, rtol, maxiter);
+    for (int c = 0; c < count; ++c) {
+        const int m = m0 + c;
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, rtol, maxiter,
+                                               m >= 1 && m >= check_from ? 1 : 0);
+    }
 }
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
     k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
@@ -1118,7 +1138,7 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         return 2 * cell;
     }
     if (n == "vel_iter") {
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0, 0.0, 1 << 30);
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0, 0.0, 1 << 30, 0);
         return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
     }
     if (n == "pcg_dir") {
