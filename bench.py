@@ -33,15 +33,15 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 # Algorithmic HBM passes per step, in fields of N^2 doubles.
 def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
     """Passes this implementation makes per step: tendencies 69 (3 x (10 + 4
-    + 9)), stage combinations 90; per velocity solve 22 (start: right-hand
-    side and four history fields read, x r p q written; the solution copied
-    into the history) + 16 per CG iteration; per projection 14 (rhs 3, the
+    + 9)), stage combinations 90; per velocity solve 18 (start: right-hand
+    side and four history fields read, x r p q written) + 16 per CG
+    iteration; per projection 14 (rhs 3, the
     tested solution 2, update 5, check 4) + 6 for the direct transform solve
     (three kernels each reading and writing the field once), or 17 + 17.3 per
     PCG iteration when the multigrid-PCG solves it."""
     cell = 8.0 * N * N
     per_proj = 14 + 6 if direct else 17
-    passes = 69 + 90 + sum(22 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
+    passes = 69 + 90 + sum(18 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
     return passes * cell
 
 

@@ -117,11 +117,13 @@ struct fsi_workspace {
     double* corr_stage[3] = {};
     double* corr_prev[3] = {};  // the step before, for the linear extrapolation 2 q^{n-1} - q^{n-2}
     // the same for the predictor solves: stage s of the previous two steps
-    // velocity stage solutions of the last three steps (newest first); the
-    // solve writes into vel_old, and the three trade roles once a step has
-    // succeeded
+    // velocity stage solutions of the last four steps (newest first) and a
+    // fifth buffer that the solve writes into (the projection then writes
+    // the projected stage velocity to Us, leaving the solution in place);
+    // the five trade roles once a step has succeeded
     double *vel_cur_u[3] = {}, *vel_cur_v[3] = {}, *vel_prev_u[3] = {}, *vel_prev_v[3] = {};
     double *vel_old_u[3] = {}, *vel_old_v[3] = {}, *vel_old2_u[3] = {}, *vel_old2_v[3] = {};
+    double *vel_next_u[3] = {}, *vel_next_v[3] = {};
     int stage_warm = 0;         // completed steps whose stage solutions the history buffers hold (0 .. 3)
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
@@ -799,7 +801,9 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2], &w->vel_old_u[0],
                           &w->vel_old_u[1],  &w->vel_old_u[2],  &w->vel_old_v[0],  &w->vel_old_v[1],
                           &w->vel_old_v[2],  &w->vel_old2_u[0], &w->vel_old2_u[1], &w->vel_old2_u[2],
-                          &w->vel_old2_v[0], &w->vel_old2_v[1], &w->vel_old2_v[2]};
+                          &w->vel_old2_v[0], &w->vel_old2_v[1], &w->vel_old2_v[2], &w->vel_next_u[0],
+                          &w->vel_next_u[1], &w->vel_next_u[2], &w->vel_next_v[0], &w->vel_next_v[1],
+                          &w->vel_next_v[2]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -908,16 +912,17 @@ static fsi_status poll_done(fsi_context* c, const KrylovScalars* dev, int* done,
 
 // fixed > 0: enqueue exactly `fixed` iteration kernels without reading the
 // residual back (speculative step); otherwise poll until converged.
+// The predictor of stage `slot` into (xu, xv).
 static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot,
-                                 int fixed) {
+                                 int fixed, double* xu, double* xv) {
     const Params& p = c->prm;
     VelBufs b{};
     b.Un = s->U;
     b.Vn = s->V;
     b.ru = w->ru;
     b.rv = w->rv;
-    b.xu = w->Us;
-    b.xv = w->Vs;
+    b.xu = xu;
+    b.xv = xv;
     for (int k = 0; k < 2; ++k) {
         b.r_u[k] = w->r_u[k];
         b.r_v[k] = w->r_v[k];
@@ -952,8 +957,6 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     if (fixed > 0) {
         launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax,
                          c->vel_check_from[slot]);
-        if (slot >= 0 && c->stage_warm)
-            launch_copy2(c->stream, p, w->Us, w->vel_old2_u[slot], w->Vs, w->vel_old2_v[slot]);
         return FSI_OK;
     }
     // kernel m tests the residual of iterate m and performs update m+1
@@ -969,19 +972,18 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         chunk = 4;
     }
     c->last_vel_iters[slot] = std::max(1, iter);
-    // the solution goes to the oldest history buffer, which the start kernel
-    // has read; the three trade roles once the step has succeeded
-    if (slot >= 0 && c->stage_warm)
-        launch_copy2(c->stream, p, w->Us, w->vel_old2_u[slot], w->Vs, w->vel_old2_v[slot]);
     return FSI_OK;
 }
 
-static fsi_status projection(fsi_context* c, fsi_workspace* w, double* U, double* V, int inject, int slot,
-                             int fixed) {
+// Projects (Uin, Vin) into (U, V); the same arrays for a projection in place.
+static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, double* Vin, double* U, double* V,
+                             int inject, int slot, int fixed) {
     const Params& p = c->prm;
     ProjBufs b{};
     b.U = U;
     b.V = V;
+    b.Uin = Uin;
+    b.Vin = Vin;
     b.inject = inject;
     b.lap_bc = c->lap_bc;
     b.rhs = w->rhs;
@@ -1090,7 +1092,7 @@ static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, 
     fsi_status rc = status_push(c);
     if (rc != FSI_OK) return rc;
     const int saved = w->corr_cur;
-    rc = projection(c, w, s->U, s->V, 0, -1, 0);
+    rc = projection(c, w, s->U, s->V, s->U, s->V, 0, -1, 0);
     if (rc != FSI_OK) return rc;
     if ((rc = status_pull(c)) != FSI_OK) return rc;
     fill_projection_result(c, res);
@@ -1166,8 +1168,13 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         launch_combine_faces(c->stream, p, s->U, s->V, w->Eu, w->Ev, w->Iu, w->Iv, si, dt, ce, ci, alpha,
                              p.visc ? c->bvisc_u : nullptr, p.visc ? c->bvisc_v : nullptr, w->ru, w->rv);
         const int k = si - 1;
-        if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0)) != FSI_OK) return rc;
-        if ((rc = projection(c, w, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
+        // with the warm-start history the predictor is kept (the next step
+        // extrapolates from it) and the projection writes the stage velocity
+        const bool keep = c->stage_warm && p.visc;
+        double* xu = keep ? w->vel_next_u[k] : w->Us;
+        double* xv = keep ? w->vel_next_v[k] : w->Vs;
+        if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0, xu, xv)) != FSI_OK) return rc;
+        if ((rc = projection(c, w, xu, xv, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
         if (si <= 2) tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0);
     }
     launch_ext_final(c->stream, p, c->st);  // the report of the last stage and the stage-1 checks
@@ -1368,9 +1375,11 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
     if (c->stage_warm) {
-        for (int k = 0; k < 3; ++k) {  // (cur, prev, old, old2) <- (old2, cur, prev, old)
-            double* nu = w->vel_old2_u[k];
-            double* nv = w->vel_old2_v[k];
+        for (int k = 0; k < 3; ++k) {  // (cur, prev, old, old2, next) <- (next, cur, prev, old, old2)
+            double* nu = w->vel_next_u[k];
+            double* nv = w->vel_next_v[k];
+            w->vel_next_u[k] = w->vel_old2_u[k];
+            w->vel_next_v[k] = w->vel_old2_v[k];
             w->vel_old2_u[k] = w->vel_old_u[k];
             w->vel_old2_v[k] = w->vel_old_v[k];
             w->vel_old_u[k] = w->vel_prev_u[k];
@@ -1459,8 +1468,8 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
     }
     vb.partials = c->partials;
     ProjBufs pb{};
-    pb.U = w->Us;
-    pb.V = w->Vs;
+    pb.U = pb.Uin = w->Us;
+    pb.V = pb.Vin = w->Vs;
     pb.lap_bc = c->lap_bc;
     pb.rhs = w->rhs;
     pb.x = w->corr[0];

@@ -113,8 +113,10 @@ void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
 
 // ---- projection (fsi_solvers.hip) ----
 struct ProjBufs {
-    double* U;  // velocity being projected (state or stage arrays)
+    double* U;  // the projected velocity (state or stage arrays)
     double* V;
+    double* Uin;  // the velocity to project (== U, V when projected in place); its
+    double* Vin;  // injected boundary faces are written here
     int inject;            // write the injected boundary faces first (_inject_compact_predictor!)
     const double* lap_bc;  // b_bc of the Laplacian
     double* rhs;
@@ -128,7 +130,6 @@ struct ProjBufs {
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
 void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap);
-void launch_copy2(hipStream_t s, const Params& p, const double* a1, double* b1, const double* a2, double* b2);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,

@@ -548,15 +548,15 @@ __global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb
     FOR_TILE(p) {
         double uR, vT;
         if (b.inject && i == p.nx - 1) {
-            uR = p.px ? b.U[at(p, 0, j)] : p.bU.r.value;
-            b.U[at(p, p.nx, j)] = uR;
-        } else uR = b.U[at(p, i + 1, j)];
+            uR = p.px ? b.Uin[at(p, 0, j)] : p.bU.r.value;
+            b.Uin[at(p, p.nx, j)] = uR;
+        } else uR = b.Uin[at(p, i + 1, j)];
         if (b.inject && j == p.ny - 1) {
-            vT = p.py ? b.V[at(p, i, 0)] : p.bV.t.value;
-            b.V[at(p, i, p.ny)] = vT;
-        } else vT = b.V[at(p, i, j + 1)];
+            vT = p.py ? b.Vin[at(p, i, 0)] : p.bV.t.value;
+            b.Vin[at(p, i, p.ny)] = vT;
+        } else vT = b.Vin[at(p, i, j + 1)];
         const size_t k = at(p, i, j);
-        double div = (uR - b.U[k]) * p.ihx + (vT - b.V[k]) * p.ihy;
+        double div = (uR - b.Uin[k]) * p.ihx + (vT - b.Vin[k]) * p.ihy;
         double rhs = (div - 0.0) - b.lap_bc[k];
         b.rhs[k] = rhs;
         acc[0] += rhs * area;
@@ -787,7 +787,8 @@ __global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int n
     put_partials<1>(acc, pset(b.partials, nb, PS_MEAN), nb);
 }
 
-// v -= grad q on every face (src/Projection.jl:266-273), q with its BC data.
+// v -= grad q on every face (src/Projection.jl:266-273), q with its BC data;
+// read from Uin/Vin, written to U/V (the same arrays for a projection in place).
 __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
     const int i = tile_id().x * TILE_X + threadIdx.x;
     GAcc Q{b.x, p.P};
@@ -811,7 +812,8 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
                     gx = (-1.0 / dist) * Q(p.nx - 1, j) + p.bQ.r.value / dist;
                 } else gx = p.bQ.r.value;
             } else gx = p.ihx * Q(i, j) - p.ihx * Q(i - 1, j);
-            b.U[at(p, i, j)] -= gx;
+            const size_t k = at(p, i, j);
+            b.U[k] = b.Uin[k] - gx;
         }
         if (i < p.nx) {  // y-face (i, j)
             double gy;
@@ -830,7 +832,8 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
                     gy = (-1.0 / dist) * Q(i, p.ny - 1) + p.bQ.t.value / dist;
                 } else gy = p.bQ.t.value;
             } else gy = p.ihy * Q(i, j) - p.ihy * Q(i, j - 1);
-            b.V[at(p, i, j)] -= gy;
+            const size_t k = at(p, i, j);
+            b.V[k] = b.Vin[k] - gy;
         }
     }
 }
@@ -1026,18 +1029,6 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restr
         if (extrap) cur[k] = 2.0 * c - prev[k];
         prev[k] = c;
     }
-}
-// the same for the two velocity components, and a two-field copy
-__global__ void __launch_bounds__(BLOCK) k_copy2(Params p, const double* __restrict__ a1, double* __restrict__ b1,
-                                                 const double* __restrict__ a2, double* __restrict__ b2) {
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        b1[k] = a1[k];
-        b2[k] = a2[k];
-    }
-}
-void launch_copy2(hipStream_t s, const Params& p, const double* a1, double* b1, const double* a2, double* b2) {
-    k_copy2<<<grid2(p), kBlock, 0, s>>>(p, a1, b1, a2, b2);
 }
 void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap) {
     k_extrapolate<<<grid2(p), kBlock, 0, s>>>(p, cur, prev, extrap);

@@ -272,6 +272,36 @@ def test_planned_steps_match_polling_steps(monkeypatch, mode):
         assert np.array_equal(a[f], b[f]), f
 
 
+def test_warm_start_history_matches_cold_starts(monkeypatch):
+    """Nine tight steps with the stage solutions of the previous steps as CG
+    warm start (linear, quadratic, then cubic extrapolation in time; five
+    history buffers per stage that rotate, so every buffer is reused) against
+    nine steps whose solves all start from the state: the solutions agree to
+    the solver tolerance, and the history saves iterations."""
+    def run(warm):
+        if warm:
+            monkeypatch.delenv("FSI_STAGE_WARM", raising=False)
+        else:
+            monkeypatch.setenv("FSI_STAGE_WARM", "0")
+        case = setups.seeded_cavity(64, "acdi", seed=13, **TIGHT)
+        ctx, work = _device(case)
+        fsi.make_velocity_compatible(case.state, ctx, work)
+        iters = 0
+        for k in range(9):
+            fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+            iters += sum(work.last_stats.velocity_iterations)
+        return {f: getattr(case.state, f).copy() for f in FIELDS}, iters
+
+    a, ia = run(True)
+    b, ib = run(False)
+    print("velocity iterations warm", ia, "cold", ib, flush=True)
+    assert ia < ib
+    for f in FIELDS:
+        e = rel_l2(a[f], b[f])
+        print(f, f"{e:.2e}", flush=True)
+        assert e <= 1e-11, (f, e)
+
+
 @pytest.mark.parametrize("N", [64, 128])
 def test_mixed_precision_step_matches_fp64(N):
     """Mixed precision (FP32 V-cycle preconditioner inside the FP64 PCG;
