@@ -201,7 +201,7 @@ __device__ __forceinline__ void put_partials(double (&v)[NV], double* dst, int n
         for (int k = 0; k < NV; ++k) dst[(size_t)k * nb + bid] = v[k];
 }
 
-template <int NV>
+template <int NV, int UNR = 4>
 __device__ __forceinline__ void get_totals(const double* src, int nb, double (&out)[NV]) {
     __shared__ double lds[NV * 16];
     __shared__ double res[NV];
@@ -210,9 +210,23 @@ __device__ __forceinline__ void get_totals(const double* src, int nb, double (&o
     double acc[NV];
 #pragma unroll
     for (int k = 0; k < NV; ++k) acc[k] = 0.0;
-    for (int b = lane; b < nb; b += nthreads)
+    // UNR partials per value in flight at once (one memory latency per UNR
+    // instead of per partial; UNR * NV doubles of registers); the additions
+    // keep the order b = lane, lane + nthreads, ... (adding 0.0 past nb
+    // leaves the sum unchanged)
+    for (int b0 = lane; b0 < nb; b0 += UNR * nthreads) {
+        double v[UNR][NV];
 #pragma unroll
-        for (int k = 0; k < NV; ++k) acc[k] += src[(size_t)k * nb + b];
+        for (int u = 0; u < UNR; ++u) {
+            const int b = b0 + u * nthreads;
+#pragma unroll
+            for (int k = 0; k < NV; ++k) v[u][k] = b < nb ? src[(size_t)k * nb + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u)
+#pragma unroll
+            for (int k = 0; k < NV; ++k) acc[k] += v[u][k];
+    }
     block_sum<NV>(acc, lds);
     if (lane == 0)
 #pragma unroll

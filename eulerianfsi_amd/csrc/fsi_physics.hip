@@ -1427,11 +1427,24 @@ __global__ void __launch_bounds__(BLOCK) k_ext_final(Params p, Status* st) {
 #pragma unroll
     for (int q = 0; q < NS; ++q) sm[q] = 0.0;
     const size_t nb = p.exnb;
-    for (size_t b = tid; b < nb; b += BLOCK) {
+    constexpr int UNR = 4;  // partials of UNR blocks in flight at once
+    for (size_t b0 = tid; b0 < nb; b0 += UNR * BLOCK) {
+        double vm[UNR][NM], vs[UNR][NS];
 #pragma unroll
-        for (int q = 0; q < NM; ++q) m[q] = fmax(m[q], p.exbuf[mslot[q] * nb + b]);
+        for (int u = 0; u < UNR; ++u) {
+            const size_t b = b0 + (size_t)u * BLOCK;
 #pragma unroll
-        for (int q = 0; q < NS; ++q) sm[q] += p.exbuf[sslot[q] * nb + b];
+            for (int q = 0; q < NM; ++q) vm[u][q] = b < nb ? p.exbuf[mslot[q] * nb + b] : -INFINITY;
+#pragma unroll
+            for (int q = 0; q < NS; ++q) vs[u][q] = b < nb ? p.exbuf[sslot[q] * nb + b] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+#pragma unroll
+            for (int q = 0; q < NM; ++q) m[q] = fmax(m[q], vm[u][q]);
+#pragma unroll
+            for (int q = 0; q < NS; ++q) sm[q] += vs[u][q];
+        }
     }
 #pragma unroll
     for (int q = 0; q < NM; ++q) m[q] = wave_max(m[q]);

@@ -380,7 +380,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
         hpu = pu0[kk];
         hpv = pv0[kk];
     }
-    if (!check_first) get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
+    // (one partial in flight at a time: the field loads hold the registers)
+    if (!check_first) get_totals<4, 1>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
     // the first iteration sets the solve's scalars from the partials of k_vel_start
     double tol2, bnorm2;
@@ -932,14 +933,31 @@ __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, in
     const int lane = lane_id();
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     double linf = 0.0, umax = 0.0, vmax = 0.0;
-    for (int k = lane; k < nb; k += BLOCK) {
-        if (pcg_m >= 0) acc[0] += rrp[k];
-        acc[1] += chk[k];
-        acc[2] += chk[(size_t)nb + k];
-        acc[3] += vm[2 * (size_t)nb + k];
-        linf = fmax(linf, chk[2 * (size_t)nb + k]);
-        umax = fmax(umax, vm[k]);
-        vmax = fmax(vmax, vm[(size_t)nb + k]);
+    constexpr int UNR = 4;  // partials of UNR blocks in flight at once
+    for (int k0 = lane; k0 < nb; k0 += UNR * BLOCK) {
+        double v[UNR][7];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int k = k0 + u * BLOCK;
+            const bool in = k < nb;
+            v[u][0] = in && pcg_m >= 0 ? rrp[k] : 0.0;
+            v[u][1] = in ? chk[k] : 0.0;
+            v[u][2] = in ? chk[(size_t)nb + k] : 0.0;
+            v[u][3] = in ? vm[2 * (size_t)nb + k] : 0.0;
+            v[u][4] = in ? chk[2 * (size_t)nb + k] : 0.0;
+            v[u][5] = in ? vm[k] : 0.0;
+            v[u][6] = in ? vm[(size_t)nb + k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            acc[0] += v[u][0];
+            acc[1] += v[u][1];
+            acc[2] += v[u][2];
+            acc[3] += v[u][3];
+            linf = fmax(linf, v[u][4]);
+            umax = fmax(umax, v[u][5]);
+            vmax = fmax(vmax, v[u][6]);
+        }
     }
     __shared__ double lds[4 * 16];
     __shared__ double lmax[3][BLOCK / 64];
