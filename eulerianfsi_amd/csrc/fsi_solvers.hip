@@ -841,60 +841,66 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
 
 // q_out = q - mean; true residual |A q - rhs| (:261-264); constraint div(v) - G
 // in L-inf and area-weighted L2 (:275-297)
+template <int RESID>
 __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
-                                                      int has_null, double lxly, int resid) {
+                                                      int has_null, double lxly) {
     double tm[1];
     get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
     const double mean = has_null ? tm[0] / lxly : 0.0;
     const double area = p.hx * p.hy;
     auto Q = [&](int a, int c) { return b.x[at(p, a, c)] - mean; };
-    double acc[2] = {0.0, 0.0};
+    // sums: true residual, constraint, NaN faces; max |U| over every x-face,
+    // max |V| over every y-face and their NaN count are the velocity extrema
+    // of the driver's timestep bound, for free here
+    double acc[3] = {0.0, 0.0, 0.0};
     double linf = 0.0;
-    // max |U| over every x-face, max |V| over every y-face and their NaN count
-    // (the velocity extrema of the driver's timestep bound, for free here)
-    double umax = 0.0, vmax = 0.0, nan_uv = 0.0;
-    FOR_TILE(p) {
-        const size_t k = at(p, i, j);
-        double q = Q(i, j);
-        b.q_out[k] = q;
-        if (resid) {
+    double umax = 0.0, vmax = 0.0;
+    // every load of the thread's rows first (index 0 stands in outside)
+    const int i = tile_id().x * TILE_X + threadIdx.x;
+    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
+    bool in[ROWS];
+    size_t kk[ROWS];
+    double xq[ROWS], uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = jb + r * TILE_Y;
+        in[r] = i < p.nx && j < p.ny;
+        kk[r] = in[r] ? at(p, i, j) : 0;
+        const size_t k = kk[r];
+        xq[r] = b.x[k];
+        uL[r] = b.U[k];
+        uR[r] = b.U[k + 1];
+        vB[r] = b.V[k];
+        vT[r] = b.V[k + p.P];
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (!in[r]) continue;
+        const int j = jb + r * TILE_Y;
+        const size_t k = kk[r];
+        b.q_out[k] = xq[r] - mean;
+        if (RESID) {
             double aq = -neg_lap(g, Q, i, j, nullptr);
             double res = aq - b.rhs[k];
             acc[0] += res * res;
         }
-        const double uL = b.U[k], uR = b.U[at(p, i + 1, j)], vB = b.V[k], vT = b.V[at(p, i, j + 1)];
-        double div = (uR - uL) * p.ihx + (vT - vB) * p.ihy;
+        double div = (uR[r] - uL[r]) * p.ihx + (vT[r] - vB[r]) * p.ihy;
         double c = div - 0.0;
         linf = fmax(linf, fabs(c));
         if (!isfinite(c)) linf = INFINITY;
         acc[1] += c * c * area;
-        umax = fmax(umax, fabs(uL));
-        vmax = fmax(vmax, fabs(vB));
-        nan_uv += (isnan(uL) ? 1.0 : 0.0) + (isnan(vB) ? 1.0 : 0.0);
-        if (i == p.nx - 1) {
-            umax = fmax(umax, fabs(uR));
-            nan_uv += isnan(uR) ? 1.0 : 0.0;
-        }
-        if (j == p.ny - 1) {
-            vmax = fmax(vmax, fabs(vT));
-            nan_uv += isnan(vT) ? 1.0 : 0.0;
-        }
+        // the far faces x = nx and y = ny belong to the last column / row
+        const bool ue = i == p.nx - 1, ve = j == p.ny - 1;
+        umax = fmax(umax, fmax(fabs(uL[r]), ue ? fabs(uR[r]) : 0.0));
+        vmax = fmax(vmax, fmax(fabs(vB[r]), ve ? fabs(vT[r]) : 0.0));
+        acc[2] += ((isnan(uL[r]) ? 1.0 : 0.0) + (isnan(vB[r]) ? 1.0 : 0.0)) +
+                  ((ue && isnan(uR[r]) ? 1.0 : 0.0) + (ve && isnan(vT[r]) ? 1.0 : 0.0));
     }
-    {  // slot 0 (true residual) comes from k_proj_direct when resid == 0
-        __shared__ double lds[2 * 16];
-        block_sum<2>(acc, lds);
-        if (lane_id() == 0) {
-            double* dst = pset(b.partials, nb, PS_CHECK);
-            if (resid) dst[block_id()] = acc[0];
-            dst[(size_t)nb + block_id()] = acc[1];
-        }
-    }
+    __shared__ double lds[3 * 16];
+    block_sum<3>(acc, lds);
     // block maxima: the constraint as a third partial of PS_CHECK, the face
-    // velocities as PS_VMAX (maxima taken by k_proj_final); the NaN count sums
+    // velocities as PS_VMAX with the NaN count (maxima taken by k_proj_final)
     __shared__ double lmax[3][BLOCK / 64];
-    __shared__ double lnan[16];
-    double nn[1] = {nan_uv};
-    block_sum<1>(nn, lnan);
     linf = wave_max(linf);
     umax = wave_max(umax);
     vmax = wave_max(vmax);
@@ -912,11 +918,14 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
             for (int w = 1; w < BLOCK / 64; ++w) m[q] = fmax(m[q], lmax[q][w]);
         }
         const int bid = blockIdx.x + blockIdx.y * gridDim.x;
-        pset(b.partials, nb, PS_CHECK)[2 * (size_t)nb + bid] = m[0];
+        double* dst = pset(b.partials, nb, PS_CHECK);
+        if (RESID) dst[bid] = acc[0];  // else slot 0 (true residual) comes from k_proj_direct
+        dst[(size_t)nb + bid] = acc[1];
+        dst[2 * (size_t)nb + bid] = m[0];
         double* vm = pset(b.partials, nb, PS_VMAX);
         vm[bid] = m[1];
         vm[(size_t)nb + bid] = m[2];
-        vm[2 * (size_t)nb + bid] = nn[0];
+        vm[2 * (size_t)nb + bid] = acc[2];
     }
 }
 
@@ -1121,7 +1130,7 @@ void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const 
     const int nb = grid_blocks(p);
     k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, nb);
     k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
-    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly, 1);
+    k_proj_check<1><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
     k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, -1, -1);
 }
 void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
@@ -1129,7 +1138,7 @@ void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const 
     const int nb = grid_blocks(p);
     k_proj_direct<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter);
     k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
-    k_proj_check<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly, 0);
+    k_proj_check<0><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
     k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
 }
 
