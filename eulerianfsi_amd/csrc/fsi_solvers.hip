@@ -546,19 +546,40 @@ __device__ __forceinline__ double pweight(int f, int I, int nf, int per, int dlo
 __global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb) {
     const double area = p.hx * p.hy;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    FOR_TILE(p) {
-        double uR, vT;
+    // every load of the thread's rows first (index 0 stands in outside)
+    const int i = tile_id().x * TILE_X + threadIdx.x;
+    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
+    bool in[ROWS];
+    size_t kk[ROWS];
+    double uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS], lb[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = jb + r * TILE_Y;
+        in[r] = i < p.nx && j < p.ny;
+        const size_t k = in[r] ? at(p, i, j) : 0;
+        kk[r] = k;
+        uL[r] = b.Uin[k];
+        uR[r] = b.Uin[k + 1];
+        vB[r] = b.Vin[k];
+        vT[r] = b.Vin[k + p.P];
+        lb[r] = b.lap_bc[k];
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (!in[r]) continue;
+        const int j = jb + r * TILE_Y;
+        const size_t k = kk[r];
+        // _inject_compact_predictor!: the far faces take the boundary data
         if (b.inject && i == p.nx - 1) {
-            uR = p.px ? b.Uin[at(p, 0, j)] : p.bU.r.value;
-            b.Uin[at(p, p.nx, j)] = uR;
-        } else uR = b.Uin[at(p, i + 1, j)];
+            uR[r] = p.px ? b.Uin[at(p, 0, j)] : p.bU.r.value;
+            b.Uin[k + 1] = uR[r];
+        }
         if (b.inject && j == p.ny - 1) {
-            vT = p.py ? b.Vin[at(p, i, 0)] : p.bV.t.value;
-            b.Vin[at(p, i, p.ny)] = vT;
-        } else vT = b.Vin[at(p, i, j + 1)];
-        const size_t k = at(p, i, j);
-        double div = (uR - b.Uin[k]) * p.ihx + (vT - b.Vin[k]) * p.ihy;
-        double rhs = (div - 0.0) - b.lap_bc[k];
+            vT[r] = p.py ? b.Vin[at(p, i, 0)] : p.bV.t.value;
+            b.Vin[k + p.P] = vT[r];
+        }
+        double div = (uR[r] - uL[r]) * p.ihx + (vT[r] - vB[r]) * p.ihy;
+        double rhs = (div - 0.0) - lb[r];
         b.rhs[k] = rhs;
         acc[0] += rhs * area;
         acc[1] += fabs(rhs) * area;
@@ -789,53 +810,81 @@ __global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int n
 }
 
 // v -= grad q on every face (src/Projection.jl:266-273), q with its BC data;
-// read from Uin/Vin, written to U/V (the same arrays for a projection in place).
+// read from Uin/Vin, written to U/V (the same arrays for a projection in
+// place).  Launched on the cell grid: the thread of cell (i, j) updates the
+// x-face (i, j) and the y-face (i, j), the last column and row also the far
+// faces x = nx and y = ny; every load of the thread's rows is issued first.
 __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
-    const int i = tile_id().x * TILE_X + threadIdx.x;
     GAcc Q{b.x, p.P};
-    for (int rr_ = 0; rr_ < ROWS; ++rr_) {
-        const int j = tile_id().y * TILE_ROWS + threadIdx.y + rr_ * TILE_Y;
-        if (i > p.nx || j > p.ny) continue;
-        if (j < p.ny) {  // x-face (i, j)
-            double gx;
-            if (p.px) {
-                int k = (i == p.nx) ? 0 : i;
-                int km = k == 0 ? p.nx - 1 : k - 1;
-                gx = p.ihx * Q(k, j) - p.ihx * Q(km, j);
-            } else if (i == 0) {
-                if (p.bQ.l.kind == FSI_BC_DIRICHLET) {
-                    double dist = p.hx / 2.0;
-                    gx = (1.0 / dist) * Q(0, j) + (-p.bQ.l.value / dist);
-                } else gx = -p.bQ.l.value;
-            } else if (i == p.nx) {
-                if (p.bQ.r.kind == FSI_BC_DIRICHLET) {
-                    double dist = p.hx / 2.0;
-                    gx = (-1.0 / dist) * Q(p.nx - 1, j) + p.bQ.r.value / dist;
-                } else gx = p.bQ.r.value;
-            } else gx = p.ihx * Q(i, j) - p.ihx * Q(i - 1, j);
-            const size_t k = at(p, i, j);
-            b.U[k] = b.Uin[k] - gx;
+    // the face gradients with their boundary rows (any face)
+    auto gx_face = [&](int i, int j) -> double {
+        if (p.px) {
+            int k = (i == p.nx) ? 0 : i;
+            int km = k == 0 ? p.nx - 1 : k - 1;
+            return p.ihx * Q(k, j) - p.ihx * Q(km, j);
+        } else if (i == 0) {
+            if (p.bQ.l.kind == FSI_BC_DIRICHLET) {
+                double dist = p.hx / 2.0;
+                return (1.0 / dist) * Q(0, j) + (-p.bQ.l.value / dist);
+            }
+            return -p.bQ.l.value;
+        } else if (i == p.nx) {
+            if (p.bQ.r.kind == FSI_BC_DIRICHLET) {
+                double dist = p.hx / 2.0;
+                return (-1.0 / dist) * Q(p.nx - 1, j) + p.bQ.r.value / dist;
+            }
+            return p.bQ.r.value;
         }
-        if (i < p.nx) {  // y-face (i, j)
-            double gy;
-            if (p.py) {
-                int k = (j == p.ny) ? 0 : j;
-                int km = k == 0 ? p.ny - 1 : k - 1;
-                gy = p.ihy * Q(i, k) - p.ihy * Q(i, km);
-            } else if (j == 0) {
-                if (p.bQ.b.kind == FSI_BC_DIRICHLET) {
-                    double dist = p.hy / 2.0;
-                    gy = (1.0 / dist) * Q(i, 0) + (-p.bQ.b.value / dist);
-                } else gy = -p.bQ.b.value;
-            } else if (j == p.ny) {
-                if (p.bQ.t.kind == FSI_BC_DIRICHLET) {
-                    double dist = p.hy / 2.0;
-                    gy = (-1.0 / dist) * Q(i, p.ny - 1) + p.bQ.t.value / dist;
-                } else gy = p.bQ.t.value;
-            } else gy = p.ihy * Q(i, j) - p.ihy * Q(i, j - 1);
-            const size_t k = at(p, i, j);
-            b.V[k] = b.Vin[k] - gy;
+        return p.ihx * Q(i, j) - p.ihx * Q(i - 1, j);
+    };
+    auto gy_face = [&](int i, int j) -> double {
+        if (p.py) {
+            int k = (j == p.ny) ? 0 : j;
+            int km = k == 0 ? p.ny - 1 : k - 1;
+            return p.ihy * Q(i, k) - p.ihy * Q(i, km);
+        } else if (j == 0) {
+            if (p.bQ.b.kind == FSI_BC_DIRICHLET) {
+                double dist = p.hy / 2.0;
+                return (1.0 / dist) * Q(i, 0) + (-p.bQ.b.value / dist);
+            }
+            return -p.bQ.b.value;
+        } else if (j == p.ny) {
+            if (p.bQ.t.kind == FSI_BC_DIRICHLET) {
+                double dist = p.hy / 2.0;
+                return (-1.0 / dist) * Q(i, p.ny - 1) + p.bQ.t.value / dist;
+            }
+            return p.bQ.t.value;
         }
+        return p.ihy * Q(i, j) - p.ihy * Q(i, j - 1);
+    };
+    const int i = tile_id().x * TILE_X + threadIdx.x;
+    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
+    bool in[ROWS];
+    size_t kk[ROWS];
+    double qc[ROWS], qw[ROWS], qs[ROWS], ui[ROWS], vi[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = jb + r * TILE_Y;
+        in[r] = i < p.nx && j < p.ny;
+        const size_t k = in[r] ? at(p, i, j) : 0;
+        kk[r] = k;
+        qc[r] = b.x[k];
+        qw[r] = b.x[in[r] && i > 0 ? k - 1 : k];
+        qs[r] = b.x[in[r] && j > 0 ? k - p.P : k];
+        ui[r] = b.Uin[k];
+        vi[r] = b.Vin[k];
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (!in[r]) continue;
+        const int j = jb + r * TILE_Y;
+        const size_t k = kk[r];
+        const double gx = i > 0 ? p.ihx * qc[r] - p.ihx * qw[r] : gx_face(0, j);
+        const double gy = j > 0 ? p.ihy * qc[r] - p.ihy * qs[r] : gy_face(i, 0);
+        b.U[k] = ui[r] - gx;
+        b.V[k] = vi[r] - gy;
+        if (i == p.nx - 1) b.U[k + 1] = b.Uin[k + 1] - gx_face(p.nx, j);
+        if (j == p.ny - 1) b.V[k + p.P] = b.Vin[k + p.P] - gy_face(i, p.ny);
     }
 }
 
@@ -1129,7 +1178,7 @@ void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const 
                         int has_null, double lxly) {
     const int nb = grid_blocks(p);
     k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, nb);
-    k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
+    k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
     k_proj_check<1><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
     k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, -1, -1);
 }
@@ -1137,7 +1186,7 @@ void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const 
                         int has_null, double rtol, int maxiter, double lxly, int slot) {
     const int nb = grid_blocks(p);
     k_proj_direct<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter);
-    k_proj_update<<<grid_of(p.nx + 1, p.ny + 1), kBlock, 0, s>>>(p, b);
+    k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
     k_proj_check<0><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
     k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
 }
