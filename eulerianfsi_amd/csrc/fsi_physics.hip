@@ -1274,50 +1274,82 @@ struct Hist {
     const double* Sxy[3];
 };
 
-__global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, Hist h, int ns, double c0,
-                                                         double c1, double c2, double* __restrict__ phi_s,
+// phi_s, sigma_s = base + sum over the NT active terms (nonzero coefficient,
+// in stage order) of c E; two rows of loads in flight at a time
+template <int NT>
+__global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, Hist h, double c0, double c1,
+                                                         double c2, double* __restrict__ phi_s,
                                                          double* __restrict__ sxx_s, double* __restrict__ syy_s,
                                                          double* __restrict__ sxy_s, Status* st, int report) {
     __shared__ double lds[4][16];
     __shared__ long long lds_n[2][16];
     const double cf[3] = {c0, c1, c2};
     const int i = tile_id().x * TILE_X + threadIdx.x;
+    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
     double pmin = INFINITY, pmax = -INFINITY, emin = INFINITY, dmax = 0.0;
     long long nonspd = 0, nonfinite = 0;
-    if (i < p.nx)
-        for (int r = 0; r < ROWS; ++r) {
-            const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
-            if (j >= p.ny) break;
-            const size_t k = at(p, i, j);
-            double ph = base.phi[k], a, d, b;
-            load_sigma(p, base, i, j, a, d, b);
-            for (int q = 0; q < ns; ++q) {
-                if (cf[q] == 0.0) continue;
-                ph += cf[q] * h.E[q][k];
-                a += cf[q] * h.Sxx[q][k];
-                d += cf[q] * h.Syy[q][k];
-                b += cf[q] * h.Sxy[q][k];
+    constexpr int RH = 2;  // rows per load batch
+#pragma unroll
+    for (int r0 = 0; r0 < ROWS; r0 += RH) {
+        bool in[RH];
+        size_t kk[RH];
+        double ph[RH], a[RH], d[RH], b[RH], e[RH][NT > 0 ? NT : 1][4];
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+            const int j = jb + (r0 + r) * TILE_Y;
+            in[r] = i < p.nx && j < p.ny;
+            const size_t k = in[r] ? at(p, i, j) : 0;
+            kk[r] = k;
+            ph[r] = base.phi[k];
+            a[r] = base.sxx[k];
+            d[r] = base.syy[k];
+            b[r] = base.sxy[k];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                e[r][q][0] = h.E[q][k];
+                e[r][q][1] = h.Sxx[q][k];
+                e[r][q][2] = h.Syy[q][k];
+                e[r][q][3] = h.Sxy[q][k];
             }
-            if (base.reset && ph < p.ext_cutoff) {
-                a = 1.0;
-                d = 1.0;
-                b = 0.0;
+        }
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+            if (!in[r]) continue;
+            const size_t k = kk[r];
+            double P = ph[r], A = a[r], D = d[r], B = b[r];
+            if (base.reset && P < p.ext_cutoff) {  // load_sigma: the safety reset of sigma_n
+                A = 1.0;
+                D = 1.0;
+                B = 0.0;
             }
-            phi_s[k] = ph;
-            sxx_s[k] = a;
-            syy_s[k] = d;
-            sxy_s[k] = b;
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                P += cf[q] * e[r][q][0];
+                A += cf[q] * e[r][q][1];
+                D += cf[q] * e[r][q][2];
+                B += cf[q] * e[r][q][3];
+            }
+            if (base.reset && P < p.ext_cutoff) {
+                A = 1.0;
+                D = 1.0;
+                B = 0.0;
+            }
+            phi_s[k] = P;
+            sxx_s[k] = A;
+            syy_s[k] = D;
+            sxy_s[k] = B;
             if (report) {  // structure_tensor_diagnostics StructureTensor.jl:627-662
-                double det = a * d - b * b;
-                double lmin = 0.5 * (a + d - hypot(a - d, 2.0 * b));
+                double det = A * D - B * B;
+                double lmin = 0.5 * (A + D - hypot(A - D, 2.0 * B));
                 dmax = fmax(dmax, fabs(det - 1.0));
                 emin = fmin(emin, lmin);
                 nonspd += (isfinite(lmin) && lmin > 0.0) ? 0 : 1;
-                pmin = fmin(pmin, ph);
-                pmax = fmax(pmax, ph);
-                nonfinite += (isfinite(ph) && isfinite(a) && isfinite(d) && isfinite(b)) ? 0 : 1;
+                pmin = fmin(pmin, P);
+                pmax = fmax(pmax, P);
+                nonfinite += (isfinite(P) && isfinite(A) && isfinite(D) && isfinite(B)) ? 0 : 1;
             }
         }
+    }
     if (!report) return;
     pmin = wave_min(pmin);
     pmax = wave_max(pmax);
@@ -1366,32 +1398,59 @@ struct VHist {
     const double* Iv[3];
 };
 
-// rhs_uv = uv_n + dt * sum(aE E_uv + aI I_uv) + alpha * b_visc
+// rhs_uv = uv_n + dt * sum(aE E_uv + aI I_uv) + alpha * b_visc over the NT
+// active terms (a nonzero coefficient, in stage order); two rows of loads in
+// flight at a time
+template <int NT>
 __global__ void __launch_bounds__(BLOCK) k_combine_faces(Params p, const double* __restrict__ Un,
-                                                         const double* __restrict__ Vn, VHist h, int ns, double dt,
+                                                         const double* __restrict__ Vn, VHist h, double dt,
                                                          double e0, double e1, double e2, double i0, double i1,
                                                          double i2, double alpha, const double* __restrict__ bu,
                                                          const double* __restrict__ bv, double* __restrict__ ru,
                                                          double* __restrict__ rv) {
     const double ce[3] = {e0, e1, e2}, ci[3] = {i0, i1, i2};
     const int i = tile_id().x * TILE_X + threadIdx.x;
-    if (i >= p.nx) return;
-    for (int r = 0; r < ROWS; ++r) {
-        const int j = tile_id().y * TILE_ROWS + threadIdx.y + r * TILE_Y;
-        if (j >= p.ny) return;
-        const size_t k = at(p, i, j);
-        double u = Un[k], v = Vn[k];
-        for (int q = 0; q < ns; ++q) {
-            if (ce[q] == 0.0 && ci[q] == 0.0) continue;
-            u += dt * (ce[q] * h.Eu[q][k] + ci[q] * h.Iu[q][k]);
-            v += dt * (ce[q] * h.Ev[q][k] + ci[q] * h.Iv[q][k]);
+    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
+    constexpr int RH = 2;  // rows per load batch
+#pragma unroll
+    for (int r0 = 0; r0 < ROWS; r0 += RH) {
+        bool in[RH];
+        size_t kk[RH];
+        double u[RH], v[RH], wu[RH], wv[RH], t[RH][NT > 0 ? NT : 1][4];
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+            const int j = jb + (r0 + r) * TILE_Y;
+            in[r] = i < p.nx && j < p.ny;
+            const size_t k = in[r] ? at(p, i, j) : 0;
+            kk[r] = k;
+            u[r] = Un[k];
+            v[r] = Vn[k];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                t[r][q][0] = h.Eu[q][k];
+                t[r][q][1] = h.Iu[q][k];
+                t[r][q][2] = h.Ev[q][k];
+                t[r][q][3] = h.Iv[q][k];
+            }
+            wu[r] = bu ? bu[k] : 0.0;
+            wv[r] = bu ? bv[k] : 0.0;
         }
-        if (bu) {
-            u += alpha * bu[k];
-            v += alpha * bv[k];
+#pragma unroll
+        for (int r = 0; r < RH; ++r) {
+            if (!in[r]) continue;
+            double U = u[r], V = v[r];
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                U += dt * (ce[q] * t[r][q][0] + ci[q] * t[r][q][1]);
+                V += dt * (ce[q] * t[r][q][2] + ci[q] * t[r][q][3]);
+            }
+            if (bu) {
+                U += alpha * wu[r];
+                V += alpha * wv[r];
+            }
+            ru[kk[r]] = U;
+            rv[kk[r]] = V;
         }
-        ru[k] = u;
-        rv[k] = v;
     }
 }
 
@@ -1507,14 +1566,23 @@ void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, c
                           Status* st, int report) {
     Stage base{nullptr, nullptr, phi_n, sxx_n, syy_n, sxy_n, reset};
     Hist h{};
+    double c[3] = {0.0, 0.0, 0.0};
+    int nt = 0;  // the terms with a nonzero coefficient, in stage order
     for (int q = 0; q < ns; ++q) {
-        h.E[q] = E[q];
-        h.Sxx[q] = Sxx[q];
-        h.Syy[q] = Syy[q];
-        h.Sxy[q] = Sxy[q];
+        if (coef[q] == 0.0) continue;
+        h.E[nt] = E[q];
+        h.Sxx[nt] = Sxx[q];
+        h.Syy[nt] = Syy[q];
+        h.Sxy[nt] = Sxy[q];
+        c[nt++] = coef[q];
     }
-    k_combine_cells<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, base, h, ns, coef[0], coef[1], coef[2], phi_s,
-                                                               sxx_s, syy_s, sxy_s, st, report);
+    const dim3 g = grid2d(p), t(TILE_X, TILE_Y);
+    switch (nt) {
+    case 0: k_combine_cells<0><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
+    case 1: k_combine_cells<1><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
+    case 2: k_combine_cells<2><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
+    default: k_combine_cells<3><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
+    }
 }
 
 void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, const double* Vn,
@@ -1522,14 +1590,24 @@ void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, cons
                           const double* const* Iv, int ns, double dt, const double* ce, const double* ci,
                           double alpha, const double* bu, const double* bv, double* ru, double* rv) {
     VHist h{};
+    double e[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
+    int nt = 0;  // the terms with a nonzero coefficient, in stage order
     for (int q = 0; q < ns; ++q) {
-        h.Eu[q] = Eu[q];
-        h.Ev[q] = Ev[q];
-        h.Iu[q] = Iu[q];
-        h.Iv[q] = Iv[q];
+        if (ce[q] == 0.0 && ci[q] == 0.0) continue;
+        h.Eu[nt] = Eu[q];
+        h.Ev[nt] = Ev[q];
+        h.Iu[nt] = Iu[q];
+        h.Iv[nt] = Iv[q];
+        e[nt] = ce[q];
+        c[nt++] = ci[q];
     }
-    k_combine_faces<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, Un, Vn, h, ns, dt, ce[0], ce[1], ce[2], ci[0],
-                                                               ci[1], ci[2], alpha, bu, bv, ru, rv);
+    const dim3 g = grid2d(p), t(TILE_X, TILE_Y);
+    switch (nt) {
+    case 0: k_combine_faces<0><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
+    case 1: k_combine_faces<1><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
+    case 2: k_combine_faces<2><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
+    default: k_combine_faces<3><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
+    }
 }
 
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv) {
