@@ -88,6 +88,73 @@ struct HTile {
     }
 };
 
+// x0 = cur, or the warm start extrapolated in time from the last two
+// (2 cur - prev), three (3 (cur - prev) + old) or four
+// (4 (cur + old) - 6 prev - old2) stage solutions, with the Dirichlet values,
+// at the tile points it * BLOCK + tid, A <= it < B, written to the LDS tile
+template <int A, int B, int N2>
+__device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx2, const int* fl, int tid, double gul,
+                                         double gvb, double* sxu, double* sxv) {
+    constexpr int M = B - A;
+    double xu[M], xv[M];
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+        xu[t] = b.Un[idx2[A + t]];
+        xv[t] = b.Vn[idx2[A + t]];
+    }
+    if (b.extrap == 1) {
+        double yu[M], yv[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            yu[t] = b.Pu[idx2[A + t]];
+            yv[t] = b.Pv[idx2[A + t]];
+        }
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            xu[t] = 2.0 * xu[t] - yu[t];
+            xv[t] = 2.0 * xv[t] - yv[t];
+        }
+    } else if (b.extrap == 2) {
+        double yu[M], yv[M], zu[M], zv[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            yu[t] = b.Pu[idx2[A + t]];
+            yv[t] = b.Pv[idx2[A + t]];
+            zu[t] = b.Ou[idx2[A + t]];
+            zv[t] = b.Ov[idx2[A + t]];
+        }
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            xu[t] = 3.0 * (xu[t] - yu[t]) + zu[t];
+            xv[t] = 3.0 * (xv[t] - yv[t]) + zv[t];
+        }
+    } else if (b.extrap == 3) {
+        double yu[M], yv[M], zu[M], zv[M], wu[M], wv[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            yu[t] = b.Pu[idx2[A + t]];
+            yv[t] = b.Pv[idx2[A + t]];
+            zu[t] = b.Ou[idx2[A + t]];
+            zv[t] = b.Ov[idx2[A + t]];
+            wu[t] = b.Wu[idx2[A + t]];
+            wv[t] = b.Wv[idx2[A + t]];
+        }
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            xu[t] = (4.0 * (xu[t] + zu[t]) - 6.0 * yu[t]) - wu[t];
+            xv[t] = (4.0 * (xv[t] + zv[t]) - 6.0 * yv[t]) - wv[t];
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < M; ++t) {
+        const int k = tid + (A + t) * BLOCK;
+        if (k >= N2) break;
+        const int f = fl[A + t];
+        sxu[k] = (f & 1) ? ((f & 2) ? gul : xu[t]) : 0.0;
+        sxv[k] = (f & 1) ? ((f & 4) ? gvb : xv[t]) : 0.0;
+    }
+}
+
 // Start of a predictor solve: x0 = the warm start with the Dirichlet
 // values, r0 = rhs - (I - alpha M) x0 (zero on Dirichlet rows), p0 = r0,
 // q0 = (I - alpha M) p0, and the partials of |r0|^2, |b_F|^2 (free rows) and
@@ -95,7 +162,7 @@ struct HTile {
 // tile and r0 on a halo-1 tile, so both stencils read LDS.
 template <bool IN>
 __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b, double alpha, int nb, int i0, int j0,
-                                               int tid, double* sxu, double* sxv, double* sru, double* srv) {
+                                               int tid, double* sxu, double* sxv) {
     using T2 = HTile<2, IN>;
     using T1 = HTile<1, IN>;
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
@@ -113,10 +180,7 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         }
         return true;
     };
-    // x0 on halo 2 and the right-hand side on halo 1: indices, then every
-    // load back to back (one latency for both tiles).  x0 = cur, or the
-    // warm start extrapolated in time from the last two (2 cur - prev) or
-    // three (3 (cur - prev) + old) stage solutions.
+    // x0 on halo 2 and the right-hand side on halo 1: indices first
     constexpr int N2 = T2::R * T2::W, NI2 = (N2 + BLOCK - 1) / BLOCK;
     constexpr int N1 = T1::R * T1::W, NI1 = (N1 + BLOCK - 1) / BLOCK;
     size_t idx2[NI2], idx[NI1];
@@ -137,71 +201,23 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         inb[it] = wrapped(i0 + k % T1::W - 1, j0 + k / T1::W - 1, gi, gj);
         idx[it] = inb[it] ? at(p, gi, gj) : 0;
     }
-    double xu[NI2], xv[NI2], bu[NI1], bv[NI1];
-#pragma unroll
-    for (int it = 0; it < NI2; ++it) {
-        xu[it] = b.Un[idx2[it]];
-        xv[it] = b.Vn[idx2[it]];
-    }
+    // the right-hand side first (needed after the x0 tile is built), then x0
+    // in two batches of tile points (half the registers in flight)
+    double bu[NI1], bv[NI1];
 #pragma unroll
     for (int it = 0; it < NI1; ++it) {
         bu[it] = b.ru[idx[it]];
         bv[it] = b.rv[idx[it]];
     }
-    if (b.extrap == 1) {
-        double yu[NI2], yv[NI2];
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            yu[it] = b.Pu[idx2[it]];
-            yv[it] = b.Pv[idx2[it]];
-        }
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            xu[it] = 2.0 * xu[it] - yu[it];
-            xv[it] = 2.0 * xv[it] - yv[it];
-        }
-    } else if (b.extrap == 2) {
-        double yu[NI2], yv[NI2], zu[NI2], zv[NI2];
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            yu[it] = b.Pu[idx2[it]];
-            yv[it] = b.Pv[idx2[it]];
-            zu[it] = b.Ou[idx2[it]];
-            zv[it] = b.Ov[idx2[it]];
-        }
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            xu[it] = 3.0 * (xu[it] - yu[it]) + zu[it];
-            xv[it] = 3.0 * (xv[it] - yv[it]) + zv[it];
-        }
-    } else if (b.extrap == 3) {
-        double yu[NI2], yv[NI2], zu[NI2], zv[NI2], wu[NI2], wv[NI2];
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            yu[it] = b.Pu[idx2[it]];
-            yv[it] = b.Pv[idx2[it]];
-            zu[it] = b.Ou[idx2[it]];
-            zv[it] = b.Ov[idx2[it]];
-            wu[it] = b.Wu[idx2[it]];
-            wv[it] = b.Wv[idx2[it]];
-        }
-#pragma unroll
-        for (int it = 0; it < NI2; ++it) {
-            xu[it] = (4.0 * (xu[it] + zu[it]) - 6.0 * yu[it]) - wu[it];
-            xv[it] = (4.0 * (xv[it] + zv[it]) - 6.0 * yv[it]) - wv[it];
-        }
-    }
-#pragma unroll
-    for (int it = 0; it < NI2; ++it) {
-        const int k = tid + it * BLOCK;
-        if (k >= N2) break;
-        sxu[k] = (fl[it] & 1) ? ((fl[it] & 2) ? gul : xu[it]) : 0.0;
-        sxv[k] = (fl[it] & 1) ? ((fl[it] & 4) ? gvb : xv[it]) : 0.0;
-    }
+    x0_batch<0, NI2 / 2, N2>(b, idx2, fl, tid, gul, gvb, sxu, sxv);
+    x0_batch<NI2 / 2, NI2, N2>(b, idx2, fl, tid, gul, gvb, sxu, sxv);
     __syncthreads();
     const T2 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
-    // r0 on halo 1; |r0|^2 and |b_F|^2 over the block's own cells
+    // r0 on halo 1; |r0|^2 and |b_F|^2 over the block's own cells.  r0 is
+    // held in registers until every read of the x0 tile is done, then takes
+    // its LDS (the block needs one tile of LDS, not two)
     double acc2[2] = {0.0, 0.0};
+    double r0u[NI1], r0v[NI1];
     {
 #pragma unroll
         for (int it = 0; it < NI1; ++it) {
@@ -233,12 +249,20 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
                     acc2[0] += ru * ru + rv * rv;
                 }
             }
-            sru[k] = ru;
-            srv[k] = rv;
+            r0u[it] = ru;
+            r0v[it] = rv;
         }
     }
     __syncthreads();
-    const T1 Ru{sru, i0, j0, p.nx, p.ny}, Rv{srv, i0, j0, p.nx, p.ny};
+#pragma unroll
+    for (int it = 0; it < NI1; ++it) {
+        const int k = tid + it * BLOCK;
+        if (k >= N1) break;
+        sxu[k] = r0u[it];
+        sxv[k] = r0v[it];
+    }
+    __syncthreads();
+    const T1 Ru{sxu, i0, j0, p.nx, p.ny}, Rv{sxv, i0, j0, p.nx, p.ny};
     double acc4[4] = {0.0, 0.0, 0.0, 0.0};
     const int i = i0 + threadIdx.x;
     if (i < p.nx)
@@ -262,8 +286,8 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
 
 __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
     constexpr int N2 = HTile<2>::R * HTile<2>::W, N1 = HTile<1>::R * HTile<1>::W;
+    static_assert(N1 <= N2, "the r0 tile reuses the x0 tile");
     __shared__ double sxu[N2], sxv[N2];
-    __shared__ double sru[N1], srv[N1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
     const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
@@ -271,8 +295,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
     // every point whose operator row the block evaluates (halo 1 included)
     // has its stencil inside the domain: no boundary rows, no wraps
     const bool inner = i0 - 1 >= 1 && i0 + TILE_X <= p.nx - 2 && j0 - 1 >= 1 && j0 + TILE_ROWS <= p.ny - 2;
-    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, sru, srv);
-    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, sru, srv);
+    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
+    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
 }
 
 // One CG iteration m >= 1 with a single reduction phase: the totals of
