@@ -107,6 +107,8 @@ struct fsi_workspace {
     double *Eu[3] = {}, *Ev[3] = {}, *Iu[3] = {}, *Iv[3] = {};
     double *Ep[3] = {}, *Exx[3] = {}, *Eyy[3] = {}, *Exy[3] = {};
     double *ru = nullptr, *rv = nullptr, *r_u[2] = {}, *r_v[2] = {};
+    // velocity right-hand sides of stages 2..4, summed by the momentum kernels
+    double *rhs_u[3] = {}, *rhs_v[3] = {};
     double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
     double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
     int corr_cur = 0;
@@ -803,7 +805,8 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->vel_old_v[2],  &w->vel_old2_u[0], &w->vel_old2_u[1], &w->vel_old2_u[2],
                           &w->vel_old2_v[0], &w->vel_old2_v[1], &w->vel_old2_v[2], &w->vel_next_u[0],
                           &w->vel_next_u[1], &w->vel_next_u[2], &w->vel_next_v[0], &w->vel_next_v[1],
-                          &w->vel_next_v[2]};
+                          &w->vel_next_v[2], &w->rhs_u[0], &w->rhs_u[1], &w->rhs_u[2], &w->rhs_v[0],
+                          &w->rhs_v[1], &w->rhs_v[2]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -912,15 +915,15 @@ static fsi_status poll_done(fsi_context* c, const KrylovScalars* dev, int* done,
 
 // fixed > 0: enqueue exactly `fixed` iteration kernels without reading the
 // residual back (speculative step); otherwise poll until converged.
-// The predictor of stage `slot` into (xu, xv).
+// The predictor of stage `slot` from the right-hand side (ru, rv) into (xu, xv).
 static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot,
-                                 int fixed, double* xu, double* xv) {
+                                 int fixed, const double* ru, const double* rv, double* xu, double* xv) {
     const Params& p = c->prm;
     VelBufs b{};
     b.Un = s->U;
     b.Vn = s->V;
-    b.ru = w->ru;
-    b.rv = w->rv;
+    b.ru = ru;
+    b.rv = rv;
     b.xu = xu;
     b.xv = xv;
     for (int k = 0; k < 2; ++k) {
@@ -1111,12 +1114,18 @@ extern "C" fsi_status fsi_project(fsi_context* c, fsi_state* s, fsi_workspace* w
     return project_state(c, s, w, res);
 }
 
+// The tendencies of stage `slot`; with `sums` the momentum kernel adds its
+// part to the velocity right-hand sides of the later stages instead of
+// storing E_u, E_v, I_u, I_v.
 static void tendencies(fsi_context* c, fsi_workspace* w, int slot, const double* U, const double* V,
                        const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                       int stage1) {
+                       int stage1, const RhsSums* sums = nullptr) {
     const Params& p = c->prm;
-    launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, w->Eu[slot], w->Ev[slot], w->Iu[slot],
-                    w->Iv[slot]);
+    if (sums)
+        launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, nullptr, nullptr, nullptr, nullptr, sums);
+    else
+        launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, w->Eu[slot], w->Ev[slot], w->Iu[slot],
+                        w->Iv[slot]);
     launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], w->Exx[slot], w->Eyy[slot],
                  w->Exy[slot], c->st, stage1);
 }
@@ -1150,32 +1159,49 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
     const Params& p = c->prm;
     const int reset = c->has_reset;
     fsi_status rc = FSI_OK;
+    // the velocity right-hand sides (_build_velocity_stage!) are summed by
+    // the momentum kernels of stages q = 0..2 (terms aE[s][q] E_q + aI[s][q]
+    // I_q for the later stages s, the lifting alpha_s b at s = q + 1)
+    auto sums_of = [&](int q) {
+        RhsSums R{};
+        for (int t = 0; t < 3; ++t) {
+            R.ru[t] = w->rhs_u[t];
+            R.rv[t] = w->rhs_v[t];
+            R.ce[t] = t >= q ? AE[t + 1][q] : 0.0;
+            R.ci[t] = t >= q ? AI[t + 1][q] : 0.0;
+        }
+        R.alpha = dt * AI[q + 1][q + 1];
+        R.dt = dt;
+        R.bu = p.visc ? c->bvisc_u : nullptr;
+        R.bv = p.visc ? c->bvisc_v : nullptr;
+        R.q = q;
+        return R;
+    };
     // stage 1: the state itself (with the safety reset on sigma_n), tendencies
-    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1);
+    RhsSums sums = sums_of(0);
+    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1, &sums);
     for (int si = 1; si < 4; ++si) {
         // phi_s, sigma_s (_build_explicit_stage!)
         double coef[3] = {0, 0, 0};
         for (int q = 0; q < si; ++q) coef[q] = dt * AE[si][q];
         launch_combine_cells(c->stream, p, s->phi, s->sxx, s->syy, s->sxy, reset, w->Ep, w->Exx, w->Eyy, w->Exy, si,
                              coef, w->phis, w->sxxs, w->syys, w->sxys, c->st, si == 3);
-        // velocity right-hand side and predictor (_build_velocity_stage!)
-        double ce[3] = {0, 0, 0}, ci[3] = {0, 0, 0};
-        for (int q = 0; q < si; ++q) {
-            ce[q] = AE[si][q];
-            ci[q] = AI[si][q];
-        }
+        // the predictor from the summed right-hand side
         const double alpha = dt * AI[si][si];
-        launch_combine_faces(c->stream, p, s->U, s->V, w->Eu, w->Ev, w->Iu, w->Iv, si, dt, ce, ci, alpha,
-                             p.visc ? c->bvisc_u : nullptr, p.visc ? c->bvisc_v : nullptr, w->ru, w->rv);
         const int k = si - 1;
         // with the warm-start history the predictor is kept (the next step
         // extrapolates from it) and the projection writes the stage velocity
         const bool keep = c->stage_warm && p.visc;
         double* xu = keep ? w->vel_next_u[k] : w->Us;
         double* xv = keep ? w->vel_next_v[k] : w->Vs;
-        if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0, xu, xv)) != FSI_OK) return rc;
+        if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0, w->rhs_u[k], w->rhs_v[k], xu,
+                                 xv)) != FSI_OK)
+            return rc;
         if ((rc = projection(c, w, xu, xv, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
-        if (si <= 2) tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0);
+        if (si <= 2) {
+            sums = sums_of(si);
+            tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0, &sums);
+        }
     }
     launch_ext_final(c->stream, p, c->st);  // the report of the last stage and the stage-1 checks
     return FSI_OK;

@@ -42,9 +42,27 @@ struct CoarsePlan {
 };
 
 // ---- physics (fsi_physics.hip) ----
+// Running sums of the velocity right-hand sides of the later stages
+// (_build_velocity_stage!, TimeIntegrationIMEX.jl:778-800): the momentum
+// kernel of stage q adds dt (aE[s][q] E_q + aI[s][q] I_q) to the sum of every
+// stage s > q (starting from u^n at q = 0, a term skipped when both
+// coefficients are zero) and, at s = q + 1, the viscous lifting
+// alpha_s b_visc; the sum of stage s is then that stage's right-hand side,
+// its terms added in the same order as a separate combination would.
+struct RhsSums {
+    double* ru[3];  // stages s = 1..3 at index s - 1
+    double* rv[3];
+    double ce[3], ci[3];  // aE[s][q], aI[s][q] at index s - 1 (s > q)
+    double alpha;         // dt aI[q+1][q+1]
+    double dt;
+    const double* bu;  // b_visc, or null without viscosity
+    const double* bv;
+    int q;  // the stage of the tendencies (0..2); -1: no sums
+};
+// E_u, E_v, I_u, I_v may be null (not stored)
 void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
                      const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
-                     double* Iu, double* Iv);
+                     double* Iu, double* Iv, const RhsSums* sums = nullptr);
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
                   const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
                   double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1);
@@ -53,10 +71,6 @@ void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, c
                           const double* const* Sxx, const double* const* Syy, const double* const* Sxy, int ns,
                           const double* coef, double* phi_s, double* sxx_s, double* syy_s, double* sxy_s,
                           Status* st, int report);
-void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, const double* Vn,
-                          const double* const* Eu, const double* const* Ev, const double* const* Iu,
-                          const double* const* Iv, int ns, double dt, const double* ce, const double* ci,
-                          double alpha, const double* bu, const double* bv, double* ru, double* rv);
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv);
 // StepReport extrema and counts, and the stage-1 checks, from the per-block
 // partials of the step's last combination and of its stage-1 tendencies

@@ -221,8 +221,8 @@ __device__ __forceinline__ double elastic_fy(const Params& p, const FT& Tyy, con
 template <bool IN>
 __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, double* __restrict__ Eu,
                                               double* __restrict__ Ev, double* __restrict__ Iu,
-                                              double* __restrict__ Iv, int i0, int j0, double* su, double* sv,
-                                              double* sxx, double* syy, double* sxy) {
+                                              double* __restrict__ Iv, const RhsSums& R, int i0, int j0,
+                                              double* su, double* sv, double* sxx, double* syy, double* sxy) {
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const bool elastic = p.elastic != 0;
     constexpr int NPT = MT_H * MT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
@@ -287,6 +287,23 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
     for (int r = 0; r < ROWS; ++r) {
         const int j = j0 + threadIdx.y + r * TILE_Y;
         if (j >= p.ny) return;
+        const size_t k = at(p, i, j);
+        // the running sums this row updates (stages q+1.., at most two read
+        // back for q >= 1) and the lifting b_visc, loaded before the face
+        // formulas so that their latency overlaps them
+        double su_old[2] = {0.0, 0.0}, sv_old[2] = {0.0, 0.0}, lbu = 0.0, lbv = 0.0;
+        if (R.q >= 1) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+                if (R.q + t < 3) {
+                    su_old[t] = R.ru[R.q + t][k];
+                    sv_old[t] = R.rv[R.q + t][k];
+                }
+        }
+        if (R.q >= 0 && R.bu) {
+            lbu = R.bu[k];
+            lbv = R.bv[k];
+        }
         // ---- x-momentum at x-face (i, j)
         double cu;
         if (!IN && !p.px && i == 0) cu = 0.0;  // wall-normal row (MomentumTransport.jl:322-327)
@@ -310,12 +327,34 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
         double ev = -cv;
         if (p.force == FSI_FORCE_CONSTANT) ev += p.ffy;
         if (elastic) ev += elastic_fy<IN>(p, Tyy, Txy, i, j);
-        const size_t k = at(p, i, j);
-        Eu[k] = eu;
-        Ev[k] = ev;
+        if (Eu) {
+            Eu[k] = eu;
+            Ev[k] = ev;
+        }
+        double iu = 0.0, iv = 0.0;
+        if (Iu || R.q >= 0) {
+            iu = p.visc ? visc_u<IN>(p, U, V, i, j, 1) : 0.0;
+            iv = p.visc ? visc_v<IN>(p, U, V, i, j, 1) : 0.0;
+        }
         if (Iu) {
-            Iu[k] = p.visc ? visc_u<IN>(p, U, V, i, j, 1) : 0.0;
-            Iv[k] = p.visc ? visc_v<IN>(p, U, V, i, j, 1) : 0.0;
+            Iu[k] = iu;
+            Iv[k] = iv;
+        }
+        if (R.q >= 0) {  // the right-hand sides of the later stages
+            for (int t = R.q; t < 3; ++t) {
+                double ru = R.q == 0 ? U(i, j) : su_old[(t - R.q) & 1];
+                double rv = R.q == 0 ? V(i, j) : sv_old[(t - R.q) & 1];
+                if (R.ce[t] != 0.0 || R.ci[t] != 0.0) {
+                    ru += R.dt * (R.ce[t] * eu + R.ci[t] * iu);
+                    rv += R.dt * (R.ce[t] * ev + R.ci[t] * iv);
+                }
+                if (t == R.q && R.bu) {
+                    ru += R.alpha * lbu;
+                    rv += R.alpha * lbv;
+                }
+                R.ru[t][k] = ru;
+                R.rv[t][k] = rv;
+            }
         }
     }
 }
@@ -323,7 +362,7 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
 // (47.5 KB of LDS: at most 3 waves per SIMD, so registers up to that occupancy)
 __global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 3)))
 k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, double* __restrict__ Iu,
-           double* __restrict__ Iv) {
+           double* __restrict__ Iv, RhsSums R) {
     __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
     __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
     const TileId T = tile_id();
@@ -331,8 +370,8 @@ k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, 
     // every face of the tile at least one cell inside the walls: no wall
     // row, no wrap, no boundary lifting
     const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
-    if (inner) momentum_tile<true>(p, s, Eu, Ev, Iu, Iv, i0, j0, su, sv, sxx, syy, sxy);
-    else momentum_tile<false>(p, s, Eu, Ev, Iu, Iv, i0, j0, su, sv, sxx, syy, sxy);
+    if (inner) momentum_tile<true>(p, s, Eu, Ev, Iu, Iv, R, i0, j0, su, sv, sxx, syy, sxy);
+    else momentum_tile<false>(p, s, Eu, Ev, Iu, Iv, R, i0, j0, su, sv, sxx, syy, sxy);
 }
 
 // ---------------------------------------------------------------------------
@@ -1391,69 +1430,6 @@ __global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, H
     }
 }
 
-struct VHist {
-    const double* Eu[3];
-    const double* Ev[3];
-    const double* Iu[3];
-    const double* Iv[3];
-};
-
-// rhs_uv = uv_n + dt * sum(aE E_uv + aI I_uv) + alpha * b_visc over the NT
-// active terms (a nonzero coefficient, in stage order); two rows of loads in
-// flight at a time
-template <int NT>
-__global__ void __launch_bounds__(BLOCK) k_combine_faces(Params p, const double* __restrict__ Un,
-                                                         const double* __restrict__ Vn, VHist h, double dt,
-                                                         double e0, double e1, double e2, double i0, double i1,
-                                                         double i2, double alpha, const double* __restrict__ bu,
-                                                         const double* __restrict__ bv, double* __restrict__ ru,
-                                                         double* __restrict__ rv) {
-    const double ce[3] = {e0, e1, e2}, ci[3] = {i0, i1, i2};
-    const int i = tile_id().x * TILE_X + threadIdx.x;
-    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
-    constexpr int RH = 2;  // rows per load batch
-#pragma unroll
-    for (int r0 = 0; r0 < ROWS; r0 += RH) {
-        bool in[RH];
-        size_t kk[RH];
-        double u[RH], v[RH], wu[RH], wv[RH], t[RH][NT > 0 ? NT : 1][4];
-#pragma unroll
-        for (int r = 0; r < RH; ++r) {
-            const int j = jb + (r0 + r) * TILE_Y;
-            in[r] = i < p.nx && j < p.ny;
-            const size_t k = in[r] ? at(p, i, j) : 0;
-            kk[r] = k;
-            u[r] = Un[k];
-            v[r] = Vn[k];
-#pragma unroll
-            for (int q = 0; q < NT; ++q) {
-                t[r][q][0] = h.Eu[q][k];
-                t[r][q][1] = h.Iu[q][k];
-                t[r][q][2] = h.Ev[q][k];
-                t[r][q][3] = h.Iv[q][k];
-            }
-            wu[r] = bu ? bu[k] : 0.0;
-            wv[r] = bu ? bv[k] : 0.0;
-        }
-#pragma unroll
-        for (int r = 0; r < RH; ++r) {
-            if (!in[r]) continue;
-            double U = u[r], V = v[r];
-#pragma unroll
-            for (int q = 0; q < NT; ++q) {
-                U += dt * (ce[q] * t[r][q][0] + ci[q] * t[r][q][1]);
-                V += dt * (ce[q] * t[r][q][2] + ci[q] * t[r][q][3]);
-            }
-            if (bu) {
-                U += alpha * wu[r];
-                V += alpha * wv[r];
-            }
-            ru[kk[r]] = U;
-            rv[kk[r]] = V;
-        }
-    }
-}
-
 // b_visc = M*0 + b (the affine part of the viscous operator), computed once.
 __global__ void __launch_bounds__(BLOCK) k_visc_boundary(Params p, const double* zero, double* bu, double* bv) {
     const int i = tile_id().x * TILE_X + threadIdx.x;
@@ -1538,9 +1514,12 @@ void launch_ext_final(hipStream_t s, const Params& p, Status* st) { k_ext_final<
 
 void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
                      const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
-                     double* Iu, double* Iv) {
+                     double* Iu, double* Iv, const RhsSums* sums) {
     Stage st{U, V, phi, sxx, syy, sxy, reset};
-    k_momentum<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, st, Eu, Ev, Iu, Iv);
+    RhsSums R{};
+    R.q = -1;
+    if (sums) R = *sums;
+    k_momentum<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, st, Eu, Ev, Iu, Iv, R);
 }
 
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
@@ -1582,31 +1561,6 @@ void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, c
     case 1: k_combine_cells<1><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
     case 2: k_combine_cells<2><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
     default: k_combine_cells<3><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
-    }
-}
-
-void launch_combine_faces(hipStream_t s, const Params& p, const double* Un, const double* Vn,
-                          const double* const* Eu, const double* const* Ev, const double* const* Iu,
-                          const double* const* Iv, int ns, double dt, const double* ce, const double* ci,
-                          double alpha, const double* bu, const double* bv, double* ru, double* rv) {
-    VHist h{};
-    double e[3] = {0.0, 0.0, 0.0}, c[3] = {0.0, 0.0, 0.0};
-    int nt = 0;  // the terms with a nonzero coefficient, in stage order
-    for (int q = 0; q < ns; ++q) {
-        if (ce[q] == 0.0 && ci[q] == 0.0) continue;
-        h.Eu[nt] = Eu[q];
-        h.Ev[nt] = Ev[q];
-        h.Iu[nt] = Iu[q];
-        h.Iv[nt] = Iv[q];
-        e[nt] = ce[q];
-        c[nt++] = ci[q];
-    }
-    const dim3 g = grid2d(p), t(TILE_X, TILE_Y);
-    switch (nt) {
-    case 0: k_combine_faces<0><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
-    case 1: k_combine_faces<1><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
-    case 2: k_combine_faces<2><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
-    default: k_combine_faces<3><<<g, t, 0, s>>>(p, Un, Vn, h, dt, e[0], e[1], e[2], c[0], c[1], c[2], alpha, bu, bv, ru, rv); break;
     }
 }
 
