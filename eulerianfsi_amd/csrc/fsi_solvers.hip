@@ -242,10 +242,8 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
                 if (own) {
                     b.xu[kk] = Xu(gi, gj);
                     b.xv[kk] = Xv(gi, gj);
-                    b.r_u[0][kk] = ru;
+                    b.r_u[0][kk] = ru;  // p0 = r0: the first iteration reads it from r
                     b.r_v[0][kk] = rv;
-                    b.p_u[0][kk] = ru;
-                    b.p_v[0][kk] = rv;
                     acc2[0] += ru * ru + rv * rv;
                 }
             }
@@ -334,7 +332,9 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
         }
     }
     const int o = m & 1, n = (m + 1) & 1;
-    const double *ru = b.r_u[o], *rv = b.r_v[o], *pu0 = b.p_u[o], *pv0 = b.p_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
+    // p0 = r0 is not stored: the first iteration reads p from r
+    const double *ru = b.r_u[o], *rv = b.r_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
+    const double *pu0 = m == 0 ? ru : b.p_u[o], *pv0 = m == 0 ? rv : b.p_v[o];
     __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
