@@ -32,10 +32,11 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 # Algorithmic HBM passes per step, in fields of N^2 doubles.
 def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
-    """Passes this implementation makes per step: tendencies 77 (momentum 6
+    """Passes this implementation makes per step: tendencies 107 (momentum 6
     reads per stage plus the running sums of the later stages' velocity
-    right-hand sides, 8 + 8 + 4; phase 3 x 4; sigma 3 x 9), stage combinations
-    of phi and sigma 44 (12 + 16 + 16); per velocity solve 18 (start: right-hand
+    right-hand sides, 8 + 8 + 4; phase 3 x 4; sigma 7 reads per stage plus the
+    running sums of the later stages' phi and sigma, 12 + 16 + 8; the stage
+    combinations happen inside these); per velocity solve 18 (start: right-hand
     side and four history fields read, x r p q written) + 16 per CG
     iteration; per projection 14 (rhs 3, the
     tested solution 2, update 5, check 4) + 6 for the direct transform solve
@@ -43,7 +44,7 @@ def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
     PCG iteration when the multigrid-PCG solves it."""
     cell = 8.0 * N * N
     per_proj = 14 + 6 if direct else 17
-    passes = 77 + 44 + sum(18 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
+    passes = 107 + sum(18 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
     return passes * cell
 
 

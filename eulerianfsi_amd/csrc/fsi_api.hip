@@ -109,6 +109,9 @@ struct fsi_workspace {
     double *ru = nullptr, *rv = nullptr, *r_u[2] = {}, *r_v[2] = {};
     // velocity right-hand sides of stages 2..4, summed by the momentum kernels
     double *rhs_u[3] = {}, *rhs_v[3] = {};
+    // phi and sigma of stages 2 and 3, summed by the sigma kernels (stage 4
+    // goes to phis, sxxs, syys, sxys, which the state swaps with)
+    double *cphi[2] = {}, *csxx[2] = {}, *csyy[2] = {}, *csxy[2] = {};
     double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
     double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
     int corr_cur = 0;
@@ -806,7 +809,8 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->vel_old2_v[0], &w->vel_old2_v[1], &w->vel_old2_v[2], &w->vel_next_u[0],
                           &w->vel_next_u[1], &w->vel_next_u[2], &w->vel_next_v[0], &w->vel_next_v[1],
                           &w->vel_next_v[2], &w->rhs_u[0], &w->rhs_u[1], &w->rhs_u[2], &w->rhs_v[0],
-                          &w->rhs_v[1], &w->rhs_v[2]};
+                          &w->rhs_v[1], &w->rhs_v[2], &w->cphi[0], &w->cphi[1], &w->csxx[0], &w->csxx[1],
+                          &w->csyy[0], &w->csyy[1], &w->csxy[0], &w->csxy[1]};
     for (auto p : singles) HIPCHECK(c, A(p, F));
     for (int s = 0; s < 3; ++s) {
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
@@ -1114,20 +1118,25 @@ extern "C" fsi_status fsi_project(fsi_context* c, fsi_state* s, fsi_workspace* w
     return project_state(c, s, w, res);
 }
 
-// The tendencies of stage `slot`; with `sums` the momentum kernel adds its
-// part to the velocity right-hand sides of the later stages instead of
-// storing E_u, E_v, I_u, I_v.
+// The tendencies of stage `slot`; with `sums` / `csums` the momentum and
+// sigma kernels add their parts to the velocity right-hand sides and to the
+// phi, sigma stage values of the later stages instead of storing E_u, E_v,
+// I_u, I_v and E_sigma (E_phi is stored for the sigma kernel).
 static void tendencies(fsi_context* c, fsi_workspace* w, int slot, const double* U, const double* V,
                        const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                       int stage1, const RhsSums* sums = nullptr) {
+                       int stage1, const RhsSums* sums = nullptr, const CellSums* csums = nullptr) {
     const Params& p = c->prm;
     if (sums)
         launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, nullptr, nullptr, nullptr, nullptr, sums);
     else
         launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, w->Eu[slot], w->Ev[slot], w->Iu[slot],
                         w->Iv[slot]);
-    launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], w->Exx[slot], w->Eyy[slot],
-                 w->Exy[slot], c->st, stage1);
+    if (csums)
+        launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], nullptr, nullptr, nullptr,
+                     c->st, stage1, csums);
+    else
+        launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], w->Exx[slot],
+                     w->Eyy[slot], w->Exy[slot], c->st, stage1);
 }
 
 extern "C" fsi_status fsi_eval_tendencies(fsi_context* c, const fsi_state* s, fsi_workspace* w, double t,
@@ -1177,15 +1186,31 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         R.q = q;
         return R;
     };
+    // phi, sigma of stage s (_build_explicit_stage!) are summed by the sigma
+    // kernels of stages q < s; stage 4's values land in phis .. sxys
+    double* const sphi[3] = {w->cphi[0], w->cphi[1], w->phis};
+    double* const ssxx[3] = {w->csxx[0], w->csxx[1], w->sxxs};
+    double* const ssyy[3] = {w->csyy[0], w->csyy[1], w->syys};
+    double* const ssxy[3] = {w->csxy[0], w->csxy[1], w->sxys};
+    auto cells_of = [&](int q) {
+        CellSums C{};
+        for (int t = 0; t < 3; ++t) {
+            C.phi[t] = sphi[t];
+            C.sxx[t] = ssxx[t];
+            C.syy[t] = ssyy[t];
+            C.sxy[t] = ssxy[t];
+            C.cf[t] = t >= q ? dt * AE[t + 1][q] : 0.0;
+        }
+        C.Ephi = w->Ep[q];
+        C.reset = reset;
+        C.q = q;
+        return C;
+    };
     // stage 1: the state itself (with the safety reset on sigma_n), tendencies
     RhsSums sums = sums_of(0);
-    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1, &sums);
+    CellSums csums = cells_of(0);
+    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1, &sums, &csums);
     for (int si = 1; si < 4; ++si) {
-        // phi_s, sigma_s (_build_explicit_stage!)
-        double coef[3] = {0, 0, 0};
-        for (int q = 0; q < si; ++q) coef[q] = dt * AE[si][q];
-        launch_combine_cells(c->stream, p, s->phi, s->sxx, s->syy, s->sxy, reset, w->Ep, w->Exx, w->Eyy, w->Exy, si,
-                             coef, w->phis, w->sxxs, w->syys, w->sxys, c->st, si == 3);
         // the predictor from the summed right-hand side
         const double alpha = dt * AI[si][si];
         const int k = si - 1;
@@ -1200,7 +1225,9 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         if ((rc = projection(c, w, xu, xv, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
         if (si <= 2) {
             sums = sums_of(si);
-            tendencies(c, w, si, w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, 0, 0, &sums);
+            csums = cells_of(si);
+            const int t = si - 1;
+            tendencies(c, w, si, w->Us, w->Vs, sphi[t], ssxx[t], ssyy[t], ssxy[t], 0, 0, &sums, &csums);
         }
     }
     launch_ext_final(c->stream, p, c->st);  // the report of the last stage and the stage-1 checks

@@ -63,14 +63,28 @@ struct RhsSums {
 void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
                      const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
                      double* Iu, double* Iv, const RhsSums* sums = nullptr);
+// Running sums of the explicit stage values of phi and sigma
+// (_build_explicit_stage!, TimeIntegrationIMEX.jl:716-740): the sigma kernel
+// of stage q adds dt aE[s][q] (E_phi, E_sigma) of stage q to the sum of every
+// later stage s (from phi^n and sigma^n with the safety reset at q = 0, a
+// term skipped when its coefficient is zero); at s = q + 1 the sum is final:
+// the safety reset where phi_s < cutoff and, for s = 3, the StepReport
+// partials.  The terms are added in the order a separate combination would.
+struct CellSums {
+    double* phi[3];  // stages s = 1..3 at index s - 1
+    double* sxx[3];
+    double* syy[3];
+    double* sxy[3];
+    const double* Ephi;  // E_phi of stage q (written by the phase kernel)
+    double cf[3];        // dt aE[s][q] at index s - 1 (s > q)
+    int reset;           // the safety reset of the stage values
+    int q;               // the stage of the tendencies (0..2); -1: no sums
+};
+// with `sums`, E_sigma is not stored (Exx, Eyy, Exy may be null)
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
                   const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1);
-void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, const double* sxx_n,
-                          const double* syy_n, const double* sxy_n, int reset, const double* const* E,
-                          const double* const* Sxx, const double* const* Syy, const double* const* Sxy, int ns,
-                          const double* coef, double* phi_s, double* sxx_s, double* syy_s, double* sxy_s,
-                          Status* st, int report);
+                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1,
+                  const CellSums* sums = nullptr);
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv);
 // StepReport extrema and counts, and the stage-1 checks, from the per-block
 // partials of the step's last combination and of its stage-1 tendencies

@@ -1192,10 +1192,12 @@ struct SymTile {
 template <bool IN>
 __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, double* __restrict__ Exx,
                                            double* __restrict__ Eyy, double* __restrict__ Exy, Status* st, int stage1,
-                                           int i0, int j0, double* su, double* sv, double* sxx, double* syy,
-                                           double* sxy) {
+                                           const CellSums& C, int i0, int j0, double* su, double* sv, double* sxx,
+                                           double* syy, double* sxy) {
     __shared__ double lds[16];
     __shared__ long long lds_n[16];
+    __shared__ double lds_r[4][16];
+    __shared__ long long lds_rn[2][16];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     constexpr int NPT = MT_H * MT_W, NIT = (NPT + BLOCK - 1) / BLOCK;
     size_t ku[NIT], kv[NIT], kc[NIT];
@@ -1243,6 +1245,11 @@ __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, doub
     const SymTile<IN> S{sxx, syy, sxy, i0, j0, p.nx, p.ny};
     double dmax = 0.0;
     long long nonfinite = 0;
+    // StepReport partials of the final stage (structure_tensor_diagnostics
+    // StructureTensor.jl:627-662), when this launch completes stage 3
+    const bool report = C.q == 2;
+    double pmin = INFINITY, pmax = -INFINITY, emin = INFINITY, demax = 0.0;
+    long long nonspd = 0, nonfin = 0;
     const int i = i0 + threadIdx.x;
     if (i < p.nx)
 #pragma unroll 1
@@ -1250,11 +1257,71 @@ __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, doub
             const int j = j0 + threadIdx.y + r * TILE_Y;
             if (j >= p.ny) break;
             const size_t k = at(p, i, j);
+            // E_phi and the sums this cell updates (at most two read back for
+            // q >= 1), loaded before the right-hand side so that their
+            // latency overlaps it
+            double ep = 0.0, old[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+            if (C.q >= 0) {
+                ep = C.Ephi[k];
+                if (C.q >= 1) {
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        if (C.q + t < 3) {
+                            old[t][0] = C.phi[C.q + t][k];
+                            old[t][1] = C.sxx[C.q + t][k];
+                            old[t][2] = C.syy[C.q + t][k];
+                            old[t][3] = C.sxy[C.q + t][k];
+                        }
+                }
+            }
             double oxx, oyy, oxy;
             sigma_rhs<IN>(p, S, U, V, s.phi[k], i, j, st, oxx, oyy, oxy);
-            Exx[k] = oxx;
-            Eyy[k] = oyy;
-            Exy[k] = oxy;
+            if (Exx) {
+                Exx[k] = oxx;
+                Eyy[k] = oyy;
+                Exy[k] = oxy;
+            }
+            if (C.q >= 0) {  // the stage values of the later stages
+                double b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+                if (C.q == 0) {  // phi^n and sigma^n with the safety reset (the tile's values)
+                    const Sym3 b = S(i, j);
+                    b0 = s.phi[k];
+                    b1 = b.xx;
+                    b2 = b.yy;
+                    b3 = b.xy;
+                }
+                for (int t = C.q; t < 3; ++t) {
+                    const int u = (t - C.q) & 1;
+                    double P = C.q == 0 ? b0 : old[u][0], A = C.q == 0 ? b1 : old[u][1];
+                    double D = C.q == 0 ? b2 : old[u][2], B = C.q == 0 ? b3 : old[u][3];
+                    const double c = C.cf[t];
+                    if (c != 0.0) {
+                        P += c * ep;
+                        A += c * oxx;
+                        D += c * oyy;
+                        B += c * oxy;
+                    }
+                    if (t == C.q && C.reset && P < p.ext_cutoff) {
+                        A = 1.0;
+                        D = 1.0;
+                        B = 0.0;
+                    }
+                    C.phi[t][k] = P;
+                    C.sxx[t][k] = A;
+                    C.syy[t][k] = D;
+                    C.sxy[t][k] = B;
+                    if (report && t == 2) {
+                        double det = A * D - B * B;
+                        double lmin = 0.5 * (A + D - hypot(A - D, 2.0 * B));
+                        demax = fmax(demax, fabs(det - 1.0));
+                        emin = fmin(emin, lmin);
+                        nonspd += (isfinite(lmin) && lmin > 0.0) ? 0 : 1;
+                        pmin = fmin(pmin, P);
+                        pmax = fmax(pmax, P);
+                        nonfin += (isfinite(P) && isfinite(A) && isfinite(D) && isfinite(B)) ? 0 : 1;
+                    }
+                }
+            }
             if (stage1) {
                 double uL = s.U[k], uR = s.U[at(p, i + 1, j)], vB = s.V[k], vT = s.V[at(p, i, j + 1)];
                 double dv = (uR - uL) * p.ihx + (vT - vB) * p.ihy;
@@ -1267,6 +1334,45 @@ __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, doub
                 if (!isfinite(dv)) dmax = INFINITY;
             }
         }
+    if (report) {
+        pmin = wave_min(pmin);
+        pmax = wave_max(pmax);
+        emin = wave_min(emin);
+        demax = wave_max(demax);
+        for (int off = 32; off > 0; off >>= 1) {
+            nonspd += __shfl_down(nonspd, off, 64);
+            nonfin += __shfl_down(nonfin, off, 64);
+        }
+        if ((tid & 63) == 0) {
+            const int w = tid >> 6;
+            lds_r[0][w] = pmin;
+            lds_r[1][w] = pmax;
+            lds_r[2][w] = emin;
+            lds_r[3][w] = demax;
+            lds_rn[0][w] = nonspd;
+            lds_rn[1][w] = nonfin;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            double v0 = lds_r[0][0], v1 = lds_r[1][0], v2 = lds_r[2][0], v3 = lds_r[3][0];
+            long long n0 = lds_rn[0][0], n1 = lds_rn[1][0];
+            for (int w = 1; w < BLOCK / 64; ++w) {
+                v0 = fmin(v0, lds_r[0][w]);
+                v1 = fmax(v1, lds_r[1][w]);
+                v2 = fmin(v2, lds_r[2][w]);
+                v3 = fmax(v3, lds_r[3][w]);
+                n0 += lds_rn[0][w];
+                n1 += lds_rn[1][w];
+            }
+            const size_t bid = blockIdx.x + blockIdx.y * gridDim.x, nb = p.exnb;
+            p.exbuf[EX_PHI_NEGMIN * nb + bid] = -v0;
+            p.exbuf[EX_PHI_MAX * nb + bid] = v1;
+            p.exbuf[EX_EIG_NEGMIN * nb + bid] = -v2;
+            p.exbuf[EX_DET_ERR * nb + bid] = v3;
+            p.exbuf[EX_NONSPD * nb + bid] = (double)n0;
+            p.exbuf[EX_NONFINITE * nb + bid] = (double)n1;
+        }
+    }
     if (!stage1) return;
     dmax = wave_max(dmax);
     for (int off = 32; off > 0; off >>= 1) nonfinite += __shfl_down(nonfinite, off, 64);
@@ -1290,144 +1396,15 @@ __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, doub
 
 __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
                                                  double* __restrict__ Eyy, double* __restrict__ Exy, Status* st,
-                                                 int stage1) {
+                                                 int stage1, CellSums C) {
     __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
     __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
     const TileId T = tile_id();
     const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
     // every cell of the tile at least one cell inside the walls
     const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
-    if (inner) sigma_tile<true>(p, s, Exx, Eyy, Exy, st, stage1, i0, j0, su, sv, sxx, syy, sxy);
-    else sigma_tile<false>(p, s, Exx, Eyy, Exy, st, stage1, i0, j0, su, sv, sxx, syy, sxy);
-}
-
-// ---------------------------------------------------------------------------
-// Stage combinations (_build_explicit_stage! :716-740, _build_velocity_stage!
-// :777-793) for s = 2..4.  coef[j] = dt * A_EXPLICIT[s][j] for the cells,
-// ce[j] = A_EXPLICIT[s][j], ci[j] = A_IMPLICIT[s][j] for the velocity.
-// ---------------------------------------------------------------------------
-struct Hist {
-    const double* E[3];
-    const double* Sxx[3];
-    const double* Syy[3];
-    const double* Sxy[3];
-};
-
-// phi_s, sigma_s = base + sum over the NT active terms (nonzero coefficient,
-// in stage order) of c E; two rows of loads in flight at a time
-template <int NT>
-__global__ void __launch_bounds__(BLOCK) k_combine_cells(Params p, Stage base, Hist h, double c0, double c1,
-                                                         double c2, double* __restrict__ phi_s,
-                                                         double* __restrict__ sxx_s, double* __restrict__ syy_s,
-                                                         double* __restrict__ sxy_s, Status* st, int report) {
-    __shared__ double lds[4][16];
-    __shared__ long long lds_n[2][16];
-    const double cf[3] = {c0, c1, c2};
-    const int i = tile_id().x * TILE_X + threadIdx.x;
-    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
-    double pmin = INFINITY, pmax = -INFINITY, emin = INFINITY, dmax = 0.0;
-    long long nonspd = 0, nonfinite = 0;
-    constexpr int RH = 2;  // rows per load batch
-#pragma unroll
-    for (int r0 = 0; r0 < ROWS; r0 += RH) {
-        bool in[RH];
-        size_t kk[RH];
-        double ph[RH], a[RH], d[RH], b[RH], e[RH][NT > 0 ? NT : 1][4];
-#pragma unroll
-        for (int r = 0; r < RH; ++r) {
-            const int j = jb + (r0 + r) * TILE_Y;
-            in[r] = i < p.nx && j < p.ny;
-            const size_t k = in[r] ? at(p, i, j) : 0;
-            kk[r] = k;
-            ph[r] = base.phi[k];
-            a[r] = base.sxx[k];
-            d[r] = base.syy[k];
-            b[r] = base.sxy[k];
-#pragma unroll
-            for (int q = 0; q < NT; ++q) {
-                e[r][q][0] = h.E[q][k];
-                e[r][q][1] = h.Sxx[q][k];
-                e[r][q][2] = h.Syy[q][k];
-                e[r][q][3] = h.Sxy[q][k];
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < RH; ++r) {
-            if (!in[r]) continue;
-            const size_t k = kk[r];
-            double P = ph[r], A = a[r], D = d[r], B = b[r];
-            if (base.reset && P < p.ext_cutoff) {  // load_sigma: the safety reset of sigma_n
-                A = 1.0;
-                D = 1.0;
-                B = 0.0;
-            }
-#pragma unroll
-            for (int q = 0; q < NT; ++q) {
-                P += cf[q] * e[r][q][0];
-                A += cf[q] * e[r][q][1];
-                D += cf[q] * e[r][q][2];
-                B += cf[q] * e[r][q][3];
-            }
-            if (base.reset && P < p.ext_cutoff) {
-                A = 1.0;
-                D = 1.0;
-                B = 0.0;
-            }
-            phi_s[k] = P;
-            sxx_s[k] = A;
-            syy_s[k] = D;
-            sxy_s[k] = B;
-            if (report) {  // structure_tensor_diagnostics StructureTensor.jl:627-662
-                double det = A * D - B * B;
-                double lmin = 0.5 * (A + D - hypot(A - D, 2.0 * B));
-                dmax = fmax(dmax, fabs(det - 1.0));
-                emin = fmin(emin, lmin);
-                nonspd += (isfinite(lmin) && lmin > 0.0) ? 0 : 1;
-                pmin = fmin(pmin, P);
-                pmax = fmax(pmax, P);
-                nonfinite += (isfinite(P) && isfinite(A) && isfinite(D) && isfinite(B)) ? 0 : 1;
-            }
-        }
-    }
-    if (!report) return;
-    pmin = wave_min(pmin);
-    pmax = wave_max(pmax);
-    emin = wave_min(emin);
-    dmax = wave_max(dmax);
-    for (int off = 32; off > 0; off >>= 1) {
-        nonspd += __shfl_down(nonspd, off, 64);
-        nonfinite += __shfl_down(nonfinite, off, 64);
-    }
-    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
-    if ((lane & 63) == 0) {
-        int w = lane >> 6;
-        lds[0][w] = pmin;
-        lds[1][w] = pmax;
-        lds[2][w] = emin;
-        lds[3][w] = dmax;
-        lds_n[0][w] = nonspd;
-        lds_n[1][w] = nonfinite;
-    }
-    __syncthreads();
-    if (lane == 0) {
-        double v0 = lds[0][0], v1 = lds[1][0], v2 = lds[2][0], v3 = lds[3][0];
-        long long n0 = lds_n[0][0], n1 = lds_n[1][0];
-        for (int w = 1; w < BLOCK / 64; ++w) {
-            v0 = fmin(v0, lds[0][w]);
-            v1 = fmax(v1, lds[1][w]);
-            v2 = fmin(v2, lds[2][w]);
-            v3 = fmax(v3, lds[3][w]);
-            n0 += lds_n[0][w];
-            n1 += lds_n[1][w];
-        }
-        const size_t bid = blockIdx.x + blockIdx.y * gridDim.x, nb = p.exnb;
-        p.exbuf[EX_PHI_NEGMIN * nb + bid] = -v0;
-        p.exbuf[EX_PHI_MAX * nb + bid] = v1;
-        p.exbuf[EX_EIG_NEGMIN * nb + bid] = -v2;
-        p.exbuf[EX_DET_ERR * nb + bid] = v3;
-        p.exbuf[EX_NONSPD * nb + bid] = (double)n0;
-        p.exbuf[EX_NONFINITE * nb + bid] = (double)n1;
-    }
+    if (inner) sigma_tile<true>(p, s, Exx, Eyy, Exy, st, stage1, C, i0, j0, su, sv, sxx, syy, sxy);
+    else sigma_tile<false>(p, s, Exx, Eyy, Exy, st, stage1, C, i0, j0, su, sv, sxx, syy, sxy);
 }
 
 // b_visc = M*0 + b (the affine part of the viscous operator), computed once.
@@ -1524,7 +1501,8 @@ void launch_momentum(hipStream_t s, const Params& p, const double* U, const doub
 
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
                   const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1) {
+                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1,
+                  const CellSums* sums) {
     Stage sg{U, V, phi, sxx, syy, sxy, reset};
     if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn)
         k_max_speed<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, st);
@@ -1535,33 +1513,10 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
         case FSI_PHASE_CDI: k_phase<FSI_PHASE_CDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
         default: k_phase<FSI_PHASE_UPWIND><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
     }
-    k_sigma<<<g, blk, 0, s>>>(p, sg, Exx, Eyy, Exy, st, stage1);
-}
-
-void launch_combine_cells(hipStream_t s, const Params& p, const double* phi_n, const double* sxx_n,
-                          const double* syy_n, const double* sxy_n, int reset, const double* const* E,
-                          const double* const* Sxx, const double* const* Syy, const double* const* Sxy, int ns,
-                          const double* coef, double* phi_s, double* sxx_s, double* syy_s, double* sxy_s,
-                          Status* st, int report) {
-    Stage base{nullptr, nullptr, phi_n, sxx_n, syy_n, sxy_n, reset};
-    Hist h{};
-    double c[3] = {0.0, 0.0, 0.0};
-    int nt = 0;  // the terms with a nonzero coefficient, in stage order
-    for (int q = 0; q < ns; ++q) {
-        if (coef[q] == 0.0) continue;
-        h.E[nt] = E[q];
-        h.Sxx[nt] = Sxx[q];
-        h.Syy[nt] = Syy[q];
-        h.Sxy[nt] = Sxy[q];
-        c[nt++] = coef[q];
-    }
-    const dim3 g = grid2d(p), t(TILE_X, TILE_Y);
-    switch (nt) {
-    case 0: k_combine_cells<0><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
-    case 1: k_combine_cells<1><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
-    case 2: k_combine_cells<2><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
-    default: k_combine_cells<3><<<g, t, 0, s>>>(p, base, h, c[0], c[1], c[2], phi_s, sxx_s, syy_s, sxy_s, st, report); break;
-    }
+    CellSums C{};
+    C.q = -1;
+    if (sums) C = *sums;
+    k_sigma<<<g, blk, 0, s>>>(p, sg, Exx, Eyy, Exy, st, stage1, C);
 }
 
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv) {
