@@ -37,8 +37,8 @@ def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
     right-hand sides, 8 + 8 + 4; phase 3 x 4; sigma 7 reads per stage plus the
     running sums of the later stages' phi and sigma, 12 + 16 + 8; the stage
     combinations happen inside these); per velocity solve 18 (start: right-hand
-    side and four history fields read, x r p q written) + 16 per CG
-    iteration; per projection 14 (rhs 3, the
+    side and four history fields read, x r written; q0: r read, q written)
+    + 16 per CG iteration; per projection 14 (rhs 3, the
     tested solution 2, update 5, check 4) + 6 for the direct transform solve
     (three kernels each reading and writing the field once), or 17 + 17.3 per
     PCG iteration when the multigrid-PCG solves it."""

@@ -91,23 +91,23 @@ struct HTile {
 // x0 = cur, or the warm start extrapolated in time from the last two
 // (2 cur - prev), three (3 (cur - prev) + old) or four
 // (4 (cur + old) - 6 prev - old2) stage solutions, with the Dirichlet values,
-// at the tile points it * BLOCK + tid, A <= it < B, written to the LDS tile
-template <int A, int B, int N2>
-__device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx2, const int* fl, int tid, double gul,
+// at the tile points it * BLOCK + tid < NPT, A <= it < B, written to the LDS tile
+template <int A, int B, int NPT>
+__device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx, const int* fl, int tid, double gul,
                                          double gvb, double* sxu, double* sxv) {
     constexpr int M = B - A;
     double xu[M], xv[M];
 #pragma unroll
     for (int t = 0; t < M; ++t) {
-        xu[t] = b.Un[idx2[A + t]];
-        xv[t] = b.Vn[idx2[A + t]];
+        xu[t] = b.Un[idx[A + t]];
+        xv[t] = b.Vn[idx[A + t]];
     }
     if (b.extrap == 1) {
         double yu[M], yv[M];
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            yu[t] = b.Pu[idx2[A + t]];
-            yv[t] = b.Pv[idx2[A + t]];
+            yu[t] = b.Pu[idx[A + t]];
+            yv[t] = b.Pv[idx[A + t]];
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
@@ -118,10 +118,10 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx2, c
         double yu[M], yv[M], zu[M], zv[M];
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            yu[t] = b.Pu[idx2[A + t]];
-            yv[t] = b.Pv[idx2[A + t]];
-            zu[t] = b.Ou[idx2[A + t]];
-            zv[t] = b.Ov[idx2[A + t]];
+            yu[t] = b.Pu[idx[A + t]];
+            yv[t] = b.Pv[idx[A + t]];
+            zu[t] = b.Ou[idx[A + t]];
+            zv[t] = b.Ov[idx[A + t]];
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
@@ -132,12 +132,12 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx2, c
         double yu[M], yv[M], zu[M], zv[M], wu[M], wv[M];
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            yu[t] = b.Pu[idx2[A + t]];
-            yv[t] = b.Pv[idx2[A + t]];
-            zu[t] = b.Ou[idx2[A + t]];
-            zv[t] = b.Ov[idx2[A + t]];
-            wu[t] = b.Wu[idx2[A + t]];
-            wv[t] = b.Wv[idx2[A + t]];
+            yu[t] = b.Pu[idx[A + t]];
+            yv[t] = b.Pv[idx[A + t]];
+            zu[t] = b.Ou[idx[A + t]];
+            zv[t] = b.Ov[idx[A + t]];
+            wu[t] = b.Wu[idx[A + t]];
+            wv[t] = b.Wv[idx[A + t]];
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
@@ -148,7 +148,7 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx2, c
 #pragma unroll
     for (int t = 0; t < M; ++t) {
         const int k = tid + (A + t) * BLOCK;
-        if (k >= N2) break;
+        if (k >= NPT) break;
         const int f = fl[A + t];
         sxu[k] = (f & 1) ? ((f & 2) ? gul : xu[t]) : 0.0;
         sxv[k] = (f & 1) ? ((f & 4) ? gvb : xv[t]) : 0.0;
@@ -156,14 +156,13 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx2, c
 }
 
 // Start of a predictor solve: x0 = the warm start with the Dirichlet
-// values, r0 = rhs - (I - alpha M) x0 (zero on Dirichlet rows), p0 = r0,
-// q0 = (I - alpha M) p0, and the partials of |r0|^2, |b_F|^2 (free rows) and
-// of the first iteration (p.q, r.q, q.q, r.r).  x0 sits on a halo-2 LDS
-// tile and r0 on a halo-1 tile, so both stencils read LDS.
+// values, r0 = rhs - (I - alpha M) x0 (zero on Dirichlet rows) and the
+// partials of |r0|^2, |b_F|^2 (free rows).  x0 sits on a halo-1 LDS tile;
+// r0 is formed on the block's own cells only.  p0 = r0 is not stored, and
+// q0 = (I - alpha M) r0 with the first iteration's partials is k_vel_q0's.
 template <bool IN>
 __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b, double alpha, int nb, int i0, int j0,
                                                int tid, double* sxu, double* sxv) {
-    using T2 = HTile<2, IN>;
     using T1 = HTile<1, IN>;
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
     auto wrapped = [&](int ui, int uj, int& gi, int& gj) -> bool {
@@ -180,121 +179,144 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         }
         return true;
     };
-    // x0 on halo 2 and the right-hand side on halo 1: indices first
-    constexpr int N2 = T2::R * T2::W, NI2 = (N2 + BLOCK - 1) / BLOCK;
     constexpr int N1 = T1::R * T1::W, NI1 = (N1 + BLOCK - 1) / BLOCK;
-    size_t idx2[NI2], idx[NI1];
-    int fl[NI2];  // 1: inside, 2: U-left Dirichlet column, 4: V-bottom Dirichlet row
-    bool inb[NI1];
-#pragma unroll
-    for (int it = 0; it < NI2; ++it) {
-        const int k = min(tid + it * BLOCK, N2 - 1);
-        int gi, gj;
-        const bool in = wrapped(i0 + k % T2::W - 2, j0 + k / T2::W - 2, gi, gj);
-        idx2[it] = in ? at(p, gi, gj) : 0;
-        fl[it] = in ? (1 | (dir_u<IN>(p, gi) ? 2 : 0) | (dir_v<IN>(p, gj) ? 4 : 0)) : 0;
-    }
+    size_t idx1[NI1];
+    int fl[NI1];  // 1: inside, 2: U-left Dirichlet column, 4: V-bottom Dirichlet row
 #pragma unroll
     for (int it = 0; it < NI1; ++it) {
         const int k = min(tid + it * BLOCK, N1 - 1);
         int gi, gj;
-        inb[it] = wrapped(i0 + k % T1::W - 1, j0 + k / T1::W - 1, gi, gj);
-        idx[it] = inb[it] ? at(p, gi, gj) : 0;
+        const bool in = wrapped(i0 + k % T1::W - 1, j0 + k / T1::W - 1, gi, gj);
+        idx1[it] = in ? at(p, gi, gj) : 0;
+        fl[it] = in ? (1 | (dir_u<IN>(p, gi) ? 2 : 0) | (dir_v<IN>(p, gj) ? 4 : 0)) : 0;
     }
-    // the right-hand side first (needed after the x0 tile is built), then x0
-    // in two batches of tile points (half the registers in flight)
-    double bu[NI1], bv[NI1];
-#pragma unroll
-    for (int it = 0; it < NI1; ++it) {
-        bu[it] = b.ru[idx[it]];
-        bv[it] = b.rv[idx[it]];
-    }
-    x0_batch<0, NI2 / 2, N2>(b, idx2, fl, tid, gul, gvb, sxu, sxv);
-    x0_batch<NI2 / 2, NI2, N2>(b, idx2, fl, tid, gul, gvb, sxu, sxv);
-    __syncthreads();
-    const T2 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
-    // r0 on halo 1; |r0|^2 and |b_F|^2 over the block's own cells.  r0 is
-    // held in registers until every read of the x0 tile is done, then takes
-    // its LDS (the block needs one tile of LDS, not two)
-    double acc2[2] = {0.0, 0.0};
-    double r0u[NI1], r0v[NI1];
-    {
-#pragma unroll
-        for (int it = 0; it < NI1; ++it) {
-            const int k = tid + it * BLOCK;
-            if (k >= N1) break;
-            const int li = k % T1::W, lj = k / T1::W;
-            const int ui = i0 + li - 1, uj = j0 + lj - 1;
-            double ru = 0.0, rv = 0.0;
-            if (inb[it]) {
-                int gi, gj;
-                wrapped(ui, uj, gi, gj);
-                const size_t kk = idx[it];
-                const bool own = li >= 1 && li <= TILE_X && lj >= 1 && lj <= TILE_ROWS && ui < p.nx && uj < p.ny;
-                if (!dir_u<IN>(p, gi)) {
-                    ru = bu[it] - (Xu(gi, gj) - alpha * visc_u<IN>(p, Xu, Xv, gi, gj, 0));
-                    if (own) acc2[1] += bu[it] * bu[it];
-                }
-                if (!dir_v<IN>(p, gj)) {
-                    rv = bv[it] - (Xv(gi, gj) - alpha * visc_v<IN>(p, Xu, Xv, gi, gj, 0));
-                    if (own) acc2[1] += bv[it] * bv[it];
-                }
-                if (own) {
-                    b.xu[kk] = Xu(gi, gj);
-                    b.xv[kk] = Xv(gi, gj);
-                    b.r_u[0][kk] = ru;  // p0 = r0: the first iteration reads it from r
-                    b.r_v[0][kk] = rv;
-                    acc2[0] += ru * ru + rv * rv;
-                }
-            }
-            r0u[it] = ru;
-            r0v[it] = rv;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < NI1; ++it) {
-        const int k = tid + it * BLOCK;
-        if (k >= N1) break;
-        sxu[k] = r0u[it];
-        sxv[k] = r0v[it];
-    }
-    __syncthreads();
-    const T1 Ru{sxu, i0, j0, p.nx, p.ny}, Rv{sxv, i0, j0, p.nx, p.ny};
-    double acc4[4] = {0.0, 0.0, 0.0, 0.0};
+    // the own cells' right-hand side first, then x0 in two batches of tile points
     const int i = i0 + threadIdx.x;
-    if (i < p.nx)
-        for (int r = 0; r < ROWS; ++r) {
-            const int j = j0 + threadIdx.y + r * TILE_Y;
-            if (j >= p.ny) break;
-            const size_t k = at(p, i, j);
-            const double vu = Ru(i, j), vv = Rv(i, j);
-            const double qu = dir_u<IN>(p, i) ? 0.0 : vu - alpha * visc_u<IN>(p, Ru, Rv, i, j, 0);
-            const double qv = dir_v<IN>(p, j) ? 0.0 : vv - alpha * visc_v<IN>(p, Ru, Rv, i, j, 0);
-            b.q_u[0][k] = qu;
-            b.q_v[0][k] = qv;
-            acc4[0] += vu * qu + vv * qv;
-            acc4[1] += vu * qu + vv * qv;  // r = p on the first iteration
-            acc4[2] += qu * qu + qv * qv;
-            acc4[3] += vu * vu + vv * vv;
+    bool own[ROWS];
+    size_t kc[ROWS];
+    double bu[ROWS], bv[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = j0 + threadIdx.y + r * TILE_Y;
+        own[r] = i < p.nx && j < p.ny;
+        kc[r] = own[r] ? at(p, i, j) : 0;
+        bu[r] = b.ru[kc[r]];
+        bv[r] = b.rv[kc[r]];
+    }
+    x0_batch<0, NI1 / 2, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
+    x0_batch<NI1 / 2, NI1, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
+    __syncthreads();
+    const T1 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
+    double acc2[2] = {0.0, 0.0};  // |r0|^2, |b_F|^2
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (!own[r]) continue;
+        const int j = j0 + threadIdx.y + r * TILE_Y;
+        const size_t k = kc[r];
+        double ru = 0.0, rv = 0.0;
+        if (!dir_u<IN>(p, i)) {
+            ru = bu[r] - (Xu(i, j) - alpha * visc_u<IN>(p, Xu, Xv, i, j, 0));
+            acc2[1] += bu[r] * bu[r];
         }
+        if (!dir_v<IN>(p, j)) {
+            rv = bv[r] - (Xv(i, j) - alpha * visc_v<IN>(p, Xu, Xv, i, j, 0));
+            acc2[1] += bv[r] * bv[r];
+        }
+        b.xu[k] = Xu(i, j);
+        b.xv[k] = Xv(i, j);
+        b.r_u[0][k] = ru;  // p0 = r0: the first iteration reads it from r
+        b.r_v[0][k] = rv;
+        acc2[0] += ru * ru + rv * rv;
+    }
     put_partials<2>(acc2, pset(b.partials, nb, PS_VELI), nb);
-    put_partials<4>(acc4, pset(b.partials, nb, PS_VEL0), nb);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
-    constexpr int N2 = HTile<2>::R * HTile<2>::W, N1 = HTile<1>::R * HTile<1>::W;
-    static_assert(N1 <= N2, "the r0 tile reuses the x0 tile");
-    __shared__ double sxu[N2], sxv[N2];
+    constexpr int N1 = HTile<1>::R * HTile<1>::W;
+    __shared__ double sxu[N1], sxv[N1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
     const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
     if (T.x == 0 && T.y == 0 && tid == 0) st->vel.done = 0;  // the iteration kernels follow this one
-    // every point whose operator row the block evaluates (halo 1 included)
-    // has its stencil inside the domain: no boundary rows, no wraps
-    const bool inner = i0 - 1 >= 1 && i0 + TILE_X <= p.nx - 2 && j0 - 1 >= 1 && j0 + TILE_ROWS <= p.ny - 2;
+    // the block's own operator rows are not the first or last face / row and
+    // their stencils stay inside the halo-1 tile: no boundary rows, no wraps
+    const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
     if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
     else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
+}
+
+// q0 = (I - alpha M) p0 with p0 = r0 (k_vel_start's) and the partials of the
+// first iteration (p.q, r.q, q.q, r.r): r0 on a halo-1 LDS tile, the block's
+// own cells and the halo ring as in k_vel_iter.
+__global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double alpha, int nb) {
+    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];
+    const int tid = threadIdx.x + threadIdx.y * TILE_X;
+    const TileId T = tile_id();
+    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    const double *ru = b.r_u[0], *rv = b.r_v[0];
+    auto wrapped = [&](int li, int lj) -> long {
+        int gi = i0 + li - 1, gj = j0 + lj - 1;
+        if (gi < 0 || gi >= p.nx) {
+            if (!p.px) return -1;
+            gi = ((gi % p.nx) + p.nx) % p.nx;
+        }
+        if (gj < 0 || gj >= p.ny) {
+            if (!p.py) return -1;
+            gj = ((gj % p.ny) + p.ny) % p.ny;
+        }
+        return (long)at(p, gi, gj);
+    };
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx;
+    size_t kc[ROWS];
+    bool own[ROWS], fill[ROWS];
+    double cru[ROWS], crv[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = j0 + ty + r * TILE_Y;
+        own[r] = i < p.nx && j < p.ny;
+        const long w = wrapped(tx + 1, ty + r * TILE_Y + 1);
+        fill[r] = w >= 0;
+        kc[r] = fill[r] ? (size_t)w : 0;
+        cru[r] = ru[kc[r]];
+        crv[r] = rv[kc[r]];
+    }
+    constexpr int NRING = 2 * VT_W + 2 * TILE_ROWS;
+    if (tid < NRING) {
+        int li, lj;
+        if (tid < VT_W) { li = tid; lj = 0; }
+        else if (tid < 2 * VT_W) { li = tid - VT_W; lj = VT_H - 1; }
+        else if (tid < 2 * VT_W + TILE_ROWS) { li = 0; lj = tid - 2 * VT_W + 1; }
+        else { li = VT_W - 1; lj = tid - 2 * VT_W - TILE_ROWS + 1; }
+        const long w = wrapped(li, lj);
+        const size_t kk = w >= 0 ? (size_t)w : 0;
+        su[lj * VT_W + li] = w >= 0 ? ru[kk] : 0.0;
+        sv[lj * VT_W + li] = w >= 0 ? rv[kk] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int slot_c = (ty + r * TILE_Y + 1) * VT_W + tx + 1;
+        su[slot_c] = fill[r] ? cru[r] : 0.0;
+        sv[slot_c] = fill[r] ? crv[r] : 0.0;
+    }
+    __syncthreads();
+    const VelTile tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (!own[r]) continue;
+        const int j = j0 + ty + r * TILE_Y;
+        const size_t k = kc[r];
+        const double qu = dir_u(p, i) ? 0.0 : cru[r] - alpha * visc_u(p, tu, tv, i, j, 0);
+        const double qv = dir_v(p, j) ? 0.0 : crv[r] - alpha * visc_v(p, tu, tv, i, j, 0);
+        b.q_u[0][k] = qu;
+        b.q_v[0][k] = qv;
+        acc[0] += cru[r] * qu + crv[r] * qv;
+        acc[1] += cru[r] * qu + crv[r] * qv;  // r = p on the first iteration
+        acc[2] += qu * qu + qv * qv;
+        acc[3] += cru[r] * cru[r] + crv[r] * crv[r];
+    }
+    put_partials<4>(acc, pset(b.partials, nb, PS_VEL0), nb);
 }
 
 // One CG iteration m >= 1 with a single reduction phase: the totals of
@@ -1095,6 +1117,7 @@ static const dim3 kBlock(TILE_X, TILE_Y);
 
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st) {
     k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
+    k_vel_q0<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p));
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot, double rtol, int maxiter, int check_from) {
