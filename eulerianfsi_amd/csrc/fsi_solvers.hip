@@ -855,6 +855,50 @@ __global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int n
     put_partials<1>(acc, pset(b.partials, nb, PS_MEAN), nb);
 }
 
+// The face gradients of q with their boundary rows (scalar_gradient,
+// Operators.jl:189-356), any face: x-face (i, j), i = 0..nx; y-face (i, j),
+// j = 0..ny.
+__device__ __forceinline__ double grad_face_x(const Params& p, const GAcc& Q, int i, int j) {
+    if (p.px) {
+        int k = (i == p.nx) ? 0 : i;
+        int km = k == 0 ? p.nx - 1 : k - 1;
+        return p.ihx * Q(k, j) - p.ihx * Q(km, j);
+    } else if (i == 0) {
+        if (p.bQ.l.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hx / 2.0;
+            return (1.0 / dist) * Q(0, j) + (-p.bQ.l.value / dist);
+        }
+        return -p.bQ.l.value;
+    } else if (i == p.nx) {
+        if (p.bQ.r.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hx / 2.0;
+            return (-1.0 / dist) * Q(p.nx - 1, j) + p.bQ.r.value / dist;
+        }
+        return p.bQ.r.value;
+    }
+    return p.ihx * Q(i, j) - p.ihx * Q(i - 1, j);
+}
+__device__ __forceinline__ double grad_face_y(const Params& p, const GAcc& Q, int i, int j) {
+    if (p.py) {
+        int k = (j == p.ny) ? 0 : j;
+        int km = k == 0 ? p.ny - 1 : k - 1;
+        return p.ihy * Q(i, k) - p.ihy * Q(i, km);
+    } else if (j == 0) {
+        if (p.bQ.b.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hy / 2.0;
+            return (1.0 / dist) * Q(i, 0) + (-p.bQ.b.value / dist);
+        }
+        return -p.bQ.b.value;
+    } else if (j == p.ny) {
+        if (p.bQ.t.kind == FSI_BC_DIRICHLET) {
+            double dist = p.hy / 2.0;
+            return (-1.0 / dist) * Q(i, p.ny - 1) + p.bQ.t.value / dist;
+        }
+        return p.bQ.t.value;
+    }
+    return p.ihy * Q(i, j) - p.ihy * Q(i, j - 1);
+}
+
 // v -= grad q on every face (src/Projection.jl:266-273), q with its BC data;
 // read from Uin/Vin, written to U/V (the same arrays for a projection in
 // place).  Launched on the cell grid: the thread of cell (i, j) updates the
@@ -862,47 +906,6 @@ __global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int n
 // faces x = nx and y = ny; every load of the thread's rows is issued first.
 __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
     GAcc Q{b.x, p.P};
-    // the face gradients with their boundary rows (any face)
-    auto gx_face = [&](int i, int j) -> double {
-        if (p.px) {
-            int k = (i == p.nx) ? 0 : i;
-            int km = k == 0 ? p.nx - 1 : k - 1;
-            return p.ihx * Q(k, j) - p.ihx * Q(km, j);
-        } else if (i == 0) {
-            if (p.bQ.l.kind == FSI_BC_DIRICHLET) {
-                double dist = p.hx / 2.0;
-                return (1.0 / dist) * Q(0, j) + (-p.bQ.l.value / dist);
-            }
-            return -p.bQ.l.value;
-        } else if (i == p.nx) {
-            if (p.bQ.r.kind == FSI_BC_DIRICHLET) {
-                double dist = p.hx / 2.0;
-                return (-1.0 / dist) * Q(p.nx - 1, j) + p.bQ.r.value / dist;
-            }
-            return p.bQ.r.value;
-        }
-        return p.ihx * Q(i, j) - p.ihx * Q(i - 1, j);
-    };
-    auto gy_face = [&](int i, int j) -> double {
-        if (p.py) {
-            int k = (j == p.ny) ? 0 : j;
-            int km = k == 0 ? p.ny - 1 : k - 1;
-            return p.ihy * Q(i, k) - p.ihy * Q(i, km);
-        } else if (j == 0) {
-            if (p.bQ.b.kind == FSI_BC_DIRICHLET) {
-                double dist = p.hy / 2.0;
-                return (1.0 / dist) * Q(i, 0) + (-p.bQ.b.value / dist);
-            }
-            return -p.bQ.b.value;
-        } else if (j == p.ny) {
-            if (p.bQ.t.kind == FSI_BC_DIRICHLET) {
-                double dist = p.hy / 2.0;
-                return (-1.0 / dist) * Q(i, p.ny - 1) + p.bQ.t.value / dist;
-            }
-            return p.bQ.t.value;
-        }
-        return p.ihy * Q(i, j) - p.ihy * Q(i, j - 1);
-    };
     const int i = tile_id().x * TILE_X + threadIdx.x;
     const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
     bool in[ROWS];
@@ -925,12 +928,12 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
         if (!in[r]) continue;
         const int j = jb + r * TILE_Y;
         const size_t k = kk[r];
-        const double gx = i > 0 ? p.ihx * qc[r] - p.ihx * qw[r] : gx_face(0, j);
-        const double gy = j > 0 ? p.ihy * qc[r] - p.ihy * qs[r] : gy_face(i, 0);
+        const double gx = i > 0 ? p.ihx * qc[r] - p.ihx * qw[r] : grad_face_x(p, Q, 0, j);
+        const double gy = j > 0 ? p.ihy * qc[r] - p.ihy * qs[r] : grad_face_y(p, Q, i, 0);
         b.U[k] = ui[r] - gx;
         b.V[k] = vi[r] - gy;
-        if (i == p.nx - 1) b.U[k + 1] = b.Uin[k + 1] - gx_face(p.nx, j);
-        if (j == p.ny - 1) b.V[k + p.P] = b.Vin[k + p.P] - gy_face(i, p.ny);
+        if (i == p.nx - 1) b.U[k + 1] = b.Uin[k + 1] - grad_face_x(p, Q, p.nx, j);
+        if (j == p.ny - 1) b.V[k + p.P] = b.Vin[k + p.P] - grad_face_y(p, Q, i, p.ny);
     }
 }
 
@@ -1015,6 +1018,97 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
         const int bid = blockIdx.x + blockIdx.y * gridDim.x;
         double* dst = pset(b.partials, nb, PS_CHECK);
         if (RESID) dst[bid] = acc[0];  // else slot 0 (true residual) comes from k_proj_direct
+        dst[(size_t)nb + bid] = acc[1];
+        dst[2 * (size_t)nb + bid] = m[0];
+        double* vm = pset(b.partials, nb, PS_VMAX);
+        vm[bid] = m[1];
+        vm[(size_t)nb + bid] = m[2];
+        vm[2 * (size_t)nb + bid] = acc[2];
+    }
+}
+
+// The tested projection's update and check in one pass: v = v* - grad q on
+// the thread's own faces (and the far faces of the last column and row), and
+// the checks of k_proj_check<0> (q_out = q - mean, constraint, velocity
+// extrema) from the updated faces of each cell, the right and top ones
+// recomputed exactly as the neighbouring thread computes them.
+__global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, ProjBufs b, Status* st, int nb, int has_null,
+                                                        double lxly) {
+    double tm[1];
+    get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
+    const double mean = has_null ? tm[0] / lxly : 0.0;
+    const double area = p.hx * p.hy;
+    GAcc Q{b.x, p.P};
+    const int i = tile_id().x * TILE_X + threadIdx.x;
+    const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
+    bool in[ROWS];
+    size_t kk[ROWS];
+    double qc[ROWS], qw[ROWS], qs[ROWS], qe[ROWS], qn[ROWS], uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        const int j = jb + r * TILE_Y;
+        in[r] = i < p.nx && j < p.ny;
+        const size_t k = in[r] ? at(p, i, j) : 0;
+        kk[r] = k;
+        qc[r] = b.x[k];
+        qw[r] = b.x[in[r] && i > 0 ? k - 1 : k];
+        qs[r] = b.x[in[r] && j > 0 ? k - p.P : k];
+        qe[r] = b.x[in[r] && i < p.nx - 1 ? k + 1 : k];
+        qn[r] = b.x[in[r] && j < p.ny - 1 ? k + p.P : k];
+        uL[r] = b.Uin[k];
+        uR[r] = b.Uin[k + 1];
+        vB[r] = b.Vin[k];
+        vT[r] = b.Vin[k + p.P];
+    }
+    double acc[3] = {0.0, 0.0, 0.0};  // (true residual: k_proj_direct's), constraint, NaN faces
+    double linf = 0.0, umax = 0.0, vmax = 0.0;
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        if (!in[r]) continue;
+        const int j = jb + r * TILE_Y;
+        const size_t k = kk[r];
+        const bool ue = i == p.nx - 1, ve = j == p.ny - 1;
+        const double gxL = i > 0 ? p.ihx * qc[r] - p.ihx * qw[r] : grad_face_x(p, Q, 0, j);
+        const double gyB = j > 0 ? p.ihy * qc[r] - p.ihy * qs[r] : grad_face_y(p, Q, i, 0);
+        const double gxR = !ue ? p.ihx * qe[r] - p.ihx * qc[r] : grad_face_x(p, Q, p.nx, j);
+        const double gyT = !ve ? p.ihy * qn[r] - p.ihy * qc[r] : grad_face_y(p, Q, i, p.ny);
+        const double u0 = uL[r] - gxL, u1 = uR[r] - gxR, v0 = vB[r] - gyB, v1 = vT[r] - gyT;
+        b.U[k] = u0;
+        b.V[k] = v0;
+        if (ue) b.U[k + 1] = u1;
+        if (ve) b.V[k + p.P] = v1;
+        b.q_out[k] = qc[r] - mean;
+        double div = (u1 - u0) * p.ihx + (v1 - v0) * p.ihy;
+        double c = div - 0.0;
+        linf = fmax(linf, fabs(c));
+        if (!isfinite(c)) linf = INFINITY;
+        acc[1] += c * c * area;
+        umax = fmax(umax, fmax(fabs(u0), ue ? fabs(u1) : 0.0));
+        vmax = fmax(vmax, fmax(fabs(v0), ve ? fabs(v1) : 0.0));
+        acc[2] += ((isnan(u0) ? 1.0 : 0.0) + (isnan(v0) ? 1.0 : 0.0)) +
+                  ((ue && isnan(u1) ? 1.0 : 0.0) + (ve && isnan(v1) ? 1.0 : 0.0));
+    }
+    __shared__ double lds[3 * 16];
+    block_sum<3>(acc, lds);
+    __shared__ double lmax[3][BLOCK / 64];
+    linf = wave_max(linf);
+    umax = wave_max(umax);
+    vmax = wave_max(vmax);
+    const int lane = threadIdx.x + threadIdx.y * blockDim.x;
+    if ((lane & 63) == 0) {
+        lmax[0][lane >> 6] = linf;
+        lmax[1][lane >> 6] = umax;
+        lmax[2][lane >> 6] = vmax;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        double m[3];
+        for (int q = 0; q < 3; ++q) {
+            m[q] = lmax[q][0];
+            for (int w = 1; w < BLOCK / 64; ++w) m[q] = fmax(m[q], lmax[q][w]);
+        }
+        const int bid = blockIdx.x + blockIdx.y * gridDim.x;
+        double* dst = pset(b.partials, nb, PS_CHECK);
         dst[(size_t)nb + bid] = acc[1];
         dst[2 * (size_t)nb + bid] = m[0];
         double* vm = pset(b.partials, nb, PS_VMAX);
@@ -1233,8 +1327,12 @@ void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const 
                         int has_null, double rtol, int maxiter, double lxly, int slot) {
     const int nb = grid_blocks(p);
     k_proj_direct<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter);
-    k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
-    k_proj_check<0><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
+    if (b.Uin != b.U && b.Vin != b.V) {  // the fused pass reads the neighbours' unprojected faces
+        k_proj_upcheck<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, has_null, lxly);
+    } else {
+        k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
+        k_proj_check<0><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
+    }
     k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
 }
 
