@@ -38,12 +38,12 @@ def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
     running sums of the later stages' phi and sigma, 12 + 16 + 8; the stage
     combinations happen inside these); per velocity solve 18 (start: right-hand
     side and four history fields read, x r written; q0: r read, q written)
-    + 16 per CG iteration; per projection 14 (rhs 3, the
-    tested solution 2, update 5, check 4) + 6 for the direct transform solve
+    + 16 per CG iteration; per projection 11 (rhs 3, the
+    tested solution 2, update and check in one pass 6) + 6 for the direct transform solve
     (three kernels each reading and writing the field once), or 17 + 17.3 per
     PCG iteration when the multigrid-PCG solves it."""
     cell = 8.0 * N * N
-    per_proj = 14 + 6 if direct else 17
+    per_proj = 11 + 6 if direct else 17
     passes = 107 + sum(18 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
     return passes * cell
 
