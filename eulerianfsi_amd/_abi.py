@@ -25,6 +25,7 @@ RELAX_NONE, RELAX_MAXWELL = 0, 1
 FORCE_NONE, FORCE_CONSTANT, FORCE_KOLMOGOROV = 0, 1, 2
 EXT_NONE, EXT_IDENTITY, EXT_GEOMETRIC = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+SOLVER_DIRECT, SOLVER_GMRES = 0, 1
 
 
 class BcSide(ctypes.Structure):
@@ -68,6 +69,7 @@ class Numerics(ctypes.Structure):
         ("projection_rtol", c_double), ("projection_maxiter", c_int32),
         ("stage1_atol", c_double), ("stage1_rtol", c_double), ("check_stage1", c_int32),
         ("precision", c_int32),
+        ("velocity_method", c_int32), ("direct_fallback", c_int32), ("fallback_rtol", c_double),
     ]
 
 
@@ -197,6 +199,8 @@ def abi_config(metrics, bcs, models, numerics):
         n.extension_kind = EXT_NONE
     ls = numerics.linear_solvers
     n.velocity_rtol, n.velocity_maxiter = ls.reltol, ls.maxiter
+    n.velocity_method = SOLVER_GMRES if ls.velocity_method == "gmres" else SOLVER_DIRECT
+    n.direct_fallback, n.fallback_rtol = int(ls.direct_fallback), ls.fallback_reltol
     n.projection_rtol, n.projection_maxiter = numerics.projection_reltol, numerics.projection_maxiter
     n.stage1_atol, n.stage1_rtol = numerics.stage1_constraint_atol, numerics.stage1_constraint_rtol
     n.check_stage1 = int(numerics.check_stage1_constraint)

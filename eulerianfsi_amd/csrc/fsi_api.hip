@@ -79,7 +79,6 @@ struct fsi_context {
     // HIP graph (measured slower on ROCm 7 for the ~460-launch step), mode 0
     // polls each solve.
     int use_graph = 2;
-    int stage_warm = 1;    // per-stage projection warm starts (FSI_STAGE_WARM=0: shared correction)
     int have_history = 0;  // iteration counts of a completed step are known
     int plan_vel[3] = {0, 0, 0};
     int vel_check_from[3] = {0, 0, 0};  // the fewest iterations of the same history
@@ -113,14 +112,20 @@ struct fsi_workspace {
     // goes to phis, sxxs, syys, sxys, which the state swaps with)
     double *cphi[2] = {}, *csxx[2] = {}, *csyy[2] = {}, *csxy[2] = {};
     double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
+    // `correction` (work.correction, :828-833) is corr[corr_cur].  A step or
+    // a projection writes its correction into corr[1 - corr_cur] and flips
+    // corr_cur only when it has succeeded, so a failing call leaves the
+    // correction as it was (project! throws before copyto!, Projection.jl:279-298).
     double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
     int corr_cur = 0;
     // Warm starts of the three stage projections of a step: stage s starts
     // from stage s of the previous step (closer than the previous stage's
-    // correction, which the shared `corr` holds).  Only iteration counts
-    // depend on it; `corr` keeps the reference's meaning for callers.
-    double* corr_stage[3] = {};
-    double* corr_prev[3] = {};  // the step before, for the linear extrapolation 2 q^{n-1} - q^{n-2}
+    // correction), extrapolated as 2 q^{n-1} - q^{n-2}.  The step solves into
+    // cq_next; the three buffers of a stage trade roles only after a
+    // successful step, so a failed or redone step starts from the same
+    // history.  Only iteration counts depend on it.
+    double *cq_cur[3] = {}, *cq_prev[3] = {}, *cq_next[3] = {};
+    double* xs = nullptr;  // the iterate of a standalone projection (fsi_project)
     // the same for the predictor solves: stage s of the previous two steps
     // velocity stage solutions of the last four steps (newest first) and a
     // fifth buffer that the solve writes into (the projection then writes
@@ -132,14 +137,11 @@ struct fsi_workspace {
     int stage_warm = 0;         // completed steps whose stage solutions the history buffers hold (0 .. 3)
     std::vector<MgLevel> lv;
     std::vector<double*> owned;
-    // captured steps, keyed by everything their kernel arguments depend on
+    // captured steps, keyed by everything their kernel arguments depend on:
+    // the plan, dt, the history depth (it changes the launch sequence) and
+    // every buffer the step reads or writes whose role rotates between steps
     struct StepGraph {
-        int vel[3], proj[3], vchk[3];
-        double dt;
-        const double* sU;  // state/stage buffers alternate between steps
-        int corr_in, corr_out;
-        int warm;          // stage warm-start history (0 .. 3): changes the launch sequence
-        const double* vcur;  // newest velocity history buffer (the three rotate between steps)
+        std::vector<uintptr_t> key;
         hipGraphExec_t exec;
     };
     std::vector<StepGraph> graphs;
@@ -297,6 +299,9 @@ static fsi_status validate(fsi_context* c) {
     if (n.projection_maxiter <= 0) return fail(c, FSI_E_ARGUMENT, "projection_maxiter must be positive.");
     if (!(n.velocity_rtol > 0.0)) return fail(c, FSI_E_ARGUMENT, "Linear-solver reltol must be positive.");
     if (n.velocity_maxiter <= 0) return fail(c, FSI_E_ARGUMENT, "Linear-solver maxiter must be positive.");
+    if (n.velocity_method != FSI_SOLVER_DIRECT && n.velocity_method != FSI_SOLVER_GMRES)
+        return fail(c, FSI_E_ARGUMENT, "velocity_method must be :direct or :gmres.");
+    if (!(n.fallback_rtol > 0.0)) return fail(c, FSI_E_ARGUMENT, "fallback_reltol must be positive.");
     if (!(n.stage1_atol >= 0.0) || !(n.stage1_rtol >= 0.0))
         return fail(c, FSI_E_ARGUMENT, "stage-1 constraint tolerances must be non-negative.");
     if (n.precision != FSI_PRECISION_FP64 && n.precision != FSI_PRECISION_MIXED)
@@ -318,8 +323,6 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     {  // FSI_STEP_GRAPH=0 keeps every step on the polling path
         const char* e = std::getenv("FSI_STEP_GRAPH");  // 0 polling, 1 planned graph, 2 planned direct
         if (e && (e[0] == '0' || e[0] == '1' || e[0] == '2')) c->use_graph = e[0] - '0';
-        e = std::getenv("FSI_STAGE_WARM");
-        if (e && e[0] == '0') c->stage_warm = 0;
         e = std::getenv("FSI_SPECTRAL");
         if (e && e[0] == '0') c->use_spectral = 0;
     }
@@ -799,8 +802,8 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
     double** singles[] = {&w->ru,     &w->rv,     &w->r_u[0], &w->r_u[1], &w->r_v[0], &w->r_v[1], &w->p_u[0],
                           &w->p_u[1], &w->p_v[0], &w->p_v[1], &w->q_u[0], &w->q_u[1], &w->q_v[0], &w->q_v[1],
                           &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w,
-                          &w->corr_stage[0], &w->corr_stage[1], &w->corr_stage[2],
-                          &w->corr_prev[0],  &w->corr_prev[1],  &w->corr_prev[2],
+                          &w->cq_cur[0], &w->cq_cur[1], &w->cq_cur[2], &w->cq_prev[0], &w->cq_prev[1],
+                          &w->cq_prev[2], &w->cq_next[0], &w->cq_next[1], &w->cq_next[2], &w->xs,
                           &w->vel_cur_u[0], &w->vel_cur_u[1], &w->vel_cur_u[2], &w->vel_cur_v[0], &w->vel_cur_v[1],
                           &w->vel_cur_v[2], &w->vel_prev_u[0], &w->vel_prev_u[1], &w->vel_prev_u[2],
                           &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2], &w->vel_old_u[0],
@@ -917,6 +920,22 @@ static fsi_status poll_done(fsi_context* c, const KrylovScalars* dev, int* done,
     return FSI_OK;
 }
 
+// The stopping rules of the predictor solve from LinearSolverOptions: the
+// direct target and the fallback get max(10 maxiter, 1000) iterations more
+// (CG on this SPD system gains about a factor 3 per iteration at the
+// benchmark sizes, so the direct target takes a few tens).
+static VelTol vel_tol(const fsi_context* c) {
+    VelTol t{};
+    t.rtol = c->num.velocity_rtol;
+    t.fb_rtol = c->num.fallback_rtol;
+    t.maxiter = c->num.velocity_maxiter;
+    const long cap = (long)c->num.velocity_maxiter + std::max(10L * c->num.velocity_maxiter, 1000L);
+    t.cap = (int)std::min(cap, 1L << 30);
+    t.method = c->num.velocity_method;
+    t.fallback = c->num.direct_fallback != 0;
+    return t;
+}
+
 // fixed > 0: enqueue exactly `fixed` iteration kernels without reading the
 // residual back (speculative step); otherwise poll until converged.
 // The predictor of stage `slot` from the right-hand side (ru, rv) into (xu, xv).
@@ -946,7 +965,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     // warm start: stage `slot` of the previous step, extrapolated in time by
     // the start kernel once two (2 u^{n-1} - u^{n-2}) or three
     // (3 (u^{n-1} - u^{n-2}) + u^{n-3}) steps are known, instead of u^n
-    const bool warm = slot >= 0 && c->stage_warm && w->stage_warm >= 1;
+    const bool warm = slot >= 0 && w->stage_warm >= 1;
     if (warm) {
         b.Un = w->vel_cur_u[slot];
         b.Vn = w->vel_cur_v[slot];
@@ -958,20 +977,24 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         b.Wv = w->vel_old2_v[slot];
         b.extrap = std::min(w->stage_warm, 4) - 1;
     }
+    const VelTol vt = vel_tol(c);
+    // gmres: the stopping rule measures against |b - A u^n| (the state velocity)
+    if (vt.method == FSI_SOLVER_GMRES) {
+        b.Su = s->U;
+        b.Sv = s->V;
+    }
     launch_vel_init(c->stream, p, b, alpha, c->st);
-    const double vrtol = c->num.velocity_rtol;
-    const int vmax = c->num.velocity_maxiter;
+    // the last iteration any rule can reach (kernel m tests iterate m)
+    const int last = (vt.method == FSI_SOLVER_DIRECT || vt.fallback) ? vt.cap : vt.maxiter;
     if (fixed > 0) {
-        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vmax + 1), slot, vrtol, vmax,
+        launch_vel_iters(c->stream, p, b, alpha, c->st, 0, std::min(fixed, last + 1), slot, vt,
                          c->vel_check_from[slot]);
         return FSI_OK;
     }
-    // kernel m tests the residual of iterate m and performs update m+1
-    const int last = c->num.velocity_maxiter;
     int m = 0, chunk = c->last_vel_iters[slot] + 1, done = 0, iter = 0;
     while (m <= last) {
         int n = std::min(chunk, last + 1 - m);
-        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot, vrtol, vmax, 1 << 30);
+        launch_vel_iters(c->stream, p, b, alpha, c->st, m, n, slot, vt, 1 << 30);
         m += n;
         fsi_status rc = poll_done(c, &c->st->vel, &done, &iter);
         if (rc != FSI_OK) return rc;
@@ -983,6 +1006,12 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
 }
 
 // Projects (Uin, Vin) into (U, V); the same arrays for a projection in place.
+// The PCG iterates in x (initial guess in, solution out) and the mean-free
+// correction goes to corr[1 - corr_cur]; nothing else of the workspace's
+// projection state is written, so the caller commits (flips corr_cur) only
+// on success.  slot >= 0: a stage projection of a step, iterating in that
+// stage's cq_next from the extrapolated history; slot -1: a standalone
+// projection from the workspace correction.
 static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, double* Vin, double* U, double* V,
                              int inject, int slot, int fixed) {
     const Params& p = c->prm;
@@ -994,9 +1023,14 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
     b.inject = inject;
     b.lap_bc = c->lap_bc;
     b.rhs = w->rhs;
-    b.x = (slot >= 0 && c->stage_warm) ? w->corr_stage[slot] : w->corr[w->corr_cur];
-    if (!c->spec.enabled && slot >= 0 && c->stage_warm && w->stage_warm >= 1)
-        launch_extrapolate(c->stream, p, b.x, w->corr_prev[slot], w->stage_warm >= 2);
+    b.x = slot >= 0 ? w->cq_next[slot] : w->xs;
+    // the initial guess (the direct solve writes x itself)
+    if (!c->spec.enabled) {
+        if (slot >= 0 && w->stage_warm >= 1)
+            launch_extrapolate(c->stream, p, b.x, w->cq_cur[slot], w->cq_prev[slot], w->stage_warm >= 2);
+        else
+            launch_extrapolate(c->stream, p, b.x, w->corr[w->corr_cur], nullptr, 0);
+    }
     b.q_out = w->corr[1 - w->corr_cur];
     b.r = w->r;
     b.z = w->z;
@@ -1013,7 +1047,6 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
         // start): the initial guess is only tested
         launch_proj_tested(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol, last,
                            c->grid.lx * c->grid.ly, slot);
-        w->corr_cur = 1 - w->corr_cur;
         return FSI_OK;
     }
     launch_pcg_init(c->stream, p, c->geo[0], b, c->st, c->has_null, c->num.projection_rtol,
@@ -1028,7 +1061,6 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
                              n - 1, slot);
         launch_pcg_test(c->stream, p, b, c->st, n - 1, slot);
         launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
-        w->corr_cur = 1 - w->corr_cur;
         return FSI_OK;
     }
     int m = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
@@ -1044,7 +1076,6 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
     }
     c->last_proj_iters[ps] = std::max(0, iter);
     launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
-    w->corr_cur = 1 - w->corr_cur;
     return FSI_OK;
 }
 
@@ -1076,9 +1107,8 @@ static fsi_status device_error(fsi_context* c) {
     case DEV_NONHOMOG_NEUMANN:
         return fail(c, FSI_E_ARGUMENT,
                     "Only homogeneous Neumann phase boundaries are supported by the ACDI/UWCDI operators.");
-    case DEV_VEL_NOCONV:
-        return fail(c, FSI_E_SOLVER, "velocity predictor CG residual is too large: %.6e after %d iterations",
-                    s.v0, (int)s.v1);
+    case DEV_VEL_NOCONV:  // _solve_linear_system! :636 (the device's direct target failing included)
+        return fail(c, FSI_E_SOLVER, "velocity stage %d GMRES residual is too large: %.6e", (int)s.v1, s.v0);
     default: return fail(c, FSI_E_HIP, "device reported error code %d", s.code);
     }
 }
@@ -1098,13 +1128,12 @@ static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, 
     status_reset(c);
     fsi_status rc = status_push(c);
     if (rc != FSI_OK) return rc;
-    const int saved = w->corr_cur;
     rc = projection(c, w, s->U, s->V, s->U, s->V, 0, -1, 0);
     if (rc != FSI_OK) return rc;
     if ((rc = status_pull(c)) != FSI_OK) return rc;
     fill_projection_result(c, res);
     rc = device_error(c);
-    if (rc != FSI_OK) w->corr_cur = saved;
+    if (rc == FSI_OK) w->corr_cur = 1 - w->corr_cur;  // copyto!(work.correction, result.correction)
     return rc;
 }
 
@@ -1216,7 +1245,7 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         const int k = si - 1;
         // with the warm-start history the predictor is kept (the next step
         // extrapolates from it) and the projection writes the stage velocity
-        const bool keep = c->stage_warm && p.visc;
+        const bool keep = p.visc;
         double* xu = keep ? w->vel_next_u[k] : w->Us;
         double* xv = keep ? w->vel_next_v[k] : w->Vs;
         if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0, w->rhs_u[k], w->rhs_v[k], xu,
@@ -1235,34 +1264,45 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
 }
 
 // Launch the planned step as a graph, capturing it on first use.  A graph
-// is valid for the buffers it was captured with: the state/stage pair and
-// the correction warm start alternate between steps, so two graphs serve a
-// run at a fixed dt and plan.
+// is valid for the buffers it was captured with: the state/stage pairs, the
+// correction and the warm-start histories rotate between steps, so a few
+// graphs serve a run at a fixed dt and plan.
+static std::vector<uintptr_t> graph_key(const fsi_context* c, const fsi_state* s, const fsi_workspace* w, double dt) {
+    std::vector<uintptr_t> k;
+    uintptr_t bits = 0;
+    std::memcpy(&bits, &dt, sizeof(bits));
+    k.push_back(bits);
+    k.push_back((uintptr_t)w->stage_warm);
+    k.push_back((uintptr_t)w->corr_cur);
+    for (int q = 0; q < 3; ++q) {
+        k.push_back((uintptr_t)c->plan_vel[q]);
+        k.push_back((uintptr_t)c->plan_proj[q]);
+        k.push_back((uintptr_t)c->vel_check_from[q]);
+    }
+    const double* bufs[] = {s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, w->Us, w->Vs, w->phis, w->sxxs, w->syys,
+                            w->sxys};
+    for (const double* b : bufs) k.push_back((uintptr_t)b);
+    for (int q = 0; q < 3; ++q) {
+        const double* h[] = {w->cq_cur[q],     w->cq_prev[q],    w->cq_next[q],    w->vel_cur_u[q],
+                             w->vel_cur_v[q],  w->vel_prev_u[q], w->vel_prev_v[q], w->vel_old_u[q],
+                             w->vel_old_v[q],  w->vel_old2_u[q], w->vel_old2_v[q], w->vel_next_u[q],
+                             w->vel_next_v[q]};
+        for (const double* b : h) k.push_back((uintptr_t)b);
+    }
+    return k;
+}
+
 static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt) {
+    const std::vector<uintptr_t> key = graph_key(c, s, w, dt);
     fsi_workspace::StepGraph* hit = nullptr;
-    for (auto& g : w->graphs) {
-        bool same = g.dt == dt && g.sU == s->U && g.corr_in == w->corr_cur && g.warm == w->stage_warm &&
-                    g.vcur == w->vel_cur_u[0];
-        for (int k = 0; k < 3; ++k)
-            same = same && g.vel[k] == c->plan_vel[k] && g.proj[k] == c->plan_proj[k] &&
-                   g.vchk[k] == c->vel_check_from[k];
-        if (same) {
+    for (auto& g : w->graphs)
+        if (g.key == key) {
             hit = &g;
             break;
         }
-    }
     if (!hit) {
         fsi_workspace::StepGraph g{};
-        for (int k = 0; k < 3; ++k) {
-            g.vel[k] = c->plan_vel[k];
-            g.vchk[k] = c->vel_check_from[k];
-            g.proj[k] = c->plan_proj[k];
-        }
-        g.dt = dt;
-        g.sU = s->U;
-        g.corr_in = w->corr_cur;
-        g.warm = w->stage_warm;
-        g.vcur = w->vel_cur_u[0];
+        g.key = key;
         hipGraph_t graph = nullptr;
         HIPCHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         fsi_status rc = enqueue_step(c, s, w, dt, true);
@@ -1272,8 +1312,6 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
             return rc;
         }
         HIPCHECK(c, e);
-        g.corr_out = w->corr_cur;
-        w->corr_cur = g.corr_in;  // nothing ran yet
         e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
         HIPCHECK(c, e);
@@ -1285,7 +1323,6 @@ static fsi_status run_step_graph(fsi_context* c, fsi_state* s, fsi_workspace* w,
         hit = &w->graphs.back();
     }
     HIPCHECK(c, hipGraphLaunch(hit->exec, c->stream));
-    w->corr_cur = hit->corr_out;
     return FSI_OK;
 }
 
@@ -1305,7 +1342,6 @@ static fsi_status step_check(fsi_context* c, fsi_state* s, fsi_workspace* w, dou
 }
 
 struct StepRun {
-    int saved_corr = 0;
     bool planned = false;  // the planned step and its status read-back are enqueued
 };
 
@@ -1313,14 +1349,8 @@ struct StepRun {
 // read-back of its Status; nothing is awaited, so the steps of several
 // contexts can be in flight together.
 static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, StepRun& run) {
-    run.saved_corr = w->corr_cur;
     run.planned = false;
     fsi_status rc = FSI_OK;
-    if (c->stage_warm && !w->stage_warm) {  // first step: every stage starts from the shared correction
-        for (int k = 0; k < 3; ++k)
-            HIPCHECK(c, hipMemcpyAsync(w->corr_stage[k], w->corr[w->corr_cur], c->felems * sizeof(double),
-                                       hipMemcpyDeviceToDevice, c->stream));
-    }
     if (!(c->use_graph && c->have_history)) return FSI_OK;
     // planned step: every solve gets its planned iteration count (kernels
     // past convergence return at once), one read-back at the end
@@ -1360,7 +1390,6 @@ static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, dou
 static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, const StepRun& run,
                               fsi_step_report* rep, fsi_step_stats* stats) {
     const Params& p = c->prm;
-    const int saved_corr = run.saved_corr;
     fsi_status rc = FSI_OK;
     bool done = false, replayed = false;
     if (run.planned) {
@@ -1374,7 +1403,6 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
             done = replayed = true;
         } else {  // a solve needed more iterations than planned: redo with polling
             ++c->spec_misses;
-            w->corr_cur = saved_corr;
         }
     }
     if (!done) {
@@ -1403,23 +1431,18 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     const Status& st = *c->st_host;
     // error priority follows the reference's order of checks
     if (st.nonfinite_in) {
-        w->corr_cur = saved_corr;
         return fail(c, FSI_E_ARGUMENT, "the state contains non-finite values.");
     }
     if (c->num.check_stage1) {  // _check_stage_one_constraint! :1116-1127
         double tol = c->num.stage1_atol + c->num.stage1_rtol * 1.0;
         if (!(st.stage1_linf <= tol)) {
-            w->corr_cur = saved_corr;
             return fail(c, FSI_E_ARGUMENT,
                         "The velocity entering stage 1 does not satisfy the prescribed divergence constraint. "
                         "Maximum |div(v)-G| = %.6e, tolerance = %.6e. Initialize the state with "
                         "make_velocity_compatible! or provide a compatible velocity.", st.stage1_linf, tol);
         }
     }
-    if ((rc = device_error(c)) != FSI_OK) {
-        w->corr_cur = saved_corr;
-        return rc;
-    }
+    if ((rc = device_error(c)) != FSI_OK) return rc;
     // the fourth stage is the new state (:1232-1238): swap buffers
     std::swap(s->U, w->Us);
     std::swap(s->V, w->Vs);
@@ -1427,7 +1450,15 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     std::swap(s->sxx, w->sxxs);
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
-    if (c->stage_warm) {
+    // commit the workspace: the step's correction, the stage histories
+    w->corr_cur = 1 - w->corr_cur;
+    {
+        for (int k = 0; k < 3; ++k) {  // (cur, prev, next) <- (next, cur, prev)
+            double* q = w->cq_next[k];
+            w->cq_next[k] = w->cq_prev[k];
+            w->cq_prev[k] = w->cq_cur[k];
+            w->cq_cur[k] = q;
+        }
         for (int k = 0; k < 3; ++k) {  // (cur, prev, old, old2, next) <- (next, cur, prev, old, old2)
             double* nu = w->vel_next_u[k];
             double* nv = w->vel_next_v[k];

@@ -96,7 +96,25 @@ struct KrylovScalars {
     double rr, bnorm2, tol2;
     double rzs[2];  // r.z of the last two PCG iterations (by parity)
     int done, iter, maxiter, pad;
+    // velocity predictor only: the stopping rules of _solve_linear_system!
+    // (src/TimeIntegrationIMEX.jl:602-641)
+    double fb2;    // accepted at the iteration limit when r.r <= fb2 (fallback_reltol)
+    double dtol2;  // the direct-solve target of :direct and of the fallback
 };
+
+// Stopping rules of the velocity predictor solve, passed to its kernels.
+//   method 0 (:direct): ||r|| <= DIRECT_RTOL max(||b||, eps), at most `cap` iterations
+//   method 1 (:gmres):  ||r|| <= rtol ||b - A u^n|| (IterativeSolvers' reltol
+//                       rule, x0 = uv_n), at most `maxiter`; then accepted when
+//                       ||r|| <= fb_rtol max(||b||, eps), else (fallback) the
+//                       direct target up to `cap` iterations in all, else an error
+// (the direct target is min(DIRECT_RTOL, fb_rtol) max(||b||, eps))
+struct VelTol {
+    double rtol, fb_rtol;
+    int maxiter, cap;
+    int method, fallback;
+};
+constexpr double VEL_DIRECT_RTOL = 1e-14;
 
 // Projection scalars (src/Projection.jl:170-307).
 struct ProjScalars {

@@ -120,6 +120,8 @@ struct VelBufs {
     const double* Wv;
     int extrap;        // 0: x0 = Un; 1: x0 = 2 Un - Pu; 2: x0 = 3 (Un - Pu) + Ou;
                        // 3: x0 = 4 (Un + Ou) - 6 Pu - Wu
+    const double* Su;  // the state velocity u^n: the gmres rule's reference residual
+    const double* Sv;  //   ||b - A u^n|| (null: not needed, the :direct method)
     const double* ru;  // right-hand side (stage sum + alpha*b)
     const double* rv;
     double* xu;  // solution, written into the stage velocity arrays
@@ -136,7 +138,7 @@ void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double al
 // kernels m0 .. m0+count-1; those with m >= max(check_from, 1) test the
 // previous iterate before loading any field
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
-                      int slot, double rtol, int maxiter, int check_from);
+                      int slot, const VelTol& tol, int check_from);
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
 
 // ---- projection (fsi_solvers.hip) ----
@@ -157,7 +159,9 @@ struct ProjBufs {
     double* partials;
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
-void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap);
+// x = 2 cur - prev (extrap) or x = cur: the initial guess of a projection
+void launch_extrapolate(hipStream_t s, const Params& p, double* x, const double* cur, const double* prev,
+                        int extrap);
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter);
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,

@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, co
     if (st) {  // finest level: convergence test on |r|^2 of the previous PCG kernel
         double tot[1];
         get_totals<1>(rr, nb, tot);
-        if (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter) {
+        if (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter || !(tot[0] <= 1.0e300)) {
             if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
                 st->pcg.done = 1;
                 st->pcg.iter = m;

@@ -157,12 +157,17 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx, co
 
 // Start of a predictor solve: x0 = the warm start with the Dirichlet
 // values, r0 = rhs - (I - alpha M) x0 (zero on Dirichlet rows) and the
-// partials of |r0|^2, |b_F|^2 (free rows).  x0 sits on a halo-1 LDS tile;
-// r0 is formed on the block's own cells only.  p0 = r0 is not stored, and
-// q0 = (I - alpha M) r0 with the first iteration's partials is k_vel_q0's.
+// partials of |r0|^2, |b|^2 and |b - A u^n|^2.  b and A are the system of
+// _impose_row_dirichlet (TimeIntegrationIMEX.jl:649-672): identity rows with
+// the Dirichlet value on the right at U-left / V-bottom, the free rows with
+// their Dirichlet columns kept, so |b - A u^n| is the initial residual
+// IterativeSolvers measures reltol against (x0 = uv_n, :807).  x0 and u^n
+// sit on halo-1 LDS tiles; r0 is formed on the block's own cells only.  p0 =
+// r0 is not stored, and q0 = (I - alpha M) r0 with the first iteration's
+// partials is k_vel_q0's.
 template <bool IN>
 __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b, double alpha, int nb, int i0, int j0,
-                                               int tid, double* sxu, double* sxv) {
+                                               int tid, double* sxu, double* sxv, double* snu, double* snv) {
     using T1 = HTile<1, IN>;
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
     auto wrapped = [&](int ui, int uj, int& gi, int& gj) -> bool {
@@ -205,35 +210,57 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
     }
     x0_batch<0, NI1 / 2, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
     x0_batch<NI1 / 2, NI1, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
+    const bool ref = b.Su != nullptr;
+    if (ref) {  // u^n as it is (no Dirichlet values substituted)
+#pragma unroll
+        for (int it = 0; it < NI1; ++it) {
+            const int k = tid + it * BLOCK;
+            if (k < N1) {
+                const bool in = fl[it] & 1;
+                snu[k] = in ? b.Su[idx1[it]] : 0.0;
+                snv[k] = in ? b.Sv[idx1[it]] : 0.0;
+            }
+        }
+    }
     __syncthreads();
     const T1 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
-    double acc2[2] = {0.0, 0.0};  // |r0|^2, |b_F|^2
+    const T1 Nu{snu, i0, j0, p.nx, p.ny}, Nv{snv, i0, j0, p.nx, p.ny};
+    double acc3[3] = {0.0, 0.0, 0.0};  // |r0|^2, |b|^2, |b - A u^n|^2
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         if (!own[r]) continue;
         const int j = j0 + threadIdx.y + r * TILE_Y;
         const size_t k = kc[r];
-        double ru = 0.0, rv = 0.0;
+        double ru = 0.0, rv = 0.0, eu = 0.0, ev = 0.0;
         if (!dir_u<IN>(p, i)) {
             ru = bu[r] - (Xu(i, j) - alpha * visc_u<IN>(p, Xu, Xv, i, j, 0));
-            acc2[1] += bu[r] * bu[r];
+            if (ref) eu = bu[r] - (Nu(i, j) - alpha * visc_u<IN>(p, Nu, Nv, i, j, 0));
+            acc3[1] += bu[r] * bu[r];
+        } else {
+            if (ref) eu = gul - Nu(i, j);
+            acc3[1] += gul * gul;
         }
         if (!dir_v<IN>(p, j)) {
             rv = bv[r] - (Xv(i, j) - alpha * visc_v<IN>(p, Xu, Xv, i, j, 0));
-            acc2[1] += bv[r] * bv[r];
+            if (ref) ev = bv[r] - (Nv(i, j) - alpha * visc_v<IN>(p, Nu, Nv, i, j, 0));
+            acc3[1] += bv[r] * bv[r];
+        } else {
+            if (ref) ev = gvb - Nv(i, j);
+            acc3[1] += gvb * gvb;
         }
         b.xu[k] = Xu(i, j);
         b.xv[k] = Xv(i, j);
         b.r_u[0][k] = ru;  // p0 = r0: the first iteration reads it from r
         b.r_v[0][k] = rv;
-        acc2[0] += ru * ru + rv * rv;
+        acc3[0] += ru * ru + rv * rv;
+        acc3[2] += eu * eu + ev * ev;
     }
-    put_partials<2>(acc2, pset(b.partials, nb, PS_VELI), nb);
+    put_partials<3>(acc3, pset(b.partials, nb, PS_VELI), nb);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
     constexpr int N1 = HTile<1>::R * HTile<1>::W;
-    __shared__ double sxu[N1], sxv[N1];
+    __shared__ double sxu[N1], sxv[N1], snu[N1], snv[N1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
     const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
@@ -241,8 +268,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
     // the block's own operator rows are not the first or last face / row and
     // their stencils stay inside the halo-1 tile: no boundary rows, no wraps
     const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
-    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
-    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
+    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, snu, snv);
+    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, snu, snv);
 }
 
 // q0 = (I - alpha M) p0 with p0 = r0 (k_vel_start's) and the partials of the
@@ -329,29 +356,45 @@ __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double al
 // convergence is expected) the totals come first, so the launch that sees
 // convergence ends without touching the fields.
 __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
-                                                    int slot, double rtol, int maxiter_arg, int check_first) {
+                                                    int slot, VelTol vt, int check_first) {
     if (st->vel.done) return;
-    // iterate m passes the test or the iteration limit: the solve ends here
-    auto finish = [&](double rr, bool conv, double bnorm2) {
-        if (block_id() == 0 && lane_id() == 0) {
+    // The test of iterate m, a function of the totals, m and the rule's
+    // constants alone, so every block decides the same.  Returns true when
+    // the solve ends here.  :direct: the direct target within `cap`
+    // iterations.  :gmres: reltol * |b - A u^n| within `maxiter`; at the
+    // limit the iterate is accepted below fallback_reltol (:629-631), else
+    // with direct_fallback the solve goes on to the direct target within
+    // `cap` iterations in all (the sparse direct solve of :632-634), else the
+    // solver error (:636).
+    auto test = [&](double rr, double tol2, double fb2, double dtol2) -> bool {
+        bool end = false, ok = false;
+        if (!(rr <= 1.0e300)) {  // NaN or Inf: no rule can be met
+            end = true;
+        } else if (vt.method == 0 || m > vt.maxiter) {
+            ok = rr <= dtol2;
+            end = ok || m >= vt.cap;
+        } else if (rr <= tol2) {
+            end = ok = true;
+        } else if (m == vt.maxiter) {
+            ok = rr <= fb2;
+            end = ok || !vt.fallback;
+        }
+        if (end && block_id() == 0 && lane_id() == 0) {
             st->vel.done = 1;
             st->vel.rr = rr;
             st->vel.iter = m;
             st->vel_iters[slot] = m;
-            if (conv) st->vel_ok[slot] = 1;
-            // fallback_reltol = 1e-6 (LinearSolverOptions, TimeIntegrationIMEX.jl:628-640)
-            if (!conv && rr > 1e-12 * bnorm2) raise_error(st, DEV_VEL_NOCONV, sqrt(rr / bnorm2), (double)m);
+            if (ok) st->vel_ok[slot] = 1;
+            else raise_error(st, DEV_VEL_NOCONV, sqrt(rr / fmax(st->vel.bnorm2, 4.930380657631324e-32)),
+                             (double)(slot + 2));
         }
+        return end;
     };
     double tot[4];
     if (check_first) {
         get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
-        const double rr = tot[3], tol2 = st->vel.tol2;
-        const bool conv = rr <= tol2;
-        if (conv || m >= st->vel.maxiter) {
-            finish(rr, conv, st->vel.bnorm2);
-            return;
-        }
+        const KrylovScalars& sk = st->vel;
+        if (test(tot[3], sk.tol2, sk.fb2, sk.dtol2)) return;
     }
     const int o = m & 1, n = (m + 1) & 1;
     // p0 = r0 is not stored: the first iteration reads p from r
@@ -430,31 +473,28 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     if (!check_first) get_totals<4, 1>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
     // the first iteration sets the solve's scalars from the partials of k_vel_start
-    double tol2, bnorm2;
-    int maxiter;
     if (m == 0) {
-        double ti[2];
-        get_totals<2>(pset(b.partials, nb, PS_VELI), nb, ti);
-        bnorm2 = ti[1];
-        tol2 = rtol * rtol * bnorm2;
-        maxiter = maxiter_arg;
+        double ti[3];
+        get_totals<3>(pset(b.partials, nb, PS_VELI), nb, ti);
+        const double bscale2 = fmax(ti[1], 4.930380657631324e-32);  // max(|b|, eps)^2
+        const double fb2 = vt.fb_rtol * vt.fb_rtol * bscale2;
+        // the direct target, never looser than the acceptance threshold
+        const double dtol2 = fmin(VEL_DIRECT_RTOL * VEL_DIRECT_RTOL * bscale2, fb2);
+        const double tol2 = vt.rtol * vt.rtol * ti[2];
         if (block_id() == 0 && lane_id() == 0) {
             KrylovScalars& sk = st->vel;
-            sk.bnorm2 = bnorm2;
+            sk.bnorm2 = ti[1];
             sk.tol2 = tol2;
-            sk.maxiter = maxiter;
+            sk.maxiter = vt.method == 0 ? vt.cap : vt.maxiter;
+            sk.fb2 = fb2;
+            sk.dtol2 = dtol2;
             sk.iter = 0;
             st->vel_iters[slot] = 0;
         }
-    } else {
-        tol2 = st->vel.tol2;
-        bnorm2 = st->vel.bnorm2;
-        maxiter = st->vel.maxiter;
-    }
-    const bool conv = rr <= tol2, stop = m >= maxiter;
-    if (conv || stop) {
-        finish(rr, conv, bnorm2);
-        return;
+        if (test(rr, tol2, fb2, dtol2)) return;
+    } else if (!check_first) {
+        const KrylovScalars& sk = st->vel;
+        if (test(rr, sk.tol2, sk.fb2, sk.dtol2)) return;
     }
     const double a = rr / pq;
     const double beta = fmax(rr - 2.0 * a * rq + a * a * qq, 0.0) / rr;
@@ -753,7 +793,7 @@ __global__ void k_pcg_check(Status* st, const double* rr, int nb, int m, int slo
     if (st->pcg.done) return;
     double tot[1];
     get_totals<1>(rr, nb, tot);
-    if (threadIdx.x == 0 && (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter)) {
+    if (threadIdx.x == 0 && (tot[0] <= st->pcg.tol2 || m >= st->pcg.maxiter || !(tot[0] <= 1.0e300))) {
         st->pcg.done = 1;
         st->pcg.iter = m;
         st->pcg.rr = tot[0];
@@ -1176,7 +1216,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, in
     }
     st->max_abs_u = acc[3] > 0.0 ? NAN : umax;  // Julia's maximum propagates NaN
     st->max_abs_v = acc[3] > 0.0 ? NAN : vmax;
-    if (pcg_m >= 0 && !st->pcg.done && (acc[0] <= st->pcg.tol2 || pcg_m >= st->pcg.maxiter)) {
+    if (pcg_m >= 0 && !st->pcg.done && (acc[0] <= st->pcg.tol2 || pcg_m >= st->pcg.maxiter || !(acc[0] <= 1.0e300))) {
         st->pcg.done = 1;
         st->pcg.iter = pcg_m;
         st->pcg.rr = acc[0];
@@ -1214,11 +1254,11 @@ void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double al
     k_vel_q0<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p));
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
-                      int slot, double rtol, int maxiter, int check_from) {
+                      int slot, const VelTol& tol, int check_from) {
     const int nb = grid_blocks(p);
     for (int c = 0; c < count; ++c) {
         const int m = m0 + c;
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, rtol, maxiter,
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, tol,
                                                m >= 1 && m >= check_from ? 1 : 0);
     }
 }
@@ -1236,19 +1276,18 @@ void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, in
     k_rz<<<grid2(p), kBlock, 0, s>>>(p, b, rz, nb, done);
 }
 
-// Warm start extrapolated in time: cur = 2 cur - prev (extrap) or unchanged,
-// and prev = the old cur.
-__global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restrict__ cur, double* __restrict__ prev,
-                                                       int extrap) {
+// Initial guess of a projection: x = 2 cur - prev (extrap, linear in time)
+// or x = cur.
+__global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restrict__ x, const double* __restrict__ cur,
+                                                       const double* __restrict__ prev, int extrap) {
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
-        const double c = cur[k];
-        if (extrap) cur[k] = 2.0 * c - prev[k];
-        prev[k] = c;
+        x[k] = extrap ? 2.0 * cur[k] - prev[k] : cur[k];
     }
 }
-void launch_extrapolate(hipStream_t s, const Params& p, double* cur, double* prev, int extrap) {
-    k_extrapolate<<<grid2(p), kBlock, 0, s>>>(p, cur, prev, extrap);
+void launch_extrapolate(hipStream_t s, const Params& p, double* x, const double* cur, const double* prev,
+                        int extrap) {
+    k_extrapolate<<<grid2(p), kBlock, 0, s>>>(p, x, cur, prev, extrap);
 }
 
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b) {
@@ -1350,7 +1389,7 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         return 2 * cell;
     }
     if (n == "vel_iter") {
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0, 0.0, 1 << 30, 0);
+        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0, VelTol{-1.0, 0.0, 1 << 30, 1 << 30, 1, 0}, 0);
         return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
     }
     if (n == "pcg_dir") {

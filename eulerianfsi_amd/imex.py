@@ -179,8 +179,13 @@ class FSIModels:
 
 @dataclass
 class LinearSolverOptions:
-    """:211-252.  On the device the velocity predictor is always CG (the
-    system is SPD); `velocity_method` is kept for API compatibility."""
+    """:211-252.  The device solves the SPD velocity predictor by CG with
+    the stopping rules of `_solve_linear_system!` (:602-641): "direct" to a
+    relative residual of 1e-14 (the exact solve), "gmres" to reltol times
+    the initial residual from uv_n within maxiter iterations, then accepted
+    below fallback_reltol, else the direct target (direct_fallback) or an
+    ErrorException.  `restart` and `resource_method` have no effect on the
+    device path (no restarted Krylov method, no resource solve)."""
     velocity_method: str = "direct"
     resource_method: str = "gmres"
     reltol: float = 1e-8
@@ -201,6 +206,8 @@ class LinearSolverOptions:
             raise ArgumentError("Linear-solver maxiter must be positive.")
         if not self.restart > 0:
             raise ArgumentError("Linear-solver restart must be positive.")
+        if not self.fallback_reltol > 0:
+            raise ArgumentError("fallback_reltol must be positive.")
 
 
 @dataclass

@@ -92,10 +92,13 @@ def _noflux(px=False):
     return BoundaryConditionSetup(Neumann(0.0), Neumann(0.0), Neumann(0.0), Neumann(0.0))
 
 
-def cavity_numerics(velocity_rtol=1e-8, projection_rtol=1e-10, precision="fp64"):
-    """build_numerics (run_sugiyama_lid_driven_cavity.jl:423-448)."""
+def cavity_numerics(velocity_rtol=1e-8, projection_rtol=1e-10, precision="fp64", velocity_method="gmres"):
+    """build_numerics (run_sugiyama_lid_driven_cavity.jl:423-448).  The
+    driver's velocity solve is GMRES to reltol 1e-8; velocity_method="direct"
+    is the reference's default (the exact solve), used for tight parity."""
     return IMEXNumerics(
-        linear_solvers=LinearSolverOptions(velocity_method="gmres", reltol=velocity_rtol, maxiter=800, restart=100),
+        linear_solvers=LinearSolverOptions(velocity_method=velocity_method, reltol=velocity_rtol, maxiter=800,
+                                           restart=100),
         structure_extension=C.GeometricStructureExtension(20.0, 4.0, safety_cutoff=1e-6),
         projection_reltol=projection_rtol,
         projection_maxiter=3000,
@@ -125,7 +128,7 @@ def phase_method(method: str, h: float, acdi_eps_over_h: float = 1.5):
 
 def lid_driven_cavity(N=128, method="acdi", radius=0.2, center=(0.6, 0.5), shear_modulus=0.1, viscosity=0.01,
                       dt=None, velocity_rtol=1e-8, projection_rtol=1e-10, acdi_eps_over_h=1.5,
-                      precision="fp64") -> Case:
+                      precision="fp64", velocity_method="gmres") -> Case:
     """Sugiyama lid-driven cavity with a deformable disk on the unit square."""
     grid = generate_grid(1.0, 1.0, N, N)
     metrics = build_mesh_metrics(grid)
@@ -145,7 +148,7 @@ def lid_driven_cavity(N=128, method="acdi", radius=0.2, center=(0.6, 0.5), shear
     if dt is None:
         dt = 1e-3 * 128.0 / N
     return Case(f"cavity_N{N}_{method}", grid, metrics, bcs, models,
-                cavity_numerics(velocity_rtol, projection_rtol, precision), state, dt,
+                cavity_numerics(velocity_rtol, projection_rtol, precision, velocity_method), state, dt,
                 dict(N=N, method=method, radius=radius, center=tuple(center), shear_modulus=shear_modulus,
                      viscosity=viscosity))
 
@@ -162,7 +165,8 @@ def seeded_cavity(N, method, seed, **kw) -> Case:
 
 
 def kolmogorov_filament(N=1024, method="uwcdi", shear_modulus=0.5, viscosity=0.01, amplitude=0.5,
-                        velocity_rtol=1e-8, projection_rtol=1e-11, precision="fp64") -> Case:
+                        velocity_rtol=1e-8, projection_rtol=1e-11, precision="fp64",
+                        velocity_method="gmres") -> Case:
     """Wall-supported filament in Kolmogorov flow on the unit square: x-periodic,
     no-slip walls at y=0,1, f_x = amplitude*sin(4 pi y), 0.05 x 0.4 filament at
     x=0.5 on the bottom wall.  phi uses Neumann(0) walls (no support layer)."""
@@ -194,7 +198,8 @@ def kolmogorov_filament(N=1024, method="uwcdi", shear_modulus=0.5, viscosity=0.0
     speed = max(1.0, math.sqrt(shear_modulus))
     dt = 0.1 * h / speed
     num = IMEXNumerics(
-        linear_solvers=LinearSolverOptions(velocity_method="gmres", reltol=velocity_rtol, maxiter=800, restart=100),
+        linear_solvers=LinearSolverOptions(velocity_method=velocity_method, reltol=velocity_rtol, maxiter=800,
+                                           restart=100),
         structure_extension=C.GeometricStructureExtension(20.0, 4.0, safety_cutoff=1e-6),
         projection_reltol=projection_rtol, precision=precision)
     return Case(f"filament_N{N}_{method}", grid, metrics, bcs, models, num, state, dt,

@@ -118,6 +118,17 @@ typedef enum {
 
 typedef enum { FSI_PRECISION_FP64 = 0, FSI_PRECISION_MIXED = 1 } fsi_precision;
 
+/* LinearSolverOptions.velocity_method (src/TimeIntegrationIMEX.jl:211-252) */
+typedef enum {
+    FSI_SOLVER_DIRECT = 0, /* :direct (the reference default): the exact solve A \ b; on the device
+                              CG to a relative residual of 1e-14 (the system is SPD with a
+                              condition number 1 + O(dt mu / h^2))                          */
+    FSI_SOLVER_GMRES = 1   /* :gmres: stop at ||b - A x|| <= reltol ||b - A uv_n|| (x0 = uv_n, :807),
+                              at most maxiter iterations; then accepted when the relative
+                              residual is <= fallback_reltol, else the direct solve
+                              (direct_fallback) or ErrorException (:602-641)               */
+} fsi_solver_method;
+
 typedef struct {
     int32_t extension_kind;
     double ext_max_rate, ext_exponent, ext_cutoff;
@@ -131,6 +142,10 @@ typedef struct {
     double stage1_atol, stage1_rtol;
     int32_t check_stage1;
     int32_t precision;         /* fsi_precision */
+    /* LinearSolverOptions.velocity_method / direct_fallback / fallback_reltol */
+    int32_t velocity_method;   /* fsi_solver_method */
+    int32_t direct_fallback;
+    double fallback_rtol;
 } fsi_numerics;
 
 /* ---- results ------------------------------------------------------------ */

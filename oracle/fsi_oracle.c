@@ -1228,7 +1228,19 @@ static double dot(const double* a, const double* b, size_t n) {
 /* (I - alpha M) x = rhs + alpha b with Dirichlet rows: _build_velocity_stage!
  * :794-812, _shifted_identity :643-647, _impose_row_dirichlet :649-672.  The
  * identity rows fix x_D = g; the remaining rows form an SPD system in x_F
- * (M is symmetric negative semidefinite on uniform grids) solved by CG. */
+ * (M is symmetric negative semidefinite on uniform grids) solved by CG.
+ *
+ * Stopping rules of _solve_linear_system! :602-641 (x0 = uv_n, :807):
+ *   :direct  the exact solve A \ b, here CG to |r| <= 1e-14 max(|b|, eps);
+ *   :gmres   IterativeSolvers' reltol rule, |r| <= reltol |b - A x0| (its
+ *            initial residual), within maxiter iterations; then accepted
+ *            when |r| / max(|b|, eps) <= fallback_reltol (:629-631), else the
+ *            direct solve (direct_fallback, :632-634: the direct target within
+ *            maxiter + max(10 maxiter, 1000) iterations in all) or the
+ *            ErrorException of :636.
+ * b and A are the whole system of _impose_row_dirichlet: the Dirichlet rows
+ * enter |b| with their value and |b - A x0| with g - x0.  A non-finite
+ * residual is an error at once. */
 int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const double* x0,
                         double* x, int* iters) {
     int N2 = 2 * c->N;
@@ -1240,18 +1252,47 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
     double* zero = dalloc(N2);
     viscous_apply(c, zero, tmp, 1); /* boundary vector b = M*0 + b */
     for (int k = 0; k < N2; ++k) b[k] = rhs_in[k] + alpha * tmp[k];
+    /* |b| and |b - A x0| of the system with identity Dirichlet rows */
+    double bn2 = 0.0, ref2 = 0.0;
+    viscous_apply(c, x0, tmp, 0);
+    for (int k = 0; k < N2; ++k) {
+        double bk = mask[k] ? g[k] : b[k];
+        double ek = mask[k] ? g[k] - x0[k] : b[k] - (x0[k] - alpha * tmp[k]);
+        bn2 += bk * bk;
+        ref2 += ek * ek;
+    }
+    const double bscale = sqrt(bn2) > 2.220446049250313e-16 ? sqrt(bn2) : 2.220446049250313e-16;
+    const double fb = c->num.fallback_rtol * bscale;
+    double dtol = 1e-14 * bscale;
+    if (dtol > fb) dtol = fb;
+    const int maxiter = c->num.velocity_maxiter;
+    const int cap = maxiter + (10 * maxiter > 1000 ? 10 * maxiter : 1000);
+    const int direct = c->num.velocity_method == FSI_SOLVER_DIRECT;
+    double tol = direct ? dtol : c->num.velocity_rtol * sqrt(ref2);
+    int limit = direct ? cap : maxiter, phase = direct;
     for (int k = 0; k < N2; ++k) x[k] = mask[k] ? g[k] : x0[k];
     viscous_apply(c, x, tmp, 0);
     for (int k = 0; k < N2; ++k) r[k] = mask[k] ? 0.0 : b[k] - (x[k] - alpha * tmp[k]);
-    double bn = 0.0;
-    for (int k = 0; k < N2; ++k)
-        if (!mask[k]) bn += b[k] * b[k];
-    bn = sqrt(bn);
-    double tol = c->num.velocity_rtol * (bn > 0 ? bn : 1.0);
     memcpy(p, r, sizeof(double) * N2);
     double rr = dot(r, r, N2);
-    int it = 0;
-    while (sqrt(rr) > tol && it < c->num.velocity_maxiter) {
+    int it = 0, rc = 0;
+    for (;;) {
+        if (!isfinite(rr)) {
+            rc = set_err(c, FSI_E_SOLVER, "velocity predictor residual is not finite");
+            break;
+        }
+        if (sqrt(rr) <= tol) break;
+        if (it >= limit) {
+            if (!phase && sqrt(rr) <= fb) break;           /* accepted (:629-631) */
+            if (!phase && c->num.direct_fallback) {       /* the direct solve (:632-634) */
+                phase = 1;
+                tol = dtol;
+                limit = cap;
+                continue;
+            }
+            rc = set_err(c, FSI_E_SOLVER, "velocity GMRES residual is too large: %.6e", sqrt(rr) / bscale);
+            break;
+        }
         viscous_apply(c, p, tmp, 0);
         for (int k = 0; k < N2; ++k) q[k] = mask[k] ? 0.0 : p[k] - alpha * tmp[k];
         double pq = dot(p, q, N2);
@@ -1265,7 +1306,7 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
     }
     if (iters) *iters = it;
     free(mask); free(g); free(b); free(r); free(p); free(q); free(tmp); free(zero);
-    return 0;
+    return rc;
 }
 
 /* ---------------------------------------------------------------------------
@@ -1818,8 +1859,9 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
                 free(mask); free(gv);
             } else {
                 int it = 0;
-                fsio_velocity_solve(c, dt * AI[s][s], rhs, uv_n, pred, &it);
+                rc = fsio_velocity_solve(c, dt * AI[s][s], rhs, uv_n, pred, &it);
                 if (stats) stats->velocity_iterations[s - 1] = it;
+                if (rc) break;
             }
             /* _inject_compact_predictor! :543-579 */
             for (int j = 0; j < ny; ++j)

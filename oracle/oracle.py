@@ -7,6 +7,7 @@ never imports this module.
 from __future__ import annotations
 
 import ctypes
+import os
 import subprocess
 from ctypes import POINTER, c_char_p, c_double, c_int, c_void_p
 from pathlib import Path
@@ -16,12 +17,16 @@ import numpy as np
 from eulerianfsi_amd import _abi
 
 HERE = Path(__file__).resolve().parent
-LIB_PATH = HERE / "liboracle_fsi.so"
+# FSI_ORACLE_LIB selects another build of the same sources (a planted-mistake
+# copy, say) for the CPU tests
+LIB_PATH = Path(os.environ["FSI_ORACLE_LIB"]) if os.environ.get("FSI_ORACLE_LIB") else HERE / "liboracle_fsi.so"
 _dp = POINTER(c_double)
 _LIB = None
 
 
 def build_oracle(verbose=False) -> Path:
+    if os.environ.get("FSI_ORACLE_LIB"):
+        return LIB_PATH
     subprocess.run(["make", "-C", str(HERE), "-s"], check=True, capture_output=not verbose)
     return LIB_PATH
 

@@ -42,3 +42,138 @@ def oracle_for(cfg):
     from oracle.oracle import Oracle
     _, metrics, bcs, models, numerics = cfg
     return Oracle(metrics, bcs, models, numerics)
+
+
+# ---- closed-form workloads of the step's implicit half and of convection ----------------
+def cn_composition_factor(z):
+    """One step of the implicit half on y' = lambda y, z = dt lambda.  The
+    stage abscissae c = (0, 8/15, 2/3, 1) are the row sums of the explicit
+    tableau; the implicit part of the fractional step is one Crank-Nicolson
+    substep over each abscissa increment (8/15, 2/15, 1/3), so the
+    amplification factor is prod_k (1 + dc_k z/2) / (1 - dc_k z/2)."""
+    c = np.array([0.0, 8.0 / 15.0, 2.0 / 3.0, 1.0])
+    R = 1.0
+    for dc in np.diff(c):
+        R *= (1.0 + 0.5 * dc * z) / (1.0 - 0.5 * dc * z)
+    return R
+
+
+def shear_mode(N=16, mu=0.05, amp=0.3, z=-1.5, mode=1):
+    """Doubly periodic shear wave U = amp sin(2 pi mode y), V = 0 with
+    ConstantViscosity(mu) and no other closure: convection ((u.grad) u = 0,
+    u^2 uniform along x, v = 0), elastic force and projection (div = 0) all
+    vanish, and the viscous operator acts on the mode with the eigenvalue
+    mu lambda_h, lambda_h = -4 sin^2(pi mode h)/h^2.  dt is chosen so that
+    z = dt mu lambda_h; one step multiplies U by cn_composition_factor(z).
+    Returns (cfg, state, dt, factor)."""
+    cfg = periodic_box(N, models=fsi.FSIModels(viscosity=C.ConstantViscosity(mu)))
+    grid = cfg[0]
+    h = 1.0 / N
+    lam = -4.0 * np.sin(np.pi * mode * h) ** 2 / h ** 2
+    dt = z / (mu * lam)
+    U = np.asfortranarray(np.tile(amp * np.sin(2 * np.pi * mode * grid.y_p), (N + 1, 1)))
+    state = fsi.FSIState(U, np.zeros((N, N + 1), order="F"), np.zeros((N, N), order="F"),
+                         np.ones((N, N), order="F"), np.ones((N, N), order="F"), np.zeros((N, N), order="F"))
+    return cfg, state, dt, cn_composition_factor(z)
+
+
+def couette(N=16, gamma=0.8, mu=0.05):
+    """Plane Couette flow between a wall at rest (y = 0) and a wall moving at
+    gamma (y = 1), x-periodic: U = gamma y at the face centres is a steady
+    state of the step only when the wall lifting alpha_s b_visc enters every
+    stage's right-hand side (M u + b = 0 for the linear profile)."""
+    cfg = shear_channel(N, gamma, models=fsi.FSIModels(viscosity=C.ConstantViscosity(mu)))
+    grid = cfg[0]
+    U = np.asfortranarray(np.tile(gamma * grid.y_p, (N + 1, 1)))
+    state = fsi.FSIState(U, np.zeros((N, N + 1), order="F"), np.zeros((N, N), order="F"),
+                         np.ones((N, N), order="F"), np.ones((N, N), order="F"), np.zeros((N, N), order="F"))
+    return cfg, state
+
+
+def _from_streamfunction(psi, N):
+    """MAC velocity of a corner streamfunction (N+1, N+1): U = d psi/dy,
+    V = -d psi/dx, discretely divergence-free."""
+    U = (psi[:, 1:] - psi[:, :-1]) * N
+    V = -(psi[1:, :] - psi[:-1, :]) * N
+    return np.asfortranarray(U), np.asfortranarray(V)
+
+
+def taylor_green(N):
+    """Doubly periodic Taylor-Green vortex psi = sin(2 pi x) sin(2 pi y) / (2 pi):
+    u = sin(2 pi x) cos(2 pi y), v = -cos(2 pi x) sin(2 pi y), and
+    (v.grad) v = (pi sin(4 pi x), pi sin(4 pi y)).  Returns
+    (cfg, U, V, Cu_exact, Cv_exact) with the exact term at the faces."""
+    cfg = periodic_box(N)
+    g = cfg[0]
+    X, Y = np.meshgrid(np.arange(N + 1) / N, np.arange(N + 1) / N, indexing="ij")
+    psi = np.sin(2 * np.pi * X) * np.sin(2 * np.pi * Y) / (2 * np.pi)
+    U, V = _from_streamfunction(psi, N)
+    XU, YU = np.meshgrid(np.arange(N + 1) / N, g.y_p, indexing="ij")
+    XV, YV = np.meshgrid(g.x_p, np.arange(N + 1) / N, indexing="ij")
+    Cu = np.pi * np.sin(4 * np.pi * XU)
+    Cv = np.pi * np.sin(4 * np.pi * YV)
+    return cfg, U, V, Cu, Cv
+
+
+def wall_channel_flow(N, gamma=0.6, w0=0.25):
+    """x-periodic channel with walls at y = 0, 1, the top one moving at gamma,
+    and a uniform transpiration w0 through both (Dirichlet V = w0):
+    psi = gamma y^2/2 - w0 x + sin(2 pi x) sin^2(pi y) / (2 pi), so
+    u = gamma y + sin(2 pi x) g'(y) / (2 pi), v = w0 - cos(2 pi x) g(y) with
+    g = sin^2(pi y): u = 0 at y = 0, u = gamma at y = 1, v = w0 on both
+    walls, and the tangential wall data enter the corner fluxes u v.
+    Returns (cfg, U, V, Cu_exact, Cv_exact) with (v.grad) v at the faces
+    (the wall-normal V rows are zero in the reference, MomentumTransport.jl:322-334)."""
+    grid = fsi.generate_grid(1.0, 1.0, N, N)
+    metrics = fsi.build_mesh_metrics(grid, periodic_x=True)
+    p, n0 = fsi.Periodic(), fsi.Neumann(0.0)
+    bcU = fsi.BoundaryConditionSetup(p, p, fsi.Dirichlet(0.0), fsi.Dirichlet(gamma))
+    bcV = fsi.BoundaryConditionSetup(p, p, fsi.Dirichlet(w0), fsi.Dirichlet(w0))
+    # the advected fields need Dirichlet data on the inflow wall
+    d0, d1 = fsi.Dirichlet(0.0), fsi.Dirichlet(1.0)
+    sc = fsi.BoundaryConditionSetup(p, p, n0, n0)
+    bphi = fsi.BoundaryConditionSetup(p, p, d0, d0)
+    bs1, bs0 = fsi.BoundaryConditionSetup(p, p, d1, d1), fsi.BoundaryConditionSetup(p, p, d0, d0)
+    bcs = fsi.FSIBoundaryConditions(bcU, bcV, sc, bphi, {"xx": bs1, "yy": bs1, "xy": bs0})
+    cfg = (grid, metrics, bcs, fsi.FSIModels(), fsi.IMEXNumerics())
+    g = grid
+    X, Y = np.meshgrid(np.arange(N + 1) / N, np.arange(N + 1) / N, indexing="ij")
+    psi = gamma * Y ** 2 / 2 + np.sin(2 * np.pi * X) * np.sin(np.pi * Y) ** 2 / (2 * np.pi)
+    U, V = _from_streamfunction(psi, N)
+    V = np.asfortranarray(V + w0)  # the uniform part -w0 x of psi (not periodic) added directly
+
+    def terms(x, y):
+        s, sp = np.sin(2 * np.pi * x), 2 * np.pi * np.cos(2 * np.pi * x)
+        gg, g1, g2 = np.sin(np.pi * y) ** 2, np.pi * np.sin(2 * np.pi * y), 2 * np.pi ** 2 * np.cos(2 * np.pi * y)
+        # psi_y = gamma y + s(x) g'(y) / (2 pi), -psi_x = w0 - cos(2 pi x) g(y)
+        u = gamma * y + s * g1 / (2 * np.pi)
+        v = w0 - np.cos(2 * np.pi * x) * gg
+        ux, uy = sp * g1 / (2 * np.pi), gamma + s * g2 / (2 * np.pi)
+        vx, vy = 2 * np.pi * s * gg, -np.cos(2 * np.pi * x) * g1
+        return u * ux + v * uy, u * vx + v * vy
+
+    XU, YU = np.meshgrid(np.arange(N + 1) / N, g.y_p, indexing="ij")
+    XV, YV = np.meshgrid(g.x_p, np.arange(N + 1) / N, indexing="ij")
+    Cu = terms(XU, YU)[0]
+    Cv = terms(XV, YV)[1]
+    Cv[:, 0] = Cv[:, -1] = 0.0
+    return cfg, U, V, Cu, Cv
+
+
+def wall_channel_flow_x(N, gamma=0.6, w0=0.25):
+    """wall_channel_flow with x and y exchanged: y-periodic, walls at x = 0, 1
+    with V = 0 and V = gamma, transpiration U = w0; exercises the tangential
+    V data of the left and right walls (MomentumTransport.jl:105-139)."""
+    cfg, U, V, Cu, Cv = wall_channel_flow(N, gamma, w0)
+    grid = fsi.generate_grid(1.0, 1.0, N, N)
+    metrics = fsi.build_mesh_metrics(grid, periodic_y=True)
+    b = cfg[2]
+
+    def tr(s):  # (left, right, bottom, top) -> (bottom, top, left, right)
+        return fsi.BoundaryConditionSetup(s.bottom, s.top, s.left, s.right)
+
+    bcs = fsi.FSIBoundaryConditions(tr(b.V), tr(b.U), tr(b.correction), tr(b.phi),
+                                    {k: tr(v) for k, v in b.sigma.items()})
+    cfg = (grid, metrics, bcs, fsi.FSIModels(), fsi.IMEXNumerics())
+    return (cfg, np.asfortranarray(V.T), np.asfortranarray(U.T), np.asfortranarray(Cv.T),
+            np.asfortranarray(Cu.T))

@@ -13,10 +13,10 @@ from tests.helpers import compact_u, compact_v, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-TIGHT = dict(velocity_rtol=1e-14, projection_rtol=1e-14)
+TIGHT = dict(velocity_method="direct", projection_rtol=1e-14)
 # at N = 256 the oracle's projection PCG cannot attain 1e-14 on every
 # right-hand side (it breaks down at its floor); 1e-13 still resolves 1e-10
-TIGHT256 = dict(velocity_rtol=1e-14, projection_rtol=1e-13)
+TIGHT256 = dict(velocity_method="direct", projection_rtol=1e-13)
 FIELDS = ("U", "V", "phi", "sigma_xx", "sigma_yy", "sigma_xy")
 
 
@@ -197,7 +197,7 @@ def test_inviscid_cavity_steps_match_oracle():
     stage velocities before projection; five tight steps against the oracle."""
     import dataclasses
     from eulerianfsi_amd import closures as C
-    case = setups.seeded_cavity(64, "acdi", seed=9, velocity_rtol=1e-14, projection_rtol=1e-12)
+    case = setups.seeded_cavity(64, "acdi", seed=9, velocity_method="direct", projection_rtol=1e-12)
     case.models = dataclasses.replace(case.models, viscosity=C.NoViscosity())
     errs, rep, rep_o, *_ = _run_pair(case, 5)
     for k, e in enumerate(errs):

@@ -102,9 +102,9 @@ def test_steps_match_multigrid_projection(monkeypatch, method):
     def run(flag):
         monkeypatch.setenv("FSI_SPECTRAL", flag)
         if method == "acdi":
-            case = setups.seeded_cavity(128, method, seed=3, velocity_rtol=1e-14, projection_rtol=1e-14)
+            case = setups.seeded_cavity(128, method, seed=3, velocity_method="direct", projection_rtol=1e-14)
         else:
-            case = setups.kolmogorov_filament(128, method, velocity_rtol=1e-14, projection_rtol=1e-14)
+            case = setups.kolmogorov_filament(128, method, velocity_method="direct", projection_rtol=1e-14)
         ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
         work = fsi.allocate_imex_workspace(ctx)
         fsi.make_velocity_compatible(case.state, ctx, work)

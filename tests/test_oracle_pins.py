@@ -333,7 +333,7 @@ def test_projection_is_idempotent_and_keeps_solenoidal_fields():
 def test_cavity_step_conserves_phase_volume(method):
     from eulerianfsi_amd import setups
     from oracle.oracle import Oracle, state_dict
-    case = setups.seeded_cavity(32, method, seed=3, velocity_rtol=1e-13, projection_rtol=1e-13)
+    case = setups.seeded_cavity(32, method, seed=3, velocity_method="direct", projection_rtol=1e-13)
     o = Oracle(case.metrics, case.bcs, case.models, case.numerics)
     s = state_dict(case.state)
     v0 = s["phi"].sum()
@@ -449,3 +449,170 @@ def test_laplacian_and_pinned_projection_match_assembly(corr):
     assert np.allclose(U.ravel(order="F"), U_ref, atol=1e-11)
     assert np.allclose(V.ravel(order="F"), V_ref, atol=1e-11)
     assert np.allclose(o.correction.ravel(order="F"), x, atol=1e-11)
+
+
+# ---- stopping rules of the predictor solve (_solve_linear_system!, TimeIntegrationIMEX.jl:602-641)
+def _assembled_predictor(name, alpha, **ls):
+    """The oracle for one viscous configuration with LinearSolverOptions(**ls)
+    and the assembled system of _shifted_identity + _impose_row_dirichlet."""
+    cfg = VISC_CFGS[name]()
+    _, metrics, bcs, models, _ = cfg
+    cfg = (cfg[0], cfg[1], cfg[2], cfg[3], fsi.IMEXNumerics(linear_solvers=fsi.LinearSolverOptions(**ls)))
+    o = cases.oracle_for(cfg)
+    m = RA.Met(o)
+    M, b = RA.viscous_operator(m, models.viscosity.value, bcs.U, bcs.V, metrics.periodic_x, metrics.periodic_y)
+    n = 2 * o.nx * o.ny
+    A = np.eye(n) - alpha * M.toarray()
+    idx, val = RA.dirichlet_rows(o.nx, o.ny, bcs.U, bcs.V)
+    for k in idx:
+        A[k, :] = 0.0
+        A[k, k] = 1.0
+
+    def system(rhs):
+        r = rhs + alpha * b
+        for k, v in zip(idx, val):
+            r[k] = v
+        return r
+
+    return o, A, system
+
+
+def test_gmres_rule_is_relative_to_the_initial_residual():
+    """reltol is measured against |b - A x0| with x0 = uv_n (IterativeSolvers'
+    gmres!: tolerance = reltol * the initial residual), not against |b|: from
+    a start 1e-6 away from the solution, reltol 1e-8 still needs iterations
+    and ends with |b - A x| <= 1e-8 |b - A x0|."""
+    alpha = 0.7
+    o, A, system = _assembled_predictor("cavity", alpha, velocity_method="gmres", reltol=1e-8, maxiter=800)
+    rng = np.random.default_rng(5)
+    rhs = rng.standard_normal(A.shape[0])
+    bb = system(rhs)
+    x_ref = np.linalg.solve(A, bb)
+    x0 = x_ref + 1e-6 * rng.standard_normal(x_ref.size)
+    ref = np.linalg.norm(bb - A @ x0)
+    assert ref < 1e-3 * np.linalg.norm(bb)  # the old |b|-relative rule would stop at once
+    x, it = o.velocity_solve(alpha, rhs, x0)
+    res = np.linalg.norm(bb - A @ x)
+    print(it, res / ref)
+    assert it > 0
+    assert res <= 1.0001e-8 * ref
+
+
+def test_gmres_limit_accepts_below_fallback_reltol():
+    """At maxiter the iterate is kept when |b - A x| / max(|b|, eps) <=
+    fallback_reltol (:629-631), whatever reltol asked for."""
+    alpha = 0.7
+    o, A, system = _assembled_predictor("cavity", alpha, velocity_method="gmres", reltol=1e-15, maxiter=3,
+                                        fallback_reltol=0.5, direct_fallback=False)
+    rhs = np.random.default_rng(6).standard_normal(A.shape[0])
+    bb = system(rhs)
+    x, it = o.velocity_solve(alpha, rhs)
+    assert it == 3
+    rel = np.linalg.norm(bb - A @ x) / np.linalg.norm(bb)
+    assert 1e-15 < rel <= 0.5
+
+
+def test_gmres_limit_falls_back_to_the_direct_solve():
+    """Above fallback_reltol at maxiter: direct_fallback gives the direct
+    solution (:632-634), otherwise ErrorException (:636)."""
+    from oracle.oracle import OracleError
+    alpha = 0.7
+    o, A, system = _assembled_predictor("cavity", alpha, velocity_method="gmres", reltol=1e-8, maxiter=1,
+                                        fallback_reltol=1e-6, direct_fallback=True)
+    rhs = np.random.default_rng(7).standard_normal(A.shape[0])
+    bb = system(rhs)
+    x_ref = np.linalg.solve(A, bb)
+    x, it = o.velocity_solve(alpha, rhs)
+    assert it > 1
+    assert np.linalg.norm(bb - A @ x) <= 1e-13 * np.linalg.norm(bb)
+    assert np.allclose(x, x_ref, rtol=0, atol=1e-12 * np.max(np.abs(x_ref)))
+    o2, _, _ = _assembled_predictor("cavity", alpha, velocity_method="gmres", reltol=1e-8, maxiter=1,
+                                    fallback_reltol=1e-6, direct_fallback=False)
+    with pytest.raises(OracleError, match="too large"):
+        o2.velocity_solve(alpha, rhs)
+
+
+def test_direct_method_is_the_exact_solve():
+    """velocity_method = :direct (the reference's default): the solution of
+    A x = b to rounding, from any start and independent of reltol."""
+    alpha = 0.7
+    for name in ("cavity", "channel"):
+        o, A, system = _assembled_predictor(name, alpha, reltol=1e-2)
+        rhs = np.random.default_rng(8).standard_normal(A.shape[0])
+        bb = system(rhs)
+        x_ref = np.linalg.solve(A, bb)
+        x, _ = o.velocity_solve(alpha, rhs, np.ones_like(rhs))
+        assert np.linalg.norm(bb - A @ x) <= 1e-13 * np.linalg.norm(bb)
+        assert np.allclose(x, x_ref, rtol=0, atol=1e-13 * np.max(np.abs(x_ref)))
+
+
+def test_nonfinite_predictor_raises():
+    from oracle.oracle import OracleError
+    o, A, _ = _assembled_predictor("cavity", 0.7, velocity_method="gmres")
+    rhs = np.zeros(A.shape[0])
+    rhs[5] = np.nan
+    with pytest.raises(OracleError, match="not finite"):
+        o.velocity_solve(0.7, rhs)
+
+
+# ---- closed forms of the implicit half and of material convection ----------------------
+@pytest.mark.parametrize("z", [-0.3, -1.5, -6.0])
+def test_implicit_half_is_three_crank_nicolson_substeps(z):
+    """A periodic shear wave under viscosity alone is multiplied by the
+    stability function of the implicit tableau, three Crank-Nicolson
+    substeps over the abscissa increments (TimeIntegrationIMEX.jl:80-85,
+    :777-812): fixes every A_I entry (a wrong coefficient changes R(z) at
+    first order in z) and the shift alpha_s = dt a_ss of each solve."""
+    from oracle.oracle import state_dict
+    cfg, state, dt, R = cases.shear_mode(16, z=z)
+    s = state_dict(state)
+    U0 = s["U"].copy()
+    cases.oracle_for(cfg).integrate_step(s, 0.0, dt)
+    assert abs(R - 1.0) > 0.05
+    assert np.allclose(s["U"], R * U0, rtol=0, atol=1e-13 * np.max(np.abs(U0)))
+    assert np.max(np.abs(s["V"])) <= 1e-14
+
+
+def test_couette_flow_is_steady():
+    """The linear Couette profile solves M u + b = 0 only through the wall
+    lifting b of the viscous operator, so it stays at rest only if
+    alpha_s b_visc enters each stage's right-hand side (:788-789) and the
+    explicit viscous terms dt aI[s][j] I_j are included (:780-784)."""
+    from oracle.oracle import state_dict
+    cfg, state = cases.couette(16)
+    s = state_dict(state)
+    U0 = s["U"].copy()
+    o = cases.oracle_for(cfg)
+    for k in range(3):
+        o.integrate_step(s, 0.01 * k, 0.01)
+    assert np.max(np.abs(s["U"] - U0)) <= 1e-13
+    assert np.max(np.abs(s["V"])) <= 1e-13
+
+
+def _convection_errors(make, Ns):
+    errs = []
+    for N in Ns:
+        cfg, U, V, Cu, Cv = make(N)
+        o = cases.oracle_for(cfg)
+        one, zero = np.ones((N, N), order="F"), np.zeros((N, N), order="F")
+        E = o.tendencies(dict(U=U, V=V, phi=zero, sigma_xx=one, sigma_yy=one, sigma_xy=zero))
+        # E_u = -material_convection (no force, no elastic stress): the sign of
+        # the explicit momentum term (TimeIntegrationIMEX.jl:962-988)
+        eu = np.max(np.abs(E["E_u"][:-1, :] + Cu[:-1, :]))
+        ev = np.max(np.abs(E["E_v"][:, :-1] + Cv[:, :-1]))
+        errs.append(max(eu, ev))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("name", ["taylor_green", "wall_channel_flow", "wall_channel_flow_x"])
+def test_material_convection_converges_to_the_exact_term(name):
+    """The explicit momentum tendency of a divergence-free field converges at
+    second order, in the max norm over every face row including those next
+    to the no-slip walls and the moving lid (MomentumTransport.jl:252-338):
+    a flipped sign or a wrong wall datum leaves an O(1) error."""
+    make = getattr(cases, name)
+    errs = _convection_errors(make, [16, 32, 64, 128])
+    rates = np.log2(errs[:-1] / errs[1:])
+    print(name, errs, rates)
+    assert errs[-1] < 2e-2
+    assert np.all(rates > 1.8), rates
