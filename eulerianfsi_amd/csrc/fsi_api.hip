@@ -79,6 +79,7 @@ struct fsi_context {
     // HIP graph (measured slower on ROCm 7 for the ~460-launch step), mode 0
     // polls each solve.
     int use_graph = 2;
+    int vel_warm = 1;  // predictor warm starts from the stage histories (FSI_STAGE_WARM=0: from u^n)
     int have_history = 0;  // iteration counts of a completed step are known
     int plan_vel[3] = {0, 0, 0};
     int vel_check_from[3] = {0, 0, 0};  // the fewest iterations of the same history
@@ -323,6 +324,8 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     {  // FSI_STEP_GRAPH=0 keeps every step on the polling path
         const char* e = std::getenv("FSI_STEP_GRAPH");  // 0 polling, 1 planned graph, 2 planned direct
         if (e && (e[0] == '0' || e[0] == '1' || e[0] == '2')) c->use_graph = e[0] - '0';
+        e = std::getenv("FSI_STAGE_WARM");
+        if (e && e[0] == '0') c->vel_warm = 0;
         e = std::getenv("FSI_SPECTRAL");
         if (e && e[0] == '0') c->use_spectral = 0;
     }
@@ -965,7 +968,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
     // warm start: stage `slot` of the previous step, extrapolated in time by
     // the start kernel once two (2 u^{n-1} - u^{n-2}) or three
     // (3 (u^{n-1} - u^{n-2}) + u^{n-3}) steps are known, instead of u^n
-    const bool warm = slot >= 0 && w->stage_warm >= 1;
+    const bool warm = slot >= 0 && c->vel_warm && w->stage_warm >= 1;
     if (warm) {
         b.Un = w->vel_cur_u[slot];
         b.Vn = w->vel_cur_v[slot];

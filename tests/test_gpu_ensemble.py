@@ -67,3 +67,28 @@ def test_ensemble_reports_an_instance_error_and_advances_the_others():
         fsi.integrate_step(ref[i].state, 0.0, ref[i].dt, ctx, work)
         assert np.array_equal(cases[i].state.U, ref[i].state.U)
         assert not np.array_equal(cases[i].state.U, before[i])
+
+
+def test_config4_ensemble_equals_single_instance_steps():
+    """The 64-cavity workload of the batched ensemble (N ~ U{128, 192, 256,
+    384}, G log-uniform [0.05, 0.5], disk r ~ U[0.15, 0.25], centre ~
+    U[0.4, 0.6]^2, ACDI, seed 4): three batched steps, every instance
+    bitwise equal to the same steps taken one instance at a time."""
+    nsteps = 3
+    alone = setups.ensemble_cases()
+    assert len(alone) == 64 and {c.params["N"] for c in alone} == {128, 192, 256, 384}
+    env = _prepare(alone)
+    for c, (ctx, work) in zip(alone, env):
+        for k in range(nsteps):
+            fsi.integrate_step(c.state, k * c.dt, c.dt, ctx, work)
+    ref = [{f: getattr(c.state, f).copy() for f in FIELDS} for c in alone]
+    del env
+    together = setups.ensemble_cases()
+    env = _prepare(together)
+    for k in range(nsteps):
+        reps = fsi.ensemble_step([c.state for c in together], [k * c.dt for c in together],
+                                 [c.dt for c in together], [e[0] for e in env], [e[1] for e in env])
+        assert all(r is not None for r in reps)
+    for i, c in enumerate(together):
+        for f in FIELDS:
+            assert np.array_equal(getattr(c.state, f), ref[i][f]), (i, f)

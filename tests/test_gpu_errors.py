@@ -1,0 +1,217 @@
+"""Failure handling of the device step, forced from the host (every kernel
+exits normally): each error maps to the reference's exception, the device
+state and the workspace correction are left as they were, and a later
+valid step is unaffected by the failed attempt.  Also the predictor
+solve's stopping rules (_solve_linear_system!, TimeIntegrationIMEX.jl:602-641)
+on the device: the direct fallback, the error without it, and the default
+LinearSolverOptions() (:direct)."""
+import dataclasses
+import os
+
+import numpy as np
+import pytest
+
+import eulerianfsi_amd as fsi
+from eulerianfsi_amd import closures as C
+from eulerianfsi_amd import setups
+from tests.helpers import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("U", "V", "phi", "sigma_xx", "sigma_yy", "sigma_xy")
+
+
+def _device(case, numerics=None):
+    if os.environ.get("FSI_TEST_DRYRUN") == "1":
+        pytest.skip("device only")
+    ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, numerics or case.numerics)
+    return ctx, fsi.allocate_imex_workspace(ctx)
+
+
+def _device_fields(state):
+    """The state as the device holds it (forces a download)."""
+    state._device_modified()
+    return {f: getattr(state, f).copy() for f in FIELDS}
+
+
+def _assert_same(a, b):
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f], equal_nan=True), f
+
+
+def test_nonfinite_state_leaves_workspace_intact():
+    """48 x 48 (not a power of two: the multigrid-PCG projection with its
+    per-stage warm-start histories).  A NaN in phi raises ArgumentError and
+    changes neither the device state nor the correction; after the valid
+    state is restored, the next step gives the same bits as a run that never
+    saw the failure (the failed attempt committed no warm start)."""
+    def fresh():
+        case = setups.lid_driven_cavity(48, "acdi", velocity_method="direct", projection_rtol=1e-12)
+        ctx, work = _device(case)
+        fsi.make_velocity_compatible(case.state, ctx, work)
+        for k in range(4):
+            fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+        return case, ctx, work
+
+    a, actx, awork = fresh()
+    fsi.integrate_step(a.state, 4 * a.dt, a.dt, actx, awork)
+    ref5 = _device_fields(a.state)
+
+    b, ctx, work = fresh()
+    assert any(i > 0 for i in work.last_stats.projection_iterations)  # the PCG path, not a direct solve
+    good = _device_fields(b.state)
+    corr0 = work.correction
+    bad_phi = good["phi"].copy()
+    bad_phi[10, 7] = np.nan
+    b.state.phi = bad_phi
+    bad = {**good, "phi": bad_phi}
+    with pytest.raises(fsi.ArgumentError, match="non-finite"):
+        fsi.integrate_step(b.state, 4 * b.dt, b.dt, ctx, work)
+    _assert_same(_device_fields(b.state), bad)
+    assert np.array_equal(work.correction, corr0)
+    b.state.phi = good["phi"]
+    fsi.integrate_step(b.state, 4 * b.dt, b.dt, ctx, work)
+    _assert_same(_device_fields(b.state), ref5)
+
+
+def test_divergent_velocity_fails_the_stage1_check():
+    """A velocity that was never projected: the stage-1 constraint check
+    raises ArgumentError (:1111-1127) and the state is unchanged."""
+    case = setups.seeded_cavity(64, "acdi", seed=2)
+    rng = np.random.default_rng(3)
+    U = case.state.U.copy()
+    U[1:-1, :] = 0.01 * rng.standard_normal((63, 64))
+    case.state.U = U
+    ctx, work = _device(case)
+    case.state._device(ctx)
+    before = _device_fields(case.state)
+    assert np.array_equal(before["U"], U)
+    with pytest.raises(fsi.ArgumentError, match="stage 1"):
+        fsi.integrate_step(case.state, 0.0, case.dt, ctx, work)
+    _assert_same(_device_fields(case.state), before)
+
+
+def test_non_spd_sigma_is_a_domain_error():
+    """GeometricStructureExtension needs an SPD tensor (StructureTensor.jl:
+    482-515): a cell inside the disk with det sigma < 0 raises DomainError
+    and the state is unchanged."""
+    case = setups.seeded_cavity(64, "acdi", seed=2)
+    sxy = case.state.sigma_xy.copy()
+    i, j = np.unravel_index(np.argmin(np.abs(case.state.phi - 0.5)), case.state.phi.shape)
+    sxy[i, j] = 2.0  # sigma = [[1, 2], [2, 1]]: eigenvalue -1, in the interface (extension rate > 0)
+    case.state.sigma_xy = sxy
+    ctx, work = _device(case)
+    fsi.make_velocity_compatible(case.state, ctx, work)
+    before = _device_fields(case.state)
+    with pytest.raises(fsi.DomainError):
+        fsi.integrate_step(case.state, 0.0, case.dt, ctx, work)
+    _assert_same(_device_fields(case.state), before)
+
+
+def test_projection_iteration_limit_keeps_the_correction():
+    """project! with projection_maxiter = 1 on the multigrid-PCG path (48 x
+    40): the true-residual check fails (ErrorException), and neither the
+    velocity nor the workspace correction changes."""
+    from tests.test_gpu_spectral import _config, _velocity
+    metrics, bcs, models, num = _config(48, 40, "NNNN", 1e-10)
+    num = dataclasses.replace(num, projection_maxiter=1)
+    if os.environ.get("FSI_TEST_DRYRUN") == "1":
+        pytest.skip("device only")
+    ctx = fsi.build_imex_context(metrics, bcs, models, num)
+    work = fsi.allocate_imex_workspace(ctx)
+    corr0 = np.asfortranarray(np.random.default_rng(4).standard_normal((48, 40)))
+    work.correction = corr0
+    U, V = _velocity(48, 40, "NNNN", seed=5)
+    one, zero = np.ones((48, 40)), np.zeros((48, 40))
+    state = fsi.FSIState(U.copy(order="F"), V.copy(order="F"), 0.5 * one, one, one, zero)
+    state._device(ctx)
+    before = _device_fields(state)
+    with pytest.raises(fsi.SolverError):
+        fsi.project(state, ctx, work)
+    _assert_same(_device_fields(state), before)
+    assert np.array_equal(work.correction, corr0)
+
+
+def _gmres_numerics(case, **ls):
+    base = case.numerics.linear_solvers
+    return dataclasses.replace(case.numerics, linear_solvers=dataclasses.replace(base, **ls))
+
+
+def test_velocity_direct_fallback_matches_oracle():
+    """velocity_method = :gmres with maxiter = 1: above fallback_reltol the
+    reference falls back to the sparse direct solve (:632-634); the device
+    continues CG to the direct target and the steps match the oracle, which
+    does the same, to 1e-10."""
+    from oracle.oracle import Oracle, state_dict
+    case = setups.seeded_cavity(64, "acdi", seed=4, projection_rtol=1e-13)
+    num = _gmres_numerics(case, velocity_method="gmres", reltol=1e-8, maxiter=1, direct_fallback=True)
+    o = Oracle(case.metrics, case.bcs, case.models, num)
+    ref = state_dict(case.state)
+    traj = []
+    for k in range(3):
+        o.integrate_step(ref, k * case.dt, case.dt)
+        traj.append({f: ref[f].copy() for f in FIELDS})
+    ctx, work = _device(case, num)
+    for k in range(3):
+        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+        its = work.last_stats.velocity_iterations
+        print(k, work.last_stats, flush=True)
+        assert all(i > 1 for i in its)
+        for f in FIELDS:
+            e = rel_l2(getattr(case.state, f), traj[k][f])
+            assert e <= 1e-10, (k, f, e)
+
+
+def test_velocity_without_fallback_is_a_solver_error():
+    """direct_fallback = false: the residual after maxiter iterations above
+    fallback_reltol is an ErrorException (:636); the state is unchanged."""
+    case = setups.seeded_cavity(64, "acdi", seed=4)
+    num = _gmres_numerics(case, velocity_method="gmres", reltol=1e-8, maxiter=1, direct_fallback=False)
+    ctx, work = _device(case, num)
+    fsi.make_velocity_compatible(case.state, ctx, work)
+    before = _device_fields(case.state)
+    with pytest.raises(fsi.SolverError, match="GMRES residual is too large"):
+        fsi.integrate_step(case.state, 0.0, case.dt, ctx, work)
+    _assert_same(_device_fields(case.state), before)
+
+
+def test_default_solver_options_match_oracle():
+    """LinearSolverOptions() as the reference defines it (velocity_method =
+    :direct, the exact solve): three steps against the oracle."""
+    from oracle.oracle import Oracle, state_dict
+    case = setups.seeded_cavity(64, "cdi", seed=6, projection_rtol=1e-13)
+    num = dataclasses.replace(case.numerics, linear_solvers=fsi.LinearSolverOptions())
+    assert num.linear_solvers.velocity_method == "direct"
+    o = Oracle(case.metrics, case.bcs, case.models, num)
+    ref = state_dict(case.state)
+    traj = []
+    for k in range(3):
+        o.integrate_step(ref, k * case.dt, case.dt)
+        traj.append({f: ref[f].copy() for f in FIELDS})
+    ctx, work = _device(case, num)
+    for k in range(3):
+        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+        for f in FIELDS:
+            e = rel_l2(getattr(case.state, f), traj[k][f])
+            assert e <= 1e-10, (k, f, e)
+
+
+def test_gmres_rule_tracks_the_exact_step():
+    """At the drivers' reltol 1e-8 the device measures the residual against
+    |b - A u^n| (the velocity change of a stage), not |b| (the velocity
+    itself): five steps stay much closer to the exact-solve oracle than
+    1e-8 of the velocity."""
+    from oracle.oracle import Oracle, state_dict
+    case = setups.seeded_cavity(64, "acdi", seed=8, projection_rtol=1e-13)
+    exact = dataclasses.replace(case.numerics, linear_solvers=fsi.LinearSolverOptions())
+    o = Oracle(case.metrics, case.bcs, case.models, exact)
+    ref = state_dict(case.state)
+    for k in range(5):
+        o.integrate_step(ref, k * case.dt, case.dt)
+    ctx, work = _device(case)
+    for k in range(5):
+        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+    vel = np.sqrt(np.sum(ref["U"] ** 2) + np.sum(ref["V"] ** 2))
+    duv = np.sqrt(np.sum((case.state.U - ref["U"]) ** 2) + np.sum((case.state.V - ref["V"]) ** 2)) / vel
+    print("gmres 1e-8 vs exact: relative velocity difference", duv, work.last_stats, flush=True)
+    assert duv <= 1e-9

@@ -343,3 +343,60 @@ def test_mixed_precision_step_matches_fp64(N):
         err = rel_l2(a[f], b[f])
         print(f, f"{err:.2e}", flush=True)
         assert err <= 1e-10, (f, err)
+
+
+# ---- closed forms on the device ------------------------------------------------------------
+@pytest.mark.parametrize("z", [-0.3, -1.5, -6.0])
+def test_device_shear_wave_follows_the_implicit_stability_function(z):
+    """The device step multiplies a periodic viscous shear wave by the
+    stability function of the implicit tableau (three Crank-Nicolson
+    substeps over the abscissa increments), independently of the oracle."""
+    from tests import cases
+    cfg, state, dt, R = cases.shear_mode(64, z=z)
+    U0 = state.U.copy()
+    if os.environ.get("FSI_TEST_DRYRUN") == "1":
+        pytest.skip("device only")
+    ctx = fsi.build_imex_context(cfg[1], cfg[2], cfg[3], fsi.IMEXNumerics(projection_reltol=1e-13))
+    work = fsi.allocate_imex_workspace(ctx)
+    fsi.integrate_step(state, 0.0, dt, ctx, work)
+    err = np.max(np.abs(state.U - R * U0)) / np.max(np.abs(U0))
+    print(z, R, err, flush=True)
+    assert err <= 1e-12
+    assert np.max(np.abs(state.V)) <= 1e-13
+
+
+def test_device_couette_flow_is_steady():
+    from tests import cases
+    cfg, state = cases.couette(64)
+    U0 = state.U.copy()
+    if os.environ.get("FSI_TEST_DRYRUN") == "1":
+        pytest.skip("device only")
+    ctx = fsi.build_imex_context(cfg[1], cfg[2], cfg[3], fsi.IMEXNumerics(projection_reltol=1e-13))
+    work = fsi.allocate_imex_workspace(ctx)
+    for k in range(3):
+        fsi.integrate_step(state, 0.01 * k, 0.01, ctx, work)
+    assert np.max(np.abs(state.U - U0)) <= 1e-12
+    assert np.max(np.abs(state.V)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["taylor_green", "wall_channel_flow", "wall_channel_flow_x"])
+def test_device_convection_converges_to_the_exact_term(name):
+    """The device's explicit momentum tendency (-material convection) of a
+    divergence-free field converges at second order to -(v.grad) v in the
+    max norm over every face row, walls and transpiration included."""
+    from tests import cases
+    if os.environ.get("FSI_TEST_DRYRUN") == "1":
+        pytest.skip("device only")
+    errs = []
+    for N in (32, 64, 128, 256):
+        cfg, U, V, Cu, Cv = getattr(cases, name)(N)
+        ctx = fsi.build_imex_context(cfg[1], cfg[2], cfg[3], cfg[4])
+        work = fsi.allocate_imex_workspace(ctx)
+        one, zero = np.ones((N, N)), np.zeros((N, N))
+        state = fsi.FSIState(U, V, zero, one, one, zero)
+        E = fsi.evaluate_stage_tendencies(state, ctx, work)
+        errs.append(max(np.max(np.abs(E["E_u"] + Cu[:-1, :])), np.max(np.abs(E["E_v"] + Cv[:, :-1]))))
+    errs = np.array(errs)
+    rates = np.log2(errs[:-1] / errs[1:])
+    print(name, errs, rates, flush=True)
+    assert np.all(rates > 1.8), rates

@@ -24,6 +24,14 @@ CONFIGS = {
     "periodic": (32, 32, "PPPP"),
     "neumann_128x256": (128, 256, "NNNN"),
     "fallback_48x40": (48, 40, "NNNN"),  # not a power of two: multigrid-PCG alone
+    # the benchmark sizes: row and column transforms of 1024 and 2048 points
+    # (dynamic LDS above 64 KiB), and columns of 2048 points, which take the
+    # one-column-pair variant of the column kernel
+    "neumann_1024": (1024, 1024, "NNNN"),
+    "dirichlet_1024": (1024, 1024, "DDDD"),
+    "neumann_2048x64": (2048, 64, "NNNN"),
+    "dirichlet_16x2048": (16, 2048, "DDDD"),
+    "periodic_16x2048": (16, 2048, "PPPP"),
 }
 
 
