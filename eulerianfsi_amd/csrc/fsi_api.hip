@@ -986,7 +986,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         b.Su = s->U;
         b.Sv = s->V;
     }
-    launch_vel_init(c->stream, p, b, alpha, c->st);
+    launch_vel_init(c->stream, p, b, alpha, c->st, slot, vt);
     // the last iteration any rule can reach (kernel m tests iterate m)
     const int last = (vt.method == FSI_SOLVER_DIRECT || vt.fallback) ? vt.cap : vt.maxiter;
     if (fixed > 0) {
@@ -1496,6 +1496,10 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
         stats->planned = replayed ? 1 : 0;
         stats->max_abs_u = st.max_abs_u;
         stats->max_abs_v = st.max_abs_v;
+        for (int k = 0; k < 3; ++k) {
+            stats->velocity_residual[k] = p.visc ? st.vel_res[k] : 0.0;
+            stats->velocity_reference_residual[k] = p.visc ? st.vel_ref[k] : 0.0;
+        }
     }
     return FSI_OK;
 }

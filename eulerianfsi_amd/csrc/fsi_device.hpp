@@ -100,6 +100,7 @@ struct KrylovScalars {
     // (src/TimeIntegrationIMEX.jl:602-641)
     double fb2;    // accepted at the iteration limit when r.r <= fb2 (fallback_reltol)
     double dtol2;  // the direct-solve target of :direct and of the fallback
+    double ref2;   // |b - A u^n|^2 (:gmres)
 };
 
 // Stopping rules of the velocity predictor solve, passed to its kernels.
@@ -139,6 +140,8 @@ struct Status {
     double max_abs_u, max_abs_v;  // face-velocity extrema after the last projection
     int vel_iters[3], proj_iters[3];
     int vel_ok[3], proj_ok[3];  // 1 once the solve of that stage has converged
+    // per predictor solve: final |r| / max(|b|, eps) and |b - A u^n| / max(|b|, eps)
+    double vel_res[3], vel_ref[3];
     KrylovScalars vel;
     KrylovScalars pcg;
     ProjScalars proj;

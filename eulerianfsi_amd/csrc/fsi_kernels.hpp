@@ -134,7 +134,9 @@ struct VelBufs {
     double* q_v[2];
     double* partials;
 };
-void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st);
+// the start of a predictor solve (x0, r0, q0 and the stopping thresholds of `tol`)
+void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
+                     const VelTol& tol);
 // kernels m0 .. m0+count-1; those with m >= max(check_from, 1) test the
 // previous iterate before loading any field
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,

@@ -274,8 +274,28 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
 
 // q0 = (I - alpha M) p0 with p0 = r0 (k_vel_start's) and the partials of the
 // first iteration (p.q, r.q, q.q, r.r): r0 on a halo-1 LDS tile, the block's
-// own cells and the halo ring as in k_vel_iter.
-__global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double alpha, int nb) {
+// own cells and the halo ring as in k_vel_iter.  Block 0 also sets the
+// solve's stopping thresholds from k_vel_start's totals (|b|^2, |b - A u^n|^2).
+__global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double alpha, int nb, Status* st, int slot,
+                                                  VelTol vt) {
+    if (block_id() == 0) {
+        double ti[3];
+        get_totals<3>(pset(b.partials, nb, PS_VELI), nb, ti);
+        if (lane_id() == 0) {
+            const double bscale2 = fmax(ti[1], 4.930380657631324e-32);  // max(|b|, eps)^2
+            const double fb2 = vt.fb_rtol * vt.fb_rtol * bscale2;
+            KrylovScalars& sk = st->vel;
+            sk.bnorm2 = ti[1];
+            sk.ref2 = ti[2];
+            sk.tol2 = vt.rtol * vt.rtol * ti[2];
+            sk.maxiter = vt.method == 0 ? vt.cap : vt.maxiter;
+            sk.fb2 = fb2;
+            // the direct target, never looser than the acceptance threshold
+            sk.dtol2 = fmin(VEL_DIRECT_RTOL * VEL_DIRECT_RTOL * bscale2, fb2);
+            sk.iter = 0;
+            st->vel_iters[slot] = 0;
+        }
+    }
     __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
@@ -384,6 +404,9 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
             st->vel.rr = rr;
             st->vel.iter = m;
             st->vel_iters[slot] = m;
+            const double bs2 = fmax(st->vel.bnorm2, 4.930380657631324e-32);
+            st->vel_res[slot] = sqrt(rr / bs2);
+            st->vel_ref[slot] = sqrt(st->vel.ref2 / bs2);
             if (ok) st->vel_ok[slot] = 1;
             else raise_error(st, DEV_VEL_NOCONV, sqrt(rr / fmax(st->vel.bnorm2, 4.930380657631324e-32)),
                              (double)(slot + 2));
@@ -472,27 +495,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     // (one partial in flight at a time: the field loads hold the registers)
     if (!check_first) get_totals<4, 1>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
-    // the first iteration sets the solve's scalars from the partials of k_vel_start
-    if (m == 0) {
-        double ti[3];
-        get_totals<3>(pset(b.partials, nb, PS_VELI), nb, ti);
-        const double bscale2 = fmax(ti[1], 4.930380657631324e-32);  // max(|b|, eps)^2
-        const double fb2 = vt.fb_rtol * vt.fb_rtol * bscale2;
-        // the direct target, never looser than the acceptance threshold
-        const double dtol2 = fmin(VEL_DIRECT_RTOL * VEL_DIRECT_RTOL * bscale2, fb2);
-        const double tol2 = vt.rtol * vt.rtol * ti[2];
-        if (block_id() == 0 && lane_id() == 0) {
-            KrylovScalars& sk = st->vel;
-            sk.bnorm2 = ti[1];
-            sk.tol2 = tol2;
-            sk.maxiter = vt.method == 0 ? vt.cap : vt.maxiter;
-            sk.fb2 = fb2;
-            sk.dtol2 = dtol2;
-            sk.iter = 0;
-            st->vel_iters[slot] = 0;
-        }
-        if (test(rr, tol2, fb2, dtol2)) return;
-    } else if (!check_first) {
+    if (!check_first) {
         const KrylovScalars& sk = st->vel;
         if (test(rr, sk.tol2, sk.fb2, sk.dtol2)) return;
     }
@@ -1249,9 +1252,10 @@ __global__ void __launch_bounds__(BLOCK) k_copy(Params p, const double* __restri
 static dim3 grid_of(int nx, int ny) { return dim3((nx + TILE_X - 1) / TILE_X, (ny + TILE_ROWS - 1) / TILE_ROWS); }
 static const dim3 kBlock(TILE_X, TILE_Y);
 
-void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st) {
+void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
+                     const VelTol& tol) {
     k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
-    k_vel_q0<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p));
+    k_vel_q0<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p), st, slot, tol);
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot, const VelTol& tol, int check_from) {

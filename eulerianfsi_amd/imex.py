@@ -269,6 +269,10 @@ class StepStats:
     # max |U|, max |V| over the faces of the new state (NaN if a face is NaN)
     max_abs_u: float = float("nan")
     max_abs_v: float = float("nan")
+    # per predictor solve: final |b - A x| / max(|b|, eps) and the initial
+    # residual from uv_n on the same scale (the :gmres rule's reference)
+    velocity_residual: tuple = (0.0, 0.0, 0.0)
+    velocity_reference_residual: tuple = (0.0, 0.0, 0.0)
 
 
 # ---------------------------------------------------------------------------
@@ -392,7 +396,8 @@ def _project(state, ctx, work, fn, what):
 
 def _step_stats(s) -> StepStats:
     return StepStats(tuple(s.velocity_iterations), tuple(s.projection_iterations), bool(s.planned),
-                     float(s.max_abs_u), float(s.max_abs_v))
+                     float(s.max_abs_u), float(s.max_abs_v), tuple(s.velocity_residual),
+                     tuple(s.velocity_reference_residual))
 
 
 def integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work: IMEXWorkspace) -> StepReport:

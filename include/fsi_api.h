@@ -182,6 +182,11 @@ typedef struct {
      * formed by the step's last projection check, so they cost no launch */
     double max_abs_u;
     double max_abs_v;
+    /* per predictor solve (stages 2..4): the final residual |b - A x| / max(|b|, eps)
+     * and the initial residual from uv_n, |b - A uv_n| / max(|b|, eps), which
+     * the :gmres rule scales reltol by (0 for :direct); 0 without viscosity */
+    double velocity_residual[3];
+    double velocity_reference_residual[3];
 } fsi_step_stats;
 
 typedef struct fsi_context fsi_context;

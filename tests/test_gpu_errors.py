@@ -110,8 +110,10 @@ def test_non_spd_sigma_is_a_domain_error():
 
 def test_projection_iteration_limit_keeps_the_correction():
     """project! with projection_maxiter = 1 on the multigrid-PCG path (48 x
-    40): the true-residual check fails (ErrorException), and neither the
-    velocity nor the workspace correction changes."""
+    40): the true-residual check fails (ErrorException) and the workspace
+    correction is unchanged.  The velocity is updated, as the reference's
+    project! subtracts the gradient before its checks throw
+    (Projection.jl:270-297)."""
     from tests.test_gpu_spectral import _config, _velocity
     metrics, bcs, models, num = _config(48, 40, "NNNN", 1e-10)
     num = dataclasses.replace(num, projection_maxiter=1)
@@ -126,9 +128,12 @@ def test_projection_iteration_limit_keeps_the_correction():
     state = fsi.FSIState(U.copy(order="F"), V.copy(order="F"), 0.5 * one, one, one, zero)
     state._device(ctx)
     before = _device_fields(state)
-    with pytest.raises(fsi.SolverError):
+    with pytest.raises(fsi.SolverError, match="true-residual"):
         fsi.project(state, ctx, work)
-    _assert_same(_device_fields(state), before)
+    after = _device_fields(state)
+    assert not np.array_equal(after["U"], before["U"])
+    for f in ("phi", "sigma_xx", "sigma_yy", "sigma_xy"):
+        assert np.array_equal(after[f], before[f])
     assert np.array_equal(work.correction, corr0)
 
 
@@ -196,22 +201,32 @@ def test_default_solver_options_match_oracle():
             assert e <= 1e-10, (k, f, e)
 
 
-def test_gmres_rule_tracks_the_exact_step():
+def test_gmres_rule_is_relative_to_the_velocity_change():
     """At the drivers' reltol 1e-8 the device measures the residual against
-    |b - A u^n| (the velocity change of a stage), not |b| (the velocity
-    itself): five steps stay much closer to the exact-solve oracle than
-    1e-8 of the velocity."""
+    |b - A u^n|, the size of a stage's velocity change, not |b| (the velocity
+    itself): from a developed flow (20 exact steps), one step at reltol 1e-8
+    differs from the exact-solve step by about 1e-8 of the step's velocity
+    change; a |b|-relative rule would allow errors about 1e-8 of the
+    velocity, some hundred times more here."""
     from oracle.oracle import Oracle, state_dict
     case = setups.seeded_cavity(64, "acdi", seed=8, projection_rtol=1e-13)
     exact = dataclasses.replace(case.numerics, linear_solvers=fsi.LinearSolverOptions())
     o = Oracle(case.metrics, case.bcs, case.models, exact)
     ref = state_dict(case.state)
-    for k in range(5):
+    for k in range(20):
         o.integrate_step(ref, k * case.dt, case.dt)
+    start = {f: ref[f].copy() for f in ref}
+    o.integrate_step(ref, 20 * case.dt, case.dt)
+    state = fsi.FSIState(*(start[f] for f in ("U", "V", "phi", "resource", "sigma_xx", "sigma_yy", "sigma_xy")))
     ctx, work = _device(case)
-    for k in range(5):
-        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
-    vel = np.sqrt(np.sum(ref["U"] ** 2) + np.sum(ref["V"] ** 2))
-    duv = np.sqrt(np.sum((case.state.U - ref["U"]) ** 2) + np.sum((case.state.V - ref["V"]) ** 2)) / vel
-    print("gmres 1e-8 vs exact: relative velocity difference", duv, work.last_stats, flush=True)
-    assert duv <= 1e-9
+    fsi.integrate_step(state, 20 * case.dt, case.dt, ctx, work)
+
+    def vnorm(U, V):
+        return np.sqrt(np.sum(U ** 2) + np.sum(V ** 2))
+
+    change = vnorm(ref["U"] - start["U"], ref["V"] - start["V"])
+    err = vnorm(state.U - ref["U"], state.V - ref["V"])
+    vel = vnorm(ref["U"], ref["V"])
+    print("velocity", vel, "step change", change, "gmres 1e-8 vs exact", err, work.last_stats, flush=True)
+    assert change < 0.05 * vel
+    assert err <= 1e-8 * change
