@@ -271,6 +271,7 @@ def _declare(lib):
         "fsi_benchmark_kernel": (c_int, [P, P, P, c_char_p, c_int32, _dp, _dp]),
         "fsi_velocity_extrema": (c_int, [P, P, _dp, _dp]),
         "fsi_phase_centroid": (c_int, [P, P, _dp, _dp, _dp]),
+        "fsi_phase_centroid_noalloc": (c_int, [P, P, c_void_p, c_void_p]),
         "fsi_diagnostics_compute": (c_int, [P, P, POINTER(DiagParamsC), POINTER(DiagnosticsC)]),
         "fsi_version": (c_char_p, []),
     }
@@ -286,7 +287,8 @@ EXPORTED = (
     "fsi_state_upload", "fsi_state_download", "fsi_workspace_create", "fsi_workspace_destroy",
     "fsi_workspace_download_correction", "fsi_workspace_upload_correction", "fsi_make_velocity_compatible",
     "fsi_integrate_step", "fsi_ensemble_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel",
-    "fsi_velocity_extrema", "fsi_phase_centroid", "fsi_diagnostics_compute", "fsi_version",
+    "fsi_velocity_extrema", "fsi_phase_centroid", "fsi_phase_centroid_noalloc", "fsi_diagnostics_compute",
+    "fsi_version",
 )
 
 

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,7 @@ struct fsi_context {
     int has_reset = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    int device = -1;  // the HIP device current at creation
     double* force_row = nullptr;
     double* ext = nullptr;  // per-block extrema of the report and checks (EX_* slots)
     double* diag = nullptr;      // per-block partials of the driver diagnostics
@@ -200,6 +202,8 @@ static const char* side_name(int k) {
     return n[k];
 }
 
+static fsi_status scratch_lookup(fsi_context* c);
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -336,6 +340,8 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(c, FSI_E_HIP, "no HIP device is available");
     HIPCHECK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHECK(c, hipGetDevice(&c->device));
+    if ((rc = scratch_lookup(c)) != FSI_OK) return rc;
 
     const int nx = grid->nx, ny = grid->ny;
     c->P = ((nx + 1 + 31) / 32) * 32;
@@ -592,6 +598,48 @@ static fsi_status centroid_of(fsi_context* c, const double* v, double& xc, doubl
     if (!(mass > 2.220446049250313e-16)) return fail(c, FSI_E_SOLVER, "Phase integral is too small to define a centroid.");
     xc = v[1] / mass;
     yc = v[2] / mass;
+    return FSI_OK;
+}
+
+// ---- the allocation-free centroid ------------------------------------------
+// `scratch` holds the per-block partials of k_diag_state (DIAG_STATE_VALUES
+// rows of up to FSI_NOALLOC_MAX_TILES values) and, after them, the values of
+// its final reduction.  Every block writes its own slot and the final
+// reduction writes every value, so nothing needs clearing between calls.  In
+// host code `scratch` only names the symbol: kernels get its device address,
+// looked up once per device with hipGetSymbolAddress (fsi_context_create).
+constexpr size_t kScratchValues = (size_t)DIAG_STATE_VALUES * FSI_NOALLOC_MAX_TILES + DIAG_STATE_VALUES;
+static __device__ __align__(256) double scratch[kScratchValues];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mutex;
+
+static fsi_status scratch_lookup(fsi_context* c) {
+    if (c->device < 0 || c->device >= kMaxDevices)
+        return fail(c, FSI_E_HIP, "device index %d is beyond the %d the library tracks", c->device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[c->device]) return FSI_OK;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(c, FSI_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[c->device] = static_cast<double*>(ptr);
+    return FSI_OK;
+}
+
+extern "C" fsi_status fsi_phase_centroid_noalloc(fsi_context* c, const fsi_state* s, double* result, void* stream) {
+    if (!c || !s || s->ctx != c || !result) return FSI_E_ARGUMENT;
+    if (c->prm.exnb > FSI_NOALLOC_MAX_TILES)
+        return fail(c, FSI_E_ARGUMENT,
+                    "fsi_phase_centroid_noalloc covers grids of at most %d tiles of 64 x 16 cells; %d x %d has %d.",
+                    FSI_NOALLOC_MAX_TILES, c->grid.nx, c->grid.ny, c->prm.exnb);
+    double* buf = g_scratch_ptr[c->device];  // set when the context was created
+    if (!buf) return fail(c, FSI_E_HIP, "the device scratch of fsi_phase_centroid_noalloc was not looked up");
+    hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    double* reduced = buf + (size_t)DIAG_STATE_VALUES * FSI_NOALLOC_MAX_TILES;
+    launch_diag_state(hs, c->prm, s->U, s->V, s->phi, c->grid.lx, c->grid.ly, 1.0, buf, reduced);
+    launch_centroid_result(hs, reduced, result);
+    HIPCHECK(c, hipGetLastError());
     return FSI_OK;
 }
 

@@ -107,6 +107,9 @@ void launch_diag_tensor(hipStream_t s, const Params& p, const double* phi, const
 void launch_diag_geometry(hipStream_t s, const Params& p, const double* phi, double lx, double ly, double xc,
                           double yc, int n0, int nmax, double* buf, double* out);
 int diag_modes_per_launch();
+// (x, y, phase integral) of the centroid from the reduced values of
+// launch_diag_state, the centroid NaN when the integral is not above eps
+void launch_centroid_result(hipStream_t s, const double* reduced, double* result);
 
 // ---- velocity predictor CG (fsi_solvers.hip) ----
 struct VelBufs {

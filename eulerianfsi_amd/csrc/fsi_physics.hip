@@ -1762,4 +1762,18 @@ void launch_diag_geometry(hipStream_t s, const Params& p, const double* phi, dou
 }
 int diag_modes_per_launch() { return DG2_MODES; }
 
+// phase_centroid (run_sugiyama_lid_driven_cavity.jl:496-509) from the sums
+// of k_diag_state: mass = sum phi dA, x = sum x phi dA / mass, y likewise.
+__global__ void k_centroid_result(const double* __restrict__ v, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    const double mass = v[0];
+    const bool ok = mass > 2.220446049250313e-16;
+    out[0] = ok ? v[1] / mass : NAN;
+    out[1] = ok ? v[2] / mass : NAN;
+    out[2] = mass;
+}
+void launch_centroid_result(hipStream_t s, const double* reduced, double* result) {
+    k_centroid_result<<<1, 64, 0, s>>>(reduced, result);
+}
+
 }  // namespace fsi

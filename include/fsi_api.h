@@ -288,6 +288,22 @@ fsi_status fsi_velocity_extrema(fsi_context* ctx, const fsi_state* st, double* m
 fsi_status fsi_phase_centroid(fsi_context* ctx, const fsi_state* st, double* x_centroid,
                               double* y_centroid, double* phase_integral);
 
+/* The phase centroid of fsi_phase_centroid computed without allocating
+ * anything: no counterpart in the reference.  For callers that cannot
+ * allocate device memory or synchronise (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The reductions use the
+ * library's static device scratch instead of the context's buffers, and the
+ * call enqueues its two kernels on `stream` (a hipStream_t; NULL: the
+ * context's stream) and returns: `result` (device memory, 3 doubles) receives
+ * x_centroid, y_centroid, phase_integral once the stream reaches it, the
+ * centroid NaN where fsi_phase_centroid reports FSI_E_SOLVER.  Grids of at
+ * most FSI_NOALLOC_MAX_TILES tiles of 64 x 16 cells (4096 x 4096, or any
+ * ceil(nx/64) * ceil(ny/16) <= 16384); larger ones give FSI_E_ARGUMENT.  The
+ * scratch (about 2.2 MB per GPU, held until the process ends) is shared by
+ * every call on a GPU: calls that overlap on one GPU race. */
+#define FSI_NOALLOC_MAX_TILES 16384
+fsi_status fsi_phase_centroid_noalloc(fsi_context* ctx, const fsi_state* st, double* result, void* stream);
+
 #define FSI_DIAG_MAX_MODES 32
 
 /* BenchmarkConfig fields the interval diagnostics read (:40-128). */
