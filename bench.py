@@ -95,6 +95,71 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
+def mixed_filament(fsi, setups, nsteps, window):
+    """The wall-supported filament in Kolmogorov flow at N = 1024 (UWCDI, the
+    drivers' tolerances) run for `nsteps` steps in FP64 and in mixed
+    precision (the predictor by FP64 refinement around an FP32 CG): steps/s
+    of both over the steps from `window` on, per-field relative L2 of the
+    mixed run against the FP64 run at the end, and the error of one mixed
+    step from the FP64 run's final state against the FP64 step."""
+    fields = ("U", "V", "phi", "sigma_xx", "sigma_yy", "sigma_xy")
+
+    def run(precision):
+        case = setups.kolmogorov_filament(1024, "uwcdi", precision=precision)
+        ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
+        work = fsi.allocate_imex_workspace(ctx)
+        fsi.make_velocity_compatible(case.state, ctx, work)
+        t, dt = 0.0, case.dt
+        for _ in range(window):
+            fsi.integrate_step(case.state, t, dt, ctx, work)
+            t += dt
+        t0 = time.perf_counter()
+        its, cyc = [], []
+        for _ in range(nsteps - window):
+            fsi.integrate_step(case.state, t, dt, ctx, work)
+            t += dt
+            its.append(work.last_stats.velocity_iterations)
+            cyc.append(work.last_stats.velocity_cycles)
+        el = time.perf_counter() - t0
+        out = {f: getattr(case.state, f).copy() for f in fields}
+        print(f"filament {precision}: {1e3 * el / (nsteps - window):.3f} ms/step, velocity iterations "
+              f"{np.mean(np.array(its), axis=0)}, cycles {np.mean(np.array(cyc), axis=0)}", file=sys.stderr,
+              flush=True)
+        return out, (nsteps - window) / el, np.mean(np.array(its), axis=0), np.mean(np.array(cyc), axis=0), \
+            (case, ctx, work, t, dt)
+
+    ref, v64, it64, _, (c64, ctx64, w64, t64, dt) = run("fp64")
+    mix, vmx, itmx, cymx, _ = run("mixed")
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    vel = np.sqrt(np.sum(ref["U"] ** 2) + np.sum(ref["V"] ** 2))
+    run_err = {f: rel(mix[f], ref[f]) for f in fields}
+    run_err["UV"] = float(np.sqrt(np.sum((mix["U"] - ref["U"]) ** 2) + np.sum((mix["V"] - ref["V"]) ** 2)) / vel)
+    # one step from the same state in both precisions
+    mcase = setups.kolmogorov_filament(1024, "uwcdi", precision="mixed")
+    mctx = fsi.build_imex_context(mcase.metrics, mcase.bcs, mcase.models, mcase.numerics)
+    mwork = fsi.allocate_imex_workspace(mctx)
+    st = fsi.FSIState(*(ref[f].copy(order="F") if f in ref else getattr(mcase.state, f).copy(order="F")
+                        for f in ("U", "V", "phi", "resource", "sigma_xx", "sigma_yy", "sigma_xy")))
+    fsi.integrate_step(st, t64, dt, mctx, mwork)
+    fsi.integrate_step(c64.state, t64, dt, ctx64, w64)
+    step_err = {f: rel(getattr(st, f), getattr(c64.state, f)) for f in fields}
+    print(f"filament mixed vs fp64: after the run {run_err}, one step {step_err}", file=sys.stderr, flush=True)
+    return {"workload": "wall-supported filament in Kolmogorov flow, 1024x1024, UWCDI, G = 0.5, mu = 0.01, "
+                        f"f_x = 0.5 sin(4 pi y), fixed dt = {dt:.4g}, {nsteps} steps from rest; velocity GMRES "
+                        "rule 1e-8, projection 1e-11",
+            "value": vmx, "unit": "IMEX steps/s", "fp64_value": v64, "speedup_vs_fp64": vmx / v64,
+            "timed_steps": f"{window}..{nsteps}", "rel_l2_vs_fp64_after_run": run_err,
+            "rel_l2_vs_fp64_one_step": step_err, "velocity_iterations_fp64": it64.tolist(),
+            "velocity_inner_iterations_mixed": itmx.tolist(), "velocity_cycles_mixed": cymx.tolist(),
+            "scheme": "velocity predictor: FP64 iterative refinement around an FP32 CG (FP32 r, p, q; FP64 x, "
+                      "residual checks and dot products; the same stopping rule); projection V-cycle in FP32; "
+                      "state, tendencies, stage sums, the direct projection solve and every reduction FP64",
+            "_states": None}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +174,8 @@ def main(argv=None):
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--only-kernel", default=None, help="time one kernel standalone and exit (profiling runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
+    ap.add_argument("--mixed-steps", type=int, default=2000, help="steps of the filament run in each precision")
+    ap.add_argument("--mixed-window", type=int, default=20, help="untimed first steps of the filament runs")
     ap.add_argument("--ensemble-steps", type=int, default=10,
                     help="timed steps of the 64-cavity ensemble, stepped one by one and batched (0: skip)")
     ap.add_argument("--ensemble-warmup", type=int, default=4)
@@ -215,30 +282,8 @@ def main(argv=None):
     # V-cycle preconditioner; error of every field against this FP64 run
     mixed = None
     if not args.no_mixed and args.precision == "fp64":
-        mcase = setups.seeded_cavity(args.N, args.method, args.seed, velocity_rtol=1e-8, projection_rtol=1e-10,
-                                     precision="mixed")
-        mctx = fsi.build_imex_context(mcase.metrics, mcase.bcs, mcase.models, mcase.numerics)
-        mwork = fsi.allocate_imex_workspace(mctx)
-        mstate, mt = mcase.state, 0.0
-        fsi.make_velocity_compatible(mstate, mctx, mwork)
-        for _ in range(args.warmup):
-            fsi.integrate_step(mstate, mt, dt, mctx, mwork)
-            mt += dt
-        m0 = time.perf_counter()
-        miters = []
-        for _ in range(args.steps):
-            fsi.integrate_step(mstate, mt, dt, mctx, mwork)
-            mt += dt
-            miters.append(mwork.last_stats.projection_iterations)
-        m_el = (time.perf_counter() - m0) / args.steps
-        errs = {f: float(np.linalg.norm(getattr(mstate, f) - final[f]) / np.linalg.norm(final[f])) for f in final}
-        mixed = {"value": 1.0 / m_el, "ms_per_step": 1e3 * m_el, "speedup_vs_fp64": (1.0 / m_el) / steps_per_s,
-                 "rel_l2_vs_fp64": errs, "projection_iterations": np.mean(np.array(miters), axis=0).tolist(),
-                 "scheme": "FP32 V-cycle preconditioner (levels, LDS tiles, arithmetic) inside the FP64 PCG; "
-                           "FP64 residuals, dot products, stencils and state"}
-        print(f"mixed: {1e3 * m_el:.3f} ms/step, speedup {mixed['speedup_vs_fp64']:.3f}, errors {errs}",
-              file=sys.stderr, flush=True)
-        del mwork, mctx
+        mixed = mixed_filament(fsi, setups, args.mixed_steps, args.mixed_window)
+        del mixed["_states"]
 
     # batched ensemble: 64 cavities (N in {128, 192, 256, 384}, G, disk drawn
     # from a seed), the same steps once one instance after another

@@ -99,7 +99,8 @@ class ProjectionResultC(ctypes.Structure):
 class StepStatsC(ctypes.Structure):
     _fields_ = [("velocity_iterations", c_int32 * 3), ("projection_iterations", c_int32 * 3),
                 ("planned", c_int32), ("max_abs_u", c_double), ("max_abs_v", c_double),
-                ("velocity_residual", c_double * 3), ("velocity_reference_residual", c_double * 3)]
+                ("velocity_residual", c_double * 3), ("velocity_reference_residual", c_double * 3),
+                ("velocity_cycles", c_int32 * 3)]
 
 
 # ---- configuration conversion ------------------------------------------------

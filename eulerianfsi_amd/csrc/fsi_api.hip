@@ -83,13 +83,17 @@ struct fsi_context {
     int use_graph = 2;
     int vel_warm = 1;  // predictor warm starts from the stage histories (FSI_STAGE_WARM=0: from u^n)
     int have_history = 0;  // iteration counts of a completed step are known
-    int plan_vel[3] = {0, 0, 0};
+    int plan_vel[3] = {0, 0, 0};   // planned iterations (mixed precision: inner iterations per cycle)
+    int plan_cyc[3] = {0, 0, 0};   // mixed precision: planned refinement cycles
+    int last_imax[3] = {8, 8, 8};  // mixed precision: most inner iterations of a cycle in the last solve
     int vel_check_from[3] = {0, 0, 0};  // the fewest iterations of the same history
     int plan_proj[3] = {0, 0, 0};
     // iteration counts of the last kHist completed steps per stage (the plan
     // is their maximum plus a margin, so alternating counts keep one plan)
     static constexpr int kHist = 4;
     int hist_vel[3][kHist] = {};
+    int hist_cyc[3][kHist] = {};   // mixed precision: cycles and most inner iterations of a cycle
+    int hist_imax[3][kHist] = {};
     int hist_proj[3][kHist] = {};
     int hist_n = 0;
     long long spec_steps = 0, spec_misses = 0;
@@ -115,6 +119,8 @@ struct fsi_workspace {
     // goes to phis, sxxs, syys, sxys, which the state swaps with)
     double *cphi[2] = {}, *csxx[2] = {}, *csyy[2] = {}, *csxy[2] = {};
     double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
+    // mixed precision: the FP32 r, p, q of the inner CG (null in FP64 mode)
+    float *fr_u[2] = {}, *fr_v[2] = {}, *fp_u[2] = {}, *fp_v[2] = {}, *fq_u[2] = {}, *fq_v[2] = {};
     // `correction` (work.correction, :828-833) is corr[corr_cur].  A step or
     // a projection writes its correction into corr[1 - corr_cur] and flips
     // corr_cur only when it has succeeded, so a failing call leaves the
@@ -870,6 +876,15 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
         double** h[8] = {&w->Eu[s], &w->Ev[s], &w->Iu[s], &w->Iv[s], &w->Ep[s], &w->Exx[s], &w->Eyy[s], &w->Exy[s]};
         for (auto p : h) HIPCHECK(c, A(p, F));
     }
+    if (c->num.precision == FSI_PRECISION_MIXED) {
+        float** fv[12] = {&w->fr_u[0], &w->fr_u[1], &w->fr_v[0], &w->fr_v[1], &w->fp_u[0], &w->fp_u[1],
+                          &w->fp_v[0], &w->fp_v[1], &w->fq_u[0], &w->fq_u[1], &w->fq_v[0], &w->fq_v[1]};
+        for (auto q : fv) {
+            HIPCHECK(c, hipMalloc((void**)q, F * sizeof(float)));
+            HIPCHECK(c, hipMemsetAsync(*q, 0, F * sizeof(float), c->stream));
+            w->owned.push_back(reinterpret_cast<double*>(*q));
+        }
+    }
     // multigrid levels: level 0 solves into z with b = r; x and r are scratch
     w->lv.resize(c->geo.size());
     for (size_t l = 0; l < c->geo.size(); ++l) {
@@ -984,7 +999,63 @@ static VelTol vel_tol(const fsi_context* c) {
     t.cap = (int)std::min(cap, 1L << 30);
     t.method = c->num.velocity_method;
     t.fallback = c->num.direct_fallback != 0;
+    t.icap = 100;
     return t;
+}
+
+// Mixed precision: the predictor by FP64 iterative refinement around an FP32
+// CG.  k_vel_start forms x0 and r0 = b - A x0 in FP64 (r0 also in FP32);
+// each cycle runs the FP32 inner CG on A d = r (x accumulating in FP64) to
+// its threshold, then k_vel_start (restart) forms r = b - A x again in FP64
+// and k_vel_check applies the stopping rules of LinearSolverOptions to it.
+// Planned: `fixed` inner iterations in each of plan_cyc[slot] cycles;
+// otherwise every cycle is polled.
+static fsi_status poll_vel(fsi_context* c, KrylovScalars* out) {
+    HIPCHECK(c, hipMemcpyAsync(out, &c->st->vel, sizeof(KrylovScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(c, hipStreamSynchronize(c->stream));
+    return FSI_OK;
+}
+static fsi_status velocity_solve_mixed(fsi_context* c, fsi_workspace* w, VelBufs b, double alpha, int slot,
+                                       int fixed, const VelTol& vt) {
+    const Params& p = c->prm;
+    for (int k = 0; k < 2; ++k) {
+        b.fr_u[k] = w->fr_u[k];
+        b.fr_v[k] = w->fr_v[k];
+        b.fp_u[k] = w->fp_u[k];
+        b.fp_v[k] = w->fp_v[k];
+        b.fq_u[k] = w->fq_u[k];
+        b.fq_v[k] = w->fq_v[k];
+    }
+    launch_vel_init(c->stream, p, b, alpha, c->st, slot, vt);
+    if (fixed > 0) {
+        const int cycles = std::max(1, c->plan_cyc[slot]);
+        for (int cy = 0; cy < cycles; ++cy) {
+            launch_vel_iters_f32(c->stream, p, b, alpha, c->st, 0, std::min(fixed, vt.icap + 1), slot, vt);
+            launch_vel_refine(c->stream, p, b, alpha, c->st, slot, vt, cy + 1 < cycles);
+        }
+        return FSI_OK;
+    }
+    // every cycle takes at least one inner iteration, so cap + 1 cycles reach any rule's limit
+    KrylovScalars ks{};
+    for (int cy = 0; cy <= vt.cap; ++cy) {
+        int m = 0, chunk = c->last_imax[slot] + 1;
+        while (m <= vt.icap) {
+            const int n = std::min(chunk, vt.icap + 1 - m);
+            launch_vel_iters_f32(c->stream, p, b, alpha, c->st, m, n, slot, vt);
+            m += n;
+            fsi_status rc = poll_vel(c, &ks);
+            if (rc != FSI_OK) return rc;
+            if (ks.done || ks.idone) break;
+            chunk = 4;
+        }
+        launch_vel_refine(c->stream, p, b, alpha, c->st, slot, vt, 1);
+        fsi_status rc = poll_vel(c, &ks);
+        if (rc != FSI_OK) return rc;
+        if (ks.done) break;
+    }
+    c->last_vel_iters[slot] = std::max(1, ks.iter);
+    c->last_imax[slot] = std::max(1, ks.imax);
+    return FSI_OK;
 }
 
 // fixed > 0: enqueue exactly `fixed` iteration kernels without reading the
@@ -1034,6 +1105,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         b.Su = s->U;
         b.Sv = s->V;
     }
+    if (c->num.precision == FSI_PRECISION_MIXED) return velocity_solve_mixed(c, w, b, alpha, slot, fixed, vt);
     launch_vel_init(c->stream, p, b, alpha, c->st, slot, vt);
     // the last iteration any rule can reach (kernel m tests iterate m)
     const int last = (vt.method == FSI_SOLVER_DIRECT || vt.fallback) ? vt.cap : vt.maxiter;
@@ -1423,6 +1495,16 @@ static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, dou
         c->plan_vel[k] = mv + kVelMargin;
         c->vel_check_from[k] = lo;
         c->plan_proj[k] = mp + kProjMargin;
+        if (c->num.precision == FSI_PRECISION_MIXED) {
+            // mixed: the most inner iterations of a cycle plus one, the most cycles
+            int mi = c->last_imax[k], mc = 1;
+            for (int h = 0; h < nh; ++h) {
+                mi = std::max(mi, c->hist_imax[k][h]);
+                mc = std::max(mc, c->hist_cyc[k][h]);
+            }
+            c->plan_vel[k] = mi + kVelMargin;
+            c->plan_cyc[k] = mc;
+        }
     }
     if ((rc = status_push_reset(c)) != FSI_OK) return rc;
     if (c->use_graph == 2) {  // planned counts, launched directly
@@ -1464,7 +1546,10 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     {
         const Status& st = *c->st_host;
         for (int k = 0; k < 3; ++k) {
-            if (!p.visc || st.vel_ok[k]) c->last_vel_iters[k] = std::max(1, st.vel_iters[k]);
+            if (!p.visc || st.vel_ok[k]) {
+                c->last_vel_iters[k] = std::max(1, st.vel_iters[k]);
+                c->last_imax[k] = std::max(1, st.vel_imax[k]);
+            }
             if (st.proj_ok[k]) c->last_proj_iters[k] = std::max(0, st.proj_iters[k]);
         }
         bool all = true;
@@ -1475,6 +1560,8 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
             for (int k = 0; k < 3; ++k) {
                 c->hist_vel[k][h] = c->last_vel_iters[k];
                 c->hist_proj[k][h] = c->last_proj_iters[k];
+                c->hist_imax[k][h] = c->last_imax[k];
+                c->hist_cyc[k][h] = std::max(1, st.vel_cycles[k]);
             }
             ++c->hist_n;
         }
@@ -1547,6 +1634,7 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
         for (int k = 0; k < 3; ++k) {
             stats->velocity_residual[k] = p.visc ? st.vel_res[k] : 0.0;
             stats->velocity_reference_residual[k] = p.visc ? st.vel_ref[k] : 0.0;
+            stats->velocity_cycles[k] = p.visc ? st.vel_cycles[k] : 0;
         }
     }
     return FSI_OK;
@@ -1604,6 +1692,12 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
         vb.p_v[k] = w->p_v[k];
         vb.q_u[k] = w->q_u[k];
         vb.q_v[k] = w->q_v[k];
+        vb.fr_u[k] = w->fr_u[k];  // mixed precision only (null otherwise)
+        vb.fr_v[k] = w->fr_v[k];
+        vb.fp_u[k] = w->fp_u[k];
+        vb.fp_v[k] = w->fp_v[k];
+        vb.fq_u[k] = w->fq_u[k];
+        vb.fq_v[k] = w->fq_v[k];
     }
     vb.partials = c->partials;
     ProjBufs pb{};

@@ -101,6 +101,12 @@ struct KrylovScalars {
     double fb2;    // accepted at the iteration limit when r.r <= fb2 (fallback_reltol)
     double dtol2;  // the direct-solve target of :direct and of the fallback
     double ref2;   // |b - A u^n|^2 (:gmres)
+    // mixed precision: the FP32 inner CG of the current refinement cycle
+    double itol2;  // its stopping threshold on r.r
+    int idone;     // 1 once the cycle's inner CG has ended (the FP64 check decides what follows)
+    int ibase;     // inner iterations of the finished cycles
+    int cycles;    // refinement cycles checked so far
+    int imax;      // most inner iterations of one cycle
 };
 
 // Stopping rules of the velocity predictor solve, passed to its kernels.
@@ -114,6 +120,7 @@ struct VelTol {
     double rtol, fb_rtol;
     int maxiter, cap;
     int method, fallback;
+    int icap;  // mixed precision: inner iterations of one refinement cycle at most
 };
 constexpr double VEL_DIRECT_RTOL = 1e-14;
 
@@ -142,6 +149,7 @@ struct Status {
     int vel_ok[3], proj_ok[3];  // 1 once the solve of that stage has converged
     // per predictor solve: final |r| / max(|b|, eps) and |b - A u^n| / max(|b|, eps)
     double vel_res[3], vel_ref[3];
+    int vel_cycles[3], vel_imax[3];  // mixed precision: refinement cycles, most inner iterations of one
     KrylovScalars vel;
     KrylovScalars pcg;
     ProjScalars proj;

@@ -136,6 +136,10 @@ struct VelBufs {
     double* q_u[2];  // q = A p
     double* q_v[2];
     double* partials;
+    // mixed precision: FP32 r, p, q of the inner CG (null in FP64 mode); x
+    // stays FP64 and accumulates the inner updates
+    float *fr_u[2], *fr_v[2], *fp_u[2], *fp_v[2], *fq_u[2], *fq_v[2];
+    int restart;  // k_vel_start: r = b - A x of the current x, for the FP64 check of a cycle
 };
 // the start of a predictor solve (x0, r0, q0 and the stopping thresholds of `tol`)
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
@@ -145,6 +149,14 @@ void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double al
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot, const VelTol& tol, int check_from);
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b);
+// mixed precision (FP32 inner CG inside FP64 iterative refinement):
+// FP32 CG iterations m0 .. m0+count-1 of the current cycle
+void launch_vel_iters_f32(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0,
+                          int count, int slot, const VelTol& tol);
+// the end of a cycle: r = b - A x in FP64 (FP32 copy for the next cycle) and
+// the outer stopping rule on it; `next`: the next cycle's q0 as well
+void launch_vel_refine(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
+                       const VelTol& tol, int next);
 
 // ---- projection (fsi_solvers.hip) ----
 struct ProjBufs {

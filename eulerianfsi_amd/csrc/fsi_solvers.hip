@@ -39,7 +39,7 @@ static __device__ __forceinline__ int lane_id() { return threadIdx.x + threadIdx
 // Partial-sum sets (see put_partials / get_totals in fsi_device.hpp): each
 // set holds up to 4 quantities x nb blocks.
 enum PartialSet : int { PS_VEL0 = 0, PS_VEL1 = 1, PS_VELI = 2, PS_RHS = 3, PS_RR = 4, PS_RZ = 5, PS_PW = 6,
-                        PS_MEAN = 7, PS_CHECK = 8, PS_VMAX = 9, PS_COUNT = 10 };
+                        PS_MEAN = 7, PS_CHECK = 8, PS_VMAX = 9, PS_VELR = 10, PS_COUNT = 11 };
 __device__ __forceinline__ double* pset(double* base, int nb, int set) { return base + (size_t)set * 4 * nb; }
 
 // ===========================================================================
@@ -56,8 +56,9 @@ __device__ __forceinline__ double* pset(double* base, int nb, int set) { return 
 // LDS tile of a velocity component on halo 1 around the 64 x 16 cells of a
 // block, read by global (wrapped) index.
 constexpr int VT_W = TILE_X + 2, VT_H = TILE_ROWS + 2;  // 66 x 18
-struct VelTile {
-    const double* a;
+template <class T>
+struct VelTileT {
+    const T* a;
     int i0, j0, nx, ny;
     __device__ __forceinline__ double operator()(int gi, int gj) const {
         int li = gi - i0 + 1, lj = gj - j0 + 1;
@@ -65,9 +66,28 @@ struct VelTile {
         else if (li >= VT_W) li -= nx;
         if (lj < 0) lj += ny;
         else if (lj >= VT_H) lj -= ny;
-        return a[lj * VT_W + li];
+        return (double)a[lj * VT_W + li];
     }
 };
+using VelTile = VelTileT<double>;
+
+// The CG vectors r, p, q (ping-pong pairs) in their storage type: FP64, or
+// FP32 for the inner solve of the mixed-precision predictor.
+template <class T>
+struct CgView {
+    using Pair = T* const*;
+    Pair r_u, r_v, p_u, p_v, q_u, q_v;
+};
+template <class T>
+__device__ __forceinline__ CgView<T> cg_view(const VelBufs& b);
+template <>
+__device__ __forceinline__ CgView<double> cg_view<double>(const VelBufs& b) {
+    return {b.r_u, b.r_v, b.p_u, b.p_v, b.q_u, b.q_v};
+}
+template <>
+__device__ __forceinline__ CgView<float> cg_view<float>(const VelBufs& b) {
+    return {b.fr_u, b.fr_v, b.fp_u, b.fp_v, b.fq_u, b.fq_v};
+}
 
 // LDS tile of halo H around the 64 x 16 cells of a block, by global index.
 // IN: the tile and its halo lie inside the domain, so no index wraps.
@@ -248,14 +268,21 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
             if (ref) ev = gvb - Nv(i, j);
             acc3[1] += gvb * gvb;
         }
-        b.xu[k] = Xu(i, j);
-        b.xv[k] = Xv(i, j);
-        b.r_u[0][k] = ru;  // p0 = r0: the first iteration reads it from r
-        b.r_v[0][k] = rv;
+        if (!b.restart) {  // a restart reads x as it is
+            b.xu[k] = Xu(i, j);
+            b.xv[k] = Xv(i, j);
+        }
+        if (b.fr_u[0]) {  // mixed precision: the inner CG starts from the FP32 copy
+            b.fr_u[0][k] = (float)ru;
+            b.fr_v[0][k] = (float)rv;
+        } else {
+            b.r_u[0][k] = ru;  // p0 = r0: the first iteration reads it from r
+            b.r_v[0][k] = rv;
+        }
         acc3[0] += ru * ru + rv * rv;
         acc3[2] += eu * eu + ev * ev;
     }
-    put_partials<3>(acc3, pset(b.partials, nb, PS_VELI), nb);
+    put_partials<3>(acc3, pset(b.partials, nb, b.restart ? PS_VELR : PS_VELI), nb);
 }
 
 __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
@@ -264,7 +291,11 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
     const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
-    if (T.x == 0 && T.y == 0 && tid == 0) st->vel.done = 0;  // the iteration kernels follow this one
+    if (b.restart) {
+        if (st->vel.done) return;  // the solve ended at an earlier check
+    } else if (T.x == 0 && T.y == 0 && tid == 0) {
+        st->vel.done = 0;  // the iteration kernels follow this one
+    }
     // the block's own operator rows are not the first or last face / row and
     // their stencils stay inside the halo-1 tile: no boundary rows, no wraps
     const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
@@ -274,11 +305,16 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
 
 // q0 = (I - alpha M) p0 with p0 = r0 (k_vel_start's) and the partials of the
 // first iteration (p.q, r.q, q.q, r.r): r0 on a halo-1 LDS tile, the block's
-// own cells and the halo ring as in k_vel_iter.  Block 0 also sets the
-// solve's stopping thresholds from k_vel_start's totals (|b|^2, |b - A u^n|^2).
+// own cells and the halo ring as in k_vel_iter.  With `first`, block 0 also
+// sets the solve's stopping thresholds from k_vel_start's totals (|b|^2,
+// |b - A u^n|^2, and for the FP32 inner CG of the mixed-precision solve the
+// threshold of its first cycle).  T: the storage of r, p, q.
+template <class T>
 __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double alpha, int nb, Status* st, int slot,
-                                                  VelTol vt) {
-    if (block_id() == 0) {
+                                                  VelTol vt, int first) {
+    constexpr bool F32 = sizeof(T) == 4;
+    if (!first && st->vel.done) return;  // a later cycle of a solve already converged
+    if (first && block_id() == 0) {
         double ti[3];
         get_totals<3>(pset(b.partials, nb, PS_VELI), nb, ti);
         if (lane_id() == 0) {
@@ -294,13 +330,22 @@ __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double al
             sk.dtol2 = fmin(VEL_DIRECT_RTOL * VEL_DIRECT_RTOL * bscale2, fb2);
             sk.iter = 0;
             st->vel_iters[slot] = 0;
+            if (F32) {  // the inner CG reduces r.r to a quarter of the target, or by 1e-12 at most
+                const double target = vt.method == 0 ? sk.dtol2 : sk.tol2;
+                sk.itol2 = fmax(0.25 * target, 1e-12 * ti[0]);
+                sk.idone = 0;
+                sk.ibase = 0;
+                sk.cycles = 0;
+                sk.imax = 0;
+            }
         }
     }
-    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];
+    const CgView<T> cv = cg_view<T>(b);
+    __shared__ T su[VT_H * VT_W], sv[VT_H * VT_W];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const TileId T = tile_id();
-    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
-    const double *ru = b.r_u[0], *rv = b.r_v[0];
+    const TileId Tl = tile_id();
+    const int i0 = Tl.x * TILE_X, j0 = Tl.y * TILE_ROWS;
+    const T *ru = cv.r_u[0], *rv = cv.r_v[0];
     auto wrapped = [&](int li, int lj) -> long {
         int gi = i0 + li - 1, gj = j0 + lj - 1;
         if (gi < 0 || gi >= p.nx) {
@@ -337,27 +382,28 @@ __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double al
         else { li = VT_W - 1; lj = tid - 2 * VT_W - TILE_ROWS + 1; }
         const long w = wrapped(li, lj);
         const size_t kk = w >= 0 ? (size_t)w : 0;
-        su[lj * VT_W + li] = w >= 0 ? ru[kk] : 0.0;
-        sv[lj * VT_W + li] = w >= 0 ? rv[kk] : 0.0;
+        su[lj * VT_W + li] = w >= 0 ? ru[kk] : T(0);
+        sv[lj * VT_W + li] = w >= 0 ? rv[kk] : T(0);
     }
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         const int slot_c = (ty + r * TILE_Y + 1) * VT_W + tx + 1;
-        su[slot_c] = fill[r] ? cru[r] : 0.0;
-        sv[slot_c] = fill[r] ? crv[r] : 0.0;
+        su[slot_c] = fill[r] ? (T)cru[r] : T(0);
+        sv[slot_c] = fill[r] ? (T)crv[r] : T(0);
     }
     __syncthreads();
-    const VelTile tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
+    const VelTileT<T> tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         if (!own[r]) continue;
         const int j = j0 + ty + r * TILE_Y;
         const size_t k = kc[r];
-        const double qu = dir_u(p, i) ? 0.0 : cru[r] - alpha * visc_u(p, tu, tv, i, j, 0);
-        const double qv = dir_v(p, j) ? 0.0 : crv[r] - alpha * visc_v(p, tu, tv, i, j, 0);
-        b.q_u[0][k] = qu;
-        b.q_v[0][k] = qv;
+        // q as stored (the FP32 rounding included, so the partials describe the stored vector)
+        const double qu = (double)(T)(dir_u(p, i) ? 0.0 : cru[r] - alpha * visc_u(p, tu, tv, i, j, 0));
+        const double qv = (double)(T)(dir_v(p, j) ? 0.0 : crv[r] - alpha * visc_v(p, tu, tv, i, j, 0));
+        cv.q_u[0][k] = (T)qu;
+        cv.q_v[0][k] = (T)qv;
         acc[0] += cru[r] * qu + crv[r] * qv;
         acc[1] += cru[r] * qu + crv[r] * qv;  // r = p on the first iteration
         acc[2] += qu * qu + qv * qv;
@@ -375,9 +421,18 @@ __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double al
 // two latencies overlap; with check_first (m >= 1, an iteration at which
 // convergence is expected) the totals come first, so the launch that sees
 // convergence ends without touching the fields.
+//
+// T = float: an iteration of the FP32 inner CG of a mixed-precision solve
+// (r, p, q stored in FP32, x in FP64, dot products summed in FP64).  Its
+// cycle ends at the cycle's threshold itol2 (set by k_vel_q0 or
+// k_vel_check), at icap iterations, or where the outer rule needs the FP64
+// check (the iteration limit); k_vel_check then decides with the FP64
+// residual whether the solve has converged.
+template <class T>
 __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
                                                     int slot, VelTol vt, int check_first) {
-    if (st->vel.done) return;
+    constexpr bool F32 = sizeof(T) == 4;
+    if (st->vel.done || (F32 && st->vel.idone)) return;
     // The test of iterate m, a function of the totals, m and the rule's
     // constants alone, so every block decides the same.  Returns true when
     // the solve ends here.  :direct: the direct target within `cap`
@@ -413,20 +468,38 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
         }
         return end;
     };
+    // the inner rule of a refinement cycle (FP32)
+    auto test_inner = [&](double rr) -> bool {
+        const KrylovScalars& sk = st->vel;
+        const int total = sk.ibase + m;
+        const bool bad = !(rr <= 1.0e300);
+        const bool limit = vt.method == 1 && total <= vt.maxiter ? total >= vt.maxiter : total >= vt.cap;
+        const bool end = bad || rr <= sk.itol2 || m >= vt.icap || limit;
+        if (end && block_id() == 0 && lane_id() == 0) {
+            st->vel.idone = 1;
+            st->vel_iters[slot] = total;
+            if (bad) {  // nothing to refine: the solve ends with the error
+                st->vel.done = 1;
+                raise_error(st, DEV_VEL_NOCONV, NAN, (double)(slot + 2));
+            }
+        }
+        return end;
+    };
+    const CgView<T> cv = cg_view<T>(b);
     double tot[4];
     if (check_first) {
         get_totals<4>(pset(b.partials, nb, (m & 1) ? PS_VEL1 : PS_VEL0), nb, tot);
         const KrylovScalars& sk = st->vel;
-        if (test(tot[3], sk.tol2, sk.fb2, sk.dtol2)) return;
+        if (F32 ? test_inner(tot[3]) : test(tot[3], sk.tol2, sk.fb2, sk.dtol2)) return;
     }
     const int o = m & 1, n = (m + 1) & 1;
     // p0 = r0 is not stored: the first iteration reads p from r
-    const double *ru = b.r_u[o], *rv = b.r_v[o], *qu0 = b.q_u[o], *qv0 = b.q_v[o];
-    const double *pu0 = m == 0 ? ru : b.p_u[o], *pv0 = m == 0 ? rv : b.p_v[o];
-    __shared__ double su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
+    const T *ru = cv.r_u[o], *rv = cv.r_v[o], *qu0 = cv.q_u[o], *qv0 = cv.q_v[o];
+    const T *pu0 = m == 0 ? ru : cv.p_u[o], *pv0 = m == 0 ? rv : cv.p_v[o];
+    __shared__ T su[VT_H * VT_W], sv[VT_H * VT_W];  // p' on halo 1
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
-    const TileId T = tile_id();
-    const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
+    const TileId Tl = tile_id();
+    const int i0 = Tl.x * TILE_X, j0 = Tl.y * TILE_ROWS;
     // wrapped global index of tile point (li, lj) of the halo-1 tile, or -1
     // outside a non-periodic wall
     auto wrapped = [&](int li, int lj) -> long {
@@ -458,7 +531,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
         fill[r] = w >= 0;
         kc[r] = fill[r] ? (size_t)w : 0;
     }
-    double cru[ROWS], crv[ROWS], cqu[ROWS], cqv[ROWS], cpu[ROWS], cpv[ROWS], cxu[ROWS], cxv[ROWS];
+    T cru[ROWS], crv[ROWS], cqu[ROWS], cqv[ROWS], cpu[ROWS], cpv[ROWS];  // in the storage type
+    double cxu[ROWS], cxv[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         cru[r] = ru[kc[r]];
@@ -474,7 +548,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     constexpr int NRING = 2 * VT_W + 2 * TILE_ROWS;
     int hslot = -1;
     bool hin = false;
-    double hru = 0.0, hrv = 0.0, hqu = 0.0, hqv = 0.0, hpu = 0.0, hpv = 0.0;
+    T hru = 0, hrv = 0, hqu = 0, hqv = 0, hpu = 0, hpv = 0;
     if (tid < NRING) {
         int li, lj;
         if (tid < VT_W) { li = tid; lj = 0; }
@@ -497,59 +571,108 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     const double pq = tot[0], rq = tot[1], qq = tot[2], rr = tot[3];
     if (!check_first) {
         const KrylovScalars& sk = st->vel;
-        if (test(rr, sk.tol2, sk.fb2, sk.dtol2)) return;
+        if (F32 ? test_inner(rr) : test(rr, sk.tol2, sk.fb2, sk.dtol2)) return;
     }
     const double a = rr / pq;
     const double beta = fmax(rr - 2.0 * a * rq + a * a * qq, 0.0) / rr;
     double hp_u = 0.0, hp_v = 0.0;
     if (hin) {
-        hp_u = (hru - a * hqu) + beta * hpu;
-        hp_v = (hrv - a * hqv) + beta * hpv;
+        hp_u = ((double)hru - a * (double)hqu) + beta * (double)hpu;
+        hp_v = ((double)hrv - a * (double)hqv) + beta * (double)hpv;
     }
+    // r', p' as stored (FP32 rounding included): the partials and the
+    // stencil then describe the stored vectors
     double r1u[ROWS], r1v[ROWS], p1u[ROWS], p1v[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-        r1u[r] = cru[r] - a * cqu[r];
-        r1v[r] = crv[r] - a * cqv[r];
-        p1u[r] = r1u[r] + beta * cpu[r];
-        p1v[r] = r1v[r] + beta * cpv[r];
+        r1u[r] = (double)(T)((double)cru[r] - a * (double)cqu[r]);
+        r1v[r] = (double)(T)((double)crv[r] - a * (double)cqv[r]);
+        p1u[r] = (double)(T)(r1u[r] + beta * (double)cpu[r]);
+        p1v[r] = (double)(T)(r1v[r] + beta * (double)cpv[r]);
         const int slot_c = (ty + r * TILE_Y + 1) * VT_W + tx + 1;
-        su[slot_c] = fill[r] ? p1u[r] : 0.0;
-        sv[slot_c] = fill[r] ? p1v[r] : 0.0;
+        su[slot_c] = fill[r] ? (T)p1u[r] : T(0);
+        sv[slot_c] = fill[r] ? (T)p1v[r] : T(0);
         if (own[r]) {
             const size_t k = kc[r];
-            b.xu[k] = cxu[r] + a * cpu[r];
-            b.xv[k] = cxv[r] + a * cpv[r];
-            b.r_u[n][k] = r1u[r];
-            b.r_v[n][k] = r1v[r];
-            b.p_u[n][k] = p1u[r];
-            b.p_v[n][k] = p1v[r];
+            b.xu[k] = cxu[r] + a * (double)cpu[r];
+            b.xv[k] = cxv[r] + a * (double)cpv[r];
+            cv.r_u[n][k] = (T)r1u[r];
+            cv.r_v[n][k] = (T)r1v[r];
+            cv.p_u[n][k] = (T)p1u[r];
+            cv.p_v[n][k] = (T)p1v[r];
         }
     }
     if (hslot >= 0) {
-        su[hslot] = hp_u;
-        sv[hslot] = hp_v;
+        su[hslot] = (T)hp_u;
+        sv[hslot] = (T)hp_v;
     }
     __syncthreads();
     // (3) q' = (I - alpha M) p' from the tile, and the four partials
-    const VelTile tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
+    const VelTileT<T> tu{su, i0, j0, p.nx, p.ny}, tv{sv, i0, j0, p.nx, p.ny};
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         if (!own[r]) continue;
         const int j = j0 + ty + r * TILE_Y;
         const size_t k = kc[r];
-        const double q1u = dir_u(p, i) ? 0.0 : p1u[r] - alpha * visc_u(p, tu, tv, i, j, 0);
-        const double q1v = dir_v(p, j) ? 0.0 : p1v[r] - alpha * visc_v(p, tu, tv, i, j, 0);
-        b.q_u[n][k] = q1u;
-        b.q_v[n][k] = q1v;
+        const double q1u = (double)(T)(dir_u(p, i) ? 0.0 : p1u[r] - alpha * visc_u(p, tu, tv, i, j, 0));
+        const double q1v = (double)(T)(dir_v(p, j) ? 0.0 : p1v[r] - alpha * visc_v(p, tu, tv, i, j, 0));
+        cv.q_u[n][k] = (T)q1u;
+        cv.q_v[n][k] = (T)q1v;
         acc[0] += p1u[r] * q1u + p1v[r] * q1v;
         acc[1] += r1u[r] * q1u + r1v[r] * q1v;
         acc[2] += q1u * q1u + q1v * q1v;
         acc[3] += r1u[r] * r1u[r] + r1v[r] * r1v[r];
     }
     put_partials<4>(acc, pset(b.partials, nb, n ? PS_VEL1 : PS_VEL0), nb);
-    if (block_id() == 0 && lane_id() == 0) st->vel_iters[slot] = m + 1;
+    if (block_id() == 0 && lane_id() == 0) st->vel_iters[slot] = (F32 ? st->vel.ibase : 0) + m + 1;
+}
+
+// The FP64 check at the end of a refinement cycle of the mixed-precision
+// solve, on the residual r = b - A x that k_vel_start (restart) formed from
+// the FP64 x: the stopping rules of k_vel_iter's `test` applied to the true
+// residual after `total` inner iterations.  When the solve goes on, the next
+// cycle's inner threshold is set from it.  One block.
+__global__ void __launch_bounds__(BLOCK) k_vel_check(VelBufs b, Status* st, int nb, int slot, VelTol vt) {
+    if (st->vel.done) return;
+    double tot[3];
+    get_totals<3>(pset(b.partials, nb, PS_VELR), nb, tot);
+    if (lane_id() != 0) return;
+    KrylovScalars& sk = st->vel;
+    const double rr = tot[0];
+    const int total = st->vel_iters[slot];
+    sk.cycles += 1;
+    sk.imax = max(sk.imax, total - sk.ibase);
+    bool end = false, ok = false;
+    const bool fallback_phase = vt.method == 0 || total > vt.maxiter;
+    if (!(rr <= 1.0e300)) {
+        end = true;
+    } else if (fallback_phase) {
+        ok = rr <= sk.dtol2;
+        end = ok || total >= vt.cap;
+    } else if (rr <= sk.tol2) {
+        end = ok = true;
+    } else if (total >= vt.maxiter) {
+        ok = rr <= sk.fb2;
+        end = ok || !vt.fallback;
+    }
+    st->vel_cycles[slot] = sk.cycles;
+    st->vel_imax[slot] = sk.imax;
+    if (end) {
+        sk.done = 1;
+        sk.rr = rr;
+        sk.iter = total;
+        const double bs2 = fmax(sk.bnorm2, 4.930380657631324e-32);
+        st->vel_res[slot] = sqrt(rr / bs2);
+        st->vel_ref[slot] = sqrt(sk.ref2 / bs2);
+        if (ok) st->vel_ok[slot] = 1;
+        else raise_error(st, DEV_VEL_NOCONV, sqrt(rr / bs2), (double)(slot + 2));
+    } else {
+        const double target = (fallback_phase || total >= vt.maxiter) ? sk.dtol2 : sk.tol2;
+        sk.itol2 = fmax(0.25 * target, 1e-12 * rr);
+        sk.ibase = total;
+        sk.idone = 0;
+    }
 }
 
 // NoViscosity: the predictor is the stage sum with Dirichlet values (:794-796).
@@ -1255,15 +1378,37 @@ static const dim3 kBlock(TILE_X, TILE_Y);
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
                      const VelTol& tol) {
     k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
-    k_vel_q0<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p), st, slot, tol);
+    if (b.fr_u[0])
+        k_vel_q0<float><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p), st, slot, tol, 1);
+    else
+        k_vel_q0<double><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p), st, slot, tol, 1);
+}
+void launch_vel_iters_f32(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0,
+                          int count, int slot, const VelTol& tol) {
+    const int nb = grid_blocks(p);
+    for (int c = 0; c < count; ++c)
+        k_vel_iter<float><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m0 + c, slot, tol, 0);
+}
+void launch_vel_refine(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
+                       const VelTol& tol, int next) {
+    const int nb = grid_blocks(p);
+    VelBufs rb = b;
+    rb.restart = 1;
+    rb.Un = b.xu;  // x as it is (its Dirichlet values already in place)
+    rb.Vn = b.xv;
+    rb.extrap = 0;
+    rb.Su = rb.Sv = nullptr;
+    k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, rb, alpha, st, nb);
+    k_vel_check<<<1, BLOCK, 0, s>>>(rb, st, nb, slot, tol);
+    if (next) k_vel_q0<float><<<grid2(p), kBlock, 0, s>>>(p, rb, alpha, nb, st, slot, tol, 0);
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot, const VelTol& tol, int check_from) {
     const int nb = grid_blocks(p);
     for (int c = 0; c < count; ++c) {
         const int m = m0 + c;
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, tol,
-                                               m >= 1 && m >= check_from ? 1 : 0);
+        k_vel_iter<double><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, tol,
+                                                       m >= 1 && m >= check_from ? 1 : 0);
     }
 }
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
@@ -1393,8 +1538,15 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         return 2 * cell;
     }
     if (n == "vel_iter") {
-        k_vel_iter<<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0, VelTol{-1.0, 0.0, 1 << 30, 1 << 30, 1, 0}, 0);
+        k_vel_iter<double><<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0,
+                                                       VelTol{-1.0, 0.0, 1 << 30, 1 << 30, 1, 0, 1 << 30}, 0);
         return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
+    }
+    if (n == "vel_iter_f32") {
+        if (!vb.fr_u[0]) return -1.0;  // FP64 mode: no FP32 vectors
+        k_vel_iter<float><<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0,
+                                                      VelTol{-1.0, 0.0, 1 << 30, 1 << 30, 1, 0, 1 << 30}, 0);
+        return 10 * cell;  // r, p, q (FP32) and x (FP64), u and v, read and written
     }
     if (n == "pcg_dir") {
         k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], pb, st, nb, 2);

@@ -273,6 +273,9 @@ class StepStats:
     # residual from uv_n on the same scale (the :gmres rule's reference)
     velocity_residual: tuple = (0.0, 0.0, 0.0)
     velocity_reference_residual: tuple = (0.0, 0.0, 0.0)
+    # mixed precision: FP64 refinement cycles per predictor solve (the
+    # iterations above are then FP32 inner iterations); 0 in FP64
+    velocity_cycles: tuple = (0, 0, 0)
 
 
 # ---------------------------------------------------------------------------
@@ -397,7 +400,7 @@ def _project(state, ctx, work, fn, what):
 def _step_stats(s) -> StepStats:
     return StepStats(tuple(s.velocity_iterations), tuple(s.projection_iterations), bool(s.planned),
                      float(s.max_abs_u), float(s.max_abs_v), tuple(s.velocity_residual),
-                     tuple(s.velocity_reference_residual))
+                     tuple(s.velocity_reference_residual), tuple(s.velocity_cycles))
 
 
 def integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work: IMEXWorkspace) -> StepReport:

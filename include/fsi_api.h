@@ -116,6 +116,11 @@ typedef enum {
     FSI_EXT_GEOMETRIC = 2  /* GeometricStructureExtension(rate, exponent; safety_cutoff) */
 } fsi_extension_kind;
 
+/* FSI_PRECISION_MIXED: the velocity predictor by FP64 iterative refinement
+ * around an FP32 CG (FP32 r, p, q; FP64 solution, residual checks and dot
+ * products; the stopping rules unchanged), and the multigrid V-cycle of the
+ * projection PCG in FP32.  State, tendencies, stage sums, the direct
+ * projection solve and every reduction stay FP64. */
 typedef enum { FSI_PRECISION_FP64 = 0, FSI_PRECISION_MIXED = 1 } fsi_precision;
 
 /* LinearSolverOptions.velocity_method (src/TimeIntegrationIMEX.jl:211-252) */
@@ -187,6 +192,9 @@ typedef struct {
      * the :gmres rule scales reltol by (0 for :direct); 0 without viscosity */
     double velocity_residual[3];
     double velocity_reference_residual[3];
+    /* FSI_PRECISION_MIXED: FP64 refinement cycles of each predictor solve
+     * (velocity_iterations then counts its FP32 inner CG iterations); 0 in FP64 */
+    int32_t velocity_cycles[3];
 } fsi_step_stats;
 
 typedef struct fsi_context fsi_context;
@@ -385,7 +393,8 @@ fsi_status fsi_diagnostics_compute(fsi_context* ctx, const fsi_state* st, const 
  * stream with the buffers of the workspace (after at least one step), timed
  * with HIP events on that stream; returns the mean duration of one launch in
  * milliseconds and the kernel's algorithmic HBM bytes per launch.  Names:
- * "copy" (calibration), "vel_iter", "spec_solve" (the projection's direct
+ * "copy" (calibration), "vel_iter", "vel_iter_f32" (the FP32 inner CG
+ * iteration of a mixed-precision context), "spec_solve" (the projection's direct
  * solve, three kernels), "pcg_dir", "pcg_upd", "mg_down0", "mg_up0",
  * "mg_coarse", "pcg_iter", "mg_jacobi", "momentum", "cells". */
 fsi_status fsi_benchmark_kernel(fsi_context* ctx, fsi_state* st, fsi_workspace* ws, const char* name,
