@@ -16,6 +16,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fsi_kernels.hpp"
@@ -135,6 +136,7 @@ struct fsi_workspace {
     // history.  Only iteration counts depend on it.
     double *cq_cur[3] = {}, *cq_prev[3] = {}, *cq_next[3] = {};
     double* xs = nullptr;  // the iterate of a standalone projection (fsi_project)
+    double *mun_u = nullptr, *mun_v = nullptr;  // M u^n of the step (the reference residual of the predictor)
     // the same for the predictor solves: stage s of the previous two steps
     // velocity stage solutions of the last four steps (newest first) and a
     // fifth buffer that the solve writes into (the projection then writes
@@ -861,6 +863,7 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w,
                           &w->cq_cur[0], &w->cq_cur[1], &w->cq_cur[2], &w->cq_prev[0], &w->cq_prev[1],
                           &w->cq_prev[2], &w->cq_next[0], &w->cq_next[1], &w->cq_next[2], &w->xs,
+                          &w->mun_u, &w->mun_v,
                           &w->vel_cur_u[0], &w->vel_cur_u[1], &w->vel_cur_u[2], &w->vel_cur_v[0], &w->vel_cur_v[1],
                           &w->vel_cur_v[2], &w->vel_prev_u[0], &w->vel_prev_u[1], &w->vel_prev_u[2],
                           &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2], &w->vel_old_u[0],
@@ -1100,10 +1103,13 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         b.extrap = std::min(w->stage_warm, 4) - 1;
     }
     const VelTol vt = vel_tol(c);
-    // gmres: the stopping rule measures against |b - A u^n| (the state velocity)
+    // gmres: the stopping rule measures against |b - A u^n| (the state
+    // velocity; M u^n from the stage-1 momentum kernel)
     if (vt.method == FSI_SOLVER_GMRES) {
         b.Su = s->U;
         b.Sv = s->V;
+        b.Mu = w->mun_u;
+        b.Mv = w->mun_v;
     }
     if (c->num.precision == FSI_PRECISION_MIXED) return velocity_solve_mixed(c, w, b, alpha, slot, fixed, vt);
     launch_vel_init(c->stream, p, b, alpha, c->st, slot, vt);
@@ -1335,6 +1341,9 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         R.dt = dt;
         R.bu = p.visc ? c->bvisc_u : nullptr;
         R.bv = p.visc ? c->bvisc_v : nullptr;
+        const bool ref = p.visc && q == 0 && c->num.velocity_method == FSI_SOLVER_GMRES;
+        R.mu = ref ? w->mun_u : nullptr;
+        R.mv = ref ? w->mun_v : nullptr;
         R.q = q;
         return R;
     };
@@ -1655,19 +1664,44 @@ extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_
     if (n < 0) return FSI_E_ARGUMENT;
     if (n == 0) return FSI_OK;
     if (!ctx || !st || !w || !t || !dt || !status) return FSI_E_ARGUMENT;
+    for (int k = 0; k < n; ++k)
+        if (!ctx[k]) return FSI_E_ARGUMENT;
     std::vector<StepRun> runs((size_t)n);
-    for (int k = 0; k < n; ++k) {  // every instance's planned step in flight before any is awaited
-        fsi_status rc = step_check(ctx[k], st[k], w[k], t[k], dt[k]);
-        if (rc == FSI_OK) rc = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
-        status[k] = rc;
+    // Host threads share the launches: thread q enqueues the planned steps of
+    // instances q, q + T, ... (every one in flight before it waits for any),
+    // then finishes them; the contexts are independent, so each instance
+    // advances exactly as fsi_integrate_step would advance it.
+    int nt = 8;
+    if (const char* e = std::getenv("FSI_ENSEMBLE_THREADS")) nt = std::max(1, std::atoi(e));
+    const unsigned hw = std::thread::hardware_concurrency();
+    if (hw > 0) nt = std::min(nt, (int)hw);
+    nt = std::min(nt, (int)n);
+    auto work = [&](int q) {
+        for (int k = q; k < n; k += nt) {
+            if (hipSetDevice(ctx[k]->device) != hipSuccess) {
+                status[k] = fail(ctx[k], FSI_E_HIP, "hipSetDevice(%d) failed", ctx[k]->device);
+                continue;
+            }
+            fsi_status rc = step_check(ctx[k], st[k], w[k], t[k], dt[k]);
+            if (rc == FSI_OK) rc = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
+            status[k] = rc;
+        }
+        for (int k = q; k < n; k += nt)
+            if (status[k] == FSI_OK)
+                status[k] = step_finish(ctx[k], st[k], w[k], dt[k], runs[k], reps ? &reps[k] : nullptr,
+                                        stats ? &stats[k] : nullptr);
+    };
+    if (nt == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        pool.reserve((size_t)nt);
+        for (int q = 0; q < nt; ++q) pool.emplace_back(work, q);
+        for (auto& th : pool) th.join();
     }
     fsi_status first = FSI_OK;
-    for (int k = 0; k < n; ++k) {
-        if (status[k] == FSI_OK)
-            status[k] = step_finish(ctx[k], st[k], w[k], dt[k], runs[k], reps ? &reps[k] : nullptr,
-                                    stats ? &stats[k] : nullptr);
+    for (int k = 0; k < n; ++k)
         if (status[k] != FSI_OK && first == FSI_OK) first = (fsi_status)status[k];
-    }
     return first;
 }
 

@@ -57,6 +57,8 @@ struct RhsSums {
     double dt;
     const double* bu;  // b_visc, or null without viscosity
     const double* bv;
+    double* mu;  // q = 0: M u^n (the viscous operator without its lifting) stored here, or null
+    double* mv;
     int q;  // the stage of the tendencies (0..2); -1: no sums
 };
 // E_u, E_v, I_u, I_v may be null (not stored)
@@ -123,8 +125,10 @@ struct VelBufs {
     const double* Wv;
     int extrap;        // 0: x0 = Un; 1: x0 = 2 Un - Pu; 2: x0 = 3 (Un - Pu) + Ou;
                        // 3: x0 = 4 (Un + Ou) - 6 Pu - Wu
-    const double* Su;  // the state velocity u^n: the gmres rule's reference residual
-    const double* Sv;  //   ||b - A u^n|| (null: not needed, the :direct method)
+    const double* Su;  // the state velocity u^n and M u^n (the stage-1 momentum kernel's):
+    const double* Sv;  //   the gmres rule's reference residual ||b - A u^n||, formed
+    const double* Mu;  //   on the own cells (null: not needed, the :direct method)
+    const double* Mv;
     const double* ru;  // right-hand side (stage sum + alpha*b)
     const double* rv;
     double* xu;  // solution, written into the stage velocity arrays

@@ -340,6 +340,10 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
             Iu[k] = iu;
             Iv[k] = iv;
         }
+        if (R.q == 0 && R.mu) {  // M u^n, for the reference residual of the predictor solves
+            R.mu[k] = iu - lbu;
+            R.mv[k] = iv - lbv;
+        }
         if (R.q >= 0) {  // the right-hand sides of the later stages
             for (int t = R.q; t < 3; ++t) {
                 double ru = R.q == 0 ? U(i, j) : su_old[(t - R.q) & 1];

@@ -181,13 +181,14 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx, co
 // _impose_row_dirichlet (TimeIntegrationIMEX.jl:649-672): identity rows with
 // the Dirichlet value on the right at U-left / V-bottom, the free rows with
 // their Dirichlet columns kept, so |b - A u^n| is the initial residual
-// IterativeSolvers measures reltol against (x0 = uv_n, :807).  x0 and u^n
-// sit on halo-1 LDS tiles; r0 is formed on the block's own cells only.  p0 =
-// r0 is not stored, and q0 = (I - alpha M) r0 with the first iteration's
-// partials is k_vel_q0's.
+// IterativeSolvers measures reltol against (x0 = uv_n, :807); with M u^n
+// from the stage-1 momentum kernel it is b - u^n + alpha M u^n on the free
+// rows.  x0 sits on a halo-1 LDS tile; r0 is formed on the block's own cells
+// only.  p0 = r0 is not stored, and q0 = (I - alpha M) r0 with the first
+// iteration's partials is k_vel_q0's.
 template <bool IN>
 __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b, double alpha, int nb, int i0, int j0,
-                                               int tid, double* sxu, double* sxv, double* snu, double* snv) {
+                                               int tid, double* sxu, double* sxv) {
     using T1 = HTile<1, IN>;
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
     auto wrapped = [&](int ui, int uj, int& gi, int& gj) -> bool {
@@ -228,23 +229,20 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         bu[r] = b.ru[kc[r]];
         bv[r] = b.rv[kc[r]];
     }
+    // u^n and M u^n on the own cells (the reference residual)
+    const bool ref = b.Su != nullptr;
+    double nu[ROWS], nv[ROWS], mu[ROWS], mv[ROWS];
+#pragma unroll
+    for (int r = 0; r < ROWS; ++r) {
+        nu[r] = ref ? b.Su[kc[r]] : 0.0;
+        nv[r] = ref ? b.Sv[kc[r]] : 0.0;
+        mu[r] = ref ? b.Mu[kc[r]] : 0.0;
+        mv[r] = ref ? b.Mv[kc[r]] : 0.0;
+    }
     x0_batch<0, NI1 / 2, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
     x0_batch<NI1 / 2, NI1, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
-    const bool ref = b.Su != nullptr;
-    if (ref) {  // u^n as it is (no Dirichlet values substituted)
-#pragma unroll
-        for (int it = 0; it < NI1; ++it) {
-            const int k = tid + it * BLOCK;
-            if (k < N1) {
-                const bool in = fl[it] & 1;
-                snu[k] = in ? b.Su[idx1[it]] : 0.0;
-                snv[k] = in ? b.Sv[idx1[it]] : 0.0;
-            }
-        }
-    }
     __syncthreads();
     const T1 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
-    const T1 Nu{snu, i0, j0, p.nx, p.ny}, Nv{snv, i0, j0, p.nx, p.ny};
     double acc3[3] = {0.0, 0.0, 0.0};  // |r0|^2, |b|^2, |b - A u^n|^2
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -254,18 +252,18 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         double ru = 0.0, rv = 0.0, eu = 0.0, ev = 0.0;
         if (!dir_u<IN>(p, i)) {
             ru = bu[r] - (Xu(i, j) - alpha * visc_u<IN>(p, Xu, Xv, i, j, 0));
-            if (ref) eu = bu[r] - (Nu(i, j) - alpha * visc_u<IN>(p, Nu, Nv, i, j, 0));
+            if (ref) eu = bu[r] - (nu[r] - alpha * mu[r]);
             acc3[1] += bu[r] * bu[r];
         } else {
-            if (ref) eu = gul - Nu(i, j);
+            if (ref) eu = gul - nu[r];
             acc3[1] += gul * gul;
         }
         if (!dir_v<IN>(p, j)) {
             rv = bv[r] - (Xv(i, j) - alpha * visc_v<IN>(p, Xu, Xv, i, j, 0));
-            if (ref) ev = bv[r] - (Nv(i, j) - alpha * visc_v<IN>(p, Nu, Nv, i, j, 0));
+            if (ref) ev = bv[r] - (nv[r] - alpha * mv[r]);
             acc3[1] += bv[r] * bv[r];
         } else {
-            if (ref) ev = gvb - Nv(i, j);
+            if (ref) ev = gvb - nv[r];
             acc3[1] += gvb * gvb;
         }
         if (!b.restart) {  // a restart reads x as it is
@@ -287,7 +285,7 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
 
 __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
     constexpr int N1 = HTile<1>::R * HTile<1>::W;
-    __shared__ double sxu[N1], sxv[N1], snu[N1], snv[N1];
+    __shared__ double sxu[N1], sxv[N1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     const TileId T = tile_id();
     const int i0 = T.x * TILE_X, j0 = T.y * TILE_ROWS;
@@ -299,8 +297,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
     // the block's own operator rows are not the first or last face / row and
     // their stencils stay inside the halo-1 tile: no boundary rows, no wraps
     const bool inner = i0 >= 1 && i0 + TILE_X - 1 <= p.nx - 2 && j0 >= 1 && j0 + TILE_ROWS - 1 <= p.ny - 2;
-    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, snu, snv);
-    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv, snu, snv);
+    if (inner) vel_start_tile<true>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
+    else vel_start_tile<false>(p, b, alpha, nb, i0, j0, tid, sxu, sxv);
 }
 
 // q0 = (I - alpha M) p0 with p0 = r0 (k_vel_start's) and the partials of the
@@ -1397,7 +1395,7 @@ void launch_vel_refine(hipStream_t s, const Params& p, const VelBufs& b, double 
     rb.Un = b.xu;  // x as it is (its Dirichlet values already in place)
     rb.Vn = b.xv;
     rb.extrap = 0;
-    rb.Su = rb.Sv = nullptr;
+    rb.Su = rb.Sv = rb.Mu = rb.Mv = nullptr;
     k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, rb, alpha, st, nb);
     k_vel_check<<<1, BLOCK, 0, s>>>(rb, st, nb, slot, tol);
     if (next) k_vel_q0<float><<<grid2(p), kBlock, 0, s>>>(p, rb, alpha, nb, st, slot, tol, 0);

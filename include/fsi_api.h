@@ -251,7 +251,8 @@ fsi_status fsi_integrate_step(fsi_context* ctx, fsi_state* st, fsi_workspace* ws
  * instances one after another from its drivers; this entry point has no
  * single counterpart there: it enqueues every instance's planned step on its
  * context's stream before waiting for any, so small instances run
- * concurrently on the device.  Each instance's result equals that of
+ * concurrently on the device, the launches shared among up to
+ * FSI_ENSEMBLE_THREADS (default 8) host threads.  Each instance's result equals that of
  * fsi_integrate_step on it; status[k] receives its status (the message via
  * fsi_last_error(ctx[k])), and the first non-OK status is returned.
  * `reports` and `stats` (n entries each) may be NULL. */
