@@ -62,7 +62,6 @@ struct fsi_context {
     double* partials = nullptr;
     Status* st = nullptr;       // device
     Status* st_host = nullptr;  // pinned mirror
-    Status* st_reset = nullptr; // pinned, the Status every step starts from (never written after set-up)
     std::vector<MgGeom> geo;
     CoarsePlan cplan{};         // single-workgroup coarse solve of geo.back()
     double* co_dense = nullptr;  // its dense bottom inverse
@@ -74,6 +73,7 @@ struct fsi_context {
     int use_spectral = 1;
     SpecPlan spec{};
     double* spec_tab = nullptr;  // device tables of both axes
+    double* spec_part = nullptr;  // partials of the fused transform passes (inside spec_tab)
     int last_vel_iters[3] = {8, 8, 8};
     int last_proj_iters[3] = {6, 6, 6};
     // Planned steps: each solve enqueued with a planned iteration count and no
@@ -522,7 +522,9 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         const int kx = spec_kind(g0.nx, g0.px, g0.dl, g0.dr), ky = spec_kind(g0.ny, g0.py, g0.db, g0.dt);
         if (c->use_spectral && kx != SPEC_NONE && ky != SPEC_NONE) {
             const std::vector<double> tx = spec_tables(g0.nx, kx, g0.ihx2), ty = spec_tables(g0.ny, ky, g0.ihy2);
-            HIPCHECK(c, dalloc(c, &c->spec_tab, tx.size() + ty.size()));
+            const size_t npart = (size_t)SPEC_PART_ROWS * ((g0.ny + 1) / 2);
+            HIPCHECK(c, dalloc(c, &c->spec_tab, tx.size() + ty.size() + npart));
+            c->spec_part = c->spec_tab + tx.size() + ty.size();
             HIPCHECK(c, hipMemcpyAsync(c->spec_tab, tx.data(), tx.size() * sizeof(double), hipMemcpyHostToDevice,
                                        c->stream));
             HIPCHECK(c, hipMemcpyAsync(c->spec_tab + tx.size(), ty.data(), ty.size() * sizeof(double),
@@ -549,11 +551,6 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     HIPCHECK(c, hipMemsetAsync(c->st, 0, sizeof(Status), c->stream));
     HIPCHECK(c, hipHostMalloc((void**)&c->st_host, sizeof(Status), hipHostMallocDefault));
     std::memset(c->st_host, 0, sizeof(Status));
-    HIPCHECK(c, hipHostMalloc((void**)&c->st_reset, sizeof(Status), hipHostMallocDefault));
-    std::memset(c->st_reset, 0, sizeof(Status));
-    c->st_reset->phi_min = INFINITY;
-    c->st_reset->phi_max = -INFINITY;
-    c->st_reset->min_eig = INFINITY;
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipGetLastError());
     return FSI_OK;
@@ -569,7 +566,6 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
-    if (c->st_reset) (void)hipHostFree(c->st_reset);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -967,9 +963,10 @@ static fsi_status status_push(fsi_context* c) {
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     return FSI_OK;
 }
-// The step's starting Status from the constant pinned copy: nothing to wait for.
+// The step's starting Status, written by a kernel: nothing to wait for.
 static fsi_status status_push_reset(fsi_context* c) {
-    HIPCHECK(c, hipMemcpyAsync(c->st, c->st_reset, sizeof(Status), hipMemcpyHostToDevice, c->stream));
+    launch_status_reset(c->stream, c->st);
+    HIPCHECK(c, hipGetLastError());
     return FSI_OK;
 }
 static fsi_status status_pull(fsi_context* c) {
@@ -1167,8 +1164,16 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
     b.p[1] = w->pp[1];
     b.w = w->w;
     b.partials = c->partials;
-    launch_proj_rhs(c->stream, p, b);
-    if (c->spec.enabled) launch_spectral_solve(c->stream, c->spec, p.P, b.rhs, b.x);
+    if (c->spec.enabled) {  // the right-hand side, its sums and the mean of x from the transform passes
+        const int np = (p.ny + 1) / 2;
+        b.rhs_part = c->spec_part;
+        b.rhs_np = np;
+        b.q_part = c->spec_part + (size_t)4 * np;
+        b.q_np = np;
+        launch_spectral_proj(c->stream, c->spec, p, b);
+    } else {
+        launch_proj_rhs(c->stream, p, b);
+    }
     const int ps = slot < 0 ? 0 : slot;
     const int last = c->num.projection_maxiter;
     if (fixed > 0 && std::min(fixed, last + 1) == 1) {

@@ -112,6 +112,7 @@ int diag_modes_per_launch();
 // (x, y, phase integral) of the centroid from the reduced values of
 // launch_diag_state, the centroid NaN when the integral is not above eps
 void launch_centroid_result(hipStream_t s, const double* reduced, double* result);
+void launch_status_reset(hipStream_t s, Status* st);
 
 // ---- velocity predictor CG (fsi_solvers.hip) ----
 struct VelBufs {
@@ -178,6 +179,14 @@ struct ProjBufs {
     double* p[2];
     double* w;
     double* partials;
+    // With the direct solve the right-hand side is formed by its first
+    // transform pass and the mean of x summed by its last: their per-workgroup
+    // partials (rhs: 4 rows of rhs_np, x: one row of q_np), or null (the
+    // per-tile partial sets of the kernels below)
+    double* rhs_part;
+    int rhs_np;
+    double* q_part;
+    int q_np;
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
 // x = 2 cur - prev (extrap) or x = cur: the initial guess of a projection
@@ -257,6 +266,14 @@ std::vector<double> spec_tables(int n, int kind, double h2inv);
 // x = solution of (-A) x = -rhs with the (0, 0) mode set to zero when A is
 // singular (the zero-mean gauge); rhs and x are cell fields of pitch P.
 void launch_spectral_solve(hipStream_t s, const SpecPlan& sp, int P, const double* rhs, double* x);
+// The projection's right-hand side and direct solve in the three transform
+// passes: the first forms rhs = div(v*) - b_bc from b.Uin, b.Vin (injecting
+// the boundary faces when b.inject) with the compatibility partials into
+// b.rhs_part, the last adds the partials of the area-weighted sum of x into
+// b.q_part; the solution goes to b.x.
+void launch_spectral_proj(hipStream_t s, const SpecPlan& sp, const Params& p, const ProjBufs& b);
+// partials (rows of (ny + 1) / 2 doubles) that launch_spectral_proj writes
+constexpr int SPEC_PART_ROWS = 5;
 // kernel attributes the plan's launches need (LDS above 64 KiB); false if the
 // device cannot run them
 bool spec_prepare(const SpecPlan& sp);

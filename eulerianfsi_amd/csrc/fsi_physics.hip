@@ -1780,4 +1780,20 @@ void launch_centroid_result(hipStream_t s, const double* reduced, double* result
     k_centroid_result<<<1, 64, 0, s>>>(reduced, result);
 }
 
+// The Status a step starts from: every counter, flag and sum zero, the
+// extrema at their identities.  A kernel rather than a copy from the host,
+// so the step's first launch does not wait on a host-to-device transfer.
+__global__ void __launch_bounds__(256) k_status_reset(Status* st) {
+    constexpr int W = (int)(sizeof(Status) / sizeof(int));
+    int* w = reinterpret_cast<int*>(st);
+    for (int k = threadIdx.x; k < W; k += blockDim.x) w[k] = 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        st->phi_min = INFINITY;
+        st->phi_max = -INFINITY;
+        st->min_eig = INFINITY;
+    }
+}
+void launch_status_reset(hipStream_t s, Status* st) { k_status_reset<<<1, 256, 0, s>>>(st); }
+
 }  // namespace fsi

@@ -229,43 +229,35 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         bu[r] = b.ru[kc[r]];
         bv[r] = b.rv[kc[r]];
     }
-    // u^n and M u^n on the own cells (the reference residual)
+    // |b|^2 and |b - A u^n|^2 from u^n and M u^n on the own cells, before x0
     const bool ref = b.Su != nullptr;
-    double nu[ROWS], nv[ROWS], mu[ROWS], mv[ROWS];
+    double acc3[3] = {0.0, 0.0, 0.0};  // |r0|^2, |b|^2, |b - A u^n|^2
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
-        nu[r] = ref ? b.Su[kc[r]] : 0.0;
-        nv[r] = ref ? b.Sv[kc[r]] : 0.0;
-        mu[r] = ref ? b.Mu[kc[r]] : 0.0;
-        mv[r] = ref ? b.Mv[kc[r]] : 0.0;
+        const double nu = ref ? b.Su[kc[r]] : 0.0, nv = ref ? b.Sv[kc[r]] : 0.0;
+        const double mu = ref ? b.Mu[kc[r]] : 0.0, mv = ref ? b.Mv[kc[r]] : 0.0;
+        if (!own[r]) continue;
+        const int j = j0 + threadIdx.y + r * TILE_Y;
+        const bool fu = !dir_u<IN>(p, i), fv = !dir_v<IN>(p, j);
+        const double cu = fu ? bu[r] : gul, cv = fv ? bv[r] : gvb;
+        const double eu = fu ? cu - (nu - alpha * mu) : cu - nu;
+        const double ev = fv ? cv - (nv - alpha * mv) : cv - nv;
+        acc3[1] += cu * cu;
+        acc3[1] += cv * cv;
+        if (ref) acc3[2] += eu * eu + ev * ev;
     }
     x0_batch<0, NI1 / 2, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
     x0_batch<NI1 / 2, NI1, N1>(b, idx1, fl, tid, gul, gvb, sxu, sxv);
     __syncthreads();
     const T1 Xu{sxu, i0, j0, p.nx, p.ny}, Xv{sxv, i0, j0, p.nx, p.ny};
-    double acc3[3] = {0.0, 0.0, 0.0};  // |r0|^2, |b|^2, |b - A u^n|^2
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         if (!own[r]) continue;
         const int j = j0 + threadIdx.y + r * TILE_Y;
         const size_t k = kc[r];
-        double ru = 0.0, rv = 0.0, eu = 0.0, ev = 0.0;
-        if (!dir_u<IN>(p, i)) {
-            ru = bu[r] - (Xu(i, j) - alpha * visc_u<IN>(p, Xu, Xv, i, j, 0));
-            if (ref) eu = bu[r] - (nu[r] - alpha * mu[r]);
-            acc3[1] += bu[r] * bu[r];
-        } else {
-            if (ref) eu = gul - nu[r];
-            acc3[1] += gul * gul;
-        }
-        if (!dir_v<IN>(p, j)) {
-            rv = bv[r] - (Xv(i, j) - alpha * visc_v<IN>(p, Xu, Xv, i, j, 0));
-            if (ref) ev = bv[r] - (nv[r] - alpha * mv[r]);
-            acc3[1] += bv[r] * bv[r];
-        } else {
-            if (ref) ev = gvb - nv[r];
-            acc3[1] += gvb * gvb;
-        }
+        double ru = 0.0, rv = 0.0;
+        if (!dir_u<IN>(p, i)) ru = bu[r] - (Xu(i, j) - alpha * visc_u<IN>(p, Xu, Xv, i, j, 0));
+        if (!dir_v<IN>(p, j)) rv = bv[r] - (Xv(i, j) - alpha * visc_v<IN>(p, Xu, Xv, i, j, 0));
         if (!b.restart) {  // a restart reads x as it is
             b.xu[k] = Xu(i, j);
             b.xv[k] = Xv(i, j);
@@ -278,7 +270,6 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
             b.r_v[0][k] = rv;
         }
         acc3[0] += ru * ru + rv * rv;
-        acc3[2] += eu * eu + ev * ev;
     }
     put_partials<3>(acc3, pset(b.partials, nb, b.restart ? PS_VELR : PS_VELI), nb);
 }
@@ -720,6 +711,21 @@ struct GAcc {
     int P;
     __device__ __forceinline__ double operator()(int i, int j) const { return a[(size_t)j * P + i]; }
 };
+// neg_lap's value from the cell and its four neighbours (periodic ones
+// wrapped) already in registers; the same operations in the same order
+__device__ __forceinline__ double neg_lap5(const MgGeom& g, int i, int j, double c, double w, double e, double s,
+                                           double n) {
+    double sx = 0.0, sy = 0.0, cx = 0.0, cy = 0.0;
+    if (i > 0 || g.px) { sx += w; cx += 1.0; }
+    else if (g.dl) cx += 2.0;
+    if (i < g.nx - 1 || g.px) { sx += e; cx += 1.0; }
+    else if (g.dr) cx += 2.0;
+    if (j > 0 || g.py) { sy += s; cy += 1.0; }
+    else if (g.db) cy += 2.0;
+    if (j < g.ny - 1 || g.py) { sy += n; cy += 1.0; }
+    else if (g.dt) cy += 2.0;
+    return g.ihx2 * (cx * c - sx) + g.ihy2 * (cy * c - sy);
+}
 
 // 1-D bilinear prolongation of fine index f from the coarse grid: parent
 // weight 3/4, nearer neighbour 1/4; outside a non-periodic wall the ghost is
@@ -802,10 +808,8 @@ __global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb
 // The projection scalars from the right-hand side totals: compatibility
 // check (src/Projection.jl:206-221) and solver tolerance.  Returns the mean
 // of the right-hand side (removed from b when A is singular).
-__device__ __forceinline__ double proj_scalars(const Params& p, const ProjBufs& b, Status* st, int nb, int has_null,
-                                               double rtol, int maxiter) {
-    double tot[4];
-    get_totals<4>(pset(b.partials, nb, PS_RHS), nb, tot);
+__device__ __forceinline__ double proj_scalars_of(const Params& p, const double (&tot)[4], Status* st,
+                                                  int has_null, double rtol, int maxiter) {
     const double area = p.hx * p.hy;
     const double n = (double)p.nx * (double)p.ny;
     const double mean = has_null ? (tot[0] / area) / n : 0.0;
@@ -829,6 +833,13 @@ __device__ __forceinline__ double proj_scalars(const Params& p, const ProjBufs& 
         s.done = 0;
     }
     return mean;
+}
+__device__ __forceinline__ double proj_scalars(const Params& p, const ProjBufs& b, Status* st, int nb, int has_null,
+                                               double rtol, int maxiter) {
+    double tot[4];
+    if (b.rhs_part) get_totals<4>(b.rhs_part, b.rhs_np, tot);  // from the first transform pass
+    else get_totals<4>(pset(b.partials, nb, PS_RHS), nb, tot);
+    return proj_scalars_of(p, tot, st, has_null, rtol, maxiter);
 }
 
 // the projection scalars and r = b - (-A) x with b = -(rhs - mean)
@@ -1195,36 +1206,73 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
 // the thread's own faces (and the far faces of the last column and row), and
 // the checks of k_proj_check<0> (q_out = q - mean, constraint, velocity
 // extrema) from the updated faces of each cell, the right and top ones
-// recomputed exactly as the neighbouring thread computes them.
-__global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, ProjBufs b, Status* st, int nb, int has_null,
-                                                        double lxly) {
-    double tm[1];
-    get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
-    const double mean = has_null ? tm[0] / lxly : 0.0;
+// recomputed exactly as the neighbouring thread computes them.  DIRECT: also
+// k_proj_direct's test of the solution (the projection scalars, r.r of PCG
+// iteration 0 and the true residual), with the mean of q from the partials
+// of the last transform pass; otherwise the test and the mean are
+// k_proj_direct's.
+template <bool DIRECT>
+__global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
+                                                        int has_null, double rtol, int maxiter, double lxly) {
     const double area = p.hx * p.hy;
     GAcc Q{b.x, p.P};
     const int i = tile_id().x * TILE_X + threadIdx.x;
     const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
     bool in[ROWS];
     size_t kk[ROWS];
-    double qc[ROWS], qw[ROWS], qs[ROWS], qe[ROWS], qn[ROWS], uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS];
+    double qc[ROWS], qw[ROWS], qs[ROWS], qe[ROWS], qn[ROWS], uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS], rh[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         const int j = jb + r * TILE_Y;
         in[r] = i < p.nx && j < p.ny;
         const size_t k = in[r] ? at(p, i, j) : 0;
         kk[r] = k;
+        // the neighbours inside the grid; across a periodic side the wrapped
+        // one (only the residual uses those; the boundary faces' gradients
+        // are grad_face_x/y's)
+        const size_t kw = i > 0 ? k - 1 : (DIRECT && p.px ? k + p.nx - 1 : k);
+        const size_t ke = i < p.nx - 1 ? k + 1 : (DIRECT && p.px ? k - (p.nx - 1) : k);
+        const size_t ks = j > 0 ? k - p.P : (DIRECT && p.py ? k + (size_t)(p.ny - 1) * p.P : k);
+        const size_t kn = j < p.ny - 1 ? k + p.P : (DIRECT && p.py ? k - (size_t)(p.ny - 1) * p.P : k);
         qc[r] = b.x[k];
-        qw[r] = b.x[in[r] && i > 0 ? k - 1 : k];
-        qs[r] = b.x[in[r] && j > 0 ? k - p.P : k];
-        qe[r] = b.x[in[r] && i < p.nx - 1 ? k + 1 : k];
-        qn[r] = b.x[in[r] && j < p.ny - 1 ? k + p.P : k];
+        qw[r] = b.x[in[r] ? kw : k];
+        qs[r] = b.x[in[r] ? ks : k];
+        qe[r] = b.x[in[r] ? ke : k];
+        qn[r] = b.x[in[r] ? kn : k];
         uL[r] = b.Uin[k];
         uR[r] = b.Uin[k + 1];
         vB[r] = b.Vin[k];
         vT[r] = b.Vin[k + p.P];
+        rh[r] = DIRECT ? b.rhs[k] : 0.0;
     }
-    double acc[3] = {0.0, 0.0, 0.0};  // (true residual: k_proj_direct's), constraint, NaN faces
+    double mean;
+    // true residual (DIRECT), constraint, NaN faces, r.r of PCG iteration 0 (DIRECT)
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (DIRECT) {
+        double nl[ROWS];
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r)
+            nl[r] = neg_lap5(g, i, jb + r * TILE_Y, qc[r], qw[r], qe[r], qs[r], qn[r]);
+        // the right-hand side totals and the sum of x in one pass (q_part
+        // follows rhs_part's four rows)
+        double tot[5];
+        get_totals<5>(b.rhs_part, b.rhs_np, tot);
+        const double trhs[4] = {tot[0], tot[1], tot[2], tot[3]};
+        const double mrhs = proj_scalars_of(p, trhs, st, has_null, rtol, maxiter);
+        mean = has_null ? tot[4] / lxly : 0.0;
+#pragma unroll
+        for (int r = 0; r < ROWS; ++r) {
+            if (!in[r]) continue;
+            const double rr = -(rh[r] - mrhs) - nl[r];
+            const double res = -nl[r] - rh[r];
+            acc[3] += rr * rr;
+            acc[0] += res * res;
+        }
+    } else {
+        double tm[1];
+        get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
+        mean = has_null ? tm[0] / lxly : 0.0;
+    }
     double linf = 0.0, umax = 0.0, vmax = 0.0;
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -1252,8 +1300,8 @@ __global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, ProjBufs b, St
         acc[2] += ((isnan(u0) ? 1.0 : 0.0) + (isnan(v0) ? 1.0 : 0.0)) +
                   ((ue && isnan(u1) ? 1.0 : 0.0) + (ve && isnan(v1) ? 1.0 : 0.0));
     }
-    __shared__ double lds[3 * 16];
-    block_sum<3>(acc, lds);
+    __shared__ double lds[4 * 16];
+    block_sum<4>(acc, lds);
     __shared__ double lmax[3][BLOCK / 64];
     linf = wave_max(linf);
     umax = wave_max(umax);
@@ -1273,6 +1321,10 @@ __global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, ProjBufs b, St
         }
         const int bid = blockIdx.x + blockIdx.y * gridDim.x;
         double* dst = pset(b.partials, nb, PS_CHECK);
+        if (DIRECT) {
+            pset(b.partials, nb, PS_RR)[bid] = acc[3];
+            dst[bid] = acc[0];
+        }
         dst[(size_t)nb + bid] = acc[1];
         dst[2 * (size_t)nb + bid] = m[0];
         double* vm = pset(b.partials, nb, PS_VMAX);
@@ -1512,9 +1564,16 @@ void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const 
 void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double rtol, int maxiter, double lxly, int slot) {
     const int nb = grid_blocks(p);
+    const bool apart = b.Uin != b.U && b.Vin != b.V;  // the fused pass reads the neighbours' unprojected faces
+    if (apart && b.rhs_part && b.q_part == b.rhs_part + (size_t)4 * b.rhs_np && b.q_np == b.rhs_np) {
+        // the direct solve's own partials: test, update and checks in one pass
+        k_proj_upcheck<true><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter, lxly);
+        k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
+        return;
+    }
     k_proj_direct<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter);
-    if (b.Uin != b.U && b.Vin != b.V) {  // the fused pass reads the neighbours' unprojected faces
-        k_proj_upcheck<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, has_null, lxly);
+    if (apart) {
+        k_proj_upcheck<false><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter, lxly);
     } else {
         k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
         k_proj_check<0><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);

@@ -177,10 +177,75 @@ __device__ __forceinline__ void line_spectrum(int n, const SpecLds& t, const dou
     }
 }
 
+// The right-hand side of the projection on the points i = tid + it NT of
+// rows j0 and j0 + 1 (for it in [C0, C0 + 4)), as k_proj_rhs forms it:
+// rhs = (div(v*) - G) - b_bc with G = 0, after _inject_compact_predictor!
+// (:543-579) has given the far faces the boundary data.  Every load of the
+// chunk is issued before the first store; acc gathers the compatibility sums
+// (rhs dA, |rhs| dA, rhs^2, div^2 dA) in the order of the thread's points.
+template <int IT, int NT, int C0>
+__device__ __forceinline__ void proj_rhs_chunk(const Params& p, const ProjBufs& pb, int tid, int j0, bool two,
+                                               double (&a)[IT], double (&b)[IT], double (&acc)[4]) {
+    constexpr int CH = IT - C0 < 4 ? IT - C0 : 4;
+    const int n = p.nx;
+    if (C0 * NT >= n) {
+#pragma unroll
+        for (int c = 0; c < CH; ++c) a[C0 + c] = b[C0 + c] = 0.0;
+        return;
+    }
+    double uL[2][CH], uR[2][CH], vB[2][CH], vT[2][CH], lb[2][CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        const int i = tid + (C0 + c) * NT;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const bool in = i < n && (h == 0 || two);
+            const size_t k = in ? (size_t)(j0 + h) * p.P + i : 0;
+            uL[h][c] = pb.Uin[k];
+            uR[h][c] = pb.Uin[k + 1];
+            vB[h][c] = pb.Vin[k];
+            vT[h][c] = pb.Vin[k + p.P];
+            lb[h][c] = pb.lap_bc[k];
+        }
+    }
+    const double area = p.hx * p.hy;
+#pragma unroll
+    for (int c = 0; c < CH; ++c) {
+        const int i = tid + (C0 + c) * NT;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            double& out = h == 0 ? a[C0 + c] : b[C0 + c];
+            out = 0.0;
+            if (!(i < n && (h == 0 || two))) continue;
+            const int j = j0 + h;
+            const size_t k = (size_t)j * p.P + i;
+            double ur = uR[h][c], vt = vT[h][c];
+            if (pb.inject && i == n - 1) {
+                ur = p.px ? pb.Uin[(size_t)j * p.P] : p.bU.r.value;
+                pb.Uin[k + 1] = ur;
+            }
+            if (pb.inject && j == p.ny - 1) {
+                vt = p.py ? pb.Vin[i] : p.bV.t.value;
+                pb.Vin[k + p.P] = vt;
+            }
+            const double div = (ur - uL[h][c]) * p.ihx + (vt - vB[h][c]) * p.ihy;
+            const double rhs = (div - 0.0) - lb[h][c];
+            pb.rhs[k] = rhs;
+            acc[0] += rhs * area;
+            acc[1] += fabs(rhs) * area;
+            acc[2] += rhs * rhs;
+            acc[3] += div * div * area;
+            out = rhs;
+        }
+    }
+}
+
 // ---- rows: forward transform along x of two rows per workgroup ----------
-template <int KIND, int NT>
+// RHS: the rows are the projection's right-hand side, formed here from the
+// velocity (proj_rhs_at) and stored, with one set of partials per workgroup.
+template <int KIND, int NT, bool RHS>
 __global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, int P, const double* __restrict__ in,
-                                                      double* __restrict__ out, double scale) {
+                                                      double* __restrict__ out, double scale, Params p, ProjBufs pb) {
     constexpr int IT = 2048 / NT;
     extern __shared__ double sp_smem[];
     const int n = ax.n, m = ax.m, tid = threadIdx.x;
@@ -190,11 +255,18 @@ __global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, in
     const bool two = j0 + 1 < nrows;
     const double* r0 = in + (size_t)j0 * P;
     double a[IT], b[IT];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (RHS) {
+        static_assert(IT <= 8, "two chunks of four points");
+        proj_rhs_chunk<IT, NT, 0>(p, pb, tid, j0, two, a, b, acc);
+        if (IT > 4) proj_rhs_chunk<IT, NT, (IT > 4 ? 4 : 0)>(p, pb, tid, j0, two, a, b, acc);
+    } else {
 #pragma unroll
-    for (int it = 0; it < IT; ++it) {  // every load in flight before the first use
-        const int i = tid + it * NT;
-        a[it] = i < n ? r0[i] : 0.0;
-        b[it] = (i < n && two) ? r0[i + P] : 0.0;
+        for (int it = 0; it < IT; ++it) {  // every load in flight before the first use
+            const int i = tid + it * NT;
+            a[it] = i < n ? r0[i] : 0.0;
+            b[it] = (i < n && two) ? r0[i + P] : 0.0;
+        }
     }
     const SpecLds t = load_tables<NT, KIND>(ax, im + pdn(n), false, tid);
 #pragma unroll
@@ -216,11 +288,15 @@ __global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, in
         o0[k] = o1;
         if (two) o0[k + P] = o2;
     }
+    if (RHS) put_partials<4>(acc, pb.rhs_part, gridDim.x);
 }
 
 // ---- rows: inverse transform along x, in place ---------------------------
 template <int KIND, int NT>
-__global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x) {
+// mpart: one partial per workgroup of the area-weighted sum of the two rows
+// written (the mean of the solution, :255-259), or null.
+__global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x, double* mpart,
+                                                      double area) {
     constexpr int IT = 2048 / NT;
     extern __shared__ double sp_smem[];
     const int n = ax.n, m = ax.m, tid = threadIdx.x;
@@ -267,12 +343,19 @@ __global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, in
     __syncthreads();
     lds_fft<NT>(re, im, t.wr, t.wi, m, 1, pdn(n), tid);
     const double inv_n = 1.0 / (double)n;
+    double acc[1] = {0.0};
     for (int i = tid; i < n; i += NT) {  // z = conj(FFT(conj Z)) / n
         const int jv = pd(KIND == SPEC_RDFT ? i : mk_pos(i, n));
         const double s = (KIND == SPEC_ALT && (i & 1)) ? -inv_n : inv_n;
-        r0[i] = s * re[jv];
-        if (two) r0[i + P] = -s * im[jv];
+        const double x0 = s * re[jv], x1 = -s * im[jv];
+        r0[i] = x0;
+        acc[0] += x0 * area;
+        if (two) {
+            r0[i + P] = x1;
+            acc[0] += x1 * area;
+        }
     }
+    if (mpart) put_partials<1>(acc, mpart, gridDim.x);
 }
 
 // ---- columns: forward along y, divide, inverse along y; 2C columns -------
@@ -421,13 +504,21 @@ static size_t cols_lds(int n, int kind, int C) {
 template <int KX>
 static void rows_fwd(hipStream_t s, const SpecPlan& sp, int P, const double* in, double* out) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_fwd<KX, SP_ROW_THREADS><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(sp.x, rows, P, in,
-                                                                                              out, -1.0);
+    k_spec_rows_fwd<KX, SP_ROW_THREADS, false><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(
+        sp.x, rows, P, in, out, -1.0, Params{}, ProjBufs{});
 }
 template <int KX>
-static void rows_inv(hipStream_t s, const SpecPlan& sp, int P, double* x) {
+static void rows_fwd_rhs(hipStream_t s, const SpecPlan& sp, const Params& p, const ProjBufs& b) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_inv<KX, SP_ROW_THREADS><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(sp.x, rows, P, x);
+    k_spec_rows_fwd<KX, SP_ROW_THREADS, true><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(
+        sp.x, rows, p.P, nullptr, b.x, -1.0, p, b);
+}
+template <int KX>
+static void rows_inv(hipStream_t s, const SpecPlan& sp, int P, double* x, double* mpart = nullptr,
+                     double area = 0.0) {
+    const int n = sp.x.n, rows = sp.y.n;
+    k_spec_rows_inv<KX, SP_ROW_THREADS><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(sp.x, rows, P, x,
+                                                                                              mpart, area);
 }
 template <int KY>
 static void cols(hipStream_t s, const SpecPlan& sp, int P, double* x) {
@@ -454,7 +545,8 @@ static bool allow_lds(const void* kernel, size_t bytes) {
 }
 template <int KX>
 static bool prepare_rows(int n) {
-    return allow_lds(reinterpret_cast<const void*>(&k_spec_rows_fwd<KX, SP_ROW_THREADS>), rows_lds(n, KX)) &&
+    return allow_lds(reinterpret_cast<const void*>(&k_spec_rows_fwd<KX, SP_ROW_THREADS, false>), rows_lds(n, KX)) &&
+           allow_lds(reinterpret_cast<const void*>(&k_spec_rows_fwd<KX, SP_ROW_THREADS, true>), rows_lds(n, KX)) &&
            allow_lds(reinterpret_cast<const void*>(&k_spec_rows_inv<KX, SP_ROW_THREADS>), rows_lds(n, KX));
 }
 template <int KY>
@@ -490,6 +582,25 @@ void launch_spectral_solve(hipStream_t s, const SpecPlan& sp, int P, const doubl
     case SPEC_COS: rows_inv<SPEC_COS>(s, sp, P, x); break;
     case SPEC_ALT: rows_inv<SPEC_ALT>(s, sp, P, x); break;
     default: rows_inv<SPEC_RDFT>(s, sp, P, x); break;
+    }
+}
+
+void launch_spectral_proj(hipStream_t s, const SpecPlan& sp, const Params& p, const ProjBufs& b) {
+    switch (sp.x.kind) {
+    case SPEC_COS: rows_fwd_rhs<SPEC_COS>(s, sp, p, b); break;
+    case SPEC_ALT: rows_fwd_rhs<SPEC_ALT>(s, sp, p, b); break;
+    default: rows_fwd_rhs<SPEC_RDFT>(s, sp, p, b); break;
+    }
+    switch (sp.y.kind) {
+    case SPEC_COS: cols<SPEC_COS>(s, sp, p.P, b.x); break;
+    case SPEC_ALT: cols<SPEC_ALT>(s, sp, p.P, b.x); break;
+    default: cols<SPEC_RDFT>(s, sp, p.P, b.x); break;
+    }
+    const double area = p.hx * p.hy;
+    switch (sp.x.kind) {
+    case SPEC_COS: rows_inv<SPEC_COS>(s, sp, p.P, b.x, b.q_part, area); break;
+    case SPEC_ALT: rows_inv<SPEC_ALT>(s, sp, p.P, b.x, b.q_part, area); break;
+    default: rows_inv<SPEC_RDFT>(s, sp, p.P, b.x, b.q_part, area); break;
     }
 }
 
