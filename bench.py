@@ -14,9 +14,11 @@ it develop the flow, the last W of them are the warm-up.
 The JSON line carries:
   roofline     -- the dominant kernel's algorithmic bytes / its mean launch
                   time, measured with HIP events on the library's own stream;
-  cpu_baseline -- the CPU oracle (a C restatement of the reference step, one
-                  thread) timed on a bounded sample: S steps from the GPU's
-                  state after the timed region, same tolerances.
+  cpu_baseline -- the CPU oracle (a C restatement of the reference step)
+                  timed on bounded samples from the GPU's state after the
+                  timed region, same tolerances: its OpenMP build on every
+                  usable core (the reported value), and the serial build on
+                  one thread ("single_thread").
 """
 from __future__ import annotations
 
@@ -171,6 +173,8 @@ def main(argv=None):
     ap.add_argument("--method", default="acdi")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-omp-steps", type=int, default=5,
+                    help="steps of the OpenMP build of the oracle on all usable cores (0: skip)")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--only-kernel", default=None, help="time one kernel standalone and exit (profiling runs)")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision run")
@@ -332,20 +336,35 @@ def main(argv=None):
         del ework, ectx, states
 
     cpu = None
-    if args.cpu_steps > 0:
-        from oracle.oracle import Oracle, state_dict
-        o = Oracle(case.metrics, case.bcs, case.models, case.numerics)
-        s = state_dict(state)
-        o.correction[:] = work.correction
-        c0 = time.perf_counter()
-        for k in range(args.cpu_steps):
-            o.integrate_step(s, t + k * dt, dt)
-            print(f"cpu oracle step {k} done", file=sys.stderr, flush=True)
-        cpu_s = (time.perf_counter() - c0) / args.cpu_steps
-        cpu = {"value": 1.0 / cpu_s, "unit": "IMEX steps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} step(s) of the same N={args.N} {args.method} cavity from the GPU state "
-                         f"after the timed region, same stopping rules, 1 thread",
-               "host": host_cpu()}
+    if args.cpu_steps > 0 or args.cpu_omp_steps > 0:
+        from oracle.oracle import OMP_LIB_PATH, Oracle, state_dict
+
+        def oracle_leg(nsteps, lib_path, label):
+            o = Oracle(case.metrics, case.bcs, case.models, case.numerics, lib_path=lib_path)
+            s = state_dict(state)
+            o.correction[:] = work.correction
+            c0 = time.perf_counter()
+            for k in range(nsteps):
+                o.integrate_step(s, t + k * dt, dt)
+                print(f"cpu oracle ({label}) step {k} done", file=sys.stderr, flush=True)
+            return nsteps / (time.perf_counter() - c0)
+
+        sample = (f"step(s) of the same N={args.N} {args.method} cavity from the GPU state after the timed region, "
+                  f"same stopping rules")
+        serial = None
+        if args.cpu_steps > 0:
+            serial = {"value": oracle_leg(args.cpu_steps, None, "1 thread"), "unit": "IMEX steps/s", "cores": 1,
+                      "kind": "port", "sample": f"{args.cpu_steps} {sample}, 1 thread"}
+        cpu = serial
+        if args.cpu_omp_steps > 0:
+            v = oracle_leg(args.cpu_omp_steps, OMP_LIB_PATH, "OpenMP")
+            import ctypes
+            threads = ctypes.CDLL("libgomp.so.1").omp_get_max_threads()
+            cpu = {"value": v, "unit": "IMEX steps/s", "cores": threads, "kind": "port",
+                   "sample": f"{args.cpu_omp_steps} {sample}, the oracle's OpenMP build (cell loops on {threads} "
+                             f"threads, red-black multigrid smoother)",
+                   "single_thread": serial}
+        cpu["host"] = host_cpu()
 
     out = {
         "metric": METRIC, "value": steps_per_s, "unit": "IMEX steps/s", "n_gpus": 1, "steps": args.steps,

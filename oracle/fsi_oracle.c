@@ -28,6 +28,22 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <malloc.h>
+
+/* The OpenMP build of this file (liboracle_fsi_omp.so, the all-cores CPU
+ * baseline of the benchmark; never the parity checker) runs the loops marked
+ * OMP_FOR in parallel: each of them writes disjoint entries.  Its dot
+ * products sum per thread and its multigrid smoother visits the cells in
+ * red-black order, so it agrees with the serial build to the solver
+ * tolerances, not bitwise.  The serial build ignores the markers. */
+#ifdef _OPENMP
+#define OMP_FOR _Pragma("omp parallel for schedule(static)")
+/* the same on a multigrid level, serial below 128 x 128 cells */
+#define OMP_FOR_LEVEL _Pragma("omp parallel for schedule(static) if (nx * ny >= 16384)")
+#else
+#define OMP_FOR
+#define OMP_FOR_LEVEL
+#endif
 
 #define UX(a, i, j) (a)[(size_t)(i) + (size_t)(j) * (size_t)(c->nx + 1)]
 #define VY(a, i, j) (a)[(size_t)(i) + (size_t)(j) * (size_t)(c->nx)]
@@ -80,7 +96,17 @@ static int set_err(fsio_ctx* c, int code, const char* fmt, ...) {
 const char* fsio_last_error(const fsio_ctx* c) { return c ? c->err : "null context"; }
 
 static double* dalloc(size_t n) {
+#ifdef _OPENMP
+    /* zeroed by the threads (calloc would clear, and fault in, the pages on
+       one core) */
+    double* p = (double*)malloc((n ? n : 1) * sizeof(double));
+    if (p) {
+        OMP_FOR
+        for (long long k = 0; k < (long long)(n ? n : 1); ++k) p[k] = 0.0;
+    }
+#else
     double* p = (double*)calloc(n ? n : 1, sizeof(double));
+#endif
     if (!p) {
         fprintf(stderr, "fsi_oracle: out of memory\n");
         abort();
@@ -189,6 +215,7 @@ const double* fsio_metric(const fsio_ctx* c, int which, int* len) {
 static void c2fx(fsio_ctx* c, const double* f, double* face) {
     const metrics_t* m = &c->mt;
     int nx = c->nx, ny = c->ny;
+    OMP_FOR
     for (int j = 0; j < ny; ++j) {
         if (c->px) {
             double v = m->wL_c2fx[0] * CC(f, nx - 1, j) + m->wR_c2fx[0] * CC(f, 0, j);
@@ -207,6 +234,7 @@ static void c2fx(fsio_ctx* c, const double* f, double* face) {
 static void c2fy(fsio_ctx* c, const double* f, double* face) {
     const metrics_t* m = &c->mt;
     int nx = c->nx, ny = c->ny;
+    OMP_FOR
     for (int i = 0; i < nx; ++i) {
         if (c->py) {
             double v = m->wB_c2fy[0] * CC(f, i, ny - 1) + m->wT_c2fy[0] * CC(f, i, 0);
@@ -224,6 +252,7 @@ static void c2fy(fsio_ctx* c, const double* f, double* face) {
 /* divergence, Operators.jl:407-425 */
 static void divergence(fsio_ctx* c, const double* U, const double* V, double* out) {
     const metrics_t* m = &c->mt;
+    OMP_FOR
     for (int j = 0; j < c->ny; ++j)
         for (int i = 0; i < c->nx; ++i)
             CC(out, i, j) = (UX(U, i + 1, j) - UX(U, i, j)) / m->dx_c[i] +
@@ -240,6 +269,7 @@ static void scalar_gradient(fsio_ctx* c, const fsi_bc_setup* bc, const double* q
                             double* gy) {
     const metrics_t* m = &c->mt;
     int nx = c->nx, ny = c->ny;
+    OMP_FOR
     for (int j = 0; j < ny; ++j) {
         /* left face (:216-238) */
         if (c->px) {
@@ -265,6 +295,7 @@ static void scalar_gradient(fsio_ctx* c, const fsi_bc_setup* bc, const double* q
             UX(gx, nx, j) = bc->right.value;
         }
     }
+    OMP_FOR
     for (int i = 0; i < nx; ++i) { /* :275-334 */
         if (c->py) {
             double inv_h = 1.0 / m->dy_f[0];
@@ -302,6 +333,7 @@ static int velocity_gradient(fsio_ctx* c, const double* U, const double* V, doub
     const fsi_bc_setup* bv = &c->bcs.V;
     double* Uc = dalloc((size_t)nx * ny);
     double* Vc = dalloc((size_t)nx * ny);
+    OMP_FOR
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             CC(du_dx, i, j) = (UX(U, i + 1, j) - UX(U, i, j)) / m->dx_c[i];
@@ -395,6 +427,7 @@ static void stress_divergence(fsio_ctx* c, const double* Txx, const double* Tyy,
     double* Txy_x = dalloc((size_t)(nx + 1) * ny);
     double* Txy_y = dalloc((size_t)nx * (ny + 1));
     /* _center_to_x_faces_linear_boundary :770-789 */
+    OMP_FOR
     for (int j = 0; j < ny; ++j) {
         if (c->px) {
             double v = m->wL_c2fx[0] * CC(Txy, nx - 1, j) + m->wR_c2fx[0] * CC(Txy, 0, j);
@@ -408,6 +441,7 @@ static void stress_divergence(fsio_ctx* c, const double* Txx, const double* Tyy,
             UX(Txy_x, i, j) = m->wL_c2fx[i] * CC(Txy, i - 1, j) + m->wR_c2fx[i] * CC(Txy, i, j);
     }
     /* _center_to_y_faces_linear_boundary :791-810 */
+    OMP_FOR
     for (int i = 0; i < nx; ++i) {
         if (c->py) {
             double v = m->wB_c2fy[0] * CC(Txy, i, ny - 1) + m->wT_c2fy[0] * CC(Txy, i, 0);
@@ -421,6 +455,7 @@ static void stress_divergence(fsio_ctx* c, const double* Txx, const double* Tyy,
             VY(Txy_y, i, j) = m->wB_c2fy[j] * CC(Txy, i, j - 1) + m->wT_c2fy[j] * CC(Txy, i, j);
     }
     /* d(Txx)/dx on x-faces :642-656 */
+    OMP_FOR
     for (int j = 0; j < ny; ++j) {
         if (c->px) {
             double w = (CC(Txx, 0, j) - CC(Txx, nx - 1, j)) / m->dx_f[0];
@@ -434,6 +469,7 @@ static void stress_divergence(fsio_ctx* c, const double* Txx, const double* Tyy,
         for (int i = 1; i < nx; ++i) UX(Fx, i, j) = (CC(Txx, i, j) - CC(Txx, i - 1, j)) / m->dx_f[i];
     }
     /* + d(Txy)/dy at U locations :659-683 (extrapolate along y of the x-face field) */
+    OMP_FOR
     for (int i = 0; i <= nx; ++i) {
         if (c->py) {
             for (int j = 0; j < ny; ++j) {
@@ -453,6 +489,7 @@ static void stress_divergence(fsio_ctx* c, const double* Txx, const double* Tyy,
         }
     }
     /* d(Tyy)/dy on y-faces :686-700 */
+    OMP_FOR
     for (int i = 0; i < nx; ++i) {
         if (c->py) {
             double w = (CC(Tyy, i, 0) - CC(Tyy, i, ny - 1)) / m->dy_f[0];
@@ -466,6 +503,7 @@ static void stress_divergence(fsio_ctx* c, const double* Txx, const double* Tyy,
         for (int j = 1; j < ny; ++j) VY(Fy, i, j) = (CC(Tyy, i, j) - CC(Tyy, i, j - 1)) / m->dy_f[j];
     }
     /* + d(Txy)/dx at V locations :702-724 */
+    OMP_FOR
     for (int j = 0; j <= ny; ++j) {
         if (c->px) {
             for (int i = 0; i < nx; ++i) {
@@ -551,6 +589,7 @@ static void material_convection(fsio_ctx* c, const double* U, const double* V, c
     double* v2 = dalloc((size_t)nx * ny);
     double* uv = dalloc(nk);
     /* _normal_flux_squares :141-159 */
+    OMP_FOR
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             double ur = (c->px && i == nx - 1) ? UX(U, 0, j) : UX(U, i + 1, j);
@@ -562,6 +601,7 @@ static void material_convection(fsio_ctx* c, const double* U, const double* V, c
         }
     /* _cross_flux_corners :221-247 */
 #define KK(a, i, j) (a)[(size_t)(i) + (size_t)(j) * (size_t)(nx + 1)]
+    OMP_FOR
     for (int j = 0; j <= ny; ++j)
         for (int i = 0; i <= nx; ++i) KK(uv, i, j) = u_at_corner(c, U, i, j) * v_at_corner(c, V, i, j);
     if (c->px)
@@ -571,6 +611,7 @@ static void material_convection(fsio_ctx* c, const double* U, const double* V, c
     /* conservative_convection :268-295 */
     memset(Cu, 0, sizeof(double) * (size_t)(nx + 1) * ny);
     memset(Cv, 0, sizeof(double) * (size_t)nx * (ny + 1));
+    OMP_FOR
     for (int j = 0; j < ny; ++j) {
         if (c->px) {
             UX(Cu, 0, j) = (CC(u2, 0, j) - CC(u2, nx - 1, j)) / m->dx_f[0] +
@@ -581,6 +622,7 @@ static void material_convection(fsio_ctx* c, const double* U, const double* V, c
             UX(Cu, i, j) = (CC(u2, i, j) - CC(u2, i - 1, j)) / m->dx_f[i] +
                            (KK(uv, i, j + 1) - KK(uv, i, j)) / m->dy_c[j];
     }
+    OMP_FOR
     for (int i = 0; i < nx; ++i) {
         if (c->py) {
             VY(Cv, i, 0) = (KK(uv, i + 1, 0) - KK(uv, i, 0)) / m->dx_c[i] +
@@ -597,7 +639,9 @@ static void material_convection(fsio_ctx* c, const double* U, const double* V, c
     double* Sy = dalloc((size_t)nx * (ny + 1));
     c2fx(c, S, Sx);
     c2fy(c, S, Sy);
+    OMP_FOR
     for (size_t k = 0; k < (size_t)(nx + 1) * ny; ++k) Cu[k] -= Sx[k] * U[k];
+    OMP_FOR
     for (size_t k = 0; k < (size_t)nx * (ny + 1); ++k) Cv[k] -= Sy[k] * V[k];
     for (int j = 0; j < ny; ++j) {
         if (c->px) UX(Cu, nx, j) = UX(Cu, 0, j);
@@ -650,10 +694,6 @@ static int upwind_advection(fsio_ctx* c, const double* f, const double* U, const
             rc = upwind_boundary_state(c, &bc->left, u, 0, CC(f, 0, j), &bs);
             UX(fxx, 0, j) = u * (u >= 0.0 ? bs : CC(f, 0, j));
         }
-        for (int i = 1; i < nx; ++i) {
-            u = UX(U, i, j);
-            UX(fxx, i, j) = u * (u >= 0.0 ? CC(f, i - 1, j) : CC(f, i, j));
-        }
         if (c->px) {
             UX(fxx, nx, j) = UX(fxx, 0, j);
         } else if (!rc) {
@@ -672,10 +712,6 @@ static int upwind_advection(fsio_ctx* c, const double* f, const double* U, const
             rc = upwind_boundary_state(c, &bc->bottom, v, 0, CC(f, i, 0), &bs);
             VY(fyy, i, 0) = v * (v >= 0.0 ? bs : CC(f, i, 0));
         }
-        for (int j = 1; j < ny; ++j) {
-            v = VY(V, i, j);
-            VY(fyy, i, j) = v * (v >= 0.0 ? CC(f, i, j - 1) : CC(f, i, j));
-        }
         if (c->py) {
             VY(fyy, i, ny) = VY(fyy, i, 0);
         } else if (!rc) {
@@ -685,11 +721,27 @@ static int upwind_advection(fsio_ctx* c, const double* f, const double* U, const
             VY(fyy, i, ny) = v * (v >= 0.0 ? CC(f, i, ny - 1) : bs);
         }
     }
-    if (!rc)
+    if (!rc) {
+        /* the interior faces (the boundary faces above, in row order, so the
+           first inflow error is the one reported) */
+        OMP_FOR
+        for (int j = 0; j < ny; ++j)
+            for (int i = 1; i < nx; ++i) {
+                double u = UX(U, i, j);
+                UX(fxx, i, j) = u * (u >= 0.0 ? CC(f, i - 1, j) : CC(f, i, j));
+            }
+        OMP_FOR
+        for (int i = 0; i < nx; ++i)
+            for (int j = 1; j < ny; ++j) {
+                double v = VY(V, i, j);
+                VY(fyy, i, j) = v * (v >= 0.0 ? CC(f, i, j - 1) : CC(f, i, j));
+            }
+        OMP_FOR
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i)
                 CC(rhs, i, j) = -((UX(fxx, i + 1, j) - UX(fxx, i, j)) / m->dx_c[i] +
                                   (VY(fyy, i, j + 1) - VY(fyy, i, j)) / m->dy_c[j]);
+    }
     free(fxx);
     free(fyy);
     return rc;
@@ -701,6 +753,7 @@ static void center_gradient(fsio_ctx* c, const double* f, const fsi_bc_setup* bc
                             double* gy, double* gxf, double* gyf) {
     const metrics_t* m = &c->mt;
     scalar_gradient(c, bc, f, gxf, gyf);
+    OMP_FOR
     for (int j = 0; j < c->ny; ++j)
         for (int i = 0; i < c->nx; ++i) {
             CC(gx, i, j) = m->wL_f2cx[i] * UX(gxf, i, j) + m->wR_f2cx[i] * UX(gxf, i + 1, j);
@@ -850,10 +903,12 @@ static int phase_rhs_acdi(fsio_ctx* c, const double* U, const double* V, const d
     double *pgx = dalloc(nc), *pgy = dalloc(nc), *pgxf = dalloc(nfx), *pgyf = dalloc(nfy);
     double *nxc = dalloc(nc), *nyc = dalloc(nc), *nxf = dalloc(nfx), *nyf = dalloc(nfy);
     double *psxf = dalloc(nfx), *psyf = dalloc(nfy), *Ax = dalloc(nfx), *Ay = dalloc(nfy);
+    OMP_FOR
     for (size_t k = 0; k < nc; ++k) psi[k] = psi_value(c, phi[k]);
     center_gradient(c, phi, &c->bcs.phi, gx, gy, gxf, gyf);
     center_gradient(c, psi, &bpsi, pgx, pgy, pgxf, pgyf);
     double d2 = sq(c->m.acdi_delta_normal);
+    OMP_FOR
     for (size_t k = 0; k < nc; ++k) { /* _center_normals :233-244 */
         double mag = sqrt(pgx[k] * pgx[k] + pgy[k] * pgy[k] + d2);
         nxc[k] = pgx[k] / mag;
@@ -876,17 +931,20 @@ static int phase_rhs_acdi(fsio_ctx* c, const double* U, const double* V, const d
         gam = gam * umax;
     }
     double eps = c->m.acdi_epsilon;
+    OMP_FOR
     for (size_t k = 0; k < nfx; ++k) {
         double t = tanh(psxf[k] / (2.0 * eps));
         double amp = 0.25 * (1.0 - t * t);
         Ax[k] = gam * (eps * gxf[k] - amp * nxf[k]);
     }
+    OMP_FOR
     for (size_t k = 0; k < nfy; ++k) {
         double t = tanh(psyf[k] / (2.0 * eps));
         double amp = 0.25 * (1.0 - t * t);
         Ay[k] = gam * (eps * gyf[k] - amp * nyf[k]);
     }
     enforce_face_flux_walls(c, Ax, Ay);
+    OMP_FOR
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             double dF = (UX(fx, i + 1, j) - UX(fx, i, j)) / m->dx_c[i] +
@@ -1005,16 +1063,13 @@ int fsio_phase_rhs(fsio_ctx* c, const double* U, const double* V, const double* 
  * Structure tensor (src/StructureTensor.jl, src/StructuralConstitutive.jl)
  * ------------------------------------------------------------------------- */
 
-/* _sigma_log_product_2d StructureTensor.jl:482-515 */
-static int sigma_log_product(fsio_ctx* c, double a, double d, double b, double* ox, double* oy,
-                             double* oxy) {
+/* _sigma_log_product_2d StructureTensor.jl:482-515; 1 where the tensor is
+ * not SPD (the DomainError of :489-493), else 0 */
+static int sigma_log_product_eval(double a, double d, double b, double* ox, double* oy, double* oxy) {
     double mm = 0.5 * (a + d), delta = 0.5 * (a - d);
     double r = hypot(delta, b);
     double lhi = mm + r, llo = mm - r;
-    if (!(isfinite(llo) && isfinite(lhi) && llo > 0.0))
-        return set_err(c, FSI_E_DOMAIN,
-                       "GeometricStructureExtension requires an SPD tensor; encountered a "
-                       "non-positive or non-finite eigenvalue.");
+    if (!(isfinite(llo) && isfinite(lhi) && llo > 0.0)) return 1;
     double ghi = lhi * log(lhi), glo = llo * log(llo);
     double scale = fabs(mm) > 1.0 ? fabs(mm) : 1.0, alpha, beta;
     if (r <= 1.0e-12 * scale) {
@@ -1030,21 +1085,41 @@ static int sigma_log_product(fsio_ctx* c, double a, double d, double b, double* 
     *oxy = alpha * b;
     return 0;
 }
+static int sigma_log_product(fsio_ctx* c, double a, double d, double b, double* ox, double* oy,
+                             double* oxy) {
+    if (sigma_log_product_eval(a, d, b, ox, oy, oxy))
+        return set_err(c, FSI_E_DOMAIN,
+                       "GeometricStructureExtension requires an SPD tensor; encountered a "
+                       "non-positive or non-finite eigenvalue.");
+    return 0;
+}
 
 /* structure_extension_rhs(::GeometricStructureExtension) :522-551 */
 static int extension_rhs(fsio_ctx* c, const double* sxx, const double* syy, const double* sxy,
                          const double* phi, size_t n, double* ex, double* ey, double* exy) {
-    for (size_t k = 0; k < n; ++k) {
+    /* the first failing cell in index order reports the DomainError */
+    long long bad = (long long)n;
+    OMP_FOR
+    for (long long k = 0; k < (long long)n; ++k) {
         ex[k] = ey[k] = exy[k] = 0.0;
         if (c->num.extension_kind != FSI_EXT_GEOMETRIC) continue;
         double kappa = c->num.ext_max_rate * pow(1.0 - clamp01(phi[k]), c->num.ext_exponent);
         if (kappa == 0.0) continue;
         double gx = 0.0, gy = 0.0, gxy = 0.0;
-        int rc = sigma_log_product(c, sxx[k], syy[k], sxy[k], &gx, &gy, &gxy);
-        if (rc) return rc;
+        if (sigma_log_product_eval(sxx[k], syy[k], sxy[k], &gx, &gy, &gxy)) {
+#ifdef _OPENMP
+#pragma omp critical(fsio_extension)
+#endif
+            if (k < bad) bad = k;
+            continue;
+        }
         ex[k] = -kappa * gx;
         ey[k] = -kappa * gy;
         exy[k] = -kappa * gxy;
+    }
+    if (bad < (long long)n) {
+        double gx, gy, gxy;
+        return sigma_log_product(c, sxx[bad], syy[bad], sxy[bad], &gx, &gy, &gxy);
     }
     return 0;
 }
@@ -1070,6 +1145,7 @@ static int structure_rhs(fsio_ctx* c, const double* sxx, const double* syy, cons
     if (!rc) {
         divergence(c, U, V, dv);
         double rate = c->m.relaxation_kind == FSI_RELAX_MAXWELL ? c->m.relaxation_rate : 0.0;
+        OMP_FOR
         for (size_t k = 0; k < nc; ++k) {
             double A11 = ux[k] - 0.0, A12 = uy[k] - 0.0, A21 = vx[k] - 0.0, A22 = vy[k] - 0.0;
             double adxx = axx[k] + sxx[k] * dv[k];
@@ -1126,6 +1202,7 @@ static void viscous_apply(fsio_ctx* c, const double* uv, double* out, int with_b
     double mu = c->m.viscosity;
     double *exx = dalloc(N), *exy = dalloc(N), *eyy = dalloc(N);
     /* E * uv (+ E_bc) :868-913, 966-979 */
+    OMP_FOR
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             int row = i + j * nx;
@@ -1149,7 +1226,9 @@ static void viscous_apply(fsio_ctx* c, const double* uv, double* out, int with_b
             exy[row] = 0.5 * duy + 0.5 * dvx;
         }
     /* stress weights 2*mu (:960-965), then Dstress (:1000-1034) */
+    OMP_FOR
     for (int k = 0; k < N; ++k) { exx[k] *= 2.0 * mu; exy[k] *= 2.0 * mu; eyy[k] *= 2.0 * mu; }
+    OMP_FOR
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             int row = i + j * nx;
@@ -1221,6 +1300,9 @@ static void dirichlet_mask(fsio_ctx* c, unsigned char* mask, double* values) {
 
 static double dot(const double* a, const double* b, size_t n) {
     double s = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : s) schedule(static)
+#endif
     for (size_t k = 0; k < n; ++k) s += a[k] * b[k];
     return s;
 }
@@ -1251,10 +1333,14 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
     /* rhs + alpha * boundary */
     double* zero = dalloc(N2);
     viscous_apply(c, zero, tmp, 1); /* boundary vector b = M*0 + b */
+    OMP_FOR
     for (int k = 0; k < N2; ++k) b[k] = rhs_in[k] + alpha * tmp[k];
     /* |b| and |b - A x0| of the system with identity Dirichlet rows */
     double bn2 = 0.0, ref2 = 0.0;
     viscous_apply(c, x0, tmp, 0);
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : bn2, ref2) schedule(static)
+#endif
     for (int k = 0; k < N2; ++k) {
         double bk = mask[k] ? g[k] : b[k];
         double ek = mask[k] ? g[k] - x0[k] : b[k] - (x0[k] - alpha * tmp[k]);
@@ -1270,8 +1356,10 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
     const int direct = c->num.velocity_method == FSI_SOLVER_DIRECT;
     double tol = direct ? dtol : c->num.velocity_rtol * sqrt(ref2);
     int limit = direct ? cap : maxiter, phase = direct;
+    OMP_FOR
     for (int k = 0; k < N2; ++k) x[k] = mask[k] ? g[k] : x0[k];
     viscous_apply(c, x, tmp, 0);
+    OMP_FOR
     for (int k = 0; k < N2; ++k) r[k] = mask[k] ? 0.0 : b[k] - (x[k] - alpha * tmp[k]);
     memcpy(p, r, sizeof(double) * N2);
     double rr = dot(r, r, N2);
@@ -1294,13 +1382,16 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
             break;
         }
         viscous_apply(c, p, tmp, 0);
+        OMP_FOR
         for (int k = 0; k < N2; ++k) q[k] = mask[k] ? 0.0 : p[k] - alpha * tmp[k];
         double pq = dot(p, q, N2);
         double a = rr / pq;
+        OMP_FOR
         for (int k = 0; k < N2; ++k) { x[k] += a * p[k]; r[k] -= a * q[k]; }
         double rr2 = dot(r, r, N2);
         double beta = rr2 / rr;
         rr = rr2;
+        OMP_FOR
         for (int k = 0; k < N2; ++k) p[k] = r[k] + beta * p[k];
         ++it;
     }
@@ -1358,6 +1449,22 @@ static double neg_lap_point(fsio_ctx* c, const mg_level* L, const double* x, int
 /* symmetric Gauss-Seidel sweep (forward then backward) on -A x = b */
 static void sgs(fsio_ctx* c, mg_level* L, int forward) {
     int nx = L->nx, ny = L->ny;
+#ifdef _OPENMP
+    /* red-black order (forward: red then black; backward: black then red),
+       where the colouring is consistent across periodic sides */
+    if (!(c->px && (nx & 1)) && !(c->py && (ny & 1))) {
+        for (int h = 0; h < 2; ++h) {
+            const int colour = forward ? h : 1 - h;
+#pragma omp parallel for schedule(static) if (nx * ny >= 16384)
+            for (int j = 0; j < ny; ++j)
+                for (int i = (j + colour) & 1; i < nx; i += 2) {
+                    double d, ax = neg_lap_point(c, L, L->x, i, j, &d);
+                    L->x[i + j * nx] += (L->b[i + j * nx] - ax) / d;
+                }
+        }
+        return;
+    }
+#endif
     if (forward) {
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
@@ -1402,10 +1509,50 @@ static void mg_vcycle(fsio_ctx* c, int l) {
     memset(L->x, 0, sizeof(double) * n);
     sgs(c, L, 1);
     sgs(c, L, 1);
+    OMP_FOR_LEVEL
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) L->r[i + j * nx] = L->b[i + j * nx] - neg_lap_point(c, L, L->x, i, j, NULL);
     mg_level* C = &c->lev[l + 1];
     const fsi_bc_setup* b = &c->bcs.correction;
+#ifdef _OPENMP
+    if (nx * ny >= 16384) {
+        /* R = P^T / 4 gathered per coarse cell from the fine cells whose
+           prolongation stencil holds it (fine 2I - 1 .. 2I + 2, wrapped
+           across periodic sides) */
+        const int cnx = C->nx, cny = C->ny;
+#pragma omp parallel for schedule(static)
+        for (int J = 0; J < cny; ++J)
+            for (int I = 0; I < cnx; ++I) {
+                double acc = 0.0;
+                for (int dy = -1; dy <= 2; ++dy) {
+                    int fy = 2 * J + dy;
+                    if (fy < 0 || fy >= ny) {
+                        if (!c->py) continue;
+                        fy = (fy + ny) % ny;
+                    }
+                    int Jy[2], ny_; double wy[2];
+                    transfer_1d(ny, c->py, b->bottom.kind, b->top.kind, fy, Jy, wy, &ny_);
+                    double wj = 0.0;
+                    for (int a = 0; a < ny_; ++a) if (Jy[a] == J) wj += wy[a];
+                    if (wj == 0.0) continue;
+                    for (int dx = -1; dx <= 2; ++dx) {
+                        int fx = 2 * I + dx;
+                        if (fx < 0 || fx >= nx) {
+                            if (!c->px) continue;
+                            fx = (fx + nx) % nx;
+                        }
+                        int Ix[2], nx_; double wx[2];
+                        transfer_1d(nx, c->px, b->left.kind, b->right.kind, fx, Ix, wx, &nx_);
+                        double wi = 0.0;
+                        for (int e = 0; e < nx_; ++e) if (Ix[e] == I) wi += wx[e];
+                        if (wi != 0.0) acc += 0.25 * wi * wj * L->r[fx + fy * nx];
+                    }
+                }
+                C->b[I + J * cnx] = acc;
+            }
+    } else
+#endif
+    {
     memset(C->b, 0, sizeof(double) * (size_t)C->nx * C->ny);
     for (int j = 0; j < ny; ++j) {
         int Jy[2], ny_; double wy[2];
@@ -1418,7 +1565,9 @@ static void mg_vcycle(fsio_ctx* c, int l) {
                 for (int e = 0; e < nx_; ++e) C->b[Ix[e] + Jy[a] * C->nx] += 0.25 * wx[e] * wy[a] * rv;
         }
     }
+    }
     mg_vcycle(c, l + 1);
+    OMP_FOR_LEVEL
     for (int j = 0; j < ny; ++j) {
         int Jy[2], ny_; double wy[2];
         transfer_1d(ny, c->py, b->bottom.kind, b->top.kind, j, Jy, wy, &ny_);
@@ -1476,12 +1625,15 @@ static int projection_solve(fsio_ctx* c, const double* rhs, double* q, int* iter
         if (it == 0) memcpy(p, z, sizeof(double) * N);
         else {
             double beta = rz2 / rz;
+            OMP_FOR
             for (int k = 0; k < N; ++k) p[k] = z[k] + beta * p[k];
         }
         rz = rz2;
         laplacian_apply(c, p, Ap, 0);
+        OMP_FOR
         for (int k = 0; k < N; ++k) Ap[k] = -Ap[k];
         double a = rz / dot(p, Ap, N);
+        OMP_FOR
         for (int k = 0; k < N; ++k) { q[k] += a * p[k]; r[k] -= a * Ap[k]; }
         rn = sqrt(dot(r, r, N));
         ++it;
@@ -1502,10 +1654,14 @@ static int project(fsio_ctx* c, double* U, double* V, const double* G, double* c
         if (!isfinite(V[k])) return set_err(c, FSI_E_ARGUMENT, "V contains non-finite values.");
     double *pdiv = dalloc(N), *rhs = dalloc(N), *Aq = dalloc(N);
     divergence(c, U, V, pdiv);
+    OMP_FOR
     for (int k = 0; k < N; ++k) rhs[k] = (pdiv[k] - G[k]) - c->lap_bc[k]; /* :201-204 */
     double defect = 0.0;
     if (c->has_nullspace) { /* :206-221 */
         double scale = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : defect, scale) schedule(static)
+#endif
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 double area = m->dx_c[i] * m->dy_c[j];
@@ -1530,22 +1686,34 @@ static int project(fsio_ctx* c, double* U, double* V, const double* G, double* c
     projection_solve(c, rhs, correction, &iters);
     if (c->has_nullspace) { /* :255-259 */
         double tot = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : tot) schedule(static)
+#endif
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) tot += CC(correction, i, j) * (m->dx_c[i] * m->dy_c[j]);
         double mean = tot / (c->lx * c->ly);
+        OMP_FOR
         for (int k = 0; k < N; ++k) correction[k] -= mean;
     }
     laplacian_apply(c, correction, Aq, 0); /* :261-264 */
     double rn = 0.0, bn = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : rn, bn) schedule(static)
+#endif
     for (int k = 0; k < N; ++k) { rn += sq(Aq[k] - rhs[k]); bn += rhs[k] * rhs[k]; }
     double lin = sqrt(rn) / (sqrt(bn) > 2.220446049250313e-16 ? sqrt(bn) : 2.220446049250313e-16);
     double* gx = dalloc((size_t)(nx + 1) * ny);
     double* gy = dalloc((size_t)nx * (ny + 1));
     scalar_gradient(c, &c->bcs.correction, correction, gx, gy); /* :266-273 */
+    OMP_FOR
     for (size_t k = 0; k < (size_t)(nx + 1) * ny; ++k) U[k] -= gx[k];
+    OMP_FOR
     for (size_t k = 0; k < (size_t)nx * (ny + 1); ++k) V[k] -= gy[k];
     divergence(c, U, V, Aq);
     double linf = 0.0, l2 = 0.0, pl2 = 0.0, tl2 = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(max : linf) reduction(+ : l2, pl2, tl2) schedule(static)
+#endif
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) {
             double area = m->dx_c[i] * m->dy_c[j];
@@ -1606,6 +1774,11 @@ static int validate_setup(fsio_ctx* c, const fsi_bc_setup* b, const char* name) 
 
 int fsio_create(const fsi_grid* g, const fsi_bcs* bcs, const fsi_models* models,
                 const fsi_numerics* num, fsio_ctx** out) {
+    /* the work arrays of every call are allocated and freed again: keep the
+       freed blocks in the heap instead of returning them to the system, so
+       the next call does not fault its pages in again */
+    mallopt(M_MMAP_THRESHOLD, 1 << 30);
+    mallopt(M_TRIM_THRESHOLD, 1 << 30);
     fsio_ctx* c = (fsio_ctx*)calloc(1, sizeof(fsio_ctx));
     *out = c;
     c->nx = g->nx; c->ny = g->ny; c->N = g->nx * g->ny;
@@ -1809,6 +1982,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
             for (int j = 0; j < s; ++j) {
                 double a = AE[s][j];
                 if (a == 0.0) continue;
+                OMP_FOR
                 for (int k = 0; k < N; ++k) {
                     phis[k] += dt * a * Ep[j][k];
                     sxxs[k] += dt * a * Exx[j][k];
@@ -1844,6 +2018,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
             for (int q = 0; q < s; ++q) {
                 double ae = AE[s][q], ai = AI[s][q];
                 if (ae == 0.0 && ai == 0.0) continue;
+                OMP_FOR
                 for (int j = 0; j < ny; ++j)
                     for (int i = 0; i < nx; ++i) {
                         int k = i + j * nx;

@@ -20,8 +20,12 @@ HERE = Path(__file__).resolve().parent
 # FSI_ORACLE_LIB selects another build of the same sources (a planted-mistake
 # copy, say) for the CPU tests
 LIB_PATH = Path(os.environ["FSI_ORACLE_LIB"]) if os.environ.get("FSI_ORACLE_LIB") else HERE / "liboracle_fsi.so"
+# the same sources with the cell loops run by OpenMP threads: the all-cores
+# CPU baseline of bench.py (agrees with the serial build to the solver
+# tolerances), never the parity checker
+OMP_LIB_PATH = HERE / "liboracle_fsi_omp.so"
 _dp = POINTER(c_double)
-_LIB = None
+_LIBS = {}
 
 
 def build_oracle(verbose=False) -> Path:
@@ -31,13 +35,13 @@ def build_oracle(verbose=False) -> Path:
     return LIB_PATH
 
 
-def load():
-    global _LIB
-    if _LIB is not None:
-        return _LIB
-    if not LIB_PATH.exists():
+def load(path: Path | None = None):
+    path = LIB_PATH if path is None else Path(path)
+    if path in _LIBS:
+        return _LIBS[path]
+    if not path.exists():
         build_oracle()
-    lib = ctypes.CDLL(str(LIB_PATH))
+    lib = ctypes.CDLL(str(path))
     P = c_void_p
     sig = {
         "fsio_create": (c_int, [POINTER(_abi.Grid), POINTER(_abi.Bcs), POINTER(_abi.Models), POINTER(_abi.Numerics),
@@ -66,7 +70,7 @@ def load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _LIB = lib
+    _LIBS[path] = lib
     return lib
 
 
@@ -87,8 +91,8 @@ class OracleError(RuntimeError):
 class Oracle:
     """One oracle context for a (metrics, bcs, models, numerics) configuration."""
 
-    def __init__(self, metrics, bcs, models, numerics):
-        self.lib = load()
+    def __init__(self, metrics, bcs, models, numerics, lib_path=None):
+        self.lib = load(lib_path)
         self.metrics = metrics
         self.cfg = _abi.abi_config(metrics, bcs, models, numerics)
         g, b, m, n = self.cfg
