@@ -37,20 +37,25 @@ METRIC = "IMEX steps/s at N=1024 (FP64 vs mixed @1e-8), HBM roofline %, adjoint 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
 
 # Algorithmic HBM passes per step, in fields of N^2 doubles.
-def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3):
+def step_bytes(N, vel_iters, proj_iters, direct=True, pcg_passes=17.3, gmres=True):
     """Passes this implementation makes per step: tendencies 107 (momentum 6
     reads per stage plus the running sums of the later stages' velocity
     right-hand sides, 8 + 8 + 4; phase 3 x 4; sigma 7 reads per stage plus the
     running sums of the later stages' phi and sigma, 12 + 16 + 8; the stage
-    combinations happen inside these); per velocity solve 18 (start: right-hand
-    side and four history fields read, x r written; q0: r read, q written)
-    + 16 per CG iteration; per projection 11 (rhs 3, the
-    tested solution 2, update and check in one pass 6) + 6 for the direct transform solve
-    (three kernels each reading and writing the field once), or 17 + 17.3 per
-    PCG iteration when the multigrid-PCG solves it."""
+    combinations happen inside these), + 2 under the GMRES stopping rule (the
+    stage-1 momentum kernel stores M u^n); per velocity solve 18 (start:
+    right-hand side and four history fields read, x r written; q0: r read, q
+    written), + 4 under the GMRES rule (u^n and M u^n read for |b - A u^n|),
+    + 16 per CG iteration; per projection with the direct transform solve 16
+    (first transform pass: U V b_bc read, rhs and the row transform written;
+    column and inverse row passes 2 each; the tested update and checks:
+    q U V rhs read, U V q_out written), or 17 + 17.3 per PCG iteration when
+    the multigrid-PCG solves it."""
     cell = 8.0 * N * N
-    per_proj = 11 + 6 if direct else 17
-    passes = 107 + sum(18 + 16 * k for k in vel_iters) + sum(per_proj + pcg_passes * k for k in proj_iters)
+    per_proj = 16 if direct else 17
+    per_solve = 18 + (4 if gmres else 0)
+    passes = 107 + (2 if gmres else 0) + sum(per_solve + 16 * k for k in vel_iters) + \
+        sum(per_proj + pcg_passes * k for k in proj_iters)
     return passes * cell
 
 
