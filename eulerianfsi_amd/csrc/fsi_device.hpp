@@ -95,7 +95,8 @@ enum DevError : int {
 struct KrylovScalars {
     double rr, bnorm2, tol2;
     double rzs[2];  // r.z of the last two PCG iterations (by parity)
-    int done, iter, maxiter, pad;
+    int done, iter, maxiter;
+    int sw;  // velocity: 1 once GMRES stopped above fallback_reltol and the solve went on to the direct target
     // velocity predictor only: the stopping rules of _solve_linear_system!
     // (src/TimeIntegrationIMEX.jl:602-641)
     double fb2;    // accepted at the iteration limit when r.r <= fb2 (fallback_reltol)

@@ -318,6 +318,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double al
             // the direct target, never looser than the acceptance threshold
             sk.dtol2 = fmin(VEL_DIRECT_RTOL * VEL_DIRECT_RTOL * bscale2, fb2);
             sk.iter = 0;
+            sk.sw = 0;
             st->vel_iters[slot] = 0;
             if (F32) {  // the inner CG reduces r.r to a quarter of the target, or by 1e-12 at most
                 const double target = vt.method == 0 ? sk.dtol2 : sk.tol2;
@@ -425,23 +426,24 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
     // The test of iterate m, a function of the totals, m and the rule's
     // constants alone, so every block decides the same.  Returns true when
     // the solve ends here.  :direct: the direct target within `cap`
-    // iterations.  :gmres: reltol * |b - A u^n| within `maxiter`; at the
-    // limit the iterate is accepted below fallback_reltol (:629-631), else
-    // with direct_fallback the solve goes on to the direct target within
-    // `cap` iterations in all (the sparse direct solve of :632-634), else the
-    // solver error (:636).
+    // iterations.  :gmres: reltol * |b - A u^n| within `maxiter`; where
+    // GMRES stops (the rule met or the limit reached) the iterate is kept
+    // below fallback_reltol (:629-631), else with direct_fallback the solve
+    // goes on to the direct target within `cap` iterations in all (the
+    // sparse direct solve of :632-634; `sw` marks it for the later
+    // launches), else the solver error (:636).
     auto test = [&](double rr, double tol2, double fb2, double dtol2) -> bool {
         bool end = false, ok = false;
         if (!(rr <= 1.0e300)) {  // NaN or Inf: no rule can be met
             end = true;
-        } else if (vt.method == 0 || m > vt.maxiter) {
+        } else if (vt.method == 0 || m > vt.maxiter || st->vel.sw) {
             ok = rr <= dtol2;
             end = ok || m >= vt.cap;
-        } else if (rr <= tol2) {
-            end = ok = true;
-        } else if (m == vt.maxiter) {
+        } else if (rr <= tol2 || m == vt.maxiter) {
             ok = rr <= fb2;
             end = ok || !vt.fallback;
+            // every block takes the same decision whether or not it sees sw
+            if (!end && block_id() == 0 && lane_id() == 0) st->vel.sw = 1;
         }
         if (end && block_id() == 0 && lane_id() == 0) {
             st->vel.done = 1;
@@ -462,7 +464,8 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
         const KrylovScalars& sk = st->vel;
         const int total = sk.ibase + m;
         const bool bad = !(rr <= 1.0e300);
-        const bool limit = vt.method == 1 && total <= vt.maxiter ? total >= vt.maxiter : total >= vt.cap;
+        const bool limit =
+            vt.method == 1 && !sk.sw && total <= vt.maxiter ? total >= vt.maxiter : total >= vt.cap;
         const bool end = bad || rr <= sk.itol2 || m >= vt.icap || limit;
         if (end && block_id() == 0 && lane_id() == 0) {
             st->vel.idone = 1;
@@ -633,17 +636,16 @@ __global__ void __launch_bounds__(BLOCK) k_vel_check(VelBufs b, Status* st, int 
     sk.cycles += 1;
     sk.imax = max(sk.imax, total - sk.ibase);
     bool end = false, ok = false;
-    const bool fallback_phase = vt.method == 0 || total > vt.maxiter;
+    const bool fallback_phase = vt.method == 0 || total > vt.maxiter || sk.sw;
     if (!(rr <= 1.0e300)) {
         end = true;
     } else if (fallback_phase) {
         ok = rr <= sk.dtol2;
         end = ok || total >= vt.cap;
-    } else if (rr <= sk.tol2) {
-        end = ok = true;
-    } else if (total >= vt.maxiter) {
+    } else if (rr <= sk.tol2 || total >= vt.maxiter) {
         ok = rr <= sk.fb2;
         end = ok || !vt.fallback;
+        if (!end) sk.sw = 1;
     }
     st->vel_cycles[slot] = sk.cycles;
     st->vel_imax[slot] = sk.imax;
@@ -657,7 +659,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_check(VelBufs b, Status* st, int 
         if (ok) st->vel_ok[slot] = 1;
         else raise_error(st, DEV_VEL_NOCONV, sqrt(rr / bs2), (double)(slot + 2));
     } else {
-        const double target = (fallback_phase || total >= vt.maxiter) ? sk.dtol2 : sk.tol2;
+        const double target = (fallback_phase || sk.sw) ? sk.dtol2 : sk.tol2;
         sk.itol2 = fmax(0.25 * target, 1e-12 * rr);
         sk.ibase = total;
         sk.idone = 0;

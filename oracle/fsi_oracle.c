@@ -1315,7 +1315,8 @@ static double dot(const double* a, const double* b, size_t n) {
  * Stopping rules of _solve_linear_system! :602-641 (x0 = uv_n, :807):
  *   :direct  the exact solve A \ b, here CG to |r| <= 1e-14 max(|b|, eps);
  *   :gmres   IterativeSolvers' reltol rule, |r| <= reltol |b - A x0| (its
- *            initial residual), within maxiter iterations; then accepted
+ *            initial residual), within maxiter iterations; where it stops
+ *            (the rule met or the limit reached) the iterate is accepted
  *            when |r| / max(|b|, eps) <= fallback_reltol (:629-631), else the
  *            direct solve (direct_fallback, :632-634: the direct target within
  *            maxiter + max(10 maxiter, 1000) iterations in all) or the
@@ -1369,10 +1370,16 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
             rc = set_err(c, FSI_E_SOLVER, "velocity predictor residual is not finite");
             break;
         }
-        if (sqrt(rr) <= tol) break;
-        if (it >= limit) {
-            if (!phase && sqrt(rr) <= fb) break;           /* accepted (:629-631) */
-            if (!phase && c->num.direct_fallback) {       /* the direct solve (:632-634) */
+        if (sqrt(rr) <= tol || it >= limit) {
+            if (phase) {
+                if (sqrt(rr) <= tol) break;
+                rc = set_err(c, FSI_E_SOLVER, "velocity GMRES residual is too large: %.6e", sqrt(rr) / bscale);
+                break;
+            }
+            /* GMRES has stopped (the rule met or the limit reached): the check
+               of :629-636 on |b - A x| / max(|b|, eps) */
+            if (sqrt(rr) <= fb) break;                     /* accepted (:629-631) */
+            if (c->num.direct_fallback) {                 /* the direct solve (:632-634) */
                 phase = 1;
                 tol = dtol;
                 limit = cap;

@@ -167,6 +167,39 @@ def test_velocity_direct_fallback_matches_oracle():
             assert e <= 1e-10, (k, f, e)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_gmres_stop_above_fallback_reltol_takes_the_direct_solve(precision):
+    """reltol 1e-2 stops GMRES far above fallback_reltol 1e-9: the fallback
+    check that follows every stop, the met rule included (:629-636), sends
+    each solve on to the direct target, on the device as in the oracle; the
+    steps match to 1e-10.  Without direct_fallback the stop is an
+    ErrorException."""
+    from oracle.oracle import Oracle, state_dict
+    case = setups.seeded_cavity(64, "acdi", seed=4, projection_rtol=1e-13)
+    kw = dict(velocity_method="gmres", reltol=1e-2, maxiter=200, fallback_reltol=1e-9)
+    num = dataclasses.replace(_gmres_numerics(case, direct_fallback=True, **kw), precision=precision)
+    o = Oracle(case.metrics, case.bcs, case.models, num)
+    ref = state_dict(case.state)
+    traj = []
+    for k in range(3):
+        o.integrate_step(ref, k * case.dt, case.dt)
+        traj.append({f: ref[f].copy() for f in FIELDS})
+    ctx, work = _device(case, num)
+    for k in range(3):
+        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+        st = work.last_stats
+        print(precision, k, st, flush=True)
+        assert all(r <= 1e-13 for r in st.velocity_residual)
+        for f in FIELDS:
+            e = rel_l2(getattr(case.state, f), traj[k][f])
+            assert e <= 1e-10, (k, f, e)
+    case2 = setups.seeded_cavity(64, "acdi", seed=4)
+    num2 = dataclasses.replace(_gmres_numerics(case2, direct_fallback=False, **kw), precision=precision)
+    ctx2, work2 = _device(case2, num2)
+    with pytest.raises(fsi.SolverError, match="GMRES residual is too large"):
+        fsi.integrate_step(case2.state, 0.0, case2.dt, ctx2, work2)
+
+
 def test_velocity_without_fallback_is_a_solver_error():
     """direct_fallback = false: the residual after maxiter iterations above
     fallback_reltol is an ErrorException (:636); the state is unchanged."""

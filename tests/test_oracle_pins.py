@@ -616,3 +616,28 @@ def test_material_convection_converges_to_the_exact_term(name):
     print(name, errs, rates)
     assert errs[-1] < 2e-2
     assert np.all(rates > 1.8), rates
+
+
+def test_gmres_stop_above_fallback_reltol_takes_the_direct_solve():
+    """The fallback check follows every GMRES stop, the met rule included
+    (:629-636): a loose reltol that stops above fallback_reltol gives the
+    direct solution with direct_fallback, otherwise ErrorException."""
+    from oracle.oracle import OracleError
+    alpha = 0.7
+    kw = dict(velocity_method="gmres", reltol=1e-2, maxiter=800, fallback_reltol=1e-9)
+    o, A, system = _assembled_predictor("cavity", alpha, direct_fallback=True, **kw)
+    rhs = np.random.default_rng(9).standard_normal(A.shape[0])
+    bb = system(rhs)
+    x_ref = np.linalg.solve(A, bb)
+    x, it = o.velocity_solve(alpha, rhs)
+    assert np.linalg.norm(bb - A @ x) <= 1e-13 * np.linalg.norm(bb)
+    assert np.allclose(x, x_ref, rtol=0, atol=1e-12 * np.max(np.abs(x_ref)))
+    # the same solve stopped by the rule alone (a fallback_reltol above it)
+    o1, _, _ = _assembled_predictor("cavity", alpha, direct_fallback=True,
+                                    **dict(kw, fallback_reltol=0.5))
+    x1, it1 = o1.velocity_solve(alpha, rhs)
+    assert it1 < it
+    assert np.linalg.norm(bb - A @ x1) > 1e-9 * np.linalg.norm(bb)
+    o2, _, _ = _assembled_predictor("cavity", alpha, direct_fallback=False, **kw)
+    with pytest.raises(OracleError, match="too large"):
+        o2.velocity_solve(alpha, rhs)
