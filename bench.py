@@ -230,7 +230,7 @@ def main(argv=None):
         print(json.dumps({"kernel": args.only_kernel, "avg_ms": ms, "bytes": nb, "GBps": nb / ms / 1e6}))
         return 0
 
-    vel_tot, proj_tot, vres, vref = [], [], [], []
+    vel_tot, proj_tot, vres, vref, planned = [], [], [], [], []
     t0 = time.perf_counter()
     for k in range(args.steps):
         rep = fsi.integrate_step(state, t, dt, ctx, work)
@@ -240,6 +240,7 @@ def main(argv=None):
         proj_tot.append(st.projection_iterations)
         vres.append(st.velocity_residual)
         vref.append(st.velocity_reference_residual)
+        planned.append(st.planned)
     elapsed = time.perf_counter() - t0
     # the predictor solves against the reference's rule: final |r| / |b - A u^n| per stage
     achieved = (np.array(vres) / np.maximum(np.array(vref), 1e-300)).max(axis=0)
@@ -382,6 +383,7 @@ def main(argv=None):
                    "velocity_residual_over_reference": achieved.tolist(),
                    "N": args.N, "method": args.method, "cell_updates_per_s": steps_per_s * args.N ** 2,
                    "velocity_iterations": vel_mean.tolist(), "projection_iterations": proj_mean.tolist(),
+                   "planned_steps": f"{sum(planned)}/{len(planned)}",
                    "precision": args.precision, "mixed_precision": mixed, "ensemble": ensemble,
                    "adjoint_grads_per_s": None},
         "roofline": roofline, "whole_step": whole_step, "kernels": kernels, "cpu_baseline": cpu,

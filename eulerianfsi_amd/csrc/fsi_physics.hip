@@ -863,9 +863,12 @@ constexpr int AF_NX = TILE_ROWS * AF_XW, AF_NY = AF_YH * TILE_X;
 template <bool IN>
 __device__ __forceinline__ void acdi_tile(const Params& p, const Stage& s, const BcSet& bpsi,
                                           double* __restrict__ Ephi, Status* st, int i0, int j0, double* s_phi,
-                                          double* s_psi, double* s_sr, double* s_n0, double* s_n1, double* s_fy) {
-    static_assert(AF_NX <= CR2 * CW2, "x-face fluxes fit the psi tile");
-    double* s_fx = s_psi;  // psi; after the normals, the x-face fluxes
+                                          double* s_psi, double* s_sr, double* s_n0, double* s_n1) {
+    static_assert(AF_NX <= CR2 * CW2 && AF_NY <= CR2 * CW2, "face fluxes fit the psi and sqrt(r) tiles");
+    // psi and sqrt(r); once the face fluxes are formed, the x- and y-face fluxes
+    // (50 KB of LDS in all: three blocks per CU)
+    double* s_fx = s_psi;
+    double* s_fy = s_sr;
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
     // 1. phi on the halo-2 tile and the tile's face velocities: indices, then
     // every load back to back
@@ -1017,7 +1020,7 @@ __device__ __forceinline__ void acdi_tile(const Params& p, const Stage& s, const
         }
         fyv[it] = F;
     }
-    __syncthreads();  // the psi tile is free now
+    __syncthreads();  // the psi and sqrt(r) tiles are free now
 #pragma unroll
     for (int it = 0; it < NUX; ++it) {
         const int q = tid + it * BLOCK;
@@ -1048,13 +1051,12 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
     __shared__ double s_sr[CR2 * CW2];  // sqrt(r)
     __shared__ double s_n0[CR1 * CW1];
     __shared__ double s_n1[CR1 * CW1];
-    __shared__ double s_fy[AF_NY];
     const int i0 = tile_id().x * TILE_X, j0 = tile_id().y * TILE_ROWS;
     // the halo-2 tile inside the domain: no wrap, every face and normal interior
     const bool inner = i0 >= CH2 && i0 + TILE_X + CH2 - 1 <= p.nx - 1 && j0 >= CH2 &&
                        j0 + TILE_ROWS + CH2 - 1 <= p.ny - 1;
-    if (inner) acdi_tile<true>(p, s, bpsi, Ephi, st, i0, j0, s_phi, s_psi, s_sr, s_n0, s_n1, s_fy);
-    else acdi_tile<false>(p, s, bpsi, Ephi, st, i0, j0, s_phi, s_psi, s_sr, s_n0, s_n1, s_fy);
+    if (inner) acdi_tile<true>(p, s, bpsi, Ephi, st, i0, j0, s_phi, s_psi, s_sr, s_n0, s_n1);
+    else acdi_tile<false>(p, s, bpsi, Ephi, st, i0, j0, s_phi, s_psi, s_sr, s_n0, s_n1);
 }
 
 // Phase-field right-hand side, one instantiation per phase method.
