@@ -1258,7 +1258,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, MgGeom g, Proj
         // the right-hand side totals and the sum of x in one pass (q_part
         // follows rhs_part's four rows)
         double tot[5];
-        get_totals<5>(b.rhs_part, b.rhs_np, tot);
+        get_totals<5, 1>(b.rhs_part, b.rhs_np, tot);  // a few partials per thread: one in flight
         const double trhs[4] = {tot[0], tot[1], tot[2], tot[3]};
         const double mrhs = proj_scalars_of(p, trhs, st, has_null, rtol, maxiter);
         mean = has_null ? tot[4] / lxly : 0.0;
