@@ -3,6 +3,7 @@ CPU oracle on identical seeded inputs.
 
   config 1  cavity 128^2, ACDI, disk r = 0.2 at (0.6, 0.5), dt = 1e-3, 200 steps
   config 2  cavity N = 512, 1024, 2048 x CDI / ACDI / UWCDI, dt = 1e-3 * 128/N
+  config 3  filament in Kolmogorov flow, 1024^2, UWCDI, FP64 and mixed precision
 
 Gates: relative L2 <= 1e-10 per step on U, V, phi, sigma with both sides'
 solves tight (velocity :direct, projection 1e-13/1e-14); phase volume
@@ -134,3 +135,40 @@ def test_config2_large_cavity_properties(N, method):
     print(N, method, "volume drift", abs(vol - vol0) / vol0, "stats", work.last_stats, flush=True)
     assert abs(vol - vol0) <= 1e-13 * vol0
     assert all(i == 0 for i in work.last_stats.projection_iterations)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config3_filament1024_step_matches_oracle(precision):
+    """BASELINE config 3's workload, the wall-supported filament in Kolmogorov
+    flow at 1024^2 (UWCDI, x-periodic: the real-DFT x cosine transform
+    solve), from a seeded perturbed state (divergence-free velocity, SPD
+    tensor field) developed by 20 device steps, then one step on the
+    device and one in the oracle from the same downloaded state, both tight
+    (velocity :direct, projection 1e-12): relative L2 <= 1e-10 on U, phi,
+    the velocity (U, V) and the tensor (sigma_xx, sigma_yy, sigma_xy) as a
+    whole (the small components V and sigma_xy carry the roundoff of the
+    large ones).  In mixed precision the FP32 inner CG refines to the same
+    FP64 target."""
+    from oracle.oracle import Oracle, state_dict
+    from tests.test_gpu_parity import _perturbed
+    case = setups.kolmogorov_filament(1024, "uwcdi", velocity_method="direct", projection_rtol=1e-12,
+                                      precision=precision)
+    case = _perturbed(case, seed=8)  # a divergence-free wall-bounded velocity and an SPD tensor field
+    ctx, work = _device(case)
+    for k in range(20):
+        fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+    t = 20 * case.dt
+    start = state_dict(case.state)
+    o = Oracle(case.metrics, case.bcs, case.models, case.numerics)
+    o.correction[:] = work.correction
+    o.integrate_step(start, t, case.dt)
+    rep = fsi.integrate_step(case.state, t, case.dt, ctx, work)
+    print(precision, work.last_stats, rep, flush=True)
+    dev = {f: getattr(case.state, f) for f in FIELDS}
+    groups = {"U": ("U",), "phi": ("phi",), "velocity": ("U", "V"), "sigma": ("sigma_xx", "sigma_yy", "sigma_xy")}
+    for name, fs in groups.items():
+        a = np.concatenate([dev[f].ravel() for f in fs])
+        b = np.concatenate([start[f].ravel() for f in fs])
+        e = rel_l2(a, b)
+        print(name, f"{e:.2e}", {f: f"{rel_l2(dev[f], start[f]):.2e}" for f in fs}, flush=True)
+        assert e <= 1e-10, (name, e)
