@@ -50,6 +50,12 @@ struct fsi_context {
     int has_null = 0;
     int has_reset = 0;
     hipStream_t stream = nullptr;
+    // The phase and sigma kernels of stage q run on `side` while `stream`
+    // solves stage q + 1 (they share no buffer: the stage-2 velocity has a
+    // buffer of its own); events join the two before each stage's tendencies.
+    hipStream_t side = nullptr;
+    hipEvent_t ev_start = nullptr, ev_proj[3] = {}, ev_cells[3] = {};
+    int dual = 0;
     std::string err;
     int device = -1;  // the HIP device current at creation
     double* force_row = nullptr;
@@ -136,6 +142,7 @@ struct fsi_workspace {
     // history.  Only iteration counts depend on it.
     double *cq_cur[3] = {}, *cq_prev[3] = {}, *cq_next[3] = {};
     double* xs = nullptr;  // the iterate of a standalone projection (fsi_project)
+    double *Us2 = nullptr, *Vs2 = nullptr;  // the stage-2 velocity (stages 1 and 3 use Us, Vs)
     double *mun_u = nullptr, *mun_v = nullptr;  // M u^n of the step (the reference residual of the predictor)
     // the same for the predictor solves: stage s of the previous two steps
     // velocity stage solutions of the last four steps (newest first) and a
@@ -348,6 +355,18 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
         return fail(c, FSI_E_HIP, "no HIP device is available");
     HIPCHECK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        const char* e = std::getenv("FSI_DUAL_STREAM");  // 0: every kernel of a step on one stream
+        c->dual = (e && e[0] == '0') ? 0 : 1;
+    }
+    if (c->dual) {
+        HIPCHECK(c, hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPCHECK(c, hipEventCreateWithFlags(&c->ev_start, hipEventDisableTiming));
+        for (int k = 0; k < 3; ++k) {
+            HIPCHECK(c, hipEventCreateWithFlags(&c->ev_proj[k], hipEventDisableTiming));
+            HIPCHECK(c, hipEventCreateWithFlags(&c->ev_cells[k], hipEventDisableTiming));
+        }
+    }
     HIPCHECK(c, hipGetDevice(&c->device));
     if ((rc = scratch_lookup(c)) != FSI_OK) return rc;
 
@@ -566,6 +585,13 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
+    if (c->side) (void)hipStreamSynchronize(c->side);
+    for (int k = 0; k < 3; ++k) {
+        if (c->ev_proj[k]) (void)hipEventDestroy(c->ev_proj[k]);
+        if (c->ev_cells[k]) (void)hipEventDestroy(c->ev_cells[k]);
+    }
+    if (c->ev_start) (void)hipEventDestroy(c->ev_start);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -859,7 +885,7 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
                           &w->rhs,    &w->corr[0], &w->corr[1], &w->r,     &w->z,      &w->pp[0],  &w->pp[1], &w->w,
                           &w->cq_cur[0], &w->cq_cur[1], &w->cq_cur[2], &w->cq_prev[0], &w->cq_prev[1],
                           &w->cq_prev[2], &w->cq_next[0], &w->cq_next[1], &w->cq_next[2], &w->xs,
-                          &w->mun_u, &w->mun_v,
+                          &w->mun_u, &w->mun_v, &w->Us2, &w->Vs2,
                           &w->vel_cur_u[0], &w->vel_cur_u[1], &w->vel_cur_u[2], &w->vel_cur_v[0], &w->vel_cur_v[1],
                           &w->vel_cur_v[2], &w->vel_prev_u[0], &w->vel_prev_u[1], &w->vel_prev_u[2],
                           &w->vel_prev_v[0], &w->vel_prev_v[1], &w->vel_prev_v[2], &w->vel_old_u[0],
@@ -1194,7 +1220,7 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
             launch_pcg_iters(c->stream, p, b, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan, c->st, 0,
                              n - 1, slot);
         launch_pcg_test(c->stream, p, b, c->st, n - 1, slot);
-        launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
+        launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly, slot);
         return FSI_OK;
     }
     int m = 0, chunk = c->last_proj_iters[ps] + 1, done = 0, iter = 0;
@@ -1209,12 +1235,35 @@ static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, doub
         chunk = 4;
     }
     c->last_proj_iters[ps] = std::max(0, iter);
-    launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly);
+    launch_proj_finish(c->stream, p, c->geo[0], b, c->st, c->has_null, c->grid.lx * c->grid.ly, slot);
     return FSI_OK;
 }
 
+// The failure the reference would raise first: its sequence within a step is
+// the tendencies of stage 1, the solves of stage 2, the tendencies of stage
+// 2, ... (the stage loop of integrate_step!, TimeIntegrationIMEX.jl:1152-1256).  The solve and projection
+// kernels record theirs in `code` (code_at tells the stage), the phase and
+// sigma kernels of each stage in cells_err, so the order holds also where
+// the latter run beside the next stage's solve.
+static ErrSlot first_error(const Status& s) {
+    int sa = 4;  // the stage (1..3) of the solve-side failure, 4: none
+    for (int k = 0; k < 3; ++k)
+        if (s.code_at[k] != DEV_OK) {
+            sa = k + 1;
+            break;
+        }
+    if (s.code != DEV_OK && sa == 4) sa = 0;  // outside a step's stages (a standalone projection)
+    for (int q = 0; q < 3; ++q)
+        if (s.cells_err[q].code != DEV_OK && q < sa) return s.cells_err[q];
+    ErrSlot e{};
+    e.code = s.code;
+    e.v0 = s.v0;
+    e.v1 = s.v1;
+    return e;
+}
+
 static fsi_status device_error(fsi_context* c) {
-    const Status& s = *c->st_host;
+    const ErrSlot s = first_error(*c->st_host);
     switch (s.code) {
     case DEV_OK: return FSI_OK;
     case DEV_NEUMANN_INFLOW:
@@ -1287,19 +1336,21 @@ extern "C" fsi_status fsi_project(fsi_context* c, fsi_state* s, fsi_workspace* w
 // I_u, I_v and E_sigma (E_phi is stored for the sigma kernel).
 static void tendencies(fsi_context* c, fsi_workspace* w, int slot, const double* U, const double* V,
                        const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                       int stage1, const RhsSums* sums = nullptr, const CellSums* csums = nullptr) {
+                       int stage1, const RhsSums* sums = nullptr, const CellSums* csums = nullptr,
+                       hipStream_t cells_stream = nullptr) {
     const Params& p = c->prm;
+    hipStream_t cs = cells_stream ? cells_stream : c->stream;
     if (sums)
         launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, nullptr, nullptr, nullptr, nullptr, sums);
     else
         launch_momentum(c->stream, p, U, V, phi, sxx, syy, sxy, reset, w->Eu[slot], w->Ev[slot], w->Iu[slot],
                         w->Iv[slot]);
     if (csums)
-        launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], nullptr, nullptr, nullptr,
-                     c->st, stage1, csums);
+        launch_cells(cs, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], nullptr, nullptr, nullptr,
+                     &c->st->cells_err[slot], stage1, csums);
     else
         launch_cells(c->stream, p, c->bpsi, U, V, phi, sxx, syy, sxy, reset, w->Ep[slot], w->Exx[slot],
-                     w->Eyy[slot], w->Exy[slot], c->st, stage1);
+                     w->Eyy[slot], w->Exy[slot], &c->st->cells_err[slot], stage1);
 }
 
 extern "C" fsi_status fsi_eval_tendencies(fsi_context* c, const fsi_state* s, fsi_workspace* w, double t,
@@ -1372,10 +1423,25 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         C.q = q;
         return C;
     };
+    // With the side stream the phase and sigma kernels of stage q overlap the
+    // solve of stage q + 1: they need the stage-q velocity (projected before
+    // them) and write only E_phi and the phi, sigma stage sums, which the
+    // momentum kernel of stage q + 1 reads after the join.
+    const bool dual = c->dual && c->side && c->use_graph != 1;
+    hipStream_t A = c->stream, B = dual ? c->side : c->stream;
+    if (dual) {
+        HIPCHECK(c, hipEventRecord(c->ev_start, A));  // after the Status reset
+        HIPCHECK(c, hipStreamWaitEvent(B, c->ev_start, 0));
+    }
+    // the stage velocity: stages 1 and 3 in Us, Vs (swapped into the state),
+    // stage 2 in Us2, Vs2 (the side stream still reads stage 1's)
+    auto stage_u = [&](int si) { return si == 2 ? w->Us2 : w->Us; };
+    auto stage_v = [&](int si) { return si == 2 ? w->Vs2 : w->Vs; };
     // stage 1: the state itself (with the safety reset on sigma_n), tendencies
     RhsSums sums = sums_of(0);
     CellSums csums = cells_of(0);
-    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1, &sums, &csums);
+    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1, &sums, &csums, B);
+    if (dual) HIPCHECK(c, hipEventRecord(c->ev_cells[0], B));
     for (int si = 1; si < 4; ++si) {
         // the predictor from the summed right-hand side
         const double alpha = dt * AI[si][si];
@@ -1383,19 +1449,29 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         // with the warm-start history the predictor is kept (the next step
         // extrapolates from it) and the projection writes the stage velocity
         const bool keep = p.visc;
-        double* xu = keep ? w->vel_next_u[k] : w->Us;
-        double* xv = keep ? w->vel_next_v[k] : w->Vs;
+        double* xu = keep ? w->vel_next_u[k] : stage_u(si);
+        double* xv = keep ? w->vel_next_v[k] : stage_v(si);
         if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0, w->rhs_u[k], w->rhs_v[k], xu,
                                  xv)) != FSI_OK)
             return rc;
-        if ((rc = projection(c, w, xu, xv, w->Us, w->Vs, 1, k, planned ? c->plan_proj[k] : 0)) != FSI_OK) return rc;
+        if ((rc = projection(c, w, xu, xv, stage_u(si), stage_v(si), 1, k, planned ? c->plan_proj[k] : 0)) !=
+            FSI_OK)
+            return rc;
         if (si <= 2) {
+            if (dual) {
+                HIPCHECK(c, hipEventRecord(c->ev_proj[si], A));
+                HIPCHECK(c, hipStreamWaitEvent(B, c->ev_proj[si], 0));     // the stage velocity
+                HIPCHECK(c, hipStreamWaitEvent(A, c->ev_cells[si - 1], 0));  // phi, sigma of the stage
+            }
             sums = sums_of(si);
             csums = cells_of(si);
             const int t = si - 1;
-            tendencies(c, w, si, w->Us, w->Vs, sphi[t], ssxx[t], ssyy[t], ssxy[t], 0, 0, &sums, &csums);
+            tendencies(c, w, si, stage_u(si), stage_v(si), sphi[t], ssxx[t], ssyy[t], ssxy[t], 0, 0, &sums, &csums,
+                       B);
+            if (dual) HIPCHECK(c, hipEventRecord(c->ev_cells[si], B));
         }
     }
+    if (dual) HIPCHECK(c, hipStreamWaitEvent(A, c->ev_cells[2], 0));
     launch_ext_final(c->stream, p, c->st);  // the report of the last stage and the stage-1 checks
     return FSI_OK;
 }
@@ -1417,7 +1493,7 @@ static std::vector<uintptr_t> graph_key(const fsi_context* c, const fsi_state* s
         k.push_back((uintptr_t)c->vel_check_from[q]);
     }
     const double* bufs[] = {s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, w->Us, w->Vs, w->phis, w->sxxs, w->syys,
-                            w->sxys};
+                            w->sxys, w->Us2, w->Vs2};
     for (const double* b : bufs) k.push_back((uintptr_t)b);
     for (int q = 0; q < 3; ++q) {
         const double* h[] = {w->cq_cur[q],     w->cq_prev[q],    w->cq_next[q],    w->vel_cur_u[q],
@@ -1769,7 +1845,7 @@ extern "C" fsi_status fsi_benchmark_kernel(fsi_context* c, fsi_state* s, fsi_wor
         }
         if (n == "cells") {
             launch_cells(c->stream, p, c->bpsi, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, c->has_reset,
-                         w->Ep[0], w->Exx[0], w->Eyy[0], w->Exy[0], c->st, 0);
+                         w->Ep[0], w->Exx[0], w->Eyy[0], w->Exy[0], &c->st->cells_err[0], 0);
             return 10.0 * 8.0 * p.nx * (double)p.ny;
         }
         return bench_launch(c->stream, name, p, vb, pb, w->lv.data(), c->geo.data(), (int)c->geo.size(), c->cplan,

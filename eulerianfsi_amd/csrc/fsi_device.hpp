@@ -132,6 +132,13 @@ struct ProjScalars {
     double lin_rel, l2;
 };
 
+// A first-failure record (a DevError code and two values describing it).
+struct ErrSlot {
+    int code;
+    int pad0;
+    double v0, v1;
+};
+
 // Everything the host reads back once per step.
 struct Status {
     int code;        // DevError of the first failure
@@ -154,6 +161,13 @@ struct Status {
     KrylovScalars vel;
     KrylovScalars pcg;
     ProjScalars proj;
+    // The phase and sigma kernels of stage q report their failures here (they
+    // may run beside the solve of stage q + 1); code_at[s] is `code` after the
+    // projection of stage s + 1, so the host can order the two as the
+    // reference's sequence of operations does.
+    ErrSlot cells_err[3];
+    int code_at[3];
+    int pad9;
 };
 
 __device__ __forceinline__ size_t at(const Params& p, int i, int j) { return (size_t)j * p.P + i; }
@@ -178,6 +192,12 @@ __device__ __forceinline__ void raise_error(Status* st, int code, double v0, dou
     if (atomicCAS(&st->code, 0, code) == 0) {
         st->v0 = v0;
         st->v1 = v1;
+    }
+}
+__device__ __forceinline__ void raise_error(ErrSlot* e, int code, double v0, double v1) {
+    if (atomicCAS(&e->code, 0, code) == 0) {
+        e->v0 = v0;
+        e->v1 = v1;
     }
 }
 

@@ -85,7 +85,7 @@ struct CellSums {
 // with `sums`, E_sigma is not stored (Exx, Eyy, Exy may be null)
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
                   const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1,
+                  double* Ephi, double* Exx, double* Eyy, double* Exy, ErrSlot* err, int stage1,
                   const CellSums* sums = nullptr);
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv);
 // StepReport extrema and counts, and the stage-1 checks, from the per-block
@@ -197,7 +197,7 @@ void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const Pro
 void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const MgLevel* lv, const MgGeom* geo,
                       int nlev, const CoarsePlan& cp, Status* st, int m0, int count, int slot);
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
-                        int has_null, double lxly);
+                        int has_null, double lxly, int slot);
 // The whole projection after the right-hand side when the warm start x is
 // only tested (PCG iteration 0; a miss is left to the caller): one pass over
 // x for the convergence test, the true residual and the mean, then the

@@ -450,7 +450,7 @@ __device__ __forceinline__ double c2fy(const Params& p, F f, int i, int k) {
 // Incoming state at a wall face for first-order upwinding
 // (InterfaceTransport.jl:112-159, StructureTensor.jl:195-265).
 __device__ __forceinline__ double upwind_state(const BcSide& b, double vel, bool low_side, double interior,
-                                               Status* st) {
+                                               ErrSlot* st) {
     bool inflow = low_side ? vel > 0.0 : vel < 0.0;
     if (vel == 0.0 || !inflow) return interior;
     if (b.kind == FSI_BC_DIRICHLET) return b.value;
@@ -461,7 +461,7 @@ __device__ __forceinline__ double upwind_state(const BcSide& b, double vel, bool
 // -div(upwind v f) at cell (i,j) (InterfaceTransport.jl:161-258).
 template <class F>
 __device__ __forceinline__ double upwind_rhs(const Params& p, const double* U, const double* V, const BcSet& bc, F f,
-                             int i, int j, Status* st) {
+                             int i, int j, ErrSlot* st) {
     auto ax = [&](int k) -> double {
         if (p.px) {
             int kk = (k == p.nx) ? 0 : k;
@@ -506,7 +506,7 @@ __device__ __forceinline__ double upwind_rhs(const Params& p, const double* U, c
 }
 
 // _phase_boundary_value (PhaseFieldMethods.jl:174-196)
-__device__ __forceinline__ double phase_bvalue(const BcSide& b, double interior, Status* st) {
+__device__ __forceinline__ double phase_bvalue(const BcSide& b, double interior, ErrSlot* st) {
     if (b.kind == FSI_BC_DIRICHLET) return b.value;
     if (fabs(b.value) > 100 * 2.220446049250313e-16) raise_error(st, DEV_NONHOMOG_NEUMANN, b.value, 0.0);
     return interior;
@@ -571,7 +571,7 @@ __device__ __forceinline__ const BcSide& side_of(const BcSet& b, int side) {
 }
 // incoming state at a wall face, per component (upwind_state)
 __device__ __forceinline__ Sym3 wall_state3(const Params& p, int side, double u, bool low_side, const Sym3& c,
-                                            Status* st) {
+                                            ErrSlot* st) {
     return {upwind_state(side_of(p.bSxx, side), u, low_side, c.xx, st),
             upwind_state(side_of(p.bSyy, side), u, low_side, c.yy, st),
             upwind_state(side_of(p.bSxy, side), u, low_side, c.xy, st)};
@@ -586,7 +586,7 @@ __device__ __forceinline__ Sym3 wall_state3(const Params& p, int side, double u,
 // and one-sided wall gradients compile out (same arithmetic).
 template <bool IN = false, class FS, class FU, class FV>
 __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU& U, const FV& V, double phic, int i,
-                                          int j, Status* st, double& oxx, double& oyy, double& oxy) {
+                                          int j, ErrSlot* st, double& oxx, double& oyy, double& oxy) {
     const Sym3 C = S(i, j);
     Sym3 fl, fr, fb, ft;
     {  // x faces i (left) and i+1 (right); periodic face nx is face 0
@@ -683,7 +683,7 @@ __device__ __forceinline__ void sigma_rhs(const Params& p, const FS& S, const FU
 // global (wrapped) indices.
 template <class F, class NX, class NY>
 __device__ __forceinline__ double phase_cdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i, int j,
-                              Status* st, int with_flux) {
+                              ErrSlot* st, int with_flux) {
     double adv = upwind_rhs(p, s.U, s.V, p.bPhi, f, i, j, st);
     if (!with_flux) return adv - 0.0;
     const double ell = 0.5 * p.cdi_eps;
@@ -701,7 +701,7 @@ __device__ __forceinline__ double phase_cdi_t(const Params& p, const Stage& s, c
 
 template <class F, class FP, class NX, class NY>
 __device__ __forceinline__ double phase_acdi_t(const Params& p, const Stage& s, const F& f, const FP& psi, const NX& N0,
-                               const NY& N1, int i, int j, double gamma, Status* st) {
+                               const NY& N1, int i, int j, double gamma, ErrSlot* st) {
     auto fx = [&](int k) -> double {
         if (p.px) {
             int kk = (k == p.nx) ? 0 : k;
@@ -742,7 +742,7 @@ __device__ __forceinline__ double phase_acdi_t(const Params& p, const Stage& s, 
 
 template <class F, class NX, class NY>
 __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i,
-                                int j, Status* st) {
+                                int j, ErrSlot* st) {
     auto fx = [&](int k) -> double {
         int kk = (p.px && k == p.nx) ? 0 : k;
         double u = s.U[at(p, kk, j)], pu;
@@ -788,7 +788,7 @@ __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s,
 }
 
 // Max |u_c| for dynamic ACDI gamma (PhaseFieldMethods.jl:576-596).
-__global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, const double* V, Status* st) {
+__global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, const double* V, ErrSlot* st) {
     __shared__ double lds[16];
     const int i = tile_id().x * TILE_X + threadIdx.x;
     double m = 0.0;
@@ -862,7 +862,7 @@ constexpr int AF_XW = TILE_X + 1, AF_YH = TILE_ROWS + 1;  // 65 x-faces per row,
 constexpr int AF_NX = TILE_ROWS * AF_XW, AF_NY = AF_YH * TILE_X;
 template <bool IN>
 __device__ __forceinline__ void acdi_tile(const Params& p, const Stage& s, const BcSet& bpsi,
-                                          double* __restrict__ Ephi, Status* st, int i0, int j0, double* s_phi,
+                                          double* __restrict__ Ephi, ErrSlot* st, int i0, int j0, double* s_phi,
                                           double* s_psi, double* s_sr, double* s_n0, double* s_n1) {
     static_assert(AF_NX <= CR2 * CW2 && AF_NY <= CR2 * CW2, "face fluxes fit the psi and sqrt(r) tiles");
     // psi and sqrt(r); once the face fluxes are formed, the x- and y-face fluxes
@@ -1045,7 +1045,7 @@ __device__ __forceinline__ void acdi_tile(const Params& p, const Stage& s, const
 }
 
 __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
-                                                      Status* st) {
+                                                      ErrSlot* st) {
     __shared__ double s_phi[CR2 * CW2];
     __shared__ double s_psi[CR2 * CW2];
     __shared__ double s_sr[CR2 * CW2];  // sqrt(r)
@@ -1062,7 +1062,7 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
 // Phase-field right-hand side, one instantiation per phase method.
 template <int PH>
 __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
-                                                 Status* st) {
+                                                 ErrSlot* st) {
     constexpr bool kPsi = PH == FSI_PHASE_ACDI;
     constexpr bool kNormals = PH != FSI_PHASE_UPWIND;
     __shared__ double s_phi[CR2 * CW2];
@@ -1197,7 +1197,7 @@ struct SymTile {
 
 template <bool IN>
 __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, double* __restrict__ Exx,
-                                           double* __restrict__ Eyy, double* __restrict__ Exy, Status* st, int stage1,
+                                           double* __restrict__ Eyy, double* __restrict__ Exy, ErrSlot* st, int stage1,
                                            const CellSums& C, int i0, int j0, double* su, double* sv, double* sxx,
                                            double* syy, double* sxy) {
     __shared__ double lds[16];
@@ -1401,7 +1401,7 @@ __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, doub
 }
 
 __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
-                                                 double* __restrict__ Eyy, double* __restrict__ Exy, Status* st,
+                                                 double* __restrict__ Eyy, double* __restrict__ Exy, ErrSlot* st,
                                                  int stage1, CellSums C) {
     __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
     __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
@@ -1507,7 +1507,7 @@ void launch_momentum(hipStream_t s, const Params& p, const double* U, const doub
 
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
                   const double* phi, const double* sxx, const double* syy, const double* sxy, int reset,
-                  double* Ephi, double* Exx, double* Eyy, double* Exy, Status* st, int stage1,
+                  double* Ephi, double* Exx, double* Eyy, double* Exy, ErrSlot* st, int stage1,
                   const CellSums* sums) {
     Stage sg{U, V, phi, sxx, syy, sxy, reset};
     if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn)

@@ -1414,6 +1414,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, in
     double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), 0.0), 1.0);
     double ctol = 1e-12 + 1e-8 * cs;
     if (!(q.l2 <= ctol)) raise_error(st, DEV_PROJ_CONSTRAINT, q.l2, ctol);
+    if (slot >= 0) st->code_at[slot] = st->code;  // the first failure up to this stage's projection
 }
 
 // Plain copy, the calibration kernel of the traffic measurement.
@@ -1556,12 +1557,12 @@ void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const M
 }
 
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
-                        int has_null, double lxly) {
+                        int has_null, double lxly, int slot) {
     const int nb = grid_blocks(p);
     k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, nb);
     k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
     k_proj_check<1><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
-    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, -1, -1);
+    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, -1, slot);
 }
 void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double rtol, int maxiter, double lxly, int slot) {

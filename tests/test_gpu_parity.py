@@ -289,6 +289,30 @@ def test_planned_steps_match_polling_steps(monkeypatch, mode):
         assert np.array_equal(a[f], b[f]), f
 
 
+def test_side_stream_steps_match_single_stream_steps(monkeypatch):
+    """The phase and sigma kernels of each stage on a second stream, beside
+    the next stage's solve (the default), give the same bits as every kernel
+    of the step on one stream (FSI_DUAL_STREAM=0), planned and polled steps
+    alike, with the stage-2 velocity in its own buffers."""
+    def run(dual):
+        monkeypatch.setenv("FSI_DUAL_STREAM", dual)
+        case = setups.seeded_cavity(64, "acdi", seed=12, **TIGHT)
+        ctx, work = _device(case)
+        fsi.make_velocity_compatible(case.state, ctx, work)
+        planned = []
+        for k in range(6):
+            fsi.integrate_step(case.state, k * case.dt, case.dt, ctx, work)
+            planned.append(work.last_stats.planned)
+        return {f: getattr(case.state, f).copy() for f in FIELDS}, planned
+
+    a, ra = run("1")
+    b, rb = run("0")
+    print("planned", ra, rb, flush=True)
+    assert ra == rb and sum(ra) >= 4
+    for f in FIELDS:
+        assert np.array_equal(a[f], b[f]), f
+
+
 def test_warm_start_history_matches_cold_starts(monkeypatch):
     """Nine tight steps with the stage solutions of the previous steps as CG
     warm start (linear, quadratic, then cubic extrapolation in time; five
