@@ -54,7 +54,7 @@ struct fsi_context {
     // solves stage q + 1 (they share no buffer: the stage-2 velocity has a
     // buffer of its own); events join the two before each stage's tendencies.
     hipStream_t side = nullptr;
-    hipEvent_t ev_start = nullptr, ev_proj[3] = {}, ev_cells[3] = {};
+    hipEvent_t ev_start = nullptr, ev_proj[3] = {}, ev_cells[3] = {}, ev_x0[3] = {};
     int dual = 0;
     std::string err;
     int device = -1;  // the HIP device current at creation
@@ -365,6 +365,7 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         for (int k = 0; k < 3; ++k) {
             HIPCHECK(c, hipEventCreateWithFlags(&c->ev_proj[k], hipEventDisableTiming));
             HIPCHECK(c, hipEventCreateWithFlags(&c->ev_cells[k], hipEventDisableTiming));
+            HIPCHECK(c, hipEventCreateWithFlags(&c->ev_x0[k], hipEventDisableTiming));
         }
     }
     HIPCHECK(c, hipGetDevice(&c->device));
@@ -589,6 +590,7 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     for (int k = 0; k < 3; ++k) {
         if (c->ev_proj[k]) (void)hipEventDestroy(c->ev_proj[k]);
         if (c->ev_cells[k]) (void)hipEventDestroy(c->ev_cells[k]);
+        if (c->ev_x0[k]) (void)hipEventDestroy(c->ev_x0[k]);
     }
     if (c->ev_start) (void)hipEventDestroy(c->ev_start);
     if (c->side) (void)hipStreamDestroy(c->side);
@@ -1084,11 +1086,43 @@ static fsi_status velocity_solve_mixed(fsi_context* c, fsi_workspace* w, VelBufs
     return FSI_OK;
 }
 
+// The warm start of stage `slot`: the stage's solutions of the previous
+// steps, extrapolated in time once two (2 u^{n-1} - u^{n-2}), three
+// (3 (u^{n-1} - u^{n-2}) + u^{n-3}) or four steps are known, instead of u^n.
+// Returns false (b untouched) before the first step or with the warm start off.
+static bool warm_history(const fsi_context* c, const fsi_workspace* w, int slot, VelBufs& b) {
+    if (!(slot >= 0 && c->vel_warm && w->stage_warm >= 1)) return false;
+    b.Un = w->vel_cur_u[slot];
+    b.Vn = w->vel_cur_v[slot];
+    b.Pu = w->vel_prev_u[slot];
+    b.Pv = w->vel_prev_v[slot];
+    b.Ou = w->vel_old_u[slot];
+    b.Ov = w->vel_old_v[slot];
+    b.Wu = w->vel_old2_u[slot];
+    b.Wv = w->vel_old2_v[slot];
+    b.extrap = std::min(w->stage_warm, 4) - 1;
+    return true;
+}
+
+// Whether the warm start of stage `slot` is formed ahead of its solve, on
+// `on` (k_vel_predict into (xu, xv)): with an extrapolation to form, so that
+// the start kernel reads one field pair instead of the history.
+static bool predict_x0(fsi_context* c, fsi_workspace* w, int slot, double* xu, double* xv, hipStream_t on) {
+    VelBufs b{};
+    if (!c->prm.visc || !warm_history(c, w, slot, b) || b.extrap < 1) return false;
+    b.xu = xu;
+    b.xv = xv;
+    launch_vel_predict(on, c->prm, b);
+    return true;
+}
+
 // fixed > 0: enqueue exactly `fixed` iteration kernels without reading the
 // residual back (speculative step); otherwise poll until converged.
-// The predictor of stage `slot` from the right-hand side (ru, rv) into (xu, xv).
+// The predictor of stage `slot` from the right-hand side (ru, rv) into (xu, xv);
+// with `x0_formed` its warm start is already in (xu, xv) (predict_x0).
 static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot,
-                                 int fixed, const double* ru, const double* rv, double* xu, double* xv) {
+                                 int fixed, const double* ru, const double* rv, double* xu, double* xv,
+                                 bool x0_formed = false) {
     const Params& p = c->prm;
     VelBufs b{};
     b.Un = s->U;
@@ -1110,20 +1144,13 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
         launch_vel_copy(c->stream, p, b);
         return FSI_OK;
     }
-    // warm start: stage `slot` of the previous step, extrapolated in time by
-    // the start kernel once two (2 u^{n-1} - u^{n-2}) or three
-    // (3 (u^{n-1} - u^{n-2}) + u^{n-3}) steps are known, instead of u^n
-    const bool warm = slot >= 0 && c->vel_warm && w->stage_warm >= 1;
-    if (warm) {
-        b.Un = w->vel_cur_u[slot];
-        b.Vn = w->vel_cur_v[slot];
-        b.Pu = w->vel_prev_u[slot];
-        b.Pv = w->vel_prev_v[slot];
-        b.Ou = w->vel_old_u[slot];
-        b.Ov = w->vel_old_v[slot];
-        b.Wu = w->vel_old2_u[slot];
-        b.Wv = w->vel_old2_v[slot];
-        b.extrap = std::min(w->stage_warm, 4) - 1;
+    // warm start: extrapolated by the start kernel, or formed ahead of it
+    if (x0_formed) {
+        b.Un = xu;
+        b.Vn = xv;
+        b.x0_in_place = 1;
+    } else {
+        (void)warm_history(c, w, slot, b);
     }
     const VelTol vt = vel_tol(c);
     // gmres: the stopping rule measures against |b - A u^n| (the state
@@ -1426,12 +1453,26 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
     // With the side stream the phase and sigma kernels of stage q overlap the
     // solve of stage q + 1: they need the stage-q velocity (projected before
     // them) and write only E_phi and the phi, sigma stage sums, which the
-    // momentum kernel of stage q + 1 reads after the join.
+    // momentum kernel of stage q + 1 reads after the join.  The warm starts
+    // of the three predictor solves (from the previous steps' solutions
+    // only) are formed there too, ahead of their solves.
     const bool dual = c->dual && c->side && c->use_graph != 1;
     hipStream_t A = c->stream, B = dual ? c->side : c->stream;
+    bool formed[3] = {false, false, false};
+    auto x0_ahead = [&](int k) -> fsi_status {
+        formed[k] = dual && predict_x0(c, w, k, w->vel_next_u[k], w->vel_next_v[k], B);
+        if (formed[k]) HIPCHECK(c, hipEventRecord(c->ev_x0[k], B));
+        return FSI_OK;
+    };
+    // an error return leaves nothing of the step running on the side stream
+    auto bail = [&](fsi_status r) {
+        if (dual) (void)hipStreamSynchronize(B);
+        return r;
+    };
     if (dual) {
         HIPCHECK(c, hipEventRecord(c->ev_start, A));  // after the Status reset
         HIPCHECK(c, hipStreamWaitEvent(B, c->ev_start, 0));
+        if ((rc = x0_ahead(0)) != FSI_OK) return bail(rc);
     }
     // the stage velocity: stages 1 and 3 in Us, Vs (swapped into the state),
     // stage 2 in Us2, Vs2 (the side stream still reads stage 1's)
@@ -1441,7 +1482,11 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
     RhsSums sums = sums_of(0);
     CellSums csums = cells_of(0);
     tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, reset, 1, &sums, &csums, B);
-    if (dual) HIPCHECK(c, hipEventRecord(c->ev_cells[0], B));
+    if (dual) {
+        HIPCHECK(c, hipEventRecord(c->ev_cells[0], B));
+        for (int k = 1; k < 3; ++k)
+            if ((rc = x0_ahead(k)) != FSI_OK) return bail(rc);
+    }
     for (int si = 1; si < 4; ++si) {
         // the predictor from the summed right-hand side
         const double alpha = dt * AI[si][si];
@@ -1451,12 +1496,13 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         const bool keep = p.visc;
         double* xu = keep ? w->vel_next_u[k] : stage_u(si);
         double* xv = keep ? w->vel_next_v[k] : stage_v(si);
+        if (formed[k]) HIPCHECK(c, hipStreamWaitEvent(A, c->ev_x0[k], 0));
         if ((rc = velocity_solve(c, s, w, alpha, k, planned ? c->plan_vel[k] : 0, w->rhs_u[k], w->rhs_v[k], xu,
-                                 xv)) != FSI_OK)
-            return rc;
+                                 xv, formed[k])) != FSI_OK)
+            return bail(rc);
         if ((rc = projection(c, w, xu, xv, stage_u(si), stage_v(si), 1, k, planned ? c->plan_proj[k] : 0)) !=
             FSI_OK)
-            return rc;
+            return bail(rc);
         if (si <= 2) {
             if (dual) {
                 HIPCHECK(c, hipEventRecord(c->ev_proj[si], A));

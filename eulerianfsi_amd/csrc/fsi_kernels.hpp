@@ -126,6 +126,7 @@ struct VelBufs {
     const double* Wv;
     int extrap;        // 0: x0 = Un; 1: x0 = 2 Un - Pu; 2: x0 = 3 (Un - Pu) + Ou;
                        // 3: x0 = 4 (Un + Ou) - 6 Pu - Wu
+    int x0_in_place;   // x0 (with its Dirichlet values) already in xu, xv: the start only reads it
     const double* Su;  // the state velocity u^n and M u^n (the stage-1 momentum kernel's):
     const double* Sv;  //   the gmres rule's reference residual ||b - A u^n||, formed
     const double* Mu;  //   on the own cells (null: not needed, the :direct method)
@@ -146,6 +147,9 @@ struct VelBufs {
     float *fr_u[2], *fr_v[2], *fp_u[2], *fp_v[2], *fq_u[2], *fq_v[2];
     int restart;  // k_vel_start: r = b - A x of the current x, for the FP64 check of a cycle
 };
+// x0 of a predictor solve formed ahead of it into b.xu, b.xv (the history
+// of b.Un .. b.Wv extrapolated as b.extrap says, the Dirichlet values set)
+void launch_vel_predict(hipStream_t s, const Params& p, const VelBufs& b);
 // the start of a predictor solve (x0, r0, q0 and the stopping thresholds of `tol`)
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
                      const VelTol& tol);

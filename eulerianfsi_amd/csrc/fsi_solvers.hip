@@ -108,6 +108,15 @@ struct HTile {
     }
 };
 
+// The warm-start extrapolations in time with their roundings fixed (explicit
+// fused multiply-adds), so that every kernel forming them gives the same bits:
+// 2 x - y, 3 (x - y) + z, 4 (x + z) - 6 y - w.
+__device__ __forceinline__ double extrap1(double x, double y) { return __builtin_fma(2.0, x, -y); }
+__device__ __forceinline__ double extrap2(double x, double y, double z) { return __builtin_fma(3.0, x - y, z); }
+__device__ __forceinline__ double extrap3(double x, double y, double z, double w) {
+    return __builtin_fma(4.0, x + z, __builtin_fma(-6.0, y, -w));
+}
+
 // x0 = cur, or the warm start extrapolated in time from the last two
 // (2 cur - prev), three (3 (cur - prev) + old) or four
 // (4 (cur + old) - 6 prev - old2) stage solutions, with the Dirichlet values,
@@ -131,8 +140,8 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx, co
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            xu[t] = 2.0 * xu[t] - yu[t];
-            xv[t] = 2.0 * xv[t] - yv[t];
+            xu[t] = extrap1(xu[t], yu[t]);
+            xv[t] = extrap1(xv[t], yv[t]);
         }
     } else if (b.extrap == 2) {
         double yu[M], yv[M], zu[M], zv[M];
@@ -145,8 +154,8 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx, co
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            xu[t] = 3.0 * (xu[t] - yu[t]) + zu[t];
-            xv[t] = 3.0 * (xv[t] - yv[t]) + zv[t];
+            xu[t] = extrap2(xu[t], yu[t], zu[t]);
+            xv[t] = extrap2(xv[t], yv[t], zv[t]);
         }
     } else if (b.extrap == 3) {
         double yu[M], yv[M], zu[M], zv[M], wu[M], wv[M];
@@ -161,8 +170,8 @@ __device__ __forceinline__ void x0_batch(const VelBufs& b, const size_t* idx, co
         }
 #pragma unroll
         for (int t = 0; t < M; ++t) {
-            xu[t] = (4.0 * (xu[t] + zu[t]) - 6.0 * yu[t]) - wu[t];
-            xv[t] = (4.0 * (xv[t] + zv[t]) - 6.0 * yv[t]) - wv[t];
+            xu[t] = extrap3(xu[t], yu[t], zu[t], wu[t]);
+            xv[t] = extrap3(xv[t], yv[t], zv[t], wv[t]);
         }
     }
 #pragma unroll
@@ -258,7 +267,7 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
         double ru = 0.0, rv = 0.0;
         if (!dir_u<IN>(p, i)) ru = bu[r] - (Xu(i, j) - alpha * visc_u<IN>(p, Xu, Xv, i, j, 0));
         if (!dir_v<IN>(p, j)) rv = bv[r] - (Xv(i, j) - alpha * visc_v<IN>(p, Xu, Xv, i, j, 0));
-        if (!b.restart) {  // a restart reads x as it is
+        if (!b.restart && !b.x0_in_place) {  // a restart (or a formed x0) reads x as it is
             b.xu[k] = Xu(i, j);
             b.xv[k] = Xv(i, j);
         }
@@ -666,6 +675,30 @@ __global__ void __launch_bounds__(BLOCK) k_vel_check(VelBufs b, Status* st, int 
     }
 }
 
+// The warm start of a predictor solve formed ahead of it (on the side
+// stream, beside the previous stage): x0_batch's extrapolation of the
+// stage's solutions of the last steps, with the Dirichlet values, on the
+// compact cells, written into the solve's iterate.  The start kernel then
+// reads it as it is (x0_in_place) instead of the four history fields.
+__global__ void __launch_bounds__(BLOCK) k_vel_predict(Params p, VelBufs b) {
+    const double gul = p.bU.l.value, gvb = p.bV.b.value;
+    FOR_TILE(p) {
+        const size_t k = at(p, i, j);
+        double xu = b.Un[k], xv = b.Vn[k];
+        if (b.extrap == 1) {
+            xu = extrap1(xu, b.Pu[k]);
+            xv = extrap1(xv, b.Pv[k]);
+        } else if (b.extrap == 2) {
+            xu = extrap2(xu, b.Pu[k], b.Ou[k]);
+            xv = extrap2(xv, b.Pv[k], b.Ov[k]);
+        } else if (b.extrap == 3) {
+            xu = extrap3(xu, b.Pu[k], b.Ou[k], b.Wu[k]);
+            xv = extrap3(xv, b.Pv[k], b.Ov[k], b.Wv[k]);
+        }
+        b.xu[k] = dir_u(p, i) ? gul : xu;
+        b.xv[k] = dir_v(p, j) ? gvb : xv;
+    }
+}
 // NoViscosity: the predictor is the stage sum with Dirichlet values (:794-796).
 __global__ void __launch_bounds__(BLOCK) k_vel_copy(Params p, VelBufs b) {
     FOR_TILE(p) {
@@ -1463,6 +1496,9 @@ void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double a
         k_vel_iter<double><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, tol,
                                                        m >= 1 && m >= check_from ? 1 : 0);
     }
+}
+void launch_vel_predict(hipStream_t s, const Params& p, const VelBufs& b) {
+    k_vel_predict<<<grid2(p), kBlock, 0, s>>>(p, b);
 }
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
     k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
