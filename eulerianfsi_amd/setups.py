@@ -57,6 +57,23 @@ def initialize_circle(grid: MACGrid, center, radius, epsilon):
     return np.asfortranarray(_profile_from_distance(d, epsilon))
 
 
+def circle_parameter_derivatives(grid: MACGrid, center, radius, epsilon):
+    """Derivatives of initialize_circle's phase field (InterfaceInitialization.jl:46-49,
+    74-95) with respect to the radius and the two centre coordinates: with
+    d = |x - c| - r and phi = (1 - tanh(d/eps))/2, dphi/dd = -(1 - tanh^2)/(2 eps),
+    dphi/dr = -dphi/dd and dphi/dc = -dphi/dd (x - c)/|x - c| (zero at the
+    centre, and wherever the clamp to [0, 1] is active)."""
+    X, Y = np.meshgrid(grid.x_p, grid.y_p, indexing="ij")
+    rho = np.hypot(X - center[0], Y - center[1])
+    t = np.tanh((rho - radius) / epsilon)
+    raw = 0.5 * (1.0 - t)
+    dphi_dd = np.where((raw > 0.0) & (raw < 1.0), -0.5 * (1.0 - t * t) / epsilon, 0.0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ex = np.where(rho > 0.0, (X - center[0]) / rho, 0.0)
+        ey = np.where(rho > 0.0, (Y - center[1]) / rho, 0.0)
+    return tuple(np.asfortranarray(a) for a in (-dphi_dd, -dphi_dd * ex, -dphi_dd * ey))
+
+
 def sharp_circle_indicator(grid: MACGrid, center, radius):
     """run_sugiyama_lid_driven_cavity.jl:450-457"""
     X, Y = np.meshgrid(grid.x_p, grid.y_p, indexing="ij")
@@ -150,7 +167,7 @@ def lid_driven_cavity(N=128, method="acdi", radius=0.2, center=(0.6, 0.5), shear
     return Case(f"cavity_N{N}_{method}", grid, metrics, bcs, models,
                 cavity_numerics(velocity_rtol, projection_rtol, precision, velocity_method), state, dt,
                 dict(N=N, method=method, radius=radius, center=tuple(center), shear_modulus=shear_modulus,
-                     viscosity=viscosity))
+                     viscosity=viscosity, init_epsilon=eps0))
 
 
 def seeded_cavity(N, method, seed, **kw) -> Case:

@@ -45,6 +45,17 @@
 #define OMP_FOR_LEVEL
 #endif
 
+/* The tangent build (fsi_tangent.cpp) compiles this file as C++ with a
+ * dual-number type in place of double: these hooks give the shear modulus
+ * its derivatives there and take the value where a double field of an
+ * fsi_api.h struct is written.  Here they are the identity. */
+#ifndef FSIO_SHEAR_MODULUS
+#define FSIO_SHEAR_MODULUS(c) ((c)->m.shear_modulus)
+#endif
+#ifndef FSIO_VALUE
+#define FSIO_VALUE(x) (x)
+#endif
+
 #define UX(a, i, j) (a)[(size_t)(i) + (size_t)(j) * (size_t)(c->nx + 1)]
 #define VY(a, i, j) (a)[(size_t)(i) + (size_t)(j) * (size_t)(c->nx)]
 #define CC(a, i, j) (a)[(size_t)(i) + (size_t)(j) * (size_t)(c->nx)]
@@ -896,7 +907,7 @@ static int phase_rhs_acdi(fsio_ctx* c, const double* U, const double* V, const d
             }
             sides[s]->value = 0.0;
         } else if (sides[s]->kind == FSI_BC_DIRICHLET) {
-            sides[s]->value = psi_value(c, sides[s]->value);
+            sides[s]->value = FSIO_VALUE(psi_value(c, sides[s]->value));
         }
     }
     double *psi = dalloc(nc), *gx = dalloc(nc), *gy = dalloc(nc), *gxf = dalloc(nfx), *gyf = dalloc(nfy);
@@ -1324,7 +1335,7 @@ static double dot(const double* a, const double* b, size_t n) {
  * b and A are the whole system of _impose_row_dirichlet: the Dirichlet rows
  * enter |b| with their value and |b - A x0| with g - x0.  A non-finite
  * residual is an error at once. */
-int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const double* x0,
+int fsio_velocity_solve(fsio_ctx* c, fsio_scalar alpha, const double* rhs_in, const double* x0,
                         double* x, int* iters) {
     int N2 = 2 * c->N;
     unsigned char* mask = (unsigned char*)calloc(N2, 1);
@@ -1373,7 +1384,7 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
         if (sqrt(rr) <= tol || it >= limit) {
             if (phase) {
                 if (sqrt(rr) <= tol) break;
-                rc = set_err(c, FSI_E_SOLVER, "velocity GMRES residual is too large: %.6e", sqrt(rr) / bscale);
+                rc = set_err(c, FSI_E_SOLVER, "velocity GMRES residual is too large: %.6e", FSIO_VALUE(sqrt(rr) / bscale));
                 break;
             }
             /* GMRES has stopped (the rule met or the limit reached): the check
@@ -1385,7 +1396,7 @@ int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs_in, const d
                 limit = cap;
                 continue;
             }
-            rc = set_err(c, FSI_E_SOLVER, "velocity GMRES residual is too large: %.6e", sqrt(rr) / bscale);
+            rc = set_err(c, FSI_E_SOLVER, "velocity GMRES residual is too large: %.6e", FSIO_VALUE(sqrt(rr) / bscale));
             break;
         }
         viscous_apply(c, p, tmp, 0);
@@ -1681,7 +1692,7 @@ static int project(fsio_ctx* c, double* U, double* V, const double* G, double* c
             return set_err(c, FSI_E_ARGUMENT,
                            "The prescribed divergence constraint is incompatible with the "
                            "Neumann/periodic correction boundaries (defect %.3e, tolerance %.3e).",
-                           defect, tol);
+                           FSIO_VALUE(defect), FSIO_VALUE(tol));
         }
     }
     for (int k = 0; k < N; ++k)
@@ -1736,23 +1747,23 @@ static int project(fsio_ctx* c, double* U, double* V, const double* G, double* c
     tl2 = sqrt(tl2 / LA);
     free(pdiv); free(rhs); free(Aq); free(gx); free(gy);
     if (res) {
-        res->linear_relative_residual = lin;
-        res->constraint_linf = linf;
-        res->constraint_l2 = l2;
-        res->compatibility_defect = defect;
+        res->linear_relative_residual = FSIO_VALUE(lin);
+        res->constraint_linf = FSIO_VALUE(linf);
+        res->constraint_l2 = FSIO_VALUE(l2);
+        res->compatibility_defect = FSIO_VALUE(defect);
         res->iterations = iters;
     }
     if (!(lin <= 1e-8)) /* :279-285 */
         return set_err(c, FSI_E_SOLVER,
                        "Projection Poisson solve did not reach the required true-residual "
-                       "accuracy. Relative residual = %.3e, tolerance = 1e-08.", lin);
+                       "accuracy. Relative residual = %.3e, tolerance = 1e-08.", FSIO_VALUE(lin));
     double cs = pl2 > tl2 ? pl2 : tl2;
     if (cs < 1.0) cs = 1.0;
     double ctol = 1e-12 + 1e-8 * cs;
     if (!(l2 <= ctol)) /* :287-297 */
         return set_err(c, FSI_E_SOLVER,
                        "Projection did not satisfy the prescribed divergence constraint. "
-                       "Weighted L2 residual = %.3e, tolerance = %.3e.", l2, ctol);
+                       "Weighted L2 residual = %.3e, tolerance = %.3e.", FSIO_VALUE(l2), FSIO_VALUE(ctol));
     return 0;
 }
 
@@ -1858,7 +1869,7 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
     /* _accumulate_elastic_force! :903-924 with elastic_stress
        StructuralConstitutive.jl:100-113 */
     if (c->m.elastic_kind == FSI_ELASTIC_LINEAR) {
-        double *Txx = dalloc(N), *Tyy = dalloc(N), *Txy = dalloc(N), Gm = c->m.shear_modulus;
+        double *Txx = dalloc(N), *Tyy = dalloc(N), *Txy = dalloc(N), Gm = FSIO_SHEAR_MODULUS(c);
         for (int k = 0; k < N; ++k) {
             Txx[k] = phi[k] * (Gm * (sxx[k] - 1.0));
             Tyy[k] = phi[k] * (Gm * (syy[k] - 1.0));
@@ -1897,7 +1908,7 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
 /* Tendencies of the stage-1 state: sigma is taken after the safety reset of
  * _copy_time_n_state! (:686-702), as the first stage of integrate_step! sees it. */
 int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const double* phi,
-                         const double* sxx, const double* syy, const double* sxy, double t,
+                         const double* sxx, const double* syy, const double* sxy, fsio_scalar t,
                          double* E_u, double* E_v, double* I_u, double* I_v, double* E_phi,
                          double* E_sxx, double* E_syy, double* E_sxy) {
     double* G = dalloc(c->N);
@@ -1944,8 +1955,8 @@ static int validate_state(fsio_ctx* c, const double* U, const double* V, const d
 }
 
 int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* resource,
-                        double* sxx, double* syy, double* sxy, double* correction, double t,
-                        double dt, fsi_step_report* rep, fsi_step_stats* stats) {
+                        double* sxx, double* syy, double* sxy, double* correction, fsio_scalar t,
+                        fsio_scalar dt, fsi_step_report* rep, fsi_step_stats* stats) {
     int nx = c->nx, ny = c->ny, N = c->N, rc;
     size_t nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
     if ((rc = validate_state(c, U, V, phi, resource, sxx, syy, sxy))) return rc;
@@ -2014,7 +2025,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
                     rc = set_err(c, FSI_E_ARGUMENT,
                                  "The velocity entering stage 1 does not satisfy the prescribed "
                                  "divergence constraint. Maximum |div(v)-G| = %.3e, tolerance = %.3e.",
-                                 linf, tol);
+                                 FSIO_VALUE(linf), FSIO_VALUE(tol));
                     break;
                 }
             }
@@ -2041,7 +2052,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
                 free(mask); free(gv);
             } else {
                 int it = 0;
-                rc = fsio_velocity_solve(c, dt * AI[s][s], rhs, uv_n, pred, &it);
+                rc = fsio_velocity_solve(c, FSIO_VALUE(dt * AI[s][s]), rhs, uv_n, pred, &it);
                 if (stats) stats->velocity_iterations[s - 1] = it;
                 if (rc) break;
             }
@@ -2086,13 +2097,13 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
                 if (phi[k] > pmax) pmax = phi[k];
                 if (resource[k] < rmin) rmin = resource[k];
             }
-            rep->maximum_projection_constraint_linf = max_linf;
+            rep->maximum_projection_constraint_linf = FSIO_VALUE(max_linf);
             rep->maximum_growth_trace_error = 0.0;
-            rep->minimum_phi = pmin;
-            rep->maximum_phi = pmax;
-            rep->minimum_resource = rmin;
-            rep->minimum_sigma_eigenvalue = mineig;
-            rep->maximum_abs_sigma_determinant_error = maxdet;
+            rep->minimum_phi = FSIO_VALUE(pmin);
+            rep->maximum_phi = FSIO_VALUE(pmax);
+            rep->minimum_resource = FSIO_VALUE(rmin);
+            rep->minimum_sigma_eigenvalue = FSIO_VALUE(mineig);
+            rep->maximum_abs_sigma_determinant_error = FSIO_VALUE(maxdet);
             rep->non_spd_cells = nonspd;
         }
     }

@@ -20,6 +20,9 @@ extern "C" {
 #endif
 
 typedef struct fsio_ctx fsio_ctx;
+/* scalar arguments (times, steps, shifts): double in every build of the
+ * oracle, the tangent build (fsi_tangent.cpp) included */
+typedef double fsio_scalar;
 
 int fsio_create(const fsi_grid* grid, const fsi_bcs* bcs, const fsi_models* models,
                 const fsi_numerics* numerics, fsio_ctx** out);
@@ -30,7 +33,7 @@ const char* fsio_last_error(const fsio_ctx* c);
  * workspace warm start (nx*ny), updated as the reference updates it. */
 int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* resource,
                         double* sxx, double* syy, double* sxy, double* correction,
-                        double t, double dt, fsi_step_report* rep, fsi_step_stats* stats);
+                        fsio_scalar t, fsio_scalar dt, fsi_step_report* rep, fsi_step_stats* stats);
 int fsio_make_velocity_compatible(fsio_ctx* c, double* U, double* V, const double* phi,
                                   const double* resource, double* correction,
                                   fsi_projection_result* res);
@@ -38,7 +41,7 @@ int fsio_make_velocity_compatible(fsio_ctx* c, double* U, double* V, const doubl
 int fsio_project(fsio_ctx* c, double* U, double* V, double* correction,
                  fsi_projection_result* res);
 int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const double* phi, /* sigma after the safety reset */
-                         const double* sxx, const double* syy, const double* sxy, double t,
+                         const double* sxx, const double* syy, const double* sxy, fsio_scalar t,
                          double* E_u, double* E_v, double* I_u, double* I_v, double* E_phi,
                          double* E_sxx, double* E_syy, double* E_sxy);
 
@@ -67,7 +70,7 @@ void fsio_laplacian_apply(fsio_ctx* c, const double* q, double* out, int with_bo
 /* Solve (I - alpha M) uv = rhs + alpha b with Dirichlet rows, as
  * _build_velocity_stage! (src/TimeIntegrationIMEX.jl:794-812).  `rhs` is the
  * stage sum before the alpha*b term.  Returns CG iterations in *iters. */
-int fsio_velocity_solve(fsio_ctx* c, double alpha, const double* rhs, const double* x0,
+int fsio_velocity_solve(fsio_ctx* c, fsio_scalar alpha, const double* rhs, const double* x0,
                         double* x, int* iters);
 /* Metric vectors: which 0..15 in MeshMetrics field order. */
 const double* fsio_metric(const fsio_ctx* c, int which, int* len);
