@@ -22,6 +22,6 @@ from .imex import (A_EXPLICIT, A_IMPLICIT, STAGE_ABSCISSA, FourStageIMEXFraction
                    ProjectionResult, StepReport, allocate_imex_workspace, build_imex_context,
                    ensemble_step, evaluate_stage_tendencies, integrate_step, make_velocity_compatible, project)
 from .mesh import MACGrid, MeshMetrics, build_mesh_metrics, generate_grid
-from . import diagnostics
+from . import adjoint, diagnostics
 
 __all__ = [n for n in dir() if not n.startswith("_")]

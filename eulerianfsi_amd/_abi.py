@@ -242,6 +242,12 @@ class DiagnosticsC(ctypes.Structure):
         + [(n, TensorRegionC) for n in ("global_region", "core", "interface_region", "tail")])
 
 
+# ---- discrete adjoint (fsi_api.h) ----------------------------------------------
+class AdjointStatsC(ctypes.Structure):
+    _fields_ = [("steps", c_int64), ("advances", c_int64), ("recordings", c_int64), ("adjoint_steps", c_int64),
+                ("snapshots_used", c_int32), ("repetitions", c_int32), ("device_bytes", c_double)]
+
+
 # ---- library -----------------------------------------------------------------
 _LIB = None
 LIB_PATH = Path(__file__).resolve().parent / "libfsi_hip.so"
@@ -275,6 +281,18 @@ def _declare(lib):
         "fsi_phase_centroid_noalloc": (c_int, [P, P, c_void_p, c_void_p]),
         "fsi_diagnostics_compute": (c_int, [P, P, POINTER(DiagParamsC), POINTER(DiagnosticsC)]),
         "fsi_version": (c_char_p, []),
+        "fsi_adjoint_create": (c_int, [P, P, c_int32, POINTER(P)]),
+        "fsi_adjoint_destroy": (None, [P]),
+        "fsi_adjoint_forward": (c_int, [P, P, c_double, c_double, c_int32]),
+        "fsi_adjoint_backward": (c_int, [P, P, _dp]),
+        "fsi_adjoint_get_stats": (c_int, [P, POINTER(AdjointStatsC)]),
+        "fsi_adjoint_record_step": (c_int, [P, P, c_double, c_double]),
+        "fsi_adjoint_step": (c_int, [P, P, _dp]),
+        "fsi_adjoint_centroid_objective": (c_int, [P, P, c_double, c_double, P, _dp]),
+        "fsi_adjoint_tendencies": (c_int, [P, P, POINTER(_dp), POINTER(_dp), _dp]),
+        "fsi_adjoint_primal_tendencies": (c_int, [P, P, POINTER(_dp)]),
+        "fsi_adjoint_plan": (c_int, [c_int32, c_int32, POINTER(c_int64), POINTER(c_int32), POINTER(c_int32),
+                                     POINTER(c_int32), c_int64, POINTER(c_int64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -289,7 +307,9 @@ EXPORTED = (
     "fsi_workspace_download_correction", "fsi_workspace_upload_correction", "fsi_make_velocity_compatible",
     "fsi_integrate_step", "fsi_ensemble_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel",
     "fsi_velocity_extrema", "fsi_phase_centroid", "fsi_phase_centroid_noalloc", "fsi_diagnostics_compute",
-    "fsi_version",
+    "fsi_version", "fsi_adjoint_create", "fsi_adjoint_destroy", "fsi_adjoint_forward", "fsi_adjoint_backward",
+    "fsi_adjoint_get_stats", "fsi_adjoint_record_step", "fsi_adjoint_step", "fsi_adjoint_centroid_objective",
+    "fsi_adjoint_tendencies", "fsi_adjoint_primal_tendencies", "fsi_adjoint_plan",
 )
 
 

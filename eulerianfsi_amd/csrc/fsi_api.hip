@@ -23,152 +23,12 @@
 
 using namespace fsi;
 
-namespace {
-
-const double AE[4][4] = {{0, 0, 0, 0},
-                         {8.0 / 15.0, 0, 0, 0},
-                         {1.0 / 4.0, 5.0 / 12.0, 0, 0},
-                         {1.0 / 4.0, 0, 3.0 / 4.0, 0}};
-const double AI[4][4] = {{0, 0, 0, 0},
-                         {4.0 / 15.0, 4.0 / 15.0, 0, 0},
-                         {4.0 / 15.0, 1.0 / 3.0, 1.0 / 15.0, 0},
-                         {4.0 / 15.0, 1.0 / 3.0, 7.0 / 30.0, 1.0 / 6.0}};
-
-constexpr int kMaxLevels = 24;
-
-}  // namespace
-
-struct fsi_context {
-    fsi_grid grid{};
-    fsi_bcs bcs{};
-    fsi_models models{};
-    fsi_numerics num{};
-    Params prm{};
-    BcSet bpsi{};
-    int P = 0;
-    size_t felems = 0;
-    int has_null = 0;
-    int has_reset = 0;
-    hipStream_t stream = nullptr;
-    // The phase and sigma kernels of stage q run on `side` while `stream`
-    // solves stage q + 1 (they share no buffer: the stage-2 velocity has a
-    // buffer of its own); events join the two before each stage's tendencies.
-    hipStream_t side = nullptr;
-    hipEvent_t ev_start = nullptr, ev_proj[3] = {}, ev_cells[3] = {}, ev_x0[3] = {};
-    int dual = 0;
-    std::string err;
-    int device = -1;  // the HIP device current at creation
-    double* force_row = nullptr;
-    double* ext = nullptr;  // per-block extrema of the report and checks (EX_* slots)
-    double* diag = nullptr;      // per-block partials of the driver diagnostics
-    double* diag_out = nullptr;  // their reduced values
-    double* lap_bc = nullptr;
-    double* bvisc_u = nullptr;
-    double* bvisc_v = nullptr;
-    double* partials = nullptr;
-    Status* st = nullptr;       // device
-    Status* st_host = nullptr;  // pinned mirror
-    std::vector<MgGeom> geo;
-    CoarsePlan cplan{};         // single-workgroup coarse solve of geo.back()
-    double* co_dense = nullptr;  // its dense bottom inverse
-    float* co_dense_f = nullptr; // the same in FP32 (mixed precision)
-    // Direct solve of the projection system by fast transforms (power-of-two
-    // axes with Neumann/Neumann, Dirichlet/Dirichlet or periodic sides): it
-    // is the PCG's initial guess, which then usually meets the tolerance at
-    // iteration 0.  FSI_SPECTRAL=0 keeps the multigrid-PCG solve alone.
-    int use_spectral = 1;
-    SpecPlan spec{};
-    double* spec_tab = nullptr;  // device tables of both axes
-    double* spec_part = nullptr;  // partials of the fused transform passes (inside spec_tab)
-    int last_vel_iters[3] = {8, 8, 8};
-    int last_proj_iters[3] = {6, 6, 6};
-    // Planned steps: each solve enqueued with a planned iteration count and no
-    // read-back until the end of the step (see fsi_integrate_step).  Mode 2
-    // launches the kernels directly, mode 1 replays the step as a captured
-    // HIP graph (measured slower on ROCm 7 for the ~460-launch step), mode 0
-    // polls each solve.
-    int use_graph = 2;
-    int vel_warm = 1;  // predictor warm starts from the stage histories (FSI_STAGE_WARM=0: from u^n)
-    int have_history = 0;  // iteration counts of a completed step are known
-    int plan_vel[3] = {0, 0, 0};   // planned iterations (mixed precision: inner iterations per cycle)
-    int plan_cyc[3] = {0, 0, 0};   // mixed precision: planned refinement cycles
-    int last_imax[3] = {8, 8, 8};  // mixed precision: most inner iterations of a cycle in the last solve
-    int vel_check_from[3] = {0, 0, 0};  // the fewest iterations of the same history
-    int plan_proj[3] = {0, 0, 0};
-    // iteration counts of the last kHist completed steps per stage (the plan
-    // is their maximum plus a margin, so alternating counts keep one plan)
-    static constexpr int kHist = 4;
-    int hist_vel[3][kHist] = {};
-    int hist_cyc[3][kHist] = {};   // mixed precision: cycles and most inner iterations of a cycle
-    int hist_imax[3][kHist] = {};
-    int hist_proj[3][kHist] = {};
-    int hist_n = 0;
-    long long spec_steps = 0, spec_misses = 0;
-};
-
-struct fsi_state {
-    fsi_context* ctx = nullptr;
-    double *U = nullptr, *V = nullptr, *phi = nullptr, *res = nullptr;
-    double *sxx = nullptr, *syy = nullptr, *sxy = nullptr;
-    double res_min = INFINITY;
-    long long res_nonfinite = 0;
-};
-
-struct fsi_workspace {
-    fsi_context* ctx = nullptr;
-    double *Us = nullptr, *Vs = nullptr, *phis = nullptr, *sxxs = nullptr, *syys = nullptr, *sxys = nullptr;
-    double *Eu[3] = {}, *Ev[3] = {}, *Iu[3] = {}, *Iv[3] = {};
-    double *Ep[3] = {}, *Exx[3] = {}, *Eyy[3] = {}, *Exy[3] = {};
-    double *ru = nullptr, *rv = nullptr, *r_u[2] = {}, *r_v[2] = {};
-    // velocity right-hand sides of stages 2..4, summed by the momentum kernels
-    double *rhs_u[3] = {}, *rhs_v[3] = {};
-    // phi and sigma of stages 2 and 3, summed by the sigma kernels (stage 4
-    // goes to phis, sxxs, syys, sxys, which the state swaps with)
-    double *cphi[2] = {}, *csxx[2] = {}, *csyy[2] = {}, *csxy[2] = {};
-    double *p_u[2] = {}, *p_v[2] = {}, *q_u[2] = {}, *q_v[2] = {};
-    // mixed precision: the FP32 r, p, q of the inner CG (null in FP64 mode)
-    float *fr_u[2] = {}, *fr_v[2] = {}, *fp_u[2] = {}, *fp_v[2] = {}, *fq_u[2] = {}, *fq_v[2] = {};
-    // `correction` (work.correction, :828-833) is corr[corr_cur].  A step or
-    // a projection writes its correction into corr[1 - corr_cur] and flips
-    // corr_cur only when it has succeeded, so a failing call leaves the
-    // correction as it was (project! throws before copyto!, Projection.jl:279-298).
-    double *rhs = nullptr, *corr[2] = {}, *r = nullptr, *z = nullptr, *pp[2] = {}, *w = nullptr;
-    int corr_cur = 0;
-    // Warm starts of the three stage projections of a step: stage s starts
-    // from stage s of the previous step (closer than the previous stage's
-    // correction), extrapolated as 2 q^{n-1} - q^{n-2}.  The step solves into
-    // cq_next; the three buffers of a stage trade roles only after a
-    // successful step, so a failed or redone step starts from the same
-    // history.  Only iteration counts depend on it.
-    double *cq_cur[3] = {}, *cq_prev[3] = {}, *cq_next[3] = {};
-    double* xs = nullptr;  // the iterate of a standalone projection (fsi_project)
-    double *Us2 = nullptr, *Vs2 = nullptr;  // the stage-2 velocity (stages 1 and 3 use Us, Vs)
-    double *mun_u = nullptr, *mun_v = nullptr;  // M u^n of the step (the reference residual of the predictor)
-    // the same for the predictor solves: stage s of the previous two steps
-    // velocity stage solutions of the last four steps (newest first) and a
-    // fifth buffer that the solve writes into (the projection then writes
-    // the projected stage velocity to Us, leaving the solution in place);
-    // the five trade roles once a step has succeeded
-    double *vel_cur_u[3] = {}, *vel_cur_v[3] = {}, *vel_prev_u[3] = {}, *vel_prev_v[3] = {};
-    double *vel_old_u[3] = {}, *vel_old_v[3] = {}, *vel_old2_u[3] = {}, *vel_old2_v[3] = {};
-    double *vel_next_u[3] = {}, *vel_next_v[3] = {};
-    int stage_warm = 0;         // completed steps whose stage solutions the history buffers hold (0 .. 3)
-    std::vector<MgLevel> lv;
-    std::vector<double*> owned;
-    // captured steps, keyed by everything their kernel arguments depend on:
-    // the plan, dt, the history depth (it changes the launch sequence) and
-    // every buffer the step reads or writes whose role rotates between steps
-    struct StepGraph {
-        std::vector<uintptr_t> key;
-        hipGraphExec_t exec;
-    };
-    std::vector<StepGraph> graphs;
-};
+#include "fsi_internal.hpp"
 
 // ---------------------------------------------------------------------------
 // helpers
 // ---------------------------------------------------------------------------
-static fsi_status fail(fsi_context* c, fsi_status code, const char* fmt, ...) {
+fsi_status fail(fsi_context* c, fsi_status code, const char* fmt, ...) {
     char buf[1024];
     va_list ap;
     va_start(ap, fmt);
@@ -178,21 +38,14 @@ static fsi_status fail(fsi_context* c, fsi_status code, const char* fmt, ...) {
     return code;
 }
 
-#define HIPCHECK(c, expr)                                                                        \
-    do {                                                                                         \
-        hipError_t e_ = (expr);                                                                  \
-        if (e_ != hipSuccess)                                                                    \
-            return fail((c), FSI_E_HIP, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, \
-                        __LINE__);                                                               \
-    } while (0)
 
-static hipError_t dalloc(fsi_context* c, double** p, size_t n) {
+hipError_t dalloc(fsi_context* c, double** p, size_t n) {
     hipError_t e = hipMalloc((void**)p, n * sizeof(double));
     if (e == hipSuccess) e = hipMemsetAsync(*p, 0, n * sizeof(double), c->stream);
     return e;
 }
 
-static BcSet to_bcset(const fsi_bc_setup& s) {
+BcSet to_bcset(const fsi_bc_setup& s) {
     BcSet b;
     b.l = {s.left.kind, s.left.value};
     b.r = {s.right.kind, s.right.value};
@@ -201,12 +54,12 @@ static BcSet to_bcset(const fsi_bc_setup& s) {
     return b;
 }
 
-static fsi_status upload2d(fsi_context* c, double* dst, const double* src, int width, int height) {
+fsi_status upload2d(fsi_context* c, double* dst, const double* src, int width, int height) {
     HIPCHECK(c, hipMemcpy2DAsync(dst, (size_t)c->P * sizeof(double), src, (size_t)width * sizeof(double),
                                  (size_t)width * sizeof(double), height, hipMemcpyHostToDevice, c->stream));
     return FSI_OK;
 }
-static fsi_status download2d(fsi_context* c, double* dst, const double* src, int width, int height) {
+fsi_status download2d(fsi_context* c, double* dst, const double* src, int width, int height) {
     HIPCHECK(c, hipMemcpy2DAsync(dst, (size_t)width * sizeof(double), src, (size_t)c->P * sizeof(double),
                                  (size_t)width * sizeof(double), height, hipMemcpyDeviceToHost, c->stream));
     return FSI_OK;
@@ -977,7 +830,7 @@ extern "C" fsi_status fsi_workspace_upload_correction(fsi_context* c, fsi_worksp
 // ---------------------------------------------------------------------------
 // step pieces
 // ---------------------------------------------------------------------------
-static void status_reset(fsi_context* c) {
+void status_reset(fsi_context* c) {
     Status& s = *c->st_host;
     std::memset(&s, 0, sizeof(Status));
     s.phi_min = INFINITY;
@@ -985,7 +838,7 @@ static void status_reset(fsi_context* c) {
     s.min_eig = INFINITY;
 }
 
-static fsi_status status_push(fsi_context* c) {
+fsi_status status_push(fsi_context* c) {
     HIPCHECK(c, hipMemcpyAsync(c->st, c->st_host, sizeof(Status), hipMemcpyHostToDevice, c->stream));
     // the pinned buffer is rewritten by the next status_pull; wait for the copy
     HIPCHECK(c, hipStreamSynchronize(c->stream));
@@ -997,7 +850,7 @@ static fsi_status status_push_reset(fsi_context* c) {
     HIPCHECK(c, hipGetLastError());
     return FSI_OK;
 }
-static fsi_status status_pull(fsi_context* c) {
+fsi_status status_pull(fsi_context* c) {
     HIPCHECK(c, hipMemcpyAsync(c->st_host, c->st, sizeof(Status), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(c, hipStreamSynchronize(c->stream));
     HIPCHECK(c, hipGetLastError());
@@ -1120,9 +973,9 @@ static bool predict_x0(fsi_context* c, fsi_workspace* w, int slot, double* xu, d
 // residual back (speculative step); otherwise poll until converged.
 // The predictor of stage `slot` from the right-hand side (ru, rv) into (xu, xv);
 // with `x0_formed` its warm start is already in (xu, xv) (predict_x0).
-static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot,
+fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha, int slot,
                                  int fixed, const double* ru, const double* rv, double* xu, double* xv,
-                                 bool x0_formed = false) {
+                                 bool x0_formed) {
     const Params& p = c->prm;
     VelBufs b{};
     b.Un = s->U;
@@ -1191,7 +1044,7 @@ static fsi_status velocity_solve(fsi_context* c, fsi_state* s, fsi_workspace* w,
 // on success.  slot >= 0: a stage projection of a step, iterating in that
 // stage's cq_next from the extrapolated history; slot -1: a standalone
 // projection from the workspace correction.
-static fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, double* Vin, double* U, double* V,
+fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, double* Vin, double* U, double* V,
                              int inject, int slot, int fixed) {
     const Params& p = c->prm;
     ProjBufs b{};
@@ -1289,7 +1142,7 @@ static ErrSlot first_error(const Status& s) {
     return e;
 }
 
-static fsi_status device_error(fsi_context* c) {
+fsi_status device_error(fsi_context* c) {
     const ErrSlot s = first_error(*c->st_host);
     switch (s.code) {
     case DEV_OK: return FSI_OK;
@@ -1503,6 +1356,10 @@ static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, d
         if ((rc = projection(c, w, xu, xv, stage_u(si), stage_v(si), 1, k, planned ? c->plan_proj[k] : 0)) !=
             FSI_OK)
             return bail(rc);
+        if (si == 1 && w->tape_U1) {  // the adjoint's tape (nothing else reads these copies)
+            HIPCHECK(c, hipMemcpyAsync(w->tape_U1, stage_u(1), c->felems * sizeof(double), hipMemcpyDeviceToDevice, A));
+            HIPCHECK(c, hipMemcpyAsync(w->tape_V1, stage_v(1), c->felems * sizeof(double), hipMemcpyDeviceToDevice, A));
+        }
         if (si <= 2) {
             if (dual) {
                 HIPCHECK(c, hipEventRecord(c->ev_proj[si], A));

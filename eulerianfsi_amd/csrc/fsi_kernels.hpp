@@ -287,4 +287,32 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
                     const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st,
                     const SpecPlan* sp = nullptr);
 
+// ---- discrete adjoint (fsi_adjoint.hip) ----
+// The linearisation point: one stage's fields (sigma after the safety reset,
+// or with `reset` the reset applied on load, as the first stage reads sigma^n)
+struct AdjStage {
+    const double *U, *V, *phi, *sxx, *syy, *sxy;
+    int reset;
+};
+// adjoints of the stage's tendencies E_u, E_v, I_u, I_v, E_phi, E_sigma (null: zero)
+struct AdjBar {
+    const double *Eu, *Ev, *Iu, *Iv, *Ephi, *Exx, *Eyy, *Exy;
+};
+// adjoints of the stage's fields, accumulated (+=); k_adj_primal writes its
+// tendencies here instead (E_u, E_v in U, V; I_u, I_v in aux_u, aux_v)
+struct AdjGrads {
+    double *U, *V, *phi, *sxx, *syy, *sxy;
+    double *aux_u, *aux_v;
+};
+int adj_block_count(const Params& p);
+void launch_adj_primal(hipStream_t s, const Params& p, const BcSet& bpsi, const AdjStage& st, const AdjGrads& out);
+// out += (d tendencies / d stage fields)^T b; *dG += b . d tendencies / d G
+// (partials: adj_block_count(p) values)
+void launch_adj_tendencies(hipStream_t s, const Params& p, const BcSet& bpsi, const AdjStage& st, const AdjBar& b,
+                           const AdjGrads& out, double* partials, double* dG);
+void launch_adj_axpy(hipStream_t s, size_t n, double* y, double a, const double* x);
+void launch_adj_mask(hipStream_t s, const Params& p, const double* phi, double* gxx, double* gyy, double* gxy);
+void launch_adj_walls(hipStream_t s, const Params& p, double* U, double* V);
+void launch_adj_affine(hipStream_t s, const Params& p, double* phi, double A, double B, double C);
+
 }  // namespace fsi

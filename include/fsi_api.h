@@ -404,6 +404,73 @@ fsi_status fsi_benchmark_kernel(fsi_context* ctx, fsi_state* st, fsi_workspace* 
 /* Library identification: device name, architecture and build. */
 const char* fsi_version(void);
 
+/* ---- discrete adjoint ----------------------------------------------------
+ * The reverse-mode derivative of integrate_step! (src/TimeIntegrationIMEX.jl
+ * :1152-1256) and of a run of steps, for the reference's differentiation
+ * use (gradients of an objective of the final state with respect to the
+ * initial state and the shear modulus, LinearInternalTensorStress(G),
+ * StructuralConstitutive.jl:100-113).  The reference has no hand-written
+ * adjoint: a Julia caller would differentiate integrate_step! with an AD
+ * package through ChainRulesCore; these entry points are what its rrule
+ * binds.  Covered: walled domains (no periodic side),
+ * ACDI with a fixed gamma, FP64, the closures of the step otherwise;
+ * FSI_E_UNSUPPORTED elsewhere.  The step is differentiated along the
+ * branches it took (upwind sides, the psi clamp, the safety reset: frozen
+ * switches) with its linear solves taken as exact, so a run meant for
+ * derivatives uses tight solver settings (velocity_method :direct).
+ * Cotangents are fsi_state objects of the same context (their resource
+ * field is ignored). */
+typedef struct fsi_adjoint fsi_adjoint;
+typedef struct {
+    int64_t steps;          /* steps of the last run                               */
+    int64_t advances;       /* steps recomputed from checkpoints                   */
+    int64_t recordings;     /* steps run with their stage fields taped             */
+    int64_t adjoint_steps;  /* transposed steps                                    */
+    int32_t snapshots_used; /* most checkpoints held at once                       */
+    int32_t repetitions;    /* the schedule's repetition number                    */
+    double device_bytes;    /* device memory the adjoint object holds              */
+} fsi_adjoint_stats;
+
+/* The adjoint of runs on (ctx, ws), holding up to `snapshots` checkpoints of
+ * the state (binomial checkpointing, Griewank & Walther's revolve). */
+fsi_status fsi_adjoint_create(fsi_context* ctx, fsi_workspace* ws, int32_t snapshots, fsi_adjoint** out);
+void fsi_adjoint_destroy(fsi_adjoint* adj);
+/* The forward half of the rrule of n_steps integrate_step! calls from t0 by
+ * dt: advances st to the final state, keeping checkpoints. */
+fsi_status fsi_adjoint_forward(fsi_adjoint* adj, fsi_state* st, double t0, double dt, int32_t n_steps);
+/* The pullback: `bar` holds the cotangent of the final state on entry and
+ * that of the initial state on return; *d_shear_modulus receives dJ/dG.
+ * Once per fsi_adjoint_forward. */
+fsi_status fsi_adjoint_backward(fsi_adjoint* adj, fsi_state* bar, double* d_shear_modulus);
+fsi_status fsi_adjoint_get_stats(const fsi_adjoint* adj, fsi_adjoint_stats* out);
+/* One step: fsi_adjoint_record_step runs integrate_step! on st (advancing
+ * it) and tapes it; fsi_adjoint_step maps the cotangent of its output
+ * (bar, in place) to that of its input. */
+fsi_status fsi_adjoint_record_step(fsi_adjoint* adj, fsi_state* st, double t, double dt);
+fsi_status fsi_adjoint_step(fsi_adjoint* adj, fsi_state* bar, double* d_shear_modulus);
+/* J = |x_c - (tx, ty)|^2 of the phase centroid (phase_centroid,
+ * scripts/benchmarks/run_sugiyama_lid_driven_cavity.jl:496-509) and its
+ * cotangent with respect to the state, written to bar. */
+fsi_status fsi_adjoint_centroid_objective(fsi_context* ctx, const fsi_state* st, double tx, double ty,
+                                          fsi_state* bar, double* J);
+/* Kernel parity.  fsi_adjoint_tendencies: the transposed linearisation of
+ * the stage tendencies of fsi_eval_tendencies at the state; bar_in[8] =
+ * cotangents of E_u, E_v, I_u, I_v, E_phi, E_sxx, E_syy, E_sxy (host arrays
+ * in their shapes, NULL = zero), grad_out[6] = cotangents of U, V, phi,
+ * sigma_xx, sigma_yy, sigma_xy (NULL = skipped), *d_shear_modulus the
+ * shear-modulus part.  fsi_adjoint_primal_tendencies: the same tendencies as
+ * fsi_eval_tendencies from the functions the adjoint differentiates
+ * (out[8] in the order of bar_in). */
+fsi_status fsi_adjoint_tendencies(fsi_context* ctx, const fsi_state* st, const double* const* bar_in,
+                                  double* const* grad_out, double* d_shear_modulus);
+fsi_status fsi_adjoint_primal_tendencies(fsi_context* ctx, const fsi_state* st, double* const* out);
+/* The checkpoint schedule of a run (host only, no device): forward steps
+ * recomputed, most snapshots held, repetition number, and optionally the
+ * actions (kind, step, slot, count; kinds 0 advance, 1 store, 2 restore,
+ * 3 record, 4 adjoint, 5 free). */
+fsi_status fsi_adjoint_plan(int32_t n_steps, int32_t snapshots, int64_t* advances, int32_t* most_held,
+                            int32_t* repetitions, int32_t* actions, int64_t max_actions, int64_t* n_actions);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,7 @@ def load():
         "fsio_last_error": (c_char_p, [P]),
         "fsio_integrate_step": (c_int, [P] + [c_void_p] * 8 + [c_double, c_double, POINTER(_abi.StepReportC),
                                                                POINTER(_abi.StepStatsC)]),
+        "fsio_eval_tendencies": (c_int, [P] + [c_void_p] * 6 + [c_double] + [c_void_p] * 8),
         "fsio_tangent_set_shear_modulus": (None, [c_double, _dp]),
         "fsio_tangent_ndir": (c_int, []),
     }
@@ -115,6 +116,21 @@ class Tangent:
         d[:len(derivs)] = derivs
         self._gd = d
         self.lib.fsio_tangent_set_shear_modulus(self.shear_modulus, d.ctypes.data_as(_dp))
+
+    def tendencies(self, s: dict, t=0.0) -> dict:
+        """Stage tendencies (E_u, E_v, I_u, I_v, E_phi, E_sigma_*) of the dual
+        state s (sigma after the safety reset, as the oracle's entry point
+        applies it), as dual fields."""
+        nx, ny = self.nx, self.ny
+        nd = ndir() + 1
+        shapes = dict(E_u=(ny, nx + 1), E_v=(ny + 1, nx), I_u=(ny, nx + 1), I_v=(ny + 1, nx), E_phi=(ny, nx),
+                      E_sigma_xx=(ny, nx), E_sigma_yy=(ny, nx), E_sigma_xy=(ny, nx))
+        out = {k: np.zeros(v + (nd,)) for k, v in shapes.items()}
+        args = [s[k].ctypes.data for k in ("U", "V", "phi", "sigma_xx", "sigma_yy", "sigma_xy")]
+        st = self.lib.fsio_eval_tendencies(self.h, *args, float(t), *(out[k].ctypes.data for k in shapes))
+        if st:
+            raise OracleError(st, self.lib.fsio_last_error(self.h).decode())
+        return out
 
     def integrate_step(self, s: dict, t, dt):
         """s: dict of dual fields (pack) for FIELDS, updated in place."""
