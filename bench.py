@@ -167,6 +167,81 @@ def mixed_filament(fsi, setups, nsteps, window):
             "_states": None}
 
 
+def adjoint_leg(fsi, setups, N, nsteps, snapshots, fd_check=True):
+    """The differentiation benchmark: lid-driven cavity, ACDI, FP64, tight
+    solves, `nsteps` steps of dt = 1e-3 * 128 / N (the largest step of the
+    cavity benchmarks that keeps ACDI's explicit diffusion stable at N = 256:
+    dt = 1e-3 there gives 8 eps Gamma dt / h^2 = 3.1, beyond the RK3
+    stability interval, and the run fails with a non-SPD sigma);
+    J = |x_c(phi(T)) - x*|^2 with x*
+    drawn from seed 5; dJ/d(shear modulus, disk radius, disk centre) by the
+    device adjoint with binomial checkpointing (`snapshots` states), and once
+    with every state kept; the plain forward run timed beside it; central
+    differences (h = 1e-6) of the device's forward runs as the check."""
+    from eulerianfsi_amd import adjoint as A
+
+    rng = setups.SplitMix64(5)
+    target = (rng.uniform(0.4, 0.6), rng.uniform(0.4, 0.6))
+    P0 = (0.1, 0.2, 0.6, 0.5)
+    dt = 1e-3 * 128 / N
+
+    def case_of(p):
+        G, r, cx, cy = p
+        return setups.lid_driven_cavity(N, "acdi", radius=r, center=(cx, cy), shear_modulus=G, dt=dt,
+                                        velocity_method="direct", projection_rtol=1e-13)
+
+    def forward_J(p):
+        c = case_of(p)
+        ctx = fsi.build_imex_context(c.metrics, c.bcs, c.models, c.numerics)
+        work = fsi.allocate_imex_workspace(ctx)
+        t0 = time.perf_counter()
+        for k in range(nsteps):
+            fsi.integrate_step(c.state, k * c.dt, c.dt, ctx, work)
+        _ = c.state.phi
+        el = time.perf_counter() - t0
+        J, _ = A.centroid_objective(c.state, ctx, target)
+        return J, el
+
+    J0, t_fwd = forward_J(P0)
+    print(f"adjoint leg: forward run {nsteps} steps at N={N}: {t_fwd:.3f} s, J = {J0:.6e}", file=sys.stderr,
+          flush=True)
+    runs = {}
+    for snaps in (snapshots, nsteps):
+        out = A.disk_gradient(case_of(P0), nsteps, target, snapshots=snaps)
+        st = out["stats"]
+        el = out["forward_s"] + out["backward_s"]
+        runs[snaps] = dict(grad=out["grad"].tolist(), seconds=el, forward_s=out["forward_s"],
+                           backward_s=out["backward_s"], grads_per_s=1.0 / el, cost_vs_forward=el / t_fwd,
+                           recomputed_steps=st.advances, taped_steps=st.recordings, snapshots_held=st.snapshots_used,
+                           repetitions=st.repetitions, adjoint_device_bytes=st.device_bytes, J=out["J"])
+        print(f"adjoint leg: {snaps} snapshots: {el:.3f} s ({el / t_fwd:.2f}x the forward run), gradient "
+              f"{out['grad']}, recomputed {st.advances}, {st.device_bytes / 2**20:.0f} MiB", file=sys.stderr,
+              flush=True)
+        del out
+    res = {"workload": f"lid-driven cavity {N}x{N}, ACDI, FP64, G = 0.1, disk r = 0.2 at (0.6, 0.5), dt = {dt:g}, "
+                       f"{nsteps} steps (T = {nsteps * dt:g}); J = |x_c(phi(T)) - x*|^2, x* = "
+                       f"({target[0]:.4f}, {target[1]:.4f}) from seed 5; dJ/d(G, r, cx, cy); velocity :direct "
+                       f"(CG to 1e-14), projection rtol 1e-13",
+           "J": J0, "forward_run_s": t_fwd, "binomial": runs[snapshots], "store_all": runs[nsteps],
+           "snapshots": snapshots}
+    if fd_check:
+        h = 1e-6
+        g = np.array(runs[snapshots]["grad"])
+        fd = []
+        for k in range(4):
+            pp, pm = list(P0), list(P0)
+            pp[k] += h
+            pm[k] -= h
+            fd.append((forward_J(pp)[0] - forward_J(pm)[0]) / (2 * h))
+            print(f"adjoint leg: central difference {k}: {fd[-1]:.10e} vs adjoint {g[k]:.10e}", file=sys.stderr,
+                  flush=True)
+        fd = np.array(fd)
+        res["central_differences"] = fd.tolist()
+        res["rel_diff_vs_central_differences"] = (np.abs(fd - g) / np.abs(g)).tolist()
+        res["fd_step"] = h
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,6 +264,10 @@ def main(argv=None):
                     help="timed steps of the 64-cavity ensemble, stepped one by one and batched (0: skip)")
     ap.add_argument("--ensemble-warmup", type=int, default=4)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="precision of the main run")
+    ap.add_argument("--adjoint-steps", type=int, default=2000, help="steps of the differentiation run (0: skip)")
+    ap.add_argument("--adjoint-N", type=int, default=256)
+    ap.add_argument("--adjoint-snapshots", type=int, default=45)
+    ap.add_argument("--no-adjoint-fd", action="store_true", help="skip the central-difference check")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
@@ -341,6 +420,11 @@ def main(argv=None):
               file=sys.stderr, flush=True)
         del ework, ectx, states
 
+    adj = None
+    if args.adjoint_steps > 0 and args.precision == "fp64":
+        adj = adjoint_leg(fsi, setups, args.adjoint_N, args.adjoint_steps, args.adjoint_snapshots,
+                          fd_check=not args.no_adjoint_fd)
+
     cpu = None
     if args.cpu_steps > 0 or args.cpu_omp_steps > 0:
         from oracle.oracle import OMP_LIB_PATH, Oracle, state_dict
@@ -385,7 +469,8 @@ def main(argv=None):
                    "velocity_iterations": vel_mean.tolist(), "projection_iterations": proj_mean.tolist(),
                    "planned_steps": f"{sum(planned)}/{len(planned)}",
                    "precision": args.precision, "mixed_precision": mixed, "ensemble": ensemble,
-                   "adjoint_grads_per_s": None},
+                   "adjoint_grads_per_s": adj["binomial"]["grads_per_s"] if adj else None,
+                   "adjoint": adj},
         "roofline": roofline, "whole_step": whole_step, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
