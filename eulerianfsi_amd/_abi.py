@@ -245,7 +245,8 @@ class DiagnosticsC(ctypes.Structure):
 # ---- discrete adjoint (fsi_api.h) ----------------------------------------------
 class AdjointStatsC(ctypes.Structure):
     _fields_ = [("steps", c_int64), ("advances", c_int64), ("recordings", c_int64), ("adjoint_steps", c_int64),
-                ("snapshots_used", c_int32), ("repetitions", c_int32), ("device_bytes", c_double)]
+                ("snapshots_used", c_int32), ("repetitions", c_int32), ("device_bytes", c_double),
+                ("replanned", c_int64), ("last_shortfall", c_int32), ("solve_iterations", c_int32 * 3)]
 
 
 # ---- library -----------------------------------------------------------------

@@ -52,6 +52,9 @@ class AdjointStats:
     snapshots_used: int
     repetitions: int
     device_bytes: float
+    replanned: int = 0
+    last_shortfall: int = 0
+    solve_iterations: tuple = (0, 0, 0)
 
 
 class StepAdjoint:
@@ -115,7 +118,7 @@ class StepAdjoint:
         s = _abi.AdjointStatsC()
         self._check(self.lib.fsi_adjoint_get_stats(self.handle, ctypes.byref(s)), "fsi_adjoint_get_stats")
         return AdjointStats(s.steps, s.advances, s.recordings, s.adjoint_steps, s.snapshots_used, s.repetitions,
-                            s.device_bytes)
+                            s.device_bytes, s.replanned, s.last_shortfall, tuple(s.solve_iterations))
 
 
 def rrule_integrate_step(state: FSIState, t: float, dt: float, ctx: IMEXContext, work: IMEXWorkspace):

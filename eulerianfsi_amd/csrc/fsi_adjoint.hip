@@ -155,24 +155,24 @@ __device__ __forceinline__ Dual<K> g_pow(const Dual<K>& a, double e) {
     const double pw = pow(a.v, e);
     return chain(a, pw, (e == 0.0 || a.v == 0.0) ? 0.0 : e * pow(a.v, e - 1.0));
 }
-// the value with derivative components added to component c
+// the value with derivative 1 in component c (none past the type's components)
 template <class T>
-__device__ __forceinline__ T seeded(double x, int c);
+struct Seed;
 template <>
-__device__ __forceinline__ double seeded<double>(double x, int) {
-    return x;
-}
-template <>
-__device__ __forceinline__ Dual<1> seeded<Dual<1>>(double x, int c) {
-    Dual<1> r(x);
-    r.d[0] = c == 0 ? 1.0 : 0.0;
-    return r;
-}
-template <>
-__device__ __forceinline__ Dual<4> seeded<Dual<4>>(double x, int c) {
-    Dual<4> r(x);
-    r.d[c & 3] = 1.0;
-    return r;
+struct Seed<double> {
+    __device__ __forceinline__ static double at(double x, int) { return x; }
+};
+template <int K>
+struct Seed<Dual<K>> {
+    __device__ __forceinline__ static Dual<K> at(double x, int c) {
+        Dual<K> r(x);
+        if (c >= 0 && c < K) r.d[c] = 1.0;
+        return r;
+    }
+};
+template <class T>
+__device__ __forceinline__ T seeded(double x, int c) {
+    return Seed<T>::at(x, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -247,21 +247,21 @@ __device__ __forceinline__ T a_v_corner(const Params& p, const FV& V, int i, int
 template <class T, class FU, class FV>
 __device__ __forceinline__ T a_conv_u(const Params& p, const FU& U, const FV& V, int i, int j) {
     if (i == 0) return T(0.0);
-    auto u2c = [&](int k) -> T {
+    auto u2c = [&](int k) __attribute__((always_inline)) -> T {
         const T uc = 0.5 * U(k, j) + 0.5 * U(k + 1, j);
         return uc * uc;
     };
-    auto uv = [&](int jc) -> T { return a_u_corner<T>(p, U, i, jc) * a_v_corner<T>(p, V, i, jc); };
+    auto uv = [&](int jc) __attribute__((always_inline)) -> T { return a_u_corner<T>(p, U, i, jc) * a_v_corner<T>(p, V, i, jc); };
     return (u2c(i) - u2c(i - 1)) * p.ihx + (uv(j + 1) - uv(j)) * p.ihy;
 }
 template <class T, class FU, class FV>
 __device__ __forceinline__ T a_conv_v(const Params& p, const FU& U, const FV& V, int i, int j) {
     if (j == 0) return T(0.0);
-    auto v2c = [&](int k) -> T {
+    auto v2c = [&](int k) __attribute__((always_inline)) -> T {
         const T vc = 0.5 * V(i, k) + 0.5 * V(i, k + 1);
         return vc * vc;
     };
-    auto uv = [&](int ic) -> T { return a_u_corner<T>(p, U, ic, j) * a_v_corner<T>(p, V, ic, j); };
+    auto uv = [&](int ic) __attribute__((always_inline)) -> T { return a_u_corner<T>(p, U, ic, j) * a_v_corner<T>(p, V, ic, j); };
     return (uv(i + 1) - uv(i)) * p.ihx + (v2c(j) - v2c(j - 1)) * p.ihy;
 }
 // symmetric_stress_divergence at x-face (i, j) / y-face (i, j) with the
@@ -276,7 +276,7 @@ __device__ __forceinline__ T a_elastic_fx(const Params& p, const FT& Txx, const 
     } else {
         fx = (Txx(i, j) - Txx(i - 1, j)) * p.ihx;
     }
-    auto txy = [&](int jj) -> T {
+    auto txy = [&](int jj) __attribute__((always_inline)) -> T {
         if (i == 0) return Txy(0, jj) - (Txy(1, jj) - Txy(0, jj)) * 0.5;
         return 0.5 * Txy(i - 1, jj) + 0.5 * Txy(i, jj);
     };
@@ -303,7 +303,7 @@ __device__ __forceinline__ T a_elastic_fy(const Params& p, const FT& Tyy, const 
     } else {
         fy = (Tyy(i, j) - Tyy(i, j - 1)) * p.ihy;
     }
-    auto txy = [&](int ii) -> T {
+    auto txy = [&](int ii) __attribute__((always_inline)) -> T {
         if (j == 0) return Txy(ii, 0) - (Txy(ii, 1) - Txy(ii, 0)) * 0.5;
         return 0.5 * Txy(ii, j - 1) + 0.5 * Txy(ii, j);
     };
@@ -393,14 +393,14 @@ __device__ __forceinline__ T a_grad_x(const Params& p, const BcSide& l, const Bc
     if (k == 0) {
         if (l.kind == FSI_BC_DIRICHLET) {
             const double dist = p.hx / 2.0;
-            return (1.0 / dist) * f(0, j) + (-l.value / dist);
+            return (1.0 / dist) * f(k, j) + (-l.value / dist);  // k = 0
         }
         return T(-l.value);
     }
     if (k == p.nx) {
         if (r.kind == FSI_BC_DIRICHLET) {
             const double dist = p.hx / 2.0;
-            return (-1.0 / dist) * f(p.nx - 1, j) + r.value / dist;
+            return (-1.0 / dist) * f(k - 1, j) + r.value / dist;  // k = nx
         }
         return T(r.value);
     }
@@ -411,14 +411,14 @@ __device__ __forceinline__ T a_grad_y(const Params& p, const BcSide& b, const Bc
     if (k == 0) {
         if (b.kind == FSI_BC_DIRICHLET) {
             const double dist = p.hy / 2.0;
-            return (1.0 / dist) * f(i, 0) + (-b.value / dist);
+            return (1.0 / dist) * f(i, k) + (-b.value / dist);  // k = 0
         }
         return T(-b.value);
     }
     if (k == p.ny) {
         if (t.kind == FSI_BC_DIRICHLET) {
             const double dist = p.hy / 2.0;
-            return (-1.0 / dist) * f(i, p.ny - 1) + t.value / dist;
+            return (-1.0 / dist) * f(i, k - 1) + t.value / dist;  // k = ny
         }
         return T(t.value);
     }
@@ -444,9 +444,9 @@ __device__ __forceinline__ T a_acdi(const Params& p, const BcSet& bpsi, const FU
             const bool use = (di < 0 ? -di : di) + (dj < 0 ? -dj : dj) <= 2;
             ps[dj + 2][di + 2] = (use && m >= 0 && m < p.nx && n >= 0 && n < p.ny) ? a_psi(p, phi(m, n)) : T(0.0);
         }
-    auto PS = [&](int m, int n) -> T { return ps[n - j + 2][m - i + 2]; };
+    auto PS = [&](int m, int n) __attribute__((always_inline)) -> T { return ps[n - j + 2][m - i + 2]; };
     // the normal at cell (i + di, j + dj), |di| + |dj| <= 1 (_center_normals :233-244)
-    auto normal = [&](int di, int dj, T& nxv, T& nyv) {
+    auto normal = [&](int di, int dj, T& nxv, T& nyv) __attribute__((always_inline)) {
         const int m = i + di, n = j + dj;
         const T gx = 0.5 * a_grad_x<T>(p, bpsi.l, bpsi.r, PS, m, n) + 0.5 * a_grad_x<T>(p, bpsi.l, bpsi.r, PS, m + 1, n);
         const T gy = 0.5 * a_grad_y<T>(p, bpsi.b, bpsi.t, PS, m, n) + 0.5 * a_grad_y<T>(p, bpsi.b, bpsi.t, PS, m, n + 1);
@@ -462,24 +462,24 @@ __device__ __forceinline__ T a_acdi(const Params& p, const BcSet& bpsi, const FU
     if (j < p.ny - 1) normal(0, 1, n0n, n1n);
     const double eps = p.acdi_eps, gam = p.acdi_gamma;
     // convective fluxes (_acdi_convective_fluxes :531-574)
-    auto fx = [&](int k) -> T {
+    auto fx = [&](int k) __attribute__((always_inline)) -> T {
         if (k == 0) return U(0, j) * a_phase_bvalue(p.bPhi.l, phi(0, j));
         if (k == p.nx) return U(p.nx, j) * a_phase_bvalue(p.bPhi.r, phi(p.nx - 1, j));
         return U(k, j) * (0.5 * phi(k - 1, j) + 0.5 * phi(k, j));
     };
-    auto fy = [&](int k) -> T {
+    auto fy = [&](int k) __attribute__((always_inline)) -> T {
         if (k == 0) return V(i, 0) * a_phase_bvalue(p.bPhi.b, phi(i, 0));
         if (k == p.ny) return V(i, p.ny) * a_phase_bvalue(p.bPhi.t, phi(i, p.ny - 1));
         return V(i, k) * (0.5 * phi(i, k - 1) + 0.5 * phi(i, k));
     };
     // regularisation fluxes (_acdi_regularization_fluxes :598-650), zero on walls
-    auto ax = [&](int k, const T& nl, const T& nr) -> T {
+    auto ax = [&](int k, const T& nl, const T& nr) __attribute__((always_inline)) -> T {
         if (k == 0 || k == p.nx) return T(0.0);
         const T t = g_tanh((0.5 * PS(k - 1, j) + 0.5 * PS(k, j)) / (2.0 * eps));
         const T amp = 0.25 * (1.0 - t * t);
         return gam * (eps * a_grad_x<T>(p, p.bPhi.l, p.bPhi.r, phi, k, j) - amp * (0.5 * nl + 0.5 * nr));
     };
-    auto ay = [&](int k, const T& nb, const T& nt) -> T {
+    auto ay = [&](int k, const T& nb, const T& nt) __attribute__((always_inline)) -> T {
         if (k == 0 || k == p.ny) return T(0.0);
         const T t = g_tanh((0.5 * PS(i, k - 1) + 0.5 * PS(i, k)) / (2.0 * eps));
         const T amp = 0.25 * (1.0 - t * t);
@@ -544,11 +544,11 @@ __device__ __forceinline__ bool a_sigma_log(const T& a, const T& d, const T& b, 
     oxy = alpha * b;
     return true;
 }
-// E_sigma at cell (i, j): upwind transport, stretching, relaxation and the
-// geometric extension (structure_tensor_rhs :377-463)
+// The transport part of E_sigma at cell (i, j): upwind advection and
+// stretching (structure_tensor_rhs :377-463 without its pointwise terms)
 template <class T, class FS, class FU, class FV>
-__device__ __forceinline__ void a_sigma_rhs(const Params& p, const FS& S, const FU& U, const FV& V, const T& phic,
-                                            int i, int j, T& oxx, T& oyy, T& oxy) {
+__device__ __forceinline__ void a_sigma_transport(const Params& p, const FS& S, const FU& U, const FV& V, int i,
+                                                  int j, T& oxx, T& oyy, T& oxy) {
     const SymT<T> C = S(i, j);
     SymT<T> fl, fr, fb, ft;
     {
@@ -569,8 +569,8 @@ __device__ __forceinline__ void a_sigma_rhs(const Params& p, const FS& S, const 
     const T uL = U(i, j), uR = U(i + 1, j), vB = V(i, j), vT = V(i, j + 1);
     const T dv = (uR - uL) * p.ihx + (vT - vB) * p.ihy;
     const T ux = (uR - uL) * p.ihx, vy = (vT - vB) * p.ihy;
-    auto Uc = [&](int jj) -> T { return 0.5 * U(i, jj) + 0.5 * U(i + 1, jj); };
-    auto Vc = [&](int ii) -> T { return 0.5 * V(ii, j) + 0.5 * V(ii, j + 1); };
+    auto Uc = [&](int jj) __attribute__((always_inline)) -> T { return 0.5 * U(i, jj) + 0.5 * U(i + 1, jj); };
+    auto Vc = [&](int ii) __attribute__((always_inline)) -> T { return 0.5 * V(ii, j) + 0.5 * V(ii, j + 1); };
     T uy, vx;
     if (j == 0) uy = p.bU.b.kind == FSI_BC_DIRICHLET ? (Uc(0) - p.bU.b.value) * (2.0 * p.ihy) : T(-p.bU.b.value);
     else if (j == p.ny - 1)
@@ -584,27 +584,46 @@ __device__ __forceinline__ void a_sigma_rhs(const Params& p, const FS& S, const 
     const T dxx = 2.0 * (ux * sa + uy * sb);
     const T dyy = 2.0 * (vx * sb + vy * sd);
     const T dxy = vx * sa + uy * sd + (ux + vy) * sb;
-    T rxx(0.0), ryy(0.0), rxy(0.0);
+    oxx = (axx + sa * dv) + dxx;
+    oyy = (ayy + sd * dv) + dyy;
+    oxy = (axy + sb * dv) + dxy;
+}
+// The pointwise part of E_sigma: relaxation and the geometric extension of
+// sigma at the cell, with phi at the cell
+template <class T>
+__device__ __forceinline__ void a_sigma_local(const Params& p, const SymT<T>& C, const T& phic, T& oxx, T& oyy,
+                                              T& oxy) {
+    const T sa = C.xx, sd = C.yy, sb = C.xy;
+    oxx = T(0.0);
+    oyy = T(0.0);
+    oxy = T(0.0);
     if (p.relax) {
-        rxx = -p.relax_rate * (sa - 1.0);
-        ryy = -p.relax_rate * (sd - 1.0);
-        rxy = -p.relax_rate * sb;
+        oxx = -p.relax_rate * (sa - 1.0);
+        oyy = -p.relax_rate * (sd - 1.0);
+        oxy = -p.relax_rate * sb;
     }
-    T exx(0.0), eyy(0.0), exy(0.0);
     if (p.ext == FSI_EXT_GEOMETRIC) {
         const T kappa = p.ext_rate * a_pow(1.0 - a_clamp01(phic), p.ext_exp);
         if (val(kappa) != 0.0) {
             T gx, gy, gxy;
             if (a_sigma_log(sa, sd, sb, gx, gy, gxy)) {
-                exx = -kappa * gx;
-                eyy = -kappa * gy;
-                exy = -kappa * gxy;
+                oxx = oxx + -kappa * gx;
+                oyy = oyy + -kappa * gy;
+                oxy = oxy + -kappa * gxy;
             }
         }
     }
-    oxx = (axx + sa * dv) + dxx + rxx + exx;
-    oyy = (ayy + sd * dv) + dyy + ryy + eyy;
-    oxy = (axy + sb * dv) + dxy + rxy + exy;
+}
+// E_sigma at cell (i, j): both parts
+template <class T, class FS, class FU, class FV>
+__device__ __forceinline__ void a_sigma_rhs(const Params& p, const FS& S, const FU& U, const FV& V, const T& phic,
+                                            int i, int j, T& oxx, T& oyy, T& oxy) {
+    T txx, tyy, txy, lxx, lyy, lxy;
+    a_sigma_transport<T>(p, S, U, V, i, j, txx, tyy, txy);
+    a_sigma_local<T>(p, S(i, j), phic, lxx, lyy, lxy);
+    oxx = txx + lxx;
+    oyy = tyy + lyy;
+    oxy = txy + lxy;
 }
 
 __device__ __forceinline__ double rd(const double* a, size_t k) { return a ? a[k] : 0.0; }
@@ -652,39 +671,60 @@ __global__ void __launch_bounds__(BLOCK) k_adj_primal(Params p, BcSet bpsi, AdjS
 }
 
 // Cells: the derivatives with respect to phi, sigma_xx, sigma_yy, sigma_xy
-// at cell (i, j) (four dual components) of E_sigma on the five-point
-// stencil, of the elastic part of E_u and E_v on the faces around it;
-// E_phi is gathered by k_adj_phase.
+// at cell (i, j): the transport part of E_sigma on the five-point stencil
+// (sigma only, three dual components), its pointwise part (relaxation and
+// extension) at the cell (four), and the elastic part of E_u and E_v on the
+// faces around the cell (four); E_phi is gathered by k_adj_phase.
 __global__ void __launch_bounds__(BLOCK) k_adj_cells(Params p, AdjStage s, AdjBar b, AdjGrads out) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j >= p.ny) return;
-    using T = Dual<4>;
-    const Acc<T> U{s.U, p.P, -1, -1, 0}, V{s.V, p.P, -1, -1, 0}, ph{s.phi, p.P, i, j, 0};
-    const SigAcc<T> S{{s.sxx, p.P, i, j, 1}, {s.syy, p.P, i, j, 2}, {s.sxy, p.P, i, j, 3}, s.phi, p.P, s.reset,
-                      p.ext_cutoff};
-    double g[4] = {0.0, 0.0, 0.0, 0.0};
-    // E_sigma at (i, j) and its four neighbours
-    const int ci[5] = {i, i - 1, i + 1, i, i}, cj[5] = {j, j, j, j - 1, j + 1};
+    double g[4] = {0.0, 0.0, 0.0, 0.0};  // phi, sigma_xx, sigma_yy, sigma_xy
+    const size_t k = at(p, i, j);
+    if (b.Exx) {
+        using T = Dual<3>;
+        const Acc<T> U{s.U, p.P, -1, -1, 0}, V{s.V, p.P, -1, -1, 0};
+        const SigAcc<T> S{{s.sxx, p.P, i, j, 0}, {s.syy, p.P, i, j, 1}, {s.sxy, p.P, i, j, 2}, s.phi, p.P, s.reset,
+                          p.ext_cutoff};
+        const int ci[5] = {i, i - 1, i + 1, i, i}, cj[5] = {j, j, j, j - 1, j + 1};
 #pragma unroll 1
-    for (int q = 0; q < 5; ++q) {
-        if (ci[q] < 0 || ci[q] >= p.nx || cj[q] < 0 || cj[q] >= p.ny) continue;
-        const size_t kc = at(p, ci[q], cj[q]);
-        const double wx = rd(b.Exx, kc), wy = rd(b.Eyy, kc), wz = rd(b.Exy, kc);
-        if (wx == 0.0 && wy == 0.0 && wz == 0.0) continue;
-        T oxx, oyy, oxy;
-        a_sigma_rhs<T>(p, S, U, V, ph(ci[q], cj[q]), ci[q], cj[q], oxx, oyy, oxy);
+        for (int q = 0; q < 5; ++q) {
+            if (ci[q] < 0 || ci[q] >= p.nx || cj[q] < 0 || cj[q] >= p.ny) continue;
+            const size_t kc = at(p, ci[q], cj[q]);
+            const double wx = b.Exx[kc], wy = b.Eyy[kc], wz = b.Exy[kc];
+            if (wx == 0.0 && wy == 0.0 && wz == 0.0) continue;
+            T oxx, oyy, oxy;
+            a_sigma_transport<T>(p, S, U, V, ci[q], cj[q], oxx, oyy, oxy);
 #pragma unroll
-        for (int c = 0; c < 4; ++c) g[c] += wx * oxx.d[c] + wy * oyy.d[c] + wz * oxy.d[c];
+            for (int c = 0; c < 3; ++c) g[1 + c] += wx * oxx.d[c] + wy * oyy.d[c] + wz * oxy.d[c];
+        }
+        const double wx = b.Exx[k], wy = b.Eyy[k], wz = b.Exy[k];
+        if (wx != 0.0 || wy != 0.0 || wz != 0.0) {
+            using T4 = Dual<4>;
+            const Acc<T4> ph{s.phi, p.P, i, j, 0};
+            const SigAcc<T4> S4{{s.sxx, p.P, i, j, 1}, {s.syy, p.P, i, j, 2}, {s.sxy, p.P, i, j, 3}, s.phi, p.P,
+                                s.reset, p.ext_cutoff};
+            T4 oxx, oyy, oxy;
+            a_sigma_local<T4>(p, S4(i, j), ph(i, j), oxx, oyy, oxy);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) g[c] += wx * oxx.d[c] + wy * oyy.d[c] + wz * oxy.d[c];
+        }
     }
-    // the elastic force on the faces within two of the cell
+    // the elastic force on the faces around the cell (x-face (fi, fj) reads
+    // the stress of cells fi - 1, fi (and 1 on the wall face 0) in rows
+    // fj - 1 .. fj + 1 (0, 1 and ny - 2, ny - 1 on the wall rows); the y-faces
+    // likewise): faces fi in [i - 1, i + 1], fj in [j - 1, j + 1]
     if (p.elastic && b.Eu) {
+        using T = Dual<4>;
+        const Acc<T> ph{s.phi, p.P, i, j, 0};
+        const SigAcc<T> S{{s.sxx, p.P, i, j, 1}, {s.syy, p.P, i, j, 2}, {s.sxy, p.P, i, j, 3}, s.phi, p.P, s.reset,
+                          p.ext_cutoff};
         const StressAcc<T, 0> Txx{&S, &ph, p.Gmod};
         const StressAcc<T, 1> Tyy{&S, &ph, p.Gmod};
         const StressAcc<T, 2> Txy{&S, &ph, p.Gmod};
 #pragma unroll 1
-        for (int fj = max(j - 2, 0); fj <= min(j + 2, p.ny - 1); ++fj)
+        for (int fj = max(j - 1, 0); fj <= min(j + 1, p.ny - 1); ++fj)
 #pragma unroll 1
-            for (int fi = max(i - 2, 0); fi <= min(i + 2, p.nx - 1); ++fi) {
+            for (int fi = max(i - 1, 0); fi <= min(i + 1, p.nx - 1); ++fi) {
                 const size_t kf = at(p, fi, fj);
                 const double wu = b.Eu[kf], wv = b.Ev[kf];
                 if (wu != 0.0) {
@@ -699,41 +739,142 @@ __global__ void __launch_bounds__(BLOCK) k_adj_cells(Params p, AdjStage s, AdjBa
                 }
             }
     }
-    const size_t k = at(p, i, j);
     out.phi[k] += g[0];
     out.sxx[k] += g[1];
     out.syy[k] += g[2];
     out.sxy[k] += g[3];
 }
 
-// Cells: the derivative of E_phi (ACDI) with respect to phi at (i, j), on
-// the cells within distance two (|di| + |dj| <= 2).
+// Cells: the derivative of E_phi (ACDI) with respect to phi at p = (i, j).
+// phi_p enters the convective fluxes of its four faces, the face gradients
+// and the psi amplitude of its four faces, and through psi_p the normals of
+// the five cells around it, hence the regularisation fluxes of those five
+// cells' faces.  Flux F on a face with left (lower) cell L and right (upper)
+// cell R adds -F/h to E_phi(L) and +F/h to E_phi(R) (the regularisation flux
+// A the opposite: E_phi = -div(convective) + div(A)), so
+//   dE^T bar = sum over those faces of (bar(R) - bar(L)) dF/h + (bar(L) - bar(R)) dA/h.
+// Only the derivative along phi_p is formed (a closed form per face from
+// psi on the cells within two of p and the five normals with their
+// derivatives), so the sum costs one psi per cell of that diamond.
 __global__ void __launch_bounds__(BLOCK) k_adj_phase(Params p, BcSet bpsi, AdjStage s, AdjBar b, AdjGrads out) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j >= p.ny || !b.Ephi) return;
-    using T = Dual<1>;
-    const Acc<T> U{s.U, p.P, -1, -1, 0}, V{s.V, p.P, -1, -1, 0}, ph{s.phi, p.P, i, j, 0};
-    double g = 0.0;
-#pragma unroll 1
+    const double* phi = s.phi;
+    auto PH = [&](int m, int n) __attribute__((always_inline)) { return phi[at(p, m, n)]; };
+    auto W = [&](int m, int n) __attribute__((always_inline)) -> double {  // bar E_phi, zero outside the cells
+        return (m >= 0 && m < p.nx && n >= 0 && n < p.ny) ? b.Ephi[at(p, m, n)] : 0.0;
+    };
+    // psi on the diamond |di| + |dj| <= 2 around p and its derivative at p
+    double ps[5][5];
+#pragma unroll
     for (int dj = -2; dj <= 2; ++dj)
-#pragma unroll 1
+#pragma unroll
         for (int di = -2; di <= 2; ++di) {
             const int m = i + di, n = j + dj;
-            if ((di < 0 ? -di : di) + (dj < 0 ? -dj : dj) > 2 || m < 0 || m >= p.nx || n < 0 || n >= p.ny) continue;
-            const double w = b.Ephi[at(p, m, n)];
-            if (w == 0.0) continue;
-            g += w * a_acdi<T>(p, bpsi, U, V, ph, m, n).d[0];
+            const bool use = (di < 0 ? -di : di) + (dj < 0 ? -dj : dj) <= 2;
+            ps[dj + 2][di + 2] = (use && m >= 0 && m < p.nx && n >= 0 && n < p.ny) ? a_psi(p, PH(m, n)) : 0.0;
         }
+    const double ph = PH(i, j), e = p.acdi_floor;
+    const double dpsi = (ph < 0.0 || ph > 1.0) ? 0.0 : p.acdi_eps * (1.0 / (ph + e) + 1.0 / (1.0 - ph + e));
+    using D = Dual<1>;
+    auto PS = [&](int m, int n) __attribute__((always_inline)) -> D {  // psi with its derivative along phi_p
+        D r(ps[n - j + 2][m - i + 2]);
+        if (m == i && n == j) r.d[0] = dpsi;
+        return r;
+    };
+    // the five normals (value and derivative), index 0 centre, 1 west, 2 east, 3 south, 4 north
+    D nxv[5], nyv[5];
+    const int oi[5] = {0, -1, 1, 0, 0}, oj[5] = {0, 0, 0, -1, 1};
+    bool has[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+        const int m = i + oi[q], n = j + oj[q];
+        has[q] = m >= 0 && m < p.nx && n >= 0 && n < p.ny;
+        nxv[q] = D(0.0);
+        nyv[q] = D(0.0);
+        if (!has[q]) continue;
+        const D gx = 0.5 * a_grad_x<D>(p, bpsi.l, bpsi.r, PS, m, n) + 0.5 * a_grad_x<D>(p, bpsi.l, bpsi.r, PS, m + 1, n);
+        const D gy = 0.5 * a_grad_y<D>(p, bpsi.b, bpsi.t, PS, m, n) + 0.5 * a_grad_y<D>(p, bpsi.b, bpsi.t, PS, m, n + 1);
+        const D mag = g_sqrt(gx * gx + gy * gy + p.acdi_d2);
+        nxv[q] = gx / mag;
+        nyv[q] = gy / mag;
+    }
+    // the index (0..4) among the five of the cell at offset (di, dj) from p,
+    // or -1 (a compile-time value at every call: the offsets are constants)
+    auto five = [&](int di, int dj) __attribute__((always_inline)) -> int {
+        return (di == 0 && dj == 0) ? 0 : (dj == 0 && di == -1) ? 1 : (dj == 0 && di == 1) ? 2
+             : (di == 0 && dj == -1) ? 3 : (di == 0 && dj == 1) ? 4 : -1;
+    };
+    const double eps = p.acdi_eps, gam = p.acdi_gamma;
+    double g = 0.0;
+    // one interior face between the cells at offsets L and R from p (x-faces:
+    // comp 0, y-faces: comp 1); the callers pass compile-time offsets
+    auto face = [&](int ldi, int ldj, int rdi, int rdj, int comp, double ih) __attribute__((always_inline)) {
+        const int lq = five(ldi, ldj), rq = five(rdi, rdj);
+        const bool lin = lq >= 0 && has[lq], rin = rq >= 0 && has[rq];
+        const bool adj = (ldi == 0 && ldj == 0) || (rdi == 0 && rdj == 0);
+        const double psf = 0.5 * ps[ldj + 2][ldi + 2] + 0.5 * ps[rdj + 2][rdi + 2];
+        const double t = tanh(psf / (2.0 * eps));
+        const double amp = 0.25 * (1.0 - t * t);
+        double dA = 0.0;
+        if (adj) {
+            const double dgrad = (rdi == 0 && rdj == 0) ? ih : -ih;  // ih (phi_R - phi_L)
+            const double damp = -0.25 * t * (1.0 - t * t) / eps * (0.5 * dpsi);  // d amp / d phi_p
+            const double nl = lin ? (comp == 0 ? nxv[lq].v : nyv[lq].v) : 0.0;
+            const double nr = rin ? (comp == 0 ? nxv[rq].v : nyv[rq].v) : 0.0;
+            dA += eps * dgrad - damp * (0.5 * nl + 0.5 * nr);
+        }
+        double dn = 0.0;
+        if (lin) dn += 0.5 * (comp == 0 ? nxv[lq].d[0] : nyv[lq].d[0]);
+        if (rin) dn += 0.5 * (comp == 0 ? nxv[rq].d[0] : nyv[rq].d[0]);
+        dA -= amp * dn;
+        g += gam * dA * (W(i + ldi, j + ldj) - W(i + rdi, j + rdj)) * ih;
+    };
+    // x-faces of the five cells: row j faces i-1 .. i+2, rows j-1 and j+1 faces i, i+1
+    // (face k lies between cells k - 1 and k)
+#pragma unroll
+    for (int t = -1; t <= 2; ++t)
+        if (i + t >= 1 && i + t <= p.nx - 1) face(t - 1, 0, t, 0, 0, p.ihx);
+#pragma unroll
+    for (int r = -1; r <= 1; r += 2)
+#pragma unroll
+        for (int t = 0; t <= 1; ++t)
+            if (j + r >= 0 && j + r < p.ny && i + t >= 1 && i + t <= p.nx - 1) face(t - 1, r, t, r, 0, p.ihx);
+    // y-faces: column i faces j-1 .. j+2, columns i-1 and i+1 faces j, j+1
+#pragma unroll
+    for (int t = -1; t <= 2; ++t)
+        if (j + t >= 1 && j + t <= p.ny - 1) face(0, t - 1, 0, t, 1, p.ihy);
+#pragma unroll
+    for (int c = -1; c <= 1; c += 2)
+#pragma unroll
+        for (int t = 0; t <= 1; ++t)
+            if (i + c >= 0 && i + c < p.nx && j + t >= 1 && j + t <= p.ny - 1) face(c, t - 1, c, t, 1, p.ihy);
+    // convective fluxes of the cell's four faces: F = u phi_f
+    {
+        const double wp = W(i, j);
+        // x-face i (left of p; p is its right cell) and i + 1 (right of p)
+        const double uL = s.U[at(p, i, j)], uR = s.U[at(p, i + 1, j)];
+        const double dl = i == 0 ? (p.bPhi.l.kind == FSI_BC_DIRICHLET ? 0.0 : 1.0) : 0.5;
+        const double dr = i == p.nx - 1 ? (p.bPhi.r.kind == FSI_BC_DIRICHLET ? 0.0 : 1.0) : 0.5;
+        g += (wp - W(i - 1, j)) * p.ihx * uL * dl;  // (bar(R) - bar(L)) dF / h, R = p
+        g += (W(i + 1, j) - wp) * p.ihx * uR * dr;  // L = p
+        const double vB = s.V[at(p, i, j)], vT = s.V[at(p, i, j + 1)];
+        const double db = j == 0 ? (p.bPhi.b.kind == FSI_BC_DIRICHLET ? 0.0 : 1.0) : 0.5;
+        const double dt = j == p.ny - 1 ? (p.bPhi.t.kind == FSI_BC_DIRICHLET ? 0.0 : 1.0) : 0.5;
+        g += (wp - W(i, j - 1)) * p.ihy * vB * db;
+        g += (W(i, j + 1) - wp) * p.ihy * vT * dt;
+    }
     out.phi[at(p, i, j)] += g;
 }
 
 // x-faces: the derivative with respect to U at face (i, j), 0 <= i <= nx, of
-// the convection, viscous, phase and sigma tendencies around it.
+// the convection, viscous and sigma-transport tendencies around it, and of
+// the ACDI convective flux u phi_f of the face itself (closed form).
 __global__ void __launch_bounds__(BLOCK) k_adj_uface(Params p, BcSet bpsi, AdjStage s, AdjBar b, AdjGrads out) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i > p.nx || j >= p.ny) return;
     using T = Dual<1>;
-    const Acc<T> U{s.U, p.P, i, j, 0}, V{s.V, p.P, -1, -1, 0}, ph{s.phi, p.P, -1, -1, 0};
+    const Acc<T> U{s.U, p.P, i, j, 0}, V{s.V, p.P, -1, -1, 0};
     const SigAcc<T> S{{s.sxx, p.P, -1, -1, 0}, {s.syy, p.P, -1, -1, 0}, {s.sxy, p.P, -1, -1, 0}, s.phi, p.P,
                       s.reset, p.ext_cutoff};
     double g = 0.0;
@@ -748,20 +889,28 @@ __global__ void __launch_bounds__(BLOCK) k_adj_uface(Params p, BcSet bpsi, AdjSt
             if (p.visc && wiu != 0.0) g += wiu * a_visc_u<T>(p, U, V, fi, fj, 0).d[0];
             if (p.visc && wiv != 0.0) g += wiv * a_visc_v<T>(p, U, V, fi, fj, 0).d[0];
         }
+    if (b.Exx) {
 #pragma unroll 1
-    for (int cj = max(j - 1, 0); cj <= min(j + 1, p.ny - 1); ++cj)
+        for (int cj = max(j - 1, 0); cj <= min(j + 1, p.ny - 1); ++cj)
 #pragma unroll 1
-        for (int ci = max(i - 1, 0); ci <= min(i, p.nx - 1); ++ci) {
-            const size_t kc = at(p, ci, cj);
-            const double wp = cj == j ? rd(b.Ephi, kc) : 0.0;
-            if (wp != 0.0) g += wp * a_acdi<T>(p, bpsi, U, V, ph, ci, cj).d[0];
-            const double wx = rd(b.Exx, kc), wy = rd(b.Eyy, kc), wz = rd(b.Exy, kc);
-            if (wx != 0.0 || wy != 0.0 || wz != 0.0) {
+            for (int ci = max(i - 1, 0); ci <= min(i, p.nx - 1); ++ci) {
+                const size_t kc = at(p, ci, cj);
+                const double wx = b.Exx[kc], wy = b.Eyy[kc], wz = b.Exy[kc];
+                if (wx == 0.0 && wy == 0.0 && wz == 0.0) continue;
                 T oxx, oyy, oxy;
-                a_sigma_rhs<T>(p, S, U, V, ph(ci, cj), ci, cj, oxx, oyy, oxy);
+                a_sigma_transport<T>(p, S, U, V, ci, cj, oxx, oyy, oxy);
                 g += wx * oxx.d[0] + wy * oyy.d[0] + wz * oxy.d[0];
             }
-        }
+    }
+    if (b.Ephi && p.phase == FSI_PHASE_ACDI) {  // F = U phi_f enters E_phi(L) as -F/h, E_phi(R) as +F/h
+        const double* phi = s.phi;
+        double pf;
+        if (i == 0) pf = p.bPhi.l.kind == FSI_BC_DIRICHLET ? p.bPhi.l.value : phi[at(p, 0, j)];
+        else if (i == p.nx) pf = p.bPhi.r.kind == FSI_BC_DIRICHLET ? p.bPhi.r.value : phi[at(p, p.nx - 1, j)];
+        else pf = 0.5 * phi[at(p, i - 1, j)] + 0.5 * phi[at(p, i, j)];
+        const double wl = i > 0 ? b.Ephi[at(p, i - 1, j)] : 0.0, wr = i < p.nx ? b.Ephi[at(p, i, j)] : 0.0;
+        g += (wr - wl) * p.ihx * pf;
+    }
     out.U[at(p, i, j)] += g;
 }
 
@@ -770,7 +919,7 @@ __global__ void __launch_bounds__(BLOCK) k_adj_vface(Params p, BcSet bpsi, AdjSt
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j > p.ny) return;
     using T = Dual<1>;
-    const Acc<T> U{s.U, p.P, -1, -1, 0}, V{s.V, p.P, i, j, 0}, ph{s.phi, p.P, -1, -1, 0};
+    const Acc<T> U{s.U, p.P, -1, -1, 0}, V{s.V, p.P, i, j, 0};
     const SigAcc<T> S{{s.sxx, p.P, -1, -1, 0}, {s.syy, p.P, -1, -1, 0}, {s.sxy, p.P, -1, -1, 0}, s.phi, p.P,
                       s.reset, p.ext_cutoff};
     double g = 0.0;
@@ -785,20 +934,28 @@ __global__ void __launch_bounds__(BLOCK) k_adj_vface(Params p, BcSet bpsi, AdjSt
             if (p.visc && wiu != 0.0) g += wiu * a_visc_u<T>(p, U, V, fi, fj, 0).d[0];
             if (p.visc && wiv != 0.0) g += wiv * a_visc_v<T>(p, U, V, fi, fj, 0).d[0];
         }
+    if (b.Exx) {
 #pragma unroll 1
-    for (int cj = max(j - 1, 0); cj <= min(j, p.ny - 1); ++cj)
+        for (int cj = max(j - 1, 0); cj <= min(j, p.ny - 1); ++cj)
 #pragma unroll 1
-        for (int ci = max(i - 1, 0); ci <= min(i + 1, p.nx - 1); ++ci) {
-            const size_t kc = at(p, ci, cj);
-            const double wp = ci == i ? rd(b.Ephi, kc) : 0.0;
-            if (wp != 0.0) g += wp * a_acdi<T>(p, bpsi, U, V, ph, ci, cj).d[0];
-            const double wx = rd(b.Exx, kc), wy = rd(b.Eyy, kc), wz = rd(b.Exy, kc);
-            if (wx != 0.0 || wy != 0.0 || wz != 0.0) {
+            for (int ci = max(i - 1, 0); ci <= min(i + 1, p.nx - 1); ++ci) {
+                const size_t kc = at(p, ci, cj);
+                const double wx = b.Exx[kc], wy = b.Eyy[kc], wz = b.Exy[kc];
+                if (wx == 0.0 && wy == 0.0 && wz == 0.0) continue;
                 T oxx, oyy, oxy;
-                a_sigma_rhs<T>(p, S, U, V, ph(ci, cj), ci, cj, oxx, oyy, oxy);
+                a_sigma_transport<T>(p, S, U, V, ci, cj, oxx, oyy, oxy);
                 g += wx * oxx.d[0] + wy * oyy.d[0] + wz * oxy.d[0];
             }
-        }
+    }
+    if (b.Ephi && p.phase == FSI_PHASE_ACDI) {
+        const double* phi = s.phi;
+        double pf;
+        if (j == 0) pf = p.bPhi.b.kind == FSI_BC_DIRICHLET ? p.bPhi.b.value : phi[at(p, i, 0)];
+        else if (j == p.ny) pf = p.bPhi.t.kind == FSI_BC_DIRICHLET ? p.bPhi.t.value : phi[at(p, i, p.ny - 1)];
+        else pf = 0.5 * phi[at(p, i, j - 1)] + 0.5 * phi[at(p, i, j)];
+        const double wb = j > 0 ? b.Ephi[at(p, i, j - 1)] : 0.0, wt = j < p.ny ? b.Ephi[at(p, i, j)] : 0.0;
+        g += (wt - wb) * p.ihy * pf;
+    }
     out.V[at(p, i, j)] += g;
 }
 
@@ -830,6 +987,23 @@ __global__ void __launch_bounds__(BLOCK) k_adj_shear_final(const double* partial
     if (threadIdx.x == 0 && threadIdx.y == 0) *dG += t[0];
 }
 
+// zero n values of each of the arrays (one array per block row)
+__global__ void __launch_bounds__(256) k_adj_zero(size_t n, AdjPtrs z) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k < n && blockIdx.y < (unsigned)z.n) z.p[blockIdx.y][k] = 0.0;
+}
+// dst[d] += coef[d] * src[from[d]] over n values, the sources read once; a
+// source with `masked` set counts only where phi >= cutoff (the safety
+// reset's constants)
+__global__ void __launch_bounds__(256) k_adj_combine(size_t n, AdjCombine c) {
+    const size_t k = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    double v[4];
+    const bool keep = !c.phi || !(c.phi[k] < c.cutoff);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v[q] = q < c.nsrc ? ((c.masked & (1 << q)) && !keep ? 0.0 : c.src[q][k]) : 0.0;
+    for (int d = 0; d < c.ndst; ++d) c.dst[d][k] += c.coef[d] * v[c.from[d]];
+}
 // y += a x over n values
 __global__ void __launch_bounds__(256) k_adj_axpy(size_t n, double* __restrict__ y, double a,
                                                   const double* __restrict__ x) {
@@ -883,6 +1057,12 @@ void launch_adj_tendencies(hipStream_t st, const Params& p, const BcSet& bpsi, c
         k_adj_shear<<<g, kAdjBlock, 0, st>>>(p, s, b, partials);
         k_adj_shear_final<<<1, kAdjBlock, 0, st>>>(partials, (int)(g.x * g.y), dG);
     }
+}
+void launch_adj_zero(hipStream_t st, size_t n, const AdjPtrs& z) {
+    if (z.n > 0) k_adj_zero<<<dim3((unsigned)((n + 255) / 256), (unsigned)z.n), 256, 0, st>>>(n, z);
+}
+void launch_adj_combine(hipStream_t st, size_t n, const AdjCombine& c) {
+    if (c.ndst > 0) k_adj_combine<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(n, c);
 }
 void launch_adj_axpy(hipStream_t st, size_t n, double* y, double a, const double* x) {
     if (a == 0.0) return;

@@ -157,6 +157,18 @@ std::vector<Act> plan_run(int n, int snapshots, int* most_held, int* rep) {
 
 }  // namespace
 
+// The stage fields of one step: stages 0..2 (stage 0 = the step's input) and
+// phi of stage 3 (19 fields)
+struct Tape {
+    double *U[3] = {}, *V[3] = {}, *phi[4] = {}, *sxx[3] = {}, *syy[3] = {}, *sxy[3] = {};
+    static constexpr int kFields = 19;
+    double** slot(int k) {
+        double** all[kFields] = {&U[0], &U[1], &U[2], &V[0], &V[1], &V[2], &phi[0], &phi[1], &phi[2], &phi[3],
+                                 &sxx[0], &sxx[1], &sxx[2], &syy[0], &syy[1], &syy[2], &sxy[0], &sxy[1], &sxy[2]};
+        return all[k];
+    }
+};
+
 struct fsi_adjoint {
     fsi_context* ctx = nullptr;
     fsi_workspace* ws = nullptr;  // the caller's (the forward part of a run)
@@ -165,16 +177,28 @@ struct fsi_adjoint {
     fsi_state* twin_zero = nullptr;  // x0 = 0 of its predictor solves
     fsi_state* work = nullptr;       // recomputations of the backward sweep
     fsi_workspace* work_ws = nullptr;
-    // tape of the last recorded step: stages 0..2 (stage 0 = the step's input) and phi of stage 3
-    double *tU[3] = {}, *tV[3] = {}, *tphi[4] = {}, *tsxx[3] = {}, *tsyy[3] = {}, *tsxy[3] = {};
+    // the tape of the last recorded step (`own`), or with every tape kept
+    // (snapshots = 0) the tapes of all steps of the run (`pool`); `cur` is
+    // the tape being written or transposed
+    Tape own;
+    Tape* cur = &own;
+    std::vector<Tape> pool;
+    double* pool_mem = nullptr;
+    int keep_tapes = 0;
     int taped = 0;
     double tape_dt = 0.0;
     // cotangents of the stage tendencies and of the stage fields (stage 0: the step's input)
     double *bEu[3] = {}, *bEv[3] = {}, *bIu[3] = {}, *bIv[3] = {}, *bEp[3] = {}, *bExx[3] = {}, *bEyy[3] = {},
            *bExy[3] = {};
     double *gU[3] = {}, *gV[3] = {}, *gphi[3] = {}, *gsxx[3] = {}, *gsyy[3] = {}, *gsxy[3] = {};
-    double *wU = nullptr, *wV = nullptr, *rU = nullptr, *rV = nullptr;
-    double *partials = nullptr, *dG = nullptr;
+    double *wU = nullptr, *wV = nullptr;
+    // the transposed predictor solutions of the three stages: each solve
+    // starts from the one of the step transposed before it
+    double *rU[3] = {}, *rV[3] = {};
+    double *partials = nullptr, *dG = nullptr, *dG_save = nullptr;
+    int planned = 0;           // the transposed solves run planned iteration counts
+    int plans = 0;             // plan_vel, plan_proj hold plans
+    int plan_vel[3] = {}, plan_proj[3] = {};
     std::vector<double*> owned;
     double bytes = 0.0;
     // the run
@@ -210,11 +234,11 @@ fsi_status record_step(fsi_adjoint* a, fsi_state* st, fsi_workspace* w, double t
     fsi_context* c = a->ctx;
     hipStream_t s = c->stream;
     const std::array<double*, 6> in = fields_of(st);
-    double* const t0[6] = {a->tU[0], a->tV[0], a->tphi[0], a->tsxx[0], a->tsyy[0], a->tsxy[0]};
+    double* const t0[6] = {a->cur->U[0], a->cur->V[0], a->cur->phi[0], a->cur->sxx[0], a->cur->syy[0], a->cur->sxy[0]};
     for (int k = 0; k < 6; ++k)
         if (fsi_status rc = copy_field(c, s, t0[k], in[k])) return rc;
-    w->tape_U1 = a->tU[1];
-    w->tape_V1 = a->tV[1];
+    w->tape_U1 = a->cur->U[1];
+    w->tape_V1 = a->cur->V[1];
     const int graph = c->use_graph;
     if (graph == 1) c->use_graph = 2;  // a captured step would not copy the tape
     fsi_status rc = fsi_integrate_step(c, st, w, t, dt, nullptr, nullptr);
@@ -224,8 +248,8 @@ fsi_status record_step(fsi_adjoint* a, fsi_state* st, fsi_workspace* w, double t
     if (rc != FSI_OK) return rc;
     const double* src[] = {w->Us2, w->Vs2, w->cphi[0], w->csxx[0], w->csyy[0], w->csxy[0],
                            w->cphi[1], w->csxx[1], w->csyy[1], w->csxy[1], st->phi};
-    double* dst[] = {a->tU[2], a->tV[2], a->tphi[1], a->tsxx[1], a->tsyy[1], a->tsxy[1],
-                     a->tphi[2], a->tsxx[2], a->tsyy[2], a->tsxy[2], a->tphi[3]};
+    double* dst[] = {a->cur->U[2], a->cur->V[2], a->cur->phi[1], a->cur->sxx[1], a->cur->syy[1], a->cur->sxy[1],
+                     a->cur->phi[2], a->cur->sxx[2], a->cur->syy[2], a->cur->sxy[2], a->cur->phi[3]};
     for (int k = 0; k < 11; ++k)
         if ((rc = copy_field(c, s, dst[k], src[k])) != FSI_OK) return rc;
     HIPCHECK(c, hipStreamSynchronize(s));
@@ -236,27 +260,31 @@ fsi_status record_step(fsi_adjoint* a, fsi_state* st, fsi_workspace* w, double t
 }
 
 AdjStage stage_of(const fsi_adjoint* a, int s) {
-    return AdjStage{a->tU[s], a->tV[s], a->tphi[s], a->tsxx[s], a->tsyy[s], a->tsxy[s],
+    return AdjStage{a->cur->U[s], a->cur->V[s], a->cur->phi[s], a->cur->sxx[s], a->cur->syy[s], a->cur->sxy[s],
                     s == 0 ? a->ctx->has_reset : 0};
 }
 
 // The transpose of the recorded step: `bar` holds the cotangent of the step's
 // output on entry and that of its input on return; *dG (device) gains the
-// shear-modulus part.
-fsi_status adjoint_step(fsi_adjoint* a, fsi_state* bar) {
+// shear-modulus part.  With `planned` the transposed solves run the
+// iteration counts of the last step's and nothing is read back until the
+// end; if one of them has not converged by then, *missed is set, `bar` is
+// left as it was, and the caller runs the step again with polling.
+fsi_status adjoint_step_once(fsi_adjoint* a, fsi_state* bar, bool planned, bool* missed) {
     fsi_context* c = a->ctx;
     fsi_context* tw = a->twin;
     const Params& p = c->prm;
     hipStream_t S = tw->stream;
-    const size_t n = c->felems, nb = n * sizeof(double);
+    const size_t n = c->felems;
     const double dt = a->tape_dt;
-    if (!a->taped) return fail(c, FSI_E_ARGUMENT, "fsi_adjoint_step: no recorded step to transpose.");
-    HIPCHECK(c, hipStreamSynchronize(c->stream));  // the cotangent and the tape, written on the step's stream
+    *missed = false;
+    AdjPtrs z{};
     for (int q = 0; q < 3; ++q) {
-        double* z[] = {a->bEu[q], a->bEv[q], a->bIu[q], a->bIv[q], a->bEp[q], a->bExx[q], a->bEyy[q], a->bExy[q],
+        double* f[] = {a->bEu[q], a->bEv[q], a->bIu[q], a->bIv[q], a->bEp[q], a->bExx[q], a->bEyy[q], a->bExy[q],
                        a->gU[q], a->gV[q], a->gphi[q], a->gsxx[q], a->gsyy[q], a->gsxy[q]};
-        for (double* f : z) HIPCHECK(c, hipMemsetAsync(f, 0, nb, S));
+        for (double* x : f) z.p[z.n++] = x;
     }
+    launch_adj_zero(S, n, z);
     status_reset(tw);
     if (fsi_status rc = status_push(tw)) return rc;
     const std::array<double*, 6> out = fields_of(bar);
@@ -268,6 +296,7 @@ fsi_status adjoint_step(fsi_adjoint* a, fsi_state* bar) {
     };
     fsi_status rc = FSI_OK;
     for (int s = 3; s >= 1; --s) {
+        const int k = s - 1;
         // the cotangent of stage s's fields: the output's, or gathered from its tendencies
         std::array<double*, 6> g = out;
         if (s <= 2) {
@@ -276,51 +305,126 @@ fsi_status adjoint_step(fsi_adjoint* a, fsi_state* bar) {
             g = {a->gU[s], a->gV[s], a->gphi[s], a->gsxx[s], a->gsyy[s], a->gsxy[s]};
         }
         // u_s = P K_s^-1 (rhs_s + alpha_s b): the wall faces are data
-        if ((rc = copy_field(c, S, a->wU, g[0])) || (rc = copy_field(c, S, a->wV, g[1]))) return rc;
-        launch_adj_walls(S, p, a->wU, a->wV);
-        if ((rc = projection(tw, a->twin_ws, a->wU, a->wV, a->wU, a->wV, 1, -1, 0)) != FSI_OK) return rc;
+        launch_adj_walls(S, p, g[0], g[1]);
+        const int pproj = planned ? a->plan_proj[k] : 0;
+        if ((rc = projection(tw, a->twin_ws, g[0], g[1], a->wU, a->wV, 1, k, pproj)) != FSI_OK) return rc;
         if (p.visc) {
-            if ((rc = velocity_solve(tw, a->twin_zero, a->twin_ws, dt * AI[s][s], s - 1, 0, a->wU, a->wV, a->rU,
-                                     a->rV)) != FSI_OK)
+            const int pvel = planned ? a->plan_vel[k] : 0;
+            if ((rc = velocity_solve(tw, a->twin_zero, a->twin_ws, dt * AI[s][s], k, pvel, a->wU, a->wV, a->rU[k],
+                                     a->rV[k], true)) != FSI_OK)
                 return rc;
         } else {
-            if ((rc = copy_field(c, S, a->rU, a->wU)) || (rc = copy_field(c, S, a->rV, a->wV))) return rc;
-            launch_adj_walls(S, p, a->rU, a->rV);  // the Dirichlet rows take the wall data
+            if ((rc = copy_field(c, S, a->rU[k], a->wU)) || (rc = copy_field(c, S, a->rV[k], a->wV))) return rc;
         }
-        launch_adj_walls(S, p, a->rU, a->rV);  // faces outside the compact predictor
-        launch_adj_axpy(S, n, a->gU[0], 1.0, a->rU);
-        launch_adj_axpy(S, n, a->gV[0], 1.0, a->rV);
+        launch_adj_walls(S, p, a->rU[k], a->rV[k]);  // the Dirichlet rows and the faces outside the compact predictor
+        // rhs_s = u^n + dt sum_q (aE E_q + aI I_q)
+        AdjCombine cv{};
+        cv.src[0] = a->rU[k];
+        cv.src[1] = a->rV[k];
+        cv.nsrc = 2;
+        auto add = [&](AdjCombine& cb, double* dst, double cf, int from) {
+            if (cf == 0.0) return;
+            cb.dst[cb.ndst] = dst;
+            cb.coef[cb.ndst] = cf;
+            cb.from[cb.ndst] = from;
+            cb.ndst++;
+        };
+        add(cv, a->gU[0], 1.0, 0);
+        add(cv, a->gV[0], 1.0, 1);
         for (int q = 0; q < s; ++q) {
-            launch_adj_axpy(S, n, a->bEu[q], dt * AE[s][q], a->rU);
-            launch_adj_axpy(S, n, a->bEv[q], dt * AE[s][q], a->rV);
-            launch_adj_axpy(S, n, a->bIu[q], dt * AI[s][q], a->rU);
-            launch_adj_axpy(S, n, a->bIv[q], dt * AI[s][q], a->rV);
+            add(cv, a->bEu[q], dt * AE[s][q], 0);
+            add(cv, a->bEv[q], dt * AE[s][q], 1);
+            add(cv, a->bIu[q], dt * AI[s][q], 0);
+            add(cv, a->bIv[q], dt * AI[s][q], 1);
         }
-        // phi_s, sigma_s: the reset cells of stage s are constants
-        if (c->has_reset) launch_adj_mask(S, p, a->tphi[s], g[3], g[4], g[5]);
-        for (int k = 2; k < 6; ++k) {
-            double* acc0[] = {nullptr, nullptr, a->gphi[0], a->gsxx[0], a->gsyy[0], a->gsxy[0]};
-            launch_adj_axpy(S, n, acc0[k], 1.0, g[k]);
-        }
+        launch_adj_combine(S, n, cv);
+        // phi_s, sigma_s = reset(phi^n + dt sum_q aE E_phi,q, sigma^n~ + ...): the reset cells are constants
+        AdjCombine cc{};
+        for (int q = 0; q < 4; ++q) cc.src[q] = g[2 + q];
+        cc.nsrc = 4;
+        cc.phi = c->has_reset ? a->cur->phi[s] : nullptr;
+        cc.cutoff = p.ext_cutoff;
+        cc.masked = 2 | 4 | 8;
+        double* acc0[4] = {a->gphi[0], a->gsxx[0], a->gsyy[0], a->gsxy[0]};
+        for (int q = 0; q < 4; ++q) add(cc, acc0[q], 1.0, q);
         for (int q = 0; q < s; ++q) {
             const double cf = dt * AE[s][q];
-            launch_adj_axpy(S, n, a->bEp[q], cf, g[2]);
-            launch_adj_axpy(S, n, a->bExx[q], cf, g[3]);
-            launch_adj_axpy(S, n, a->bEyy[q], cf, g[4]);
-            launch_adj_axpy(S, n, a->bExy[q], cf, g[5]);
+            add(cc, a->bEp[q], cf, 0);
+            add(cc, a->bExx[q], cf, 1);
+            add(cc, a->bEyy[q], cf, 2);
+            add(cc, a->bExy[q], cf, 3);
         }
+        launch_adj_combine(S, n, cc);
     }
     // the stage-1 tendencies of the input (sigma^n with the safety reset)
     launch_adj_tendencies(S, p, c->bpsi, stage_of(a, 0), bar_of(0), grads_of(0), a->partials,
                           p.elastic ? a->dG : nullptr);
-    if (c->has_reset) launch_adj_mask(S, p, a->tphi[0], a->gsxx[0], a->gsyy[0], a->gsxy[0]);
+    if (c->has_reset) launch_adj_mask(S, p, a->cur->phi[0], a->gsxx[0], a->gsyy[0], a->gsxy[0]);
     launch_adj_walls(S, p, a->gU[0], a->gV[0]);
+    HIPCHECK(c, hipGetLastError());
+    if ((rc = status_pull(tw)) != FSI_OK) return rc;
+    if (planned) {
+        const Status& st = *tw->st_host;
+        int mask = 0;
+        for (int k = 0; k < 3; ++k) {
+            if (p.visc && !st.vel_ok[k]) mask |= 1 << k;
+            if (!st.proj_ok[k]) mask |= 8 << k;
+        }
+        if (mask) {  // those solves get more iterations from now on
+            *missed = true;
+            a->stats.last_shortfall = mask;
+            for (int k = 0; k < 3; ++k) {
+                if (mask & (1 << k)) a->plan_vel[k] += 2 + a->plan_vel[k] / 4;
+                if (mask & (8 << k)) a->plan_proj[k] += 2 + a->plan_proj[k] / 4;
+            }
+            return FSI_OK;
+        }
+    }
+    if ((rc = device_error(tw)) != FSI_OK) return fail(c, rc, "adjoint solve: %s", fsi_last_error(tw));
+    {
+        const Status& st = *tw->st_host;
+        // the plans: from the first (polled) step; after a planned step they
+        // shrink by one iteration at a time towards what the solves took (a
+        // polled rerun after a shortfall teaches nothing: it starts from the
+        // shortfall's iterate)
+        for (int k = 0; k < 3; ++k) {
+            const int v = std::max(1, st.vel_iters[k]) + 1, q = std::max(0, st.proj_iters[k]) + 1;
+            if (!a->plans) {
+                a->plan_vel[k] = v;
+                a->plan_proj[k] = q;
+            } else if (planned) {
+                a->plan_vel[k] = std::max(v, a->plan_vel[k] - 1);
+                a->plan_proj[k] = std::max(q, a->plan_proj[k] - 1);
+            }
+            a->stats.solve_iterations[k] = st.vel_iters[k];
+        }
+        a->plans = 1;
+    }
     double* const g0[6] = {a->gU[0], a->gV[0], a->gphi[0], a->gsxx[0], a->gsyy[0], a->gsxy[0]};
     for (int k = 0; k < 6; ++k)
         if ((rc = copy_field(c, S, out[k], g0[k])) != FSI_OK) return rc;
-    HIPCHECK(c, hipGetLastError());
-    if ((rc = status_pull(tw)) != FSI_OK) return rc;
-    if ((rc = device_error(tw)) != FSI_OK) return fail(c, rc, "adjoint solve: %s", fsi_last_error(tw));
+    HIPCHECK(c, hipStreamSynchronize(S));
+    return FSI_OK;
+}
+
+fsi_status adjoint_step(fsi_adjoint* a, fsi_state* bar) {
+    fsi_context* c = a->ctx;
+    if (!a->taped) return fail(c, FSI_E_ARGUMENT, "fsi_adjoint_step: no recorded step to transpose.");
+    HIPCHECK(c, hipStreamSynchronize(c->stream));  // the cotangent and the tape, written on the step's stream
+    bool missed = false;
+    fsi_status rc = FSI_OK;
+    if (a->planned) {
+        HIPCHECK(c, hipMemcpyAsync(a->dG_save, a->dG, sizeof(double), hipMemcpyDeviceToDevice, a->twin->stream));
+        if ((rc = adjoint_step_once(a, bar, true, &missed)) != FSI_OK) return rc;
+        if (missed) {
+            a->stats.replanned++;
+            HIPCHECK(c, hipMemcpyAsync(a->dG, a->dG_save, sizeof(double), hipMemcpyDeviceToDevice, a->twin->stream));
+        }
+    }
+    if (!a->planned || missed) {
+        if ((rc = adjoint_step_once(a, bar, false, &missed)) != FSI_OK) return rc;
+        a->planned = 1;
+    }
     a->stats.adjoint_steps++;
     return FSI_OK;
 }
@@ -365,7 +469,7 @@ fsi_status make_twin(fsi_adjoint* a) {
 }  // namespace
 
 extern "C" fsi_status fsi_adjoint_create(fsi_context* c, fsi_workspace* ws, int32_t snapshots, fsi_adjoint** out) {
-    if (!c || !ws || !out || ws->ctx != c || snapshots < 1) return FSI_E_ARGUMENT;
+    if (!c || !ws || !out || ws->ctx != c || snapshots < 0) return FSI_E_ARGUMENT;
     *out = nullptr;
     if (fsi_status rc = adjoint_supported(c)) return rc;
     size_t free0 = 0, total = 0;
@@ -374,24 +478,27 @@ extern "C" fsi_status fsi_adjoint_create(fsi_context* c, fsi_workspace* ws, int3
     a->ctx = c;
     a->ws = ws;
     a->snapshots = snapshots;
+    a->keep_tapes = snapshots == 0;
     fsi_status rc = make_twin(a);
     if (rc == FSI_OK && (rc = fsi_state_create(c, &a->work)) == FSI_OK) rc = fsi_workspace_create(c, &a->work_ws);
     for (int s = 0; s < 3 && rc == FSI_OK; ++s) {
-        double** f[] = {&a->tU[s], &a->tV[s], &a->tphi[s], &a->tsxx[s], &a->tsyy[s], &a->tsxy[s],
+        double** f[] = {&a->own.U[s], &a->own.V[s], &a->own.phi[s], &a->own.sxx[s], &a->own.syy[s], &a->own.sxy[s],
                         &a->bEu[s], &a->bEv[s], &a->bIu[s], &a->bIv[s], &a->bEp[s], &a->bExx[s], &a->bEyy[s],
-                        &a->bExy[s], &a->gU[s], &a->gV[s], &a->gphi[s], &a->gsxx[s], &a->gsyy[s], &a->gsxy[s]};
+                        &a->bExy[s], &a->gU[s], &a->gV[s], &a->gphi[s], &a->gsxx[s], &a->gsyy[s], &a->gsxy[s],
+                        &a->rU[s], &a->rV[s]};
         for (double** pp : f)
             if ((rc = adj_alloc(a, pp)) != FSI_OK) break;
     }
-    double** extra[] = {&a->tphi[3], &a->wU, &a->wV, &a->rU, &a->rV};
+    double** extra[] = {&a->own.phi[3], &a->wU, &a->wV};
     for (double** pp : extra)
         if (rc == FSI_OK) rc = adj_alloc(a, pp);
     if (rc == FSI_OK) {
         const size_t np = (size_t)adj_block_count(c->prm) + 1;
         if (hipMalloc((void**)&a->partials, np * sizeof(double)) != hipSuccess ||
-            hipMalloc((void**)&a->dG, sizeof(double)) != hipSuccess)
+            hipMalloc((void**)&a->dG, 2 * sizeof(double)) != hipSuccess)
             rc = fail(c, FSI_E_HIP, "adjoint buffers: out of device memory");
-        a->bytes += (double)(np + 1) * sizeof(double);
+        a->bytes += (double)(np + 2) * sizeof(double);
+        a->dG_save = a->dG + 1;
     }
     a->slots.resize(snapshots);
     for (auto& sl : a->slots)
@@ -416,6 +523,7 @@ extern "C" void fsi_adjoint_destroy(fsi_adjoint* a) {
     for (double* p : a->owned) (void)hipFree(p);
     if (a->partials) (void)hipFree(a->partials);
     if (a->dG) (void)hipFree(a->dG);
+    if (a->pool_mem) (void)hipFree(a->pool_mem);
     fsi_workspace_destroy(a->work_ws);
     fsi_state_destroy(a->work);
     fsi_state_destroy(a->twin_zero);
@@ -427,6 +535,7 @@ extern "C" void fsi_adjoint_destroy(fsi_adjoint* a) {
 extern "C" fsi_status fsi_adjoint_record_step(fsi_adjoint* a, fsi_state* st, double t, double dt) {
     if (!a || !st || st->ctx != a->ctx) return FSI_E_ARGUMENT;
     fsi_context* c = a->ctx;
+    a->cur = &a->own;
     const int warm = c->vel_warm;
     c->vel_warm = 0;
     fsi_status rc = record_step(a, st, a->ws, t, dt);
@@ -497,7 +606,7 @@ extern "C" fsi_status fsi_adjoint_forward(fsi_adjoint* a, fsi_state* st, double 
     if (!(dt > 0.0) || !std::isfinite(dt) || !std::isfinite(t0))
         return fail(c, FSI_E_ARGUMENT, "dt must be finite and strictly positive.");
     int held = 0, rep = 0;
-    a->acts = plan_run(n_steps, a->snapshots, &held, &rep);
+    a->acts = a->keep_tapes ? std::vector<Act>{} : plan_run(n_steps, a->snapshots, &held, &rep);
     a->next_act = 0;
     a->run_steps = n_steps;
     a->t0 = t0;
@@ -508,8 +617,42 @@ extern "C" fsi_status fsi_adjoint_forward(fsi_adjoint* a, fsi_state* st, double 
     a->stats.steps = n_steps;
     a->stats.snapshots_used = held;
     a->stats.repetitions = rep;
+    for (int k = 0; k < 3; ++k) {  // the transposed solves of the run start from zero
+        HIPCHECK(c, hipMemsetAsync(a->rU[k], 0, c->felems * sizeof(double), c->stream));
+        HIPCHECK(c, hipMemsetAsync(a->rV[k], 0, c->felems * sizeof(double), c->stream));
+    }
+    if (a->keep_tapes) {  // every step taped as it runs: nothing is recomputed
+        if ((int)a->pool.size() < n_steps) {
+            if (a->pool_mem) (void)hipFree(a->pool_mem);
+            a->pool_mem = nullptr;
+            a->pool.clear();
+            const size_t per = (size_t)Tape::kFields * c->felems;
+            if (hipMalloc((void**)&a->pool_mem, (size_t)n_steps * per * sizeof(double)) != hipSuccess) {
+                a->run_steps = -1;
+                return fail(c, FSI_E_HIP, "adjoint: no device memory for %d tapes of %.1f MB (use checkpoints)",
+                            n_steps, per * sizeof(double) / 1e6);
+            }
+            a->pool.resize(n_steps);
+            for (int k = 0; k < n_steps; ++k)
+                for (int f = 0; f < Tape::kFields; ++f) *a->pool[k].slot(f) = a->pool_mem + ((size_t)k * Tape::kFields + f) * c->felems;
+        }
+        a->stats.snapshots_used = 0;
+        a->stats.repetitions = 0;
+        size_t fr = 0, tot = 0;
+        (void)hipMemGetInfo(&fr, &tot);
+        for (int k = 0; k < n_steps; ++k) {
+            a->cur = &a->pool[k];
+            if (fsi_status rc = record_step(a, st, a->ws, t0 + (double)k * dt, dt)) {
+                a->run_steps = -1;
+                return rc;
+            }
+        }
+        a->stats.device_bytes = a->bytes + (double)a->pool.size() * Tape::kFields * c->felems * sizeof(double);
+        return FSI_OK;
+    }
     // the recomputations start from checkpoints of the six evolving fields;
     // the resource (constant under NoGrowth) is the caller's
+    a->cur = &a->own;
     if (fsi_status rc = copy_field(c, c->stream, a->work->res, st->res)) return rc;
     const int warm = c->vel_warm;
     c->vel_warm = 0;
@@ -522,9 +665,27 @@ extern "C" fsi_status fsi_adjoint_forward(fsi_adjoint* a, fsi_state* st, double 
 extern "C" fsi_status fsi_adjoint_backward(fsi_adjoint* a, fsi_state* bar, double* d_shear_modulus) {
     if (!a || !bar || bar->ctx != a->ctx) return FSI_E_ARGUMENT;
     fsi_context* c = a->ctx;
-    if (a->run_steps < 1 || a->next_act >= a->acts.size() || a->acts[a->next_act].kind != ACT_ADJOINT)
+    if (a->run_steps < 1 ||
+        (!a->keep_tapes && (a->next_act >= a->acts.size() || a->acts[a->next_act].kind != ACT_ADJOINT)))
         return fail(c, FSI_E_ARGUMENT, "fsi_adjoint_backward: no run recorded by fsi_adjoint_forward.");
     HIPCHECK(c, hipMemsetAsync(a->dG, 0, sizeof(double), a->twin->stream));
+    if (a->keep_tapes) {
+        fsi_status rc = FSI_OK;
+        for (int k = a->run_steps - 1; k >= 0 && rc == FSI_OK; --k) {
+            a->cur = &a->pool[k];
+            a->taped = 1;
+            a->tape_dt = a->dt;
+            rc = adjoint_step(a, bar);
+        }
+        a->run_steps = -1;
+        a->taped = 0;
+        if (rc != FSI_OK) return rc;
+        double g = 0.0;
+        HIPCHECK(c, hipMemcpyAsync(&g, a->dG, sizeof(double), hipMemcpyDeviceToHost, a->twin->stream));
+        HIPCHECK(c, hipStreamSynchronize(a->twin->stream));
+        if (d_shear_modulus) *d_shear_modulus = g;
+        return FSI_OK;
+    }
     const int warm = c->vel_warm;
     c->vel_warm = 0;
     fsi_status rc = run_actions(a, a->work, a->work_ws, bar, false);

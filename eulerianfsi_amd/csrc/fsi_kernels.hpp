@@ -311,6 +311,26 @@ void launch_adj_primal(hipStream_t s, const Params& p, const BcSet& bpsi, const 
 void launch_adj_tendencies(hipStream_t s, const Params& p, const BcSet& bpsi, const AdjStage& st, const AdjBar& b,
                            const AdjGrads& out, double* partials, double* dG);
 void launch_adj_axpy(hipStream_t s, size_t n, double* y, double a, const double* x);
+// zero each of up to 48 arrays of n values
+struct AdjPtrs {
+    double* p[48];
+    int n;
+};
+void launch_adj_zero(hipStream_t s, size_t n, const AdjPtrs& z);
+// dst[d] += coef[d] * src[from[d]] (d < ndst <= 16, nsrc <= 4); sources with
+// bit q of `masked` read as zero where phi < cutoff (phi null: never)
+struct AdjCombine {
+    const double* src[4];
+    int nsrc;
+    double* dst[16];
+    double coef[16];
+    int from[16];
+    int ndst;
+    const double* phi;
+    double cutoff;
+    int masked;
+};
+void launch_adj_combine(hipStream_t s, size_t n, const AdjCombine& c);
 void launch_adj_mask(hipStream_t s, const Params& p, const double* phi, double* gxx, double* gyy, double* gxy);
 void launch_adj_walls(hipStream_t s, const Params& p, double* U, double* V);
 void launch_adj_affine(hipStream_t s, const Params& p, double* phi, double A, double B, double C);

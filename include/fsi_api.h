@@ -429,10 +429,17 @@ typedef struct {
     int32_t snapshots_used; /* most checkpoints held at once                       */
     int32_t repetitions;    /* the schedule's repetition number                    */
     double device_bytes;    /* device memory the adjoint object holds              */
+    int64_t replanned;      /* transposed steps whose planned solves fell short and ran again polled */
+    int32_t last_shortfall; /* which: bit k velocity solve of stage k + 2, bit 3 + k its projection */
+    int32_t solve_iterations[3]; /* iterations of the last transposed predictor solves */
 } fsi_adjoint_stats;
 
 /* The adjoint of runs on (ctx, ws), holding up to `snapshots` checkpoints of
- * the state (binomial checkpointing, Griewank & Walther's revolve). */
+ * the state (binomial checkpointing, Griewank & Walther's revolve; the
+ * recomputed steps run without the velocity warm-start history, so they
+ * reproduce the recorded ones exactly).  snapshots = 0 keeps instead every
+ * step's stage fields (19 fields per step, allocated by the forward run):
+ * nothing is recomputed. */
 fsi_status fsi_adjoint_create(fsi_context* ctx, fsi_workspace* ws, int32_t snapshots, fsi_adjoint** out);
 void fsi_adjoint_destroy(fsi_adjoint* adj);
 /* The forward half of the rrule of n_steps integrate_step! calls from t0 by
