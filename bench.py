@@ -269,12 +269,20 @@ def main(argv=None):
     ap.add_argument("--adjoint-N", type=int, default=256)
     ap.add_argument("--adjoint-snapshots", type=int, default=45)
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip the central-difference check")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run the differentiation leg alone and print its JSON (profiling runs)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build's step runs on a single MI355X (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         return 2
+    if args.only_adjoint:
+        import eulerianfsi_amd as fsi
+        from eulerianfsi_amd import setups
+        print(json.dumps(adjoint_leg(fsi, setups, args.adjoint_N, args.adjoint_steps, args.adjoint_snapshots,
+                                     fd_check=not args.no_adjoint_fd)))
+        return 0
 
     import eulerianfsi_amd as fsi
     from eulerianfsi_amd import _abi, setups

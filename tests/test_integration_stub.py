@@ -16,7 +16,7 @@ JULIA_TO_C = {
     "BcSide": "fsi_bc_side", "BcSetup": "fsi_bc_setup", "Bcs": "fsi_bcs", "Grid": "fsi_grid",
     "Models": "fsi_models", "Numerics": "fsi_numerics", "CStepReport": "fsi_step_report",
     "StepStats": "fsi_step_stats", "ProjectionResultC": "fsi_projection_result", "DiagParams": "fsi_diag_params",
-    "TensorRegion": "fsi_tensor_region", "Diagnostics": "fsi_diagnostics",
+    "TensorRegion": "fsi_tensor_region", "Diagnostics": "fsi_diagnostics", "AdjointStats": "fsi_adjoint_stats",
 }
 SCALARS = {"Int32": "int32_t", "Int64": "int64_t", "Float64": "double"}
 
