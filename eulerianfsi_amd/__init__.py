@@ -12,7 +12,7 @@ from .boundary import (AbstractBoundaryCondition, BoundaryConditionSetup, Dirich
 from .closures import (ConservativeDiffuseInterfaceFlux, ConstantBodyForce, ConstantViscosity, CurrentCDIPhaseField,
                        GeometricStructureExtension, IdentityOutsideStructure, IsotropicGrowthTensor,
                        JainACDIPhaseField, KolmogorovBodyForce, LegacySplitPhaseFieldMethod,
-                       LinearInternalTensorStress, MaxwellRelaxation, NoBodyForce, NoCapillarity, NoDiffusion,
+                       LinearInternalTensorStress, MaxwellRelaxation, NoBodyForce, NoCapillarity, DiffuseAreaCapillarity, NoDiffusion,
                        NoElasticStress, NoGrowth, NoInterfaceFlux, NoPhaseFieldRegularization, NoRelaxation,
                        NoResourceConsumption, NoStructureExtension, NoViscosity, PirozzoliUWCDIPhaseField,
                        jain_equilibrium_initialization_epsilon, phase_field_label)
@@ -20,7 +20,8 @@ from .errors import ArgumentError, DeviceError, DimensionMismatch, DomainError, 
 from .imex import (A_EXPLICIT, A_IMPLICIT, STAGE_ABSCISSA, FourStageIMEXFractionalStep, FSIBoundaryConditions,
                    FSIModels, FSIState, IMEXContext, IMEXNumerics, IMEXWorkspace, LinearSolverOptions,
                    ProjectionResult, StepReport, allocate_imex_workspace, build_imex_context,
-                   ensemble_step, evaluate_stage_tendencies, integrate_step, make_velocity_compatible, project)
+                   ensemble_step, evaluate_stage_tendencies, integrate_step, make_velocity_compatible, project,
+                   uwcdi_strang_step)
 from .mesh import MACGrid, MeshMetrics, build_mesh_metrics, generate_grid
 from . import adjoint, diagnostics
 

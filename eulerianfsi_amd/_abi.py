@@ -25,6 +25,8 @@ RELAX_NONE, RELAX_MAXWELL = 0, 1
 FORCE_NONE, FORCE_CONSTANT, FORCE_KOLMOGOROV = 0, 1, 2
 EXT_NONE, EXT_IDENTITY, EXT_GEOMETRIC = 0, 1, 2
 PRECISION_FP64, PRECISION_MIXED = 0, 1
+UWCDI_NORMALS = {"standard": 0, "youngs": 1, "smoothed9": 2}
+CAPILLARITY_NONE, CAPILLARITY_DIFFUSE_AREA = 0, 1
 SOLVER_DIRECT, SOLVER_GMRES = 0, 1
 
 
@@ -58,6 +60,9 @@ class Models(ctypes.Structure):
         ("body_force_kind", c_int32),
         ("force_fx", c_double), ("force_fy", c_double),
         ("force_amplitude", c_double), ("force_wavenumber", c_double), ("force_phase", c_double),
+        ("uwcdi_normal_method", c_int32),
+        ("capillarity_kind", c_int32), ("capillary_coefficient", c_double), ("capillary_delta_normal", c_double),
+        ("capillary_smoothing_passes", c_int32),
     ]
 
 
@@ -146,10 +151,9 @@ def abi_config(metrics, bcs, models, numerics):
         m.acdi_gamma, m.acdi_epsilon, m.acdi_psi_floor = pf.gamma, pf.epsilon, pf.psi_floor
         m.acdi_delta_normal, m.acdi_dynamic_gamma = pf.delta_normal, int(pf.dynamic_gamma)
     elif isinstance(pf, _cl.PirozzoliUWCDIPhaseField):
-        if pf.normal_reconstruction != "standard":
-            raise _cl.UnsupportedModel("UWCDI normal reconstruction other than :standard is outside the device path")
         m.phase_method = PHASE_UWCDI
         m.uwcdi_alpha, m.uwcdi_delta_normal = pf.alpha, pf.delta_normal
+        m.uwcdi_normal_method = UWCDI_NORMALS[pf.normal_reconstruction]
     else:
         raise _cl.UnsupportedModel(f"{type(pf).__name__} is outside the device path")
 
@@ -184,10 +188,17 @@ def abi_config(metrics, bcs, models, numerics):
         m.body_force_kind = FORCE_NONE
     else:
         raise _cl.UnsupportedModel(f"{type(f).__name__} is outside the device path")
-    for name in ("growth_source", "growth_tensor", "diffusivity", "resource_consumption", "capillarity"):
+    cap = models.capillarity
+    if isinstance(cap, _cl.DiffuseAreaCapillarity):
+        m.capillarity_kind = CAPILLARITY_DIFFUSE_AREA
+        m.capillary_coefficient, m.capillary_delta_normal = cap.coefficient, cap.delta_normal
+        m.capillary_smoothing_passes = cap.smoothing_passes
+    elif not isinstance(cap, _cl.NoCapillarity):
+        raise _cl.UnsupportedModel(f"capillarity={type(cap).__name__} is outside the device path")
+    for name in ("growth_source", "growth_tensor", "diffusivity", "resource_consumption"):
         obj = getattr(models, name)
         if not isinstance(obj, (_cl.NoGrowth, _cl.IsotropicGrowthTensor, _cl.NoDiffusion,
-                                _cl.NoResourceConsumption, _cl.NoCapillarity)):
+                                _cl.NoResourceConsumption)):
             raise _cl.UnsupportedModel(f"{name}={type(obj).__name__} is outside the device path")
 
     n = Numerics()
@@ -276,6 +287,7 @@ def _declare(lib):
                                       POINTER(StepStatsC), POINTER(c_int32)]),
         "fsi_project": (c_int, [P, P, P, POINTER(ProjectionResultC)]),
         "fsi_eval_tendencies": (c_int, [P, P, P, c_double] + [_dp] * 8),
+        "fsi_uwcdi_strang_step": (c_int, [P, P, P, c_double, c_int32]),
         "fsi_benchmark_kernel": (c_int, [P, P, P, c_char_p, c_int32, _dp, _dp]),
         "fsi_velocity_extrema": (c_int, [P, P, _dp, _dp]),
         "fsi_phase_centroid": (c_int, [P, P, _dp, _dp, _dp]),
@@ -306,7 +318,7 @@ EXPORTED = (
     "fsi_context_create", "fsi_context_destroy", "fsi_last_error", "fsi_state_create", "fsi_state_destroy",
     "fsi_state_upload", "fsi_state_download", "fsi_workspace_create", "fsi_workspace_destroy",
     "fsi_workspace_download_correction", "fsi_workspace_upload_correction", "fsi_make_velocity_compatible",
-    "fsi_integrate_step", "fsi_ensemble_step", "fsi_project", "fsi_eval_tendencies", "fsi_benchmark_kernel",
+    "fsi_integrate_step", "fsi_ensemble_step", "fsi_project", "fsi_eval_tendencies", "fsi_uwcdi_strang_step", "fsi_benchmark_kernel",
     "fsi_velocity_extrema", "fsi_phase_centroid", "fsi_phase_centroid_noalloc", "fsi_diagnostics_compute",
     "fsi_version", "fsi_adjoint_create", "fsi_adjoint_destroy", "fsi_adjoint_forward", "fsi_adjoint_backward",
     "fsi_adjoint_get_stats", "fsi_adjoint_record_step", "fsi_adjoint_step", "fsi_adjoint_centroid_objective",

@@ -16,8 +16,8 @@ SOURCES = ["fsi_api.hip", "fsi_physics.hip", "fsi_solvers.hip", "fsi_mg.hip", "f
 LIB = PKG / "libfsi_hip.so"
 ARCH = os.environ.get("FSI_OFFLOAD_ARCH", "gfx950")
 
-# -packed-fp32-ops: no v_pk_*_f32 generation (wrong results observed on the
-# target pool); harmless for this FP64 code, kept for the FP32 paths.
+# -packed-fp32-ops: no v_pk_*_f32 generation (some of their forms have given
+# wrong results on gfx950); harmless for this FP64 code, kept for the FP32 paths.
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
     "-O3",

@@ -106,6 +106,26 @@ class NoCapillarity:
     pass
 
 
+@dataclass
+class DiffuseAreaCapillarity:
+    """DiffuseAreaCapillarity(coefficient; delta_normal, smoothing_passes)
+    (Capillarity.jl:21-41): f = coefficient * kappa * grad(phi), kappa = -div(n)."""
+    coefficient: float
+    delta_normal: float = 1e-6
+    smoothing_passes: int = 0
+
+    def __post_init__(self):
+        c = _finite(self.coefficient, "coefficient must be finite.")
+        if not c >= 0.0:
+            raise ArgumentError("coefficient must be nonnegative.")
+        d = _finite(self.delta_normal, "delta_normal must be finite.")
+        if not d > 0.0:
+            raise ArgumentError("delta_normal must be positive.")
+        if int(self.smoothing_passes) != self.smoothing_passes or self.smoothing_passes < 0:
+            raise ArgumentError("smoothing_passes must be nonnegative.")
+        self.coefficient, self.delta_normal, self.smoothing_passes = float(c), float(d), int(self.smoothing_passes)
+
+
 # ---- growth / resource (only the no-growth closures are on the path) -----------
 class NoGrowth:
     pass

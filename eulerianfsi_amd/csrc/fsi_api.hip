@@ -125,8 +125,33 @@ static fsi_status validate(fsi_context* c) {
         if (!(m.uwcdi_alpha >= 1.0) || !std::isfinite(m.uwcdi_alpha))
             return fail(c, FSI_E_ARGUMENT, "alpha must be finite and >= 1 for the published UWCDI stability range.");
         if (!(m.uwcdi_delta_normal > 0.0)) return fail(c, FSI_E_ARGUMENT, "delta_normal must be strictly positive.");
+        if (m.uwcdi_normal_method < FSI_UWCDI_NORMAL_STANDARD || m.uwcdi_normal_method > FSI_UWCDI_NORMAL_SMOOTHED9)
+            return fail(c, FSI_E_ARGUMENT, "normal_reconstruction must be :standard, :youngs, or :smoothed9.");
+        if (m.uwcdi_normal_method == FSI_UWCDI_NORMAL_SMOOTHED9) {  // _validate_smoothed9_boundaries :347-362
+            const fsi_bc_side* ps[4] = {&c->bcs.phi.left, &c->bcs.phi.right, &c->bcs.phi.bottom, &c->bcs.phi.top};
+            const char* pn[4] = {"left", "right", "bottom", "top"};
+            for (int q = 0; q < 4; ++q) {
+                if (ps[q]->kind == FSI_BC_PERIODIC) continue;
+                if (ps[q]->kind == FSI_BC_NEUMANN && std::fabs(ps[q]->value) <= 100 * 2.220446049250313e-16) continue;
+                return fail(c, FSI_E_ARGUMENT,
+                            "smoothed9 normal reconstruction requires homogeneous Neumann or periodic phase "
+                            "boundaries; %s does not.", pn[q]);
+            }
+        }
+        // :youngs needs a uniform grid (:285-296): every device grid is uniform
         break;
     default: return fail(c, FSI_E_UNSUPPORTED, "phase method %d is not supported on the device.", m.phase_method);
+    }
+    switch (m.capillarity_kind) {
+    case FSI_CAPILLARITY_NONE: break;
+    case FSI_CAPILLARITY_DIFFUSE_AREA:  // Capillarity.jl:21-41
+        if (!std::isfinite(m.capillary_coefficient)) return fail(c, FSI_E_ARGUMENT, "coefficient must be finite.");
+        if (!(m.capillary_coefficient >= 0.0)) return fail(c, FSI_E_ARGUMENT, "coefficient must be nonnegative.");
+        if (!std::isfinite(m.capillary_delta_normal)) return fail(c, FSI_E_ARGUMENT, "delta_normal must be finite.");
+        if (!(m.capillary_delta_normal > 0.0)) return fail(c, FSI_E_ARGUMENT, "delta_normal must be positive.");
+        if (m.capillary_smoothing_passes < 0) return fail(c, FSI_E_ARGUMENT, "smoothing_passes must be nonnegative.");
+        break;
+    default: return fail(c, FSI_E_UNSUPPORTED, "capillarity %d is not supported on the device.", m.capillarity_kind);
     }
     if (m.viscosity_kind == FSI_VISC_CONSTANT) {
         if (!(m.viscosity > 0.0) || !std::isfinite(m.viscosity))
@@ -251,6 +276,10 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     p.acdi_dyn = m.acdi_dynamic_gamma;
     p.uw_alpha = m.uwcdi_alpha;
     p.uw_d2 = m.uwcdi_delta_normal * m.uwcdi_delta_normal;
+    p.uw_normal = p.phase == FSI_PHASE_UWCDI ? m.uwcdi_normal_method : FSI_UWCDI_NORMAL_STANDARD;
+    p.cap_coef = m.capillary_coefficient;
+    p.cap_d2 = m.capillary_delta_normal * m.capillary_delta_normal;
+    p.cap_passes = m.capillary_smoothing_passes;
     p.visc = m.viscosity_kind == FSI_VISC_CONSTANT;
     p.mu = m.viscosity;
     p.elastic = m.elastic_kind == FSI_ELASTIC_LINEAR;
@@ -295,6 +324,16 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
         HIPCHECK(c, hipMemcpyAsync(c->force_row, host.data(), ny * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
     p.force_row = c->force_row;
+    if (p.uw_normal == FSI_UWCDI_NORMAL_SMOOTHED9) {
+        HIPCHECK(c, dalloc(c, &c->uw_smooth, c->felems));
+        p.uw_field = c->uw_smooth;
+    }
+    if (m.capillarity_kind == FSI_CAPILLARITY_DIFFUSE_AREA) {
+        HIPCHECK(c, dalloc(c, &c->cap_buf, 7 * c->felems));
+        p.cap_fx = c->cap_buf;
+        p.cap_fy = c->cap_buf + c->felems;
+        p.cap_work = c->cap_buf + 2 * c->felems;
+    }
     p.exnb = grid_blocks(p);
     HIPCHECK(c, dalloc(c, &c->ext, (size_t)EX_COUNT * p.exnb));
     {
@@ -426,8 +465,8 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
 extern "C" void fsi_context_destroy(fsi_context* c) {
     if (!c) return;
     if (c->stream) (void)hipStreamSynchronize(c->stream);
-    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u, c->bvisc_v, c->partials, c->co_dense, c->spec_tab,
-                      c->ext,       c->diag,   c->diag_out};
+    double* bufs[] = {c->force_row, c->lap_bc, c->bvisc_u,  c->bvisc_v, c->partials, c->co_dense,
+                      c->spec_tab,  c->ext,    c->diag,     c->diag_out, c->uw_smooth, c->cap_buf};
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
@@ -1245,6 +1284,30 @@ extern "C" fsi_status fsi_eval_tendencies(fsi_context* c, const fsi_state* s, fs
     if (E_sxx && (rc = download2d(c, E_sxx, w->Exx[0], nx, ny)) != FSI_OK) return rc;
     if (E_syy && (rc = download2d(c, E_syy, w->Eyy[0], nx, ny)) != FSI_OK) return rc;
     if (E_sxy && (rc = download2d(c, E_sxy, w->Exy[0], nx, ny)) != FSI_OK) return rc;
+    if ((rc = status_pull(c)) != FSI_OK) return rc;
+    return device_error(c);
+}
+
+// uwcdi_strang_step (PhaseFieldMethods.jl:793-850): Euler sweeps of the
+// directional right-hand sides, x (dt/2), y (dt), x (dt/2) (first_direction
+// 1: y, x, y), through E_phi of stage slots 0..2 of the workspace.
+extern "C" fsi_status fsi_uwcdi_strang_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt,
+                                            int32_t first_direction) {
+    if (!c || !s || !w) return FSI_E_ARGUMENT;
+    if (c->prm.phase != FSI_PHASE_UWCDI) return fail(c, FSI_E_ARGUMENT, "uwcdi_strang_step needs the UWCDI phase field.");
+    if (!(std::isfinite(dt) && dt >= 0.0)) return fail(c, FSI_E_ARGUMENT, "dt must be finite and non-negative.");
+    if (first_direction != 0 && first_direction != 1)
+        return fail(c, FSI_E_ARGUMENT, "first_direction must be :x or :y; received %d.", (int)first_direction);
+    if (dt == 0.0) return FSI_OK;  // every sweep returns phi unchanged (:802)
+    status_reset(c);
+    fsi_status rc = status_push(c);
+    if (rc != FSI_OK) return rc;
+    const Params& p = c->prm;
+    const int d1 = first_direction == 0 ? 1 : 2, d2 = 3 - d1;
+    ErrSlot* err = &c->st->cells_err[0];
+    launch_uwcdi_sweep(c->stream, p, s->U, s->V, s->phi, d1, 0.5 * dt, w->Ep[0], w->Ep[1], err);
+    launch_uwcdi_sweep(c->stream, p, s->U, s->V, w->Ep[1], d2, dt, w->Ep[0], w->Ep[2], err);
+    launch_uwcdi_sweep(c->stream, p, s->U, s->V, w->Ep[2], d1, 0.5 * dt, w->Ep[0], s->phi, err);
     if ((rc = status_pull(c)) != FSI_OK) return rc;
     return device_error(c);
 }

@@ -46,6 +46,15 @@ struct Params {
     double acdi_gamma, acdi_eps, acdi_floor, acdi_d2;
     int acdi_dyn;
     double uw_alpha, uw_d2;
+    int uw_normal;      // fsi_uwcdi_normal
+    double* uw_field;   // the 9-point-filtered phi of a smoothed9 stage (cells), or null
+    // DiffuseAreaCapillarity: the face force read by the momentum kernel
+    // (null: no capillarity), its work cells and coefficients
+    const double* cap_fx;
+    const double* cap_fy;
+    double* cap_work;  // 5 fields: two filter buffers, n_x, n_y, kappa
+    double cap_coef, cap_d2;
+    int cap_passes;
     // closures
     int visc;
     double mu;

@@ -51,6 +51,8 @@ struct fsi_context {
     std::string err;
     int device = -1;  // the HIP device current at creation
     double* force_row = nullptr;
+    double* uw_smooth = nullptr;  // UWCDI :smoothed9: the filtered phi of the stage being evaluated
+    double* cap_buf = nullptr;    // DiffuseAreaCapillarity: force x, force y, then 5 work fields
     double* ext = nullptr;  // per-block extrema of the report and checks (EX_* slots)
     double* diag = nullptr;      // per-block partials of the driver diagnostics
     double* diag_out = nullptr;  // their reduced values

@@ -88,6 +88,11 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
                   double* Ephi, double* Exx, double* Eyy, double* Exy, ErrSlot* err, int stage1,
                   const CellSums* sums = nullptr);
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv);
+// uwcdi_directional_rhs (PhaseFieldMethods.jl:768-791) of phi: dir 1 the
+// x-fluxes, 2 the y-fluxes; then phi_out = phi + h rhs (the Euler sweep
+// :793-810).  rhs and phi_out may not alias phi.
+void launch_uwcdi_sweep(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi, int dir,
+                        double h, double* rhs, double* phi_out, ErrSlot* err);
 // StepReport extrema and counts, and the stage-1 checks, from the per-block
 // partials of the step's last combination and of its stage-1 tendencies
 void launch_ext_final(hipStream_t s, const Params& p, Status* st);

@@ -315,6 +315,7 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
         double eu = -cu;
         if (p.force == FSI_FORCE_CONSTANT) eu += p.ffx;
         else if (p.force == FSI_FORCE_KOLMOGOROV) eu += p.force_row[j];
+        if (p.cap_fx) eu += p.cap_fx[k];  // capillary_force, :980-982
         if (elastic) eu += elastic_fx<IN>(p, Txx, Txy, i, j);
         // ---- y-momentum at y-face (i, j)
         double cv;
@@ -326,6 +327,7 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
         }
         double ev = -cv;
         if (p.force == FSI_FORCE_CONSTANT) ev += p.ffy;
+        if (p.cap_fy) ev += p.cap_fy[k];
         if (elastic) ev += elastic_fy<IN>(p, Tyy, Txy, i, j);
         if (Eu) {
             Eu[k] = eu;
@@ -384,8 +386,14 @@ k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, 
 
 // scalar_gradient on x-face k (Operators.jl:189-356) of the cell function f.
 // IN = true: 1 <= k <= nx-1, an interior face (no wall row, no wrap).
+// Both products are rounded before the difference, as the reference's
+// sparse product forms them: a fused multiply-add would turn the exact zero
+// of two equal neighbours into the product's rounding error, which the
+// regularised normals g / sqrt(g^2 + delta^2) (delta down to 1e-14) amplify
+// to O(1) in flat regions of phi.
 template <bool IN = false, class F>
 __device__ __forceinline__ double grad_xface(const Params& p, const BcSet& bc, F f, int k, int j) {
+#pragma clang fp contract(off)
     if (IN) return p.ihx * f(k, j) - p.ihx * f(k - 1, j);
     if (p.px) {
         if (k == 0 || k == p.nx) return p.ihx * f(0, j) - p.ihx * f(p.nx - 1, j);
@@ -406,6 +414,7 @@ __device__ __forceinline__ double grad_xface(const Params& p, const BcSet& bc, F
 }
 template <bool IN = false, class F>
 __device__ __forceinline__ double grad_yface(const Params& p, const BcSet& bc, F f, int i, int k) {
+#pragma clang fp contract(off)
     if (IN) return p.ihy * f(i, k) - p.ihy * f(i, k - 1);
     if (p.py) {
         if (k == 0 || k == p.ny) return p.ihy * f(i, 0) - p.ihy * f(i, p.ny - 1);
@@ -742,7 +751,7 @@ __device__ __forceinline__ double phase_acdi_t(const Params& p, const Stage& s, 
 
 template <class F, class NX, class NY>
 __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s, const F& f, const NX& N0, const NY& N1, int i,
-                                int j, ErrSlot* st) {
+                                int j, ErrSlot* st, int dir) {
     auto fx = [&](int k) -> double {
         int kk = (p.px && k == p.nx) ? 0 : k;
         double u = s.U[at(p, kk, j)], pu;
@@ -783,6 +792,9 @@ __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s,
         double beta = fabs(v) / ay;
         return pu * (v + beta * nf * (1.0 - pu));
     };
+    // dir 1 / 2: -divergence of (fx, 0) / (0, fy) (uwcdi_directional_rhs :768-791)
+    if (dir == 1) return -((fx(i + 1) - fx(i)) * p.ihx + 0.0);
+    if (dir == 2) return -(0.0 + (fy(j + 1) - fy(j)) * p.ihy);
     double dF = (fx(i + 1) - fx(i)) * p.ihx + (fy(j + 1) - fy(j)) * p.ihy;
     return -dF;
 }
@@ -1062,11 +1074,14 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
 // Phase-field right-hand side, one instantiation per phase method.
 template <int PH>
 __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
-                                                 ErrSlot* st) {
+                                                 ErrSlot* st, int dir) {
     constexpr bool kPsi = PH == FSI_PHASE_ACDI;
     constexpr bool kNormals = PH != FSI_PHASE_UPWIND;
+    // UWCDI :smoothed9 takes its normals from the filtered phi, held in the psi tile
+    constexpr bool kSecond = kPsi || PH == FSI_PHASE_UWCDI;
+    const bool smooth = PH == FSI_PHASE_UWCDI && p.uw_field != nullptr;
     __shared__ double s_phi[CR2 * CW2];
-    __shared__ double s_psi[kPsi ? CR2 * CW2 : 1];
+    __shared__ double s_psi[kSecond ? CR2 * CW2 : 1];
     __shared__ double s_n0[kNormals ? CR1 * CW1 : 1];
     __shared__ double s_n1[kNormals ? CR1 * CW1 : 1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
@@ -1092,9 +1107,13 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
             inb[it] = in;
             idx[it] = in ? at(p, gi, gj) : 0;
         }
-        double vals[NIT];
+        double vals[NIT], sm[NIT];
 #pragma unroll
         for (int it = 0; it < NIT; ++it) vals[it] = s.phi[idx[it]];
+        if (smooth) {
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) sm[it] = p.uw_field[idx[it]];
+        }
 #pragma unroll
         for (int it = 0; it < NIT; ++it) {
             const int k = tid + it * BLOCK;
@@ -1102,6 +1121,7 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
             const double v = inb[it] ? vals[it] : 0.0;
             s_phi[k] = v;
             if (kPsi) s_psi[k] = inb[it] ? psi_value(p, v) : 0.0;
+            if (smooth) s_psi[k] = inb[it] ? sm[it] : 0.0;
         }
     }
     __syncthreads();
@@ -1129,8 +1149,19 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
                     double mag = sqrt(gx * gx + gy * gy + p.acdi_d2);
                     n0 = gx / mag;
                     n1 = gy / mag;
-                } else if (PH == FSI_PHASE_UWCDI) {  // _center_normals of phi
-                    grad_center(p, p.bPhi, f, gi, gj, gx, gy);
+                } else if (PH == FSI_PHASE_UWCDI) {
+                    // _center_normals of phi, or of the filtered phi (:smoothed9, :385-390)
+                    if (smooth) grad_center(p, p.bPhi, fpsi, gi, gj, gx, gy);
+                    else grad_center(p, p.bPhi, f, gi, gj, gx, gy);
+                    // :youngs (:280-333): the 3x3 stencil on every cell off the non-periodic walls
+                    if (p.uw_normal == FSI_UWCDI_NORMAL_YOUNGS && (p.px || (gi > 0 && gi < p.nx - 1)) &&
+                        (p.py || (gj > 0 && gj < p.ny - 1))) {
+                        const int im = gi - 1, ip = gi + 1, jm = gj - 1, jp = gj + 1;
+                        gx = (f(ip, jm) + 2.0 * f(ip, gj) + f(ip, jp) - f(im, jm) - 2.0 * f(im, gj) - f(im, jp)) /
+                             (8.0 * p.hx);
+                        gy = (f(im, jp) + 2.0 * f(gi, jp) + f(ip, jp) - f(im, jm) - 2.0 * f(gi, jm) - f(ip, jm)) /
+                             (8.0 * p.hy);
+                    }
                     double mag = sqrt(gx * gx + gy * gy + p.uw_d2);
                     n0 = gx / mag;
                     n1 = gy / mag;
@@ -1169,7 +1200,7 @@ __global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, 
         if (j >= p.ny) break;
         double e;
         if (PH == FSI_PHASE_ACDI) e = phase_acdi_t(p, s, f, fpsi, N0, N1, i, j, gamma, st);
-        else if (PH == FSI_PHASE_UWCDI) e = phase_uwcdi_t(p, s, f, N0, N1, i, j, st);
+        else if (PH == FSI_PHASE_UWCDI) e = phase_uwcdi_t(p, s, f, N0, N1, i, j, st, dir);
         else e = phase_cdi_t(p, s, f, N0, N1, i, j, st, PH == FSI_PHASE_CDI);
         Ephi[at(p, i, j)] = e;
     }
@@ -1493,11 +1524,100 @@ __global__ void __launch_bounds__(BLOCK) k_ext_final(Params p, Status* st) {
     st->nonfinite_in = (long long)ls[2][0];
 }
 
+// ---------------------------------------------------------------------------
+// Point kernels of the UWCDI :smoothed9 normals, the directional Strang sweep
+// and DiffuseAreaCapillarity: one thread per cell (faces: `faces` = 1), blocks
+// of 64 x 4.
+// ---------------------------------------------------------------------------
+static inline dim3 point_grid(const Params& p, int faces) {
+    return dim3((unsigned)((p.nx + faces + TILE_X - 1) / TILE_X), (unsigned)((p.ny + faces + TILE_Y - 1) / TILE_Y));
+}
+__device__ __forceinline__ int clamp_or_wrap(int k, int n, int periodic) {
+    if (periodic) return k < 0 ? k + n : (k >= n ? k - n : k);
+    return k < 0 ? 0 : (k > n - 1 ? n - 1 : k);
+}
+
+// The 9-point filter of _smoothed9_field (PhaseFieldMethods.jl:364-383) and
+// _smooth_phase_for_capillarity (Capillarity.jl:81-111): the index is
+// clamped at a wall and wrapped across a periodic side.
+__global__ void __launch_bounds__(BLOCK) k_smooth9(Params p, const double* __restrict__ f, double* __restrict__ out) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
+    if (i >= p.nx || j >= p.ny) return;
+    const int im = clamp_or_wrap(i - 1, p.nx, p.px), ip = clamp_or_wrap(i + 1, p.nx, p.px);
+    const int jm = clamp_or_wrap(j - 1, p.ny, p.py), jp = clamp_or_wrap(j + 1, p.ny, p.py);
+    auto F = [&](int a, int b) { return f[at(p, a, b)]; };
+    out[at(p, i, j)] = 0.25 * F(i, j) + 0.125 * (F(ip, j) + F(im, j) + F(i, jp) + F(i, jm)) +
+                       0.0625 * (F(ip, jp) + F(im, jp) + F(ip, jm) + F(im, jm));
+}
+
+// phi_out = phi + h rhs on the cells (_uwcdi_directional_euler_sweep :793-810)
+__global__ void __launch_bounds__(BLOCK) k_cell_axpy(Params p, const double* __restrict__ phi, double h,
+                                                     const double* __restrict__ rhs, double* __restrict__ out) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
+    if (i >= p.nx || j >= p.ny) return;
+    const size_t k = at(p, i, j);
+    out[k] = phi[k] + h * rhs[k];
+}
+
+// Regularised centre normal of the geometry field (Capillarity.jl:59-79):
+// the face gradients with the phase boundary conditions averaged to the centre.
+__global__ void __launch_bounds__(BLOCK) k_cap_normals(Params p, const double* __restrict__ geo,
+                                                       double* __restrict__ nxc, double* __restrict__ nyc) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
+    if (i >= p.nx || j >= p.ny) return;
+    auto G = [=](int a, int b) { return geo[at(p, a, b)]; };
+    double gx, gy;
+    grad_center(p, p.bPhi, G, i, j, gx, gy);
+    const double den = sqrt(gx * gx + gy * gy + p.cap_d2);
+    nxc[at(p, i, j)] = gx / den;
+    nyc[at(p, i, j)] = gy / den;
+}
+
+// kappa = -div(n) with n carried to the faces by interpolate_center_to_*_faces (Capillarity.jl:168-173)
+__global__ void __launch_bounds__(BLOCK) k_cap_kappa(Params p, const double* __restrict__ nxc,
+                                                     const double* __restrict__ nyc, double* __restrict__ kap) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
+    if (i >= p.nx || j >= p.ny) return;
+    auto NX = [=](int a, int b) { return nxc[at(p, a, b)]; };
+    auto NY = [=](int a, int b) { return nyc[at(p, a, b)]; };
+    kap[at(p, i, j)] = -((c2fx(p, NX, i + 1, j) - c2fx(p, NX, i, j)) / p.hx +
+                         (c2fy(p, NY, i, j + 1) - c2fy(p, NY, i, j)) / p.hy);
+}
+
+// f = coefficient kappa_face grad(phi) on every x- and y-face (Capillarity.jl:175-186)
+__global__ void __launch_bounds__(BLOCK) k_cap_force(Params p, const double* __restrict__ phi,
+                                                     const double* __restrict__ kap, double* __restrict__ fx,
+                                                     double* __restrict__ fy) {
+    const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
+    if (i > p.nx || j > p.ny) return;
+    auto PH = [=](int a, int b) { return phi[at(p, a, b)]; };
+    auto K = [=](int a, int b) { return kap[at(p, a, b)]; };
+    if (j < p.ny) fx[at(p, i, j)] = p.cap_coef * c2fx(p, K, i, j) * grad_xface(p, p.bPhi, PH, i, j);
+    if (i < p.nx) fy[at(p, i, j)] = p.cap_coef * c2fy(p, K, i, j) * grad_yface(p, p.bPhi, PH, i, j);
+}
+
 void launch_ext_final(hipStream_t s, const Params& p, Status* st) { k_ext_final<<<1, BLOCK, 0, s>>>(p, st); }
+
+// capillary_force (Capillarity.jl:134-187) of the stage phi into p.cap_fx, p.cap_fy
+static void launch_capillary(hipStream_t s, const Params& p, const double* phi) {
+    const size_t F = (size_t)(p.ny + 1) * p.P;
+    double* w = p.cap_work;
+    const double* geo = phi;
+    for (int k = 0; k < p.cap_passes; ++k) {
+        double* out = w + (size_t)(k & 1) * F;
+        k_smooth9<<<point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s>>>(p, geo, out);
+        geo = out;
+    }
+    k_cap_normals<<<point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s>>>(p, geo, w + 2 * F, w + 3 * F);
+    k_cap_kappa<<<point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s>>>(p, w + 2 * F, w + 3 * F, w + 4 * F);
+    k_cap_force<<<point_grid(p, 1), dim3(TILE_X, TILE_Y), 0, s>>>(p, phi, w + 4 * F, const_cast<double*>(p.cap_fx),
+                                                                   const_cast<double*>(p.cap_fy));
+}
 
 void launch_momentum(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
                      const double* sxx, const double* syy, const double* sxy, int reset, double* Eu, double* Ev,
                      double* Iu, double* Iv, const RhsSums* sums) {
+    if (p.cap_fx) launch_capillary(s, p, phi);
     Stage st{U, V, phi, sxx, syy, sxy, reset};
     RhsSums R{};
     R.q = -1;
@@ -1515,14 +1635,26 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
     const dim3 g = grid2d(p), blk(TILE_X, TILE_Y);
     switch (p.phase) {
         case FSI_PHASE_ACDI: k_phase_acdi<<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
-        case FSI_PHASE_UWCDI: k_phase<FSI_PHASE_UWCDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
-        case FSI_PHASE_CDI: k_phase<FSI_PHASE_CDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
-        default: k_phase<FSI_PHASE_UPWIND><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+        case FSI_PHASE_UWCDI:
+            if (p.uw_field) k_smooth9<<<point_grid(p, 0), blk, 0, s>>>(p, phi, p.uw_field);
+            k_phase<FSI_PHASE_UWCDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st, 0);
+            break;
+        case FSI_PHASE_CDI: k_phase<FSI_PHASE_CDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st, 0); break;
+        default: k_phase<FSI_PHASE_UPWIND><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st, 0); break;
     }
     CellSums C{};
     C.q = -1;
     if (sums) C = *sums;
     k_sigma<<<g, blk, 0, s>>>(p, sg, Exx, Eyy, Exy, st, stage1, C);
+}
+
+void launch_uwcdi_sweep(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi, int dir,
+                        double h, double* rhs, double* phi_out, ErrSlot* err) {
+    Stage sg{U, V, phi, nullptr, nullptr, nullptr, 0};
+    const dim3 blk(TILE_X, TILE_Y);
+    if (p.uw_field) k_smooth9<<<point_grid(p, 0), blk, 0, s>>>(p, phi, p.uw_field);
+    k_phase<FSI_PHASE_UWCDI><<<grid2d(p), blk, 0, s>>>(p, sg, p.bPhi, rhs, err, dir);
+    k_cell_axpy<<<point_grid(p, 0), blk, 0, s>>>(p, phi, h, rhs, phi_out);
 }
 
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv) {

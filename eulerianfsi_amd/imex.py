@@ -479,3 +479,18 @@ def evaluate_stage_tendencies(state: FSIState, ctx: IMEXContext, work: IMEXWorks
     st = ctx.lib.fsi_eval_tendencies(ctx.handle, h, work.handle, float(t), *(_abi.as_ptr(out[k]) for k in out))
     _abi.check(ctx.lib, ctx.handle, st, "evaluate_stage_tendencies")
     return out
+
+
+def uwcdi_strang_step(state: FSIState, dt: float, ctx: IMEXContext, work: IMEXWorkspace,
+                      first_direction: str = "x") -> np.ndarray:
+    """uwcdi_strang_step (PhaseFieldMethods.jl:793-850) of the context's
+    PirozzoliUWCDIPhaseField: the Strang-split directional update of
+    state.phi by the state's velocity.  state.phi is replaced by the result,
+    which is also returned."""
+    if first_direction not in ("x", "y"):
+        raise ArgumentError(f"first_direction must be :x or :y; received {first_direction}.")
+    h = state._device(ctx)
+    st = ctx.lib.fsi_uwcdi_strang_step(ctx.handle, h, work.handle, float(dt), 0 if first_direction == "x" else 1)
+    state._device_modified()
+    _abi.check(ctx.lib, ctx.handle, st, "uwcdi_strang_step")
+    return state.phi

@@ -91,6 +91,13 @@ typedef enum {
     FSI_FORCE_KOLMOGOROV = 2 /* KolmogorovBodyForce(amplitude, wavenumber; phase) */
 } fsi_body_force_kind;
 
+typedef enum {
+    FSI_UWCDI_NORMAL_STANDARD = 0,  /* :standard  centred gradient of phi                 */
+    FSI_UWCDI_NORMAL_YOUNGS = 1,    /* :youngs    Youngs' 3x3 stencil off the walls       */
+    FSI_UWCDI_NORMAL_SMOOTHED9 = 2  /* :smoothed9 centred gradient of the 9-point-filtered phi */
+} fsi_uwcdi_normal;
+typedef enum { FSI_CAPILLARITY_NONE = 0, FSI_CAPILLARITY_DIFFUSE_AREA = 1 } fsi_capillarity_kind;
+
 typedef struct {
     int32_t phase_method;      /* fsi_phase_method */
     /* CDI: ConservativeDiffuseInterfaceFlux(inverse_peclet, epsilon; delta_normal) */
@@ -107,6 +114,13 @@ typedef struct {
     int32_t body_force_kind;
     double force_fx, force_fy;                         /* ConstantBodyForce              */
     double force_amplitude, force_wavenumber, force_phase; /* KolmogorovBodyForce        */
+    /* UWCDI normal_reconstruction (PhaseFieldMethods.jl:96-120, :246-404): fsi_uwcdi_normal */
+    int32_t uwcdi_normal_method;
+    /* capillarity (Capillarity.jl:21-41): fsi_capillarity_kind;
+     * DiffuseAreaCapillarity(coefficient; delta_normal, smoothing_passes) */
+    int32_t capillarity_kind;
+    double capillary_coefficient, capillary_delta_normal;
+    int32_t capillary_smoothing_passes;
 } fsi_models;
 
 /* ---- numerics (IMEXNumerics, src/TimeIntegrationIMEX.jl:262-301) -------- */
@@ -277,6 +291,16 @@ fsi_status fsi_eval_tendencies(fsi_context* ctx, const fsi_state* st, fsi_worksp
                                double t,
                                double* E_u, double* E_v, double* I_u, double* I_v,
                                double* E_phi, double* E_sxx, double* E_syy, double* E_sxy);
+
+/* uwcdi_strang_step (src/PhaseFieldMethods.jl:793-850): the Strang-split
+ * directional UWCDI update of the state's phi by the state's velocity,
+ * Euler sweeps x (dt/2), y (dt), x (dt/2) of uwcdi_directional_rhs
+ * (:768-791); first_direction 0 = :x, 1 = :y (y, x, y).  phi is replaced by
+ * the result (the reference returns it as a new array).  FSI_E_ARGUMENT when
+ * the context's phase field is not UWCDI, dt is negative or not finite, or
+ * first_direction is neither.  Uses the workspace's stage E_phi buffers. */
+fsi_status fsi_uwcdi_strang_step(fsi_context* ctx, fsi_state* st, fsi_workspace* ws, double dt,
+                                 int32_t first_direction);
 
 /* ---- driver diagnostics ---------------------------------------------------
  * The per-step and per-interval reductions of the reference's cavity driver

@@ -972,20 +972,104 @@ static int phase_rhs_acdi(fsio_ctx* c, const double* U, const double* V, const d
 
 /* UWCDI phase_field_rhs PhaseFieldMethods.jl:852-867 with _uwcdi_fluxes
  * :675-766 and _normal_components_on_faces :246-278 (:standard). */
-static int phase_rhs_uwcdi(fsio_ctx* c, const double* U, const double* V, const double* phi,
-                           const double* S, double* out) {
+/* The 9-point filter of _smoothed9_field (PhaseFieldMethods.jl:364-383) and
+ * of _smooth_phase_for_capillarity (Capillarity.jl:81-111): index clamped at a
+ * wall, wrapped across a periodic side. */
+static void smooth9(fsio_ctx* c, const double* f, double* out) {
+    int nx = c->nx, ny = c->ny;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            int im = i - 1, ip = i + 1, jm = j - 1, jp = j + 1;
+            if (c->px) { im = (im + nx) % nx; ip = ip % nx; }
+            else { im = im < 0 ? 0 : im; ip = ip > nx - 1 ? nx - 1 : ip; }
+            if (c->py) { jm = (jm + ny) % ny; jp = jp % ny; }
+            else { jm = jm < 0 ? 0 : jm; jp = jp > ny - 1 ? ny - 1 : jp; }
+            CC(out, i, j) = 0.25 * CC(f, i, j) +
+                            0.125 * (CC(f, ip, j) + CC(f, im, j) + CC(f, i, jp) + CC(f, i, jm)) +
+                            0.0625 * (CC(f, ip, jp) + CC(f, im, jp) + CC(f, ip, jm) + CC(f, im, jm));
+        }
+}
+
+/* Cell normals of the UWCDI model: _center_normals of phi (:standard), of
+ * the 9-point-filtered phi (:smoothed9, :341-404, homogeneous Neumann or
+ * periodic phase sides only), or Youngs' 3x3 stencil on every cell off the
+ * non-periodic walls (:youngs, :280-339). */
+static int uwcdi_center_normals(fsio_ctx* c, const double* phi, double* nxc, double* nyc) {
     const metrics_t* m = &c->mt;
-    int nx = c->nx, ny = c->ny, rc = 0;
+    int nx = c->nx, ny = c->ny, method = c->m.uwcdi_normal_method;
     size_t nc = (size_t)nx * ny, nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
-    double *gx = dalloc(nc), *gy = dalloc(nc), *gxf = dalloc(nfx), *gyf = dalloc(nfy);
-    double *nxc = dalloc(nc), *nyc = dalloc(nc), *nxf = dalloc(nfx), *nyf = dalloc(nfy);
-    double *nyx = dalloc(nfx), *nxy = dalloc(nfy), *fx = dalloc(nfx), *fy = dalloc(nfy);
     double dn = c->m.uwcdi_delta_normal, d2 = dn * dn;
-    center_gradient(c, phi, &c->bcs.phi, gx, gy, gxf, gyf);
+    const double* field = phi;
+    double* smooth = NULL;
+    if (method == FSI_UWCDI_NORMAL_SMOOTHED9) {
+        const fsi_bc_side* sides[4] = {&c->bcs.phi.left, &c->bcs.phi.right, &c->bcs.phi.bottom, &c->bcs.phi.top};
+        const char* names[4] = {"left", "right", "bottom", "top"};
+        for (int k = 0; k < 4; ++k) {
+            if (sides[k]->kind == FSI_BC_PERIODIC) continue;
+            if (sides[k]->kind == FSI_BC_NEUMANN && fabs(sides[k]->value) <= 100 * 2.220446049250313e-16) continue;
+            return set_err(c, FSI_E_ARGUMENT,
+                           "smoothed9 normal reconstruction requires homogeneous Neumann or periodic phase "
+                           "boundaries; %s does not.", names[k]);
+        }
+        smooth = dalloc(nc);
+        smooth9(c, phi, smooth);
+        field = smooth;
+    }
+    double *gx = dalloc(nc), *gy = dalloc(nc), *gxf = dalloc(nfx), *gyf = dalloc(nfy);
+    center_gradient(c, field, &c->bcs.phi, gx, gy, gxf, gyf);
     for (size_t k = 0; k < nc; ++k) {
         double mag = sqrt(gx[k] * gx[k] + gy[k] * gy[k] + d2);
         nxc[k] = gx[k] / mag;
         nyc[k] = gy[k] / mag;
+    }
+    if (method == FSI_UWCDI_NORMAL_YOUNGS) {
+        double hx = m->dx_c[0], hy = m->dy_c[0], ex = 0.0, ey = 0.0;
+        for (int i = 0; i < nx; ++i) ex = fmax(ex, fabs(m->dx_c[i] - hx));
+        for (int j = 0; j < ny; ++j) ey = fmax(ey, fabs(m->dy_c[j] - hy));
+        if (!(ex <= 100 * 2.220446049250313e-16 * fmax(1.0, fabs(hx))) ||
+            !(ey <= 100 * 2.220446049250313e-16 * fmax(1.0, fabs(hy)))) {
+            free(gx); free(gy); free(gxf); free(gyf);
+            return set_err(c, FSI_E_ARGUMENT, "Youngs normal reconstruction currently requires a uniform %s grid.",
+                           ex <= 100 * 2.220446049250313e-16 * fmax(1.0, fabs(hx)) ? "y" : "x");
+        }
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                if ((!c->px && (i == 0 || i == nx - 1)) || (!c->py && (j == 0 || j == ny - 1))) continue;
+                int im = c->px ? (i - 1 + nx) % nx : i - 1, ip = c->px ? (i + 1) % nx : i + 1;
+                int jm = c->py ? (j - 1 + ny) % ny : j - 1, jp = c->py ? (j + 1) % ny : j + 1;
+                double ggx = (CC(phi, ip, jm) + 2.0 * CC(phi, ip, j) + CC(phi, ip, jp) - CC(phi, im, jm) -
+                              2.0 * CC(phi, im, j) - CC(phi, im, jp)) / (8.0 * hx);
+                double ggy = (CC(phi, im, jp) + 2.0 * CC(phi, i, jp) + CC(phi, ip, jp) - CC(phi, im, jm) -
+                              2.0 * CC(phi, i, jm) - CC(phi, ip, jm)) / (8.0 * hy);
+                double mag = sqrt(ggx * ggx + ggy * ggy + d2);
+                CC(nxc, i, j) = ggx / mag;
+                CC(nyc, i, j) = ggy / mag;
+            }
+    }
+    free(gx); free(gy); free(gxf); free(gyf);
+    if (smooth) free(smooth);
+    return 0;
+}
+
+int fsio_uwcdi_center_normals(fsio_ctx* c, const double* phi, double* nxc, double* nyc) {
+    return uwcdi_center_normals(c, phi, nxc, nyc);
+}
+void fsio_smooth9(fsio_ctx* c, const double* f, double* out) { smooth9(c, f, out); }
+
+/* dir: 0 both directions (phase_field_rhs, with the phi S source), 1 the
+ * x-fluxes alone, 2 the y-fluxes alone (uwcdi_directional_rhs :768-791) */
+static int phase_rhs_uwcdi_dir(fsio_ctx* c, const double* U, const double* V, const double* phi,
+                               const double* S, double* out, int dir) {
+    const metrics_t* m = &c->mt;
+    int nx = c->nx, ny = c->ny, rc = 0;
+    size_t nc = (size_t)nx * ny, nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
+    double *nxc = dalloc(nc), *nyc = dalloc(nc), *nxf = dalloc(nfx), *nyf = dalloc(nfy);
+    double *nyx = dalloc(nfx), *nxy = dalloc(nfy), *fx = dalloc(nfx), *fy = dalloc(nfy);
+    double dn = c->m.uwcdi_delta_normal, d2 = dn * dn;
+    rc = uwcdi_center_normals(c, phi, nxc, nyc);
+    if (rc) {
+        free(nxc); free(nyc); free(nxf); free(nyf); free(nyx); free(nxy); free(fx); free(fy);
+        return rc;
     }
     c2fx(c, nxc, nxf);
     c2fy(c, nyc, nyf);
@@ -1045,14 +1129,98 @@ static int phase_rhs_uwcdi(fsio_ctx* c, const double* U, const double* V, const 
             for (int i = 0; i < nx; ++i) VY(fy, i, ny) = VY(fy, i, 0);
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
-                double dF = (UX(fx, i + 1, j) - UX(fx, i, j)) / m->dx_c[i] +
-                            (VY(fy, i, j + 1) - VY(fy, i, j)) / m->dy_c[j];
-                CC(out, i, j) = -dF + CC(phi, i, j) * CC(S, i, j);
+                double dFx = (UX(fx, i + 1, j) - UX(fx, i, j)) / m->dx_c[i];
+                double dFy = (VY(fy, i, j + 1) - VY(fy, i, j)) / m->dy_c[j];
+                if (dir == 1) CC(out, i, j) = -(dFx + 0.0);
+                else if (dir == 2) CC(out, i, j) = -(0.0 + dFy);
+                else CC(out, i, j) = -(dFx + dFy) + CC(phi, i, j) * CC(S, i, j);
             }
     }
-    free(gx); free(gy); free(gxf); free(gyf); free(nxc); free(nyc); free(nxf); free(nyf);
+    free(nxc); free(nyc); free(nxf); free(nyf);
     free(nyx); free(nxy); free(fx); free(fy);
     return rc;
+}
+
+static int phase_rhs_uwcdi(fsio_ctx* c, const double* U, const double* V, const double* phi,
+                           const double* S, double* out) {
+    return phase_rhs_uwcdi_dir(c, U, V, phi, S, out, 0);
+}
+
+/* uwcdi_strang_step PhaseFieldMethods.jl:793-850: Euler sweeps x(dt/2),
+ * y(dt), x(dt/2) of the directional right-hand sides (first_direction 1 = :y
+ * swaps the roles); phi is replaced by the result. */
+int fsio_uwcdi_strang_step(fsio_ctx* c, const double* U, const double* V, double* phi, fsio_scalar dt,
+                           int first_direction) {
+    if (c->m.phase_method != FSI_PHASE_UWCDI)
+        return set_err(c, FSI_E_ARGUMENT, "uwcdi_strang_step needs the UWCDI phase field.");
+    if (!(isfinite(dt) && dt >= 0.0)) return set_err(c, FSI_E_ARGUMENT, "dt must be finite and non-negative.");
+    if (first_direction != 0 && first_direction != 1)
+        return set_err(c, FSI_E_ARGUMENT, "first_direction must be :x or :y.");
+    int N = c->N, rc = 0;
+    const int d1 = first_direction == 0 ? 1 : 2, d2 = first_direction == 0 ? 2 : 1;
+    const int dirs[3] = {d1, d2, d1};
+    const double h[3] = {0.5 * dt, dt, 0.5 * dt};
+    double *rhs = dalloc(N), *q = dalloc(N);
+    memcpy(q, phi, sizeof(double) * N);
+    for (int k = 0; k < 3 && !rc; ++k) {
+        if (h[k] == 0.0) continue;  /* _uwcdi_directional_euler_sweep returns phi at dt = 0 */
+        rc = phase_rhs_uwcdi_dir(c, U, V, q, NULL, rhs, dirs[k]);
+        for (int n = 0; n < N && !rc; ++n) q[n] = q[n] + h[k] * rhs[n];
+    }
+    if (!rc) memcpy(phi, q, sizeof(double) * N);
+    free(rhs);
+    free(q);
+    return rc;
+}
+
+/* capillary_force(::DiffuseAreaCapillarity) Capillarity.jl:134-187:
+ * f = coefficient kappa grad(phi) on the faces, kappa = -div(n) of the
+ * regularised normal of the (optionally filtered) phase field, the normal
+ * and the curvature carried to the faces by interpolate_center_to_*_faces. */
+static void capillary_force(fsio_ctx* c, const double* phi, double* fxo, double* fyo) {
+    const metrics_t* m = &c->mt;
+    int nx = c->nx, ny = c->ny;
+    size_t nc = (size_t)nx * ny, nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
+    double *geo = dalloc(nc), *tmp = dalloc(nc);
+    memcpy(geo, phi, sizeof(double) * nc);
+    for (int k = 0; k < c->m.capillary_smoothing_passes; ++k) {
+        smooth9(c, geo, tmp);
+        double* t = geo; geo = tmp; tmp = t;
+    }
+    double *ggx = dalloc(nfx), *ggy = dalloc(nfy), *gx = dalloc(nfx), *gy = dalloc(nfy);
+    scalar_gradient(c, &c->bcs.phi, geo, ggx, ggy);
+    scalar_gradient(c, &c->bcs.phi, phi, gx, gy);
+    double *nxc = dalloc(nc), *nyc = dalloc(nc), *nxf = dalloc(nfx), *nyf = dalloc(nfy), *kap = dalloc(nc);
+    double d2 = c->m.capillary_delta_normal * c->m.capillary_delta_normal;
+    for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+            double gxc = m->wL_f2cx[i] * UX(ggx, i, j) + m->wR_f2cx[i] * UX(ggx, i + 1, j);
+            double gyc = m->wB_f2cy[j] * VY(ggy, i, j) + m->wT_f2cy[j] * VY(ggy, i, j + 1);
+            double den = sqrt(gxc * gxc + gyc * gyc + d2);
+            CC(nxc, i, j) = gxc / den;
+            CC(nyc, i, j) = gyc / den;
+        }
+    c2fx(c, nxc, nxf);
+    c2fy(c, nyc, nyf);
+    divergence(c, nxf, nyf, kap);
+    for (size_t k = 0; k < nc; ++k) kap[k] = -kap[k];
+    double *kx = dalloc(nfx), *ky = dalloc(nfy);
+    c2fx(c, kap, kx);
+    c2fy(c, kap, ky);
+    double coef = c->m.capillary_coefficient;
+    for (size_t k = 0; k < nfx; ++k) fxo[k] = coef * kx[k] * gx[k];
+    for (size_t k = 0; k < nfy; ++k) fyo[k] = coef * ky[k] * gy[k];
+    free(geo); free(tmp); free(ggx); free(ggy); free(gx); free(gy);
+    free(nxc); free(nyc); free(nxf); free(nyf); free(kap); free(kx); free(ky);
+}
+
+void fsio_capillary_force(fsio_ctx* c, const double* phi, double* fx, double* fy) {
+    if (c->m.capillarity_kind == FSI_CAPILLARITY_DIFFUSE_AREA) {
+        capillary_force(c, phi, fx, fy);
+        return;
+    }
+    memset(fx, 0, sizeof(double) * (size_t)(c->nx + 1) * c->ny);
+    memset(fy, 0, sizeof(double) * (size_t)c->nx * (c->ny + 1));
 }
 
 static int phase_rhs(fsio_ctx* c, const double* U, const double* V, const double* phi,
@@ -1865,6 +2033,12 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i <= nx; ++i) UX(E_u, i, j) += c->force_row[j];
         for (size_t k = 0; k < nfy; ++k) E_v[k] += 0.0;
+    }
+    /* capillary_force (Capillarity.jl:134-187), :975-977 */
+    if (c->m.capillarity_kind == FSI_CAPILLARITY_DIFFUSE_AREA) {
+        capillary_force(c, phi, Fx, Fy);
+        for (size_t k = 0; k < nfx; ++k) E_u[k] += Fx[k];
+        for (size_t k = 0; k < nfy; ++k) E_v[k] += Fy[k];
     }
     /* _accumulate_elastic_force! :903-924 with elastic_stress
        StructuralConstitutive.jl:100-113 */

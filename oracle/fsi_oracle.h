@@ -72,6 +72,16 @@ void fsio_laplacian_apply(fsio_ctx* c, const double* q, double* out, int with_bo
  * stage sum before the alpha*b term.  Returns CG iterations in *iters. */
 int fsio_velocity_solve(fsio_ctx* c, fsio_scalar alpha, const double* rhs, const double* x0,
                         double* x, int* iters);
+/* uwcdi_strang_step (PhaseFieldMethods.jl:793-850): phi replaced by the
+ * Strang-split directional step; first_direction 0 = :x, 1 = :y. */
+int fsio_uwcdi_strang_step(fsio_ctx* c, const double* U, const double* V, double* phi, fsio_scalar dt,
+                           int first_direction);
+/* The UWCDI centre normals of the context's normal_reconstruction, and the
+ * 9-point filter they and the capillary geometry use. */
+int fsio_uwcdi_center_normals(fsio_ctx* c, const double* phi, double* nxc, double* nyc);
+void fsio_smooth9(fsio_ctx* c, const double* f, double* out);
+/* capillary_force of the context's capillarity model (zero for NoCapillarity). */
+void fsio_capillary_force(fsio_ctx* c, const double* phi, double* fx, double* fy);
 /* Metric vectors: which 0..15 in MeshMetrics field order. */
 const double* fsio_metric(const fsio_ctx* c, int which, int* len);
 

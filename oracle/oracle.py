@@ -65,6 +65,10 @@ def load(path: Path | None = None):
         "fsio_laplacian_apply": (None, [P, _dp, _dp, c_int]),
         "fsio_velocity_solve": (c_int, [P, c_double, _dp, _dp, _dp, POINTER(c_int)]),
         "fsio_metric": (_dp, [P, c_int, POINTER(c_int)]),
+        "fsio_uwcdi_strang_step": (c_int, [P, _dp, _dp, _dp, c_double, c_int]),
+        "fsio_capillary_force": (None, [P, _dp, _dp, _dp]),
+        "fsio_uwcdi_center_normals": (c_int, [P, _dp, _dp, _dp]),
+        "fsio_smooth9": (None, [P, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -191,6 +195,32 @@ class Oracle:
         out = np.zeros((self.nx, self.ny), order="F")
         self._check(self.lib.fsio_phase_rhs(self.h, _p(_F(U)), _p(_F(V)), _p(_F(phi)), _p(S), _p(out)))
         return out
+
+    def uwcdi_strang_step(self, U, V, phi, dt, first_direction="x"):
+        """uwcdi_strang_step (PhaseFieldMethods.jl:793-850); returns the new phi."""
+        out = np.array(phi, dtype=np.float64, order="F", copy=True)
+        fd = {"x": 0, "y": 1}.get(first_direction, 2)
+        self._check(self.lib.fsio_uwcdi_strang_step(self.h, _p(_F(U)), _p(_F(V)), _p(out), float(dt), fd))
+        return out
+
+    def uwcdi_center_normals(self, phi):
+        """The UWCDI centre normals (PhaseFieldMethods.jl:234-245, 280-339, 385-390)."""
+        nxc = np.zeros((self.nx, self.ny), order="F")
+        nyc = np.zeros((self.nx, self.ny), order="F")
+        self._check(self.lib.fsio_uwcdi_center_normals(self.h, _p(_F(phi)), _p(nxc), _p(nyc)))
+        return nxc, nyc
+
+    def smooth9(self, f):
+        out = np.zeros((self.nx, self.ny), order="F")
+        self.lib.fsio_smooth9(self.h, _p(_F(f)), _p(out))
+        return out
+
+    def capillary_force(self, phi):
+        """capillary_force of the context's model (Capillarity.jl:121-187)."""
+        fx = np.zeros((self.nx + 1, self.ny), order="F")
+        fy = np.zeros((self.nx, self.ny + 1), order="F")
+        self.lib.fsio_capillary_force(self.h, _p(_F(phi)), _p(fx), _p(fy))
+        return fx, fy
 
     def structure_rhs(self, sxx, syy, sxy, U, V, phi):
         outs = [np.zeros((self.nx, self.ny), order="F") for _ in range(3)]

@@ -1,6 +1,8 @@
 """Writes tests/golden/phase_rhs.npz: seeded inputs (U, V, phi, S) and the
 phase right-hand side of each (configuration, phase model) pair below,
-evaluated by the NumPy restatement in tests/reference_phase.py.
+evaluated by the NumPy restatement in tests/reference_phase.py; the
+DiffuseAreaCapillarity force of seeded phase fields (capillary/...) and
+uwcdi_strang_step of seeded states (strang/...) from the same restatement.
 
 The configurations cover every boundary branch of the reference's phase
 operators (src/InterfaceTransport.jl, src/InterfaceRegularization.jl,
@@ -47,7 +49,14 @@ def models():
         "acdi_dynamic": C.JainACDIPhaseField(1.3, 0.06, psi_floor=1e-12, delta_normal=1e-10),
         "uwcdi": C.PirozzoliUWCDIPhaseField(1.0),
         "uwcdi_alpha": C.PirozzoliUWCDIPhaseField(1.6, delta_normal=1e-8),
+        "uwcdi_youngs": C.PirozzoliUWCDIPhaseField(1.0, normal_reconstruction="youngs"),
+        "uwcdi_smoothed9": C.PirozzoliUWCDIPhaseField(1.3, delta_normal=1e-8, normal_reconstruction="smoothed9"),
     }
+
+
+def model_applies(cname, mname):
+    """:smoothed9 accepts homogeneous Neumann or periodic phase sides only."""
+    return mname != "uwcdi_smoothed9" or all(s in ("N", "P") for s in CONFIGS[cname]["phi"])
 
 
 def _side(s):
@@ -143,18 +152,56 @@ def cases():
     out = []
     for ci, cname in enumerate(CONFIGS):
         for mi, mname in enumerate(models()):
-            out.append((f"{cname}/{mname}", cname, mname, 100 * ci + mi))
+            if model_applies(cname, mname):
+                out.append((f"{cname}/{mname}", cname, mname, 100 * ci + mi))
     return out
 
 
-def restated_rhs(cname, mname, U, V, phi, S):
+# DiffuseAreaCapillarity(coefficient; delta_normal, smoothing_passes) of the capillary fixtures
+CAPILLARITY = {"plain": (0.7, 1e-6, 0), "filtered": (0.45, 1e-5, 2)}
+STRANG_DT = 0.01
+
+
+def capillary_cases():
+    """(key, configuration, capillarity name, seed): the force of the seeded phi."""
+    return [(f"capillary/{cname}/{k}", cname, k, 1000 + 10 * ci + n)
+            for ci, cname in enumerate(CONFIGS) for n, k in enumerate(CAPILLARITY)]
+
+
+def strang_cases():
+    """(key, configuration, UWCDI model name, first direction, seed)."""
+    out = []
+    for ci, cname in enumerate(CONFIGS):
+        for mname in ("uwcdi", "uwcdi_youngs", "uwcdi_smoothed9"):
+            if model_applies(cname, mname):
+                for d in ("x", "y"):
+                    out.append((f"strang/{cname}/{mname}/{d}", cname, mname, d, 2000 + 10 * ci + len(out) % 10))
+    return out
+
+
+def _geometry(cname, mname):
     from tests import reference_assembly as RA
     from tests import reference_phase as RP
     from oracle.oracle import Oracle
     metrics, bcs, mdl, num = configuration(cname, models()[mname])
     met = RA.Met(Oracle(metrics, bcs, mdl, num))
-    g = RP.PhaseGeometry(met, bcs.phi, metrics.periodic_x, metrics.periodic_y)
+    return RP, RP.PhaseGeometry(met, bcs.phi, metrics.periodic_x, metrics.periodic_y), mdl
+
+
+def restated_rhs(cname, mname, U, V, phi, S):
+    RP, g, mdl = _geometry(cname, mname)
     return RP.phase_rhs(g, mdl.phase_field_method, U, V, phi, S)
+
+
+def restated_capillary(cname, kname, phi):
+    RP, g, _ = _geometry(cname, "uwcdi")
+    coef, delta, passes = CAPILLARITY[kname]
+    return RP.capillary_force(g, phi, coef, delta, passes)
+
+
+def restated_strang(cname, mname, d, U, V, phi):
+    RP, g, mdl = _geometry(cname, mname)
+    return RP.uwcdi_strang_step(g, mdl.phase_field_method, U, V, phi, STRANG_DT, d)
 
 
 def main():
@@ -165,6 +212,18 @@ def main():
         for f, a in (("U", U), ("V", V), ("phi", phi), ("S", S), ("rhs", rhs)):
             data[f"{key}/{f}"] = a
         print(key, f"max|rhs| = {np.max(np.abs(rhs)):.4e}")
+    for key, cname, kname, seed in capillary_cases():
+        phi = inputs(cname, seed)[2]
+        fx, fy = restated_capillary(cname, kname, phi)
+        for f, a in (("phi", phi), ("fx", fx), ("fy", fy)):
+            data[f"{key}/{f}"] = a
+        print(key, f"max|f| = {max(np.max(np.abs(fx)), np.max(np.abs(fy))):.4e}")
+    for key, cname, mname, d, seed in strang_cases():
+        U, V, phi, _ = inputs(cname, seed)
+        out = restated_strang(cname, mname, d, U, V, phi)
+        for f, a in (("U", U), ("V", V), ("phi", phi), ("out", out)):
+            data[f"{key}/{f}"] = a
+        print(key, f"max|dphi| = {np.max(np.abs(out - phi)):.4e}")
     np.savez_compressed(PATH, **data)
     print("wrote", PATH)
 

@@ -4,7 +4,10 @@ sides, written from the Julia sources independently of the C oracle:
   upwind material flux        src/InterfaceTransport.jl:79-276
   CDI sharpening flux         src/InterfaceRegularization.jl:99-228
   ACDI (Jain)                 src/PhaseFieldMethods.jl:478-671
-  UWCDI (Pirozzoli, standard) src/PhaseFieldMethods.jl:219-283, 675-867
+  UWCDI (Pirozzoli)           src/PhaseFieldMethods.jl:219-404, 675-867 (the
+                              three normal reconstructions, the directional
+                              right-hand sides and uwcdi_strang_step)
+  DiffuseAreaCapillarity      src/Capillarity.jl:59-187
   face gradient / c2f / div   src/Operators.jl:110-159, 395-425 (gradient rows
                               from the sparse assembly in reference_assembly)
 
@@ -238,10 +241,52 @@ def acdi_rhs(g, U, V, phi, S, gamma, epsilon, floor=1e-100, delta=1e-14, dynamic
 # ---------------------------------------------------------------------------
 # UWCDI (PhaseFieldMethods.jl:675-867)
 
-def uwcdi_face_normals(g, phi, delta):
+def smoothed9(g, f):
+    """The 9-point filter (PhaseFieldMethods.jl:364-383, Capillarity.jl:81-111):
+    weights 1/4 (centre), 1/8 (edges), 1/16 (corners); an index past a wall
+    is clamped, one across a periodic side wrapped."""
+    nx, ny = f.shape
+    ix = lambda off: (np.arange(nx) + off) % nx if g.px else np.clip(np.arange(nx) + off, 0, nx - 1)  # noqa: E731
+    iy = lambda off: (np.arange(ny) + off) % ny if g.py else np.clip(np.arange(ny) + off, 0, ny - 1)  # noqa: E731
+    F = lambda a, b: f[np.ix_(ix(a), iy(b))]  # noqa: E731
+    return (0.25 * f + 0.125 * (F(1, 0) + F(-1, 0) + F(0, 1) + F(0, -1)) +
+            0.0625 * (F(1, 1) + F(-1, 1) + F(1, -1) + F(-1, -1)))
+
+
+def uwcdi_center_normals(g, phi, delta, method="standard"):
+    """Centre normals of the UWCDI normal_reconstruction options
+    (PhaseFieldMethods.jl:684-707): :standard, :youngs (:280-339),
+    :smoothed9 (:341-404)."""
+    if method == "smoothed9":
+        for side in (g.bc.left, g.bc.right, g.bc.bottom, g.bc.top):
+            kind = type(side).__name__
+            assert kind == "Periodic" or (kind == "Neumann" and abs(float(side.value)) <= 100 * np.finfo(float).eps)
+        return center_normals(g, smoothed9(g, phi), g.bc, delta)
+    nxc, nyc = center_normals(g, phi, g.bc, delta)
+    if method == "youngs":
+        nx, ny = phi.shape
+        hx, hy = g.m.dx_c[0], g.m.dy_c[0]
+        im, ip = (np.arange(nx) - 1) % nx, (np.arange(nx) + 1) % nx
+        jm, jp = (np.arange(ny) - 1) % ny, (np.arange(ny) + 1) % ny
+        I, J = np.arange(nx), np.arange(ny)
+        P = lambda a, b: phi[np.ix_(a, b)]  # noqa: E731
+        gx = (P(ip, jm) + 2.0 * P(ip, J) + P(ip, jp) - P(im, jm) - 2.0 * P(im, J) - P(im, jp)) / (8.0 * hx)
+        gy = (P(im, jp) + 2.0 * P(I, jp) + P(ip, jp) - P(im, jm) - 2.0 * P(I, jm) - P(ip, jm)) / (8.0 * hy)
+        mag = np.sqrt(gx ** 2 + gy ** 2 + delta ** 2)
+        keep = np.ones((nx, ny), dtype=bool)   # cells whose 3x3 stencil crosses a wall keep :standard
+        if not g.px:
+            keep[[0, -1], :] = False
+        if not g.py:
+            keep[:, [0, -1]] = False
+        nxc = np.where(keep, gx / mag, nxc)
+        nyc = np.where(keep, gy / mag, nyc)
+    return nxc, nyc
+
+
+def uwcdi_face_normals(g, phi, delta, method="standard"):
     """_normal_components_on_faces: centre normals interpolated to the faces
     and renormalised with the transverse component."""
-    nxc, nyc = center_normals(g, phi, g.bc, delta)
+    nxc, nyc = uwcdi_center_normals(g, phi, delta, method)
     nxf, nyx = g.c2f_x(nxc), g.c2f_x(nyc)
     nyf, nxy = g.c2f_y(nyc), g.c2f_y(nxc)
     nxf = nxf / np.sqrt(nxf ** 2 + nyx ** 2 + delta ** 2)
@@ -253,9 +298,10 @@ def uwcdi_face_normals(g, phi, delta):
     return nxf, nyf
 
 
-def uwcdi_rhs(g, U, V, phi, S, alpha=1.0, delta=1e-14):
+def uwcdi_fluxes(g, U, V, phi, alpha=1.0, delta=1e-14, method="standard"):
+    """_uwcdi_fluxes (PhaseFieldMethods.jl:675-766)."""
     m = g.m
-    nxf, nyf = uwcdi_face_normals(g, phi, delta)
+    nxf, nyf = uwcdi_face_normals(g, phi, delta, method)
     # upwind face values
     up_x = np.empty((g.nx + 1, g.ny))
     up_x[1:-1] = np.where(U[1:-1] >= 0.0, phi[:-1], phi[1:])
@@ -280,7 +326,47 @@ def uwcdi_rhs(g, U, V, phi, S, alpha=1.0, delta=1e-14):
         fx[-1] = fx[0]
     if g.py:
         fy[:, -1] = fy[:, 0]
+    return fx, fy
+
+
+def uwcdi_rhs(g, U, V, phi, S, alpha=1.0, delta=1e-14, method="standard"):
+    fx, fy = uwcdi_fluxes(g, U, V, phi, alpha, delta, method)
     return -g.div(fx, fy) + phi * S
+
+
+def uwcdi_directional_rhs(g, model, direction, U, V, phi):
+    """uwcdi_directional_rhs (PhaseFieldMethods.jl:768-791): minus the
+    divergence of one flux component, the other set to zero."""
+    fx, fy = uwcdi_fluxes(g, U, V, phi, model.alpha, model.delta_normal, model.normal_reconstruction)
+    if direction == "x":
+        return -g.div(fx, np.zeros((g.nx, g.ny + 1)))
+    return -g.div(np.zeros((g.nx + 1, g.ny)), fy)
+
+
+def uwcdi_strang_step(g, model, U, V, phi, dt, first_direction="x"):
+    """uwcdi_strang_step (PhaseFieldMethods.jl:793-850): Euler sweeps a (dt/2),
+    b (dt), a (dt/2) with (a, b) = (x, y) or (y, x)."""
+    a, b = ("x", "y") if first_direction == "x" else ("y", "x")
+    q = np.array(phi, dtype=np.float64)
+    for d, h in ((a, 0.5 * dt), (b, dt), (a, 0.5 * dt)):
+        if h != 0.0:
+            q = q + h * uwcdi_directional_rhs(g, model, d, U, V, q)
+    return q
+
+
+def capillary_force(g, phi, coefficient, delta_normal=1e-6, smoothing_passes=0):
+    """capillary_force(::DiffuseAreaCapillarity) (Capillarity.jl:134-187):
+    the normal of the (optionally filtered) field, kappa = -div(n), and
+    coefficient kappa grad(phi) on both face sets."""
+    geo = np.array(phi, dtype=np.float64)
+    for _ in range(smoothing_passes):
+        geo = smoothed9(g, geo)
+    ggx, ggy = g.gradient(geo)
+    gx, gy = g.gradient(phi)
+    gxc, gyc = g.f2c(ggx, ggy)
+    den = np.sqrt(gxc * gxc + gyc * gyc + delta_normal ** 2)
+    kappa = -g.div(g.c2f_x(gxc / den), g.c2f_y(gyc / den))
+    return coefficient * g.c2f_x(kappa) * gx, coefficient * g.c2f_y(kappa) * gy
 
 
 def phase_rhs(g, model, U, V, phi, S):
@@ -300,5 +386,5 @@ def phase_rhs(g, model, U, V, phi, S):
         return acdi_rhs(g, U, V, phi, S, model.gamma, model.epsilon, model.psi_floor, model.delta_normal,
                         model.dynamic_gamma)
     if name == "PirozzoliUWCDIPhaseField":
-        return uwcdi_rhs(g, U, V, phi, S, model.alpha, model.delta_normal)
+        return uwcdi_rhs(g, U, V, phi, S, model.alpha, model.delta_normal, model.normal_reconstruction)
     raise ValueError(name)

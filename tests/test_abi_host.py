@@ -71,9 +71,30 @@ def test_abi_config_of_filament():
 
 def test_unsupported_closures_are_rejected():
     case = setups.lid_driven_cavity(16, "uwcdi")
-    models = fsi.FSIModels(phase_field_method=C.PirozzoliUWCDIPhaseField(1.0, normal_reconstruction="youngs"))
-    with pytest.raises(fsi.UnsupportedError):
-        _abi.abi_config(case.metrics, case.bcs, models, case.numerics)
+
+    class SomeGrowth:
+        pass
+
+    for kw in (dict(growth_source=SomeGrowth()), dict(capillarity=SomeGrowth())):
+        models = fsi.FSIModels(phase_field_method=C.PirozzoliUWCDIPhaseField(1.0), **kw)
+        with pytest.raises(fsi.UnsupportedError):
+            _abi.abi_config(case.metrics, case.bcs, models, case.numerics)
+
+
+def test_uwcdi_normals_and_capillarity_reach_the_abi():
+    case = setups.lid_driven_cavity(16, "uwcdi")
+    for name, code in (("standard", 0), ("youngs", 1), ("smoothed9", 2)):
+        models = fsi.FSIModels(phase_field_method=C.PirozzoliUWCDIPhaseField(1.0, normal_reconstruction=name))
+        m = _abi.abi_config(case.metrics, case.bcs, models, case.numerics)[2]
+        assert m.uwcdi_normal_method == code and m.capillarity_kind == 0
+    models = fsi.FSIModels(capillarity=C.DiffuseAreaCapillarity(0.3, delta_normal=1e-5, smoothing_passes=2))
+    m = _abi.abi_config(case.metrics, case.bcs, models, case.numerics)[2]
+    assert (m.capillarity_kind, m.capillary_coefficient, m.capillary_delta_normal,
+            m.capillary_smoothing_passes) == (1, 0.3, 1e-5, 2)
+    for bad in (dict(coefficient=-1.0), dict(coefficient=float("nan")), dict(coefficient=1.0, delta_normal=0.0),
+                dict(coefficient=1.0, smoothing_passes=-1)):
+        with pytest.raises(fsi.ArgumentError):
+            C.DiffuseAreaCapillarity(**bad)
 
 
 def test_constructor_validation_mirrors_reference():
