@@ -39,6 +39,8 @@ fsi_status fail(fsi_context* c, fsi_status code, const char* fmt, ...) {
 }
 
 
+thread_local LaunchRecorder* fsi::g_recorder = nullptr;
+
 hipError_t dalloc(fsi_context* c, double** p, size_t n) {
     hipError_t e = hipMalloc((void**)p, n * sizeof(double));
     if (e == hipSuccess) e = hipMemsetAsync(*p, 0, n * sizeof(double), c->stream);

@@ -17,6 +17,7 @@
 #include <stdint.h>
 
 #include "../../include/fsi_api.h"
+#include "fsi_launch.hpp"
 
 namespace fsi {
 

@@ -246,7 +246,7 @@ __device__ __forceinline__ void down_tile(const MgGeom& gf, const MgGeom& gc, co
 // must line up with the PCG kernels' grid; smaller on coarser levels so that
 // they still spread over many CUs).
 template <int FX, int FY, typename TB, typename T>
-__global__ void __launch_bounds__(MG_THREADS) k_mg_down(MgGeom gf, MgGeom gc, const TB* __restrict__ b,
+FSI_KERNEL void k_mg_down(MgGeom gf, MgGeom gc, const TB* __restrict__ b,
                                                         T* __restrict__ x2out, T* __restrict__ bc, const int* done,
                                                         Status* st, const double* __restrict__ rr, int nb, int m,
                                                         int slot) {
@@ -391,7 +391,7 @@ __device__ __forceinline__ double up_tile(const MgGeom& gf, const MgGeom& gc, co
 }
 
 template <int FX, int FY, typename TB, typename T, typename TO>
-__global__ void __launch_bounds__(MG_THREADS) k_mg_up(MgGeom gf, MgGeom gc, const T* __restrict__ x2,
+FSI_KERNEL void k_mg_up(MgGeom gf, MgGeom gc, const T* __restrict__ x2,
                                                       const TB* __restrict__ b, const T* __restrict__ ec,
                                                       TO* __restrict__ xout, const int* done, double* rz, int nb) {
     using UT = UpTile<FX, FY>;
@@ -503,7 +503,7 @@ __device__ __forceinline__ void co_pw(int f, int nf, int per, int dlo, int dhi, 
         for (int i = tx; i < (g).nx; i += 32)
 
 template <typename T>
-__global__ void __launch_bounds__(CO_THREADS) k_mg_coarse(MgGeom g0, int levels, const T* __restrict__ dense,
+FSI_KERNEL void k_mg_coarse(MgGeom g0, int levels, const T* __restrict__ dense,
                                                           int dense_n, const T* __restrict__ bglob,
                                                           T* __restrict__ tglob, const int* done) {
     extern __shared__ __align__(16) unsigned char co_smem[];
@@ -769,10 +769,8 @@ int coarse_max_cells() {
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 1024;
     const size_t need = 3 * sizeof(double) * (4096 + 1024 + 256 + 64);
     if ((size_t)lds < need) return 1024;
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse<double>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) != hipSuccess ||
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mg_coarse<float>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) != hipSuccess) {
+    if (fsi_allow_dynamic_lds<&k_mg_coarse<double>, CO_THREADS>((int)need) != hipSuccess ||
+        fsi_allow_dynamic_lds<&k_mg_coarse<float>, CO_THREADS>((int)need) != hipSuccess) {
         (void)hipGetLastError();
         return 1024;
     }
@@ -781,12 +779,12 @@ int coarse_max_cells() {
 
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const double* b, double* t,
                       const int* done) {
-    k_mg_coarse<double><<<1, CO_THREADS, cp.lds_cells * sizeof(double), s>>>(top, cp.levels, cp.dense, cp.dense_n,
+    fsi_launch<&k_mg_coarse<double>, CO_THREADS, 0>(1, CO_THREADS, cp.lds_cells * sizeof(double), s, top, cp.levels, cp.dense, cp.dense_n,
                                                                              b, t, done);
 }
 void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, const float* b, float* t,
                       const int* done) {
-    k_mg_coarse<float><<<1, CO_THREADS, cp.lds_cells * sizeof(float), s>>>(top, cp.levels, cp.dense_f, cp.dense_n,
+    fsi_launch<&k_mg_coarse<float>, CO_THREADS, 0>(1, CO_THREADS, cp.lds_cells * sizeof(float), s, top, cp.levels, cp.dense_f, cp.dense_n,
                                                                            b, t, done);
 }
 
@@ -805,13 +803,13 @@ template <int FX, int FY, typename TB, typename T>
 static void down_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const TB* b, T* x2, T* bc, const int* done,
                    Status* st, const double* rr, int nb, int m, int slot) {
     dim3 g((gf.nx + FX - 1) / FX, (gf.ny + FY - 1) / FY);
-    k_mg_down<FX, FY, TB, T><<<g, MG_THREADS, 0, s>>>(gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
+    fsi_launch<&k_mg_down<FX, FY, TB, T>, MG_THREADS, 0>(g, MG_THREADS, 0, s, gf, gc, b, x2, bc, done, st, rr, nb, m, slot);
 }
 template <int FX, int FY, typename TB, typename T, typename TO>
 static void up_t(hipStream_t s, const MgGeom& gf, const MgGeom& gc, const T* x2, const TB* b, const T* ec, TO* xout,
                  const int* done, double* rz, int nb) {
     dim3 g((gf.nx + FX - 1) / FX, (gf.ny + FY - 1) / FY);
-    k_mg_up<FX, FY, TB, T, TO><<<g, MG_THREADS, 0, s>>>(gf, gc, x2, b, ec, xout, done, rz, nb);
+    fsi_launch<&k_mg_up<FX, FY, TB, T, TO>, MG_THREADS, 0>(g, MG_THREADS, 0, s, gf, gc, x2, b, ec, xout, done, rz, nb);
 }
 
 template <typename TB, typename T>

@@ -366,8 +366,7 @@ __device__ __forceinline__ void momentum_tile(const Params& p, const Stage& s, d
 }
 
 // (47.5 KB of LDS: at most 3 waves per SIMD, so registers up to that occupancy)
-__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(1, 3)))
-k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, double* __restrict__ Iu,
+FSI_KERNEL void k_momentum(Params p, Stage s, double* __restrict__ Eu, double* __restrict__ Ev, double* __restrict__ Iu,
            double* __restrict__ Iv, RhsSums R) {
     __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
     __shared__ double sxx[MT_H * MT_W], syy[MT_H * MT_W], sxy[MT_H * MT_W];
@@ -800,7 +799,7 @@ __device__ __forceinline__ double phase_uwcdi_t(const Params& p, const Stage& s,
 }
 
 // Max |u_c| for dynamic ACDI gamma (PhaseFieldMethods.jl:576-596).
-__global__ void __launch_bounds__(BLOCK) k_max_speed(Params p, const double* U, const double* V, ErrSlot* st) {
+FSI_KERNEL void k_max_speed(Params p, const double* U, const double* V, ErrSlot* st) {
     __shared__ double lds[16];
     const int i = tile_id().x * TILE_X + threadIdx.x;
     double m = 0.0;
@@ -1056,7 +1055,7 @@ __device__ __forceinline__ void acdi_tile(const Params& p, const Stage& s, const
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
+FSI_KERNEL void k_phase_acdi(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
                                                       ErrSlot* st) {
     __shared__ double s_phi[CR2 * CW2];
     __shared__ double s_psi[CR2 * CW2];
@@ -1073,7 +1072,7 @@ __global__ void __launch_bounds__(BLOCK) k_phase_acdi(Params p, Stage s, BcSet b
 
 // Phase-field right-hand side, one instantiation per phase method.
 template <int PH>
-__global__ void __launch_bounds__(BLOCK) k_phase(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
+FSI_KERNEL void k_phase(Params p, Stage s, BcSet bpsi, double* __restrict__ Ephi,
                                                  ErrSlot* st, int dir) {
     constexpr bool kPsi = PH == FSI_PHASE_ACDI;
     constexpr bool kNormals = PH != FSI_PHASE_UPWIND;
@@ -1431,7 +1430,7 @@ __device__ __forceinline__ void sigma_tile(const Params& p, const Stage& s, doub
     }
 }
 
-__global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __restrict__ Exx,
+FSI_KERNEL void k_sigma(Params p, Stage s, double* __restrict__ Exx,
                                                  double* __restrict__ Eyy, double* __restrict__ Exy, ErrSlot* st,
                                                  int stage1, CellSums C) {
     __shared__ double su[MT_H * MT_W], sv[MT_H * MT_W];
@@ -1445,7 +1444,7 @@ __global__ void __launch_bounds__(BLOCK) k_sigma(Params p, Stage s, double* __re
 }
 
 // b_visc = M*0 + b (the affine part of the viscous operator), computed once.
-__global__ void __launch_bounds__(BLOCK) k_visc_boundary(Params p, const double* zero, double* bu, double* bv) {
+FSI_KERNEL void k_visc_boundary(Params p, const double* zero, double* bu, double* bv) {
     const int i = tile_id().x * TILE_X + threadIdx.x;
     if (i >= p.nx) return;
     for (int r = 0; r < ROWS; ++r) {
@@ -1464,7 +1463,7 @@ static dim3 grid2d(const Params& p) {
 }
 
 // One block: maxima and sums of the per-block partials in a fixed order.
-__global__ void __launch_bounds__(BLOCK) k_ext_final(Params p, Status* st) {
+FSI_KERNEL void k_ext_final(Params p, Status* st) {
     constexpr int NM = 5, NS = 3;  // maxima: -phi_min, phi_max, -eig_min, det_err, stage1; sums: counts
     const int mslot[NM] = {EX_PHI_NEGMIN, EX_PHI_MAX, EX_EIG_NEGMIN, EX_DET_ERR, EX_STAGE1};
     const int sslot[NS] = {EX_NONSPD, EX_NONFINITE, EX_NONFINITE_IN};
@@ -1540,7 +1539,7 @@ __device__ __forceinline__ int clamp_or_wrap(int k, int n, int periodic) {
 // The 9-point filter of _smoothed9_field (PhaseFieldMethods.jl:364-383) and
 // _smooth_phase_for_capillarity (Capillarity.jl:81-111): the index is
 // clamped at a wall and wrapped across a periodic side.
-__global__ void __launch_bounds__(BLOCK) k_smooth9(Params p, const double* __restrict__ f, double* __restrict__ out) {
+FSI_KERNEL void k_smooth9(Params p, const double* __restrict__ f, double* __restrict__ out) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j >= p.ny) return;
     const int im = clamp_or_wrap(i - 1, p.nx, p.px), ip = clamp_or_wrap(i + 1, p.nx, p.px);
@@ -1551,7 +1550,7 @@ __global__ void __launch_bounds__(BLOCK) k_smooth9(Params p, const double* __res
 }
 
 // phi_out = phi + h rhs on the cells (_uwcdi_directional_euler_sweep :793-810)
-__global__ void __launch_bounds__(BLOCK) k_cell_axpy(Params p, const double* __restrict__ phi, double h,
+FSI_KERNEL void k_cell_axpy(Params p, const double* __restrict__ phi, double h,
                                                      const double* __restrict__ rhs, double* __restrict__ out) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j >= p.ny) return;
@@ -1561,7 +1560,7 @@ __global__ void __launch_bounds__(BLOCK) k_cell_axpy(Params p, const double* __r
 
 // Regularised centre normal of the geometry field (Capillarity.jl:59-79):
 // the face gradients with the phase boundary conditions averaged to the centre.
-__global__ void __launch_bounds__(BLOCK) k_cap_normals(Params p, const double* __restrict__ geo,
+FSI_KERNEL void k_cap_normals(Params p, const double* __restrict__ geo,
                                                        double* __restrict__ nxc, double* __restrict__ nyc) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j >= p.ny) return;
@@ -1574,7 +1573,7 @@ __global__ void __launch_bounds__(BLOCK) k_cap_normals(Params p, const double* _
 }
 
 // kappa = -div(n) with n carried to the faces by interpolate_center_to_*_faces (Capillarity.jl:168-173)
-__global__ void __launch_bounds__(BLOCK) k_cap_kappa(Params p, const double* __restrict__ nxc,
+FSI_KERNEL void k_cap_kappa(Params p, const double* __restrict__ nxc,
                                                      const double* __restrict__ nyc, double* __restrict__ kap) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
     if (i >= p.nx || j >= p.ny) return;
@@ -1585,7 +1584,7 @@ __global__ void __launch_bounds__(BLOCK) k_cap_kappa(Params p, const double* __r
 }
 
 // f = coefficient kappa_face grad(phi) on every x- and y-face (Capillarity.jl:175-186)
-__global__ void __launch_bounds__(BLOCK) k_cap_force(Params p, const double* __restrict__ phi,
+FSI_KERNEL void k_cap_force(Params p, const double* __restrict__ phi,
                                                      const double* __restrict__ kap, double* __restrict__ fx,
                                                      double* __restrict__ fy) {
     const int i = blockIdx.x * TILE_X + threadIdx.x, j = blockIdx.y * TILE_Y + threadIdx.y;
@@ -1596,7 +1595,7 @@ __global__ void __launch_bounds__(BLOCK) k_cap_force(Params p, const double* __r
     if (i < p.nx) fy[at(p, i, j)] = p.cap_coef * c2fy(p, K, i, j) * grad_yface(p, p.bPhi, PH, i, j);
 }
 
-void launch_ext_final(hipStream_t s, const Params& p, Status* st) { k_ext_final<<<1, BLOCK, 0, s>>>(p, st); }
+void launch_ext_final(hipStream_t s, const Params& p, Status* st) { fsi_launch<&k_ext_final, BLOCK, 0>(1, BLOCK, 0, s, p, st); }
 
 // capillary_force (Capillarity.jl:134-187) of the stage phi into p.cap_fx, p.cap_fy
 static void launch_capillary(hipStream_t s, const Params& p, const double* phi) {
@@ -1605,12 +1604,12 @@ static void launch_capillary(hipStream_t s, const Params& p, const double* phi) 
     const double* geo = phi;
     for (int k = 0; k < p.cap_passes; ++k) {
         double* out = w + (size_t)(k & 1) * F;
-        k_smooth9<<<point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s>>>(p, geo, out);
+        fsi_launch<&k_smooth9, BLOCK, 0>(point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s, p, geo, out);
         geo = out;
     }
-    k_cap_normals<<<point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s>>>(p, geo, w + 2 * F, w + 3 * F);
-    k_cap_kappa<<<point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s>>>(p, w + 2 * F, w + 3 * F, w + 4 * F);
-    k_cap_force<<<point_grid(p, 1), dim3(TILE_X, TILE_Y), 0, s>>>(p, phi, w + 4 * F, const_cast<double*>(p.cap_fx),
+    fsi_launch<&k_cap_normals, BLOCK, 0>(point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s, p, geo, w + 2 * F, w + 3 * F);
+    fsi_launch<&k_cap_kappa, BLOCK, 0>(point_grid(p, 0), dim3(TILE_X, TILE_Y), 0, s, p, w + 2 * F, w + 3 * F, w + 4 * F);
+    fsi_launch<&k_cap_force, BLOCK, 0>(point_grid(p, 1), dim3(TILE_X, TILE_Y), 0, s, p, phi, w + 4 * F, const_cast<double*>(p.cap_fx),
                                                                    const_cast<double*>(p.cap_fy));
 }
 
@@ -1622,7 +1621,7 @@ void launch_momentum(hipStream_t s, const Params& p, const double* U, const doub
     RhsSums R{};
     R.q = -1;
     if (sums) R = *sums;
-    k_momentum<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, st, Eu, Ev, Iu, Iv, R);
+    fsi_launch<&k_momentum, BLOCK, 3>(grid2d(p), dim3(TILE_X, TILE_Y), 0, s, p, st, Eu, Ev, Iu, Iv, R);
 }
 
 void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const double* U, const double* V,
@@ -1631,34 +1630,34 @@ void launch_cells(hipStream_t s, const Params& p, const BcSet& bpsi, const doubl
                   const CellSums* sums) {
     Stage sg{U, V, phi, sxx, syy, sxy, reset};
     if (p.phase == FSI_PHASE_ACDI && p.acdi_dyn)
-        k_max_speed<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, st);
+        fsi_launch<&k_max_speed, BLOCK, 0>(grid2d(p), dim3(TILE_X, TILE_Y), 0, s, p, U, V, st);
     const dim3 g = grid2d(p), blk(TILE_X, TILE_Y);
     switch (p.phase) {
-        case FSI_PHASE_ACDI: k_phase_acdi<<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st); break;
+        case FSI_PHASE_ACDI: fsi_launch<&k_phase_acdi, BLOCK, 0>(g, blk, 0, s, p, sg, bpsi, Ephi, st); break;
         case FSI_PHASE_UWCDI:
-            if (p.uw_field) k_smooth9<<<point_grid(p, 0), blk, 0, s>>>(p, phi, p.uw_field);
-            k_phase<FSI_PHASE_UWCDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st, 0);
+            if (p.uw_field) fsi_launch<&k_smooth9, BLOCK, 0>(point_grid(p, 0), blk, 0, s, p, phi, p.uw_field);
+            fsi_launch<&k_phase<FSI_PHASE_UWCDI>, BLOCK, 0>(g, blk, 0, s, p, sg, bpsi, Ephi, st, 0);
             break;
-        case FSI_PHASE_CDI: k_phase<FSI_PHASE_CDI><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st, 0); break;
-        default: k_phase<FSI_PHASE_UPWIND><<<g, blk, 0, s>>>(p, sg, bpsi, Ephi, st, 0); break;
+        case FSI_PHASE_CDI: fsi_launch<&k_phase<FSI_PHASE_CDI>, BLOCK, 0>(g, blk, 0, s, p, sg, bpsi, Ephi, st, 0); break;
+        default: fsi_launch<&k_phase<FSI_PHASE_UPWIND>, BLOCK, 0>(g, blk, 0, s, p, sg, bpsi, Ephi, st, 0); break;
     }
     CellSums C{};
     C.q = -1;
     if (sums) C = *sums;
-    k_sigma<<<g, blk, 0, s>>>(p, sg, Exx, Eyy, Exy, st, stage1, C);
+    fsi_launch<&k_sigma, BLOCK, 0>(g, blk, 0, s, p, sg, Exx, Eyy, Exy, st, stage1, C);
 }
 
 void launch_uwcdi_sweep(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi, int dir,
                         double h, double* rhs, double* phi_out, ErrSlot* err) {
     Stage sg{U, V, phi, nullptr, nullptr, nullptr, 0};
     const dim3 blk(TILE_X, TILE_Y);
-    if (p.uw_field) k_smooth9<<<point_grid(p, 0), blk, 0, s>>>(p, phi, p.uw_field);
-    k_phase<FSI_PHASE_UWCDI><<<grid2d(p), blk, 0, s>>>(p, sg, p.bPhi, rhs, err, dir);
-    k_cell_axpy<<<point_grid(p, 0), blk, 0, s>>>(p, phi, h, rhs, phi_out);
+    if (p.uw_field) fsi_launch<&k_smooth9, BLOCK, 0>(point_grid(p, 0), blk, 0, s, p, phi, p.uw_field);
+    fsi_launch<&k_phase<FSI_PHASE_UWCDI>, BLOCK, 0>(grid2d(p), blk, 0, s, p, sg, p.bPhi, rhs, err, dir);
+    fsi_launch<&k_cell_axpy, BLOCK, 0>(point_grid(p, 0), blk, 0, s, p, phi, h, rhs, phi_out);
 }
 
 void launch_visc_boundary(hipStream_t s, const Params& p, const double* zero, double* bu, double* bv) {
-    k_visc_boundary<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, zero, bu, bv);
+    fsi_launch<&k_visc_boundary, BLOCK, 0>(grid2d(p), dim3(TILE_X, TILE_Y), 0, s, p, zero, bu, bv);
 }
 
 // ===========================================================================
@@ -1708,7 +1707,7 @@ __device__ __forceinline__ double centre_x(double lx, int nx, int i) { return lx
 //       NaN counts: 8 phi, 9 speed or divergence, 10 U or V
 // maxima: 0 phi max, 1 -phi min, 2 speed, 3 |div|, 4 |U|, 5 |V|
 constexpr int DG1_S = 11, DG1_M = 6;
-__global__ void __launch_bounds__(BLOCK) k_diag_state(Params p, const double* U, const double* V, const double* phi,
+FSI_KERNEL void k_diag_state(Params p, const double* U, const double* V, const double* phi,
                                                       double lx, double ly, double density, double* buf) {
     double sm[DG1_S], mx[DG1_M];
 #pragma unroll
@@ -1764,7 +1763,7 @@ __global__ void __launch_bounds__(BLOCK) k_diag_state(Params p, const double* U,
 // sums: 0 cells, 1 area, 2 sum a |det - 1|, 3 sum a stress, 4 non-SPD cells, 5 NaN count
 // maxima: 0 det, 1 -det, 2 |det - 1|, 3 lambda_max, 4 -lambda_min, 5 stress
 constexpr int DGT_S = 6, DGT_M = 6;
-__global__ void __launch_bounds__(BLOCK) k_diag_tensor(Params p, const double* phi, const double* sxx,
+FSI_KERNEL void k_diag_tensor(Params p, const double* phi, const double* sxx,
                                                        const double* syy, const double* sxy, double G, int all,
                                                        double lo, int lo_in, double hi, int hi_in, double* buf) {
     double sm[DGT_S], mx[DGT_M];
@@ -1810,7 +1809,7 @@ __global__ void __launch_bounds__(BLOCK) k_diag_tensor(Params p, const double* p
 // series, and with n0 == 0 the perimeter proxy and the second moments.
 // sums: 0 perimeter, 1 cxx, 2 cyy, 3 cxy, 4 + 2q Rc[n0 + q + 1], 5 + 2q Rs[...]
 constexpr int DG2_MODES = 8, DG2_S = 4 + 2 * DG2_MODES;
-__global__ void __launch_bounds__(BLOCK) k_diag_geometry(Params p, const double* phi, double lx, double ly, double xc,
+FSI_KERNEL void k_diag_geometry(Params p, const double* phi, double lx, double ly, double xc,
                                                          double yc, int n0, int nmax, double* buf) {
     double sm[DG2_S], mx[1] = {-INFINITY};
 #pragma unroll
@@ -1849,7 +1848,7 @@ __global__ void __launch_bounds__(BLOCK) k_diag_geometry(Params p, const double*
 
 // one block: out[q] = total of row q (sums in a fixed order, then maxima)
 template <int NS, int NM>
-__global__ void __launch_bounds__(BLOCK) k_diag_final(const double* buf, int nb, double* out) {
+FSI_KERNEL void k_diag_final(const double* buf, int nb, double* out) {
     double sm[NS], mx[NM];
 #pragma unroll
     for (int q = 0; q < NS; ++q) sm[q] = 0.0;
@@ -1884,25 +1883,25 @@ __global__ void __launch_bounds__(BLOCK) k_diag_final(const double* buf, int nb,
 
 void launch_diag_state(hipStream_t s, const Params& p, const double* U, const double* V, const double* phi,
                        double lx, double ly, double density, double* buf, double* out) {
-    k_diag_state<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, U, V, phi, lx, ly, density, buf);
-    k_diag_final<DG1_S, DG1_M><<<1, BLOCK, 0, s>>>(buf, p.exnb, out);
+    fsi_launch<&k_diag_state, BLOCK, 0>(grid2d(p), dim3(TILE_X, TILE_Y), 0, s, p, U, V, phi, lx, ly, density, buf);
+    fsi_launch<&k_diag_final<DG1_S, DG1_M>, BLOCK, 0>(1, BLOCK, 0, s, buf, p.exnb, out);
 }
 void launch_diag_tensor(hipStream_t s, const Params& p, const double* phi, const double* sxx, const double* syy,
                         const double* sxy, double G, const DiagRegion& rg, double* buf, double* out) {
-    k_diag_tensor<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, phi, sxx, syy, sxy, G, rg.all, rg.lo, rg.lo_in,
+    fsi_launch<&k_diag_tensor, BLOCK, 0>(grid2d(p), dim3(TILE_X, TILE_Y), 0, s, p, phi, sxx, syy, sxy, G, rg.all, rg.lo, rg.lo_in,
                                                              rg.hi, rg.hi_in, buf);
-    k_diag_final<DGT_S, DGT_M><<<1, BLOCK, 0, s>>>(buf, p.exnb, out);
+    fsi_launch<&k_diag_final<DGT_S, DGT_M>, BLOCK, 0>(1, BLOCK, 0, s, buf, p.exnb, out);
 }
 void launch_diag_geometry(hipStream_t s, const Params& p, const double* phi, double lx, double ly, double xc,
                           double yc, int n0, int nmax, double* buf, double* out) {
-    k_diag_geometry<<<grid2d(p), dim3(TILE_X, TILE_Y), 0, s>>>(p, phi, lx, ly, xc, yc, n0, nmax, buf);
-    k_diag_final<DG2_S, 1><<<1, BLOCK, 0, s>>>(buf, p.exnb, out);
+    fsi_launch<&k_diag_geometry, BLOCK, 0>(grid2d(p), dim3(TILE_X, TILE_Y), 0, s, p, phi, lx, ly, xc, yc, n0, nmax, buf);
+    fsi_launch<&k_diag_final<DG2_S, 1>, BLOCK, 0>(1, BLOCK, 0, s, buf, p.exnb, out);
 }
 int diag_modes_per_launch() { return DG2_MODES; }
 
 // phase_centroid (run_sugiyama_lid_driven_cavity.jl:496-509) from the sums
 // of k_diag_state: mass = sum phi dA, x = sum x phi dA / mass, y likewise.
-__global__ void k_centroid_result(const double* __restrict__ v, double* __restrict__ out) {
+FSI_KERNEL void k_centroid_result(const double* __restrict__ v, double* __restrict__ out) {
     if (threadIdx.x != 0) return;
     const double mass = v[0];
     const bool ok = mass > 2.220446049250313e-16;
@@ -1911,13 +1910,13 @@ __global__ void k_centroid_result(const double* __restrict__ v, double* __restri
     out[2] = mass;
 }
 void launch_centroid_result(hipStream_t s, const double* reduced, double* result) {
-    k_centroid_result<<<1, 64, 0, s>>>(reduced, result);
+    fsi_launch<&k_centroid_result, 1024, 0>(1, 64, 0, s, reduced, result);
 }
 
 // The Status a step starts from: every counter, flag and sum zero, the
 // extrema at their identities.  A kernel rather than a copy from the host,
 // so the step's first launch does not wait on a host-to-device transfer.
-__global__ void __launch_bounds__(256) k_status_reset(Status* st) {
+FSI_KERNEL void k_status_reset(Status* st) {
     constexpr int W = (int)(sizeof(Status) / sizeof(int));
     int* w = reinterpret_cast<int*>(st);
     for (int k = threadIdx.x; k < W; k += blockDim.x) w[k] = 0;
@@ -1928,6 +1927,6 @@ __global__ void __launch_bounds__(256) k_status_reset(Status* st) {
         st->min_eig = INFINITY;
     }
 }
-void launch_status_reset(hipStream_t s, Status* st) { k_status_reset<<<1, 256, 0, s>>>(st); }
+void launch_status_reset(hipStream_t s, Status* st) { fsi_launch<&k_status_reset, 256, 0>(1, 256, 0, s, st); }
 
 }  // namespace fsi

@@ -283,7 +283,7 @@ __device__ __forceinline__ void vel_start_tile(const Params& p, const VelBufs& b
     put_partials<3>(acc3, pset(b.partials, nb, b.restart ? PS_VELR : PS_VELI), nb);
 }
 
-__global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
+FSI_KERNEL void k_vel_start(Params p, VelBufs b, double alpha, Status* st, int nb) {
     constexpr int N1 = HTile<1>::R * HTile<1>::W;
     __shared__ double sxu[N1], sxv[N1];
     const int tid = threadIdx.x + threadIdx.y * TILE_X;
@@ -308,7 +308,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_start(Params p, VelBufs b, double
 // |b - A u^n|^2, and for the FP32 inner CG of the mixed-precision solve the
 // threshold of its first cycle).  T: the storage of r, p, q.
 template <class T>
-__global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double alpha, int nb, Status* st, int slot,
+FSI_KERNEL void k_vel_q0(Params p, VelBufs b, double alpha, int nb, Status* st, int slot,
                                                   VelTol vt, int first) {
     constexpr bool F32 = sizeof(T) == 4;
     if (!first && st->vel.done) return;  // a later cycle of a solve already converged
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_q0(Params p, VelBufs b, double al
 // check (the iteration limit); k_vel_check then decides with the FP64
 // residual whether the solve has converged.
 template <class T>
-__global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
+FSI_KERNEL void k_vel_iter(Params p, VelBufs b, double alpha, Status* st, int nb, int m,
                                                     int slot, VelTol vt, int check_first) {
     constexpr bool F32 = sizeof(T) == 4;
     if (st->vel.done || (F32 && st->vel.idone)) return;
@@ -634,7 +634,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_iter(Params p, VelBufs b, double 
 // the FP64 x: the stopping rules of k_vel_iter's `test` applied to the true
 // residual after `total` inner iterations.  When the solve goes on, the next
 // cycle's inner threshold is set from it.  One block.
-__global__ void __launch_bounds__(BLOCK) k_vel_check(VelBufs b, Status* st, int nb, int slot, VelTol vt) {
+FSI_KERNEL void k_vel_check(VelBufs b, Status* st, int nb, int slot, VelTol vt) {
     if (st->vel.done) return;
     double tot[3];
     get_totals<3>(pset(b.partials, nb, PS_VELR), nb, tot);
@@ -680,7 +680,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_check(VelBufs b, Status* st, int 
 // stage's solutions of the last steps, with the Dirichlet values, on the
 // compact cells, written into the solve's iterate.  The start kernel then
 // reads it as it is (x0_in_place) instead of the four history fields.
-__global__ void __launch_bounds__(BLOCK) k_vel_predict(Params p, VelBufs b) {
+FSI_KERNEL void k_vel_predict(Params p, VelBufs b) {
     const double gul = p.bU.l.value, gvb = p.bV.b.value;
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
@@ -700,7 +700,7 @@ __global__ void __launch_bounds__(BLOCK) k_vel_predict(Params p, VelBufs b) {
     }
 }
 // NoViscosity: the predictor is the stage sum with Dirichlet values (:794-796).
-__global__ void __launch_bounds__(BLOCK) k_vel_copy(Params p, VelBufs b) {
+FSI_KERNEL void k_vel_copy(Params p, VelBufs b) {
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
         b.xu[k] = dir_u(p, i) ? p.bU.l.value : b.ru[k];
@@ -794,7 +794,7 @@ __device__ __forceinline__ double pweight(int f, int I, int nf, int per, int dlo
 
 // rhs = (div(v*) - G) - b_bc, with the injected boundary faces of the
 // predictor (_inject_compact_predictor! :543-579) written first.
-__global__ void __launch_bounds__(BLOCK) k_proj_rhs(Params p, ProjBufs b, int nb) {
+FSI_KERNEL void k_proj_rhs(Params p, ProjBufs b, int nb) {
     const double area = p.hx * p.hy;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     // every load of the thread's rows first (index 0 stands in outside)
@@ -878,7 +878,7 @@ __device__ __forceinline__ double proj_scalars(const Params& p, const ProjBufs& 
 }
 
 // the projection scalars and r = b - (-A) x with b = -(rhs - mean)
-__global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int has_null,
+FSI_KERNEL void k_pcg_init(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int has_null,
                                                     double rtol, int maxiter) {
     const double mean = proj_scalars(p, b, st, nb, has_null, rtol, maxiter);
     GAcc X{b.x, p.P};
@@ -899,7 +899,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_init(Params p, MgGeom g, ProjBufs
 // iteration 0), of the true residual |A x - rhs|^2 (:261-264; x and
 // x - mean(x) have the same image under A) and of the area-weighted sum of x
 // (its mean, :255-259).  r is not stored.
-__global__ void __launch_bounds__(BLOCK) k_proj_direct(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
+FSI_KERNEL void k_proj_direct(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
                                                        int has_null, double rtol, int maxiter) {
     // the block's own cells loaded before the right-hand side totals are
     // summed, so the two latencies overlap
@@ -938,7 +938,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_direct(Params p, MgGeom g, ProjB
 }
 
 // ---- multigrid level kernels (every one returns at once once PCG is done)
-__global__ void __launch_bounds__(BLOCK) k_mg_jacobi0(MgGeom g, const double* b, double* x, double omega,
+FSI_KERNEL void k_mg_jacobi0(MgGeom g, const double* b, double* x, double omega,
                                                       const int* done) {
     if (*done) return;
     FOR_TILE(g) {
@@ -947,7 +947,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_jacobi0(MgGeom g, const double* b,
         x[(size_t)j * g.P + i] = omega * b[(size_t)j * g.P + i] / d;
     }
 }
-__global__ void __launch_bounds__(BLOCK) k_mg_jacobi(MgGeom g, const double* xin, const double* b, double* xout,
+FSI_KERNEL void k_mg_jacobi(MgGeom g, const double* xin, const double* b, double* xout,
                                                      double omega, const int* done) {
     if (*done) return;
     FOR_TILE(g) {
@@ -959,7 +959,7 @@ __global__ void __launch_bounds__(BLOCK) k_mg_jacobi(MgGeom g, const double* xin
 }
 // single-level hierarchy: the convergence test the finest down-kernel
 // would do, and the r.z partials the finest up-kernel would write
-__global__ void k_pcg_check(Status* st, const double* rr, int nb, int m, int slot) {
+FSI_KERNEL void k_pcg_check(Status* st, const double* rr, int nb, int m, int slot) {
     if (st->pcg.done) return;
     double tot[1];
     get_totals<1>(rr, nb, tot);
@@ -971,7 +971,7 @@ __global__ void k_pcg_check(Status* st, const double* rr, int nb, int m, int slo
         if (slot >= 0 && tot[0] <= st->pcg.tol2) st->proj_ok[slot] = 1;
     }
 }
-__global__ void __launch_bounds__(BLOCK) k_rz(Params p, ProjBufs b, double* rz, int nb, const int* done) {
+FSI_KERNEL void k_rz(Params p, ProjBufs b, double* rz, int nb, const int* done) {
     if (*done) return;
     double acc[1] = {0.0};
     FOR_TILE(p) {
@@ -982,7 +982,7 @@ __global__ void __launch_bounds__(BLOCK) k_rz(Params p, ProjBufs b, double* rz, 
 }
 
 // p_m = z + beta p_{m-1} (beta = rz_m / rz_{m-1}), w = (-A) p_m, partial p.w
-__global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int m) {
+FSI_KERNEL void k_pcg_dir(Params p, MgGeom g, ProjBufs b, Status* st, int nb, int m) {
     if (st->pcg.done) return;
     double tot[1];
     get_totals<1>(pset(b.partials, nb, PS_RZ), nb, tot);
@@ -1039,7 +1039,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_dir(Params p, MgGeom g, ProjBufs 
     put_partials<1>(acc, pset(b.partials, nb, PS_PW), nb);
 }
 // x += a p_m; r -= a w (a = rz_m / p.w); partial r.r
-__global__ void __launch_bounds__(BLOCK) k_pcg_upd(Params p, ProjBufs b, Status* st, int nb, int m, int slot) {
+FSI_KERNEL void k_pcg_upd(Params p, ProjBufs b, Status* st, int nb, int m, int slot) {
     if (st->pcg.done) return;
     double tot[1];
     get_totals<1>(pset(b.partials, nb, PS_PW), nb, tot);
@@ -1058,7 +1058,7 @@ __global__ void __launch_bounds__(BLOCK) k_pcg_upd(Params p, ProjBufs b, Status*
 }
 
 // mean of q (area weighted, src/Projection.jl:255-259)
-__global__ void __launch_bounds__(BLOCK) k_proj_mean(Params p, ProjBufs b, int nb) {
+FSI_KERNEL void k_proj_mean(Params p, ProjBufs b, int nb) {
     double acc[1] = {0.0};
     const double area = p.hx * p.hy;
     FOR_TILE(p) { acc[0] += b.x[at(p, i, j)] * area; }
@@ -1114,7 +1114,7 @@ __device__ __forceinline__ double grad_face_y(const Params& p, const GAcc& Q, in
 // place).  Launched on the cell grid: the thread of cell (i, j) updates the
 // x-face (i, j) and the y-face (i, j), the last column and row also the far
 // faces x = nx and y = ny; every load of the thread's rows is issued first.
-__global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
+FSI_KERNEL void k_proj_update(Params p, ProjBufs b) {
     GAcc Q{b.x, p.P};
     const int i = tile_id().x * TILE_X + threadIdx.x;
     const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
@@ -1150,7 +1150,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_update(Params p, ProjBufs b) {
 // q_out = q - mean; true residual |A q - rhs| (:261-264); constraint div(v) - G
 // in L-inf and area-weighted L2 (:275-297)
 template <int RESID>
-__global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
+FSI_KERNEL void k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
                                                       int has_null, double lxly) {
     double tm[1];
     get_totals<1>(pset(b.partials, nb, PS_MEAN), nb, tm);
@@ -1247,7 +1247,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_check(Params p, MgGeom g, ProjBu
 // of the last transform pass; otherwise the test and the mean are
 // k_proj_direct's.
 template <bool DIRECT>
-__global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
+FSI_KERNEL void k_proj_upcheck(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
                                                         int has_null, double rtol, int maxiter, double lxly) {
     const double area = p.hx * p.hy;
     GAcc Q{b.x, p.P};
@@ -1370,7 +1370,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_upcheck(Params p, MgGeom g, Proj
 }
 
 // the checks of project! (:279-298), one block
-__global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, int nb, double lxly, int pcg_m,
+FSI_KERNEL void k_proj_final(ProjBufs b, Status* st, int nb, double lxly, int pcg_m,
                                                       int slot) {
     // one pass over the partials: the r.r of PCG iteration pcg_m (its
     // convergence test, as k_pcg_check makes it), the true-residual and
@@ -1451,7 +1451,7 @@ __global__ void __launch_bounds__(BLOCK) k_proj_final(ProjBufs b, Status* st, in
 }
 
 // Plain copy, the calibration kernel of the traffic measurement.
-__global__ void __launch_bounds__(BLOCK) k_copy(Params p, const double* __restrict__ a, double* __restrict__ b) {
+FSI_KERNEL void k_copy(Params p, const double* __restrict__ a, double* __restrict__ b) {
     FOR_TILE(p) { b[at(p, i, j)] = a[at(p, i, j)]; }
 }
 
@@ -1463,17 +1463,17 @@ static const dim3 kBlock(TILE_X, TILE_Y);
 
 void launch_vel_init(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
                      const VelTol& tol) {
-    k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, grid_blocks(p));
+    fsi_launch<&k_vel_start, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, alpha, st, grid_blocks(p));
     if (b.fr_u[0])
-        k_vel_q0<float><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p), st, slot, tol, 1);
+        fsi_launch<&k_vel_q0<float>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, alpha, grid_blocks(p), st, slot, tol, 1);
     else
-        k_vel_q0<double><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, grid_blocks(p), st, slot, tol, 1);
+        fsi_launch<&k_vel_q0<double>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, alpha, grid_blocks(p), st, slot, tol, 1);
 }
 void launch_vel_iters_f32(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0,
                           int count, int slot, const VelTol& tol) {
     const int nb = grid_blocks(p);
     for (int c = 0; c < count; ++c)
-        k_vel_iter<float><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m0 + c, slot, tol, 0);
+        fsi_launch<&k_vel_iter<float>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, alpha, st, nb, m0 + c, slot, tol, 0);
 }
 void launch_vel_refine(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int slot,
                        const VelTol& tol, int next) {
@@ -1484,39 +1484,39 @@ void launch_vel_refine(hipStream_t s, const Params& p, const VelBufs& b, double 
     rb.Vn = b.xv;
     rb.extrap = 0;
     rb.Su = rb.Sv = rb.Mu = rb.Mv = nullptr;
-    k_vel_start<<<grid2(p), kBlock, 0, s>>>(p, rb, alpha, st, nb);
-    k_vel_check<<<1, BLOCK, 0, s>>>(rb, st, nb, slot, tol);
-    if (next) k_vel_q0<float><<<grid2(p), kBlock, 0, s>>>(p, rb, alpha, nb, st, slot, tol, 0);
+    fsi_launch<&k_vel_start, BLOCK, 0>(grid2(p), kBlock, 0, s, p, rb, alpha, st, nb);
+    fsi_launch<&k_vel_check, BLOCK, 0>(1, BLOCK, 0, s, rb, st, nb, slot, tol);
+    if (next) fsi_launch<&k_vel_q0<float>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, rb, alpha, nb, st, slot, tol, 0);
 }
 void launch_vel_iters(hipStream_t s, const Params& p, const VelBufs& b, double alpha, Status* st, int m0, int count,
                       int slot, const VelTol& tol, int check_from) {
     const int nb = grid_blocks(p);
     for (int c = 0; c < count; ++c) {
         const int m = m0 + c;
-        k_vel_iter<double><<<grid2(p), kBlock, 0, s>>>(p, b, alpha, st, nb, m, slot, tol,
+        fsi_launch<&k_vel_iter<double>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, alpha, st, nb, m, slot, tol,
                                                        m >= 1 && m >= check_from ? 1 : 0);
     }
 }
 void launch_vel_predict(hipStream_t s, const Params& p, const VelBufs& b) {
-    k_vel_predict<<<grid2(p), kBlock, 0, s>>>(p, b);
+    fsi_launch<&k_vel_predict, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b);
 }
 void launch_vel_copy(hipStream_t s, const Params& p, const VelBufs& b) {
-    k_vel_copy<<<grid2(p), kBlock, 0, s>>>(p, b);
+    fsi_launch<&k_vel_copy, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b);
 }
 void launch_pcg_check(hipStream_t s, Status* st, const double* rr, int nb, int m, int slot) {
-    k_pcg_check<<<1, BLOCK, 0, s>>>(st, rr, nb, m, slot);
+    fsi_launch<&k_pcg_check, 1024, 0>(1, BLOCK, 0, s, st, rr, nb, m, slot);
 }
 void launch_pcg_test(hipStream_t s, const Params& p, const ProjBufs& b, Status* st, int m, int slot) {
     const int nb = grid_blocks(p);
-    k_pcg_check<<<1, BLOCK, 0, s>>>(st, b.partials + (size_t)PS_RR * 4 * nb, nb, m, slot);
+    fsi_launch<&k_pcg_check, 1024, 0>(1, BLOCK, 0, s, st, b.partials + (size_t)PS_RR * 4 * nb, nb, m, slot);
 }
 void launch_rz(hipStream_t s, const Params& p, const ProjBufs& b, double* rz, int nb, const int* done) {
-    k_rz<<<grid2(p), kBlock, 0, s>>>(p, b, rz, nb, done);
+    fsi_launch<&k_rz, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, rz, nb, done);
 }
 
 // Initial guess of a projection: x = 2 cur - prev (extrap, linear in time)
 // or x = cur.
-__global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restrict__ x, const double* __restrict__ cur,
+FSI_KERNEL void k_extrapolate(Params p, double* __restrict__ x, const double* __restrict__ cur,
                                                        const double* __restrict__ prev, int extrap) {
     FOR_TILE(p) {
         const size_t k = at(p, i, j);
@@ -1525,15 +1525,15 @@ __global__ void __launch_bounds__(BLOCK) k_extrapolate(Params p, double* __restr
 }
 void launch_extrapolate(hipStream_t s, const Params& p, double* x, const double* cur, const double* prev,
                         int extrap) {
-    k_extrapolate<<<grid2(p), kBlock, 0, s>>>(p, x, cur, prev, extrap);
+    fsi_launch<&k_extrapolate, BLOCK, 0>(grid2(p), kBlock, 0, s, p, x, cur, prev, extrap);
 }
 
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b) {
-    k_proj_rhs<<<grid2(p), kBlock, 0, s>>>(p, b, grid_blocks(p));
+    fsi_launch<&k_proj_rhs, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, grid_blocks(p));
 }
 void launch_pcg_init(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st, int has_null,
                      double rtol, int maxiter) {
-    k_pcg_init<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, grid_blocks(p), has_null, rtol, maxiter);
+    fsi_launch<&k_pcg_init, BLOCK, 0>(grid2(p), kBlock, 0, s, p, g0, b, st, grid_blocks(p), has_null, rtol, maxiter);
 }
 
 static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp,
@@ -1547,13 +1547,13 @@ static void coarse_solve(hipStream_t s, const MgLevel* lv, const MgGeom* geo, in
     } else {  // odd-sized large coarsest level: a fixed number of damped-Jacobi sweeps
         dim3 gr = grid_of(g.nx, g.ny);
         const double omega = 0.8;  // damped-Jacobi sweeps of the fallback
-        k_mg_jacobi0<<<gr, kBlock, 0, s>>>(g, lv[l].b, lv[l].x, omega, done);
+        fsi_launch<&k_mg_jacobi0, BLOCK, 0>(gr, kBlock, 0, s, g, lv[l].b, lv[l].x, omega, done);
         const int sweeps = 2 * (g.nx > g.ny ? g.nx : g.ny);
         for (int it = 0; it < sweeps; ++it) {
-            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
-            k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].t, lv[l].b, lv[l].x, omega, done);
+            fsi_launch<&k_mg_jacobi, BLOCK, 0>(gr, kBlock, 0, s, g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+            fsi_launch<&k_mg_jacobi, BLOCK, 0>(gr, kBlock, 0, s, g, lv[l].t, lv[l].b, lv[l].x, omega, done);
         }
-        k_mg_jacobi<<<gr, kBlock, 0, s>>>(g, lv[l].x, lv[l].b, lv[l].t, omega, done);
+        fsi_launch<&k_mg_jacobi, BLOCK, 0>(gr, kBlock, 0, s, g, lv[l].x, lv[l].b, lv[l].t, omega, done);
     }
 }
 
@@ -1587,18 +1587,18 @@ void launch_pcg_iters(hipStream_t s, const Params& p, const ProjBufs& b, const M
             coarse_solve(s, lv, geo, nlev, cp, done);
             launch_rz(s, p, b, rz, nb, done);
         }
-        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], b, st, nb, m);
-        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, b, st, nb, m, slot);
+        fsi_launch<&k_pcg_dir, BLOCK, 0>(grid2(p), kBlock, 0, s, p, geo[0], b, st, nb, m);
+        fsi_launch<&k_pcg_upd, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, st, nb, m, slot);
     }
 }
 
 void launch_proj_finish(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double lxly, int slot) {
     const int nb = grid_blocks(p);
-    k_proj_mean<<<grid2(p), kBlock, 0, s>>>(p, b, nb);
-    k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
-    k_proj_check<1><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
-    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, -1, slot);
+    fsi_launch<&k_proj_mean, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b, nb);
+    fsi_launch<&k_proj_update, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b);
+    fsi_launch<&k_proj_check<1>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, g0, b, st, nb, has_null, lxly);
+    fsi_launch<&k_proj_final, BLOCK, 0>(1, BLOCK, 0, s, b, st, nb, lxly, -1, slot);
 }
 void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const ProjBufs& b, Status* st,
                         int has_null, double rtol, int maxiter, double lxly, int slot) {
@@ -1606,18 +1606,18 @@ void launch_proj_tested(hipStream_t s, const Params& p, const MgGeom& g0, const 
     const bool apart = b.Uin != b.U && b.Vin != b.V;  // the fused pass reads the neighbours' unprojected faces
     if (apart && b.rhs_part && b.q_part == b.rhs_part + (size_t)4 * b.rhs_np && b.q_np == b.rhs_np) {
         // the direct solve's own partials: test, update and checks in one pass
-        k_proj_upcheck<true><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter, lxly);
-        k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
+        fsi_launch<&k_proj_upcheck<true>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, g0, b, st, nb, has_null, rtol, maxiter, lxly);
+        fsi_launch<&k_proj_final, BLOCK, 0>(1, BLOCK, 0, s, b, st, nb, lxly, 0, slot);
         return;
     }
-    k_proj_direct<<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter);
+    fsi_launch<&k_proj_direct, BLOCK, 0>(grid2(p), kBlock, 0, s, p, g0, b, st, nb, has_null, rtol, maxiter);
     if (apart) {
-        k_proj_upcheck<false><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, rtol, maxiter, lxly);
+        fsi_launch<&k_proj_upcheck<false>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, g0, b, st, nb, has_null, rtol, maxiter, lxly);
     } else {
-        k_proj_update<<<grid2(p), kBlock, 0, s>>>(p, b);
-        k_proj_check<0><<<grid2(p), kBlock, 0, s>>>(p, g0, b, st, nb, has_null, lxly);
+        fsi_launch<&k_proj_update, BLOCK, 0>(grid2(p), kBlock, 0, s, p, b);
+        fsi_launch<&k_proj_check<0>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, g0, b, st, nb, has_null, lxly);
     }
-    k_proj_final<<<1, BLOCK, 0, s>>>(b, st, nb, lxly, 0, slot);
+    fsi_launch<&k_proj_final, BLOCK, 0>(1, BLOCK, 0, s, b, st, nb, lxly, 0, slot);
 }
 
 // One launch of a named solver kernel with the buffers of a live solve, for
@@ -1630,26 +1630,26 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
     const double cell = 8.0 * (double)p.nx * (double)p.ny;
     const std::string n(name);
     if (n == "copy") {
-        k_copy<<<grid2(p), kBlock, 0, s>>>(p, pb.rhs, pb.w);
+        fsi_launch<&k_copy, BLOCK, 0>(grid2(p), kBlock, 0, s, p, pb.rhs, pb.w);
         return 2 * cell;
     }
     if (n == "vel_iter") {
-        k_vel_iter<double><<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0,
+        fsi_launch<&k_vel_iter<double>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, vb, 1e-3, st, nb, 2, 0,
                                                        VelTol{-1.0, 0.0, 1 << 30, 1 << 30, 1, 0, 1 << 30}, 0);
         return 16 * cell;  // r, p, q, x (u, v) read; r', p', q', x (u, v) written
     }
     if (n == "vel_iter_f32") {
         if (!vb.fr_u[0]) return -1.0;  // FP64 mode: no FP32 vectors
-        k_vel_iter<float><<<grid2(p), kBlock, 0, s>>>(p, vb, 1e-3, st, nb, 2, 0,
+        fsi_launch<&k_vel_iter<float>, BLOCK, 0>(grid2(p), kBlock, 0, s, p, vb, 1e-3, st, nb, 2, 0,
                                                       VelTol{-1.0, 0.0, 1 << 30, 1 << 30, 1, 0, 1 << 30}, 0);
         return 10 * cell;  // r, p, q (FP32) and x (FP64), u and v, read and written
     }
     if (n == "pcg_dir") {
-        k_pcg_dir<<<grid2(p), kBlock, 0, s>>>(p, geo[0], pb, st, nb, 2);
+        fsi_launch<&k_pcg_dir, BLOCK, 0>(grid2(p), kBlock, 0, s, p, geo[0], pb, st, nb, 2);
         return 4 * cell;  // z, p_old read; p_new, w written
     }
     if (n == "pcg_upd") {
-        k_pcg_upd<<<grid2(p), kBlock, 0, s>>>(p, pb, st, nb, 2, 0);
+        fsi_launch<&k_pcg_upd, BLOCK, 0>(grid2(p), kBlock, 0, s, p, pb, st, nb, 2, 0);
         return 6 * cell;  // x, p, r, w read; x, r written
     }
     if (n == "mg_down0" && nlev > 1 && lv[0].xf) {
@@ -1686,7 +1686,7 @@ double bench_launch(hipStream_t s, const char* name, const Params& p, const VelB
         return 0.0;
     }
     if (n == "mg_jacobi") {
-        k_mg_jacobi<<<grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s>>>(geo[0], lv[0].x, lv[0].b, lv[0].t, 0.8,
+        fsi_launch<&k_mg_jacobi, BLOCK, 0>(grid_of(geo[0].nx, geo[0].ny), kBlock, 0, s, geo[0], lv[0].x, lv[0].b, lv[0].t, 0.8,
                                                                     &st->pcg.done);
         return 3 * cell;  // x, b read; x' written
     }

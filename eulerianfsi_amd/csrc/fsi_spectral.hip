@@ -244,7 +244,7 @@ __device__ __forceinline__ void proj_rhs_chunk(const Params& p, const ProjBufs& 
 // RHS: the rows are the projection's right-hand side, formed here from the
 // velocity (proj_rhs_at) and stored, with one set of partials per workgroup.
 template <int KIND, int NT, bool RHS>
-__global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, int P, const double* __restrict__ in,
+FSI_KERNEL void k_spec_rows_fwd(SpecAxis ax, int nrows, int P, const double* __restrict__ in,
                                                       double* __restrict__ out, double scale, Params p, ProjBufs pb) {
     constexpr int IT = 2048 / NT;
     extern __shared__ double sp_smem[];
@@ -295,7 +295,7 @@ __global__ void __launch_bounds__(NT) k_spec_rows_fwd(SpecAxis ax, int nrows, in
 template <int KIND, int NT>
 // mpart: one partial per workgroup of the area-weighted sum of the two rows
 // written (the mean of the solution, :255-259), or null.
-__global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x, double* mpart,
+FSI_KERNEL void k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x, double* mpart,
                                                       double area) {
     constexpr int IT = 2048 / NT;
     extern __shared__ double sp_smem[];
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(NT) k_spec_rows_inv(SpecAxis ax, int nrows, in
 
 // ---- columns: forward along y, divide, inverse along y; 2C columns -------
 template <int KIND, int C, int NT>
-__global__ void __launch_bounds__(NT) k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, double* x) {
+FSI_KERNEL void k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, double* x) {
     constexpr int IT = 2048 / NT;  // C n <= 2048 line-pair values per block
     extern __shared__ double sp_smem[];
     const int n = ay.n, m = ay.m, tid = threadIdx.x;
@@ -504,40 +504,40 @@ static size_t cols_lds(int n, int kind, int C) {
 template <int KX>
 static void rows_fwd(hipStream_t s, const SpecPlan& sp, int P, const double* in, double* out) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_fwd<KX, SP_ROW_THREADS, false><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(
+    fsi_launch<&k_spec_rows_fwd<KX, SP_ROW_THREADS, false>, SP_ROW_THREADS, 0>((rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s, 
         sp.x, rows, P, in, out, -1.0, Params{}, ProjBufs{});
 }
 template <int KX>
 static void rows_fwd_rhs(hipStream_t s, const SpecPlan& sp, const Params& p, const ProjBufs& b) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_fwd<KX, SP_ROW_THREADS, true><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(
+    fsi_launch<&k_spec_rows_fwd<KX, SP_ROW_THREADS, true>, SP_ROW_THREADS, 0>((rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s, 
         sp.x, rows, p.P, nullptr, b.x, -1.0, p, b);
 }
 template <int KX>
 static void rows_inv(hipStream_t s, const SpecPlan& sp, int P, double* x, double* mpart = nullptr,
                      double area = 0.0) {
     const int n = sp.x.n, rows = sp.y.n;
-    k_spec_rows_inv<KX, SP_ROW_THREADS><<<(rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s>>>(sp.x, rows, P, x,
+    fsi_launch<&k_spec_rows_inv<KX, SP_ROW_THREADS>, SP_ROW_THREADS, 0>((rows + 1) / 2, SP_ROW_THREADS, rows_lds(n, KX), s, sp.x, rows, P, x,
                                                                                               mpart, area);
 }
 template <int KY>
 static void cols(hipStream_t s, const SpecPlan& sp, int P, double* x) {
     const int n = sp.y.n, nx = sp.x.n;
     if (cols_pairs(n, nx) == 2)
-        k_spec_cols<KY, 2, SP_COL_THREADS><<<nx / 4, SP_COL_THREADS, cols_lds(n, KY, 2), s>>>(sp.y, sp.x.mu, P, x);
+        fsi_launch<&k_spec_cols<KY, 2, SP_COL_THREADS>, SP_COL_THREADS, 0>(nx / 4, SP_COL_THREADS, cols_lds(n, KY, 2), s, sp.y, sp.x.mu, P, x);
     else
-        k_spec_cols<KY, 1, SP_COL_THREADS><<<nx / 2, SP_COL_THREADS, cols_lds(n, KY, 1), s>>>(sp.y, sp.x.mu, P, x);
+        fsi_launch<&k_spec_cols<KY, 1, SP_COL_THREADS>, SP_COL_THREADS, 0>(nx / 2, SP_COL_THREADS, cols_lds(n, KY, 1), s, sp.y, sp.x.mu, P, x);
 }
 
 // dynamic LDS above the 64 KiB default needs the kernel attribute (up to the
 // device's per-block maximum)
-static bool allow_lds(const void* kernel, size_t bytes) {
+static bool allow_lds(hipError_t (*allow)(int), size_t bytes) {
     if (bytes <= 65536) return true;
     int dev = 0, mx = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&mx, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
         bytes > (size_t)mx ||
-        hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) != hipSuccess) {
+        allow((int)bytes) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
@@ -545,15 +545,17 @@ static bool allow_lds(const void* kernel, size_t bytes) {
 }
 template <int KX>
 static bool prepare_rows(int n) {
-    return allow_lds(reinterpret_cast<const void*>(&k_spec_rows_fwd<KX, SP_ROW_THREADS, false>), rows_lds(n, KX)) &&
-           allow_lds(reinterpret_cast<const void*>(&k_spec_rows_fwd<KX, SP_ROW_THREADS, true>), rows_lds(n, KX)) &&
-           allow_lds(reinterpret_cast<const void*>(&k_spec_rows_inv<KX, SP_ROW_THREADS>), rows_lds(n, KX));
+    return allow_lds(&fsi_allow_dynamic_lds<&k_spec_rows_fwd<KX, SP_ROW_THREADS, false>, SP_ROW_THREADS>,
+                     rows_lds(n, KX)) &&
+           allow_lds(&fsi_allow_dynamic_lds<&k_spec_rows_fwd<KX, SP_ROW_THREADS, true>, SP_ROW_THREADS>,
+                     rows_lds(n, KX)) &&
+           allow_lds(&fsi_allow_dynamic_lds<&k_spec_rows_inv<KX, SP_ROW_THREADS>, SP_ROW_THREADS>, rows_lds(n, KX));
 }
 template <int KY>
 static bool prepare_cols(int n, int nx) {
     const int C = cols_pairs(n, nx);
-    const void* k = C == 2 ? reinterpret_cast<const void*>(&k_spec_cols<KY, 2, SP_COL_THREADS>)
-                           : reinterpret_cast<const void*>(&k_spec_cols<KY, 1, SP_COL_THREADS>);
+    hipError_t (*k)(int) = C == 2 ? &fsi_allow_dynamic_lds<&k_spec_cols<KY, 2, SP_COL_THREADS>, SP_COL_THREADS>
+                                  : &fsi_allow_dynamic_lds<&k_spec_cols<KY, 1, SP_COL_THREADS>, SP_COL_THREADS>;
     return allow_lds(k, cols_lds(n, KY, C));
 }
 

@@ -95,11 +95,18 @@ def test_tendency_transpose_matches_the_oracle_tangent():
     bar = {k: np.asfortranarray(v) for k, v in bar.items()}
     lhs = sum(float(np.sum(bar[k] * T.unpack(tout[k])[1][0])) for k in bar)
     lhs_G = sum(float(np.sum(bar[k] * T.unpack(tout[k])[1][1])) for k in bar)
+    # the sums cancel (random signs): the bound is 1e-12 of the sum of the
+    # terms' magnitudes, a per-term agreement far below any transposition
+    # error (observed 2e-13 at most, on sigma_xy: the device's fused
+    # multiply-adds against the oracle's separate roundings)
+    mag = sum(float(np.sum(np.abs(bar[k] * T.unpack(tout[k])[1][0]))) for k in bar)
+    mag_G = sum(float(np.sum(np.abs(bar[k] * T.unpack(tout[k])[1][1]))) for k in bar)
     grads, dG = A.tendencies_transpose(case.state, ctx, bar)
     rhs = _dot(grads, dx, GRADS)
-    print(f"<bar, J dx> = {lhs:.15e}, <J^T bar, dx> = {rhs:.15e}; dG {lhs_G:.15e} vs {dG:.15e}", flush=True)
-    assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), 1.0)
-    assert abs(lhs_G - dG) <= 1e-11 * max(abs(lhs_G), 1.0)
+    print(f"<bar, J dx> = {lhs:.15e}, <J^T bar, dx> = {rhs:.15e} (terms {mag:.3e}); dG {lhs_G:.15e} vs "
+          f"{dG:.15e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-12 * max(mag, 1.0)
+    assert abs(lhs_G - dG) <= 1e-12 * max(mag_G, 1.0)
     # each input field on its own
     for k in GRADS:
         only = {kk: (dx[kk] if kk == k else np.zeros_like(dx[kk])) for kk in dx}
@@ -107,8 +114,9 @@ def test_tendency_transpose_matches_the_oracle_tangent():
         out = tg.tendencies(ds)
         l = sum(float(np.sum(bar[q] * T.unpack(out[q])[1][0])) for q in bar)
         r = float(np.sum(grads[k] * dx[k]))
-        print(f"  {k}: {l:.15e} vs {r:.15e}", flush=True)
-        assert abs(l - r) <= 1e-11 * max(abs(l), 1.0), k
+        m = float(np.sum(np.abs(grads[k] * dx[k])))
+        print(f"  {k}: {l:.15e} vs {r:.15e} (terms {m:.3e}, {abs(l - r) / max(m, 1.0):.1e})", flush=True)
+        assert abs(l - r) <= 1e-12 * max(m, 1.0), k
 
 
 def test_step_transpose_matches_the_oracle_tangent():
