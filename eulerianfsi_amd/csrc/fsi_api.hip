@@ -472,6 +472,8 @@ extern "C" void fsi_context_destroy(fsi_context* c) {
     for (double* b : bufs)
         if (b) (void)hipFree(b);
     if (c->co_dense_f) (void)hipFree(c->co_dense_f);
+    if (c->batch_dev) (void)hipFree(c->batch_dev);
+    if (c->batch_host) (void)hipHostFree(c->batch_host);
     if (c->st) (void)hipFree(c->st);
     if (c->st_host) (void)hipHostFree(c->st_host);
     if (c->side) (void)hipStreamSynchronize(c->side);
@@ -1533,10 +1535,10 @@ struct StepRun {
 // Enqueues the planned step (once the iteration history allows one) and the
 // read-back of its Status; nothing is awaited, so the steps of several
 // contexts can be in flight together.
-static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, StepRun& run) {
-    run.planned = false;
-    fsi_status rc = FSI_OK;
-    if (!(c->use_graph && c->have_history)) return FSI_OK;
+// The iteration plans of a planned step from the context's history; false
+// when the step has to poll (no history yet, or planning switched off).
+static bool plan_step(fsi_context* c) {
+    if (!(c->use_graph && c->have_history)) return false;
     // planned step: every solve gets its planned iteration count (kernels
     // past convergence return at once), one read-back at the end
     // a solve of k iterations is seen converged by launch k + 1 (kernel m
@@ -1568,6 +1570,13 @@ static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, dou
             c->plan_cyc[k] = mc;
         }
     }
+    return true;
+}
+
+static fsi_status step_begin(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, StepRun& run) {
+    run.planned = false;
+    fsi_status rc = FSI_OK;
+    if (!plan_step(c)) return FSI_OK;
     if ((rc = status_push_reset(c)) != FSI_OK) return rc;
     if (c->use_graph == 2) {  // planned counts, launched directly
         if ((rc = enqueue_step(c, s, w, dt, true)) != FSI_OK) return rc;
@@ -1711,6 +1720,108 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
     return step_finish(c, s, w, dt, run, rep, stats);
 }
 
+// One planned step of several instances of one grid shape, batched: the
+// plans are equalised (the largest iteration counts of the group: kernels
+// past a solve's convergence return at once, so each instance's result is
+// the one its own planned step gives), each instance's launch sequence is
+// recorded, and when the sequences agree kernel by kernel every kernel runs
+// once for the whole group (grid.z = instances) on the leader's stream, its
+// arguments read from a table per launch.  Returns false, with nothing
+// enqueued, when the sequences differ; the caller then steps the instances
+// one by one.  On success each run is a planned step awaiting step_finish.
+static bool batched_group_step(const std::vector<int>& ids, fsi_context* const* ctx, fsi_state* const* st,
+                               fsi_workspace* const* w, const double* dt, std::vector<StepRun>& runs,
+                               int32_t* status) {
+    const int K = (int)ids.size();
+    fsi_context* lead = ctx[ids[0]];
+    for (int k = 0; k < 3; ++k) {
+        int pv = 0, pp = 0, pc = 0, cf = 1 << 30;
+        for (int i : ids) {
+            pv = std::max(pv, ctx[i]->plan_vel[k]);
+            pp = std::max(pp, ctx[i]->plan_proj[k]);
+            pc = std::max(pc, ctx[i]->plan_cyc[k]);
+            cf = std::min(cf, ctx[i]->vel_check_from[k]);
+        }
+        for (int i : ids) {
+            ctx[i]->plan_vel[k] = pv;
+            ctx[i]->plan_proj[k] = pp;
+            ctx[i]->plan_cyc[k] = pc;
+            ctx[i]->vel_check_from[k] = cf;
+        }
+    }
+    std::vector<LaunchRecorder> rec((size_t)K);
+    bool ok = true;
+    for (int q = 0; q < K && ok; ++q) {
+        fsi_context* c = ctx[ids[q]];
+        g_recorder = &rec[q];
+        const int dual = c->dual;
+        c->dual = 0;
+        ok = status_push_reset(c) == FSI_OK && enqueue_step(c, st[ids[q]], w[ids[q]], dt[ids[q]], true) == FSI_OK;
+        c->dual = dual;
+        g_recorder = nullptr;
+        ok = ok && !rec[q].overflow;
+    }
+    const size_t L = rec[0].launches.size();
+    for (int q = 1; q < K && ok; ++q) {
+        ok = rec[q].launches.size() == L;
+        for (size_t j = 0; j < L && ok; ++j) {
+            const RecordedLaunch &a = rec[0].launches[j], &b = rec[q].launches[j];
+            ok = a.launch == b.launch && a.size == b.size && a.shared == b.shared && a.grid.x == b.grid.x &&
+                 a.grid.y == b.grid.y && a.block.x == b.block.x && a.block.y == b.block.y &&
+                 a.block.z == b.block.z;
+        }
+    }
+    if (!ok || L == 0) return false;
+    // the tables: launch j's K tuples `stride` bytes apart, every table 256-byte aligned
+    std::vector<size_t> off(L), stride(L);
+    size_t total = 0;
+    for (size_t j = 0; j < L; ++j) {
+        stride[j] = (rec[0].launches[j].size + 15) / 16 * 16;
+        off[j] = total;
+        total = (total + (size_t)K * stride[j] + 255) / 256 * 256;
+    }
+    if (total > lead->batch_cap) {
+        if (lead->batch_dev) (void)hipFree(lead->batch_dev);
+        if (lead->batch_host) (void)hipHostFree(lead->batch_host);
+        lead->batch_dev = nullptr;
+        lead->batch_host = nullptr;
+        lead->batch_cap = 0;
+        const size_t cap = total + total / 2;
+        if (hipMalloc((void**)&lead->batch_dev, cap) != hipSuccess ||
+            hipHostMalloc((void**)&lead->batch_host, cap, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        lead->batch_cap = cap;
+    }
+    for (size_t j = 0; j < L; ++j)
+        for (int q = 0; q < K; ++q)
+            std::memcpy(lead->batch_host + off[j] + (size_t)q * stride[j],
+                        rec[q].args.data() + rec[q].launches[j].offset, rec[q].launches[j].size);
+    // after whatever the instances' own streams still hold (state uploads)
+    hipStream_t S = lead->stream;
+    for (int i : ids)
+        if (ctx[i]->stream != S && hipStreamSynchronize(ctx[i]->stream) != hipSuccess) {
+            status[i] = fail(ctx[i], FSI_E_HIP, "hipStreamSynchronize failed");
+            return false;
+        }
+    if (hipMemcpyAsync(lead->batch_dev, lead->batch_host, total, hipMemcpyHostToDevice, S) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    for (size_t j = 0; j < L; ++j) {
+        const RecordedLaunch& a = rec[0].launches[j];
+        a.launch(a.grid, a.block, a.shared, S, lead->batch_dev + off[j], K, (int)stride[j]);
+    }
+    for (int i : ids) (void)hipMemcpyAsync(ctx[i]->st_host, ctx[i]->st, sizeof(Status), hipMemcpyDeviceToHost, S);
+    const hipError_t e = hipStreamSynchronize(S);
+    for (int i : ids) {
+        if (e != hipSuccess) status[i] = fail(ctx[i], FSI_E_HIP, "batched step: %s", hipGetErrorString(e));
+        runs[i].planned = e == hipSuccess;
+    }
+    return true;
+}
+
 extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_state* const* st,
                                         fsi_workspace* const* w, const double* t, const double* dt,
                                         fsi_step_report* reps, fsi_step_stats* stats, int32_t* status) {
@@ -1734,34 +1845,86 @@ extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_
     // instance (the single-step overlap of the phase and sigma kernels with
     // the next solve) would put two queues per instance on the hardware's
     // few (measured: 64 cavities 95 M against 166 M cell updates/s).
+    // Instances with a planned step and the same grid shape (and device)
+    // form a group stepped by batched launches; the rest, and any group
+    // whose launch sequences turn out to differ, are stepped one by one.
+    // FSI_ENSEMBLE_BATCH=0 steps every instance on its own.
+    const char* eb = std::getenv("FSI_ENSEMBLE_BATCH");
+    const bool batch = !(eb && eb[0] == '0');
+    std::vector<std::vector<int>> groups;  // batched groups (>= 2 instances)
+    std::vector<int> single;               // instances stepped on their own
+    {
+        std::vector<char> planned((size_t)n, 0);
+        for (int k = 0; k < n; ++k) {
+            status[k] = step_check(ctx[k], st[k], w[k], t[k], dt[k]);
+            planned[k] = batch && status[k] == FSI_OK && plan_step(ctx[k]);
+        }
+        std::vector<char> taken((size_t)n, 0);
+        for (int k = 0; k < n; ++k) {
+            if (!planned[k] || taken[k]) continue;
+            std::vector<int> g{k};
+            taken[k] = 1;
+            for (int j = k + 1; j < n; ++j)
+                if (planned[j] && !taken[j] && ctx[j]->grid.nx == ctx[k]->grid.nx &&
+                    ctx[j]->grid.ny == ctx[k]->grid.ny && ctx[j]->device == ctx[k]->device) {
+                    g.push_back(j);
+                    taken[j] = 1;
+                }
+            if (g.size() >= 2) groups.push_back(g);
+            else single.push_back(k);
+        }
+        for (int k = 0; k < n; ++k)
+            if (!planned[k] && status[k] == FSI_OK) single.push_back(k);
+    }
+    auto finish = [&](int k) {
+        if (status[k] != FSI_OK) return;
+        const int dual = ctx[k]->dual;
+        ctx[k]->dual = 0;
+        status[k] = step_finish(ctx[k], st[k], w[k], dt[k], runs[k], reps ? &reps[k] : nullptr,
+                                stats ? &stats[k] : nullptr);
+        ctx[k]->dual = dual;
+    };
+    // jobs: a batched group, or a share of the single instances
+    const int nj = (int)groups.size() + std::min<int>(nt, (int)single.size());
     auto work = [&](int q) {
-        for (int k = q; k < n; k += nt) {
+        if (q < (int)groups.size()) {
+            const std::vector<int>& g = groups[q];
+            if (hipSetDevice(ctx[g[0]]->device) != hipSuccess) {
+                for (int k : g) status[k] = fail(ctx[k], FSI_E_HIP, "hipSetDevice(%d) failed", ctx[k]->device);
+                return;
+            }
+            if (!batched_group_step(g, ctx, st, w, dt, runs, status))
+                for (int k : g)  // one by one, each with its own plan again
+                    if (status[k] == FSI_OK) {
+                        const int dual = ctx[k]->dual;
+                        ctx[k]->dual = 0;
+                        status[k] = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
+                        ctx[k]->dual = dual;
+                    }
+            for (int k : g) finish(k);
+            return;
+        }
+        const int share = q - (int)groups.size(), ns = nj - (int)groups.size();
+        for (size_t i = (size_t)share; i < single.size(); i += (size_t)ns) {
+            const int k = single[i];
+            if (status[k] != FSI_OK) continue;
             if (hipSetDevice(ctx[k]->device) != hipSuccess) {
                 status[k] = fail(ctx[k], FSI_E_HIP, "hipSetDevice(%d) failed", ctx[k]->device);
                 continue;
             }
             const int dual = ctx[k]->dual;
             ctx[k]->dual = 0;
-            fsi_status rc = step_check(ctx[k], st[k], w[k], t[k], dt[k]);
-            if (rc == FSI_OK) rc = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
+            status[k] = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
             ctx[k]->dual = dual;
-            status[k] = rc;
         }
-        for (int k = q; k < n; k += nt)
-            if (status[k] == FSI_OK) {
-                const int dual = ctx[k]->dual;
-                ctx[k]->dual = 0;
-                status[k] = step_finish(ctx[k], st[k], w[k], dt[k], runs[k], reps ? &reps[k] : nullptr,
-                                        stats ? &stats[k] : nullptr);
-                ctx[k]->dual = dual;
-            }
+        for (size_t i = (size_t)share; i < single.size(); i += (size_t)ns) finish(single[i]);
     };
-    if (nt == 1) {
+    if (nj == 1) {
         work(0);
-    } else {
+    } else if (nj > 1) {
         std::vector<std::thread> pool;
-        pool.reserve((size_t)nt);
-        for (int q = 0; q < nt; ++q) pool.emplace_back(work, q);
+        pool.reserve((size_t)nj);
+        for (int q = 0; q < nj; ++q) pool.emplace_back(work, q);
         for (auto& th : pool) th.join();
     }
     fsi_status first = FSI_OK;

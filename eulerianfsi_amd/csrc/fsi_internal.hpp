@@ -51,6 +51,11 @@ struct fsi_context {
     std::string err;
     int device = -1;  // the HIP device current at creation
     double* force_row = nullptr;
+    // argument tables of the batched ensemble steps this context leads
+    // (pinned host staging and its device copy)
+    char* batch_host = nullptr;
+    char* batch_dev = nullptr;
+    size_t batch_cap = 0;
     double* uw_smooth = nullptr;  // UWCDI :smoothed9: the filtered phi of the stage being evaluated
     double* cap_buf = nullptr;    // DiffuseAreaCapillarity: force x, force y, then 5 work fields
     double* ext = nullptr;  // per-block extrema of the report and checks (EX_* slots)
