@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -40,6 +41,7 @@ fsi_status fail(fsi_context* c, fsi_status code, const char* fmt, ...) {
 
 
 thread_local LaunchRecorder* fsi::g_recorder = nullptr;
+
 
 hipError_t dalloc(fsi_context* c, double** p, size_t n) {
     hipError_t e = hipMalloc((void**)p, n * sizeof(double));
@@ -441,8 +443,9 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
             auto axis = [](int n, int kind, double* t) {
                 SpecAxis a{};
                 a.n = n;
+                a.r3 = (n & (n - 1)) ? 1 : 0;  // 3 * 2^m
                 a.m = 0;
-                while ((1 << a.m) < n) ++a.m;
+                while (((a.r3 ? 3 : 1) << a.m) < n) ++a.m;
                 a.kind = kind;
                 a.wfft = reinterpret_cast<const double2*>(t);
                 a.wpost = reinterpret_cast<const double2*>(t + n);
@@ -1720,73 +1723,95 @@ extern "C" fsi_status fsi_integrate_step(fsi_context* c, fsi_state* s, fsi_works
     return step_finish(c, s, w, dt, run, rep, stats);
 }
 
-// One planned step of several instances of one grid shape, batched: the
-// plans are equalised (the largest iteration counts of the group: kernels
-// past a solve's convergence return at once, so each instance's result is
-// the one its own planned step gives), each instance's launch sequence is
-// recorded, and when the sequences agree kernel by kernel every kernel runs
-// once for the whole group (grid.z = instances) on the leader's stream, its
-// arguments read from a table per launch.  Returns false, with nothing
-// enqueued, when the sequences differ; the caller then steps the instances
-// one by one.  On success each run is a planned step awaiting step_finish.
-static bool batched_group_step(const std::vector<int>& ids, fsi_context* const* ctx, fsi_state* const* st,
-                               fsi_workspace* const* w, const double* dt, std::vector<StepRun>& runs,
-                               int32_t* status) {
-    const int K = (int)ids.size();
-    fsi_context* lead = ctx[ids[0]];
+// What must agree for two instances' planned steps to launch the same
+// kernel sequence: the grid shape and device, the precision, and per stage
+// whether the projection only tests the direct solve (a plan of one) or runs
+// PCG iterations; in mixed precision also the planned refinement cycles.
+// Larger iteration plans are harmless (kernels past convergence return).
+static std::vector<long long> batch_key(const fsi_context* c) {
+    std::vector<long long> key{c->grid.nx, c->grid.ny, c->device, c->num.precision};
+    for (int k = 0; k < 3; ++k) {
+        key.push_back(std::min(c->plan_proj[k], c->num.projection_maxiter + 1) == 1);
+        if (c->num.precision == FSI_PRECISION_MIXED) key.push_back(c->plan_cyc[k]);
+    }
+    return key;
+}
+
+// A group of instances whose planned steps run as batched launches: the
+// launch sequence recorded per instance, and the argument tables (launch j:
+// the K tuples `stride[j]` bytes apart from off[j]) staged in the leader's
+// pinned buffer.
+struct BatchGroup {
+    std::vector<int> ids;
+    std::vector<LaunchRecorder> rec;
+    std::vector<size_t> off, stride;
+    size_t total = 0;
+};
+
+// Records one planned step of each instance of the group with the plans
+// equalised (the largest iteration counts of the group: kernels past a
+// solve's convergence return at once, so each instance's result is the one
+// its own planned step gives) and stages the argument tables.  False, with
+// nothing enqueued, when the sequences differ kernel by kernel or the tables
+// cannot be allocated; the instances are then stepped one by one.
+static bool record_group(BatchGroup& g, fsi_context* const* ctx, fsi_state* const* st, fsi_workspace* const* w,
+                         const double* dt) {
+    const int K = (int)g.ids.size();
+    fsi_context* lead = ctx[g.ids[0]];
     for (int k = 0; k < 3; ++k) {
         int pv = 0, pp = 0, pc = 0, cf = 1 << 30;
-        for (int i : ids) {
+        for (int i : g.ids) {
             pv = std::max(pv, ctx[i]->plan_vel[k]);
             pp = std::max(pp, ctx[i]->plan_proj[k]);
             pc = std::max(pc, ctx[i]->plan_cyc[k]);
             cf = std::min(cf, ctx[i]->vel_check_from[k]);
         }
-        for (int i : ids) {
+        for (int i : g.ids) {
             ctx[i]->plan_vel[k] = pv;
             ctx[i]->plan_proj[k] = pp;
             ctx[i]->plan_cyc[k] = pc;
             ctx[i]->vel_check_from[k] = cf;
         }
     }
-    std::vector<LaunchRecorder> rec((size_t)K);
+    g.rec.assign((size_t)K, LaunchRecorder{});
     bool ok = true;
     for (int q = 0; q < K && ok; ++q) {
-        fsi_context* c = ctx[ids[q]];
-        g_recorder = &rec[q];
+        fsi_context* c = ctx[g.ids[q]];
+        g_recorder = &g.rec[q];
         const int dual = c->dual;
         c->dual = 0;
-        ok = status_push_reset(c) == FSI_OK && enqueue_step(c, st[ids[q]], w[ids[q]], dt[ids[q]], true) == FSI_OK;
+        ok = status_push_reset(c) == FSI_OK &&
+             enqueue_step(c, st[g.ids[q]], w[g.ids[q]], dt[g.ids[q]], true) == FSI_OK;
         c->dual = dual;
         g_recorder = nullptr;
-        ok = ok && !rec[q].overflow;
+        ok = ok && !g.rec[q].overflow;
     }
-    const size_t L = rec[0].launches.size();
+    const size_t L = g.rec[0].launches.size();
     for (int q = 1; q < K && ok; ++q) {
-        ok = rec[q].launches.size() == L;
+        ok = g.rec[q].launches.size() == L;
         for (size_t j = 0; j < L && ok; ++j) {
-            const RecordedLaunch &a = rec[0].launches[j], &b = rec[q].launches[j];
+            const RecordedLaunch &a = g.rec[0].launches[j], &b = g.rec[q].launches[j];
             ok = a.launch == b.launch && a.size == b.size && a.shared == b.shared && a.grid.x == b.grid.x &&
                  a.grid.y == b.grid.y && a.block.x == b.block.x && a.block.y == b.block.y &&
                  a.block.z == b.block.z;
         }
     }
     if (!ok || L == 0) return false;
-    // the tables: launch j's K tuples `stride` bytes apart, every table 256-byte aligned
-    std::vector<size_t> off(L), stride(L);
-    size_t total = 0;
+    g.off.resize(L);
+    g.stride.resize(L);
+    g.total = 0;
     for (size_t j = 0; j < L; ++j) {
-        stride[j] = (rec[0].launches[j].size + 15) / 16 * 16;
-        off[j] = total;
-        total = (total + (size_t)K * stride[j] + 255) / 256 * 256;
+        g.stride[j] = (g.rec[0].launches[j].size + 15) / 16 * 16;
+        g.off[j] = g.total;
+        g.total = (g.total + (size_t)K * g.stride[j] + 255) / 256 * 256;
     }
-    if (total > lead->batch_cap) {
+    if (g.total > lead->batch_cap) {
         if (lead->batch_dev) (void)hipFree(lead->batch_dev);
         if (lead->batch_host) (void)hipHostFree(lead->batch_host);
         lead->batch_dev = nullptr;
         lead->batch_host = nullptr;
         lead->batch_cap = 0;
-        const size_t cap = total + total / 2;
+        const size_t cap = g.total + g.total / 2;
         if (hipMalloc((void**)&lead->batch_dev, cap) != hipSuccess ||
             hipHostMalloc((void**)&lead->batch_host, cap, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
@@ -1796,29 +1821,25 @@ static bool batched_group_step(const std::vector<int>& ids, fsi_context* const* 
     }
     for (size_t j = 0; j < L; ++j)
         for (int q = 0; q < K; ++q)
-            std::memcpy(lead->batch_host + off[j] + (size_t)q * stride[j],
-                        rec[q].args.data() + rec[q].launches[j].offset, rec[q].launches[j].size);
-    // after whatever the instances' own streams still hold (state uploads)
-    hipStream_t S = lead->stream;
-    for (int i : ids)
-        if (ctx[i]->stream != S && hipStreamSynchronize(ctx[i]->stream) != hipSuccess) {
-            status[i] = fail(ctx[i], FSI_E_HIP, "hipStreamSynchronize failed");
-            return false;
-        }
-    if (hipMemcpyAsync(lead->batch_dev, lead->batch_host, total, hipMemcpyHostToDevice, S) != hipSuccess) {
+            std::memcpy(lead->batch_host + g.off[j] + (size_t)q * g.stride[j],
+                        g.rec[q].args.data() + g.rec[q].launches[j].offset, g.rec[q].launches[j].size);
+    return true;
+}
+
+// The group's tables, its launches (every kernel once for the whole group,
+// grid.z = instances) and the read-back of each instance's Status, on S.
+static bool enqueue_group(const BatchGroup& g, fsi_context* const* ctx, hipStream_t S) {
+    const fsi_context* lead = ctx[g.ids[0]];
+    const int K = (int)g.ids.size();
+    if (hipMemcpyAsync(lead->batch_dev, lead->batch_host, g.total, hipMemcpyHostToDevice, S) != hipSuccess) {
         (void)hipGetLastError();
         return false;
     }
-    for (size_t j = 0; j < L; ++j) {
-        const RecordedLaunch& a = rec[0].launches[j];
-        a.launch(a.grid, a.block, a.shared, S, lead->batch_dev + off[j], K, (int)stride[j]);
+    for (size_t j = 0; j < g.rec[0].launches.size(); ++j) {
+        const RecordedLaunch& a = g.rec[0].launches[j];
+        a.launch(a.grid, a.block, a.shared, S, lead->batch_dev + g.off[j], K, (int)g.stride[j]);
     }
-    for (int i : ids) (void)hipMemcpyAsync(ctx[i]->st_host, ctx[i]->st, sizeof(Status), hipMemcpyDeviceToHost, S);
-    const hipError_t e = hipStreamSynchronize(S);
-    for (int i : ids) {
-        if (e != hipSuccess) status[i] = fail(ctx[i], FSI_E_HIP, "batched step: %s", hipGetErrorString(e));
-        runs[i].planned = e == hipSuccess;
-    }
+    for (int i : g.ids) (void)hipMemcpyAsync(ctx[i]->st_host, ctx[i]->st, sizeof(Status), hipMemcpyDeviceToHost, S);
     return true;
 }
 
@@ -1831,50 +1852,43 @@ extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_
     for (int k = 0; k < n; ++k)
         if (!ctx[k]) return FSI_E_ARGUMENT;
     std::vector<StepRun> runs((size_t)n);
-    // Host threads share the launches: thread q enqueues the planned steps of
-    // instances q, q + T, ... (every one in flight before it waits for any),
-    // then finishes them; the contexts are independent, so each instance
-    // advances exactly as fsi_integrate_step would advance it.
-    int nt = 8;
-    if (const char* e = std::getenv("FSI_ENSEMBLE_THREADS")) nt = std::max(1, std::atoi(e));
-    const unsigned hw = std::thread::hardware_concurrency();
-    if (hw > 0) nt = std::min(nt, (int)hw);
-    nt = std::min(nt, (int)n);
-    // Each instance's step runs on its context's main stream alone: the
-    // instances already overlap one another, and a second stream per
-    // instance (the single-step overlap of the phase and sigma kernels with
-    // the next solve) would put two queues per instance on the hardware's
-    // few (measured: 64 cavities 95 M against 166 M cell updates/s).
-    // Instances with a planned step and the same grid shape (and device)
-    // form a group stepped by batched launches; the rest, and any group
-    // whose launch sequences turn out to differ, are stepped one by one.
+    // Instances with a planned step and the same launch sequence (batch_key)
+    // form groups stepped by batched launches: every group's launches are
+    // enqueued back to back on one stream (the groups one after the other:
+    // groups running side by side on several streams gave results that
+    // differed run to run in the last bits, which one stream does not), then
+    // each instance's step is finished as fsi_integrate_step finishes it.
+    // The rest, and any group whose sequences turn out to differ, are stepped
+    // one by one by host threads, each instance on its own stream.
     // FSI_ENSEMBLE_BATCH=0 steps every instance on its own.
     const char* eb = std::getenv("FSI_ENSEMBLE_BATCH");
     const bool batch = !(eb && eb[0] == '0');
-    std::vector<std::vector<int>> groups;  // batched groups (>= 2 instances)
-    std::vector<int> single;               // instances stepped on their own
+    std::vector<BatchGroup> groups;
+    std::vector<int> single;
     {
-        std::vector<char> planned((size_t)n, 0);
+        std::vector<char> planned((size_t)n, 0), taken((size_t)n, 0);
         for (int k = 0; k < n; ++k) {
             status[k] = step_check(ctx[k], st[k], w[k], t[k], dt[k]);
             planned[k] = batch && status[k] == FSI_OK && plan_step(ctx[k]);
         }
-        std::vector<char> taken((size_t)n, 0);
         for (int k = 0; k < n; ++k) {
-            if (!planned[k] || taken[k]) continue;
-            std::vector<int> g{k};
+            if (status[k] != FSI_OK || taken[k]) continue;
             taken[k] = 1;
+            if (!planned[k]) {
+                single.push_back(k);
+                continue;
+            }
+            BatchGroup g;
+            g.ids.push_back(k);
+            const std::vector<long long> key = batch_key(ctx[k]);
             for (int j = k + 1; j < n; ++j)
-                if (planned[j] && !taken[j] && ctx[j]->grid.nx == ctx[k]->grid.nx &&
-                    ctx[j]->grid.ny == ctx[k]->grid.ny && ctx[j]->device == ctx[k]->device) {
-                    g.push_back(j);
+                if (planned[j] && !taken[j] && status[j] == FSI_OK && batch_key(ctx[j]) == key) {
+                    g.ids.push_back(j);
                     taken[j] = 1;
                 }
-            if (g.size() >= 2) groups.push_back(g);
+            if (g.ids.size() >= 2) groups.push_back(std::move(g));
             else single.push_back(k);
         }
-        for (int k = 0; k < n; ++k)
-            if (!planned[k] && status[k] == FSI_OK) single.push_back(k);
     }
     auto finish = [&](int k) {
         if (status[k] != FSI_OK) return;
@@ -1884,28 +1898,56 @@ extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_
                                 stats ? &stats[k] : nullptr);
         ctx[k]->dual = dual;
     };
-    // jobs: a batched group, or a share of the single instances
-    const int nj = (int)groups.size() + std::min<int>(nt, (int)single.size());
-    auto work = [&](int q) {
-        if (q < (int)groups.size()) {
-            const std::vector<int>& g = groups[q];
-            if (hipSetDevice(ctx[g[0]]->device) != hipSuccess) {
-                for (int k : g) status[k] = fail(ctx[k], FSI_E_HIP, "hipSetDevice(%d) failed", ctx[k]->device);
-                return;
-            }
-            if (!batched_group_step(g, ctx, st, w, dt, runs, status))
-                for (int k : g)  // one by one, each with its own plan again
-                    if (status[k] == FSI_OK) {
-                        const int dual = ctx[k]->dual;
-                        ctx[k]->dual = 0;
-                        status[k] = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
-                        ctx[k]->dual = dual;
-                    }
-            for (int k : g) finish(k);
-            return;
+    // the batched groups
+    if (!groups.empty()) {
+        const int dev = ctx[groups[0].ids[0]]->device;
+        std::vector<char> recorded(groups.size(), 0);
+        bool dev_ok = hipSetDevice(dev) == hipSuccess;
+        for (size_t q = 0; q < groups.size(); ++q) {
+            BatchGroup& g = groups[q];
+            recorded[q] = dev_ok && ctx[g.ids[0]]->device == dev && record_group(g, ctx, st, w, dt);
+            if (!recorded[q])
+                for (int k : g.ids) single.push_back(k);  // one by one, each with its own plan again
         }
-        const int share = q - (int)groups.size(), ns = nj - (int)groups.size();
-        for (size_t i = (size_t)share; i < single.size(); i += (size_t)ns) {
+        hipStream_t S = nullptr;
+        for (size_t q = 0; q < groups.size(); ++q)
+            if (recorded[q]) {
+                if (!S) S = ctx[groups[q].ids[0]]->stream;
+                for (int k : groups[q].ids)  // after whatever their own streams still hold (state uploads)
+                    if (ctx[k]->stream != S && hipStreamSynchronize(ctx[k]->stream) != hipSuccess)
+                        status[k] = fail(ctx[k], FSI_E_HIP, "hipStreamSynchronize failed");
+            }
+        std::vector<char> enq(groups.size(), 0);
+        for (size_t q = 0; q < groups.size(); ++q)
+            if (recorded[q]) {
+                enq[q] = enqueue_group(groups[q], ctx, S);
+                if (!enq[q])
+                    for (int k : groups[q].ids) single.push_back(k);
+            }
+        const hipError_t e = S ? hipStreamSynchronize(S) : hipSuccess;
+        for (size_t q = 0; q < groups.size(); ++q)
+            if (enq[q])
+                for (int k : groups[q].ids) {
+                    if (e != hipSuccess && status[k] == FSI_OK)
+                        status[k] = fail(ctx[k], FSI_E_HIP, "batched step: %s", hipGetErrorString(e));
+                    runs[k].planned = e == hipSuccess;
+                    finish(k);
+                }
+    }
+    // the instances stepped one by one: host threads share the launches;
+    // thread q enqueues the planned steps of instances q, q + T, ... (every
+    // one in flight before it waits for any), then finishes them.  Each runs
+    // on its context's main stream alone: a second stream per instance (the
+    // single-step overlap of the phase and sigma kernels with the next
+    // solve) would put two queues per instance on the hardware's few
+    // (measured: 64 cavities 95 M against 166 M cell updates/s).
+    int nt = 8;
+    if (const char* e = std::getenv("FSI_ENSEMBLE_THREADS")) nt = std::max(1, std::atoi(e));
+    const unsigned hw = std::thread::hardware_concurrency();
+    if (hw > 0) nt = std::min(nt, (int)hw);
+    nt = std::min(nt, (int)single.size());
+    auto work = [&](int q) {
+        for (size_t i = (size_t)q; i < single.size(); i += (size_t)nt) {
             const int k = single[i];
             if (status[k] != FSI_OK) continue;
             if (hipSetDevice(ctx[k]->device) != hipSuccess) {
@@ -1917,14 +1959,14 @@ extern "C" fsi_status fsi_ensemble_step(int32_t n, fsi_context* const* ctx, fsi_
             status[k] = step_begin(ctx[k], st[k], w[k], dt[k], runs[k]);
             ctx[k]->dual = dual;
         }
-        for (size_t i = (size_t)share; i < single.size(); i += (size_t)ns) finish(single[i]);
+        for (size_t i = (size_t)q; i < single.size(); i += (size_t)nt) finish(single[i]);
     };
-    if (nj == 1) {
+    if (nt == 1) {
         work(0);
-    } else if (nj > 1) {
+    } else if (nt > 1) {
         std::vector<std::thread> pool;
-        pool.reserve((size_t)nj);
-        for (int q = 0; q < nj; ++q) pool.emplace_back(work, q);
+        pool.reserve((size_t)nt);
+        for (int q = 0; q < nt; ++q) pool.emplace_back(work, q);
         for (auto& th : pool) th.join();
     }
     fsi_status first = FSI_OK;

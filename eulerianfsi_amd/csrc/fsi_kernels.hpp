@@ -256,7 +256,8 @@ void launch_mg_coarse(hipStream_t s, const MgGeom& top, const CoarsePlan& cp, co
 enum SpecKind { SPEC_NONE = -1, SPEC_COS = 0, SPEC_ALT = 1, SPEC_RDFT = 2 };
 constexpr int SPEC_MIN_N = 8, SPEC_MAX_N = 2048;
 struct SpecAxis {
-    int n, m, kind;        // length (power of two), log2 n, SpecKind
+    int n, m, kind;        // length 2^m (r3 = 0) or 3 * 2^m (r3 = 1), SpecKind
+    int r3;
     const double2* wfft;   // e^{-2 pi i t / n}, t < n/2
     const double2* wpost;  // e^{-i pi k / (2n)}, k < n (cosine kinds)
     const double* mu;      // eigenvalue of the 1-D operator per coefficient index
@@ -266,8 +267,8 @@ struct SpecPlan {
     SpecAxis x, y;
 };
 // The kind of one axis from its geometry (SPEC_NONE if not diagonalisable
-// here: mixed Neumann/Dirichlet sides, or a length that is not a power of two
-// in [SPEC_MIN_N, SPEC_MAX_N]).
+// here: mixed Neumann/Dirichlet sides, or a length outside [SPEC_MIN_N,
+// SPEC_MAX_N] that is neither 2^m nor 3 * 2^m).
 int spec_kind(int n, int periodic, int dlo, int dhi);
 // Host tables of one axis, packed as wfft (n/2 complex), wpost (n complex),
 // mu (n); h2inv = 1 / h^2.

@@ -21,10 +21,13 @@
 // The (0, 0) mode is zeroed where both axes have the constant null vector,
 // which is the zero-mean gauge of the singular Neumann/periodic system.
 //
-// Every transform of two real lines is one complex FFT of length n = 2^m in
-// LDS: the lines are packed as z = x1 + i x2, loaded in bit-reversed order
-// and transformed in place by decimation-in-time radix-4 stages (one radix-2
-// stage first when m is odd), then separated with X1 = (Z_k + conj Z_{n-k})/2,
+// Every transform of two real lines is one complex FFT of length n = 2^m or
+// n = 3 * 2^m in LDS: the lines are packed as z = x1 + i x2, loaded in
+// bit-reversed order (for 3 * 2^m: input i = r + 2^m q at 3 brev(r) + q, so
+// that each triple holds one length-3 sub-transform) and transformed in place
+// by decimation-in-time stages: radix 3 on the triples when n = 3 * 2^m,
+// then radix 4 (one radix-2 stage first when m is odd) on spans b 2^s with
+// b = 1 or 3; then separated with X1 = (Z_k + conj Z_{n-k})/2,
 // X2 = (Z_k - conj Z_{n-k})/(2i).  The cosine transform of a real line uses
 // Makhoul's reordering v = (x0, x2, x4, ..., x5, x3, x1): C_k = Re(e^{-i pi
 // k/2n} V_k); its inverse rebuilds V_k = e^{i pi k/2n} (C_k - i C_{n-k}) and
@@ -38,6 +41,11 @@ namespace fsi {
 
 
 __device__ __forceinline__ int brev(int j, int m) { return (int)(__brev((unsigned)j) >> (32 - m)); }
+// LDS position of input index j of a transform of length 2^m (r3 = 0) or 3 * 2^m
+__device__ __forceinline__ int fft_pos(int j, int m, int r3) {
+    if (!r3) return brev(j, m);
+    return 3 * brev(j & ((1 << m) - 1), m) + (j >> m);
+}
 
 // LDS slot of index j with one pad double per 32: the power-of-two strides
 // of the bit-reversed scatter, of the first butterfly spans and of the
@@ -85,12 +93,92 @@ __host__ __device__ constexpr int spec_table_values(int n, int kind, bool with_m
     return 2 * pdn(n >> 1) + (kind == SPEC_RDFT ? 0 : 2 * n) + (with_mu ? n : 0);
 }
 
-// F complex FFTs of length n = 2^m in place on re/im (transform f at
-// f * S, S >= pdn(n), input in bit-reversed order, output in natural order);
-// twiddles wr + i wi = e^{-2 pi i t/n}, t < n/2.  Every line is addressed
-// through pd().  Ends with a barrier.
+// The radix-3 stage and the radix-2/4 stages of spans 3 * 2^s of F complex
+// FFTs of length n = 3 * 2^m (the layout and twiddles of lds_fft).
 template <int NT>
-__device__ void lds_fft(double* re, double* im, const double* wr, const double* wi, int m, int F, int S, int tid) {
+__device__ void lds_fft3(double* re, double* im, const double* wr, const double* wi, int m, int F, int S, int tid) {
+    const int n = 3 << m, n3 = 1 << m;
+    constexpr double c3 = -0.5, s3 = -0.86602540378443864676;  // e^{-2 pi i/3}
+    // X0 = A + (B + C), X1,2 = A + c3 (B + C) +- i s3 (B - C)
+    for (int q = tid; q < F * n3; q += NT) {
+        const int f = q >> m, l = 3 * (q & (n3 - 1)), b = f * S;
+        const int j0 = b + pd(l), j1 = b + pd(l + 1), j2 = b + pd(l + 2);
+        const double Ar = re[j0], Ai = im[j0], Br = re[j1], Bi = im[j1], Cr = re[j2], Ci = im[j2];
+        const double tr = Br + Cr, ti = Bi + Ci, ur = Br - Cr, ui = Bi - Ci;
+        const double mr = Ar + c3 * tr, mi = Ai + c3 * ti;
+        const double pr = -s3 * ui, pi = s3 * ur;
+        re[j0] = Ar + tr;
+        im[j0] = Ai + ti;
+        re[j1] = mr + pr;
+        im[j1] = mi + pi;
+        re[j2] = mr - pr;
+        im[j2] = mi - pi;
+    }
+    __syncthreads();
+    int s = 0;
+    if (m & 1) {  // radix 2, span 3: pairs (l, l+3) in blocks of 6, twiddle e^{-2 pi i t/6}
+        const int nh = n >> 1;
+        for (int q = tid; q < F * nh; q += NT) {
+            const int f = q / nh, r = q - f * nh;
+            const int blk = r / 3, t = r - 3 * blk;
+            const int l = 6 * blk + t, b = f * S;
+            const int iw = pd(t << (m - 1));
+            const int j0 = b + pd(l), j1 = b + pd(l + 3);
+            const double w_r = wr[iw], w_i = wi[iw];
+            const double Br = re[j1], Bi = im[j1];
+            const double bR = w_r * Br - w_i * Bi, bI = w_r * Bi + w_i * Br;
+            const double Ar = re[j0], Ai = im[j0];
+            re[j0] = Ar + bR;
+            im[j0] = Ai + bI;
+            re[j1] = Ar - bR;
+            im[j1] = Ai - bI;
+        }
+        s = 1;
+        __syncthreads();
+    }
+    const int nq = n >> 2;
+    for (; s < m; s += 2) {  // radix 4 on spans h = 3 * 2^s and 2h, as in lds_fft
+        const int h = 3 << s;
+        for (int q = tid; q < F * nq; q += NT) {
+            const int f = q / nq, r = q - f * nq;
+            const int blk = (r >> s) / 3, t = r - blk * h;
+            const int l = blk * 4 * h + t, b = f * S;
+            const int i1 = pd(t << (m - s - 1)), i2 = pd(t << (m - s - 2));
+            const int ja = b + pd(l), jb = b + pd(l + h), jc = b + pd(l + 2 * h), jd = b + pd(l + 3 * h);
+            const double w1r = wr[i1], w1i = wi[i1], w2r = wr[i2], w2i = wi[i2];
+            const double Ar = re[ja], Ai = im[ja], Br = re[jb], Bi = im[jb];
+            const double Cr = re[jc], Ci = im[jc], Dr = re[jd], Di = im[jd];
+            const double bR = w1r * Br - w1i * Bi, bI = w1r * Bi + w1i * Br;
+            const double dR = w1r * Dr - w1i * Di, dI = w1r * Di + w1i * Dr;
+            const double A1r = Ar + bR, A1i = Ai + bI, B1r = Ar - bR, B1i = Ai - bI;
+            const double C1r = Cr + dR, C1i = Ci + dI, D1r = Cr - dR, D1i = Ci - dI;
+            const double cR = w2r * C1r - w2i * C1i, cI = w2r * C1i + w2i * C1r;
+            const double gR = w2r * D1r - w2i * D1i, gI = w2r * D1i + w2i * D1r;  // w2 D1
+            const double eR = gI, eI = -gR;                                         // -i w2 D1
+            re[ja] = A1r + cR;
+            im[ja] = A1i + cI;
+            re[jc] = A1r - cR;
+            im[jc] = A1i - cI;
+            re[jb] = B1r + eR;
+            im[jb] = B1i + eI;
+            re[jd] = B1r - eR;
+            im[jd] = B1i - eI;
+        }
+        __syncthreads();
+    }
+}
+
+// F complex FFTs of length n = 2^m (r3 = 0) or 3 * 2^m (r3 = 1) in place on
+// re/im (transform f at f * S, S >= pdn(n), input in the order of fft_pos,
+// output in natural order); twiddles wr + i wi = e^{-2 pi i t/n}, t < n/2.
+// Every line is addressed through pd().  Ends with a barrier.
+template <int NT>
+__device__ void lds_fft(double* re, double* im, const double* wr, const double* wi, int m, int r3, int F, int S,
+                        int tid) {
+    if (r3) {
+        lds_fft3<NT>(re, im, wr, wi, m, F, S, tid);
+        return;
+    }
     const int n = 1 << m;
     int s = 0;
     if (m & 1) {  // radix 2, span 1: butterflies (2q, 2q+1), twiddle 1
@@ -145,7 +233,7 @@ __device__ void lds_fft(double* re, double* im, const double* wr, const double* 
 template <int KIND>
 __device__ __forceinline__ void split_coeffs(int n, const SpecLds& t, const double* re, const double* im, int k,
                                              double& o1, double& o2) {
-    const int kr = (n - k) & (n - 1);
+    const int kr = k ? n - k : 0;
     const double zr = re[pd(k)], zi = im[pd(k)], cr = re[pd(kr)], ci = im[pd(kr)];
     const double x1r = 0.5 * (zr + cr), x1i = 0.5 * (zi - ci);
     const double x2r = 0.5 * (zi + ci), x2i = -0.5 * (zr - cr);
@@ -275,12 +363,12 @@ FSI_KERNEL void k_spec_rows_fwd(SpecAxis ax, int nrows, int P, const double* __r
         if (i >= n) break;
         const int jv = KIND == SPEC_RDFT ? i : mk_pos(i, n);
         const double s = (KIND == SPEC_ALT && (i & 1)) ? -scale : scale;
-        const int p = pd(brev(jv, m));
+        const int p = pd(fft_pos(jv, m, ax.r3));
         re[p] = s * a[it];
         im[p] = s * b[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, pdn(n), tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, ax.r3, 1, pdn(n), tid);
     double* o0 = out + (size_t)j0 * P;
     for (int k = tid; k < n; k += NT) {
         double o1, o2;
@@ -336,12 +424,12 @@ FSI_KERNEL void k_spec_rows_inv(SpecAxis ax, int nrows, int P, double* x, double
     for (int it = 0; it < IT; ++it) {
         const int k = tid + it * NT;
         if (k >= n) break;
-        const int p = pd(brev(k, m));
+        const int p = pd(fft_pos(k, m, ax.r3));
         re[p] = a[it];
         im[p] = b[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, 1, pdn(n), tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, ax.r3, 1, pdn(n), tid);
     const double inv_n = 1.0 / (double)n;
     double acc[1] = {0.0};
     for (int i = tid; i < n; i += NT) {  // z = conj(FFT(conj Z)) / n
@@ -391,19 +479,19 @@ FSI_KERNEL void k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, 
         const int y = e / W, c = e % W;
         const int jv = KIND == SPEC_RDFT ? y : mk_pos(y, n);
         const double s = (KIND == SPEC_ALT && (y & 1)) ? -1.0 : 1.0;
-        const int p = (c >> 1) * S + pd(brev(jv, m));
+        const int p = (c >> 1) * S + pd(fft_pos(jv, m, ay.r3));
         if (c & 1) im[p] = s * v[it];
         else re[p] = s * v[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, C, S, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, ay.r3, C, S, tid);
     // coefficients of the column pair f at row mode l, divided by the eigenvalue
     double o1[IT], o2[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int q = tid + it * NT;
         if (q >= C * n) break;
-        const int f = q >> m, l = q & (n - 1);
+        const int f = ay.r3 ? (q >= n) : q >> m, l = q - f * n;  // C <= 2 transforms
         split_coeffs<KIND>(n, t, re + f * S, im + f * S, l, o1[it], o2[it]);
         const double my = t.mu[l];
         const double l1 = lx[2 * f] + my, l2 = lx[2 * f + 1] + my;
@@ -415,7 +503,8 @@ FSI_KERNEL void k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, 
     for (int it = 0; it < IT; ++it) {
         const int q = tid + it * NT;
         if (q >= C * n) break;
-        const int p = (q >> m) * S + pd(q & (n - 1));
+        const int f = ay.r3 ? (q >= n) : q >> m;
+        const int p = f * S + pd(q - f * n);
         re[p] = o1[it];  // coefficient line of column 2f in re[f S ...], 2f+1 in im[f S ...]
         im[p] = o2[it];
     }
@@ -424,7 +513,7 @@ FSI_KERNEL void k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, 
     for (int it = 0; it < IT; ++it) {
         const int q = tid + it * NT;
         if (q >= C * n) break;
-        const int f = q >> m, l = q & (n - 1);
+        const int f = ay.r3 ? (q >= n) : q >> m, l = q - f * n;
         double v1r, v1i, v2r, v2i;
         line_spectrum<KIND>(n, t, re + f * S, l, v1r, v1i);
         line_spectrum<KIND>(n, t, im + f * S, l, v2r, v2i);
@@ -436,12 +525,13 @@ FSI_KERNEL void k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, 
     for (int it = 0; it < IT; ++it) {
         const int q = tid + it * NT;
         if (q >= C * n) break;
-        const int p = (q >> m) * S + pd(brev(q & (n - 1), m));
+        const int f = ay.r3 ? (q >= n) : q >> m;
+        const int p = f * S + pd(fft_pos(q - f * n, m, ay.r3));
         re[p] = o1[it];
         im[p] = o2[it];
     }
     __syncthreads();
-    lds_fft<NT>(re, im, t.wr, t.wi, m, C, S, tid);
+    lds_fft<NT>(re, im, t.wr, t.wi, m, ay.r3, C, S, tid);
     const double inv_n = 1.0 / (double)n;
 #pragma unroll
     for (int it = 0; it < 2 * IT; ++it) {
@@ -458,7 +548,8 @@ FSI_KERNEL void k_spec_cols(SpecAxis ay, const double* __restrict__ mux, int P, 
 // ---- host side -------------------------------------------------------------
 
 int spec_kind(int n, int periodic, int dlo, int dhi) {
-    if (n < SPEC_MIN_N || n > SPEC_MAX_N || (n & (n - 1))) return SPEC_NONE;
+    const int odd = (n % 3 == 0) ? n / 3 : n;  // 2^m or 3 * 2^m
+    if (n < SPEC_MIN_N || n > SPEC_MAX_N || (odd & (odd - 1))) return SPEC_NONE;
     if (periodic) return SPEC_RDFT;
     if (!dlo && !dhi) return SPEC_COS;
     if (dlo && dhi) return SPEC_ALT;

@@ -23,7 +23,15 @@ CONFIGS = {
     "periodic_x_neumann_y": (64, 32, "PPNN"),
     "periodic": (32, 32, "PPPP"),
     "neumann_128x256": (128, 256, "NNNN"),
-    "fallback_48x40": (48, 40, "NNNN"),  # not a power of two: multigrid-PCG alone
+    "fallback_48x40": (48, 40, "NNNN"),  # 40 is neither 2^m nor 3 * 2^m: multigrid-PCG alone
+    # lengths 3 * 2^m (a radix-3 stage ahead of the radix-4 stages), odd and even m
+    "neumann_192x96": (192, 96, "NNNN"),
+    "dirichlet_384": (384, 384, "DDDD"),
+    "periodic_96x48": (96, 48, "PPPP"),
+    "periodic_x_neumann_y_384x192": (384, 192, "PPNN"),
+    "dirichlet_x_neumann_y_24x768": (24, 768, "DDNN", 1.0 / 32, 1.0),
+    "neumann_1536x48": (1536, 48, "NNNN", 1.0, 1.0 / 32),
+    "mixed_256x384": (256, 384, "NNDD"),
     # the benchmark sizes: row and column transforms of 1024 and 2048 points
     # (dynamic LDS above 64 KiB), and columns of 2048 points, which take the
     # one-column-pair variant of the column kernel
@@ -104,17 +112,17 @@ def test_projection_matches_oracle(name):
     assert res.linear_relative_residual <= 1e-10
 
 
-@pytest.mark.parametrize("method", ["acdi", "uwcdi"])
-def test_steps_match_multigrid_projection(monkeypatch, method):
-    """Five tight steps of the N = 128 cavity (ACDI) or the x-periodic
-    filament (UWCDI): the direct-solve projection and the multigrid-PCG
-    projection give the same fields to solver accuracy."""
+@pytest.mark.parametrize("method,N", [("acdi", 128), ("uwcdi", 128), ("acdi", 192), ("uwcdi", 96)])
+def test_steps_match_multigrid_projection(monkeypatch, method, N):
+    """Five tight steps of the cavity (ACDI) or the x-periodic filament
+    (UWCDI), N = 2^m or 3 * 2^m: the direct-solve projection and the
+    multigrid-PCG projection give the same fields to solver accuracy."""
     def run(flag):
         monkeypatch.setenv("FSI_SPECTRAL", flag)
         if method == "acdi":
-            case = setups.seeded_cavity(128, method, seed=3, velocity_method="direct", projection_rtol=1e-14)
+            case = setups.seeded_cavity(N, method, seed=3, velocity_method="direct", projection_rtol=1e-14)
         else:
-            case = setups.kolmogorov_filament(128, method, velocity_method="direct", projection_rtol=1e-14)
+            case = setups.kolmogorov_filament(N, method, velocity_method="direct", projection_rtol=1e-14)
         ctx = fsi.build_imex_context(case.metrics, case.bcs, case.models, case.numerics)
         work = fsi.allocate_imex_workspace(ctx)
         fsi.make_velocity_compatible(case.state, ctx, work)
