@@ -15,6 +15,8 @@ from .closures import (ConservativeDiffuseInterfaceFlux, ConstantBodyForce, Cons
                        LinearInternalTensorStress, MaxwellRelaxation, NoBodyForce, NoCapillarity, DiffuseAreaCapillarity, NoDiffusion,
                        NoElasticStress, NoGrowth, NoInterfaceFlux, NoPhaseFieldRegularization, NoRelaxation,
                        NoResourceConsumption, NoStructureExtension, NoViscosity, PirozzoliUWCDIPhaseField,
+                       ConstantIntrinsicGrowth, SaturatingResourceGrowth, DirectionalGrowthTensor,
+                       ConstantDiffusivity, HarmonicPhaseDiffusivity, SaturatingResourceConsumption,
                        jain_equilibrium_initialization_epsilon, phase_field_label)
 from .errors import ArgumentError, DeviceError, DimensionMismatch, DomainError, SolverError, UnsupportedError
 from .imex import (A_EXPLICIT, A_IMPLICIT, STAGE_ABSCISSA, FourStageIMEXFractionalStep, FSIBoundaryConditions,

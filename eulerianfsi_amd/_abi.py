@@ -28,6 +28,10 @@ PRECISION_FP64, PRECISION_MIXED = 0, 1
 UWCDI_NORMALS = {"standard": 0, "youngs": 1, "smoothed9": 2}
 CAPILLARITY_NONE, CAPILLARITY_DIFFUSE_AREA = 0, 1
 SOLVER_DIRECT, SOLVER_GMRES = 0, 1
+GROWTH_NONE, GROWTH_CONSTANT, GROWTH_SATURATING = 0, 1, 2
+GROWTH_TENSOR_ISOTROPIC, GROWTH_TENSOR_DIRECTIONAL = 0, 1
+DIFFUSION_NONE, DIFFUSION_CONSTANT, DIFFUSION_HARMONIC = 0, 1, 2
+CONSUMPTION_NONE, CONSUMPTION_SATURATING = 0, 1
 
 
 class BcSide(ctypes.Structure):
@@ -63,6 +67,10 @@ class Models(ctypes.Structure):
         ("uwcdi_normal_method", c_int32),
         ("capillarity_kind", c_int32), ("capillary_coefficient", c_double), ("capillary_delta_normal", c_double),
         ("capillary_smoothing_passes", c_int32),
+        ("growth_source_kind", c_int32), ("growth_rate", c_double), ("growth_half_saturation", c_double),
+        ("growth_tensor_kind", c_int32), ("growth_direction_x", c_double), ("growth_direction_y", c_double),
+        ("diffusivity_kind", c_int32), ("diffusivity_fluid", c_double), ("diffusivity_structure", c_double),
+        ("consumption_kind", c_int32), ("consumption_rate", c_double), ("consumption_half_saturation", c_double),
     ]
 
 
@@ -75,6 +83,7 @@ class Numerics(ctypes.Structure):
         ("stage1_atol", c_double), ("stage1_rtol", c_double), ("check_stage1", c_int32),
         ("precision", c_int32),
         ("velocity_method", c_int32), ("direct_fallback", c_int32), ("fallback_rtol", c_double),
+        ("resource_method", c_int32),
     ]
 
 
@@ -195,11 +204,33 @@ def abi_config(metrics, bcs, models, numerics):
         m.capillary_smoothing_passes = cap.smoothing_passes
     elif not isinstance(cap, _cl.NoCapillarity):
         raise _cl.UnsupportedModel(f"capillarity={type(cap).__name__} is outside the device path")
-    for name in ("growth_source", "growth_tensor", "diffusivity", "resource_consumption"):
-        obj = getattr(models, name)
-        if not isinstance(obj, (_cl.NoGrowth, _cl.IsotropicGrowthTensor, _cl.NoDiffusion,
-                                _cl.NoResourceConsumption)):
-            raise _cl.UnsupportedModel(f"{name}={type(obj).__name__} is outside the device path")
+    gs = models.growth_source
+    if isinstance(gs, _cl.SaturatingResourceGrowth):
+        m.growth_source_kind, m.growth_rate, m.growth_half_saturation = GROWTH_SATURATING, gs.rate, gs.half_saturation
+    elif isinstance(gs, _cl.ConstantIntrinsicGrowth):
+        m.growth_source_kind, m.growth_rate = GROWTH_CONSTANT, gs.rate
+    elif not isinstance(gs, _cl.NoGrowth):
+        raise _cl.UnsupportedModel(f"growth_source={type(gs).__name__} is outside the device path")
+    gt = models.growth_tensor
+    if isinstance(gt, _cl.DirectionalGrowthTensor):
+        m.growth_tensor_kind = GROWTH_TENSOR_DIRECTIONAL
+        m.growth_direction_x, m.growth_direction_y = gt.direction_x, gt.direction_y
+    elif not isinstance(gt, _cl.IsotropicGrowthTensor):
+        raise _cl.UnsupportedModel(f"growth_tensor={type(gt).__name__} is outside the device path")
+    d = models.diffusivity
+    if isinstance(d, _cl.HarmonicPhaseDiffusivity):
+        m.diffusivity_kind, m.diffusivity_fluid, m.diffusivity_structure = (
+            DIFFUSION_HARMONIC, d.fluid_value, d.structure_value)
+    elif isinstance(d, _cl.ConstantDiffusivity):
+        m.diffusivity_kind, m.diffusivity_fluid = DIFFUSION_CONSTANT, d.value
+    elif not isinstance(d, _cl.NoDiffusion):
+        raise _cl.UnsupportedModel(f"diffusivity={type(d).__name__} is outside the device path")
+    rc = models.resource_consumption
+    if isinstance(rc, _cl.SaturatingResourceConsumption):
+        m.consumption_kind, m.consumption_rate, m.consumption_half_saturation = (
+            CONSUMPTION_SATURATING, rc.rate, rc.half_saturation)
+    elif not isinstance(rc, _cl.NoResourceConsumption):
+        raise _cl.UnsupportedModel(f"resource_consumption={type(rc).__name__} is outside the device path")
 
     n = Numerics()
     ext = numerics.structure_extension
@@ -214,6 +245,7 @@ def abi_config(metrics, bcs, models, numerics):
     n.velocity_rtol, n.velocity_maxiter = ls.reltol, ls.maxiter
     n.velocity_method = SOLVER_GMRES if ls.velocity_method == "gmres" else SOLVER_DIRECT
     n.direct_fallback, n.fallback_rtol = int(ls.direct_fallback), ls.fallback_reltol
+    n.resource_method = SOLVER_GMRES if ls.resource_method == "gmres" else SOLVER_DIRECT
     n.projection_rtol, n.projection_maxiter = numerics.projection_reltol, numerics.projection_maxiter
     n.stage1_atol, n.stage1_rtol = numerics.stage1_constraint_atol, numerics.stage1_constraint_rtol
     n.check_stage1 = int(numerics.check_stage1_constraint)

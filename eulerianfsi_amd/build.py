@@ -12,7 +12,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 SOURCES = ["fsi_api.hip", "fsi_physics.hip", "fsi_solvers.hip", "fsi_mg.hip", "fsi_spectral.hip", "fsi_adjoint.hip",
-           "fsi_adjoint_api.hip"]
+           "fsi_adjoint_api.hip", "fsi_growth.hip"]
 LIB = PKG / "libfsi_hip.so"
 ARCH = os.environ.get("FSI_OFFLOAD_ARCH", "gfx950")
 

@@ -126,24 +126,102 @@ class DiffuseAreaCapillarity:
         self.coefficient, self.delta_normal, self.smoothing_passes = float(c), float(d), int(self.smoothing_passes)
 
 
-# ---- growth / resource (only the no-growth closures are on the path) -----------
+# ---- growth / resource (GrowthModels.jl, TransportConstitutive.jl:181-335) -----
 class NoGrowth:
-    pass
+    """G = 0 (GrowthModels.jl:214-223)."""
+
+
+@dataclass
+class ConstantIntrinsicGrowth:
+    """G = rate * phi (GrowthModels.jl:55-75, 225-235)."""
+    rate: float
+
+    def __post_init__(self):
+        self.rate = _finite(self.rate, "Intrinsic growth rate must be finite.")
+
+
+@dataclass
+class SaturatingResourceGrowth:
+    """G = rate * phi * c / (half_saturation + c) (GrowthModels.jl:77-104, 237-252)."""
+    rate: float
+    half_saturation: float
+
+    def __post_init__(self):
+        self.rate = _finite(self.rate, "Resource-limited growth rate must be finite.")
+        self.half_saturation = _finite(self.half_saturation, "Growth half-saturation constant must be finite.")
+        if not self.half_saturation > 0.0:
+            raise ArgumentError("Growth half-saturation constant must be strictly positive.")
 
 
 class IsotropicGrowthTensor:
-    pass
+    """K_g = G/2 I (GrowthModels.jl:114-130, 268-289)."""
+
+
+@dataclass
+class DirectionalGrowthTensor:
+    """K_g = G t (x) t with t normalised by the constructor (GrowthModels.jl:132-166)."""
+    direction_x: float
+    direction_y: float
+
+    def __post_init__(self):
+        tx, ty = float(self.direction_x), float(self.direction_y)
+        if not (math.isfinite(tx) and math.isfinite(ty)):
+            raise ArgumentError("Directional-growth components must be finite.")
+        mag = math.hypot(tx, ty)
+        if not mag > 0.0:
+            raise ArgumentError("Directional growth requires a non-zero direction vector.")
+        self.direction_x, self.direction_y = tx / mag, ty / mag
 
 
 class NoDiffusion:
-    pass
+    """No resource diffusion; no implicit resource solve (TransportConstitutive.jl:183-184)."""
+
+
+@dataclass
+class ConstantDiffusivity:
+    """TransportConstitutive.jl:186-196."""
+    value: float
+
+    def __post_init__(self):
+        self.value = _finite(self.value, "Diffusivity must be finite.")
+        if not self.value > 0.0:
+            raise ArgumentError("Diffusivity must be strictly positive.")
+
+
+@dataclass
+class HarmonicPhaseDiffusivity:
+    """D = 1 / ((1 - phi)/D_f + phi/D_s), phi unclamped (TransportConstitutive.jl:198-225, 255-263)."""
+    fluid_value: float
+    structure_value: float
+
+    def __post_init__(self):
+        self.fluid_value = _finite(self.fluid_value, "Fluid diffusivity must be finite.")
+        self.structure_value = _finite(self.structure_value, "Structure diffusivity must be finite.")
+        if not self.fluid_value > 0.0:
+            raise ArgumentError("Fluid diffusivity must be strictly positive.")
+        if not self.structure_value > 0.0:
+            raise ArgumentError("Structure diffusivity must be strictly positive.")
 
 
 class NoResourceConsumption:
-    pass
+    """C = 0 (TransportConstitutive.jl:267-268)."""
 
 
-# ---- phase-field methods ----------------------------------------------------------
+@dataclass
+class SaturatingResourceConsumption:
+    """C = phi * rate * c / (half_saturation + c) (TransportConstitutive.jl:270-295, 303-335)."""
+    rate: float
+    half_saturation: float
+
+    def __post_init__(self):
+        self.rate = _finite(self.rate, "Consumption rate must be finite.")
+        self.half_saturation = _finite(self.half_saturation, "Consumption half-saturation constant must be finite.")
+        if not self.rate >= 0.0:
+            raise ArgumentError("Consumption rate must be non-negative.")
+        if not self.half_saturation > 0.0:
+            raise ArgumentError("Consumption half-saturation constant must be strictly positive.")
+
+
 class NoInterfaceFlux:
     pass
 

@@ -438,6 +438,8 @@ fsi_status adjoint_supported(fsi_context* c) {
         return fail(c, FSI_E_UNSUPPORTED, "The device adjoint covers the ACDI phase field with a fixed gamma.");
     if (c->num.precision != FSI_PRECISION_FP64)
         return fail(c, FSI_E_UNSUPPORTED, "The device adjoint differentiates the FP64 step.");
+    if (c->growth)
+        return fail(c, FSI_E_UNSUPPORTED, "The device adjoint does not cover growth or resource closures.");
     if (c->models.capillarity_kind != FSI_CAPILLARITY_NONE)
         return fail(c, FSI_E_UNSUPPORTED, "The device adjoint does not cover capillary forcing.");
     const fsi_bc_setup* v[2] = {&c->bcs.U, &c->bcs.V};

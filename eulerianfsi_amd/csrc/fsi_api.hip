@@ -197,6 +197,55 @@ static fsi_status validate(fsi_context* c) {
             return fail(c, FSI_E_ARGUMENT, "Structure-extension cutoff must lie in [0, 1].");
     } else if (n.extension_kind != FSI_EXT_NONE)
         return fail(c, FSI_E_UNSUPPORTED, "extension model %d is not supported on the device.", n.extension_kind);
+    // growth and resource closures (GrowthModels.jl:55-166, TransportConstitutive.jl:186-295)
+    switch (m.growth_source_kind) {
+    case FSI_GROWTH_NONE: break;
+    case FSI_GROWTH_CONSTANT:
+        if (!std::isfinite(m.growth_rate)) return fail(c, FSI_E_ARGUMENT, "Intrinsic growth rate must be finite.");
+        break;
+    case FSI_GROWTH_SATURATING:
+        if (!std::isfinite(m.growth_rate)) return fail(c, FSI_E_ARGUMENT, "Resource-limited growth rate must be finite.");
+        if (!std::isfinite(m.growth_half_saturation))
+            return fail(c, FSI_E_ARGUMENT, "Growth half-saturation constant must be finite.");
+        if (!(m.growth_half_saturation > 0.0))
+            return fail(c, FSI_E_ARGUMENT, "Growth half-saturation constant must be strictly positive.");
+        break;
+    default: return fail(c, FSI_E_UNSUPPORTED, "growth source %d is not supported on the device.", m.growth_source_kind);
+    }
+    if (m.growth_tensor_kind == FSI_GROWTH_TENSOR_DIRECTIONAL) {
+        if (!(std::isfinite(m.growth_direction_x) && std::isfinite(m.growth_direction_y)))
+            return fail(c, FSI_E_ARGUMENT, "Directional-growth components must be finite.");
+        if (!(std::hypot(m.growth_direction_x, m.growth_direction_y) > 0.0))
+            return fail(c, FSI_E_ARGUMENT, "Directional growth requires a non-zero direction vector.");
+    } else if (m.growth_tensor_kind != FSI_GROWTH_TENSOR_ISOTROPIC)
+        return fail(c, FSI_E_UNSUPPORTED, "growth tensor %d is not supported on the device.", m.growth_tensor_kind);
+    switch (m.diffusivity_kind) {
+    case FSI_DIFFUSION_NONE: break;
+    case FSI_DIFFUSION_CONSTANT:
+        if (!std::isfinite(m.diffusivity_fluid)) return fail(c, FSI_E_ARGUMENT, "Diffusivity must be finite.");
+        if (!(m.diffusivity_fluid > 0.0)) return fail(c, FSI_E_ARGUMENT, "Diffusivity must be strictly positive.");
+        break;
+    case FSI_DIFFUSION_HARMONIC:
+        if (!std::isfinite(m.diffusivity_fluid)) return fail(c, FSI_E_ARGUMENT, "Fluid diffusivity must be finite.");
+        if (!std::isfinite(m.diffusivity_structure))
+            return fail(c, FSI_E_ARGUMENT, "Structure diffusivity must be finite.");
+        if (!(m.diffusivity_fluid > 0.0)) return fail(c, FSI_E_ARGUMENT, "Fluid diffusivity must be strictly positive.");
+        if (!(m.diffusivity_structure > 0.0))
+            return fail(c, FSI_E_ARGUMENT, "Structure diffusivity must be strictly positive.");
+        break;
+    default: return fail(c, FSI_E_UNSUPPORTED, "diffusivity %d is not supported on the device.", m.diffusivity_kind);
+    }
+    if (m.consumption_kind == FSI_CONSUMPTION_SATURATING) {
+        if (!std::isfinite(m.consumption_rate)) return fail(c, FSI_E_ARGUMENT, "Consumption rate must be finite.");
+        if (!std::isfinite(m.consumption_half_saturation))
+            return fail(c, FSI_E_ARGUMENT, "Consumption half-saturation constant must be finite.");
+        if (!(m.consumption_rate >= 0.0)) return fail(c, FSI_E_ARGUMENT, "Consumption rate must be non-negative.");
+        if (!(m.consumption_half_saturation > 0.0))
+            return fail(c, FSI_E_ARGUMENT, "Consumption half-saturation constant must be strictly positive.");
+    } else if (m.consumption_kind != FSI_CONSUMPTION_NONE)
+        return fail(c, FSI_E_UNSUPPORTED, "resource consumption %d is not supported on the device.", m.consumption_kind);
+    if (n.resource_method != FSI_SOLVER_DIRECT && n.resource_method != FSI_SOLVER_GMRES)
+        return fail(c, FSI_E_ARGUMENT, "resource_method must be :direct or :gmres.");
     if (!(n.projection_rtol > 0.0)) return fail(c, FSI_E_ARGUMENT, "projection_reltol must be positive.");
     if (n.projection_maxiter <= 0) return fail(c, FSI_E_ARGUMENT, "projection_maxiter must be positive.");
     if (!(n.velocity_rtol > 0.0)) return fail(c, FSI_E_ARGUMENT, "Linear-solver reltol must be positive.");
@@ -340,6 +389,36 @@ extern "C" fsi_status fsi_context_create(const fsi_grid* grid, const fsi_bcs* bc
     }
     p.exnb = grid_blocks(p);
     HIPCHECK(c, dalloc(c, &c->ext, (size_t)EX_COUNT * p.exnb));
+    // the growth slots stay zero (neutral) where the no-growth step never writes them
+    HIPCHECK(c, hipMemsetAsync(c->ext, 0, (size_t)EX_COUNT * p.exnb * sizeof(double), c->stream));
+    {
+        const fsi_models& m = c->models;
+        GrowthP& g = c->gp;
+        c->growth = m.growth_source_kind != FSI_GROWTH_NONE || m.diffusivity_kind != FSI_DIFFUSION_NONE ||
+                    m.consumption_kind != FSI_CONSUMPTION_NONE;
+        g.src = m.growth_source_kind;
+        g.rate = m.growth_rate;
+        g.K = m.growth_half_saturation;
+        g.kxx = 0.5;  // IsotropicGrowthTensor: G/2 I (GrowthModels.jl:268-289)
+        g.kxy = 0.0;
+        g.kyy = 0.5;
+        if (m.growth_tensor_kind == FSI_GROWTH_TENSOR_DIRECTIONAL) {  // G t (x) t, t normalised (:150-165, :291-318)
+            const double mag = std::hypot(m.growth_direction_x, m.growth_direction_y);
+            const double tx = m.growth_direction_x / mag, ty = m.growth_direction_y / mag;
+            g.kxx = tx * tx;
+            g.kxy = tx * ty;
+            g.kyy = ty * ty;
+        }
+        g.diff = m.diffusivity_kind;
+        g.Df = m.diffusivity_fluid;
+        g.Ds = m.diffusivity_structure;
+        g.cons = m.consumption_kind;
+        g.crate = m.consumption_rate;
+        g.cK = m.consumption_half_saturation;
+        g.visc = p.visc;
+        g.mu = p.mu;
+        g.bC = to_bcset(c->bcs.resource);
+    }
     {
         const int nv = std::max(DIAG_STATE_VALUES, std::max(DIAG_TENSOR_VALUES, DIAG_GEOMETRY_VALUES));
         HIPCHECK(c, dalloc(c, &c->diag, (size_t)nv * p.exnb));
@@ -805,6 +884,13 @@ extern "C" fsi_status fsi_workspace_create(fsi_context* c, fsi_workspace** out) 
             w->owned.push_back(reinterpret_cast<double*>(*q));
         }
     }
+    if (c->growth) {  // the growth step's buffers
+        double** gb[] = {&w->G, &w->Ec[0], &w->Ec[1], &w->Ec[2], &w->Ic[0], &w->Ic[1], &w->Ic[2], &w->Dx, &w->Dy,
+                         &w->crhs, &w->rc_r, &w->rc_p, &w->rc_q};
+        for (auto q : gb) HIPCHECK(c, A(q, F));
+        HIPCHECK(c, A(&w->gpart, (size_t)2 * c->prm.exnb));
+        HIPCHECK(c, dalloc(c, &w->cs, F));  // trades places with the state's resource (freed through its owner)
+    }
     // multigrid levels: level 0 solves into z with b = r; x and r are scratch
     w->lv.resize(c->geo.size());
     for (size_t l = 0; l < c->geo.size(); ++l) {
@@ -843,7 +929,7 @@ extern "C" void fsi_workspace_destroy(fsi_workspace* w) {
     if (w->ctx && w->ctx->stream) (void)hipStreamSynchronize(w->ctx->stream);
     for (auto& g : w->graphs) (void)hipGraphExecDestroy(g.exec);
     for (double* p : w->owned) (void)hipFree(p);
-    double* stage[] = {w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys};
+    double* stage[] = {w->Us, w->Vs, w->phis, w->sxxs, w->syys, w->sxys, w->cs};
     for (double* p : stage)
         if (p) (void)hipFree(p);
     delete w;
@@ -1104,6 +1190,7 @@ fsi_status projection(fsi_context* c, fsi_workspace* w, double* Uin, double* Vin
             launch_extrapolate(c->stream, p, b.x, w->corr[w->corr_cur], nullptr, 0);
     }
     b.q_out = w->corr[1 - w->corr_cur];
+    b.G = w->proj_target;
     b.r = w->r;
     b.z = w->z;
     b.p[0] = w->pp[0];
@@ -1210,6 +1297,14 @@ fsi_status device_error(fsi_context* c) {
     case DEV_NONHOMOG_NEUMANN:
         return fail(c, FSI_E_ARGUMENT,
                     "Only homogeneous Neumann phase boundaries are supported by the ACDI/UWCDI operators.");
+    case DEV_DIFFUSIVITY:  // _validate_positive_field, TransportConstitutive.jl:53-59
+        return fail(c, FSI_E_ARGUMENT,
+                    std::isfinite(s.v0) ? "evaluated diffusivity must be strictly positive everywhere."
+                                        : "evaluated diffusivity contains non-finite values.");
+    case DEV_RES_SINK:  // resource_sink! TransportConstitutive.jl:331-333
+        return fail(c, FSI_E_ARGUMENT, "Evaluated resource sink contains non-finite values.");
+    case DEV_RES_NOCONV:  // _solve_linear_system! :636 for the resource stage
+        return fail(c, FSI_E_SOLVER, "resource stage GMRES residual is too large: %.6e", s.v0);
     case DEV_VEL_NOCONV:  // _solve_linear_system! :636 (the device's direct target failing included)
         return fail(c, FSI_E_SOLVER, "velocity stage %d GMRES residual is too large: %.6e", (int)s.v1, s.v0);
     default: return fail(c, FSI_E_HIP, "device reported error code %d", s.code);
@@ -1226,12 +1321,19 @@ static void fill_projection_result(const fsi_context* c, fsi_projection_result* 
     r->iterations = s.pcg.iter;
 }
 
-static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, fsi_projection_result* res) {
+static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, fsi_projection_result* res,
+                                bool growth = false) {
     if (!c || !s || !w) return FSI_E_ARGUMENT;
     status_reset(c);
     fsi_status rc = status_push(c);
     if (rc != FSI_OK) return rc;
+    if (growth) {  // G of the state's phi and resource (growth_source! :1094)
+        GrSrc a{s->phi, s->res, s->U, s->V, s->sxx, s->syy, s->sxy, w->G, w->gpart, 1, 0};
+        launch_gr_source(c->stream, c->prm, c->gp, a, c->st);
+        w->proj_target = w->G;
+    }
     rc = projection(c, w, s->U, s->V, s->U, s->V, 0, -1, 0);
+    w->proj_target = nullptr;
     if (rc != FSI_OK) return rc;
     if ((rc = status_pull(c)) != FSI_OK) return rc;
     fill_projection_result(c, res);
@@ -1242,8 +1344,8 @@ static fsi_status project_state(fsi_context* c, fsi_state* s, fsi_workspace* w, 
 
 extern "C" fsi_status fsi_make_velocity_compatible(fsi_context* c, fsi_state* s, fsi_workspace* w,
                                                    fsi_projection_result* res) {
-    // growth_source!(NoGrowth) gives G = 0 (GrowthModels.jl:214-223)
-    return project_state(c, s, w, res);
+    // project onto div v = G(phi, c) (:1088-1109); G = 0 under NoGrowth
+    return project_state(c, s, w, res, c && w && c->growth && c->gp.src != FSI_GROWTH_NONE);
 }
 
 extern "C" fsi_status fsi_project(fsi_context* c, fsi_state* s, fsi_workspace* w, fsi_projection_result* res) {
@@ -1319,10 +1421,144 @@ extern "C" fsi_status fsi_uwcdi_strang_step(fsi_context* c, fsi_state* s, fsi_wo
     return device_error(c);
 }
 
+// The resource stage solve (_build_resource_stage! :742-774) into w->cs from
+// x0 = c_n, polled every few iterations.
+static fsi_status resource_solve(fsi_context* c, fsi_state* s, fsi_workspace* w, double alpha) {
+    const Params& p = c->prm;
+    ResCg a{};
+    a.Dx = w->Dx;
+    a.Dy = w->Dy;
+    a.b = w->crhs;
+    a.x0 = s->res;
+    a.x = w->cs;
+    a.r = w->rc_r;
+    a.p = w->rc_p;
+    a.q = w->rc_q;
+    a.part = w->gpart;
+    a.alpha = alpha;
+    const int maxiter = c->num.velocity_maxiter;
+    const int cap = maxiter + std::max(10 * maxiter, 1000);  // the direct target's budget, as the predictor's
+    a.rtol = c->num.velocity_rtol;
+    a.fb = c->num.fallback_rtol;
+    a.maxiter_d = cap;
+    a.method = c->num.resource_method;
+    a.fallback = c->num.direct_fallback;
+    a.maxiter = maxiter;
+    launch_rcg_start(c->stream, p, c->gp, a, c->st);
+    int m = 0, chunk = 8, done = 0, iter = 0;
+    fsi_status rc = poll_done(c, &c->st->res, &done, &iter);
+    while (rc == FSI_OK && !done && m < cap) {
+        const int n = std::min(chunk, cap - m);
+        launch_rcg_iters(c->stream, p, c->gp, a, c->st, m, n);
+        m += n;
+        rc = poll_done(c, &c->st->res, &done, &iter);
+        chunk = 16;
+    }
+    return rc;
+}
+
+// The step with growth, resource and diffusion closures
+// (TimeIntegrationIMEX.jl:1152-1256): per stage phi_s, sigma_s and the
+// resource right-hand side, the resource solve (diffusion), G_s and K_g,s,
+// the predictor and the projection onto div v = G_s, then the stored
+// tendencies of the no-growth kernels with the growth terms added.  Every
+// solve is polled (no planned steps); everything runs on the context's stream.
+static fsi_status enqueue_growth_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt) {
+    const Params& p = c->prm;
+    const GrowthP& g = c->gp;
+    const bool diff = g.diff != FSI_DIFFUSION_NONE;
+    hipStream_t S = c->stream;
+    fsi_status rc = FSI_OK;
+    double* const sphi[3] = {w->cphi[0], w->cphi[1], w->phis};
+    double* const ssxx[3] = {w->csxx[0], w->csxx[1], w->sxxs};
+    double* const ssyy[3] = {w->csyy[0], w->csyy[1], w->syys};
+    double* const ssxy[3] = {w->csxy[0], w->csxy[1], w->sxys};
+    auto stage_u = [&](int si) { return si == 2 ? w->Us2 : w->Us; };
+    auto stage_v = [&](int si) { return si == 2 ? w->Vs2 : w->Vs; };
+    // stage 1: G of the state, the stage-1 checks, the tendencies
+    GrSrc src{s->phi, s->res, s->U, s->V, s->sxx, s->syy, s->sxy, w->G, w->gpart, 1, 1};
+    launch_gr_source(S, p, g, src, c->st);
+    if (diff) launch_gr_diff(S, p, g, s->phi, w->Dx, w->Dy, c->st);
+    tendencies(c, w, 0, s->U, s->V, s->phi, s->sxx, s->syy, s->sxy, c->has_reset, 0);
+    GrTend t0{s->U,   s->V,      s->phi,    s->sxx,    s->syy,    s->sxy,    w->G,  s->res,
+              w->Dx,  w->Dy,     c->has_reset, w->Eu[0], w->Ev[0], w->Ep[0], w->Exx[0], w->Eyy[0],
+              w->Exy[0], w->Ec[0], diff ? w->Ic[0] : nullptr};
+    launch_gr_tend(S, p, g, t0, c->st);
+    for (int si = 1; si < 4; ++si) {
+        const int k = si - 1;
+        const double alpha = dt * AI[si][si];
+        GrStage a{};
+        a.phin = s->phi;
+        a.sxxn = s->sxx;
+        a.syyn = s->syy;
+        a.sxyn = s->sxy;
+        a.cn = s->res;
+        a.Un = s->U;
+        a.Vn = s->V;
+        for (int q = 0; q < si; ++q) {
+            a.Ep[q] = w->Ep[q];
+            a.Exx[q] = w->Exx[q];
+            a.Eyy[q] = w->Eyy[q];
+            a.Exy[q] = w->Exy[q];
+            a.Ec[q] = w->Ec[q];
+            a.Ic[q] = diff ? w->Ic[q] : nullptr;
+            a.Eu[q] = w->Eu[q];
+            a.Ev[q] = w->Ev[q];
+            a.Iu[q] = w->Iu[q];
+            a.Iv[q] = w->Iv[q];
+            a.ce[q] = AE[si][q];
+            a.ci[q] = AI[si][q];
+        }
+        a.dt = dt;
+        a.alpha = alpha;
+        a.bu = p.visc ? c->bvisc_u : nullptr;
+        a.bv = p.visc ? c->bvisc_v : nullptr;
+        a.phi = sphi[k];
+        a.sxx = ssxx[k];
+        a.syy = ssyy[k];
+        a.sxy = ssxy[k];
+        a.crhs = diff ? w->crhs : w->cs;  // without diffusion c_s is the right-hand side itself
+        a.ru = w->rhs_u[k];
+        a.rv = w->rhs_v[k];
+        const bool ref = p.visc && si == 1 && c->num.velocity_method == FSI_SOLVER_GMRES;
+        a.mu = ref ? w->mun_u : nullptr;
+        a.mv = ref ? w->mun_v : nullptr;
+        a.ns = si;
+        a.reset_n = c->has_reset;
+        a.reset = c->has_reset;
+        launch_gr_stage(S, p, a);
+        if (diff) {
+            launch_gr_diff(S, p, g, sphi[k], w->Dx, w->Dy, c->st);
+            if ((rc = resource_solve(c, s, w, alpha)) != FSI_OK) return rc;
+        }
+        GrSrc sk{sphi[k], w->cs, nullptr, nullptr, nullptr, nullptr, nullptr, w->G, w->gpart, 0, 0};
+        launch_gr_source(S, p, g, sk, c->st);
+        const bool keep = p.visc;
+        double* xu = keep ? w->vel_next_u[k] : stage_u(si);
+        double* xv = keep ? w->vel_next_v[k] : stage_v(si);
+        if ((rc = velocity_solve(c, s, w, alpha, k, 0, w->rhs_u[k], w->rhs_v[k], xu, xv)) != FSI_OK) return rc;
+        w->proj_target = w->G;
+        rc = projection(c, w, xu, xv, stage_u(si), stage_v(si), 1, k, 0);
+        w->proj_target = nullptr;
+        if (rc != FSI_OK) return rc;
+        if (si <= 2) {
+            tendencies(c, w, si, stage_u(si), stage_v(si), sphi[k], ssxx[k], ssyy[k], ssxy[k], 0, 0);
+            GrTend tk{stage_u(si), stage_v(si), sphi[k], ssxx[k], ssyy[k], ssxy[k], w->G, w->cs, w->Dx, w->Dy, 0,
+                      w->Eu[si], w->Ev[si], w->Ep[si], w->Exx[si], w->Eyy[si], w->Exy[si], w->Ec[si],
+                      diff ? w->Ic[si] : nullptr};
+            launch_gr_tend(S, p, g, tk, c->st);
+        }
+    }
+    launch_gr_report(S, p, w->phis, w->sxxs, w->syys, w->sxys, w->cs);
+    launch_ext_final(S, p, c->st);
+    return FSI_OK;
+}
+
 // The stages of one step (TimeIntegrationIMEX.jl:1152-1256) on the stream;
 // with `planned` each solve runs the planned number of iterations and the
 // launch sequence is fixed (it can be captured into a graph).
 static fsi_status enqueue_step(fsi_context* c, fsi_state* s, fsi_workspace* w, double dt, bool planned) {
+    if (c->growth) return enqueue_growth_step(c, s, w, dt);
     const Params& p = c->prm;
     const int reset = c->has_reset;
     fsi_status rc = FSI_OK;
@@ -1541,6 +1777,7 @@ struct StepRun {
 // The iteration plans of a planned step from the context's history; false
 // when the step has to poll (no history yet, or planning switched off).
 static bool plan_step(fsi_context* c) {
+    if (c->growth) return false;  // the growth step polls its solves
     if (!(c->use_graph && c->have_history)) return false;
     // planned step: every solve gets its planned iteration count (kernels
     // past convergence return at once), one read-back at the end
@@ -1646,7 +1883,7 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
         return fail(c, FSI_E_ARGUMENT, "the state contains non-finite values.");
     }
     if (c->num.check_stage1) {  // _check_stage_one_constraint! :1116-1127
-        double tol = c->num.stage1_atol + c->num.stage1_rtol * 1.0;
+        double tol = c->num.stage1_atol + c->num.stage1_rtol * std::fmax(st.stage1_gmax, 1.0);
         if (!(st.stage1_linf <= tol)) {
             return fail(c, FSI_E_ARGUMENT,
                         "The velocity entering stage 1 does not satisfy the prescribed divergence constraint. "
@@ -1662,6 +1899,10 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     std::swap(s->sxx, w->sxxs);
     std::swap(s->syy, w->syys);
     std::swap(s->sxy, w->sxys);
+    if (c->growth) {  // copyto!(state.resource, work.c_stage) :1235
+        std::swap(s->res, w->cs);
+        s->res_min = st.res_min;
+    }
     // commit the workspace: the step's correction, the stage histories
     w->corr_cur = 1 - w->corr_cur;
     {
@@ -1689,7 +1930,7 @@ static fsi_status step_finish(fsi_context* c, fsi_state* s, fsi_workspace* w, do
     }
     if (rep) {
         rep->maximum_projection_constraint_linf = std::fmax(st.stage1_linf, st.max_constraint_linf);
-        rep->maximum_growth_trace_error = 0.0;
+        rep->maximum_growth_trace_error = st.growth_trace;
         rep->minimum_phi = st.phi_min;
         rep->maximum_phi = st.phi_max;
         rep->minimum_resource = s->res_min;

@@ -84,7 +84,10 @@ enum ExtSlot : int {
     EX_STAGE1 = 6,      // max |div v - G| of the input velocity
     EX_NONFINITE_IN = 7,
     EX_SPEED = 8,       // max |u_c| (dynamic ACDI gamma)
-    EX_COUNT = 9
+    EX_GMAX = 9,        // max |G| at stage 1 (growth step)
+    EX_TRACE = 10,      // max |tr K_g - G| over the stages (growth step)
+    EX_RES_NEGMIN = 11, // -min of the new resource (growth step)
+    EX_COUNT = 12
 };
 
 // Error codes raised from kernels into Status::code (first error wins).
@@ -98,7 +101,10 @@ enum DevError : int {
     DEV_PROJ_RESIDUAL = 6,       // ErrorException: projection true residual
     DEV_PROJ_CONSTRAINT = 7,     // ErrorException: projection constraint
     DEV_NONHOMOG_NEUMANN = 8,    // ArgumentError: non-homogeneous Neumann phase BC
-    DEV_VEL_NOCONV = 9           // ErrorException: velocity predictor solve not converged
+    DEV_VEL_NOCONV = 9,          // ErrorException: velocity predictor solve not converged
+    DEV_DIFFUSIVITY = 10,        // ArgumentError: evaluated diffusivity not finite and positive
+    DEV_RES_SINK = 11,           // ArgumentError: evaluated resource sink not finite
+    DEV_RES_NOCONV = 12          // ErrorException: resource stage solve not converged
 };
 
 // Scalars of one Krylov solve, resident in device memory.
@@ -118,6 +124,7 @@ struct KrylovScalars {
     int ibase;     // inner iterations of the finished cycles
     int cycles;    // refinement cycles checked so far
     int imax;      // most inner iterations of one cycle
+    int cap;       // resource solve: the iteration limit of its current phase
 };
 
 // Stopping rules of the velocity predictor solve, passed to its kernels.
@@ -140,6 +147,7 @@ struct ProjScalars {
     double defect, scale, mean_rhs, rhs2, pdiv_l2sq, mean_q;
     double resid2, linf, l2sq;
     double lin_rel, l2;
+    double tgt_l2sq;  // sum of G^2 dA of the target divergence (growth step; scales the constraint check)
 };
 
 // A first-failure record (a DevError code and two values describing it).
@@ -171,6 +179,10 @@ struct Status {
     KrylovScalars vel;
     KrylovScalars pcg;
     ProjScalars proj;
+    KrylovScalars res;    // the resource stage solve (growth step)
+    double stage1_gmax;   // max |G| at stage 1 (the scale of the stage-1 check)
+    double growth_trace;  // maximum_growth_trace_error of the step
+    double res_min;       // minimum_resource of the new state (growth step)
     // The phase and sigma kernels of stage q report their failures here (they
     // may run beside the solve of stage q + 1); code_at[s] is `code` after the
     // projection of stage s + 1, so the host can order the two as the

@@ -103,6 +103,10 @@ struct fsi_context {
     int hist_proj[3][kHist] = {};
     int hist_n = 0;
     long long spec_steps = 0, spec_misses = 0;
+    // growth and resource path: any growth, diffusion or consumption closure
+    // active (TimeIntegrationIMEX.jl:1004-1009) selects the growth step
+    int growth = 0;
+    GrowthP gp{};
 };
 
 struct fsi_state {
@@ -166,6 +170,13 @@ struct fsi_workspace {
     // velocity (the first projected stage, overwritten by the last) here
     double* tape_U1 = nullptr;
     double* tape_V1 = nullptr;
+    // growth step: G_s, the resource tendencies E_c, I_c of stages 1..3, the
+    // face diffusivities of phi_s, the resource right-hand side, the resource
+    // stage value (swapped into the state with the other fields), the CG
+    // vectors of the resource solve and its partials
+    double *G = nullptr, *Ec[3] = {}, *Ic[3] = {}, *Dx = nullptr, *Dy = nullptr, *crhs = nullptr, *cs = nullptr;
+    double *rc_r = nullptr, *rc_p = nullptr, *rc_q = nullptr, *gpart = nullptr;
+    const double* proj_target = nullptr;  // the target divergence of the projections in flight (null: G = 0)
 };
 
 

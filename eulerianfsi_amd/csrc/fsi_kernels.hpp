@@ -196,6 +196,7 @@ struct ProjBufs {
     int rhs_np;
     double* q_part;
     int q_np;
+    const double* G;  // target divergence (cells), or null for G = 0
 };
 void launch_proj_rhs(hipStream_t s, const Params& p, const ProjBufs& b);
 // x = 2 cur - prev (extrap) or x = cur: the initial guess of a projection
@@ -292,6 +293,69 @@ size_t partials_size(const Params& p);
 double bench_launch(hipStream_t s, const char* name, const Params& p, const VelBufs& vb, const ProjBufs& pb,
                     const MgLevel* lv, const MgGeom* geo, int nlev, const CoarsePlan& cp, Status* st,
                     const SpecPlan* sp = nullptr);
+
+// ---- growth and resource path (fsi_growth.hip) ----
+// Closure parameters: G(phi, c), K_g = G (kxx, kxy; kxy, kyy), the diffusivity
+// and the consumption law, the resource boundary conditions.
+struct GrowthP {
+    int src;           // fsi_growth_source
+    double rate, K;
+    double kxx, kxy, kyy;
+    int diff;          // fsi_diffusivity_kind
+    double Df, Ds;
+    int cons;          // fsi_consumption_kind
+    double crate, cK;
+    int visc;          // the growth viscous stress -2 mu K_g is added
+    double mu;
+    BcSet bC;          // resource
+};
+// G of (phi, c) into G; the trace error (first: overwrite, else max with the
+// earlier stages); stage1: max |div v - G|, max |G| and the non-finite count
+// of the input state; the sum of G^2 dA into st->proj.tgt_l2sq
+struct GrSrc {
+    const double *phi, *c, *U, *V, *sxx, *syy, *sxy;
+    double* G;
+    double* part;
+    int first, stage1;
+};
+void launch_gr_source(hipStream_t s, const Params& p, const GrowthP& g, const GrSrc& a, Status* st);
+// stage values: phi_s, sigma_s (safety reset), the resource right-hand side and
+// the velocity right-hand side (+ alpha b_visc); mu, mv: M u^n = I_0 - b_visc
+struct GrStage {
+    const double *phin, *sxxn, *syyn, *sxyn, *cn, *Un, *Vn;
+    const double *Ep[3], *Exx[3], *Eyy[3], *Exy[3], *Ec[3], *Ic[3], *Eu[3], *Ev[3], *Iu[3], *Iv[3];
+    double ce[3], ci[3];
+    double dt, alpha;
+    const double *bu, *bv;
+    double *phi, *sxx, *syy, *sxy, *crhs, *ru, *rv, *mu, *mv;
+    int ns, reset_n, reset;
+};
+void launch_gr_stage(hipStream_t s, const Params& p, const GrStage& a);
+// face diffusivities of phi (and the diffusivity's validation)
+void launch_gr_diff(hipStream_t s, const Params& p, const GrowthP& g, const double* phi, double* Dx, double* Dy,
+                    Status* st);
+// the resource stage solve (I - alpha L) x = b + alpha b_L by CG (x0 = c_n)
+struct ResCg {
+    const double *Dx, *Dy, *b, *x0;
+    double *x, *r, *p, *q, *part;
+    double alpha;
+    double rtol, fb, maxiter_d;
+    int method, fallback, maxiter;
+};
+void launch_rcg_start(hipStream_t s, const Params& p, const GrowthP& g, const ResCg& a, Status* st);
+void launch_rcg_iters(hipStream_t s, const Params& p, const GrowthP& g, const ResCg& a, Status* st, int m0,
+                      int count);
+// growth terms added to the stored tendencies of a stage, and the resource
+// tendencies E_c, I_c
+struct GrTend {
+    const double *U, *V, *phi, *sxx, *syy, *sxy, *G, *c, *Dx, *Dy;
+    int reset;
+    double *Eu, *Ev, *Ep, *Exx, *Eyy, *Exy, *Ec, *Ic;
+};
+void launch_gr_tend(hipStream_t s, const Params& p, const GrowthP& g, const GrTend& a, Status* st);
+// the StepReport partials of the new state (EX_* slots)
+void launch_gr_report(hipStream_t s, const Params& p, const double* phi, const double* sxx, const double* syy,
+                      const double* sxy, const double* c);
 
 // ---- discrete adjoint (fsi_adjoint.hip) ----
 // The linearisation point: one stage's fields (sigma after the safety reset,

@@ -1464,8 +1464,11 @@ static dim3 grid2d(const Params& p) {
 
 // One block: maxima and sums of the per-block partials in a fixed order.
 FSI_KERNEL void k_ext_final(Params p, Status* st) {
-    constexpr int NM = 5, NS = 3;  // maxima: -phi_min, phi_max, -eig_min, det_err, stage1; sums: counts
-    const int mslot[NM] = {EX_PHI_NEGMIN, EX_PHI_MAX, EX_EIG_NEGMIN, EX_DET_ERR, EX_STAGE1};
+    // maxima: -phi_min, phi_max, -eig_min, det_err, stage1, and of the growth
+    // step max |G| at stage 1, the trace error, -min resource; sums: counts
+    constexpr int NM = 8, NS = 3;
+    const int mslot[NM] = {EX_PHI_NEGMIN, EX_PHI_MAX, EX_EIG_NEGMIN, EX_DET_ERR, EX_STAGE1, EX_GMAX, EX_TRACE,
+                           EX_RES_NEGMIN};
     const int sslot[NS] = {EX_NONSPD, EX_NONFINITE, EX_NONFINITE_IN};
     __shared__ double lm[NM][BLOCK / 64], ls[NS][BLOCK / 64];
     const int tid = threadIdx.x;
@@ -1518,6 +1521,9 @@ FSI_KERNEL void k_ext_final(Params p, Status* st) {
     st->min_eig = -lm[2][0];
     st->max_det_err = fmax(lm[3][0], 0.0);
     st->stage1_linf = fmax(lm[4][0], 0.0);
+    st->stage1_gmax = fmax(lm[5][0], 0.0);
+    st->growth_trace = fmax(lm[6][0], 0.0);
+    st->res_min = -lm[7][0];
     st->nonspd = (long long)ls[0][0];
     st->nonfinite = (long long)ls[1][0];
     st->nonfinite_in = (long long)ls[2][0];

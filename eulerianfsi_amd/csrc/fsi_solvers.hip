@@ -802,7 +802,7 @@ FSI_KERNEL void k_proj_rhs(Params p, ProjBufs b, int nb) {
     const int jb = tile_id().y * TILE_ROWS + threadIdx.y;
     bool in[ROWS];
     size_t kk[ROWS];
-    double uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS], lb[ROWS];
+    double uL[ROWS], uR[ROWS], vB[ROWS], vT[ROWS], lb[ROWS], gk[ROWS];
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
         const int j = jb + r * TILE_Y;
@@ -814,6 +814,7 @@ FSI_KERNEL void k_proj_rhs(Params p, ProjBufs b, int nb) {
         vB[r] = b.Vin[k];
         vT[r] = b.Vin[k + p.P];
         lb[r] = b.lap_bc[k];
+        gk[r] = b.G ? b.G[k] : 0.0;
     }
 #pragma unroll
     for (int r = 0; r < ROWS; ++r) {
@@ -830,7 +831,7 @@ FSI_KERNEL void k_proj_rhs(Params p, ProjBufs b, int nb) {
             b.Vin[k + p.P] = vT[r];
         }
         double div = (uR[r] - uL[r]) * p.ihx + (vT[r] - vB[r]) * p.ihy;
-        double rhs = (div - 0.0) - lb[r];
+        double rhs = (div - gk[r]) - lb[r];
         b.rhs[k] = rhs;
         acc[0] += rhs * area;
         acc[1] += fabs(rhs) * area;
@@ -1193,7 +1194,7 @@ FSI_KERNEL void k_proj_check(Params p, MgGeom g, ProjBufs b, Status* st, int nb,
             acc[0] += res * res;
         }
         double div = (uR[r] - uL[r]) * p.ihx + (vT[r] - vB[r]) * p.ihy;
-        double c = div - 0.0;
+        double c = div - (b.G ? b.G[k] : 0.0);
         linf = fmax(linf, fabs(c));
         if (!isfinite(c)) linf = INFINITY;
         acc[1] += c * c * area;
@@ -1326,7 +1327,7 @@ FSI_KERNEL void k_proj_upcheck(Params p, MgGeom g, ProjBufs b, Status* st, int n
         if (ve) b.V[k + p.P] = v1;
         b.q_out[k] = qc[r] - mean;
         double div = (u1 - u0) * p.ihx + (v1 - v0) * p.ihy;
-        double c = div - 0.0;
+        double c = div - (b.G ? b.G[k] : 0.0);
         linf = fmax(linf, fabs(c));
         if (!isfinite(c)) linf = INFINITY;
         acc[1] += c * c * area;
@@ -1444,7 +1445,7 @@ FSI_KERNEL void k_proj_final(ProjBufs b, Status* st, int nb, double lxly, int pc
     q.l2 = sqrt(acc[2] / lxly);
     st->max_constraint_linf = fmax(st->max_constraint_linf, q.linf);
     if (!(q.lin_rel <= 1e-8)) raise_error(st, DEV_PROJ_RESIDUAL, q.lin_rel, 1e-8);
-    double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), 0.0), 1.0);
+    double cs = fmax(fmax(sqrt(q.pdiv_l2sq / lxly), b.G ? sqrt(q.tgt_l2sq / lxly) : 0.0), 1.0);
     double ctol = 1e-12 + 1e-8 * cs;
     if (!(q.l2 <= ctol)) raise_error(st, DEV_PROJ_CONSTRAINT, q.l2, ctol);
     if (slot >= 0) st->code_at[slot] = st->code;  // the first failure up to this stage's projection

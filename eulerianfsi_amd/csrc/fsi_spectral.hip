@@ -317,7 +317,7 @@ __device__ __forceinline__ void proj_rhs_chunk(const Params& p, const ProjBufs& 
                 pb.Vin[k + p.P] = vt;
             }
             const double div = (ur - uL[h][c]) * p.ihx + (vt - vB[h][c]) * p.ihy;
-            const double rhs = (div - 0.0) - lb[h][c];
+            const double rhs = (div - (pb.G ? pb.G[k] : 0.0)) - lb[h][c];
             pb.rhs[k] = rhs;
             acc[0] += rhs * area;
             acc[1] += fabs(rhs) * area;

@@ -98,6 +98,26 @@ typedef enum {
 } fsi_uwcdi_normal;
 typedef enum { FSI_CAPILLARITY_NONE = 0, FSI_CAPILLARITY_DIFFUSE_AREA = 1 } fsi_capillarity_kind;
 
+/* growth and resource closures (src/GrowthModels.jl, src/TransportConstitutive.jl:181-335) */
+typedef enum {
+    FSI_GROWTH_NONE = 0,        /* NoGrowth: G = 0                                        GrowthModels.jl:214-223 */
+    FSI_GROWTH_CONSTANT = 1,    /* ConstantIntrinsicGrowth(rate): G = rate phi            :225-235 */
+    FSI_GROWTH_SATURATING = 2   /* SaturatingResourceGrowth(rate, K): G = rate phi c/(K+c) :237-252 */
+} fsi_growth_source;
+typedef enum {
+    FSI_GROWTH_TENSOR_ISOTROPIC = 0,   /* IsotropicGrowthTensor: K_g = G/2 I                  :268-289 */
+    FSI_GROWTH_TENSOR_DIRECTIONAL = 1  /* DirectionalGrowthTensor(tx, ty): K_g = G t (x) t     :291-318 */
+} fsi_growth_tensor;
+typedef enum {
+    FSI_DIFFUSION_NONE = 0,      /* NoDiffusion: the resource is not diffused (no implicit solve) */
+    FSI_DIFFUSION_CONSTANT = 1,  /* ConstantDiffusivity(value): diffusivity_fluid          */
+    FSI_DIFFUSION_HARMONIC = 2   /* HarmonicPhaseDiffusivity(D_f, D_s): 1/((1-phi)/D_f + phi/D_s) */
+} fsi_diffusivity_kind;
+typedef enum {
+    FSI_CONSUMPTION_NONE = 0,       /* NoResourceConsumption                                  */
+    FSI_CONSUMPTION_SATURATING = 1  /* SaturatingResourceConsumption(rate, K): phi rate c/(K+c) */
+} fsi_consumption_kind;
+
 typedef struct {
     int32_t phase_method;      /* fsi_phase_method */
     /* CDI: ConservativeDiffuseInterfaceFlux(inverse_peclet, epsilon; delta_normal) */
@@ -121,6 +141,20 @@ typedef struct {
     int32_t capillarity_kind;
     double capillary_coefficient, capillary_delta_normal;
     int32_t capillary_smoothing_passes;
+    /* growth and resource (FSIModels growth_source, growth_tensor, diffusivity,
+     * resource_consumption, TimeIntegrationIMEX.jl:156-208).  Any of them
+     * active selects the growth step: G_s and K_g,s from phi_s and c_s, the
+     * projection onto div v = G_s, the S v convection term, phi S, the K_g terms
+     * of the structure tensor, the growth viscous stress -2 mu K_g, and the
+     * resource stage (upwind advection, sink, implicit diffusion solve). */
+    int32_t growth_source_kind;     /* fsi_growth_source */
+    double growth_rate, growth_half_saturation;
+    int32_t growth_tensor_kind;     /* fsi_growth_tensor */
+    double growth_direction_x, growth_direction_y;  /* normalised by the library, as the constructor does */
+    int32_t diffusivity_kind;       /* fsi_diffusivity_kind */
+    double diffusivity_fluid, diffusivity_structure;  /* ConstantDiffusivity: diffusivity_fluid */
+    int32_t consumption_kind;       /* fsi_consumption_kind */
+    double consumption_rate, consumption_half_saturation;
 } fsi_models;
 
 /* ---- numerics (IMEXNumerics, src/TimeIntegrationIMEX.jl:262-301) -------- */
@@ -165,6 +199,9 @@ typedef struct {
     int32_t velocity_method;   /* fsi_solver_method */
     int32_t direct_fallback;
     double fallback_rtol;
+    /* LinearSolverOptions.resource_method (the resource stage solve, :742-774):
+     * fsi_solver_method with the same reltol, maxiter and fallback rules */
+    int32_t resource_method;
 } fsi_numerics;
 
 /* ---- results ------------------------------------------------------------ */

@@ -91,6 +91,8 @@ struct fsio_ctx {
     mg_level lev[32];
     /* Kolmogorov body force per row (TransportConstitutive.jl:385-397) */
     double* force_row;
+    /* growth tensor per unit G (GrowthModels.jl:268-318): K_g = G (kxx, kxy; kxy, kyy) */
+    double kxx, kxy, kyy;
 };
 
 /* ------------------------------------------------------------------------- */
@@ -1310,8 +1312,8 @@ int fsio_structure_extension_rhs(fsio_ctx* c, const double* sxx, const double* s
 
 /* structure_tensor_rhs StructureTensor.jl:377-463 (K_g = 0 under NoGrowth). */
 static int structure_rhs(fsio_ctx* c, const double* sxx, const double* syy, const double* sxy,
-                         const double* U, const double* V, const double* phi, double* oxx,
-                         double* oyy, double* oxy) {
+                         const double* U, const double* V, const double* phi, const double* G,
+                         double* oxx, double* oyy, double* oxy) {
     size_t nc = (size_t)c->N;
     double *axx = dalloc(nc), *ayy = dalloc(nc), *axy = dalloc(nc), *dv = dalloc(nc);
     double *ux = dalloc(nc), *uy = dalloc(nc), *vx = dalloc(nc), *vy = dalloc(nc);
@@ -1326,7 +1328,9 @@ static int structure_rhs(fsio_ctx* c, const double* sxx, const double* syy, cons
         double rate = c->m.relaxation_kind == FSI_RELAX_MAXWELL ? c->m.relaxation_rate : 0.0;
         OMP_FOR
         for (size_t k = 0; k < nc; ++k) {
-            double A11 = ux[k] - 0.0, A12 = uy[k] - 0.0, A21 = vx[k] - 0.0, A22 = vy[k] - 0.0;
+            /* A = grad v - K_g (StructureTensor.jl:431-434); K_g = 0 without growth */
+            double Kxx = G ? c->kxx * G[k] : 0.0, Kxy = G ? c->kxy * G[k] : 0.0, Kyy = G ? c->kyy * G[k] : 0.0;
+            double A11 = ux[k] - Kxx, A12 = uy[k] - Kxy, A21 = vx[k] - Kxy, A22 = vy[k] - Kyy;
             double adxx = axx[k] + sxx[k] * dv[k];
             double adyy = ayy[k] + syy[k] * dv[k];
             double adxy = axy[k] + sxy[k] * dv[k];
@@ -1352,7 +1356,7 @@ static int structure_rhs(fsio_ctx* c, const double* sxx, const double* syy, cons
 int fsio_structure_rhs(fsio_ctx* c, const double* sxx, const double* syy, const double* sxy,
                        const double* U, const double* V, const double* phi, double* oxx,
                        double* oyy, double* oxy) {
-    return structure_rhs(c, sxx, syy, sxy, U, V, phi, oxx, oyy, oxy);
+    return structure_rhs(c, sxx, syy, sxy, U, V, phi, NULL, oxx, oyy, oxy);
 }
 
 /* apply_structure_extension! StructureTensor.jl:553-613 (safety reset). */
@@ -1366,6 +1370,192 @@ static void apply_extension_reset(fsio_ctx* c, double* sxx, double* syy, double*
     } else return;
     for (int k = 0; k < c->N; ++k)
         if (phi[k] < cutoff) { sxx[k] = 1.0; syy[k] = 1.0; sxy[k] = 0.0; }
+}
+
+/* ---------------------------------------------------------------------------
+ * Growth and resource closures.
+ * ------------------------------------------------------------------------- */
+/* the resource is transported only when a growth, diffusion or consumption
+   closure is active (TimeIntegrationIMEX.jl:1004-1009) */
+static int growth_active(const fsio_ctx* c) {
+    return c->m.growth_source_kind != FSI_GROWTH_NONE || c->m.diffusivity_kind != FSI_DIFFUSION_NONE ||
+           c->m.consumption_kind != FSI_CONSUMPTION_NONE;
+}
+
+/* growth_source! GrowthModels.jl:214-256 and growth_diagnostics :336-352
+   (the pointwise trace error of K_g, growth_tensor! :268-318); returns
+   max |K_xx + K_yy - G| */
+static double growth_source(fsio_ctx* c, const double* phi, const double* res, double* G) {
+    const double rate = c->m.growth_rate, K = c->m.growth_half_saturation;
+    double err = 0.0;
+    for (int k = 0; k < c->N; ++k) {
+        double g = 0.0;
+        if (c->m.growth_source_kind == FSI_GROWTH_CONSTANT) g = rate * phi[k];
+        else if (c->m.growth_source_kind == FSI_GROWTH_SATURATING) g = rate * phi[k] * res[k] / (K + res[k]);
+        G[k] = g;
+        double e = fabs(c->kxx * g + c->kyy * g - g);
+        if (e > err || isnan(e)) err = e;
+    }
+    return err;
+}
+double fsio_growth_source(fsio_ctx* c, const double* phi, const double* res, double* G) {
+    return FSIO_VALUE(growth_source(c, phi, res, G));
+}
+
+/* diffusivity_field! TransportConstitutive.jl:241-263 (the harmonic law takes
+   phi unclamped) with its validation */
+static int diffusivity_field(fsio_ctx* c, const double* phi, double* D) {
+    const double Df = c->m.diffusivity_fluid, Ds = c->m.diffusivity_structure;
+    for (int k = 0; k < c->N; ++k) {
+        if (!isfinite(phi[k])) return set_err(c, FSI_E_ARGUMENT, "phi contains non-finite values.");
+        D[k] = c->m.diffusivity_kind == FSI_DIFFUSION_HARMONIC ? 1.0 / ((1.0 - phi[k]) / Df + phi[k] / Ds) : Df;
+    }
+    for (int k = 0; k < c->N; ++k)
+        if (!isfinite(D[k])) return set_err(c, FSI_E_ARGUMENT, "evaluated diffusivity contains non-finite values.");
+    for (int k = 0; k < c->N; ++k)
+        if (!(D[k] > 0.0))
+            return set_err(c, FSI_E_ARGUMENT, "evaluated diffusivity must be strictly positive everywhere.");
+    return 0;
+}
+
+/* build_diffusion_operator Operators.jl:361-384 applied matrix-free:
+   L q + b = D_x (D_xf (G_x q + g_x)) + D_y (D_yf (G_y q + g_y)), the face
+   diffusivities interpolated from the cells (:110-159), the gradient with the
+   resource BC rows (:189-356; their data only `with_boundary`) */
+static void diffusion_apply(fsio_ctx* c, const double* Dc, const double* q, double* out, int with_boundary) {
+    int nx = c->nx, ny = c->ny;
+    size_t nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
+    double *gx = dalloc(nfx), *gy = dalloc(nfy), *dx = dalloc(nfx), *dy = dalloc(nfy);
+    fsi_bc_setup bc = c->bcs.resource;
+    if (!with_boundary) { bc.left.value = bc.right.value = bc.bottom.value = bc.top.value = 0.0; }
+    scalar_gradient(c, &bc, q, gx, gy);
+    c2fx(c, Dc, dx);
+    c2fy(c, Dc, dy);
+    for (size_t k = 0; k < nfx; ++k) gx[k] *= dx[k];
+    for (size_t k = 0; k < nfy; ++k) gy[k] *= dy[k];
+    divergence(c, gx, gy, out);
+    free(gx); free(gy); free(dx); free(dy);
+}
+int fsio_diffusion_apply(fsio_ctx* c, const double* phi, const double* q, double* out, int with_boundary) {
+    double* D = dalloc(c->N);
+    int rc = diffusivity_field(c, phi, D);
+    if (!rc) diffusion_apply(c, D, q, out, with_boundary);
+    free(D);
+    return rc;
+}
+
+/* resource_sink! TransportConstitutive.jl:303-335 and resource_explicit_rhs
+   ResourceTransport.jl:292-342: -div(upwind c v) - C(phi, c), with the
+   advective boundary states of :84-159 (the rule upwind_advection follows) */
+static int resource_explicit_rhs(fsio_ctx* c, const double* U, const double* V, const double* phi,
+                                 const double* res, double* out) {
+    for (int k = 0; k < c->N; ++k) {
+        if (!isfinite(phi[k])) return set_err(c, FSI_E_ARGUMENT, "phi contains non-finite values.");
+    }
+    for (int k = 0; k < c->N; ++k) {
+        if (!isfinite(res[k])) return set_err(c, FSI_E_ARGUMENT, "resource contains non-finite values.");
+    }
+    double* sink = dalloc(c->N);
+    if (c->m.consumption_kind == FSI_CONSUMPTION_SATURATING) {
+        const double rate = c->m.consumption_rate, K = c->m.consumption_half_saturation;
+        for (int k = 0; k < c->N; ++k) sink[k] = rate * phi[k] * res[k] / (K + res[k]);
+        for (int k = 0; k < c->N; ++k)
+            if (!isfinite(sink[k])) {
+                free(sink);
+                return set_err(c, FSI_E_ARGUMENT, "Evaluated resource sink contains non-finite values.");
+            }
+    }
+    int rc = upwind_advection(c, res, U, V, &c->bcs.resource, out);
+    if (!rc)
+        for (int k = 0; k < c->N; ++k) out[k] = out[k] - sink[k];
+    free(sink);
+    return rc;
+}
+int fsio_resource_explicit_rhs(fsio_ctx* c, const double* U, const double* V, const double* phi,
+                               const double* res, double* out) {
+    return resource_explicit_rhs(c, U, V, phi, res, out);
+}
+
+static double dot(const double* a, const double* b, size_t n);
+/* The resource stage solve, _build_resource_stage! :742-774: (I - alpha L) c =
+ * rhs + alpha b, L the diffusion operator of phi_s.  L is symmetric negative
+ * semidefinite on uniform grids (the face terms are symmetric, a Dirichlet
+ * face adds to the diagonal only), so the system is SPD and solved by CG with
+ * the stopping rules of _solve_linear_system! :602-641 (x0 = c_n, method
+ * resource_method): :direct to |r| <= 1e-14 max(|b|, eps); :gmres to
+ * |r| <= reltol |b - A x0| within maxiter, then accepted below
+ * fallback_reltol, else the direct target (direct_fallback) or the
+ * ErrorException of :636. */
+static int resource_solve(fsio_ctx* c, double alpha, const double* Dc, const double* rhs_in, const double* x0,
+                          double* x, int* iters) {
+    int N = c->N;
+    double *b = dalloc(N), *r = dalloc(N), *p = dalloc(N), *q = dalloc(N), *tmp = dalloc(N), *zero = dalloc(N);
+    diffusion_apply(c, Dc, zero, tmp, 1); /* the boundary vector */
+    for (int k = 0; k < N; ++k) b[k] = rhs_in[k] + alpha * tmp[k];
+    diffusion_apply(c, Dc, x0, tmp, 0);
+    double bn2 = 0.0, ref2 = 0.0;
+    for (int k = 0; k < N; ++k) {
+        double e = b[k] - (x0[k] - alpha * tmp[k]);
+        bn2 += b[k] * b[k];
+        ref2 += e * e;
+    }
+    const double bscale = sqrt(bn2) > 2.220446049250313e-16 ? sqrt(bn2) : 2.220446049250313e-16;
+    const double fb = c->num.fallback_rtol * bscale;
+    double dtol = 1e-14 * bscale;
+    if (dtol > fb) dtol = fb;
+    const int maxiter = c->num.velocity_maxiter;
+    const int cap = maxiter + (10 * maxiter > 1000 ? 10 * maxiter : 1000);
+    const int direct = c->num.resource_method == FSI_SOLVER_DIRECT;
+    double tol = direct ? dtol : c->num.velocity_rtol * sqrt(ref2);
+    int limit = direct ? cap : maxiter, phase = direct;
+    memcpy(x, x0, sizeof(double) * N);
+    for (int k = 0; k < N; ++k) r[k] = b[k] - (x[k] - alpha * tmp[k]);
+    memcpy(p, r, sizeof(double) * N);
+    double rr = dot(r, r, N);
+    int it = 0, rc = 0;
+    for (;;) {
+        if (!isfinite(rr)) {
+            rc = set_err(c, FSI_E_SOLVER, "resource stage residual is not finite");
+            break;
+        }
+        if (sqrt(rr) <= tol || it >= limit) {
+            if (phase) {
+                if (sqrt(rr) <= tol) break;
+                rc = set_err(c, FSI_E_SOLVER, "resource GMRES residual is too large: %.6e", FSIO_VALUE(sqrt(rr) / bscale));
+                break;
+            }
+            if (sqrt(rr) <= fb) break;
+            if (c->num.direct_fallback) {
+                phase = 1;
+                tol = dtol;
+                limit = cap;
+                continue;
+            }
+            rc = set_err(c, FSI_E_SOLVER, "resource GMRES residual is too large: %.6e", FSIO_VALUE(sqrt(rr) / bscale));
+            break;
+        }
+        diffusion_apply(c, Dc, p, tmp, 0);
+        for (int k = 0; k < N; ++k) q[k] = p[k] - alpha * tmp[k];
+        double pq = dot(p, q, N);
+        double a = rr / pq;
+        for (int k = 0; k < N; ++k) { x[k] += a * p[k]; r[k] -= a * q[k]; }
+        double rr2 = dot(r, r, N);
+        double beta = rr2 / rr;
+        rr = rr2;
+        for (int k = 0; k < N; ++k) p[k] = r[k] + beta * p[k];
+        ++it;
+    }
+    if (iters) *iters = it;
+    free(b); free(r); free(p); free(q); free(tmp); free(zero);
+    return rc;
+}
+int fsio_resource_solve(fsio_ctx* c, fsio_scalar alpha, const double* phi, const double* rhs, const double* x0,
+                        double* x, int* iters) {
+    double* D = dalloc(c->N);
+    int rc = diffusivity_field(c, phi, D);
+    if (!rc) rc = resource_solve(c, alpha, D, rhs, x0, x, iters);
+    free(D);
+    return rc;
 }
 
 /* ---------------------------------------------------------------------------
@@ -1987,6 +2177,17 @@ int fsio_create(const fsi_grid* g, const fsi_bcs* bcs, const fsi_models* models,
     laplacian_apply(c, zero, c->lap_bc, 1);
     free(zero);
     build_mg(c);
+    /* growth tensor per unit G: G/2 I, or G t (x) t with t normalised as the
+       DirectionalGrowthTensor constructor does (GrowthModels.jl:150-165) */
+    c->kxx = 0.5; c->kxy = 0.0; c->kyy = 0.5;
+    if (models->growth_tensor_kind == FSI_GROWTH_TENSOR_DIRECTIONAL) {
+        double tx = models->growth_direction_x, ty = models->growth_direction_y, mag = hypot(tx, ty);
+        if (!(isfinite(tx) && isfinite(ty))) return set_err(c, FSI_E_ARGUMENT, "Directional-growth components must be finite.");
+        if (!(mag > 0.0)) return set_err(c, FSI_E_ARGUMENT, "Directional growth requires a non-zero direction vector.");
+        tx = tx / mag;
+        ty = ty / mag;
+        c->kxx = tx * tx; c->kxy = tx * ty; c->kyy = ty * ty;
+    }
     c->force_row = dalloc(c->ny);
     if (models->body_force_kind == FSI_FORCE_KOLMOGOROV) { /* TransportConstitutive.jl:385-397 */
         double y = 0.0;
@@ -2017,7 +2218,8 @@ void fsio_destroy(fsio_ctx* c) {
 static int tendencies(fsio_ctx* c, const double* U, const double* V, const double* phi,
                       const double* sxx, const double* syy, const double* sxy, const double* G,
                       double t_stage, double* E_u, double* E_v, double* I_u, double* I_v,
-                      double* E_phi, double* E_sxx, double* E_syy, double* E_sxy) {
+                      double* E_phi, double* E_sxx, double* E_syy, double* E_sxy,
+                      const double* res, const double* Dc, double* E_c, double* I_c) {
     int nx = c->nx, ny = c->ny, N = c->N;
     size_t nfx = (size_t)(nx + 1) * ny, nfy = (size_t)nx * (ny + 1);
     (void)t_stage; /* the supported body forces are steady */
@@ -2054,10 +2256,33 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
         for (size_t k = 0; k < nfy; ++k) E_v[k] += Fy[k];
         free(Txx); free(Tyy); free(Txy);
     }
-    /* _accumulate_growth_viscous_force! :926-942 adds the divergence of
-       -2 mu K_g = 0 exactly under NoGrowth. */
+    /* _accumulate_growth_viscous_force! :926-942: the divergence of the
+       stress -2 mu K_g (exactly zero under NoGrowth, so skipped there) */
+    if (c->m.viscosity_kind == FSI_VISC_CONSTANT && growth_active(c)) {
+        double *Txx = dalloc(N), *Tyy = dalloc(N), *Txy = dalloc(N), mu = c->m.viscosity;
+        for (int k = 0; k < N; ++k) {
+            double Kxx = c->kxx * G[k], Kxy = c->kxy * G[k], Kyy = c->kyy * G[k];
+            Txx[k] = -2.0 * mu * Kxx;
+            Tyy[k] = -2.0 * mu * Kyy;
+            Txy[k] = -mu * (Kxy + Kxy);
+        }
+        stress_divergence(c, Txx, Tyy, Txy, Fx, Fy);
+        for (size_t k = 0; k < nfx; ++k) E_u[k] += Fx[k];
+        for (size_t k = 0; k < nfy; ++k) E_v[k] += Fy[k];
+        free(Txx); free(Tyy); free(Txy);
+    }
     int rc = phase_rhs(c, U, V, phi, G, E_phi);
-    if (!rc) rc = structure_rhs(c, sxx, syy, sxy, U, V, phi, E_sxx, E_syy, E_sxy);
+    if (!rc) rc = structure_rhs(c, sxx, syy, sxy, U, V, phi, G, E_sxx, E_syy, E_sxy);
+    /* resource transport and consumption :1002-1026; the implicit history
+       L c + b of the diffusion operator :1064-1069 */
+    if (!rc && E_c) {
+        if (growth_active(c)) rc = resource_explicit_rhs(c, U, V, phi, res, E_c);
+        else memset(E_c, 0, sizeof(double) * N);
+    }
+    if (!rc && I_c) {
+        if (c->m.diffusivity_kind != FSI_DIFFUSION_NONE) diffusion_apply(c, Dc, res, I_c, 1);
+        else memset(I_c, 0, sizeof(double) * N);
+    }
     if (!rc && (I_u || I_v)) {
         double *uv = dalloc(2 * (size_t)N), *iu = dalloc(2 * (size_t)N);
         for (int j = 0; j < ny; ++j)
@@ -2091,7 +2316,8 @@ int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const do
     memcpy(d, syy, sizeof(double) * c->N);
     memcpy(b, sxy, sizeof(double) * c->N);
     apply_extension_reset(c, a, d, b, phi);
-    int rc = tendencies(c, U, V, phi, a, d, b, G, t, E_u, E_v, I_u, I_v, E_phi, E_sxx, E_syy, E_sxy);
+    int rc = tendencies(c, U, V, phi, a, d, b, G, t, E_u, E_v, I_u, I_v, E_phi, E_sxx, E_syy, E_sxy, NULL, NULL,
+                        NULL, NULL);
     free(G);
     free(a);
     free(d);
@@ -2143,11 +2369,17 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
     double *uv_n = dalloc(2 * (size_t)N), *phi_n = dalloc(N), *sxx_n = dalloc(N), *syy_n = dalloc(N), *sxy_n = dalloc(N);
     double *Us = dalloc(nfx), *Vs = dalloc(nfy), *phis = dalloc(N), *sxxs = dalloc(N), *syys = dalloc(N), *sxys = dalloc(N);
     double *rhs = dalloc(2 * (size_t)N), *pred = dalloc(2 * (size_t)N), *G = dalloc(N), *dv = dalloc(N);
-    double *Eu[3], *Ev[3], *Iu[3], *Iv[3], *Ep[3], *Exx[3], *Eyy[3], *Exy[3];
+    double *Eu[3], *Ev[3], *Iu[3], *Iv[3], *Ep[3], *Exx[3], *Eyy[3], *Exy[3], *Ec[3], *Ic[3];
     for (int s = 0; s < 3; ++s) {
         Eu[s] = dalloc(nfx); Ev[s] = dalloc(nfy); Iu[s] = dalloc(nfx); Iv[s] = dalloc(nfy);
         Ep[s] = dalloc(N); Exx[s] = dalloc(N); Eyy[s] = dalloc(N); Exy[s] = dalloc(N);
+        Ec[s] = dalloc(N); Ic[s] = dalloc(N);
     }
+    /* the resource of the stage (c_n, c_stage, rhs_c) and the diffusivity of phi_s */
+    double *c_n = dalloc(N), *cs = dalloc(N), *rhs_c = dalloc(N), *Dc = dalloc(N);
+    const int growth = growth_active(c), diffusion = c->m.diffusivity_kind != FSI_DIFFUSION_NONE;
+    double max_trace = 0.0;
+    memcpy(c_n, resource, sizeof(double) * N);
     /* _copy_time_n_state! :686-702 */
     for (int j = 0; j < ny; ++j)
         for (int i = 0; i < nx; ++i) { uv_n[i + j * nx] = UX(U, i, j); uv_n[N + i + j * nx] = VY(V, i, j); }
@@ -2166,6 +2398,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
             memcpy(sxxs, sxx_n, sizeof(double) * N);
             memcpy(syys, syy_n, sizeof(double) * N);
             memcpy(sxys, sxy_n, sizeof(double) * N);
+            memcpy(cs, c_n, sizeof(double) * N);
         } else { /* _build_explicit_stage! :716-740 */
             memcpy(phis, phi_n, sizeof(double) * N);
             memcpy(sxxs, sxx_n, sizeof(double) * N);
@@ -2184,7 +2417,36 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
             }
             apply_extension_reset(c, sxxs, syys, sxys, phis);
         }
-        /* growth (NoGrowth): G = 0, K_g = 0, trace error 0 (:842-865) */
+        /* _build_diffusion_operator! :867-880 (phi validated even without
+           diffusion), then the resource stage :742-774 for s > 1 */
+        if (growth || diffusion) {
+            if (diffusion) {
+                if ((rc = diffusivity_field(c, phis, Dc))) break;
+            } else {
+                for (int k = 0; k < N; ++k)
+                    if (!isfinite(phis[k])) { rc = set_err(c, FSI_E_ARGUMENT, "phi contains non-finite values."); break; }
+                if (rc) break;
+            }
+        }
+        if (s > 0 && growth) {
+            memcpy(rhs_c, c_n, sizeof(double) * N);
+            for (int q = 0; q < s; ++q) {
+                double ae = AE[s][q], ai = AI[s][q];
+                if (ae == 0.0 && ai == 0.0) continue;
+                for (int k = 0; k < N; ++k) rhs_c[k] += dt * (ae * Ec[q][k] + ai * Ic[q][k]);
+            }
+            if (diffusion) {
+                int it = 0;
+                if ((rc = resource_solve(c, FSIO_VALUE(dt * AI[s][s]), Dc, rhs_c, c_n, cs, &it))) break;
+            } else {
+                memcpy(cs, rhs_c, sizeof(double) * N);
+            }
+        }
+        /* _evaluate_growth! :842-865: G_s, K_g,s and the trace error */
+        if (growth) {
+            double te = growth_source(c, phis, cs, G);
+            if (te > max_trace || isnan(te)) max_trace = te;
+        }
         if (s == 0) { /* stage-one constraint :1194-1201, :1111-1127 */
             divergence(c, Us, Vs, dv);
             double linf = 0.0, gmax = 0.0;
@@ -2247,7 +2509,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
         }
         if (s <= 2) {
             rc = tendencies(c, Us, Vs, phis, sxxs, syys, sxys, G, t + CABS[s] * dt, Eu[s], Ev[s],
-                            Iu[s], Iv[s], Ep[s], Exx[s], Eyy[s], Exy[s]);
+                            Iu[s], Iv[s], Ep[s], Exx[s], Eyy[s], Exy[s], cs, Dc, Ec[s], Ic[s]);
         }
     }
     if (!rc) {
@@ -2257,6 +2519,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
         memcpy(sxx, sxxs, sizeof(double) * N);
         memcpy(syy, syys, sizeof(double) * N);
         memcpy(sxy, sxys, sizeof(double) * N);
+        memcpy(resource, cs, sizeof(double) * N);
         if (rep) { /* structure_tensor_diagnostics StructureTensor.jl:627-662 */
             double mineig = INFINITY, maxdet = 0.0, pmin = INFINITY, pmax = -INFINITY, rmin = INFINITY;
             long long nonspd = 0;
@@ -2272,7 +2535,7 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
                 if (resource[k] < rmin) rmin = resource[k];
             }
             rep->maximum_projection_constraint_linf = FSIO_VALUE(max_linf);
-            rep->maximum_growth_trace_error = 0.0;
+            rep->maximum_growth_trace_error = FSIO_VALUE(max_trace);
             rep->minimum_phi = FSIO_VALUE(pmin);
             rep->maximum_phi = FSIO_VALUE(pmax);
             rep->minimum_resource = FSIO_VALUE(rmin);
@@ -2284,8 +2547,10 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
     free(uv_n); free(phi_n); free(sxx_n); free(syy_n); free(sxy_n);
     free(Us); free(Vs); free(phis); free(sxxs); free(syys); free(sxys);
     free(rhs); free(pred); free(G); free(dv);
+    free(c_n); free(cs); free(rhs_c); free(Dc);
     for (int s = 0; s < 3; ++s) {
         free(Eu[s]); free(Ev[s]); free(Iu[s]); free(Iv[s]); free(Ep[s]); free(Exx[s]); free(Eyy[s]); free(Exy[s]);
+        free(Ec[s]); free(Ic[s]);
     }
     return rc;
 }
@@ -2294,9 +2559,8 @@ int fsio_integrate_step(fsio_ctx* c, double* U, double* V, double* phi, double* 
 int fsio_make_velocity_compatible(fsio_ctx* c, double* U, double* V, const double* phi,
                                   const double* resource, double* correction,
                                   fsi_projection_result* res) {
-    (void)phi;
-    (void)resource;
     double* G = dalloc(c->N);
+    growth_source(c, phi, resource, G); /* growth_source! :1094 */
     int rc = project(c, U, V, G, correction, res);
     free(G);
     return rc;

@@ -69,6 +69,10 @@ def load(path: Path | None = None):
         "fsio_capillary_force": (None, [P, _dp, _dp, _dp]),
         "fsio_uwcdi_center_normals": (c_int, [P, _dp, _dp, _dp]),
         "fsio_smooth9": (None, [P, _dp, _dp]),
+        "fsio_growth_source": (c_double, [P, _dp, _dp, _dp]),
+        "fsio_resource_explicit_rhs": (c_int, [P] + [_dp] * 5),
+        "fsio_diffusion_apply": (c_int, [P, _dp, _dp, _dp, c_int]),
+        "fsio_resource_solve": (c_int, [P, c_double, _dp, _dp, _dp, _dp, POINTER(c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -263,6 +267,34 @@ class Oracle:
         x = np.zeros_like(rhs)
         it = c_int(0)
         self._check(self.lib.fsio_velocity_solve(self.h, float(alpha), _p(rhs), _p(x0), _p(x), ctypes.byref(it)))
+        return x, it.value
+
+
+    # ---- growth and resource closures ----
+    def growth_source(self, phi, res):
+        """(G, maximum trace error of K_g) of growth_source! / growth_tensor! / growth_diagnostics."""
+        G = np.zeros((self.nx, self.ny), order="F")
+        err = self.lib.fsio_growth_source(self.h, _p(_F(phi)), _p(_F(res)), _p(G))
+        return G, err
+
+    def resource_explicit_rhs(self, U, V, phi, res):
+        out = np.zeros((self.nx, self.ny), order="F")
+        self._check(self.lib.fsio_resource_explicit_rhs(self.h, _p(_F(U)), _p(_F(V)), _p(_F(phi)), _p(_F(res)),
+                                                        _p(out)))
+        return out
+
+    def diffusion_apply(self, phi, q, with_boundary=True):
+        out = np.zeros((self.nx, self.ny), order="F")
+        self._check(self.lib.fsio_diffusion_apply(self.h, _p(_F(phi)), _p(_F(q)), _p(out), int(with_boundary)))
+        return out
+
+    def resource_solve(self, alpha, phi, rhs, x0=None):
+        rhs = _F(rhs)
+        x0 = np.zeros_like(rhs) if x0 is None else _F(x0)
+        x = np.zeros((self.nx, self.ny), order="F")
+        it = c_int(0)
+        self._check(self.lib.fsio_resource_solve(self.h, float(alpha), _p(_F(phi)), _p(rhs), _p(x0), _p(x),
+                                                 ctypes.byref(it)))
         return x, it.value
 
 

@@ -177,3 +177,69 @@ def wall_channel_flow_x(N, gamma=0.6, w0=0.25):
     cfg = (grid, metrics, bcs, fsi.FSIModels(), fsi.IMEXNumerics())
     return (cfg, np.asfortranarray(V.T), np.asfortranarray(U.T), np.asfortranarray(Cv.T),
             np.asfortranarray(Cu.T))
+
+
+# ---- growth and resource workloads ---------------------------------------------------------
+def growth_smoke(N=8, models=None, numerics=None):
+    """The coupled growth configuration of test/growth_tests.jl:96-141: x-periodic,
+    an impermeable top (V = 0), an open bottom (q = 0 there), resource held at 1
+    on both walls, SaturatingResourceGrowth(0.02, 0.5), IsotropicGrowthTensor,
+    ConstantDiffusivity(0.01), SaturatingResourceConsumption(0.002, 0.5)."""
+    grid = fsi.generate_grid(1.0, 1.0, N, N)
+    metrics = fsi.build_mesh_metrics(grid, periodic_x=True)
+    px, n0 = fsi.Periodic(), fsi.Neumann(0.0)
+    bc_n = fsi.BoundaryConditionSetup(px, px, n0, n0)
+    bc_v = fsi.BoundaryConditionSetup(px, px, n0, fsi.Dirichlet(0.0))
+    bc_q = fsi.BoundaryConditionSetup(px, px, fsi.Dirichlet(0.0), n0)
+    bc_c = fsi.BoundaryConditionSetup(px, px, fsi.Dirichlet(1.0), fsi.Dirichlet(1.0))
+    bcs = fsi.FSIBoundaryConditions(bc_n, bc_v, bc_q, bc_n, bc_c, {"xx": bc_n, "yy": bc_n, "xy": bc_n})
+    models = models or fsi.FSIModels(growth_source=C.SaturatingResourceGrowth(0.02, 0.5),
+                                     growth_tensor=C.IsotropicGrowthTensor(),
+                                     diffusivity=C.ConstantDiffusivity(0.01),
+                                     resource_consumption=C.SaturatingResourceConsumption(0.002, 0.5))
+    return grid, metrics, bcs, models, numerics or fsi.IMEXNumerics()
+
+
+def growth_smoke_state(N=8):
+    return fsi.FSIState(np.zeros((N + 1, N)), np.zeros((N, N + 1)), np.full((N, N), 0.5), np.ones((N, N)),
+                        np.ones((N, N)), np.ones((N, N)), np.zeros((N, N)))
+
+
+def uniaxial_growth(N=16, g0=0.3, tensor=None, models_kw=None):
+    """Uniform growth G = g0 (ConstantIntrinsicGrowth(g0), phi = 1) in an
+    x-periodic strip whose top face carries the flux out, V = g0 at y = 1:
+    the solution of div v = g0 is v = (0, g0 y) exactly on the MAC grid (its
+    convection is a discrete gradient, which the projection removes), so the
+    stage velocity never changes and sigma obeys the linear ODE
+    sigma' = A sigma + sigma A^T with A = grad v - K_g constant."""
+    grid = fsi.generate_grid(1.0, 1.0, N, N)
+    metrics = fsi.build_mesh_metrics(grid, periodic_x=True)
+    px, n0 = fsi.Periodic(), fsi.Neumann(0.0)
+    bc_n = fsi.BoundaryConditionSetup(px, px, n0, n0)
+    bc_v = fsi.BoundaryConditionSetup(px, px, fsi.Dirichlet(0.0), fsi.Dirichlet(g0))
+    bcs = fsi.FSIBoundaryConditions(bc_n, bc_v, bc_n, bc_n, bc_n, {"xx": bc_n, "yy": bc_n, "xy": bc_n})
+    models = fsi.FSIModels(growth_source=C.ConstantIntrinsicGrowth(g0),
+                           growth_tensor=tensor or C.IsotropicGrowthTensor(), **(models_kw or {}))
+    numerics = fsi.IMEXNumerics(projection_reltol=1e-13)
+    yv = np.arange(N + 1) / N
+    state = fsi.FSIState(np.zeros((N + 1, N)), np.tile(g0 * yv, (N, 1)), np.ones((N, N)), np.ones((N, N)),
+                         np.ones((N, N)), np.ones((N, N)), np.zeros((N, N)))
+    return (grid, metrics, bcs, models, numerics), state
+
+
+def resource_box(N=16, periodic=True, models=None, bc_c=None, numerics=None):
+    """A box at rest (v = 0) carrying a resource field: periodic, or walled
+    with the resource condition bc_c."""
+    grid = fsi.generate_grid(1.0, 1.0, N, N)
+    if periodic:
+        metrics = fsi.build_mesh_metrics(grid, periodic_x=True, periodic_y=True)
+        p = fsi.Periodic()
+        bc = fsi.BoundaryConditionSetup(p, p, p, p)
+        bcs = fsi.FSIBoundaryConditions(bc, bc, bc, bc, bc_c or bc, {"xx": bc, "yy": bc, "xy": bc})
+    else:
+        metrics = fsi.build_mesh_metrics(grid)
+        w, n0 = fsi.Dirichlet(0.0), fsi.Neumann(0.0)
+        wall = fsi.BoundaryConditionSetup(w, w, w, w)
+        nf = fsi.BoundaryConditionSetup(n0, n0, n0, n0)
+        bcs = fsi.FSIBoundaryConditions(wall, wall, nf, nf, bc_c or nf, {"xx": nf, "yy": nf, "xy": nf})
+    return grid, metrics, bcs, models or fsi.FSIModels(), numerics or fsi.IMEXNumerics()

@@ -1,0 +1,269 @@
+"""Pins of the oracle's growth and resource path (oracle/fsi_oracle.c), on the CPU.
+
+The reference's own known answers (test/growth_tests.jl) are re-expressed;
+the rest is fixed by closed forms chosen so that a plausible transcription
+mistake (a wrong coefficient, sign, face average, boundary row or tableau
+entry) fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import eulerianfsi_amd as fsi
+from eulerianfsi_amd import closures as C
+from oracle.oracle import OracleError, state_dict
+from tests import cases
+
+AE = [[0, 0, 0], [8 / 15, 0, 0], [1 / 4, 5 / 12, 0], [1 / 4, 0, 3 / 4]]
+
+
+def _oracle(cfg):
+    return cases.oracle_for(cfg)
+
+
+# ---------------------------------------------------------------------------
+# test/growth_tests.jl known answers
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("source,tensor,expect,trace", [
+    (C.NoGrowth(), C.IsotropicGrowthTensor(), lambda p: 0.0 * p, 0.0),                      # :10-11
+    (C.ConstantIntrinsicGrowth(0.2), C.IsotropicGrowthTensor(), lambda p: 0.2 * p, 0.0),   # :13-14, :20-23
+    (C.SaturatingResourceGrowth(0.6, 0.25), C.IsotropicGrowthTensor(), lambda p: 0.3 * p, 0.0),  # :15-16
+    (C.SaturatingResourceGrowth(0.6, 0.25), C.DirectionalGrowthTensor(3.0, 4.0), lambda p: 0.3 * p, 1e-14),  # :25-28
+])
+def test_growth_closures_known_answers(source, tensor, expect, trace):
+    o = _oracle(cases.periodic_box(4, models=fsi.FSIModels(growth_source=source, growth_tensor=tensor)))
+    phi = np.linspace(0.0, 1.0, 16).reshape(4, 4, order="F")
+    G, err = o.growth_source(phi, np.full((4, 4), 0.25))
+    assert np.allclose(G, expect(phi), rtol=1e-15, atol=0)
+    if trace == 0.0:
+        assert err == 0.0          # iszero(maximum_trace_error), :23
+    else:
+        assert err < trace         # :26
+
+
+def test_growth_closure_constructors_reject_invalid_values():  # :17, :29, :51
+    with pytest.raises(fsi.ArgumentError):
+        C.SaturatingResourceGrowth(0.6, 0.0)
+    with pytest.raises(fsi.ArgumentError):
+        C.DirectionalGrowthTensor(0.0, 0.0)
+    with pytest.raises(fsi.ArgumentError):
+        C.ConstantDiffusivity(0.0)
+    t = C.DirectionalGrowthTensor(3.0, 4.0)
+    assert (t.direction_x, t.direction_y) == (0.6, 0.8)
+
+
+def test_periodic_resource_advection_integral_vanishes():  # :32-43
+    models = fsi.FSIModels(resource_consumption=C.SaturatingResourceConsumption(0.0, 0.5))
+    o = _oracle(cases.periodic_box(12, models=models))
+    rng = np.random.default_rng(3)
+    U = rng.standard_normal((13, 12))
+    U[-1] = U[0]
+    V = rng.standard_normal((12, 13))
+    V[:, -1] = V[:, 0]
+    c = rng.random((12, 12))
+    rhs = o.resource_explicit_rhs(U, V, np.full((12, 12), 0.5), c)
+    assert abs(rhs.sum() / 144.0) < 1e-12
+
+
+def test_harmonic_diffusivity_and_sink_values():  # :47-55
+    models = fsi.FSIModels(diffusivity=C.HarmonicPhaseDiffusivity(0.1, 0.01),
+                           resource_consumption=C.SaturatingResourceConsumption(0.4, 0.5))
+    o = _oracle(cases.resource_box(4, models=models))
+    phi = np.linspace(0.0, 1.0, 16).reshape(4, 4, order="F")
+    # L applied to the linear field c = x (Neumann data -1 / +1 make every
+    # face gradient 1): (D_{i+1/2} - D_{i-1/2}) / h, faces averaging the cells
+    # and taking the cell value on a wall
+    n = fsi.Neumann
+    bc_c = fsi.BoundaryConditionSetup(n(-1.0), n(1.0), n(0.0), n(0.0))
+    o = _oracle(cases.resource_box(4, periodic=False, models=models, bc_c=bc_c))
+    D = 1.0 / ((1.0 - phi) / 0.1 + phi / 0.01)
+    x = (np.arange(4) + 0.5) / 4
+    Lc = o.diffusion_apply(phi, np.tile(x[:, None], (1, 4)), with_boundary=True)
+    Dface = np.vstack([D[:1], 0.5 * (D[:-1] + D[1:]), D[-1:]])
+    assert np.allclose(Lc, (Dface[1:] - Dface[:-1]) * 4.0, rtol=0, atol=1e-12)
+    # the sink, with v = 0: rhs = -rate phi c / (K + c), nonnegative sink (:54-55)
+    c = np.full((4, 4), 0.3)
+    rhs = o.resource_explicit_rhs(np.zeros((5, 4)), np.zeros((4, 5)), phi, c)
+    assert np.allclose(rhs, -0.4 * phi * 0.3 / 0.8, rtol=1e-15, atol=0)
+
+
+def test_legacy_constructors_zero_the_resource():  # :59-70
+    st = fsi.FSIState(np.zeros((9, 8)), np.zeros((8, 9)), np.full((8, 8), 0.5), np.ones((8, 8)), np.ones((8, 8)),
+                      np.zeros((8, 8)))
+    assert np.all(state_dict(st)["resource"] == 0.0)
+    m = fsi.FSIModels()
+    assert isinstance(m.growth_source, C.NoGrowth) and isinstance(m.diffusivity, C.NoDiffusion)
+
+
+def test_coupled_growth_step_invariants():  # :96-141
+    cfg = cases.growth_smoke(8)
+    o = _oracle(cfg)
+    s = state_dict(cases.growth_smoke_state(8))
+    res = o.make_velocity_compatible(s)
+    assert res.constraint_linf < 1e-7 and np.max(np.abs(s["U"])) + np.max(np.abs(s["V"])) > 0.0
+    o.correction[:] = 0.0
+    rep, _ = o.integrate_step(s, 0.0, 1e-4)
+    assert rep.maximum_projection_constraint_linf < 1e-6
+    assert np.max(np.abs(s["U"][-1] - s["U"][0])) < 1e-12
+    assert np.max(np.abs(s["V"][:, -1])) < 1e-12
+    assert rep.maximum_growth_trace_error < 1e-12
+    assert rep.minimum_resource > 0.0
+    assert np.all(np.isfinite(s["phi"])) and np.all(np.isfinite(s["sigma_xx"]))
+
+
+# ---------------------------------------------------------------------------
+# closed forms
+# ---------------------------------------------------------------------------
+def test_upwind_resource_advection_closed_form():
+    """Uniform u0 > 0, periodic: rhs_i = -u0 (c_i - c_{i-1}) / h exactly."""
+    N, u0 = 16, 0.7
+    models = fsi.FSIModels(diffusivity=C.ConstantDiffusivity(0.1))
+    o = _oracle(cases.periodic_box(N, models=models))
+    x = (np.arange(N) + 0.5) / N
+    c = np.tile(np.cos(2 * np.pi * x)[:, None], (1, N))
+    rhs = o.resource_explicit_rhs(np.full((N + 1, N), u0), np.zeros((N, N + 1)), np.ones((N, N)), c)
+    assert np.allclose(rhs, -u0 * (c - np.roll(c, 1, axis=0)) * N, rtol=0, atol=1e-13)
+
+
+def test_dirichlet_inflow_state_and_neumann_inflow_error():
+    N, u0, cb = 8, 0.5, 2.0
+    models = fsi.FSIModels(diffusivity=C.ConstantDiffusivity(0.1))
+    d, n0 = fsi.Dirichlet, fsi.Neumann(0.0)
+    bc_c = fsi.BoundaryConditionSetup(d(cb), n0, n0, n0)
+    o = _oracle(cases.resource_box(N, periodic=False, models=models, bc_c=bc_c))
+    c = np.ones((N, N))
+    U = np.full((N + 1, N), u0)
+    rhs = o.resource_explicit_rhs(U, np.zeros((N, N + 1)), np.ones((N, N)), c)
+    assert np.allclose(rhs[0], -u0 * (1.0 - cb) * N, rtol=1e-15)  # inflow carries the wall value
+    assert np.allclose(rhs[1:], 0.0, atol=1e-14)                 # outflow right: interior state
+    o2 = _oracle(cases.resource_box(N, periodic=False, models=models))  # Neumann on every side
+    with pytest.raises(OracleError) as e:
+        o2.resource_explicit_rhs(U, np.zeros((N, N + 1)), np.ones((N, N)), c)
+    assert e.value.status == 1
+
+
+def test_material_convection_source_term():
+    """A uniform periodic flow has zero conservative convection, so
+    C_u = -S_x u0 with S_x the face average of S (MomentumTransport.jl:313-319)."""
+    N, u0 = 8, 0.6
+    o = _oracle(cases.periodic_box(N))
+    S = np.random.default_rng(1).random((N, N))
+    Cu, Cv = o.material_convection(np.full((N + 1, N), u0), np.zeros((N, N + 1)), S)
+    Sx = 0.5 * (np.roll(S, 1, axis=0) + S)
+    assert np.allclose(Cu[:N], -Sx * u0, rtol=0, atol=1e-15)
+    assert np.allclose(Cu[N], Cu[0]) and np.all(np.abs(Cv) <= 1e-15)
+
+
+def _dense(o, phi, N, with_boundary=False):
+    cols = []
+    for k in range(N * N):
+        e = np.zeros(N * N)
+        e[k] = 1.0
+        cols.append(o.diffusion_apply(phi, e.reshape(N, N, order="F"), with_boundary=with_boundary).ravel(order="F"))
+    return np.array(cols).T
+
+
+def test_diffusion_operator_symmetric_negative_definite_and_solve():
+    """Harmonic diffusivity with Dirichlet and Neumann resource sides: L is
+    symmetric negative definite (a Dirichlet face adds to the diagonal only),
+    and the resource solve equals the dense solve of (I - alpha L) x = rhs + alpha b."""
+    N = 6
+    d, n = fsi.Dirichlet, fsi.Neumann
+    bc_c = fsi.BoundaryConditionSetup(d(1.5), n(0.2), d(-0.5), n(0.0))
+    models = fsi.FSIModels(diffusivity=C.HarmonicPhaseDiffusivity(0.1, 0.01))
+    o = _oracle(cases.resource_box(N, periodic=False, models=models, bc_c=bc_c,
+                                   numerics=fsi.IMEXNumerics(linear_solvers=fsi.LinearSolverOptions(
+                                       resource_method="direct"))))
+    phi = np.random.default_rng(2).random((N, N))
+    L = _dense(o, phi, N)
+    assert np.allclose(L, L.T, rtol=0, atol=1e-12 * np.abs(L).max())
+    assert np.max(np.linalg.eigvalsh(0.5 * (L + L.T))) < 0.0
+    b = o.diffusion_apply(phi, np.zeros((N, N)), with_boundary=True).ravel(order="F")
+    alpha = 0.05
+    rhs = np.random.default_rng(5).random((N, N))
+    x, it = o.resource_solve(alpha, phi, rhs)
+    exact = np.linalg.solve(np.eye(N * N) - alpha * L, rhs.ravel(order="F") + alpha * b)
+    assert it > 0 and np.allclose(x.ravel(order="F"), exact, rtol=0, atol=1e-13)
+
+
+@pytest.mark.parametrize("z", [-0.8, -3.0])
+def test_resource_diffusion_mode_decays_by_the_implicit_stability_function(z):
+    """v = 0, constant D, periodic: a cosine mode is multiplied by the implicit
+    tableau's stability function (three Crank-Nicolson substeps) at
+    z = dt D lambda_h; fixes A_I, the shift alpha_s = dt a_ss and the
+    implicit history I_c of the resource stage (TimeIntegrationIMEX.jl:742-774)."""
+    N, D = 16, 0.05
+    lam = -4.0 * math.sin(math.pi / N) ** 2 * N * N
+    dt = z / (D * lam)
+    models = fsi.FSIModels(diffusivity=C.ConstantDiffusivity(D))
+    num = fsi.IMEXNumerics(linear_solvers=fsi.LinearSolverOptions(resource_method="direct"))
+    o = _oracle(cases.periodic_box(N, models=models, numerics=num))
+    x = (np.arange(N) + 0.5) / N
+    c0 = np.tile((0.3 * np.cos(2 * np.pi * x))[:, None], (1, N)) + 1.0
+    st = fsi.FSIState(np.zeros((N + 1, N)), np.zeros((N, N + 1)), np.ones((N, N)), c0, np.ones((N, N)),
+                      np.ones((N, N)), np.zeros((N, N)))
+    s = state_dict(st)
+    o.integrate_step(s, 0.0, dt)
+    R = cases.cn_composition_factor(z)
+    assert abs(R - 1.0) > 0.05
+    assert np.allclose(s["resource"], 1.0 + R * (c0 - 1.0), rtol=0, atol=1e-13)
+
+
+def test_uniform_resource_consumed_by_the_explicit_rk3():
+    """v = 0, uniform c, no diffusion: one step is the explicit tableau (Wray
+    RK3) on c' = -rate phi c / (K + c); fixes A_E and the sink's sign."""
+    N, rate, K, phi0, c0, dt = 8, 0.9, 0.4, 0.7, 1.3, 0.2
+    models = fsi.FSIModels(resource_consumption=C.SaturatingResourceConsumption(rate, K))
+    o = _oracle(cases.periodic_box(N, models=models))
+    st = fsi.FSIState(np.zeros((N + 1, N)), np.zeros((N, N + 1)), np.full((N, N), phi0), np.full((N, N), c0),
+                      np.ones((N, N)), np.ones((N, N)), np.zeros((N, N)))
+    s = state_dict(st)
+    o.integrate_step(s, 0.0, dt)
+    f = lambda c: -rate * phi0 * c / (K + c)
+    k = []
+    for i in range(3):
+        ci = c0 + dt * sum(AE[i][j] * k[j] for j in range(i))
+        k.append(f(ci))
+    c1 = c0 + dt * sum(AE[3][j] * k[j] for j in range(3))
+    assert np.allclose(s["resource"], c1, rtol=1e-15, atol=0)
+    assert abs(c1 - c0) > 0.05
+
+
+@pytest.mark.parametrize("tensor,rates", [
+    (C.IsotropicGrowthTensor(), (-1.0, 1.0)),             # A = diag(-g/2, g/2): sigma_xx' = -g, sigma_yy' = +g
+    (C.DirectionalGrowthTensor(0.0, 1.0), (0.0, 0.0)),     # K_g = G e_y e_y matches the flow: sigma stays I
+    (C.DirectionalGrowthTensor(1.0, 0.0), (-2.0, 2.0)),    # A = diag(-g, g)
+])
+def test_uniaxial_growth_structure_tensor_closed_form(tensor, rates):
+    """Uniform growth with the exact expansion flow v = (0, g y): sigma stays
+    diagonal and each component follows the RK3 polynomial of its linear ODE
+    sigma_kk' = r_k g sigma_kk; fixes the K_g terms of the structure tensor
+    (StructureTensor.jl:431-437) and phi S of the phase transport, which keeps
+    phi = 1."""
+    g0, dt = 0.3, 0.1
+    cfg, st = cases.uniaxial_growth(16, g0, tensor)
+    o = _oracle(cfg)
+    s = state_dict(st)
+    V0 = s["V"].copy()
+    rep, _ = o.integrate_step(s, 0.0, dt)
+    rk3 = lambda z: 1.0 + z + z * z / 2.0 + z ** 3 / 6.0
+    assert np.allclose(s["V"], V0, rtol=0, atol=1e-13)
+    assert np.max(np.abs(s["U"])) <= 1e-13
+    assert np.allclose(s["phi"], 1.0, rtol=0, atol=1e-13)
+    assert np.allclose(s["sigma_xx"], rk3(rates[0] * g0 * dt), rtol=0, atol=1e-13)
+    assert np.allclose(s["sigma_yy"], rk3(rates[1] * g0 * dt), rtol=0, atol=1e-13)
+    assert np.max(np.abs(s["sigma_xy"])) <= 1e-13
+    assert rep.maximum_projection_constraint_linf < 1e-10
+
+
+def test_growth_viscous_stress_is_a_gradient_of_mu_g():
+    """Isotropic growth stress -2 mu K_g = -mu G I: its divergence is the
+    face gradient of -mu G (the shear part vanishes), the term of
+    _accumulate_growth_viscous_force! (TimeIntegrationIMEX.jl:926-942)."""
+    N, mu = 8, 0.05
+    o = _oracle(cases.periodic_box(N))
+    G = np.random.default_rng(7).random((N, N))
+    Fx, Fy = o.stress_divergence(-mu * G, -mu * G, 0.0 * G)
+    assert np.allclose(Fx[:N], -mu * (G - np.roll(G, 1, axis=0)) * N, atol=1e-14)
+    assert np.allclose(Fy[:, :N], -mu * (G - np.roll(G, 1, axis=1)) * N, atol=1e-14)
