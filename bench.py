@@ -243,6 +243,41 @@ def adjoint_leg(fsi, setups, N, nsteps, snapshots, fd_check=True):
     return res
 
 
+def growth_leg(fsi, setups, N, nsteps, warmup):
+    """The growth and resource step (growing disk, setups.growing_disk) under
+    the reference's stopping rules: velocity and resource GMRES rule 1e-8,
+    projection 1e-10.  Every solve is polled (no planned steps)."""
+    import torch
+    numerics = fsi.IMEXNumerics(projection_reltol=1e-10, linear_solvers=fsi.LinearSolverOptions(
+        velocity_method="gmres", resource_method="gmres", reltol=1e-8))
+    grid, metrics, bcs, models, numerics, state = setups.growing_disk(N, numerics=numerics)
+    ctx = fsi.build_imex_context(metrics, bcs, models, numerics)
+    work = fsi.allocate_imex_workspace(ctx)
+    dt = 1e-3 * 128 / N
+    fsi.make_velocity_compatible(state, ctx, work)
+    t = 0.0
+    for _ in range(warmup):
+        fsi.integrate_step(state, t, dt, ctx, work)
+        t += dt
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rep = None
+    for _ in range(nsteps):
+        rep = fsi.integrate_step(state, t, dt, ctx, work)
+        t += dt
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    return {"workload": f"elastic ACDI disk growing on a diffused, consumed resource, {N}x{N}, FP64, dt = {dt:.3g}, "
+                        f"steps {warmup}..{warmup + nsteps} from rest; SaturatingResourceGrowth(0.5, 0.5), "
+                        "DirectionalGrowthTensor(1, 2), HarmonicPhaseDiffusivity(0.05, 0.01), "
+                        "SaturatingResourceConsumption(0.2, 0.5); velocity and resource GMRES rule 1e-8, "
+                        "projection 1e-10 (multigrid PCG: the opened top has no transform basis)",
+            "value": nsteps / el, "unit": "IMEX steps/s", "ms_per_step": 1e3 * el / nsteps,
+            "cell_updates_per_s": N * N * nsteps / el,
+            "maximum_growth_trace_error": rep.maximum_growth_trace_error, "minimum_resource": rep.minimum_resource,
+            "maximum_projection_constraint_linf": rep.maximum_projection_constraint_linf}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +304,10 @@ def main(argv=None):
     ap.add_argument("--adjoint-N", type=int, default=256)
     ap.add_argument("--adjoint-snapshots", type=int, default=45)
     ap.add_argument("--no-adjoint-fd", action="store_true", help="skip the central-difference check")
+    ap.add_argument("--growth-steps", type=int, default=10, help="timed steps of the growth leg (0: skip)")
+    ap.add_argument("--growth-N", type=int, default=1024)
+    ap.add_argument("--only-growth", action="store_true",
+                    help="run the growth leg alone and print its JSON (profiling runs)")
     ap.add_argument("--only-adjoint", action="store_true",
                     help="run the differentiation leg alone and print its JSON (profiling runs)")
     args = ap.parse_args(argv)
@@ -277,6 +316,11 @@ def main(argv=None):
         print(f"bench.py measures one GPU: this build's step runs on a single MI355X (asked for --gpus {args.gpus}).",
               file=sys.stderr)
         return 2
+    if args.only_growth:
+        import eulerianfsi_amd as fsi
+        from eulerianfsi_amd import setups
+        print(json.dumps(growth_leg(fsi, setups, args.growth_N, args.growth_steps, 3)))
+        return 0
     if args.only_adjoint:
         import eulerianfsi_amd as fsi
         from eulerianfsi_amd import setups
@@ -465,6 +509,8 @@ def main(argv=None):
                    "single_thread": serial}
         cpu["host"] = host_cpu()
 
+    growth = growth_leg(fsi, setups, args.growth_N, args.growth_steps, 3) if args.growth_steps > 0 else None
+
     out = {
         "metric": METRIC, "value": steps_per_s, "unit": "IMEX steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
@@ -480,7 +526,7 @@ def main(argv=None):
                    "precision": args.precision, "mixed_precision": mixed, "ensemble": ensemble,
                    "adjoint_grads_per_s": (max(adj["binomial"]["grads_per_s"], adj["all_tapes_kept"]["grads_per_s"])
                                            if adj else None),
-                   "adjoint": adj},
+                   "adjoint": adj, "growth": growth},
         "roofline": roofline, "whole_step": whole_step, "kernels": kernels, "cpu_baseline": cpu,
     }
     print(json.dumps(out))

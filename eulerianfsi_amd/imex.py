@@ -184,8 +184,9 @@ class LinearSolverOptions:
     relative residual of 1e-14 (the exact solve), "gmres" to reltol times
     the initial residual from uv_n within maxiter iterations, then accepted
     below fallback_reltol, else the direct target (direct_fallback) or an
-    ErrorException.  `restart` and `resource_method` have no effect on the
-    device path (no restarted Krylov method, no resource solve)."""
+    ErrorException.  The resource stage solve follows the same rules with
+    `resource_method`.  `restart` has no effect on the device path (no
+    restarted Krylov method)."""
     velocity_method: str = "direct"
     resource_method: str = "gmres"
     reltol: float = 1e-8

@@ -238,3 +238,37 @@ def ensemble_cases(n=64, seed=4, sizes=(128, 192, 256, 384), method="acdi", **kw
         case.params["instance"] = k
         out.append(case)
     return out
+
+
+def growing_disk(N=64, tensor=None, acdi=True, numerics=None):
+    """An elastic disk growing on a transported resource (the growth API of
+    README.md:48-94 in a closed box): a walled unit box whose top is opened by
+    the projection (correction q = 0 there, zero flux elsewhere, so the volume
+    the disk produces leaves through the top), an ACDI disk of radius 0.2 at
+    (0.5, 0.4), SaturatingResourceGrowth(0.5, 0.5), a directional growth
+    tensor t = (1, 2)/sqrt(5) (or `tensor`), HarmonicPhaseDiffusivity(0.05,
+    0.01), SaturatingResourceConsumption(0.2, 0.5), viscosity 0.01, shear
+    modulus 0.1; the resource starts at 1 + 0.2 cos(2 pi x) sin(pi y) and is
+    held at 1 on the top.  Returns (grid, metrics, bcs, models, numerics, state)."""
+    grid = generate_grid(1.0, 1.0, N, N)
+    metrics = build_mesh_metrics(grid)
+    w, n0 = Dirichlet(0.0), Neumann(0.0)
+    wall = BoundaryConditionSetup(w, w, w, w)
+    nf = BoundaryConditionSetup(n0, n0, n0, n0)
+    q = BoundaryConditionSetup(n0, n0, n0, Dirichlet(0.0))
+    bc_c = BoundaryConditionSetup(n0, n0, n0, Dirichlet(1.0))
+    bcs = FSIBoundaryConditions(wall, wall, q, nf, bc_c, {"xx": nf, "yy": nf, "xy": nf})
+    h = 1.0 / N
+    pm, eps = phase_method("acdi" if acdi else "upwind", h)
+    models = FSIModels(phase_field_method=pm, viscosity=C.ConstantViscosity(0.01),
+                       elastic_stress=C.LinearInternalTensorStress(0.1),
+                       growth_source=C.SaturatingResourceGrowth(0.5, 0.5),
+                       growth_tensor=tensor or C.DirectionalGrowthTensor(1.0, 2.0),
+                       diffusivity=C.HarmonicPhaseDiffusivity(0.05, 0.01),
+                       resource_consumption=C.SaturatingResourceConsumption(0.2, 0.5))
+    phi = initialize_circle(grid, (0.5, 0.4), 0.2, eps)
+    x = (np.arange(N) + 0.5) * h
+    c0 = 1.0 + 0.2 * np.outer(np.cos(2 * np.pi * x), np.sin(np.pi * x))
+    state = FSIState(np.zeros((N + 1, N)), np.zeros((N, N + 1)), phi, c0, np.ones((N, N)), np.ones((N, N)),
+                     np.zeros((N, N)))
+    return grid, metrics, bcs, models, numerics or IMEXNumerics(), state
