@@ -246,8 +246,9 @@ def adjoint_leg(fsi, setups, N, nsteps, snapshots, fd_check=True):
 def growth_leg(fsi, setups, N, nsteps, warmup):
     """The growth and resource step (growing disk, setups.growing_disk) under
     the reference's stopping rules: velocity and resource GMRES rule 1e-8,
-    projection 1e-10.  Every solve is polled (no planned steps)."""
-    import torch
+    projection 1e-10.  Every solve is polled (no planned steps); each
+    integrate_step returns after its stream has finished (the Status read-back),
+    so the host clock brackets completed work."""
     numerics = fsi.IMEXNumerics(projection_reltol=1e-10, linear_solvers=fsi.LinearSolverOptions(
         velocity_method="gmres", resource_method="gmres", reltol=1e-8))
     grid, metrics, bcs, models, numerics, state = setups.growing_disk(N, numerics=numerics)
@@ -259,13 +260,11 @@ def growth_leg(fsi, setups, N, nsteps, warmup):
     for _ in range(warmup):
         fsi.integrate_step(state, t, dt, ctx, work)
         t += dt
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     rep = None
     for _ in range(nsteps):
         rep = fsi.integrate_step(state, t, dt, ctx, work)
         t += dt
-    torch.cuda.synchronize()
     el = time.perf_counter() - t0
     return {"workload": f"elastic ACDI disk growing on a diffused, consumed resource, {N}x{N}, FP64, dt = {dt:.3g}, "
                         f"steps {warmup}..{warmup + nsteps} from rest; SaturatingResourceGrowth(0.5, 0.5), "
