@@ -2257,7 +2257,11 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
         free(Txx); free(Tyy); free(Txy);
     }
     /* _accumulate_growth_viscous_force! :926-942: the divergence of the
-       stress -2 mu K_g (exactly zero under NoGrowth, so skipped there) */
+       stress -2 mu K_g (exactly zero under NoGrowth, so skipped there).
+       Parity unpinned for the coefficient of this term: with constant mu an
+       isotropic K_g gives a pure gradient that the projection removes, and no
+       closed form here fixes the anisotropic part; the stress divergence it
+       feeds is pinned (exact on linear fields, the gradient of -mu G). */
     if (c->m.viscosity_kind == FSI_VISC_CONSTANT && growth_active(c)) {
         double *Txx = dalloc(N), *Tyy = dalloc(N), *Txy = dalloc(N), mu = c->m.viscosity;
         for (int k = 0; k < N; ++k) {

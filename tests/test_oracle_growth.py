@@ -258,9 +258,10 @@ def test_uniaxial_growth_structure_tensor_closed_form(tensor, rates):
 
 
 def test_growth_viscous_stress_is_a_gradient_of_mu_g():
-    """Isotropic growth stress -2 mu K_g = -mu G I: its divergence is the
-    face gradient of -mu G (the shear part vanishes), the term of
-    _accumulate_growth_viscous_force! (TimeIntegrationIMEX.jl:926-942)."""
+    """The stress divergence that _accumulate_growth_viscous_force!
+    (TimeIntegrationIMEX.jl:926-942) feeds: for an isotropic stress -mu G I
+    it is the face gradient of -mu G (the shear part vanishes).  The
+    coefficient -2 mu of that term in the tendencies is not fixed here."""
     N, mu = 8, 0.05
     o = _oracle(cases.periodic_box(N))
     G = np.random.default_rng(7).random((N, N))
