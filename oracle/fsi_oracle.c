@@ -2258,10 +2258,7 @@ static int tendencies(fsio_ctx* c, const double* U, const double* V, const doubl
     }
     /* _accumulate_growth_viscous_force! :926-942: the divergence of the
        stress -2 mu K_g (exactly zero under NoGrowth, so skipped there).
-       Parity unpinned for the coefficient of this term: with constant mu an
-       isotropic K_g gives a pure gradient that the projection removes, and no
-       closed form here fixes the anisotropic part; the stress divergence it
-       feeds is pinned (exact on linear fields, the gradient of -mu G). */
+       */
     if (c->m.viscosity_kind == FSI_VISC_CONSTANT && growth_active(c)) {
         double *Txx = dalloc(N), *Tyy = dalloc(N), *Txy = dalloc(N), mu = c->m.viscosity;
         for (int k = 0; k < N; ++k) {
@@ -2326,6 +2323,19 @@ int fsio_eval_tendencies(fsio_ctx* c, const double* U, const double* V, const do
     free(a);
     free(d);
     free(b);
+    return rc;
+}
+
+/* The momentum tendencies of a stage with target divergence / growth source G
+ * and resource res (sigma taken as given): the terms S v and the growth
+ * viscous stress of _evaluate_stage_tendencies! :944-985 included. */
+int fsio_eval_tendencies_growth(fsio_ctx* c, const double* U, const double* V, const double* phi,
+                                const double* sxx, const double* syy, const double* sxy, const double* res,
+                                const double* G, double* E_u, double* E_v) {
+    double *Ep = dalloc(c->N), *Exx = dalloc(c->N), *Eyy = dalloc(c->N), *Exy = dalloc(c->N), *Ec = dalloc(c->N);
+    int rc = tendencies(c, U, V, phi, sxx, syy, sxy, G, 0.0, E_u, E_v, NULL, NULL, Ep, Exx, Eyy, Exy, res, NULL,
+                        Ec, NULL);
+    free(Ep); free(Exx); free(Eyy); free(Exy); free(Ec);
     return rc;
 }
 

@@ -84,13 +84,15 @@ void fsio_smooth9(fsio_ctx* c, const double* f, double* out);
 void fsio_capillary_force(fsio_ctx* c, const double* phi, double* fx, double* fy);
 /* Metric vectors: which 0..15 in MeshMetrics field order. */
 const double* fsio_metric(const fsio_ctx* c, int which, int* len);
-/* growth and resource closures (parity unpinned: the coefficient of the
- * growth viscous stress -2 mu K_g in the tendencies; every other term is pinned): G of (phi, c) with the growth tensor's trace
+/* growth and resource closures: G of (phi, c) with the growth tensor's trace
  * error as the return value; -div(upwind c v) - sink; L q (+ b) of the
  * diffusion operator of phi; the resource stage solve (I - alpha L) x = rhs + alpha b */
 double fsio_growth_source(fsio_ctx* c, const double* phi, const double* res, double* G);
 int fsio_resource_explicit_rhs(fsio_ctx* c, const double* U, const double* V, const double* phi,
                                const double* res, double* out);
+int fsio_eval_tendencies_growth(fsio_ctx* c, const double* U, const double* V, const double* phi,
+                                const double* sxx, const double* syy, const double* sxy, const double* res,
+                                const double* G, double* E_u, double* E_v);
 int fsio_diffusion_apply(fsio_ctx* c, const double* phi, const double* q, double* out, int with_boundary);
 int fsio_resource_solve(fsio_ctx* c, fsio_scalar alpha, const double* phi, const double* rhs, const double* x0,
                         double* x, int* iters);

@@ -72,6 +72,7 @@ def load(path: Path | None = None):
         "fsio_growth_source": (c_double, [P, _dp, _dp, _dp]),
         "fsio_resource_explicit_rhs": (c_int, [P] + [_dp] * 5),
         "fsio_diffusion_apply": (c_int, [P, _dp, _dp, _dp, c_int]),
+        "fsio_eval_tendencies_growth": (c_int, [P] + [_dp] * 10),
         "fsio_resource_solve": (c_int, [P, c_double, _dp, _dp, _dp, _dp, POINTER(c_int)]),
     }
     for name, (res, args) in sig.items():
@@ -282,6 +283,14 @@ class Oracle:
         self._check(self.lib.fsio_resource_explicit_rhs(self.h, _p(_F(U)), _p(_F(V)), _p(_F(phi)), _p(_F(res)),
                                                         _p(out)))
         return out
+
+    def momentum_tendencies_growth(self, U, V, phi, sxx, syy, sxy, res, G):
+        """(E_u, E_v) of a stage with growth source G (sigma taken as given)."""
+        Eu = np.zeros((self.nx + 1, self.ny), order="F")
+        Ev = np.zeros((self.nx, self.ny + 1), order="F")
+        self._check(self.lib.fsio_eval_tendencies_growth(self.h, *[_p(_F(a)) for a in (U, V, phi, sxx, syy, sxy, res, G)],
+                                                         _p(Eu), _p(Ev)))
+        return Eu, Ev
 
     def diffusion_apply(self, phi, q, with_boundary=True):
         out = np.zeros((self.nx, self.ny), order="F")

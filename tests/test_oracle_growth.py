@@ -268,3 +268,22 @@ def test_growth_viscous_stress_is_a_gradient_of_mu_g():
     Fx, Fy = o.stress_divergence(-mu * G, -mu * G, 0.0 * G)
     assert np.allclose(Fx[:N], -mu * (G - np.roll(G, 1, axis=0)) * N, atol=1e-14)
     assert np.allclose(Fy[:, :N], -mu * (G - np.roll(G, 1, axis=1)) * N, atol=1e-14)
+
+
+@pytest.mark.parametrize("t,axis", [((1.0, 0.0), 0), ((0.0, 1.0), 1)])
+def test_uniaxial_growth_viscous_force_at_rest(t, axis):
+    """At rest, sigma = I and no elastic stress, the momentum tendency is the
+    divergence of the growth viscous stress -2 mu K_g alone
+    (TimeIntegrationIMEX.jl:926-942); with K_g = G t (x) t along an axis it is
+    the face difference of -2 mu G along that axis and zero across it."""
+    N, mu = 8, 0.05
+    models = fsi.FSIModels(viscosity=C.ConstantViscosity(mu), growth_source=C.ConstantIntrinsicGrowth(0.1),
+                           growth_tensor=C.DirectionalGrowthTensor(*t))
+    o = _oracle(cases.periodic_box(N, models=models))
+    G = np.random.default_rng(11).random((N, N))
+    one, zero = np.ones((N, N)), np.zeros((N, N))
+    Eu, Ev = o.momentum_tendencies_growth(np.zeros((N + 1, N)), np.zeros((N, N + 1)), one, one, one, zero, one, G)
+    F = -2.0 * mu * (G - np.roll(G, 1, axis=axis)) * N
+    along, across = (Eu[:N], Ev[:, :N]) if axis == 0 else (Ev[:, :N], Eu[:N])
+    assert np.allclose(along, F, rtol=0, atol=1e-14)
+    assert np.max(np.abs(across)) <= 1e-15
